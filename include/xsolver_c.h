@@ -1,0 +1,164 @@
+/*
+ * xsolver_c.h -- C-ABI of the MI355X (gfx950) xsolver Krylov library.
+ *
+ * Drop-in for the reference API hip/xsolver_c.h:37-49 (implemented in
+ * hip/xsolver_c.cpp:41-333).  The reference declares those functions with C++
+ * linkage (the extern "C" is commented out), takes `int &usediter` and gives
+ * `restart` a default.  Here the core is real C: plain pointers and sizes,
+ * int status codes, no C++ or torch types.  For C++ callers the header also
+ * provides inline overloads with the reference's exact signatures
+ * (reference, default argument), so code written against the reference
+ * header (e.g. hip/solve_base.h:4-24) recompiles unchanged.
+ *
+ * Everything runs on one GPU and one rank; the reference's MPI path
+ * (advanced_pam[0] == 1, scatter_mpi/gather_mpi) is not provided.
+ * The library is not thread-safe: the legacy entry points share one global
+ * context, like the reference's singletons (hip/xsolver_c.cpp:12-21).
+ */
+#ifndef XSOLVER_C_H_
+#define XSOLVER_C_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum xs_status {
+  XS_OK = 0,
+  XS_NOT_CONVERGED = 1,     /* maxiter reached (the reference's usediter = maxiter) */
+  XS_ERR_ARG = -1,          /* invalid argument / unknown type string */
+  XS_ERR_STATE = -2,        /* call out of order (e.g. solve before mat_setup) */
+  XS_ERR_UNSUPPORTED = -3,  /* e.g. IC0/ICT on GPU, MPI distribution, CPU backend */
+  XS_ERR_DEVICE = -4,       /* HIP runtime error or no device */
+  XS_ERR_BREAKDOWN = -5,    /* non-finite residual / Krylov breakdown */
+  XS_ERR_PIVOT = -6,        /* missing diagonal entry / zero Jacobi pivot */
+  XS_ERR_MEMORY = -7
+};
+
+/* ======================================================================== */
+/* Legacy singleton API (reference hip/xsolver_c.h:37-49)                   */
+/* ======================================================================== */
+
+/* hip/xsolver_c.cpp:41-102.  CSR arrays are 0- or 1-based (rowptr[0] decides,
+ * as in hip/HostMatrix/HostMatrixCSR.cpp:119-128); they are copied to the GPU
+ * here.  advanced_pam[0] = 1 (MPI) -> XS_ERR_UNSUPPORTED; advanced_pam[1] = 1
+ * selects ELL SpMV; advanced_pam[2] must be 1 (GPU). */
+void xsolver_mat_setup(int n, int *rowptr, int *colidx, double *val, int *advanced_pam);
+/* hip/xsolver_c.cpp:103-130: "Bicgstab|bicgstab|BICGSTAB", "Igcr|igcr|IGCR",
+ * "Cg|cg|CG"; additionally "Gmres|gmres|GMRES" and "Fgmres|fgmres|FGMRES". */
+void xsolver_ksp_settype(char *fmt);
+/* hip/xsolver_c.cpp:131-133 */
+void xsolver_ksp_set_absolute_tol(double absolute_tol);
+/* hip/xsolver_c.cpp:134-149.  *usediter and resvec[0..usediter] (absolute
+ * residual norms, resvec[0] = ||r0||) are written when xsolver_solve returns;
+ * resvec may be NULL, otherwise it must hold maxiter + 1 doubles. */
+void xsolver_ksp_setoption(double tol, int maxiter, int *usediter, double *resvec, int restart);
+/* hip/xsolver_c.cpp:150-262: "Jacobi", "Ilu0", "Ilut", "Ilu0_mkl", "Ilut_mkl"
+ * in the reference's spellings (on the GPU the *_mkl names map to ParILU/
+ * ParILUT as in :196-211); "Ic0"/"Ict" -> XS_ERR_UNSUPPORTED (:174-177). */
+void xsolver_pc_settype(char *fmt);
+/* hip/xsolver_c.cpp:264-273 */
+void xsolver_pc_setilut(double tol, int max_fill);
+void xsolver_pc_setict(double tol, int max_fill);
+/* hip/xsolver_c.cpp:274-292.  sol holds x0 on entry and the solution on exit. */
+void xsolver_solve(double *rhs, double *sol);
+/* hip/xsolver_c.cpp:308-333 */
+void xsolver_free(void);
+
+/* Result / iteration queries (the reference returns results only through the
+ * usediter/resvec pointers captured by xsolver_ksp_setoption). */
+int xsolver_last_error(void);
+int xsolver_get_iterations(void);
+double xsolver_get_residual(void);
+
+/* ======================================================================== */
+/* Handle API (same semantics, one context per system)                      */
+/* ======================================================================== */
+typedef struct xs_ctx xs_ctx;
+
+/* Create a context on the current HIP device (no reference counterpart: the
+ * reference keeps one global context, hip/xsolver_c.cpp:12-21). */
+int xs_create(xs_ctx **ctx);
+int xs_destroy(xs_ctx *ctx);
+const char *xs_status_string(int status);
+
+/* Matrix upload from host CSR (hip/xsolver_c.cpp:41-102). */
+int xs_mat_setup(xs_ctx *ctx, int n, const int *rowptr, const int *colidx,
+                 const double *val, const int *advanced_pam);
+/* Matrix already resident on the GPU (0-based CSR device pointers, borrowed
+ * until xs_destroy / the next setup).  No reference counterpart: used when the
+ * caller owns HBM-resident data. */
+int xs_mat_setup_device(xs_ctx *ctx, int n, int nnz, const int *d_rowptr,
+                        const int *d_colidx, const double *d_val, int use_ell);
+/* Replace the values of the current pattern (host or device pointer), the
+ * "*_mat_update" of the reference's Fortran-style API (intel/HostMatrix.h:197-201);
+ * the preconditioner is re-factorised on the next solve. */
+int xs_mat_update_values(xs_ctx *ctx, const double *val, int on_device);
+
+int xs_ksp_settype(xs_ctx *ctx, const char *type);
+int xs_ksp_set_absolute_tol(xs_ctx *ctx, double abstol);
+int xs_ksp_setoption(xs_ctx *ctx, double tol, int maxiter, int restart);
+int xs_pc_settype(xs_ctx *ctx, const char *type);
+int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
+
+/* Named options (no reference counterpart; the reference hard-codes them):
+ *   "parilu_sweeps"  ParILU(0) fixed-point sweeps (reference: 20, precon_gpuilu0.cpp:5)
+ *   "parilu_async"   1 = in-place asynchronous sweeps (reference behaviour),
+ *                    0 = deterministic double-buffered sweeps
+ *   "check_every"    iterations launched between host convergence checks
+ *   "precision"      0 = FP64, 1 = FP32 inner / FP64 outer
+ *   "use_ell"        1 = ELL SpMV (same as advanced_pam[1] = 1) */
+int xs_set_option(xs_ctx *ctx, const char *name, double value);
+
+/* Factorise / prepare the preconditioner now (otherwise done by the first
+ * solve, like Xsolver::xsolver_init, hip/Solver/xsolver_bicgstab.cpp:3-19). */
+int xs_pc_setup(xs_ctx *ctx);
+/* Solve with host vectors (hip/xsolver_c.cpp:274-292): sol = x0 in, x out. */
+int xs_solve(xs_ctx *ctx, const double *rhs, double *sol);
+/* Solve with device vectors already in HBM (d_sol = x0 in, x out). */
+int xs_solve_device(xs_ctx *ctx, const double *d_rhs, double *d_sol);
+
+int xs_get_iterations(const xs_ctx *ctx);
+double xs_get_residual(const xs_ctx *ctx);        /* last ||r_k|| (absolute) */
+double xs_get_initial_residual(const xs_ctx *ctx);
+/* Copies min(len, iterations+1) residual norms; returns the count copied. */
+int xs_get_resvec(const xs_ctx *ctx, double *out, int len);
+/* Milliseconds of the last pc setup and the last solve (HIP events). */
+double xs_get_time_ms(const xs_ctx *ctx, const char *phase);
+
+/* ======================================================================== */
+/* Kernel-level entry points on device pointers (stream = hipStream_t or 0). */
+/* They operate on the matrix/factors held by a context.                     */
+/* ======================================================================== */
+/* y = A x (CSR row-binned or ELL, per the context's format); reference
+ * DeviceMatrixCSR::SpMV, hip/HostMatrix/DeviceMatrix/DeviceMatrixCSR.cpp:157-170,
+ * and DeviceMatrixELL::SpMV, DeviceMatrixELL.cpp:147-162. */
+int xs_spmv(xs_ctx *ctx, const double *d_x, double *d_y, void *stream);
+/* y = b - A x; reference bmAxCSR/bmAxELL (DeviceMatrixCSR.cpp:72-84, 171-189). */
+int xs_bmax(xs_ctx *ctx, const double *d_b, const double *d_x, double *d_y, void *stream);
+/* y = A^T x (adjoint path; no reference counterpart). */
+int xs_spmv_t(xs_ctx *ctx, const double *d_x, double *d_y, void *stream);
+/* Preconditioner apply y = M^{-1} x; for ILU the sync-free L then U solves
+ * (reference Lcsr_trsv_sync_free / Ucsc_trsv_sync_free,
+ * hip/Precond/gpuparilu.cpp:257-391). */
+int xs_pc_apply(xs_ctx *ctx, const double *d_x, double *d_y, void *stream);
+/* Transposed preconditioner apply y = M^{-T} x (adjoint path). */
+int xs_pc_apply_t(xs_ctx *ctx, const double *d_x, double *d_y, void *stream);
+/* Single triangular solves with the ILU factors: which = 0 lower (unit),
+ * 1 upper (with diagonal). */
+int xs_trsv(xs_ctx *ctx, int which, const double *d_b, double *d_x, void *stream);
+/* Copy the ILU factors to the host in combined CSR form (A's pattern for
+ * ILU(0)): nnz values, plus the position of each pivot.  Returns the nnz. */
+int xs_pc_get_factors(xs_ctx *ctx, int *rowptr, int *colidx, double *val, int *diagpos, int cap);
+
+#ifdef __cplusplus
+} /* extern "C" */
+
+/* C++ overloads with the reference's exact signatures (hip/xsolver_c.h:40). */
+inline void xsolver_ksp_setoption(double tol, int maxiter, int &usediter,
+                                  double *resvec, int restart = 0) {
+  xsolver_ksp_setoption(tol, maxiter, &usediter, resvec, restart);
+}
+#endif
+
+#endif /* XSOLVER_C_H_ */

@@ -1,0 +1,196 @@
+"""ctypes bindings for the CPU oracle (test infrastructure only).
+
+The oracle is the checker: only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py import this module.  The product library
+(xsolver_amd/libxsolver_amd.so) never calls it.
+
+``liboracle.so`` is the C restatement in xs_oracle.c.  ``_ref/libxsref.so`` is
+the reference's own hip/Precond/ilut_diag_in_csr.cpp compiled in place (it
+exists only where the reference tree was available at build time).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+
+
+def build(ref=None):
+    """Compile liboracle.so (and _ref/libxsref.so when the reference exists)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+    if ref is None:
+        ref = os.path.isdir("/root/reference")
+    if ref:
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build(ref=False)
+        L = ctypes.CDLL(path)
+        L.or_spmv.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.or_bmax.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
+        L.or_spmv_t.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.or_lower_solve.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.or_upper_solve.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
+        L.or_jacobi_setup.argtypes = [ctypes.c_int, _ip, _ip, _dp, ctypes.c_double, _dp]
+        L.or_ilu0.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _ip]
+        L.or_ilut.argtypes = [ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_double,
+                              ctypes.c_int, _dp, _ip, _ip, _ip, _dp]
+        L.or_ilu_split.argtypes = [ctypes.c_int, _ip, _ip, _dp, _ip, _ip, _ip, _dp,
+                                   _ip, _ip, _dp, _dp]
+        L.or_parilu_sweep.argtypes = [ctypes.c_int, _ip, _ip, _dp, _ip, _ip, _dp,
+                                      _ip, _ip, _dp, _dp, _dp]
+        L.or_solve.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _ip, _ip, _dp,
+                               _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                               ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _c_int_p]
+        for f in ("or_jacobi_setup", "or_ilu0", "or_ilut", "or_solve"):
+            getattr(L, f).restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def ref_lib():
+    """The reference's compiled ILU code, or None where it was not built."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libxsref.so")
+        if not os.path.exists(path):
+            return None
+        R = ctypes.CDLL(path)
+        R.ref_ilu0_simple.argtypes = [_dp, _dp, _ip, _ip, ctypes.c_int, _ip]
+        R.ref_ilutp_final.argtypes = [_dp, _dp, _ip, _ip, ctypes.c_int, _dp, ctypes.c_int,
+                                      _ip, _ip, _ip, ctypes.c_double]
+        _REF = R
+    return _REF
+
+
+def _csr(A):
+    rp, ci, v = A
+    return (np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32),
+            np.ascontiguousarray(v, np.float64))
+
+
+def spmv(A, x):
+    rp, ci, v = _csr(A)
+    y = np.empty(len(rp) - 1)
+    lib().or_spmv(len(rp) - 1, rp, ci, v, np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def bmax(A, b, x):
+    rp, ci, v = _csr(A)
+    y = np.empty(len(rp) - 1)
+    lib().or_bmax(len(rp) - 1, rp, ci, v, np.ascontiguousarray(b, np.float64),
+                  np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def spmv_t(A, x):
+    rp, ci, v = _csr(A)
+    y = np.empty(len(rp) - 1)
+    lib().or_spmv_t(len(rp) - 1, rp, ci, v, np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def lower_solve(L, b):
+    rp, ci, v = _csr(L)
+    x = np.empty(len(rp) - 1)
+    lib().or_lower_solve(len(rp) - 1, rp, ci, v, np.ascontiguousarray(b, np.float64), x)
+    return x
+
+
+def upper_solve(U, dinv, b):
+    rp, ci, v = _csr(U)
+    x = np.empty(len(rp) - 1)
+    lib().or_upper_solve(len(rp) - 1, rp, ci, v, np.ascontiguousarray(dinv, np.float64),
+                         np.ascontiguousarray(b, np.float64), x)
+    return x
+
+
+def jacobi_dinv(A, small=0.0):
+    rp, ci, v = _csr(A)
+    d = np.empty(len(rp) - 1)
+    bad = lib().or_jacobi_setup(len(rp) - 1, rp, ci, v, small, d)
+    return d, bad
+
+
+def ilu0(A):
+    """Combined ILU(0) values on A's pattern and the pivot positions."""
+    rp, ci, v = _csr(A)
+    n = len(rp) - 1
+    lu = np.empty(len(v))
+    uptr = np.empty(n, np.int32)
+    rc = lib().or_ilu0(n, rp, ci, v, lu, uptr)
+    if rc:
+        raise ValueError("row %d has no diagonal entry" % (rc - 1))
+    return lu, uptr
+
+
+def ilut(A, lfil, droptol):
+    """Combined ILUT rows (lurows, lucols, luval), pivot positions, 1/pivot."""
+    rp, ci, v = _csr(A)
+    n = len(rp) - 1
+    cap = (2 * lfil + 2) * n
+    luval = np.zeros(cap)
+    lucols = np.zeros(cap, np.int32)
+    lurows = np.zeros(n + 1, np.int32)
+    uptr = np.zeros(n + 1, np.int32)
+    diag = np.zeros(n)
+    rc = lib().or_ilut(n, rp, ci, v, lfil, droptol, cap, luval, lucols, lurows, uptr, diag)
+    if rc:
+        raise ValueError("ILUT capacity exceeded")
+    nnz = lurows[n]
+    return lurows, lucols[:nnz].copy(), luval[:nnz].copy(), uptr[:n].copy(), diag
+
+
+def ilu_split(rp, ci, luval, uptr):
+    """Split combined factors into L (strict lower), U (strict upper scaled by
+    1/u_ii) and dinv, as the reference's seqilu0/seqilut do."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    luval = np.ascontiguousarray(luval, np.float64)
+    uptr = np.ascontiguousarray(uptr, np.int32)
+    n = len(rp) - 1
+    cap = max(int(rp[n]), 1)
+    lrp = np.zeros(n + 1, np.int32); urp = np.zeros(n + 1, np.int32)
+    lci = np.zeros(cap, np.int32); uci = np.zeros(cap, np.int32)
+    lv = np.zeros(cap); uv = np.zeros(cap); dinv = np.zeros(n)
+    lib().or_ilu_split(n, rp, ci, luval, uptr, lrp, lci, lv, urp, uci, uv, dinv)
+    nl, nu = lrp[n], urp[n]
+    return ((lrp, lci[:nl].copy(), lv[:nl].copy()),
+            (urp, uci[:nu].copy(), uv[:nu].copy()), dinv)
+
+
+def parilu_sweep(A, L, Ucsc, n):
+    rp, ci, a = _csr(A)
+    lrp, lci, lv = _csr(L)
+    ucp, uri, uv = _csr(Ucsc)
+    lo = np.empty_like(lv); uo = np.empty_like(uv)
+    lib().or_parilu_sweep(n, rp, ci, a, lrp, lci, lv, ucp, uri, uv, lo, uo)
+    return lo, uo
+
+
+def solve(A, b, ksp="cg", pc="jacobi", tol=1e-8, abstol=0.0, maxiter=1000,
+          restart=0, x0=None, ilut_tol=1e-4, ilut_fill=10):
+    """Run one oracle solve.  Returns (x, status, usediter, resvec)."""
+    rp, ci, v = _csr(A)
+    n = len(rp) - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64, copy=True)
+    resvec = np.zeros(maxiter + 2)
+    it = ctypes.c_int(0)
+    st = lib().or_solve(ksp.encode(), pc.encode(), n, rp, ci, v,
+                        np.ascontiguousarray(b, np.float64), x, tol, abstol, maxiter,
+                        restart, ilut_tol, ilut_fill, resvec, ctypes.byref(it))
+    return x, st, it.value, resvec[: it.value + 1].copy()

@@ -1,0 +1,735 @@
+/*
+ * xs_oracle.c -- CPU restatement of the xsolver reference's preconditioned
+ * Krylov path.  TEST INFRASTRUCTURE ONLY (the checker, never the product):
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Each function cites the reference file:line it restates.  How each function
+ * is pinned (golden vectors from the compiled reference, closed forms,
+ * library routines, brute force) is listed in the test files under tests/.
+ */
+#include "xs_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* BLAS-1 helpers (sequential summation order)                              */
+/* ------------------------------------------------------------------------ */
+static double dot(int n, const double *a, const double *b) {
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+static void axpy(int n, double a, const double *x, double *y) {
+  for (int i = 0; i < n; i++) y[i] += a * x[i];
+}
+static void scal(int n, double a, double *x) {
+  for (int i = 0; i < n; i++) x[i] *= a;
+}
+static void copy(int n, const double *x, double *y) { memcpy(y, x, sizeof(double) * (size_t)n); }
+
+static void apply_M(or_precond_fn M, void *ctx, int n, const double *x, double *y) {
+  if (M) M(ctx, x, y);
+  else if (x != y) copy(n, x, y);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sparse kernels                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* y = A x.  intel/HostMatrix.h:83-103: the off-diagonal entries of the row are
+ * accumulated in storage order and the diagonal term is added last. */
+void or_spmv(int n, const int *rp, const int *ci, const double *v,
+             const double *x, double *y) {
+  for (int i = 0; i < n; i++) {
+    double r = 0.0, d = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; e++) {
+      if (ci[e] == i) d += v[e];
+      else r = r + v[e] * x[ci[e]];
+    }
+    y[i] = r + d * x[i];
+  }
+}
+
+/* y = b - A x.  intel/HostMatrix.h:105-125 (r = rhs; r -= offdiag; y = r - diag*x). */
+void or_bmax(int n, const int *rp, const int *ci, const double *v,
+             const double *b, const double *x, double *y) {
+  for (int i = 0; i < n; i++) {
+    double r = b[i], d = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; e++) {
+      if (ci[e] == i) d += v[e];
+      else r = r - v[e] * x[ci[e]];
+    }
+    y[i] = r - d * x[i];
+  }
+}
+
+/* y = A^T x (adjoint path; no reference counterpart). */
+void or_spmv_t(int n, const int *rp, const int *ci, const double *v,
+               const double *x, double *y) {
+  memset(y, 0, sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; i++)
+    for (int e = rp[i]; e < rp[i + 1]; e++) y[ci[e]] += v[e] * x[i];
+}
+
+/* Forward solve with a strictly-lower CSR factor and an implied unit diagonal:
+ * x_i = b_i - sum_j L_ij x_j.  hip/HostMatrix/HostMatrixCSR.cpp:386-394. */
+void or_lower_solve(int n, const int *rp, const int *ci, const double *v,
+                    const double *b, double *x) {
+  for (int i = 0; i < n; i++) {
+    double s = b[i];
+    for (int e = rp[i]; e < rp[i + 1]; e++) s -= x[ci[e]] * v[e];
+    x[i] = s;
+  }
+}
+
+/* Backward solve with a strictly-upper CSR factor and the inverted diagonal:
+ * x_i = dinv_i (b_i - sum_j U_ij x_j).  Same recurrence as
+ * hip/HostMatrix/HostMatrixCSR.cpp:398-408 but with an unscaled U. */
+void or_upper_solve(int n, const int *rp, const int *ci, const double *v,
+                    const double *dinv, const double *b, double *x) {
+  for (int i = n - 1; i >= 0; i--) {
+    double s = b[i];
+    for (int e = rp[i]; e < rp[i + 1]; e++) s -= x[ci[e]] * v[e];
+    x[i] = s * dinv[i];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Preconditioners                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* dinv_i = 1/(a_ii + small).  intel/HostMatrix.h:127-137; the HIP framework
+ * uses small = 0 (hip/Precond/precon_jacobi.h:18).  Returns the first row
+ * with a zero (or missing) pivot + 1, or 0 on success
+ * (hip/HostVector/HostVector.cpp:84-95 exits there). */
+int or_jacobi_setup(int n, const int *rp, const int *ci, const double *v,
+                    double small, double *dinv) {
+  int bad = 0;
+  for (int i = 0; i < n; i++) {
+    double d = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; e++)
+      if (ci[e] == i) d = v[e];
+    double t = d + small;
+    if (t == 0.0 && !bad) bad = i + 1;
+    dinv[i] = 1.0 / t;
+  }
+  return bad;
+}
+
+/* y = dinv o x.  intel/HostMatrix.h:139-146. */
+typedef struct { int n; const double *dinv; } or_jac_ctx_t;
+void or_jacobi_apply(void *ctx, const double *x, double *y) {
+  const or_jac_ctx_t *c = (const or_jac_ctx_t *)ctx;
+  for (int i = 0; i < c->n; i++) y[i] = c->dinv[i] * x[i];
+}
+
+/* In-place textbook IKJ ILU(0) on a copy of A's values; pivots below 1e-20 in
+ * magnitude are replaced by 1e-20.  hip/Precond/ilut_diag_in_csr.cpp:347-406.
+ * uptr[i] = position of a_ii (or -1 when absent).  Returns 0, or i+1 for the
+ * first row without a diagonal entry. */
+int or_ilu0(int n, const int *rp, const int *ci, const double *v,
+            double *luval, int *uptr) {
+  int nnz = rp[n];
+  for (int i = 0; i < n; i++) uptr[i] = -1;
+  for (int e = 0; e < nnz; e++) luval[e] = v[e];
+  for (int i = 0; i < n; i++)
+    for (int e = rp[i]; e < rp[i + 1]; e++)
+      if (ci[e] == i) { uptr[i] = e; break; }
+  for (int i = 0; i < n; i++)
+    if (uptr[i] < 0) return i + 1;
+  for (int i = 0; i < n; i++) {
+    int upart = uptr[i];
+    for (int j = rp[i]; j < upart; j++) {
+      int k = ci[j];
+      if (fabs(luval[uptr[k]]) < 1e-20) luval[uptr[k]] = 1e-20;
+      luval[j] = luval[j] / luval[uptr[k]];
+      int sub = uptr[k];
+      for (int jj = j + 1; jj < rp[i + 1]; jj++) {
+        int kk = ci[jj];
+        while (ci[sub] < kk && sub < rp[k + 1]) sub++;
+        if (sub < rp[k + 1]) {
+          if (ci[sub] == kk) luval[jj] = luval[jj] - luval[j] * luval[sub];
+        } else {
+          break;
+        }
+      }
+    }
+    if (fabs(luval[uptr[i]]) < 1e-20) luval[uptr[i]] = 1e-20;
+  }
+  return 0;
+}
+
+/* Split a combined LU (unit-lower strict part | diagonal | upper part) into
+ * L (strict lower), dinv = 1/u_ii and U (strict upper, each row scaled by
+ * dinv_i).  hip/HostMatrix/HostMatrixCSR.cpp:164-197 (ILU0) and :230-268
+ * (ILUT): both use the same split. */
+void or_ilu_split(int n, const int *rp, const int *ci, const double *luval,
+                  const int *uptr, int *lrp, int *lci, double *lv,
+                  int *urp, int *uci, double *uv, double *dinv) {
+  int il = 0, iu = 0;
+  lrp[0] = 0; urp[0] = 0;
+  for (int i = 0; i < n; i++) {
+    for (int e = rp[i]; e < uptr[i]; e++) { lv[il] = luval[e]; lci[il++] = ci[e]; }
+    dinv[i] = 1.0 / luval[uptr[i]];
+    for (int e = uptr[i] + 1; e < rp[i + 1]; e++) {
+      uv[iu] = luval[e] * dinv[i];
+      uci[iu++] = ci[e];
+    }
+    lrp[i + 1] = il; urp[i + 1] = iu;
+  }
+}
+
+/* y = (LU)^{-1} x with the split factors.  hip/HostMatrix/HostMatrixCSR.cpp:385-409:
+ * forward with unit L, y *= dinv, backward with the pre-scaled U visiting each
+ * row's entries from last to first. */
+void or_lusolve(void *ctx, const double *x, double *y) {
+  const or_lu_t *f = (const or_lu_t *)ctx;
+  int n = f->n;
+  double *t = (double *)malloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    double s = x[i];
+    for (int e = f->lrp[i]; e < f->lrp[i + 1]; e++) s -= t[f->lci[e]] * f->lv[e];
+    t[i] = s;
+  }
+  for (int i = 0; i < n; i++) t[i] *= f->dinv[i];
+  for (int i = n - 1; i >= 0; i--) {
+    double s = t[i];
+    for (int e = f->urp[i + 1] - 1; e >= f->urp[i]; e--) s -= t[f->uci[e]] * f->uv[e];
+    t[i] = s;
+  }
+  memcpy(y, t, sizeof(double) * (size_t)n);
+  free(t);
+}
+
+/* --- ILUT (dual threshold, per-row heaps) -------------------------------- */
+/* Min-heap of column ids ordered by |w[id]| ("less" = larger magnitude, so the
+ * top is the smallest magnitude).  vh_push restates libstdc++'s push_heap
+ * sift-up (strict comparison) as used by hip/Precond/ilut_diag_in_csr.cpp:120,
+ * vh_replace_top restates heap_swap (ilut_diag_in_csr.cpp:2-18). */
+static int vh_less(const double *w, int a, int b) { return fabs(w[a]) > fabs(w[b]); }
+static void vh_push(int *h, int len, const double *w) {
+  int hole = len - 1, val = h[hole];
+  int parent = (hole - 1) / 2;
+  while (hole > 0 && vh_less(w, h[parent], val)) {
+    h[hole] = h[parent];
+    hole = parent;
+    parent = (hole - 1) / 2;
+  }
+  h[hole] = val;
+}
+static void vh_replace_top(int *h, int len, const double *w) {
+  int i = 0, j = 1, k = h[0];
+  double vk = fabs(w[k]);
+  while (j < len) {
+    if (j + 1 < len && fabs(w[h[j]]) > fabs(w[h[j + 1]])) j = j + 1;
+    if (fabs(w[h[j]]) > vk) break;
+    h[i] = h[j];
+    i = j;
+    j = 2 * j + 1;
+  }
+  h[i] = k;
+}
+/* Plain binary min-heap of ints (column order of the elimination). */
+static void ih_push(int *h, int len) {
+  int hole = len - 1, val = h[hole], parent = (hole - 1) / 2;
+  while (hole > 0 && h[parent] > val) { h[hole] = h[parent]; hole = parent; parent = (hole - 1) / 2; }
+  h[hole] = val;
+}
+static int ih_pop(int *h, int len) {
+  int top = h[0], last = h[len - 1];
+  len -= 1;
+  int hole = 0;
+  for (;;) {
+    int c = 2 * hole + 1;
+    if (c >= len) break;
+    if (c + 1 < len && h[c + 1] < h[c]) c++;
+    if (h[c] >= last) break;
+    h[hole] = h[c];
+    hole = c;
+  }
+  if (len > 0) h[hole] = last;
+  return top;
+}
+static void sort_prefix_by_col(int *cols, int len) {
+  /* selection sort, ilut_diag_in_csr.cpp:147-158 */
+  for (int a = 0; a < len; a++) {
+    int im = a;
+    for (int b = a + 1; b < len; b++) if (cols[b] < cols[im]) im = b;
+    int t = cols[a]; cols[a] = cols[im]; cols[im] = t;
+  }
+}
+
+/* Dual-threshold ILUT: drop |v| < droptol*||a_i||_2, keep at most lfil entries
+ * in each of the L and U parts of a row (the largest in magnitude), substitute
+ * (1e-4/droptol + 1)*droptol*||a_i||_2 for a zero pivot.
+ * hip/Precond/ilut_diag_in_csr.cpp:20-251 (ilutp_final, permtol/iperm unused).
+ * Output: combined rows lurows/lucols/luval with uptr[i] = position of the
+ * pivot, diag[i] = 1/pivot.  cap = capacity of lucols/luval.  Returns 0, or -1
+ * when cap is exceeded. */
+int or_ilut(int n, const int *rp, const int *ci, const double *v, int lfil,
+            double droptol, int cap, double *luval, int *lucols, int *lurows,
+            int *uptr, double *diag) {
+  double *w = (double *)calloc((size_t)n, sizeof(double));
+  int *uj = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+  int *lj = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+  int nnz = 0, rc = 0;
+  lurows[0] = 0;
+  for (int i = 0; i < n; i++) diag[i] = 0.0;
+  for (int i = 0; i < n && rc == 0; i++) {
+    int nu = 0, nl = 0;
+    double t1 = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; e++) {
+      int k = ci[e];
+      w[k] = v[e];
+      t1 += v[e] * v[e];
+      if (k < i) { lj[nl++] = k; ih_push(lj, nl); }
+      if (k > i) uj[nu++] = k;
+    }
+    double tnorm = sqrt(t1) * droptol;
+    if (nnz + 2 * lfil + 1 > cap) { rc = -1; break; }
+    int *vh = lucols + nnz;
+    int nv = 0;
+    while (nl > 0) {
+      int k = ih_pop(lj, nl);
+      nl -= 1;
+      double factor = w[k] * diag[k];
+      if (fabs(factor) < tnorm) { w[k] = 0.0; continue; }
+      for (int e = uptr[k] + 1; e < lurows[k + 1]; e++) {
+        int j = lucols[e];
+        double twj = w[j];
+        if (twj == 0.0) {
+          if (j < i) { lj[nl++] = j; ih_push(lj, nl); }
+          if (i < j) uj[nu++] = j;
+        }
+        w[j] = twj - factor * luval[e];
+      }
+      w[k] = factor;
+      if (nv < lfil) {
+        vh[nv++] = k;
+        vh_push(vh, nv, w);
+      } else {
+        int top = vh[0];
+        if (fabs(factor) > fabs(w[top])) {
+          w[top] = 0.0;
+          vh[0] = k;
+          vh_replace_top(vh, nv, w);
+        } else {
+          w[k] = 0.0;
+        }
+      }
+    }
+    sort_prefix_by_col(vh, nv);
+    for (int t = 0; t < nv; t++) { luval[nnz] = w[vh[t]]; nnz++; w[vh[t]] = 0.0; }
+    uptr[i] = nnz;
+    if (w[i] != 0.0) {
+      diag[i] = 1.0 / w[i];
+      luval[nnz] = w[i];
+      lucols[nnz] = i;
+      w[i] = 0.0;
+      nnz++;
+    } else {
+      double sub = 1e-4 / droptol * tnorm + tnorm;
+      diag[i] = 1.0 / sub;
+      luval[nnz] = sub;
+      lucols[nnz] = i;
+      nnz++;
+    }
+    vh = lucols + nnz;
+    nv = 0;
+    for (int t = 0; t < nu; t++) {
+      int ki = uj[t];
+      if (fabs(w[ki]) > tnorm) {
+        if (nv < lfil) {
+          vh[nv++] = ki;
+          vh_push(vh, nv, w);
+        } else {
+          int top = vh[0];
+          if (fabs(w[ki]) > fabs(w[top])) {
+            w[top] = 0.0;
+            vh[0] = ki;
+            vh_replace_top(vh, nv, w);
+          } else {
+            w[ki] = 0.0;
+          }
+        }
+      } else {
+        w[ki] = 0.0;
+      }
+    }
+    sort_prefix_by_col(vh, nv);
+    for (int t = 0; t < nv; t++) { luval[nnz] = w[vh[t]]; nnz++; w[vh[t]] = 0.0; }
+    lurows[i + 1] = nnz;
+  }
+  free(w); free(uj); free(lj);
+  return rc;
+}
+
+/* One synchronous ParILU(0) fixed-point sweep over pattern(A).
+ * hip/Precond/gpuparilu.cpp:46-102 (ilu0_ij_aw): for (i,j) in pattern(A),
+ *   s = a_ij - sum_{k<min(i,j)} l_ik u_kj   (sorted merge of L row i, U col j)
+ *   i > j : l_ij = s / u_jj  (|u_jj| < 1e-10 clamped to +-1e-10)
+ *   i <= j: u_ij = s         (diagonal |s| < 1e-10 clamped to +-1e-10)
+ *   a non-finite result leaves the old value in place.
+ * The reference sweeps asynchronously in place; this is the Jacobi
+ * (read-old / write-new) form.  L: strict lower CSR (unit diagonal implied).
+ * U: upper incl. diagonal in CSC with the diagonal last in each column. */
+void or_parilu_sweep(int n, const int *rp, const int *ci, const double *a,
+                     const int *lrp, const int *lci, const double *lv_in,
+                     const int *ucp, const int *uri, const double *uv_in,
+                     double *lv_out, double *uv_out) {
+  memcpy(lv_out, lv_in, sizeof(double) * (size_t)lrp[n]);
+  memcpy(uv_out, uv_in, sizeof(double) * (size_t)ucp[n]);
+  for (int i = 0; i < n; i++) {
+    for (int e = rp[i]; e < rp[i + 1]; e++) {
+      int j = ci[e];
+      int m = i < j ? i : j;
+      double s = a[e];
+      int il = lrp[i], iu = ucp[j];
+      int le = lrp[i + 1], ue = ucp[j + 1];
+      while (il < le && iu < ue) {
+        int kl = lci[il], ku = uri[iu];
+        if (kl >= m || ku >= m) break;
+        if (kl == ku) { s -= lv_in[il] * uv_in[iu]; il++; iu++; }
+        else if (kl < ku) il++;
+        else iu++;
+      }
+      if (i > j) {
+        double d = uv_in[ucp[j + 1] - 1];
+        if (fabs(d) < 1e-10) d = d >= 0 ? 1e-10 : -1e-10;
+        double t = s / d;
+        /* locate l_ij */
+        for (int q = lrp[i]; q < le; q++)
+          if (lci[q] == j) { if (isfinite(t)) lv_out[q] = t; break; }
+      } else {
+        if (i == j && fabs(s) < 1e-10) s = s >= 0 ? 1e-10 : -1e-10;
+        for (int q = ucp[j]; q < ue; q++)
+          if (uri[q] == i) { if (isfinite(s)) uv_out[q] = s; break; }
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Krylov solvers                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* Preconditioned CG.  hip/Solver/xsolver_cg.cpp:16-74 (small = 0), which is
+ * the same recurrence as intel/mpisolver-C-v1.cpp:193-272 (cpu_cgm_solve).
+ * Returns 0 on convergence, 1 when maxiter is reached, 2 on a non-finite
+ * residual. */
+int or_cg(int n, const int *rp, const int *ci, const double *v,
+          const double *b, double *x, double tol, double abstol, int maxiter,
+          or_precond_fn M, void *Mctx, double *resvec, int *usediter) {
+  double *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+  double *p = malloc(sizeof(double) * n), *Ap = malloc(sizeof(double) * n);
+  int rc = 1;
+  or_bmax(n, rp, ci, v, b, x, r);
+  apply_M(M, Mctx, n, r, z);
+  double tau = dot(n, r, z);
+  double res0 = sqrt(dot(n, r, r));
+  copy(n, z, p);
+  if (resvec) resvec[0] = res0;
+  *usediter = 0;
+  if (res0 == 0.0 || res0 <= abstol) { rc = 0; goto done; }
+  for (int it = 0; it < maxiter; it++) {
+    double taoo = tau;
+    or_spmv(n, rp, ci, v, p, Ap);
+    double sigma = dot(n, Ap, p);
+    double alpha = tau / sigma;
+    axpy(n, alpha, p, x);
+    axpy(n, -alpha, Ap, r);
+    apply_M(M, Mctx, n, r, z);
+    tau = dot(n, z, r);
+    double beta = tau / taoo;
+    scal(n, beta, p);
+    axpy(n, 1.0, z, p);
+    double resl = sqrt(dot(n, r, r));
+    if (resvec) resvec[it + 1] = resl;
+    *usediter = it + 1;
+    if (!isfinite(resl)) { rc = 2; goto done; }
+    if (resl / res0 < tol || resl <= abstol) { rc = 0; goto done; }
+  }
+done:
+  free(r); free(z); free(p); free(Ap);
+  return rc;
+}
+
+/* Right-preconditioned BiCGSTAB in x-space.  hip/Solver/xsolver_bicgstab.cpp:20-124
+ * with the fused updates of hip/HostVector/DeviceVector/DeviceVector.cpp:55-72
+ * (p = r + omega (p - alpha u); s = r - gamma u; x += gamma z + alpha z1),
+ * small = 0 except the 1e-40 guard on omega's denominator.  In exact arithmetic
+ * the iterates equal intel/mpisolver-C-v1.cpp:365-482 (y-space, x = M^{-1} y). */
+int or_bicgstab(int n, const int *rp, const int *ci, const double *v,
+                const double *b, double *x, double tol, double abstol,
+                int maxiter, or_precond_fn M, void *Mctx, double *resvec,
+                int *usediter) {
+  double *r = calloc(n, sizeof(double)), *rh = calloc(n, sizeof(double));
+  double *p = calloc(n, sizeof(double)), *u = calloc(n, sizeof(double));
+  double *s = calloc(n, sizeof(double)), *vv = calloc(n, sizeof(double));
+  double *z = calloc(n, sizeof(double)), *z1 = calloc(n, sizeof(double));
+  int rc = 1;
+  or_bmax(n, rp, ci, v, b, x, r);
+  double res0 = sqrt(dot(n, r, r));
+  copy(n, r, rh);
+  if (resvec) resvec[0] = res0;
+  *usediter = 0;
+  if (res0 == 0.0 || res0 <= abstol) { rc = 0; goto done; }
+  double alpha = 1.0, beto = 1.0, gama = 1.0;
+  for (int it = 0; it < maxiter; it++) {
+    double beta = dot(n, rh, r);
+    double omega = (beta * gama) / (alpha * beto + 1e-40);
+    beto = beta;
+    for (int i = 0; i < n; i++) p[i] = r[i] + omega * (p[i] - alpha * u[i]);
+    apply_M(M, Mctx, n, p, z);
+    or_spmv(n, rp, ci, v, z, u);
+    double tmp = dot(n, rh, u);
+    gama = beta / tmp;
+    for (int i = 0; i < n; i++) s[i] = r[i] - gama * u[i];
+    apply_M(M, Mctx, n, s, z1);
+    or_spmv(n, rp, ci, v, z1, vv);
+    double t0 = dot(n, s, vv), t1 = dot(n, vv, vv);
+    alpha = t0 / t1;
+    for (int i = 0; i < n; i++) x[i] = x[i] + gama * z[i] + alpha * z1[i];
+    copy(n, s, r);
+    axpy(n, -alpha, vv, r);
+    double resl = sqrt(dot(n, r, r));
+    if (resvec) resvec[it + 1] = resl;
+    *usediter = it + 1;
+    if (!isfinite(resl)) { rc = 2; goto done; }
+    if (resl / res0 < tol || resl <= abstol) { rc = 0; goto done; }
+  }
+done:
+  free(r); free(rh); free(p); free(u); free(s); free(vv); free(z); free(z1);
+  return rc;
+}
+
+/* Right-preconditioned GCR(restart) with modified Gram-Schmidt on c = A u.
+ * hip/Solver/xsolver_igcr.cpp:28-108 (small = 0). */
+int or_gcr(int n, const int *rp, const int *ci, const double *v,
+           const double *b, double *x, double tol, double abstol, int maxiter,
+           int restart, or_precond_fn M, void *Mctx, double *resvec,
+           int *usediter) {
+  if (restart <= 0) restart = maxiter;
+  double *r = malloc(sizeof(double) * n);
+  double *U = malloc(sizeof(double) * (size_t)n * restart);
+  double *C = malloc(sizeof(double) * (size_t)n * restart);
+  int rc = 1;
+  or_bmax(n, rp, ci, v, b, x, r);
+  double res0 = sqrt(dot(n, r, r));
+  if (resvec) resvec[0] = res0;
+  *usediter = 0;
+  if (res0 == 0.0 || res0 <= abstol) { rc = 0; goto done; }
+  int iiter = 0;
+  while (iiter < maxiter) {
+    int k = 0;
+    while (k < restart && iiter < maxiter) {
+      double *uk = U + (size_t)k * n, *ck = C + (size_t)k * n;
+      apply_M(M, Mctx, n, r, uk);
+      or_spmv(n, rp, ci, v, uk, ck);
+      for (int i = 0; i < k; i++) {
+        double a = dot(n, C + (size_t)i * n, ck);
+        axpy(n, -a, C + (size_t)i * n, ck);
+        axpy(n, -a, U + (size_t)i * n, uk);
+      }
+      double t0 = sqrt(dot(n, ck, ck)), t1 = dot(n, ck, r);
+      double inv = 1.0 / t0;
+      scal(n, inv, uk);
+      scal(n, inv, ck);
+      double tmp = t1 / t0;
+      axpy(n, tmp, uk, x);
+      axpy(n, -tmp, ck, r);
+      double rn = sqrt(dot(n, r, r));
+      if (resvec) resvec[iiter + 1] = rn;
+      *usediter = iiter + 1;
+      if (!isfinite(rn)) { rc = 2; goto done; }
+      if (rn / res0 < tol || rn <= abstol) { rc = 0; goto done; }
+      k++; iiter++;
+    }
+  }
+done:
+  free(r); free(U); free(C);
+  return rc;
+}
+
+/* Givens rotation helpers.  intel/mpisolver-C-v1.cpp:1249-1279. */
+static void rot_apply(double *dx, double *dy, double cs, double sn) {
+  double t = cs * *dx + sn * *dy;
+  *dy = -sn * *dx + cs * *dy;
+  *dx = t;
+}
+static void rot_gen(double dx, double dy, double *cs, double *sn) {
+  if (dy == 0.0) { *cs = 1.0; *sn = 0.0; }
+  else if (fabs(dy) > fabs(dx)) {
+    double t = dx / dy;
+    *sn = 1.0 / sqrt(1.0 + t * t);
+    *cs = t * *sn;
+  } else {
+    double t = dy / dx;
+    *cs = 1.0 / sqrt(1.0 + t * t);
+    *sn = t * *cs;
+  }
+}
+/* y = H(0:k,0:k)^{-1} s (H_ii + 1e-20), x += sum_i y_i B_i.
+ * intel/mpisolver-C-v1.cpp:1281-1301 (updatec). */
+static void gm_update(int n, double *x, int k, const double *H, int ldh,
+                      const double *s, const double *B) {
+  double *y = malloc(sizeof(double) * (size_t)(k + 1));
+  for (int i = 0; i < k; i++) y[i] = s[i];
+  for (int i = k - 1; i >= 0; i--) {
+    y[i] = y[i] / (H[i * ldh + i] + 1e-20);
+    for (int j = i - 1; j >= 0; j--) y[j] = y[j] - H[j * ldh + i] * y[i];
+  }
+  for (int i = 0; i < k; i++) axpy(n, y[i], B + (size_t)i * n, x);
+  free(y);
+}
+
+/* GMRES(restart) with MGS Arnoldi and Givens rotations.
+ * flexible = 0: left preconditioning, intel/mpisolver-C-v1.cpp:933-1087
+ *   (residuals are norms of M^{-1}(b - A x)).
+ * flexible = 1: right preconditioning with stored Z_i = M V_i,
+ *   intel/mpisolver-C-v1.cpp:1089-1247 (FGMRES; residuals unpreconditioned).
+ * Restart keeps the reference's rule beta = |s[m]| (:1069).  Unlike the
+ * reference (which calls updatec(phi, i, ...) on convergence and so drops the
+ * newest direction), the converged update uses all i+1 directions.
+ * resvec[0] = res0, resvec[j+1] = |s_{i+1}| after global step j. */
+int or_gmres(int n, const int *rp, const int *ci, const double *v,
+             const double *b, double *x, double tol, int maxiter, int restart,
+             int flexible, or_precond_fn M, void *Mctx, double *resvec,
+             int *usediter) {
+  int m = restart;
+  double *V = calloc((size_t)n * (m + 1), sizeof(double));
+  double *Z = flexible ? calloc((size_t)n * (m + 1), sizeof(double)) : NULL;
+  double *H = calloc((size_t)(m + 1) * m, sizeof(double));
+  double *s = calloc(m + 1, sizeof(double)), *cs = calloc(m + 1, sizeof(double));
+  double *sn = calloc(m + 1, sizeof(double));
+  double *r = malloc(sizeof(double) * n), *w = malloc(sizeof(double) * n);
+  double *t = malloc(sizeof(double) * n);
+  int rc = 1;
+  or_bmax(n, rp, ci, v, b, x, r);
+  if (!flexible) { apply_M(M, Mctx, n, r, t); copy(n, t, r); }
+  double res0 = sqrt(dot(n, r, r));
+  double beta = res0;
+  if (resvec) resvec[0] = res0;
+  *usediter = 0;
+  if (res0 == 0.0) { rc = 0; goto done; }
+  int j = 0;
+  while (j < maxiter) {
+    memset(H, 0, sizeof(double) * (size_t)(m + 1) * m);
+    memset(s, 0, sizeof(double) * (m + 1));
+    memset(cs, 0, sizeof(double) * (m + 1));
+    memset(sn, 0, sizeof(double) * (m + 1));
+    scal(n, 1.0 / (beta + 1e-20), r);
+    copy(n, r, V);
+    s[0] = beta;
+    int i = 0;
+    while (i < m && j < maxiter) {
+      double *vi = V + (size_t)i * n;
+      if (flexible) {
+        double *zi = Z + (size_t)i * n;
+        apply_M(M, Mctx, n, vi, zi);
+        or_spmv(n, rp, ci, v, zi, w);
+      } else {
+        or_spmv(n, rp, ci, v, vi, t);
+        apply_M(M, Mctx, n, t, w);
+      }
+      for (int k = 0; k <= i; k++) {
+        double h = dot(n, w, V + (size_t)k * n);
+        H[k * m + i] = h;
+        axpy(n, -h, V + (size_t)k * n, w);
+      }
+      double h1 = sqrt(dot(n, w, w));
+      H[(i + 1) * m + i] = h1;
+      scal(n, 1.0 / (h1 + 1e-20), w);
+      copy(n, w, V + (size_t)(i + 1) * n);
+      for (int k = 0; k < i; k++) rot_apply(&H[k * m + i], &H[(k + 1) * m + i], cs[k], sn[k]);
+      rot_gen(H[i * m + i], H[(i + 1) * m + i], &cs[i], &sn[i]);
+      rot_apply(&H[i * m + i], &H[(i + 1) * m + i], cs[i], sn[i]);
+      rot_apply(&s[i], &s[i + 1], cs[i], sn[i]);
+      double err = fabs(s[i + 1]);
+      if (resvec) resvec[j + 1] = err;
+      *usediter = j + 1;
+      if (!isfinite(err)) { rc = 2; goto done; }
+      if (err / (res0 + 1e-20) < tol) {
+        gm_update(n, x, i + 1, H, m, s, flexible ? Z : V);
+        rc = 0;
+        goto done;
+      }
+      i++; j++;
+    }
+    gm_update(n, x, m, H, m, s, flexible ? Z : V);
+    or_bmax(n, rp, ci, v, b, x, r);
+    if (!flexible) { apply_M(M, Mctx, n, r, t); copy(n, t, r); }
+    beta = fabs(s[m]);
+  }
+done:
+  free(V); free(Z); free(H); free(s); free(cs); free(sn); free(r); free(w); free(t);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Driver: build the preconditioner named by pc and run the solver named by  */
+/* ksp, the way hip/xsolver_c.cpp:150-292 wires them (Precond_jacobi,        */
+/* Precond_ilu0 = seqilu0 + LUsolve, Precond_ilup = seqilut + LUsolve).      */
+/* Returns the solver status, or -1 for an unknown name / setup failure.     */
+/* ------------------------------------------------------------------------ */
+static int streq(const char *a, const char *b) { return strcmp(a, b) == 0; }
+
+int or_solve(const char *ksp, const char *pc, int n, const int *rp,
+             const int *ci, const double *v, const double *b, double *x,
+             double tol, double abstol, int maxiter, int restart,
+             double ilut_tol, int ilut_fill, double *resvec, int *usediter) {
+  or_precond_fn M = NULL;
+  void *Mctx = NULL;
+  or_jac_ctx_t jac;
+  or_lu_t lu;
+  double *dinv = NULL, *luval = NULL, *lv = NULL, *uv = NULL, *diag = NULL;
+  int *uptr = NULL, *lrp = NULL, *lci = NULL, *urp = NULL, *uci = NULL;
+  int *lucols = NULL, *lurows = NULL;
+  int rc = -1;
+  if (streq(pc, "none")) {
+  } else if (streq(pc, "jacobi")) {
+    dinv = malloc(sizeof(double) * n);
+    if (or_jacobi_setup(n, rp, ci, v, 0.0, dinv)) goto out;
+    jac.n = n; jac.dinv = dinv;
+    M = or_jacobi_apply; Mctx = &jac;
+  } else if (streq(pc, "ilu0") || streq(pc, "ilut")) {
+    int cap, nnz = rp[n];
+    const int *frp, *fci;
+    uptr = malloc(sizeof(int) * (n + 1));
+    dinv = malloc(sizeof(double) * n);
+    if (streq(pc, "ilu0")) {
+      cap = nnz;
+      luval = malloc(sizeof(double) * nnz);
+      if (or_ilu0(n, rp, ci, v, luval, uptr)) goto out;
+      frp = rp; fci = ci;
+    } else {
+      cap = (2 * ilut_fill + 2) * n;
+      luval = malloc(sizeof(double) * (size_t)cap);
+      lucols = malloc(sizeof(int) * (size_t)cap);
+      lurows = malloc(sizeof(int) * (n + 1));
+      diag = malloc(sizeof(double) * n);
+      if (or_ilut(n, rp, ci, v, ilut_fill, ilut_tol, cap, luval, lucols, lurows, uptr, diag)) goto out;
+      frp = lurows; fci = lucols;
+    }
+    lrp = malloc(sizeof(int) * (n + 1)); urp = malloc(sizeof(int) * (n + 1));
+    lci = malloc(sizeof(int) * (size_t)cap); uci = malloc(sizeof(int) * (size_t)cap);
+    lv = malloc(sizeof(double) * (size_t)cap); uv = malloc(sizeof(double) * (size_t)cap);
+    or_ilu_split(n, frp, fci, luval, uptr, lrp, lci, lv, urp, uci, uv, dinv);
+    lu.n = n; lu.lrp = lrp; lu.lci = lci; lu.lv = lv;
+    lu.urp = urp; lu.uci = uci; lu.uv = uv; lu.dinv = dinv;
+    M = or_lusolve; Mctx = &lu;
+  } else {
+    goto out;
+  }
+  if (streq(ksp, "cg")) rc = or_cg(n, rp, ci, v, b, x, tol, abstol, maxiter, M, Mctx, resvec, usediter);
+  else if (streq(ksp, "bicgstab")) rc = or_bicgstab(n, rp, ci, v, b, x, tol, abstol, maxiter, M, Mctx, resvec, usediter);
+  else if (streq(ksp, "gcr")) rc = or_gcr(n, rp, ci, v, b, x, tol, abstol, maxiter, restart, M, Mctx, resvec, usediter);
+  else if (streq(ksp, "gmres")) rc = or_gmres(n, rp, ci, v, b, x, tol, maxiter, restart > 0 ? restart : 30, 0, M, Mctx, resvec, usediter);
+  else if (streq(ksp, "fgmres")) rc = or_gmres(n, rp, ci, v, b, x, tol, maxiter, restart > 0 ? restart : 30, 1, M, Mctx, resvec, usediter);
+out:
+  free(dinv); free(luval); free(lv); free(uv); free(diag); free(uptr);
+  free(lrp); free(lci); free(urp); free(uci); free(lucols); free(lurows);
+  return rc;
+}

@@ -1,0 +1,159 @@
+"""GPU parity of the kernels through the C-ABI (xs_spmv, xs_bmax, xs_trsv,
+ParILU set-up) against the oracle on the same seeded inputs."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import random_sparse, rel_err, say, split_factors
+from xsolver_amd import XSolver, gen
+
+pytestmark = pytest.mark.gpu
+
+MATS = {
+    "poisson2d_64": lambda: gen.poisson2d(64),
+    "convdiff3d_24": lambda: gen.convdiff3d(24),
+    "lognormal27_14": lambda: gen.lognormal27(14, seed=2),
+    "grf2d_48": lambda: gen.vardiff2d(gen.grf_kappa(48, 0.1, 4)),
+    "random_3000": lambda: random_sparse(3000, 6, 21),
+    "longrows_2500": lambda: random_sparse(2500, 4, 22, long_rows=[(7, 2400), (1000, 5000 // 3), (2499, 2300)]),
+}
+
+
+def torch_stream():
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def dev_tensor(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+@pytest.mark.parametrize("ell", [0, 1], ids=["csr", "ell"])
+@pytest.mark.parametrize("name", list(MATS))
+def test_spmv_and_bmax_parity(dev, name, ell):
+    import torch
+    A = MATS[name]()
+    n = len(A[0]) - 1
+    say(f"[spmv] {name} ell={ell} n={n} nnz={len(A[2])}")
+    S = XSolver()
+    S.mat_setup(*A, advanced_pam=(0, ell, 1))
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, n)
+    b = rng.uniform(-1, 1, n)
+    xd, bd = dev_tensor(x, dev), dev_tensor(b, dev)
+    yd = torch.empty(n, dtype=torch.float64, device=dev)
+    S.spmv(xd, yd, torch_stream())
+    torch.cuda.synchronize()
+    ref = oracle.spmv(A, x)
+    scale = np.abs(A[2]).max() * np.abs(x).max() * np.diff(A[0]).max()
+    assert np.abs(yd.cpu().numpy() - ref).max() <= 1e-14 * scale
+    S.bmax(bd, xd, yd, torch_stream())
+    torch.cuda.synchronize()
+    assert np.abs(yd.cpu().numpy() - oracle.bmax(A, b, x)).max() <= 1e-14 * scale
+    S.close()
+
+
+def test_spmv_full_size_convdiff128(dev):
+    import torch
+    A = gen.convdiff3d(128)
+    n = len(A[0]) - 1
+    say(f"[spmv] convdiff3d_128 n={n} nnz={len(A[2])}")
+    x = np.random.default_rng(1).uniform(-1, 1, n)
+    ref = gen.spmv(A, x)
+    for ell in (0, 1):
+        S = XSolver()
+        S.mat_setup(*A, advanced_pam=(0, ell, 1))
+        yd = torch.empty(n, dtype=torch.float64, device=dev)
+        S.spmv(dev_tensor(x, dev), yd, torch_stream())
+        torch.cuda.synchronize()
+        assert rel_err(yd.cpu().numpy(), ref) < 1e-14
+        S.close()
+
+
+TRSV_MATS = {
+    "convdiff3d_20": lambda: gen.convdiff3d(20),     # in-wave chains (scan path)
+    "poisson2d_16": lambda: gen.poisson2d(16),       # in-wave deps r-1, r-16 (loop path)
+    "lognormal27_12": lambda: gen.lognormal27(12, seed=2),
+    "random_4000": lambda: random_sparse(4000, 7, 23),
+    "grf2d_40": lambda: gen.vardiff2d(gen.grf_kappa(40, 0.1, 4)),
+}
+
+
+@pytest.mark.parametrize("name", list(TRSV_MATS))
+def test_trsv_parity(dev, name):
+    import torch
+    A = TRSV_MATS[name]()
+    n = len(A[0]) - 1
+    say(f"[trsv] {name} n={n}")
+    S = XSolver()
+    S.mat_setup(*A)
+    S.ksp_settype("bicgstab")
+    S.pc_settype("ilu0")
+    S.pc_setup()
+    rp, ci, lu, dpos = S.factors(n, len(A[2]))
+    L, U, dinv = split_factors(rp, ci, lu, dpos)
+    b = np.random.default_rng(5).uniform(-1, 1, n)
+    bd = dev_tensor(b, dev)
+    xd = torch.empty(n, dtype=torch.float64, device=dev)
+    for rep in range(3):  # repeated calls reuse the self-resetting tickets
+        S.trsv(0, bd, xd, torch_stream())
+        torch.cuda.synchronize()
+        ref = oracle.lower_solve(L, b)
+        assert rel_err(xd.cpu().numpy(), ref) < 1e-12
+        S.trsv(1, bd, xd, torch_stream())
+        torch.cuda.synchronize()
+        ref = oracle.upper_solve(U, dinv, b)
+        assert rel_err(xd.cpu().numpy(), ref) < 1e-12
+    yd = torch.empty(n, dtype=torch.float64, device=dev)
+    S.pc_apply(bd, yd, torch_stream())
+    torch.cuda.synchronize()
+    ref = oracle.upper_solve(U, dinv, oracle.lower_solve(L, b))
+    assert rel_err(yd.cpu().numpy(), ref) < 1e-12
+    S.close()
+
+
+@pytest.mark.parametrize("name", ["convdiff3d_20", "lognormal27_12", "grf2d_40", "random_4000"])
+def test_parilu_converges_to_ilu0(dev, name):
+    A = TRSV_MATS[name]()
+    n = len(A[0]) - 1
+    say(f"[parilu] {name} n={n}")
+    ref_lu, uptr = oracle.ilu0(A)
+    for mode, sweeps, tol in (("sync", 60, 1e-11), ("async", 20, 1e-5)):
+        S = XSolver()
+        S.mat_setup(*A)
+        S.ksp_settype("cg")
+        S.pc_settype("ilu0")
+        S.set_option("parilu_async", mode == "async")
+        S.set_option("parilu_sweeps", sweeps)
+        S.pc_setup()
+        rp, ci, lu, dpos = S.factors(n, len(A[2]))
+        assert np.array_equal(dpos, uptr)
+        err = rel_err(lu, ref_lu)
+        say(f"   {mode} sweeps={sweeps} max rel diff to ILU(0) = {err:.3e}")
+        assert err < tol
+        S.close()
+
+
+def test_parilu_sync_sweep_matches_oracle_sweep(dev):
+    # one and three deterministic sweeps from the reference's initial guess
+    A = gen.convdiff3d(9)
+    n = len(A[0]) - 1
+    from tests.test_oracle import _split_for_sweep
+    lu0, uptr = oracle.ilu0(A)
+    L0, U0, _, _ = _split_for_sweep(A, A[2], uptr)
+    l, u = L0[2], U0[2]
+    for sweeps in (1, 3):
+        for _ in range(sweeps if sweeps == 1 else 2):
+            l, u = oracle.parilu_sweep(A, (L0[0], L0[1], l), (U0[0], U0[1], u), n)
+        S = XSolver()
+        S.mat_setup(*A)
+        S.ksp_settype("cg")
+        S.pc_settype("ilu0")
+        S.set_option("parilu_async", 0)
+        S.set_option("parilu_sweeps", sweeps)
+        S.pc_setup()
+        rp, ci, lu, dpos = S.factors(n, len(A[2]))
+        Lg, Ug, _, _ = _split_for_sweep(A, lu, uptr)
+        assert rel_err(Lg[2], l) < 1e-14 and rel_err(Ug[2], u) < 1e-14
+        S.close()

@@ -1,0 +1,192 @@
+"""GPU solver parity through the C-ABI against the oracle (same seeded
+matrices, same x0 = 0): iterations within +-2 (HIP conventions: iterations
+performed, absolute resvec), true FP64 relative residual below rtol, and the
+solution close to the oracle's solution."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+import xsolver_amd
+from tests.helpers import rel_err, say, true_relres
+from xsolver_amd import XSolver, XSolverError, gen
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_solve(A, b, ksp, pc, tol, maxiter, restart=0, ell=0, **opts):
+    S = XSolver()
+    S.mat_setup(*A, advanced_pam=(0, ell, 1))
+    S.ksp_settype(ksp)
+    S.ksp_setoption(tol, maxiter, restart)
+    S.pc_settype(pc)
+    for k, v in opts.items():
+        S.set_option(k, v)
+    x, st = S.solve(b)
+    out = dict(x=x, st=st, it=S.iterations, res=S.resvec(), S=S)
+    return out
+
+
+CASES = [
+    # name, builder, ksp, pc, tol, maxiter, x* seed
+    ("poisson2d_256_cg_jacobi", lambda: gen.poisson2d(256), "cg", "jacobi", 1e-8, 5000, 0),
+    ("convdiff3d_32_bicgstab_ilu0", lambda: gen.convdiff3d(32), "bicgstab", "ilu0", 1e-10, 1000, 1),
+    ("convdiff3d_32_bicgstab_jacobi", lambda: gen.convdiff3d(32), "bicgstab", "jacobi", 1e-10, 2000, 1),
+    ("grf2d_64_cg_ilu0", lambda: gen.vardiff2d(gen.grf_kappa(64, 0.1, 4)), "cg", "ilu0", 1e-10, 2000, 4),
+    ("lognormal27_16_bicgstab_ilu0", lambda: gen.lognormal27(16, seed=2), "bicgstab", "ilu0", 1e-10, 500, 2),
+    ("poisson2d_64_cg_none", lambda: gen.poisson2d(64), "cg", "none", 1e-9, 2000, 0),
+]
+
+
+@pytest.mark.parametrize("ell", [0, 1], ids=["csr", "ell"])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_solve_parity(dev, case, ell):
+    name, build, ksp, pc, tol, maxiter, seed = case
+    A = build()
+    xs, b = gen.manufactured(A, seed)
+    say(f"[solve] {name} ell={ell} n={len(xs)}")
+    xo, sto, ito, reso = oracle.solve(A, b, ksp, pc, tol=tol, maxiter=maxiter)
+    kw = {"parilu_async": 0, "parilu_sweeps": 60} if pc == "ilu0" else {}
+    g = gpu_solve(A, b, ksp, pc, tol, maxiter, ell=ell, **kw)
+    say(f"   oracle it={ito} gpu it={g['it']} status={g['st']} "
+        f"x-diff={rel_err(g['x'], xo):.2e} relres={true_relres(A, b, g['x']):.2e}")
+    assert sto == 0 and g["st"] == xsolver_amd.XS_OK
+    assert abs(g["it"] - ito) <= 2
+    assert true_relres(A, b, g["x"]) < tol * 1.05
+    assert np.linalg.norm(g["x"] - xo) / np.linalg.norm(xo) < 50 * tol
+    res = g["res"]
+    assert len(res) == g["it"] + 1
+    assert res[0] == pytest.approx(np.linalg.norm(b), rel=1e-12)
+    assert res[-1] / res[0] < tol
+    n = min(len(res), len(reso), 20)
+    assert np.allclose(res[:n], reso[:n], rtol=1e-6)
+    g["S"].close()
+
+
+def test_parilu_async_reference_default(dev):
+    # the reference's own setting: 20 asynchronous in-place sweeps
+    A = gen.convdiff3d(32)
+    xs, b = gen.manufactured(A, 1)
+    xo, sto, ito, _ = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=1000)
+    g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000)
+    say(f"[async] oracle it={ito} gpu it={g['it']}")
+    assert g["st"] == 0 and abs(g["it"] - ito) <= 2
+    assert true_relres(A, b, g["x"]) < 1.05e-10
+    g["S"].close()
+
+
+def test_device_resident_solve_matches_host_solve(dev):
+    import torch
+    A = gen.convdiff3d(24)
+    xs, b = gen.manufactured(A, 1)
+    g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 500, parilu_async=0, parilu_sweeps=40)
+    S = XSolver()
+    rp, ci, v = (torch.from_numpy(a).to(dev) for a in A)
+    S.mat_setup_device(rp, ci, v)
+    S.ksp_settype("bicgstab"); S.ksp_setoption(1e-10, 500); S.pc_settype("ilu0")
+    S.set_option("parilu_async", 0); S.set_option("parilu_sweeps", 40)
+    bd = torch.from_numpy(b).to(dev)
+    xd = torch.zeros_like(bd)
+    torch.cuda.synchronize()
+    st = S.solve_device(bd, xd)
+    assert st == 0 and S.iterations == g["it"]
+    assert np.array_equal(xd.cpu().numpy(), g["x"])  # same kernels, same order
+    S.close(); g["S"].close()
+
+
+def test_warm_start_and_repeat(dev):
+    A = gen.poisson2d(64)
+    xs, b = gen.manufactured(A, 0)
+    S = XSolver()
+    S.mat_setup(*A); S.ksp_settype("cg"); S.ksp_setoption(1e-10, 3000); S.pc_settype("jacobi")
+    x1, st1 = S.solve(b)
+    it1 = S.iterations
+    x2, st2 = S.solve(b)               # repeat: identical
+    assert np.array_equal(x1, x2) and S.iterations == it1
+    x3, st3 = S.solve(b, x0=x1)        # warm start from the solution
+    assert S.iterations <= 1
+    S.close()
+
+
+def test_edge_cases(dev):
+    A = gen.poisson2d(16)
+    n = 256
+    S = XSolver()
+    S.mat_setup(*A); S.ksp_settype("bicgstab"); S.ksp_setoption(1e-8, 100); S.pc_settype("jacobi")
+    x, st = S.solve(np.zeros(n))                       # zero rhs: res0 = 0
+    assert st == 0 and S.iterations == 0 and np.all(x == 0)
+    S.ksp_setoption(1e-30, 5)                           # maxiter exhaustion
+    x, st = S.solve(np.ones(n))
+    assert st == xsolver_amd.XS_NOT_CONVERGED and S.iterations == 5 and len(S.resvec()) == 6
+    with pytest.raises(XSolverError):
+        S.ksp_settype("minres")
+    with pytest.raises(XSolverError) as e:
+        S.pc_settype("ic0")
+    assert e.value.status == xsolver_amd.XS_ERR_UNSUPPORTED
+    S.close()
+    # zero diagonal -> Jacobi pivot error (reference exits, HostVector.cpp:88-92)
+    rp, ci, v = A
+    v = v.copy(); v[rp[3] + 1] = 0.0
+    S = XSolver()
+    S.mat_setup(rp, ci, v); S.ksp_settype("cg"); S.ksp_setoption(1e-8, 10); S.pc_settype("jacobi")
+    with pytest.raises(XSolverError) as e:
+        S.solve(np.ones(n))
+    assert e.value.status == xsolver_amd.XS_ERR_PIVOT
+    S.close()
+    # MPI distribution and CPU backend are not provided
+    S = XSolver()
+    with pytest.raises(XSolverError):
+        S.mat_setup(*A, advanced_pam=(1, 0, 1))
+    with pytest.raises(XSolverError):
+        S.mat_setup(*A, advanced_pam=(0, 0, 0))
+    S.close()
+
+
+def test_one_based_and_unsorted_input(dev):
+    A = gen.convdiff3d(10)
+    rp, ci, v = A
+    xs, b = gen.manufactured(A, 1)
+    ref = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 300, parilu_async=0)
+    # 1-based (hip/HostMatrix/HostMatrixCSR.cpp:119-128 accepts both)
+    g1 = gpu_solve((rp + 1, ci + 1, v), b, "bicgstab", "ilu0", 1e-10, 300, parilu_async=0)
+    assert np.array_equal(g1["x"], ref["x"])
+    # unsorted columns within rows are sorted on upload
+    rng = np.random.default_rng(0)
+    ci2, v2 = ci.copy(), v.copy()
+    for i in range(len(rp) - 1):
+        p = rng.permutation(rp[i + 1] - rp[i]) + rp[i]
+        ci2[rp[i]:rp[i + 1]] = ci[p]; v2[rp[i]:rp[i + 1]] = v[p]
+    g2 = gpu_solve((rp, ci2, v2), b, "bicgstab", "ilu0", 1e-10, 300, parilu_async=0)
+    assert np.array_equal(g2["x"], ref["x"])
+    for g in (ref, g1, g2):
+        g["S"].close()
+
+
+def test_legacy_singleton_api(dev):
+    """The reference call sequence (hip/solve_base.h:4-24) through the C symbols."""
+    L = xsolver_amd.lib()
+    A = gen.convdiff3d(16)
+    rp, ci, v = (np.ascontiguousarray(a) for a in A)
+    n = len(rp) - 1
+    xs, b = gen.manufactured(A, 1)
+    pam = np.array([0, 0, 1], np.int32)
+    used = ctypes.c_int(-1)
+    resvec = np.zeros(301)
+    sol = np.zeros(n)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    L.xsolver_mat_setup(n, P(rp), P(ci), P(v), P(pam))
+    L.xsolver_ksp_settype(b"bicgstab")
+    L.xsolver_ksp_setoption(1e-10, 300, ctypes.byref(used), P(resvec), 0)
+    L.xsolver_pc_settype(b"Ilu0")
+    L.xsolver_solve(P(b), P(sol))
+    assert L.xsolver_last_error() == 0
+    xo, sto, ito, reso = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=300)
+    assert abs(used.value - ito) <= 2
+    assert used.value == L.xsolver_get_iterations()
+    assert resvec[0] == pytest.approx(np.linalg.norm(b))
+    assert resvec[used.value] / resvec[0] < 1e-10
+    assert true_relres(A, b, sol) < 1.05e-10
+    L.xsolver_free()
+    L.xsolver_ksp_settype(b"cg")
+    assert L.xsolver_last_error() == xsolver_amd.XS_ERR_STATE

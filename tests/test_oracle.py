@@ -1,0 +1,294 @@
+"""The oracle (oracle/xs_oracle.c) pinned before it is trusted.
+
+* ILU(0) and ILUT: bit-exact against golden vectors written by the
+  reference's own hip/Precond/ilut_diag_in_csr.cpp (tests/golden/).
+* ILU(0): the defining invariant (LU)_ij = a_ij on pattern(A), and the
+  closed form "ILU(0) of a tridiagonal matrix is its exact LU".
+* LU apply: (LU) y = x checked with dense algebra.
+* SpMV/bmAx/transpose: brute force against dense products.
+* Krylov loops: CG iteration counts equal scipy's textbook PCG; CG finite
+  termination; GMRES/FGMRES/GCR residual norms equal the minimal residuals of
+  a dense least-squares problem over the Krylov space; BiCGSTAB residual
+  history equals scipy's bicgstab (same right-preconditioned recurrence) and
+  the recursive residual matches the true residual.
+* ParILU sweep: the ILU(0) factors are its fixed point; repeated synchronous
+  sweeps converge to them.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+import oracle
+from xsolver_amd import gen
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "ilu_golden.npz")
+
+
+def dense(A):
+    rp, ci, v = A
+    n = len(rp) - 1
+    return sp.csr_matrix((v, ci, rp), shape=(n, n)).toarray()
+
+
+def golden_names():
+    G = np.load(GOLD)
+    return sorted({k.split("/")[0] for k in G.files})
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_ilu0_matches_reference_golden(name):
+    G = np.load(GOLD)
+    A = (G[name + "/rowptr"], G[name + "/colidx"], G[name + "/val"])
+    lu, uptr = oracle.ilu0(A)
+    assert np.array_equal(lu, G[name + "/ilu0/luval"])
+    assert np.array_equal(uptr, G[name + "/ilu0/uptr"])
+
+
+@pytest.mark.parametrize("name", golden_names())
+@pytest.mark.parametrize("lfil,tol", [(1, 1e-4), (2, 1e-2), (5, 1e-4), (3, 0.1)])
+def test_ilut_matches_reference_golden(name, lfil, tol):
+    G = np.load(GOLD)
+    A = (G[name + "/rowptr"], G[name + "/colidx"], G[name + "/val"])
+    k = f"{name}/ilut_{lfil}_{tol:g}"
+    lr, lc, lv, up, d = oracle.ilut(A, lfil, tol)
+    assert np.array_equal(lr, G[k + "/lurows"])
+    assert np.array_equal(lc, G[k + "/lucols"])
+    assert np.array_equal(lv, G[k + "/luval"])
+    assert np.array_equal(up, G[k + "/uptr"])
+    assert np.array_equal(d, G[k + "/diag"])
+
+
+def split_dense(A, lu, uptr):
+    rp, ci, _ = A
+    n = len(rp) - 1
+    L = np.eye(n)
+    U = np.zeros((n, n))
+    for i in range(n):
+        for e in range(rp[i], rp[i + 1]):
+            if e < uptr[i]:
+                L[i, ci[e]] = lu[e]
+            else:
+                U[i, ci[e]] = lu[e]
+    return L, U
+
+
+@pytest.mark.parametrize("A", [gen.convdiff3d(5), gen.lognormal27(4), gen.vardiff2d(gen.grf_kappa(9, 0.3, 1))],
+                         ids=["convdiff", "lognormal27", "grf"])
+def test_ilu0_pattern_invariant(A):
+    lu, uptr = oracle.ilu0(A)
+    L, U = split_dense(A, lu, uptr)
+    P = L @ U
+    D = dense(A)
+    rp, ci, _ = A
+    mask = np.zeros_like(D, bool)
+    for i in range(len(rp) - 1):
+        mask[i, ci[rp[i]:rp[i + 1]]] = True
+    assert np.abs(P[mask] - D[mask]).max() <= 1e-12 * np.abs(D).max()
+
+
+def test_ilu0_tridiagonal_is_exact_lu():
+    n = 30
+    rng = np.random.default_rng(5)
+    T = np.diag(4 + rng.random(n)) + np.diag(-rng.random(n - 1), 1) + np.diag(-rng.random(n - 1), -1)
+    S = sp.csr_matrix(T)
+    S.sort_indices()
+    A = (S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data)
+    lu, uptr = oracle.ilu0(A)
+    L, U = split_dense(A, lu, uptr)
+    P, Ld, Ud = sla.lu(T)
+    assert np.allclose(P, np.eye(n))
+    assert np.abs(L - Ld).max() < 1e-14 and np.abs(U - Ud).max() < 1e-13
+
+
+def test_ilu_split_and_lusolve_dense():
+    A = gen.convdiff3d(5)
+    lu, uptr = oracle.ilu0(A)
+    Lf, Uf, dinv = oracle.ilu_split(A[0], A[1], lu, uptr)
+    L, U = split_dense(A, lu, uptr)
+    x = np.random.default_rng(1).standard_normal(len(dinv))
+    y = oracle.upper_solve((Uf[0], Uf[1], Uf[2] / dinv[np.repeat(np.arange(len(dinv)), np.diff(Uf[0]))]),
+                           dinv, oracle.lower_solve(Lf, x))
+    assert np.allclose(L @ U @ y, x, rtol=0, atol=1e-12)
+    # the reference's apply (forward, scale, backward with pre-scaled U)
+    xs, st, it, _ = oracle.solve(A, L @ U @ x, "gmres", "ilu0", tol=1e-14, maxiter=1, restart=1)
+    assert st in (0, 1)
+
+
+def test_spmv_bmax_transpose_bruteforce():
+    for A in (gen.convdiff3d(6), gen.lognormal27(4), gen.poisson2d(9)):
+        D = dense(A)
+        x = np.random.default_rng(2).standard_normal(D.shape[0])
+        b = np.random.default_rng(3).standard_normal(D.shape[0])
+        assert np.allclose(oracle.spmv(A, x), D @ x, rtol=0, atol=1e-12)
+        assert np.allclose(oracle.bmax(A, b, x), b - D @ x, rtol=0, atol=1e-12)
+        assert np.allclose(oracle.spmv_t(A, x), D.T @ x, rtol=0, atol=1e-12)
+
+
+def test_spmv_diagonal_added_last():
+    # intel/HostMatrix.h:83-103 sums off-diagonals first and adds a_ii x_i last:
+    # with x = (1, 1e16, -1e16) the order is visible in floating point:
+    # storage order gives (1 + 1e16) - 1e16 = 0, the reference's order 1.
+    rp = np.array([0, 3, 4, 5], np.int32)
+    ci = np.array([0, 1, 2, 1, 2], np.int32)
+    v = np.array([1.0, 1.0, 1.0, 1.0, 1.0])
+    x = np.array([1.0, 1e16, -1e16])
+    y = oracle.spmv((rp, ci, v), x)
+    assert y[0] == 1.0
+
+
+def test_cg_iterations_match_scipy_and_jacobi():
+    A = gen.poisson2d(48)
+    xs, b = gen.manufactured(A, 0)
+    x, st, it, res = oracle.solve(A, b, "cg", "jacobi", tol=1e-8, maxiter=2000)
+    rp, ci, v = A
+    S = sp.csr_matrix((v, ci, rp))
+    cnt = [0]
+    xx, info = spl.cg(S, b, rtol=1e-8, atol=0, maxiter=2000, M=sp.diags(1 / S.diagonal()),
+                      callback=lambda _: cnt.__setitem__(0, cnt[0] + 1))
+    assert st == 0 and info == 0
+    assert it == cnt[0]
+    assert np.linalg.norm(x - xx) / np.linalg.norm(xx) < 1e-9
+    assert res[0] == pytest.approx(np.linalg.norm(b))
+    assert res[-1] / res[0] < 1e-8 <= res[-2] / res[0]
+
+
+def test_cg_finite_termination_and_diagonal():
+    n = 12
+    rng = np.random.default_rng(7)
+    Q = rng.standard_normal((n, n))
+    D = Q @ Q.T + n * np.eye(n)
+    S = sp.csr_matrix(D)
+    A = (S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data)
+    b = rng.standard_normal(n)
+    x, st, it, _ = oracle.solve(A, b, "cg", "none", tol=1e-12, maxiter=50)
+    assert st == 0 and it <= n + 2
+    assert np.allclose(D @ x, b, atol=1e-9)
+    Dg = sp.diags(rng.random(n) + 1).tocsr()
+    Ad = (Dg.indptr.astype(np.int32), Dg.indices.astype(np.int32), Dg.data)
+    x, st, it, _ = oracle.solve(Ad, b, "cg", "jacobi", tol=1e-12, maxiter=5)
+    assert st == 0 and it == 1
+
+
+def krylov_min_residuals(Dm, b, M, k, left):
+    """min ||r|| over x in K_k (dense, via least squares)."""
+    n = len(b)
+    if left:
+        B = M @ Dm
+        r0 = M @ b
+        K = [r0]
+        for _ in range(k - 1):
+            K.append(B @ K[-1])
+        K = np.array(K).T
+        Q, _ = np.linalg.qr(K)
+        y, *_ = np.linalg.lstsq(B @ Q, r0, rcond=None)
+        return np.linalg.norm(r0 - B @ Q @ y)
+    B = Dm @ M
+    K = [b]
+    for _ in range(k - 1):
+        K.append(B @ K[-1])
+    K = np.array(K).T
+    Q, _ = np.linalg.qr(K)
+    y, *_ = np.linalg.lstsq(B @ Q, b, rcond=None)
+    return np.linalg.norm(b - B @ Q @ y)
+
+
+@pytest.mark.parametrize("ksp,left", [("gmres", True), ("fgmres", False), ("gcr", False)])
+def test_minimal_residual_methods_match_dense_lstsq(ksp, left):
+    A = gen.convdiff3d(4)
+    Dm = dense(A)
+    n = Dm.shape[0]
+    b = np.random.default_rng(4).standard_normal(n)
+    M = np.diag(1 / np.diag(Dm))
+    x, st, it, res = oracle.solve(A, b, ksp, "jacobi", tol=1e-13, maxiter=25, restart=40)
+    for k in range(1, min(it, 12) + 1):
+        ref = krylov_min_residuals(Dm, b, M, k, left)
+        assert res[k] == pytest.approx(ref, rel=1e-7, abs=1e-13 * res[0])
+
+
+def test_gmres_restart_reaches_tolerance_true_residual():
+    A = gen.convdiff3d(8)
+    xs, b = gen.manufactured(A, 1)
+    x, st, it, res = oracle.solve(A, b, "fgmres", "ilu0", tol=1e-10, maxiter=200, restart=5)
+    assert st == 0
+    r = b - gen.spmv(A, x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 2e-10
+    x2, st2, it2, res2 = oracle.solve(A, b, "gmres", "jacobi", tol=1e-10, maxiter=400, restart=7)
+    assert st2 == 0
+    assert np.linalg.norm(x2 - xs) / np.linalg.norm(xs) < 1e-7
+
+
+def test_bicgstab_matches_scipy_and_true_residual():
+    A = gen.convdiff3d(10)
+    xs, b = gen.manufactured(A, 1)
+    rp, ci, v = A
+    S = sp.csr_matrix((v, ci, rp))
+    x, st, it, res = oracle.solve(A, b, "bicgstab", "jacobi", tol=1e-10, maxiter=500)
+    cnt = [0]
+    xx, info = spl.bicgstab(S, b, rtol=1e-10, atol=0, maxiter=500, M=sp.diags(1 / S.diagonal()),
+                            callback=lambda _: cnt.__setitem__(0, cnt[0] + 1))
+    assert st == 0 and info == 0
+    assert abs(it - cnt[0]) <= 1
+    assert np.linalg.norm(x - xx) / np.linalg.norm(xx) < 1e-8
+    true = np.linalg.norm(b - S @ x)
+    assert true == pytest.approx(res[-1], rel=1e-3, abs=1e-12 * res[0])
+
+
+def test_bicgstab_ilu0_converges_fewer_iterations():
+    A = gen.convdiff3d(12)
+    xs, b = gen.manufactured(A, 1)
+    _, st1, it1, _ = oracle.solve(A, b, "bicgstab", "jacobi", tol=1e-10, maxiter=500)
+    x, st2, it2, _ = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=500)
+    assert st1 == 0 and st2 == 0 and it2 < it1
+    assert np.linalg.norm(b - gen.spmv(A, x)) / np.linalg.norm(b) < 1e-9
+
+
+def test_parilu_fixed_point_is_ilu0():
+    A = gen.convdiff3d(5)
+    rp, ci, a = A
+    n = len(rp) - 1
+    lu, uptr = oracle.ilu0(A)
+    L, Uc, lpos, upos_map = _split_for_sweep(A, lu, uptr)
+    lo, uo = oracle.parilu_sweep(A, L, Uc, n)
+    assert np.allclose(lo, L[2], rtol=1e-13, atol=1e-15)
+    assert np.allclose(uo, Uc[2], rtol=1e-13, atol=1e-15)
+    # from the reference's initial guess (L = strict lower of A, U = upper of A)
+    L0, U0, _, _ = _split_for_sweep(A, a, uptr)
+    l, u = L0[2], U0[2]
+    for _ in range(40):
+        l, u = oracle.parilu_sweep(A, (L0[0], L0[1], l), (U0[0], U0[1], u), n)
+    assert np.abs(l - L[2]).max() < 1e-12 and np.abs(u - Uc[2]).max() < 1e-12
+
+
+def _split_for_sweep(A, vals, uptr):
+    rp, ci, _ = A
+    n = len(rp) - 1
+    lrp = [0]; lci = []; lv = []
+    cols = [[] for _ in range(n)]
+    for i in range(n):
+        for e in range(rp[i], rp[i + 1]):
+            if ci[e] < i:
+                lci.append(ci[e]); lv.append(vals[e])
+            else:
+                cols[ci[e]].append((i, vals[e]))
+        lrp.append(len(lci))
+    ucp = [0]; uri = []; uv = []
+    for j in range(n):
+        for (i, val) in cols[j]:
+            uri.append(i); uv.append(val)
+        ucp.append(len(uri))
+    L = (np.array(lrp, np.int32), np.array(lci, np.int32), np.array(lv))
+    U = (np.array(ucp, np.int32), np.array(uri, np.int32), np.array(uv))
+    return L, U, None, None
+
+
+def test_jacobi_zero_pivot_reported():
+    A = gen.poisson2d(4)
+    rp, ci, v = A
+    v = v.copy()
+    v[rp[2] + 1] = 0.0  # a_22 sits second in row 2 (cols 1,2,3,6)
+    d, bad = oracle.jacobi_dinv((rp, ci, v))
+    assert bad == 3

@@ -1,0 +1,250 @@
+"""MI355X (gfx950) preconditioned Krylov solver -- Python host mirror.
+
+The product is the C-ABI library ``libxsolver_amd.so`` (include/xsolver_c.h).
+This module binds it with ctypes and mirrors the reference's API
+(hip/xsolver_c.h:37-49: mat_setup -> ksp_settype -> ksp_setoption ->
+pc_settype -> solve -> free) as a class, so tests read like the reference's
+call sequence (hip/solve_base.h:4-24).  Device vectors are passed as torch
+tensors (``data_ptr()``) for the kernel-level entry points; torch is only
+used for device memory and streams.
+
+There is no CPU fallback: if the library is missing, importing the solver
+raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxsolver_amd.so")
+_LIB = None
+
+XS_OK = 0
+XS_NOT_CONVERGED = 1
+XS_ERR_ARG = -1
+XS_ERR_STATE = -2
+XS_ERR_UNSUPPORTED = -3
+XS_ERR_DEVICE = -4
+XS_ERR_BREAKDOWN = -5
+XS_ERR_PIVOT = -6
+XS_ERR_MEMORY = -7
+
+EXPORTS = [
+    "xsolver_mat_setup", "xsolver_ksp_settype", "xsolver_ksp_set_absolute_tol",
+    "xsolver_ksp_setoption", "xsolver_pc_settype", "xsolver_pc_setilut",
+    "xsolver_pc_setict", "xsolver_solve", "xsolver_free", "xsolver_last_error",
+    "xsolver_get_iterations", "xsolver_get_residual",
+    "xs_create", "xs_destroy", "xs_status_string", "xs_mat_setup", "xs_mat_setup_device",
+    "xs_mat_update_values", "xs_ksp_settype", "xs_ksp_set_absolute_tol", "xs_ksp_setoption",
+    "xs_pc_settype", "xs_pc_setilut", "xs_set_option", "xs_pc_setup", "xs_solve",
+    "xs_solve_device", "xs_get_iterations", "xs_get_residual", "xs_get_initial_residual",
+    "xs_get_resvec", "xs_get_time_ms", "xs_spmv", "xs_bmax", "xs_spmv_t", "xs_pc_apply",
+    "xs_pc_apply_t", "xs_trsv", "xs_pc_get_factors",
+]
+
+
+class XSolverError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(msg)
+        self.status = status
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_cp = ctypes.c_char_p
+
+
+def lib():
+    """Load libxsolver_amd.so; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built: run `python -m xsolver_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "xs_create": (_i, [ctypes.POINTER(_vp)]),
+        "xs_destroy": (_i, [_vp]),
+        "xs_status_string": (_cp, [_i]),
+        "xs_mat_setup": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+        "xs_mat_setup_device": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i]),
+        "xs_mat_update_values": (_i, [_vp, _vp, _i]),
+        "xs_ksp_settype": (_i, [_vp, _cp]),
+        "xs_ksp_set_absolute_tol": (_i, [_vp, _d]),
+        "xs_ksp_setoption": (_i, [_vp, _d, _i, _i]),
+        "xs_pc_settype": (_i, [_vp, _cp]),
+        "xs_pc_setilut": (_i, [_vp, _d, _i]),
+        "xs_set_option": (_i, [_vp, _cp, _d]),
+        "xs_pc_setup": (_i, [_vp]),
+        "xs_solve": (_i, [_vp, _vp, _vp]),
+        "xs_solve_device": (_i, [_vp, _vp, _vp]),
+        "xs_get_iterations": (_i, [_vp]),
+        "xs_get_residual": (_d, [_vp]),
+        "xs_get_initial_residual": (_d, [_vp]),
+        "xs_get_resvec": (_i, [_vp, _vp, _i]),
+        "xs_get_time_ms": (_d, [_vp, _cp]),
+        "xs_spmv": (_i, [_vp, _vp, _vp, _vp]),
+        "xs_bmax": (_i, [_vp, _vp, _vp, _vp, _vp]),
+        "xs_spmv_t": (_i, [_vp, _vp, _vp, _vp]),
+        "xs_pc_apply": (_i, [_vp, _vp, _vp, _vp]),
+        "xs_pc_apply_t": (_i, [_vp, _vp, _vp, _vp]),
+        "xs_trsv": (_i, [_vp, _i, _vp, _vp, _vp]),
+        "xs_pc_get_factors": (_i, [_vp, _vp, _vp, _vp, _vp, _i]),
+        "xsolver_mat_setup": (None, [_i, _vp, _vp, _vp, _vp]),
+        "xsolver_ksp_settype": (None, [_cp]),
+        "xsolver_ksp_set_absolute_tol": (None, [_d]),
+        "xsolver_ksp_setoption": (None, [_d, _i, _vp, _vp, _i]),
+        "xsolver_pc_settype": (None, [_cp]),
+        "xsolver_pc_setilut": (None, [_d, _i]),
+        "xsolver_pc_setict": (None, [_d, _i]),
+        "xsolver_solve": (None, [_vp, _vp]),
+        "xsolver_free": (None, []),
+        "xsolver_last_error": (_i, []),
+        "xsolver_get_iterations": (_i, []),
+        "xsolver_get_residual": (_d, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def _check(rc, what):
+    if rc < 0:
+        msg = lib().xs_status_string(rc).decode()
+        raise XSolverError(rc, f"{what}: {msg} ({rc})")
+    return rc
+
+
+def _ptr(a):
+    """Address of a numpy array or torch tensor."""
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class XSolver:
+    """One solver context (handle API of include/xsolver_c.h)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().xs_create(ctypes.byref(h)), "xs_create")
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            lib().xs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- reference call sequence ------------------------------------------
+    def mat_setup(self, rowptr, colidx, val, advanced_pam=(0, 0, 1)):
+        rp = np.ascontiguousarray(rowptr, np.int32)
+        ci = np.ascontiguousarray(colidx, np.int32)
+        v = np.ascontiguousarray(val, np.float64)
+        pam = np.ascontiguousarray(advanced_pam, np.int32)
+        return _check(lib().xs_mat_setup(self.h, len(rp) - 1, _ptr(rp), _ptr(ci), _ptr(v),
+                                         _ptr(pam)), "xs_mat_setup")
+
+    def mat_setup_device(self, rowptr, colidx, val, use_ell=False):
+        """rowptr/colidx (int32) and val (float64) are device tensors kept alive here."""
+        self._keep = [rowptr, colidx, val]
+        return _check(lib().xs_mat_setup_device(self.h, rowptr.numel() - 1, val.numel(),
+                                                _ptr(rowptr), _ptr(colidx), _ptr(val),
+                                                int(use_ell)), "xs_mat_setup_device")
+
+    def ksp_settype(self, name):
+        return _check(lib().xs_ksp_settype(self.h, name.encode()), "xs_ksp_settype")
+
+    def ksp_set_absolute_tol(self, tol):
+        return _check(lib().xs_ksp_set_absolute_tol(self.h, tol), "xs_ksp_set_absolute_tol")
+
+    def ksp_setoption(self, tol, maxiter, restart=0):
+        return _check(lib().xs_ksp_setoption(self.h, tol, maxiter, restart), "xs_ksp_setoption")
+
+    def pc_settype(self, name):
+        return _check(lib().xs_pc_settype(self.h, name.encode()), "xs_pc_settype")
+
+    def pc_setilut(self, tol, max_fill):
+        return _check(lib().xs_pc_setilut(self.h, tol, max_fill), "xs_pc_setilut")
+
+    def set_option(self, name, value):
+        return _check(lib().xs_set_option(self.h, name.encode(), float(value)), "xs_set_option")
+
+    def pc_setup(self):
+        return _check(lib().xs_pc_setup(self.h), "xs_pc_setup")
+
+    def solve(self, rhs, x0=None):
+        """Host solve; returns (x, status)."""
+        b = np.ascontiguousarray(rhs, np.float64)
+        x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64, copy=True)
+        st = _check(lib().xs_solve(self.h, _ptr(b), _ptr(x)), "xs_solve")
+        return x, st
+
+    def solve_device(self, d_rhs, d_sol):
+        return _check(lib().xs_solve_device(self.h, _ptr(d_rhs), _ptr(d_sol)), "xs_solve_device")
+
+    # -- queries ------------------------------------------------------------
+    @property
+    def iterations(self):
+        return lib().xs_get_iterations(self.h)
+
+    @property
+    def residual(self):
+        return lib().xs_get_residual(self.h)
+
+    @property
+    def initial_residual(self):
+        return lib().xs_get_initial_residual(self.h)
+
+    def resvec(self):
+        n = self.iterations + 1
+        out = np.zeros(max(n, 1))
+        m = lib().xs_get_resvec(self.h, _ptr(out), len(out))
+        return out[:m]
+
+    def time_ms(self, phase):
+        return lib().xs_get_time_ms(self.h, phase.encode())
+
+    # -- kernel-level (device tensors) ---------------------------------------
+    def spmv(self, x, y, stream=None):
+        return _check(lib().xs_spmv(self.h, _ptr(x), _ptr(y), stream), "xs_spmv")
+
+    def bmax(self, b, x, y, stream=None):
+        return _check(lib().xs_bmax(self.h, _ptr(b), _ptr(x), _ptr(y), stream), "xs_bmax")
+
+    def spmv_t(self, x, y, stream=None):
+        return _check(lib().xs_spmv_t(self.h, _ptr(x), _ptr(y), stream), "xs_spmv_t")
+
+    def pc_apply(self, x, y, stream=None):
+        return _check(lib().xs_pc_apply(self.h, _ptr(x), _ptr(y), stream), "xs_pc_apply")
+
+    def pc_apply_t(self, x, y, stream=None):
+        return _check(lib().xs_pc_apply_t(self.h, _ptr(x), _ptr(y), stream), "xs_pc_apply_t")
+
+    def trsv(self, which, b, x, stream=None):
+        return _check(lib().xs_trsv(self.h, int(which), _ptr(b), _ptr(x), stream), "xs_trsv")
+
+    def factors(self, n, cap):
+        rp = np.zeros(n + 1, np.int32)
+        ci = np.zeros(cap, np.int32)
+        v = np.zeros(cap)
+        dp = np.zeros(n, np.int32)
+        nnz = _check(lib().xs_pc_get_factors(self.h, _ptr(rp), _ptr(ci), _ptr(v), _ptr(dp), cap),
+                     "xs_pc_get_factors")
+        return rp, ci[:nnz], v[:nnz], dp
+
+
+def legacy():
+    """The reference-style global API functions (ctypes)."""
+    return lib()

@@ -1,0 +1,826 @@
+// xs_host.cpp -- host driver of the gfx950 Krylov library: contexts, matrix
+// upload and analysis, preconditioner set-up, device-resident Krylov loops and
+// the extern "C" entry points declared in include/xsolver_c.h.
+//
+// Reference call sequence mirrored: hip/xsolver_c.cpp:41-333 (mat_setup ->
+// ksp_settype -> ksp_setoption -> pc_settype -> solve -> free), solver loops
+// hip/Solver/xsolver_{cg,bicgstab,igcr}.cpp, GMRES from
+// intel/mpisolver-C-v1.cpp:933-1247.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/xsolver_c.h"
+#include "xs_internal.h"
+#include "xs_solvers.h"
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+#define XS_HIP(call)                                                    \
+  do {                                                                  \
+    hipError_t e_ = (call);                                             \
+    if (e_ != hipSuccess) {                                             \
+      snprintf(xs_errbuf, sizeof(xs_errbuf), "%s: %s", #call,           \
+               hipGetErrorString(e_));                                  \
+      return XS_ERR_DEVICE;                                             \
+    }                                                                   \
+  } while (0)
+
+static thread_local char xs_errbuf[512];
+
+template <class T>
+static int dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  XS_HIP(hipMalloc((void **)p, sizeof(T) * count));
+  return XS_OK;
+}
+template <class T>
+static void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+static bool streq_any(const char *s, std::initializer_list<const char *> names) {
+  for (const char *n : names)
+    if (strcmp(s, n) == 0) return true;
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+xs_ctx::xs_ctx() {}
+
+static void free_precond(xs_ctx *c) {
+  dfree(c->jac_dinv);
+  xs_ilu &F = c->ilu;
+  dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.ucp); dfree(F.uri); dfree(F.upos);
+  dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own); dfree(F.trp); dfree(F.tci);
+  dfree(F.tdpos); dfree(F.tlu);
+  F = xs_ilu();
+  dfree(c->lu_tmp);
+  c->pc_ready = false;
+}
+
+static void free_matrix(xs_ctx *c) {
+  if (c->own_matrix) { dfree(c->A.rp); dfree(c->A.ci); dfree(c->A.v); }
+  c->A = xs_csr();
+  dfree(c->plan.blk_row);
+  c->plan = xs_spmv_plan();
+  dfree(c->ell.ci); dfree(c->ell.v);
+  c->ell = xs_ell();
+  dfree(c->At.rp); dfree(c->At.ci); dfree(c->At.v);
+  c->At = xs_csr();
+  dfree(c->planT.blk_row);
+  c->planT = xs_spmv_plan();
+  c->have_At = false;
+  c->h_rp.clear(); c->h_ci.clear();
+  free_precond(c);
+}
+
+static void free_work(xs_ctx *c) {
+  dfree(c->work);
+  c->work_n = 0;
+  c->work_vecs = 0;
+}
+
+int xs_ctx_ensure_work(xs_ctx *c, int nvec) {
+  size_t need = (size_t)nvec * c->A.n;
+  if (c->work && c->work_vecs >= nvec && c->work_n == (size_t)c->A.n) return XS_OK;
+  free_work(c);
+  int rc = dalloc(&c->work, need);
+  if (rc) return rc;
+  c->work_n = c->A.n;
+  c->work_vecs = nvec;
+  return XS_OK;
+}
+
+double *xs_ctx_vec(xs_ctx *c, int k) { return c->work + (size_t)k * c->work_n; }
+
+static int ensure_common(xs_ctx *c) {
+  if (!c->st) {
+    int rc;
+    if ((rc = dalloc(&c->st, 1))) return rc;
+    if ((rc = dalloc(&c->counters, 64))) return rc;
+    XS_HIP(hipMemset(c->counters, 0, sizeof(unsigned) * 64));
+    if ((rc = dalloc(&c->d_int, 8))) return rc;
+    XS_HIP(hipHostMalloc((void **)&c->h_state, sizeof(xs_state), hipHostMallocDefault));
+  }
+  size_t pblocks = std::max<size_t>(XS_MAX_REDUCE_BLOCKS, (size_t)c->plan.nblk);
+  pblocks = std::max<size_t>(pblocks, (size_t)c->planT.nblk);
+  if (pblocks > c->partial_blocks) {
+    dfree(c->partial);
+    int rc = dalloc(&c->partial, pblocks * 4);
+    if (rc) return rc;
+    c->partial_blocks = pblocks;
+  }
+  return XS_OK;
+}
+
+xs_reduce xs_ctx_reduce(xs_ctx *c, int op, int slot) {
+  xs_reduce r{};
+  r.partial = c->partial;
+  r.counter = c->counters;   // counters[0]: reductions; [2..3]: trsv tickets
+  r.st = c->st;
+  r.resvec = c->resvec_d;
+  r.op = op;
+  r.slot = slot;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// matrix setup and analysis
+// ---------------------------------------------------------------------------
+static int analyse_pattern(xs_ctx *c) {
+  // Host copy of the pattern (needed for the ILU column structure).  Checks
+  // sorted columns within rows (the reference's GPU factorisations assume it,
+  // hip/Precond/par_ilu_hip/Asplit_gpu.cpp:15-25).
+  const int n = c->A.n;
+  const std::vector<int> &rp = c->h_rp;
+  const std::vector<int> &ci = c->h_ci;
+  c->sorted = true;
+  c->has_diag = true;
+  for (int i = 0; i < n && c->sorted; i++) {
+    bool d = false;
+    for (int e = rp[i]; e < rp[i + 1]; e++) {
+      if (e > rp[i] && ci[e] <= ci[e - 1]) { c->sorted = false; break; }
+      if (ci[e] < 0 || ci[e] >= n) { c->sorted = false; break; }
+      if (ci[e] == i) d = true;
+    }
+    if (!d) c->has_diag = false;
+  }
+  return XS_OK;
+}
+
+static int build_spmv(xs_ctx *c) {
+  int rc;
+  xs_csr &A = c->A;
+  double avg = A.n ? (double)A.nnz / A.n : 1.0;
+  int chunk = std::min(2048, std::max(512, ((int)(avg * XS_BLOCK) + XS_BLOCK - 1) / XS_BLOCK * XS_BLOCK));
+  int nblk = std::max(1, (A.nnz + chunk - 1) / chunk);
+  if ((rc = dalloc(&c->plan.blk_row, (size_t)nblk + 1))) return rc;
+  xs_build_plan(c->stream, A, c->plan, c->plan.blk_row);
+  if (c->use_ell) {
+    int W = xs_max_row_len(c->stream, A, c->d_int);
+    c->ell.n = A.n;
+    c->ell.W = W;
+    c->ell.ld = (A.n + 511) / 512 * 512;
+    if ((rc = dalloc(&c->ell.ci, (size_t)W * c->ell.ld))) return rc;
+    if ((rc = dalloc(&c->ell.v, (size_t)W * c->ell.ld))) return rc;
+    xs_build_ell(c->stream, A, W, c->ell);
+  }
+  XS_HIP(hipStreamSynchronize(c->stream));
+  return XS_OK;
+}
+
+static int setup_common(xs_ctx *c, int n, int nnz, int use_ell) {
+  c->use_ell = use_ell ? 1 : 0;
+  c->A.n = n;
+  c->A.nnz = nnz;
+  int rc = build_spmv(c);
+  if (rc) return rc;
+  c->mat_ready = true;
+  c->pc_ready = false;
+  return ensure_common(c);
+}
+
+extern "C" int xs_create(xs_ctx **out) {
+  if (!out) return XS_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    snprintf(xs_errbuf, sizeof(xs_errbuf), "no HIP device");
+    *out = nullptr;
+    return XS_ERR_DEVICE;
+  }
+  xs_ctx *c = new xs_ctx();
+  if (hipGetDevice(&c->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    *out = nullptr;
+    return XS_ERR_DEVICE;
+  }
+  (void)hipEventCreate(&c->ev0);
+  (void)hipEventCreate(&c->ev1);
+  *out = c;
+  return XS_OK;
+}
+
+extern "C" int xs_destroy(xs_ctx *c) {
+  if (!c) return XS_OK;
+  (void)hipStreamSynchronize(c->stream);
+  free_matrix(c);
+  free_work(c);
+  dfree(c->st); dfree(c->counters); dfree(c->d_int); dfree(c->partial); dfree(c->resvec_d);
+  if (c->h_state) (void)hipHostFree(c->h_state);
+  xs_solvers_free(c);
+  for (auto &e : c->prof_ev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return XS_OK;
+}
+
+extern "C" const char *xs_status_string(int s) {
+  switch (s) {
+    case XS_OK: return "ok";
+    case XS_NOT_CONVERGED: return "maximum iterations reached";
+    case XS_ERR_ARG: return "invalid argument";
+    case XS_ERR_STATE: return "call out of order";
+    case XS_ERR_UNSUPPORTED: return "unsupported";
+    case XS_ERR_DEVICE: return xs_errbuf[0] ? xs_errbuf : "device error";
+    case XS_ERR_BREAKDOWN: return "breakdown (non-finite residual)";
+    case XS_ERR_PIVOT: return "zero or missing pivot";
+    case XS_ERR_MEMORY: return "out of memory";
+  }
+  return "unknown status";
+}
+
+extern "C" int xs_mat_setup(xs_ctx *c, int n, const int *rowptr, const int *colidx,
+                            const double *val, const int *advanced_pam) {
+  if (!c || n <= 0 || !rowptr || !colidx || !val) return XS_ERR_ARG;
+  int use_ell = 0;
+  if (advanced_pam) {
+    if (advanced_pam[0] == 1) return XS_ERR_UNSUPPORTED;  // MPI distribution
+    if (advanced_pam[2] != 1) return XS_ERR_UNSUPPORTED;  // CPU backend not provided
+    if (advanced_pam[1] == 1) use_ell = 1;
+    else if (advanced_pam[1] != 0) return XS_ERR_UNSUPPORTED;
+  }
+  const int base = rowptr[0];
+  if (base != 0 && base != 1) return XS_ERR_ARG;
+  const int nnz = rowptr[n] - base;
+  if (nnz <= 0) return XS_ERR_ARG;
+  (void)hipStreamSynchronize(c->stream);
+  free_matrix(c);
+  c->h_rp.resize(n + 1);
+  c->h_ci.resize(nnz);
+  std::vector<double> hv(val, val + nnz);
+  for (int i = 0; i <= n; i++) c->h_rp[i] = rowptr[i] - base;
+  for (int e = 0; e < nnz; e++) c->h_ci[e] = colidx[e] - base;
+  for (int i = 0; i < n; i++)
+    if (c->h_rp[i + 1] < c->h_rp[i]) return XS_ERR_ARG;
+  analyse_pattern(c);
+  if (!c->sorted) {
+    // sort each row by column (values follow); rejects out-of-range columns
+    for (int i = 0; i < n; i++) {
+      int a = c->h_rp[i], b = c->h_rp[i + 1];
+      std::vector<std::pair<int, double>> row;
+      for (int e = a; e < b; e++) {
+        if (c->h_ci[e] < 0 || c->h_ci[e] >= n) return XS_ERR_ARG;
+        row.push_back({c->h_ci[e], hv[e]});
+      }
+      std::sort(row.begin(), row.end(),
+                [](const std::pair<int, double> &x, const std::pair<int, double> &y) { return x.first < y.first; });
+      for (int e = a; e < b; e++) { c->h_ci[e] = row[e - a].first; hv[e] = row[e - a].second; }
+    }
+    analyse_pattern(c);
+  }
+  int rc;
+  c->own_matrix = true;
+  if ((rc = dalloc(&c->A.rp, n + 1))) return rc;
+  if ((rc = dalloc(&c->A.ci, nnz))) return rc;
+  if ((rc = dalloc(&c->A.v, nnz))) return rc;
+  XS_HIP(hipMemcpy(c->A.rp, c->h_rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  XS_HIP(hipMemcpy(c->A.ci, c->h_ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+  XS_HIP(hipMemcpy(c->A.v, hv.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
+  return setup_common(c, n, nnz, use_ell);
+}
+
+extern "C" int xs_mat_setup_device(xs_ctx *c, int n, int nnz, const int *d_rowptr,
+                                   const int *d_colidx, const double *d_val, int use_ell) {
+  if (!c || n <= 0 || nnz <= 0 || !d_rowptr || !d_colidx || !d_val) return XS_ERR_ARG;
+  (void)hipStreamSynchronize(c->stream);
+  free_matrix(c);
+  c->own_matrix = false;
+  c->A.rp = const_cast<int *>(d_rowptr);
+  c->A.ci = const_cast<int *>(d_colidx);
+  c->A.v = const_cast<double *>(d_val);
+  c->h_rp.resize(n + 1);
+  c->h_ci.resize(nnz);
+  XS_HIP(hipMemcpy(c->h_rp.data(), d_rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  XS_HIP(hipMemcpy(c->h_ci.data(), d_colidx, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+  if (c->h_rp[0] != 0 || c->h_rp[n] != nnz) return XS_ERR_ARG;
+  analyse_pattern(c);
+  if (!c->sorted) return XS_ERR_ARG;
+  return setup_common(c, n, nnz, use_ell);
+}
+
+extern "C" int xs_mat_update_values(xs_ctx *c, const double *val, int on_device) {
+  if (!c || !c->mat_ready || !val) return XS_ERR_STATE;
+  if (!c->own_matrix) return XS_ERR_STATE;
+  XS_HIP(hipMemcpy(c->A.v, val, sizeof(double) * c->A.nnz,
+                   on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  if (c->use_ell) xs_build_ell(c->stream, c->A, c->ell.W, c->ell);
+  if (c->have_At) c->have_At = false;
+  c->pc_ready = false;
+  XS_HIP(hipStreamSynchronize(c->stream));
+  return XS_OK;
+}
+
+extern "C" int xs_ksp_settype(xs_ctx *c, const char *t) {
+  if (!c || !t) return XS_ERR_ARG;
+  if (streq_any(t, {"Bicgstab", "bicgstab", "BICGSTAB"})) c->ksp = XS_KSP_BICGSTAB;
+  else if (streq_any(t, {"Igcr", "igcr", "IGCR", "Gcr", "gcr", "GCR"})) c->ksp = XS_KSP_GCR;
+  else if (streq_any(t, {"Cg", "cg", "CG"})) c->ksp = XS_KSP_CG;
+  else if (streq_any(t, {"Gmres", "gmres", "GMRES"})) c->ksp = XS_KSP_GMRES;
+  else if (streq_any(t, {"Fgmres", "fgmres", "FGMRES"})) c->ksp = XS_KSP_FGMRES;
+  else return XS_ERR_ARG;
+  return XS_OK;
+}
+
+extern "C" int xs_ksp_set_absolute_tol(xs_ctx *c, double a) {
+  if (!c) return XS_ERR_ARG;
+  c->abstol = a;
+  return XS_OK;
+}
+
+extern "C" int xs_ksp_setoption(xs_ctx *c, double tol, int maxiter, int restart) {
+  if (!c || maxiter < 0 || restart < 0) return XS_ERR_ARG;
+  c->tol = tol;
+  c->maxiter = maxiter;
+  c->restart = restart;
+  return XS_OK;
+}
+
+extern "C" int xs_pc_settype(xs_ctx *c, const char *t) {
+  if (!c || !t) return XS_ERR_ARG;
+  int pc;
+  if (streq_any(t, {"Jacobi", "jacobi", "JACOBI"})) pc = XS_PC_JACOBI;
+  else if (streq_any(t, {"Ilu0", "ilu0", "ILU0", "Ilu0_mkl", "ilu0_mkl", "ILU0_MKL"})) pc = XS_PC_ILU0;
+  else if (streq_any(t, {"Ilut", "ilut", "ILUT", "Ilut_mkl", "ilut_mkl", "ILUT_MKL"})) pc = XS_PC_ILUT;
+  else if (streq_any(t, {"None", "none", "NONE"})) pc = XS_PC_NONE;
+  else if (streq_any(t, {"Ic0", "ic0", "IC0", "Ict", "ict", "ICT"})) return XS_ERR_UNSUPPORTED;
+  else return XS_ERR_ARG;
+  if (pc != c->pc) free_precond(c);
+  c->pc = pc;
+  c->pc_ready = false;
+  return XS_OK;
+}
+
+extern "C" int xs_pc_setilut(xs_ctx *c, double tol, int max_fill) {
+  if (!c || tol < 0 || max_fill <= 0) return XS_ERR_ARG;
+  c->ilut_tol = tol;
+  c->ilut_fill = max_fill;
+  c->pc_ready = false;
+  return XS_OK;
+}
+
+extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
+  if (!c || !name) return XS_ERR_ARG;
+  std::string k(name);
+  if (k == "parilu_sweeps") { if (value < 0) return XS_ERR_ARG; c->parilu_sweeps = (int)value; c->pc_ready = false; }
+  else if (k == "parilu_async") { c->parilu_async = value != 0; c->pc_ready = false; }
+  else if (k == "check_every") { if (value < 1) return XS_ERR_ARG; c->check_every = (int)value; }
+  else if (k == "refactor") { c->pc_ready = false; }
+  else if (k == "profile") { c->prof_kind = (int)value; for (int i = 0; i < 4; i++) { c->prof_ms[i] = 0; c->prof_cnt[i] = 0; } }
+  else if (k == "precision") { if (value != 0 && value != 1) return XS_ERR_ARG; c->precision = (int)value; }
+  else if (k == "use_ell") {
+    if (!c->mat_ready) { c->use_ell = value != 0; return XS_OK; }
+    int want = value != 0;
+    if (want && !c->ell.v) {
+      int W = xs_max_row_len(c->stream, c->A, c->d_int);
+      c->ell.n = c->A.n; c->ell.W = W; c->ell.ld = (c->A.n + 511) / 512 * 512;
+      int rc;
+      if ((rc = dalloc(&c->ell.ci, (size_t)W * c->ell.ld))) return rc;
+      if ((rc = dalloc(&c->ell.v, (size_t)W * c->ell.ld))) return rc;
+      xs_build_ell(c->stream, c->A, W, c->ell);
+      XS_HIP(hipStreamSynchronize(c->stream));
+    }
+    c->use_ell = want;
+  } else return XS_ERR_ARG;
+  return XS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// preconditioner set-up
+// ---------------------------------------------------------------------------
+static int setup_ilu_pattern(xs_ctx *c) {
+  // Combined-factor layout on A's pattern plus the CSC of the upper triangle
+  // used by the ParILU sweep (the reference builds L CSR / U CSC with
+  // hipsparseDcsr2csc, hip/Precond/par_ilu_hip/Asplit_gpu.cpp:62-161).
+  const int n = c->A.n, nnz = c->A.nnz;
+  const std::vector<int> &rp = c->h_rp, &ci = c->h_ci;
+  std::vector<int> dpos(n), rowidx(nnz), ucnt(n + 1, 0);
+  for (int i = 0; i < n; i++) {
+    dpos[i] = -1;
+    for (int e = rp[i]; e < rp[i + 1]; e++) {
+      rowidx[e] = i;
+      if (ci[e] == i) dpos[i] = e;
+      if (ci[e] >= i) ucnt[ci[e] + 1]++;
+    }
+    if (dpos[i] < 0) return XS_ERR_PIVOT;
+  }
+  for (int j = 0; j < n; j++) ucnt[j + 1] += ucnt[j];
+  const int nu = ucnt[n];
+  std::vector<int> uri(nu), upos(nu), fill(ucnt.begin(), ucnt.end() - 1);
+  for (int i = 0; i < n; i++)
+    for (int e = dpos[i]; e < rp[i + 1]; e++) {
+      const int q = fill[ci[e]]++;
+      uri[q] = i;
+      upos[q] = e;
+    }
+  xs_ilu &F = c->ilu;
+  F.n = n; F.nnz = nnz; F.rp = c->A.rp; F.ci = c->A.ci;
+  int rc;
+  if ((rc = dalloc(&F.lu, nnz))) return rc;
+  if ((rc = dalloc(&F.dpos, n))) return rc;
+  if ((rc = dalloc(&F.dinv, n))) return rc;
+  if ((rc = dalloc(&F.rowidx, nnz))) return rc;
+  if ((rc = dalloc(&F.ucp, n + 1))) return rc;
+  if ((rc = dalloc(&F.uri, nu))) return rc;
+  if ((rc = dalloc(&F.upos, nu))) return rc;
+  XS_HIP(hipMemcpy(F.dpos, dpos.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  XS_HIP(hipMemcpy(F.rowidx, rowidx.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+  XS_HIP(hipMemcpy(F.ucp, ucnt.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  XS_HIP(hipMemcpy(F.uri, uri.data(), sizeof(int) * nu, hipMemcpyHostToDevice));
+  XS_HIP(hipMemcpy(F.upos, upos.data(), sizeof(int) * nu, hipMemcpyHostToDevice));
+  return XS_OK;
+}
+
+static int factor_parilu(xs_ctx *c) {
+  xs_ilu &F = c->ilu;
+  if (!F.lu) {
+    int rc = setup_ilu_pattern(c);
+    if (rc) return rc;
+  }
+  xs_ilu_copy_values(c->stream, c->A, F);
+  if (!c->parilu_async && !c->lu_tmp) {
+    int rc = dalloc(&c->lu_tmp, c->A.nnz);
+    if (rc) return rc;
+  }
+  for (int s = 0; s < c->parilu_sweeps; s++) {
+    if (c->parilu_async) {
+      xs_parilu_sweep(c->stream, c->A, F, nullptr, true);
+    } else {
+      xs_parilu_sweep(c->stream, c->A, F, c->lu_tmp, false);
+      std::swap(F.lu, c->lu_tmp);
+    }
+  }
+  XS_HIP(hipMemsetAsync(c->d_int, 0x7f, sizeof(int), c->stream));
+  xs_ilu_dinv(c->stream, F, c->d_int);
+  int bad = 0;
+  XS_HIP(hipMemcpyAsync(&bad, c->d_int, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  XS_HIP(hipStreamSynchronize(c->stream));
+  if (bad != 0x7f7f7f7f) return XS_ERR_PIVOT;
+  return XS_OK;
+}
+
+extern "C" int xs_pc_setup(xs_ctx *c) {
+  if (!c) return XS_ERR_ARG;
+  if (!c->mat_ready) return XS_ERR_STATE;
+  if (c->pc_ready) return XS_OK;
+  int rc = XS_OK;
+  XS_HIP(hipEventRecord(c->ev0, c->stream));
+  switch (c->pc) {
+    case XS_PC_NONE: break;
+    case XS_PC_JACOBI: {
+      if (!c->jac_dinv && (rc = dalloc(&c->jac_dinv, c->A.n))) return rc;
+      XS_HIP(hipMemsetAsync(c->d_int, 0x7f, sizeof(int), c->stream));
+      xs_jacobi_dinv(c->stream, c->A, c->jac_dinv, c->d_int);
+      int bad = 0;
+      XS_HIP(hipMemcpyAsync(&bad, c->d_int, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      XS_HIP(hipStreamSynchronize(c->stream));
+      if (bad != 0x7f7f7f7f) rc = XS_ERR_PIVOT;
+      break;
+    }
+    case XS_PC_ILU0:
+      rc = factor_parilu(c);
+      break;
+    case XS_PC_ILUT:
+      rc = xs_parilut_factor(c);
+      break;
+    default:
+      return XS_ERR_STATE;
+  }
+  XS_HIP(hipEventRecord(c->ev1, c->stream));
+  XS_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->t_pc_ms = ms;
+  if (rc == XS_OK) c->pc_ready = true;
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// kernel timing: event pairs from a pool, summed after the solve
+// ---------------------------------------------------------------------------
+static const int XS_PROF_POOL = 16384;
+
+void xs_prof_begin(xs_ctx *c, int kind) {
+  if (!(c->prof_kind & (1 << kind))) return;
+  if (c->prof_ev.empty()) {
+    c->prof_ev.resize(XS_PROF_POOL);
+    for (auto &e : c->prof_ev) (void)hipEventCreate(&e);
+  }
+  if (c->prof_used + 2 > XS_PROF_POOL) return;
+  (void)hipEventRecord(c->prof_ev[c->prof_used], c->stream);
+  c->prof_next_kind = kind;
+}
+
+void xs_prof_end(xs_ctx *c, int kind) {
+  if (!(c->prof_kind & (1 << kind))) return;
+  if (c->prof_used + 2 > XS_PROF_POOL) return;
+  (void)hipEventRecord(c->prof_ev[c->prof_used + 1], c->stream);
+  c->prof_kinds.push_back(kind);
+  c->prof_used += 2;
+}
+
+void xs_prof_collect(xs_ctx *c) {
+  for (int i = 0; i < c->prof_used; i += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->prof_ev[i], c->prof_ev[i + 1]) == hipSuccess) {
+      int k = c->prof_kinds[i / 2];
+      c->prof_ms[k] += ms;
+      c->prof_cnt[k] += 1;
+    }
+  }
+  c->prof_used = 0;
+  c->prof_kinds.clear();
+}
+
+// ---------------------------------------------------------------------------
+// operators used by the solver loops
+// ---------------------------------------------------------------------------
+void xs_op_spmv(xs_ctx *c, const double *x, double *y, int epi, const double *b,
+                const double *w, const xs_reduce *red) {
+  xs_prof_begin(c, 1);
+  if (c->use_ell && c->ell.v)
+    xs_launch_spmv_ell(c->stream, c->ell, x, y, epi, b, w, red);
+  else
+    xs_launch_spmv_csr(c->stream, c->A, c->plan, x, y, epi, b, w, red);
+  xs_prof_end(c, 1);
+}
+
+// y = M^{-1} x on the device (tmp: scratch vector for the ILU forward solve)
+void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state *st) {
+  switch (c->pc) {
+    case XS_PC_JACOBI:
+      xs_scale(c->stream, c->A.n, c->jac_dinv, x, y);
+      break;
+    case XS_PC_ILU0:
+    case XS_PC_ILUT:
+      xs_fill_sentinel(c->stream, tmp, c->A.n);
+      xs_prof_begin(c, 2);
+      xs_trsv_lower(c->stream, c->ilu, x, tmp, c->counters + 2, st);
+      xs_prof_end(c, 2);
+      xs_fill_sentinel(c->stream, y, c->A.n);
+      xs_prof_begin(c, 3);
+      xs_trsv_upper(c->stream, c->ilu, tmp, y, c->counters + 4, st);
+      xs_prof_end(c, 3);
+      break;
+    default:
+      xs_copy(c->stream, c->A.n, x, y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// solve
+// ---------------------------------------------------------------------------
+static int run_solve(xs_ctx *c, const double *d_b, double *d_x) {
+  int rc;
+  if (!c->mat_ready) return XS_ERR_STATE;
+  if (c->ksp == XS_KSP_NONE) return XS_ERR_STATE;
+  if (c->maxiter <= 0) return XS_ERR_STATE;
+  if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
+  if ((rc = ensure_common(c))) return rc;
+  if (c->resvec_cap < c->maxiter + 2) {
+    dfree(c->resvec_d);
+    if ((rc = dalloc(&c->resvec_d, (size_t)c->maxiter + 2))) return rc;
+    c->resvec_cap = c->maxiter + 2;
+  }
+  XS_HIP(hipEventRecord(c->ev0, c->stream));
+  switch (c->ksp) {
+    case XS_KSP_CG: rc = xs_solve_cg(c, d_b, d_x); break;
+    case XS_KSP_BICGSTAB: rc = xs_solve_bicgstab(c, d_b, d_x); break;
+    case XS_KSP_GCR: rc = xs_solve_gcr(c, d_b, d_x); break;
+    case XS_KSP_GMRES: rc = xs_solve_gmres(c, d_b, d_x, false); break;
+    case XS_KSP_FGMRES: rc = xs_solve_gmres(c, d_b, d_x, true); break;
+    default: return XS_ERR_STATE;
+  }
+  XS_HIP(hipEventRecord(c->ev1, c->stream));
+  XS_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->t_solve_ms = ms;
+  xs_prof_collect(c);
+  {
+    // wait-expired flags of the lower/upper triangular solves
+    unsigned fl[4] = {0, 0, 0, 0};
+    XS_HIP(hipMemcpy(fl, c->counters + 6, sizeof(fl), hipMemcpyDeviceToHost));
+    if (fl[0] | fl[2]) {
+      XS_HIP(hipMemset(c->counters + 6, 0, sizeof(fl)));
+      snprintf(xs_errbuf, sizeof(xs_errbuf), "triangular solve dependency wait expired");
+      return XS_ERR_DEVICE;
+    }
+  }
+  if (rc < 0 && rc != XS_ERR_BREAKDOWN) return rc;
+  // results
+  int nres = std::min(c->iters + 1, c->resvec_cap);
+  c->resvec.resize(nres);
+  XS_HIP(hipMemcpy(c->resvec.data(), c->resvec_d, sizeof(double) * nres, hipMemcpyDeviceToHost));
+  return rc;
+}
+
+extern "C" int xs_solve_device(xs_ctx *c, const double *d_rhs, double *d_sol) {
+  if (!c || !d_rhs || !d_sol) return XS_ERR_ARG;
+  // the caller's producers of d_rhs/d_sol may run on any stream
+  XS_HIP(hipDeviceSynchronize());
+  return run_solve(c, d_rhs, d_sol);
+}
+
+extern "C" int xs_solve(xs_ctx *c, const double *rhs, double *sol) {
+  if (!c || !rhs || !sol) return XS_ERR_ARG;
+  if (!c->mat_ready) return XS_ERR_STATE;
+  int rc;
+  const int n = c->A.n;
+  if (c->hb_n != n) {
+    dfree(c->hb); dfree(c->hx);
+    if ((rc = dalloc(&c->hb, n))) return rc;
+    if ((rc = dalloc(&c->hx, n))) return rc;
+    c->hb_n = n;
+  }
+  XS_HIP(hipMemcpy(c->hb, rhs, sizeof(double) * n, hipMemcpyHostToDevice));
+  XS_HIP(hipMemcpy(c->hx, sol, sizeof(double) * n, hipMemcpyHostToDevice));
+  rc = run_solve(c, c->hb, c->hx);
+  if (rc < 0 && rc != XS_ERR_BREAKDOWN) return rc;
+  XS_HIP(hipMemcpy(sol, c->hx, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return rc;
+}
+
+extern "C" int xs_get_iterations(const xs_ctx *c) { return c ? c->iters : -1; }
+extern "C" double xs_get_residual(const xs_ctx *c) { return c ? c->resl : -1.0; }
+extern "C" double xs_get_initial_residual(const xs_ctx *c) { return c ? c->res0 : -1.0; }
+extern "C" int xs_get_resvec(const xs_ctx *c, double *out, int len) {
+  if (!c || !out || len < 0) return XS_ERR_ARG;
+  int m = std::min<int>(len, (int)c->resvec.size());
+  for (int i = 0; i < m; i++) out[i] = c->resvec[i];
+  return m;
+}
+extern "C" double xs_get_time_ms(const xs_ctx *c, const char *phase) {
+  if (!c || !phase) return -1.0;
+  if (!strcmp(phase, "pc_setup")) return c->t_pc_ms;
+  if (!strcmp(phase, "solve")) return c->t_solve_ms;
+  static const char *kn[4] = {"", "spmv", "trsv_lower", "trsv_upper"};
+  for (int k = 1; k < 4; k++) {
+    std::string tot = std::string("total:") + kn[k], cnt = std::string("count:") + kn[k];
+    if (tot == phase) return c->prof_ms[k];
+    if (cnt == phase) return (double)c->prof_cnt[k];
+  }
+  return -1.0;
+}
+
+// ---------------------------------------------------------------------------
+// kernel-level entry points
+// ---------------------------------------------------------------------------
+static hipStream_t pick(xs_ctx *c, void *s) { return s ? (hipStream_t)s : c->stream; }
+
+extern "C" int xs_spmv(xs_ctx *c, const double *x, double *y, void *stream) {
+  if (!c || !x || !y) return XS_ERR_ARG;
+  if (!c->mat_ready) return XS_ERR_STATE;
+  hipStream_t keep = c->stream;
+  c->stream = pick(c, stream);
+  xs_op_spmv(c, x, y, EPI_Y, nullptr, nullptr, nullptr);
+  c->stream = keep;
+  XS_HIP(hipGetLastError());
+  return XS_OK;
+}
+
+extern "C" int xs_bmax(xs_ctx *c, const double *b, const double *x, double *y, void *stream) {
+  if (!c || !b || !x || !y) return XS_ERR_ARG;
+  if (!c->mat_ready) return XS_ERR_STATE;
+  hipStream_t keep = c->stream;
+  c->stream = pick(c, stream);
+  xs_op_spmv(c, x, y, EPI_BMAX, b, nullptr, nullptr);
+  c->stream = keep;
+  XS_HIP(hipGetLastError());
+  return XS_OK;
+}
+
+extern "C" int xs_spmv_t(xs_ctx *c, const double *x, double *y, void *stream) {
+  if (!c || !x || !y) return XS_ERR_ARG;
+  if (!c->mat_ready) return XS_ERR_STATE;
+  int rc = xs_build_transpose(c);
+  if (rc) return rc;
+  xs_launch_spmv_csr(pick(c, stream), c->At, c->planT, x, y, EPI_Y, nullptr, nullptr, nullptr);
+  XS_HIP(hipGetLastError());
+  return XS_OK;
+}
+
+extern "C" int xs_pc_apply(xs_ctx *c, const double *x, double *y, void *stream) {
+  if (!c || !x || !y) return XS_ERR_ARG;
+  int rc;
+  if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
+  if ((rc = xs_ctx_ensure_work(c, XS_NWORK))) return rc;
+  hipStream_t keep = c->stream;
+  c->stream = pick(c, stream);
+  xs_op_pc(c, x, y, xs_ctx_vec(c, XS_NWORK - 1), nullptr);
+  c->stream = keep;
+  XS_HIP(hipGetLastError());
+  return XS_OK;
+}
+
+extern "C" int xs_pc_apply_t(xs_ctx *c, const double *x, double *y, void *stream) {
+  if (!c || !x || !y) return XS_ERR_ARG;
+  int rc;
+  if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
+  if ((rc = xs_ctx_ensure_work(c, XS_NWORK))) return rc;
+  return xs_op_pc_t(c, x, y, xs_ctx_vec(c, XS_NWORK - 1), pick(c, stream));
+}
+
+extern "C" int xs_trsv(xs_ctx *c, int which, const double *b, double *x, void *stream) {
+  if (!c || !b || !x || (which != 0 && which != 1)) return XS_ERR_ARG;
+  int rc;
+  if (c->pc != XS_PC_ILU0 && c->pc != XS_PC_ILUT) return XS_ERR_STATE;
+  if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
+  xs_fill_sentinel(pick(c, stream), x, c->A.n);
+  if (which == 0) xs_trsv_lower(pick(c, stream), c->ilu, b, x, c->counters + 2, nullptr);
+  else xs_trsv_upper(pick(c, stream), c->ilu, b, x, c->counters + 4, nullptr);
+  XS_HIP(hipGetLastError());
+  return XS_OK;
+}
+
+extern "C" int xs_pc_get_factors(xs_ctx *c, int *rowptr, int *colidx, double *val, int *diagpos,
+                                 int cap) {
+  if (!c) return XS_ERR_ARG;
+  if (c->pc != XS_PC_ILU0 && c->pc != XS_PC_ILUT) return XS_ERR_STATE;
+  int rc;
+  if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
+  const xs_ilu &F = c->ilu;
+  if (cap < F.nnz) return F.nnz;
+  XS_HIP(hipDeviceSynchronize());
+  if (rowptr) XS_HIP(hipMemcpy(rowptr, F.rp, sizeof(int) * (F.n + 1), hipMemcpyDeviceToHost));
+  if (colidx) XS_HIP(hipMemcpy(colidx, F.ci, sizeof(int) * F.nnz, hipMemcpyDeviceToHost));
+  if (val) XS_HIP(hipMemcpy(val, F.lu, sizeof(double) * F.nnz, hipMemcpyDeviceToHost));
+  if (diagpos) XS_HIP(hipMemcpy(diagpos, F.dpos, sizeof(int) * F.n, hipMemcpyDeviceToHost));
+  return F.nnz;
+}
+
+// ---------------------------------------------------------------------------
+// legacy singleton API (hip/xsolver_c.cpp)
+// ---------------------------------------------------------------------------
+static xs_ctx *g_ctx = nullptr;
+static int g_err = XS_OK;
+static int *g_usediter = nullptr;
+static double *g_resvec = nullptr;
+
+static int g_set(int rc) {
+  g_err = rc;
+  return rc;
+}
+
+extern "C" void xsolver_mat_setup(int n, int *rowptr, int *colidx, double *val, int *advanced_pam) {
+  if (!g_ctx && g_set(xs_create(&g_ctx)) != XS_OK) return;
+  g_set(xs_mat_setup(g_ctx, n, rowptr, colidx, val, advanced_pam));
+}
+extern "C" void xsolver_ksp_settype(char *fmt) {
+  if (!g_ctx) { g_set(XS_ERR_STATE); return; }
+  g_set(xs_ksp_settype(g_ctx, fmt));
+}
+extern "C" void xsolver_ksp_set_absolute_tol(double a) {
+  if (!g_ctx) { g_set(XS_ERR_STATE); return; }
+  g_set(xs_ksp_set_absolute_tol(g_ctx, a));
+}
+extern "C" void xsolver_ksp_setoption(double tol, int maxiter, int *usediter, double *resvec,
+                                      int restart) {
+  if (!g_ctx) { g_set(XS_ERR_STATE); return; }
+  g_usediter = usediter;
+  g_resvec = resvec;
+  // IGCR restart = maxiter when 0 (hip/Solver/xsolver_igcr.h:36)
+  g_set(xs_ksp_setoption(g_ctx, tol, maxiter, restart));
+}
+extern "C" void xsolver_pc_settype(char *fmt) {
+  if (!g_ctx) { g_set(XS_ERR_STATE); return; }
+  g_set(xs_pc_settype(g_ctx, fmt));
+}
+extern "C" void xsolver_pc_setilut(double tol, int max_fill) {
+  if (!g_ctx) { g_set(XS_ERR_STATE); return; }
+  g_set(xs_pc_setilut(g_ctx, tol, max_fill));
+}
+extern "C" void xsolver_pc_setict(double, int) { g_set(XS_ERR_UNSUPPORTED); }
+extern "C" void xsolver_solve(double *rhs, double *sol) {
+  if (!g_ctx) { g_set(XS_ERR_STATE); return; }
+  int rc = xs_solve(g_ctx, rhs, sol);
+  g_set(rc);
+  if (rc >= 0 || rc == XS_ERR_BREAKDOWN) {
+    if (g_usediter) *g_usediter = g_ctx->iters;
+    if (g_resvec)
+      for (size_t i = 0; i < g_ctx->resvec.size(); i++) g_resvec[i] = g_ctx->resvec[i];
+  }
+}
+extern "C" void xsolver_free(void) {
+  if (g_ctx) xs_destroy(g_ctx);
+  g_ctx = nullptr;
+  g_usediter = nullptr;
+  g_resvec = nullptr;
+}
+extern "C" int xsolver_last_error(void) { return g_err; }
+extern "C" int xsolver_get_iterations(void) { return g_ctx ? g_ctx->iters : -1; }
+extern "C" double xsolver_get_residual(void) { return g_ctx ? g_ctx->resl : -1.0; }

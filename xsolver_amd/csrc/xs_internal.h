@@ -1,0 +1,148 @@
+// xs_internal.h -- internal structures shared by the host driver and the
+// gfx950 kernels.  Not part of the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#define XS_BLOCK 256          // threads per workgroup for streaming kernels
+#define XS_WAVE 64            // CDNA wavefront
+#define XS_SPMV_LDS 4096      // products staged in LDS per CSR row block (32 KiB)
+#define XS_MAX_REDUCE_BLOCKS 2048
+#define XS_NSCAL 64
+
+// Readiness marker for the sync-free triangular solves: a signalling-NaN bit
+// pattern.  Arithmetic never produces a signalling NaN, and stored values that
+// happen to carry it are replaced by the canonical quiet NaN.
+#define XS_SENTINEL 0xFFF4DEADBEEF0001ull
+
+// ---------------------------------------------------------------------------
+// Device-resident solver state: scalars, flags, iteration counter.
+// ---------------------------------------------------------------------------
+enum xs_slot {
+  S_RES0 = 0, S_RESL, S_TOL, S_ABSTOL,
+  // BiCGSTAB (hip/Solver/xsolver_bicgstab.cpp naming)
+  S_ALPHA, S_BETA, S_BETO, S_GAMA, S_OMEGA,
+  // CG
+  S_TAU, S_TAUO, S_CGALPHA, S_CGBETA,
+  // GCR
+  S_GCR_T0, S_GCR_T1,
+  S_SCRATCH0 = 32
+};
+
+struct xs_state {
+  double sc[XS_NSCAL];
+  int done;       // 1 = converged, 2 = breakdown / non-finite
+  int iter;       // iterations completed
+  int maxiter;
+  int pad;
+};
+
+// Fused reduction ops: what the last workgroup does with the totals.
+enum xs_finish_op {
+  F_NONE = 0,
+  F_STORE,          // sc[slot+k] = tot[k]
+  F_RES0,           // res0 = sqrt(tot0); resvec[0] = res0; done if res0 == 0 or <= abstol
+  F_CG_ALPHA,       // alpha = tau / tot0
+  F_CG_RES,         // tau' = tot0 (z.r), rr = tot1; beta = tau'/tau; convergence
+  F_CG_INIT,        // tau = tot0 (r.z), res0 = sqrt(tot1)
+  F_BICG_GAMA,      // gama = beta / tot0
+  F_BICG_ALPHA,     // alpha = tot0 / tot1
+  F_BICG_RES,       // resl = sqrt(tot0), beta' = tot1, omega, convergence
+  F_GCR_COEF        // t0 = sqrt(tot0), t1 = tot1 -> scale and step
+};
+
+struct xs_reduce {
+  double *partial;     // [XS_MAX_REDUCE_BLOCKS * 4] (>= gridDim * nv)
+  unsigned *counter;   // last-block ticket (self-resetting)
+  xs_state *st;
+  double *resvec;      // device residual history (may be null)
+  int op;
+  int slot;
+};
+
+// ---------------------------------------------------------------------------
+// Matrix on the device.
+// ---------------------------------------------------------------------------
+struct xs_csr {
+  int n = 0, nnz = 0;
+  int *rp = nullptr, *ci = nullptr;
+  double *v = nullptr;
+};
+
+struct xs_spmv_plan {   // row blocks for the LDS-staged CSR kernel
+  int nblk = 0;
+  int chunk = 0;        // nnz budget per block
+  int *blk_row = nullptr;  // [nblk + 1]
+};
+
+struct xs_ell {         // column-major slabs: v[k * ld + row], width W
+  int n = 0, ld = 0, W = 0;
+  int *ci = nullptr;
+  double *v = nullptr;
+};
+
+// Combined ILU factors aligned with a CSR pattern: strict lower part of row i
+// in [rp[i], dpos[i]), pivot at dpos[i], strict upper part in (dpos[i], rp[i+1]).
+struct xs_ilu {
+  int n = 0, nnz = 0;
+  const int *rp = nullptr, *ci = nullptr;  // pattern (owned elsewhere or below)
+  int *rp_own = nullptr, *ci_own = nullptr;
+  double *lu = nullptr;     // factor values
+  int *dpos = nullptr;      // pivot positions
+  double *dinv = nullptr;   // 1 / u_ii
+  // CSC of the upper triangle (incl. diagonal) for the ParILU sweep:
+  int *ucp = nullptr, *uri = nullptr, *upos = nullptr;
+  int *rowidx = nullptr;    // row of each nnz (COO)
+  // transposed factors for the adjoint (built on demand)
+  bool have_t = false;
+  int *trp = nullptr, *tci = nullptr, *tdpos = nullptr;
+  double *tlu = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (xs_kernels.hip)
+// ---------------------------------------------------------------------------
+// SpMV epilogues: y = Ax | y = b - Ax | y = Ax, w.y | y = Ax, w.y, y.y | y = b - Ax, y.y
+enum xs_spmv_epi { EPI_Y = 0, EPI_BMAX, EPI_DOT1, EPI_DOT2, EPI_BMAXN };
+
+void xs_launch_spmv_csr(hipStream_t s, const xs_csr &A, const xs_spmv_plan &P,
+                        const double *x, double *y, int epi, const double *b,
+                        const double *w, const xs_reduce *red);
+void xs_launch_spmv_ell(hipStream_t s, const xs_ell &E, const double *x, double *y,
+                        int epi, const double *b, const double *w, const xs_reduce *red);
+void xs_launch_spmv_t(hipStream_t s, const xs_csr &A, const double *x, double *y);
+void xs_build_plan(hipStream_t s, const xs_csr &A, xs_spmv_plan &P, int *scratch_blk);
+void xs_build_ell(hipStream_t s, const xs_csr &A, int W, xs_ell &E);
+int xs_max_row_len(hipStream_t s, const xs_csr &A, int *d_tmp);
+
+// fills / vector ops
+void xs_fill_sentinel(hipStream_t s, double *x, int n);
+void xs_launch_dot(hipStream_t s, int n, const double *a, const double *b,
+                   const double *c, const double *d, const xs_reduce &red);
+
+// Krylov fused vector kernels
+void xs_bicg_p(hipStream_t s, int n, const double *r, const double *u, double *p,
+               const double *dinv, double *z, xs_state *st);
+void xs_bicg_s(hipStream_t s, int n, const double *r, const double *u, double *sv,
+               const double *dinv, double *z1, xs_state *st);
+void xs_bicg_xr(hipStream_t s, int n, double *x, double *r, const double *z,
+                const double *z1, const double *sv, const double *v,
+                const double *rh, const xs_reduce &red);
+void xs_cg_xr(hipStream_t s, int n, double *x, double *r, const double *p,
+              const double *Ap, const double *dinv, double *z, const xs_reduce &red);
+void xs_cg_zr_dot(hipStream_t s, int n, const double *z, const double *r, const xs_reduce &red);
+void xs_cg_p(hipStream_t s, int n, double *p, const double *z, xs_state *st);
+void xs_scale(hipStream_t s, int n, const double *d, const double *x, double *y);
+void xs_copy(hipStream_t s, int n, const double *x, double *y);
+void xs_init_state(hipStream_t s, xs_state *st, double tol, double abstol, int maxiter);
+
+// ILU
+void xs_ilu_copy_values(hipStream_t s, const xs_csr &A, xs_ilu &F);
+void xs_parilu_sweep(hipStream_t s, const xs_csr &A, xs_ilu &F, double *lu_out, bool async);
+void xs_ilu_dinv(hipStream_t s, xs_ilu &F, int *d_bad);
+void xs_trsv_lower(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
+                   const xs_state *st);
+void xs_trsv_upper(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
+                   const xs_state *st);
+void xs_jacobi_dinv(hipStream_t s, const xs_csr &A, double *dinv, int *d_bad);

@@ -1,0 +1,103 @@
+// xs_solvers.h -- the context structure and the host-side solver drivers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "xs_internal.h"
+
+enum { XS_KSP_NONE = 0, XS_KSP_CG, XS_KSP_BICGSTAB, XS_KSP_GCR, XS_KSP_GMRES, XS_KSP_FGMRES };
+enum { XS_PC_NONE = 0, XS_PC_JACOBI, XS_PC_ILU0, XS_PC_ILUT };
+
+#define XS_NWORK 12   // work vectors of length n held by a context
+
+struct xs_ctx {
+  xs_ctx();
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // solver settings (defaults as the reference: tol/maxiter come from setoption)
+  int ksp = XS_KSP_NONE;
+  int pc = XS_PC_NONE;
+  double tol = 1e-6, abstol = 0.0;
+  int maxiter = 0, restart = 0;
+  double ilut_tol = 1e-4;       // hip/Precond/precon_ilup.h:19-24 uses 1e-5/20
+  int ilut_fill = 20;
+  int parilu_sweeps = 20;       // hip/Precond/precon_gpuilu0.cpp:5
+  bool parilu_async = true;
+  int check_every = 8;
+  int precision = 0;
+  int use_ell = 0;
+
+  // matrix
+  bool mat_ready = false, own_matrix = false, sorted = true, has_diag = true;
+  xs_csr A;
+  xs_spmv_plan plan;
+  xs_ell ell;
+  std::vector<int> h_rp, h_ci;
+  bool have_At = false;
+  xs_csr At;
+  xs_spmv_plan planT;
+
+  // preconditioner
+  bool pc_ready = false;
+  double *jac_dinv = nullptr;
+  xs_ilu ilu;
+  double *lu_tmp = nullptr;
+
+  // device state and workspace
+  xs_state *st = nullptr;
+  xs_state *h_state = nullptr;   // pinned mirror
+  unsigned *counters = nullptr;  // [0] reductions, [2..3] lower trsv, [4..5] upper trsv
+  int *d_int = nullptr;
+  double *partial = nullptr;
+  size_t partial_blocks = 0;
+  double *resvec_d = nullptr;
+  int resvec_cap = 0;
+  double *work = nullptr;
+  size_t work_n = 0;
+  int work_vecs = 0;
+  double *hb = nullptr, *hx = nullptr;
+  int hb_n = 0;
+  // GMRES/GCR workspace (basis vectors, small dense data)
+  double *basis = nullptr;
+  size_t basis_len = 0;
+  double *small_d = nullptr;
+
+  // kernel timing (HIP events on the context stream around launches of one
+  // kernel family while a solve runs): 0 off, 1 SpMV, 2 lower trsv, 3 upper trsv
+  int prof_kind = 0;
+  std::vector<hipEvent_t> prof_ev;
+  int prof_used = 0;
+  double prof_ms[4] = {0, 0, 0, 0};
+  long prof_cnt[4] = {0, 0, 0, 0};
+  int prof_next_kind = 0;
+  std::vector<int> prof_kinds;
+
+  // results
+  int iters = 0;
+  double res0 = 0.0, resl = 0.0;
+  std::vector<double> resvec;
+  float t_pc_ms = 0.f, t_solve_ms = 0.f;
+};
+
+int xs_ctx_ensure_work(xs_ctx *c, int nvec);
+// kernel timing helpers: bracket one launch of kernel family `kind`
+void xs_prof_begin(xs_ctx *c, int kind);
+void xs_prof_end(xs_ctx *c, int kind);
+void xs_prof_collect(xs_ctx *c);
+double *xs_ctx_vec(xs_ctx *c, int k);
+xs_reduce xs_ctx_reduce(xs_ctx *c, int op, int slot);
+void xs_op_spmv(xs_ctx *c, const double *x, double *y, int epi, const double *b,
+                const double *w, const xs_reduce *red);
+void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state *st);
+int xs_op_pc_t(xs_ctx *c, const double *x, double *y, double *tmp, hipStream_t s);
+
+int xs_solve_cg(xs_ctx *c, const double *b, double *x);
+int xs_solve_bicgstab(xs_ctx *c, const double *b, double *x);
+int xs_solve_gcr(xs_ctx *c, const double *b, double *x);
+int xs_solve_gmres(xs_ctx *c, const double *b, double *x, bool flexible);
+int xs_parilut_factor(xs_ctx *c);
+int xs_build_transpose(xs_ctx *c);
+void xs_solvers_free(xs_ctx *c);
