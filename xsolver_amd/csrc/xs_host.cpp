@@ -105,15 +105,23 @@ int xs_ctx_ensure_work(xs_ctx *c, int nvec) {
 
 double *xs_ctx_vec(xs_ctx *c, int k) { return c->work + (size_t)k * c->work_n; }
 
+// Base device buffers of a context (state, counters, flags); allocated once at
+// creation so every later launch finds them.
+static int alloc_base(xs_ctx *c) {
+  int rc;
+  if ((rc = dalloc(&c->st, 1))) return rc;
+  if ((rc = dalloc(&c->counters, 64))) return rc;
+  XS_HIP(hipMemset(c->counters, 0, sizeof(unsigned) * 64));
+  if ((rc = dalloc(&c->d_int, 8))) return rc;
+  XS_HIP(hipMemset(c->d_int, 0, sizeof(int) * 8));
+  XS_HIP(hipHostMalloc((void **)&c->h_state, sizeof(xs_state), hipHostMallocDefault));
+  return XS_OK;
+}
+
+// Reduction partials sized for the largest grid that reduces (>= the SpMV
+// row-block count).
 static int ensure_common(xs_ctx *c) {
-  if (!c->st) {
-    int rc;
-    if ((rc = dalloc(&c->st, 1))) return rc;
-    if ((rc = dalloc(&c->counters, 64))) return rc;
-    XS_HIP(hipMemset(c->counters, 0, sizeof(unsigned) * 64));
-    if ((rc = dalloc(&c->d_int, 8))) return rc;
-    XS_HIP(hipHostMalloc((void **)&c->h_state, sizeof(xs_state), hipHostMallocDefault));
-  }
+  if (!c->st || !c->counters || !c->d_int) return XS_ERR_STATE;
   size_t pblocks = std::max<size_t>(XS_MAX_REDUCE_BLOCKS, (size_t)c->plan.nblk);
   pblocks = std::max<size_t>(pblocks, (size_t)c->planT.nblk);
   if (pblocks > c->partial_blocks) {
@@ -163,6 +171,7 @@ static int analyse_pattern(xs_ctx *c) {
 static int build_spmv(xs_ctx *c) {
   int rc;
   xs_csr &A = c->A;
+  if (!A.rp || !A.ci || !A.v || !c->d_int) return XS_ERR_STATE;
   double avg = A.n ? (double)A.nnz / A.n : 1.0;
   int chunk = std::min(2048, std::max(512, ((int)(avg * XS_BLOCK) + XS_BLOCK - 1) / XS_BLOCK * XS_BLOCK));
   int nblk = std::max(1, (A.nnz + chunk - 1) / chunk);
@@ -178,6 +187,7 @@ static int build_spmv(xs_ctx *c) {
     xs_build_ell(c->stream, A, W, c->ell);
   }
   XS_HIP(hipStreamSynchronize(c->stream));
+  if (xs_take_launch_guard()) return XS_ERR_STATE;
   return XS_OK;
 }
 
@@ -209,6 +219,12 @@ extern "C" int xs_create(xs_ctx **out) {
   }
   (void)hipEventCreate(&c->ev0);
   (void)hipEventCreate(&c->ev1);
+  int rc = alloc_base(c);
+  if (rc) {
+    xs_destroy(c);
+    *out = nullptr;
+    return rc;
+  }
   *out = c;
   return XS_OK;
 }
@@ -505,6 +521,7 @@ extern "C" int xs_pc_setup(xs_ctx *c) {
   float ms = 0;
   (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
   c->t_pc_ms = ms;
+  if (xs_take_launch_guard()) rc = XS_ERR_STATE;
   if (rc == XS_OK) c->pc_ready = true;
   return rc;
 }
@@ -611,6 +628,10 @@ static int run_solve(xs_ctx *c, const double *d_b, double *d_x) {
   (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
   c->t_solve_ms = ms;
   xs_prof_collect(c);
+  if (xs_take_launch_guard()) {
+    snprintf(xs_errbuf, sizeof(xs_errbuf), "internal: a kernel launch found a null buffer");
+    return XS_ERR_STATE;
+  }
   {
     // wait-expired flags of the lower/upper triangular solves
     unsigned fl[4] = {0, 0, 0, 0};
@@ -680,7 +701,9 @@ extern "C" double xs_get_time_ms(const xs_ctx *c, const char *phase) {
 // ---------------------------------------------------------------------------
 // kernel-level entry points
 // ---------------------------------------------------------------------------
-static hipStream_t pick(xs_ctx *c, void *s) { return s ? (hipStream_t)s : c->stream; }
+// Kernel-level entry points run on the caller's stream (0 = the default
+// stream), so they order with the caller's own work on that stream.
+static hipStream_t pick(xs_ctx *, void *s) { return (hipStream_t)s; }
 
 extern "C" int xs_spmv(xs_ctx *c, const double *x, double *y, void *stream) {
   if (!c || !x || !y) return XS_ERR_ARG;
@@ -690,7 +713,7 @@ extern "C" int xs_spmv(xs_ctx *c, const double *x, double *y, void *stream) {
   xs_op_spmv(c, x, y, EPI_Y, nullptr, nullptr, nullptr);
   c->stream = keep;
   XS_HIP(hipGetLastError());
-  return XS_OK;
+  return xs_take_launch_guard() ? XS_ERR_STATE : XS_OK;
 }
 
 extern "C" int xs_bmax(xs_ctx *c, const double *b, const double *x, double *y, void *stream) {
@@ -701,7 +724,7 @@ extern "C" int xs_bmax(xs_ctx *c, const double *b, const double *x, double *y, v
   xs_op_spmv(c, x, y, EPI_BMAX, b, nullptr, nullptr);
   c->stream = keep;
   XS_HIP(hipGetLastError());
-  return XS_OK;
+  return xs_take_launch_guard() ? XS_ERR_STATE : XS_OK;
 }
 
 extern "C" int xs_spmv_t(xs_ctx *c, const double *x, double *y, void *stream) {
@@ -711,7 +734,7 @@ extern "C" int xs_spmv_t(xs_ctx *c, const double *x, double *y, void *stream) {
   if (rc) return rc;
   xs_launch_spmv_csr(pick(c, stream), c->At, c->planT, x, y, EPI_Y, nullptr, nullptr, nullptr);
   XS_HIP(hipGetLastError());
-  return XS_OK;
+  return xs_take_launch_guard() ? XS_ERR_STATE : XS_OK;
 }
 
 extern "C" int xs_pc_apply(xs_ctx *c, const double *x, double *y, void *stream) {
@@ -724,7 +747,7 @@ extern "C" int xs_pc_apply(xs_ctx *c, const double *x, double *y, void *stream) 
   xs_op_pc(c, x, y, xs_ctx_vec(c, XS_NWORK - 1), nullptr);
   c->stream = keep;
   XS_HIP(hipGetLastError());
-  return XS_OK;
+  return xs_take_launch_guard() ? XS_ERR_STATE : XS_OK;
 }
 
 extern "C" int xs_pc_apply_t(xs_ctx *c, const double *x, double *y, void *stream) {
@@ -744,7 +767,7 @@ extern "C" int xs_trsv(xs_ctx *c, int which, const double *b, double *x, void *s
   if (which == 0) xs_trsv_lower(pick(c, stream), c->ilu, b, x, c->counters + 2, nullptr);
   else xs_trsv_upper(pick(c, stream), c->ilu, b, x, c->counters + 4, nullptr);
   XS_HIP(hipGetLastError());
-  return XS_OK;
+  return xs_take_launch_guard() ? XS_ERR_STATE : XS_OK;
 }
 
 extern "C" int xs_pc_get_factors(xs_ctx *c, int *rowptr, int *colidx, double *val, int *diagpos,

@@ -146,3 +146,5 @@ void xs_trsv_lower(hipStream_t s, const xs_ilu &F, const double *b, double *x, u
 void xs_trsv_upper(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
                    const xs_state *st);
 void xs_jacobi_dinv(hipStream_t s, const xs_csr &A, double *dinv, int *d_bad);
+// 1 if a launcher refused to launch since the last call (and clears it)
+int xs_take_launch_guard();

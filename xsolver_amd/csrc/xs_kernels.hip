@@ -12,7 +12,21 @@
 //    iterations launched ahead of the host's convergence check are no-ops.
 #include "xs_internal.h"
 
+#include <atomic>
 #include <climits>
+
+// Launch guard: a launcher that finds a required pointer null (a set-up
+// ordering bug) does not launch; it raises this flag, which the host driver
+// turns into an error instead of letting a kernel touch address 0.
+static std::atomic<int> g_launch_guard{0};
+int xs_take_launch_guard() { return g_launch_guard.exchange(0); }
+#define XS_GUARD(cond)                 \
+  do {                                 \
+    if (!(cond)) {                     \
+      g_launch_guard.store(1);         \
+      return;                          \
+    }                                  \
+  } while (0)
 
 // ---------------------------------------------------------------------------
 // Reductions
@@ -263,6 +277,10 @@ static xs_reduce no_reduce() {
 void xs_launch_spmv_csr(hipStream_t s, const xs_csr &A, const xs_spmv_plan &P,
                         const double *x, double *y, int epi, const double *b,
                         const double *w, const xs_reduce *red) {
+  XS_GUARD(A.rp && A.ci && A.v && P.blk_row && P.nblk > 0 && x && y);
+  XS_GUARD(!(epi == EPI_BMAX || epi == EPI_BMAXN) || b);
+  XS_GUARD(!(epi == EPI_DOT1 || epi == EPI_DOT2) || w);
+  XS_GUARD(!(epi == EPI_DOT1 || epi == EPI_DOT2 || epi == EPI_BMAXN) || (red && red->partial && red->counter && red->st));
   xs_reduce R = red ? *red : no_reduce();
   dim3 grid(P.nblk), blk(XS_BLOCK);
   switch (epi) {
@@ -276,6 +294,10 @@ void xs_launch_spmv_csr(hipStream_t s, const xs_csr &A, const xs_spmv_plan &P,
 
 void xs_launch_spmv_ell(hipStream_t s, const xs_ell &E, const double *x, double *y,
                         int epi, const double *b, const double *w, const xs_reduce *red) {
+  XS_GUARD(E.ci && E.v && E.ld % 2 == 0 && E.ld >= E.n && x && y);
+  XS_GUARD(!(epi == EPI_BMAX || epi == EPI_BMAXN) || b);
+  XS_GUARD(!(epi == EPI_DOT1 || epi == EPI_DOT2) || w);
+  XS_GUARD(!(epi == EPI_DOT1 || epi == EPI_DOT2 || epi == EPI_BMAXN) || (red && red->partial && red->counter && red->st));
   xs_reduce R = red ? *red : no_reduce();
   int threads = (E.n + 1) / 2;
   dim3 grid((threads + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
@@ -306,6 +328,7 @@ __global__ void k_build_plan(int n, const int *__restrict__ rp, int chunk, int n
 }
 
 void xs_build_plan(hipStream_t s, const xs_csr &A, xs_spmv_plan &P, int *blk) {
+  XS_GUARD(A.rp && blk);
   // nnz budget per block: about 256 rows at the average row length, a
   // multiple of 256, between 512 and 2048 so a straddling row still fits.
   double avg = A.n ? (double)A.nnz / A.n : 1.0;
@@ -331,6 +354,7 @@ __global__ void k_max_row_len(int n, const int *__restrict__ rp, int *out) {
 }
 
 int xs_max_row_len(hipStream_t s, const xs_csr &A, int *d_tmp) {
+  if (!A.rp || !d_tmp) { g_launch_guard.store(1); return 0; }
   int h = 0;
   (void)hipMemsetAsync(d_tmp, 0, sizeof(int), s);
   hipLaunchKernelGGL(k_max_row_len, dim3((A.n + 255) / 256), dim3(256), 0, s, A.n, A.rp, d_tmp);
@@ -355,6 +379,7 @@ __global__ void k_build_ell(int n, int ld, int W, const int *__restrict__ rp,
 }
 
 void xs_build_ell(hipStream_t s, const xs_csr &A, int W, xs_ell &E) {
+  XS_GUARD(A.rp && A.ci && A.v && E.ci && E.v);
   hipLaunchKernelGGL(k_build_ell, dim3((E.ld + 255) / 256), dim3(256), 0, s, A.n, E.ld, W, A.rp,
                      A.ci, A.v, E.ci, E.v);
 }
@@ -372,6 +397,7 @@ __global__ void k_spmv_t_scatter(int n, const int *__restrict__ rp, const int *_
 }
 
 void xs_launch_spmv_t(hipStream_t s, const xs_csr &A, const double *x, double *y) {
+  XS_GUARD(A.rp && A.ci && A.v && x && y);
   (void)hipMemsetAsync(y, 0, sizeof(double) * A.n, s);
   hipLaunchKernelGGL(k_spmv_t_scatter, dim3((A.n + 255) / 256), dim3(256), 0, s, A.n, A.rp, A.ci,
                      A.v, x, y);
@@ -392,6 +418,7 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long val)
 }
 
 void xs_fill_sentinel(hipStream_t s, double *x, int n) {
+  XS_GUARD(x);
   hipLaunchKernelGGL(k_fill_u64, dim3(vec_grid(n)), dim3(XS_BLOCK), 0, s,
                      reinterpret_cast<unsigned long long *>(x), n, XS_SENTINEL);
 }
@@ -408,6 +435,7 @@ __global__ void k_init_state(xs_state *st, double tol, double abstol, int maxite
 }
 
 void xs_init_state(hipStream_t s, xs_state *st, double tol, double abstol, int maxiter) {
+  XS_GUARD(st);
   hipLaunchKernelGGL(k_init_state, dim3(1), dim3(64), 0, s, st, tol, abstol, maxiter);
 }
 
@@ -430,6 +458,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_dot(int n, const double *__restric
 
 void xs_launch_dot(hipStream_t s, int n, const double *a, const double *b, const double *c,
                    const double *d, const xs_reduce &red) {
+  XS_GUARD(a && b && red.partial && red.counter && red.st && (!c || d));
   if (c) hipLaunchKernelGGL(k_dot<2>, dim3(vec_grid(n)), dim3(XS_BLOCK), 0, s, n, a, b, c, d, red);
   else hipLaunchKernelGGL(k_dot<1>, dim3(vec_grid(n)), dim3(XS_BLOCK), 0, s, n, a, b, c, d, red);
 }
@@ -452,6 +481,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_bicg_p(int n, const double *__rest
 
 void xs_bicg_p(hipStream_t s, int n, const double *r, const double *u, double *p,
                const double *dinv, double *z, xs_state *st) {
+  XS_GUARD(r && u && p && st && (!dinv || z));
   hipLaunchKernelGGL(k_bicg_p, dim3(vec_grid(n)), dim3(XS_BLOCK), 0, s, n, r, u, p, dinv, z, st);
 }
 
@@ -472,6 +502,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_bicg_s(int n, const double *__rest
 
 void xs_bicg_s(hipStream_t s, int n, const double *r, const double *u, double *sv,
                const double *dinv, double *z1, xs_state *st) {
+  XS_GUARD(r && u && sv && st && (!dinv || z1));
   hipLaunchKernelGGL(k_bicg_s, dim3(vec_grid(n)), dim3(XS_BLOCK), 0, s, n, r, u, sv, dinv, z1, st);
 }
 
@@ -500,6 +531,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_bicg_xr(int n, double *__restrict_
 
 void xs_bicg_xr(hipStream_t s, int n, double *x, double *r, const double *z, const double *z1,
                 const double *sv, const double *v, const double *rh, const xs_reduce &red) {
+  XS_GUARD(x && r && z && z1 && sv && v && rh && red.partial && red.counter && red.st);
   hipLaunchKernelGGL(k_bicg_xr, dim3(vec_grid(n)), dim3(XS_BLOCK), 0, s, n, x, r, z, z1, sv, v, rh, red);
 }
 
@@ -535,11 +567,13 @@ __global__ __launch_bounds__(XS_BLOCK) void k_cg_xr(int n, double *__restrict__ 
 
 void xs_cg_xr(hipStream_t s, int n, double *x, double *r, const double *p, const double *Ap,
               const double *dinv, double *z, const xs_reduce &red) {
+  XS_GUARD(x && r && p && Ap && (!dinv || z) && red.partial && red.counter && red.st);
   if (dinv) hipLaunchKernelGGL(k_cg_xr<true>, dim3(vec_grid(n)), dim3(XS_BLOCK), 0, s, n, x, r, p, Ap, dinv, z, red);
   else hipLaunchKernelGGL(k_cg_xr<false>, dim3(vec_grid(n)), dim3(XS_BLOCK), 0, s, n, x, r, p, Ap, dinv, z, red);
 }
 
 void xs_cg_zr_dot(hipStream_t s, int n, const double *z, const double *r, const xs_reduce &red) {
+  XS_GUARD(z && r && red.partial && red.counter && red.st);
   hipLaunchKernelGGL(k_dot<1>, dim3(vec_grid(n)), dim3(XS_BLOCK), 0, s, n, z, r,
                      (const double *)nullptr, (const double *)nullptr, red);
 }
@@ -554,6 +588,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_cg_p(int n, double *__restrict__ p
 }
 
 void xs_cg_p(hipStream_t s, int n, double *p, const double *z, xs_state *st) {
+  XS_GUARD(p && z && st);
   hipLaunchKernelGGL(k_cg_p, dim3(vec_grid(n)), dim3(XS_BLOCK), 0, s, n, p, z, st);
 }
 
@@ -562,12 +597,14 @@ __global__ void k_scale(int n, const double *__restrict__ d, const double *__res
   for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) y[i] = d[i] * x[i];
 }
 void xs_scale(hipStream_t s, int n, const double *d, const double *x, double *y) {
+  XS_GUARD(d && x && y);
   hipLaunchKernelGGL(k_scale, dim3(vec_grid(n)), dim3(XS_BLOCK), 0, s, n, d, x, y);
 }
 __global__ void k_copy(int n, const double *__restrict__ x, double *__restrict__ y) {
   for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) y[i] = x[i];
 }
 void xs_copy(hipStream_t s, int n, const double *x, double *y) {
+  XS_GUARD(x && y);
   hipLaunchKernelGGL(k_copy, dim3(vec_grid(n)), dim3(XS_BLOCK), 0, s, n, x, y);
 }
 
@@ -589,6 +626,7 @@ __global__ void k_jacobi_dinv(int n, const int *__restrict__ rp, const int *__re
 }
 
 void xs_jacobi_dinv(hipStream_t s, const xs_csr &A, double *dinv, int *d_bad) {
+  XS_GUARD(A.rp && A.ci && A.v && dinv && d_bad);
   hipLaunchKernelGGL(k_jacobi_dinv, dim3((A.n + 255) / 256), dim3(256), 0, s, A.n, A.rp, A.ci, A.v,
                      dinv, d_bad);
 }
@@ -645,12 +683,15 @@ __global__ __launch_bounds__(XS_BLOCK) void k_parilu_sweep(
 }
 
 void xs_parilu_sweep(hipStream_t s, const xs_csr &A, xs_ilu &F, double *lu_out, bool async) {
+  XS_GUARD(A.rp && A.ci && A.v && F.rowidx && F.dpos && F.ucp && F.uri && F.upos && F.lu);
+  XS_GUARD(async || lu_out);
   double *out = async ? F.lu : lu_out;
   hipLaunchKernelGGL(k_parilu_sweep, dim3((A.nnz + XS_BLOCK - 1) / XS_BLOCK), dim3(XS_BLOCK), 0, s,
                      A.nnz, F.rowidx, A.rp, A.ci, A.v, F.dpos, F.ucp, F.uri, F.upos, F.lu, out);
 }
 
 void xs_ilu_copy_values(hipStream_t s, const xs_csr &A, xs_ilu &F) {
+  XS_GUARD(A.v && F.lu);
   (void)hipMemcpyAsync(F.lu, A.v, sizeof(double) * A.nnz, hipMemcpyDeviceToDevice, s);
 }
 
@@ -664,6 +705,7 @@ __global__ void k_ilu_dinv(int n, const double *__restrict__ lu, const int *__re
 }
 
 void xs_ilu_dinv(hipStream_t s, xs_ilu &F, int *d_bad) {
+  XS_GUARD(F.lu && F.dpos && F.dinv && d_bad);
   hipLaunchKernelGGL(k_ilu_dinv, dim3((F.n + 255) / 256), dim3(256), 0, s, F.n, F.lu, F.dpos,
                      F.dinv, d_bad);
 }
@@ -833,6 +875,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
 static void trsv_launch(hipStream_t s, bool upper, int n, const int *rp, const int *ci,
                         const double *lu, const int *dpos, const double *dinv, const double *b,
                         double *x, unsigned *tickets, const xs_state *st) {
+  XS_GUARD(rp && ci && lu && dpos && (!upper || dinv) && b && x && tickets);
   const int nw = (n + XS_WAVE - 1) / XS_WAVE;
   dim3 grid((nw + (XS_BLOCK / XS_WAVE) - 1) / (XS_BLOCK / XS_WAVE)), blk(XS_BLOCK);
   if (upper) hipLaunchKernelGGL(k_trsv<true>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, st);
