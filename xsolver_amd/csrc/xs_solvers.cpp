@@ -9,6 +9,7 @@
 #include <cstdio>
 
 #include "../../include/xsolver_c.h"
+#include "xs_krylov.h"
 #include "xs_solvers.h"
 
 #define XS_CHK(call)                            \
@@ -103,14 +104,210 @@ int xs_solve_bicgstab(xs_ctx *c, const double *b, double *x) {
   });
 }
 
-int xs_solve_gcr(xs_ctx *, const double *, double *) { return XS_ERR_UNSUPPORTED; }
-int xs_solve_gmres(xs_ctx *, const double *, double *, bool) { return XS_ERR_UNSUPPORTED; }
+// ---------------------------------------------------------------------------
+// basis workspace
+// ---------------------------------------------------------------------------
+static int ensure_dev(double **p, size_t *len, size_t need) {
+  if (*p && *len >= need) return XS_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *len = 0;
+  if (hipMalloc((void **)p, sizeof(double) * (need ? need : 1)) != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return XS_ERR_MEMORY;
+  }
+  *len = need;
+  return XS_OK;
+}
+
+static int ensure_ts(xs_ctx *c) {
+  const size_t need = (size_t)std::max(xs_ts_blocks(c->A.n), XS_MAX_REDUCE_BLOCKS) * XS_TS_KMAX;
+  return ensure_dev(&c->ts_partial, &c->ts_partial_len, need);
+}
+
+static xs_gmres_small carve_small(double *base, int m) {
+  xs_gmres_small g;
+  double *p = base;
+  g.H = p; p += (size_t)(m + 1) * m;
+  g.cs = p; p += m + 1;
+  g.sn = p; p += m + 1;
+  g.s = p; p += m + 1;
+  g.h1 = p; p += m + 2;
+  g.h2 = p; p += m + 2;
+  g.y = p; p += m + 1;
+  g.misc = p;
+  return g;
+}
+static size_t small_size(int m) { return (size_t)(m + 1) * m + 6 * (size_t)(m + 2) + 16; }
+
+// GMRES(m) (flexible = false: left preconditioning,
+// intel/mpisolver-C-v1.cpp:933-1087) and FGMRES(m) (flexible = true: right
+// preconditioning with stored Z_i = M^{-1} V_i, :1089-1247).  Arnoldi with
+// CGS2 on the device; Hessenberg, Givens rotations, residual estimate and
+// back-substitution in FP64 on the device.  On convergence the update uses
+// all Krylov directions of the cycle.
+int xs_solve_gmres(xs_ctx *c, const double *b, double *x, bool flexible) {
+  int rc = xs_ctx_ensure_work(c, XS_NWORK);
+  if (rc) return rc;
+  const int n = c->A.n;
+  const int m = c->restart > 0 ? c->restart : 30;
+  hipStream_t s = c->stream;
+  const size_t ld = (size_t)n;
+  const size_t nbasis = (size_t)(m + 1) + (flexible ? (size_t)m : 0);
+  if ((rc = ensure_dev(&c->basis, &c->basis_len, nbasis * ld))) return rc;
+  if ((rc = ensure_dev(&c->small_d, &c->small_len, small_size(m)))) return rc;
+  if ((rc = ensure_ts(c))) return rc;
+  double *V = c->basis;
+  double *Z = flexible ? c->basis + (size_t)(m + 1) * ld : nullptr;
+  xs_gmres_small g = carve_small(c->small_d, m);
+  double *r = xs_ctx_vec(c, 0), *t = xs_ctx_vec(c, 1), *tmp = xs_ctx_vec(c, 2);
+  double *w = xs_ctx_vec(c, 3);
+  unsigned *tc = c->counters + 10;
+  double *P = c->ts_partial;
+  xs_state *st = c->st;
+  double *rv = c->resvec_d;
+
+  xs_init_state(s, st, c->tol, c->abstol, c->maxiter);
+  auto residual = [&]() {   // r = [M^{-1}] (b - A x)
+    xs_op_spmv(c, x, flexible ? r : t, EPI_BMAX, b, nullptr, nullptr);
+    if (!flexible) xs_op_pc(c, t, r, tmp, nullptr);
+  };
+  residual();
+  xs_ts_dot(s, n, 1, r, ld, r, P, tc, g.misc + XS_G_NRM2, nullptr);
+  xs_gmres_cycle(s, m, 1, g, st, rv);
+  xs_ts_scale(s, n, r, V, g.misc + XS_G_SCALE, st);
+
+  int launched = 0, i = 0, cycle_start = 0;
+  XS_CHK(hipMemcpyAsync(c->h_state, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
+  XS_CHK(hipStreamSynchronize(s));
+  bool done = c->h_state->done != 0;
+  while (!done && launched < c->maxiter) {
+    double *vi = V + (size_t)i * ld, *vn = V + (size_t)(i + 1) * ld;
+    if (flexible) {
+      double *zi = Z + (size_t)i * ld;
+      xs_op_pc(c, vi, zi, tmp, st);
+      xs_op_spmv(c, zi, w, EPI_Y, nullptr, nullptr, nullptr);
+    } else {
+      xs_op_spmv(c, vi, t, EPI_Y, nullptr, nullptr, nullptr);
+      xs_op_pc(c, t, w, tmp, st);
+    }
+    xs_ts_dot(s, n, i + 1, V, ld, w, P, tc, g.h1, st);
+    xs_ts_update(s, n, i + 1, V, nullptr, ld, g.h1, w, w, nullptr, st);
+    xs_ts_dot(s, n, i + 1, V, ld, w, P, tc, g.h2, st);
+    xs_ts_update(s, n, i + 1, V, nullptr, ld, g.h2, w, vn, nullptr, st);
+    xs_ts_dot(s, n, 1, vn, ld, vn, P, tc, g.misc + XS_G_NRM2, st);
+    xs_gmres_column(s, i, m, g, st, rv);
+    xs_ts_scale(s, n, vn, vn, g.misc + XS_G_SCALE, st);
+    i++;
+    launched++;
+    if (i == m || launched == c->maxiter || launched % c->check_every == 0) {
+      XS_CHK(hipGetLastError());
+      XS_CHK(hipMemcpyAsync(c->h_state, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
+      XS_CHK(hipStreamSynchronize(s));
+      done = c->h_state->done != 0;
+      if (done) break;
+    }
+    if (i == m && launched < c->maxiter) {      // restart (intel :1065-1070)
+      xs_gmres_backsub(s, m, m, g);
+      xs_ts_axpy(s, n, m, flexible ? Z : V, ld, g.y, x);
+      residual();
+      xs_gmres_cycle(s, m, 0, g, st, rv);
+      xs_ts_scale(s, n, r, V, g.misc + XS_G_SCALE, st);
+      i = 0;
+      cycle_start = launched;
+    }
+  }
+  const xs_state &h = *c->h_state;
+  int k;
+  if (h.done == 1) k = h.iter - cycle_start;   // directions of the converged cycle
+  else k = i;                                  // maxiter: use what this cycle built
+  if (h.done != 2 && k > 0) {
+    xs_gmres_backsub(s, k, m, g);
+    xs_ts_axpy(s, n, k, flexible ? Z : V, ld, g.y, x);
+  }
+  XS_CHK(hipGetLastError());
+  XS_CHK(hipStreamSynchronize(s));
+  c->iters = h.iter;
+  c->res0 = h.sc[S_RES0];
+  c->resl = h.sc[S_RESL];
+  if (h.done == 2) return XS_ERR_BREAKDOWN;
+  if (h.done == 1) return XS_OK;
+  return XS_NOT_CONVERGED;
+}
+
+// Right-preconditioned GCR(restart), hip/Solver/xsolver_igcr.cpp:28-108 (the
+// reference's default restart is maxiter, xsolver_igcr.h:36); c = A u is
+// orthogonalised against the stored c's with CGS2, the same coefficients
+// applied to u.
+int xs_solve_gcr(xs_ctx *c, const double *b, double *x) {
+  int rc = xs_ctx_ensure_work(c, XS_NWORK);
+  if (rc) return rc;
+  const int n = c->A.n;
+  const int R = c->restart > 0 ? c->restart : c->maxiter;
+  hipStream_t s = c->stream;
+  const size_t ld = (size_t)n;
+  if ((rc = ensure_dev(&c->basis, &c->basis_len, 2 * (size_t)R * ld))) return rc;
+  if ((rc = ensure_dev(&c->small_d, &c->small_len, 2 * (size_t)R + 16))) return rc;
+  if ((rc = ensure_ts(c))) return rc;
+  double *U = c->basis, *C = c->basis + (size_t)R * ld;
+  double *h1 = c->small_d, *h2 = c->small_d + R, *misc = c->small_d + 2 * R;
+  double *r = xs_ctx_vec(c, 0), *tmp = xs_ctx_vec(c, 1);
+  unsigned *tc = c->counters + 10;
+  double *P = c->ts_partial;
+  xs_state *st = c->st;
+  double *rv = c->resvec_d;
+
+  xs_init_state(s, st, c->tol, c->abstol, c->maxiter);
+  xs_op_spmv(c, x, r, EPI_BMAX, b, nullptr, nullptr);
+  xs_ts_dot(s, n, 1, r, ld, r, P, tc, misc + XS_G_NRM2, nullptr);
+  xs_res0_from(s, misc + XS_G_NRM2, st, rv);
+  int k = 0, launched = 0;
+  XS_CHK(hipMemcpyAsync(c->h_state, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
+  XS_CHK(hipStreamSynchronize(s));
+  bool done = c->h_state->done != 0;
+  double *t01 = misc + XS_G_T0;
+  while (!done && launched < c->maxiter) {
+    double *uk = U + (size_t)k * ld, *ck = C + (size_t)k * ld;
+    xs_op_pc(c, r, uk, tmp, st);                                   // u_k = M^-1 r
+    xs_op_spmv(c, uk, ck, EPI_Y, nullptr, nullptr, nullptr);       // c_k = A u_k
+    if (k > 0) {
+      xs_ts_dot(s, n, k, C, ld, ck, P, tc, h1, st);
+      xs_ts_update(s, n, k, C, U, ld, h1, ck, ck, uk, st);
+      xs_ts_dot(s, n, k, C, ld, ck, P, tc, h2, st);
+      xs_ts_update(s, n, k, C, U, ld, h2, ck, ck, uk, st);
+    }
+    xs_ts_dot(s, n, 1, ck, ld, ck, P, tc, t01, st);                // c.c
+    xs_ts_dot(s, n, 1, ck, ld, r, P, tc, t01 + 1, st);             // c.r
+    xs_gcr_apply(s, n, uk, ck, x, r, t01, P, tc, st, rv);          // x, r, ||r||
+    k = (k + 1 == R) ? 0 : k + 1;
+    launched++;
+    if (launched == c->maxiter || launched % c->check_every == 0) {
+      XS_CHK(hipGetLastError());
+      XS_CHK(hipMemcpyAsync(c->h_state, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
+      XS_CHK(hipStreamSynchronize(s));
+      done = c->h_state->done != 0;
+    }
+  }
+  const xs_state &h = *c->h_state;
+  c->iters = h.iter;
+  c->res0 = h.sc[S_RES0];
+  c->resl = h.sc[S_RESL];
+  if (h.done == 2) return XS_ERR_BREAKDOWN;
+  if (h.done == 1) return XS_OK;
+  return XS_NOT_CONVERGED;
+}
 int xs_parilut_factor(xs_ctx *) { return XS_ERR_UNSUPPORTED; }
 int xs_build_transpose(xs_ctx *) { return XS_ERR_UNSUPPORTED; }
 int xs_op_pc_t(xs_ctx *, const double *, double *, double *, hipStream_t) { return XS_ERR_UNSUPPORTED; }
 void xs_solvers_free(xs_ctx *c) {
   if (c->basis) (void)hipFree(c->basis);
   c->basis = nullptr;
+  c->basis_len = 0;
   if (c->small_d) (void)hipFree(c->small_d);
   c->small_d = nullptr;
+  c->small_len = 0;
+  if (c->ts_partial) (void)hipFree(c->ts_partial);
+  c->ts_partial = nullptr;
+  c->ts_partial_len = 0;
 }

@@ -64,6 +64,9 @@ struct xs_ctx {
   double *basis = nullptr;
   size_t basis_len = 0;
   double *small_d = nullptr;
+  size_t small_len = 0;
+  double *ts_partial = nullptr;
+  size_t ts_partial_len = 0;
 
   // kernel timing (HIP events on the context stream around launches of one
   // kernel family while a solve runs): 0 off, 1 SpMV, 2 lower trsv, 3 upper trsv
