@@ -55,7 +55,9 @@ def lib():
         L.or_solve.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _ip, _ip, _dp,
                                _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _c_int_p]
-        for f in ("or_jacobi_setup", "or_ilu0", "or_ilut", "or_solve"):
+        L.or_parilut.argtypes = [ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_int, _ip, _ip, _dp, _ip, _ip, _dp]
+        for f in ("or_jacobi_setup", "or_ilu0", "or_ilut", "or_solve", "or_parilut"):
             getattr(L, f).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -180,6 +182,21 @@ def parilu_sweep(A, L, Ucsc, n):
     lo = np.empty_like(lv); uo = np.empty_like(uv)
     lib().or_parilu_sweep(n, rp, ci, a, lrp, lci, lv, ucp, uri, uv, lo, uo)
     return lo, uo
+
+
+def parilut(A, sweeps=10, fill=2.0, droptol=1e-4):
+    """ParILUT factors: L (strict lower CSR), U (upper CSR, diagonal first)."""
+    rp, ci, v = _csr(A)
+    n = len(rp) - 1
+    cap = int((fill + 1.0) * len(v)) + n + 16
+    lrp = np.zeros(n + 1, np.int32); urp = np.zeros(n + 1, np.int32)
+    lci = np.zeros(cap, np.int32); uci = np.zeros(cap, np.int32)
+    lv = np.zeros(cap); uv = np.zeros(cap)
+    rc = lib().or_parilut(n, rp, ci, v, sweeps, fill, droptol, cap, lrp, lci, lv, urp, uci, uv)
+    if rc:
+        raise ValueError("ParILUT capacity exceeded")
+    return ((lrp, lci[:lrp[n]].copy(), lv[:lrp[n]].copy()),
+            (urp, uci[:urp[n]].copy(), uv[:urp[n]].copy()))
 
 
 def solve(A, b, ksp="cg", pc="jacobi", tol=1e-8, abstol=0.0, maxiter=1000,

@@ -675,6 +675,9 @@ done:
 /* Returns the solver status, or -1 for an unknown name / setup failure.     */
 /* ------------------------------------------------------------------------ */
 static int streq(const char *a, const char *b) { return strcmp(a, b) == 0; }
+int or_parilut(int n, const int *arp, const int *aci, const double *av, int sweeps,
+               double fill, double droptol, int cap, int *lrp, int *lci, double *lv,
+               int *urp, int *uci, double *uv);
 
 int or_solve(const char *ksp, const char *pc, int n, const int *rp,
              const int *ci, const double *v, const double *b, double *x,
@@ -720,6 +723,35 @@ int or_solve(const char *ksp, const char *pc, int n, const int *rp,
     lu.n = n; lu.lrp = lrp; lu.lci = lci; lu.lv = lv;
     lu.urp = urp; lu.uci = uci; lu.uv = uv; lu.dinv = dinv;
     M = or_lusolve; Mctx = &lu;
+  } else if (streq(pc, "parilut")) {
+    /* ParILUT: 10 outer steps (hip/Precond/precon_gpuilut.cpp:5), fill factor
+     * ilut_fill, relative drop tolerance ilut_tol; split as seqilut does. */
+    int cap = (int)((ilut_fill + 1.0) * rp[n]) + n + 16;
+    lrp = malloc(sizeof(int) * (n + 1)); urp = malloc(sizeof(int) * (n + 1));
+    lci = malloc(sizeof(int) * (size_t)cap); uci = malloc(sizeof(int) * (size_t)cap);
+    lv = malloc(sizeof(double) * (size_t)cap); uv = malloc(sizeof(double) * (size_t)cap);
+    dinv = malloc(sizeof(double) * n);
+    if (or_parilut(n, rp, ci, v, 10, (double)ilut_fill, ilut_tol, cap, lrp, lci, lv, urp, uci, uv))
+      goto out;
+    /* U rows: diagonal first; strict part scaled by 1/u_ii, stored from the
+     * second entry on (the split of hip/HostMatrix/HostMatrixCSR.cpp:230-268) */
+    for (int i = 0; i < n; i++) {
+      dinv[i] = 1.0 / uv[urp[i]];
+      for (int e = urp[i] + 1; e < urp[i + 1]; e++) uv[e] *= dinv[i];
+    }
+    /* drop the diagonal from U's index range: shift each row's pointer */
+    int *urp2 = malloc(sizeof(int) * (n + 1));
+    int w = 0;
+    urp2[0] = 0;
+    for (int i = 0; i < n; i++) {
+      for (int e = urp[i] + 1; e < urp[i + 1]; e++) { uci[w] = uci[e]; uv[w++] = uv[e]; }
+      urp2[i + 1] = w;
+    }
+    memcpy(urp, urp2, sizeof(int) * (n + 1));
+    free(urp2);
+    lu.n = n; lu.lrp = lrp; lu.lci = lci; lu.lv = lv;
+    lu.urp = urp; lu.uci = uci; lu.uv = uv; lu.dinv = dinv;
+    M = or_lusolve; Mctx = &lu;
   } else {
     goto out;
   }
@@ -731,5 +763,228 @@ int or_solve(const char *ksp, const char *pc, int n, const int *rp,
 out:
   free(dinv); free(luval); free(lv); free(uv); free(diag); free(uptr);
   free(lrp); free(lci); free(urp); free(uci); free(lucols); free(lurows);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ParILUT (deterministic, synchronous form)                                 */
+/* ------------------------------------------------------------------------ */
+/* Restates the structure of hip/Precond/parilut.cpp:621-1206 (parilut_clean):
+ * per outer step s (of `sweeps`):
+ *   1. candidates C_i = pattern(A_i) u L_i u U_i u (union over k in L_i of U_k)
+ *      (the pattern of A - L U, :760-767);
+ *   2. new candidate entries are initialised from R = A - L U:
+ *      l_ij = r_ij / u_jj, u_ij = r_ij; existing entries keep their values
+ *      (embedA_intoB_L2/U2, :178-233);
+ *   3. one fixed-point sweep over the candidates (LU_sweep, :508);
+ *   4. per factor, entries below the threshold are dropped: the threshold is
+ *      the value of rank (count - keep) in ascending |v| (find_kth_element,
+ *      select_kth.h:501-634; thresold_remove2, parilut.cpp:303-384) with
+ *      keep = (1 + (fill - 1) (s + 1) / sweeps) * (strict nnz of that factor
+ *      in A) (the reference's ramp, :692, with fill = 3); the diagonal of U
+ *      is always kept; additionally |v| < droptol * ||a_i||_2 is dropped
+ *      (the dual-threshold rule of ilutp_final, ilut_diag_in_csr.cpp:67-80);
+ *   5. one fixed-point sweep over the kept pattern (L_sweep/U_sweep, :1155-1169).
+ * Sweeps are Jacobi (old values in, new values out); the reference sweeps
+ * asynchronously in place.  The sum over k in each sweep runs in ascending k.
+ * Factors: L strict lower CSR (unit diagonal implied), U upper CSR including
+ * the diagonal (first in each row).  Returns 0, -1 on capacity overflow.
+ * The factorisation runs on the symmetrically scaled matrix
+ * S A S, S = |diag A|^{-1/2} (the reference starts from the unscaled values;
+ * from that start the synchronous sweep diverges on the 27-point lognormal
+ * systems), and the factors are scaled back: l_ij = l~_ij s_j / s_i,
+ * u_ij = u~_ij / (s_i s_j).  ILU is equivariant under diagonal scaling, so
+ * the fixed points are unchanged. */
+typedef struct { int n; int *rp, *ci; double *v; int cap; } or_fac_t;
+
+static int fac_find(const int *ci, int a, int b, int col) {
+  while (a < b) {
+    int m = (a + b) >> 1;
+    if (ci[m] < col) a = m + 1; else b = m;
+  }
+  return a;
+}
+static double fac_get(const or_fac_t *F, int row, int col) {
+  int a = F->rp[row], b = F->rp[row + 1];
+  int p = fac_find(F->ci, a, b, col);
+  return (p < b && F->ci[p] == col) ? F->v[p] : 0.0;
+}
+/* S_ij = sum_{k in L_i, k < min(i,j)} l_ik u_kj */
+static double lu_partial(const or_fac_t *L, const or_fac_t *U, int i, int j) {
+  int m = i < j ? i : j;
+  double s = 0.0;
+  for (int e = L->rp[i]; e < L->rp[i + 1]; e++) {
+    int k = L->ci[e];
+    if (k >= m) break;
+    s += L->v[e] * fac_get(U, k, j);
+  }
+  return s;
+}
+static double clamp_piv(double d) {
+  if (fabs(d) < 1e-10) d = d >= 0 ? 1e-10 : -1e-10;
+  return d;
+}
+/* one Jacobi sweep over the current patterns of L and U (in place via copies) */
+static void fac_sweep(int n, const int *arp, const int *aci, const double *av,
+                      or_fac_t *L, or_fac_t *U) {
+  double *lv = malloc(sizeof(double) * (size_t)(L->rp[n] + 1));
+  double *uv = malloc(sizeof(double) * (size_t)(U->rp[n] + 1));
+  for (int i = 0; i < n; i++) {
+    for (int e = L->rp[i]; e < L->rp[i + 1]; e++) {
+      int j = L->ci[e];
+      double a = 0.0;
+      int p = fac_find(aci, arp[i], arp[i + 1], j);
+      if (p < arp[i + 1] && aci[p] == j) a = av[p];
+      double t = (a - lu_partial(L, U, i, j)) / clamp_piv(U->v[U->rp[j]]);
+      lv[e] = isfinite(t) ? t : L->v[e];
+    }
+    for (int e = U->rp[i]; e < U->rp[i + 1]; e++) {
+      int j = U->ci[e];
+      double a = 0.0;
+      int p = fac_find(aci, arp[i], arp[i + 1], j);
+      if (p < arp[i + 1] && aci[p] == j) a = av[p];
+      double t = a - lu_partial(L, U, i, j);
+      if (i == j) t = clamp_piv(t);
+      uv[e] = isfinite(t) ? t : U->v[e];
+    }
+  }
+  memcpy(L->v, lv, sizeof(double) * (size_t)L->rp[n]);
+  memcpy(U->v, uv, sizeof(double) * (size_t)U->rp[n]);
+  free(lv); free(uv);
+}
+static int cmp_dbl(const void *a, const void *b) {
+  double x = *(const double *)a, y = *(const double *)b;
+  return (x > y) - (x < y);
+}
+/* value of rank k (0-based) in ascending |v| over the entries selected by
+ * `mask` (NULL = all) */
+static double kth_abs(const double *v, int m, int k) {
+  double *t = malloc(sizeof(double) * (size_t)(m + 1));
+  for (int i = 0; i < m; i++) t[i] = fabs(v[i]);
+  qsort(t, (size_t)m, sizeof(double), cmp_dbl);
+  double r = t[k];
+  free(t);
+  return r;
+}
+
+int or_parilut(int n, const int *arp, const int *aci, const double *av_in, int sweeps,
+               double fill, double droptol, int cap, int *lrp, int *lci, double *lv,
+               int *urp, int *uci, double *uv) {
+  or_fac_t L = {n, lrp, lci, lv, cap}, U = {n, urp, uci, uv, cap};
+  double *sc = malloc(sizeof(double) * n);
+  double *av = malloc(sizeof(double) * (size_t)(arp[n] + 1));
+  for (int i = 0; i < n; i++) {
+    double d = 0.0;
+    for (int e = arp[i]; e < arp[i + 1]; e++) if (aci[e] == i) d = av_in[e];
+    sc[i] = d != 0.0 ? 1.0 / sqrt(fabs(d)) : 1.0;
+  }
+  for (int i = 0; i < n; i++)
+    for (int e = arp[i]; e < arp[i + 1]; e++) av[e] = av_in[e] * sc[i] * sc[aci[e]];
+  int nl0 = 0, nu0 = 0;
+  lrp[0] = urp[0] = 0;
+  for (int i = 0; i < n; i++) {
+    for (int e = arp[i]; e < arp[i + 1]; e++) {
+      if (aci[e] < i) { lci[nl0] = aci[e]; lv[nl0++] = av[e]; }
+      else { uci[nu0] = aci[e]; uv[nu0++] = av[e]; }
+    }
+    lrp[i + 1] = nl0; urp[i + 1] = nu0;
+  }
+  const int strictU0 = nu0 - n;
+  double *anorm = malloc(sizeof(double) * n);
+  for (int i = 0; i < n; i++) {
+    double s = 0.0;
+    for (int e = arp[i]; e < arp[i + 1]; e++) s += av[e] * av[e];
+    anorm[i] = sqrt(s);
+  }
+  int *mark = malloc(sizeof(int) * n);
+  int *list = malloc(sizeof(int) * n);
+  for (int i = 0; i < n; i++) mark[i] = -1;
+  int rc = 0;
+  for (int s = 0; s < sweeps && rc == 0; s++) {
+    /* 1-2: candidate pattern and embedded values (new arrays) */
+    int *nlrp = malloc(sizeof(int) * (n + 1)), *nurp = malloc(sizeof(int) * (n + 1));
+    int *nlci = malloc(sizeof(int) * (size_t)cap), *nuci = malloc(sizeof(int) * (size_t)cap);
+    double *nlv = malloc(sizeof(double) * (size_t)cap), *nuv = malloc(sizeof(double) * (size_t)cap);
+    int nl = 0, nu = 0;
+    nlrp[0] = nurp[0] = 0;
+    for (int i = 0; i < n && rc == 0; i++) {
+      int cnt = 0;
+#define ADDC(c) do { int c_ = (c); if (mark[c_] != i) { mark[c_] = i; list[cnt++] = c_; } } while (0)
+      for (int e = arp[i]; e < arp[i + 1]; e++) ADDC(aci[e]);
+      for (int e = L.rp[i]; e < L.rp[i + 1]; e++) {
+        ADDC(L.ci[e]);
+        int k = L.ci[e];
+        for (int q = U.rp[k]; q < U.rp[k + 1]; q++) ADDC(U.ci[q]);
+      }
+      for (int e = U.rp[i]; e < U.rp[i + 1]; e++) ADDC(U.ci[e]);
+#undef ADDC
+      /* sort the row's candidate columns */
+      for (int a = 1; a < cnt; a++) {
+        int key = list[a], b = a - 1;
+        while (b >= 0 && list[b] > key) { list[b + 1] = list[b]; b--; }
+        list[b + 1] = key;
+      }
+      if (nl + cnt > cap || nu + cnt > cap) { rc = -1; break; }
+      for (int t = 0; t < cnt; t++) {
+        int j = list[t];
+        double a = 0.0;
+        int p = fac_find(aci, arp[i], arp[i + 1], j);
+        if (p < arp[i + 1] && aci[p] == j) a = av[p];
+        if (j < i) {
+          int q = fac_find(L.ci, L.rp[i], L.rp[i + 1], j);
+          double val;
+          if (q < L.rp[i + 1] && L.ci[q] == j) val = L.v[q];
+          else val = (a - lu_partial(&L, &U, i, j)) / clamp_piv(U.v[U.rp[j]]);
+          nlci[nl] = j; nlv[nl++] = isfinite(val) ? val : 0.0;
+        } else {
+          int q = fac_find(U.ci, U.rp[i], U.rp[i + 1], j);
+          double val;
+          if (q < U.rp[i + 1] && U.ci[q] == j) val = U.v[q];
+          else val = a - lu_partial(&L, &U, i, j);
+          nuci[nu] = j; nuv[nu++] = isfinite(val) ? val : 0.0;
+        }
+      }
+      nlrp[i + 1] = nl; nurp[i + 1] = nu;
+    }
+    if (rc) { free(nlrp); free(nurp); free(nlci); free(nuci); free(nlv); free(nuv); break; }
+    or_fac_t CL = {n, nlrp, nlci, nlv, cap}, CU = {n, nurp, nuci, nuv, cap};
+    /* 3: sweep over the candidates */
+    fac_sweep(n, arp, aci, av, &CL, &CU);
+    /* 4: thresholds */
+    double rate = 1.0 + (fill - 1.0) * (double)(s + 1) / (double)sweeps;
+    long keepL = (long)(rate * (double)nl0), keepU = (long)(rate * (double)strictU0);
+    double tL = 0.0, tU = 0.0;
+    if (nl > keepL) tL = kth_abs(nlv, nl, (int)(nl - keepL));
+    {
+      double *su = malloc(sizeof(double) * (size_t)(nu + 1));
+      int m = 0;
+      for (int i = 0; i < n; i++)
+        for (int e = nurp[i] + 1; e < nurp[i + 1]; e++) su[m++] = nuv[e];
+      if (m > keepU) tU = kth_abs(su, m, (int)(m - keepU));
+      free(su);
+    }
+    int ol = 0, ou = 0;
+    for (int i = 0; i < n; i++) {
+      double dt = droptol * anorm[i];
+      for (int e = nlrp[i]; e < nlrp[i + 1]; e++) {
+        double a = fabs(nlv[e]);
+        if (a >= tL && a >= dt) { L.ci[ol] = nlci[e]; L.v[ol++] = nlv[e]; }
+      }
+      L.rp[i + 1] = ol;
+      for (int e = nurp[i]; e < nurp[i + 1]; e++) {
+        double a = fabs(nuv[e]);
+        if (nuci[e] == i || (a >= tU && a >= dt)) { U.ci[ou] = nuci[e]; U.v[ou++] = nuv[e]; }
+      }
+      U.rp[i + 1] = ou;
+    }
+    free(nlrp); free(nurp); free(nlci); free(nuci); free(nlv); free(nuv);
+    /* 5: sweep over the kept pattern */
+    fac_sweep(n, arp, aci, av, &L, &U);
+  }
+  for (int i = 0; i < n; i++) {
+    for (int e = L.rp[i]; e < L.rp[i + 1]; e++) L.v[e] *= sc[L.ci[e]] / sc[i];
+    for (int e = U.rp[i]; e < U.rp[i + 1]; e++) U.v[e] /= sc[i] * sc[U.ci[e]];
+  }
+  free(anorm); free(mark); free(list); free(sc); free(av);
   return rc;
 }

@@ -79,6 +79,11 @@ int or_gmres(int n, const int *rp, const int *ci, const double *v,
              int flexible, or_precond_fn M, void *Mctx, double *resvec,
              int *usediter);
 
+/* ---- ParILUT (deterministic synchronous form) --------------------------- */
+int or_parilut(int n, const int *arp, const int *aci, const double *av, int sweeps,
+               double fill, double droptol, int cap, int *lrp, int *lci, double *lv,
+               int *urp, int *uci, double *uv);
+
 /* ---- driver: pc in {"none","jacobi","ilu0","ilut"},
  *      ksp in {"cg","bicgstab","gcr","gmres","fgmres"} ------------------- */
 int or_solve(const char *ksp, const char *pc, int n, const int *rp,

@@ -292,3 +292,46 @@ def test_jacobi_zero_pivot_reported():
     v[rp[2] + 1] = 0.0  # a_22 sits second in row 2 (cols 1,2,3,6)
     d, bad = oracle.jacobi_dinv((rp, ci, v))
     assert bad == 3
+
+
+def test_parilut_without_dropping_is_exact_lu():
+    # keep everything (fill 1000, droptol 0): the candidate pattern grows to
+    # the full fill of LU and the sweeps converge to the exact factors.
+    A = gen.lognormal27(3, seed=2)
+    n = len(A[0]) - 1
+    L, U = oracle.parilut(A, 40, 1000.0, 0.0)
+    Ld = np.eye(n) + sp.csr_matrix((L[2], L[1], L[0]), shape=(n, n)).toarray()
+    Ud = sp.csr_matrix((U[2], U[1], U[0]), shape=(n, n)).toarray()
+    D = dense(A)
+    assert np.abs(Ld @ Ud - D).max() <= 1e-10 * np.abs(D).max()
+    assert np.allclose(np.tril(Ld, -1), np.tril(Ld, -1))
+
+
+def test_parilut_invariants():
+    A = gen.lognormal27(7, seed=2)
+    rp, ci, v = A
+    n = len(rp) - 1
+    L, U = oracle.parilut(A, 10, 2.0, 1e-4)
+    nl0 = sum(1 for i in range(n) for c in ci[rp[i]:rp[i + 1]] if c < i)
+    nu0 = sum(1 for i in range(n) for c in ci[rp[i]:rp[i + 1]] if c > i)
+    # L strictly lower, U upper with its pivot first, sorted rows
+    for i in range(n):
+        lc = L[1][L[0][i]:L[0][i + 1]]
+        uc = U[1][U[0][i]:U[0][i + 1]]
+        assert np.all(lc < i) and np.all(np.diff(lc) > 0)
+        assert uc[0] == i and np.all(np.diff(uc) > 0)
+    # per-factor counts within the final keep (ties may add a few)
+    assert len(L[2]) <= 2 * nl0 + 8
+    assert len(U[2]) - n <= 2 * nu0 + 8
+    # residual A - LU on the kept pattern is small relative to A
+    Ld = sp.csr_matrix((L[2], L[1], L[0]), shape=(n, n)) + sp.eye(n)
+    Ud = sp.csr_matrix((U[2], U[1], U[0]), shape=(n, n))
+    As = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    R = (As - Ld @ Ud).multiply((abs(Ld) + abs(Ud)) != 0)
+    assert abs(R).max() < 1e-2 * abs(As).max()
+    # as a preconditioner it beats ILU(0) on this system
+    xs, b = gen.manufactured(A, [2, 1])
+    _, s1, it1, _ = oracle.solve(A, b, "fgmres", "ilu0", tol=1e-10, maxiter=300, restart=30)
+    _, s2, it2, _ = oracle.solve(A, b, "fgmres", "parilut", tol=1e-10, maxiter=300, restart=30,
+                                 ilut_tol=1e-4, ilut_fill=2)
+    assert s1 == 0 and s2 == 0 and it2 < it1

@@ -18,6 +18,7 @@
 
 #include "../../include/xsolver_c.h"
 #include "xs_internal.h"
+#include "xs_parilut.h"
 #include "xs_solvers.h"
 
 // ---------------------------------------------------------------------------
@@ -64,7 +65,7 @@ static void free_precond(xs_ctx *c) {
   xs_ilu &F = c->ilu;
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.ucp); dfree(F.uri); dfree(F.upos);
   dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own); dfree(F.trp); dfree(F.tci);
-  dfree(F.tdpos); dfree(F.tlu);
+  dfree(F.tdpos); dfree(F.tlu); dfree(F.scale);
   F = xs_ilu();
   dfree(c->lu_tmp);
   c->pc_ready = false;
@@ -396,6 +397,8 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   else if (k == "parilu_async") { c->parilu_async = value != 0; c->pc_ready = false; }
   else if (k == "check_every") { if (value < 1) return XS_ERR_ARG; c->check_every = (int)value; }
   else if (k == "refactor") { c->pc_ready = false; }
+  else if (k == "parilu_scale") { c->parilu_scale = value != 0; c->pc_ready = false; }
+  else if (k == "parilut_steps") { if (value < 1) return XS_ERR_ARG; c->parilut_steps = (int)value; c->pc_ready = false; }
   else if (k == "profile") { c->prof_kind = (int)value; for (int i = 0; i < 4; i++) { c->prof_ms[i] = 0; c->prof_cnt[i] = 0; } }
   else if (k == "precision") { if (value != 0 && value != 1) return XS_ERR_ARG; c->precision = (int)value; }
   else if (k == "use_ell") {
@@ -461,6 +464,11 @@ static int setup_ilu_pattern(xs_ctx *c) {
   return XS_OK;
 }
 
+// ParILU(0) (hip/Precond/gpuparilu.cpp:150-165, 20 sweeps by default).  With
+// parilu_scale (default) the sweeps run on S A S, S = |diag A|^{-1/2}, and the
+// factors are scaled back; ILU(0) is unchanged by the scaling, but from the
+// reference's unscaled start the sweeps converge slowly or diverge on the
+// 27-point lognormal systems.
 static int factor_parilu(xs_ctx *c) {
   xs_ilu &F = c->ilu;
   if (!F.lu) {
@@ -472,19 +480,159 @@ static int factor_parilu(xs_ctx *c) {
     int rc = dalloc(&c->lu_tmp, c->A.nnz);
     if (rc) return rc;
   }
+  const double *sc = nullptr;
+  if (c->parilu_scale) {
+    if (!F.scale) {
+      int rc = dalloc(&F.scale, c->A.n);
+      if (rc) return rc;
+    }
+    xs_pt_scale_vec(c->stream, c->A.n, c->A.rp, c->A.ci, c->A.v, F.scale);
+    xs_pt_rescale(c->stream, c->A.nnz, F.rowidx, c->A.ci, F.lu, F.scale, 0);
+    sc = F.scale;
+  }
   for (int s = 0; s < c->parilu_sweeps; s++) {
     if (c->parilu_async) {
-      xs_parilu_sweep(c->stream, c->A, F, nullptr, true);
+      xs_parilu_sweep(c->stream, c->A, F, nullptr, true, sc);
     } else {
-      xs_parilu_sweep(c->stream, c->A, F, c->lu_tmp, false);
+      xs_parilu_sweep(c->stream, c->A, F, c->lu_tmp, false, sc);
       std::swap(F.lu, c->lu_tmp);
     }
   }
+  if (sc) xs_pt_rescale(c->stream, c->A.nnz, F.rowidx, c->A.ci, F.lu, F.scale, 1);
   XS_HIP(hipMemsetAsync(c->d_int, 0x7f, sizeof(int), c->stream));
   xs_ilu_dinv(c->stream, F, c->d_int);
   int bad = 0;
   XS_HIP(hipMemcpyAsync(&bad, c->d_int, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   XS_HIP(hipStreamSynchronize(c->stream));
+  if (bad != 0x7f7f7f7f) return XS_ERR_PIVOT;
+  return XS_OK;
+}
+
+// ParILUT (hip/Precond/parilut.cpp:621-1206, precon_gpuilut.cpp:5: 10 outer
+// steps): kernels in xs_parilut.hip.  Fill factor = ilut_fill (final keep
+// rate per factor relative to A's strict parts), relative drop tolerance =
+// ilut_tol.  The factor pattern is owned by the context (rp_own/ci_own).
+int xs_parilut_factor(xs_ctx *c) {
+  hipStream_t s = c->stream;
+  const int n = c->A.n, nnz = c->A.nnz;
+  xs_ilu &F = c->ilu;
+  // free a previous factor
+  dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own);
+  dfree(F.ucp); dfree(F.uri); dfree(F.upos);
+  int rc;
+  double *dtol = nullptr, *thr = nullptr;
+  int *cnt = nullptr, *bsum = nullptr, *ovf = nullptr;
+  unsigned long long *sel = nullptr, *cnt2 = nullptr;
+  unsigned *hist = nullptr;
+  int *crp = nullptr, *cci = nullptr, *crow = nullptr, *cdpos = nullptr;
+  double *cv = nullptr, *cv2 = nullptr, *fv2 = nullptr;
+  auto cleanup = [&]() {
+    dfree(dtol); dfree(thr); dfree(cnt); dfree(bsum); dfree(ovf); dfree(sel); dfree(cnt2);
+    dfree(hist); dfree(crp); dfree(cci); dfree(crow); dfree(cdpos); dfree(cv); dfree(cv2); dfree(fv2);
+  };
+#define PT_TRY(x) do { if ((rc = (x))) { cleanup(); return rc; } } while (0)
+  PT_TRY(F.scale ? XS_OK : dalloc(&F.scale, n));
+  PT_TRY(dalloc(&dtol, n));
+  PT_TRY(dalloc(&thr, 2));
+  PT_TRY(dalloc(&cnt, n));
+  PT_TRY(dalloc(&bsum, xs_scan_bsum_len(n)));
+  PT_TRY(dalloc(&ovf, 1));
+  PT_TRY(dalloc(&sel, 3));
+  PT_TRY(dalloc(&cnt2, 2));
+  PT_TRY(dalloc(&hist, 256));
+  PT_TRY(dalloc(&crp, n + 1));
+  PT_TRY(dalloc(&cdpos, n));
+  PT_TRY(dalloc(&F.rp_own, n + 1));
+  PT_TRY(dalloc(&F.ci_own, nnz));
+  PT_TRY(dalloc(&F.lu, nnz));
+  PT_TRY(dalloc(&F.rowidx, nnz));
+  PT_TRY(dalloc(&F.dpos, n));
+  PT_TRY(dalloc(&F.dinv, n));
+  XS_HIP(hipMemsetAsync(ovf, 0, sizeof(int), s));
+  // initial factor = A (scaled): L = strict lower part, U = upper part
+  XS_HIP(hipMemcpyAsync(F.rp_own, c->A.rp, sizeof(int) * (n + 1), hipMemcpyDeviceToDevice, s));
+  XS_HIP(hipMemcpyAsync(F.ci_own, c->A.ci, sizeof(int) * nnz, hipMemcpyDeviceToDevice, s));
+  XS_HIP(hipMemcpyAsync(F.lu, c->A.v, sizeof(double) * nnz, hipMemcpyDeviceToDevice, s));
+  xs_pt_rowidx(s, n, F.rp_own, F.rowidx);
+  xs_pt_scale_vec(s, n, c->A.rp, c->A.ci, c->A.v, F.scale);
+  xs_pt_rescale(s, nnz, F.rowidx, F.ci_own, F.lu, F.scale, 0);
+  xs_pt_dtol(s, n, c->A.rp, c->A.ci, c->A.v, F.scale, c->ilut_tol, dtol);
+  xs_pt_dpos(s, n, F.rp_own, F.ci_own, F.dpos, ovf);
+  long nl0 = 0, nu0 = 0;
+  for (int i = 0; i < n; i++)
+    for (int e = c->h_rp[i]; e < c->h_rp[i + 1]; e++) {
+      if (c->h_ci[e] < i) nl0++;
+      else if (c->h_ci[e] > i) nu0++;
+    }
+  int fnnz = nnz;
+  const int steps = c->parilut_steps;
+  const double fill = (double)c->ilut_fill;
+  for (int step = 0; step < steps; step++) {
+    const double rate = 1.0 + (fill - 1.0) * (double)(step + 1) / (double)steps;
+    const long keepL = (long)(rate * (double)nl0), keepU = (long)(rate * (double)nu0);
+    // 1. candidate pattern
+    xs_pt_symbolic(s, false, n, c->A.rp, c->A.ci, F.rp_own, F.ci_own, F.dpos, cnt, nullptr,
+                   nullptr, nullptr, ovf);
+    xs_scan_excl(s, n, cnt, crp, bsum);
+    int hdr[2] = {0, 0};
+    XS_HIP(hipMemcpyAsync(&hdr[0], crp + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    XS_HIP(hipMemcpyAsync(&hdr[1], ovf, sizeof(int), hipMemcpyDeviceToHost, s));
+    XS_HIP(hipStreamSynchronize(s));
+    if (hdr[1]) { snprintf(xs_errbuf, sizeof(xs_errbuf), "ParILUT: pattern overflow"); cleanup(); return XS_ERR_MEMORY; }
+    const int nc = hdr[0];
+    dfree(cci); dfree(crow); dfree(cv); dfree(cv2);
+    PT_TRY(dalloc(&cci, nc));
+    PT_TRY(dalloc(&crow, nc));
+    PT_TRY(dalloc(&cv, nc));
+    PT_TRY(dalloc(&cv2, nc));
+    xs_pt_symbolic(s, true, n, c->A.rp, c->A.ci, F.rp_own, F.ci_own, F.dpos, cnt, crp, cci, crow, ovf);
+    xs_pt_dpos(s, n, crp, cci, cdpos, ovf);
+    // 2-3. embed and sweep over the candidates
+    xs_pt_embed(s, nc, crow, cci, c->A.rp, c->A.ci, c->A.v, F.scale, F.rp_own, F.ci_own, F.lu,
+                F.dpos, cv);
+    xs_pt_sweep(s, nc, crow, crp, cci, cdpos, cv, cv2, c->A.rp, c->A.ci, c->A.v, F.scale);
+    std::swap(cv, cv2);
+    // 4. thresholds
+    xs_pt_count_parts(s, nc, crow, cci, cnt2);
+    unsigned long long parts[2] = {0, 0};
+    XS_HIP(hipMemcpyAsync(parts, cnt2, sizeof(parts), hipMemcpyDeviceToHost, s));
+    XS_HIP(hipStreamSynchronize(s));
+    const bool dropL = (long long)parts[0] > keepL, dropU = (long long)parts[1] > keepU;
+    if (dropL) xs_pt_select(s, nc, crow, cci, cv, 0, parts[0] - (unsigned long long)keepL, sel, hist);
+    xs_pt_thr(s, sel, dropL, thr);
+    if (dropU) xs_pt_select(s, nc, crow, cci, cv, 2, parts[1] - (unsigned long long)keepU, sel, hist);
+    xs_pt_thr(s, sel, dropU, thr + 1);
+    // compaction into the new factor
+    xs_pt_keep(s, false, n, crp, cci, cv, thr, dtol, cnt, nullptr, nullptr, nullptr, nullptr);
+    xs_scan_excl(s, n, cnt, F.rp_own, bsum);
+    XS_HIP(hipMemcpyAsync(&fnnz, F.rp_own + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    XS_HIP(hipStreamSynchronize(s));
+    dfree(F.ci_own); dfree(F.lu); dfree(F.rowidx); dfree(fv2);
+    PT_TRY(dalloc(&F.ci_own, fnnz));
+    PT_TRY(dalloc(&F.lu, fnnz));
+    PT_TRY(dalloc(&F.rowidx, fnnz));
+    PT_TRY(dalloc(&fv2, fnnz));
+    xs_pt_keep(s, true, n, crp, cci, cv, thr, dtol, nullptr, F.rp_own, F.ci_own, F.lu, F.rowidx);
+    xs_pt_dpos(s, n, F.rp_own, F.ci_own, F.dpos, ovf);
+    // 5. sweep over the kept pattern
+    xs_pt_sweep(s, fnnz, F.rowidx, F.rp_own, F.ci_own, F.dpos, F.lu, fv2, c->A.rp, c->A.ci,
+                c->A.v, F.scale);
+    std::swap(F.lu, fv2);
+  }
+  xs_pt_rescale(s, fnnz, F.rowidx, F.ci_own, F.lu, F.scale, 1);
+  F.n = n;
+  F.nnz = fnnz;
+  F.rp = F.rp_own;
+  F.ci = F.ci_own;
+  XS_HIP(hipMemsetAsync(c->d_int, 0x7f, sizeof(int), s));
+  xs_ilu_dinv(s, F, c->d_int);
+  int bad = 0, ov = 0;
+  XS_HIP(hipMemcpyAsync(&bad, c->d_int, sizeof(int), hipMemcpyDeviceToHost, s));
+  XS_HIP(hipMemcpyAsync(&ov, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
+  XS_HIP(hipStreamSynchronize(s));
+  cleanup();
+#undef PT_TRY
+  if (ov) return XS_ERR_MEMORY;
   if (bad != 0x7f7f7f7f) return XS_ERR_PIVOT;
   return XS_OK;
 }

@@ -91,6 +91,7 @@ struct xs_ilu {
   double *lu = nullptr;     // factor values
   int *dpos = nullptr;      // pivot positions
   double *dinv = nullptr;   // 1 / u_ii
+  double *scale = nullptr;  // |a_ii|^{-1/2} used while factorising
   // CSC of the upper triangle (incl. diagonal) for the ParILU sweep:
   int *ucp = nullptr, *uri = nullptr, *upos = nullptr;
   int *rowidx = nullptr;    // row of each nnz (COO)
@@ -139,7 +140,8 @@ void xs_init_state(hipStream_t s, xs_state *st, double tol, double abstol, int m
 
 // ILU
 void xs_ilu_copy_values(hipStream_t s, const xs_csr &A, xs_ilu &F);
-void xs_parilu_sweep(hipStream_t s, const xs_csr &A, xs_ilu &F, double *lu_out, bool async);
+void xs_parilu_sweep(hipStream_t s, const xs_csr &A, xs_ilu &F, double *lu_out, bool async,
+                     const double *sc);
 void xs_ilu_dinv(hipStream_t s, xs_ilu &F, int *d_bad);
 void xs_trsv_lower(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
                    const xs_state *st);

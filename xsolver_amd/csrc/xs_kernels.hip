@@ -641,18 +641,19 @@ void xs_jacobi_dinv(hipStream_t s, const xs_csr &A, double *dinv, int *d_bad) {
 //   i <= j: u_ij = s         (|u_ii| < 1e-10 -> +-1e-10)
 //   non-finite results keep the previous value.
 // async = true updates in place (the reference's behaviour); false reads
-// lu_in and writes lu_out (deterministic Jacobi sweep).
+// lu_in and writes lu_out (deterministic Jacobi sweep).  With sc, the sweep
+// factors the symmetrically scaled matrix S A S (s = |a_ii|^{-1/2}).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(XS_BLOCK) void k_parilu_sweep(
     int nnz, const int *__restrict__ rowidx, const int *__restrict__ rp, const int *__restrict__ ci,
     const double *__restrict__ a, const int *__restrict__ dpos, const int *__restrict__ ucp,
     const int *__restrict__ uri, const int *__restrict__ upos, const double *lu_in,
-    double *lu_out) {
+    double *lu_out, const double *__restrict__ sc) {
   const int k = blockIdx.x * XS_BLOCK + threadIdx.x;
   if (k >= nnz) return;
   const int i = rowidx[k], j = ci[k];
   const int m = i < j ? i : j;
-  double s = a[k];
+  double s = sc ? a[k] * sc[i] * sc[j] : a[k];
   int il = rp[i];
   const int le = dpos[i];
   int iu = ucp[j];
@@ -682,12 +683,13 @@ __global__ __launch_bounds__(XS_BLOCK) void k_parilu_sweep(
   lu_out[k] = isfinite(nv) ? nv : old;
 }
 
-void xs_parilu_sweep(hipStream_t s, const xs_csr &A, xs_ilu &F, double *lu_out, bool async) {
+void xs_parilu_sweep(hipStream_t s, const xs_csr &A, xs_ilu &F, double *lu_out, bool async,
+                     const double *sc) {
   XS_GUARD(A.rp && A.ci && A.v && F.rowidx && F.dpos && F.ucp && F.uri && F.upos && F.lu);
   XS_GUARD(async || lu_out);
   double *out = async ? F.lu : lu_out;
   hipLaunchKernelGGL(k_parilu_sweep, dim3((A.nnz + XS_BLOCK - 1) / XS_BLOCK), dim3(XS_BLOCK), 0, s,
-                     A.nnz, F.rowidx, A.rp, A.ci, A.v, F.dpos, F.ucp, F.uri, F.upos, F.lu, out);
+                     A.nnz, F.rowidx, A.rp, A.ci, A.v, F.dpos, F.ucp, F.uri, F.upos, F.lu, out, sc);
 }
 
 void xs_ilu_copy_values(hipStream_t s, const xs_csr &A, xs_ilu &F) {

@@ -1,0 +1,31 @@
+// xs_parilut.h -- ParILUT kernels (xs_parilut.hip) and shared scan/scale helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+void xs_scan_excl(hipStream_t s, int n, const int *in, int *out, int *bsum);
+int xs_scan_bsum_len(int n);
+void xs_pt_symbolic(hipStream_t s, bool fill, int n, const int *arp, const int *aci,
+                    const int *frp, const int *fci, const int *fdpos, int *cnt, const int *crp,
+                    int *cci, int *crow, int *ovf);
+void xs_pt_dpos(hipStream_t s, int n, const int *rp, const int *ci, int *dpos, int *bad);
+void xs_pt_embed(hipStream_t s, int nc, const int *crow, const int *cci, const int *arp,
+                 const int *aci, const double *av, const double *sc, const int *frp,
+                 const int *fci, const double *fv, const int *fdpos, double *cv);
+void xs_pt_sweep(hipStream_t s, int nnz, const int *row, const int *rp, const int *ci,
+                 const int *dpos, const double *vin, double *vout, const int *arp,
+                 const int *aci, const double *av, const double *sc);
+void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const double *v,
+                  int part, unsigned long long k, unsigned long long *sel, unsigned *hist);
+void xs_pt_count_parts(hipStream_t s, int nnz, const int *row, const int *ci,
+                       unsigned long long *cnt2);
+void xs_pt_thr(hipStream_t s, const unsigned long long *sel, int use, double *thr_slot);
+void xs_pt_dtol(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
+                const double *sc, double droptol, double *dtol);
+void xs_pt_keep(hipStream_t s, bool write, int n, const int *crp, const int *cci,
+                const double *cv, const double *thr, const double *dtol, int *cnt,
+                const int *frp, int *fci, double *fv, int *frow);
+void xs_pt_scale_vec(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
+                     double *sc);
+void xs_pt_rescale(hipStream_t s, int nnz, const int *row, const int *ci, double *v,
+                   const double *sc, int unscale);
+void xs_pt_rowidx(hipStream_t s, int n, const int *rp, int *row);

@@ -1,0 +1,480 @@
+// xs_parilut.hip -- ParILUT on the device (threshold-based incomplete LU by
+// fixed-point sweeps over a changing pattern).
+//
+// Structure of the reference's parilut_clean (hip/Precond/parilut.cpp:621-1206),
+// redesigned for gfx950: all factors live in one combined CSR F (row i: strict
+// lower part = L without its unit diagonal, the pivot, the strict upper part
+// = U), so no L/U splits, no CSC transposes and no SpGEMM library calls.
+// Per outer step:
+//   1. candidate pattern C_i = A_i u F_i u (union over k in L_i of U_k incl.
+//      the pivot): one wavefront per row inserts the columns into an LDS hash
+//      set, counts (pass 1), then writes them sorted (pass 2);
+//   2. embed: existing entries keep their values, new entries take
+//      (a_ij - S_ij)/u_jj below the diagonal, a_ij - S_ij on/above it, where
+//      S_ij = sum_{k in L_i, k < min(i,j)} l_ik u_kj (embedA_intoB_L2/U2);
+//   3. one Jacobi sweep over C (LU_sweep);
+//   4. per factor, drop entries below the value of rank (count - keep) in
+//      ascending |v| (8-bit radix select on the IEEE bit patterns, all on the
+//      device) and below droptol * ||a_i||_2; the pivot is always kept
+//      (find_kth_element + thresold_remove2);
+//   5. one Jacobi sweep over the kept pattern.
+// The sweeps read the symmetrically scaled values S A S, S = |diag A|^{-1/2};
+// the caller scales A's values in and the factors back out.
+#include <climits>
+
+#include "xs_internal.h"
+#include "xs_parilut.h"
+
+#define PT_HASH 2048          // LDS hash slots per wavefront
+#define PT_WAVES 4            // wavefronts per workgroup
+#define PT_EMPTY (-1)
+
+__device__ __forceinline__ int pt_wave_sum_i(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, XS_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ void pt_insert(int *tab, int c, int *ovf) {
+  unsigned h = ((unsigned)c * 2654435761u) & (PT_HASH - 1);
+  for (int probe = 0; probe < PT_HASH; ++probe) {
+    const int old = atomicCAS(&tab[h], PT_EMPTY, c);
+    if (old == PT_EMPTY || old == c) return;
+    h = (h + 1) & (PT_HASH - 1);
+  }
+  atomicOr(ovf, 1);
+}
+
+// Pass 1 (fill == false): cnt[i] = |C_i|.  Pass 2 (fill == true): write C_i
+// sorted into cci[crp[i]..] and the row index into crow.
+template <bool FILL>
+__global__ __launch_bounds__(XS_WAVE *PT_WAVES) void k_pt_symbolic(
+    int n, const int *__restrict__ arp, const int *__restrict__ aci, const int *__restrict__ frp,
+    const int *__restrict__ fci, const int *__restrict__ fdpos, int *__restrict__ cnt,
+    const int *__restrict__ crp, int *__restrict__ cci, int *__restrict__ crow, int *ovf) {
+  __shared__ int tab_all[PT_WAVES][PT_HASH];
+  __shared__ int keys_all[PT_WAVES][PT_HASH];
+  const int lane = threadIdx.x & (XS_WAVE - 1), wv = threadIdx.x / XS_WAVE;
+  const int i = blockIdx.x * PT_WAVES + wv;
+  int *tab = tab_all[wv];
+  int *keys = keys_all[wv];
+  for (int q = lane; q < PT_HASH; q += XS_WAVE) tab[q] = PT_EMPTY;
+  __syncthreads();
+  if (i < n) {
+    for (int e = arp[i] + lane; e < arp[i + 1]; e += XS_WAVE) pt_insert(tab, aci[e], ovf);
+    for (int e = frp[i] + lane; e < frp[i + 1]; e += XS_WAVE) pt_insert(tab, fci[e], ovf);
+    // union of the upper parts (incl. pivot) of the rows k in L_i
+    for (int e = frp[i]; e < fdpos[i]; ++e) {
+      const int k = fci[e];
+      for (int q = fdpos[k] + lane; q < frp[k + 1]; q += XS_WAVE) pt_insert(tab, fci[q], ovf);
+    }
+  }
+  __syncthreads();
+  if (i < n) {
+    // compact the keys (slot order), then rank-sort them
+    int base = 0;
+    for (int q0 = 0; q0 < PT_HASH; q0 += XS_WAVE) {
+      const int c = tab[q0 + lane];
+      const bool has = c != PT_EMPTY;
+      const unsigned long long m = __ballot(has);
+      const int before = __popcll(m & ((1ull << lane) - 1ull));
+      if (has) keys[base + before] = c;
+      base += __popcll(m);
+    }
+    const int u = base;
+    if (!FILL) {
+      if (lane == 0) cnt[i] = u;
+    } else {
+      const int off = crp[i];
+      for (int a = lane; a < u; a += XS_WAVE) {
+        const int key = keys[a];
+        int rank = 0;
+        for (int b = 0; b < u; ++b) rank += keys[b] < key;
+        cci[off + rank] = key;
+        crow[off + rank] = i;
+      }
+    }
+  }
+}
+
+// value of F at (row, col) or 0 (binary search in [a, b))
+__device__ __forceinline__ int pt_find(const int *__restrict__ ci, int a, int b, int col) {
+  while (a < b) {
+    const int m = (a + b) >> 1;
+    if (ci[m] < col) a = m + 1;
+    else b = m;
+  }
+  return a;
+}
+
+__device__ __forceinline__ double pt_a(const int *__restrict__ arp, const int *__restrict__ aci,
+                                       const double *__restrict__ av,
+                                       const double *__restrict__ sc, int i, int j) {
+  const int a = arp[i], b = arp[i + 1];
+  const int p = pt_find(aci, a, b, j);
+  return (p < b && aci[p] == j) ? av[p] * sc[i] * sc[j] : 0.0;
+}
+
+// S_ij = sum_{k in L_i, k < min(i,j)} l_ik u_kj over factor (rp, ci, v, dpos)
+__device__ __forceinline__ double pt_partial(const int *__restrict__ rp, const int *__restrict__ ci,
+                                             const double *__restrict__ v,
+                                             const int *__restrict__ dpos, int i, int j) {
+  const int m = i < j ? i : j;
+  double s = 0.0;
+  for (int e = rp[i]; e < dpos[i]; ++e) {
+    const int k = ci[e];
+    if (k >= m) break;
+    const int a = dpos[k], b = rp[k + 1];
+    const int p = pt_find(ci, a, b, j);
+    if (p < b && ci[p] == j) s += v[e] * v[p];
+  }
+  return s;
+}
+
+__device__ __forceinline__ double pt_clamp(double d) {
+  if (fabs(d) < 1e-10) d = d >= 0.0 ? 1e-10 : -1e-10;
+  return d;
+}
+
+// pivot positions of a sorted CSR pattern
+__global__ void k_pt_dpos(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                          int *__restrict__ dpos, int *bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = rp[i], b = rp[i + 1];
+  const int p = pt_find(ci, a, b, i);
+  if (p < b && ci[p] == i) dpos[i] = p;
+  else { dpos[i] = a; atomicOr(bad, 1); }
+}
+
+// Embed values of the old factor F into the candidate pattern C.
+__global__ __launch_bounds__(XS_BLOCK) void k_pt_embed(
+    int nc, const int *__restrict__ crow, const int *__restrict__ cci, const int *__restrict__ arp,
+    const int *__restrict__ aci, const double *__restrict__ av, const double *__restrict__ sc,
+    const int *__restrict__ frp, const int *__restrict__ fci, const double *__restrict__ fv,
+    const int *__restrict__ fdpos, double *__restrict__ cv) {
+  const int e = blockIdx.x * XS_BLOCK + threadIdx.x;
+  if (e >= nc) return;
+  const int i = crow[e], j = cci[e];
+  const int a = frp[i], b = frp[i + 1];
+  const int p = pt_find(fci, a, b, j);
+  double val;
+  if (p < b && fci[p] == j) {
+    val = fv[p];
+  } else {
+    const double r = pt_a(arp, aci, av, sc, i, j) - pt_partial(frp, fci, fv, fdpos, i, j);
+    val = (i > j) ? r / pt_clamp(fv[fdpos[j]]) : r;
+  }
+  cv[e] = isfinite(val) ? val : 0.0;
+}
+
+// One Jacobi sweep over the pattern (rp, ci) with values vin -> vout.
+__global__ __launch_bounds__(XS_BLOCK) void k_pt_sweep(
+    int nnz, const int *__restrict__ row, const int *__restrict__ rp, const int *__restrict__ ci,
+    const int *__restrict__ dpos, const double *__restrict__ vin, double *__restrict__ vout,
+    const int *__restrict__ arp, const int *__restrict__ aci, const double *__restrict__ av,
+    const double *__restrict__ sc) {
+  const int e = blockIdx.x * XS_BLOCK + threadIdx.x;
+  if (e >= nnz) return;
+  const int i = row[e], j = ci[e];
+  const double s = pt_a(arp, aci, av, sc, i, j) - pt_partial(rp, ci, vin, dpos, i, j);
+  double t;
+  if (i > j) {
+    t = s / pt_clamp(vin[dpos[j]]);
+  } else {
+    t = (i == j) ? pt_clamp(s) : s;
+  }
+  vout[e] = isfinite(t) ? t : vin[e];
+}
+
+// ---------------------------------------------------------------------------
+// Radix select: value of rank k (0-based) in ascending |v| over the entries
+// of part `part` (0: col < row, 2: col > row).  |v| bit patterns are ordered
+// like the values; 8 passes of 8-bit digits from the most significant.
+// sel[0] = prefix bits, sel[1] = remaining rank, sel[2] = mask of fixed bits.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int pt_part(int i, int j) { return j < i ? 0 : (j == i ? 1 : 2); }
+
+__global__ __launch_bounds__(XS_BLOCK) void k_pt_hist(int nnz, const int *__restrict__ row,
+                                                      const int *__restrict__ ci,
+                                                      const double *__restrict__ v, int part,
+                                                      int shift, const unsigned long long *sel,
+                                                      unsigned *__restrict__ hist) {
+  __shared__ unsigned h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const unsigned long long prefix = sel[0], mask = sel[2];
+  for (int e = blockIdx.x * XS_BLOCK + threadIdx.x; e < nnz; e += gridDim.x * XS_BLOCK) {
+    if (pt_part(row[e], ci[e]) != part) continue;
+    const unsigned long long bits =
+        (unsigned long long)__double_as_longlong(fabs(v[e])) & 0x7FFFFFFFFFFFFFFFull;
+    if ((bits & mask) != prefix) continue;
+    atomicAdd(&h[(bits >> shift) & 0xFF], 1u);
+  }
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_pt_pick(int shift, unsigned long long *sel, unsigned *hist) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long k = sel[1];
+  unsigned long long cum = 0;
+  int d = 0;
+  for (; d < 256; ++d) {
+    if (cum + hist[d] > k) break;
+    cum += hist[d];
+  }
+  if (d == 256) d = 255;
+  sel[0] |= ((unsigned long long)d) << shift;
+  sel[1] = k - cum;
+  sel[2] |= 0xFFull << shift;
+  for (int q = 0; q < 256; ++q) hist[q] = 0;
+}
+
+__global__ void k_pt_sel_init(unsigned long long *sel, unsigned long long k, unsigned *hist) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sel[0] = 0; sel[1] = k; sel[2] = 0;
+  }
+  for (int q = threadIdx.x; q < 256; q += blockDim.x) hist[q] = 0;
+}
+
+// count of entries per part (0, 2) -> cnt2[0], cnt2[1]
+__global__ __launch_bounds__(XS_BLOCK) void k_pt_count_parts(int nnz, const int *__restrict__ row,
+                                                             const int *__restrict__ ci,
+                                                             unsigned long long *cnt2) {
+  unsigned long long a = 0, b = 0;
+  for (int e = blockIdx.x * XS_BLOCK + threadIdx.x; e < nnz; e += gridDim.x * XS_BLOCK) {
+    const int p = pt_part(row[e], ci[e]);
+    a += p == 0;
+    b += p == 2;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, XS_WAVE);
+    b += __shfl_xor(b, off, XS_WAVE);
+  }
+  if ((threadIdx.x & (XS_WAVE - 1)) == 0) {
+    atomicAdd(&cnt2[0], a);
+    atomicAdd(&cnt2[1], b);
+  }
+}
+
+// keep predicate for the compaction
+__device__ __forceinline__ bool pt_keep(int i, int j, double v, double tl, double tu,
+                                        double dt) {
+  if (i == j) return true;
+  const double a = fabs(v);
+  if (a < dt) return false;
+  return j < i ? a >= tl : a >= tu;
+}
+
+// per-row kept counts: one thread per row
+__global__ void k_pt_keep_count(int n, const int *__restrict__ crp, const int *__restrict__ cci,
+                                const double *__restrict__ cv, const double *thr,
+                                const double *__restrict__ dtol, int *__restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double tl = thr[0], tu = thr[1], dt = dtol[i];
+  int c = 0;
+  for (int e = crp[i]; e < crp[i + 1]; ++e) c += pt_keep(i, cci[e], cv[e], tl, tu, dt);
+  cnt[i] = c;
+}
+
+__global__ void k_pt_keep_write(int n, const int *__restrict__ crp, const int *__restrict__ cci,
+                                const double *__restrict__ cv, const double *thr,
+                                const double *__restrict__ dtol, const int *__restrict__ frp,
+                                int *__restrict__ fci, double *__restrict__ fv,
+                                int *__restrict__ frow) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double tl = thr[0], tu = thr[1], dt = dtol[i];
+  int w = frp[i];
+  for (int e = crp[i]; e < crp[i + 1]; ++e) {
+    if (pt_keep(i, cci[e], cv[e], tl, tu, dt)) {
+      fci[w] = cci[e];
+      fv[w] = cv[e];
+      frow[w] = i;
+      ++w;
+    }
+  }
+}
+
+// threshold value from the select state (0 when nothing is dropped)
+__global__ void k_pt_thr(const unsigned long long *sel, int use, double *thr_slot) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    *thr_slot = use ? __longlong_as_double((long long)sel[0]) : 0.0;
+}
+
+// droptol * ||a_i||_2 of the scaled matrix
+__global__ void k_pt_dtol(int n, const int *__restrict__ arp, const int *__restrict__ aci,
+                          const double *__restrict__ av, const double *__restrict__ sc,
+                          double droptol, double *__restrict__ dtol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int e = arp[i]; e < arp[i + 1]; ++e) {
+    const double a = av[e] * sc[i] * sc[aci[e]];
+    s += a * a;
+  }
+  dtol[i] = droptol * sqrt(s);
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of n ints into out[0..n] (out[n] = total); 3 phases
+// ---------------------------------------------------------------------------
+#define SCAN_B 1024
+__global__ void k_scan_blocks(int n, const int *__restrict__ in, int *__restrict__ out,
+                              int *__restrict__ bsum) {
+  __shared__ int s[SCAN_B];
+  const int g = blockIdx.x * SCAN_B + threadIdx.x;
+  s[threadIdx.x] = g < n ? in[g] : 0;
+  __syncthreads();
+  for (int off = 1; off < SCAN_B; off <<= 1) {
+    const int t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (g < n) out[g + 1] = s[threadIdx.x];
+  if (threadIdx.x == SCAN_B - 1) bsum[blockIdx.x] = s[threadIdx.x];
+}
+__global__ void k_scan_sums(int nb, int *__restrict__ bsum) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int acc = 0;
+  for (int b = 0; b < nb; ++b) {
+    const int t = bsum[b];
+    bsum[b] = acc;
+    acc += t;
+  }
+}
+__global__ void k_scan_add(int n, int *__restrict__ out, const int *__restrict__ bsum) {
+  const int g = blockIdx.x * SCAN_B + threadIdx.x;
+  if (g < n) out[g + 1] += bsum[blockIdx.x];
+  if (g == 0) out[0] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// scaling helpers
+// ---------------------------------------------------------------------------
+__global__ void k_pt_scale_vec(int n, const int *__restrict__ arp, const int *__restrict__ aci,
+                               const double *__restrict__ av, double *__restrict__ sc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int e = arp[i]; e < arp[i + 1]; ++e)
+    if (aci[e] == i) d = av[e];
+  sc[i] = d != 0.0 ? 1.0 / sqrt(fabs(d)) : 1.0;
+}
+
+// combined factor (values on a CSR pattern with row index array) scaled by
+// l_ij *= s_j / s_i below the diagonal, u_ij /= s_i s_j on and above it
+// (unscale = true), or the matrix values multiplied by s_i s_j (unscale = false).
+__global__ void k_pt_rescale(int nnz, const int *__restrict__ row, const int *__restrict__ ci,
+                             double *__restrict__ v, const double *__restrict__ sc, int unscale) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  const int i = row[e], j = ci[e];
+  if (!unscale) v[e] *= sc[i] * sc[j];
+  else if (j < i) v[e] *= sc[j] / sc[i];
+  else v[e] /= sc[i] * sc[j];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void xs_scan_excl(hipStream_t s, int n, const int *in, int *out, int *bsum) {
+  const int nb = (n + SCAN_B - 1) / SCAN_B;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_B), 0, s, n, in, out, bsum);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, s, nb, bsum);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(SCAN_B), 0, s, n, out, bsum);
+}
+int xs_scan_bsum_len(int n) { return (n + SCAN_B - 1) / SCAN_B + 1; }
+
+void xs_pt_symbolic(hipStream_t s, bool fill, int n, const int *arp, const int *aci,
+                    const int *frp, const int *fci, const int *fdpos, int *cnt, const int *crp,
+                    int *cci, int *crow, int *ovf) {
+  dim3 grid((n + PT_WAVES - 1) / PT_WAVES), blk(XS_WAVE * PT_WAVES);
+  if (fill)
+    hipLaunchKernelGGL(k_pt_symbolic<true>, grid, blk, 0, s, n, arp, aci, frp, fci, fdpos, cnt, crp, cci, crow, ovf);
+  else
+    hipLaunchKernelGGL(k_pt_symbolic<false>, grid, blk, 0, s, n, arp, aci, frp, fci, fdpos, cnt, crp, cci, crow, ovf);
+}
+
+void xs_pt_dpos(hipStream_t s, int n, const int *rp, const int *ci, int *dpos, int *bad) {
+  hipLaunchKernelGGL(k_pt_dpos, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, dpos, bad);
+}
+
+void xs_pt_embed(hipStream_t s, int nc, const int *crow, const int *cci, const int *arp,
+                 const int *aci, const double *av, const double *sc, const int *frp,
+                 const int *fci, const double *fv, const int *fdpos, double *cv) {
+  hipLaunchKernelGGL(k_pt_embed, dim3((nc + XS_BLOCK - 1) / XS_BLOCK), dim3(XS_BLOCK), 0, s, nc,
+                     crow, cci, arp, aci, av, sc, frp, fci, fv, fdpos, cv);
+}
+
+void xs_pt_sweep(hipStream_t s, int nnz, const int *row, const int *rp, const int *ci,
+                 const int *dpos, const double *vin, double *vout, const int *arp,
+                 const int *aci, const double *av, const double *sc) {
+  hipLaunchKernelGGL(k_pt_sweep, dim3((nnz + XS_BLOCK - 1) / XS_BLOCK), dim3(XS_BLOCK), 0, s, nnz,
+                     row, rp, ci, dpos, vin, vout, arp, aci, av, sc);
+}
+
+void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const double *v,
+                  int part, unsigned long long k, unsigned long long *sel, unsigned *hist) {
+  hipLaunchKernelGGL(k_pt_sel_init, dim3(1), dim3(256), 0, s, sel, k, hist);
+  int g = (nnz + XS_BLOCK * 8 - 1) / (XS_BLOCK * 8);
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(k_pt_hist, dim3(g), dim3(XS_BLOCK), 0, s, nnz, row, ci, v, part, shift, sel, hist);
+    hipLaunchKernelGGL(k_pt_pick, dim3(1), dim3(64), 0, s, shift, sel, hist);
+  }
+}
+
+void xs_pt_count_parts(hipStream_t s, int nnz, const int *row, const int *ci,
+                       unsigned long long *cnt2) {
+  int g = (nnz + XS_BLOCK * 8 - 1) / (XS_BLOCK * 8);
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  (void)hipMemsetAsync(cnt2, 0, 2 * sizeof(unsigned long long), s);
+  hipLaunchKernelGGL(k_pt_count_parts, dim3(g), dim3(XS_BLOCK), 0, s, nnz, row, ci, cnt2);
+}
+
+void xs_pt_thr(hipStream_t s, const unsigned long long *sel, int use, double *thr_slot) {
+  hipLaunchKernelGGL(k_pt_thr, dim3(1), dim3(64), 0, s, sel, use, thr_slot);
+}
+
+void xs_pt_dtol(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
+                const double *sc, double droptol, double *dtol) {
+  hipLaunchKernelGGL(k_pt_dtol, dim3((n + 255) / 256), dim3(256), 0, s, n, arp, aci, av, sc,
+                     droptol, dtol);
+}
+
+void xs_pt_keep(hipStream_t s, bool write, int n, const int *crp, const int *cci,
+                const double *cv, const double *thr, const double *dtol, int *cnt,
+                const int *frp, int *fci, double *fv, int *frow) {
+  if (!write)
+    hipLaunchKernelGGL(k_pt_keep_count, dim3((n + 255) / 256), dim3(256), 0, s, n, crp, cci, cv,
+                       thr, dtol, cnt);
+  else
+    hipLaunchKernelGGL(k_pt_keep_write, dim3((n + 255) / 256), dim3(256), 0, s, n, crp, cci, cv,
+                       thr, dtol, frp, fci, fv, frow);
+}
+
+void xs_pt_scale_vec(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
+                     double *sc) {
+  hipLaunchKernelGGL(k_pt_scale_vec, dim3((n + 255) / 256), dim3(256), 0, s, n, arp, aci, av, sc);
+}
+
+void xs_pt_rescale(hipStream_t s, int nnz, const int *row, const int *ci, double *v,
+                   const double *sc, int unscale) {
+  hipLaunchKernelGGL(k_pt_rescale, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, row, ci, v, sc,
+                     unscale);
+}
+
+__global__ void k_pt_rowidx(int n, const int *__restrict__ rp, int *__restrict__ row) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int e = rp[i]; e < rp[i + 1]; ++e) row[e] = i;
+}
+void xs_pt_rowidx(hipStream_t s, int n, const int *rp, int *row) {
+  hipLaunchKernelGGL(k_pt_rowidx, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, row);
+}

@@ -22,10 +22,12 @@ struct xs_ctx {
   int pc = XS_PC_NONE;
   double tol = 1e-6, abstol = 0.0;
   int maxiter = 0, restart = 0;
-  double ilut_tol = 1e-4;       // hip/Precond/precon_ilup.h:19-24 uses 1e-5/20
-  int ilut_fill = 20;
+  double ilut_tol = 1e-4;       // ParILUT relative drop tolerance
+  int ilut_fill = 2;            // ParILUT final fill factor (nnz per factor / A's)
   int parilu_sweeps = 20;       // hip/Precond/precon_gpuilu0.cpp:5
   bool parilu_async = true;
+  bool parilu_scale = true;
+  int parilut_steps = 10;       // hip/Precond/precon_gpuilut.cpp:5
   int check_every = 8;
   int precision = 0;
   int use_ell = 0;
