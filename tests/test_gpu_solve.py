@@ -225,3 +225,26 @@ def test_basis_krylov_parity(dev, case):
     assert rr < bound
     assert np.linalg.norm(g["x"] - xo) / np.linalg.norm(xo) < 1e3 * tol
     g["S"].close()
+
+
+@pytest.mark.parametrize("factor", [0, 1, 2], ids=["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("pc", ["ilu0", "jacobi"])
+def test_mixed_precision_reaches_fp64_accuracy(dev, pc, factor):
+    """FP32 inner BiCGSTAB (FP32 SpMV / SpTRSV, factors stored FP32/BF16/FP16)
+    inside FP64 iterative refinement reaches the same FP64-verified residual as
+    the plain FP64 solve; its error against the oracle's FP64 solution is
+    reported and bounded."""
+    A = gen.convdiff3d(32)
+    xs, b = gen.manufactured(A, 1)
+    say(f"[mixed] convdiff3d_32 pc={pc} factor={factor}")
+    xo, sto, ito, _ = oracle.solve(A, b, "bicgstab", pc, tol=1e-10, maxiter=2000)
+    g = gpu_solve(A, b, "bicgstab", pc, 1e-10, 4000, precision=1, factor_storage=factor)
+    rr = true_relres(A, b, g["x"])
+    err = np.linalg.norm(g["x"] - xo) / np.linalg.norm(xo)
+    say(f"   oracle it={ito} mixed inner it={g['it']} outer={len(g['res']) - 1} relres={rr:.2e} err-vs-fp64={err:.2e}")
+    assert g["st"] == 0
+    assert rr < 1.05e-10
+    assert err < 1e-7
+    res = g["res"]
+    assert res[-1] / res[0] < 1e-10 and len(res) >= 2
+    g["S"].close()

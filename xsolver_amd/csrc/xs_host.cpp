@@ -85,6 +85,7 @@ static void free_matrix(xs_ctx *c) {
   c->have_At = false;
   c->h_rp.clear(); c->h_ci.clear();
   free_precond(c);
+  xs_mixed_free(c);
 }
 
 static void free_work(xs_ctx *c) {
@@ -238,6 +239,7 @@ extern "C" int xs_destroy(xs_ctx *c) {
   dfree(c->st); dfree(c->counters); dfree(c->d_int); dfree(c->partial); dfree(c->resvec_d);
   if (c->h_state) (void)hipHostFree(c->h_state);
   xs_solvers_free(c);
+  xs_mixed_free(c);
   for (auto &e : c->prof_ev) (void)hipEventDestroy(e);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
@@ -396,7 +398,9 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   if (k == "parilu_sweeps") { if (value < 0) return XS_ERR_ARG; c->parilu_sweeps = (int)value; c->pc_ready = false; }
   else if (k == "parilu_async") { c->parilu_async = value != 0; c->pc_ready = false; }
   else if (k == "check_every") { if (value < 1) return XS_ERR_ARG; c->check_every = (int)value; }
-  else if (k == "refactor") { c->pc_ready = false; }
+  else if (k == "refactor") { c->pc_ready = false; c->mixed_ready = false; }
+  else if (k == "factor_storage") { if (value < 0 || value > 2) return XS_ERR_ARG; c->factor_type = (int)value; c->mixed_ready = false; }
+  else if (k == "inner_tol") { if (!(value > 0 && value < 1)) return XS_ERR_ARG; c->inner_tol = value; }
   else if (k == "parilu_scale") { c->parilu_scale = value != 0; c->pc_ready = false; }
   else if (k == "parilut_steps") { if (value < 1) return XS_ERR_ARG; c->parilut_steps = (int)value; c->pc_ready = false; }
   else if (k == "profile") { c->prof_kind = (int)value; for (int i = 0; i < 4; i++) { c->prof_ms[i] = 0; c->prof_cnt[i] = 0; } }
@@ -762,7 +766,9 @@ static int run_solve(xs_ctx *c, const double *d_b, double *d_x) {
     c->resvec_cap = c->maxiter + 2;
   }
   XS_HIP(hipEventRecord(c->ev0, c->stream));
-  switch (c->ksp) {
+  if (c->precision == 1) {
+    rc = xs_solve_mixed(c, d_b, d_x);
+  } else switch (c->ksp) {
     case XS_KSP_CG: rc = xs_solve_cg(c, d_b, d_x); break;
     case XS_KSP_BICGSTAB: rc = xs_solve_bicgstab(c, d_b, d_x); break;
     case XS_KSP_GCR: rc = xs_solve_gcr(c, d_b, d_x); break;
@@ -792,7 +798,7 @@ static int run_solve(xs_ctx *c, const double *d_b, double *d_x) {
   }
   if (rc < 0 && rc != XS_ERR_BREAKDOWN) return rc;
   // results
-  int nres = std::min(c->iters + 1, c->resvec_cap);
+  int nres = std::min(c->precision == 1 ? c->outer_iters + 1 : c->iters + 1, c->resvec_cap);
   c->resvec.resize(nres);
   XS_HIP(hipMemcpy(c->resvec.data(), c->resvec_d, sizeof(double) * nres, hipMemcpyDeviceToHost));
   return rc;

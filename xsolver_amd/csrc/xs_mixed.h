@@ -1,0 +1,31 @@
+// xs_mixed.h -- FP32 inner-iteration kernels of the mixed-precision solve.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "xs_internal.h"
+
+enum { XS_F32 = 0, XS_BF16 = 1, XS_F16 = 2 };   // factor storage types
+
+void xs_launch_spmv_csr32(hipStream_t s, const xs_csr &A, const float *v32,
+                          const xs_spmv_plan &P, const float *x, float *y, int epi,
+                          const float *b, const float *w, const xs_reduce *red);
+void xs_launch_spmv_ell32(hipStream_t s, const xs_ell &E, const float *v32, const float *x,
+                          float *y, int epi, const float *b, const float *w,
+                          const xs_reduce *red);
+void xs_trsv32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, const void *lu,
+               int ftype, const int *dpos, const float *dinv, const float *b, float *x,
+               unsigned *tickets, const xs_state *st);
+void xs_cvt_factor(hipStream_t s, int nnz, const double *src, void *dst, int ftype);
+void xs_d2f(hipStream_t s, int n, const double *a, float *b);
+void xs_vec_d2f(hipStream_t s, int n, const double *d, float *f);
+void xs_d2f_scaled(hipStream_t s, int n, const double *a, float *b, const xs_state *sref, int slot);
+void xs_add_f2d(hipStream_t s, int n, const float *d, double *x, const xs_state *sref, int slot);
+void xs_zero32(hipStream_t s, int n, float *a);
+void xs_copy32(hipStream_t s, int n, const float *a, float *b);
+void xs_scale32(hipStream_t s, int n, const float *d, const float *a, float *b);
+void xs_bicg_p32(hipStream_t s, int n, const float *r, const float *u, float *p,
+                 const float *dinv, float *z, xs_state *st);
+void xs_bicg_s32(hipStream_t s, int n, const float *r, const float *u, float *sv,
+                 const float *dinv, float *z1, xs_state *st);
+void xs_bicg_xr32(hipStream_t s, int n, float *x, float *r, const float *z, const float *z1,
+                  const float *sv, const float *v, const float *rh, const xs_reduce &red);

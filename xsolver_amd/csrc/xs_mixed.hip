@@ -1,0 +1,430 @@
+// xs_mixed.hip -- FP32 inner-iteration kernels for the mixed-precision solve:
+// FP32 CSR / ELL SpMV, sync-free SpTRSV over factors stored as FP32, BF16 or
+// FP16 (accumulated in FP32), FP32 BiCGSTAB vector updates, and the
+// FP64 <-> FP32 conversions of the iterative-refinement outer loop.  Inner
+// products are accumulated and reduced in FP64; every Krylov scalar stays
+// FP64 on the device.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <climits>
+
+#include "xs_device.h"
+#include "xs_mixed.h"
+
+__device__ __forceinline__ float fl(const float *p, size_t i) { return p[i]; }
+__device__ __forceinline__ float fl(const __hip_bfloat16 *p, size_t i) { return __bfloat162float(p[i]); }
+__device__ __forceinline__ float fl(const __half *p, size_t i) { return __half2float(p[i]); }
+
+template <int EPI, int NV>
+__device__ __forceinline__ void epi32(int r, float s, float *__restrict__ y,
+                                      const float *__restrict__ b, const float *__restrict__ w,
+                                      double (&acc)[NV]) {
+  if (EPI == EPI_BMAX || EPI == EPI_BMAXN) {
+    const float t = b[r] - s;
+    y[r] = t;
+    if (EPI == EPI_BMAXN) acc[0] += (double)t * (double)t;
+  } else {
+    y[r] = s;
+    if (EPI == EPI_DOT1 || EPI == EPI_DOT2) acc[0] += (double)w[r] * (double)s;
+    if (EPI == EPI_DOT2) acc[NV - 1] += (double)s * (double)s;
+  }
+}
+
+// CSR SpMV in FP32 over the same row-block plan as the FP64 kernel.
+template <int EPI>
+__global__ __launch_bounds__(XS_BLOCK) void k_spmv_csr32(
+    const int *__restrict__ rp, const int *__restrict__ ci, const float *__restrict__ v,
+    const float *__restrict__ x, float *__restrict__ y, const int *__restrict__ blk_row,
+    const float *__restrict__ b, const float *__restrict__ w, xs_reduce R) {
+  __shared__ float prod[XS_SPMV_LDS];
+  constexpr int NV = (EPI == EPI_DOT2) ? 2 : 1;
+  constexpr bool RED = (EPI == EPI_DOT1 || EPI == EPI_DOT2 || EPI == EPI_BMAXN);
+  if (R.st != nullptr && R.st->done) return;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  const int r0 = blk_row[blockIdx.x], r1 = blk_row[blockIdx.x + 1];
+  if (r1 > r0) {
+    const int e0 = rp[r0], e1 = rp[r1];
+    const int ne = e1 - e0;
+    if (ne <= XS_SPMV_LDS) {
+      for (int k = threadIdx.x; k < ne; k += XS_BLOCK) prod[k] = v[e0 + k] * x[ci[e0 + k]];
+      __syncthreads();
+      for (int r = r0 + threadIdx.x; r < r1; r += XS_BLOCK) {
+        const int a = rp[r] - e0, bnd = rp[r + 1] - e0;
+        float s = 0.f;
+        for (int k = a; k < bnd; ++k) s += prod[k];
+        epi32<EPI, NV>(r, s, y, b, w, acc);
+      }
+    } else {
+      const int lane = threadIdx.x & (XS_WAVE - 1);
+      for (int r = r0 + (int)(threadIdx.x / XS_WAVE); r < r1; r += XS_BLOCK / XS_WAVE) {
+        const int a = rp[r], bnd = rp[r + 1];
+        double s = 0.0;
+        for (int k = a + lane; k < bnd; k += XS_WAVE) s += (double)(v[k] * x[ci[k]]);
+        s = wave_sum(s);
+        if (lane == 0) epi32<EPI, NV>(r, (float)s, y, b, w, acc);
+      }
+    }
+  }
+  if (RED) reduce_finish<NV>(acc, R);
+}
+
+// ELL SpMV in FP32, two rows per thread (8-byte value and index loads).
+template <int EPI>
+__global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell32(
+    int n, int ld, int W, const int *__restrict__ ci, const float *__restrict__ v,
+    const float *__restrict__ x, float *__restrict__ y, const float *__restrict__ b,
+    const float *__restrict__ w, xs_reduce R) {
+  constexpr int NV = (EPI == EPI_DOT2) ? 2 : 1;
+  constexpr bool RED = (EPI == EPI_DOT1 || EPI == EPI_DOT2 || EPI == EPI_BMAXN);
+  if (R.st != nullptr && R.st->done) return;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  const int r = 2 * (blockIdx.x * XS_BLOCK + threadIdx.x);
+  if (r < n) {
+    float s0 = 0.f, s1 = 0.f;
+    for (int k = 0; k < W; ++k) {
+      const int2 c = *reinterpret_cast<const int2 *>(ci + (size_t)k * ld + r);
+      const float2 a = *reinterpret_cast<const float2 *>(v + (size_t)k * ld + r);
+      s0 += a.x * x[c.x];
+      s1 += a.y * x[c.y];
+    }
+    if (r < n) epi32<EPI, NV>(r, s0, y, b, w, acc);
+    if (r + 1 < n) epi32<EPI, NV>(r + 1, s1, y, b, w, acc);
+  }
+  if (RED) reduce_finish<NV>(acc, R);
+}
+
+void xs_launch_spmv_csr32(hipStream_t s, const xs_csr &A, const float *v32,
+                          const xs_spmv_plan &P, const float *x, float *y, int epi,
+                          const float *b, const float *w, const xs_reduce *red) {
+  xs_reduce R{};
+  if (red) R = *red;
+  if (!(A.rp && A.ci && v32 && P.blk_row && x && y)) return;
+  dim3 grid(P.nblk), blk(XS_BLOCK);
+  switch (epi) {
+    case EPI_Y: hipLaunchKernelGGL(k_spmv_csr32<EPI_Y>, grid, blk, 0, s, A.rp, A.ci, v32, x, y, P.blk_row, b, w, R); break;
+    case EPI_BMAX: hipLaunchKernelGGL(k_spmv_csr32<EPI_BMAX>, grid, blk, 0, s, A.rp, A.ci, v32, x, y, P.blk_row, b, w, R); break;
+    case EPI_DOT1: hipLaunchKernelGGL(k_spmv_csr32<EPI_DOT1>, grid, blk, 0, s, A.rp, A.ci, v32, x, y, P.blk_row, b, w, R); break;
+    case EPI_DOT2: hipLaunchKernelGGL(k_spmv_csr32<EPI_DOT2>, grid, blk, 0, s, A.rp, A.ci, v32, x, y, P.blk_row, b, w, R); break;
+    case EPI_BMAXN: hipLaunchKernelGGL(k_spmv_csr32<EPI_BMAXN>, grid, blk, 0, s, A.rp, A.ci, v32, x, y, P.blk_row, b, w, R); break;
+  }
+}
+
+void xs_launch_spmv_ell32(hipStream_t s, const xs_ell &E, const float *v32, const float *x,
+                          float *y, int epi, const float *b, const float *w,
+                          const xs_reduce *red) {
+  xs_reduce R{};
+  if (red) R = *red;
+  if (!(E.ci && v32 && x && y)) return;
+  const int threads = (E.n + 1) / 2;
+  dim3 grid((threads + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+  switch (epi) {
+    case EPI_Y: hipLaunchKernelGGL(k_spmv_ell32<EPI_Y>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, v32, x, y, b, w, R); break;
+    case EPI_BMAX: hipLaunchKernelGGL(k_spmv_ell32<EPI_BMAX>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, v32, x, y, b, w, R); break;
+    case EPI_DOT1: hipLaunchKernelGGL(k_spmv_ell32<EPI_DOT1>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, v32, x, y, b, w, R); break;
+    case EPI_DOT2: hipLaunchKernelGGL(k_spmv_ell32<EPI_DOT2>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, v32, x, y, b, w, R); break;
+    case EPI_BMAXN: hipLaunchKernelGGL(k_spmv_ell32<EPI_BMAXN>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, v32, x, y, b, w, R); break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sync-free SpTRSV in FP32 (same algorithm as k_trsv in xs_kernels.hip):
+// a 32-bit signalling-NaN sentinel marks rows not yet solved.
+// ---------------------------------------------------------------------------
+#define XS_SENTINEL32 0xFFA0DEADu
+#define XS_SPIN_LIMIT32 (1u << 22)
+
+__device__ __forceinline__ float ld_ready32(const float *p, unsigned *err) {
+  unsigned u = __hip_atomic_load(reinterpret_cast<unsigned *>(const_cast<float *>(p)),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned polls = 0;
+  while (u == XS_SENTINEL32) {
+    if (++polls > XS_SPIN_LIMIT32) {
+      atomicOr(err, 1u);
+      u = 0x7FC00000u;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    u = __hip_atomic_load(reinterpret_cast<unsigned *>(const_cast<float *>(p)), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return __uint_as_float(u);
+}
+__device__ __forceinline__ void st_ready32(float *p, float v) {
+  unsigned u = __float_as_uint(v);
+  if (u == XS_SENTINEL32) u = 0x7FC00000u;
+  __hip_atomic_store(reinterpret_cast<unsigned *>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <bool UPPER, class TF>
+__global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restrict__ rp,
+                                                     const int *__restrict__ ci,
+                                                     const TF *__restrict__ lu,
+                                                     const int *__restrict__ dpos,
+                                                     const float *__restrict__ dinv,
+                                                     const float *__restrict__ b, float *x,
+                                                     unsigned *tickets, const xs_state *st) {
+  if (st != nullptr && st->done) return;
+  unsigned *err = tickets + 4;
+  const int lane = threadIdx.x & (XS_WAVE - 1);
+  unsigned w = 0;
+  if (lane == 0) w = atomicAdd(&tickets[0], 1u);
+  w = __shfl(w, 0, XS_WAVE);
+  const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
+  if (w < nw) {
+    int r0, r1;
+    if (!UPPER) { r0 = (int)w * XS_WAVE; r1 = min(n, r0 + XS_WAVE); }
+    else { r1 = n - (int)w * XS_WAVE; r0 = max(0, r1 - XS_WAVE); }
+    const int r = r0 + lane;
+    const bool act = r < r1;
+    float acc = 0.f, di = 1.f;
+    int eb = 0, ee = 0;
+    if (act) {
+      acc = b[r];
+      if (!UPPER) {
+        int e = rp[r];
+        const int end = dpos[r];
+        int xe = e;
+        while (xe < end && ci[xe] < r0) ++xe;
+        for (; e + 4 <= xe; e += 4) {
+          const float x0 = ld_ready32(x + ci[e], err), x1 = ld_ready32(x + ci[e + 1], err);
+          const float x2 = ld_ready32(x + ci[e + 2], err), x3 = ld_ready32(x + ci[e + 3], err);
+          acc -= fl(lu, e) * x0; acc -= fl(lu, e + 1) * x1;
+          acc -= fl(lu, e + 2) * x2; acc -= fl(lu, e + 3) * x3;
+        }
+        for (; e < xe; ++e) acc -= fl(lu, e) * ld_ready32(x + ci[e], err);
+        eb = xe; ee = end;
+      } else {
+        di = dinv[r];
+        const int beg = dpos[r] + 1;
+        int e = rp[r + 1] - 1;
+        int xb = e + 1;
+        while (xb > beg && ci[xb - 1] >= r1) --xb;
+        for (; e - 3 >= xb; e -= 4) {
+          const float x0 = ld_ready32(x + ci[e], err), x1 = ld_ready32(x + ci[e - 1], err);
+          const float x2 = ld_ready32(x + ci[e - 2], err), x3 = ld_ready32(x + ci[e - 3], err);
+          acc -= fl(lu, e) * x0; acc -= fl(lu, e - 1) * x1;
+          acc -= fl(lu, e - 2) * x2; acc -= fl(lu, e - 3) * x3;
+        }
+        for (; e >= xb; --e) acc -= fl(lu, e) * ld_ready32(x + ci[e], err);
+        eb = beg; ee = xb;
+      }
+    }
+    const int nint = ee - eb;
+    const int nb = UPPER ? r + 1 : r - 1;
+    const bool chain = !act || nint == 0 || (nint == 1 && ci[eb] == nb);
+    float val;
+    if (__all(chain)) {
+      float c, m;
+      if (!UPPER) { c = acc; m = (act && nint == 1) ? -fl(lu, eb) : 0.f; }
+      else { c = act ? di * acc : 0.f; m = (act && nint == 1) ? -di * fl(lu, eb) : 0.f; }
+#pragma unroll
+      for (int d = 1; d < XS_WAVE; d <<= 1) {
+        float cj, mj;
+        if (!UPPER) { cj = __shfl_up(c, d, XS_WAVE); mj = __shfl_up(m, d, XS_WAVE); }
+        else { cj = __shfl_down(c, d, XS_WAVE); mj = __shfl_down(m, d, XS_WAVE); }
+        const bool has = UPPER ? (lane + d < XS_WAVE) : (lane >= d);
+        if (has) { c = c + m * cj; m = m * mj; }
+      }
+      val = c;
+    } else {
+      const int nrows = r1 - r0;
+      if (!UPPER) {
+        int e = eb;
+        int nc = (act && e < ee) ? ci[e] : INT_MAX;
+        for (int k = 0; k < nrows; ++k) {
+          const float xk = __shfl(acc, k, XS_WAVE);
+          if (nc == r0 + k) {
+            acc -= fl(lu, e) * xk;
+            ++e;
+            nc = (e < ee) ? ci[e] : INT_MAX;
+          }
+        }
+        val = acc;
+      } else {
+        int e = ee - 1;
+        int nc = (act && e >= eb) ? ci[e] : INT_MIN;
+        for (int k = nrows - 1; k >= 0; --k) {
+          const float xk = __shfl(acc * di, k, XS_WAVE);
+          if (nc == r0 + k) {
+            acc -= fl(lu, e) * xk;
+            --e;
+            nc = (e >= eb) ? ci[e] : INT_MIN;
+          }
+        }
+        val = acc * di;
+      }
+    }
+    if (act) st_ready32(x + r, val);
+  }
+  if (lane == 0) {
+    const unsigned total = gridDim.x * (XS_BLOCK / XS_WAVE);
+    const unsigned f = atomicAdd(&tickets[1], 1u);
+    if (f == total - 1) {
+      __hip_atomic_store(&tickets[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&tickets[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+__global__ void k_fill_u32(unsigned *p, int n, unsigned val) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = val;
+}
+
+static inline int vgrid(int n) {
+  int g = (n + XS_BLOCK * 4 - 1) / (XS_BLOCK * 4);
+  if (g > XS_MAX_REDUCE_BLOCKS) g = XS_MAX_REDUCE_BLOCKS;
+  return g < 1 ? 1 : g;
+}
+
+void xs_trsv32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, const void *lu,
+               int ftype, const int *dpos, const float *dinv, const float *b, float *x,
+               unsigned *tickets, const xs_state *st) {
+  if (!(rp && ci && lu && dpos && b && x && tickets) || (upper && !dinv)) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s,
+                     reinterpret_cast<unsigned *>(x), n, XS_SENTINEL32);
+  const int nw = (n + XS_WAVE - 1) / XS_WAVE;
+  dim3 grid((nw + 3) / 4), blk(XS_BLOCK);
+#define TRSV32(U, T) hipLaunchKernelGGL((k_trsv32<U, T>), grid, blk, 0, s, n, rp, ci, (const T *)lu, dpos, dinv, b, x, tickets, st)
+  if (ftype == XS_F32) { if (upper) TRSV32(true, float); else TRSV32(false, float); }
+  else if (ftype == XS_BF16) { if (upper) TRSV32(true, __hip_bfloat16); else TRSV32(false, __hip_bfloat16); }
+  else { if (upper) TRSV32(true, __half); else TRSV32(false, __half); }
+#undef TRSV32
+}
+
+// ---------------------------------------------------------------------------
+// conversions and FP32 BiCGSTAB updates
+// ---------------------------------------------------------------------------
+__global__ void k_cvt_factor(int nnz, const double *__restrict__ src, void *dst, int ftype) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  const double v = src[e];
+  if (ftype == XS_F32) reinterpret_cast<float *>(dst)[e] = (float)v;
+  else if (ftype == XS_BF16) reinterpret_cast<__hip_bfloat16 *>(dst)[e] = __float2bfloat16((float)v);
+  else reinterpret_cast<__half *>(dst)[e] = __float2half((float)v);
+}
+void xs_cvt_factor(hipStream_t s, int nnz, const double *src, void *dst, int ftype) {
+  hipLaunchKernelGGL(k_cvt_factor, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, src, dst, ftype);
+}
+
+__global__ void k_d2f(int n, const double *__restrict__ a, float *__restrict__ b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+void xs_d2f(hipStream_t s, int n, const double *a, float *b) {
+  hipLaunchKernelGGL(k_d2f, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s, n, a, b);
+}
+
+// r32 = r64 / ||r64|| (norm from sc[slot])
+__global__ void k_d2f_scaled(int n, const double *__restrict__ a, float *__restrict__ b,
+                             const xs_state *sref, int slot) {
+  const double inv = 1.0 / sqrt(sref->sc[slot]);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    b[i] = (float)(a[i] * inv);
+}
+void xs_d2f_scaled(hipStream_t s, int n, const double *a, float *b, const xs_state *sref, int slot) {
+  hipLaunchKernelGGL(k_d2f_scaled, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s, n, a, b, sref, slot);
+}
+
+// x64 += ||r64|| * d32
+__global__ void k_add_f2d(int n, const float *__restrict__ d, double *__restrict__ x,
+                          const xs_state *sref, int slot) {
+  const double sc = sqrt(sref->sc[slot]);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    x[i] += sc * (double)d[i];
+}
+void xs_add_f2d(hipStream_t s, int n, const float *d, double *x, const xs_state *sref, int slot) {
+  hipLaunchKernelGGL(k_add_f2d, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s, n, d, x, sref, slot);
+}
+
+__global__ void k_zero32(int n, float *__restrict__ a) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0.f;
+}
+void xs_zero32(hipStream_t s, int n, float *a) {
+  hipLaunchKernelGGL(k_zero32, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s, n, a);
+}
+__global__ void k_copy32(int n, const float *__restrict__ a, float *__restrict__ b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = a[i];
+}
+void xs_copy32(hipStream_t s, int n, const float *a, float *b) {
+  hipLaunchKernelGGL(k_copy32, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s, n, a, b);
+}
+__global__ void k_scale32(int n, const float *__restrict__ d, const float *__restrict__ a,
+                          float *__restrict__ b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = d[i] * a[i];
+}
+void xs_scale32(hipStream_t s, int n, const float *d, const float *a, float *b) {
+  hipLaunchKernelGGL(k_scale32, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s, n, d, a, b);
+}
+
+__global__ __launch_bounds__(XS_BLOCK) void k_bicg_p32(int n, const float *__restrict__ r,
+                                                       const float *__restrict__ u,
+                                                       float *__restrict__ p,
+                                                       const float *__restrict__ dinv,
+                                                       float *__restrict__ z, xs_state *st) {
+  if (st->done) return;
+  const float omega = (float)st->sc[S_OMEGA], alpha = (float)st->sc[S_ALPHA];
+  for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) {
+    const float pi = r[i] + omega * (p[i] - alpha * u[i]);
+    p[i] = pi;
+    if (dinv) z[i] = dinv[i] * pi;
+  }
+}
+void xs_bicg_p32(hipStream_t s, int n, const float *r, const float *u, float *p,
+                 const float *dinv, float *z, xs_state *st) {
+  hipLaunchKernelGGL(k_bicg_p32, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s, n, r, u, p, dinv, z, st);
+}
+
+__global__ __launch_bounds__(XS_BLOCK) void k_bicg_s32(int n, const float *__restrict__ r,
+                                                       const float *__restrict__ u,
+                                                       float *__restrict__ sv,
+                                                       const float *__restrict__ dinv,
+                                                       float *__restrict__ z1, xs_state *st) {
+  if (st->done) return;
+  const float gama = (float)st->sc[S_GAMA];
+  for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) {
+    const float si = r[i] - gama * u[i];
+    sv[i] = si;
+    if (dinv) z1[i] = dinv[i] * si;
+  }
+}
+void xs_bicg_s32(hipStream_t s, int n, const float *r, const float *u, float *sv,
+                 const float *dinv, float *z1, xs_state *st) {
+  hipLaunchKernelGGL(k_bicg_s32, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s, n, r, u, sv, dinv, z1, st);
+}
+
+__global__ __launch_bounds__(XS_BLOCK) void k_bicg_xr32(int n, float *__restrict__ x,
+                                                        float *__restrict__ r,
+                                                        const float *__restrict__ z,
+                                                        const float *__restrict__ z1,
+                                                        const float *__restrict__ sv,
+                                                        const float *__restrict__ v,
+                                                        const float *__restrict__ rh, xs_reduce R) {
+  xs_state *st = R.st;
+  if (st->done) return;
+  const float gama = (float)st->sc[S_GAMA], alpha = (float)st->sc[S_ALPHA];
+  double acc[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) {
+    x[i] = x[i] + gama * z[i] + alpha * z1[i];
+    const float ri = sv[i] - alpha * v[i];
+    r[i] = ri;
+    acc[0] += (double)ri * (double)ri;
+    acc[1] += (double)rh[i] * (double)ri;
+  }
+  reduce_finish<2>(acc, R);
+}
+void xs_bicg_xr32(hipStream_t s, int n, float *x, float *r, const float *z, const float *z1,
+                  const float *sv, const float *v, const float *rh, const xs_reduce &red) {
+  hipLaunchKernelGGL(k_bicg_xr32, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s, n, x, r, z, z1, sv, v, rh, red);
+}
+
+__global__ void k_diag_f32(int n, const double *__restrict__ d, float *__restrict__ f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = (float)d[i];
+}
+void xs_vec_d2f(hipStream_t s, int n, const double *d, float *f) {
+  hipLaunchKernelGGL(k_diag_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, d, f);
+}

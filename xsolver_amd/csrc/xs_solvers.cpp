@@ -10,6 +10,7 @@
 
 #include "../../include/xsolver_c.h"
 #include "xs_krylov.h"
+#include "xs_mixed.h"
 #include "xs_solvers.h"
 
 #define XS_CHK(call)                            \
@@ -309,4 +310,180 @@ void xs_solvers_free(xs_ctx *c) {
   if (c->ts_partial) (void)hipFree(c->ts_partial);
   c->ts_partial = nullptr;
   c->ts_partial_len = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: FP64 iterative refinement around an FP32 inner solve.
+//   r = b - A x (FP64 SpMV, FP64 norm: the convergence-deciding quantity)
+//   repeat: d ~= A^{-1} (r/||r||) by the FP32 inner Krylov method with FP32
+//           SpMV and FP32 triangular solves over FP32/BF16/FP16 factors;
+//           x += ||r|| d (FP64); r = b - A x (FP64)
+// until ||r||/||r0|| < tol.  Iterations reported = inner iterations in total;
+// resvec holds the FP64 true residual after each refinement step.
+// ---------------------------------------------------------------------------
+template <class T>
+static int ensure_buf(T **p, size_t need) {
+  if (*p) return XS_OK;
+  if (hipMalloc((void **)p, sizeof(T) * (need ? need : 1)) != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return XS_ERR_MEMORY;
+  }
+  return XS_OK;
+}
+
+void xs_mixed_free(xs_ctx *c) {
+  auto f = [](auto *&p) { if (p) (void)hipFree((void *)p); p = nullptr; };
+  f(c->A32); f(c->ell32); f(c->dinv32); f(c->jac32); f(c->F32); f(c->work32); f(c->st_in);
+  c->work32_n = 0;
+  c->mixed_ready = false;
+}
+
+static int mixed_prepare(xs_ctx *c) {
+  hipStream_t s = c->stream;
+  const int n = c->A.n;
+  int rc;
+  if ((rc = ensure_buf(&c->A32, (size_t)c->A.nnz))) return rc;
+  xs_d2f(s, c->A.nnz, c->A.v, c->A32);
+  if (c->use_ell && c->ell.v) {
+    if ((rc = ensure_buf(&c->ell32, (size_t)c->ell.W * c->ell.ld))) return rc;
+    xs_d2f(s, c->ell.W * c->ell.ld, c->ell.v, c->ell32);
+  }
+  if (c->pc == XS_PC_JACOBI) {
+    if ((rc = ensure_buf(&c->jac32, (size_t)n))) return rc;
+    xs_vec_d2f(s, n, c->jac_dinv, c->jac32);
+  } else if (c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) {
+    if (c->F32) { (void)hipFree(c->F32); c->F32 = nullptr; }
+    const size_t es = c->factor_type == XS_F32 ? 4 : 2;
+    if (hipMalloc(&c->F32, es * (size_t)c->ilu.nnz) != hipSuccess) return XS_ERR_MEMORY;
+    xs_cvt_factor(s, c->ilu.nnz, c->ilu.lu, c->F32, c->factor_type);
+    if ((rc = ensure_buf(&c->dinv32, (size_t)n))) return rc;
+    xs_vec_d2f(s, n, c->ilu.dinv, c->dinv32);
+  }
+  if (c->work32_n != (size_t)n) {
+    if (c->work32) (void)hipFree(c->work32);
+    c->work32 = nullptr;
+    if ((rc = ensure_buf(&c->work32, (size_t)12 * n))) return rc;
+    c->work32_n = n;
+  }
+  if ((rc = ensure_buf(&c->st_in, 1))) return rc;
+  c->mixed_ready = true;
+  return XS_OK;
+}
+
+static void op_spmv32(xs_ctx *c, const float *x, float *y, int epi, const float *b,
+                      const float *w, const xs_reduce *red) {
+  xs_prof_begin(c, 1);
+  if (c->use_ell && c->ell32)
+    xs_launch_spmv_ell32(c->stream, c->ell, c->ell32, x, y, epi, b, w, red);
+  else
+    xs_launch_spmv_csr32(c->stream, c->A, c->A32, c->plan, x, y, epi, b, w, red);
+  xs_prof_end(c, 1);
+}
+
+static void op_pc32(xs_ctx *c, const float *x, float *y, float *tmp, const xs_state *st) {
+  const int n = c->A.n;
+  if (c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) {
+    xs_prof_begin(c, 2);
+    xs_trsv32(c->stream, false, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type, c->ilu.dpos,
+              c->dinv32, x, tmp, c->counters + 2, st);
+    xs_prof_end(c, 2);
+    xs_prof_begin(c, 3);
+    xs_trsv32(c->stream, true, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type, c->ilu.dpos,
+              c->dinv32, tmp, y, c->counters + 4, st);
+    xs_prof_end(c, 3);
+  } else if (c->pc == XS_PC_JACOBI) {
+    xs_scale32(c->stream, n, c->jac32, x, y);
+  } else {
+    xs_copy32(c->stream, n, x, y);
+  }
+}
+
+// FP32 BiCGSTAB on A d = rhs from d = 0 (inner solve); returns iterations.
+static int inner_bicgstab32(xs_ctx *c, const float *rhs, float *d, int maxit, int *iters) {
+  hipStream_t s = c->stream;
+  const int n = c->A.n;
+  float *W = c->work32;
+  float *r = W, *rh = W + (size_t)n, *p = W + 2 * (size_t)n, *u = W + 3 * (size_t)n;
+  float *sv = W + 4 * (size_t)n, *v = W + 5 * (size_t)n, *z = W + 6 * (size_t)n;
+  float *z1 = W + 7 * (size_t)n, *t = W + 8 * (size_t)n;
+  xs_state *st = c->st_in;
+  const float *jac = (c->pc == XS_PC_JACOBI) ? c->jac32 : nullptr;
+  xs_init_state(s, st, c->inner_tol, 0.0, maxit);
+  xs_zero32(s, n, d);
+  xs_zero32(s, n, p);
+  xs_zero32(s, n, u);
+  xs_reduce r0{}; r0.partial = c->partial; r0.counter = c->counters; r0.st = st; r0.op = F_RES0;
+  op_spmv32(c, d, r, EPI_BMAXN, rhs, nullptr, &r0);
+  xs_copy32(s, n, r, rh);
+  xs_reduce rg = r0, ra = r0, rr = r0;
+  rg.op = F_BICG_GAMA; ra.op = F_BICG_ALPHA; rr.op = F_BICG_RES;
+  int launched = 0;
+  xs_state h{};
+  XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
+  XS_CHK(hipStreamSynchronize(s));
+  while (!h.done && launched < maxit) {
+    const int k = std::min(c->check_every, maxit - launched);
+    for (int j = 0; j < k; j++) {
+      xs_bicg_p32(s, n, r, u, p, jac, z, st);
+      if (!jac) op_pc32(c, p, z, t, st);
+      op_spmv32(c, z, u, EPI_DOT1, nullptr, rh, &rg);
+      xs_bicg_s32(s, n, r, u, sv, jac, z1, st);
+      if (!jac) op_pc32(c, sv, z1, t, st);
+      op_spmv32(c, z1, v, EPI_DOT2, nullptr, sv, &ra);
+      xs_bicg_xr32(s, n, d, r, z, z1, sv, v, rh, rr);
+    }
+    launched += k;
+    XS_CHK(hipGetLastError());
+    XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
+    XS_CHK(hipStreamSynchronize(s));
+  }
+  *iters = h.iter;
+  return h.done == 2 ? XS_ERR_BREAKDOWN : XS_OK;
+}
+
+int xs_solve_mixed(xs_ctx *c, const double *b, double *x) {
+  if (c->ksp != XS_KSP_BICGSTAB) return XS_ERR_UNSUPPORTED;
+  int rc = xs_ctx_ensure_work(c, XS_NWORK);
+  if (rc) return rc;
+  if ((rc = mixed_prepare(c))) return rc;
+  hipStream_t s = c->stream;
+  const int n = c->A.n;
+  double *r64 = xs_ctx_vec(c, 0);
+  float *rhs32 = c->work32 + 9 * (size_t)n, *d32 = c->work32 + 10 * (size_t)n;
+  xs_state *st = c->st;
+  xs_init_state(s, st, c->tol, c->abstol, c->maxiter);
+  xs_reduce rn = xs_ctx_reduce(c, F_STORE, S_SCRATCH0 + 1);   // ||r||^2 -> sc[S_SCRATCH0+1]
+  std::vector<double> hist;
+  double res0 = 0.0, res = 0.0;
+  int total = 0, outer = 0;
+  rc = XS_NOT_CONVERGED;
+  for (;;) {
+    xs_op_spmv(c, x, r64, EPI_BMAXN, b, nullptr, &rn);           // FP64 true residual
+    xs_state h{};
+    XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
+    XS_CHK(hipStreamSynchronize(s));
+    res = std::sqrt(h.sc[S_SCRATCH0 + 1]);
+    if (outer == 0) res0 = res;
+    hist.push_back(res);
+    if (!std::isfinite(res)) { rc = XS_ERR_BREAKDOWN; break; }
+    if (res == 0.0 || res / res0 < c->tol || res <= c->abstol) { rc = XS_OK; break; }
+    if (total >= c->maxiter) break;
+    xs_d2f_scaled(s, n, r64, rhs32, st, S_SCRATCH0 + 1);
+    int it = 0;
+    int irc = inner_bicgstab32(c, rhs32, d32, c->maxiter - total, &it);
+    total += it;
+    xs_add_f2d(s, n, d32, x, st, S_SCRATCH0 + 1);
+    outer++;
+    if (irc == XS_ERR_BREAKDOWN && it == 0) { rc = XS_ERR_BREAKDOWN; break; }
+  }
+  XS_CHK(hipGetLastError());
+  c->iters = total;
+  c->outer_iters = outer;
+  c->res0 = res0;
+  c->resl = res;
+  // residual history on the host side of the context (resvec_d holds room)
+  const int m = std::min<int>((int)hist.size(), c->resvec_cap);
+  XS_CHK(hipMemcpy(c->resvec_d, hist.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+  return rc;
 }

@@ -80,6 +80,18 @@ struct xs_ctx {
   int prof_next_kind = 0;
   std::vector<int> prof_kinds;
 
+  // mixed precision (precision = 1): FP32 inner BiCGSTAB / CG inside an FP64
+  // iterative-refinement loop; factors stored as FP32, BF16 or FP16
+  int factor_type = 0;          // XS_F32 / XS_BF16 / XS_F16
+  double inner_tol = 1e-4;      // relative reduction asked of each inner solve
+  bool mixed_ready = false;
+  float *A32 = nullptr, *ell32 = nullptr, *dinv32 = nullptr, *jac32 = nullptr;
+  void *F32 = nullptr;
+  float *work32 = nullptr;
+  size_t work32_n = 0;
+  xs_state *st_in = nullptr;
+  int outer_iters = 0;
+
   // results
   int iters = 0;
   double res0 = 0.0, resl = 0.0;
@@ -104,5 +116,7 @@ int xs_solve_bicgstab(xs_ctx *c, const double *b, double *x);
 int xs_solve_gcr(xs_ctx *c, const double *b, double *x);
 int xs_solve_gmres(xs_ctx *c, const double *b, double *x, bool flexible);
 int xs_parilut_factor(xs_ctx *c);
+int xs_solve_mixed(xs_ctx *c, const double *b, double *x);
+void xs_mixed_free(xs_ctx *c);
 int xs_build_transpose(xs_ctx *c);
 void xs_solvers_free(xs_ctx *c);
