@@ -147,9 +147,38 @@ int xs_pc_apply_t(xs_ctx *ctx, const double *d_x, double *d_y, void *stream);
 /* Single triangular solves with the ILU factors: which = 0 lower (unit),
  * 1 upper (with diagonal). */
 int xs_trsv(xs_ctx *ctx, int which, const double *d_b, double *d_x, void *stream);
+/* Adjoint solve A^T x = b with the transposed preconditioner M^{-T}
+ * (U^T then L^T on the transposed factors), same Krylov method, device
+ * vectors (d_sol = x0 in, x out).  No reference counterpart. */
+int xs_solve_transpose_device(xs_ctx *ctx, const double *d_rhs, double *d_sol);
+/* dL/dA_ij = -lambda_i x_j on the pattern of A (nnz values in CSR order).
+ * No reference counterpart. */
+int xs_grad_values_device(xs_ctx *ctx, const double *d_lambda, const double *d_x,
+                          double *d_grad, void *stream);
 /* Copy the ILU factors to the host in combined CSR form (A's pattern for
  * ILU(0)): nnz values, plus the position of each pivot.  Returns the nnz. */
 int xs_pc_get_factors(xs_ctx *ctx, int *rowptr, int *colidx, double *val, int *diagpos, int cap);
+
+/* ======================================================================== */
+/* Batched solve: many independent systems in one persistent launch          */
+/* (no reference counterpart; the reference solves one system per call).     */
+/* ======================================================================== */
+typedef struct xs_batch xs_batch;
+/* Block-diagonal CSR (0-based): rows sys_ptr[s]..sys_ptr[s+1]-1 form system s
+ * and its columns stay in that range.  Copied to the GPU. */
+int xs_batch_create(xs_batch **bt, int nsys, const int *sys_ptr, const int *rowptr,
+                    const int *colidx, const double *val);
+/* CG + Jacobi on every system to ||b - A x|| < tol ||b|| (FP64 check), each
+ * system with its own iteration count; precision 0 = FP64, 1 = FP32 CG inside
+ * FP64 iterative refinement.  sol = x0 in, x out (host or device vectors). */
+int xs_batch_solve(xs_batch *bt, const double *rhs, double *sol, double tol, int maxiter,
+                   int precision);
+int xs_batch_solve_device(xs_batch *bt, const double *d_rhs, double *d_sol, double tol,
+                          int maxiter, int precision);
+/* Per-system iterations and final FP64 relative residual; returns nsys. */
+int xs_batch_get_results(const xs_batch *bt, int *iters, double *relres);
+double xs_batch_get_time_ms(const xs_batch *bt);
+int xs_batch_destroy(xs_batch *bt);
 
 #ifdef __cplusplus
 } /* extern "C" */

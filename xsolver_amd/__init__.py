@@ -41,6 +41,9 @@ EXPORTS = [
     "xs_solve_device", "xs_get_iterations", "xs_get_residual", "xs_get_initial_residual",
     "xs_get_resvec", "xs_get_time_ms", "xs_spmv", "xs_bmax", "xs_spmv_t", "xs_pc_apply",
     "xs_pc_apply_t", "xs_trsv", "xs_pc_get_factors",
+    "xs_batch_create", "xs_batch_solve", "xs_batch_solve_device", "xs_batch_get_results",
+    "xs_batch_get_time_ms", "xs_batch_destroy", "xs_solve_transpose_device",
+    "xs_grad_values_device",
 ]
 
 
@@ -92,6 +95,14 @@ def lib():
         "xs_pc_apply_t": (_i, [_vp, _vp, _vp, _vp]),
         "xs_trsv": (_i, [_vp, _i, _vp, _vp, _vp]),
         "xs_pc_get_factors": (_i, [_vp, _vp, _vp, _vp, _vp, _i]),
+        "xs_batch_create": (_i, [ctypes.POINTER(_vp), _i, _vp, _vp, _vp, _vp]),
+        "xs_batch_solve": (_i, [_vp, _vp, _vp, _d, _i, _i]),
+        "xs_batch_solve_device": (_i, [_vp, _vp, _vp, _d, _i, _i]),
+        "xs_batch_get_results": (_i, [_vp, _vp, _vp]),
+        "xs_batch_get_time_ms": (_d, [_vp]),
+        "xs_batch_destroy": (_i, [_vp]),
+        "xs_solve_transpose_device": (_i, [_vp, _vp, _vp]),
+        "xs_grad_values_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
         "xsolver_mat_setup": (None, [_i, _vp, _vp, _vp, _vp]),
         "xsolver_ksp_settype": (None, [_cp]),
         "xsolver_ksp_set_absolute_tol": (None, [_d]),
@@ -235,6 +246,14 @@ class XSolver:
     def trsv(self, which, b, x, stream=None):
         return _check(lib().xs_trsv(self.h, int(which), _ptr(b), _ptr(x), stream), "xs_trsv")
 
+    def solve_transpose_device(self, d_rhs, d_sol):
+        return _check(lib().xs_solve_transpose_device(self.h, _ptr(d_rhs), _ptr(d_sol)),
+                      "xs_solve_transpose_device")
+
+    def grad_values(self, d_lambda, d_x, d_grad, stream=None):
+        return _check(lib().xs_grad_values_device(self.h, _ptr(d_lambda), _ptr(d_x), _ptr(d_grad),
+                                                  stream), "xs_grad_values_device")
+
     def factors(self, n, cap):
         rp = np.zeros(n + 1, np.int32)
         ci = np.zeros(cap, np.int32)
@@ -243,6 +262,71 @@ class XSolver:
         nnz = _check(lib().xs_pc_get_factors(self.h, _ptr(rp), _ptr(ci), _ptr(v), _ptr(dp), cap),
                      "xs_pc_get_factors")
         return rp, ci[:nnz], v[:nnz], dp
+
+
+class XBatch:
+    """Many independent systems solved in one persistent launch
+    (block-diagonal CSR, sys_ptr = row offsets of the systems)."""
+
+    def __init__(self, sys_ptr, rowptr, colidx, val):
+        self.sys_ptr = np.ascontiguousarray(sys_ptr, np.int32)
+        rp = np.ascontiguousarray(rowptr, np.int32)
+        ci = np.ascontiguousarray(colidx, np.int32)
+        v = np.ascontiguousarray(val, np.float64)
+        h = ctypes.c_void_p()
+        _check(lib().xs_batch_create(ctypes.byref(h), len(self.sys_ptr) - 1, _ptr(self.sys_ptr),
+                                     _ptr(rp), _ptr(ci), _ptr(v)), "xs_batch_create")
+        self.h = h
+        self.nsys = len(self.sys_ptr) - 1
+
+    def solve(self, rhs, tol=1e-8, maxiter=5000, precision=0, x0=None):
+        b = np.ascontiguousarray(rhs, np.float64)
+        x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64, copy=True)
+        st = _check(lib().xs_batch_solve(self.h, _ptr(b), _ptr(x), tol, maxiter, precision),
+                    "xs_batch_solve")
+        return x, st
+
+    def solve_device(self, d_rhs, d_sol, tol=1e-8, maxiter=5000, precision=0):
+        return _check(lib().xs_batch_solve_device(self.h, _ptr(d_rhs), _ptr(d_sol), tol, maxiter,
+                                                  precision), "xs_batch_solve_device")
+
+    def results(self):
+        it = np.zeros(self.nsys, np.int32)
+        rr = np.zeros(self.nsys)
+        lib().xs_batch_get_results(self.h, _ptr(it), _ptr(rr))
+        return it, rr
+
+    @property
+    def time_ms(self):
+        return lib().xs_batch_get_time_ms(self.h)
+
+    def close(self):
+        if self.h:
+            lib().xs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_concat(systems):
+    """Stack CSR systems into one block-diagonal CSR; returns (sys_ptr, rp, ci, v)."""
+    sys_ptr = [0]
+    rps, cis, vs = [], [], []
+    nnz_off = 0
+    for rp, ci, v in systems:
+        n = len(rp) - 1
+        rps.append(np.asarray(rp[:-1], np.int64) + nnz_off)
+        cis.append(np.asarray(ci, np.int64) + sys_ptr[-1])
+        vs.append(np.asarray(v, np.float64))
+        nnz_off += len(v)
+        sys_ptr.append(sys_ptr[-1] + n)
+    rp = np.concatenate(rps + [np.array([nnz_off])]).astype(np.int32)
+    return (np.array(sys_ptr, np.int32), rp, np.concatenate(cis).astype(np.int32),
+            np.concatenate(vs))
 
 
 def legacy():

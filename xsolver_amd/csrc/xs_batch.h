@@ -1,0 +1,11 @@
+// xs_batch.h -- persistent batched CG kernels (xs_batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+void xs_batch_prep(hipStream_t s, int n, const int *rp, const int *ci, const double *v,
+                   double *dinv, float *dinv32, float *v32, int *bad);
+void xs_batch_cg(hipStream_t s, int grid, int nsys, const int *order, const int *sys_ptr,
+                 const int *rp, const int *ci, const double *v, const float *v32,
+                 const double *dinv, const float *dinv32, const double *b, double *x,
+                 double *w64, float *w32, double tol, int maxiter, int precision,
+                 double inner_tol, unsigned *queue, int *iters, double *relres);

@@ -1,0 +1,217 @@
+// xs_batch.hip -- many independent systems in one persistent launch.
+//
+// The systems are the diagonal blocks of one block-diagonal CSR matrix
+// (sys_ptr[s]..sys_ptr[s+1] are the rows of system s, with columns in the
+// same range).  Each workgroup takes systems from an atomic queue (largest
+// first, a longest-processing-time order) and runs preconditioned CG with
+// Jacobi (hip/Solver/xsolver_cg.cpp:16-74) on it to convergence with
+// workgroup barriers only: every system stops at its own iteration count
+// (per-system convergence), and no kernel is launched per iteration.
+// Precision 1 runs the CG in FP32 inside FP64 iterative refinement: the FP64
+// residual decides convergence, FP32 CG solves each correction to inner_tol.
+// No reference counterpart (the reference solves one system per call).
+#include "xs_device.h"
+#include "xs_batch.h"
+
+#define BT_THREADS 1024
+#define BT_WAVES (BT_THREADS / XS_WAVE)
+
+template <class T>
+__device__ __forceinline__ T bt_wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, XS_WAVE);
+  return v;
+}
+
+// workgroup sum of two values (double accumulation)
+__device__ __forceinline__ void bt_sum2(double &a, double &b, double (*red)[2]) {
+  const int lane = threadIdx.x & (XS_WAVE - 1), w = threadIdx.x / XS_WAVE;
+  a = bt_wave_sum(a);
+  b = bt_wave_sum(b);
+  __syncthreads();
+  if (lane == 0) { red[w][0] = a; red[w][1] = b; }
+  __syncthreads();
+  double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+  for (int q = 0; q < BT_WAVES; ++q) { s0 += red[q][0]; s1 += red[q][1]; }
+  a = s0;
+  b = s1;
+}
+
+// y = A p over rows [r0, r1) (thread-per-row, block-stride) and sum p.y
+template <class TV, class TX>
+__device__ __forceinline__ double bt_spmv_dot(int r0, int r1, const int *__restrict__ rp,
+                                              const int *__restrict__ ci,
+                                              const TV *__restrict__ v, const TX *p, TX *y) {
+  double acc = 0.0;
+  for (int r = r0 + threadIdx.x; r < r1; r += BT_THREADS) {
+    TX s = 0;
+    for (int e = rp[r]; e < rp[r + 1]; ++e) s += (TX)v[e] * p[ci[e]];
+    y[r] = s;
+    acc += (double)s * (double)p[r];
+  }
+  return acc;
+}
+
+// CG + Jacobi on rows [r0, r1) for A d = rhs from d = 0 (FP32 or FP64);
+// stops when ||r|| <= tol * ||rhs||.  Returns iterations (barrier-uniform).
+template <class TV, class TX>
+__device__ int bt_cg(int r0, int r1, const int *__restrict__ rp, const int *__restrict__ ci,
+                     const TV *__restrict__ v, const TX *__restrict__ dinv, const TX *rhs, TX *d,
+                     TX *r, TX *z, TX *p, TX *q, double tol, int maxit, double (*red)[2]) {
+  double tau = 0.0, rr = 0.0;
+  for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) {
+    const TX ri = rhs[i];
+    const TX zi = dinv[i] * ri;
+    d[i] = 0;
+    r[i] = ri;
+    z[i] = zi;
+    p[i] = zi;
+    tau += (double)ri * (double)zi;
+    rr += (double)ri * (double)ri;
+  }
+  bt_sum2(tau, rr, red);
+  const double res0 = sqrt(rr);
+  if (res0 == 0.0) return 0;
+  int it = 0;
+  while (it < maxit) {
+    __syncthreads();
+    double sigma = bt_spmv_dot<TV, TX>(r0, r1, rp, ci, v, p, q), dummy = 0.0;
+    bt_sum2(sigma, dummy, red);
+    const TX alpha = (TX)(tau / sigma);
+    double tn = 0.0, rn = 0.0;
+    for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) {
+      d[i] += alpha * p[i];
+      const TX ri = r[i] - alpha * q[i];
+      r[i] = ri;
+      const TX zi = dinv[i] * ri;
+      z[i] = zi;
+      tn += (double)zi * (double)ri;
+      rn += (double)ri * (double)ri;
+    }
+    bt_sum2(tn, rn, red);
+    const TX beta = (TX)(tn / tau);
+    tau = tn;
+    ++it;
+    if (!(sqrt(rn) > tol * res0)) break;   // also stops on NaN
+    for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) p[i] = z[i] + beta * p[i];
+  }
+  __syncthreads();
+  return it;
+}
+
+__global__ __launch_bounds__(BT_THREADS) void k_batch_cg(
+    int nsys, const int *__restrict__ order, const int *__restrict__ sys_ptr,
+    const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v,
+    const float *__restrict__ v32, const double *__restrict__ dinv,
+    const float *__restrict__ dinv32, const double *__restrict__ b, double *__restrict__ x,
+    double *__restrict__ w64, float *__restrict__ w32, double tol, int maxiter, int precision,
+    double inner_tol, unsigned *queue, int *iters, double *relres) {
+  __shared__ double red[BT_WAVES][2];
+  __shared__ int s_sys;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const unsigned t = atomicAdd(queue, 1u);
+      s_sys = t < (unsigned)nsys ? order[t] : -1;
+    }
+    __syncthreads();
+    const int sys = s_sys;
+    __syncthreads();
+    if (sys < 0) break;
+    const int r0 = sys_ptr[sys], r1 = sys_ptr[sys + 1];
+    int it = 0;
+    double rel = 0.0;
+    if (precision == 0) {
+      // FP64 CG from x0 (the workspace holds r, z, p, q, rhs)
+      double *r = w64, *z = w64 + sys_ptr[nsys], *p = z + sys_ptr[nsys], *q = p + sys_ptr[nsys];
+      double *rhs = q + sys_ptr[nsys], *d = rhs + sys_ptr[nsys];
+      double bb = 0.0, dummy = 0.0;
+      for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) {
+        double s = 0.0;
+        for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
+        rhs[i] = b[i] - s;
+        bb += b[i] * b[i];
+      }
+      bt_sum2(bb, dummy, red);
+      it = bt_cg<double, double>(r0, r1, rp, ci, v, dinv, rhs, d, r, z, p, q, tol, maxiter, red);
+      double rn = 0.0, dm = 0.0;
+      for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) {
+        x[i] += d[i];
+      }
+      __syncthreads();
+      for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) {
+        double s = 0.0;
+        for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
+        const double t = b[i] - s;
+        rn += t * t;
+      }
+      bt_sum2(rn, dm, red);
+      rel = bb > 0.0 ? sqrt(rn / bb) : 0.0;
+    } else {
+      // FP64 refinement around FP32 CG corrections
+      const int N = sys_ptr[nsys];
+      float *r = w32, *z = w32 + N, *p = z + N, *q = p + N, *rhs = q + N, *d = rhs + N;
+      double *res64 = w64;
+      double bb = 0.0, dm = 0.0;
+      for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) bb += b[i] * b[i];
+      bt_sum2(bb, dm, red);
+      for (int outer = 0; outer < 50 && it < maxiter; ++outer) {
+        double rn = 0.0, d2 = 0.0;
+        __syncthreads();
+        for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) {
+          double s = 0.0;
+          for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
+          const double t = b[i] - s;
+          res64[i] = t;
+          rn += t * t;
+        }
+        bt_sum2(rn, d2, red);
+        rel = bb > 0.0 ? sqrt(rn / bb) : 0.0;
+        if (!(rel >= tol)) break;
+        const double sc = 1.0 / sqrt(rn);
+        for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) rhs[i] = (float)(res64[i] * sc);
+        __syncthreads();
+        it += bt_cg<float, float>(r0, r1, rp, ci, v32, dinv32, rhs, d, r, z, p, q, inner_tol,
+                                  maxiter - it, red);
+        const double back = sqrt(rn);
+        for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) x[i] += back * (double)d[i];
+      }
+    }
+    if (threadIdx.x == 0) {
+      iters[sys] = it;
+      relres[sys] = rel;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_batch_prep(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                             const double *__restrict__ v, double *__restrict__ dinv,
+                             float *__restrict__ dinv32, float *__restrict__ v32, int *bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int e = rp[i]; e < rp[i + 1]; ++e) {
+    if (ci[e] == i) d = v[e];
+    v32[e] = (float)v[e];
+  }
+  if (d == 0.0) atomicMin(bad, i + 1);
+  dinv[i] = 1.0 / d;
+  dinv32[i] = (float)(1.0 / d);
+}
+
+void xs_batch_prep(hipStream_t s, int n, const int *rp, const int *ci, const double *v,
+                   double *dinv, float *dinv32, float *v32, int *bad) {
+  hipLaunchKernelGGL(k_batch_prep, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, v, dinv,
+                     dinv32, v32, bad);
+}
+
+void xs_batch_cg(hipStream_t s, int grid, int nsys, const int *order, const int *sys_ptr,
+                 const int *rp, const int *ci, const double *v, const float *v32,
+                 const double *dinv, const float *dinv32, const double *b, double *x,
+                 double *w64, float *w32, double tol, int maxiter, int precision,
+                 double inner_tol, unsigned *queue, int *iters, double *relres) {
+  hipLaunchKernelGGL(k_batch_cg, dim3(grid), dim3(BT_THREADS), 0, s, nsys, order, sys_ptr, rp, ci,
+                     v, v32, dinv, dinv32, b, x, w64, w32, tol, maxiter, precision, inner_tol,
+                     queue, iters, relres);
+}

@@ -65,7 +65,7 @@ static void free_precond(xs_ctx *c) {
   xs_ilu &F = c->ilu;
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.ucp); dfree(F.uri); dfree(F.upos);
   dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own); dfree(F.trp); dfree(F.tci);
-  dfree(F.tdpos); dfree(F.tlu); dfree(F.scale);
+  dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
   F = xs_ilu();
   dfree(c->lu_tmp);
   c->pc_ready = false;
@@ -82,6 +82,7 @@ static void free_matrix(xs_ctx *c) {
   c->At = xs_csr();
   dfree(c->planT.blk_row);
   c->planT = xs_spmv_plan();
+  dfree(c->perm_At);
   c->have_At = false;
   c->h_rp.clear(); c->h_ci.clear();
   free_precond(c);
@@ -338,7 +339,7 @@ extern "C" int xs_mat_update_values(xs_ctx *c, const double *val, int on_device)
   XS_HIP(hipMemcpy(c->A.v, val, sizeof(double) * c->A.nnz,
                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
   if (c->use_ell) xs_build_ell(c->stream, c->A, c->ell.W, c->ell);
-  if (c->have_At) c->have_At = false;
+  if (c->have_At) xs_permute_vals(c->stream, c->A.nnz, c->perm_At, c->A.v, c->At.v);
   c->pc_ready = false;
   XS_HIP(hipStreamSynchronize(c->stream));
   return XS_OK;
@@ -675,6 +676,7 @@ extern "C" int xs_pc_setup(xs_ctx *c) {
   c->t_pc_ms = ms;
   if (xs_take_launch_guard()) rc = XS_ERR_STATE;
   if (rc == XS_OK) c->pc_ready = true;
+  c->ilu.have_t = false;
   return rc;
 }
 
@@ -721,7 +723,9 @@ void xs_prof_collect(xs_ctx *c) {
 void xs_op_spmv(xs_ctx *c, const double *x, double *y, int epi, const double *b,
                 const double *w, const xs_reduce *red) {
   xs_prof_begin(c, 1);
-  if (c->use_ell && c->ell.v)
+  if (c->use_t)
+    xs_launch_spmv_csr(c->stream, c->At, c->planT, x, y, epi, b, w, red);
+  else if (c->use_ell && c->ell.v)
     xs_launch_spmv_ell(c->stream, c->ell, x, y, epi, b, w, red);
   else
     xs_launch_spmv_csr(c->stream, c->A, c->plan, x, y, epi, b, w, red);
@@ -736,6 +740,13 @@ void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state
       break;
     case XS_PC_ILU0:
     case XS_PC_ILUT:
+      if (c->use_t) {
+        xs_fill_sentinel(c->stream, tmp, c->A.n);
+        xs_trsv_lower_t(c->stream, c->ilu, x, tmp, c->counters + 2, st);
+        xs_fill_sentinel(c->stream, y, c->A.n);
+        xs_trsv_upper_t(c->stream, c->ilu, tmp, y, c->counters + 4, st);
+        break;
+      }
       xs_fill_sentinel(c->stream, tmp, c->A.n);
       xs_prof_begin(c, 2);
       xs_trsv_lower(c->stream, c->ilu, x, tmp, c->counters + 2, st);
@@ -850,6 +861,102 @@ extern "C" double xs_get_time_ms(const xs_ctx *c, const char *phase) {
     if (cnt == phase) return (double)c->prof_cnt[k];
   }
   return -1.0;
+}
+
+// ---------------------------------------------------------------------------
+// transposes for the adjoint (no reference counterpart)
+// ---------------------------------------------------------------------------
+// Transposed pattern on the host (rows of the transpose are produced in
+// ascending column order, so they come out sorted); perm[k] = position of
+// entry k in the transpose.
+static void transpose_pattern(int n, const int *rp, const int *ci, std::vector<int> &trp,
+                              std::vector<int> &tci, std::vector<int> &perm) {
+  const int nnz = rp[n];
+  trp.assign(n + 1, 0);
+  for (int e = 0; e < nnz; e++) trp[ci[e] + 1]++;
+  for (int j = 0; j < n; j++) trp[j + 1] += trp[j];
+  std::vector<int> fill(trp.begin(), trp.end() - 1);
+  tci.resize(nnz);
+  perm.resize(nnz);
+  for (int i = 0; i < n; i++)
+    for (int e = rp[i]; e < rp[i + 1]; e++) {
+      const int q = fill[ci[e]]++;
+      tci[q] = i;
+      perm[e] = q;
+    }
+}
+
+int xs_build_transpose(xs_ctx *c) {
+  if (c->have_At) return XS_OK;
+  const int n = c->A.n, nnz = c->A.nnz;
+  std::vector<int> trp, tci, perm;
+  transpose_pattern(n, c->h_rp.data(), c->h_ci.data(), trp, tci, perm);
+  int rc;
+  dfree(c->At.rp); dfree(c->At.ci); dfree(c->At.v); dfree(c->perm_At); dfree(c->planT.blk_row);
+  if ((rc = dalloc(&c->At.rp, n + 1)) || (rc = dalloc(&c->At.ci, nnz)) ||
+      (rc = dalloc(&c->At.v, nnz)) || (rc = dalloc(&c->perm_At, nnz)))
+    return rc;
+  XS_HIP(hipMemcpy(c->At.rp, trp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  XS_HIP(hipMemcpy(c->At.ci, tci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+  XS_HIP(hipMemcpy(c->perm_At, perm.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+  c->At.n = n;
+  c->At.nnz = nnz;
+  xs_permute_vals(c->stream, nnz, c->perm_At, c->A.v, c->At.v);
+  double avg = (double)nnz / n;
+  int chunk = std::min(2048, std::max(512, ((int)(avg * XS_BLOCK) + XS_BLOCK - 1) / XS_BLOCK * XS_BLOCK));
+  int nblk = std::max(1, (nnz + chunk - 1) / chunk);
+  if ((rc = dalloc(&c->planT.blk_row, (size_t)nblk + 1))) return rc;
+  xs_build_plan(c->stream, c->At, c->planT, c->planT.blk_row);
+  XS_HIP(hipStreamSynchronize(c->stream));
+  if (xs_take_launch_guard()) return XS_ERR_STATE;
+  c->have_At = true;
+  return ensure_common(c);
+}
+
+int xs_build_factor_transpose(xs_ctx *c) {
+  xs_ilu &F = c->ilu;
+  if (F.have_t) return XS_OK;
+  const int n = F.n, nnz = F.nnz;
+  std::vector<int> frp(n + 1), fci(nnz), trp, tci, perm;
+  XS_HIP(hipMemcpy(frp.data(), F.rp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  XS_HIP(hipMemcpy(fci.data(), F.ci, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+  transpose_pattern(n, frp.data(), fci.data(), trp, tci, perm);
+  std::vector<int> tdpos(n, -1);
+  for (int i = 0; i < n; i++)
+    for (int e = trp[i]; e < trp[i + 1]; e++)
+      if (tci[e] == i) tdpos[i] = e;
+  int rc;
+  dfree(F.trp); dfree(F.tci); dfree(F.tdpos); dfree(F.tlu); dfree(F.tperm);
+  if ((rc = dalloc(&F.trp, n + 1)) || (rc = dalloc(&F.tci, nnz)) || (rc = dalloc(&F.tdpos, n)) ||
+      (rc = dalloc(&F.tlu, nnz)) || (rc = dalloc(&F.tperm, nnz)))
+    return rc;
+  XS_HIP(hipMemcpy(F.trp, trp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  XS_HIP(hipMemcpy(F.tci, tci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+  XS_HIP(hipMemcpy(F.tdpos, tdpos.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  XS_HIP(hipMemcpy(F.tperm, perm.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+  xs_permute_vals(c->stream, nnz, F.tperm, F.lu, F.tlu);
+  XS_HIP(hipStreamSynchronize(c->stream));
+  if (xs_take_launch_guard()) return XS_ERR_STATE;
+  F.have_t = true;
+  return XS_OK;
+}
+
+// y = M^{-T} x: U^T (lower, 1/u_ii) then L^T (upper, unit diagonal)
+int xs_op_pc_t(xs_ctx *c, const double *x, double *y, double *tmp, hipStream_t s) {
+  if (c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) {
+    int rc = xs_build_factor_transpose(c);
+    if (rc) return rc;
+    xs_fill_sentinel(s, tmp, c->A.n);
+    xs_trsv_lower_t(s, c->ilu, x, tmp, c->counters + 2, nullptr);
+    xs_fill_sentinel(s, y, c->A.n);
+    xs_trsv_upper_t(s, c->ilu, tmp, y, c->counters + 4, nullptr);
+  } else if (c->pc == XS_PC_JACOBI) {
+    xs_scale(s, c->A.n, c->jac_dinv, x, y);
+  } else {
+    xs_copy(s, c->A.n, x, y);
+  }
+  XS_HIP(hipGetLastError());
+  return xs_take_launch_guard() ? XS_ERR_STATE : XS_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1001,3 +1108,30 @@ extern "C" void xsolver_free(void) {
 extern "C" int xsolver_last_error(void) { return g_err; }
 extern "C" int xsolver_get_iterations(void) { return g_ctx ? g_ctx->iters : -1; }
 extern "C" double xsolver_get_residual(void) { return g_ctx ? g_ctx->resl : -1.0; }
+
+// ---------------------------------------------------------------------------
+// adjoint (no reference counterpart)
+// ---------------------------------------------------------------------------
+extern "C" int xs_solve_transpose_device(xs_ctx *c, const double *d_rhs, double *d_sol) {
+  if (!c || !d_rhs || !d_sol) return XS_ERR_ARG;
+  if (!c->mat_ready) return XS_ERR_STATE;
+  if (c->precision != 0) return XS_ERR_UNSUPPORTED;
+  XS_HIP(hipDeviceSynchronize());
+  int rc;
+  if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
+  if ((rc = xs_build_transpose(c))) return rc;
+  if ((c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) && (rc = xs_build_factor_transpose(c))) return rc;
+  c->use_t = true;
+  rc = run_solve(c, d_rhs, d_sol);
+  c->use_t = false;
+  return rc;
+}
+
+extern "C" int xs_grad_values_device(xs_ctx *c, const double *d_lambda, const double *d_x,
+                                     double *d_grad, void *stream) {
+  if (!c || !d_lambda || !d_x || !d_grad) return XS_ERR_ARG;
+  if (!c->mat_ready) return XS_ERR_STATE;
+  xs_grad_vals(pick(c, stream), c->A, d_lambda, d_x, d_grad);
+  XS_HIP(hipGetLastError());
+  return xs_take_launch_guard() ? XS_ERR_STATE : XS_OK;
+}

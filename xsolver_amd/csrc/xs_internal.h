@@ -97,7 +97,7 @@ struct xs_ilu {
   int *rowidx = nullptr;    // row of each nnz (COO)
   // transposed factors for the adjoint (built on demand)
   bool have_t = false;
-  int *trp = nullptr, *tci = nullptr, *tdpos = nullptr;
+  int *trp = nullptr, *tci = nullptr, *tdpos = nullptr, *tperm = nullptr;
   double *tlu = nullptr;
 };
 
@@ -147,6 +147,12 @@ void xs_trsv_lower(hipStream_t s, const xs_ilu &F, const double *b, double *x, u
                    const xs_state *st);
 void xs_trsv_upper(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
                    const xs_state *st);
+void xs_trsv_lower_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
+                     unsigned *tickets, const xs_state *st);
+void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
+                     unsigned *tickets, const xs_state *st);
+void xs_permute_vals(hipStream_t s, int nnz, const int *perm, const double *src, double *dst);
+void xs_grad_vals(hipStream_t s, const xs_csr &A, const double *lam, const double *x, double *g);
 void xs_jacobi_dinv(hipStream_t s, const xs_csr &A, double *dinv, int *d_bad);
 // 1 if a launcher refused to launch since the last call (and clears it)
 int xs_take_launch_guard();

@@ -579,6 +579,8 @@ void xs_ilu_dinv(hipStream_t s, xs_ilu &F, int *d_bad) {
 // is pre-filled with a signalling-NaN sentinel, each row's value is published
 // with an agent-scope atomic store, and a dependent row spins (with s_sleep)
 // on an agent-scope atomic load until the sentinel is gone.
+// The diagonal is unit when dinv is null (the L factor, and L^T in the
+// transposed solve) and 1/u_ii otherwise (U, and U^T).
 // Dependencies inside a wave are resolved in registers: a log-step affine scan
 // when each row depends at most on its neighbour (x_i = c_i + m_i x_{i-1}), a
 // 64-step broadcast loop otherwise.
@@ -642,6 +644,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
     int eb = 0, ee = 0;  // internal entries [eb, ee)
     if (act) {
       acc = b[r];
+      if (dinv) di = dinv[r];
       if (!UPPER) {
         int e = rp[r];
         const int end = dpos[r];
@@ -656,7 +659,6 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
         for (; e < xe; ++e) acc -= lu[e] * ld_ready(x + ci[e], err);
         eb = xe; ee = end;
       } else {
-        di = dinv[r];
         const int beg = dpos[r] + 1;
         int e = rp[r + 1] - 1;
         int xb = e + 1;
@@ -678,8 +680,8 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
     if (__all(chain)) {
       // x_r = c + m * x_nb, composed along the wave with a log-step scan
       double c, m;
-      if (!UPPER) { c = acc; m = (act && nint == 1) ? -lu[eb] : 0.0; }
-      else { c = act ? di * acc : 0.0; m = (act && nint == 1) ? -di * lu[eb] : 0.0; }
+      c = act ? di * acc : 0.0;
+      m = (act && nint == 1) ? -di * lu[eb] : 0.0;
 #pragma unroll
       for (int d = 1; d < XS_WAVE; d <<= 1) {
         double cj, mj;
@@ -695,14 +697,14 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
         int e = eb;
         int nc = (act && e < ee) ? ci[e] : INT_MAX;
         for (int k = 0; k < nrows; ++k) {
-          const double xk = __shfl(acc, k, XS_WAVE);
+          const double xk = __shfl(acc * di, k, XS_WAVE);
           if (nc == r0 + k) {
             acc -= lu[e] * xk;
             ++e;
             nc = (e < ee) ? ci[e] : INT_MAX;
           }
         }
-        val = acc;
+        val = acc * di;
       } else {
         int e = ee - 1;
         int nc = (act && e >= eb) ? ci[e] : INT_MIN;
@@ -734,7 +736,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
 static void trsv_launch(hipStream_t s, bool upper, int n, const int *rp, const int *ci,
                         const double *lu, const int *dpos, const double *dinv, const double *b,
                         double *x, unsigned *tickets, const xs_state *st) {
-  XS_GUARD(rp && ci && lu && dpos && (!upper || dinv) && b && x && tickets);
+  XS_GUARD(rp && ci && lu && dpos && b && x && tickets);
   const int nw = (n + XS_WAVE - 1) / XS_WAVE;
   dim3 grid((nw + (XS_BLOCK / XS_WAVE) - 1) / (XS_BLOCK / XS_WAVE)), blk(XS_BLOCK);
   if (upper) hipLaunchKernelGGL(k_trsv<true>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, st);
@@ -743,9 +745,44 @@ static void trsv_launch(hipStream_t s, bool upper, int n, const int *rp, const i
 
 void xs_trsv_lower(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
                    const xs_state *st) {
-  trsv_launch(s, false, F.n, F.rp, F.ci, F.lu, F.dpos, F.dinv, b, x, tickets, st);
+  trsv_launch(s, false, F.n, F.rp, F.ci, F.lu, F.dpos, nullptr, b, x, tickets, st);
 }
 void xs_trsv_upper(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
                    const xs_state *st) {
   trsv_launch(s, true, F.n, F.rp, F.ci, F.lu, F.dpos, F.dinv, b, x, tickets, st);
+}
+// Transposed factors (F^T = U^T | L^T on the transposed pattern): the lower
+// solve divides by u_ii, the upper one has a unit diagonal.
+void xs_trsv_lower_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
+                     unsigned *tickets, const xs_state *st) {
+  trsv_launch(s, false, F.n, F.trp, F.tci, F.tlu, F.tdpos, F.dinv, b, x, tickets, st);
+}
+void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
+                     unsigned *tickets, const xs_state *st) {
+  trsv_launch(s, true, F.n, F.trp, F.tci, F.tlu, F.tdpos, nullptr, b, x, tickets, st);
+}
+
+// values of a transposed pattern: dst[perm[k]] = src[k]
+__global__ void k_permute_vals(int nnz, const int *__restrict__ perm, const double *__restrict__ src,
+                               double *__restrict__ dst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nnz) dst[perm[k]] = src[k];
+}
+void xs_permute_vals(hipStream_t s, int nnz, const int *perm, const double *src, double *dst) {
+  XS_GUARD(perm && src && dst);
+  hipLaunchKernelGGL(k_permute_vals, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, perm, src, dst);
+}
+
+// dL/dA_ij = -lambda_i x_j on the pattern of A (thread per row)
+__global__ void k_grad_vals(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                            const double *__restrict__ lam, const double *__restrict__ x,
+                            double *__restrict__ g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double li = lam[i];
+  for (int e = rp[i]; e < rp[i + 1]; ++e) g[e] = -li * x[ci[e]];
+}
+void xs_grad_vals(hipStream_t s, const xs_csr &A, const double *lam, const double *x, double *g) {
+  XS_GUARD(A.rp && A.ci && lam && x && g);
+  hipLaunchKernelGGL(k_grad_vals, dim3((A.n + 255) / 256), dim3(256), 0, s, A.n, A.rp, A.ci, lam, x, g);
 }

@@ -298,8 +298,6 @@ int xs_solve_gcr(xs_ctx *c, const double *b, double *x) {
   if (h.done == 1) return XS_OK;
   return XS_NOT_CONVERGED;
 }
-int xs_build_transpose(xs_ctx *) { return XS_ERR_UNSUPPORTED; }
-int xs_op_pc_t(xs_ctx *, const double *, double *, double *, hipStream_t) { return XS_ERR_UNSUPPORTED; }
 void xs_solvers_free(xs_ctx *c) {
   if (c->basis) (void)hipFree(c->basis);
   c->basis = nullptr;

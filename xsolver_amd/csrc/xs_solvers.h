@@ -39,6 +39,8 @@ struct xs_ctx {
   xs_ell ell;
   std::vector<int> h_rp, h_ci;
   bool have_At = false;
+  bool use_t = false;            // solve with A^T and M^T (adjoint)
+  int *perm_At = nullptr;        // position in A^T of each entry of A
   xs_csr At;
   xs_spmv_plan planT;
 
@@ -119,4 +121,5 @@ int xs_parilut_factor(xs_ctx *c);
 int xs_solve_mixed(xs_ctx *c, const double *b, double *x);
 void xs_mixed_free(xs_ctx *c);
 int xs_build_transpose(xs_ctx *c);
+int xs_build_factor_transpose(xs_ctx *c);
 void xs_solvers_free(xs_ctx *c);
