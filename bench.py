@@ -37,7 +37,7 @@ def algorithmic_bytes(n, nnz, nnz_l, nnz_u, ell_w):
     }
 
 
-def run_gpu(A, b, fmt, steps, warmup, profile=True):
+def run_gpu(A, b, fmt, steps, warmup, profile=True, precision=0, factor_storage=0):
     import torch
     from xsolver_amd import XSolver
 
@@ -52,6 +52,8 @@ def run_gpu(A, b, fmt, steps, warmup, profile=True):
     S.pc_settype("ilu0")
     S.set_option("parilu_sweeps", 20)
     S.set_option("parilu_async", 1)
+    S.set_option("precision", precision)
+    S.set_option("factor_storage", factor_storage)
 
     def step():
         S.set_option("refactor", 1)
@@ -124,6 +126,7 @@ def main():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ell", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -156,6 +159,12 @@ def main():
     relres = float(np.linalg.norm(b - gen.spmv(A, csr["x"])) / np.linalg.norm(b))
     err_xs = float(np.linalg.norm(csr["x"] - xs) / np.linalg.norm(xs))
     ell = None if args.no_ell else run_gpu(A, b, "ell", args.steps, args.warmup)
+    mixed = None if args.no_mixed else run_gpu(A, b, "csr", args.steps, args.warmup,
+                                               precision=1, factor_storage=0)
+    mixed_relres = (float(np.linalg.norm(b - gen.spmv(A, mixed["x"])) / np.linalg.norm(b))
+                    if mixed else None)
+    mixed_err = (float(np.linalg.norm(mixed["x"] - csr["x"]) / np.linalg.norm(csr["x"]))
+                 if mixed else None)
 
     # dominant kernel by total time in the timed CSR steps
     k = csr["kernels"]
@@ -204,6 +213,12 @@ def main():
             "rel_err_vs_xstar": err_xs,
             "ell_sec_per_step": round(ell["sec_per_step"], 5) if ell else None,
             "ell_iterations": int(np.median(ell["iterations"])) if ell else None,
+            "mixed_sec_per_step": round(mixed["sec_per_step"], 5) if mixed else None,
+            "mixed_inner_iterations": int(np.median(mixed["iterations"])) if mixed else None,
+            "mixed_true_relres": mixed_relres,
+            "mixed_rel_err_vs_fp64_gpu": mixed_err,
+            "mixed_speedup_vs_fp64": (round(csr["sec_per_step"] / mixed["sec_per_step"], 3)
+                                      if mixed else None),
         },
         "kernels": kernels_gbs,
         "roofline": roof,

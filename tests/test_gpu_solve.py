@@ -252,3 +252,25 @@ def test_mixed_precision_reaches_fp64_accuracy(dev, pc, factor):
     res = g["res"]
     assert res[-1] / res[0] < 1e-10 and len(res) >= 2
     g["S"].close()
+
+
+@pytest.mark.parametrize("factor", [0, 1], ids=["fp32", "bf16"])
+def test_mixed_gmres_parilut(dev, factor):
+    """GMRES(30) + ParILUT in mixed precision (FP32 inner FGMRES, FP32/BF16
+    factors, FP64 refinement) reaches the FP64-verified 1e-10 residual."""
+    A = gen.lognormal27(16, seed=2)
+    xs, b = gen.manufactured(A, [2, 1])
+    say(f"[mixed gmres] lognormal27_16 factor={factor}")
+    xo, sto, ito, _ = oracle.solve(A, b, "fgmres", "parilut", tol=1e-10, maxiter=500, restart=30,
+                                   ilut_tol=1e-4, ilut_fill=2)
+    S = XSolver()
+    S.mat_setup(*A); S.ksp_settype("fgmres"); S.ksp_setoption(1e-10, 1000, 30)
+    S.pc_settype("ilut"); S.pc_setilut(1e-4, 2)
+    S.set_option("precision", 1); S.set_option("factor_storage", factor)
+    x, st = S.solve(b)
+    it = S.iterations
+    S.close()
+    rr = true_relres(A, b, x)
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    say(f"   oracle it={ito} mixed inner it={it} relres={rr:.2e} err-vs-fp64={err:.2e}")
+    assert st == 0 and rr < 1.05e-10 and err < 1e-7

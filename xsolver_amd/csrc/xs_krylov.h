@@ -18,12 +18,17 @@ struct xs_gmres_small {
 };
 
 int xs_ts_blocks(int n);
-void xs_ts_dot(hipStream_t s, int n, int k, const double *V, size_t ld, const double *w,
+// T = double or float (instantiated in xs_krylov.hip); inner products in double
+template <class T>
+void xs_ts_dot(hipStream_t s, int n, int k, const T *V, size_t ld, const T *w,
                double *partial, unsigned *counter, double *out, const xs_state *st);
-void xs_ts_update(hipStream_t s, int n, int k, const double *V, const double *U, size_t ld,
-                  const double *h, double *w, double *wout, double *u, const xs_state *st);
-void xs_ts_axpy(hipStream_t s, int n, int k, const double *V, size_t ld, const double *y, double *x);
-void xs_ts_scale(hipStream_t s, int n, const double *src, double *dst, const double *scale,
+template <class T>
+void xs_ts_update(hipStream_t s, int n, int k, const T *V, const T *U, size_t ld,
+                  const double *h, T *w, T *wout, T *u, const xs_state *st);
+template <class T>
+void xs_ts_axpy(hipStream_t s, int n, int k, const T *V, size_t ld, const double *y, T *x);
+template <class T>
+void xs_ts_scale(hipStream_t s, int n, const T *src, T *dst, const double *scale,
                  const xs_state *st);
 void xs_gmres_column(hipStream_t s, int i, int m, const xs_gmres_small &g, xs_state *st,
                      double *resvec);

@@ -59,9 +59,11 @@ __device__ __forceinline__ void ts_finish(int k, double (*red)[XS_BLOCK / XS_WAV
   if (threadIdx.x == 0) *counter = 0u;
 }
 
-// out[j] = V_j . w  for j < k (k <= XS_TS_KMAX)
-__global__ __launch_bounds__(XS_BLOCK) void k_ts_dot(int n, int k, const double *__restrict__ V,
-                                                     size_t ld, const double *__restrict__ w,
+// out[j] = V_j . w  for j < k (k <= XS_TS_KMAX); T = double or float
+// vectors, products accumulated in double.
+template <class T>
+__global__ __launch_bounds__(XS_BLOCK) void k_ts_dot(int n, int k, const T *__restrict__ V,
+                                                     size_t ld, const T *__restrict__ w,
                                                      double *partial, unsigned *counter,
                                                      double *out, const xs_state *st) {
   if (st && st->done) return;
@@ -72,15 +74,15 @@ __global__ __launch_bounds__(XS_BLOCK) void k_ts_dot(int n, int k, const double 
 #pragma unroll
   for (int q = 0; q < TS_ROWS; ++q) {
     const int row = base + q * XS_BLOCK;
-    wr[q] = row < n ? w[row] : 0.0;
+    wr[q] = row < n ? (double)w[row] : 0.0;
   }
   for (int j = 0; j < k; ++j) {
-    const double *vj = V + (size_t)j * ld;
+    const T *vj = V + (size_t)j * ld;
     double a = 0.0;
 #pragma unroll
     for (int q = 0; q < TS_ROWS; ++q) {
       const int row = base + q * XS_BLOCK;
-      if (row < n) a += vj[row] * wr[q];
+      if (row < n) a += (double)vj[row] * wr[q];
     }
     a = ts_wave_sum(a);
     if (lane == 0) red[j][wv] = a;
@@ -91,10 +93,11 @@ __global__ __launch_bounds__(XS_BLOCK) void k_ts_dot(int n, int k, const double 
 
 // w -= sum_j h[j] V_j  (and u -= sum_j h[j] U_j when U is given), k <= XS_TS_KMAX;
 // the result may be written to wout instead of w.
-__global__ __launch_bounds__(XS_BLOCK) void k_ts_update(int n, int k, const double *__restrict__ V,
-                                                        const double *__restrict__ U, size_t ld,
-                                                        const double *h, double *w, double *wout,
-                                                        double *u, double sign, const xs_state *st) {
+template <class T>
+__global__ __launch_bounds__(XS_BLOCK) void k_ts_update(int n, int k, const T *__restrict__ V,
+                                                        const T *__restrict__ U, size_t ld,
+                                                        const double *h, T *w, T *wout,
+                                                        T *u, double sign, const xs_state *st) {
   if (st && st->done) return;
   __shared__ double hs[XS_TS_KMAX];
   for (int j = threadIdx.x; j < k; j += XS_BLOCK) hs[j] = sign * h[j];
@@ -104,24 +107,25 @@ __global__ __launch_bounds__(XS_BLOCK) void k_ts_update(int n, int k, const doub
   for (int q = 0; q < TS_ROWS; ++q) {
     const int row = base + q * XS_BLOCK;
     if (row >= n) continue;
-    double a = w[row];
-    for (int j = 0; j < k; ++j) a -= hs[j] * V[(size_t)j * ld + row];
-    wout[row] = a;
+    double a = (double)w[row];
+    for (int j = 0; j < k; ++j) a -= hs[j] * (double)V[(size_t)j * ld + row];
+    wout[row] = (T)a;
     if (U) {
-      double b = u[row];
-      for (int j = 0; j < k; ++j) b -= hs[j] * U[(size_t)j * ld + row];
-      u[row] = b;
+      double b = (double)u[row];
+      for (int j = 0; j < k; ++j) b -= hs[j] * (double)U[(size_t)j * ld + row];
+      u[row] = (T)b;
     }
   }
 }
 
 // dst = src * (*scale)
-__global__ void k_ts_scale(int n, const double *__restrict__ src, double *__restrict__ dst,
+template <class T>
+__global__ void k_ts_scale(int n, const T *__restrict__ src, T *__restrict__ dst,
                            const double *scale, const xs_state *st) {
   if (st && st->done) return;
   const double a = *scale;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    dst[i] = src[i] * a;
+    dst[i] = (T)((double)src[i] * a);
 }
 
 // ---------------------------------------------------------------------------
@@ -264,45 +268,58 @@ __global__ void k_res0_from(const double *nrm2, xs_state *st, double *resvec) {
 static inline int ts_grid(int n) { return (n + XS_BLOCK * TS_ROWS - 1) / (XS_BLOCK * TS_ROWS); }
 int xs_ts_blocks(int n) { return ts_grid(n); }
 
-void xs_ts_dot(hipStream_t s, int n, int k, const double *V, size_t ld, const double *w,
+template <class T>
+void xs_ts_dot(hipStream_t s, int n, int k, const T *V, size_t ld, const T *w,
                double *partial, unsigned *counter, double *out, const xs_state *st) {
   for (int j0 = 0; j0 < k; j0 += XS_TS_KMAX) {
     const int kk = k - j0 < XS_TS_KMAX ? k - j0 : XS_TS_KMAX;
-    hipLaunchKernelGGL(k_ts_dot, dim3(ts_grid(n)), dim3(XS_BLOCK), 0, s, n, kk, V + (size_t)j0 * ld,
-                       ld, w, partial, counter, out + j0, st);
+    hipLaunchKernelGGL(k_ts_dot<T>, dim3(ts_grid(n)), dim3(XS_BLOCK), 0, s, n, kk,
+                       V + (size_t)j0 * ld, ld, w, partial, counter, out + j0, st);
   }
 }
 
-void xs_ts_update(hipStream_t s, int n, int k, const double *V, const double *U, size_t ld,
-                  const double *h, double *w, double *wout, double *u, const xs_state *st) {
-  const double *src = w;
+template <class T>
+void xs_ts_update(hipStream_t s, int n, int k, const T *V, const T *U, size_t ld,
+                  const double *h, T *w, T *wout, T *u, const xs_state *st) {
+  const T *src = w;
   for (int j0 = 0; j0 < k; j0 += XS_TS_KMAX) {
     const int kk = k - j0 < XS_TS_KMAX ? k - j0 : XS_TS_KMAX;
-    double *dst = (j0 + kk >= k) ? wout : w;
-    hipLaunchKernelGGL(k_ts_update, dim3(ts_grid(n)), dim3(XS_BLOCK), 0, s, n, kk,
-                       V + (size_t)j0 * ld, U ? U + (size_t)j0 * ld : nullptr, ld, h + j0,
-                       const_cast<double *>(src), dst, u, 1.0, st);
+    T *dst = (j0 + kk >= k) ? wout : w;
+    hipLaunchKernelGGL(k_ts_update<T>, dim3(ts_grid(n)), dim3(XS_BLOCK), 0, s, n, kk,
+                       V + (size_t)j0 * ld, U ? U + (size_t)j0 * ld : (const T *)nullptr, ld,
+                       h + j0, const_cast<T *>(src), dst, u, 1.0, st);
     src = dst;
   }
 }
 
-void xs_ts_axpy(hipStream_t s, int n, int k, const double *V, size_t ld, const double *y,
-                double *x) {
+template <class T>
+void xs_ts_axpy(hipStream_t s, int n, int k, const T *V, size_t ld, const double *y, T *x) {
   // x += sum_j y_j V_j  ==  x -= sum_j (-y_j) V_j
   for (int j0 = 0; j0 < k; j0 += XS_TS_KMAX) {
     const int kk = k - j0 < XS_TS_KMAX ? k - j0 : XS_TS_KMAX;
-    hipLaunchKernelGGL(k_ts_update, dim3(ts_grid(n)), dim3(XS_BLOCK), 0, s, n, kk,
-                       V + (size_t)j0 * ld, (const double *)nullptr, ld, y + j0, x, x,
-                       (double *)nullptr, -1.0, (const xs_state *)nullptr);
+    hipLaunchKernelGGL(k_ts_update<T>, dim3(ts_grid(n)), dim3(XS_BLOCK), 0, s, n, kk,
+                       V + (size_t)j0 * ld, (const T *)nullptr, ld, y + j0, x, x, (T *)nullptr,
+                       -1.0, (const xs_state *)nullptr);
   }
 }
 
-void xs_ts_scale(hipStream_t s, int n, const double *src, double *dst, const double *scale,
+template <class T>
+void xs_ts_scale(hipStream_t s, int n, const T *src, T *dst, const double *scale,
                  const xs_state *st) {
   int g = (n + XS_BLOCK * 4 - 1) / (XS_BLOCK * 4);
   if (g > 2048) g = 2048;
-  hipLaunchKernelGGL(k_ts_scale, dim3(g), dim3(XS_BLOCK), 0, s, n, src, dst, scale, st);
+  hipLaunchKernelGGL(k_ts_scale<T>, dim3(g), dim3(XS_BLOCK), 0, s, n, src, dst, scale, st);
 }
+
+#define XS_TS_INST(T)                                                                       \
+  template void xs_ts_dot<T>(hipStream_t, int, int, const T *, size_t, const T *, double *, \
+                             unsigned *, double *, const xs_state *);                      \
+  template void xs_ts_update<T>(hipStream_t, int, int, const T *, const T *, size_t,        \
+                                const double *, T *, T *, T *, const xs_state *);          \
+  template void xs_ts_axpy<T>(hipStream_t, int, int, const T *, size_t, const double *, T *); \
+  template void xs_ts_scale<T>(hipStream_t, int, const T *, T *, const double *, const xs_state *);
+XS_TS_INST(double)
+XS_TS_INST(float)
 
 void xs_gmres_column(hipStream_t s, int i, int m, const xs_gmres_small &g, xs_state *st,
                      double *resvec) {

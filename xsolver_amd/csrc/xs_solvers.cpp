@@ -175,9 +175,9 @@ int xs_solve_gmres(xs_ctx *c, const double *b, double *x, bool flexible) {
     if (!flexible) xs_op_pc(c, t, r, tmp, nullptr);
   };
   residual();
-  xs_ts_dot(s, n, 1, r, ld, r, P, tc, g.misc + XS_G_NRM2, nullptr);
+  xs_ts_dot<double>(s, n, 1, r, ld, r, P, tc, g.misc + XS_G_NRM2, nullptr);
   xs_gmres_cycle(s, m, 1, g, st, rv);
-  xs_ts_scale(s, n, r, V, g.misc + XS_G_SCALE, st);
+  xs_ts_scale<double>(s, n, r, V, g.misc + XS_G_SCALE, st);
 
   int launched = 0, i = 0, cycle_start = 0;
   XS_CHK(hipMemcpyAsync(c->h_state, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
@@ -193,13 +193,13 @@ int xs_solve_gmres(xs_ctx *c, const double *b, double *x, bool flexible) {
       xs_op_spmv(c, vi, t, EPI_Y, nullptr, nullptr, nullptr);
       xs_op_pc(c, t, w, tmp, st);
     }
-    xs_ts_dot(s, n, i + 1, V, ld, w, P, tc, g.h1, st);
-    xs_ts_update(s, n, i + 1, V, nullptr, ld, g.h1, w, w, nullptr, st);
-    xs_ts_dot(s, n, i + 1, V, ld, w, P, tc, g.h2, st);
-    xs_ts_update(s, n, i + 1, V, nullptr, ld, g.h2, w, vn, nullptr, st);
-    xs_ts_dot(s, n, 1, vn, ld, vn, P, tc, g.misc + XS_G_NRM2, st);
+    xs_ts_dot<double>(s, n, i + 1, V, ld, w, P, tc, g.h1, st);
+    xs_ts_update<double>(s, n, i + 1, V, nullptr, ld, g.h1, w, w, nullptr, st);
+    xs_ts_dot<double>(s, n, i + 1, V, ld, w, P, tc, g.h2, st);
+    xs_ts_update<double>(s, n, i + 1, V, nullptr, ld, g.h2, w, vn, nullptr, st);
+    xs_ts_dot<double>(s, n, 1, vn, ld, vn, P, tc, g.misc + XS_G_NRM2, st);
     xs_gmres_column(s, i, m, g, st, rv);
-    xs_ts_scale(s, n, vn, vn, g.misc + XS_G_SCALE, st);
+    xs_ts_scale<double>(s, n, vn, vn, g.misc + XS_G_SCALE, st);
     i++;
     launched++;
     if (i == m || launched == c->maxiter || launched % c->check_every == 0) {
@@ -211,10 +211,10 @@ int xs_solve_gmres(xs_ctx *c, const double *b, double *x, bool flexible) {
     }
     if (i == m && launched < c->maxiter) {      // restart (intel :1065-1070)
       xs_gmres_backsub(s, m, m, g);
-      xs_ts_axpy(s, n, m, flexible ? Z : V, ld, g.y, x);
+      xs_ts_axpy<double>(s, n, m, flexible ? Z : V, ld, g.y, x);
       residual();
       xs_gmres_cycle(s, m, 0, g, st, rv);
-      xs_ts_scale(s, n, r, V, g.misc + XS_G_SCALE, st);
+      xs_ts_scale<double>(s, n, r, V, g.misc + XS_G_SCALE, st);
       i = 0;
       cycle_start = launched;
     }
@@ -225,7 +225,7 @@ int xs_solve_gmres(xs_ctx *c, const double *b, double *x, bool flexible) {
   else k = i;                                  // maxiter: use what this cycle built
   if (h.done != 2 && k > 0) {
     xs_gmres_backsub(s, k, m, g);
-    xs_ts_axpy(s, n, k, flexible ? Z : V, ld, g.y, x);
+    xs_ts_axpy<double>(s, n, k, flexible ? Z : V, ld, g.y, x);
   }
   XS_CHK(hipGetLastError());
   XS_CHK(hipStreamSynchronize(s));
@@ -261,7 +261,7 @@ int xs_solve_gcr(xs_ctx *c, const double *b, double *x) {
 
   xs_init_state(s, st, c->tol, c->abstol, c->maxiter);
   xs_op_spmv(c, x, r, EPI_BMAX, b, nullptr, nullptr);
-  xs_ts_dot(s, n, 1, r, ld, r, P, tc, misc + XS_G_NRM2, nullptr);
+  xs_ts_dot<double>(s, n, 1, r, ld, r, P, tc, misc + XS_G_NRM2, nullptr);
   xs_res0_from(s, misc + XS_G_NRM2, st, rv);
   int k = 0, launched = 0;
   XS_CHK(hipMemcpyAsync(c->h_state, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
@@ -273,13 +273,13 @@ int xs_solve_gcr(xs_ctx *c, const double *b, double *x) {
     xs_op_pc(c, r, uk, tmp, st);                                   // u_k = M^-1 r
     xs_op_spmv(c, uk, ck, EPI_Y, nullptr, nullptr, nullptr);       // c_k = A u_k
     if (k > 0) {
-      xs_ts_dot(s, n, k, C, ld, ck, P, tc, h1, st);
-      xs_ts_update(s, n, k, C, U, ld, h1, ck, ck, uk, st);
-      xs_ts_dot(s, n, k, C, ld, ck, P, tc, h2, st);
-      xs_ts_update(s, n, k, C, U, ld, h2, ck, ck, uk, st);
+      xs_ts_dot<double>(s, n, k, C, ld, ck, P, tc, h1, st);
+      xs_ts_update<double>(s, n, k, C, U, ld, h1, ck, ck, uk, st);
+      xs_ts_dot<double>(s, n, k, C, ld, ck, P, tc, h2, st);
+      xs_ts_update<double>(s, n, k, C, U, ld, h2, ck, ck, uk, st);
     }
-    xs_ts_dot(s, n, 1, ck, ld, ck, P, tc, t01, st);                // c.c
-    xs_ts_dot(s, n, 1, ck, ld, r, P, tc, t01 + 1, st);             // c.r
+    xs_ts_dot<double>(s, n, 1, ck, ld, ck, P, tc, t01, st);                // c.c
+    xs_ts_dot<double>(s, n, 1, ck, ld, r, P, tc, t01 + 1, st);             // c.r
     xs_gcr_apply(s, n, uk, ck, x, r, t01, P, tc, st, rv);          // x, r, ||r||
     k = (k + 1 == R) ? 0 : k + 1;
     launched++;
@@ -333,6 +333,8 @@ static int ensure_buf(T **p, size_t need) {
 void xs_mixed_free(xs_ctx *c) {
   auto f = [](auto *&p) { if (p) (void)hipFree((void *)p); p = nullptr; };
   f(c->A32); f(c->ell32); f(c->dinv32); f(c->jac32); f(c->F32); f(c->work32); f(c->st_in);
+  f(c->basis32);
+  c->basis32_len = 0;
   c->work32_n = 0;
   c->mixed_ready = false;
 }
@@ -440,8 +442,87 @@ static int inner_bicgstab32(xs_ctx *c, const float *rhs, float *d, int maxit, in
   return h.done == 2 ? XS_ERR_BREAKDOWN : XS_OK;
 }
 
+// FP32 flexible GMRES(m) on A d = rhs from d = 0 (inner solve of the
+// mixed-precision refinement): FP32 basis, SpMV and triangular solves; FP64
+// inner products, Hessenberg, Givens rotations and residual estimate.
+static int inner_fgmres32(xs_ctx *c, const float *rhs, float *d, int maxit, int *iters) {
+  hipStream_t s = c->stream;
+  const int n = c->A.n;
+  const int m = c->restart > 0 ? c->restart : 30;
+  const size_t ld = (size_t)n;
+  const size_t need = (size_t)(2 * m + 1) * ld;
+  if (c->basis32 && c->basis32_len < need) { (void)hipFree(c->basis32); c->basis32 = nullptr; }
+  if (!c->basis32) {
+    if (hipMalloc((void **)&c->basis32, sizeof(float) * need) != hipSuccess) {
+      (void)hipGetLastError();
+      return XS_ERR_MEMORY;
+    }
+    c->basis32_len = need;
+  }
+  int rc;
+  if ((rc = ensure_dev(&c->small_d, &c->small_len, small_size(m)))) return rc;
+  if ((rc = ensure_ts(c))) return rc;
+  float *V = c->basis32, *Z = c->basis32 + (size_t)(m + 1) * ld;
+  xs_gmres_small g = carve_small(c->small_d, m);
+  float *W = c->work32;
+  float *r = W, *tmp = W + (size_t)n, *w = W + 2 * (size_t)n;
+  unsigned *tc = c->counters + 10;
+  double *P = c->ts_partial;
+  xs_state *st = c->st_in;
+  xs_init_state(s, st, c->inner_tol, 0.0, maxit);
+  xs_zero32(s, n, d);
+  xs_copy32(s, n, rhs, r);
+  xs_ts_dot<float>(s, n, 1, r, ld, r, P, tc, g.misc + XS_G_NRM2, nullptr);
+  xs_gmres_cycle(s, m, 1, g, st, nullptr);
+  xs_ts_scale<float>(s, n, r, V, g.misc + XS_G_SCALE, st);
+  int launched = 0, i = 0, cycle_start = 0;
+  xs_state h{};
+  XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
+  XS_CHK(hipStreamSynchronize(s));
+  bool done = h.done != 0;
+  while (!done && launched < maxit) {
+    float *vi = V + (size_t)i * ld, *vn = V + (size_t)(i + 1) * ld, *zi = Z + (size_t)i * ld;
+    op_pc32(c, vi, zi, tmp, st);
+    op_spmv32(c, zi, w, EPI_Y, nullptr, nullptr, nullptr);
+    xs_ts_dot<float>(s, n, i + 1, V, ld, w, P, tc, g.h1, st);
+    xs_ts_update<float>(s, n, i + 1, V, nullptr, ld, g.h1, w, w, nullptr, st);
+    xs_ts_dot<float>(s, n, i + 1, V, ld, w, P, tc, g.h2, st);
+    xs_ts_update<float>(s, n, i + 1, V, nullptr, ld, g.h2, w, vn, nullptr, st);
+    xs_ts_dot<float>(s, n, 1, vn, ld, vn, P, tc, g.misc + XS_G_NRM2, st);
+    xs_gmres_column(s, i, m, g, st, nullptr);
+    xs_ts_scale<float>(s, n, vn, vn, g.misc + XS_G_SCALE, st);
+    i++;
+    launched++;
+    if (i == m || launched == maxit || launched % c->check_every == 0) {
+      XS_CHK(hipGetLastError());
+      XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
+      XS_CHK(hipStreamSynchronize(s));
+      done = h.done != 0;
+      if (done) break;
+    }
+    if (i == m && launched < maxit) {
+      xs_gmres_backsub(s, m, m, g);
+      xs_ts_axpy<float>(s, n, m, Z, ld, g.y, d);
+      op_spmv32(c, d, r, EPI_BMAX, rhs, nullptr, nullptr);
+      xs_gmres_cycle(s, m, 0, g, st, nullptr);
+      xs_ts_scale<float>(s, n, r, V, g.misc + XS_G_SCALE, st);
+      i = 0;
+      cycle_start = launched;
+    }
+  }
+  const int k = (h.done == 1) ? h.iter - cycle_start : i;
+  if (h.done != 2 && k > 0) {
+    xs_gmres_backsub(s, k, m, g);
+    xs_ts_axpy<float>(s, n, k, Z, ld, g.y, d);
+  }
+  XS_CHK(hipGetLastError());
+  *iters = h.iter;
+  return h.done == 2 ? XS_ERR_BREAKDOWN : XS_OK;
+}
+
 int xs_solve_mixed(xs_ctx *c, const double *b, double *x) {
-  if (c->ksp != XS_KSP_BICGSTAB) return XS_ERR_UNSUPPORTED;
+  if (c->ksp != XS_KSP_BICGSTAB && c->ksp != XS_KSP_GMRES && c->ksp != XS_KSP_FGMRES)
+    return XS_ERR_UNSUPPORTED;
   int rc = xs_ctx_ensure_work(c, XS_NWORK);
   if (rc) return rc;
   if ((rc = mixed_prepare(c))) return rc;
@@ -469,7 +550,9 @@ int xs_solve_mixed(xs_ctx *c, const double *b, double *x) {
     if (total >= c->maxiter) break;
     xs_d2f_scaled(s, n, r64, rhs32, st, S_SCRATCH0 + 1);
     int it = 0;
-    int irc = inner_bicgstab32(c, rhs32, d32, c->maxiter - total, &it);
+    int irc = (c->ksp == XS_KSP_BICGSTAB)
+                  ? inner_bicgstab32(c, rhs32, d32, c->maxiter - total, &it)
+                  : inner_fgmres32(c, rhs32, d32, c->maxiter - total, &it);
     total += it;
     xs_add_f2d(s, n, d32, x, st, S_SCRATCH0 + 1);
     outer++;

@@ -90,6 +90,8 @@ struct xs_ctx {
   float *A32 = nullptr, *ell32 = nullptr, *dinv32 = nullptr, *jac32 = nullptr;
   void *F32 = nullptr;
   float *work32 = nullptr;
+  float *basis32 = nullptr;
+  size_t basis32_len = 0;
   size_t work32_n = 0;
   xs_state *st_in = nullptr;
   int outer_iters = 0;
