@@ -160,6 +160,34 @@ int xs_grad_values_device(xs_ctx *ctx, const double *d_lambda, const double *d_x
 int xs_pc_get_factors(xs_ctx *ctx, int *rowptr, int *colidx, double *val, int *diagpos, int cap);
 
 /* ======================================================================== */
+/* Block of right-hand sides: A X = B for k <= 64 columns at once             */
+/* (no reference counterpart; the reference solves one right-hand side).      */
+/* ======================================================================== */
+enum xs_rhs_layout {
+  XS_RHS_COLMAJOR = 0,     /* column j at rhs + j*n (k vectors back to back) */
+  XS_RHS_INTERLEAVED = 1   /* row-major n x k: entry (i, j) at rhs[i*k + j] */
+};
+/* The context's Krylov method (BiCGSTAB or CG, FP64) and preconditioner run
+ * on every column with its own scalars and convergence test, sharing one
+ * SpMM and one k-column triangular solve per operator application.  sol =
+ * X0 in, X out.  Returns XS_OK when every column converged, XS_NOT_CONVERGED
+ * if some column reached maxiter, XS_ERR_BREAKDOWN if one broke down.
+ * GMRES/GCR or precision 1 -> XS_ERR_UNSUPPORTED. */
+int xs_solve_multi_device(xs_ctx *ctx, int nrhs, const double *d_rhs, double *d_sol,
+                          int layout);
+int xs_solve_multi(xs_ctx *ctx, int nrhs, const double *rhs, double *sol, int layout);
+/* Per-column iterations, initial and final residual norms of the last block
+ * solve (arrays of cap entries; any may be NULL).  Returns the column count. */
+int xs_get_multi_results(const xs_ctx *ctx, int *iters, double *res0, double *resl, int cap);
+/* Residual history of column col (absolute norms, entry 0 = ||r0||). */
+int xs_get_multi_resvec(const xs_ctx *ctx, int col, double *out, int len);
+/* Kernel level, k <= 64 interleaved columns (n x k row-major device blocks):
+ * Y = A X (multi-RHS SpMM, the matrix read once for all columns) and
+ * Y = M^{-1} X (k-column sync-free triangular solves for ILU). */
+int xs_spmm(xs_ctx *ctx, int nrhs, const double *d_x, double *d_y, void *stream);
+int xs_pc_apply_multi(xs_ctx *ctx, int nrhs, const double *d_x, double *d_y, void *stream);
+
+/* ======================================================================== */
 /* Batched solve: many independent systems in one persistent launch          */
 /* (no reference counterpart; the reference solves one system per call).     */
 /* ======================================================================== */

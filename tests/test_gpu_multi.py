@@ -1,0 +1,244 @@
+"""Block of right-hand sides through the C-ABI (xs_spmm, xs_pc_apply_multi,
+xs_solve_multi[_device]) against the oracle column by column: SpMM and the
+k-column triangular solves to rounding, and the block BiCGSTAB/CG solve per
+column to the single right-hand side bar (iterations within +-2 of the
+oracle, FP64 relative residual below rtol, solution close to the oracle's)."""
+import numpy as np
+import pytest
+
+import oracle
+import xsolver_amd
+from tests.helpers import random_sparse, rel_err, say, split_factors, true_relres
+from xsolver_amd import XSolver, gen
+
+pytestmark = pytest.mark.gpu
+
+
+def torch_stream():
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def dev_tensor(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+SPMM_MATS = {
+    "convdiff3d_20": lambda: gen.convdiff3d(20),
+    "lognormal27_10": lambda: gen.lognormal27(10, seed=2),
+    "longrows_1500": lambda: random_sparse(1500, 4, 22, long_rows=[(3, 1400), (900, 700)]),
+}
+
+
+@pytest.mark.parametrize("k", [1, 5, 32, 64])
+@pytest.mark.parametrize("name", list(SPMM_MATS))
+def test_spmm_matches_oracle(dev, name, k):
+    import torch
+    A = SPMM_MATS[name]()
+    n = len(A[0]) - 1
+    say(f"[spmm] {name} n={n} k={k}")
+    S = XSolver()
+    S.mat_setup(*A)
+    X = np.random.default_rng(k).uniform(-1, 1, (n, k))
+    Yd = torch.empty((n, k), dtype=torch.float64, device=dev)
+    S.spmm(dev_tensor(X, dev), Yd, torch_stream())
+    torch.cuda.synchronize()
+    Y = Yd.cpu().numpy()
+    scale = np.abs(A[2]).max() * np.diff(A[0]).max()
+    for j in range(k):
+        assert np.abs(Y[:, j] - oracle.spmv(A, X[:, j])).max() <= 1e-14 * scale
+    S.close()
+
+
+TRSV_MATS = {
+    "poisson2d_128": lambda: gen.poisson2d(128),     # only r-1 inside a wave: scan path
+    "convdiff3d_12": lambda: gen.convdiff3d(12),     # r-1, r-12 inside a wave: loop path
+    "lognormal27_10": lambda: gen.lognormal27(10, seed=2),
+    "random_3000": lambda: random_sparse(3000, 7, 23),
+}
+
+
+@pytest.mark.parametrize("k", [1, 3, 16, 21, 32])
+@pytest.mark.parametrize("name", list(TRSV_MATS))
+def test_pc_apply_multi_matches_oracle(dev, name, k):
+    import torch
+    A = TRSV_MATS[name]()
+    n = len(A[0]) - 1
+    say(f"[mtrsv] {name} n={n} k={k}")
+    S = XSolver()
+    S.mat_setup(*A)
+    S.ksp_settype("bicgstab")
+    S.pc_settype("ilu0")
+    S.pc_setup()
+    rp, ci, lu, dpos = S.factors(n, len(A[2]))
+    L, U, dinv = split_factors(rp, ci, lu, dpos)
+    B = np.random.default_rng(5 + k).uniform(-1, 1, (n, k))
+    Bd = dev_tensor(B, dev)
+    Yd = torch.empty((n, k), dtype=torch.float64, device=dev)
+    for rep in range(2):  # the tickets reset themselves between calls
+        S.pc_apply_multi(Bd, Yd, torch_stream())
+        torch.cuda.synchronize()
+        Y = Yd.cpu().numpy()
+        for j in range(k):
+            ref = oracle.upper_solve(U, dinv, oracle.lower_solve(L, B[:, j]))
+            assert rel_err(Y[:, j], ref) < 1e-12, (rep, j)
+    # one column against the single right-hand side kernel
+    yd = torch.empty(n, dtype=torch.float64, device=dev)
+    S.pc_apply(dev_tensor(B[:, 0], dev), yd, torch_stream())
+    torch.cuda.synchronize()
+    assert rel_err(Y[:, 0], yd.cpu().numpy()) < 1e-13
+    S.close()
+
+
+def test_pc_apply_multi_parilut_factors(dev):
+    import torch
+    A = gen.lognormal27(10, seed=2)
+    n = len(A[0]) - 1
+    S = XSolver()
+    S.mat_setup(*A)
+    S.ksp_settype("gmres")
+    S.pc_settype("ilut")
+    S.pc_setilut(1e-4, 2)
+    S.pc_setup()
+    rp, ci, lu, dpos = S.factors(n, 4 * len(A[2]))
+    L, U, dinv = split_factors(rp, ci, lu, dpos)
+    B = np.random.default_rng(11).uniform(-1, 1, (n, 8))
+    Yd = torch.empty((n, 8), dtype=torch.float64, device=dev)
+    S.pc_apply_multi(dev_tensor(B, dev), Yd, torch_stream())
+    torch.cuda.synchronize()
+    Y = Yd.cpu().numpy()
+    for j in range(8):
+        ref = oracle.upper_solve(U, dinv, oracle.lower_solve(L, B[:, j]))
+        assert rel_err(Y[:, j], ref) < 1e-12
+    S.close()
+
+
+BLOCK_CASES = [
+    # name, builder, ksp, pc, tol, maxiter, k
+    ("convdiff3d_24_bicgstab_ilu0", lambda: gen.convdiff3d(24), "bicgstab", "ilu0", 1e-10, 1000, 8),
+    ("convdiff3d_24_bicgstab_jacobi", lambda: gen.convdiff3d(24), "bicgstab", "jacobi", 1e-10, 2000, 5),
+    ("poisson2d_96_cg_jacobi", lambda: gen.poisson2d(96), "cg", "jacobi", 1e-8, 5000, 4),
+    ("grf2d_48_cg_ilu0", lambda: gen.vardiff2d(gen.grf_kappa(48, 0.1, 4)), "cg", "ilu0", 1e-10, 2000, 3),
+    ("poisson2d_48_cg_none", lambda: gen.poisson2d(48), "cg", "none", 1e-9, 2000, 2),
+]
+
+
+def rhs_block(A, k, seed0):
+    xs = []
+    bs = []
+    for j in range(k):
+        x, b = gen.manufactured(A, seed0 + j)
+        xs.append(x)
+        bs.append(b)
+    return np.stack(xs, 1), np.stack(bs, 1)
+
+
+@pytest.mark.parametrize("case", BLOCK_CASES, ids=[c[0] for c in BLOCK_CASES])
+def test_block_solve_matches_oracle_per_column(dev, case):
+    name, build, ksp, pc, tol, maxiter, k = case
+    A = build()
+    Xs, B = rhs_block(A, k, 1)
+    say(f"[block] {name} n={len(Xs)} k={k}")
+    S = XSolver()
+    S.mat_setup(*A)
+    S.ksp_settype(ksp)
+    S.ksp_setoption(tol, maxiter)
+    S.pc_settype(pc)
+    if pc == "ilu0":
+        S.set_option("parilu_async", 0)
+        S.set_option("parilu_sweeps", 60)
+    X, st = S.solve_multi(B)
+    its, r0, rl = S.multi_results()
+    assert st == xsolver_amd.XS_OK and len(its) == k
+    for j in range(k):
+        xo, sto, ito, reso = oracle.solve(A, B[:, j], ksp, pc, tol=tol, maxiter=maxiter)
+        rr = true_relres(A, B[:, j], X[:, j])
+        say(f"   col {j}: oracle it={ito} gpu it={its[j]} relres={rr:.2e} "
+            f"x-diff={rel_err(X[:, j], xo):.2e}")
+        assert sto == 0
+        assert abs(int(its[j]) - ito) <= 2
+        assert rr < tol * 1.05
+        assert np.linalg.norm(X[:, j] - xo) / np.linalg.norm(xo) < 50 * tol
+        assert r0[j] == pytest.approx(np.linalg.norm(B[:, j]), rel=1e-12)
+        res = S.multi_resvec(j)
+        assert len(res) == its[j] + 1 and res[-1] == pytest.approx(rl[j])
+        n = min(len(res), len(reso), 20)
+        assert np.allclose(res[:n], reso[:n], rtol=1e-6)
+    S.close()
+
+
+def test_block_device_layouts_and_single_column(dev):
+    """Column-major and interleaved device blocks give the same answer, and a
+    1-column block matches the single right-hand side solve's iterations."""
+    import torch
+    A = gen.convdiff3d(20)
+    Xs, B = rhs_block(A, 6, 3)
+    n = B.shape[0]
+    S = XSolver()
+    S.mat_setup(*A)
+    S.ksp_settype("bicgstab")
+    S.ksp_setoption(1e-10, 1000)
+    S.pc_settype("ilu0")
+    Bi = dev_tensor(B, dev)
+    Xi = torch.zeros_like(Bi)
+    torch.cuda.synchronize()
+    assert S.solve_multi_device(Bi, Xi, xsolver_amd.XS_RHS_INTERLEAVED) == 0
+    it_i = S.multi_results()[0].copy()
+    Bc = dev_tensor(B.T, dev)
+    Xc = torch.zeros_like(Bc)
+    torch.cuda.synchronize()
+    assert S.solve_multi_device(Bc, Xc, xsolver_amd.XS_RHS_COLMAJOR) == 0
+    it_c = S.multi_results()[0].copy()
+    assert np.array_equal(it_i, it_c)
+    assert np.array_equal(Xi.cpu().numpy(), Xc.cpu().numpy().T)   # same kernels, same order
+    x1, st = S.solve(B[:, 2])
+    it1 = S.iterations
+    X1, st1 = S.solve_multi(B[:, 2:3])
+    assert st == 0 and st1 == 0
+    assert abs(int(S.multi_results()[0][0]) - it1) <= 1
+    assert rel_err(X1[:, 0], x1) < 1e-8
+    S.close()
+
+
+def test_block_per_column_masks(dev):
+    """A zero column and an already-solved column finish at once; the others
+    iterate to their own counts."""
+    A = gen.convdiff3d(16)
+    Xs, B = rhs_block(A, 4, 5)
+    B[:, 1] = 0.0
+    S = XSolver()
+    S.mat_setup(*A)
+    S.ksp_settype("bicgstab")
+    S.ksp_setoption(1e-10, 1000)
+    S.pc_settype("jacobi")
+    X0 = np.zeros_like(B)
+    X0[:, 3] = Xs[:, 3]                       # exact start: r0 = 0 to rounding
+    S.ksp_set_absolute_tol(1e-12 * np.linalg.norm(B[:, 3]))
+    X, st = S.solve_multi(B, x0=X0)
+    its, r0, rl = S.multi_results()
+    say(f"[masks] its={its}")
+    assert st == 0
+    assert its[1] == 0 and its[3] == 0 and its[0] > 0 and its[2] > 0
+    assert np.array_equal(X[:, 1], np.zeros(B.shape[0]))
+    for j in (0, 2):
+        assert true_relres(A, B[:, j], X[:, j]) < 1.05e-10
+    S.close()
+
+
+def test_block_errors(dev):
+    A = gen.poisson2d(16)
+    S = XSolver()
+    S.mat_setup(*A)
+    S.ksp_settype("gmres")
+    S.ksp_setoption(1e-8, 100)
+    S.pc_settype("jacobi")
+    B = np.ones((256, 3))
+    with pytest.raises(xsolver_amd.XSolverError) as e:
+        S.solve_multi(B)
+    assert e.value.status == xsolver_amd.XS_ERR_UNSUPPORTED
+    S.ksp_settype("cg")
+    with pytest.raises(xsolver_amd.XSolverError) as e:
+        S.solve_multi(np.ones((256, 65)))
+    assert e.value.status == xsolver_amd.XS_ERR_ARG
+    S.close()

@@ -43,8 +43,13 @@ EXPORTS = [
     "xs_pc_apply_t", "xs_trsv", "xs_pc_get_factors",
     "xs_batch_create", "xs_batch_solve", "xs_batch_solve_device", "xs_batch_get_results",
     "xs_batch_get_time_ms", "xs_batch_destroy", "xs_solve_transpose_device",
-    "xs_grad_values_device",
+    "xs_grad_values_device", "xs_solve_multi_device", "xs_solve_multi", "xs_get_multi_results",
+    "xs_get_multi_resvec", "xs_spmm", "xs_pc_apply_multi",
 ]
+
+XS_RHS_COLMAJOR = 0
+XS_RHS_INTERLEAVED = 1
+MAX_RHS = 64
 
 
 class XSolverError(RuntimeError):
@@ -103,6 +108,12 @@ def lib():
         "xs_batch_destroy": (_i, [_vp]),
         "xs_solve_transpose_device": (_i, [_vp, _vp, _vp]),
         "xs_grad_values_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
+        "xs_solve_multi_device": (_i, [_vp, _i, _vp, _vp, _i]),
+        "xs_solve_multi": (_i, [_vp, _i, _vp, _vp, _i]),
+        "xs_get_multi_results": (_i, [_vp, _vp, _vp, _vp, _i]),
+        "xs_get_multi_resvec": (_i, [_vp, _i, _vp, _i]),
+        "xs_spmm": (_i, [_vp, _i, _vp, _vp, _vp]),
+        "xs_pc_apply_multi": (_i, [_vp, _i, _vp, _vp, _vp]),
         "xsolver_mat_setup": (None, [_i, _vp, _vp, _vp, _vp]),
         "xsolver_ksp_settype": (None, [_cp]),
         "xsolver_ksp_set_absolute_tol": (None, [_d]),
@@ -204,6 +215,48 @@ class XSolver:
 
     def solve_device(self, d_rhs, d_sol):
         return _check(lib().xs_solve_device(self.h, _ptr(d_rhs), _ptr(d_sol)), "xs_solve_device")
+
+    # -- k right-hand sides -------------------------------------------------
+    def solve_multi(self, rhs, x0=None):
+        """Host block solve: rhs is (n, k); returns (X (n, k), status)."""
+        b = np.ascontiguousarray(rhs, np.float64)
+        x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64, copy=True, order="C")
+        st = _check(lib().xs_solve_multi(self.h, b.shape[1], _ptr(b), _ptr(x),
+                                         XS_RHS_INTERLEAVED), "xs_solve_multi")
+        return x, st
+
+    def solve_multi_device(self, d_rhs, d_sol, layout=XS_RHS_INTERLEAVED):
+        """Device block solve.  Interleaved: (n, k) row-major tensors;
+        column-major: (k, n) row-major tensors (k vectors back to back)."""
+        k = d_rhs.shape[1] if layout == XS_RHS_INTERLEAVED else d_rhs.shape[0]
+        return _check(lib().xs_solve_multi_device(self.h, k, _ptr(d_rhs), _ptr(d_sol), layout),
+                      "xs_solve_multi_device")
+
+    def spmm(self, X, Y, stream=None):
+        """Y = A X for (n, k) row-major device tensors."""
+        return _check(lib().xs_spmm(self.h, X.shape[1], _ptr(X), _ptr(Y), stream), "xs_spmm")
+
+    def pc_apply_multi(self, X, Y, stream=None):
+        """Y = M^{-1} X for (n, k) row-major device tensors."""
+        return _check(lib().xs_pc_apply_multi(self.h, X.shape[1], _ptr(X), _ptr(Y), stream),
+                      "xs_pc_apply_multi")
+
+    def multi_results(self):
+        """(iterations, res0, resl) per column of the last block solve."""
+        cap = MAX_RHS
+        it = np.zeros(cap, np.int32)
+        r0 = np.zeros(cap)
+        rl = np.zeros(cap)
+        k = _check(lib().xs_get_multi_results(self.h, _ptr(it), _ptr(r0), _ptr(rl), cap),
+                   "xs_get_multi_results")
+        return it[:k], r0[:k], rl[:k]
+
+    def multi_resvec(self, col):
+        it, _, _ = self.multi_results()
+        out = np.zeros(int(it[col]) + 1)
+        m = _check(lib().xs_get_multi_resvec(self.h, col, _ptr(out), len(out)),
+                   "xs_get_multi_resvec")
+        return out[:m]
 
     # -- queries ------------------------------------------------------------
     @property

@@ -149,3 +149,36 @@ __device__ __forceinline__ void reduce_finish(double (&v)[NV], const xs_reduce &
   }
 }
 
+// ---------------------------------------------------------------------------
+// Ready-flag loads/stores of the sync-free triangular solves: a value slot
+// holds XS_SENTINEL until its row is published with an agent-scope store.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long ld_relaxed(const double *p) {
+  return __hip_atomic_load(reinterpret_cast<unsigned long long *>(const_cast<double *>(p)),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Spin until the row value is published.  The wait is bounded (about a
+// second of polling, far beyond any legitimate wait): on expiry the flag
+// *err is raised and a quiet NaN is used, so every wave always exits and the
+// solve reports the failure instead of hanging.
+#define XS_SPIN_LIMIT (1u << 22)
+__device__ __forceinline__ double ld_ready(const double *p, unsigned *err) {
+  unsigned long long u = ld_relaxed(p);
+  unsigned polls = 0;
+  while (u == XS_SENTINEL) {
+    if (++polls > XS_SPIN_LIMIT) {
+      atomicOr(err, 1u);
+      u = 0x7FF8000000000000ull;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    u = ld_relaxed(p);
+  }
+  return __longlong_as_double((long long)u);
+}
+__device__ __forceinline__ void st_ready(double *p, double v) {
+  unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  if (u == XS_SENTINEL) u = 0x7FF8000000000000ull;
+  __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), u, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}

@@ -18,6 +18,7 @@
 
 #include "../../include/xsolver_c.h"
 #include "xs_internal.h"
+#include "xs_multi.h"
 #include "xs_parilut.h"
 #include "xs_solvers.h"
 
@@ -764,13 +765,45 @@ void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state
 // ---------------------------------------------------------------------------
 // solve
 // ---------------------------------------------------------------------------
-static int run_solve(xs_ctx *c, const double *d_b, double *d_x) {
+// Solve epilogue: stop the timer, collect kernel timings, and turn a refused
+// launch or an expired triangular-solve wait into an error.  Returns rc
+// (the solver's status) when nothing went wrong.
+static int finish_solve(xs_ctx *c, int rc) {
+  XS_HIP(hipEventRecord(c->ev1, c->stream));
+  XS_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->t_solve_ms = ms;
+  xs_prof_collect(c);
+  if (xs_take_launch_guard()) {
+    snprintf(xs_errbuf, sizeof(xs_errbuf), "internal: a kernel launch found a null buffer");
+    return XS_ERR_STATE;
+  }
+  // wait-expired flags of the lower/upper triangular solves
+  unsigned fl[4] = {0, 0, 0, 0};
+  XS_HIP(hipMemcpy(fl, c->counters + 6, sizeof(fl), hipMemcpyDeviceToHost));
+  if (fl[0] | fl[2]) {
+    XS_HIP(hipMemset(c->counters + 6, 0, sizeof(fl)));
+    snprintf(xs_errbuf, sizeof(xs_errbuf), "triangular solve dependency wait expired");
+    return XS_ERR_DEVICE;
+  }
+  return rc;
+}
+
+// common checks and lazy set-up before a solve
+static int prepare_solve(xs_ctx *c) {
   int rc;
   if (!c->mat_ready) return XS_ERR_STATE;
   if (c->ksp == XS_KSP_NONE) return XS_ERR_STATE;
   if (c->maxiter <= 0) return XS_ERR_STATE;
   if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
   if ((rc = ensure_common(c))) return rc;
+  return XS_OK;
+}
+
+static int run_solve(xs_ctx *c, const double *d_b, double *d_x) {
+  int rc;
+  if ((rc = prepare_solve(c))) return rc;
   if (c->resvec_cap < c->maxiter + 2) {
     dfree(c->resvec_d);
     if ((rc = dalloc(&c->resvec_d, (size_t)c->maxiter + 2))) return rc;
@@ -787,27 +820,9 @@ static int run_solve(xs_ctx *c, const double *d_b, double *d_x) {
     case XS_KSP_FGMRES: rc = xs_solve_gmres(c, d_b, d_x, true); break;
     default: return XS_ERR_STATE;
   }
-  XS_HIP(hipEventRecord(c->ev1, c->stream));
-  XS_HIP(hipEventSynchronize(c->ev1));
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-  c->t_solve_ms = ms;
-  xs_prof_collect(c);
-  if (xs_take_launch_guard()) {
-    snprintf(xs_errbuf, sizeof(xs_errbuf), "internal: a kernel launch found a null buffer");
-    return XS_ERR_STATE;
+  if ((rc = finish_solve(c, rc))) {
+    if (rc < 0 && rc != XS_ERR_BREAKDOWN) return rc;
   }
-  {
-    // wait-expired flags of the lower/upper triangular solves
-    unsigned fl[4] = {0, 0, 0, 0};
-    XS_HIP(hipMemcpy(fl, c->counters + 6, sizeof(fl), hipMemcpyDeviceToHost));
-    if (fl[0] | fl[2]) {
-      XS_HIP(hipMemset(c->counters + 6, 0, sizeof(fl)));
-      snprintf(xs_errbuf, sizeof(xs_errbuf), "triangular solve dependency wait expired");
-      return XS_ERR_DEVICE;
-    }
-  }
-  if (rc < 0 && rc != XS_ERR_BREAKDOWN) return rc;
   // results
   int nres = std::min(c->precision == 1 ? c->outer_iters + 1 : c->iters + 1, c->resvec_cap);
   c->resvec.resize(nres);
@@ -820,6 +835,88 @@ extern "C" int xs_solve_device(xs_ctx *c, const double *d_rhs, double *d_sol) {
   // the caller's producers of d_rhs/d_sol may run on any stream
   XS_HIP(hipDeviceSynchronize());
   return run_solve(c, d_rhs, d_sol);
+}
+
+// ---------------------------------------------------------------------------
+// k right-hand sides
+// ---------------------------------------------------------------------------
+static int run_multi(xs_ctx *c, int k, const double *d_b, double *d_x, int layout) {
+  int rc;
+  if ((rc = prepare_solve(c))) return rc;
+  if (c->precision != 0) {
+    snprintf(xs_errbuf, sizeof(xs_errbuf), "block solve: FP64 only (precision 0)");
+    return XS_ERR_UNSUPPORTED;
+  }
+  if (c->ksp != XS_KSP_BICGSTAB && c->ksp != XS_KSP_CG) {
+    snprintf(xs_errbuf, sizeof(xs_errbuf), "block solve: BiCGSTAB or CG only");
+    return XS_ERR_UNSUPPORTED;
+  }
+  if ((rc = xs_multi_reserve(c, k, layout == XS_RHS_COLMAJOR ? 2 : 0))) return rc;
+  const int n = c->A.n;
+  const double *B = d_b;
+  double *X = d_x;
+  if (layout == XS_RHS_COLMAJOR) {
+    double *mB = xs_multi_vec(c, k, 9), *mX = xs_multi_vec(c, k, 10);
+    xs_mlayout(c->stream, n, k, d_b, mB, true);
+    xs_mlayout(c->stream, n, k, d_x, mX, true);
+    B = mB;
+    X = mX;
+  }
+  XS_HIP(hipEventRecord(c->ev0, c->stream));
+  rc = xs_solve_block(c, k, B, X);
+  if (layout == XS_RHS_COLMAJOR && (rc >= 0 || rc == XS_ERR_BREAKDOWN))
+    xs_mlayout(c->stream, n, k, X, d_x, false);
+  return finish_solve(c, rc);
+}
+
+extern "C" int xs_solve_multi_device(xs_ctx *c, int nrhs, const double *d_rhs, double *d_sol,
+                                     int layout) {
+  if (!c || !d_rhs || !d_sol || nrhs < 1 || nrhs > XS_MAX_RHS) return XS_ERR_ARG;
+  if (layout != XS_RHS_COLMAJOR && layout != XS_RHS_INTERLEAVED) return XS_ERR_ARG;
+  XS_HIP(hipDeviceSynchronize());
+  return run_multi(c, nrhs, d_rhs, d_sol, layout);
+}
+
+extern "C" int xs_solve_multi(xs_ctx *c, int nrhs, const double *rhs, double *sol, int layout) {
+  if (!c || !rhs || !sol || nrhs < 1 || nrhs > XS_MAX_RHS) return XS_ERR_ARG;
+  if (layout != XS_RHS_COLMAJOR && layout != XS_RHS_INTERLEAVED) return XS_ERR_ARG;
+  if (!c->mat_ready) return XS_ERR_STATE;
+  const size_t m = (size_t)c->A.n * nrhs;
+  double *db = nullptr, *dx = nullptr;
+  int rc = dalloc(&db, m);
+  if (rc) return rc;
+  if ((rc = dalloc(&dx, m))) { dfree(db); return rc; }
+  if (hipMemcpy(db, rhs, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dx, sol, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess) {
+    dfree(db); dfree(dx);
+    return XS_ERR_DEVICE;
+  }
+  rc = run_multi(c, nrhs, db, dx, layout);
+  if ((rc >= 0 || rc == XS_ERR_BREAKDOWN) &&
+      hipMemcpy(sol, dx, sizeof(double) * m, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = XS_ERR_DEVICE;
+  dfree(db);
+  dfree(dx);
+  return rc;
+}
+
+extern "C" int xs_get_multi_results(const xs_ctx *c, int *iters, double *res0, double *resl,
+                                    int cap) {
+  if (!c) return XS_ERR_ARG;
+  const int k = std::min(c->mk, cap);
+  for (int j = 0; j < k; ++j) {
+    if (iters) iters[j] = c->m_iters[j];
+    if (res0) res0[j] = c->m_res0[j];
+    if (resl) resl[j] = c->m_resl[j];
+  }
+  return c->mk;
+}
+
+extern "C" int xs_get_multi_resvec(const xs_ctx *c, int col, double *out, int len) {
+  if (!c || !out || col < 0 || col >= c->mk) return XS_ERR_ARG;
+  const int cnt = std::min(len, c->m_iters[col] + 1);
+  for (int i = 0; i < cnt; ++i) out[i] = c->m_resvec[(size_t)i * c->mk + col];
+  return cnt;
 }
 
 extern "C" int xs_solve(xs_ctx *c, const double *rhs, double *sol) {
@@ -1017,6 +1114,45 @@ extern "C" int xs_pc_apply_t(xs_ctx *c, const double *x, double *y, void *stream
   if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
   if ((rc = xs_ctx_ensure_work(c, XS_NWORK))) return rc;
   return xs_op_pc_t(c, x, y, xs_ctx_vec(c, XS_NWORK - 1), pick(c, stream));
+}
+
+// Y = A X for k interleaved columns (n x k row-major device blocks).
+extern "C" int xs_spmm(xs_ctx *c, int nrhs, const double *x, double *y, void *stream) {
+  if (!c || !x || !y || nrhs < 1 || nrhs > XS_MAX_RHS) return XS_ERR_ARG;
+  if (!c->mat_ready) return XS_ERR_STATE;
+  xs_mspmm(pick(c, stream), c->A, nrhs, x, y, EPI_Y, nullptr, nullptr, nullptr, nullptr);
+  XS_HIP(hipGetLastError());
+  return xs_take_launch_guard() ? XS_ERR_STATE : XS_OK;
+}
+
+// Y = M^{-1} X for k interleaved columns.
+extern "C" int xs_pc_apply_multi(xs_ctx *c, int nrhs, const double *x, double *y, void *stream) {
+  if (!c || !x || !y || nrhs < 1 || nrhs > XS_MAX_RHS) return XS_ERR_ARG;
+  if (!c->mat_ready) return XS_ERR_STATE;
+  int rc;
+  if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
+  if ((rc = ensure_common(c))) return rc;
+  hipStream_t s = pick(c, stream);
+  const int n = c->A.n;
+  switch (c->pc) {
+    case XS_PC_JACOBI:
+      xs_mscale(s, n, nrhs, c->jac_dinv, x, y);
+      break;
+    case XS_PC_ILU0:
+    case XS_PC_ILUT: {
+      if ((rc = xs_multi_reserve(c, nrhs, 0))) return rc;
+      double *tmp = xs_multi_vec(c, nrhs, 8);
+      xs_mtrsv(s, false, n, nrhs, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, nullptr, x, tmp,
+               c->counters + 2, nullptr);
+      xs_mtrsv(s, true, n, nrhs, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, c->ilu.dinv, tmp, y,
+               c->counters + 4, nullptr);
+      break;
+    }
+    default:
+      XS_HIP(hipMemcpyAsync(y, x, sizeof(double) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, s));
+  }
+  XS_HIP(hipGetLastError());
+  return xs_take_launch_guard() ? XS_ERR_STATE : XS_OK;
 }
 
 extern "C" int xs_trsv(xs_ctx *c, int which, const double *b, double *x, void *stream) {

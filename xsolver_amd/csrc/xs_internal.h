@@ -156,3 +156,15 @@ void xs_grad_vals(hipStream_t s, const xs_csr &A, const double *lam, const doubl
 void xs_jacobi_dinv(hipStream_t s, const xs_csr &A, double *dinv, int *d_bad);
 // 1 if a launcher refused to launch since the last call (and clears it)
 int xs_take_launch_guard();
+void xs_raise_launch_guard();
+// Launch guard: a launcher that finds a required pointer null (a set-up
+// ordering bug) or a size out of range does not launch; it raises the flag,
+// which the host driver turns into an error instead of letting a kernel touch
+// address 0.
+#define XS_GUARD(cond)            \
+  do {                            \
+    if (!(cond)) {                \
+      xs_raise_launch_guard();    \
+      return;                     \
+    }                             \
+  } while (0)

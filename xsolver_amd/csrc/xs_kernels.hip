@@ -20,13 +20,7 @@
 // turns into an error instead of letting a kernel touch address 0.
 static std::atomic<int> g_launch_guard{0};
 int xs_take_launch_guard() { return g_launch_guard.exchange(0); }
-#define XS_GUARD(cond)                 \
-  do {                                 \
-    if (!(cond)) {                     \
-      g_launch_guard.store(1);         \
-      return;                          \
-    }                                  \
-  } while (0)
+void xs_raise_launch_guard() { g_launch_guard.store(1); }
 
 #include "xs_device.h"
 
@@ -211,7 +205,7 @@ __global__ void k_max_row_len(int n, const int *__restrict__ rp, int *out) {
 }
 
 int xs_max_row_len(hipStream_t s, const xs_csr &A, int *d_tmp) {
-  if (!A.rp || !d_tmp) { g_launch_guard.store(1); return 0; }
+  if (!A.rp || !d_tmp) { xs_raise_launch_guard(); return 0; }
   int h = 0;
   (void)hipMemsetAsync(d_tmp, 0, sizeof(int), s);
   hipLaunchKernelGGL(k_max_row_len, dim3((A.n + 255) / 256), dim3(256), 0, s, A.n, A.rp, d_tmp);
@@ -589,36 +583,6 @@ void xs_ilu_dinv(hipStream_t s, xs_ilu &F, int *d_bad) {
 // (hip/Precond/gpuparilu.cpp:257-391): there one wave per row spins on a
 // separate volatile int flag; the backward solve scatters with double atomics.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ unsigned long long ld_relaxed(const double *p) {
-  return __hip_atomic_load(reinterpret_cast<unsigned long long *>(const_cast<double *>(p)),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Spin until the row value is published.  The wait is bounded (about a
-// second of polling, far beyond any legitimate wait): on expiry the flag
-// *err is raised and a quiet NaN is used, so every wave always exits and the
-// solve reports the failure instead of hanging.
-#define XS_SPIN_LIMIT (1u << 22)
-__device__ __forceinline__ double ld_ready(const double *p, unsigned *err) {
-  unsigned long long u = ld_relaxed(p);
-  unsigned polls = 0;
-  while (u == XS_SENTINEL) {
-    if (++polls > XS_SPIN_LIMIT) {
-      atomicOr(err, 1u);
-      u = 0x7FF8000000000000ull;
-      break;
-    }
-    __builtin_amdgcn_s_sleep(1);
-    u = ld_relaxed(p);
-  }
-  return __longlong_as_double((long long)u);
-}
-__device__ __forceinline__ void st_ready(double *p, double v) {
-  unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  if (u == XS_SENTINEL) u = 0x7FF8000000000000ull;
-  __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), u, __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
-
 template <bool UPPER>
 __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict__ rp,
                                                    const int *__restrict__ ci,

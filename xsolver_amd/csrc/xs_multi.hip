@@ -1,0 +1,637 @@
+// xs_multi.hip -- k right-hand sides at once.
+//
+// Vectors are n x k row-major blocks (X[i*k + j]): the k values that one
+// matrix entry multiplies are contiguous, so the SpMM and the triangular
+// solves read the matrix once for all k columns (SpMM bytes:
+// 12 nnz + 4(n+1) + 16 n k).  A workgroup of XS_BLOCK threads covers
+// XS_BLOCK / k rows x k columns (thread t: column t % k, row t / k), so a
+// row block of the vectors is one contiguous, coalesced segment.
+// Every column keeps its own Krylov scalars and convergence flag; a column
+// that has converged stops updating while the others go on, and once every
+// column is done all kernels exit at entry.
+// No reference counterpart (the reference solves one right-hand side).
+#include <climits>
+
+#include "xs_device.h"
+#include "xs_multi.h"
+
+__device__ __forceinline__ double &MSC(xs_mstate *ms, int slot, int j) { return ms->sc[slot][j]; }
+
+__device__ inline void m_conv(const xs_mred &R, int j, double resl) {
+  xs_mstate *ms = R.ms;
+  MSC(ms, MS_RESL, j) = resl;
+  if (!isfinite(resl)) ms->done[j] = 2;
+  else if (resl / MSC(ms, MS_RES0, j) < R.tol || resl <= R.abstol) ms->done[j] = 1;
+}
+
+__device__ inline void m_res0(const xs_mred &R, int j, double r0) {
+  xs_mstate *ms = R.ms;
+  MSC(ms, MS_RES0, j) = r0;
+  MSC(ms, MS_RESL, j) = r0;
+  if (R.resvec) R.resvec[j] = r0;
+  if (!isfinite(r0)) ms->done[j] = 2;
+  else if (r0 == 0.0 || r0 <= R.abstol) ms->done[j] = 1;
+}
+
+// Per-column scalar updates; the same formulas as the single right-hand
+// side finishing ops (hip/Solver/xsolver_bicgstab.cpp:36-103,
+// hip/Solver/xsolver_cg.cpp:28-58).
+__device__ inline void m_finish(const xs_mred &R, int j, int k, const double *tot, int NV) {
+  xs_mstate *ms = R.ms;
+  if (ms->done[j]) return;
+  switch (R.op) {
+    case MF_BICG_RES0:
+      m_res0(R, j, sqrt(tot[0]));
+      MSC(ms, MS_ALPHA, j) = 1.0;
+      MSC(ms, MS_GAMA, j) = 1.0;
+      MSC(ms, MS_BETA, j) = tot[0];
+      MSC(ms, MS_OMEGA, j) = (tot[0] * 1.0) / (1.0 * 1.0 + 1e-40);
+      MSC(ms, MS_BETO, j) = tot[0];
+      break;
+    case MF_BICG_GAMA:
+      MSC(ms, MS_GAMA, j) = MSC(ms, MS_BETA, j) / tot[0];
+      break;
+    case MF_BICG_ALPHA:
+      MSC(ms, MS_ALPHA, j) = tot[0] / tot[1];
+      break;
+    case MF_BICG_RES: {
+      const double resl = sqrt(tot[0]);
+      const int it = ms->iter[j] + 1;
+      ms->iter[j] = it;
+      if (R.resvec) R.resvec[(size_t)it * k + j] = resl;
+      m_conv(R, j, resl);
+      const double beta = tot[1];
+      MSC(ms, MS_OMEGA, j) =
+          (beta * MSC(ms, MS_GAMA, j)) / (MSC(ms, MS_ALPHA, j) * MSC(ms, MS_BETO, j) + 1e-40);
+      MSC(ms, MS_BETO, j) = beta;
+      MSC(ms, MS_BETA, j) = beta;
+      break;
+    }
+    case MF_CG_INIT:
+      MSC(ms, MS_TAU, j) = tot[0];
+      m_res0(R, j, sqrt(tot[1]));
+      break;
+    case MF_CG_ALPHA:
+      MSC(ms, MS_CGALPHA, j) = MSC(ms, MS_TAU, j) / tot[0];
+      break;
+    case MF_CG_RR:
+      MSC(ms, MS_RR, j) = tot[0];
+      break;
+    case MF_CG_RES: {
+      const double taoo = MSC(ms, MS_TAU, j);
+      MSC(ms, MS_TAU, j) = tot[0];
+      MSC(ms, MS_CGBETA, j) = tot[0] / taoo;
+      const double rr = NV > 1 ? tot[1] : MSC(ms, MS_RR, j);
+      const double resl = sqrt(rr);
+      const int it = ms->iter[j] + 1;
+      ms->iter[j] = it;
+      if (R.resvec) R.resvec[(size_t)it * k + j] = resl;
+      m_conv(R, j, resl);
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// Column sums of the workgroup's (row, column) values, published per
+// workgroup; the last workgroup of the grid combines the partials in a fixed
+// order (deterministic) and runs the finishing op for every column.
+template <int NV>
+__device__ __forceinline__ void mcol_reduce(int k, const double (&v)[NV], const xs_mred &R) {
+  __shared__ double red[NV][XS_BLOCK];
+  __shared__ int is_last;
+  const int per = XS_BLOCK / k;
+  for (int q = 0; q < NV; ++q) red[q][threadIdx.x] = v[q];
+  __syncthreads();
+  if ((int)threadIdx.x < k) {
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.0;
+      for (int r = 0; r < per; ++r) s += red[q][r * k + threadIdx.x];
+      R.partial[((size_t)blockIdx.x * NV + q) * k + threadIdx.x] = s;
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned t = atomicAdd(R.counter, 1u);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  const int j = threadIdx.x % k, g = threadIdx.x / k;
+  double s[NV];
+  for (int q = 0; q < NV; ++q) s[q] = 0.0;
+  if (g < per) {
+    for (unsigned b = g; b < gridDim.x; b += per)
+      for (int q = 0; q < NV; ++q)
+        s[q] += __longlong_as_double((long long)__hip_atomic_load(
+            reinterpret_cast<unsigned long long *>(&R.partial[((size_t)b * NV + q) * k + j]),
+            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+  __syncthreads();
+  for (int q = 0; q < NV; ++q) red[q][threadIdx.x] = s[q];
+  __syncthreads();
+  if ((int)threadIdx.x < k) {
+    double tot[2] = {0.0, 0.0};
+    for (int q = 0; q < NV; ++q)
+      for (int r = 0; r < per; ++r) tot[q] += red[q][r * k + threadIdx.x];
+    m_finish(R, threadIdx.x, k, tot, NV);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int all = 1;
+    for (int c = 0; c < k; ++c) all &= (R.ms->done[c] != 0);
+    R.ms->all = all;
+    *R.counter = 0u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SpMM with epilogues: Y = A X (EPI_Y), + w.y (EPI_DOT1), + w.y and y.y
+// (EPI_DOT2), Y = B - A X with y.y (EPI_BMAXN).  Columns already done are
+// skipped.
+// ---------------------------------------------------------------------------
+template <int EPI>
+__global__ __launch_bounds__(XS_BLOCK) void k_mspmm(int n, int k, const int *__restrict__ rp,
+                                                    const int *__restrict__ ci,
+                                                    const double *__restrict__ v,
+                                                    const double *__restrict__ X,
+                                                    double *__restrict__ Y,
+                                                    const double *__restrict__ B,
+                                                    const double *__restrict__ W, xs_mred R,
+                                                    const xs_mstate *ms) {
+  if (ms && ms->all) return;
+  constexpr int NV = (EPI == EPI_DOT2) ? 2 : 1;
+  const int per = XS_BLOCK / k;
+  const int j = threadIdx.x % k, rl = threadIdx.x / k;
+  double acc[NV];
+  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+  if (rl < per && !(ms && ms->done[j])) {
+    for (int row = blockIdx.x * per + rl; row < n; row += gridDim.x * per) {
+      double s = 0.0;
+      const int e1 = rp[row + 1];
+      for (int e = rp[row]; e < e1; ++e) s += v[e] * X[(size_t)ci[e] * k + j];
+      const size_t p = (size_t)row * k + j;
+      if (EPI == EPI_BMAXN) {
+        const double t = B[p] - s;
+        Y[p] = t;
+        acc[0] += t * t;
+      } else {
+        Y[p] = s;
+        if (EPI == EPI_DOT1 || EPI == EPI_DOT2) acc[0] += W[p] * s;
+        if (EPI == EPI_DOT2) acc[NV - 1] += s * s;
+      }
+    }
+  }
+  if (EPI == EPI_Y) return;
+  mcol_reduce<NV>(k, acc, R);
+}
+
+// a.b (and c.d) per column
+template <int NV>
+__global__ __launch_bounds__(XS_BLOCK) void k_mdot(int n, int k, const double *__restrict__ a,
+                                                   const double *__restrict__ b,
+                                                   const double *__restrict__ c,
+                                                   const double *__restrict__ d, xs_mred R) {
+  if (R.ms->all) return;
+  const int per = XS_BLOCK / k;
+  const int j = threadIdx.x % k, rl = threadIdx.x / k;
+  double acc[NV];
+  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+  if (rl < per && !R.ms->done[j]) {
+    for (int row = blockIdx.x * per + rl; row < n; row += gridDim.x * per) {
+      const size_t p = (size_t)row * k + j;
+      acc[0] += a[p] * b[p];
+      if (NV == 2) acc[NV - 1] += c[p] * d[p];
+    }
+  }
+  mcol_reduce<NV>(k, acc, R);
+}
+
+// P = R + omega (P - alpha U); Z = D^{-1} P with Jacobi
+__global__ __launch_bounds__(XS_BLOCK) void k_mbicg_p(int n, int k, const double *__restrict__ R,
+                                                      const double *__restrict__ U,
+                                                      double *__restrict__ P,
+                                                      const double *__restrict__ dinv,
+                                                      double *__restrict__ Z,
+                                                      const xs_mstate *ms) {
+  if (ms->all) return;
+  const int per = XS_BLOCK / k;
+  const int j = threadIdx.x % k, rl = threadIdx.x / k;
+  if (rl >= per || ms->done[j]) return;
+  const double om = ms->sc[MS_OMEGA][j], al = ms->sc[MS_ALPHA][j];
+  for (int row = blockIdx.x * per + rl; row < n; row += gridDim.x * per) {
+    const size_t p = (size_t)row * k + j;
+    const double t = R[p] + om * (P[p] - al * U[p]);
+    P[p] = t;
+    if (dinv) Z[p] = dinv[row] * t;
+  }
+}
+
+// S = R - gama U; Z1 = D^{-1} S with Jacobi
+__global__ __launch_bounds__(XS_BLOCK) void k_mbicg_s(int n, int k, const double *__restrict__ R,
+                                                      const double *__restrict__ U,
+                                                      double *__restrict__ Sv,
+                                                      const double *__restrict__ dinv,
+                                                      double *__restrict__ Z1,
+                                                      const xs_mstate *ms) {
+  if (ms->all) return;
+  const int per = XS_BLOCK / k;
+  const int j = threadIdx.x % k, rl = threadIdx.x / k;
+  if (rl >= per || ms->done[j]) return;
+  const double ga = ms->sc[MS_GAMA][j];
+  for (int row = blockIdx.x * per + rl; row < n; row += gridDim.x * per) {
+    const size_t p = (size_t)row * k + j;
+    const double t = R[p] - ga * U[p];
+    Sv[p] = t;
+    if (dinv) Z1[p] = dinv[row] * t;
+  }
+}
+
+// X += gama Z + alpha Z1; R = S - alpha V; per column ||r|| and rhat.r
+__global__ __launch_bounds__(XS_BLOCK) void k_mbicg_xr(int n, int k, double *__restrict__ X,
+                                                       double *__restrict__ Rv,
+                                                       const double *__restrict__ Z,
+                                                       const double *__restrict__ Z1,
+                                                       const double *__restrict__ Sv,
+                                                       const double *__restrict__ V,
+                                                       const double *__restrict__ Rh, xs_mred R) {
+  xs_mstate *ms = R.ms;
+  if (ms->all) return;
+  const int per = XS_BLOCK / k;
+  const int j = threadIdx.x % k, rl = threadIdx.x / k;
+  double acc[2] = {0.0, 0.0};
+  if (rl < per && !ms->done[j]) {
+    const double ga = ms->sc[MS_GAMA][j], al = ms->sc[MS_ALPHA][j];
+    for (int row = blockIdx.x * per + rl; row < n; row += gridDim.x * per) {
+      const size_t p = (size_t)row * k + j;
+      X[p] = X[p] + ga * Z[p] + al * Z1[p];
+      const double ri = Sv[p] - al * V[p];
+      Rv[p] = ri;
+      acc[0] += ri * ri;
+      acc[1] += Rh[p] * ri;
+    }
+  }
+  mcol_reduce<2>(k, acc, R);
+}
+
+// X += alpha P; R -= alpha AP; with Jacobi also Z = D^{-1} R and (z.r, r.r),
+// otherwise r.r only
+template <bool JAC>
+__global__ __launch_bounds__(XS_BLOCK) void k_mcg_xr(int n, int k, double *__restrict__ X,
+                                                     double *__restrict__ Rv,
+                                                     const double *__restrict__ P,
+                                                     const double *__restrict__ AP,
+                                                     const double *__restrict__ dinv,
+                                                     double *__restrict__ Z, xs_mred R) {
+  constexpr int NV = JAC ? 2 : 1;
+  xs_mstate *ms = R.ms;
+  if (ms->all) return;
+  const int per = XS_BLOCK / k;
+  const int j = threadIdx.x % k, rl = threadIdx.x / k;
+  double acc[NV];
+  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+  if (rl < per && !ms->done[j]) {
+    const double al = ms->sc[MS_CGALPHA][j];
+    for (int row = blockIdx.x * per + rl; row < n; row += gridDim.x * per) {
+      const size_t p = (size_t)row * k + j;
+      X[p] += al * P[p];
+      const double ri = Rv[p] - al * AP[p];
+      Rv[p] = ri;
+      if (JAC) {
+        const double zi = dinv[row] * ri;
+        Z[p] = zi;
+        acc[0] += zi * ri;
+        acc[NV - 1] += ri * ri;
+      } else {
+        acc[0] += ri * ri;
+      }
+    }
+  }
+  mcol_reduce<NV>(k, acc, R);
+}
+
+// P = Z + beta P
+__global__ __launch_bounds__(XS_BLOCK) void k_mcg_p(int n, int k, double *__restrict__ P,
+                                                    const double *__restrict__ Z,
+                                                    const xs_mstate *ms) {
+  if (ms->all) return;
+  const int per = XS_BLOCK / k;
+  const int j = threadIdx.x % k, rl = threadIdx.x / k;
+  if (rl >= per || ms->done[j]) return;
+  const double be = ms->sc[MS_CGBETA][j];
+  for (int row = blockIdx.x * per + rl; row < n; row += gridDim.x * per) {
+    const size_t p = (size_t)row * k + j;
+    P[p] = Z[p] + be * P[p];
+  }
+}
+
+// Y = D^{-1} X (all columns)
+__global__ __launch_bounds__(XS_BLOCK) void k_mscale(int n, int k, const double *__restrict__ d,
+                                                     const double *__restrict__ X,
+                                                     double *__restrict__ Y) {
+  const int per = XS_BLOCK / k;
+  const int j = threadIdx.x % k, rl = threadIdx.x / k;
+  if (rl >= per) return;
+  for (int row = blockIdx.x * per + rl; row < n; row += gridDim.x * per) {
+    const size_t p = (size_t)row * k + j;
+    Y[p] = d[row] * X[p];
+  }
+}
+
+__global__ void k_mfill(size_t m, unsigned long long *__restrict__ p, unsigned long long val) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < m;
+       q += (size_t)gridDim.x * blockDim.x)
+    p[q] = val;
+}
+
+// column-major (in[j*n + i]) <-> interleaved (out[i*k + j])
+__global__ void k_mlayout(int n, int k, const double *__restrict__ in, double *__restrict__ out,
+                          int to_interleaved) {
+  const size_t m = (size_t)n * k;
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < m;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const size_t i = q / k, j = q % k;
+    if (to_interleaved) out[q] = in[j * n + i];
+    else out[j * n + i] = in[q];
+  }
+}
+
+__global__ void k_mstate_init(xs_mstate *ms, int k) {
+  const int t = threadIdx.x;
+  for (int q = t; q < MS_NSLOT * XS_MAX_RHS; q += blockDim.x) (&ms->sc[0][0])[q] = 0.0;
+  for (int q = t; q < XS_MAX_RHS; q += blockDim.x) {
+    ms->done[q] = 0;
+    ms->iter[q] = 0;
+  }
+  if (t == 0) {
+    ms->all = 0;
+    ms->k = k;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sync-free triangular solve with KC right-hand sides: the ticket / sentinel
+// scheme of the single right-hand side solve (each wave owns 64 consecutive
+// rows, taken in ticket order; a row's value slot holds the sentinel until
+// published), with each lane carrying its row for KC columns.  X must hold
+// the sentinel in columns [j0, j0 + KC) when the kernel starts.
+// ---------------------------------------------------------------------------
+template <bool UPPER, int KC>
+__global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
+                                                    const int *__restrict__ rp,
+                                                    const int *__restrict__ ci,
+                                                    const double *__restrict__ lu,
+                                                    const int *__restrict__ dpos,
+                                                    const double *__restrict__ dinv,
+                                                    const double *__restrict__ B, double *X,
+                                                    unsigned *tickets, const xs_mstate *ms) {
+  if (ms && ms->all) return;
+  unsigned *err = tickets + 4;
+  const int lane = threadIdx.x & (XS_WAVE - 1);
+  unsigned w = 0;
+  if (lane == 0) w = atomicAdd(&tickets[0], 1u);
+  w = __shfl(w, 0, XS_WAVE);
+  const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
+  if (w < nw) {
+    int r0, r1;
+    if (!UPPER) { r0 = (int)w * XS_WAVE; r1 = min(n, r0 + XS_WAVE); }
+    else { r1 = n - (int)w * XS_WAVE; r0 = max(0, r1 - XS_WAVE); }
+    const int r = r0 + lane;
+    const bool act = r < r1;
+    double acc[KC];
+    double di = 1.0;
+    int eb = 0, ee = 0;  // in-wave entries [eb, ee)
+#pragma unroll
+    for (int q = 0; q < KC; ++q) acc[q] = 0.0;
+    if (act) {
+      const double *br = B + (size_t)r * k + j0;
+#pragma unroll
+      for (int q = 0; q < KC; ++q) acc[q] = br[q];
+      if (dinv) di = dinv[r];
+      int xb, xe;  // entries outside the wave [xb, xe)
+      if (!UPPER) {
+        xb = rp[r];
+        const int end = dpos[r];
+        xe = xb;
+        while (xe < end && ci[xe] < r0) ++xe;
+        eb = xe; ee = end;
+      } else {
+        const int beg = dpos[r] + 1;
+        xe = rp[r + 1];
+        xb = xe;
+        while (xb > beg && ci[xb - 1] >= r1) --xb;
+        eb = beg; ee = xb;
+      }
+      for (int e = xb; e < xe; ++e) {
+        const double a = lu[e];
+        const double *xc = X + (size_t)ci[e] * k + j0;
+        // issue all KC loads first; spin only on values not yet published
+        unsigned long long u[KC];
+#pragma unroll
+        for (int q = 0; q < KC; ++q) u[q] = ld_relaxed(xc + q);
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+          const double xv = (u[q] == XS_SENTINEL) ? ld_ready(xc + q, err)
+                                                  : __longlong_as_double((long long)u[q]);
+          acc[q] -= a * xv;
+        }
+      }
+    }
+    const int nint = ee - eb;
+    const int nb = UPPER ? r + 1 : r - 1;
+    const bool chain = !act || nint == 0 || (nint == 1 && ci[eb] == nb);
+    if (__all(chain)) {
+      // x_r = c + m x_nb per column, composed with a log-step scan
+      double m = (act && nint == 1) ? -di * lu[eb] : 0.0;
+#pragma unroll
+      for (int q = 0; q < KC; ++q) acc[q] = act ? di * acc[q] : 0.0;
+#pragma unroll
+      for (int d = 1; d < XS_WAVE; d <<= 1) {
+        const bool has = UPPER ? (lane + d < XS_WAVE) : (lane >= d);
+        const double mj = UPPER ? __shfl_down(m, d, XS_WAVE) : __shfl_up(m, d, XS_WAVE);
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+          const double cj = UPPER ? __shfl_down(acc[q], d, XS_WAVE) : __shfl_up(acc[q], d, XS_WAVE);
+          if (has) acc[q] = acc[q] + m * cj;
+        }
+        if (has) m = m * mj;
+      }
+    } else {
+      const int nrows = r1 - r0;
+      if (!UPPER) {
+        int e = eb;
+        int nc = (act && e < ee) ? ci[e] : INT_MAX;
+        for (int kk = 0; kk < nrows; ++kk) {
+          const bool hit = nc == r0 + kk;
+          const double a = hit ? lu[e] : 0.0;  // read once per row step
+#pragma unroll
+          for (int q = 0; q < KC; ++q) {
+            const double xk = __shfl(acc[q] * di, kk, XS_WAVE);
+            if (hit) acc[q] -= a * xk;
+          }
+          if (hit) { ++e; nc = (e < ee) ? ci[e] : INT_MAX; }
+        }
+      } else {
+        int e = ee - 1;
+        int nc = (act && e >= eb) ? ci[e] : INT_MIN;
+        for (int kk = nrows - 1; kk >= 0; --kk) {
+          const bool hit = nc == r0 + kk;
+          const double a = hit ? lu[e] : 0.0;  // read once per row step
+#pragma unroll
+          for (int q = 0; q < KC; ++q) {
+            const double xk = __shfl(acc[q] * di, kk, XS_WAVE);
+            if (hit) acc[q] -= a * xk;
+          }
+          if (hit) { --e; nc = (e >= eb) ? ci[e] : INT_MIN; }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) acc[q] *= di;
+    }
+    if (act) {
+      double *xr = X + (size_t)r * k + j0;
+#pragma unroll
+      for (int q = 0; q < KC; ++q) st_ready(xr + q, acc[q]);
+    }
+  }
+  if (lane == 0) {
+    const unsigned total = gridDim.x * (XS_BLOCK / XS_WAVE);
+    const unsigned f = atomicAdd(&tickets[1], 1u);
+    if (f == total - 1) {
+      __hip_atomic_store(&tickets[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&tickets[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int mgrid_rows(int n, int k) {
+  const int per = XS_BLOCK / k;
+  long g = ((long)n + per - 1) / per;
+  if (g > XS_MGRID) g = XS_MGRID;
+  return g < 1 ? 1 : (int)g;
+}
+static inline int mgrid_flat(size_t m) {
+  size_t g = (m + XS_BLOCK * 4 - 1) / (XS_BLOCK * 4);
+  if (g > 8192) g = 8192;
+  return g < 1 ? 1 : (int)g;
+}
+static inline bool k_ok(int k) { return k >= 1 && k <= XS_MAX_RHS; }
+
+void xs_mspmm(hipStream_t s, const xs_csr &A, int k, const double *X, double *Y, int epi,
+              const double *B, const double *W, const xs_mred *red, const xs_mstate *ms) {
+  XS_GUARD(A.rp && A.ci && A.v && X && Y && k_ok(k));
+  XS_GUARD(epi == EPI_Y || (red && red->partial && red->counter && red->ms));
+  XS_GUARD(epi != EPI_BMAXN || B);
+  XS_GUARD(!(epi == EPI_DOT1 || epi == EPI_DOT2) || W);
+  const dim3 grid(mgrid_rows(A.n, k)), blk(XS_BLOCK);
+  const xs_mred R = red ? *red : xs_mred{};
+  switch (epi) {
+    case EPI_Y:
+      hipLaunchKernelGGL(k_mspmm<EPI_Y>, grid, blk, 0, s, A.n, k, A.rp, A.ci, A.v, X, Y, B, W, R, ms);
+      break;
+    case EPI_DOT1:
+      hipLaunchKernelGGL(k_mspmm<EPI_DOT1>, grid, blk, 0, s, A.n, k, A.rp, A.ci, A.v, X, Y, B, W, R, ms);
+      break;
+    case EPI_DOT2:
+      hipLaunchKernelGGL(k_mspmm<EPI_DOT2>, grid, blk, 0, s, A.n, k, A.rp, A.ci, A.v, X, Y, B, W, R, ms);
+      break;
+    case EPI_BMAXN:
+      hipLaunchKernelGGL(k_mspmm<EPI_BMAXN>, grid, blk, 0, s, A.n, k, A.rp, A.ci, A.v, X, Y, B, W, R, ms);
+      break;
+    default:
+      XS_GUARD(false);
+  }
+}
+
+void xs_mdot(hipStream_t s, int n, int k, const double *a, const double *b, const double *c,
+             const double *d, const xs_mred &red) {
+  XS_GUARD(a && b && k_ok(k) && red.partial && red.counter && red.ms && (!c || d));
+  const dim3 grid(mgrid_rows(n, k)), blk(XS_BLOCK);
+  if (c) hipLaunchKernelGGL(k_mdot<2>, grid, blk, 0, s, n, k, a, b, c, d, red);
+  else hipLaunchKernelGGL(k_mdot<1>, grid, blk, 0, s, n, k, a, b, c, d, red);
+}
+
+void xs_mbicg_p(hipStream_t s, int n, int k, const double *R, const double *U, double *P,
+                const double *dinv, double *Z, const xs_mstate *ms) {
+  XS_GUARD(R && U && P && ms && k_ok(k) && (!dinv || Z));
+  hipLaunchKernelGGL(k_mbicg_p, dim3(mgrid_rows(n, k)), dim3(XS_BLOCK), 0, s, n, k, R, U, P, dinv, Z, ms);
+}
+void xs_mbicg_s(hipStream_t s, int n, int k, const double *R, const double *U, double *Sv,
+                const double *dinv, double *Z1, const xs_mstate *ms) {
+  XS_GUARD(R && U && Sv && ms && k_ok(k) && (!dinv || Z1));
+  hipLaunchKernelGGL(k_mbicg_s, dim3(mgrid_rows(n, k)), dim3(XS_BLOCK), 0, s, n, k, R, U, Sv, dinv, Z1, ms);
+}
+void xs_mbicg_xr(hipStream_t s, int n, int k, double *X, double *R, const double *Z,
+                 const double *Z1, const double *Sv, const double *V, const double *Rh,
+                 const xs_mred &red) {
+  XS_GUARD(X && R && Z && Z1 && Sv && V && Rh && k_ok(k) && red.partial && red.counter && red.ms);
+  hipLaunchKernelGGL(k_mbicg_xr, dim3(mgrid_rows(n, k)), dim3(XS_BLOCK), 0, s, n, k, X, R, Z, Z1,
+                     Sv, V, Rh, red);
+}
+void xs_mcg_xr(hipStream_t s, int n, int k, double *X, double *R, const double *P,
+               const double *AP, const double *dinv, double *Z, const xs_mred &red) {
+  XS_GUARD(X && R && P && AP && k_ok(k) && (!dinv || Z) && red.partial && red.counter && red.ms);
+  const dim3 grid(mgrid_rows(n, k)), blk(XS_BLOCK);
+  if (dinv) hipLaunchKernelGGL(k_mcg_xr<true>, grid, blk, 0, s, n, k, X, R, P, AP, dinv, Z, red);
+  else hipLaunchKernelGGL(k_mcg_xr<false>, grid, blk, 0, s, n, k, X, R, P, AP, dinv, Z, red);
+}
+void xs_mcg_p(hipStream_t s, int n, int k, double *P, const double *Z, const xs_mstate *ms) {
+  XS_GUARD(P && Z && ms && k_ok(k));
+  hipLaunchKernelGGL(k_mcg_p, dim3(mgrid_rows(n, k)), dim3(XS_BLOCK), 0, s, n, k, P, Z, ms);
+}
+void xs_mscale(hipStream_t s, int n, int k, const double *d, const double *X, double *Y) {
+  XS_GUARD(d && X && Y && k_ok(k));
+  hipLaunchKernelGGL(k_mscale, dim3(mgrid_rows(n, k)), dim3(XS_BLOCK), 0, s, n, k, d, X, Y);
+}
+void xs_mlayout(hipStream_t s, int n, int k, const double *in, double *out, bool to_interleaved) {
+  XS_GUARD(in && out && k_ok(k));
+  hipLaunchKernelGGL(k_mlayout, dim3(mgrid_flat((size_t)n * k)), dim3(XS_BLOCK), 0, s, n, k, in,
+                     out, to_interleaved ? 1 : 0);
+}
+void xs_mstate_init(hipStream_t s, xs_mstate *ms, int k) {
+  XS_GUARD(ms && k_ok(k));
+  hipLaunchKernelGGL(k_mstate_init, dim3(1), dim3(256), 0, s, ms, k);
+}
+
+template <bool UPPER>
+static void mtrsv_chunk(hipStream_t s, dim3 grid, int kc, int n, int k, int j0, const int *rp,
+                        const int *ci, const double *lu, const int *dpos, const double *dinv,
+                        const double *B, double *X, unsigned *tickets, const xs_mstate *ms) {
+  const dim3 blk(XS_BLOCK);
+#define XS_MT(KC) \
+  hipLaunchKernelGGL((k_mtrsv<UPPER, KC>), grid, blk, 0, s, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, ms)
+  switch (kc) {
+    case 16: XS_MT(16); break;
+    case 8: XS_MT(8); break;
+    case 4: XS_MT(4); break;
+    case 2: XS_MT(2); break;
+    default: XS_MT(1); break;
+  }
+#undef XS_MT
+}
+
+// X = T^{-1} B for all k columns; the columns are carried in chunks of
+// 16/8/4/2/1 per launch.  X is sentinel-filled here.
+void xs_mtrsv(hipStream_t s, bool upper, int n, int k, const int *rp, const int *ci,
+              const double *lu, const int *dpos, const double *dinv, const double *B, double *X,
+              unsigned *tickets, const xs_mstate *ms) {
+  XS_GUARD(rp && ci && lu && dpos && B && X && tickets && k_ok(k));
+  const size_t m = (size_t)n * k;
+  hipLaunchKernelGGL(k_mfill, dim3(mgrid_flat(m)), dim3(XS_BLOCK), 0, s, m,
+                     reinterpret_cast<unsigned long long *>(X), XS_SENTINEL);
+  const int nw = (n + XS_WAVE - 1) / XS_WAVE;
+  const dim3 grid((nw + (XS_BLOCK / XS_WAVE) - 1) / (XS_BLOCK / XS_WAVE));
+  for (int j0 = 0; j0 < k;) {
+    const int left = k - j0;
+    const int kc = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+    if (upper) mtrsv_chunk<true>(s, grid, kc, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, ms);
+    else mtrsv_chunk<false>(s, grid, kc, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, ms);
+    j0 += kc;
+  }
+}

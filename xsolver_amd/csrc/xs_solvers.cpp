@@ -299,6 +299,7 @@ int xs_solve_gcr(xs_ctx *c, const double *b, double *x) {
   return XS_NOT_CONVERGED;
 }
 void xs_solvers_free(xs_ctx *c) {
+  xs_multi_free(c);
   if (c->basis) (void)hipFree(c->basis);
   c->basis = nullptr;
   c->basis_len = 0;

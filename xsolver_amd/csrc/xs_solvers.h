@@ -96,6 +96,17 @@ struct xs_ctx {
   xs_state *st_in = nullptr;
   int outer_iters = 0;
 
+  // k right-hand sides solved together (interleaved n x k blocks)
+  double *mwork = nullptr;
+  size_t mwork_len = 0;
+  void *ms = nullptr;            // xs_mstate on the device
+  void *h_ms = nullptr;          // pinned mirror
+  double *mpartial = nullptr;
+  double *mresvec = nullptr;
+  size_t mresvec_len = 0;
+  int mk = 0;                    // columns of the last block solve
+  std::vector<int> m_iters;
+  std::vector<double> m_res0, m_resl, m_resvec;
   // results
   int iters = 0;
   double res0 = 0.0, resl = 0.0;
@@ -125,3 +136,8 @@ void xs_mixed_free(xs_ctx *c);
 int xs_build_transpose(xs_ctx *c);
 int xs_build_factor_transpose(xs_ctx *c);
 void xs_solvers_free(xs_ctx *c);
+// k right-hand sides (interleaved n x k device blocks; X = X0 in, solution out)
+int xs_solve_block(xs_ctx *c, int k, const double *B, double *X);
+int xs_multi_reserve(xs_ctx *c, int k, int extra_vecs);
+double *xs_multi_vec(xs_ctx *c, int k, int idx);
+void xs_multi_free(xs_ctx *c);
