@@ -106,9 +106,30 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *   "parilu_async"   1 = in-place asynchronous sweeps (reference behaviour),
  *                    0 = deterministic double-buffered sweeps
  *   "check_every"    iterations launched between host convergence checks
- *   "precision"      0 = FP64, 1 = FP32 inner / FP64 outer
+ *   "use_graph"      1 (default) = replay each batch of check_every iterations
+ *                    as a captured HIP graph, 0 = launch kernels one by one
+ *   "precision"      0 = FP64, 1 = FP32 inner Krylov solve inside FP64
+ *                    iterative refinement (BiCGSTAB/CG) or FP64 flexible
+ *                    GMRES (GMRES/FGMRES)
+ *   "factor_storage" ILU factors of the FP32 inner solve: 0 FP32, 1 BF16, 2 FP16
+ *   "inner_tol"      relative residual reduction asked of each inner solve (1e-4)
+ *   "parilu_scale"   1 (default) = factorise S A S, S = |diag(A)|^-1/2, and
+ *                    scale back (ParILU/ParILUT fixed-point stability)
+ *   "parilut_steps"  ParILUT add/sweep/select/sweep steps (reference: 10)
+ *   "refactor"       any value: recompute the preconditioner at the next solve
+ *   "profile"        bit mask of kernel families timed with HIP events
+ *                    (2 SpMV, 4 lower solve, 8 upper solve); read back with
+ *                    xs_get_time_ms("total:spmv"|"count:spmv"|...)
  *   "use_ell"        1 = ELL SpMV (same as advanced_pam[1] = 1) */
 int xs_set_option(xs_ctx *ctx, const char *name, double value);
+
+/* SpMV autotuner (no reference counterpart): times the CSR row-block kernel
+ * at nnz budgets per workgroup of 512..4096 and the ELL kernel (when its
+ * padding is under 50%) on this matrix and keeps the fastest; with the
+ * environment variable XS_TUNE_CACHE naming a file, results are read from
+ * and appended to it per matrix signature.  Choice and time are queried with
+ * xs_get_time_ms("tune:chunk" | "tune:ell" | "tune:us"). */
+int xs_autotune(xs_ctx *ctx);
 
 /* Factorise / prepare the preconditioner now (otherwise done by the first
  * solve, like Xsolver::xsolver_init, hip/Solver/xsolver_bicgstab.cpp:3-19). */

@@ -54,7 +54,10 @@ def lib():
                                       _ip, _ip, _dp, _dp, _dp]
         L.or_solve.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _ip, _ip, _dp,
                                _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
-                               ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _c_int_p]
+                               ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, _dp, _c_int_p]
+        L.or_ilu_sweeps.argtypes = [ctypes.c_int, _ip, _ip, _dp, _ip, ctypes.c_int,
+                                    ctypes.c_int, _dp, _dp]
         L.or_parilut.argtypes = [ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_int, _ip, _ip, _dp, _ip, _ip, _dp]
         for f in ("or_jacobi_setup", "or_ilu0", "or_ilut", "or_solve", "or_parilut"):
@@ -119,6 +122,17 @@ def upper_solve(U, dinv, b):
     lib().or_upper_solve(len(rp) - 1, rp, ci, v, np.ascontiguousarray(dinv, np.float64),
                          np.ascontiguousarray(b, np.float64), x)
     return x
+
+
+def ilu_sweeps(rp, ci, luval, uptr, kl, ku, x):
+    """Reference GPU ILU apply: kl Jacobi sweeps on L, ku on U (combined factors)."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    y = np.empty(len(rp) - 1)
+    lib().or_ilu_sweeps(len(rp) - 1, rp, ci, np.ascontiguousarray(luval, np.float64),
+                        np.ascontiguousarray(uptr, np.int32), kl, ku,
+                        np.ascontiguousarray(x, np.float64), y)
+    return y
 
 
 def jacobi_dinv(A, small=0.0):
@@ -200,7 +214,7 @@ def parilut(A, sweeps=10, fill=2.0, droptol=1e-4):
 
 
 def solve(A, b, ksp="cg", pc="jacobi", tol=1e-8, abstol=0.0, maxiter=1000,
-          restart=0, x0=None, ilut_tol=1e-4, ilut_fill=10):
+          restart=0, x0=None, ilut_tol=1e-4, ilut_fill=10, lsweeps=0, usweeps=0):
     """Run one oracle solve.  Returns (x, status, usediter, resvec)."""
     rp, ci, v = _csr(A)
     n = len(rp) - 1
@@ -209,5 +223,6 @@ def solve(A, b, ksp="cg", pc="jacobi", tol=1e-8, abstol=0.0, maxiter=1000,
     it = ctypes.c_int(0)
     st = lib().or_solve(ksp.encode(), pc.encode(), n, rp, ci, v,
                         np.ascontiguousarray(b, np.float64), x, tol, abstol, maxiter,
-                        restart, ilut_tol, ilut_fill, resvec, ctypes.byref(it))
+                        restart, ilut_tol, ilut_fill, lsweeps, usweeps, resvec,
+                        ctypes.byref(it))
     return x, st, it.value, resvec[: it.value + 1].copy()

@@ -203,6 +203,51 @@ void or_lusolve(void *ctx, const double *x, double *y) {
   free(t);
 }
 
+/* Iterative application of combined ILU factors (the reference's GPU ILU
+ * apply, hip/Precond/precon_gpuilu0.cpp:16-19 and precon_gpuilut.cpp:18-21,
+ * DeviceMatrixCSR::Lsolve_iter / Usolve_iter,
+ * hip/HostMatrix/DeviceMatrix/DeviceMatrixCSR.cpp:281-304, kernels
+ * LMultCSR / lsolveiterCSR / UMultCSR / usolveiterCSR :14-58):
+ *   L: t_0 = 0, t_{k+1} = x - L_strict t_k           (kl sweeps)
+ *   U: y_0 = 0, y_{k+1} = (t - U_strict y_k) / u_ii  (ku sweeps)
+ * Off-diagonal sums run over each row's entries in ascending column order.
+ * luval/uptr: combined factor on pattern (rp, ci) with the pivot at uptr[i].
+ * kl = 0 copies x (Lsolve_iter's maxiter == 0 branch); ku = 0 scales by
+ * 1/u_ii (Usolve_iter's jacobiSolve branch). */
+void or_ilu_sweeps(int n, const int *rp, const int *ci, const double *luval, const int *uptr,
+                   int kl, int ku, const double *x, double *y) {
+  double *t = (double *)calloc((size_t)n, sizeof(double));
+  double *u = (double *)calloc((size_t)n, sizeof(double));
+  double *tmp = (double *)malloc(sizeof(double) * (size_t)n);
+  if (kl == 0) memcpy(t, x, sizeof(double) * (size_t)n);
+  for (int k = 0; k < kl; k++) {
+    for (int i = 0; i < n; i++) {
+      double r = 0.0;
+      for (int e = rp[i]; e < uptr[i]; e++) r += luval[e] * t[ci[e]];
+      tmp[i] = r;
+    }
+    for (int i = 0; i < n; i++) t[i] = x[i] - tmp[i];
+  }
+  if (ku == 0)
+    for (int i = 0; i < n; i++) u[i] = (1.0 / luval[uptr[i]]) * t[i];
+  for (int k = 0; k < ku; k++) {
+    for (int i = 0; i < n; i++) {
+      double r = 0.0;
+      for (int e = uptr[i] + 1; e < rp[i + 1]; e++) r += luval[e] * u[ci[e]];
+      tmp[i] = r;
+    }
+    for (int i = 0; i < n; i++) u[i] = (1.0 / luval[uptr[i]]) * (t[i] - tmp[i]);
+  }
+  memcpy(y, u, sizeof(double) * (size_t)n);
+  free(t); free(u); free(tmp);
+}
+
+typedef struct { int n; const int *rp, *ci, *uptr; const double *luval; int kl, ku; } or_sweep_t;
+static void or_sweep_apply(void *ctx, const double *x, double *y) {
+  const or_sweep_t *c = (const or_sweep_t *)ctx;
+  or_ilu_sweeps(c->n, c->rp, c->ci, c->luval, c->uptr, c->kl, c->ku, x, y);
+}
+
 /* --- ILUT (dual threshold, per-row heaps) -------------------------------- */
 /* Min-heap of column ids ordered by |w[id]| ("less" = larger magnitude, so the
  * top is the smallest magnitude).  vh_push restates libstdc++'s push_heap
@@ -682,11 +727,13 @@ int or_parilut(int n, const int *arp, const int *aci, const double *av, int swee
 int or_solve(const char *ksp, const char *pc, int n, const int *rp,
              const int *ci, const double *v, const double *b, double *x,
              double tol, double abstol, int maxiter, int restart,
-             double ilut_tol, int ilut_fill, double *resvec, int *usediter) {
+             double ilut_tol, int ilut_fill, int lsweeps, int usweeps,
+             double *resvec, int *usediter) {
   or_precond_fn M = NULL;
   void *Mctx = NULL;
   or_jac_ctx_t jac;
   or_lu_t lu;
+  or_sweep_t sw;
   double *dinv = NULL, *luval = NULL, *lv = NULL, *uv = NULL, *diag = NULL;
   int *uptr = NULL, *lrp = NULL, *lci = NULL, *urp = NULL, *uci = NULL;
   int *lucols = NULL, *lurows = NULL;
@@ -716,13 +763,20 @@ int or_solve(const char *ksp, const char *pc, int n, const int *rp,
       if (or_ilut(n, rp, ci, v, ilut_fill, ilut_tol, cap, luval, lucols, lurows, uptr, diag)) goto out;
       frp = lurows; fci = lucols;
     }
-    lrp = malloc(sizeof(int) * (n + 1)); urp = malloc(sizeof(int) * (n + 1));
-    lci = malloc(sizeof(int) * (size_t)cap); uci = malloc(sizeof(int) * (size_t)cap);
-    lv = malloc(sizeof(double) * (size_t)cap); uv = malloc(sizeof(double) * (size_t)cap);
-    or_ilu_split(n, frp, fci, luval, uptr, lrp, lci, lv, urp, uci, uv, dinv);
-    lu.n = n; lu.lrp = lrp; lu.lci = lci; lu.lv = lv;
-    lu.urp = urp; lu.uci = uci; lu.uv = uv; lu.dinv = dinv;
-    M = or_lusolve; Mctx = &lu;
+    if (lsweeps > 0 || usweeps > 0) {
+      /* iterative application (the reference's GPU apply) */
+      sw.n = n; sw.rp = frp; sw.ci = fci; sw.uptr = uptr; sw.luval = luval;
+      sw.kl = lsweeps; sw.ku = usweeps;
+      M = or_sweep_apply; Mctx = &sw;
+    } else {
+      lrp = malloc(sizeof(int) * (n + 1)); urp = malloc(sizeof(int) * (n + 1));
+      lci = malloc(sizeof(int) * (size_t)cap); uci = malloc(sizeof(int) * (size_t)cap);
+      lv = malloc(sizeof(double) * (size_t)cap); uv = malloc(sizeof(double) * (size_t)cap);
+      or_ilu_split(n, frp, fci, luval, uptr, lrp, lci, lv, urp, uci, uv, dinv);
+      lu.n = n; lu.lrp = lrp; lu.lci = lci; lu.lv = lv;
+      lu.urp = urp; lu.uci = uci; lu.uv = uv; lu.dinv = dinv;
+      M = or_lusolve; Mctx = &lu;
+    }
   } else if (streq(pc, "parilut")) {
     /* ParILUT: 10 outer steps (hip/Precond/precon_gpuilut.cpp:5), fill factor
      * ilut_fill, relative drop tolerance ilut_tol; split as seqilut does. */

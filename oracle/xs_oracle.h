@@ -55,6 +55,9 @@ void or_ilu_split(int n, const int *rp, const int *ci, const double *luval,
                   const int *uptr, int *lrp, int *lci, double *lv,
                   int *urp, int *uci, double *uv, double *dinv);
 void or_lusolve(void *lu, const double *x, double *y);
+/* kl / ku Jacobi sweeps on the combined factors (reference GPU apply) */
+void or_ilu_sweeps(int n, const int *rp, const int *ci, const double *luval, const int *uptr,
+                   int kl, int ku, const double *x, double *y);
 
 /* ---- ParILU fixed-point sweep (synchronous / Jacobi form) --------------- */
 void or_parilu_sweep(int n, const int *rp, const int *ci, const double *a,
@@ -89,7 +92,8 @@ int or_parilut(int n, const int *arp, const int *aci, const double *av, int swee
 int or_solve(const char *ksp, const char *pc, int n, const int *rp,
              const int *ci, const double *v, const double *b, double *x,
              double tol, double abstol, int maxiter, int restart,
-             double ilut_tol, int ilut_fill, double *resvec, int *usediter);
+             double ilut_tol, int ilut_fill, int lsweeps, int usweeps,
+             double *resvec, int *usediter);
 
 #ifdef __cplusplus
 }

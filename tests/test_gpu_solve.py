@@ -165,6 +165,19 @@ def test_one_based_and_unsorted_input(dev):
     assert np.array_equal(g2["x"], ref["x"])
     for g in (ref, g1, g2):
         g["S"].close()
+    # a repeated column within a row is rejected (host upload) ...
+    ci3 = ci.copy()
+    ci3[rp[5] + 1] = ci3[rp[5]]
+    S = XSolver()
+    with pytest.raises(XSolverError) as e:
+        S.mat_setup(rp, ci3, v)
+    assert e.value.status == xsolver_amd.XS_ERR_ARG
+    # ... and a device-resident matrix must come sorted
+    import torch
+    with pytest.raises(XSolverError) as e:
+        S.mat_setup_device(*(torch.from_numpy(a).to(dev) for a in (rp, ci2, v2)))
+    assert e.value.status == xsolver_amd.XS_ERR_ARG
+    S.close()
 
 
 def test_legacy_singleton_api(dev):
@@ -274,3 +287,24 @@ def test_mixed_gmres_parilut(dev, factor):
     err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
     say(f"   oracle it={ito} mixed inner it={it} relres={rr:.2e} err-vs-fp64={err:.2e}")
     assert st == 0 and rr < 1.05e-10 and err < 1e-7
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_iterative_ilu_apply_solve(dev, precision):
+    """BiCGSTAB + ILU(0) with the reference's 20/20 Jacobi-sweep apply:
+    iterations within +-2 of the oracle running the same apply, FP64 true
+    residual below rtol (mixed: FP32 sweeps inside FP64 refinement)."""
+    A = gen.convdiff3d(24)
+    xs, b = gen.manufactured(A, 1)
+    xo, sto, ito, _ = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=1000,
+                                   lsweeps=20, usweeps=20)
+    g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000, parilu_async=0, parilu_sweeps=60,
+                  lsolve_sweeps=20, usolve_sweeps=20, precision=precision)
+    say(f"[sweep-solve] precision={precision} oracle it={ito} gpu it={g['it']} "
+        f"relres={true_relres(A, b, g['x']):.2e}")
+    assert sto == 0 and g["st"] == 0
+    if precision == 0:
+        assert abs(g["it"] - ito) <= 2
+        assert np.linalg.norm(g["x"] - xo) / np.linalg.norm(xo) < 50 * 1e-10
+    assert true_relres(A, b, g["x"]) < 1.05e-10
+    g["S"].close()

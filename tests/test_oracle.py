@@ -335,3 +335,68 @@ def test_parilut_invariants():
     _, s2, it2, _ = oracle.solve(A, b, "fgmres", "parilut", tol=1e-10, maxiter=300, restart=30,
                                  ilut_tol=1e-4, ilut_fill=2)
     assert s1 == 0 and s2 == 0 and it2 < it1
+
+
+def _dense_factors(A, lu, uptr):
+    """Dense strict L, strict U and the pivots of combined factors on A's pattern."""
+    rp, ci, _ = A
+    n = len(rp) - 1
+    Ls = np.zeros((n, n)); Us = np.zeros((n, n)); piv = np.zeros(n)
+    for i in range(n):
+        for e in range(rp[i], rp[i + 1]):
+            j = ci[e]
+            if j < i: Ls[i, j] = lu[e]
+            elif j > i: Us[i, j] = lu[e]
+            else: piv[i] = lu[e]
+    return Ls, Us, piv
+
+
+def test_ilu_sweeps_closed_forms():
+    """The reference's iterative GPU ILU apply (k Jacobi sweeps per factor):
+    k = 0 gives the copy / diagonal scaling branches, k sweeps on the unit
+    lower factor give the truncated Neumann series sum_{j<k} (-L)^j x, and
+    k >= the dependency depth gives the exact triangular solves."""
+    A = gen.convdiff3d(5)
+    rp, ci, _ = A
+    n = len(rp) - 1
+    lu, uptr = oracle.ilu0(A)
+    Ls, Us, piv = _dense_factors(A, lu, uptr)
+    x = np.random.default_rng(4).uniform(-1, 1, n)
+    # no sweeps: t = x, y = x / u_ii
+    assert np.array_equal(oracle.ilu_sweeps(rp, ci, lu, uptr, 0, 0, x), (1.0 / piv) * x)
+    for k in (1, 2, 3, 7):
+        neu = np.zeros(n); term = x.copy()
+        for _ in range(k):
+            neu += term
+            term = -Ls @ term
+        y = oracle.ilu_sweeps(rp, ci, lu, uptr, k, 0, x)
+        assert np.allclose(y, neu / piv, rtol=1e-13, atol=1e-15)
+        # U sweeps: y_{j+1} = D^{-1} (t - U_s y_j), y_0 = 0, by dense algebra
+        t = neu
+        yy = np.zeros(n)
+        for _ in range(k):
+            yy = (t - Us @ yy) / piv
+        y2 = oracle.ilu_sweeps(rp, ci, lu, uptr, k, k, x)
+        assert np.allclose(y2, yy, rtol=1e-13, atol=1e-15)
+    # depth of the 7-point lower triangle on 5^3 is 3*4 = 12 < 20: exact
+    ex = sla.solve_triangular(np.diag(piv) + Us,
+                              sla.solve_triangular(np.eye(n) + Ls, x, lower=True), lower=False)
+    ye = oracle.ilu_sweeps(rp, ci, lu, uptr, 20, 20, x)
+    assert np.allclose(ye, ex, rtol=1e-12, atol=1e-14)
+
+
+def test_solve_with_iterative_ilu_apply():
+    """BiCGSTAB + ILU(0) with the reference's 20/20 sweep apply converges; with
+    as many sweeps as the dependency depth it reproduces the exact-apply run."""
+    A = gen.convdiff3d(6)
+    _, b = gen.manufactured(A, 1)
+    xe, se, ie, re = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=500)
+    xd, sd, idp, rd = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=500,
+                                   lsweeps=16, usweeps=16)      # depth 15
+    assert se == 0 and sd == 0 and idp == ie
+    assert np.allclose(rd, re, rtol=1e-9)
+    x2, s2, i2, _ = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=500,
+                                 lsweeps=2, usweeps=2)
+    assert s2 == 0 and i2 >= ie
+    r = b - gen.spmv(A, x2)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1.05e-10

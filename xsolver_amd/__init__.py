@@ -44,7 +44,7 @@ EXPORTS = [
     "xs_batch_create", "xs_batch_solve", "xs_batch_solve_device", "xs_batch_get_results",
     "xs_batch_get_time_ms", "xs_batch_destroy", "xs_solve_transpose_device",
     "xs_grad_values_device", "xs_solve_multi_device", "xs_solve_multi", "xs_get_multi_results",
-    "xs_get_multi_resvec", "xs_spmm", "xs_pc_apply_multi",
+    "xs_get_multi_resvec", "xs_spmm", "xs_pc_apply_multi", "xs_autotune",
 ]
 
 XS_RHS_COLMAJOR = 0
@@ -113,6 +113,7 @@ def lib():
         "xs_get_multi_results": (_i, [_vp, _vp, _vp, _vp, _i]),
         "xs_get_multi_resvec": (_i, [_vp, _i, _vp, _i]),
         "xs_spmm": (_i, [_vp, _i, _vp, _vp, _vp]),
+        "xs_autotune": (_i, [_vp]),
         "xs_pc_apply_multi": (_i, [_vp, _i, _vp, _vp, _vp]),
         "xsolver_mat_setup": (None, [_i, _vp, _vp, _vp, _vp]),
         "xsolver_ksp_settype": (None, [_cp]),
@@ -202,6 +203,13 @@ class XSolver:
 
     def set_option(self, name, value):
         return _check(lib().xs_set_option(self.h, name.encode(), float(value)), "xs_set_option")
+
+    def autotune(self):
+        """Pick the SpMV format / row-block size by timing; returns
+        (chunk, ell, microseconds per SpMV)."""
+        _check(lib().xs_autotune(self.h), "xs_autotune")
+        return (int(self.time_ms("tune:chunk")), int(self.time_ms("tune:ell")),
+                self.time_ms("tune:us"))
 
     def pc_setup(self):
         return _check(lib().xs_pc_setup(self.h), "xs_pc_setup")

@@ -5,6 +5,62 @@
 #include "xs_internal.h"
 
 // ---------------------------------------------------------------------------
+// Ready-flag loads/stores of the sync-free triangular solves: a value slot
+// holds XS_SENTINEL until its row is published with an agent-scope store.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long ld_relaxed(const double *p) {
+  return __hip_atomic_load(reinterpret_cast<unsigned long long *>(const_cast<double *>(p)),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Spin until the row value is published.  The wait is bounded (about a
+// second of polling, far beyond any legitimate wait): on expiry the flag
+// *err is raised and a quiet NaN is used, so every wave always exits and the
+// solve reports the failure instead of hanging.
+#define XS_SPIN_LIMIT (1u << 22)
+__device__ __forceinline__ double ld_ready(const double *p, unsigned *err) {
+  unsigned long long u = ld_relaxed(p);
+  unsigned polls = 0;
+  while (u == XS_SENTINEL) {
+    if (++polls > XS_SPIN_LIMIT) {
+      atomicOr(err, 1u);
+      u = 0x7FF8000000000000ull;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    u = ld_relaxed(p);
+  }
+  return __longlong_as_double((long long)u);
+}
+__device__ __forceinline__ void st_ready(double *p, double v) {
+  unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  if (u == XS_SENTINEL) u = 0x7FF8000000000000ull;
+  __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), u, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Work order of the sync-free triangular solves: one ticket per workgroup
+// (the XS_BLOCK / XS_WAVE waves of the workgroup own consecutive 64-row
+// groups), so a wave only ever waits for rows of waves that took an earlier
+// ticket or sit in its own workgroup -- all resident.  One same-address
+// atomic per workgroup instead of per wave (same-address atomics serialise).
+__device__ __forceinline__ unsigned trsv_ticket(unsigned *tickets) {
+  __shared__ unsigned s_ticket;
+  if (threadIdx.x == 0) s_ticket = atomicAdd(&tickets[0], 1u);
+  __syncthreads();
+  return s_ticket * (XS_BLOCK / XS_WAVE) + threadIdx.x / XS_WAVE;
+}
+// The workgroup that retires last resets both counters for the next launch.
+__device__ __forceinline__ void trsv_retire(unsigned *tickets) {
+  if (threadIdx.x == 0) {
+    const unsigned f = atomicAdd(&tickets[1], 1u);
+    if (f == gridDim.x - 1) {
+      __hip_atomic_store(&tickets[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&tickets[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Reductions
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v) {
@@ -113,72 +169,54 @@ __device__ inline void do_finish(const xs_reduce &R, const double *tot, int NV) 
   }
 }
 
-// Block-reduce v, publish the block partial, and let the last workgroup of the
-// grid combine all partials (in block order) and run the finishing op.
+// Block-reduce v and publish the block's partials; the workgroup with the
+// highest index combines all partials in block order and runs the finishing
+// op.  The partial slots hold XS_SENTINEL between reductions: a block stores
+// its value with an agent-scope store (the value is its own ready flag), the
+// combining block waits for each slot and writes the sentinel back.  No
+// fence and no same-address atomic per workgroup: on gfx950 a device-scope
+// fence writes back the XCD's L2 and same-address atomics serialise
+// (~10 ns each), which made a counter-based last-block reduction over 8k
+// workgroups cost ~150 us.  The combining block may start before others (the
+// 8 XCDs dispatch independently); it only waits, holding one slot, while
+// every other block runs to its store.
 template <int NV>
 __device__ __forceinline__ void reduce_finish(double (&v)[NV], const xs_reduce &R) {
-  __shared__ int is_last;
   block_sum<NV>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) R.partial[(size_t)blockIdx.x * NV + k] = v[k];
-    __threadfence();
-    unsigned t = atomicAdd(R.counter, 1u);
-    is_last = (t == gridDim.x - 1);
+    for (int k = 0; k < NV; ++k) st_ready(&R.partial[(size_t)blockIdx.x * NV + k], v[k]);
   }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
+  if (blockIdx.x != gridDim.x - 1) return;
   double tot[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) tot[k] = 0.0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += XS_BLOCK) {
+  const unsigned m = gridDim.x * NV;
+  // 8 slots in flight per thread, then wait on the ones not yet published
+  for (unsigned b0 = threadIdx.x; b0 < m; b0 += 8 * XS_BLOCK) {
+    unsigned long long u[8];
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-      tot[k] += __longlong_as_double((long long)__hip_atomic_load(
-          reinterpret_cast<unsigned long long *>(&R.partial[(size_t)b * NV + k]), __ATOMIC_RELAXED,
-          __HIP_MEMORY_SCOPE_AGENT));
+    for (int q = 0; q < 8; ++q) {
+      const unsigned s = b0 + q * XS_BLOCK;
+      u[q] = s < m ? ld_relaxed(&R.partial[s]) : 0ull;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const unsigned s = b0 + q * XS_BLOCK;
+      if (s < m) {
+        const double val = u[q] == XS_SENTINEL ? ld_ready(&R.partial[s], R.counter)
+                                               : __longlong_as_double((long long)u[q]);
+        tot[s % NV] += val;
+        __hip_atomic_store(reinterpret_cast<unsigned long long *>(&R.partial[s]), XS_SENTINEL,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   }
   block_sum<NV>(tot);
   if (threadIdx.x == 0) {
-    *R.counter = 0u;
     double t4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < NV; ++k) t4[k] = tot[k];
     do_finish(R, t4, NV);
   }
-}
-
-// ---------------------------------------------------------------------------
-// Ready-flag loads/stores of the sync-free triangular solves: a value slot
-// holds XS_SENTINEL until its row is published with an agent-scope store.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ unsigned long long ld_relaxed(const double *p) {
-  return __hip_atomic_load(reinterpret_cast<unsigned long long *>(const_cast<double *>(p)),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Spin until the row value is published.  The wait is bounded (about a
-// second of polling, far beyond any legitimate wait): on expiry the flag
-// *err is raised and a quiet NaN is used, so every wave always exits and the
-// solve reports the failure instead of hanging.
-#define XS_SPIN_LIMIT (1u << 22)
-__device__ __forceinline__ double ld_ready(const double *p, unsigned *err) {
-  unsigned long long u = ld_relaxed(p);
-  unsigned polls = 0;
-  while (u == XS_SENTINEL) {
-    if (++polls > XS_SPIN_LIMIT) {
-      atomicOr(err, 1u);
-      u = 0x7FF8000000000000ull;
-      break;
-    }
-    __builtin_amdgcn_s_sleep(1);
-    u = ld_relaxed(p);
-  }
-  return __longlong_as_double((long long)u);
-}
-__device__ __forceinline__ void st_ready(double *p, double v) {
-  unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  if (u == XS_SENTINEL) u = 0x7FF8000000000000ull;
-  __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), u, __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
 }

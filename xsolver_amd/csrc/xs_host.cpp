@@ -128,11 +128,16 @@ static int ensure_common(xs_ctx *c) {
   if (!c->st || !c->counters || !c->d_int) return XS_ERR_STATE;
   size_t pblocks = std::max<size_t>(XS_MAX_REDUCE_BLOCKS, (size_t)c->plan.nblk);
   pblocks = std::max<size_t>(pblocks, (size_t)c->planT.nblk);
+  // ELL launch: two rows per thread
+  pblocks = std::max<size_t>(pblocks, ((size_t)c->A.n + 2 * XS_BLOCK - 1) / (2 * XS_BLOCK));
   if (pblocks > c->partial_blocks) {
     dfree(c->partial);
     int rc = dalloc(&c->partial, pblocks * 4);
     if (rc) return rc;
     c->partial_blocks = pblocks;
+    // reduction slots hold the sentinel between reductions
+    xs_fill_sentinel(c->stream, c->partial, (int)(pblocks * 4));
+    XS_HIP(hipStreamSynchronize(c->stream));
   }
   return XS_OK;
 }
@@ -155,9 +160,9 @@ static int analyse_pattern(xs_ctx *c) {
   // Host copy of the pattern (needed for the ILU column structure).  Checks
   // sorted columns within rows (the reference's GPU factorisations assume it,
   // hip/Precond/par_ilu_hip/Asplit_gpu.cpp:15-25).
-  const int n = c->A.n;
   const std::vector<int> &rp = c->h_rp;
   const std::vector<int> &ci = c->h_ci;
+  const int n = rp.empty() ? 0 : (int)rp.size() - 1;
   c->sorted = true;
   c->has_diag = true;
   for (int i = 0; i < n && c->sorted; i++) {
@@ -172,27 +177,148 @@ static int analyse_pattern(xs_ctx *c) {
   return XS_OK;
 }
 
+// (Re)build the CSR row-block plan of M with the given nnz budget per block.
+static int make_plan(xs_ctx *c, const xs_csr &M, xs_spmv_plan &P, int chunk) {
+  dfree(P.blk_row);
+  P = xs_spmv_plan();
+  const int nblk = std::max(1, (M.nnz + chunk - 1) / chunk);
+  int rc = dalloc(&P.blk_row, (size_t)nblk + 1);
+  if (rc) return rc;
+  xs_build_plan(c->stream, M, P, P.blk_row, chunk);
+  return XS_OK;
+}
+
+static int make_ell(xs_ctx *c) {
+  int rc;
+  xs_csr &A = c->A;
+  if (c->ell.v) return XS_OK;
+  int W = xs_max_row_len(c->stream, A, c->d_int);
+  c->ell.n = A.n;
+  c->ell.W = W;
+  c->ell.ld = (A.n + 511) / 512 * 512;
+  if ((rc = dalloc(&c->ell.ci, (size_t)W * c->ell.ld))) return rc;
+  if ((rc = dalloc(&c->ell.v, (size_t)W * c->ell.ld))) return rc;
+  xs_build_ell(c->stream, A, W, c->ell);
+  return XS_OK;
+}
+
 static int build_spmv(xs_ctx *c) {
   int rc;
   xs_csr &A = c->A;
   if (!A.rp || !A.ci || !A.v || !c->d_int) return XS_ERR_STATE;
-  double avg = A.n ? (double)A.nnz / A.n : 1.0;
-  int chunk = std::min(2048, std::max(512, ((int)(avg * XS_BLOCK) + XS_BLOCK - 1) / XS_BLOCK * XS_BLOCK));
-  int nblk = std::max(1, (A.nnz + chunk - 1) / chunk);
-  if ((rc = dalloc(&c->plan.blk_row, (size_t)nblk + 1))) return rc;
-  xs_build_plan(c->stream, A, c->plan, c->plan.blk_row);
-  if (c->use_ell) {
-    int W = xs_max_row_len(c->stream, A, c->d_int);
-    c->ell.n = A.n;
-    c->ell.W = W;
-    c->ell.ld = (A.n + 511) / 512 * 512;
-    if ((rc = dalloc(&c->ell.ci, (size_t)W * c->ell.ld))) return rc;
-    if ((rc = dalloc(&c->ell.v, (size_t)W * c->ell.ld))) return rc;
-    xs_build_ell(c->stream, A, W, c->ell);
-  }
+  if ((rc = make_plan(c, A, c->plan, xs_default_chunk(A)))) return rc;
+  if (c->use_ell && (rc = make_ell(c))) return rc;
   XS_HIP(hipStreamSynchronize(c->stream));
   if (xs_take_launch_guard()) return XS_ERR_STATE;
   return XS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SpMV autotuner (no reference counterpart).  Times the LDS-staged CSR
+// kernel at several nnz budgets per workgroup (the row-block size: rows that
+// do not fit the LDS tile of their block go one wavefront per row) and the
+// ELL kernel when its padding is modest, on the context's own matrix, and
+// keeps the fastest.  Results are cached per matrix signature in the text
+// file named by XS_TUNE_CACHE (one "key chunk ell us" line per matrix).
+// ---------------------------------------------------------------------------
+static std::string tune_key(const xs_ctx *c) {
+  // n, nnz and an FNV-1a hash of about 4096 sampled row lengths
+  const int n = c->A.n;
+  unsigned long long h = 1469598103934665603ull;
+  const int step = std::max(1, n / 4096);
+  for (int i = 0; i < n; i += step) {
+    h ^= (unsigned long long)(c->h_rp[i + 1] - c->h_rp[i]);
+    h *= 1099511628211ull;
+  }
+  char buf[96];
+  snprintf(buf, sizeof(buf), "spmv:%d:%d:%016llx", n, c->A.nnz, h);
+  return buf;
+}
+
+static bool tune_cache_get(const std::string &key, int *chunk, int *ell, double *us) {
+  const char *path = getenv("XS_TUNE_CACHE");
+  if (!path) return false;
+  FILE *f = fopen(path, "r");
+  if (!f) return false;
+  char k[96];
+  int ch, el;
+  double t;
+  bool hit = false;
+  while (fscanf(f, "%95s %d %d %lf", k, &ch, &el, &t) == 4)
+    if (key == k) { *chunk = ch; *ell = el; *us = t; hit = true; }
+  fclose(f);
+  return hit;
+}
+
+static void tune_cache_put(const std::string &key, int chunk, int ell, double us) {
+  const char *path = getenv("XS_TUNE_CACHE");
+  if (!path) return;
+  FILE *f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%s %d %d %.3f\n", key.c_str(), chunk, ell, us);
+  fclose(f);
+}
+
+// mean microseconds of one y = A x with the current format settings
+static int time_spmv(xs_ctx *c, const double *x, double *y, int reps, double *us) {
+  for (int r = 0; r < 3; r++) xs_op_spmv(c, x, y, EPI_Y, nullptr, nullptr, nullptr);
+  XS_HIP(hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; r++) xs_op_spmv(c, x, y, EPI_Y, nullptr, nullptr, nullptr);
+  XS_HIP(hipEventRecord(c->ev1, c->stream));
+  XS_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  XS_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *us = 1000.0 * ms / reps;
+  return XS_OK;
+}
+
+extern "C" int xs_autotune(xs_ctx *c) {
+  if (!c) return XS_ERR_ARG;
+  if (!c->mat_ready) return XS_ERR_STATE;
+  int rc;
+  if ((rc = xs_ctx_ensure_work(c, XS_NWORK))) return rc;
+  const std::string key = tune_key(c);
+  int best_chunk = 0, best_ell = 0;
+  double best_us = 0.0;
+  if (!tune_cache_get(key, &best_chunk, &best_ell, &best_us)) {
+    double *x = xs_ctx_vec(c, 0), *y = xs_ctx_vec(c, 1);
+    XS_HIP(hipMemsetAsync(x, 0, sizeof(double) * c->A.n, c->stream));
+    const int reps = 20;
+    struct keep_prof {  // no kernel timing while tuning; restored on every path
+      xs_ctx *c;
+      int kind;
+      ~keep_prof() { c->prof_kind = kind; }
+    } kp{c, c->prof_kind};
+    c->prof_kind = 0;
+    best_us = 1e30;
+    c->use_ell = 0;
+    static const int chunks[] = {512, 1024, 1536, 2048, 3072, 4096};
+    for (int ch : chunks) {
+      if ((rc = make_plan(c, c->A, c->plan, ch))) return rc;
+      double us;
+      if ((rc = time_spmv(c, x, y, reps, &us))) return rc;
+      if (us < best_us) { best_us = us; best_chunk = ch; best_ell = 0; }
+    }
+    // ELL when padding stays under 50%
+    int W = c->ell.v ? c->ell.W : xs_max_row_len(c->stream, c->A, c->d_int);
+    if ((double)W * c->A.n <= 1.5 * c->A.nnz) {
+      if ((rc = make_ell(c))) return rc;
+      c->use_ell = 1;
+      double us;
+      if ((rc = time_spmv(c, x, y, reps, &us))) return rc;
+      if (us < best_us) { best_us = us; best_ell = 1; }
+    }
+    tune_cache_put(key, best_chunk, best_ell, best_us);
+  }
+  if ((rc = make_plan(c, c->A, c->plan, best_chunk))) return rc;
+  c->use_ell = best_ell;
+  if (best_ell && (rc = make_ell(c))) return rc;
+  c->tune_chunk = best_chunk;
+  c->tune_us = best_us;
+  c->tuned = true;
+  XS_HIP(hipStreamSynchronize(c->stream));
+  if (xs_take_launch_guard()) return XS_ERR_STATE;
+  return ensure_common(c);
 }
 
 static int setup_common(xs_ctx *c, int n, int nnz, int use_ell) {
@@ -303,6 +429,10 @@ extern "C" int xs_mat_setup(xs_ctx *c, int n, const int *rowptr, const int *coli
       for (int e = a; e < b; e++) { c->h_ci[e] = row[e - a].first; hv[e] = row[e - a].second; }
     }
     analyse_pattern(c);
+    if (!c->sorted) {
+      snprintf(xs_errbuf, sizeof(xs_errbuf), "duplicate column index within a row");
+      return XS_ERR_ARG;
+    }
   }
   int rc;
   c->own_matrix = true;
@@ -400,6 +530,9 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   if (k == "parilu_sweeps") { if (value < 0) return XS_ERR_ARG; c->parilu_sweeps = (int)value; c->pc_ready = false; }
   else if (k == "parilu_async") { c->parilu_async = value != 0; c->pc_ready = false; }
   else if (k == "check_every") { if (value < 1) return XS_ERR_ARG; c->check_every = (int)value; }
+  else if (k == "use_graph") { c->use_graph = value != 0; }
+  else if (k == "lsolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->lsweeps = (int)value; }
+  else if (k == "usolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->usweeps = (int)value; }
   else if (k == "refactor") { c->pc_ready = false; c->mixed_ready = false; }
   else if (k == "factor_storage") { if (value < 0 || value > 2) return XS_ERR_ARG; c->factor_type = (int)value; c->mixed_ready = false; }
   else if (k == "inner_tol") { if (!(value > 0 && value < 1)) return XS_ERR_ARG; c->inner_tol = value; }
@@ -733,6 +866,21 @@ void xs_op_spmv(xs_ctx *c, const double *x, double *y, int epi, const double *b,
   xs_prof_end(c, 1);
 }
 
+// out = T^{-1} b approximated by k Jacobi sweeps from 0 (k = 0 and k = 1
+// coincide: b for L, b / u_ii for U); ping-pong between out and w so that the
+// last sweep lands in out.
+static void ilu_sweeps(xs_ctx *c, bool upper, int k, const double *b, double *out, double *w,
+                       const xs_state *st) {
+  const int m = k < 1 ? 1 : k;
+  double *cur = ((m - 1) % 2 == 0) ? out : w;
+  xs_tri_sweep(c->stream, c->ilu, upper, b, nullptr, cur, st);
+  for (int j = 2; j <= m; ++j) {
+    double *nxt = (cur == out) ? w : out;
+    xs_tri_sweep(c->stream, c->ilu, upper, b, cur, nxt, st);
+    cur = nxt;
+  }
+}
+
 // y = M^{-1} x on the device (tmp: scratch vector for the ILU forward solve)
 void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state *st) {
   switch (c->pc) {
@@ -748,13 +896,21 @@ void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state
         xs_trsv_upper_t(c->stream, c->ilu, tmp, y, c->counters + 4, st);
         break;
       }
-      xs_fill_sentinel(c->stream, tmp, c->A.n);
       xs_prof_begin(c, 2);
-      xs_trsv_lower(c->stream, c->ilu, x, tmp, c->counters + 2, st);
+      if (c->lsweeps >= 0) {
+        ilu_sweeps(c, false, c->lsweeps, x, tmp, xs_ctx_vec(c, 10), st);
+      } else {
+        xs_fill_sentinel(c->stream, tmp, c->A.n);
+        xs_trsv_lower(c->stream, c->ilu, x, tmp, c->counters + 2, st);
+      }
       xs_prof_end(c, 2);
-      xs_fill_sentinel(c->stream, y, c->A.n);
       xs_prof_begin(c, 3);
-      xs_trsv_upper(c->stream, c->ilu, tmp, y, c->counters + 4, st);
+      if (c->usweeps >= 0) {
+        ilu_sweeps(c, true, c->usweeps, tmp, y, xs_ctx_vec(c, 10), st);
+      } else {
+        xs_fill_sentinel(c->stream, y, c->A.n);
+        xs_trsv_upper(c->stream, c->ilu, tmp, y, c->counters + 4, st);
+      }
       xs_prof_end(c, 3);
       break;
     default:
@@ -779,12 +935,14 @@ static int finish_solve(xs_ctx *c, int rc) {
     snprintf(xs_errbuf, sizeof(xs_errbuf), "internal: a kernel launch found a null buffer");
     return XS_ERR_STATE;
   }
-  // wait-expired flags of the lower/upper triangular solves
-  unsigned fl[4] = {0, 0, 0, 0};
-  XS_HIP(hipMemcpy(fl, c->counters + 6, sizeof(fl), hipMemcpyDeviceToHost));
-  if (fl[0] | fl[2]) {
-    XS_HIP(hipMemset(c->counters + 6, 0, sizeof(fl)));
-    snprintf(xs_errbuf, sizeof(xs_errbuf), "triangular solve dependency wait expired");
+  // wait-expired flags: [0] grid reductions, [6]/[8] lower/upper triangular
+  // solves, [10] tall-skinny reductions
+  unsigned fl[12] = {0};
+  XS_HIP(hipMemcpy(fl, c->counters, sizeof(fl), hipMemcpyDeviceToHost));
+  if (fl[0] | fl[6] | fl[8] | fl[10]) {
+    XS_HIP(hipMemset(c->counters, 0, sizeof(fl)));
+    snprintf(xs_errbuf, sizeof(xs_errbuf), "%s wait expired",
+             (fl[6] | fl[8]) ? "triangular solve dependency" : "grid reduction");
     return XS_ERR_DEVICE;
   }
   return rc;
@@ -951,6 +1109,9 @@ extern "C" double xs_get_time_ms(const xs_ctx *c, const char *phase) {
   if (!c || !phase) return -1.0;
   if (!strcmp(phase, "pc_setup")) return c->t_pc_ms;
   if (!strcmp(phase, "solve")) return c->t_solve_ms;
+  if (!strcmp(phase, "tune:chunk")) return c->tuned ? c->tune_chunk : -1.0;
+  if (!strcmp(phase, "tune:ell")) return c->tuned ? c->use_ell : -1.0;
+  if (!strcmp(phase, "tune:us")) return c->tuned ? c->tune_us : -1.0;
   static const char *kn[4] = {"", "spmv", "trsv_lower", "trsv_upper"};
   for (int k = 1; k < 4; k++) {
     std::string tot = std::string("total:") + kn[k], cnt = std::string("count:") + kn[k];
@@ -999,11 +1160,7 @@ int xs_build_transpose(xs_ctx *c) {
   c->At.n = n;
   c->At.nnz = nnz;
   xs_permute_vals(c->stream, nnz, c->perm_At, c->A.v, c->At.v);
-  double avg = (double)nnz / n;
-  int chunk = std::min(2048, std::max(512, ((int)(avg * XS_BLOCK) + XS_BLOCK - 1) / XS_BLOCK * XS_BLOCK));
-  int nblk = std::max(1, (nnz + chunk - 1) / chunk);
-  if ((rc = dalloc(&c->planT.blk_row, (size_t)nblk + 1))) return rc;
-  xs_build_plan(c->stream, c->At, c->planT, c->planT.blk_row);
+  if ((rc = make_plan(c, c->At, c->planT, xs_default_chunk(c->At)))) return rc;
   XS_HIP(hipStreamSynchronize(c->stream));
   if (xs_take_launch_guard()) return XS_ERR_STATE;
   c->have_At = true;

@@ -53,8 +53,8 @@ enum xs_finish_op {
 };
 
 struct xs_reduce {
-  double *partial;     // [XS_MAX_REDUCE_BLOCKS * 4] (>= gridDim * nv)
-  unsigned *counter;   // last-block ticket (self-resetting)
+  double *partial;     // >= gridDim * nv slots, XS_SENTINEL between reductions
+  unsigned *counter;   // raised if a partial never arrives (bounded wait)
   xs_state *st;
   double *resvec;      // device residual history (may be null)
   int op;
@@ -113,7 +113,8 @@ void xs_launch_spmv_csr(hipStream_t s, const xs_csr &A, const xs_spmv_plan &P,
 void xs_launch_spmv_ell(hipStream_t s, const xs_ell &E, const double *x, double *y,
                         int epi, const double *b, const double *w, const xs_reduce *red);
 void xs_launch_spmv_t(hipStream_t s, const xs_csr &A, const double *x, double *y);
-void xs_build_plan(hipStream_t s, const xs_csr &A, xs_spmv_plan &P, int *scratch_blk);
+void xs_build_plan(hipStream_t s, const xs_csr &A, xs_spmv_plan &P, int *blk, int chunk);
+int xs_default_chunk(const xs_csr &A);
 void xs_build_ell(hipStream_t s, const xs_csr &A, int W, xs_ell &E);
 int xs_max_row_len(hipStream_t s, const xs_csr &A, int *d_tmp);
 
@@ -151,6 +152,9 @@ void xs_trsv_lower_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st);
 void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st);
+// one Jacobi sweep of a factor: yn = D (b - T_off yo); yo == null: yn = D b
+void xs_tri_sweep(hipStream_t s, const xs_ilu &F, bool upper, const double *b, const double *yo,
+                  double *yn, const xs_state *st);
 void xs_permute_vals(hipStream_t s, int nnz, const int *perm, const double *src, double *dst);
 void xs_grad_vals(hipStream_t s, const xs_csr &A, const double *lam, const double *x, double *g);
 void xs_jacobi_dinv(hipStream_t s, const xs_csr &A, double *dinv, int *d_bad);

@@ -178,15 +178,20 @@ __global__ void k_build_plan(int n, const int *__restrict__ rp, int chunk, int n
   blk_row[k] = lo;
 }
 
-void xs_build_plan(hipStream_t s, const xs_csr &A, xs_spmv_plan &P, int *blk) {
-  XS_GUARD(A.rp && blk);
-  // nnz budget per block: about 256 rows at the average row length, a
-  // multiple of 256, between 512 and 2048 so a straddling row still fits.
+// Default nnz budget per block: about 256 rows at the average row length, a
+// multiple of 256, between 512 and 2048 so a straddling row still fits.
+int xs_default_chunk(const xs_csr &A) {
   double avg = A.n ? (double)A.nnz / A.n : 1.0;
   int chunk = (int)(avg * XS_BLOCK);
   chunk = ((chunk + XS_BLOCK - 1) / XS_BLOCK) * XS_BLOCK;
   if (chunk < 512) chunk = 512;
   if (chunk > 2048) chunk = 2048;
+  return chunk;
+}
+
+// blk must hold (nnz + chunk - 1) / chunk + 1 entries.
+void xs_build_plan(hipStream_t s, const xs_csr &A, xs_spmv_plan &P, int *blk, int chunk) {
+  XS_GUARD(A.rp && blk && chunk > 0);
   P.chunk = chunk;
   P.nblk = (A.nnz + chunk - 1) / chunk;
   if (P.nblk < 1) P.nblk = 1;
@@ -583,6 +588,35 @@ void xs_ilu_dinv(hipStream_t s, xs_ilu &F, int *d_bad) {
 // (hip/Precond/gpuparilu.cpp:257-391): there one wave per row spins on a
 // separate volatile int flag; the backward solve scatters with double atomics.
 // ---------------------------------------------------------------------------
+// acc - sum of lu[e] * x[ci[e]] over up to 4 entries e, e+dir, ... (stopping
+// at `stop`), in that order: the 4 loads are issued together, then only the
+// values still holding the sentinel are waited for.
+__device__ __forceinline__ double trsv_gather4(double acc, int e, int stop, int dir,
+                                               const int *__restrict__ ci,
+                                               const double *__restrict__ lu, const double *x,
+                                               unsigned *err) {
+  unsigned long long u[4];
+  int c[4];
+  double a[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int eq = e + q * dir;
+    const bool in = dir > 0 ? eq < stop : eq > stop;
+    c[q] = in ? ci[eq] : -1;
+    a[q] = in ? lu[eq] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) u[q] = c[q] >= 0 ? ld_relaxed(x + c[q]) : 0ull;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (c[q] < 0) break;
+    const double v = u[q] == XS_SENTINEL ? ld_ready(x + c[q], err)
+                                         : __longlong_as_double((long long)u[q]);
+    acc -= a[q] * v;
+  }
+  return acc;
+}
+
 template <bool UPPER>
 __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict__ rp,
                                                    const int *__restrict__ ci,
@@ -594,9 +628,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
   if (st != nullptr && st->done) return;
   unsigned *err = tickets + 4;  // wait-expired flag, 4 words past the tickets
   const int lane = threadIdx.x & (XS_WAVE - 1);
-  unsigned w = 0;
-  if (lane == 0) w = atomicAdd(&tickets[0], 1u);
-  w = __shfl(w, 0, XS_WAVE);
+  const unsigned w = trsv_ticket(tickets);
   const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
   if (w < nw) {
     int r0, r1;
@@ -614,26 +646,14 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
         const int end = dpos[r];
         int xe = e;
         while (xe < end && ci[xe] < r0) ++xe;
-        for (; e + 4 <= xe; e += 4) {
-          const int c0 = ci[e], c1 = ci[e + 1], c2 = ci[e + 2], c3 = ci[e + 3];
-          const double x0 = ld_ready(x + c0, err), x1 = ld_ready(x + c1, err);
-          const double x2 = ld_ready(x + c2, err), x3 = ld_ready(x + c3, err);
-          acc -= lu[e] * x0; acc -= lu[e + 1] * x1; acc -= lu[e + 2] * x2; acc -= lu[e + 3] * x3;
-        }
-        for (; e < xe; ++e) acc -= lu[e] * ld_ready(x + ci[e], err);
+        for (; e < xe; e += 4) acc = trsv_gather4(acc, e, xe, 1, ci, lu, x, err);
         eb = xe; ee = end;
       } else {
         const int beg = dpos[r] + 1;
         int e = rp[r + 1] - 1;
         int xb = e + 1;
         while (xb > beg && ci[xb - 1] >= r1) --xb;
-        for (; e - 3 >= xb; e -= 4) {
-          const int c0 = ci[e], c1 = ci[e - 1], c2 = ci[e - 2], c3 = ci[e - 3];
-          const double x0 = ld_ready(x + c0, err), x1 = ld_ready(x + c1, err);
-          const double x2 = ld_ready(x + c2, err), x3 = ld_ready(x + c3, err);
-          acc -= lu[e] * x0; acc -= lu[e - 1] * x1; acc -= lu[e - 2] * x2; acc -= lu[e - 3] * x3;
-        }
-        for (; e >= xb; --e) acc -= lu[e] * ld_ready(x + ci[e], err);
+        for (; e >= xb; e -= 4) acc = trsv_gather4(acc, e, xb - 1, -1, ci, lu, x, err);
         eb = beg; ee = xb;
       }
     }
@@ -685,14 +705,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
     }
     if (act) st_ready(x + r, val);
   }
-  if (lane == 0) {
-    const unsigned total = gridDim.x * (XS_BLOCK / XS_WAVE);
-    const unsigned f = atomicAdd(&tickets[1], 1u);
-    if (f == total - 1) {
-      __hip_atomic_store(&tickets[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&tickets[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+  trsv_retire(tickets);
 }
 
 // x must hold the sentinel in every entry when the kernel starts
@@ -724,6 +737,46 @@ void xs_trsv_lower_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
 void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
   trsv_launch(s, true, F.n, F.trp, F.tci, F.tlu, F.tdpos, nullptr, b, x, tickets, st);
+}
+
+// ---------------------------------------------------------------------------
+// Iterative application of one ILU factor: the reference's GPU ILU apply
+// (DeviceMatrixCSR::Lsolve_iter / Usolve_iter,
+// hip/HostMatrix/DeviceMatrix/DeviceMatrixCSR.cpp:281-304, kernel pairs
+// LMultCSR + lsolveiterCSR / UMultCSR + usolveiterCSR :14-58), one fused
+// kernel per sweep:  y_new = D (b - T_off y_old), D = I for L, 1/u_ii for U;
+// y_old == null is the first sweep from y = 0 (y_new = D b).  Off-diagonal
+// sums in ascending column order as the reference.  Thread per row.
+// ---------------------------------------------------------------------------
+template <bool UPPER>
+__global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep(int n, const int *__restrict__ rp,
+                                                        const int *__restrict__ ci,
+                                                        const double *__restrict__ lu,
+                                                        const int *__restrict__ dpos,
+                                                        const double *__restrict__ dinv,
+                                                        const double *__restrict__ b,
+                                                        const double *__restrict__ yo,
+                                                        double *__restrict__ yn,
+                                                        const xs_state *st) {
+  if (st != nullptr && st->done) return;
+  const int i = blockIdx.x * XS_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  double r = 0.0;
+  if (yo) {
+    const int e0 = UPPER ? dpos[i] + 1 : rp[i], e1 = UPPER ? rp[i + 1] : dpos[i];
+    for (int e = e0; e < e1; ++e) r += lu[e] * yo[ci[e]];
+  }
+  yn[i] = UPPER ? dinv[i] * (b[i] - r) : b[i] - r;
+}
+
+void xs_tri_sweep(hipStream_t s, const xs_ilu &F, bool upper, const double *b, const double *yo,
+                  double *yn, const xs_state *st) {
+  XS_GUARD(F.rp && F.ci && F.lu && F.dpos && b && yn && (!upper || F.dinv));
+  const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+  if (upper)
+    hipLaunchKernelGGL(k_tri_sweep<true>, grid, blk, 0, s, F.n, F.rp, F.ci, F.lu, F.dpos, F.dinv, b, yo, yn, st);
+  else
+    hipLaunchKernelGGL(k_tri_sweep<false>, grid, blk, 0, s, F.n, F.rp, F.ci, F.lu, F.dpos, F.dinv, b, yo, yn, st);
 }
 
 // values of a transposed pattern: dst[perm[k]] = src[k]

@@ -12,7 +12,7 @@
 // O(1) instead of O(k) global reductions per step.
 #include <cmath>
 
-#include "xs_internal.h"
+#include "xs_device.h"
 #include "xs_krylov.h"
 
 #define TS_ROWS 4  // rows per thread in the tall-skinny kernels
@@ -23,40 +23,43 @@ __device__ __forceinline__ double ts_wave_sum(double v) {
   return v;
 }
 
-__device__ __forceinline__ double ts_load_relaxed(const double *p) {
-  return __longlong_as_double((long long)__hip_atomic_load(
-      reinterpret_cast<unsigned long long *>(const_cast<double *>(p)), __ATOMIC_RELAXED,
-      __HIP_MEMORY_SCOPE_AGENT));
-}
-
-// Publish this block's k partial sums (red[j][wave]) and let the last block
-// reduce them in block order into out[0..k).
+// Publish this block's k partial sums (red[j][wave]) and let the block with
+// the highest index reduce them in block order into out[0..k).  Partial slots
+// hold XS_SENTINEL between reductions (the grid-reduction scheme of
+// reduce_finish in xs_device.h: agent-scope stores, no fence, no counter).
 __device__ __forceinline__ void ts_finish(int k, double (*red)[XS_BLOCK / XS_WAVE],
-                                          double *partial, unsigned *counter, double *out) {
-  __shared__ int is_last;
+                                          double *partial, unsigned *err, double *out) {
   for (int j = threadIdx.x; j < k; j += XS_BLOCK) {
     double a = 0.0;
 #pragma unroll
     for (int q = 0; q < XS_BLOCK / XS_WAVE; ++q) a += red[j][q];
-    partial[(size_t)blockIdx.x * k + j] = a;
+    st_ready(&partial[(size_t)blockIdx.x * k + j], a);
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned t = atomicAdd(counter, 1u);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
+  if (blockIdx.x != gridDim.x - 1) return;
   const int lane = threadIdx.x & (XS_WAVE - 1), wv = threadIdx.x / XS_WAVE;
   for (int j = wv; j < k; j += XS_BLOCK / XS_WAVE) {
     double a = 0.0;
-    for (unsigned b = lane; b < gridDim.x; b += XS_WAVE) a += ts_load_relaxed(&partial[(size_t)b * k + j]);
+    for (unsigned b0 = lane; b0 < gridDim.x; b0 += 4 * XS_WAVE) {
+      unsigned long long u[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const unsigned b = b0 + q * XS_WAVE;
+        u[q] = b < gridDim.x ? ld_relaxed(&partial[(size_t)b * k + j]) : 0ull;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const unsigned b = b0 + q * XS_WAVE;
+        if (b < gridDim.x) {
+          double *slot = &partial[(size_t)b * k + j];
+          a += u[q] == XS_SENTINEL ? ld_ready(slot, err) : __longlong_as_double((long long)u[q]);
+          __hip_atomic_store(reinterpret_cast<unsigned long long *>(slot), XS_SENTINEL,
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
     a = ts_wave_sum(a);
     if (lane == 0) out[j] = a;
   }
-  if (threadIdx.x == 0) *counter = 0u;
 }
 
 // out[j] = V_j . w  for j < k (k <= XS_TS_KMAX); T = double or float

@@ -12,6 +12,10 @@ void xs_launch_spmv_csr32(hipStream_t s, const xs_csr &A, const float *v32,
 void xs_launch_spmv_ell32(hipStream_t s, const xs_ell &E, const float *v32, const float *x,
                           float *y, int epi, const float *b, const float *w,
                           const xs_reduce *red);
+// one Jacobi sweep of an FP32/BF16/FP16 factor (yo == null: yn = D b)
+void xs_tri_sweep32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, const void *lu,
+                    int ftype, const int *dpos, const float *dinv, const float *b, const float *yo,
+                    float *yn, const xs_state *st);
 void xs_trsv32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, const void *lu,
                int ftype, const int *dpos, const float *dinv, const float *b, float *x,
                unsigned *tickets, const xs_state *st);

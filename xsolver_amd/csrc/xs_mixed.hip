@@ -160,6 +160,38 @@ __device__ __forceinline__ void st_ready32(float *p, float v) {
   __hip_atomic_store(reinterpret_cast<unsigned *>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// acc - sum of lu[e] * x[ci[e]] over up to 4 entries e, e+dir, ... (stopping
+// at `stop`): the loads are issued together, then only values still holding
+// the sentinel are waited for.
+template <class TF>
+__device__ __forceinline__ float trsv_gather4_32(float acc, int e, int stop, int dir,
+                                                 const int *__restrict__ ci,
+                                                 const TF *__restrict__ lu, const float *x,
+                                                 unsigned *err) {
+  unsigned u[4];
+  int c[4];
+  float a[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int eq = e + q * dir;
+    const bool in = dir > 0 ? eq < stop : eq > stop;
+    c[q] = in ? ci[eq] : -1;
+    a[q] = in ? fl(lu, eq) : 0.f;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    u[q] = c[q] >= 0 ? __hip_atomic_load(reinterpret_cast<unsigned *>(const_cast<float *>(x + c[q])),
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                     : 0u;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (c[q] < 0) break;
+    const float v = u[q] == XS_SENTINEL32 ? ld_ready32(x + c[q], err) : __uint_as_float(u[q]);
+    acc -= a[q] * v;
+  }
+  return acc;
+}
+
 template <bool UPPER, class TF>
 __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restrict__ rp,
                                                      const int *__restrict__ ci,
@@ -171,9 +203,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
   if (st != nullptr && st->done) return;
   unsigned *err = tickets + 4;
   const int lane = threadIdx.x & (XS_WAVE - 1);
-  unsigned w = 0;
-  if (lane == 0) w = atomicAdd(&tickets[0], 1u);
-  w = __shfl(w, 0, XS_WAVE);
+  const unsigned w = trsv_ticket(tickets);
   const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
   if (w < nw) {
     int r0, r1;
@@ -190,13 +220,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
         const int end = dpos[r];
         int xe = e;
         while (xe < end && ci[xe] < r0) ++xe;
-        for (; e + 4 <= xe; e += 4) {
-          const float x0 = ld_ready32(x + ci[e], err), x1 = ld_ready32(x + ci[e + 1], err);
-          const float x2 = ld_ready32(x + ci[e + 2], err), x3 = ld_ready32(x + ci[e + 3], err);
-          acc -= fl(lu, e) * x0; acc -= fl(lu, e + 1) * x1;
-          acc -= fl(lu, e + 2) * x2; acc -= fl(lu, e + 3) * x3;
-        }
-        for (; e < xe; ++e) acc -= fl(lu, e) * ld_ready32(x + ci[e], err);
+        for (; e < xe; e += 4) acc = trsv_gather4_32(acc, e, xe, 1, ci, lu, x, err);
         eb = xe; ee = end;
       } else {
         di = dinv[r];
@@ -204,13 +228,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
         int e = rp[r + 1] - 1;
         int xb = e + 1;
         while (xb > beg && ci[xb - 1] >= r1) --xb;
-        for (; e - 3 >= xb; e -= 4) {
-          const float x0 = ld_ready32(x + ci[e], err), x1 = ld_ready32(x + ci[e - 1], err);
-          const float x2 = ld_ready32(x + ci[e - 2], err), x3 = ld_ready32(x + ci[e - 3], err);
-          acc -= fl(lu, e) * x0; acc -= fl(lu, e - 1) * x1;
-          acc -= fl(lu, e - 2) * x2; acc -= fl(lu, e - 3) * x3;
-        }
-        for (; e >= xb; --e) acc -= fl(lu, e) * ld_ready32(x + ci[e], err);
+        for (; e >= xb; e -= 4) acc = trsv_gather4_32(acc, e, xb - 1, -1, ci, lu, x, err);
         eb = beg; ee = xb;
       }
     }
@@ -261,14 +279,47 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
     }
     if (act) st_ready32(x + r, val);
   }
-  if (lane == 0) {
-    const unsigned total = gridDim.x * (XS_BLOCK / XS_WAVE);
-    const unsigned f = atomicAdd(&tickets[1], 1u);
-    if (f == total - 1) {
-      __hip_atomic_store(&tickets[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&tickets[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+  trsv_retire(tickets);
+}
+
+// One Jacobi sweep of an FP32/BF16/FP16 factor with FP32 vectors (the
+// reference's iterative GPU ILU apply, see k_tri_sweep in xs_kernels.hip).
+template <bool UPPER, class TF>
+__global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep32(int n, const int *__restrict__ rp,
+                                                          const int *__restrict__ ci,
+                                                          const TF *__restrict__ lu,
+                                                          const int *__restrict__ dpos,
+                                                          const float *__restrict__ dinv,
+                                                          const float *__restrict__ b,
+                                                          const float *__restrict__ yo,
+                                                          float *__restrict__ yn,
+                                                          const xs_state *st) {
+  if (st != nullptr && st->done) return;
+  const int i = blockIdx.x * XS_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  float r = 0.f;
+  if (yo) {
+    const int e0 = UPPER ? dpos[i] + 1 : rp[i], e1 = UPPER ? rp[i + 1] : dpos[i];
+    for (int e = e0; e < e1; ++e) r += fl(lu, e) * yo[ci[e]];
   }
+  yn[i] = UPPER ? dinv[i] * (b[i] - r) : b[i] - r;
+}
+
+void xs_tri_sweep32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, const void *lu,
+                    int ftype, const int *dpos, const float *dinv, const float *b, const float *yo,
+                    float *yn, const xs_state *st) {
+  if (!(rp && ci && lu && dpos && b && yn && (!upper || dinv))) { xs_raise_launch_guard(); return; }
+  const dim3 grid((n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+#define XS_SW(TF)                                                                                 \
+  do {                                                                                            \
+    const TF *f = static_cast<const TF *>(lu);                                                    \
+    if (upper) hipLaunchKernelGGL((k_tri_sweep32<true, TF>), grid, blk, 0, s, n, rp, ci, f, dpos, dinv, b, yo, yn, st); \
+    else hipLaunchKernelGGL((k_tri_sweep32<false, TF>), grid, blk, 0, s, n, rp, ci, f, dpos, dinv, b, yo, yn, st); \
+  } while (0)
+  if (ftype == XS_BF16) XS_SW(__hip_bfloat16);
+  else if (ftype == XS_F16) XS_SW(__half);
+  else XS_SW(float);
+#undef XS_SW
 }
 
 __global__ void k_fill_u32(unsigned *p, int n, unsigned val) {

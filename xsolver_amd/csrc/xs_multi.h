@@ -38,8 +38,8 @@ enum xs_mfinish {
 };
 
 struct xs_mred {
-  double *partial;     // >= XS_MGRID * 2 * k doubles
-  unsigned *counter;   // last-block ticket (self-resetting)
+  double *partial;     // >= XS_MGRID * 2 * k doubles, XS_SENTINEL between reductions
+  unsigned *counter;   // raised if a partial never arrives (bounded wait)
   xs_mstate *ms;
   double *resvec;      // [(maxiter + 1) * k] absolute residuals, may be null
   double tol, abstol;
@@ -67,3 +67,5 @@ void xs_mtrsv(hipStream_t s, bool upper, int n, int k, const int *rp, const int 
 // column-major (k vectors of length n) <-> interleaved n x k
 void xs_mlayout(hipStream_t s, int n, int k, const double *in, double *out, bool to_interleaved);
 void xs_mstate_init(hipStream_t s, xs_mstate *ms, int k);
+// out = in over m doubles (a kernel, so it can sit in a captured graph)
+void xs_mcopy(hipStream_t s, size_t m, const double *in, double *out);

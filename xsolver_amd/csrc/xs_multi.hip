@@ -95,12 +95,14 @@ __device__ inline void m_finish(const xs_mred &R, int j, int k, const double *to
 }
 
 // Column sums of the workgroup's (row, column) values, published per
-// workgroup; the last workgroup of the grid combines the partials in a fixed
-// order (deterministic) and runs the finishing op for every column.
+// workgroup with agent-scope stores into slots that hold XS_SENTINEL between
+// reductions; the workgroup with the highest index waits for every slot,
+// combines the partials in a fixed order (deterministic), writes the sentinel
+// back and runs the finishing op for every column (the grid-reduction scheme
+// of reduce_finish, no fence and no counter).
 template <int NV>
 __device__ __forceinline__ void mcol_reduce(int k, const double (&v)[NV], const xs_mred &R) {
   __shared__ double red[NV][XS_BLOCK];
-  __shared__ int is_last;
   const int per = XS_BLOCK / k;
   for (int q = 0; q < NV; ++q) red[q][threadIdx.x] = v[q];
   __syncthreads();
@@ -108,27 +110,24 @@ __device__ __forceinline__ void mcol_reduce(int k, const double (&v)[NV], const 
     for (int q = 0; q < NV; ++q) {
       double s = 0.0;
       for (int r = 0; r < per; ++r) s += red[q][r * k + threadIdx.x];
-      R.partial[((size_t)blockIdx.x * NV + q) * k + threadIdx.x] = s;
+      st_ready(&R.partial[((size_t)blockIdx.x * NV + q) * k + threadIdx.x], s);
     }
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned t = atomicAdd(R.counter, 1u);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
+  if (blockIdx.x != gridDim.x - 1) return;
   const int j = threadIdx.x % k, g = threadIdx.x / k;
   double s[NV];
   for (int q = 0; q < NV; ++q) s[q] = 0.0;
   if (g < per) {
-    for (unsigned b = g; b < gridDim.x; b += per)
-      for (int q = 0; q < NV; ++q)
-        s[q] += __longlong_as_double((long long)__hip_atomic_load(
-            reinterpret_cast<unsigned long long *>(&R.partial[((size_t)b * NV + q) * k + j]),
-            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    for (unsigned b = g; b < gridDim.x; b += per) {
+      unsigned long long u[NV];
+      for (int q = 0; q < NV; ++q) u[q] = ld_relaxed(&R.partial[((size_t)b * NV + q) * k + j]);
+      for (int q = 0; q < NV; ++q) {
+        double *slot = &R.partial[((size_t)b * NV + q) * k + j];
+        s[q] += u[q] == XS_SENTINEL ? ld_ready(slot, R.counter) : __longlong_as_double((long long)u[q]);
+        __hip_atomic_store(reinterpret_cast<unsigned long long *>(slot), XS_SENTINEL,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   }
   __syncthreads();
   for (int q = 0; q < NV; ++q) red[q][threadIdx.x] = s[q];
@@ -144,7 +143,6 @@ __device__ __forceinline__ void mcol_reduce(int k, const double (&v)[NV], const 
     int all = 1;
     for (int c = 0; c < k; ++c) all &= (R.ms->done[c] != 0);
     R.ms->all = all;
-    *R.counter = 0u;
   }
 }
 
@@ -347,6 +345,12 @@ __global__ void k_mfill(size_t m, unsigned long long *__restrict__ p, unsigned l
     p[q] = val;
 }
 
+__global__ void k_mcopy(size_t m, const double *__restrict__ in, double *__restrict__ out) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < m;
+       q += (size_t)gridDim.x * blockDim.x)
+    out[q] = in[q];
+}
+
 // column-major (in[j*n + i]) <-> interleaved (out[i*k + j])
 __global__ void k_mlayout(int n, int k, const double *__restrict__ in, double *__restrict__ out,
                           int to_interleaved) {
@@ -391,9 +395,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
   if (ms && ms->all) return;
   unsigned *err = tickets + 4;
   const int lane = threadIdx.x & (XS_WAVE - 1);
-  unsigned w = 0;
-  if (lane == 0) w = atomicAdd(&tickets[0], 1u);
-  w = __shfl(w, 0, XS_WAVE);
+  const unsigned w = trsv_ticket(tickets);
   const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
   if (w < nw) {
     int r0, r1;
@@ -497,14 +499,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
       for (int q = 0; q < KC; ++q) st_ready(xr + q, acc[q]);
     }
   }
-  if (lane == 0) {
-    const unsigned total = gridDim.x * (XS_BLOCK / XS_WAVE);
-    const unsigned f = atomicAdd(&tickets[1], 1u);
-    if (f == total - 1) {
-      __hip_atomic_store(&tickets[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&tickets[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+  trsv_retire(tickets);
 }
 
 // ---------------------------------------------------------------------------
@@ -593,6 +588,10 @@ void xs_mlayout(hipStream_t s, int n, int k, const double *in, double *out, bool
   XS_GUARD(in && out && k_ok(k));
   hipLaunchKernelGGL(k_mlayout, dim3(mgrid_flat((size_t)n * k)), dim3(XS_BLOCK), 0, s, n, k, in,
                      out, to_interleaved ? 1 : 0);
+}
+void xs_mcopy(hipStream_t s, size_t m, const double *in, double *out) {
+  XS_GUARD(in && out);
+  hipLaunchKernelGGL(k_mcopy, dim3(mgrid_flat(m)), dim3(XS_BLOCK), 0, s, m, in, out);
 }
 void xs_mstate_init(hipStream_t s, xs_mstate *ms, int k) {
   XS_GUARD(ms && k_ok(k));

@@ -37,8 +37,11 @@ int xs_multi_reserve(xs_ctx *c, int k, int extra_vecs) {
   }
   if (!c->ms) XS_CHK(hipMalloc(&c->ms, sizeof(xs_mstate)));
   if (!c->h_ms) XS_CHK(hipHostMalloc(&c->h_ms, sizeof(xs_mstate), hipHostMallocDefault));
-  if (!c->mpartial)
+  if (!c->mpartial) {
     XS_CHK(hipMalloc((void **)&c->mpartial, sizeof(double) * XS_MGRID * 2 * XS_MAX_RHS));
+    // reduction slots hold the sentinel between reductions
+    xs_fill_sentinel(c->stream, c->mpartial, XS_MGRID * 2 * XS_MAX_RHS);
+  }
   const size_t rv = (size_t)(c->maxiter + 1) * (size_t)k;
   if (c->mresvec_len < rv) {
     if (c->mresvec) (void)hipFree(c->mresvec);
@@ -100,7 +103,7 @@ struct multi_run {
                  out, c->counters + 4, ms);
         break;
       default:
-        (void)hipMemcpyAsync(out, in, sizeof(double) * (size_t)n * k, hipMemcpyDeviceToDevice, s);
+        xs_mcopy(s, (size_t)n * k, in, out);
     }
   }
   // launch `iteration` until every column is done or maxiter were launched
@@ -110,9 +113,10 @@ struct multi_run {
     XS_CHK(hipMemcpyAsync(h, ms, sizeof(xs_mstate), hipMemcpyDeviceToHost, s));
     XS_CHK(hipStreamSynchronize(s));
     int launched = 0;
+    xs_graph_cache gc;
     while (!h->all && launched < c->maxiter) {
       const int step = std::min(c->check_every, c->maxiter - launched);
-      for (int j = 0; j < step; j++) iteration();
+      if (xs_run_batch(c, gc, step, iteration)) return XS_ERR_DEVICE;
       launched += step;
       XS_CHK(hipGetLastError());
       XS_CHK(hipMemcpyAsync(h, ms, sizeof(xs_mstate), hipMemcpyDeviceToHost, s));

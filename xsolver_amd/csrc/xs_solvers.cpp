@@ -27,9 +27,10 @@ static int drive(xs_ctx *c, F iteration) {
   XS_CHK(hipMemcpyAsync(c->h_state, c->st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
   XS_CHK(hipStreamSynchronize(s));
   int launched = 0;
+  xs_graph_cache gc;
   while (!c->h_state->done && launched < c->maxiter) {
     const int k = std::min(c->check_every, c->maxiter - launched);
-    for (int j = 0; j < k; j++) iteration();
+    XS_CHK(xs_run_batch(c, gc, k, iteration) == 0 ? hipSuccess : hipErrorUnknown);
     launched += k;
     XS_CHK(hipGetLastError());
     XS_CHK(hipMemcpyAsync(c->h_state, c->st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
@@ -122,9 +123,14 @@ static int ensure_dev(double **p, size_t *len, size_t need) {
   return XS_OK;
 }
 
+// reduction slots start out (and are left) holding the sentinel
 static int ensure_ts(xs_ctx *c) {
   const size_t need = (size_t)std::max(xs_ts_blocks(c->A.n), XS_MAX_REDUCE_BLOCKS) * XS_TS_KMAX;
-  return ensure_dev(&c->ts_partial, &c->ts_partial_len, need);
+  if (c->ts_partial && c->ts_partial_len >= need) return XS_OK;
+  int rc = ensure_dev(&c->ts_partial, &c->ts_partial_len, need);
+  if (rc) return rc;
+  xs_fill_sentinel(c->stream, c->ts_partial, (int)c->ts_partial_len);
+  return XS_OK;
 }
 
 static xs_gmres_small carve_small(double *base, int m) {
@@ -382,16 +388,38 @@ static void op_spmv32(xs_ctx *c, const float *x, float *y, int epi, const float 
   xs_prof_end(c, 1);
 }
 
+// FP32 counterpart of the host's ilu_sweeps (k Jacobi sweeps of one factor)
+static void ilu_sweeps32(xs_ctx *c, bool upper, int k, const float *b, float *out, float *w,
+                         const xs_state *st) {
+  const int n = c->A.n, m = k < 1 ? 1 : k;
+  float *cur = ((m - 1) % 2 == 0) ? out : w;
+  xs_tri_sweep32(c->stream, upper, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type, c->ilu.dpos,
+                 c->dinv32, b, nullptr, cur, st);
+  for (int j = 2; j <= m; ++j) {
+    float *nxt = (cur == out) ? w : out;
+    xs_tri_sweep32(c->stream, upper, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type,
+                   c->ilu.dpos, c->dinv32, b, cur, nxt, st);
+    cur = nxt;
+  }
+}
+
 static void op_pc32(xs_ctx *c, const float *x, float *y, float *tmp, const xs_state *st) {
   const int n = c->A.n;
   if (c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) {
+    float *w = c->work32 + 11 * (size_t)n;   // sweep scratch
     xs_prof_begin(c, 2);
-    xs_trsv32(c->stream, false, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type, c->ilu.dpos,
-              c->dinv32, x, tmp, c->counters + 2, st);
+    if (c->lsweeps >= 0)
+      ilu_sweeps32(c, false, c->lsweeps, x, tmp, w, st);
+    else
+      xs_trsv32(c->stream, false, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type, c->ilu.dpos,
+                c->dinv32, x, tmp, c->counters + 2, st);
     xs_prof_end(c, 2);
     xs_prof_begin(c, 3);
-    xs_trsv32(c->stream, true, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type, c->ilu.dpos,
-              c->dinv32, tmp, y, c->counters + 4, st);
+    if (c->usweeps >= 0)
+      ilu_sweeps32(c, true, c->usweeps, tmp, y, w, st);
+    else
+      xs_trsv32(c->stream, true, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type, c->ilu.dpos,
+                c->dinv32, tmp, y, c->counters + 4, st);
     xs_prof_end(c, 3);
   } else if (c->pc == XS_PC_JACOBI) {
     xs_scale32(c->stream, n, c->jac32, x, y);
@@ -423,17 +451,19 @@ static int inner_bicgstab32(xs_ctx *c, const float *rhs, float *d, int maxit, in
   xs_state h{};
   XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
   XS_CHK(hipStreamSynchronize(s));
+  auto iteration = [&]() {
+    xs_bicg_p32(s, n, r, u, p, jac, z, st);
+    if (!jac) op_pc32(c, p, z, t, st);
+    op_spmv32(c, z, u, EPI_DOT1, nullptr, rh, &rg);
+    xs_bicg_s32(s, n, r, u, sv, jac, z1, st);
+    if (!jac) op_pc32(c, sv, z1, t, st);
+    op_spmv32(c, z1, v, EPI_DOT2, nullptr, sv, &ra);
+    xs_bicg_xr32(s, n, d, r, z, z1, sv, v, rh, rr);
+  };
+  xs_graph_cache gc;
   while (!h.done && launched < maxit) {
     const int k = std::min(c->check_every, maxit - launched);
-    for (int j = 0; j < k; j++) {
-      xs_bicg_p32(s, n, r, u, p, jac, z, st);
-      if (!jac) op_pc32(c, p, z, t, st);
-      op_spmv32(c, z, u, EPI_DOT1, nullptr, rh, &rg);
-      xs_bicg_s32(s, n, r, u, sv, jac, z1, st);
-      if (!jac) op_pc32(c, sv, z1, t, st);
-      op_spmv32(c, z1, v, EPI_DOT2, nullptr, sv, &ra);
-      xs_bicg_xr32(s, n, d, r, z, z1, sv, v, rh, rr);
-    }
+    XS_CHK(xs_run_batch(c, gc, k, iteration) == 0 ? hipSuccess : hipErrorUnknown);
     launched += k;
     XS_CHK(hipGetLastError());
     XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));

@@ -28,7 +28,15 @@ struct xs_ctx {
   bool parilu_async = true;
   bool parilu_scale = true;
   int parilut_steps = 10;       // hip/Precond/precon_gpuilut.cpp:5
+  // ILU application: -1 = exact sync-free triangular solve; k >= 0 = k Jacobi
+  // sweeps per factor (the reference's GPU apply: 20/20 for ILU(0),
+  // hip/Precond/precon_gpuilu0.cpp:17-18; 10/15 for ILUT, precon_gpuilut.cpp:19-20)
+  int lsweeps = -1, usweeps = -1;
   int check_every = 8;
+  bool tuned = false;           // SpMV format chosen by the autotuner
+  int tune_chunk = 0;
+  double tune_us = 0.0;
+  bool use_graph = true;        // replay batches of iterations as HIP graphs
   int precision = 0;
   int use_ell = 0;
 
@@ -115,6 +123,44 @@ struct xs_ctx {
 };
 
 int xs_ctx_ensure_work(xs_ctx *c, int nvec);
+
+// Replay of a batch of Krylov iterations as a HIP graph.  The first full
+// batch (check_every iterations) of a solve is captured from the stream into
+// a graph and every later full batch of that solve replays it: one graph
+// launch instead of ~10 kernel launches per iteration.  A batch contains
+// kernels only (no memset/memcpy nodes).  Off while kernel timing is on, and
+// whenever capture fails (then the batch runs eagerly).
+struct xs_graph_cache {
+  hipGraphExec_t exec = nullptr;
+  bool failed = false;
+  ~xs_graph_cache() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
+template <class F>
+int xs_run_batch(xs_ctx *c, xs_graph_cache &gc, int count, F &iteration) {
+  const bool want = c->use_graph && c->prof_kind == 0 && !gc.failed && count == c->check_every;
+  if (want && !gc.exec) {
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      for (int j = 0; j < count; j++) iteration();
+      const hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (e != hipSuccess || !g || hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        gc.exec = nullptr;
+        gc.failed = true;
+      }
+      if (g) (void)hipGraphDestroy(g);
+    } else {
+      gc.failed = true;
+    }
+    (void)hipGetLastError();
+  }
+  if (want && gc.exec)
+    return hipGraphLaunch(gc.exec, c->stream) == hipSuccess ? 0 : -4;
+  for (int j = 0; j < count; j++) iteration();
+  return 0;
+}
 // kernel timing helpers: bracket one launch of kernel family `kind`
 void xs_prof_begin(xs_ctx *c, int kind);
 void xs_prof_end(xs_ctx *c, int kind);
