@@ -93,6 +93,23 @@ def run_gpu(A, b, fmt, steps, warmup, profile=True, precision=0, factor_storage=
     }
 
 
+# kernel families timed by the library -> kernel names in the rocprofv3 summary
+PROFILE_NAMES = {"spmv": "k_spmv_csr<2>", "trsv_lower": "k_trsv<false>",
+                 "trsv_upper": "k_trsv<true>", "spmv_ell": "k_spmv_ell<2>"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the 128^3 workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "convdiff3d_128_hbm_traffic.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block).get(PROFILE_NAMES.get(kernel, ""))
+    except (OSError, IndexError, ValueError):
+        return None
+
+
 def cpu_baseline(A, b, iters_needed, sample_iters=8):
     """Oracle (C restatement, 1 core): ILU(0) set-up plus `sample_iters`
     BiCGSTAB iterations on the same system, scaled to `iters_needed`."""
@@ -173,8 +190,9 @@ def main():
     if dom:
         bytes_per = by["spmv_csr"] if dom == "spmv" else by[dom]
         achieved = bytes_per / (k[dom]["avg_us"] * 1e-6) / 1e9
+        traffic = measured_traffic(dom) if N == 128 else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": dom, "bytes_per_launch": bytes_per,
                 "avg_us": round(k[dom]["avg_us"], 2)}
     kernels_gbs = {}

@@ -120,6 +120,13 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *   "profile"        bit mask of kernel families timed with HIP events
  *                    (2 SpMV, 4 lower solve, 8 upper solve); read back with
  *                    xs_get_time_ms("total:spmv"|"count:spmv"|...)
+ *   "lsolve_sweeps", "usolve_sweeps"
+ *                    -1 (default) = exact sync-free triangular solve; k >= 0 =
+ *                    k Jacobi sweeps per factor, the reference's GPU ILU apply
+ *                    (20/20 for ILU(0), precon_gpuilu0.cpp:17-18; 10/15 for
+ *                    ILUT, precon_gpuilut.cpp:19-20)
+ *   "trsv_schedule"  1 (default) = hand out the triangular-solve row groups in
+ *                    dependency-level order, 0 = row order
  *   "use_ell"        1 = ELL SpMV (same as advanced_pam[1] = 1) */
 int xs_set_option(xs_ctx *ctx, const char *name, double value);
 

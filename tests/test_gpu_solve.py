@@ -308,3 +308,35 @@ def test_iterative_ilu_apply_solve(dev, precision):
         assert np.linalg.norm(g["x"] - xo) / np.linalg.norm(xo) < 50 * 1e-10
     assert true_relres(A, b, g["x"]) < 1.05e-10
     g["S"].close()
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_update_values_then_solve(dev, precision):
+    """xs_mat_update_values (the reference's *_mat_update): new values on the
+    same pattern are used by the SpMV, the refactorised ILU and, in mixed
+    precision, the FP32 copies."""
+    A = gen.convdiff3d(16)
+    rp, ci, v = A
+    S = XSolver()
+    S.mat_setup(*A)
+    S.ksp_settype("bicgstab"); S.ksp_setoption(1e-10, 1000); S.pc_settype("ilu0")
+    S.set_option("precision", precision)
+    _, b = gen.manufactured(A, 1)
+    x1, st1 = S.solve(b)
+    assert st1 == 0 and true_relres(A, b, x1) < 1.05e-10
+    v2 = v.copy()
+    for i in range(len(rp) - 1):                  # heavier diagonal, new values
+        seg = slice(rp[i], rp[i + 1])
+        d = rp[i] + int(np.where(ci[seg] == i)[0][0])
+        v2[d] *= 1.5
+    A2 = (rp, ci, v2)
+    import ctypes
+    assert xsolver_amd.lib().xs_mat_update_values(S.h, v2.ctypes.data_as(ctypes.c_void_p), 0) == 0
+    _, b2 = gen.manufactured(A2, 2)
+    x2, st2 = S.solve(b2)
+    xo, sto, ito, _ = oracle.solve(A2, b2, "bicgstab", "ilu0", tol=1e-10, maxiter=1000)
+    say(f"[update] precision={precision} it={S.iterations} oracle it={ito}")
+    assert st2 == 0 and true_relres(A2, b2, x2) < 1.05e-10
+    if precision == 0:
+        assert abs(S.iterations - ito) <= 2
+    S.close()

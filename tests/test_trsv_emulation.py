@@ -1,9 +1,10 @@
 """CPU emulation of the sync-free triangular-solve kernel's per-wavefront
-algorithm (xsolver_amd/csrc/xs_kernels.hip, k_trsv): 64 rows per ticket,
-dependencies before the wave's first row (lower) / after its last row (upper)
-read from the solution vector, dependencies inside the wave resolved by the
-log-step affine scan when every row depends at most on its neighbour, by the
-64-step broadcast loop otherwise.  Waves are emulated in ticket order, which
+algorithm (xsolver_amd/csrc/xs_kernels.hip, k_trsv): 64 rows per group, lanes
+along the solve order (upper solves: lane l holds row r1-1-l), dependencies
+outside the group read from the solution vector, dependencies inside the
+group resolved by the DPP affine scan (row shifts 1/2/4/8, then the
+row_bcast:15 / row_bcast:31 steps) when every row depends at most on the
+previous one, by the 64-step broadcast loop otherwise.  Waves are emulated in ticket order, which
 is the order the kernel's waits resolve in.  Checked against the oracle's
 sequential solves (oracle/xs_oracle.c)."""
 import numpy as np
@@ -27,16 +28,16 @@ def emulate(rp, ci, lu, dpos, dinv, b, upper):
         else:
             r1 = n - w * W
             r0 = max(0, r1 - W)
+        nrows = r1 - r0
+        # lanes along the solve order: upper solves run rows backwards
+        row = [(r1 - 1 - l) if upper else (r0 + l) for l in range(W)]
         acc = np.zeros(W)
         di = np.ones(W)
         eb = np.zeros(W, int)
         ee = np.zeros(W, int)
-        act = np.zeros(W, bool)
-        for lane in range(W):
-            r = r0 + lane
-            if r >= r1:
-                continue
-            act[lane] = True
+        act = np.arange(W) < nrows
+        for lane in range(nrows):
+            r = row[lane]
             acc[lane] = b[r]
             if not upper:
                 e, end = rp[r], dpos[r]
@@ -60,52 +61,45 @@ def emulate(rp, ci, lu, dpos, dinv, b, upper):
                 eb[lane], ee[lane] = beg, xb
         nint = ee - eb
         chain = all((not act[l]) or nint[l] == 0 or
-                    (nint[l] == 1 and ci[eb[l]] == (r0 + l + (1 if upper else -1)))
+                    (nint[l] == 1 and ci[eb[l]] == row[l] + (1 if upper else -1))
                     for l in range(W))
         if chain:
+            # inclusive affine scan in lane order (the kernel's DPP steps:
+            # shifts 1, 2, 4, 8 inside 16-lane rows, then row broadcasts)
             paths.add("scan")
-            if not upper:
-                c = acc.copy()
-                m = np.array([-lu[eb[l]] if act[l] and nint[l] == 1 else 0.0 for l in range(W)])
-            else:
-                c = np.where(act, di * acc, 0.0)
-                m = np.array([-di[l] * lu[eb[l]] if act[l] and nint[l] == 1 else 0.0 for l in range(W)])
-            d = 1
-            while d < W:
-                if not upper:
-                    cj = np.concatenate([c[:d], c[:-d]]); mj = np.concatenate([m[:d], m[:-d]])
-                    has = np.arange(W) >= d
-                else:
-                    cj = np.concatenate([c[d:], c[-d:]]); mj = np.concatenate([m[d:], m[-d:]])
-                    has = np.arange(W) + d < W
-                c = np.where(has, c + m * cj, c)
-                m = np.where(has, m * mj, m)
-                d <<= 1
+            c = np.where(act, di * acc, 0.0)
+            m = np.array([-di[l] * lu[eb[l]] if act[l] and nint[l] == 1 else 0.0 for l in range(W)])
+            lanes = np.arange(W)
+            for d in (1, 2, 4, 8):
+                src = lanes - d
+                ok = (lanes % 16) >= d
+                cj = np.where(ok, c[np.clip(src, 0, W - 1)], 0.0)
+                mj = np.where(ok, m[np.clip(src, 0, W - 1)], 1.0)
+                c, m = c + m * cj, m * mj
+            for srclane, rows_in in ((15, (1,)), (47, (3,)), (31, (2, 3))):
+                ok = np.isin(lanes // 16, rows_in)
+                if srclane == 15:
+                    ok = ok & (lanes // 16 == 1)
+                cj = np.where(ok, c[srclane], 0.0)
+                mj = np.where(ok, m[srclane], 1.0)
+                c, m = c + m * cj, m * mj
             val = c
         else:
             paths.add("loop")
-            nrows = r1 - r0
-            if not upper:
-                e = eb.copy()
-                for k in range(nrows):
-                    xk = acc[k]
-                    for l in range(W):
-                        if act[l] and e[l] < ee[l] and ci[e[l]] == r0 + k:
-                            acc[l] -= lu[e[l]] * xk
-                            e[l] += 1
-                val = acc
-            else:
-                e = ee - 1
-                for k in range(nrows - 1, -1, -1):
-                    xk = acc[k] * di[k]
-                    for l in range(W):
-                        if act[l] and e[l] >= eb[l] and ci[e[l]] == r0 + k:
-                            acc[l] -= lu[e[l]] * xk
-                            e[l] -= 1
-                val = acc * di
-        for l in range(W):
-            if act[l]:
-                x[r0 + l] = val[l]
+            e = np.where(upper, ee - 1, eb)
+            for k in range(nrows):
+                xk = acc[k] * di[k]
+                rk = row[k]
+                for l in range(nrows):
+                    if nint[l] == 0:
+                        continue
+                    inside = (e[l] >= eb[l]) if upper else (e[l] < ee[l])
+                    if inside and ci[e[l]] == rk:
+                        acc[l] -= lu[e[l]] * xk
+                        e[l] += -1 if upper else 1
+            val = acc * di
+        for l in range(nrows):
+            x[row[l]] = val[l]
     return x, paths
 
 

@@ -67,6 +67,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.ucp); dfree(F.uri); dfree(F.upos);
   dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own); dfree(F.trp); dfree(F.tci);
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
+  dfree(F.lsched); dfree(F.usched);
   F = xs_ilu();
   dfree(c->lu_tmp);
   c->pc_ready = false;
@@ -472,6 +473,7 @@ extern "C" int xs_mat_update_values(xs_ctx *c, const double *val, int on_device)
   if (c->use_ell) xs_build_ell(c->stream, c->A, c->ell.W, c->ell);
   if (c->have_At) xs_permute_vals(c->stream, c->A.nnz, c->perm_At, c->A.v, c->At.v);
   c->pc_ready = false;
+  c->mixed_ready = false;
   XS_HIP(hipStreamSynchronize(c->stream));
   return XS_OK;
 }
@@ -531,6 +533,7 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   else if (k == "parilu_async") { c->parilu_async = value != 0; c->pc_ready = false; }
   else if (k == "check_every") { if (value < 1) return XS_ERR_ARG; c->check_every = (int)value; }
   else if (k == "use_graph") { c->use_graph = value != 0; }
+  else if (k == "trsv_schedule") { c->trsv_schedule = value != 0; c->pc_ready = false; free_precond(c); }
   else if (k == "lsolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->lsweeps = (int)value; }
   else if (k == "usolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->usweeps = (int)value; }
   else if (k == "refactor") { c->pc_ready = false; c->mixed_ready = false; }
@@ -560,6 +563,70 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
 // ---------------------------------------------------------------------------
 // preconditioner set-up
 // ---------------------------------------------------------------------------
+// SpTRSV work order (no reference counterpart).  The sync-free solve hands
+// out 64-row groups in ticket order and a resident wave spins until its
+// dependencies are published; with groups taken in row order, the resident
+// waves (at most ~8k on the chip) fill up with groups whose dependencies are
+// far from ready -- on the 128^3 7-point factors only ~32 planes fit, so
+// the solve takes ~2x its dependency depth in hops.  Ordering the groups by
+// dependency level (longest chain of group dependencies below them, from
+// the factor pattern once per pattern) keeps the resident waves on the
+// dependency frontier.  Every dependency of a group has a smaller level, so
+// it still holds an earlier ticket: the no-deadlock argument is unchanged.
+// This is an order for the tickets only -- no level-synchronous execution.
+static void group_order(int n, const int *rp, const int *ci, const int *dpos, bool upper,
+                        std::vector<int> &order) {
+  const int G = (n + XS_WAVE - 1) / XS_WAVE;
+  std::vector<int> lev(G, 0);
+  int maxlev = 0;
+  for (int g = 0; g < G; ++g) {
+    int r0, r1;
+    if (!upper) { r0 = g * XS_WAVE; r1 = std::min(n, r0 + XS_WAVE); }
+    else { r1 = n - g * XS_WAVE; r0 = std::max(0, r1 - XS_WAVE); }
+    int L = 0;
+    for (int r = r0; r < r1; ++r) {
+      if (!upper) {
+        for (int e = rp[r]; e < dpos[r]; ++e) {
+          const int col = ci[e];
+          if (col >= r0) break;
+          L = std::max(L, lev[col / XS_WAVE] + 1);
+        }
+      } else {
+        for (int e = rp[r + 1] - 1; e > dpos[r]; --e) {
+          const int col = ci[e];
+          if (col < r1) break;
+          L = std::max(L, lev[(n - 1 - col) / XS_WAVE] + 1);
+        }
+      }
+    }
+    lev[g] = L;
+    maxlev = std::max(maxlev, L);
+  }
+  std::vector<int> start(maxlev + 2, 0);
+  for (int g = 0; g < G; ++g) start[lev[g] + 1]++;
+  for (int l = 0; l <= maxlev; ++l) start[l + 1] += start[l];
+  order.resize(G);
+  for (int g = 0; g < G; ++g) order[start[lev[g]]++] = g;
+}
+
+// host copies of a factor pattern -> c->ilu.lsched / usched
+static int build_trsv_schedules(xs_ctx *c, const int *rp, const int *ci, const int *dpos) {
+  xs_ilu &F = c->ilu;
+  dfree(F.lsched);
+  dfree(F.usched);
+  if (!c->trsv_schedule) return XS_OK;
+  const int n = c->A.n, G = (n + XS_WAVE - 1) / XS_WAVE;
+  std::vector<int> ord;
+  int rc;
+  group_order(n, rp, ci, dpos, false, ord);
+  if ((rc = dalloc(&F.lsched, G))) return rc;
+  XS_HIP(hipMemcpy(F.lsched, ord.data(), sizeof(int) * G, hipMemcpyHostToDevice));
+  group_order(n, rp, ci, dpos, true, ord);
+  if ((rc = dalloc(&F.usched, G))) return rc;
+  XS_HIP(hipMemcpy(F.usched, ord.data(), sizeof(int) * G, hipMemcpyHostToDevice));
+  return XS_OK;
+}
+
 static int setup_ilu_pattern(xs_ctx *c) {
   // Combined-factor layout on A's pattern plus the CSC of the upper triangle
   // used by the ParILU sweep (the reference builds L CSR / U CSC with
@@ -600,7 +667,7 @@ static int setup_ilu_pattern(xs_ctx *c) {
   XS_HIP(hipMemcpy(F.ucp, ucnt.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
   XS_HIP(hipMemcpy(F.uri, uri.data(), sizeof(int) * nu, hipMemcpyHostToDevice));
   XS_HIP(hipMemcpy(F.upos, upos.data(), sizeof(int) * nu, hipMemcpyHostToDevice));
-  return XS_OK;
+  return build_trsv_schedules(c, rp.data(), ci.data(), dpos.data());
 }
 
 // ParILU(0) (hip/Precond/gpuparilu.cpp:150-165, 20 sweeps by default).  With
@@ -773,7 +840,12 @@ int xs_parilut_factor(xs_ctx *c) {
 #undef PT_TRY
   if (ov) return XS_ERR_MEMORY;
   if (bad != 0x7f7f7f7f) return XS_ERR_PIVOT;
-  return XS_OK;
+  // SpTRSV order of the new pattern
+  std::vector<int> hrp(n + 1), hci(fnnz), hdp(n);
+  XS_HIP(hipMemcpy(hrp.data(), F.rp_own, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  XS_HIP(hipMemcpy(hci.data(), F.ci_own, sizeof(int) * fnnz, hipMemcpyDeviceToHost));
+  XS_HIP(hipMemcpy(hdp.data(), F.dpos, sizeof(int) * n, hipMemcpyDeviceToHost));
+  return build_trsv_schedules(c, hrp.data(), hci.data(), hdp.data());
 }
 
 extern "C" int xs_pc_setup(xs_ctx *c) {
@@ -811,6 +883,7 @@ extern "C" int xs_pc_setup(xs_ctx *c) {
   if (xs_take_launch_guard()) rc = XS_ERR_STATE;
   if (rc == XS_OK) c->pc_ready = true;
   c->ilu.have_t = false;
+  c->mixed_ready = false;   // FP32 copies of A and of the factors are rebuilt
   return rc;
 }
 
@@ -1300,9 +1373,9 @@ extern "C" int xs_pc_apply_multi(xs_ctx *c, int nrhs, const double *x, double *y
       if ((rc = xs_multi_reserve(c, nrhs, 0))) return rc;
       double *tmp = xs_multi_vec(c, nrhs, 8);
       xs_mtrsv(s, false, n, nrhs, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, nullptr, x, tmp,
-               c->counters + 2, nullptr);
+               c->counters + 2, c->ilu.lsched, nullptr);
       xs_mtrsv(s, true, n, nrhs, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, c->ilu.dinv, tmp, y,
-               c->counters + 4, nullptr);
+               c->counters + 4, c->ilu.usched, nullptr);
       break;
     }
     default:

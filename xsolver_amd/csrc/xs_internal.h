@@ -95,6 +95,9 @@ struct xs_ilu {
   // CSC of the upper triangle (incl. diagonal) for the ParILU sweep:
   int *ucp = nullptr, *uri = nullptr, *upos = nullptr;
   int *rowidx = nullptr;    // row of each nnz (COO)
+  // SpTRSV work order: the 64-row groups of L (resp. U, counted from the
+  // bottom) sorted by dependency level; ticket t solves group sched[t]
+  int *lsched = nullptr, *usched = nullptr;
   // transposed factors for the adjoint (built on demand)
   bool have_t = false;
   int *trp = nullptr, *tci = nullptr, *tdpos = nullptr, *tperm = nullptr;

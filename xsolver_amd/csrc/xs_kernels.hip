@@ -571,26 +571,31 @@ void xs_ilu_dinv(hipStream_t s, xs_ilu &F, int *d_bad) {
 // ---------------------------------------------------------------------------
 // Sync-free triangular solves (no level sets).
 //
-// Each wavefront takes a ticket from a global counter and owns the 64
-// consecutive rows of that ticket (in solve order), one row per lane, so a
-// wave only ever waits on rows owned by waves that started before it: every
-// wait ends.  The solution vector itself carries the per-row ready flag: it
-// is pre-filled with a signalling-NaN sentinel, each row's value is published
-// with an agent-scope atomic store, and a dependent row spins (with s_sleep)
-// on an agent-scope atomic load until the sentinel is gone.
+// Work is handed out in 64-row groups, one row per lane: each workgroup takes
+// a ticket from a global counter and its waves solve the groups of that
+// ticket (tickets map to groups in dependency-level order when a schedule is
+// given, in row order otherwise), so a wave only ever waits on rows of waves
+// that started before it or sit in its workgroup: every wait ends.
+// The solution vector itself carries the per-row ready flag: it is
+// pre-filled with a signalling-NaN sentinel, each row's value is published
+// with an agent-scope atomic store, and a dependent row polls (s_sleep
+// between polls) with agent-scope atomic loads until the sentinel is gone;
+// all pending values of a row are re-polled together, one round trip per
+// poll.
 // The diagonal is unit when dinv is null (the L factor, and L^T in the
 // transposed solve) and 1/u_ii otherwise (U, and U^T).
-// Dependencies inside a wave are resolved in registers: a log-step affine scan
-// when each row depends at most on its neighbour (x_i = c_i + m_i x_{i-1}), a
-// 64-step broadcast loop otherwise.
-// A second counter lets the last wave reset both tickets for the next call.
+// Lanes run along the solve order (upper solves: lane l holds row r1-1-l), so
+// dependencies inside a wave always point to lower lanes.  They are resolved
+// in registers: a DPP affine scan when each row depends at most on the
+// previous one (x_i = c_i + m_i x_{i-1}), a 64-step broadcast loop otherwise.
+// The last workgroup to retire resets the tickets for the next call.
 // Reference kernels replaced: Lcsr_trsv_sync_free / Ucsc_trsv_sync_free
 // (hip/Precond/gpuparilu.cpp:257-391): there one wave per row spins on a
 // separate volatile int flag; the backward solve scatters with double atomics.
 // ---------------------------------------------------------------------------
 // acc - sum of lu[e] * x[ci[e]] over up to 4 entries e, e+dir, ... (stopping
-// at `stop`), in that order: the 4 loads are issued together, then only the
-// values still holding the sentinel are waited for.
+// at `stop`), subtracted in that order: the 4 loads are issued together and
+// the values still holding the sentinel are re-polled together.
 __device__ __forceinline__ double trsv_gather4(double acc, int e, int stop, int dir,
                                                const int *__restrict__ ci,
                                                const double *__restrict__ lu, const double *x,
@@ -607,13 +612,27 @@ __device__ __forceinline__ double trsv_gather4(double acc, int e, int stop, int 
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) u[q] = c[q] >= 0 ? ld_relaxed(x + c[q]) : 0ull;
+  unsigned polls = 0;
+  for (;;) {
+    bool pend = false;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    if (c[q] < 0) break;
-    const double v = u[q] == XS_SENTINEL ? ld_ready(x + c[q], err)
-                                         : __longlong_as_double((long long)u[q]);
-    acc -= a[q] * v;
+    for (int q = 0; q < 4; ++q) pend |= (c[q] >= 0 && u[q] == XS_SENTINEL);
+    if (!pend) break;
+    if (++polls > XS_SPIN_LIMIT) {  // bounded wait: flag it, continue with NaN
+      atomicOr(err, 1u);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (u[q] == XS_SENTINEL) u[q] = 0x7FF8000000000000ull;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c[q] >= 0 && u[q] == XS_SENTINEL) u[q] = ld_relaxed(x + c[q]);
   }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (c[q] >= 0) acc -= a[q] * __longlong_as_double((long long)u[q]);
   return acc;
 }
 
@@ -624,20 +643,23 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
                                                    const int *__restrict__ dpos,
                                                    const double *__restrict__ dinv,
                                                    const double *__restrict__ b, double *x,
-                                                   unsigned *tickets, const xs_state *st) {
+                                                   unsigned *tickets, const int *__restrict__ sched,
+                                                   const xs_state *st) {
   if (st != nullptr && st->done) return;
   unsigned *err = tickets + 4;  // wait-expired flag, 4 words past the tickets
   const int lane = threadIdx.x & (XS_WAVE - 1);
-  const unsigned w = trsv_ticket(tickets);
+  const unsigned t = trsv_ticket(tickets);
   const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
-  if (w < nw) {
+  if (t < nw) {
+    const unsigned w = sched ? (unsigned)sched[t] : t;
     int r0, r1;
     if (!UPPER) { r0 = (int)w * XS_WAVE; r1 = min(n, r0 + XS_WAVE); }
     else { r1 = n - (int)w * XS_WAVE; r0 = max(0, r1 - XS_WAVE); }
-    const int r = r0 + lane;
-    const bool act = r < r1;
+    const int nrows = r1 - r0;
+    const bool act = lane < nrows;
+    const int r = UPPER ? r1 - 1 - lane : r0 + lane;
     double acc = 0.0, di = 1.0;
-    int eb = 0, ee = 0;  // internal entries [eb, ee)
+    int eb = 0, ee = 0;  // in-wave entries [eb, ee)
     if (act) {
       acc = b[r];
       if (dinv) di = dinv[r];
@@ -662,46 +684,25 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
     const bool chain = !act || nint == 0 || (nint == 1 && ci[eb] == nb);
     double val;
     if (__all(chain)) {
-      // x_r = c + m * x_nb, composed along the wave with a log-step scan
-      double c, m;
-      c = act ? di * acc : 0.0;
-      m = (act && nint == 1) ? -di * lu[eb] : 0.0;
-#pragma unroll
-      for (int d = 1; d < XS_WAVE; d <<= 1) {
-        double cj, mj;
-        if (!UPPER) { cj = __shfl_up(c, d, XS_WAVE); mj = __shfl_up(m, d, XS_WAVE); }
-        else { cj = __shfl_down(c, d, XS_WAVE); mj = __shfl_down(m, d, XS_WAVE); }
-        const bool has = UPPER ? (lane + d < XS_WAVE) : (lane >= d);
-        if (has) { c = c + m * cj; m = m * mj; }
-      }
-      val = c;
+      // x_r = c + m * x_nb, composed along the lanes
+      double c[1] = {act ? di * acc : 0.0};
+      double m = (act && nint == 1) ? -di * lu[eb] : 0.0;
+      wave_affine_scan<1>(c, m);
+      val = c[0];
     } else {
-      const int nrows = r1 - r0;
-      if (!UPPER) {
-        int e = eb;
-        int nc = (act && e < ee) ? ci[e] : INT_MAX;
-        for (int k = 0; k < nrows; ++k) {
-          const double xk = __shfl(acc * di, k, XS_WAVE);
-          if (nc == r0 + k) {
-            acc -= lu[e] * xk;
-            ++e;
-            nc = (e < ee) ? ci[e] : INT_MAX;
-          }
+      // lane k's row is final once the rows of lanes < k are: broadcast it
+      int e = UPPER ? ee - 1 : eb;
+      int nc = act && nint > 0 ? ci[e] : -1;
+      for (int k = 0; k < nrows; ++k) {
+        const double xk = __shfl(acc * di, k, XS_WAVE);
+        const int rk = UPPER ? r1 - 1 - k : r0 + k;
+        if (nc == rk) {
+          acc -= lu[e] * xk;
+          if (UPPER) { --e; nc = (e >= eb) ? ci[e] : -1; }
+          else { ++e; nc = (e < ee) ? ci[e] : -1; }
         }
-        val = acc * di;
-      } else {
-        int e = ee - 1;
-        int nc = (act && e >= eb) ? ci[e] : INT_MIN;
-        for (int k = nrows - 1; k >= 0; --k) {
-          const double xk = __shfl(acc * di, k, XS_WAVE);
-          if (nc == r0 + k) {
-            acc -= lu[e] * xk;
-            --e;
-            nc = (e >= eb) ? ci[e] : INT_MIN;
-          }
-        }
-        val = acc * di;
       }
+      val = acc * di;
     }
     if (act) st_ready(x + r, val);
   }
@@ -712,31 +713,31 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
 // (xs_fill_sentinel); the kernel leaves no sentinel behind.
 static void trsv_launch(hipStream_t s, bool upper, int n, const int *rp, const int *ci,
                         const double *lu, const int *dpos, const double *dinv, const double *b,
-                        double *x, unsigned *tickets, const xs_state *st) {
+                        double *x, unsigned *tickets, const int *sched, const xs_state *st) {
   XS_GUARD(rp && ci && lu && dpos && b && x && tickets);
   const int nw = (n + XS_WAVE - 1) / XS_WAVE;
   dim3 grid((nw + (XS_BLOCK / XS_WAVE) - 1) / (XS_BLOCK / XS_WAVE)), blk(XS_BLOCK);
-  if (upper) hipLaunchKernelGGL(k_trsv<true>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, st);
-  else hipLaunchKernelGGL(k_trsv<false>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, st);
+  if (upper) hipLaunchKernelGGL(k_trsv<true>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, sched, st);
+  else hipLaunchKernelGGL(k_trsv<false>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, sched, st);
 }
 
 void xs_trsv_lower(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
                    const xs_state *st) {
-  trsv_launch(s, false, F.n, F.rp, F.ci, F.lu, F.dpos, nullptr, b, x, tickets, st);
+  trsv_launch(s, false, F.n, F.rp, F.ci, F.lu, F.dpos, nullptr, b, x, tickets, F.lsched, st);
 }
 void xs_trsv_upper(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
                    const xs_state *st) {
-  trsv_launch(s, true, F.n, F.rp, F.ci, F.lu, F.dpos, F.dinv, b, x, tickets, st);
+  trsv_launch(s, true, F.n, F.rp, F.ci, F.lu, F.dpos, F.dinv, b, x, tickets, F.usched, st);
 }
 // Transposed factors (F^T = U^T | L^T on the transposed pattern): the lower
 // solve divides by u_ii, the upper one has a unit diagonal.
 void xs_trsv_lower_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
-  trsv_launch(s, false, F.n, F.trp, F.tci, F.tlu, F.tdpos, F.dinv, b, x, tickets, st);
+  trsv_launch(s, false, F.n, F.trp, F.tci, F.tlu, F.tdpos, F.dinv, b, x, tickets, nullptr, st);
 }
 void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
-  trsv_launch(s, true, F.n, F.trp, F.tci, F.tlu, F.tdpos, nullptr, b, x, tickets, st);
+  trsv_launch(s, true, F.n, F.trp, F.tci, F.tlu, F.tdpos, nullptr, b, x, tickets, nullptr, st);
 }
 
 // ---------------------------------------------------------------------------

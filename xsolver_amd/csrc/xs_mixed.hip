@@ -161,8 +161,8 @@ __device__ __forceinline__ void st_ready32(float *p, float v) {
 }
 
 // acc - sum of lu[e] * x[ci[e]] over up to 4 entries e, e+dir, ... (stopping
-// at `stop`): the loads are issued together, then only values still holding
-// the sentinel are waited for.
+// at `stop`), subtracted in that order: the loads are issued together and
+// the values still holding the sentinel are re-polled together.
 template <class TF>
 __device__ __forceinline__ float trsv_gather4_32(float acc, int e, int stop, int dir,
                                                  const int *__restrict__ ci,
@@ -178,20 +178,38 @@ __device__ __forceinline__ float trsv_gather4_32(float acc, int e, int stop, int
     c[q] = in ? ci[eq] : -1;
     a[q] = in ? fl(lu, eq) : 0.f;
   }
+  auto ld = [&](int q) {
+    return __hip_atomic_load(reinterpret_cast<unsigned *>(const_cast<float *>(x + c[q])),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+#pragma unroll
+  for (int q = 0; q < 4; ++q) u[q] = c[q] >= 0 ? ld(q) : 0u;
+  unsigned polls = 0;
+  for (;;) {
+    bool pend = false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pend |= (c[q] >= 0 && u[q] == XS_SENTINEL32);
+    if (!pend) break;
+    if (++polls > XS_SPIN_LIMIT32) {  // bounded wait: flag it, continue with NaN
+      atomicOr(err, 1u);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (u[q] == XS_SENTINEL32) u[q] = 0x7FC00000u;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c[q] >= 0 && u[q] == XS_SENTINEL32) u[q] = ld(q);
+  }
 #pragma unroll
   for (int q = 0; q < 4; ++q)
-    u[q] = c[q] >= 0 ? __hip_atomic_load(reinterpret_cast<unsigned *>(const_cast<float *>(x + c[q])),
-                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                     : 0u;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    if (c[q] < 0) break;
-    const float v = u[q] == XS_SENTINEL32 ? ld_ready32(x + c[q], err) : __uint_as_float(u[q]);
-    acc -= a[q] * v;
-  }
+    if (c[q] >= 0) acc -= a[q] * __uint_as_float(u[q]);
   return acc;
 }
 
+// FP32 sync-free solve: the algorithm of k_trsv (xs_kernels.hip) with FP32
+// vectors and FP32/BF16/FP16 factor values.
 template <bool UPPER, class TF>
 __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restrict__ rp,
                                                      const int *__restrict__ ci,
@@ -199,18 +217,22 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
                                                      const int *__restrict__ dpos,
                                                      const float *__restrict__ dinv,
                                                      const float *__restrict__ b, float *x,
-                                                     unsigned *tickets, const xs_state *st) {
+                                                     unsigned *tickets,
+                                                     const int *__restrict__ sched,
+                                                     const xs_state *st) {
   if (st != nullptr && st->done) return;
   unsigned *err = tickets + 4;
   const int lane = threadIdx.x & (XS_WAVE - 1);
-  const unsigned w = trsv_ticket(tickets);
+  const unsigned t = trsv_ticket(tickets);
   const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
-  if (w < nw) {
+  if (t < nw) {
+    const unsigned w = sched ? (unsigned)sched[t] : t;
     int r0, r1;
     if (!UPPER) { r0 = (int)w * XS_WAVE; r1 = min(n, r0 + XS_WAVE); }
     else { r1 = n - (int)w * XS_WAVE; r0 = max(0, r1 - XS_WAVE); }
-    const int r = r0 + lane;
-    const bool act = r < r1;
+    const int nrows = r1 - r0;
+    const bool act = lane < nrows;
+    const int r = UPPER ? r1 - 1 - lane : r0 + lane;
     float acc = 0.f, di = 1.f;
     int eb = 0, ee = 0;
     if (act) {
@@ -237,45 +259,23 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
     const bool chain = !act || nint == 0 || (nint == 1 && ci[eb] == nb);
     float val;
     if (__all(chain)) {
-      float c, m;
-      if (!UPPER) { c = acc; m = (act && nint == 1) ? -fl(lu, eb) : 0.f; }
-      else { c = act ? di * acc : 0.f; m = (act && nint == 1) ? -di * fl(lu, eb) : 0.f; }
-#pragma unroll
-      for (int d = 1; d < XS_WAVE; d <<= 1) {
-        float cj, mj;
-        if (!UPPER) { cj = __shfl_up(c, d, XS_WAVE); mj = __shfl_up(m, d, XS_WAVE); }
-        else { cj = __shfl_down(c, d, XS_WAVE); mj = __shfl_down(m, d, XS_WAVE); }
-        const bool has = UPPER ? (lane + d < XS_WAVE) : (lane >= d);
-        if (has) { c = c + m * cj; m = m * mj; }
-      }
+      float c = act ? di * acc : 0.f;
+      float m = (act && nint == 1) ? -di * fl(lu, eb) : 0.f;
+      wave_affine_scan_f(c, m);
       val = c;
     } else {
-      const int nrows = r1 - r0;
-      if (!UPPER) {
-        int e = eb;
-        int nc = (act && e < ee) ? ci[e] : INT_MAX;
-        for (int k = 0; k < nrows; ++k) {
-          const float xk = __shfl(acc, k, XS_WAVE);
-          if (nc == r0 + k) {
-            acc -= fl(lu, e) * xk;
-            ++e;
-            nc = (e < ee) ? ci[e] : INT_MAX;
-          }
+      int e = UPPER ? ee - 1 : eb;
+      int nc = act && nint > 0 ? ci[e] : -1;
+      for (int k = 0; k < nrows; ++k) {
+        const float xk = __shfl(acc * di, k, XS_WAVE);
+        const int rk = UPPER ? r1 - 1 - k : r0 + k;
+        if (nc == rk) {
+          acc -= fl(lu, e) * xk;
+          if (UPPER) { --e; nc = (e >= eb) ? ci[e] : -1; }
+          else { ++e; nc = (e < ee) ? ci[e] : -1; }
         }
-        val = acc;
-      } else {
-        int e = ee - 1;
-        int nc = (act && e >= eb) ? ci[e] : INT_MIN;
-        for (int k = nrows - 1; k >= 0; --k) {
-          const float xk = __shfl(acc * di, k, XS_WAVE);
-          if (nc == r0 + k) {
-            acc -= fl(lu, e) * xk;
-            --e;
-            nc = (e >= eb) ? ci[e] : INT_MIN;
-          }
-        }
-        val = acc * di;
       }
+      val = acc * di;
     }
     if (act) st_ready32(x + r, val);
   }
@@ -334,13 +334,13 @@ static inline int vgrid(int n) {
 
 void xs_trsv32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, const void *lu,
                int ftype, const int *dpos, const float *dinv, const float *b, float *x,
-               unsigned *tickets, const xs_state *st) {
-  if (!(rp && ci && lu && dpos && b && x && tickets) || (upper && !dinv)) return;
+               unsigned *tickets, const int *sched, const xs_state *st) {
+  if (!(rp && ci && lu && dpos && b && x && tickets) || (upper && !dinv)) { xs_raise_launch_guard(); return; }
   hipLaunchKernelGGL(k_fill_u32, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s,
                      reinterpret_cast<unsigned *>(x), n, XS_SENTINEL32);
   const int nw = (n + XS_WAVE - 1) / XS_WAVE;
   dim3 grid((nw + 3) / 4), blk(XS_BLOCK);
-#define TRSV32(U, T) hipLaunchKernelGGL((k_trsv32<U, T>), grid, blk, 0, s, n, rp, ci, (const T *)lu, dpos, dinv, b, x, tickets, st)
+#define TRSV32(U, T) hipLaunchKernelGGL((k_trsv32<U, T>), grid, blk, 0, s, n, rp, ci, (const T *)lu, dpos, dinv, b, x, tickets, sched, st)
   if (ftype == XS_F32) { if (upper) TRSV32(true, float); else TRSV32(false, float); }
   else if (ftype == XS_BF16) { if (upper) TRSV32(true, __hip_bfloat16); else TRSV32(false, __hip_bfloat16); }
   else { if (upper) TRSV32(true, __half); else TRSV32(false, __half); }

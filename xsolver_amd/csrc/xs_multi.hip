@@ -391,18 +391,22 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
                                                     const int *__restrict__ dpos,
                                                     const double *__restrict__ dinv,
                                                     const double *__restrict__ B, double *X,
-                                                    unsigned *tickets, const xs_mstate *ms) {
+                                                    unsigned *tickets,
+                                                    const int *__restrict__ sched,
+                                                    const xs_mstate *ms) {
   if (ms && ms->all) return;
   unsigned *err = tickets + 4;
   const int lane = threadIdx.x & (XS_WAVE - 1);
-  const unsigned w = trsv_ticket(tickets);
+  const unsigned t = trsv_ticket(tickets);
   const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
-  if (w < nw) {
+  if (t < nw) {
+    const unsigned w = sched ? (unsigned)sched[t] : t;
     int r0, r1;
     if (!UPPER) { r0 = (int)w * XS_WAVE; r1 = min(n, r0 + XS_WAVE); }
     else { r1 = n - (int)w * XS_WAVE; r0 = max(0, r1 - XS_WAVE); }
-    const int r = r0 + lane;
-    const bool act = r < r1;
+    const int nrows = r1 - r0;
+    const bool act = lane < nrows;
+    const int r = UPPER ? r1 - 1 - lane : r0 + lane;  // lanes along the solve order
     double acc[KC];
     double di = 1.0;
     int eb = 0, ee = 0;  // in-wave entries [eb, ee)
@@ -430,64 +434,57 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
       for (int e = xb; e < xe; ++e) {
         const double a = lu[e];
         const double *xc = X + (size_t)ci[e] * k + j0;
-        // issue all KC loads first; spin only on values not yet published
+        // issue all KC loads, then re-poll the pending ones together
         unsigned long long u[KC];
 #pragma unroll
         for (int q = 0; q < KC; ++q) u[q] = ld_relaxed(xc + q);
+        unsigned polls = 0;
+        for (;;) {
+          bool pend = false;
 #pragma unroll
-        for (int q = 0; q < KC; ++q) {
-          const double xv = (u[q] == XS_SENTINEL) ? ld_ready(xc + q, err)
-                                                  : __longlong_as_double((long long)u[q]);
-          acc[q] -= a * xv;
+          for (int q = 0; q < KC; ++q) pend |= u[q] == XS_SENTINEL;
+          if (!pend) break;
+          if (++polls > XS_SPIN_LIMIT) {
+            atomicOr(err, 1u);
+#pragma unroll
+            for (int q = 0; q < KC; ++q)
+              if (u[q] == XS_SENTINEL) u[q] = 0x7FF8000000000000ull;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+          for (int q = 0; q < KC; ++q)
+            if (u[q] == XS_SENTINEL) u[q] = ld_relaxed(xc + q);
         }
+#pragma unroll
+        for (int q = 0; q < KC; ++q) acc[q] -= a * __longlong_as_double((long long)u[q]);
       }
     }
     const int nint = ee - eb;
     const int nb = UPPER ? r + 1 : r - 1;
     const bool chain = !act || nint == 0 || (nint == 1 && ci[eb] == nb);
     if (__all(chain)) {
-      // x_r = c + m x_nb per column, composed with a log-step scan
+      // x_r = c + m x_nb per column, composed along the lanes
       double m = (act && nint == 1) ? -di * lu[eb] : 0.0;
 #pragma unroll
       for (int q = 0; q < KC; ++q) acc[q] = act ? di * acc[q] : 0.0;
-#pragma unroll
-      for (int d = 1; d < XS_WAVE; d <<= 1) {
-        const bool has = UPPER ? (lane + d < XS_WAVE) : (lane >= d);
-        const double mj = UPPER ? __shfl_down(m, d, XS_WAVE) : __shfl_up(m, d, XS_WAVE);
+      wave_affine_scan<KC>(acc, m);
+    } else {
+      // lane kk's rows are final once the lanes below are: broadcast them
+      int e = UPPER ? ee - 1 : eb;
+      int nc = act && nint > 0 ? ci[e] : -1;
+      for (int kk = 0; kk < nrows; ++kk) {
+        const int rk = UPPER ? r1 - 1 - kk : r0 + kk;
+        const bool hit = nc == rk;
+        const double a = hit ? lu[e] : 0.0;
 #pragma unroll
         for (int q = 0; q < KC; ++q) {
-          const double cj = UPPER ? __shfl_down(acc[q], d, XS_WAVE) : __shfl_up(acc[q], d, XS_WAVE);
-          if (has) acc[q] = acc[q] + m * cj;
+          const double xk = __shfl(acc[q] * di, kk, XS_WAVE);
+          if (hit) acc[q] -= a * xk;
         }
-        if (has) m = m * mj;
-      }
-    } else {
-      const int nrows = r1 - r0;
-      if (!UPPER) {
-        int e = eb;
-        int nc = (act && e < ee) ? ci[e] : INT_MAX;
-        for (int kk = 0; kk < nrows; ++kk) {
-          const bool hit = nc == r0 + kk;
-          const double a = hit ? lu[e] : 0.0;  // read once per row step
-#pragma unroll
-          for (int q = 0; q < KC; ++q) {
-            const double xk = __shfl(acc[q] * di, kk, XS_WAVE);
-            if (hit) acc[q] -= a * xk;
-          }
-          if (hit) { ++e; nc = (e < ee) ? ci[e] : INT_MAX; }
-        }
-      } else {
-        int e = ee - 1;
-        int nc = (act && e >= eb) ? ci[e] : INT_MIN;
-        for (int kk = nrows - 1; kk >= 0; --kk) {
-          const bool hit = nc == r0 + kk;
-          const double a = hit ? lu[e] : 0.0;  // read once per row step
-#pragma unroll
-          for (int q = 0; q < KC; ++q) {
-            const double xk = __shfl(acc[q] * di, kk, XS_WAVE);
-            if (hit) acc[q] -= a * xk;
-          }
-          if (hit) { --e; nc = (e >= eb) ? ci[e] : INT_MIN; }
+        if (hit) {
+          if (UPPER) { --e; nc = (e >= eb) ? ci[e] : -1; }
+          else { ++e; nc = (e < ee) ? ci[e] : -1; }
         }
       }
 #pragma unroll
@@ -601,10 +598,11 @@ void xs_mstate_init(hipStream_t s, xs_mstate *ms, int k) {
 template <bool UPPER>
 static void mtrsv_chunk(hipStream_t s, dim3 grid, int kc, int n, int k, int j0, const int *rp,
                         const int *ci, const double *lu, const int *dpos, const double *dinv,
-                        const double *B, double *X, unsigned *tickets, const xs_mstate *ms) {
+                        const double *B, double *X, unsigned *tickets, const int *sched,
+                        const xs_mstate *ms) {
   const dim3 blk(XS_BLOCK);
 #define XS_MT(KC) \
-  hipLaunchKernelGGL((k_mtrsv<UPPER, KC>), grid, blk, 0, s, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, ms)
+  hipLaunchKernelGGL((k_mtrsv<UPPER, KC>), grid, blk, 0, s, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, sched, ms)
   switch (kc) {
     case 16: XS_MT(16); break;
     case 8: XS_MT(8); break;
@@ -619,7 +617,7 @@ static void mtrsv_chunk(hipStream_t s, dim3 grid, int kc, int n, int k, int j0, 
 // 16/8/4/2/1 per launch.  X is sentinel-filled here.
 void xs_mtrsv(hipStream_t s, bool upper, int n, int k, const int *rp, const int *ci,
               const double *lu, const int *dpos, const double *dinv, const double *B, double *X,
-              unsigned *tickets, const xs_mstate *ms) {
+              unsigned *tickets, const int *sched, const xs_mstate *ms) {
   XS_GUARD(rp && ci && lu && dpos && B && X && tickets && k_ok(k));
   const size_t m = (size_t)n * k;
   hipLaunchKernelGGL(k_mfill, dim3(mgrid_flat(m)), dim3(XS_BLOCK), 0, s, m,
@@ -629,8 +627,8 @@ void xs_mtrsv(hipStream_t s, bool upper, int n, int k, const int *rp, const int 
   for (int j0 = 0; j0 < k;) {
     const int left = k - j0;
     const int kc = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
-    if (upper) mtrsv_chunk<true>(s, grid, kc, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, ms);
-    else mtrsv_chunk<false>(s, grid, kc, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, ms);
+    if (upper) mtrsv_chunk<true>(s, grid, kc, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, sched, ms);
+    else mtrsv_chunk<false>(s, grid, kc, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, sched, ms);
     j0 += kc;
   }
 }

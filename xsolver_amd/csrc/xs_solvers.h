@@ -37,6 +37,7 @@ struct xs_ctx {
   int tune_chunk = 0;
   double tune_us = 0.0;
   bool use_graph = true;        // replay batches of iterations as HIP graphs
+  bool trsv_schedule = true;    // SpTRSV groups in dependency-level order
   int precision = 0;
   int use_ell = 0;
 
