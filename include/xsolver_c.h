@@ -127,12 +127,16 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *                    ILUT, precon_gpuilut.cpp:19-20)
  *   "trsv_schedule"  1 (default) = hand out the triangular-solve row groups in
  *                    dependency-level order, 0 = row order
+ *   "spmv_kernel"    CSR SpMV kernel: 0 = LDS-staged row blocks (default),
+ *                    1 = row-binned vector CSR (lanes per row = mean row length
+ *                    rounded up to a power of two, long rows one wavefront each)
  *   "use_ell"        1 = ELL SpMV (same as advanced_pam[1] = 1) */
 int xs_set_option(xs_ctx *ctx, const char *name, double value);
 
 /* SpMV autotuner (no reference counterpart): times the CSR row-block kernel
- * at nnz budgets per workgroup of 512..4096 and the ELL kernel (when its
- * padding is under 50%) on this matrix and keeps the fastest; with the
+ * at nnz budgets per workgroup of 512..4096, the row-binned vector-CSR kernel
+ * (reported as chunk 0) and the ELL kernel (when its padding is under 50%) on
+ * this matrix and keeps the fastest; with the
  * environment variable XS_TUNE_CACHE naming a file, results are read from
  * and appended to it per matrix signature.  Choice and time are queried with
  * xs_get_time_ms("tune:chunk" | "tune:ell" | "tune:us"). */

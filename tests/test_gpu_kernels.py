@@ -29,15 +29,16 @@ def dev_tensor(a, dev):
     return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
 
-@pytest.mark.parametrize("ell", [0, 1], ids=["csr", "ell"])
+@pytest.mark.parametrize("fmt", ["csr", "vec", "ell"])
 @pytest.mark.parametrize("name", list(MATS))
-def test_spmv_and_bmax_parity(dev, name, ell):
+def test_spmv_and_bmax_parity(dev, name, fmt):
     import torch
     A = MATS[name]()
     n = len(A[0]) - 1
-    say(f"[spmv] {name} ell={ell} n={n} nnz={len(A[2])}")
+    say(f"[spmv] {name} {fmt} n={n} nnz={len(A[2])}")
     S = XSolver()
-    S.mat_setup(*A, advanced_pam=(0, ell, 1))
+    S.mat_setup(*A, advanced_pam=(0, 1 if fmt == "ell" else 0, 1))
+    S.set_option("spmv_kernel", 1 if fmt == "vec" else 0)
     rng = np.random.default_rng(3)
     x = rng.uniform(-1, 1, n)
     b = rng.uniform(-1, 1, n)
@@ -61,9 +62,10 @@ def test_spmv_full_size_convdiff128(dev):
     say(f"[spmv] convdiff3d_128 n={n} nnz={len(A[2])}")
     x = np.random.default_rng(1).uniform(-1, 1, n)
     ref = gen.spmv(A, x)
-    for ell in (0, 1):
+    for ell, vec in ((0, 0), (0, 1), (1, 0)):
         S = XSolver()
         S.mat_setup(*A, advanced_pam=(0, ell, 1))
+        S.set_option("spmv_kernel", vec)
         yd = torch.empty(n, dtype=torch.float64, device=dev)
         S.spmv(dev_tensor(x, dev), yd, torch_stream())
         torch.cuda.synchronize()

@@ -39,16 +39,17 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("ell", [0, 1], ids=["csr", "ell"])
+@pytest.mark.parametrize("fmt", ["csr", "vec", "ell"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_solve_parity(dev, case, ell):
+def test_solve_parity(dev, case, fmt):
     name, build, ksp, pc, tol, maxiter, seed = case
     A = build()
     xs, b = gen.manufactured(A, seed)
-    say(f"[solve] {name} ell={ell} n={len(xs)}")
+    say(f"[solve] {name} {fmt} n={len(xs)}")
     xo, sto, ito, reso = oracle.solve(A, b, ksp, pc, tol=tol, maxiter=maxiter)
     kw = {"parilu_async": 0, "parilu_sweeps": 60} if pc == "ilu0" else {}
-    g = gpu_solve(A, b, ksp, pc, tol, maxiter, ell=ell, **kw)
+    kw["spmv_kernel"] = 1 if fmt == "vec" else 0
+    g = gpu_solve(A, b, ksp, pc, tol, maxiter, ell=1 if fmt == "ell" else 0, **kw)
     say(f"   oracle it={ito} gpu it={g['it']} status={g['st']} "
         f"x-diff={rel_err(g['x'], xo):.2e} relres={true_relres(A, b, g['x']):.2e}")
     assert sto == 0 and g["st"] == xsolver_amd.XS_OK
@@ -340,3 +341,12 @@ def test_update_values_then_solve(dev, precision):
     if precision == 0:
         assert abs(S.iterations - ito) <= 2
     S.close()
+
+
+def test_mixed_with_vector_csr(dev):
+    """FP32 inner solve over the row-binned vector-CSR SpMV (FP32 kernel)."""
+    A = gen.convdiff3d(24)
+    _, b = gen.manufactured(A, 1)
+    g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000, precision=1, spmv_kernel=1)
+    assert g["st"] == 0 and true_relres(A, b, g["x"]) < 1.05e-10
+    g["S"].close()

@@ -77,12 +77,14 @@ static void free_matrix(xs_ctx *c) {
   if (c->own_matrix) { dfree(c->A.rp); dfree(c->A.ci); dfree(c->A.v); }
   c->A = xs_csr();
   dfree(c->plan.blk_row);
+  dfree(c->plan.long_rows);
   c->plan = xs_spmv_plan();
   dfree(c->ell.ci); dfree(c->ell.v);
   c->ell = xs_ell();
   dfree(c->At.rp); dfree(c->At.ci); dfree(c->At.v);
   c->At = xs_csr();
   dfree(c->planT.blk_row);
+  dfree(c->planT.long_rows);
   c->planT = xs_spmv_plan();
   dfree(c->perm_At);
   c->have_At = false;
@@ -131,6 +133,9 @@ static int ensure_common(xs_ctx *c) {
   pblocks = std::max<size_t>(pblocks, (size_t)c->planT.nblk);
   // ELL launch: two rows per thread
   pblocks = std::max<size_t>(pblocks, ((size_t)c->A.n + 2 * XS_BLOCK - 1) / (2 * XS_BLOCK));
+  // vector CSR: long-row workgroups + at most XS_MAX_REDUCE_BLOCKS
+  for (const xs_spmv_plan *P : {&c->plan, &c->planT})
+    pblocks = std::max<size_t>(pblocks, (size_t)(P->nlong + 3) / 4 + XS_MAX_REDUCE_BLOCKS);
   if (pblocks > c->partial_blocks) {
     dfree(c->partial);
     int rc = dalloc(&c->partial, pblocks * 4);
@@ -180,12 +185,37 @@ static int analyse_pattern(xs_ctx *c) {
 
 // (Re)build the CSR row-block plan of M with the given nnz budget per block.
 static int make_plan(xs_ctx *c, const xs_csr &M, xs_spmv_plan &P, int chunk) {
-  dfree(P.blk_row);
-  P = xs_spmv_plan();
+  dfree(P.blk_row);   // the vector-CSR fields are kept
+  P.nblk = 0;
+  P.chunk = 0;
   const int nblk = std::max(1, (M.nnz + chunk - 1) / chunk);
   int rc = dalloc(&P.blk_row, (size_t)nblk + 1);
   if (rc) return rc;
   xs_build_plan(c->stream, M, P, P.blk_row, chunk);
+  return XS_OK;
+}
+
+// Row-binned vector-CSR parameters from the host row pointer: lanes per row
+// = average row length rounded up to a power of two (2..64); rows longer than
+// 8 such groups are listed for one wavefront each.
+static int make_vec(xs_ctx *c, const std::vector<int> &hrp, xs_spmv_plan &P) {
+  (void)c;
+  const int n = (int)hrp.size() - 1;
+  const double avg = n > 0 ? (double)hrp[n] / n : 1.0;
+  int L = 2;
+  while (L < avg && L < XS_WAVE) L <<= 1;
+  P.vec_L = L;
+  P.long_thr = 8 * L;
+  std::vector<int> lr;
+  for (int i = 0; i < n; i++)
+    if (hrp[i + 1] - hrp[i] > P.long_thr) lr.push_back(i);
+  dfree(P.long_rows);
+  P.nlong = (int)lr.size();
+  if (P.nlong) {
+    int rc = dalloc(&P.long_rows, lr.size());
+    if (rc) return rc;
+    XS_HIP(hipMemcpy(P.long_rows, lr.data(), sizeof(int) * lr.size(), hipMemcpyHostToDevice));
+  }
   return XS_OK;
 }
 
@@ -208,6 +238,8 @@ static int build_spmv(xs_ctx *c) {
   xs_csr &A = c->A;
   if (!A.rp || !A.ci || !A.v || !c->d_int) return XS_ERR_STATE;
   if ((rc = make_plan(c, A, c->plan, xs_default_chunk(A)))) return rc;
+  if ((rc = make_vec(c, c->h_rp, c->plan))) return rc;
+  c->plan.vec = c->spmv_vec;
   if (c->use_ell && (rc = make_ell(c))) return rc;
   XS_HIP(hipStreamSynchronize(c->stream));
   if (xs_take_launch_guard()) return XS_ERR_STATE;
@@ -293,12 +325,20 @@ extern "C" int xs_autotune(xs_ctx *c) {
     c->prof_kind = 0;
     best_us = 1e30;
     c->use_ell = 0;
+    c->plan.vec = 0;
     static const int chunks[] = {512, 1024, 1536, 2048, 3072, 4096};
     for (int ch : chunks) {
       if ((rc = make_plan(c, c->A, c->plan, ch))) return rc;
       double us;
       if ((rc = time_spmv(c, x, y, reps, &us))) return rc;
       if (us < best_us) { best_us = us; best_chunk = ch; best_ell = 0; }
+    }
+    {  // row-binned vector CSR (chunk reported as 0)
+      c->plan.vec = 1;
+      double us;
+      if ((rc = time_spmv(c, x, y, reps, &us))) return rc;
+      if (us < best_us) { best_us = us; best_chunk = 0; best_ell = 0; }
+      c->plan.vec = 0;
     }
     // ELL when padding stays under 50%
     int W = c->ell.v ? c->ell.W : xs_max_row_len(c->stream, c->A, c->d_int);
@@ -311,7 +351,10 @@ extern "C" int xs_autotune(xs_ctx *c) {
     }
     tune_cache_put(key, best_chunk, best_ell, best_us);
   }
-  if ((rc = make_plan(c, c->A, c->plan, best_chunk))) return rc;
+  if ((rc = make_plan(c, c->A, c->plan, best_chunk > 0 ? best_chunk : xs_default_chunk(c->A))))
+    return rc;
+  c->plan.vec = (best_chunk == 0 && !best_ell) ? 1 : 0;
+  c->spmv_vec = c->plan.vec;
   c->use_ell = best_ell;
   if (best_ell && (rc = make_ell(c))) return rc;
   c->tune_chunk = best_chunk;
@@ -533,6 +576,12 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   else if (k == "parilu_async") { c->parilu_async = value != 0; c->pc_ready = false; }
   else if (k == "check_every") { if (value < 1) return XS_ERR_ARG; c->check_every = (int)value; }
   else if (k == "use_graph") { c->use_graph = value != 0; }
+  else if (k == "spmv_kernel") {   // 0 = LDS row blocks, 1 = row-binned vector CSR
+    if (value != 0 && value != 1) return XS_ERR_ARG;
+    c->spmv_vec = (int)value;
+    c->plan.vec = c->spmv_vec;
+    c->planT.vec = c->spmv_vec;
+  }
   else if (k == "trsv_schedule") { c->trsv_schedule = value != 0; c->pc_ready = false; free_precond(c); }
   else if (k == "lsolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->lsweeps = (int)value; }
   else if (k == "usolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->usweeps = (int)value; }
@@ -1234,6 +1283,8 @@ int xs_build_transpose(xs_ctx *c) {
   c->At.nnz = nnz;
   xs_permute_vals(c->stream, nnz, c->perm_At, c->A.v, c->At.v);
   if ((rc = make_plan(c, c->At, c->planT, xs_default_chunk(c->At)))) return rc;
+  if ((rc = make_vec(c, trp, c->planT))) return rc;
+  c->planT.vec = c->spmv_vec;
   XS_HIP(hipStreamSynchronize(c->stream));
   if (xs_take_launch_guard()) return XS_ERR_STATE;
   c->have_At = true;

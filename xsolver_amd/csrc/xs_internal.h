@@ -74,6 +74,10 @@ struct xs_spmv_plan {   // row blocks for the LDS-staged CSR kernel
   int nblk = 0;
   int chunk = 0;        // nnz budget per block
   int *blk_row = nullptr;  // [nblk + 1]
+  // row-binned vector CSR (used when vec != 0): rows of length <= long_thr
+  // by groups of vec_L lanes, the nlong longer rows one wavefront each
+  int vec = 0, vec_L = 0, long_thr = 0, nlong = 0;
+  int *long_rows = nullptr;
 };
 
 struct xs_ell {         // column-major slabs: v[k * ld + row], width W

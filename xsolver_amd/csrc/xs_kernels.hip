@@ -12,6 +12,7 @@
 //    iterations launched ahead of the host's convergence check are no-ops.
 #include "xs_internal.h"
 
+#include <algorithm>
 #include <atomic>
 #include <climits>
 
@@ -86,6 +87,74 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_csr(
   if (RED) reduce_finish<NV>(acc, R);
 }
 
+// Row-binned vector CSR: rows of length <= long_thr are reduced by groups of
+// L lanes (L = the average row length rounded up to a power of two), each
+// group reading its row's entries with consecutive lanes (coalesced across
+// the wave) and combining them with xor shuffles inside the group; the
+// longer rows (listed at set-up) go one wavefront each, in the first
+// workgroups.  Grid-stride over rows with at most XS_MAX_REDUCE_BLOCKS
+// workgroups so the fused dot epilogues reduce over few partials.
+template <int L, int EPI>
+__global__ __launch_bounds__(XS_BLOCK) void k_spmv_vec(
+    int n, const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v,
+    const double *__restrict__ x, double *__restrict__ y, const int *__restrict__ long_rows,
+    int nlong, int long_thr, const double *__restrict__ b, const double *__restrict__ w,
+    xs_reduce R) {
+  constexpr int NV = (EPI == EPI_DOT2) ? 2 : 1;
+  constexpr bool RED = (EPI == EPI_DOT1 || EPI == EPI_DOT2 || EPI == EPI_BMAXN);
+  if (R.st != nullptr && R.st->done) return;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  const int nlb = (nlong + XS_BLOCK / XS_WAVE - 1) / (XS_BLOCK / XS_WAVE);
+  if ((int)blockIdx.x < nlb) {
+    const int lane = threadIdx.x & (XS_WAVE - 1);
+    const int idx = blockIdx.x * (XS_BLOCK / XS_WAVE) + threadIdx.x / XS_WAVE;
+    if (idx < nlong) {
+      const int r = long_rows[idx];
+      double s = 0.0;
+      for (int e = rp[r] + lane; e < rp[r + 1]; e += XS_WAVE) s += v[e] * x[ci[e]];
+      s = wave_sum(s);
+      if (lane == 0) epilogue<EPI, NV>(r, s, y, b, w, acc);
+    }
+  } else {
+    constexpr int GPB = XS_BLOCK / L;  // row groups per workgroup
+    const int sub = threadIdx.x % L;
+    const int stride = ((int)gridDim.x - nlb) * GPB;
+    for (int row = ((int)blockIdx.x - nlb) * GPB + (int)threadIdx.x / L; row < n; row += stride) {
+      const int a = rp[row], e1 = rp[row + 1];
+      if (e1 - a > long_thr) continue;
+      double s = 0.0;
+      for (int e = a + sub; e < e1; e += L) s += v[e] * x[ci[e]];
+#pragma unroll
+      for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+      if (sub == 0) epilogue<EPI, NV>(row, s, y, b, w, acc);
+    }
+  }
+  if (RED) reduce_finish<NV>(acc, R);
+}
+
+template <int L>
+static void spmv_vec_launch(hipStream_t s, const xs_csr &A, const xs_spmv_plan &P,
+                            const double *x, double *y, int epi, const double *b,
+                            const double *w, const xs_reduce &R) {
+  const int nlb = (P.nlong + XS_BLOCK / XS_WAVE - 1) / (XS_BLOCK / XS_WAVE);
+  long g = ((long)A.n + XS_BLOCK / L - 1) / (XS_BLOCK / L);
+  const long cap = std::max(1, XS_MAX_REDUCE_BLOCKS - nlb);
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  const dim3 grid((unsigned)(nlb + g)), blk(XS_BLOCK);
+#define XS_VEC(E) hipLaunchKernelGGL((k_spmv_vec<L, E>), grid, blk, 0, s, A.n, A.rp, A.ci, A.v, x, y, P.long_rows, P.nlong, P.long_thr, b, w, R)
+  switch (epi) {
+    case EPI_Y: XS_VEC(EPI_Y); break;
+    case EPI_BMAX: XS_VEC(EPI_BMAX); break;
+    case EPI_DOT1: XS_VEC(EPI_DOT1); break;
+    case EPI_DOT2: XS_VEC(EPI_DOT2); break;
+    case EPI_BMAXN: XS_VEC(EPI_BMAXN); break;
+  }
+#undef XS_VEC
+}
+
 // ELL SpMV over column-major slabs v[k*ld + row]; two rows per thread with
 // 16-byte loads.  Padding uses the row's own index and a zero value.
 // Reference: matMultELL / bmAxELL (DeviceMatrixELL.cpp:3-34), width fixed at
@@ -133,6 +202,18 @@ void xs_launch_spmv_csr(hipStream_t s, const xs_csr &A, const xs_spmv_plan &P,
   XS_GUARD(!(epi == EPI_DOT1 || epi == EPI_DOT2) || w);
   XS_GUARD(!(epi == EPI_DOT1 || epi == EPI_DOT2 || epi == EPI_BMAXN) || (red && red->partial && red->counter && red->st));
   xs_reduce R = red ? *red : no_reduce();
+  if (P.vec) {
+    XS_GUARD(P.nlong == 0 || P.long_rows);
+    switch (P.vec_L) {
+      case 2: spmv_vec_launch<2>(s, A, P, x, y, epi, b, w, R); break;
+      case 4: spmv_vec_launch<4>(s, A, P, x, y, epi, b, w, R); break;
+      case 8: spmv_vec_launch<8>(s, A, P, x, y, epi, b, w, R); break;
+      case 16: spmv_vec_launch<16>(s, A, P, x, y, epi, b, w, R); break;
+      case 32: spmv_vec_launch<32>(s, A, P, x, y, epi, b, w, R); break;
+      default: spmv_vec_launch<64>(s, A, P, x, y, epi, b, w, R); break;
+    }
+    return;
+  }
   dim3 grid(P.nblk), blk(XS_BLOCK);
   switch (epi) {
     case EPI_Y: hipLaunchKernelGGL(k_spmv_csr<EPI_Y>, grid, blk, 0, s, A.rp, A.ci, A.v, x, y, P.blk_row, b, w, R); break;

@@ -71,6 +71,68 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_csr32(
   if (RED) reduce_finish<NV>(acc, R);
 }
 
+// Row-binned vector CSR in FP32 (the scheme of k_spmv_vec in xs_kernels.hip).
+template <int L, int EPI>
+__global__ __launch_bounds__(XS_BLOCK) void k_spmv_vec32(
+    int n, const int *__restrict__ rp, const int *__restrict__ ci, const float *__restrict__ v,
+    const float *__restrict__ x, float *__restrict__ y, const int *__restrict__ long_rows,
+    int nlong, int long_thr, const float *__restrict__ b, const float *__restrict__ w,
+    xs_reduce R) {
+  constexpr int NV = (EPI == EPI_DOT2) ? 2 : 1;
+  constexpr bool RED = (EPI == EPI_DOT1 || EPI == EPI_DOT2 || EPI == EPI_BMAXN);
+  if (R.st != nullptr && R.st->done) return;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  const int nlb = (nlong + XS_BLOCK / XS_WAVE - 1) / (XS_BLOCK / XS_WAVE);
+  if ((int)blockIdx.x < nlb) {
+    const int lane = threadIdx.x & (XS_WAVE - 1);
+    const int idx = blockIdx.x * (XS_BLOCK / XS_WAVE) + threadIdx.x / XS_WAVE;
+    if (idx < nlong) {
+      const int r = long_rows[idx];
+      double s = 0.0;
+      for (int e = rp[r] + lane; e < rp[r + 1]; e += XS_WAVE) s += (double)(v[e] * x[ci[e]]);
+      s = wave_sum(s);
+      if (lane == 0) epi32<EPI, NV>(r, (float)s, y, b, w, acc);
+    }
+  } else {
+    constexpr int GPB = XS_BLOCK / L;
+    const int sub = threadIdx.x % L;
+    const int stride = ((int)gridDim.x - nlb) * GPB;
+    for (int row = ((int)blockIdx.x - nlb) * GPB + (int)threadIdx.x / L; row < n; row += stride) {
+      const int a = rp[row], e1 = rp[row + 1];
+      if (e1 - a > long_thr) continue;
+      float s = 0.f;
+      for (int e = a + sub; e < e1; e += L) s += v[e] * x[ci[e]];
+#pragma unroll
+      for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+      if (sub == 0) epi32<EPI, NV>(row, s, y, b, w, acc);
+    }
+  }
+  if (RED) reduce_finish<NV>(acc, R);
+}
+
+template <int L>
+static void spmv_vec32_launch(hipStream_t s, const xs_csr &A, const float *v32,
+                              const xs_spmv_plan &P, const float *x, float *y, int epi,
+                              const float *b, const float *w, const xs_reduce &R) {
+  const int nlb = (P.nlong + XS_BLOCK / XS_WAVE - 1) / (XS_BLOCK / XS_WAVE);
+  long g = ((long)A.n + XS_BLOCK / L - 1) / (XS_BLOCK / L);
+  const long cap = XS_MAX_REDUCE_BLOCKS - nlb > 1 ? XS_MAX_REDUCE_BLOCKS - nlb : 1;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  const dim3 grid((unsigned)(nlb + g)), blk(XS_BLOCK);
+#define XS_VEC32(E) hipLaunchKernelGGL((k_spmv_vec32<L, E>), grid, blk, 0, s, A.n, A.rp, A.ci, v32, x, y, P.long_rows, P.nlong, P.long_thr, b, w, R)
+  switch (epi) {
+    case EPI_Y: XS_VEC32(EPI_Y); break;
+    case EPI_BMAX: XS_VEC32(EPI_BMAX); break;
+    case EPI_DOT1: XS_VEC32(EPI_DOT1); break;
+    case EPI_DOT2: XS_VEC32(EPI_DOT2); break;
+    case EPI_BMAXN: XS_VEC32(EPI_BMAXN); break;
+  }
+#undef XS_VEC32
+}
+
 // ELL SpMV in FP32, two rows per thread (8-byte value and index loads).
 template <int EPI>
 __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell32(
@@ -103,7 +165,19 @@ void xs_launch_spmv_csr32(hipStream_t s, const xs_csr &A, const float *v32,
                           const float *b, const float *w, const xs_reduce *red) {
   xs_reduce R{};
   if (red) R = *red;
-  if (!(A.rp && A.ci && v32 && P.blk_row && x && y)) return;
+  if (!(A.rp && A.ci && v32 && P.blk_row && x && y)) { xs_raise_launch_guard(); return; }
+  if (P.vec) {
+    if (P.nlong > 0 && !P.long_rows) { xs_raise_launch_guard(); return; }
+    switch (P.vec_L) {
+      case 2: spmv_vec32_launch<2>(s, A, v32, P, x, y, epi, b, w, R); break;
+      case 4: spmv_vec32_launch<4>(s, A, v32, P, x, y, epi, b, w, R); break;
+      case 8: spmv_vec32_launch<8>(s, A, v32, P, x, y, epi, b, w, R); break;
+      case 16: spmv_vec32_launch<16>(s, A, v32, P, x, y, epi, b, w, R); break;
+      case 32: spmv_vec32_launch<32>(s, A, v32, P, x, y, epi, b, w, R); break;
+      default: spmv_vec32_launch<64>(s, A, v32, P, x, y, epi, b, w, R); break;
+    }
+    return;
+  }
   dim3 grid(P.nblk), blk(XS_BLOCK);
   switch (epi) {
     case EPI_Y: hipLaunchKernelGGL(k_spmv_csr32<EPI_Y>, grid, blk, 0, s, A.rp, A.ci, v32, x, y, P.blk_row, b, w, R); break;
@@ -119,7 +193,7 @@ void xs_launch_spmv_ell32(hipStream_t s, const xs_ell &E, const float *v32, cons
                           const xs_reduce *red) {
   xs_reduce R{};
   if (red) R = *red;
-  if (!(E.ci && v32 && x && y)) return;
+  if (!(E.ci && v32 && x && y)) { xs_raise_launch_guard(); return; }
   const int threads = (E.n + 1) / 2;
   dim3 grid((threads + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
   switch (epi) {

@@ -170,7 +170,19 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mspmm(int n, int k, const int *__r
     for (int row = blockIdx.x * per + rl; row < n; row += gridDim.x * per) {
       double s = 0.0;
       const int e1 = rp[row + 1];
-      for (int e = rp[row]; e < e1; ++e) s += v[e] * X[(size_t)ci[e] * k + j];
+      int e = rp[row];
+      // four independent gathers in flight per thread, summed in entry order
+      for (; e + 4 <= e1; e += 4) {
+        const int c0 = ci[e], c1 = ci[e + 1], c2 = ci[e + 2], c3 = ci[e + 3];
+        const double a0 = v[e], a1 = v[e + 1], a2 = v[e + 2], a3 = v[e + 3];
+        const double x0 = X[(size_t)c0 * k + j], x1 = X[(size_t)c1 * k + j];
+        const double x2 = X[(size_t)c2 * k + j], x3 = X[(size_t)c3 * k + j];
+        s += a0 * x0;
+        s += a1 * x1;
+        s += a2 * x2;
+        s += a3 * x3;
+      }
+      for (; e < e1; ++e) s += v[e] * X[(size_t)ci[e] * k + j];
       const size_t p = (size_t)row * k + j;
       if (EPI == EPI_BMAXN) {
         const double t = B[p] - s;
