@@ -240,6 +240,22 @@ int xs_batch_get_results(const xs_batch *bt, int *iters, double *relres);
 double xs_batch_get_time_ms(const xs_batch *bt);
 int xs_batch_destroy(xs_batch *bt);
 
+/* ======================================================================== */
+/* Matrix Market input (host only, no device needed)                          */
+/* ======================================================================== */
+/* Coordinate-format file -> 0-based CSR with sorted columns (reference reader
+ * hip/Tool/ReadMatrix/csr_mmio_highlevel.h:8-233, mmio_info + mmio_data).
+ * Call once with rowptr/colidx/val NULL to get *n and *nnz (after the
+ * expansion of symmetric / skew-symmetric / Hermitian files), then with
+ * arrays of those capacities (*n, *nnz in = capacities).  diag (optional, n
+ * entries) receives the diagonal values as read (the reference's a_p).
+ * Duplicate coordinates are summed; pattern files get value 1; complex and
+ * non-square files -> XS_ERR_UNSUPPORTED.  The reference negates the values
+ * of general files (:67, :178); XS_MTX_NEGATE_GENERAL reproduces that. */
+#define XS_MTX_NEGATE_GENERAL 1
+int xs_mtx_read(const char *path, int flags, int *n, int *nnz, int *rowptr, int *colidx,
+                double *val, double *diag);
+
 #ifdef __cplusplus
 } /* extern "C" */
 

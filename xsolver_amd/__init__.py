@@ -44,7 +44,7 @@ EXPORTS = [
     "xs_batch_create", "xs_batch_solve", "xs_batch_solve_device", "xs_batch_get_results",
     "xs_batch_get_time_ms", "xs_batch_destroy", "xs_solve_transpose_device",
     "xs_grad_values_device", "xs_solve_multi_device", "xs_solve_multi", "xs_get_multi_results",
-    "xs_get_multi_resvec", "xs_spmm", "xs_pc_apply_multi", "xs_autotune",
+    "xs_get_multi_resvec", "xs_spmm", "xs_pc_apply_multi", "xs_autotune", "xs_mtx_read",
 ]
 
 XS_RHS_COLMAJOR = 0
@@ -114,6 +114,7 @@ def lib():
         "xs_get_multi_resvec": (_i, [_vp, _i, _vp, _i]),
         "xs_spmm": (_i, [_vp, _i, _vp, _vp, _vp]),
         "xs_autotune": (_i, [_vp]),
+        "xs_mtx_read": (_i, [_cp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
         "xs_pc_apply_multi": (_i, [_vp, _i, _vp, _vp, _vp]),
         "xsolver_mat_setup": (None, [_i, _vp, _vp, _vp, _vp]),
         "xsolver_ksp_settype": (None, [_cp]),
@@ -371,6 +372,29 @@ class XBatch:
             self.close()
         except Exception:
             pass
+
+
+XS_MTX_NEGATE_GENERAL = 1
+
+
+def read_mtx(path, negate_general=False):
+    """Matrix Market file -> (rowptr, colidx, val, diag), 0-based CSR with
+    sorted columns (xs_mtx_read; host only).  negate_general reproduces the
+    reference reader's sign flip for general files."""
+    L = lib()
+    flags = XS_MTX_NEGATE_GENERAL if negate_general else 0
+    n = ctypes.c_int(0)
+    nnz = ctypes.c_int(0)
+    p = str(path).encode()
+    _check(L.xs_mtx_read(p, flags, ctypes.byref(n), ctypes.byref(nnz), None, None, None, None),
+           "xs_mtx_read")
+    rp = np.zeros(n.value + 1, np.int32)
+    ci = np.zeros(max(nnz.value, 1), np.int32)
+    v = np.zeros(max(nnz.value, 1))
+    d = np.zeros(n.value)
+    _check(L.xs_mtx_read(p, flags, ctypes.byref(n), ctypes.byref(nnz), _ptr(rp), _ptr(ci),
+                         _ptr(v), _ptr(d)), "xs_mtx_read")
+    return rp, ci[:nnz.value], v[:nnz.value], d
 
 
 def batch_concat(systems):

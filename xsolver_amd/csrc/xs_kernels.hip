@@ -741,33 +741,44 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
     const int r = UPPER ? r1 - 1 - lane : r0 + lane;
     double acc = 0.0, di = 1.0;
     int eb = 0, ee = 0;  // in-wave entries [eb, ee)
+    int c_eb = -1;       // first in-wave column and its value, loaded before
+    double l_eb = 0.0;   // any wait so that their latency stays off the chain
     if (act) {
       acc = b[r];
       if (dinv) di = dinv[r];
+      int xb, xe;        // entries outside the group
       if (!UPPER) {
-        int e = rp[r];
+        xb = rp[r];
         const int end = dpos[r];
-        int xe = e;
+        xe = xb;
         while (xe < end && ci[xe] < r0) ++xe;
-        for (; e < xe; e += 4) acc = trsv_gather4(acc, e, xe, 1, ci, lu, x, err);
         eb = xe; ee = end;
       } else {
         const int beg = dpos[r] + 1;
-        int e = rp[r + 1] - 1;
-        int xb = e + 1;
+        xe = rp[r + 1];
+        xb = xe;
         while (xb > beg && ci[xb - 1] >= r1) --xb;
-        for (; e >= xb; e -= 4) acc = trsv_gather4(acc, e, xb - 1, -1, ci, lu, x, err);
         eb = beg; ee = xb;
+      }
+      if (eb < ee) {
+        const int ef = UPPER ? ee - 1 : eb;   // the in-wave entry nearest the row
+        c_eb = ci[ef];
+        l_eb = lu[ef];
+      }
+      if (!UPPER) {
+        for (int e = xb; e < xe; e += 4) acc = trsv_gather4(acc, e, xe, 1, ci, lu, x, err);
+      } else {
+        for (int e = xe - 1; e >= xb; e -= 4) acc = trsv_gather4(acc, e, xb - 1, -1, ci, lu, x, err);
       }
     }
     const int nint = ee - eb;
     const int nb = UPPER ? r + 1 : r - 1;
-    const bool chain = !act || nint == 0 || (nint == 1 && ci[eb] == nb);
+    const bool chain = !act || nint == 0 || (nint == 1 && c_eb == nb);
     double val;
     if (__all(chain)) {
       // x_r = c + m * x_nb, composed along the lanes
       double c[1] = {act ? di * acc : 0.0};
-      double m = (act && nint == 1) ? -di * lu[eb] : 0.0;
+      double m = (act && nint == 1) ? -di * l_eb : 0.0;
       wave_affine_scan<1>(c, m);
       val = c[0];
     } else {

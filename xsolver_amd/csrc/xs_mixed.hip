@@ -309,32 +309,43 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
     const int r = UPPER ? r1 - 1 - lane : r0 + lane;
     float acc = 0.f, di = 1.f;
     int eb = 0, ee = 0;
+    int c_eb = -1;       // nearest in-wave entry, loaded before any wait
+    float l_eb = 0.f;
     if (act) {
       acc = b[r];
+      int xb, xe;
       if (!UPPER) {
-        int e = rp[r];
+        xb = rp[r];
         const int end = dpos[r];
-        int xe = e;
+        xe = xb;
         while (xe < end && ci[xe] < r0) ++xe;
-        for (; e < xe; e += 4) acc = trsv_gather4_32(acc, e, xe, 1, ci, lu, x, err);
         eb = xe; ee = end;
       } else {
         di = dinv[r];
         const int beg = dpos[r] + 1;
-        int e = rp[r + 1] - 1;
-        int xb = e + 1;
+        xe = rp[r + 1];
+        xb = xe;
         while (xb > beg && ci[xb - 1] >= r1) --xb;
-        for (; e >= xb; e -= 4) acc = trsv_gather4_32(acc, e, xb - 1, -1, ci, lu, x, err);
         eb = beg; ee = xb;
+      }
+      if (eb < ee) {
+        const int ef = UPPER ? ee - 1 : eb;
+        c_eb = ci[ef];
+        l_eb = fl(lu, ef);
+      }
+      if (!UPPER) {
+        for (int e = xb; e < xe; e += 4) acc = trsv_gather4_32(acc, e, xe, 1, ci, lu, x, err);
+      } else {
+        for (int e = xe - 1; e >= xb; e -= 4) acc = trsv_gather4_32(acc, e, xb - 1, -1, ci, lu, x, err);
       }
     }
     const int nint = ee - eb;
     const int nb = UPPER ? r + 1 : r - 1;
-    const bool chain = !act || nint == 0 || (nint == 1 && ci[eb] == nb);
+    const bool chain = !act || nint == 0 || (nint == 1 && c_eb == nb);
     float val;
     if (__all(chain)) {
       float c = act ? di * acc : 0.f;
-      float m = (act && nint == 1) ? -di * fl(lu, eb) : 0.f;
+      float m = (act && nint == 1) ? -di * l_eb : 0.f;
       wave_affine_scan_f(c, m);
       val = c;
     } else {

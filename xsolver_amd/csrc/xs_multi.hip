@@ -422,6 +422,8 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
     double acc[KC];
     double di = 1.0;
     int eb = 0, ee = 0;  // in-wave entries [eb, ee)
+    int c_eb = -1;       // nearest in-wave entry, loaded before any wait
+    double l_eb = 0.0;
 #pragma unroll
     for (int q = 0; q < KC; ++q) acc[q] = 0.0;
     if (act) {
@@ -442,6 +444,11 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
         xb = xe;
         while (xb > beg && ci[xb - 1] >= r1) --xb;
         eb = beg; ee = xb;
+      }
+      if (eb < ee) {
+        const int ef = UPPER ? ee - 1 : eb;
+        c_eb = ci[ef];
+        l_eb = lu[ef];
       }
       for (int e = xb; e < xe; ++e) {
         const double a = lu[e];
@@ -474,10 +481,10 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
     }
     const int nint = ee - eb;
     const int nb = UPPER ? r + 1 : r - 1;
-    const bool chain = !act || nint == 0 || (nint == 1 && ci[eb] == nb);
+    const bool chain = !act || nint == 0 || (nint == 1 && c_eb == nb);
     if (__all(chain)) {
       // x_r = c + m x_nb per column, composed along the lanes
-      double m = (act && nint == 1) ? -di * lu[eb] : 0.0;
+      double m = (act && nint == 1) ? -di * l_eb : 0.0;
 #pragma unroll
       for (int q = 0; q < KC; ++q) acc[q] = act ? di * acc[q] : 0.0;
       wave_affine_scan<KC>(acc, m);
