@@ -127,6 +127,8 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *                    ILUT, precon_gpuilut.cpp:19-20)
  *   "trsv_schedule"  1 (default) = hand out the triangular-solve row groups in
  *                    dependency-level order, 0 = row order
+ *   "trsv_workgroups" persistent workgroups of a triangular solve (default 256):
+ *                    bounds the number of polling waves
  *   "spmv_kernel"    CSR SpMV kernel: 0 = LDS-staged row blocks (default),
  *                    1 = row-binned vector CSR (lanes per row = mean row length
  *                    rounded up to a power of two, long rows one wavefront each)

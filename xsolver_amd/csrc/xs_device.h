@@ -94,16 +94,22 @@ __device__ __forceinline__ void wave_affine_scan_f(float &c, float &m) {
   affine_step_f<0x143, 0xC>(c, m);
 }
 
-// Work order of the sync-free triangular solves: one ticket per workgroup
-// (the XS_BLOCK / XS_WAVE waves of the workgroup own consecutive 64-row
-// groups), so a wave only ever waits for rows of waves that took an earlier
-// ticket or sit in its own workgroup -- all resident.  One same-address
-// atomic per workgroup instead of per wave (same-address atomics serialise).
-__device__ __forceinline__ unsigned trsv_ticket(unsigned *tickets) {
+// Work order of the sync-free triangular solves.  The grid is persistent and
+// small (a few hundred workgroups): each workgroup repeatedly takes one ticket
+// (one same-address atomic per ticket) and its XS_BLOCK / XS_WAVE waves solve
+// the consecutive 64-row groups of that ticket, until the tickets run out.
+// A wave only ever waits for rows of groups with earlier tickets, held by
+// running workgroups that finish them before taking another ticket, or for
+// rows in its own group: every wait ends.  Few resident waves also means
+// little polling traffic: only the waves near the dependency frontier poll.
+// Returns this wave's ticket slot; `base` is the first slot of the ticket.
+__device__ __forceinline__ unsigned trsv_next(unsigned *tickets, unsigned &base) {
   __shared__ unsigned s_ticket;
+  __syncthreads();   // every wave has read the previous ticket
   if (threadIdx.x == 0) s_ticket = atomicAdd(&tickets[0], 1u);
   __syncthreads();
-  return s_ticket * (XS_BLOCK / XS_WAVE) + threadIdx.x / XS_WAVE;
+  base = s_ticket * (XS_BLOCK / XS_WAVE);
+  return base + threadIdx.x / XS_WAVE;
 }
 // The workgroup that retires last resets both counters for the next launch.
 __device__ __forceinline__ void trsv_retire(unsigned *tickets) {

@@ -69,6 +69,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
   dfree(F.lsched); dfree(F.usched);
   F = xs_ilu();
+  F.trsv_wg = c->trsv_wg;
   dfree(c->lu_tmp);
   c->pc_ready = false;
 }
@@ -581,6 +582,11 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     c->spmv_vec = (int)value;
     c->plan.vec = c->spmv_vec;
     c->planT.vec = c->spmv_vec;
+  }
+  else if (k == "trsv_workgroups") {   // persistent workgroups of the SpTRSV
+    if (value < 1 || value > 65536) return XS_ERR_ARG;
+    c->trsv_wg = (int)value;
+    c->ilu.trsv_wg = c->trsv_wg;
   }
   else if (k == "trsv_schedule") { c->trsv_schedule = value != 0; c->pc_ready = false; free_precond(c); }
   else if (k == "lsolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->lsweeps = (int)value; }
@@ -1424,9 +1430,9 @@ extern "C" int xs_pc_apply_multi(xs_ctx *c, int nrhs, const double *x, double *y
       if ((rc = xs_multi_reserve(c, nrhs, 0))) return rc;
       double *tmp = xs_multi_vec(c, nrhs, 8);
       xs_mtrsv(s, false, n, nrhs, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, nullptr, x, tmp,
-               c->counters + 2, c->ilu.lsched, nullptr);
+               c->counters + 2, c->ilu.lsched, c->ilu.trsv_wg, nullptr);
       xs_mtrsv(s, true, n, nrhs, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, c->ilu.dinv, tmp, y,
-               c->counters + 4, c->ilu.usched, nullptr);
+               c->counters + 4, c->ilu.usched, c->ilu.trsv_wg, nullptr);
       break;
     }
     default:

@@ -102,6 +102,7 @@ struct xs_ilu {
   // SpTRSV work order: the 64-row groups of L (resp. U, counted from the
   // bottom) sorted by dependency level; ticket t solves group sched[t]
   int *lsched = nullptr, *usched = nullptr;
+  int trsv_wg = 256;        // persistent workgroups of the triangular solves
   // transposed factors for the adjoint (built on demand)
   bool have_t = false;
   int *trp = nullptr, *tci = nullptr, *tdpos = nullptr, *tperm = nullptr;

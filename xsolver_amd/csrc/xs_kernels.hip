@@ -729,8 +729,11 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
   if (st != nullptr && st->done) return;
   unsigned *err = tickets + 4;  // wait-expired flag, 4 words past the tickets
   const int lane = threadIdx.x & (XS_WAVE - 1);
-  const unsigned t = trsv_ticket(tickets);
   const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
+  for (;;) {
+  unsigned base;
+  const unsigned t = trsv_next(tickets, base);
+  if (base >= nw) break;
   if (t < nw) {
     const unsigned w = sched ? (unsigned)sched[t] : t;
     int r0, r1;
@@ -798,6 +801,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
     }
     if (act) st_ready(x + r, val);
   }
+  }
   trsv_retire(tickets);
 }
 
@@ -805,31 +809,33 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
 // (xs_fill_sentinel); the kernel leaves no sentinel behind.
 static void trsv_launch(hipStream_t s, bool upper, int n, const int *rp, const int *ci,
                         const double *lu, const int *dpos, const double *dinv, const double *b,
-                        double *x, unsigned *tickets, const int *sched, const xs_state *st) {
-  XS_GUARD(rp && ci && lu && dpos && b && x && tickets);
+                        double *x, unsigned *tickets, const int *sched, int max_wg,
+                        const xs_state *st) {
+  XS_GUARD(rp && ci && lu && dpos && b && x && tickets && max_wg > 0);
   const int nw = (n + XS_WAVE - 1) / XS_WAVE;
-  dim3 grid((nw + (XS_BLOCK / XS_WAVE) - 1) / (XS_BLOCK / XS_WAVE)), blk(XS_BLOCK);
+  const int tk = (nw + (XS_BLOCK / XS_WAVE) - 1) / (XS_BLOCK / XS_WAVE);
+  dim3 grid(tk < max_wg ? tk : max_wg), blk(XS_BLOCK);
   if (upper) hipLaunchKernelGGL(k_trsv<true>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, sched, st);
   else hipLaunchKernelGGL(k_trsv<false>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, sched, st);
 }
 
 void xs_trsv_lower(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
                    const xs_state *st) {
-  trsv_launch(s, false, F.n, F.rp, F.ci, F.lu, F.dpos, nullptr, b, x, tickets, F.lsched, st);
+  trsv_launch(s, false, F.n, F.rp, F.ci, F.lu, F.dpos, nullptr, b, x, tickets, F.lsched, F.trsv_wg, st);
 }
 void xs_trsv_upper(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
                    const xs_state *st) {
-  trsv_launch(s, true, F.n, F.rp, F.ci, F.lu, F.dpos, F.dinv, b, x, tickets, F.usched, st);
+  trsv_launch(s, true, F.n, F.rp, F.ci, F.lu, F.dpos, F.dinv, b, x, tickets, F.usched, F.trsv_wg, st);
 }
 // Transposed factors (F^T = U^T | L^T on the transposed pattern): the lower
 // solve divides by u_ii, the upper one has a unit diagonal.
 void xs_trsv_lower_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
-  trsv_launch(s, false, F.n, F.trp, F.tci, F.tlu, F.tdpos, F.dinv, b, x, tickets, nullptr, st);
+  trsv_launch(s, false, F.n, F.trp, F.tci, F.tlu, F.tdpos, F.dinv, b, x, tickets, nullptr, F.trsv_wg, st);
 }
 void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
-  trsv_launch(s, true, F.n, F.trp, F.tci, F.tlu, F.tdpos, nullptr, b, x, tickets, nullptr, st);
+  trsv_launch(s, true, F.n, F.trp, F.tci, F.tlu, F.tdpos, nullptr, b, x, tickets, nullptr, F.trsv_wg, st);
 }
 
 // ---------------------------------------------------------------------------

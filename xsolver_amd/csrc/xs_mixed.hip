@@ -297,8 +297,11 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
   if (st != nullptr && st->done) return;
   unsigned *err = tickets + 4;
   const int lane = threadIdx.x & (XS_WAVE - 1);
-  const unsigned t = trsv_ticket(tickets);
   const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
+  for (;;) {
+  unsigned base;
+  const unsigned t = trsv_next(tickets, base);
+  if (base >= nw) break;
   if (t < nw) {
     const unsigned w = sched ? (unsigned)sched[t] : t;
     int r0, r1;
@@ -364,6 +367,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
     }
     if (act) st_ready32(x + r, val);
   }
+  }
   trsv_retire(tickets);
 }
 
@@ -419,12 +423,16 @@ static inline int vgrid(int n) {
 
 void xs_trsv32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, const void *lu,
                int ftype, const int *dpos, const float *dinv, const float *b, float *x,
-               unsigned *tickets, const int *sched, const xs_state *st) {
-  if (!(rp && ci && lu && dpos && b && x && tickets) || (upper && !dinv)) { xs_raise_launch_guard(); return; }
+               unsigned *tickets, const int *sched, int max_wg, const xs_state *st) {
+  if (!(rp && ci && lu && dpos && b && x && tickets) || (upper && !dinv) || max_wg <= 0) {
+    xs_raise_launch_guard();
+    return;
+  }
   hipLaunchKernelGGL(k_fill_u32, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s,
                      reinterpret_cast<unsigned *>(x), n, XS_SENTINEL32);
   const int nw = (n + XS_WAVE - 1) / XS_WAVE;
-  dim3 grid((nw + 3) / 4), blk(XS_BLOCK);
+  const int tk = (nw + 3) / 4;
+  dim3 grid(tk < max_wg ? tk : max_wg), blk(XS_BLOCK);
 #define TRSV32(U, T) hipLaunchKernelGGL((k_trsv32<U, T>), grid, blk, 0, s, n, rp, ci, (const T *)lu, dpos, dinv, b, x, tickets, sched, st)
   if (ftype == XS_F32) { if (upper) TRSV32(true, float); else TRSV32(false, float); }
   else if (ftype == XS_BF16) { if (upper) TRSV32(true, __hip_bfloat16); else TRSV32(false, __hip_bfloat16); }

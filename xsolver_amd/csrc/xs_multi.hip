@@ -409,8 +409,11 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
   if (ms && ms->all) return;
   unsigned *err = tickets + 4;
   const int lane = threadIdx.x & (XS_WAVE - 1);
-  const unsigned t = trsv_ticket(tickets);
   const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
+  for (;;) {
+  unsigned base;
+  const unsigned t = trsv_next(tickets, base);
+  if (base >= nw) break;
   if (t < nw) {
     const unsigned w = sched ? (unsigned)sched[t] : t;
     int r0, r1;
@@ -514,6 +517,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
 #pragma unroll
       for (int q = 0; q < KC; ++q) st_ready(xr + q, acc[q]);
     }
+  }
   }
   trsv_retire(tickets);
 }
@@ -636,13 +640,14 @@ static void mtrsv_chunk(hipStream_t s, dim3 grid, int kc, int n, int k, int j0, 
 // 16/8/4/2/1 per launch.  X is sentinel-filled here.
 void xs_mtrsv(hipStream_t s, bool upper, int n, int k, const int *rp, const int *ci,
               const double *lu, const int *dpos, const double *dinv, const double *B, double *X,
-              unsigned *tickets, const int *sched, const xs_mstate *ms) {
-  XS_GUARD(rp && ci && lu && dpos && B && X && tickets && k_ok(k));
+              unsigned *tickets, const int *sched, int max_wg, const xs_mstate *ms) {
+  XS_GUARD(rp && ci && lu && dpos && B && X && tickets && k_ok(k) && max_wg > 0);
   const size_t m = (size_t)n * k;
   hipLaunchKernelGGL(k_mfill, dim3(mgrid_flat(m)), dim3(XS_BLOCK), 0, s, m,
                      reinterpret_cast<unsigned long long *>(X), XS_SENTINEL);
   const int nw = (n + XS_WAVE - 1) / XS_WAVE;
-  const dim3 grid((nw + (XS_BLOCK / XS_WAVE) - 1) / (XS_BLOCK / XS_WAVE));
+  const int tk = (nw + (XS_BLOCK / XS_WAVE) - 1) / (XS_BLOCK / XS_WAVE);
+  const dim3 grid(tk < max_wg ? tk : max_wg);
   for (int j0 = 0; j0 < k;) {
     const int left = k - j0;
     const int kc = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;

@@ -98,9 +98,9 @@ struct multi_run {
       case XS_PC_ILU0:
       case XS_PC_ILUT:
         xs_mtrsv(s, false, n, k, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, nullptr, in, tmp,
-                 c->counters + 2, c->ilu.lsched, ms);
+                 c->counters + 2, c->ilu.lsched, c->ilu.trsv_wg, ms);
         xs_mtrsv(s, true, n, k, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, c->ilu.dinv, tmp,
-                 out, c->counters + 4, c->ilu.usched, ms);
+                 out, c->counters + 4, c->ilu.usched, c->ilu.trsv_wg, ms);
         break;
       default:
         xs_mcopy(s, (size_t)n * k, in, out);
