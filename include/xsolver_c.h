@@ -129,9 +129,11 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *                    dependency-level order, 0 = row order
  *   "trsv_workgroups" persistent workgroups of a triangular solve (default 256):
  *                    bounds the number of polling waves
- *   "spmv_kernel"    CSR SpMV kernel: 0 = LDS-staged row blocks (default),
- *                    1 = row-binned vector CSR (lanes per row = mean row length
- *                    rounded up to a power of two, long rows one wavefront each)
+ *   "spmv_kernel"    CSR SpMV kernel: -1 (default) = vector CSR when the row
+ *                    lengths are regular, else LDS row blocks; 0 = LDS-staged
+ *                    row blocks; 1 = row-binned vector CSR (lanes per row = mean
+ *                    row length rounded up to a power of two, long rows one
+ *                    wavefront each)
  *   "use_ell"        1 = ELL SpMV (same as advanced_pam[1] = 1) */
 int xs_set_option(xs_ctx *ctx, const char *name, double value);
 

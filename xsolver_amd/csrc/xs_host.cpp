@@ -220,6 +220,16 @@ static int make_vec(xs_ctx *c, const std::vector<int> &hrp, xs_spmv_plan &P) {
   return XS_OK;
 }
 
+// Rows regular enough for the vector-CSR kernel by default: no long rows and
+// no row longer than two lane groups (the LDS row blocks cope better with
+// irregular lengths).
+static bool regular_rows(const std::vector<int> &hrp, const xs_spmv_plan &P) {
+  const int n = (int)hrp.size() - 1;
+  int mx = 0;
+  for (int i = 0; i < n; i++) mx = std::max(mx, hrp[i + 1] - hrp[i]);
+  return P.nlong == 0 && mx <= 2 * P.vec_L;
+}
+
 static int make_ell(xs_ctx *c) {
   int rc;
   xs_csr &A = c->A;
@@ -240,7 +250,7 @@ static int build_spmv(xs_ctx *c) {
   if (!A.rp || !A.ci || !A.v || !c->d_int) return XS_ERR_STATE;
   if ((rc = make_plan(c, A, c->plan, xs_default_chunk(A)))) return rc;
   if ((rc = make_vec(c, c->h_rp, c->plan))) return rc;
-  c->plan.vec = c->spmv_vec;
+  c->plan.vec = c->spmv_vec < 0 ? (regular_rows(c->h_rp, c->plan) ? 1 : 0) : c->spmv_vec;
   if (c->use_ell && (rc = make_ell(c))) return rc;
   XS_HIP(hipStreamSynchronize(c->stream));
   if (xs_take_launch_guard()) return XS_ERR_STATE;
@@ -577,11 +587,13 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   else if (k == "parilu_async") { c->parilu_async = value != 0; c->pc_ready = false; }
   else if (k == "check_every") { if (value < 1) return XS_ERR_ARG; c->check_every = (int)value; }
   else if (k == "use_graph") { c->use_graph = value != 0; }
-  else if (k == "spmv_kernel") {   // 0 = LDS row blocks, 1 = row-binned vector CSR
-    if (value != 0 && value != 1) return XS_ERR_ARG;
+  else if (k == "spmv_kernel") {   // -1 = by row regularity, 0 = LDS row blocks, 1 = vector CSR
+    if (value != 0 && value != 1 && value != -1) return XS_ERR_ARG;
     c->spmv_vec = (int)value;
-    c->plan.vec = c->spmv_vec;
-    c->planT.vec = c->spmv_vec;
+    if (c->mat_ready) {
+      c->plan.vec = c->spmv_vec < 0 ? (regular_rows(c->h_rp, c->plan) ? 1 : 0) : c->spmv_vec;
+      c->planT.vec = c->spmv_vec < 0 ? c->plan.vec : c->spmv_vec;
+    }
   }
   else if (k == "trsv_workgroups") {   // persistent workgroups of the SpTRSV
     if (value < 1 || value > 65536) return XS_ERR_ARG;
@@ -1290,7 +1302,7 @@ int xs_build_transpose(xs_ctx *c) {
   xs_permute_vals(c->stream, nnz, c->perm_At, c->A.v, c->At.v);
   if ((rc = make_plan(c, c->At, c->planT, xs_default_chunk(c->At)))) return rc;
   if ((rc = make_vec(c, trp, c->planT))) return rc;
-  c->planT.vec = c->spmv_vec;
+  c->planT.vec = c->spmv_vec < 0 ? (regular_rows(trp, c->planT) ? 1 : 0) : c->spmv_vec;
   XS_HIP(hipStreamSynchronize(c->stream));
   if (xs_take_launch_guard()) return XS_ERR_STATE;
   c->have_At = true;

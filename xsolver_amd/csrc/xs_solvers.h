@@ -39,7 +39,7 @@ struct xs_ctx {
   bool use_graph = true;        // replay batches of iterations as HIP graphs
   bool trsv_schedule = true;    // SpTRSV groups in dependency-level order
   int trsv_wg = 256;            // persistent workgroups of the SpTRSV
-  int spmv_vec = 0;             // CSR kernel: 0 LDS row blocks, 1 row-binned vector CSR
+  int spmv_vec = -1;            // CSR kernel: -1 by row regularity, 0 LDS row blocks, 1 vector CSR
   int precision = 0;
   int use_ell = 0;
 
