@@ -94,6 +94,17 @@ __device__ __forceinline__ void wave_affine_scan_f(float &c, float &m) {
   affine_step_f<0x143, 0xC>(c, m);
 }
 
+// Chunk of work for workgroup `b` of `nb`.  Workgroups are dealt to the 8
+// XCDs round-robin (b % 8); this gives each XCD one contiguous run of chunks,
+// so rows that read neighbouring parts of a vector share one XCD's L2.  It is
+// a bijection on [0, nb) whatever the dispatch does; only locality depends on
+// the round-robin order.
+#define XS_NXCD 8
+__device__ __forceinline__ int xcd_chunk(int b, int nb) {
+  const int xcd = b % XS_NXCD, q = nb / XS_NXCD, rem = nb % XS_NXCD;
+  return xcd * q + (xcd < rem ? xcd : rem) + b / XS_NXCD;
+}
+
 // Work order of the sync-free triangular solves.  The grid is persistent and
 // small (a few hundred workgroups): each workgroup repeatedly takes one ticket
 // (one same-address atomic per ticket) and its XS_BLOCK / XS_WAVE waves solve

@@ -92,8 +92,12 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_csr(
 // group reading its row's entries with consecutive lanes (coalesced across
 // the wave) and combining them with xor shuffles inside the group; the
 // longer rows (listed at set-up) go one wavefront each, in the first
-// workgroups.  Grid-stride over rows with at most XS_MAX_REDUCE_BLOCKS
-// workgroups so the fused dot epilogues reduce over few partials.
+// workgroups.  At most XS_MAX_REDUCE_BLOCKS workgroups (few partials for the
+// fused dot epilogues), each owning one contiguous chunk of rows placed by
+// xcd_chunk; a group takes XS_VEC_UR rows per step and issues the row
+// pointers, then the first entry of every row, then the x gathers together,
+// so the three dependent loads of a row overlap those of the other rows.
+#define XS_VEC_UR 4
 template <int L, int EPI>
 __global__ __launch_bounds__(XS_BLOCK) void k_spmv_vec(
     int n, const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v,
@@ -119,16 +123,43 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_vec(
     }
   } else {
     constexpr int GPB = XS_BLOCK / L;  // row groups per workgroup
+    constexpr int UR = XS_VEC_UR;
     const int sub = threadIdx.x % L;
-    const int stride = ((int)gridDim.x - nlb) * GPB;
-    for (int row = ((int)blockIdx.x - nlb) * GPB + (int)threadIdx.x / L; row < n; row += stride) {
-      const int a = rp[row], e1 = rp[row + 1];
-      if (e1 - a > long_thr) continue;
-      double s = 0.0;
-      for (int e = a + sub; e < e1; e += L) s += v[e] * x[ci[e]];
+    const int G = (int)gridDim.x - nlb;
+    const int rpc = ((n + G - 1) / G + GPB - 1) / GPB * GPB;  // rows per chunk
+    const int c0 = xcd_chunk((int)blockIdx.x - nlb, G) * rpc;
+    const int c1 = min(n, c0 + rpc);
+    for (int row0 = c0 + (int)threadIdx.x / L; row0 < c1; row0 += GPB * UR) {
+      int a[UR], e1[UR];
+      bool on[UR];
 #pragma unroll
-      for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
-      if (sub == 0) epilogue<EPI, NV>(row, s, y, b, w, acc);
+      for (int u = 0; u < UR; ++u) {
+        const int row = row0 + u * GPB;
+        a[u] = row < c1 ? rp[row] : 0;
+        e1[u] = row < c1 ? rp[row + 1] : 0;
+        on[u] = row < c1 && e1[u] - a[u] <= long_thr;
+      }
+      int cc[UR];
+      double vv[UR], s[UR];
+#pragma unroll
+      for (int u = 0; u < UR; ++u) {
+        const int e = a[u] + sub;
+        const bool in = on[u] && e < e1[u];
+        cc[u] = in ? ci[e] : -1;
+        vv[u] = in ? v[e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < UR; ++u) s[u] = cc[u] >= 0 ? vv[u] * x[cc[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < UR; ++u)
+        if (on[u])
+          for (int e = a[u] + sub + L; e < e1[u]; e += L) s[u] += v[e] * x[ci[e]];
+#pragma unroll
+      for (int u = 0; u < UR; ++u) {
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) s[u] += __shfl_xor(s[u], off, L);
+        if (sub == 0 && on[u]) epilogue<EPI, NV>(row0 + u * GPB, s[u], y, b, w, acc);
+      }
     }
   }
   if (RED) reduce_finish<NV>(acc, R);
@@ -156,9 +187,12 @@ static void spmv_vec_launch(hipStream_t s, const xs_csr &A, const xs_spmv_plan &
 }
 
 // ELL SpMV over column-major slabs v[k*ld + row]; two rows per thread with
-// 16-byte loads.  Padding uses the row's own index and a zero value.
+// 16-byte loads, the slabs taken eight at a time (all index and value loads
+// of a batch issued, then all x gathers) and the 512-row blocks placed by
+// xcd_chunk.  Padding uses the row's own index and a zero value.
 // Reference: matMultELL / bmAxELL (DeviceMatrixELL.cpp:3-34), width fixed at
 // 16 there; here the width is the true maximum row length.
+#define XS_ELL_UK 8
 template <int EPI>
 __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell(
     int n, int ld, int W, const int *__restrict__ ci, const double *__restrict__ v,
@@ -170,14 +204,28 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell(
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  const int r = 2 * (blockIdx.x * XS_BLOCK + threadIdx.x);
+  const int r = 2 * (xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x);
   if (r < n) {
     double s0 = 0.0, s1 = 0.0;
-    for (int k = 0; k < W; ++k) {
-      const int2 c = *reinterpret_cast<const int2 *>(ci + (size_t)k * ld + r);
-      const double2 a = *reinterpret_cast<const double2 *>(v + (size_t)k * ld + r);
-      s0 += a.x * x[c.x];
-      s1 += a.y * x[c.y];
+    for (int k0 = 0; k0 < W; k0 += XS_ELL_UK) {
+      int2 c[XS_ELL_UK];
+      double2 a[XS_ELL_UK];
+#pragma unroll
+      for (int q = 0; q < XS_ELL_UK; ++q) {
+        if (k0 + q < W) {
+          c[q] = *reinterpret_cast<const int2 *>(ci + (size_t)(k0 + q) * ld + r);
+          a[q] = *reinterpret_cast<const double2 *>(v + (size_t)(k0 + q) * ld + r);
+        }
+      }
+      double x0[XS_ELL_UK], x1[XS_ELL_UK];
+#pragma unroll
+      for (int q = 0; q < XS_ELL_UK; ++q) {
+        if (k0 + q < W) { x0[q] = x[c[q].x]; x1[q] = x[c[q].y]; }
+      }
+#pragma unroll
+      for (int q = 0; q < XS_ELL_UK; ++q) {
+        if (k0 + q < W) { s0 += a[q].x * x0[q]; s1 += a[q].y * x1[q]; }
+      }
     }
     double s[2] = {s0, s1};
 #pragma unroll

@@ -96,17 +96,45 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_vec32(
       if (lane == 0) epi32<EPI, NV>(r, (float)s, y, b, w, acc);
     }
   } else {
+    // same chunking and load batching as the FP64 vector-CSR kernel
     constexpr int GPB = XS_BLOCK / L;
+    constexpr int UR = 4;
     const int sub = threadIdx.x % L;
-    const int stride = ((int)gridDim.x - nlb) * GPB;
-    for (int row = ((int)blockIdx.x - nlb) * GPB + (int)threadIdx.x / L; row < n; row += stride) {
-      const int a = rp[row], e1 = rp[row + 1];
-      if (e1 - a > long_thr) continue;
-      float s = 0.f;
-      for (int e = a + sub; e < e1; e += L) s += v[e] * x[ci[e]];
+    const int G = (int)gridDim.x - nlb;
+    const int rpc = ((n + G - 1) / G + GPB - 1) / GPB * GPB;
+    const int c0 = xcd_chunk((int)blockIdx.x - nlb, G) * rpc;
+    const int c1 = min(n, c0 + rpc);
+    for (int row0 = c0 + (int)threadIdx.x / L; row0 < c1; row0 += GPB * UR) {
+      int a[UR], e1[UR];
+      bool on[UR];
 #pragma unroll
-      for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
-      if (sub == 0) epi32<EPI, NV>(row, s, y, b, w, acc);
+      for (int u = 0; u < UR; ++u) {
+        const int row = row0 + u * GPB;
+        a[u] = row < c1 ? rp[row] : 0;
+        e1[u] = row < c1 ? rp[row + 1] : 0;
+        on[u] = row < c1 && e1[u] - a[u] <= long_thr;
+      }
+      int cc[UR];
+      float vv[UR], s[UR];
+#pragma unroll
+      for (int u = 0; u < UR; ++u) {
+        const int e = a[u] + sub;
+        const bool in = on[u] && e < e1[u];
+        cc[u] = in ? ci[e] : -1;
+        vv[u] = in ? v[e] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < UR; ++u) s[u] = cc[u] >= 0 ? vv[u] * x[cc[u]] : 0.f;
+#pragma unroll
+      for (int u = 0; u < UR; ++u)
+        if (on[u])
+          for (int e = a[u] + sub + L; e < e1[u]; e += L) s[u] += v[e] * x[ci[e]];
+#pragma unroll
+      for (int u = 0; u < UR; ++u) {
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) s[u] += __shfl_xor(s[u], off, L);
+        if (sub == 0 && on[u]) epi32<EPI, NV>(row0 + u * GPB, s[u], y, b, w, acc);
+      }
     }
   }
   if (RED) reduce_finish<NV>(acc, R);
@@ -145,14 +173,29 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell32(
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  const int r = 2 * (blockIdx.x * XS_BLOCK + threadIdx.x);
+  const int r = 2 * (xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x);
   if (r < n) {
     float s0 = 0.f, s1 = 0.f;
-    for (int k = 0; k < W; ++k) {
-      const int2 c = *reinterpret_cast<const int2 *>(ci + (size_t)k * ld + r);
-      const float2 a = *reinterpret_cast<const float2 *>(v + (size_t)k * ld + r);
-      s0 += a.x * x[c.x];
-      s1 += a.y * x[c.y];
+    constexpr int UK = 8;   // slabs per batch: all loads of a batch, then all gathers
+    for (int k0 = 0; k0 < W; k0 += UK) {
+      int2 c[UK];
+      float2 a[UK];
+#pragma unroll
+      for (int q = 0; q < UK; ++q) {
+        if (k0 + q < W) {
+          c[q] = *reinterpret_cast<const int2 *>(ci + (size_t)(k0 + q) * ld + r);
+          a[q] = *reinterpret_cast<const float2 *>(v + (size_t)(k0 + q) * ld + r);
+        }
+      }
+      float x0[UK], x1[UK];
+#pragma unroll
+      for (int q = 0; q < UK; ++q) {
+        if (k0 + q < W) { x0[q] = x[c[q].x]; x1[q] = x[c[q].y]; }
+      }
+#pragma unroll
+      for (int q = 0; q < UK; ++q) {
+        if (k0 + q < W) { s0 += a[q].x * x0[q]; s1 += a[q].y * x1[q]; }
+      }
     }
     if (r < n) epi32<EPI, NV>(r, s0, y, b, w, acc);
     if (r + 1 < n) epi32<EPI, NV>(r + 1, s1, y, b, w, acc);
