@@ -232,22 +232,28 @@ def test_gmres_parilut_solve_parity(dev):
     assert relres < 1.05e-10
 
 
-@pytest.mark.parametrize("kl,ku", [(0, 0), (1, 3), (20, 20), (10, 15)])
-def test_ilu_sweep_apply_matches_oracle(dev, kl, ku):
+@pytest.mark.parametrize("pc,kl,ku", [("ilu0", 0, 0), ("ilu0", 1, 3), ("ilu0", 2, 2),
+                                       ("ilu0", 20, 20), ("ilu0", 10, 15), ("ilut", 2, 3),
+                                       ("ilut", 10, 15)])
+def test_ilu_sweep_apply_matches_oracle(dev, pc, kl, ku):
     """The reference's iterative GPU ILU apply (kl / ku Jacobi sweeps per
-    factor) against the oracle's restatement on the GPU's own factors."""
+    factor) against the oracle's restatement on the GPU's own factors; two or
+    more sweeps run over the ELL copies of L and U (first two fused), one or
+    none over the CSR factors."""
     import torch
     A = gen.convdiff3d(16)
     n = len(A[0]) - 1
-    say(f"[sweeps] kl={kl} ku={ku}")
+    say(f"[sweeps] {pc} kl={kl} ku={ku}")
     S = XSolver()
     S.mat_setup(*A)
     S.ksp_settype("bicgstab")
-    S.pc_settype("ilu0")
+    S.pc_settype(pc)
+    if pc == "ilut":
+        S.pc_setilut(1e-3, 3)
     S.set_option("lsolve_sweeps", kl)
     S.set_option("usolve_sweeps", ku)
     S.pc_setup()
-    rp, ci, lu, dpos = S.factors(n, len(A[2]))
+    rp, ci, lu, dpos = S.factors(n, 8 * len(A[2]))
     b = np.random.default_rng(6).uniform(-1, 1, n)
     yd = torch.empty(n, dtype=torch.float64, device=dev)
     S.pc_apply(dev_tensor(b, dev), yd, torch_stream())

@@ -290,18 +290,20 @@ def test_mixed_gmres_parilut(dev, factor):
     assert st == 0 and rr < 1.05e-10 and err < 1e-7
 
 
+@pytest.mark.parametrize("sweeps", [20, 2])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
-def test_iterative_ilu_apply_solve(dev, precision):
-    """BiCGSTAB + ILU(0) with the reference's 20/20 Jacobi-sweep apply:
-    iterations within +-2 of the oracle running the same apply, FP64 true
-    residual below rtol (mixed: FP32 sweeps inside FP64 refinement)."""
+def test_iterative_ilu_apply_solve(dev, precision, sweeps):
+    """BiCGSTAB + ILU(0) with the reference's Jacobi-sweep apply (its 20/20
+    and a 2/2 apply): iterations within +-2 of the oracle running the same
+    apply, FP64 true residual below rtol (mixed: FP32 sweeps inside FP64
+    refinement)."""
     A = gen.convdiff3d(24)
     xs, b = gen.manufactured(A, 1)
     xo, sto, ito, _ = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=1000,
-                                   lsweeps=20, usweeps=20)
+                                   lsweeps=sweeps, usweeps=sweeps)
     g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000, parilu_async=0, parilu_sweeps=60,
-                  lsolve_sweeps=20, usolve_sweeps=20, precision=precision)
-    say(f"[sweep-solve] precision={precision} oracle it={ito} gpu it={g['it']} "
+                  lsolve_sweeps=sweeps, usolve_sweeps=sweeps, precision=precision)
+    say(f"[sweep-solve] precision={precision} sweeps={sweeps} oracle it={ito} gpu it={g['it']} "
         f"relres={true_relres(A, b, g['x']):.2e}")
     assert sto == 0 and g["st"] == 0
     if precision == 0:

@@ -68,6 +68,9 @@ static void free_precond(xs_ctx *c) {
   dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own); dfree(F.trp); dfree(F.tci);
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
   dfree(F.lsched); dfree(F.usched);
+  dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.v32);
+  dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.v32);
+  c->tell_ready = c->tell32_ready = false;
   F = xs_ilu();
   F.trsv_wg = c->trsv_wg;
   dfree(c->lu_tmp);
@@ -528,6 +531,7 @@ extern "C" int xs_mat_update_values(xs_ctx *c, const double *val, int on_device)
   if (c->have_At) xs_permute_vals(c->stream, c->A.nnz, c->perm_At, c->A.v, c->At.v);
   c->pc_ready = false;
   c->mixed_ready = false;
+  c->tell_ready = c->tell32_ready = false;
   XS_HIP(hipStreamSynchronize(c->stream));
   return XS_OK;
 }
@@ -603,7 +607,7 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   else if (k == "trsv_schedule") { c->trsv_schedule = value != 0; c->pc_ready = false; free_precond(c); }
   else if (k == "lsolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->lsweeps = (int)value; }
   else if (k == "usolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->usweeps = (int)value; }
-  else if (k == "refactor") { c->pc_ready = false; c->mixed_ready = false; }
+  else if (k == "refactor") { c->pc_ready = false; c->mixed_ready = false; c->tell_ready = c->tell32_ready = false; }
   else if (k == "factor_storage") { if (value < 0 || value > 2) return XS_ERR_ARG; c->factor_type = (int)value; c->mixed_ready = false; }
   else if (k == "inner_tol") { if (!(value > 0 && value < 1)) return XS_ERR_ARG; c->inner_tol = value; }
   else if (k == "parilu_scale") { c->parilu_scale = value != 0; c->pc_ready = false; }
@@ -951,6 +955,7 @@ extern "C" int xs_pc_setup(xs_ctx *c) {
   if (rc == XS_OK) c->pc_ready = true;
   c->ilu.have_t = false;
   c->mixed_ready = false;   // FP32 copies of A and of the factors are rebuilt
+  c->tell_ready = c->tell32_ready = false;   // and the sweep copies of the factors
   return rc;
 }
 
@@ -1021,6 +1026,55 @@ static void ilu_sweeps(xs_ctx *c, bool upper, int k, const double *b, double *ou
   }
 }
 
+// ELL copies of L and U for the sweep apply (pattern and values of the
+// current factors); built before a solve so that no allocation happens inside
+// a captured iteration batch.
+static int ensure_tri_ell(xs_ctx *c) {
+  if (c->tell_ready) return XS_OK;
+  xs_ilu &F = c->ilu;
+  int rc, w[2];
+  if (xs_tri_widths(c->stream, F, c->d_int, &w[0], &w[1])) return XS_ERR_DEVICE;
+  for (int part = 0; part < 2; ++part) {
+    xs_tell &E = part ? F.tu : F.tl;
+    const size_t need = (size_t)w[part] * F.n;
+    if (need > E.cap) {
+      dfree(E.ci); dfree(E.v); dfree(E.v32);
+      E.cap = 0;
+      if ((rc = dalloc(&E.ci, need)) || (rc = dalloc(&E.v, need))) return rc;
+      E.cap = need;
+    }
+    E.W = w[part];
+    E.ld = F.n;
+    xs_build_tell(c->stream, F, part == 1, E);
+  }
+  XS_HIP(hipStreamSynchronize(c->stream));
+  if (xs_take_launch_guard()) return XS_ERR_STATE;
+  c->tell_ready = true;
+  c->tell32_ready = false;
+  c->mixed_ready = false;   // the FP32 copies are refreshed with the sweep values
+  return XS_OK;
+}
+
+// k >= 2 sweeps over the ELL copy: sweeps 1 and 2 fused, then k - 2 more,
+// ping-ponging so that the last lands in out (identical results to
+// ilu_sweeps)
+static void ilu_sweeps_ell(xs_ctx *c, bool upper, int k, const double *b, double *out,
+                           double *w, const xs_state *st) {
+  double *cur = ((k - 2) % 2 == 0) ? out : w;
+  xs_tri_sweep_ell(c->stream, c->ilu, upper, true, b, nullptr, cur, st);
+  for (int j = 3; j <= k; ++j) {
+    double *nxt = (cur == out) ? w : out;
+    xs_tri_sweep_ell(c->stream, c->ilu, upper, false, b, cur, nxt, st);
+    cur = nxt;
+  }
+}
+
+static void apply_sweeps(xs_ctx *c, bool upper, int k, const double *b, double *out, double *w,
+                         const xs_state *st) {
+  if (k >= 2 && c->tell_ready) ilu_sweeps_ell(c, upper, k, b, out, w, st);
+  else ilu_sweeps(c, upper, k, b, out, w, st);
+}
+
 // y = M^{-1} x on the device (tmp: scratch vector for the ILU forward solve)
 void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state *st) {
   switch (c->pc) {
@@ -1038,7 +1092,7 @@ void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state
       }
       xs_prof_begin(c, 2);
       if (c->lsweeps >= 0) {
-        ilu_sweeps(c, false, c->lsweeps, x, tmp, xs_ctx_vec(c, 10), st);
+        apply_sweeps(c, false, c->lsweeps, x, tmp, xs_ctx_vec(c, 10), st);
       } else {
         xs_fill_sentinel(c->stream, tmp, c->A.n);
         xs_trsv_lower(c->stream, c->ilu, x, tmp, c->counters + 2, st);
@@ -1046,7 +1100,7 @@ void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state
       xs_prof_end(c, 2);
       xs_prof_begin(c, 3);
       if (c->usweeps >= 0) {
-        ilu_sweeps(c, true, c->usweeps, tmp, y, xs_ctx_vec(c, 10), st);
+        apply_sweeps(c, true, c->usweeps, tmp, y, xs_ctx_vec(c, 10), st);
       } else {
         xs_fill_sentinel(c->stream, y, c->A.n);
         xs_trsv_upper(c->stream, c->ilu, tmp, y, c->counters + 4, st);
@@ -1096,6 +1150,9 @@ static int prepare_solve(xs_ctx *c) {
   if (c->maxiter <= 0) return XS_ERR_STATE;
   if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
   if ((rc = ensure_common(c))) return rc;
+  if ((c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) && (c->lsweeps >= 2 || c->usweeps >= 2) &&
+      (rc = ensure_tri_ell(c)))
+    return rc;
   return XS_OK;
 }
 
@@ -1399,6 +1456,9 @@ extern "C" int xs_pc_apply(xs_ctx *c, const double *x, double *y, void *stream) 
   int rc;
   if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
   if ((rc = xs_ctx_ensure_work(c, XS_NWORK))) return rc;
+  if ((c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) && (c->lsweeps >= 2 || c->usweeps >= 2) &&
+      (rc = ensure_tri_ell(c)))
+    return rc;
   hipStream_t keep = c->stream;
   c->stream = pick(c, stream);
   xs_op_pc(c, x, y, xs_ctx_vec(c, XS_NWORK - 1), nullptr);

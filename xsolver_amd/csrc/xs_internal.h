@@ -88,6 +88,17 @@ struct xs_ell {         // column-major slabs: v[k * ld + row], width W
 
 // Combined ILU factors aligned with a CSR pattern: strict lower part of row i
 // in [rp[i], dpos[i]), pivot at dpos[i], strict upper part in (dpos[i], rp[i+1]).
+// ELL copy of the strictly lower (or strictly upper) part of the factors for
+// the Jacobi-sweep apply: W column-major slabs of ld entries (entry k of row i
+// at k * ld + i, in the row's column order), padding column -1 / value 0.
+struct xs_tell {
+  int W = 0, ld = 0;
+  size_t cap = 0;            // allocated entries
+  int *ci = nullptr;
+  double *v = nullptr;
+  float *v32 = nullptr;      // FP32 values for the mixed-precision inner solver
+};
+
 struct xs_ilu {
   int n = 0, nnz = 0;
   const int *rp = nullptr, *ci = nullptr;  // pattern (owned elsewhere or below)
@@ -103,6 +114,7 @@ struct xs_ilu {
   // bottom) sorted by dependency level; ticket t solves group sched[t]
   int *lsched = nullptr, *usched = nullptr;
   int trsv_wg = 256;        // persistent workgroups of the triangular solves
+  xs_tell tl, tu;           // sweep copies of L and U (built on demand)
   // transposed factors for the adjoint (built on demand)
   bool have_t = false;
   int *trp = nullptr, *tci = nullptr, *tdpos = nullptr, *tperm = nullptr;
@@ -163,6 +175,13 @@ void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
 // one Jacobi sweep of a factor: yn = D (b - T_off yo); yo == null: yn = D b
 void xs_tri_sweep(hipStream_t s, const xs_ilu &F, bool upper, const double *b, const double *yo,
                   double *yn, const xs_state *st);
+// Jacobi sweeps over the ELL copies: fused = sweeps 1 and 2 in one pass
+// (yo implied: b for L, D b for U), else one sweep from yo.  xs_tri_widths
+// returns the widths of L and U (synchronous; nonzero on failure).
+int xs_tri_widths(hipStream_t s, const xs_ilu &F, int *d_tmp, int *wl, int *wu);
+void xs_build_tell(hipStream_t s, const xs_ilu &F, bool upper, xs_tell &E);
+void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, const double *b,
+                      const double *yo, double *yn, const xs_state *st);
 void xs_permute_vals(hipStream_t s, int nnz, const int *perm, const double *src, double *dst);
 void xs_grad_vals(hipStream_t s, const xs_csr &A, const double *lam, const double *x, double *g);
 void xs_jacobi_dinv(hipStream_t s, const xs_csr &A, double *dinv, int *d_bad);

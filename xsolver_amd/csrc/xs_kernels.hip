@@ -887,6 +887,119 @@ void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
 }
 
 // ---------------------------------------------------------------------------
+// Jacobi sweeps over ELL copies of the factor parts (same arithmetic as
+// k_tri_sweep, entry by entry in column order, so the results are identical;
+// coalesced slab loads instead of a thread walking its CSR row).
+// ---------------------------------------------------------------------------
+// strictly-lower / strictly-upper entry counts: maxima into w[0], w[1]
+__global__ void k_tri_widths(int n, const int *__restrict__ rp, const int *__restrict__ dpos,
+                             int *w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int lo = 0, up = 0;
+  if (i < n) { lo = dpos[i] - rp[i]; up = rp[i + 1] - dpos[i] - 1; }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = max(lo, __shfl_xor(lo, off, XS_WAVE));
+    up = max(up, __shfl_xor(up, off, XS_WAVE));
+  }
+  if ((threadIdx.x & (XS_WAVE - 1)) == 0) { atomicMax(&w[0], lo); atomicMax(&w[1], up); }
+}
+
+int xs_tri_widths(hipStream_t s, const xs_ilu &F, int *d_tmp, int *wl, int *wu) {
+  if (!F.rp || !F.dpos || !d_tmp) { xs_raise_launch_guard(); return 1; }
+  int h[2] = {0, 0};
+  if (hipMemsetAsync(d_tmp, 0, 2 * sizeof(int), s) != hipSuccess) return 1;
+  hipLaunchKernelGGL(k_tri_widths, dim3((F.n + 255) / 256), dim3(256), 0, s, F.n, F.rp, F.dpos, d_tmp);
+  if (hipMemcpyAsync(h, d_tmp, 2 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return 1;
+  *wl = h[0];
+  *wu = h[1];
+  return 0;
+}
+
+template <bool UPPER>
+__global__ void k_build_tell(int n, int ld, int W, const int *__restrict__ rp,
+                             const int *__restrict__ ci, const double *__restrict__ lu,
+                             const int *__restrict__ dpos, int *__restrict__ eci,
+                             double *__restrict__ ev) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ld) return;
+  int a = 0, len = 0;
+  if (i < n) {
+    a = UPPER ? dpos[i] + 1 : rp[i];
+    len = UPPER ? rp[i + 1] - dpos[i] - 1 : dpos[i] - rp[i];
+  }
+  for (int k = 0; k < W; ++k) {
+    const size_t p = (size_t)k * ld + i;
+    if (k < len) { eci[p] = ci[a + k]; ev[p] = lu[a + k]; }
+    else { eci[p] = -1; ev[p] = 0.0; }
+  }
+}
+
+void xs_build_tell(hipStream_t s, const xs_ilu &F, bool upper, xs_tell &E) {
+  XS_GUARD(F.rp && F.ci && F.lu && F.dpos && (E.W == 0 || (E.ci && E.v)) && E.ld >= F.n);
+  if (E.W == 0) return;
+  const dim3 grid((E.ld + 255) / 256), blk(256);
+  if (upper)
+    hipLaunchKernelGGL(k_build_tell<true>, grid, blk, 0, s, F.n, E.ld, E.W, F.rp, F.ci, F.lu, F.dpos, E.ci, E.v);
+  else
+    hipLaunchKernelGGL(k_build_tell<false>, grid, blk, 0, s, F.n, E.ld, E.W, F.rp, F.ci, F.lu, F.dpos, E.ci, E.v);
+}
+
+// yn = D (b - T yo), T given by its ELL slabs; FUSED: yo = b (L) or D b (U),
+// i.e. sweeps 1 and 2 from y = 0 in one pass.  Slabs four at a time: all
+// index/value loads of a batch, then all gathers.  Rows in 256-row blocks
+// placed by xcd_chunk.
+template <bool UPPER, bool FUSED>
+__global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W,
+                                                            const int *__restrict__ eci,
+                                                            const double *__restrict__ ev,
+                                                            const double *__restrict__ dinv,
+                                                            const double *__restrict__ b,
+                                                            const double *__restrict__ yo,
+                                                            double *__restrict__ yn,
+                                                            const xs_state *st) {
+  if (st != nullptr && st->done) return;
+  const int i = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
+  if (i >= n) return;
+  double r = 0.0;
+  for (int k0 = 0; k0 < W; k0 += 4) {
+    int c[4];
+    double a[4], y[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      c[q] = k0 + q < W ? eci[(size_t)(k0 + q) * ld + i] : -1;
+      a[q] = c[q] >= 0 ? ev[(size_t)(k0 + q) * ld + i] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (c[q] >= 0) {
+        if (FUSED) y[q] = UPPER ? dinv[c[q]] * (b[c[q]] - 0.0) : b[c[q]] - 0.0;
+        else y[q] = yo[c[q]];
+      } else {
+        y[q] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c[q] >= 0) r += a[q] * y[q];
+  }
+  yn[i] = UPPER ? dinv[i] * (b[i] - r) : b[i] - r;
+}
+
+void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, const double *b,
+                      const double *yo, double *yn, const xs_state *st) {
+  const xs_tell &E = upper ? F.tu : F.tl;
+  XS_GUARD((E.W == 0 || (E.ci && E.v)) && E.ld >= F.n && b && yn && (!upper || F.dinv) &&
+           (fused || yo));
+  const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+#define XS_SWE(U, FU) hipLaunchKernelGGL((k_tri_sweep_ell<U, FU>), grid, blk, 0, s, F.n, E.ld, E.W, E.ci, E.v, F.dinv, b, yo, yn, st)
+  if (upper) { if (fused) XS_SWE(true, true); else XS_SWE(true, false); }
+  else { if (fused) XS_SWE(false, true); else XS_SWE(false, false); }
+#undef XS_SWE
+}
+
+// ---------------------------------------------------------------------------
 // Iterative application of one ILU factor: the reference's GPU ILU apply
 // (DeviceMatrixCSR::Lsolve_iter / Usolve_iter,
 // hip/HostMatrix/DeviceMatrix/DeviceMatrixCSR.cpp:281-304, kernel pairs

@@ -454,6 +454,60 @@ void xs_tri_sweep32(hipStream_t s, bool upper, int n, const int *rp, const int *
 #undef XS_SW
 }
 
+// FP32 Jacobi sweep over the ELL copy of a factor part (FP32 values); the
+// arithmetic of k_tri_sweep32 entry by entry, FUSED as in the FP64 kernel.
+template <bool UPPER, bool FUSED>
+__global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int W,
+                                                              const int *__restrict__ eci,
+                                                              const float *__restrict__ ev,
+                                                              const float *__restrict__ dinv,
+                                                              const float *__restrict__ b,
+                                                              const float *__restrict__ yo,
+                                                              float *__restrict__ yn,
+                                                              const xs_state *st) {
+  if (st != nullptr && st->done) return;
+  const int i = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
+  if (i >= n) return;
+  float r = 0.f;
+  for (int k0 = 0; k0 < W; k0 += 4) {
+    int c[4];
+    float a[4], y[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      c[q] = k0 + q < W ? eci[(size_t)(k0 + q) * ld + i] : -1;
+      a[q] = c[q] >= 0 ? ev[(size_t)(k0 + q) * ld + i] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (c[q] >= 0) {
+        if (FUSED) y[q] = UPPER ? dinv[c[q]] * (b[c[q]] - 0.f) : b[c[q]] - 0.f;
+        else y[q] = yo[c[q]];
+      } else {
+        y[q] = 0.f;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c[q] >= 0) r += a[q] * y[q];
+  }
+  yn[i] = UPPER ? dinv[i] * (b[i] - r) : b[i] - r;
+}
+
+void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_tell &E,
+                        const float *dinv, const float *b, const float *yo, float *yn,
+                        const xs_state *st) {
+  if (!((E.W == 0 || (E.ci && E.v32)) && E.ld >= n && b && yn && (!upper || dinv) &&
+        (fused || yo))) {
+    xs_raise_launch_guard();
+    return;
+  }
+  const dim3 grid((n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+#define XS_SWE32(U, FU) hipLaunchKernelGGL((k_tri_sweep_ell32<U, FU>), grid, blk, 0, s, n, E.ld, E.W, E.ci, E.v32, dinv, b, yo, yn, st)
+  if (upper) { if (fused) XS_SWE32(true, true); else XS_SWE32(true, false); }
+  else { if (fused) XS_SWE32(false, true); else XS_SWE32(false, false); }
+#undef XS_SWE32
+}
+
 __global__ void k_fill_u32(unsigned *p, int n, unsigned val) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = val;
 }

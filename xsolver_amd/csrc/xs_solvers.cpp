@@ -366,6 +366,16 @@ static int mixed_prepare(xs_ctx *c) {
     xs_cvt_factor(s, c->ilu.nnz, c->ilu.lu, c->F32, c->factor_type);
     if ((rc = ensure_buf(&c->dinv32, (size_t)n))) return rc;
     xs_vec_d2f(s, n, c->ilu.dinv, c->dinv32);
+    c->tell32_ready = false;
+    if (c->tell_ready && c->factor_type == XS_F32) {   // FP32 values of the sweep copies
+      for (xs_tell *E : {&c->ilu.tl, &c->ilu.tu}) {
+        if (E->W == 0) continue;
+        if (!E->v32 && hipMalloc(&E->v32, sizeof(float) * E->cap) != hipSuccess)
+          return XS_ERR_MEMORY;
+        xs_d2f(s, E->W * E->ld, E->v, E->v32);
+      }
+      c->tell32_ready = true;
+    }
   }
   if (c->work32_n != (size_t)n) {
     if (c->work32) (void)hipFree(c->work32);
@@ -388,10 +398,22 @@ static void op_spmv32(xs_ctx *c, const float *x, float *y, int epi, const float 
   xs_prof_end(c, 1);
 }
 
-// FP32 counterpart of the host's ilu_sweeps (k Jacobi sweeps of one factor)
+// FP32 counterpart of the host's ilu_sweeps (k Jacobi sweeps of one factor);
+// k >= 2 over the FP32 ELL copies when they are current (sweeps 1 and 2 fused)
 static void ilu_sweeps32(xs_ctx *c, bool upper, int k, const float *b, float *out, float *w,
                          const xs_state *st) {
   const int n = c->A.n, m = k < 1 ? 1 : k;
+  if (m >= 2 && c->tell32_ready) {
+    const xs_tell &E = upper ? c->ilu.tu : c->ilu.tl;
+    float *cur = ((m - 2) % 2 == 0) ? out : w;
+    xs_tri_sweep_ell32(c->stream, upper, true, n, E, c->dinv32, b, nullptr, cur, st);
+    for (int j = 3; j <= m; ++j) {
+      float *nxt = (cur == out) ? w : out;
+      xs_tri_sweep_ell32(c->stream, upper, false, n, E, c->dinv32, b, cur, nxt, st);
+      cur = nxt;
+    }
+    return;
+  }
   float *cur = ((m - 1) % 2 == 0) ? out : w;
   xs_tri_sweep32(c->stream, upper, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type, c->ilu.dpos,
                  c->dinv32, b, nullptr, cur, st);

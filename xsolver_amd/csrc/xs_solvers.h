@@ -98,6 +98,8 @@ struct xs_ctx {
   int factor_type = 0;          // XS_F32 / XS_BF16 / XS_F16
   double inner_tol = 1e-4;      // relative reduction asked of each inner solve
   bool mixed_ready = false;
+  bool tell_ready = false;      // ELL sweep copies of the factors match ilu.lu
+  bool tell32_ready = false;    // and their FP32 values (mixed inner solver)
   float *A32 = nullptr, *ell32 = nullptr, *dinv32 = nullptr, *jac32 = nullptr;
   void *F32 = nullptr;
   float *work32 = nullptr;
