@@ -4,14 +4,18 @@ Workload (one "step" = one complete solve of one system with the matrix and
 right-hand side already resident in HBM): 3D 7-point convection-diffusion on a
 128^3 grid (n = 2,097,152, nnz = 14,581,760; diffusion 1, velocity direction
 (1, 0.5, 0.25) scaled to cell Peclet 0.5, central differences), b = A x* with
-x* ~ U[-1, 1] (seed 1), x0 = 0, BiCGSTAB + ParILU(0) (20 sweeps, factorisation
-inside the step), FP64, CSR SpMV, to relative residual 1e-10.  The ELL format
-is timed beside it.
+x* ~ U[-1, 1] (seed 1), x0 = 0, CSR input, BiCGSTAB + ParILU(0) (20
+asynchronous sweeps, factorisation inside the step), FP64, to relative
+residual 1e-10.  The ILU factors are applied the way the reference's GPU
+backend applies them -- Jacobi sweeps per factor -- with 4 sweeps per factor
+in the headline configuration (the reference hard-codes 20); the exact
+sync-free SpTRSV apply, the reference's 20/20 sweeps and the FP32-inner mixed
+solve are timed beside it.
 
 Printed: one JSON line (value = seconds per solve, lower is better) with the
-dominant kernel's roofline (algorithmic bytes / mean kernel time from HIP
-events recorded on the library's stream during the timed steps) and a CPU
-baseline (the oracle's C restatement, one core, on a bounded sample).
+dominant kernel's roofline (algorithmic bytes / mean time from HIP events
+recorded on the library's stream during the timed steps) and a CPU baseline
+(the oracle's C restatement of the same solve, one core, on a bounded sample).
 """
 import argparse
 import json
@@ -28,12 +32,24 @@ METRIC = "time-to-rtol 1e-10 (s) + SpMV/SpTRSV HBM GB/s; mixed-prec vs FP64 on G
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(n, nnz, nnz_l, nnz_u, ell_w):
+APPLY = (4, 4)   # Jacobi sweeps per factor (L, U) in the headline configuration
+
+
+def algorithmic_bytes(n, nnz, nnz_l, nnz_u, sweeps):
+    """Bytes each kernel (or apply sequence) must move at least: 12 B per
+    stored entry (value + column), 8 B per vector entry read or written.
+    nnz_l / nnz_u: strictly lower / upper entries of the factors.  An apply of
+    k >= 2 sweeps is one fused launch (sweeps 1-2, no previous iterate) plus
+    k - 2 plain ones; U also reads 1/u_ii."""
+    kl, ku = sweeps
+    l_fused, l_plain = 12 * nnz_l + 16 * n, 12 * nnz_l + 24 * n
+    u_fused, u_plain = 12 * nnz_u + 24 * n, 12 * nnz_u + 32 * n
     return {
-        "spmv_csr": 12 * nnz + 4 * (n + 1) + 16 * n,
-        "spmv_ell": 12 * n * ell_w + 16 * n,
-        "trsv_lower": 12 * nnz_l + 4 * (n + 1) + 16 * n,
-        "trsv_upper": 12 * nnz_u + 4 * (n + 1) + 16 * n,
+        "spmv": 12 * nnz + 16 * n,                  # ELL / CSR values+columns, x, y
+        "ilu_lower_apply": l_fused + (kl - 2) * l_plain,
+        "ilu_upper_apply": u_fused + (ku - 2) * u_plain,
+        "trsv_lower_exact": 12 * nnz_l + 4 * (n + 1) + 4 * n + 16 * n,
+        "trsv_upper_exact": 12 * nnz_u + 4 * (n + 1) + 4 * n + 24 * n,
     }
 
 
@@ -96,34 +112,46 @@ def run_gpu(A, b, fmt, steps, warmup, profile=True, precision=0, factor_storage=
     }
 
 
-# kernel families timed by the library -> kernel names in the rocprofv3 summary
-PROFILE_NAMES = {"spmv": "k_spmv_csr<2>", "trsv_lower": "k_trsv<false>",
-                 "trsv_upper": "k_trsv<true>", "spmv_ell": "k_spmv_ell<2>"}
+# library timing families -> (kernel name in the rocprofv3 summary, launches)
+PROFILE_NAMES = {
+    "spmv": [("k_spmv_ell<2>", 1)],
+    "ilu_lower_apply": [("k_tri_sweep_ell<false, true>", 1), ("k_tri_sweep_ell<false, false>", APPLY[0] - 2)],
+    "ilu_upper_apply": [("k_tri_sweep_ell<true, true>", 1), ("k_tri_sweep_ell<true, false>", APPLY[1] - 2)],
+    "trsv_lower_exact": [("k_trsv<false>", 1)],
+    "trsv_upper_exact": [("k_trsv<true>", 1)],
+}
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` on the 128^3 workload from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+def measured_traffic(family):
+    """HBM bytes per launch (per apply sequence) of `family` on the 128^3
+    workload from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (copy-calibrated), or None."""
     path = os.path.join(ROOT, "profiles", "convdiff3d_128_hbm_traffic.md")
     try:
         text = open(path).read()
-        block = text.split("```json", 1)[1].split("```", 1)[0]
-        return json.loads(block).get(PROFILE_NAMES.get(kernel, ""))
+        block = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
+        parts = [block.get(k) for k, m in PROFILE_NAMES.get(family, []) if m > 0]
+        if not parts or any(p is None for p in parts):
+            return None
+        return int(sum(block[k] * m for k, m in PROFILE_NAMES[family] if m > 0))
     except (OSError, IndexError, ValueError):
         return None
 
 
-def cpu_baseline(A, b, iters_needed, sample_iters=8):
+def cpu_baseline(A, b, iters_needed, sweeps, sample_iters=8):
     """Oracle (C restatement, 1 core): ILU(0) set-up plus `sample_iters`
-    BiCGSTAB iterations on the same system, scaled to `iters_needed`."""
+    BiCGSTAB iterations with the same sweep apply on the same system, scaled
+    to `iters_needed`."""
     import oracle
 
     oracle.lib()
     t0 = time.perf_counter()
-    oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=0)
+    oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=0,
+                 lsweeps=sweeps[0], usweeps=sweeps[1])
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
-    _, st, it, _ = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=sample_iters)
+    _, st, it, _ = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=sample_iters,
+                                lsweeps=sweeps[0], usweeps=sweeps[1])
     t_run = time.perf_counter() - t0
     per_iter = max(t_run - t_setup, 0.0) / max(it, 1)
     est = t_setup + per_iter * iters_needed
@@ -133,8 +161,8 @@ def cpu_baseline(A, b, iters_needed, sample_iters=8):
         "cores": 1,
         "kind": "port",
         "sample": (f"oracle ILU(0) set-up ({t_setup:.2f} s) + {it} BiCGSTAB iterations "
-                   f"({per_iter:.3f} s/iter) on the same 128^3 system, scaled to "
-                   f"{iters_needed} iterations"),
+                   f"with {sweeps[0]}/{sweeps[1]} Jacobi-sweep ILU apply ({per_iter:.3f} s/iter) "
+                   f"on the same system, scaled to {iters_needed} iterations"),
     }
 
 
@@ -305,89 +333,96 @@ def main():
     A = gen.convdiff3d(N)
     xs, b = gen.manufactured(A, 1)
     n, nnz = len(A[0]) - 1, len(A[2])
-    nnz_l = (nnz - n) // 2
-    nnz_u = nnz - nnz_l
-    by = algorithmic_bytes(n, nnz, nnz_l, nnz_u, int(np.diff(A[0]).max()))
+    nnz_l = (nnz - n) // 2          # symmetric pattern: strictly lower = strictly upper
+    by = algorithmic_bytes(n, nnz, nnz_l, nnz_l, APPLY)
 
-    csr = run_gpu(A, b, "csr", args.steps, args.warmup)
-    relres = float(np.linalg.norm(b - gen.spmv(A, csr["x"])) / np.linalg.norm(b))
-    err_xs = float(np.linalg.norm(csr["x"] - xs) / np.linalg.norm(xs))
-    ell = None if args.no_ell else run_gpu(A, b, "ell", args.steps, args.warmup)
-    mixed = None if args.no_mixed else run_gpu(A, b, "csr", args.steps, args.warmup,
-                                               precision=1, factor_storage=0)
-    mixed_relres = (float(np.linalg.norm(b - gen.spmv(A, mixed["x"])) / np.linalg.norm(b))
-                    if mixed else None)
-    # the reference GPU backend's ILU apply: 20 Jacobi sweeps per factor
-    # (hip/Precond/precon_gpuilu0.cpp:17-18), FP64, and FP32 inside refinement
-    sw = None if args.no_sweeps else run_gpu(A, b, "csr", args.steps, args.warmup,
-                                             profile=False, sweeps=(20, 20))
-    swm = None if args.no_sweeps or args.no_mixed else run_gpu(
-        A, b, "csr", args.steps, args.warmup, profile=False, precision=1, sweeps=(20, 20))
     def relres_of(r):
         return float(np.linalg.norm(b - gen.spmv(A, r["x"])) / np.linalg.norm(b)) if r else None
-    mixed_err = (float(np.linalg.norm(mixed["x"] - csr["x"]) / np.linalg.norm(csr["x"]))
-                 if mixed else None)
 
-    # dominant kernel by total time in the timed CSR steps
-    k = csr["kernels"]
-    dom = max(k, key=lambda name: k[name]["total_ms"]) if k else None
+    def its(r):
+        return int(np.median(r["iterations"])) if r else None
+
+    def sec(r):
+        return round(r["sec_per_step"], 5) if r else None
+
+    # timed without per-kernel events (they add gaps between launches); the
+    # kernel figures come from separate profiled runs of the same solves
+    head = run_gpu(A, b, "csr", args.steps, args.warmup, profile=False, sweeps=APPLY)
+    relres = relres_of(head)
+    err_xs = float(np.linalg.norm(head["x"] - xs) / np.linalg.norm(xs))
+    headp = run_gpu(A, b, "csr", 1, 1, sweeps=APPLY)
+    exact = run_gpu(A, b, "csr", args.steps, args.warmup, profile=False)   # sync-free SpTRSV
+    exactp = run_gpu(A, b, "csr", 1, 1)
+    mixed = None if args.no_mixed else run_gpu(A, b, "csr", args.steps, args.warmup,
+                                               profile=False, precision=1, sweeps=APPLY)
+    mixed_exact = None if args.no_mixed else run_gpu(A, b, "csr", args.steps, args.warmup,
+                                                     profile=False, precision=1)
+    # the reference GPU backend's own apply: 20 Jacobi sweeps per factor
+    # (hip/Precond/precon_gpuilu0.cpp:17-18)
+    sw = None if args.no_sweeps else run_gpu(A, b, "csr", args.steps, args.warmup,
+                                             profile=False, sweeps=(20, 20))
+    ell = None if args.no_ell else run_gpu(A, b, "ell", args.steps, args.warmup,
+                                           profile=False, sweeps=APPLY)
+
+    # kernel families: the headline run's SpMV and ILU applies, the exact run's SpTRSV
+    fam = {"spmv": headp["kernels"].get("spmv"),
+           "ilu_lower_apply": headp["kernels"].get("trsv_lower"),
+           "ilu_upper_apply": headp["kernels"].get("trsv_upper"),
+           "trsv_lower_exact": exactp["kernels"].get("trsv_lower"),
+           "trsv_upper_exact": exactp["kernels"].get("trsv_upper")}
+    fam = {k: v for k, v in fam.items() if v}
+    kernels_gbs = {name: {"avg_us": round(rec["avg_us"], 2),
+                          "GB/s": round(by[name] / (rec["avg_us"] * 1e-6) / 1e9, 1),
+                          "launches": rec["count"]} for name, rec in fam.items()}
+    # dominant family by total time in the profiled headline solve
+    headfam = {k: v for k, v in fam.items() if not k.endswith("_exact")}
+    dom = max(headfam, key=lambda name: headfam[name]["total_ms"]) if headfam else None
     roof = None
     if dom:
-        bytes_per = by["spmv_csr"] if dom == "spmv" else by[dom]
-        achieved = bytes_per / (k[dom]["avg_us"] * 1e-6) / 1e9
-        traffic = measured_traffic(dom) if N == 128 else None
+        rec = headfam[dom]
+        achieved = by[dom] / (rec["avg_us"] * 1e-6) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": dom, "bytes_per_launch": bytes_per,
-                "avg_us": round(k[dom]["avg_us"], 2)}
-    kernels_gbs = {}
-    for name, rec in k.items():
-        bp = by["spmv_csr"] if name == "spmv" else by[name]
-        kernels_gbs[name] = {"avg_us": round(rec["avg_us"], 2),
-                             "GB/s": round(bp / (rec["avg_us"] * 1e-6) / 1e9, 1),
-                             "launches": rec["count"]}
-    if ell and "spmv" in ell["kernels"]:
-        r = ell["kernels"]["spmv"]
-        kernels_gbs["spmv_ell"] = {"avg_us": round(r["avg_us"], 2),
-                                   "GB/s": round(by["spmv_ell"] / (r["avg_us"] * 1e-6) / 1e9, 1),
-                                   "launches": r["count"]}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dom) if N == 128 else None,
+                "kernel": dom, "launches": PROFILE_NAMES[dom],
+                "bytes_per_launch": by[dom], "avg_us": round(rec["avg_us"], 2)}
 
-    iters = int(np.median(csr["iterations"]))
-    cpu = None if args.no_cpu else cpu_baseline(A, b, iters)
+    iters = its(head)
+    cpu = None if args.no_cpu else cpu_baseline(A, b, iters, APPLY)
 
     out = {
         "metric": METRIC,
-        "value": round(csr["sec_per_step"], 5),
+        "value": round(head["sec_per_step"], 5),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1000 * csr["sec_per_step"], 3),
+        "ms_per_step": round(1000 * head["sec_per_step"], 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded generator, xsolver_amd/gen.py)",
         "config": {
-            "workload": f"convdiff3d_{N} BiCGSTAB+ParILU(0) FP64 rtol 1e-10 (CSR)",
+            "workload": (f"convdiff3d_{N} BiCGSTAB+ParILU(0) FP64 rtol 1e-10, CSR input, "
+                         f"ILU apply {APPLY[0]}/{APPLY[1]} Jacobi sweeps"),
             "n": n, "nnz": nnz, "format": "csr", "parilu_sweeps": 20,
-            "iterations": iters, "pc_setup_ms": round(csr["pc_setup_ms"], 3),
-            "solve_ms": round(csr["solve_ms"], 3), "true_relres": relres,
-            "rel_err_vs_xstar": err_xs,
-            "ell_sec_per_step": round(ell["sec_per_step"], 5) if ell else None,
-            "ell_iterations": int(np.median(ell["iterations"])) if ell else None,
-            "mixed_sec_per_step": round(mixed["sec_per_step"], 5) if mixed else None,
-            "mixed_inner_iterations": int(np.median(mixed["iterations"])) if mixed else None,
-            "mixed_true_relres": mixed_relres,
-            "mixed_rel_err_vs_fp64_gpu": mixed_err,
-            "mixed_speedup_vs_fp64": (round(csr["sec_per_step"] / mixed["sec_per_step"], 3)
+            "ilu_apply_sweeps": list(APPLY), "iterations": iters,
+            "pc_setup_ms": round(head["pc_setup_ms"], 3), "solve_ms": round(head["solve_ms"], 3),
+            "true_relres": relres, "rel_err_vs_xstar": err_xs,
+            "exact_trsv_sec_per_step": sec(exact), "exact_trsv_iterations": its(exact),
+            "exact_trsv_true_relres": relres_of(exact),
+            "mixed_sec_per_step": sec(mixed), "mixed_inner_iterations": its(mixed),
+            "mixed_true_relres": relres_of(mixed),
+            "mixed_speedup_vs_fp64": (round(head["sec_per_step"] / mixed["sec_per_step"], 3)
                                       if mixed else None),
-            "sweep20_sec_per_step": round(sw["sec_per_step"], 5) if sw else None,
-            "sweep20_iterations": int(np.median(sw["iterations"])) if sw else None,
+            "mixed_exact_trsv_sec_per_step": sec(mixed_exact),
+            "mixed_exact_trsv_speedup_vs_fp64": (
+                round(exact["sec_per_step"] / mixed_exact["sec_per_step"], 3)
+                if mixed_exact else None),
+            "sweep20_sec_per_step": sec(sw), "sweep20_iterations": its(sw),
             "sweep20_true_relres": relres_of(sw),
-            "sweep20_mixed_sec_per_step": round(swm["sec_per_step"], 5) if swm else None,
-            "sweep20_mixed_inner_iterations": int(np.median(swm["iterations"])) if swm else None,
-            "sweep20_mixed_true_relres": relres_of(swm),
+            "ell_input_sec_per_step": sec(ell), "ell_input_iterations": its(ell),
         },
         "kernels": kernels_gbs,
         "roofline": roof,

@@ -129,11 +129,13 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *                    dependency-level order, 0 = row order
  *   "trsv_workgroups" persistent workgroups of a triangular solve (default 256):
  *                    bounds the number of polling waves
- *   "spmv_kernel"    CSR SpMV kernel: -1 (default) = vector CSR when the row
- *                    lengths are regular, else LDS row blocks; 0 = LDS-staged
- *                    row blocks; 1 = row-binned vector CSR (lanes per row = mean
- *                    row length rounded up to a power of two, long rows one
- *                    wavefront each)
+ *   "spmv_kernel"    SpMV kernel for CSR input: -1 (default) = automatic: an
+ *                    internal ELL copy when the padding to the longest row is
+ *                    under 20% of nnz, else vector CSR when the row lengths are
+ *                    regular, else LDS row blocks; 0 = LDS-staged row blocks;
+ *                    1 = row-binned vector CSR (lanes per row = mean row length
+ *                    rounded up to a power of two, long rows one wavefront
+ *                    each).  0 and 1 also drop the automatic ELL copy.
  *   "use_ell"        1 = ELL SpMV (same as advanced_pam[1] = 1) */
 int xs_set_option(xs_ctx *ctx, const char *name, double value);
 
@@ -143,7 +145,8 @@ int xs_set_option(xs_ctx *ctx, const char *name, double value);
  * this matrix and keeps the fastest; with the
  * environment variable XS_TUNE_CACHE naming a file, results are read from
  * and appended to it per matrix signature.  Choice and time are queried with
- * xs_get_time_ms("tune:chunk" | "tune:ell" | "tune:us"). */
+ * xs_get_time_ms("tune:chunk" | "tune:ell" | "tune:us").  The SpMV kernel in
+ * use: xs_get_time_ms("spmv:kernel") = 0 LDS row blocks, 1 vector CSR, 2 ELL. */
 int xs_autotune(xs_ctx *ctx);
 
 /* Factorise / prepare the preconditioner now (otherwise done by the first

@@ -73,6 +73,32 @@ def test_spmv_full_size_convdiff128(dev):
         S.close()
 
 
+def test_default_spmv_kernel_choice(dev):
+    """Automatic SpMV kernel for CSR input: ELL copy for near-uniform rows
+    (7-point stencil), vector CSR for regular rows with a wider spread, LDS
+    row blocks when some rows are long; every choice matches the oracle."""
+    import torch
+    cases = [("convdiff3d_24", gen.convdiff3d(24), 2),
+             ("mixedlen_3000", random_sparse(3000, 3, 15, long_rows=[(i, 12) for i in range(0, 3000, 3)]), 1),
+             ("longrows_3000", random_sparse(3000, 5, 31, long_rows=[(7, 2500)]), 0)]
+    for name, A, want in cases:
+        n = len(A[0]) - 1
+        S = XSolver()
+        S.mat_setup(*A)
+        got = S.time_ms("spmv:kernel")
+        say(f"[spmv-default] {name}: kernel {got}")
+        assert got == want
+        x = np.random.default_rng(4).uniform(-1, 1, n)
+        yd = torch.empty(n, dtype=torch.float64, device=dev)
+        S.spmv(dev_tensor(x, dev), yd, torch_stream())
+        torch.cuda.synchronize()
+        scale = np.abs(A[2]).max() * np.diff(A[0]).max()
+        assert np.abs(yd.cpu().numpy() - oracle.spmv(A, x)).max() <= 1e-14 * scale
+        S.set_option("spmv_kernel", 0)            # explicit choice drops the ELL copy
+        assert S.time_ms("spmv:kernel") == 0
+        S.close()
+
+
 TRSV_MATS = {
     "convdiff3d_20": lambda: gen.convdiff3d(20),     # in-wave chains (scan path)
     "poisson2d_16": lambda: gen.poisson2d(16),       # in-wave deps r-1, r-16 (loop path)

@@ -233,6 +233,16 @@ static bool regular_rows(const std::vector<int> &hrp, const xs_spmv_plan &P) {
   return P.nlong == 0 && mx <= 2 * P.vec_L;
 }
 
+// ELL pays off over the CSR kernels when the padding to the longest row stays
+// small: max row length * n <= 1.2 nnz.
+static bool ell_fits(const std::vector<int> &hrp) {
+  const int n = (int)hrp.size() - 1;
+  if (n <= 0) return false;
+  int mx = 0;
+  for (int i = 0; i < n; i++) mx = std::max(mx, hrp[i + 1] - hrp[i]);
+  return mx > 0 && (double)mx * n <= 1.2 * (double)hrp[n];
+}
+
 static int make_ell(xs_ctx *c) {
   int rc;
   xs_csr &A = c->A;
@@ -254,6 +264,11 @@ static int build_spmv(xs_ctx *c) {
   if ((rc = make_plan(c, A, c->plan, xs_default_chunk(A)))) return rc;
   if ((rc = make_vec(c, c->h_rp, c->plan))) return rc;
   c->plan.vec = c->spmv_vec < 0 ? (regular_rows(c->h_rp, c->plan) ? 1 : 0) : c->spmv_vec;
+  c->ell_auto = false;
+  if (!c->use_ell && c->spmv_vec < 0 && ell_fits(c->h_rp)) {
+    c->use_ell = 1;   // CSR input, ELL kernel: padding under 20% of nnz
+    c->ell_auto = true;
+  }
   if (c->use_ell && (rc = make_ell(c))) return rc;
   XS_HIP(hipStreamSynchronize(c->stream));
   if (xs_take_launch_guard()) return XS_ERR_STATE;
@@ -591,9 +606,10 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   else if (k == "parilu_async") { c->parilu_async = value != 0; c->pc_ready = false; }
   else if (k == "check_every") { if (value < 1) return XS_ERR_ARG; c->check_every = (int)value; }
   else if (k == "use_graph") { c->use_graph = value != 0; }
-  else if (k == "spmv_kernel") {   // -1 = by row regularity, 0 = LDS row blocks, 1 = vector CSR
+  else if (k == "spmv_kernel") {   // -1 = automatic, 0 = LDS row blocks, 1 = vector CSR
     if (value != 0 && value != 1 && value != -1) return XS_ERR_ARG;
     c->spmv_vec = (int)value;
+    if (c->ell_auto && value >= 0) { c->use_ell = 0; c->ell_auto = false; }
     if (c->mat_ready) {
       c->plan.vec = c->spmv_vec < 0 ? (regular_rows(c->h_rp, c->plan) ? 1 : 0) : c->spmv_vec;
       c->planT.vec = c->spmv_vec < 0 ? c->plan.vec : c->spmv_vec;
@@ -627,6 +643,7 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
       XS_HIP(hipStreamSynchronize(c->stream));
     }
     c->use_ell = want;
+    c->ell_auto = false;
   } else return XS_ERR_ARG;
   return XS_OK;
 }
@@ -1309,6 +1326,9 @@ extern "C" double xs_get_time_ms(const xs_ctx *c, const char *phase) {
   if (!strcmp(phase, "tune:chunk")) return c->tuned ? c->tune_chunk : -1.0;
   if (!strcmp(phase, "tune:ell")) return c->tuned ? c->use_ell : -1.0;
   if (!strcmp(phase, "tune:us")) return c->tuned ? c->tune_us : -1.0;
+  // SpMV kernel in use: 0 LDS row blocks, 1 vector CSR, 2 ELL
+  if (!strcmp(phase, "spmv:kernel"))
+    return !c->mat_ready ? -1.0 : (c->use_ell && c->ell.v) ? 2.0 : (double)c->plan.vec;
   static const char *kn[4] = {"", "spmv", "trsv_lower", "trsv_upper"};
   for (int k = 1; k < 4; k++) {
     std::string tot = std::string("total:") + kn[k], cnt = std::string("count:") + kn[k];

@@ -42,6 +42,7 @@ struct xs_ctx {
   int spmv_vec = -1;            // CSR kernel: -1 by row regularity, 0 LDS row blocks, 1 vector CSR
   int precision = 0;
   int use_ell = 0;
+  bool ell_auto = false;        // use_ell chosen by the library (regular rows)
 
   // matrix
   bool mat_ready = false, own_matrix = false, sorted = true, has_diag = true;
