@@ -163,9 +163,12 @@ def test_parilu_converges_to_ilu0(dev, name):
         S.close()
 
 
-def test_parilu_sync_sweep_matches_oracle_sweep(dev):
-    # one and three deterministic sweeps from the reference's initial guess
-    A = gen.convdiff3d(9)
+@pytest.mark.parametrize("name", ["convdiff3d_9", "random_600"])
+def test_parilu_sync_sweep_matches_oracle_sweep(dev, name):
+    # one and three deterministic sweeps from the reference's initial guess;
+    # the GPU sweeps run over precomputed update lists (random_600: a
+    # nonsymmetric pattern, so entries off the diagonal also get products)
+    A = gen.convdiff3d(9) if name == "convdiff3d_9" else random_sparse(600, 5, 9)
     n = len(A[0]) - 1
     from tests.test_oracle import _split_for_sweep
     lu0, uptr = oracle.ilu0(A)

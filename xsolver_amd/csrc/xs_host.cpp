@@ -68,6 +68,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own); dfree(F.trp); dfree(F.tci);
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
   dfree(F.lsched); dfree(F.usched);
+  dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as);
   dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.v32);
   dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.v32);
   c->tell_ready = c->tell32_ready = false;
@@ -715,6 +716,29 @@ static int build_trsv_schedules(xs_ctx *c, const int *rp, const int *ci, const i
   return XS_OK;
 }
 
+// Update lists of the ParILU(0) sweeps (once per pattern, on the device):
+// count the products of every entry, scan, fill.
+static int build_parilu_lists(xs_ctx *c) {
+  xs_ilu &F = c->ilu;
+  const int nnz = c->A.nnz;
+  int rc, *cnt = nullptr, *bsum = nullptr, np = 0;
+  auto done = [&](int r) { dfree(cnt); dfree(bsum); return r; };
+  if ((rc = dalloc(&cnt, nnz)) || (rc = dalloc(&bsum, xs_scan_bsum_len(nnz))) ||
+      (rc = dalloc(&F.pd, nnz)) || (rc = dalloc(&F.pp, (size_t)nnz + 1)))
+    return done(rc);
+  xs_parilu_pairs(c->stream, c->A, F, cnt, nullptr, nullptr, nullptr, F.pd);
+  xs_scan_excl(c->stream, nnz, cnt, F.pp, bsum);
+  if (hipMemcpyAsync(&np, F.pp + nnz, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return done(XS_ERR_DEVICE);
+  if ((rc = dalloc(&F.pl, (size_t)std::max(np, 1))) || (rc = dalloc(&F.pu, (size_t)std::max(np, 1))))
+    return done(rc);
+  xs_parilu_pairs(c->stream, c->A, F, nullptr, F.pp, F.pl, F.pu, nullptr);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return done(XS_ERR_DEVICE);
+  if (xs_take_launch_guard()) return done(XS_ERR_STATE);
+  return done(XS_OK);
+}
+
 static int setup_ilu_pattern(xs_ctx *c) {
   // Combined-factor layout on A's pattern plus the CSC of the upper triangle
   // used by the ParILU sweep (the reference builds L CSR / U CSC with
@@ -755,6 +779,7 @@ static int setup_ilu_pattern(xs_ctx *c) {
   XS_HIP(hipMemcpy(F.ucp, ucnt.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
   XS_HIP(hipMemcpy(F.uri, uri.data(), sizeof(int) * nu, hipMemcpyHostToDevice));
   XS_HIP(hipMemcpy(F.upos, upos.data(), sizeof(int) * nu, hipMemcpyHostToDevice));
+  if ((rc = build_parilu_lists(c))) return rc;
   return build_trsv_schedules(c, rp.data(), ci.data(), dpos.data());
 }
 
@@ -784,11 +809,22 @@ static int factor_parilu(xs_ctx *c) {
     xs_pt_rescale(c->stream, c->A.nnz, F.rowidx, c->A.ci, F.lu, F.scale, 0);
     sc = F.scale;
   }
+  const double *a0 = c->A.v;   // the values the sweeps start from
+  if (sc && F.pp) {
+    if (!F.as) {
+      int rc = dalloc(&F.as, c->A.nnz);
+      if (rc) return rc;
+    }
+    xs_parilu_scaled(c->stream, c->A, F, sc, F.as);
+    a0 = F.as;
+  }
   for (int s = 0; s < c->parilu_sweeps; s++) {
     if (c->parilu_async) {
-      xs_parilu_sweep(c->stream, c->A, F, nullptr, true, sc);
+      if (F.pp) xs_parilu_sweep_pairs(c->stream, F, a0, F.lu, F.lu);
+      else xs_parilu_sweep(c->stream, c->A, F, nullptr, true, sc);
     } else {
-      xs_parilu_sweep(c->stream, c->A, F, c->lu_tmp, false, sc);
+      if (F.pp) xs_parilu_sweep_pairs(c->stream, F, a0, F.lu, c->lu_tmp);
+      else xs_parilu_sweep(c->stream, c->A, F, c->lu_tmp, false, sc);
       std::swap(F.lu, c->lu_tmp);
     }
   }
@@ -813,6 +849,7 @@ int xs_parilut_factor(xs_ctx *c) {
   // free a previous factor
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own);
   dfree(F.ucp); dfree(F.uri); dfree(F.upos);
+  dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as);   // ILU(0) lists only
   int rc;
   double *dtol = nullptr, *thr = nullptr;
   int *cnt = nullptr, *bsum = nullptr, *ovf = nullptr;

@@ -110,6 +110,12 @@ struct xs_ilu {
   // CSC of the upper triangle (incl. diagonal) for the ParILU sweep:
   int *ucp = nullptr, *uri = nullptr, *upos = nullptr;
   int *rowidx = nullptr;    // row of each nnz (COO)
+  // ParILU(0) update lists (no reference counterpart): entry k's products
+  // l_ik' u_k'j are lu[pl[e]] * lu[pu[e]] for e in [pp[k], pp[k+1]) in
+  // ascending k'; pd[k] = position of u_jj below the diagonal, -2 on it, -1
+  // above.  as = the (scaled) matrix values the sweeps start from.
+  int *pp = nullptr, *pl = nullptr, *pu = nullptr, *pd = nullptr;
+  double *as = nullptr;
   // SpTRSV work order: the 64-row groups of L (resp. U, counted from the
   // bottom) sorted by dependency level; ticket t solves group sched[t]
   int *lsched = nullptr, *usched = nullptr;
@@ -161,6 +167,15 @@ void xs_init_state(hipStream_t s, xs_state *st, double tol, double abstol, int m
 
 // ILU
 void xs_ilu_copy_values(hipStream_t s, const xs_csr &A, xs_ilu &F);
+// ParILU(0) update lists: count (pl == null: matches into cnt, pd filled) or
+// fill (pl, pu at offsets pp)
+void xs_parilu_pairs(hipStream_t s, const xs_csr &A, const xs_ilu &F, int *cnt, const int *pp,
+                     int *pl, int *pu, int *pd);
+void xs_parilu_scaled(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
+                      double *as);
+// one ParILU sweep over the update lists (same arithmetic as xs_parilu_sweep)
+void xs_parilu_sweep_pairs(hipStream_t s, const xs_ilu &F, const double *as, const double *lu_in,
+                           double *lu_out);
 void xs_parilu_sweep(hipStream_t s, const xs_csr &A, xs_ilu &F, double *lu_out, bool async,
                      const double *sc);
 void xs_ilu_dinv(hipStream_t s, xs_ilu &F, int *d_bad);

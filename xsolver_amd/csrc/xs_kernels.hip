@@ -677,6 +677,104 @@ void xs_parilu_sweep(hipStream_t s, const xs_csr &A, xs_ilu &F, double *lu_out, 
                      A.nnz, F.rowidx, A.rp, A.ci, A.v, F.dpos, F.ucp, F.uri, F.upos, F.lu, out, sc);
 }
 
+// Update lists of the ParILU(0) sweep.  The products entry (i, j) subtracts
+// depend only on the pattern, so the merge of L's row i with U's column j is
+// done once per pattern: one pass counts the matches (and records where u_jj
+// sits), a scan turns the counts into offsets, a second pass writes the
+// matched factor positions in the merge order.
+template <bool FILL>
+__global__ __launch_bounds__(XS_BLOCK) void k_parilu_pairs(
+    int nnz, const int *__restrict__ rowidx, const int *__restrict__ rp,
+    const int *__restrict__ ci, const int *__restrict__ dpos, const int *__restrict__ ucp,
+    const int *__restrict__ uri, const int *__restrict__ upos, int *__restrict__ cnt,
+    const int *__restrict__ pp, int *__restrict__ pl, int *__restrict__ pu, int *__restrict__ pd) {
+  const int k = blockIdx.x * XS_BLOCK + threadIdx.x;
+  if (k >= nnz) return;
+  const int i = rowidx[k], j = ci[k];
+  const int m = i < j ? i : j;
+  int il = rp[i], iu = ucp[j], c = 0;
+  const int le = dpos[i], ue = ucp[j + 1];
+  const int base = FILL ? pp[k] : 0;
+  while (il < le && iu < ue) {
+    const int kl = ci[il], ku = uri[iu];
+    if (kl >= m || ku >= m) break;
+    if (kl == ku) {
+      if (FILL) { pl[base + c] = il; pu[base + c] = upos[iu]; }
+      ++c; ++il; ++iu;
+    } else if (kl < ku) {
+      ++il;
+    } else {
+      ++iu;
+    }
+  }
+  if (!FILL) {
+    cnt[k] = c;
+    pd[k] = i > j ? dpos[j] : (i == j ? -2 : -1);
+  }
+}
+
+void xs_parilu_pairs(hipStream_t s, const xs_csr &A, const xs_ilu &F, int *cnt, const int *pp,
+                     int *pl, int *pu, int *pd) {
+  XS_GUARD(A.rp && A.ci && F.rowidx && F.dpos && F.ucp && F.uri && F.upos);
+  const dim3 grid((A.nnz + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+  if (!pl) {
+    XS_GUARD(cnt && pd);
+    hipLaunchKernelGGL(k_parilu_pairs<false>, grid, blk, 0, s, A.nnz, F.rowidx, A.rp, A.ci,
+                       F.dpos, F.ucp, F.uri, F.upos, cnt, nullptr, nullptr, nullptr, pd);
+  } else {
+    XS_GUARD(pp && pu);
+    hipLaunchKernelGGL(k_parilu_pairs<true>, grid, blk, 0, s, A.nnz, F.rowidx, A.rp, A.ci,
+                       F.dpos, F.ucp, F.uri, F.upos, nullptr, pp, pl, pu, nullptr);
+  }
+}
+
+// as[k] = a_k s_i s_j, the starting values of the scaled sweeps
+__global__ void k_parilu_scaled(int nnz, const int *__restrict__ rowidx,
+                                const int *__restrict__ ci, const double *__restrict__ a,
+                                const double *__restrict__ sc, double *__restrict__ as) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  as[k] = a[k] * sc[rowidx[k]] * sc[ci[k]];
+}
+
+void xs_parilu_scaled(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
+                      double *as) {
+  XS_GUARD(A.ci && A.v && F.rowidx && sc && as);
+  hipLaunchKernelGGL(k_parilu_scaled, dim3((A.nnz + 255) / 256), dim3(256), 0, s, A.nnz,
+                     F.rowidx, A.ci, A.v, sc, as);
+}
+
+// One sweep over the update lists: the arithmetic of k_parilu_sweep (same
+// products in the same order, same pivot guard), without the merge.  The
+// previous value is read only when the new one is not finite.
+__global__ __launch_bounds__(XS_BLOCK) void k_parilu_sweep_pairs(
+    int nnz, const double *__restrict__ as, const int *__restrict__ pp,
+    const int *__restrict__ pl, const int *__restrict__ pu, const int *__restrict__ pd,
+    const double *lu_in, double *lu_out) {
+  const int k = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
+  if (k >= nnz) return;
+  double s = as[k];
+  const int e0 = pp[k], e1 = pp[k + 1], d = pd[k];
+  for (int e = e0; e < e1; ++e) s -= lu_in[pl[e]] * lu_in[pu[e]];
+  double nv;
+  if (d >= 0) {
+    double dd = lu_in[d];
+    if (fabs(dd) < 1e-10) dd = dd >= 0.0 ? 1e-10 : -1e-10;
+    nv = s / dd;
+  } else {
+    if (d == -2 && fabs(s) < 1e-10) s = s >= 0.0 ? 1e-10 : -1e-10;
+    nv = s;
+  }
+  lu_out[k] = isfinite(nv) ? nv : lu_in[k];
+}
+
+void xs_parilu_sweep_pairs(hipStream_t s, const xs_ilu &F, const double *as, const double *lu_in,
+                           double *lu_out) {
+  XS_GUARD(F.pp && F.pd && F.pl && F.pu && as && lu_in && lu_out);
+  hipLaunchKernelGGL(k_parilu_sweep_pairs, dim3((F.nnz + XS_BLOCK - 1) / XS_BLOCK), dim3(XS_BLOCK),
+                     0, s, F.nnz, as, F.pp, F.pl, F.pu, F.pd, lu_in, lu_out);
+}
+
 void xs_ilu_copy_values(hipStream_t s, const xs_csr &A, xs_ilu &F) {
   XS_GUARD(A.v && F.lu);
   (void)hipMemcpyAsync(F.lu, A.v, sizeof(double) * A.nnz, hipMemcpyDeviceToDevice, s);
