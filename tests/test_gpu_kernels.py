@@ -141,9 +141,12 @@ def test_trsv_parity(dev, name):
     S.close()
 
 
-@pytest.mark.parametrize("name", ["convdiff3d_20", "lognormal27_12", "grf2d_40", "random_4000"])
+@pytest.mark.parametrize("name", ["convdiff3d_20", "lognormal27_12", "grf2d_40", "random_4000",
+                                  "convdiff3d_64"])
 def test_parilu_converges_to_ilu0(dev, name):
-    A = TRSV_MATS[name]()
+    # convdiff3d_64: 1.8 M entries, so the update lists' scan runs over
+    # 1,770 block sums (more than one per scan thread, not a multiple of 1024)
+    A = gen.convdiff3d(64) if name == "convdiff3d_64" else TRSV_MATS[name]()
     n = len(A[0]) - 1
     say(f"[parilu] {name} n={n}")
     ref_lu, uptr = oracle.ilu0(A)
@@ -262,8 +265,8 @@ def test_gmres_parilut_solve_parity(dev):
 
 
 @pytest.mark.parametrize("pc,kl,ku", [("ilu0", 0, 0), ("ilu0", 1, 3), ("ilu0", 2, 2),
-                                       ("ilu0", 20, 20), ("ilu0", 10, 15), ("ilut", 2, 3),
-                                       ("ilut", 10, 15)])
+                                       ("ilu0", 4, 4), ("ilu0", 5, 6), ("ilu0", 20, 20),
+                                       ("ilu0", 10, 15), ("ilut", 2, 3), ("ilut", 10, 15)])
 def test_ilu_sweep_apply_matches_oracle(dev, pc, kl, ku):
     """The reference's iterative GPU ILU apply (kl / ku Jacobi sweeps per
     factor) against the oracle's restatement on the GPU's own factors; two or

@@ -850,6 +850,7 @@ int xs_parilut_factor(xs_ctx *c) {
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own);
   dfree(F.ucp); dfree(F.uri); dfree(F.upos);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as);   // ILU(0) lists only
+  F.tell_shape = false;   // a new pattern: the sweep copies are re-shaped
   int rc;
   double *dtol = nullptr, *thr = nullptr;
   int *cnt = nullptr, *bsum = nullptr, *ovf = nullptr;
@@ -1087,20 +1088,23 @@ static int ensure_tri_ell(xs_ctx *c) {
   if (c->tell_ready) return XS_OK;
   xs_ilu &F = c->ilu;
   int rc, w[2];
-  if (xs_tri_widths(c->stream, F, c->d_int, &w[0], &w[1])) return XS_ERR_DEVICE;
-  for (int part = 0; part < 2; ++part) {
-    xs_tell &E = part ? F.tu : F.tl;
-    const size_t need = (size_t)w[part] * F.n;
-    if (need > E.cap) {
-      dfree(E.ci); dfree(E.v); dfree(E.v32);
-      E.cap = 0;
-      if ((rc = dalloc(&E.ci, need)) || (rc = dalloc(&E.v, need))) return rc;
-      E.cap = need;
+  if (!F.tell_shape) {   // widths once per factor pattern
+    if (xs_tri_widths(c->stream, F, c->d_int, &w[0], &w[1])) return XS_ERR_DEVICE;
+    for (int part = 0; part < 2; ++part) {
+      xs_tell &E = part ? F.tu : F.tl;
+      const size_t need = (size_t)w[part] * F.n;
+      if (need > E.cap) {
+        dfree(E.ci); dfree(E.v); dfree(E.v32);
+        E.cap = 0;
+        if ((rc = dalloc(&E.ci, need)) || (rc = dalloc(&E.v, need))) return rc;
+        E.cap = need;
+      }
+      E.W = w[part];
+      E.ld = F.n;
     }
-    E.W = w[part];
-    E.ld = F.n;
-    xs_build_tell(c->stream, F, part == 1, E);
+    F.tell_shape = true;
   }
+  for (int part = 0; part < 2; ++part) xs_build_tell(c->stream, F, part == 1, part ? F.tu : F.tl);
   XS_HIP(hipStreamSynchronize(c->stream));
   if (xs_take_launch_guard()) return XS_ERR_STATE;
   c->tell_ready = true;

@@ -121,6 +121,7 @@ struct xs_ilu {
   int *lsched = nullptr, *usched = nullptr;
   int trsv_wg = 256;        // persistent workgroups of the triangular solves
   xs_tell tl, tu;           // sweep copies of L and U (built on demand)
+  bool tell_shape = false;  // tl/tu widths and buffers match the pattern
   // transposed factors for the adjoint (built on demand)
   bool have_t = false;
   int *trp = nullptr, *tci = nullptr, *tdpos = nullptr, *tperm = nullptr;

@@ -330,19 +330,34 @@ void xs_build_plan(hipStream_t s, const xs_csr &A, xs_spmv_plan &P, int *blk, in
                      chunk, P.nblk, P.blk_row);
 }
 
-__global__ void k_max_row_len(int n, const int *__restrict__ rp, int *out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int m = 0;
-  if (i < n) m = rp[i + 1] - rp[i];
+// Grid-stride maxima with one atomic per workgroup (same-address atomics
+// serialise, so a per-wave atomic over millions of rows costs ~0.5 ms).
+__device__ __forceinline__ int block_max(int m) {
+  __shared__ int wm[XS_BLOCK / XS_WAVE];
   for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, XS_WAVE));
-  if ((threadIdx.x & (XS_WAVE - 1)) == 0) atomicMax(out, m);
+  __syncthreads();
+  if ((threadIdx.x & (XS_WAVE - 1)) == 0) wm[threadIdx.x / XS_WAVE] = m;
+  __syncthreads();
+  m = wm[0];
+  for (int k = 1; k < XS_BLOCK / XS_WAVE; ++k) m = max(m, wm[k]);
+  return m;
+}
+#define XS_MAX_GRID 512
+
+__global__ __launch_bounds__(XS_BLOCK) void k_max_row_len(int n, const int *__restrict__ rp, int *out) {
+  int m = 0;
+  for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK)
+    m = max(m, rp[i + 1] - rp[i]);
+  m = block_max(m);
+  if (threadIdx.x == 0) atomicMax(out, m);
 }
 
 int xs_max_row_len(hipStream_t s, const xs_csr &A, int *d_tmp) {
   if (!A.rp || !d_tmp) { xs_raise_launch_guard(); return 0; }
   int h = 0;
   (void)hipMemsetAsync(d_tmp, 0, sizeof(int), s);
-  hipLaunchKernelGGL(k_max_row_len, dim3((A.n + 255) / 256), dim3(256), 0, s, A.n, A.rp, d_tmp);
+  hipLaunchKernelGGL(k_max_row_len, dim3(std::max(1, std::min((A.n + XS_BLOCK - 1) / XS_BLOCK, XS_MAX_GRID))),
+                     dim3(XS_BLOCK), 0, s, A.n, A.rp, d_tmp);
   (void)hipMemcpyAsync(&h, d_tmp, sizeof(int), hipMemcpyDeviceToHost, s);
   (void)hipStreamSynchronize(s);
   return h;
@@ -990,23 +1005,24 @@ void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
 // coalesced slab loads instead of a thread walking its CSR row).
 // ---------------------------------------------------------------------------
 // strictly-lower / strictly-upper entry counts: maxima into w[0], w[1]
-__global__ void k_tri_widths(int n, const int *__restrict__ rp, const int *__restrict__ dpos,
-                             int *w) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(XS_BLOCK) void k_tri_widths(int n, const int *__restrict__ rp,
+                                                         const int *__restrict__ dpos, int *w) {
   int lo = 0, up = 0;
-  if (i < n) { lo = dpos[i] - rp[i]; up = rp[i + 1] - dpos[i] - 1; }
-  for (int off = 32; off > 0; off >>= 1) {
-    lo = max(lo, __shfl_xor(lo, off, XS_WAVE));
-    up = max(up, __shfl_xor(up, off, XS_WAVE));
+  for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) {
+    lo = max(lo, dpos[i] - rp[i]);
+    up = max(up, rp[i + 1] - dpos[i] - 1);
   }
-  if ((threadIdx.x & (XS_WAVE - 1)) == 0) { atomicMax(&w[0], lo); atomicMax(&w[1], up); }
+  lo = block_max(lo);
+  up = block_max(up);
+  if (threadIdx.x == 0) { atomicMax(&w[0], lo); atomicMax(&w[1], up); }
 }
 
 int xs_tri_widths(hipStream_t s, const xs_ilu &F, int *d_tmp, int *wl, int *wu) {
   if (!F.rp || !F.dpos || !d_tmp) { xs_raise_launch_guard(); return 1; }
   int h[2] = {0, 0};
   if (hipMemsetAsync(d_tmp, 0, 2 * sizeof(int), s) != hipSuccess) return 1;
-  hipLaunchKernelGGL(k_tri_widths, dim3((F.n + 255) / 256), dim3(256), 0, s, F.n, F.rp, F.dpos, d_tmp);
+  hipLaunchKernelGGL(k_tri_widths, dim3(std::max(1, std::min((F.n + XS_BLOCK - 1) / XS_BLOCK, XS_MAX_GRID))),
+                     dim3(XS_BLOCK), 0, s, F.n, F.rp, F.dpos, d_tmp);
   if (hipMemcpyAsync(h, d_tmp, 2 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return 1;

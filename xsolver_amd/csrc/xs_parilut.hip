@@ -337,10 +337,25 @@ __global__ void k_scan_blocks(int n, const int *__restrict__ in, int *__restrict
   if (g < n) out[g + 1] = s[threadIdx.x];
   if (threadIdx.x == SCAN_B - 1) bsum[blockIdx.x] = s[threadIdx.x];
 }
-__global__ void k_scan_sums(int nb, int *__restrict__ bsum) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int acc = 0;
-  for (int b = 0; b < nb; ++b) {
+// exclusive scan of the nb block sums in place, one workgroup of SCAN_B
+// threads: each thread owns a contiguous chunk, the chunk totals are scanned
+// in LDS, then every chunk is rewritten with its offset
+__global__ __launch_bounds__(SCAN_B) void k_scan_sums(int nb, int *__restrict__ bsum) {
+  __shared__ int s[SCAN_B];
+  const int per = (nb + SCAN_B - 1) / SCAN_B;
+  const int b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+  int tot = 0;
+  for (int b = b0; b < b1; ++b) tot += bsum[b];
+  s[threadIdx.x] = tot;
+  __syncthreads();
+  for (int off = 1; off < SCAN_B; off <<= 1) {
+    const int t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int acc = s[threadIdx.x] - tot;   // exclusive offset of this chunk
+  for (int b = b0; b < b1; ++b) {
     const int t = bsum[b];
     bsum[b] = acc;
     acc += t;
@@ -384,7 +399,7 @@ __global__ void k_pt_rescale(int nnz, const int *__restrict__ row, const int *__
 void xs_scan_excl(hipStream_t s, int n, const int *in, int *out, int *bsum) {
   const int nb = (n + SCAN_B - 1) / SCAN_B;
   hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_B), 0, s, n, in, out, bsum);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, s, nb, bsum);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_B), 0, s, nb, bsum);
   hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(SCAN_B), 0, s, n, out, bsum);
 }
 int xs_scan_bsum_len(int n) { return (n + SCAN_B - 1) / SCAN_B + 1; }
