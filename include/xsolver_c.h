@@ -146,7 +146,10 @@ int xs_set_option(xs_ctx *ctx, const char *name, double value);
  * environment variable XS_TUNE_CACHE naming a file, results are read from
  * and appended to it per matrix signature.  Choice and time are queried with
  * xs_get_time_ms("tune:chunk" | "tune:ell" | "tune:us").  The SpMV kernel in
- * use: xs_get_time_ms("spmv:kernel") = 0 LDS row blocks, 1 vector CSR, 2 ELL. */
+ * use: xs_get_time_ms("spmv:kernel") = 0 LDS row blocks, 1 vector CSR, 2 ELL;
+ * xs_get_time_ms("spmv:idx16") = 1 when the ELL copy stores 16-bit column
+ * offsets (every entry within 32767 of its row), 0 when it reads int32
+ * columns. */
 int xs_autotune(xs_ctx *ctx);
 
 /* Factorise / prepare the preconditioner now (otherwise done by the first

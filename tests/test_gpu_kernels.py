@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests.helpers import random_sparse, rel_err, say, split_factors
+from tests.helpers import random_sparse, rel_err, say, split_factors, true_relres
 from xsolver_amd import XSolver, gen
 
 pytestmark = pytest.mark.gpu
@@ -96,6 +96,43 @@ def test_default_spmv_kernel_choice(dev):
         assert np.abs(yd.cpu().numpy() - oracle.spmv(A, x)).max() <= 1e-14 * scale
         S.set_option("spmv_kernel", 0)            # explicit choice drops the ELL copy
         assert S.time_ms("spmv:kernel") == 0
+        S.close()
+
+
+def test_ell_column_offsets_and_wide_band(dev):
+    """The ELL copy stores 16-bit column offsets when every entry lies within
+    32767 of its row, and falls back to int32 columns otherwise; both give
+    the oracle's SpMV, and a solve over the sweep copies works either way."""
+    import torch
+    n = 40000
+    rows = np.repeat(np.arange(n), 3)
+    cols = np.stack([np.arange(n), (np.arange(n) + 1) % n, (np.arange(n) + 35000) % n], 1).ravel()
+    order = np.lexsort((cols, rows))
+    rows, cols = rows[order], cols[order]
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, rows + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    v = np.where(rows == cols, 4.0, -1.0)
+    wide = (rp, cols.astype(np.int32), v)
+    for name, A, want in (("convdiff3d_20", gen.convdiff3d(20), 1), ("wide_40000", wide, 0)):
+        nn = len(A[0]) - 1
+        S = XSolver()
+        S.mat_setup(*A)
+        assert S.time_ms("spmv:kernel") == 2 and S.time_ms("spmv:idx16") == want
+        x = np.random.default_rng(5).uniform(-1, 1, nn)
+        yd = torch.empty(nn, dtype=torch.float64, device=dev)
+        S.spmv(dev_tensor(x, dev), yd, torch_stream())
+        torch.cuda.synchronize()
+        assert rel_err(yd.cpu().numpy(), oracle.spmv(A, x)) < 1e-14
+        xs_, b = gen.manufactured(A, 2)
+        S.ksp_settype("bicgstab")
+        S.ksp_setoption(1e-10, 500)
+        S.pc_settype("ilu0")
+        S.set_option("lsolve_sweeps", 3)
+        S.set_option("usolve_sweeps", 3)
+        xg, st = S.solve(b)
+        say(f"[ell-offsets] {name}: idx16={want} it={S.iterations}")
+        assert st == 0 and true_relres(A, b, xg) < 1.05e-10
         S.close()
 
 

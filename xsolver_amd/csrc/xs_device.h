@@ -242,54 +242,69 @@ __device__ inline void do_finish(const xs_reduce &R, const double *tot, int NV) 
   }
 }
 
-// Block-reduce v and publish the block's partials; the workgroup with the
-// highest index combines all partials in block order and runs the finishing
-// op.  The partial slots hold XS_SENTINEL between reductions: a block stores
-// its value with an agent-scope store (the value is its own ready flag), the
-// combining block waits for each slot and writes the sentinel back.  No
-// fence and no same-address atomic per workgroup: on gfx950 a device-scope
-// fence writes back the XCD's L2 and same-address atomics serialise
-// (~10 ns each), which made a counter-based last-block reduction over 8k
-// workgroups cost ~150 us.  The combining block may start before others (the
-// 8 XCDs dispatch independently); it only waits, holding one slot, while
-// every other block runs to its store.
+// Block-reduce v and combine the blocks' partials in two levels: blocks are
+// grouped by XS_RED_GROUP consecutive indices; the group's highest-index block
+// (dispatched last in its group) waits for the group's other partials, adds
+// its own and publishes the group partial; the grid's highest-index block,
+// the leader of the last group, then combines the group partials and runs the
+// finishing op.  Fixed combination order (deterministic).  Partial slots hold
+// XS_SENTINEL between reductions: a value is published with an agent-scope
+// store (it is its own ready flag) and the reader writes the sentinel back.
+// No fence and no same-address atomic per workgroup: on gfx950 a
+// device-scope fence writes back the XCD's L2 and same-address atomics
+// serialise (~10 ns each), which made a counter-based last-block reduction
+// over 8k workgroups cost ~150 us.  Each combining step is one round trip of
+// at most one slot per thread for up to 64 x 64 blocks.  Layout: block
+// partials [gridDim.x][NV], then group partials [ngroups][NV].
+#define XS_RED_GROUP 64
+template <int NV>
+__device__ __forceinline__ void group_gather(const double *slots, unsigned m, const xs_reduce &R,
+                                             double (&tot)[NV]) {
+  // slots[0..m) in NV-interleaved order; thread t sums slots t, t + XS_BLOCK, ...
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = 0.0;
+  for (unsigned s = threadIdx.x; s < m; s += XS_BLOCK) {
+    double *p = const_cast<double *>(slots) + s;
+    const unsigned long long u = ld_relaxed(p);
+    const double val = u == XS_SENTINEL ? ld_ready(p, R.counter) : __longlong_as_double((long long)u);
+    tot[s % NV] += val;
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), XS_SENTINEL, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  block_sum<NV>(tot);
+}
+
 template <int NV>
 __device__ __forceinline__ void reduce_finish(double (&v)[NV], const xs_reduce &R) {
   block_sum<NV>(v);
-  if (threadIdx.x == 0) {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned g = b / XS_RED_GROUP, ng = (nb + XS_RED_GROUP - 1) / XS_RED_GROUP;
+  const unsigned g0 = g * XS_RED_GROUP, g1 = min(nb, g0 + XS_RED_GROUP);
+  double *gpart = R.partial + (size_t)nb * NV;
+  if (b != g1 - 1) {   // not the group's leader: publish and leave
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) st_ready(&R.partial[(size_t)blockIdx.x * NV + k], v[k]);
+      for (int k = 0; k < NV; ++k) st_ready(&R.partial[(size_t)b * NV + k], v[k]);
+    }
+    return;
   }
-  if (blockIdx.x != gridDim.x - 1) return;
   double tot[NV];
+  group_gather<NV>(R.partial + (size_t)g0 * NV, (g1 - 1 - g0) * NV, R, tot);
 #pragma unroll
-  for (int k = 0; k < NV; ++k) tot[k] = 0.0;
-  const unsigned m = gridDim.x * NV;
-  // 8 slots in flight per thread, then wait on the ones not yet published
-  for (unsigned b0 = threadIdx.x; b0 < m; b0 += 8 * XS_BLOCK) {
-    unsigned long long u[8];
+  for (int k = 0; k < NV; ++k) tot[k] += v[k];   // valid in thread 0
+  if (g != ng - 1) {
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const unsigned s = b0 + q * XS_BLOCK;
-      u[q] = s < m ? ld_relaxed(&R.partial[s]) : 0ull;
+      for (int k = 0; k < NV; ++k) st_ready(&gpart[(size_t)g * NV + k], tot[k]);
     }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const unsigned s = b0 + q * XS_BLOCK;
-      if (s < m) {
-        const double val = u[q] == XS_SENTINEL ? ld_ready(&R.partial[s], R.counter)
-                                               : __longlong_as_double((long long)u[q]);
-        tot[s % NV] += val;
-        __hip_atomic_store(reinterpret_cast<unsigned long long *>(&R.partial[s]), XS_SENTINEL,
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
+    return;
   }
-  block_sum<NV>(tot);
+  double all[NV];
+  group_gather<NV>(gpart, (ng - 1) * NV, R, all);
   if (threadIdx.x == 0) {
     double t4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < NV; ++k) t4[k] = tot[k];
+    for (int k = 0; k < NV; ++k) t4[k] = all[k] + tot[k];
     do_finish(R, t4, NV);
   }
 }

@@ -69,8 +69,8 @@ static void free_precond(xs_ctx *c) {
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
   dfree(F.lsched); dfree(F.usched);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as);
-  dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.v32);
-  dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.v32);
+  dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.v32); dfree(F.tl.co);
+  dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.v32); dfree(F.tu.co);
   c->tell_ready = c->tell32_ready = false;
   F = xs_ilu();
   F.trsv_wg = c->trsv_wg;
@@ -84,7 +84,7 @@ static void free_matrix(xs_ctx *c) {
   dfree(c->plan.blk_row);
   dfree(c->plan.long_rows);
   c->plan = xs_spmv_plan();
-  dfree(c->ell.ci); dfree(c->ell.v);
+  dfree(c->ell.ci); dfree(c->ell.v); dfree(c->ell.co);
   c->ell = xs_ell();
   dfree(c->At.rp); dfree(c->At.ci); dfree(c->At.v);
   c->At = xs_csr();
@@ -143,11 +143,13 @@ static int ensure_common(xs_ctx *c) {
     pblocks = std::max<size_t>(pblocks, (size_t)(P->nlong + 3) / 4 + XS_MAX_REDUCE_BLOCKS);
   if (pblocks > c->partial_blocks) {
     dfree(c->partial);
-    int rc = dalloc(&c->partial, pblocks * 4);
+    // up to 4 values per block, then per group of 64 blocks (two-level combine)
+    const size_t slots = (pblocks + pblocks / 64 + 1) * 4;
+    int rc = dalloc(&c->partial, slots);
     if (rc) return rc;
     c->partial_blocks = pblocks;
     // reduction slots hold the sentinel between reductions
-    xs_fill_sentinel(c->stream, c->partial, (int)(pblocks * 4));
+    xs_fill_sentinel(c->stream, c->partial, (int)slots);
     XS_HIP(hipStreamSynchronize(c->stream));
   }
   return XS_OK;
@@ -244,6 +246,21 @@ static bool ell_fits(const std::vector<int> &hrp) {
   return mx > 0 && (double)mx * n <= 1.2 * (double)hrp[n];
 }
 
+// 16-bit column offsets beside the int32 columns of an ELL layout when every
+// entry is within XS_OFF16_MAX of its row (the kernels then read 2 instead of
+// 4 index bytes per entry); *co stays null otherwise.
+static int make_offsets(xs_ctx *c, int n, int ld, int W, const int *eci, short **co) {
+  dfree(*co);
+  if (W == 0 || n == 0) return XS_OK;
+  const int m = xs_ell_max_offset(c->stream, n, ld, W, eci, c->d_int);
+  if (m < 0) return XS_ERR_DEVICE;
+  if (m > XS_OFF16_MAX) return XS_OK;
+  int rc;
+  if ((rc = dalloc(co, (size_t)W * ld))) return rc;
+  xs_ell_build_offsets(c->stream, n, ld, W, eci, *co);
+  return XS_OK;
+}
+
 static int make_ell(xs_ctx *c) {
   int rc;
   xs_csr &A = c->A;
@@ -255,7 +272,7 @@ static int make_ell(xs_ctx *c) {
   if ((rc = dalloc(&c->ell.ci, (size_t)W * c->ell.ld))) return rc;
   if ((rc = dalloc(&c->ell.v, (size_t)W * c->ell.ld))) return rc;
   xs_build_ell(c->stream, A, W, c->ell);
-  return XS_OK;
+  return make_offsets(c, A.n, c->ell.ld, W, c->ell.ci, &c->ell.co);
 }
 
 static int build_spmv(xs_ctx *c) {
@@ -635,12 +652,8 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     if (!c->mat_ready) { c->use_ell = value != 0; return XS_OK; }
     int want = value != 0;
     if (want && !c->ell.v) {
-      int W = xs_max_row_len(c->stream, c->A, c->d_int);
-      c->ell.n = c->A.n; c->ell.W = W; c->ell.ld = (c->A.n + 511) / 512 * 512;
-      int rc;
-      if ((rc = dalloc(&c->ell.ci, (size_t)W * c->ell.ld))) return rc;
-      if ((rc = dalloc(&c->ell.v, (size_t)W * c->ell.ld))) return rc;
-      xs_build_ell(c->stream, c->A, W, c->ell);
+      int rc = make_ell(c);
+      if (rc) return rc;
       XS_HIP(hipStreamSynchronize(c->stream));
     }
     c->use_ell = want;
@@ -1094,7 +1107,7 @@ static int ensure_tri_ell(xs_ctx *c) {
       xs_tell &E = part ? F.tu : F.tl;
       const size_t need = (size_t)w[part] * F.n;
       if (need > E.cap) {
-        dfree(E.ci); dfree(E.v); dfree(E.v32);
+        dfree(E.ci); dfree(E.v); dfree(E.v32); dfree(E.co);
         E.cap = 0;
         if ((rc = dalloc(&E.ci, need)) || (rc = dalloc(&E.v, need))) return rc;
         E.cap = need;
@@ -1102,9 +1115,15 @@ static int ensure_tri_ell(xs_ctx *c) {
       E.W = w[part];
       E.ld = F.n;
     }
-    F.tell_shape = true;
   }
   for (int part = 0; part < 2; ++part) xs_build_tell(c->stream, F, part == 1, part ? F.tu : F.tl);
+  if (!F.tell_shape) {   // offsets depend on the pattern only
+    for (int part = 0; part < 2; ++part) {
+      xs_tell &E = part ? F.tu : F.tl;
+      if ((rc = make_offsets(c, F.n, E.ld, E.W, E.ci, &E.co))) return rc;
+    }
+    F.tell_shape = true;
+  }
   XS_HIP(hipStreamSynchronize(c->stream));
   if (xs_take_launch_guard()) return XS_ERR_STATE;
   c->tell_ready = true;
@@ -1370,6 +1389,8 @@ extern "C" double xs_get_time_ms(const xs_ctx *c, const char *phase) {
   // SpMV kernel in use: 0 LDS row blocks, 1 vector CSR, 2 ELL
   if (!strcmp(phase, "spmv:kernel"))
     return !c->mat_ready ? -1.0 : (c->use_ell && c->ell.v) ? 2.0 : (double)c->plan.vec;
+  // 1 when the ELL copy carries 16-bit column offsets
+  if (!strcmp(phase, "spmv:idx16")) return c->ell.co ? 1.0 : 0.0;
   static const char *kn[4] = {"", "spmv", "trsv_lower", "trsv_upper"};
   for (int k = 1; k < 4; k++) {
     std::string tot = std::string("total:") + kn[k], cnt = std::string("count:") + kn[k];

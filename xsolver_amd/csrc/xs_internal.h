@@ -84,7 +84,12 @@ struct xs_ell {         // column-major slabs: v[k * ld + row], width W
   int n = 0, ld = 0, W = 0;
   int *ci = nullptr;
   double *v = nullptr;
+  short *co = nullptr;  // column - row as 16 bits (when every entry fits), else null
 };
+
+// 16-bit column offsets: padding of the sweep copies (column -1)
+#define XS_OFF16_PAD (-32768)
+#define XS_OFF16_MAX 32767
 
 // Combined ILU factors aligned with a CSR pattern: strict lower part of row i
 // in [rp[i], dpos[i]), pivot at dpos[i], strict upper part in (dpos[i], rp[i+1]).
@@ -97,6 +102,7 @@ struct xs_tell {
   int *ci = nullptr;
   double *v = nullptr;
   float *v32 = nullptr;      // FP32 values for the mixed-precision inner solver
+  short *co = nullptr;       // column - row as 16 bits, XS_OFF16_PAD = padding
 };
 
 struct xs_ilu {
@@ -143,6 +149,11 @@ void xs_launch_spmv_t(hipStream_t s, const xs_csr &A, const double *x, double *y
 void xs_build_plan(hipStream_t s, const xs_csr &A, xs_spmv_plan &P, int *blk, int chunk);
 int xs_default_chunk(const xs_csr &A);
 void xs_build_ell(hipStream_t s, const xs_csr &A, int W, xs_ell &E);
+// Largest |column - row| over the non-padding entries of ELL slabs (rows < n,
+// column >= 0); synchronous, -1 on failure.  Then the 16-bit offsets: padding
+// columns (-1) become XS_OFF16_PAD, rows >= n point at row n - 1.
+int xs_ell_max_offset(hipStream_t s, int n, int ld, int W, const int *eci, int *d_tmp);
+void xs_ell_build_offsets(hipStream_t s, int n, int ld, int W, const int *eci, short *eco);
 int xs_max_row_len(hipStream_t s, const xs_csr &A, int *d_tmp);
 
 // fills / vector ops

@@ -193,11 +193,11 @@ static void spmv_vec_launch(hipStream_t s, const xs_csr &A, const xs_spmv_plan &
 // Reference: matMultELL / bmAxELL (DeviceMatrixELL.cpp:3-34), width fixed at
 // 16 there; here the width is the true maximum row length.
 #define XS_ELL_UK 8
-template <int EPI>
+template <int EPI, bool O16>
 __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell(
-    int n, int ld, int W, const int *__restrict__ ci, const double *__restrict__ v,
-    const double *__restrict__ x, double *__restrict__ y, const double *__restrict__ b,
-    const double *__restrict__ w, xs_reduce R) {
+    int n, int ld, int W, const int *__restrict__ ci, const short *__restrict__ co,
+    const double *__restrict__ v, const double *__restrict__ x, double *__restrict__ y,
+    const double *__restrict__ b, const double *__restrict__ w, xs_reduce R) {
   constexpr int NV = (EPI == EPI_DOT2) ? 2 : 1;
   constexpr bool RED = (EPI == EPI_DOT1 || EPI == EPI_DOT2 || EPI == EPI_BMAXN);
   if (R.st != nullptr && R.st->done) return;
@@ -213,7 +213,12 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell(
 #pragma unroll
       for (int q = 0; q < XS_ELL_UK; ++q) {
         if (k0 + q < W) {
-          c[q] = *reinterpret_cast<const int2 *>(ci + (size_t)(k0 + q) * ld + r);
+          if (O16) {   // 16-bit offsets: 4 bytes of indices for the two rows
+            const short2 o = *reinterpret_cast<const short2 *>(co + (size_t)(k0 + q) * ld + r);
+            c[q] = make_int2(r + o.x, r + 1 + o.y);
+          } else {
+            c[q] = *reinterpret_cast<const int2 *>(ci + (size_t)(k0 + q) * ld + r);
+          }
           a[q] = *reinterpret_cast<const double2 *>(v + (size_t)(k0 + q) * ld + r);
         }
       }
@@ -281,13 +286,17 @@ void xs_launch_spmv_ell(hipStream_t s, const xs_ell &E, const double *x, double 
   xs_reduce R = red ? *red : no_reduce();
   int threads = (E.n + 1) / 2;
   dim3 grid((threads + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+#define XS_ELLK(E_, O) hipLaunchKernelGGL((k_spmv_ell<E_, O>), grid, blk, 0, s, E.n, E.ld, E.W, E.ci, E.co, E.v, x, y, b, w, R)
+#define XS_ELLE(E_) do { if (E.co) XS_ELLK(E_, true); else XS_ELLK(E_, false); } while (0)
   switch (epi) {
-    case EPI_Y: hipLaunchKernelGGL(k_spmv_ell<EPI_Y>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, E.v, x, y, b, w, R); break;
-    case EPI_BMAX: hipLaunchKernelGGL(k_spmv_ell<EPI_BMAX>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, E.v, x, y, b, w, R); break;
-    case EPI_DOT1: hipLaunchKernelGGL(k_spmv_ell<EPI_DOT1>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, E.v, x, y, b, w, R); break;
-    case EPI_DOT2: hipLaunchKernelGGL(k_spmv_ell<EPI_DOT2>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, E.v, x, y, b, w, R); break;
-    case EPI_BMAXN: hipLaunchKernelGGL(k_spmv_ell<EPI_BMAXN>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, E.v, x, y, b, w, R); break;
+    case EPI_Y: XS_ELLE(EPI_Y); break;
+    case EPI_BMAX: XS_ELLE(EPI_BMAX); break;
+    case EPI_DOT1: XS_ELLE(EPI_DOT1); break;
+    case EPI_DOT2: XS_ELLE(EPI_DOT2); break;
+    case EPI_BMAXN: XS_ELLE(EPI_BMAXN); break;
   }
+#undef XS_ELLE
+#undef XS_ELLK
 }
 
 // --- analysis kernels -------------------------------------------------------
@@ -376,6 +385,47 @@ __global__ void k_build_ell(int n, int ld, int W, const int *__restrict__ rp,
     if (k < len) { eci[p] = ci[a + k]; ev[p] = v[a + k]; }
     else { eci[p] = own; ev[p] = 0.0; }
   }
+}
+
+__global__ __launch_bounds__(XS_BLOCK) void k_ell_max_offset(int n, int ld, int W,
+                                                             const int *__restrict__ eci,
+                                                             int *out) {
+  int m = 0;
+  for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK)
+    for (int k = 0; k < W; ++k) {
+      const int c = eci[(size_t)k * ld + i];
+      if (c >= 0) m = max(m, abs(c - i));
+    }
+  m = block_max(m);
+  if (threadIdx.x == 0) atomicMax(out, m);
+}
+
+int xs_ell_max_offset(hipStream_t s, int n, int ld, int W, const int *eci, int *d_tmp) {
+  if (!eci || !d_tmp) { xs_raise_launch_guard(); return -1; }
+  int h = 0;
+  if (hipMemsetAsync(d_tmp, 0, sizeof(int), s) != hipSuccess) return -1;
+  hipLaunchKernelGGL(k_ell_max_offset, dim3(std::max(1, std::min((n + XS_BLOCK - 1) / XS_BLOCK, XS_MAX_GRID))),
+                     dim3(XS_BLOCK), 0, s, n, ld, W, eci, d_tmp);
+  if (hipMemcpyAsync(&h, d_tmp, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return -1;
+  return h;
+}
+
+__global__ void k_ell_build_offsets(int n, int ld, int W, const int *__restrict__ eci,
+                                    short *__restrict__ eco) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ld) return;
+  for (int k = 0; k < W; ++k) {
+    const size_t p = (size_t)k * ld + i;
+    const int c = eci[p];
+    eco[p] = (short)(i >= n ? (n - 1) - i : c < 0 ? XS_OFF16_PAD : c - i);
+  }
+}
+
+void xs_ell_build_offsets(hipStream_t s, int n, int ld, int W, const int *eci, short *eco) {
+  XS_GUARD(eci && eco && n > 0 && ld - n <= XS_OFF16_MAX);
+  hipLaunchKernelGGL(k_ell_build_offsets, dim3((ld + 255) / 256), dim3(256), 0, s, n, ld, W, eci, eco);
 }
 
 void xs_build_ell(hipStream_t s, const xs_csr &A, int W, xs_ell &E) {
@@ -1064,9 +1114,10 @@ void xs_build_tell(hipStream_t s, const xs_ilu &F, bool upper, xs_tell &E) {
 // i.e. sweeps 1 and 2 from y = 0 in one pass.  Slabs four at a time: all
 // index/value loads of a batch, then all gathers.  Rows in 256-row blocks
 // placed by xcd_chunk.
-template <bool UPPER, bool FUSED>
+template <bool UPPER, bool FUSED, bool O16>
 __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W,
                                                             const int *__restrict__ eci,
+                                                            const short *__restrict__ eco,
                                                             const double *__restrict__ ev,
                                                             const double *__restrict__ dinv,
                                                             const double *__restrict__ b,
@@ -1082,7 +1133,12 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W
     double a[4], y[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      c[q] = k0 + q < W ? eci[(size_t)(k0 + q) * ld + i] : -1;
+      if (O16) {
+        const int o = k0 + q < W ? (int)eco[(size_t)(k0 + q) * ld + i] : XS_OFF16_PAD;
+        c[q] = o == XS_OFF16_PAD ? -1 : i + o;
+      } else {
+        c[q] = k0 + q < W ? eci[(size_t)(k0 + q) * ld + i] : -1;
+      }
       a[q] = c[q] >= 0 ? ev[(size_t)(k0 + q) * ld + i] : 0.0;
     }
 #pragma unroll
@@ -1107,10 +1163,12 @@ void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, co
   XS_GUARD((E.W == 0 || (E.ci && E.v)) && E.ld >= F.n && b && yn && (!upper || F.dinv) &&
            (fused || yo));
   const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
-#define XS_SWE(U, FU) hipLaunchKernelGGL((k_tri_sweep_ell<U, FU>), grid, blk, 0, s, F.n, E.ld, E.W, E.ci, E.v, F.dinv, b, yo, yn, st)
+#define XS_SWK(U, FU, O) hipLaunchKernelGGL((k_tri_sweep_ell<U, FU, O>), grid, blk, 0, s, F.n, E.ld, E.W, E.ci, E.co, E.v, F.dinv, b, yo, yn, st)
+#define XS_SWE(U, FU) do { if (E.co) XS_SWK(U, FU, true); else XS_SWK(U, FU, false); } while (0)
   if (upper) { if (fused) XS_SWE(true, true); else XS_SWE(true, false); }
   else { if (fused) XS_SWE(false, true); else XS_SWE(false, false); }
 #undef XS_SWE
+#undef XS_SWK
 }
 
 // ---------------------------------------------------------------------------

@@ -162,11 +162,11 @@ static void spmv_vec32_launch(hipStream_t s, const xs_csr &A, const float *v32,
 }
 
 // ELL SpMV in FP32, two rows per thread (8-byte value and index loads).
-template <int EPI>
+template <int EPI, bool O16>
 __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell32(
-    int n, int ld, int W, const int *__restrict__ ci, const float *__restrict__ v,
-    const float *__restrict__ x, float *__restrict__ y, const float *__restrict__ b,
-    const float *__restrict__ w, xs_reduce R) {
+    int n, int ld, int W, const int *__restrict__ ci, const short *__restrict__ co,
+    const float *__restrict__ v, const float *__restrict__ x, float *__restrict__ y,
+    const float *__restrict__ b, const float *__restrict__ w, xs_reduce R) {
   constexpr int NV = (EPI == EPI_DOT2) ? 2 : 1;
   constexpr bool RED = (EPI == EPI_DOT1 || EPI == EPI_DOT2 || EPI == EPI_BMAXN);
   if (R.st != nullptr && R.st->done) return;
@@ -183,7 +183,12 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell32(
 #pragma unroll
       for (int q = 0; q < UK; ++q) {
         if (k0 + q < W) {
-          c[q] = *reinterpret_cast<const int2 *>(ci + (size_t)(k0 + q) * ld + r);
+          if (O16) {
+            const short2 o = *reinterpret_cast<const short2 *>(co + (size_t)(k0 + q) * ld + r);
+            c[q] = make_int2(r + o.x, r + 1 + o.y);
+          } else {
+            c[q] = *reinterpret_cast<const int2 *>(ci + (size_t)(k0 + q) * ld + r);
+          }
           a[q] = *reinterpret_cast<const float2 *>(v + (size_t)(k0 + q) * ld + r);
         }
       }
@@ -239,13 +244,17 @@ void xs_launch_spmv_ell32(hipStream_t s, const xs_ell &E, const float *v32, cons
   if (!(E.ci && v32 && x && y)) { xs_raise_launch_guard(); return; }
   const int threads = (E.n + 1) / 2;
   dim3 grid((threads + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+#define XS_ELLK32(E_, O) hipLaunchKernelGGL((k_spmv_ell32<E_, O>), grid, blk, 0, s, E.n, E.ld, E.W, E.ci, E.co, v32, x, y, b, w, R)
+#define XS_ELLE32(E_) do { if (E.co) XS_ELLK32(E_, true); else XS_ELLK32(E_, false); } while (0)
   switch (epi) {
-    case EPI_Y: hipLaunchKernelGGL(k_spmv_ell32<EPI_Y>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, v32, x, y, b, w, R); break;
-    case EPI_BMAX: hipLaunchKernelGGL(k_spmv_ell32<EPI_BMAX>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, v32, x, y, b, w, R); break;
-    case EPI_DOT1: hipLaunchKernelGGL(k_spmv_ell32<EPI_DOT1>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, v32, x, y, b, w, R); break;
-    case EPI_DOT2: hipLaunchKernelGGL(k_spmv_ell32<EPI_DOT2>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, v32, x, y, b, w, R); break;
-    case EPI_BMAXN: hipLaunchKernelGGL(k_spmv_ell32<EPI_BMAXN>, grid, blk, 0, s, E.n, E.ld, E.W, E.ci, v32, x, y, b, w, R); break;
+    case EPI_Y: XS_ELLE32(EPI_Y); break;
+    case EPI_BMAX: XS_ELLE32(EPI_BMAX); break;
+    case EPI_DOT1: XS_ELLE32(EPI_DOT1); break;
+    case EPI_DOT2: XS_ELLE32(EPI_DOT2); break;
+    case EPI_BMAXN: XS_ELLE32(EPI_BMAXN); break;
   }
+#undef XS_ELLE32
+#undef XS_ELLK32
 }
 
 // ---------------------------------------------------------------------------
@@ -456,9 +465,10 @@ void xs_tri_sweep32(hipStream_t s, bool upper, int n, const int *rp, const int *
 
 // FP32 Jacobi sweep over the ELL copy of a factor part (FP32 values); the
 // arithmetic of k_tri_sweep32 entry by entry, FUSED as in the FP64 kernel.
-template <bool UPPER, bool FUSED>
+template <bool UPPER, bool FUSED, bool O16>
 __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int W,
                                                               const int *__restrict__ eci,
+                                                              const short *__restrict__ eco,
                                                               const float *__restrict__ ev,
                                                               const float *__restrict__ dinv,
                                                               const float *__restrict__ b,
@@ -474,7 +484,12 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
     float a[4], y[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      c[q] = k0 + q < W ? eci[(size_t)(k0 + q) * ld + i] : -1;
+      if (O16) {
+        const int o = k0 + q < W ? (int)eco[(size_t)(k0 + q) * ld + i] : XS_OFF16_PAD;
+        c[q] = o == XS_OFF16_PAD ? -1 : i + o;
+      } else {
+        c[q] = k0 + q < W ? eci[(size_t)(k0 + q) * ld + i] : -1;
+      }
       a[q] = c[q] >= 0 ? ev[(size_t)(k0 + q) * ld + i] : 0.f;
     }
 #pragma unroll
@@ -502,10 +517,12 @@ void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_t
     return;
   }
   const dim3 grid((n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
-#define XS_SWE32(U, FU) hipLaunchKernelGGL((k_tri_sweep_ell32<U, FU>), grid, blk, 0, s, n, E.ld, E.W, E.ci, E.v32, dinv, b, yo, yn, st)
+#define XS_SWK32(U, FU, O) hipLaunchKernelGGL((k_tri_sweep_ell32<U, FU, O>), grid, blk, 0, s, n, E.ld, E.W, E.ci, E.co, E.v32, dinv, b, yo, yn, st)
+#define XS_SWE32(U, FU) do { if (E.co) XS_SWK32(U, FU, true); else XS_SWK32(U, FU, false); } while (0)
   if (upper) { if (fused) XS_SWE32(true, true); else XS_SWE32(true, false); }
   else { if (fused) XS_SWE32(false, true); else XS_SWE32(false, false); }
 #undef XS_SWE32
+#undef XS_SWK32
 }
 
 __global__ void k_fill_u32(unsigned *p, int n, unsigned val) {
