@@ -313,6 +313,22 @@ def test_iterative_ilu_apply_solve(dev, precision, sweeps):
     g["S"].close()
 
 
+@pytest.mark.parametrize("factor", [0, 1, 2], ids=["f32", "bf16", "f16"])
+def test_mixed_sweep_apply_factor_storage(dev, factor):
+    """FP32-inner refinement with the sweep apply reading the factors' sweep
+    copies stored as FP32 / BF16 / FP16: FP64 true residual below rtol and
+    the solution within the FP64 solve's accuracy."""
+    A = gen.convdiff3d(24)
+    xs, b = gen.manufactured(A, 3)
+    g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000, lsolve_sweeps=5, usolve_sweeps=2,
+                  precision=1, factor_storage=factor)
+    rr = true_relres(A, b, g["x"])
+    say(f"[mixed-sweeps] factor={factor} inner it={g['it']} relres={rr:.2e}")
+    assert g["st"] == 0 and rr < 1.05e-10
+    assert np.linalg.norm(g["x"] - xs) / np.linalg.norm(xs) < 1e-6
+    g["S"].close()
+
+
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
 def test_update_values_then_solve(dev, precision):
     """xs_mat_update_values (the reference's *_mat_update): new values on the

@@ -69,8 +69,8 @@ static void free_precond(xs_ctx *c) {
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
   dfree(F.lsched); dfree(F.usched);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as);
-  dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.v32); dfree(F.tl.co);
-  dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.v32); dfree(F.tu.co);
+  dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.vlo); dfree(F.tl.co);
+  dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co);
   c->tell_ready = c->tell32_ready = false;
   F = xs_ilu();
   F.trsv_wg = c->trsv_wg;
@@ -1107,7 +1107,7 @@ static int ensure_tri_ell(xs_ctx *c) {
       xs_tell &E = part ? F.tu : F.tl;
       const size_t need = (size_t)w[part] * F.n;
       if (need > E.cap) {
-        dfree(E.ci); dfree(E.v); dfree(E.v32); dfree(E.co);
+        dfree(E.ci); dfree(E.v); dfree(E.vlo); dfree(E.co);
         E.cap = 0;
         if ((rc = dalloc(&E.ci, need)) || (rc = dalloc(&E.v, need))) return rc;
         E.cap = need;

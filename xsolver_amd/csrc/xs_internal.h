@@ -101,7 +101,7 @@ struct xs_tell {
   size_t cap = 0;            // allocated entries
   int *ci = nullptr;
   double *v = nullptr;
-  float *v32 = nullptr;      // FP32 values for the mixed-precision inner solver
+  void *vlo = nullptr;       // values for the mixed inner solver (factor storage type)
   short *co = nullptr;       // column - row as 16 bits, XS_OFF16_PAD = padding
 };
 

@@ -16,8 +16,9 @@ void xs_launch_spmv_ell32(hipStream_t s, const xs_ell &E, const float *v32, cons
 void xs_tri_sweep32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, const void *lu,
                     int ftype, const int *dpos, const float *dinv, const float *b, const float *yo,
                     float *yn, const xs_state *st);
-// FP32 sweep over the ELL copy of a factor part (E.v32); fused = sweeps 1+2
-void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_tell &E,
+// FP32 sweep over the ELL copy of a factor part (values E.vlo stored as
+// ftype: FP32 / BF16 / FP16); fused = sweeps 1+2
+void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_tell &E, int ftype,
                         const float *dinv, const float *b, const float *yo, float *yn,
                         const xs_state *st);
 void xs_trsv32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, const void *lu,

@@ -465,11 +465,11 @@ void xs_tri_sweep32(hipStream_t s, bool upper, int n, const int *rp, const int *
 
 // FP32 Jacobi sweep over the ELL copy of a factor part (FP32 values); the
 // arithmetic of k_tri_sweep32 entry by entry, FUSED as in the FP64 kernel.
-template <bool UPPER, bool FUSED, bool O16>
+template <bool UPPER, bool FUSED, bool O16, typename TF>
 __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int W,
                                                               const int *__restrict__ eci,
                                                               const short *__restrict__ eco,
-                                                              const float *__restrict__ ev,
+                                                              const TF *__restrict__ ev,
                                                               const float *__restrict__ dinv,
                                                               const float *__restrict__ b,
                                                               const float *__restrict__ yo,
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
       } else {
         c[q] = k0 + q < W ? eci[(size_t)(k0 + q) * ld + i] : -1;
       }
-      a[q] = c[q] >= 0 ? ev[(size_t)(k0 + q) * ld + i] : 0.f;
+      a[q] = c[q] >= 0 ? fl(ev, (size_t)(k0 + q) * ld + i) : 0.f;
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -508,20 +508,22 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
   yn[i] = UPPER ? dinv[i] * (b[i] - r) : b[i] - r;
 }
 
-void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_tell &E,
+void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_tell &E, int ftype,
                         const float *dinv, const float *b, const float *yo, float *yn,
                         const xs_state *st) {
-  if (!((E.W == 0 || (E.ci && E.v32)) && E.ld >= n && b && yn && (!upper || dinv) &&
+  if (!((E.W == 0 || (E.ci && E.vlo)) && E.ld >= n && b && yn && (!upper || dinv) &&
         (fused || yo))) {
     xs_raise_launch_guard();
     return;
   }
   const dim3 grid((n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
-#define XS_SWK32(U, FU, O) hipLaunchKernelGGL((k_tri_sweep_ell32<U, FU, O>), grid, blk, 0, s, n, E.ld, E.W, E.ci, E.co, E.v32, dinv, b, yo, yn, st)
-#define XS_SWE32(U, FU) do { if (E.co) XS_SWK32(U, FU, true); else XS_SWK32(U, FU, false); } while (0)
+#define XS_SWK32(U, FU, O, T) hipLaunchKernelGGL((k_tri_sweep_ell32<U, FU, O, T>), grid, blk, 0, s, n, E.ld, E.W, E.ci, E.co, static_cast<const T *>(E.vlo), dinv, b, yo, yn, st)
+#define XS_SWT32(U, FU, O) do { if (ftype == XS_BF16) XS_SWK32(U, FU, O, __hip_bfloat16); else if (ftype == XS_F16) XS_SWK32(U, FU, O, __half); else XS_SWK32(U, FU, O, float); } while (0)
+#define XS_SWE32(U, FU) do { if (E.co) XS_SWT32(U, FU, true); else XS_SWT32(U, FU, false); } while (0)
   if (upper) { if (fused) XS_SWE32(true, true); else XS_SWE32(true, false); }
   else { if (fused) XS_SWE32(false, true); else XS_SWE32(false, false); }
 #undef XS_SWE32
+#undef XS_SWT32
 #undef XS_SWK32
 }
 

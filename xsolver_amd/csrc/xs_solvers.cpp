@@ -367,12 +367,12 @@ static int mixed_prepare(xs_ctx *c) {
     if ((rc = ensure_buf(&c->dinv32, (size_t)n))) return rc;
     xs_vec_d2f(s, n, c->ilu.dinv, c->dinv32);
     c->tell32_ready = false;
-    if (c->tell_ready && c->factor_type == XS_F32) {   // FP32 values of the sweep copies
+    if (c->tell_ready) {   // sweep copies in the factor storage type
       for (xs_tell *E : {&c->ilu.tl, &c->ilu.tu}) {
         if (E->W == 0) continue;
-        if (!E->v32 && hipMalloc(&E->v32, sizeof(float) * E->cap) != hipSuccess)
-          return XS_ERR_MEMORY;
-        xs_d2f(s, E->W * E->ld, E->v, E->v32);
+        if (E->vlo) { (void)hipFree(E->vlo); E->vlo = nullptr; }
+        if (hipMalloc(&E->vlo, es * E->cap) != hipSuccess) return XS_ERR_MEMORY;
+        xs_cvt_factor(s, E->W * E->ld, E->v, E->vlo, c->factor_type);
       }
       c->tell32_ready = true;
     }
@@ -406,10 +406,10 @@ static void ilu_sweeps32(xs_ctx *c, bool upper, int k, const float *b, float *ou
   if (m >= 2 && c->tell32_ready) {
     const xs_tell &E = upper ? c->ilu.tu : c->ilu.tl;
     float *cur = ((m - 2) % 2 == 0) ? out : w;
-    xs_tri_sweep_ell32(c->stream, upper, true, n, E, c->dinv32, b, nullptr, cur, st);
+    xs_tri_sweep_ell32(c->stream, upper, true, n, E, c->factor_type, c->dinv32, b, nullptr, cur, st);
     for (int j = 3; j <= m; ++j) {
       float *nxt = (cur == out) ? w : out;
-      xs_tri_sweep_ell32(c->stream, upper, false, n, E, c->dinv32, b, cur, nxt, st);
+      xs_tri_sweep_ell32(c->stream, upper, false, n, E, c->factor_type, c->dinv32, b, cur, nxt, st);
       cur = nxt;
     }
     return;
