@@ -1283,7 +1283,7 @@ static int run_multi(xs_ctx *c, int k, const double *d_b, double *d_x, int layou
     snprintf(xs_errbuf, sizeof(xs_errbuf), "block solve: BiCGSTAB or CG only");
     return XS_ERR_UNSUPPORTED;
   }
-  if ((rc = xs_multi_reserve(c, k, layout == XS_RHS_COLMAJOR ? 2 : 0))) return rc;
+  if ((rc = xs_multi_reserve(c, k, 3))) return rc;   // 9-10: column-major copies, 11: sweeps
   const int n = c->A.n;
   const double *B = d_b;
   double *X = d_x;
@@ -1581,12 +1581,10 @@ extern "C" int xs_pc_apply_multi(xs_ctx *c, int nrhs, const double *x, double *y
       break;
     case XS_PC_ILU0:
     case XS_PC_ILUT: {
-      if ((rc = xs_multi_reserve(c, nrhs, 0))) return rc;
-      double *tmp = xs_multi_vec(c, nrhs, 8);
-      xs_mtrsv(s, false, n, nrhs, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, nullptr, x, tmp,
-               c->counters + 2, c->ilu.lsched, c->ilu.trsv_wg, nullptr);
-      xs_mtrsv(s, true, n, nrhs, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, c->ilu.dinv, tmp, y,
-               c->counters + 4, c->ilu.usched, c->ilu.trsv_wg, nullptr);
+      if ((rc = xs_multi_reserve(c, nrhs, 1))) return rc;
+      if ((c->lsweeps >= 2 || c->usweeps >= 2) && (rc = ensure_tri_ell(c))) return rc;
+      xs_multi_ilu_apply(c, s, nrhs, x, y, xs_multi_vec(c, nrhs, 8), xs_multi_vec(c, nrhs, 9),
+                         nullptr);
       break;
     }
     default:

@@ -69,3 +69,8 @@ void xs_mlayout(hipStream_t s, int n, int k, const double *in, double *out, bool
 void xs_mstate_init(hipStream_t s, xs_mstate *ms, int k);
 // out = in over m doubles (a kernel, so it can sit in a captured graph)
 void xs_mcopy(hipStream_t s, size_t m, const double *in, double *out);
+// one k-column Jacobi sweep of a factor part over its ELL copy (fused: sweeps
+// 1 and 2 from zero)
+void xs_msweep_ell(hipStream_t s, const xs_tell &E, int n, int k, bool upper, bool fused,
+                   const double *dinv, const double *B, const double *Yo, double *Yn,
+                   const xs_mstate *ms);

@@ -536,6 +536,60 @@ static inline int mgrid_flat(size_t m) {
   if (g > 8192) g = 8192;
   return g < 1 ? 1 : (int)g;
 }
+// k-column Jacobi sweep over the ELL copy of a factor part (n x k row-major
+// blocks): thread (row, j) of a workgroup covering 256/k rows x k columns;
+// the row's slab entries are the same for its k threads, the iterate is
+// gathered k columns at a time.  Per column the arithmetic of
+// k_tri_sweep_ell (FUSED: sweeps 1 and 2 from zero), so every column equals
+// the single right-hand side sweep.
+template <bool UPPER, bool FUSED, bool O16>
+__global__ __launch_bounds__(XS_BLOCK) void k_msweep_ell(int n, int k, int ld, int W,
+                                                         const int *__restrict__ eci,
+                                                         const short *__restrict__ eco,
+                                                         const double *__restrict__ ev,
+                                                         const double *__restrict__ dinv,
+                                                         const double *__restrict__ B,
+                                                         const double *__restrict__ Yo,
+                                                         double *__restrict__ Yn,
+                                                         const xs_mstate *ms) {
+  if (ms && ms->all) return;
+  const int per = XS_BLOCK / k;
+  const int j = threadIdx.x % k, rl = threadIdx.x / k;
+  if (rl >= per || (ms && ms->done[j])) return;
+  for (int i = blockIdx.x * per + rl; i < n; i += gridDim.x * per) {
+    double r = 0.0;
+    for (int k0 = 0; k0 < W; k0 += 4) {
+      int c[4];
+      double a[4], y[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (O16) {
+          const int o = k0 + q < W ? (int)eco[(size_t)(k0 + q) * ld + i] : XS_OFF16_PAD;
+          c[q] = o == XS_OFF16_PAD ? -1 : i + o;
+        } else {
+          c[q] = k0 + q < W ? eci[(size_t)(k0 + q) * ld + i] : -1;
+        }
+        a[q] = c[q] >= 0 ? ev[(size_t)(k0 + q) * ld + i] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (c[q] >= 0) {
+          const double bc = B[(size_t)c[q] * k + j];
+          if (FUSED) y[q] = UPPER ? dinv[c[q]] * (bc - 0.0) : bc - 0.0;
+          else y[q] = Yo[(size_t)c[q] * k + j];
+        } else {
+          y[q] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (c[q] >= 0) r += a[q] * y[q];
+    }
+    const double bi = B[(size_t)i * k + j];
+    Yn[(size_t)i * k + j] = UPPER ? dinv[i] * (bi - r) : bi - r;
+  }
+}
+
 static inline bool k_ok(int k) { return k >= 1 && k <= XS_MAX_RHS; }
 
 void xs_mspmm(hipStream_t s, const xs_csr &A, int k, const double *X, double *Y, int epi,
@@ -609,6 +663,20 @@ void xs_mlayout(hipStream_t s, int n, int k, const double *in, double *out, bool
   hipLaunchKernelGGL(k_mlayout, dim3(mgrid_flat((size_t)n * k)), dim3(XS_BLOCK), 0, s, n, k, in,
                      out, to_interleaved ? 1 : 0);
 }
+void xs_msweep_ell(hipStream_t s, const xs_tell &E, int n, int k, bool upper, bool fused,
+                   const double *dinv, const double *B, const double *Yo, double *Yn,
+                   const xs_mstate *ms) {
+  XS_GUARD((E.W == 0 || (E.ci && E.v)) && E.ld >= n && k_ok(k) && B && Yn && (!upper || dinv) &&
+           (fused || Yo));
+  const dim3 grid(mgrid_rows(n, k)), blk(XS_BLOCK);
+#define XS_MSK(U, FU, O) hipLaunchKernelGGL((k_msweep_ell<U, FU, O>), grid, blk, 0, s, n, k, E.ld, E.W, E.ci, E.co, E.v, dinv, B, Yo, Yn, ms)
+#define XS_MSE(U, FU) do { if (E.co) XS_MSK(U, FU, true); else XS_MSK(U, FU, false); } while (0)
+  if (upper) { if (fused) XS_MSE(true, true); else XS_MSE(true, false); }
+  else { if (fused) XS_MSE(false, true); else XS_MSE(false, false); }
+#undef XS_MSE
+#undef XS_MSK
+}
+
 void xs_mcopy(hipStream_t s, size_t m, const double *in, double *out) {
   XS_GUARD(in && out);
   hipLaunchKernelGGL(k_mcopy, dim3(mgrid_flat(m)), dim3(XS_BLOCK), 0, s, m, in, out);

@@ -70,6 +70,45 @@ void xs_multi_free(xs_ctx *c) {
   c->mresvec_len = 0;
 }
 
+// Jacobi sweeps of one factor part for k columns: m <= 1 is the first sweep
+// from zero (b, or D b), m >= 2 runs over the ELL copy (sweeps 1 and 2 in one
+// pass), ping-ponging with w so that the last pass lands in out.
+static void msweeps(xs_ctx *c, hipStream_t s, int k, bool upper, int m, const double *b,
+                    double *out, double *w, const xs_mstate *ms) {
+  const int n = c->A.n;
+  if (m <= 1) {
+    if (upper) xs_mscale(s, n, k, c->ilu.dinv, b, out);
+    else xs_mcopy(s, (size_t)n * k, b, out);
+    return;
+  }
+  if (!c->tell_ready) { xs_raise_launch_guard(); return; }
+  const xs_tell &E = upper ? c->ilu.tu : c->ilu.tl;
+  double *cur = ((m - 2) % 2 == 0) ? out : w;
+  xs_msweep_ell(s, E, n, k, upper, true, c->ilu.dinv, b, nullptr, cur, ms);
+  for (int j = 3; j <= m; ++j) {
+    double *nxt = (cur == out) ? w : out;
+    xs_msweep_ell(s, E, n, k, upper, false, c->ilu.dinv, b, cur, nxt, ms);
+    cur = nxt;
+  }
+}
+
+// k-column ILU apply: Jacobi sweeps per factor when lsolve_sweeps /
+// usolve_sweeps >= 0, else the k-column sync-free triangular solves.
+void xs_multi_ilu_apply(xs_ctx *c, hipStream_t s, int k, const double *in, double *out,
+                        double *tmp, double *w, const xs_mstate *ms) {
+  const int n = c->A.n;
+  if (c->lsweeps >= 0)
+    msweeps(c, s, k, false, c->lsweeps, in, tmp, w, ms);
+  else
+    xs_mtrsv(s, false, n, k, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, nullptr, in, tmp,
+             c->counters + 2, c->ilu.lsched, c->ilu.trsv_wg, ms);
+  if (c->usweeps >= 0)
+    msweeps(c, s, k, true, c->usweeps, tmp, out, w, ms);
+  else
+    xs_mtrsv(s, true, n, k, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, c->ilu.dinv, tmp, out,
+             c->counters + 4, c->ilu.usched, c->ilu.trsv_wg, ms);
+}
+
 namespace {
 
 struct multi_run {
@@ -97,10 +136,7 @@ struct multi_run {
         break;
       case XS_PC_ILU0:
       case XS_PC_ILUT:
-        xs_mtrsv(s, false, n, k, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, nullptr, in, tmp,
-                 c->counters + 2, c->ilu.lsched, c->ilu.trsv_wg, ms);
-        xs_mtrsv(s, true, n, k, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, c->ilu.dinv, tmp,
-                 out, c->counters + 4, c->ilu.usched, c->ilu.trsv_wg, ms);
+        xs_multi_ilu_apply(c, s, k, in, out, tmp, xs_multi_vec(c, k, 11), ms);
         break;
       default:
         xs_mcopy(s, (size_t)n * k, in, out);
@@ -152,7 +188,7 @@ int xs_solve_block(xs_ctx *c, int k, const double *B, double *X) {
   if (k < 1 || k > XS_MAX_RHS) return XS_ERR_ARG;
   if (c->precision != 0) return XS_ERR_UNSUPPORTED;
   if (c->ksp != XS_KSP_BICGSTAB && c->ksp != XS_KSP_CG) return XS_ERR_UNSUPPORTED;
-  int rc = xs_multi_reserve(c, k, 0);
+  int rc = xs_multi_reserve(c, k, 3);   // vectors 9-10: column-major copies, 11: sweep scratch
   if (rc) return rc;
   multi_run m{c, c->A.n, k, c->stream, MS(c)};
   const int n = m.n;

@@ -193,3 +193,7 @@ int xs_solve_block(xs_ctx *c, int k, const double *B, double *X);
 int xs_multi_reserve(xs_ctx *c, int k, int extra_vecs);
 double *xs_multi_vec(xs_ctx *c, int k, int idx);
 void xs_multi_free(xs_ctx *c);
+// k-column ILU apply (sweeps or sync-free solves; xs_multi_host.cpp)
+struct xs_mstate;
+void xs_multi_ilu_apply(xs_ctx *c, hipStream_t s, int k, const double *in, double *out,
+                        double *tmp, double *w, const xs_mstate *ms);
