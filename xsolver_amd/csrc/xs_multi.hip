@@ -531,6 +531,15 @@ static inline int mgrid_rows(int n, int k) {
   if (g > XS_MGRID) g = XS_MGRID;
   return g < 1 ? 1 : (int)g;
 }
+// kernels without a grid reduction: enough workgroups that the dependent
+// loads of a row (slabs or row pointers, then gathers) overlap across waves
+// instead of running back to back in a grid-stride loop
+static inline int mgrid_rows_wide(int n, int k) {
+  const int per = XS_BLOCK / k;
+  long g = ((long)n + per - 1) / per;
+  if (g > 65536) g = 65536;
+  return g < 1 ? 1 : (int)g;
+}
 static inline int mgrid_flat(size_t m) {
   size_t g = (m + XS_BLOCK * 4 - 1) / (XS_BLOCK * 4);
   if (g > 8192) g = 8192;
@@ -629,12 +638,12 @@ void xs_mdot(hipStream_t s, int n, int k, const double *a, const double *b, cons
 void xs_mbicg_p(hipStream_t s, int n, int k, const double *R, const double *U, double *P,
                 const double *dinv, double *Z, const xs_mstate *ms) {
   XS_GUARD(R && U && P && ms && k_ok(k) && (!dinv || Z));
-  hipLaunchKernelGGL(k_mbicg_p, dim3(mgrid_rows(n, k)), dim3(XS_BLOCK), 0, s, n, k, R, U, P, dinv, Z, ms);
+  hipLaunchKernelGGL(k_mbicg_p, dim3(mgrid_rows_wide(n, k)), dim3(XS_BLOCK), 0, s, n, k, R, U, P, dinv, Z, ms);
 }
 void xs_mbicg_s(hipStream_t s, int n, int k, const double *R, const double *U, double *Sv,
                 const double *dinv, double *Z1, const xs_mstate *ms) {
   XS_GUARD(R && U && Sv && ms && k_ok(k) && (!dinv || Z1));
-  hipLaunchKernelGGL(k_mbicg_s, dim3(mgrid_rows(n, k)), dim3(XS_BLOCK), 0, s, n, k, R, U, Sv, dinv, Z1, ms);
+  hipLaunchKernelGGL(k_mbicg_s, dim3(mgrid_rows_wide(n, k)), dim3(XS_BLOCK), 0, s, n, k, R, U, Sv, dinv, Z1, ms);
 }
 void xs_mbicg_xr(hipStream_t s, int n, int k, double *X, double *R, const double *Z,
                  const double *Z1, const double *Sv, const double *V, const double *Rh,
@@ -652,11 +661,11 @@ void xs_mcg_xr(hipStream_t s, int n, int k, double *X, double *R, const double *
 }
 void xs_mcg_p(hipStream_t s, int n, int k, double *P, const double *Z, const xs_mstate *ms) {
   XS_GUARD(P && Z && ms && k_ok(k));
-  hipLaunchKernelGGL(k_mcg_p, dim3(mgrid_rows(n, k)), dim3(XS_BLOCK), 0, s, n, k, P, Z, ms);
+  hipLaunchKernelGGL(k_mcg_p, dim3(mgrid_rows_wide(n, k)), dim3(XS_BLOCK), 0, s, n, k, P, Z, ms);
 }
 void xs_mscale(hipStream_t s, int n, int k, const double *d, const double *X, double *Y) {
   XS_GUARD(d && X && Y && k_ok(k));
-  hipLaunchKernelGGL(k_mscale, dim3(mgrid_rows(n, k)), dim3(XS_BLOCK), 0, s, n, k, d, X, Y);
+  hipLaunchKernelGGL(k_mscale, dim3(mgrid_rows_wide(n, k)), dim3(XS_BLOCK), 0, s, n, k, d, X, Y);
 }
 void xs_mlayout(hipStream_t s, int n, int k, const double *in, double *out, bool to_interleaved) {
   XS_GUARD(in && out && k_ok(k));
