@@ -36,17 +36,20 @@ HBM_PEAK_GBS = 8000.0
 APPLY = (5, 2)   # Jacobi sweeps per factor (L, U) in the headline configuration
 
 
-def algorithmic_bytes(n, nnz, nnz_l, nnz_u, sweeps):
-    """Bytes each kernel (or apply sequence) must move at least: 12 B per
-    stored entry (value + column), 8 B per vector entry read or written.
-    nnz_l / nnz_u: strictly lower / upper entries of the factors.  An apply of
-    k >= 2 sweeps is one fused launch (sweeps 1-2, no previous iterate) plus
-    k - 2 plain ones; U also reads 1/u_ii."""
+def algorithmic_bytes(n, nnz, nnz_l, nnz_u, sweeps, idx_a=4, idx_f=4):
+    """Bytes each kernel (or apply sequence) must move at least: 8 B of value
+    plus the stored column (idx_a for A's ELL copy, idx_f for the factors'
+    sweep copies: 2 with 16-bit offsets, else 4) per entry, 8 B per vector
+    entry read or written.  nnz_l / nnz_u: strictly lower / upper entries of
+    the factors.  An apply of k >= 2 sweeps is one fused launch (sweeps 1-2,
+    no previous iterate) plus k - 2 plain ones; U also reads 1/u_ii.  The
+    exact SpTRSV reads CSR factors (4-byte columns)."""
     kl, ku = sweeps
-    l_fused, l_plain = 12 * nnz_l + 16 * n, 12 * nnz_l + 24 * n
-    u_fused, u_plain = 12 * nnz_u + 24 * n, 12 * nnz_u + 32 * n
+    ea, ef = 8 + idx_a, 8 + idx_f
+    l_fused, l_plain = ef * nnz_l + 16 * n, ef * nnz_l + 24 * n
+    u_fused, u_plain = ef * nnz_u + 24 * n, ef * nnz_u + 32 * n
     return {
-        "spmv": 12 * nnz + 16 * n,                  # ELL / CSR values+columns, x, y
+        "spmv": ea * nnz + 16 * n,                  # ELL values + columns, x, y
         "ilu_lower_apply": l_fused + (kl - 2) * l_plain,
         "ilu_upper_apply": u_fused + (ku - 2) * u_plain,
         "trsv_lower_exact": 12 * nnz_l + 4 * (n + 1) + 4 * n + 16 * n,
@@ -102,8 +105,10 @@ def run_gpu(A, b, fmt, steps, warmup, profile=True, precision=0, factor_storage=
         if cnt > 0:
             kern[k] = {"count": int(cnt), "total_ms": tot, "avg_us": 1000.0 * tot / cnt}
     x = xd.cpu().numpy()
+    idx = (S.time_ms("spmv:idx16"), S.time_ms("pc:idx16"))
     S.close()
     return {
+        "idx16": idx,
         "sec_per_step": (t1 - t0) / steps,
         "pc_setup_ms": float(np.mean(pc_ms)),
         "solve_ms": float(np.mean(solve_ms)),
@@ -115,9 +120,11 @@ def run_gpu(A, b, fmt, steps, warmup, profile=True, precision=0, factor_storage=
 
 # library timing families -> (kernel name in the rocprofv3 summary, launches)
 PROFILE_NAMES = {
-    "spmv": [("k_spmv_ell<2>", 1)],
-    "ilu_lower_apply": [("k_tri_sweep_ell<false, true>", 1), ("k_tri_sweep_ell<false, false>", APPLY[0] - 2)],
-    "ilu_upper_apply": [("k_tri_sweep_ell<true, true>", 1), ("k_tri_sweep_ell<true, false>", APPLY[1] - 2)],
+    "spmv": [("k_spmv_ell<2, true>", 1)],
+    "ilu_lower_apply": [("k_tri_sweep_ell<false, true, true>", 1),
+                        ("k_tri_sweep_ell<false, false, true>", APPLY[0] - 2)],
+    "ilu_upper_apply": [("k_tri_sweep_ell<true, true, true>", 1),
+                        ("k_tri_sweep_ell<true, false, true>", APPLY[1] - 2)],
     "trsv_lower_exact": [("k_trsv<false>", 1)],
     "trsv_upper_exact": [("k_trsv<true>", 1)],
 }
@@ -339,7 +346,6 @@ def main():
     xs, b = gen.manufactured(A, 1)
     n, nnz = len(A[0]) - 1, len(A[2])
     nnz_l = (nnz - n) // 2          # symmetric pattern: strictly lower = strictly upper
-    by = algorithmic_bytes(n, nnz, nnz_l, nnz_l, APPLY)
 
     def relres_of(r):
         return float(np.linalg.norm(b - gen.spmv(A, r["x"])) / np.linalg.norm(b)) if r else None
@@ -369,6 +375,8 @@ def main():
     ell = None if args.no_ell else run_gpu(A, b, "ell", args.steps, args.warmup,
                                            profile=False, sweeps=APPLY)
 
+    ia, ifac = headp["idx16"]
+    by = algorithmic_bytes(n, nnz, nnz_l, nnz_l, APPLY, 2 if ia == 1 else 4, 2 if ifac == 1 else 4)
     # kernel families: the headline run's SpMV and ILU applies, the exact run's SpTRSV
     fam = {"spmv": headp["kernels"].get("spmv"),
            "ilu_lower_apply": headp["kernels"].get("trsv_lower"),

@@ -149,7 +149,8 @@ int xs_set_option(xs_ctx *ctx, const char *name, double value);
  * use: xs_get_time_ms("spmv:kernel") = 0 LDS row blocks, 1 vector CSR, 2 ELL;
  * xs_get_time_ms("spmv:idx16") = 1 when the ELL copy stores 16-bit column
  * offsets (every entry within 32767 of its row), 0 when it reads int32
- * columns. */
+ * columns; xs_get_time_ms("pc:idx16") the same for the ELL copies of L and U
+ * used by the Jacobi-sweep apply (-1 before they are built). */
 int xs_autotune(xs_ctx *ctx);
 
 /* Factorise / prepare the preconditioner now (otherwise done by the first

@@ -1391,6 +1391,10 @@ extern "C" double xs_get_time_ms(const xs_ctx *c, const char *phase) {
     return !c->mat_ready ? -1.0 : (c->use_ell && c->ell.v) ? 2.0 : (double)c->plan.vec;
   // 1 when the ELL copy carries 16-bit column offsets
   if (!strcmp(phase, "spmv:idx16")) return c->ell.co ? 1.0 : 0.0;
+  // 1 when both sweep copies of the factors carry 16-bit offsets (-1: not built)
+  if (!strcmp(phase, "pc:idx16"))
+    return !c->tell_ready ? -1.0
+                          : ((c->ilu.tl.co || c->ilu.tl.W == 0) && (c->ilu.tu.co || c->ilu.tu.W == 0)) ? 1.0 : 0.0;
   static const char *kn[4] = {"", "spmv", "trsv_lower", "trsv_upper"};
   for (int k = 1; k < 4; k++) {
     std::string tot = std::string("total:") + kn[k], cnt = std::string("count:") + kn[k];
