@@ -535,7 +535,10 @@ int or_bicgstab(int n, const int *rp, const int *ci, const double *v,
     apply_M(M, Mctx, n, s, z1);
     or_spmv(n, rp, ci, v, z1, vv);
     double t0 = dot(n, s, vv), t1 = dot(n, vv, vv);
-    alpha = t0 / t1;
+    /* the reference divides unguarded (xsolver_bicgstab.cpp:95); when the
+       half step already solved the system (s = 0, so v = 0) that is 0/0 and
+       every later value is NaN -- take alpha = 0, keeping x + gama z */
+    alpha = t1 != 0.0 ? t0 / t1 : 0.0;
     for (int i = 0; i < n; i++) x[i] = x[i] + gama * z[i] + alpha * z1[i];
     copy(n, s, r);
     axpy(n, -alpha, vv, r);

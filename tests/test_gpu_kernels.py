@@ -136,6 +136,67 @@ def test_ell_column_offsets_and_wide_band(dev):
         S.close()
 
 
+def _tridiag(n):
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j, a in ((i - 1, -1.0), (i, 3.0), (i + 1, -1.2)):
+            if 0 <= j < n:
+                rows.append(i); cols.append(j); vals.append(a)
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, np.array(rows) + 1, 1)
+    return np.cumsum(rp).astype(np.int32), np.array(cols, np.int32), np.array(vals)
+
+
+@pytest.mark.parametrize("name", ["diag_100", "tridiag_1", "tridiag_65", "tridiag_1000",
+                                  "random_257"])
+def test_small_and_ragged_sizes(dev, name):
+    """Edge sizes for the ELL copies, 16-bit offsets and sweep apply: a
+    diagonal matrix (empty strict parts, width 0), n = 1, n not a multiple of
+    64 / 256 / 512; SpMV, the 3/2 sweep apply (FP64 and k-column) and a solve
+    against the oracle."""
+    import torch
+    if name == "diag_100":
+        n = 100
+        A = (np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.linspace(1, 3, n))
+    elif name.startswith("tridiag"):
+        A = _tridiag(int(name.split("_")[1]))
+    else:
+        A = random_sparse(257, 4, 17)
+    n = len(A[0]) - 1
+    S = XSolver()
+    S.mat_setup(*A)
+    x = np.random.default_rng(3).uniform(-1, 1, n)
+    yd = torch.empty(n, dtype=torch.float64, device=dev)
+    S.spmv(dev_tensor(x, dev), yd, torch_stream())
+    torch.cuda.synchronize()
+    assert rel_err(yd.cpu().numpy(), oracle.spmv(A, x)) < 1e-14
+    S.ksp_settype("bicgstab")
+    S.ksp_setoption(1e-10, 500)
+    S.pc_settype("ilu0")
+    S.set_option("parilu_async", 0)
+    S.set_option("parilu_sweeps", 60)
+    S.set_option("lsolve_sweeps", 3)
+    S.set_option("usolve_sweeps", 2)
+    S.pc_setup()
+    rp, ci, lu, dpos = S.factors(n, len(A[2]))
+    S.pc_apply(dev_tensor(x, dev), yd, torch_stream())
+    torch.cuda.synchronize()
+    ref = oracle.ilu_sweeps(rp, ci, lu, dpos, 3, 2, x)
+    assert rel_err(yd.cpu().numpy(), ref) < 1e-13
+    X = np.random.default_rng(4).uniform(-1, 1, (n, 3))
+    Yd = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    S.pc_apply_multi(dev_tensor(X, dev), Yd, torch_stream())
+    torch.cuda.synchronize()
+    for j in range(3):
+        assert rel_err(Yd.cpu().numpy()[:, j], oracle.ilu_sweeps(rp, ci, lu, dpos, 3, 2, X[:, j])) < 1e-13
+    _, b = gen.manufactured(A, 5)
+    xg, st = S.solve(b)
+    say(f"[edge] {name}: n={n} it={S.iterations} idx16={S.time_ms('spmv:idx16')} "
+        f"pc_idx16={S.time_ms('pc:idx16')}")
+    assert st == 0 and true_relres(A, b, xg) < 1.05e-10
+    S.close()
+
+
 TRSV_MATS = {
     "convdiff3d_20": lambda: gen.convdiff3d(20),     # in-wave chains (scan path)
     "poisson2d_16": lambda: gen.poisson2d(16),       # in-wave deps r-1, r-16 (loop path)

@@ -400,3 +400,29 @@ def test_solve_with_iterative_ilu_apply():
     assert s2 == 0 and i2 >= ie
     r = b - gen.spmv(A, x2)
     assert np.linalg.norm(r) / np.linalg.norm(b) < 1.05e-10
+
+
+def test_bicgstab_exact_preconditioner_stops_cleanly():
+    """With M = A (n = 1 ILU(0), or Jacobi on a diagonal matrix) the first
+    half step solves the system, s = v = 0; (s.v)/(v.v) would be 0/0 (the
+    reference divides unguarded): the restatement takes 0 and converges in
+    one iteration with the exact solution."""
+    for A, pc in ((_csr1(np.array([[3.0]])), "ilu0"),
+                  (_csr1(np.diag([1.0, 2.0, 4.0])), "jacobi")):
+        n = len(A[0]) - 1
+        b = np.arange(1.0, n + 1.0)
+        x, st, it, res = oracle.solve(A, b, "bicgstab", pc, tol=1e-10, maxiter=10)
+        assert st == 0 and it == 1
+        assert np.allclose(x, b / np.diag(_dense(A)), rtol=1e-15)
+
+
+def _csr1(D):
+    import scipy.sparse as sp
+    M = sp.csr_matrix(D)
+    return M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.astype(np.float64)
+
+
+def _dense(A):
+    import scipy.sparse as sp
+    rp, ci, v = A
+    return sp.csr_matrix((v, ci, rp), shape=(len(rp) - 1, len(rp) - 1)).toarray()

@@ -222,8 +222,8 @@ __device__ inline void do_finish(const xs_reduce &R, const double *tot, int NV) 
     case F_BICG_GAMA:   // gama = beta / (rhat.u)  xsolver_bicgstab.cpp:80
       sc[S_GAMA] = sc[S_BETA] / tot[0];
       break;
-    case F_BICG_ALPHA:  // alpha = (s.v)/(v.v)      :95
-      sc[S_ALPHA] = tot[0] / tot[1];
+    case F_BICG_ALPHA:  // alpha = (s.v)/(v.v)      :95; v = 0 (s solved exactly): 0, not 0/0
+      sc[S_ALPHA] = tot[1] != 0.0 ? tot[0] / tot[1] : 0.0;
       break;
     case F_BICG_RES: {  // resl = ||r|| (:101-103); next beta/omega (:50-53)
       double resl = sqrt(tot[0]);

@@ -52,7 +52,7 @@ __device__ inline void m_finish(const xs_mred &R, int j, int k, const double *to
       MSC(ms, MS_GAMA, j) = MSC(ms, MS_BETA, j) / tot[0];
       break;
     case MF_BICG_ALPHA:
-      MSC(ms, MS_ALPHA, j) = tot[0] / tot[1];
+      MSC(ms, MS_ALPHA, j) = tot[1] != 0.0 ? tot[0] / tot[1] : 0.0;   // v = 0: s solved exactly
       break;
     case MF_BICG_RES: {
       const double resl = sqrt(tot[0]);
