@@ -122,6 +122,21 @@ __device__ __forceinline__ unsigned trsv_next(unsigned *tickets, unsigned &base)
   base = s_ticket * (XS_BLOCK / XS_WAVE);
   return base + threadIdx.x / XS_WAVE;
 }
+// Groups of a triangular solve and the rows [r0, r1) of group w.
+__device__ __forceinline__ unsigned trsv_ngroups(int n, const xs_tsched &S) {
+  return S.gb ? (unsigned)S.ngroups : (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
+}
+template <bool UPPER>
+__device__ __forceinline__ void trsv_rows(int n, const xs_tsched &S, unsigned w, int &r0, int &r1) {
+  if (S.gb) {
+    if (!UPPER) { r0 = S.gb[w]; r1 = S.gb[w + 1]; }
+    else { r1 = S.gb[w]; r0 = S.gb[w + 1]; }
+  } else if (!UPPER) {
+    r0 = (int)w * XS_WAVE; r1 = min(n, r0 + XS_WAVE);
+  } else {
+    r1 = n - (int)w * XS_WAVE; r0 = max(0, r1 - XS_WAVE);
+  }
+}
 // The workgroup that retires last resets both counters for the next launch.
 __device__ __forceinline__ void trsv_retire(unsigned *tickets) {
   if (threadIdx.x == 0) {
@@ -257,6 +272,8 @@ __device__ inline void do_finish(const xs_reduce &R, const double *tot, int NV) 
 // at most one slot per thread for up to 64 x 64 blocks.  Layout: block
 // partials [gridDim.x][NV], then group partials [ngroups][NV].
 #define XS_RED_GROUP 64
+// in-wave entries a triangular-solve lane keeps in registers
+#define XS_INW 8
 template <int NV>
 __device__ __forceinline__ void group_gather(const double *slots, unsigned m, const xs_reduce &R,
                                              double (&tot)[NV]) {

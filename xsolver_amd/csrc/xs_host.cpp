@@ -67,7 +67,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.ucp); dfree(F.uri); dfree(F.upos);
   dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own); dfree(F.trp); dfree(F.tci);
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
-  dfree(F.lsched); dfree(F.usched);
+  dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as);
   dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.vlo); dfree(F.tl.co);
   dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co);
@@ -676,28 +676,59 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
 // dependency frontier.  Every dependency of a group has a smaller level, so
 // it still holds an earlier ticket: the no-deadlock argument is unchanged.
 // This is an order for the tickets only -- no level-synchronous execution.
+// Row groups of a triangular solve in solve order (L: ascending rows, U:
+// descending): at most XS_WAVE rows, cut at the last chain break within
+// reach -- a row that does not depend on the row before it in solve order --
+// so that groups follow the matrix's own chains (grid lines of 100 rows give
+// groups of 64 + 36 rows instead of groups straddling lines, each waiting on
+// the one before it).  gb holds the G + 1 boundaries in solve order.
+static void trsv_groups(int n, const int *rp, const int *ci, const int *dpos, bool upper,
+                        std::vector<int> &gb) {
+  auto brk = [&](int q) {   // position q does not depend on position q - 1
+    if (!upper) return !(dpos[q] > rp[q] && ci[dpos[q] - 1] == q - 1);
+    const int i = n - 1 - q;
+    return !(dpos[i] + 1 < rp[i + 1] && ci[dpos[i] + 1] == i + 1);
+  };
+  gb.assign(1, upper ? n : 0);
+  int p = 0;
+  while (p < n) {
+    const int reach = std::min(n, p + XS_WAVE);
+    int cut = reach;
+    if (reach < n) {
+      for (int q = reach; q > p + 1; --q)
+        if (brk(q)) { cut = q; break; }
+    }
+    p = cut;
+    gb.push_back(upper ? n - p : p);
+  }
+}
+
+// Dependency level of each group (longest chain of group dependencies below
+// it) and the groups sorted by level: the ticket order of the solve.
 static void group_order(int n, const int *rp, const int *ci, const int *dpos, bool upper,
-                        std::vector<int> &order) {
-  const int G = (n + XS_WAVE - 1) / XS_WAVE;
-  std::vector<int> lev(G, 0);
+                        const std::vector<int> &gb, std::vector<int> &order) {
+  const int G = (int)gb.size() - 1;
+  std::vector<int> rg(n), lev(G, 0);
+  for (int g = 0; g < G; ++g) {
+    const int r0 = upper ? gb[g + 1] : gb[g], r1 = upper ? gb[g] : gb[g + 1];
+    for (int r = r0; r < r1; ++r) rg[r] = g;
+  }
   int maxlev = 0;
   for (int g = 0; g < G; ++g) {
-    int r0, r1;
-    if (!upper) { r0 = g * XS_WAVE; r1 = std::min(n, r0 + XS_WAVE); }
-    else { r1 = n - g * XS_WAVE; r0 = std::max(0, r1 - XS_WAVE); }
+    const int r0 = upper ? gb[g + 1] : gb[g], r1 = upper ? gb[g] : gb[g + 1];
     int L = 0;
     for (int r = r0; r < r1; ++r) {
       if (!upper) {
         for (int e = rp[r]; e < dpos[r]; ++e) {
           const int col = ci[e];
           if (col >= r0) break;
-          L = std::max(L, lev[col / XS_WAVE] + 1);
+          L = std::max(L, lev[rg[col]] + 1);
         }
       } else {
         for (int e = rp[r + 1] - 1; e > dpos[r]; --e) {
           const int col = ci[e];
           if (col < r1) break;
-          L = std::max(L, lev[(n - 1 - col) / XS_WAVE] + 1);
+          L = std::max(L, lev[rg[col]] + 1);
         }
       }
     }
@@ -711,21 +742,32 @@ static void group_order(int n, const int *rp, const int *ci, const int *dpos, bo
   for (int g = 0; g < G; ++g) order[start[lev[g]]++] = g;
 }
 
-// host copies of a factor pattern -> c->ilu.lsched / usched
+// host copies of a factor pattern -> row groups (c->ilu.lgb / ugb) and, with
+// trsv_schedule, their level order (c->ilu.lsched / usched)
 static int build_trsv_schedules(xs_ctx *c, const int *rp, const int *ci, const int *dpos) {
   xs_ilu &F = c->ilu;
   dfree(F.lsched);
   dfree(F.usched);
-  if (!c->trsv_schedule) return XS_OK;
-  const int n = c->A.n, G = (n + XS_WAVE - 1) / XS_WAVE;
-  std::vector<int> ord;
+  dfree(F.lgb);
+  dfree(F.ugb);
+  F.lgroups = F.ugroups = 0;
+  const int n = c->A.n;
+  std::vector<int> gb, ord;
   int rc;
-  group_order(n, rp, ci, dpos, false, ord);
-  if ((rc = dalloc(&F.lsched, G))) return rc;
-  XS_HIP(hipMemcpy(F.lsched, ord.data(), sizeof(int) * G, hipMemcpyHostToDevice));
-  group_order(n, rp, ci, dpos, true, ord);
-  if ((rc = dalloc(&F.usched, G))) return rc;
-  XS_HIP(hipMemcpy(F.usched, ord.data(), sizeof(int) * G, hipMemcpyHostToDevice));
+  for (int part = 0; part < 2; ++part) {
+    const bool upper = part == 1;
+    trsv_groups(n, rp, ci, dpos, upper, gb);
+    const int G = (int)gb.size() - 1;
+    int *&dgb = upper ? F.ugb : F.lgb;
+    if ((rc = dalloc(&dgb, (size_t)G + 1))) return rc;
+    XS_HIP(hipMemcpy(dgb, gb.data(), sizeof(int) * (G + 1), hipMemcpyHostToDevice));
+    (upper ? F.ugroups : F.lgroups) = G;
+    if (!c->trsv_schedule) continue;
+    group_order(n, rp, ci, dpos, upper, gb, ord);
+    int *&ds = upper ? F.usched : F.lsched;
+    if ((rc = dalloc(&ds, (size_t)G))) return rc;
+    XS_HIP(hipMemcpy(ds, ord.data(), sizeof(int) * G, hipMemcpyHostToDevice));
+  }
   return XS_OK;
 }
 

@@ -105,6 +105,17 @@ struct xs_tell {
   short *co = nullptr;       // column - row as 16 bits, XS_OFF16_PAD = padding
 };
 
+// Work order of a sync-free triangular solve: the row groups (gb: ngroups + 1
+// boundaries in solve order -- ascending for L, descending for U; null =
+// consecutive 64-row groups), the ticket -> group order (null = solve order)
+// and the persistent workgroup count.
+struct xs_tsched {
+  const int *gb = nullptr;
+  const int *order = nullptr;
+  int ngroups = 0;
+  int max_wg = 256;
+};
+
 struct xs_ilu {
   int n = 0, nnz = 0;
   const int *rp = nullptr, *ci = nullptr;  // pattern (owned elsewhere or below)
@@ -125,6 +136,10 @@ struct xs_ilu {
   // SpTRSV work order: the 64-row groups of L (resp. U, counted from the
   // bottom) sorted by dependency level; ticket t solves group sched[t]
   int *lsched = nullptr, *usched = nullptr;
+  // row-group boundaries (at most 64 rows, cut where a row does not depend
+  // on its neighbour in solve order) and their counts
+  int *lgb = nullptr, *ugb = nullptr;
+  int lgroups = 0, ugroups = 0;
   int trsv_wg = 256;        // persistent workgroups of the triangular solves
   xs_tell tl, tu;           // sweep copies of L and U (built on demand)
   bool tell_shape = false;  // tl/tu widths and buffers match the pattern
@@ -133,6 +148,15 @@ struct xs_ilu {
   int *trp = nullptr, *tci = nullptr, *tdpos = nullptr, *tperm = nullptr;
   double *tlu = nullptr;
 };
+
+inline xs_tsched xs_ilu_tsched(const xs_ilu &F, bool upper) {
+  xs_tsched t;
+  t.gb = upper ? F.ugb : F.lgb;
+  t.ngroups = upper ? F.ugroups : F.lgroups;
+  t.order = upper ? F.usched : F.lsched;
+  t.max_wg = F.trsv_wg;
+  return t;
+}
 
 // ---------------------------------------------------------------------------
 // Kernel launchers (xs_kernels.hip)

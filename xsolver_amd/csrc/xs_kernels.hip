@@ -935,28 +935,31 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
                                                    const int *__restrict__ dpos,
                                                    const double *__restrict__ dinv,
                                                    const double *__restrict__ b, double *x,
-                                                   unsigned *tickets, const int *__restrict__ sched,
+                                                   unsigned *tickets, const xs_tsched S,
                                                    const xs_state *st) {
   if (st != nullptr && st->done) return;
   unsigned *err = tickets + 4;  // wait-expired flag, 4 words past the tickets
   const int lane = threadIdx.x & (XS_WAVE - 1);
-  const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
+  const unsigned nw = trsv_ngroups(n, S);
   for (;;) {
   unsigned base;
   const unsigned t = trsv_next(tickets, base);
   if (base >= nw) break;
   if (t < nw) {
-    const unsigned w = sched ? (unsigned)sched[t] : t;
+    const unsigned w = S.order ? (unsigned)S.order[t] : t;
     int r0, r1;
-    if (!UPPER) { r0 = (int)w * XS_WAVE; r1 = min(n, r0 + XS_WAVE); }
-    else { r1 = n - (int)w * XS_WAVE; r0 = max(0, r1 - XS_WAVE); }
+    trsv_rows<UPPER>(n, S, w, r0, r1);
     const int nrows = r1 - r0;
     const bool act = lane < nrows;
     const int r = UPPER ? r1 - 1 - lane : r0 + lane;
     double acc = 0.0, di = 1.0;
     int eb = 0, ee = 0;  // in-wave entries [eb, ee)
-    int c_eb = -1;       // first in-wave column and its value, loaded before
-    double l_eb = 0.0;   // any wait so that their latency stays off the chain
+    // the first XS_INW in-wave entries in solve order, loaded before any wait
+    // so that no load sits on the in-wave chain
+    int iw_c[XS_INW];
+    double iw_v[XS_INW];
+#pragma unroll
+    for (int q = 0; q < XS_INW; ++q) { iw_c[q] = -1; iw_v[q] = 0.0; }
     if (act) {
       acc = b[r];
       if (dinv) di = dinv[r];
@@ -974,10 +977,10 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
         while (xb > beg && ci[xb - 1] >= r1) --xb;
         eb = beg; ee = xb;
       }
-      if (eb < ee) {
-        const int ef = UPPER ? ee - 1 : eb;   // the in-wave entry nearest the row
-        c_eb = ci[ef];
-        l_eb = lu[ef];
+#pragma unroll
+      for (int q = 0; q < XS_INW; ++q) {
+        const int e = UPPER ? ee - 1 - q : eb + q;
+        if (q < ee - eb) { iw_c[q] = ci[e]; iw_v[q] = lu[e]; }
       }
       if (!UPPER) {
         for (int e = xb; e < xe; e += 4) acc = trsv_gather4(acc, e, xe, 1, ci, lu, x, err);
@@ -987,25 +990,35 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
     }
     const int nint = ee - eb;
     const int nb = UPPER ? r + 1 : r - 1;
-    const bool chain = !act || nint == 0 || (nint == 1 && c_eb == nb);
+    const bool chain = !act || nint == 0 || (nint == 1 && iw_c[0] == nb);
     double val;
     if (__all(chain)) {
       // x_r = c + m * x_nb, composed along the lanes
       double c[1] = {act ? di * acc : 0.0};
-      double m = (act && nint == 1) ? -di * l_eb : 0.0;
+      double m = (act && nint == 1) ? -di * iw_v[0] : 0.0;
       wave_affine_scan<1>(c, m);
       val = c[0];
     } else {
-      // lane k's row is final once the rows of lanes < k are: broadcast it
-      int e = UPPER ? ee - 1 : eb;
-      int nc = act && nint > 0 ? ci[e] : -1;
+      // lane k's row is final once the rows of lanes < k are: broadcast it;
+      // each lane consumes its in-wave entries from registers in solve order
+      // (shifted down as they are used, refilled from memory past XS_INW)
+      int left = act ? nint - XS_INW : 0;
+      int enext = UPPER ? ee - 1 - XS_INW : eb + XS_INW;
       for (int k = 0; k < nrows; ++k) {
         const double xk = __shfl(acc * di, k, XS_WAVE);
         const int rk = UPPER ? r1 - 1 - k : r0 + k;
-        if (nc == rk) {
-          acc -= lu[e] * xk;
-          if (UPPER) { --e; nc = (e >= eb) ? ci[e] : -1; }
-          else { ++e; nc = (e < ee) ? ci[e] : -1; }
+        if (iw_c[0] == rk) {
+          acc -= iw_v[0] * xk;
+#pragma unroll
+          for (int q = 0; q + 1 < XS_INW; ++q) { iw_c[q] = iw_c[q + 1]; iw_v[q] = iw_v[q + 1]; }
+          if (left > 0) {
+            iw_c[XS_INW - 1] = ci[enext];
+            iw_v[XS_INW - 1] = lu[enext];
+            enext += UPPER ? -1 : 1;
+            --left;
+          } else {
+            iw_c[XS_INW - 1] = -1;
+          }
         }
       }
       val = acc * di;
@@ -1020,33 +1033,37 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
 // (xs_fill_sentinel); the kernel leaves no sentinel behind.
 static void trsv_launch(hipStream_t s, bool upper, int n, const int *rp, const int *ci,
                         const double *lu, const int *dpos, const double *dinv, const double *b,
-                        double *x, unsigned *tickets, const int *sched, int max_wg,
-                        const xs_state *st) {
-  XS_GUARD(rp && ci && lu && dpos && b && x && tickets && max_wg > 0);
-  const int nw = (n + XS_WAVE - 1) / XS_WAVE;
+                        double *x, unsigned *tickets, const xs_tsched &S, const xs_state *st) {
+  XS_GUARD(rp && ci && lu && dpos && b && x && tickets && S.max_wg > 0 && (!S.gb || S.ngroups > 0));
+  const int nw = S.gb ? S.ngroups : (n + XS_WAVE - 1) / XS_WAVE;
   const int tk = (nw + (XS_BLOCK / XS_WAVE) - 1) / (XS_BLOCK / XS_WAVE);
-  dim3 grid(tk < max_wg ? tk : max_wg), blk(XS_BLOCK);
-  if (upper) hipLaunchKernelGGL(k_trsv<true>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, sched, st);
-  else hipLaunchKernelGGL(k_trsv<false>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, sched, st);
+  dim3 grid(tk < S.max_wg ? tk : S.max_wg), blk(XS_BLOCK);
+  if (upper) hipLaunchKernelGGL(k_trsv<true>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, S, st);
+  else hipLaunchKernelGGL(k_trsv<false>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, S, st);
 }
 
 void xs_trsv_lower(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
                    const xs_state *st) {
-  trsv_launch(s, false, F.n, F.rp, F.ci, F.lu, F.dpos, nullptr, b, x, tickets, F.lsched, F.trsv_wg, st);
+  trsv_launch(s, false, F.n, F.rp, F.ci, F.lu, F.dpos, nullptr, b, x, tickets, xs_ilu_tsched(F, false), st);
 }
 void xs_trsv_upper(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
                    const xs_state *st) {
-  trsv_launch(s, true, F.n, F.rp, F.ci, F.lu, F.dpos, F.dinv, b, x, tickets, F.usched, F.trsv_wg, st);
+  trsv_launch(s, true, F.n, F.rp, F.ci, F.lu, F.dpos, F.dinv, b, x, tickets, xs_ilu_tsched(F, true), st);
 }
 // Transposed factors (F^T = U^T | L^T on the transposed pattern): the lower
-// solve divides by u_ii, the upper one has a unit diagonal.
+// solve divides by u_ii, the upper one has a unit diagonal; consecutive
+// 64-row groups in solve order.
 void xs_trsv_lower_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
-  trsv_launch(s, false, F.n, F.trp, F.tci, F.tlu, F.tdpos, F.dinv, b, x, tickets, nullptr, F.trsv_wg, st);
+  xs_tsched S;
+  S.max_wg = F.trsv_wg;
+  trsv_launch(s, false, F.n, F.trp, F.tci, F.tlu, F.tdpos, F.dinv, b, x, tickets, S, st);
 }
 void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
-  trsv_launch(s, true, F.n, F.trp, F.tci, F.tlu, F.tdpos, nullptr, b, x, tickets, nullptr, F.trsv_wg, st);
+  xs_tsched S;
+  S.max_wg = F.trsv_wg;
+  trsv_launch(s, true, F.n, F.trp, F.tci, F.tlu, F.tdpos, nullptr, b, x, tickets, S, st);
 }
 
 // ---------------------------------------------------------------------------

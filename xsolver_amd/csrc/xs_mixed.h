@@ -23,7 +23,7 @@ void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_t
                         const xs_state *st);
 void xs_trsv32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, const void *lu,
                int ftype, const int *dpos, const float *dinv, const float *b, float *x,
-               unsigned *tickets, const int *sched, int max_wg, const xs_state *st);
+               unsigned *tickets, const xs_tsched &S, const xs_state *st);
 void xs_cvt_factor(hipStream_t s, int nnz, const double *src, void *dst, int ftype);
 void xs_d2f(hipStream_t s, int n, const double *a, float *b);
 void xs_vec_d2f(hipStream_t s, int n, const double *d, float *f);

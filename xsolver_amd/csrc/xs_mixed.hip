@@ -343,29 +343,29 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
                                                      const int *__restrict__ dpos,
                                                      const float *__restrict__ dinv,
                                                      const float *__restrict__ b, float *x,
-                                                     unsigned *tickets,
-                                                     const int *__restrict__ sched,
+                                                     unsigned *tickets, const xs_tsched S,
                                                      const xs_state *st) {
   if (st != nullptr && st->done) return;
   unsigned *err = tickets + 4;
   const int lane = threadIdx.x & (XS_WAVE - 1);
-  const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
+  const unsigned nw = trsv_ngroups(n, S);
   for (;;) {
   unsigned base;
   const unsigned t = trsv_next(tickets, base);
   if (base >= nw) break;
   if (t < nw) {
-    const unsigned w = sched ? (unsigned)sched[t] : t;
+    const unsigned w = S.order ? (unsigned)S.order[t] : t;
     int r0, r1;
-    if (!UPPER) { r0 = (int)w * XS_WAVE; r1 = min(n, r0 + XS_WAVE); }
-    else { r1 = n - (int)w * XS_WAVE; r0 = max(0, r1 - XS_WAVE); }
+    trsv_rows<UPPER>(n, S, w, r0, r1);
     const int nrows = r1 - r0;
     const bool act = lane < nrows;
     const int r = UPPER ? r1 - 1 - lane : r0 + lane;
     float acc = 0.f, di = 1.f;
     int eb = 0, ee = 0;
-    int c_eb = -1;       // nearest in-wave entry, loaded before any wait
-    float l_eb = 0.f;
+    int iw_c[XS_INW];    // first in-wave entries in solve order, loaded before any wait
+    float iw_v[XS_INW];
+#pragma unroll
+    for (int q = 0; q < XS_INW; ++q) { iw_c[q] = -1; iw_v[q] = 0.f; }
     if (act) {
       acc = b[r];
       int xb, xe;
@@ -383,10 +383,10 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
         while (xb > beg && ci[xb - 1] >= r1) --xb;
         eb = beg; ee = xb;
       }
-      if (eb < ee) {
-        const int ef = UPPER ? ee - 1 : eb;
-        c_eb = ci[ef];
-        l_eb = fl(lu, ef);
+#pragma unroll
+      for (int q = 0; q < XS_INW; ++q) {
+        const int e = UPPER ? ee - 1 - q : eb + q;
+        if (q < ee - eb) { iw_c[q] = ci[e]; iw_v[q] = fl(lu, e); }
       }
       if (!UPPER) {
         for (int e = xb; e < xe; e += 4) acc = trsv_gather4_32(acc, e, xe, 1, ci, lu, x, err);
@@ -396,23 +396,32 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
     }
     const int nint = ee - eb;
     const int nb = UPPER ? r + 1 : r - 1;
-    const bool chain = !act || nint == 0 || (nint == 1 && c_eb == nb);
+    const bool chain = !act || nint == 0 || (nint == 1 && iw_c[0] == nb);
     float val;
     if (__all(chain)) {
       float c = act ? di * acc : 0.f;
-      float m = (act && nint == 1) ? -di * l_eb : 0.f;
+      float m = (act && nint == 1) ? -di * iw_v[0] : 0.f;
       wave_affine_scan_f(c, m);
       val = c;
     } else {
-      int e = UPPER ? ee - 1 : eb;
-      int nc = act && nint > 0 ? ci[e] : -1;
+      // broadcast loop over register-resident in-wave entries (as k_trsv)
+      int left = act ? nint - XS_INW : 0;
+      int enext = UPPER ? ee - 1 - XS_INW : eb + XS_INW;
       for (int k = 0; k < nrows; ++k) {
         const float xk = __shfl(acc * di, k, XS_WAVE);
         const int rk = UPPER ? r1 - 1 - k : r0 + k;
-        if (nc == rk) {
-          acc -= fl(lu, e) * xk;
-          if (UPPER) { --e; nc = (e >= eb) ? ci[e] : -1; }
-          else { ++e; nc = (e < ee) ? ci[e] : -1; }
+        if (iw_c[0] == rk) {
+          acc -= iw_v[0] * xk;
+#pragma unroll
+          for (int q = 0; q + 1 < XS_INW; ++q) { iw_c[q] = iw_c[q + 1]; iw_v[q] = iw_v[q + 1]; }
+          if (left > 0) {
+            iw_c[XS_INW - 1] = ci[enext];
+            iw_v[XS_INW - 1] = fl(lu, enext);
+            enext += UPPER ? -1 : 1;
+            --left;
+          } else {
+            iw_c[XS_INW - 1] = -1;
+          }
         }
       }
       val = acc * di;
@@ -539,17 +548,18 @@ static inline int vgrid(int n) {
 
 void xs_trsv32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, const void *lu,
                int ftype, const int *dpos, const float *dinv, const float *b, float *x,
-               unsigned *tickets, const int *sched, int max_wg, const xs_state *st) {
-  if (!(rp && ci && lu && dpos && b && x && tickets) || (upper && !dinv) || max_wg <= 0) {
+               unsigned *tickets, const xs_tsched &S, const xs_state *st) {
+  if (!(rp && ci && lu && dpos && b && x && tickets) || (upper && !dinv) || S.max_wg <= 0 ||
+      (S.gb && S.ngroups <= 0)) {
     xs_raise_launch_guard();
     return;
   }
   hipLaunchKernelGGL(k_fill_u32, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s,
                      reinterpret_cast<unsigned *>(x), n, XS_SENTINEL32);
-  const int nw = (n + XS_WAVE - 1) / XS_WAVE;
+  const int nw = S.gb ? S.ngroups : (n + XS_WAVE - 1) / XS_WAVE;
   const int tk = (nw + 3) / 4;
-  dim3 grid(tk < max_wg ? tk : max_wg), blk(XS_BLOCK);
-#define TRSV32(U, T) hipLaunchKernelGGL((k_trsv32<U, T>), grid, blk, 0, s, n, rp, ci, (const T *)lu, dpos, dinv, b, x, tickets, sched, st)
+  dim3 grid(tk < S.max_wg ? tk : S.max_wg), blk(XS_BLOCK);
+#define TRSV32(U, T) hipLaunchKernelGGL((k_trsv32<U, T>), grid, blk, 0, s, n, rp, ci, (const T *)lu, dpos, dinv, b, x, tickets, S, st)
   if (ftype == XS_F32) { if (upper) TRSV32(true, float); else TRSV32(false, float); }
   else if (ftype == XS_BF16) { if (upper) TRSV32(true, __hip_bfloat16); else TRSV32(false, __hip_bfloat16); }
   else { if (upper) TRSV32(true, __half); else TRSV32(false, __half); }

@@ -63,7 +63,7 @@ void xs_mcg_p(hipStream_t s, int n, int k, double *P, const double *Z, const xs_
 void xs_mscale(hipStream_t s, int n, int k, const double *d, const double *X, double *Y);
 void xs_mtrsv(hipStream_t s, bool upper, int n, int k, const int *rp, const int *ci,
               const double *lu, const int *dpos, const double *dinv, const double *B, double *X,
-              unsigned *tickets, const int *sched, int max_wg, const xs_mstate *ms);
+              unsigned *tickets, const xs_tsched &S, const xs_mstate *ms);
 // column-major (k vectors of length n) <-> interleaved n x k
 void xs_mlayout(hipStream_t s, int n, int k, const double *in, double *out, bool to_interleaved);
 void xs_mstate_init(hipStream_t s, xs_mstate *ms, int k);

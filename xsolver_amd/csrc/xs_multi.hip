@@ -404,29 +404,30 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
                                                     const double *__restrict__ dinv,
                                                     const double *__restrict__ B, double *X,
                                                     unsigned *tickets,
-                                                    const int *__restrict__ sched,
+                                                    const xs_tsched S,
                                                     const xs_mstate *ms) {
   if (ms && ms->all) return;
   unsigned *err = tickets + 4;
   const int lane = threadIdx.x & (XS_WAVE - 1);
-  const unsigned nw = (unsigned)((n + XS_WAVE - 1) / XS_WAVE);
+  const unsigned nw = trsv_ngroups(n, S);
   for (;;) {
   unsigned base;
   const unsigned t = trsv_next(tickets, base);
   if (base >= nw) break;
   if (t < nw) {
-    const unsigned w = sched ? (unsigned)sched[t] : t;
+    const unsigned w = S.order ? (unsigned)S.order[t] : t;
     int r0, r1;
-    if (!UPPER) { r0 = (int)w * XS_WAVE; r1 = min(n, r0 + XS_WAVE); }
-    else { r1 = n - (int)w * XS_WAVE; r0 = max(0, r1 - XS_WAVE); }
+    trsv_rows<UPPER>(n, S, w, r0, r1);
     const int nrows = r1 - r0;
     const bool act = lane < nrows;
     const int r = UPPER ? r1 - 1 - lane : r0 + lane;  // lanes along the solve order
     double acc[KC];
     double di = 1.0;
     int eb = 0, ee = 0;  // in-wave entries [eb, ee)
-    int c_eb = -1;       // nearest in-wave entry, loaded before any wait
-    double l_eb = 0.0;
+    int iw_c[XS_INW];    // first in-wave entries in solve order, loaded before any wait
+    double iw_v[XS_INW];
+#pragma unroll
+    for (int q = 0; q < XS_INW; ++q) { iw_c[q] = -1; iw_v[q] = 0.0; }
 #pragma unroll
     for (int q = 0; q < KC; ++q) acc[q] = 0.0;
     if (act) {
@@ -448,10 +449,10 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
         while (xb > beg && ci[xb - 1] >= r1) --xb;
         eb = beg; ee = xb;
       }
-      if (eb < ee) {
-        const int ef = UPPER ? ee - 1 : eb;
-        c_eb = ci[ef];
-        l_eb = lu[ef];
+#pragma unroll
+      for (int q = 0; q < XS_INW; ++q) {
+        const int e = UPPER ? ee - 1 - q : eb + q;
+        if (q < ee - eb) { iw_c[q] = ci[e]; iw_v[q] = lu[e]; }
       }
       for (int e = xb; e < xe; ++e) {
         const double a = lu[e];
@@ -484,29 +485,38 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mtrsv(int n, int k, int j0,
     }
     const int nint = ee - eb;
     const int nb = UPPER ? r + 1 : r - 1;
-    const bool chain = !act || nint == 0 || (nint == 1 && c_eb == nb);
+    const bool chain = !act || nint == 0 || (nint == 1 && iw_c[0] == nb);
     if (__all(chain)) {
       // x_r = c + m x_nb per column, composed along the lanes
-      double m = (act && nint == 1) ? -di * l_eb : 0.0;
+      double m = (act && nint == 1) ? -di * iw_v[0] : 0.0;
 #pragma unroll
       for (int q = 0; q < KC; ++q) acc[q] = act ? di * acc[q] : 0.0;
       wave_affine_scan<KC>(acc, m);
     } else {
-      // lane kk's rows are final once the lanes below are: broadcast them
-      int e = UPPER ? ee - 1 : eb;
-      int nc = act && nint > 0 ? ci[e] : -1;
+      // lane kk's rows are final once the lanes below are: broadcast them;
+      // in-wave entries consumed from registers in solve order (as k_trsv)
+      int left = act ? nint - XS_INW : 0;
+      int enext = UPPER ? ee - 1 - XS_INW : eb + XS_INW;
       for (int kk = 0; kk < nrows; ++kk) {
         const int rk = UPPER ? r1 - 1 - kk : r0 + kk;
-        const bool hit = nc == rk;
-        const double a = hit ? lu[e] : 0.0;
+        const bool hit = iw_c[0] == rk;
+        const double a = hit ? iw_v[0] : 0.0;
 #pragma unroll
         for (int q = 0; q < KC; ++q) {
           const double xk = __shfl(acc[q] * di, kk, XS_WAVE);
           if (hit) acc[q] -= a * xk;
         }
         if (hit) {
-          if (UPPER) { --e; nc = (e >= eb) ? ci[e] : -1; }
-          else { ++e; nc = (e < ee) ? ci[e] : -1; }
+#pragma unroll
+          for (int q = 0; q + 1 < XS_INW; ++q) { iw_c[q] = iw_c[q + 1]; iw_v[q] = iw_v[q + 1]; }
+          if (left > 0) {
+            iw_c[XS_INW - 1] = ci[enext];
+            iw_v[XS_INW - 1] = lu[enext];
+            enext += UPPER ? -1 : 1;
+            --left;
+          } else {
+            iw_c[XS_INW - 1] = -1;
+          }
         }
       }
 #pragma unroll
@@ -698,11 +708,11 @@ void xs_mstate_init(hipStream_t s, xs_mstate *ms, int k) {
 template <bool UPPER>
 static void mtrsv_chunk(hipStream_t s, dim3 grid, int kc, int n, int k, int j0, const int *rp,
                         const int *ci, const double *lu, const int *dpos, const double *dinv,
-                        const double *B, double *X, unsigned *tickets, const int *sched,
+                        const double *B, double *X, unsigned *tickets, const xs_tsched &S,
                         const xs_mstate *ms) {
   const dim3 blk(XS_BLOCK);
 #define XS_MT(KC) \
-  hipLaunchKernelGGL((k_mtrsv<UPPER, KC>), grid, blk, 0, s, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, sched, ms)
+  hipLaunchKernelGGL((k_mtrsv<UPPER, KC>), grid, blk, 0, s, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, S, ms)
   switch (kc) {
     case 16: XS_MT(16); break;
     case 8: XS_MT(8); break;
@@ -717,19 +727,20 @@ static void mtrsv_chunk(hipStream_t s, dim3 grid, int kc, int n, int k, int j0, 
 // 16/8/4/2/1 per launch.  X is sentinel-filled here.
 void xs_mtrsv(hipStream_t s, bool upper, int n, int k, const int *rp, const int *ci,
               const double *lu, const int *dpos, const double *dinv, const double *B, double *X,
-              unsigned *tickets, const int *sched, int max_wg, const xs_mstate *ms) {
-  XS_GUARD(rp && ci && lu && dpos && B && X && tickets && k_ok(k) && max_wg > 0);
+              unsigned *tickets, const xs_tsched &S, const xs_mstate *ms) {
+  XS_GUARD(rp && ci && lu && dpos && B && X && tickets && k_ok(k) && S.max_wg > 0 &&
+           (!S.gb || S.ngroups > 0));
   const size_t m = (size_t)n * k;
   hipLaunchKernelGGL(k_mfill, dim3(mgrid_flat(m)), dim3(XS_BLOCK), 0, s, m,
                      reinterpret_cast<unsigned long long *>(X), XS_SENTINEL);
-  const int nw = (n + XS_WAVE - 1) / XS_WAVE;
+  const int nw = S.gb ? S.ngroups : (n + XS_WAVE - 1) / XS_WAVE;
   const int tk = (nw + (XS_BLOCK / XS_WAVE) - 1) / (XS_BLOCK / XS_WAVE);
-  const dim3 grid(tk < max_wg ? tk : max_wg);
+  const dim3 grid(tk < S.max_wg ? tk : S.max_wg);
   for (int j0 = 0; j0 < k;) {
     const int left = k - j0;
     const int kc = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
-    if (upper) mtrsv_chunk<true>(s, grid, kc, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, sched, ms);
-    else mtrsv_chunk<false>(s, grid, kc, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, sched, ms);
+    if (upper) mtrsv_chunk<true>(s, grid, kc, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, S, ms);
+    else mtrsv_chunk<false>(s, grid, kc, n, k, j0, rp, ci, lu, dpos, dinv, B, X, tickets, S, ms);
     j0 += kc;
   }
 }

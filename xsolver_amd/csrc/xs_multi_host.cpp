@@ -101,12 +101,12 @@ void xs_multi_ilu_apply(xs_ctx *c, hipStream_t s, int k, const double *in, doubl
     msweeps(c, s, k, false, c->lsweeps, in, tmp, w, ms);
   else
     xs_mtrsv(s, false, n, k, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, nullptr, in, tmp,
-             c->counters + 2, c->ilu.lsched, c->ilu.trsv_wg, ms);
+             c->counters + 2, xs_ilu_tsched(c->ilu, false), ms);
   if (c->usweeps >= 0)
     msweeps(c, s, k, true, c->usweeps, tmp, out, w, ms);
   else
     xs_mtrsv(s, true, n, k, c->ilu.rp, c->ilu.ci, c->ilu.lu, c->ilu.dpos, c->ilu.dinv, tmp, out,
-             c->counters + 4, c->ilu.usched, c->ilu.trsv_wg, ms);
+             c->counters + 4, xs_ilu_tsched(c->ilu, true), ms);
 }
 
 namespace {

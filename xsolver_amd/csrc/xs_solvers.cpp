@@ -434,14 +434,14 @@ static void op_pc32(xs_ctx *c, const float *x, float *y, float *tmp, const xs_st
       ilu_sweeps32(c, false, c->lsweeps, x, tmp, w, st);
     else
       xs_trsv32(c->stream, false, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type, c->ilu.dpos,
-                c->dinv32, x, tmp, c->counters + 2, c->ilu.lsched, c->ilu.trsv_wg, st);
+                c->dinv32, x, tmp, c->counters + 2, xs_ilu_tsched(c->ilu, false), st);
     xs_prof_end(c, 2);
     xs_prof_begin(c, 3);
     if (c->usweeps >= 0)
       ilu_sweeps32(c, true, c->usweeps, tmp, y, w, st);
     else
       xs_trsv32(c->stream, true, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type, c->ilu.dpos,
-                c->dinv32, tmp, y, c->counters + 4, c->ilu.usched, c->ilu.trsv_wg, st);
+                c->dinv32, tmp, y, c->counters + 4, xs_ilu_tsched(c->ilu, true), st);
     xs_prof_end(c, 3);
   } else if (c->pc == XS_PC_JACOBI) {
     xs_scale32(c->stream, n, c->jac32, x, y);
