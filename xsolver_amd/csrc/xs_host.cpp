@@ -68,7 +68,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own); dfree(F.trp); dfree(F.tci);
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
   dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
-  dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as);
+  dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);
   dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.vlo); dfree(F.tl.co);
   dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co);
   c->tell_ready = c->tell32_ready = false;
@@ -789,9 +789,22 @@ static int build_parilu_lists(xs_ctx *c) {
   if ((rc = dalloc(&F.pl, (size_t)std::max(np, 1))) || (rc = dalloc(&F.pu, (size_t)std::max(np, 1))))
     return done(rc);
   xs_parilu_pairs(c->stream, c->A, F, nullptr, F.pp, F.pl, F.pu, nullptr);
-  if (hipStreamSynchronize(c->stream) != hipSuccess) return done(XS_ERR_DEVICE);
-  if (xs_take_launch_guard()) return done(XS_ERR_STATE);
-  return done(XS_OK);
+  // active entries (a sweep can change them): flag, scan, compact
+  int *flag = nullptr, *pos = nullptr;
+  auto done2 = [&](int r) { dfree(flag); dfree(pos); return done(r); };
+  if ((rc = dalloc(&flag, nnz)) || (rc = dalloc(&pos, (size_t)nnz + 1))) return done2(rc);
+  xs_parilu_active_flags(c->stream, nnz, cnt, F.pd, flag);
+  xs_scan_excl(c->stream, nnz, flag, pos, bsum);
+  int na = 0;
+  if (hipMemcpyAsync(&na, pos + nnz, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return done2(XS_ERR_DEVICE);
+  if ((rc = dalloc(&F.act, (size_t)std::max(na, 1)))) return done2(rc);
+  xs_compact_flagged(c->stream, nnz, flag, pos, F.act);
+  F.nact = na;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return done2(XS_ERR_DEVICE);
+  if (xs_take_launch_guard()) return done2(XS_ERR_STATE);
+  return done2(XS_OK);
 }
 
 static int setup_ilu_pattern(xs_ctx *c) {
@@ -849,7 +862,6 @@ static int factor_parilu(xs_ctx *c) {
     int rc = setup_ilu_pattern(c);
     if (rc) return rc;
   }
-  xs_ilu_copy_values(c->stream, c->A, F);
   if (!c->parilu_async && !c->lu_tmp) {
     int rc = dalloc(&c->lu_tmp, c->A.nnz);
     if (rc) return rc;
@@ -861,26 +873,41 @@ static int factor_parilu(xs_ctx *c) {
       if (rc) return rc;
     }
     xs_pt_scale_vec(c->stream, c->A.n, c->A.rp, c->A.ci, c->A.v, F.scale);
-    xs_pt_rescale(c->stream, c->A.nnz, F.rowidx, c->A.ci, F.lu, F.scale, 0);
     sc = F.scale;
   }
-  const double *a0 = c->A.v;   // the values the sweeps start from
-  if (sc && F.pp) {
-    if (!F.as) {
-      int rc = dalloc(&F.as, c->A.nnz);
-      if (rc) return rc;
+  if (F.pp) {
+    // start values: the (scaled) matrix, in both buffers of a deterministic
+    // run; entries off the active list keep them
+    const double *a0 = c->A.v;
+    if (sc) {
+      if (!F.as) {
+        int rc = dalloc(&F.as, c->A.nnz);
+        if (rc) return rc;
+      }
+      xs_parilu_scaled(c->stream, c->A, F, sc, F.as);
+      a0 = F.as;
     }
-    xs_parilu_scaled(c->stream, c->A, F, sc, F.as);
-    a0 = F.as;
-  }
-  for (int s = 0; s < c->parilu_sweeps; s++) {
-    if (c->parilu_async) {
-      if (F.pp) xs_parilu_sweep_pairs(c->stream, F, a0, F.lu, F.lu);
-      else xs_parilu_sweep(c->stream, c->A, F, nullptr, true, sc);
-    } else {
-      if (F.pp) xs_parilu_sweep_pairs(c->stream, F, a0, F.lu, c->lu_tmp);
-      else xs_parilu_sweep(c->stream, c->A, F, c->lu_tmp, false, sc);
-      std::swap(F.lu, c->lu_tmp);
+    const size_t bytes = sizeof(double) * c->A.nnz;
+    XS_HIP(hipMemcpyAsync(F.lu, a0, bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (!c->parilu_async) XS_HIP(hipMemcpyAsync(c->lu_tmp, a0, bytes, hipMemcpyDeviceToDevice, c->stream));
+    for (int s = 0; s < c->parilu_sweeps; s++) {
+      if (c->parilu_async) {
+        xs_parilu_sweep_pairs(c->stream, F, a0, F.lu, F.lu);
+      } else {
+        xs_parilu_sweep_pairs(c->stream, F, a0, F.lu, c->lu_tmp);
+        std::swap(F.lu, c->lu_tmp);
+      }
+    }
+  } else {
+    xs_ilu_copy_values(c->stream, c->A, F);
+    if (sc) xs_pt_rescale(c->stream, c->A.nnz, F.rowidx, c->A.ci, F.lu, F.scale, 0);
+    for (int s = 0; s < c->parilu_sweeps; s++) {
+      if (c->parilu_async) {
+        xs_parilu_sweep(c->stream, c->A, F, nullptr, true, sc);
+      } else {
+        xs_parilu_sweep(c->stream, c->A, F, c->lu_tmp, false, sc);
+        std::swap(F.lu, c->lu_tmp);
+      }
     }
   }
   if (sc) xs_pt_rescale(c->stream, c->A.nnz, F.rowidx, c->A.ci, F.lu, F.scale, 1);
@@ -904,7 +931,8 @@ int xs_parilut_factor(xs_ctx *c) {
   // free a previous factor
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own);
   dfree(F.ucp); dfree(F.uri); dfree(F.upos);
-  dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as);   // ILU(0) lists only
+  dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);   // ILU(0) lists only
+  F.nact = 0;
   F.tell_shape = false;   // a new pattern: the sweep copies are re-shaped
   int rc;
   double *dtol = nullptr, *thr = nullptr;

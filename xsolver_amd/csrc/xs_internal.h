@@ -133,6 +133,10 @@ struct xs_ilu {
   // above.  as = the (scaled) matrix values the sweeps start from.
   int *pp = nullptr, *pl = nullptr, *pu = nullptr, *pd = nullptr;
   double *as = nullptr;
+  // entries a sweep can change (those with products, the pivots and the
+  // entries below the diagonal); the others keep their start value
+  int *act = nullptr;
+  int nact = 0;
   // SpTRSV work order: the 64-row groups of L (resp. U, counted from the
   // bottom) sorted by dependency level; ticket t solves group sched[t]
   int *lsched = nullptr, *usched = nullptr;
@@ -212,6 +216,10 @@ void xs_parilu_scaled(hipStream_t s, const xs_csr &A, const xs_ilu &F, const dou
 // one ParILU sweep over the update lists (same arithmetic as xs_parilu_sweep)
 void xs_parilu_sweep_pairs(hipStream_t s, const xs_ilu &F, const double *as, const double *lu_in,
                            double *lu_out);
+// active-entry list: flag[k] = 1 for entries a sweep can change; compaction
+// of the flagged indices at their scanned positions
+void xs_parilu_active_flags(hipStream_t s, int nnz, const int *cnt, const int *pd, int *flag);
+void xs_compact_flagged(hipStream_t s, int nnz, const int *flag, const int *pos, int *out);
 void xs_parilu_sweep(hipStream_t s, const xs_csr &A, xs_ilu &F, double *lu_out, bool async,
                      const double *sc);
 void xs_ilu_dinv(hipStream_t s, xs_ilu &F, int *d_bad);

@@ -810,14 +810,17 @@ void xs_parilu_scaled(hipStream_t s, const xs_csr &A, const xs_ilu &F, const dou
 }
 
 // One sweep over the update lists: the arithmetic of k_parilu_sweep (same
-// products in the same order, same pivot guard), without the merge.  The
-// previous value is read only when the new one is not finite.
+// products in the same order, same pivot guard), without the merge, over the
+// active entries only (an entry above the diagonal without products is its
+// start value at every sweep).  The previous value is read only when the new
+// one is not finite.
 __global__ __launch_bounds__(XS_BLOCK) void k_parilu_sweep_pairs(
-    int nnz, const double *__restrict__ as, const int *__restrict__ pp,
-    const int *__restrict__ pl, const int *__restrict__ pu, const int *__restrict__ pd,
-    const double *lu_in, double *lu_out) {
-  const int k = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
-  if (k >= nnz) return;
+    int nact, const int *__restrict__ act, const double *__restrict__ as,
+    const int *__restrict__ pp, const int *__restrict__ pl, const int *__restrict__ pu,
+    const int *__restrict__ pd, const double *lu_in, double *lu_out) {
+  const int t = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
+  if (t >= nact) return;
+  const int k = act[t];
   double s = as[k];
   const int e0 = pp[k], e1 = pp[k + 1], d = pd[k];
   for (int e = e0; e < e1; ++e) s -= lu_in[pl[e]] * lu_in[pu[e]];
@@ -835,9 +838,29 @@ __global__ __launch_bounds__(XS_BLOCK) void k_parilu_sweep_pairs(
 
 void xs_parilu_sweep_pairs(hipStream_t s, const xs_ilu &F, const double *as, const double *lu_in,
                            double *lu_out) {
-  XS_GUARD(F.pp && F.pd && F.pl && F.pu && as && lu_in && lu_out);
-  hipLaunchKernelGGL(k_parilu_sweep_pairs, dim3((F.nnz + XS_BLOCK - 1) / XS_BLOCK), dim3(XS_BLOCK),
-                     0, s, F.nnz, as, F.pp, F.pl, F.pu, F.pd, lu_in, lu_out);
+  XS_GUARD(F.pp && F.pd && F.pl && F.pu && (F.act || F.nact == 0) && as && lu_in && lu_out);
+  if (F.nact == 0) return;
+  hipLaunchKernelGGL(k_parilu_sweep_pairs, dim3((F.nact + XS_BLOCK - 1) / XS_BLOCK), dim3(XS_BLOCK),
+                     0, s, F.nact, F.act, as, F.pp, F.pl, F.pu, F.pd, lu_in, lu_out);
+}
+
+__global__ void k_parilu_active_flags(int nnz, const int *__restrict__ cnt,
+                                      const int *__restrict__ pd, int *__restrict__ flag) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nnz) flag[k] = (cnt[k] > 0 || pd[k] != -1) ? 1 : 0;
+}
+void xs_parilu_active_flags(hipStream_t s, int nnz, const int *cnt, const int *pd, int *flag) {
+  XS_GUARD(cnt && pd && flag);
+  hipLaunchKernelGGL(k_parilu_active_flags, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, cnt, pd, flag);
+}
+__global__ void k_compact_flagged(int nnz, const int *__restrict__ flag, const int *__restrict__ pos,
+                                  int *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nnz && flag[k]) out[pos[k]] = k;
+}
+void xs_compact_flagged(hipStream_t s, int nnz, const int *flag, const int *pos, int *out) {
+  XS_GUARD(flag && pos && out);
+  hipLaunchKernelGGL(k_compact_flagged, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, flag, pos, out);
 }
 
 void xs_ilu_copy_values(hipStream_t s, const xs_csr &A, xs_ilu &F) {
