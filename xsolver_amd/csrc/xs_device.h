@@ -272,8 +272,10 @@ __device__ inline void do_finish(const xs_reduce &R, const double *tot, int NV) 
 // at most one slot per thread for up to 64 x 64 blocks.  Layout: block
 // partials [gridDim.x][NV], then group partials [ngroups][NV].
 #define XS_RED_GROUP 64
-// in-wave entries a triangular-solve lane keeps in registers
+// in-wave entries a triangular-solve lane keeps in registers, and the
+// off-group entries whose values it loads together before waiting
 #define XS_INW 8
+#define XS_GATHER 4
 template <int NV>
 __device__ __forceinline__ void group_gather(const double *slots, unsigned m, const xs_reduce &R,
                                              double (&tot)[NV]) {

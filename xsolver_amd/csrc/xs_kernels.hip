@@ -908,45 +908,46 @@ void xs_ilu_dinv(hipStream_t s, xs_ilu &F, int *d_bad) {
 // (hip/Precond/gpuparilu.cpp:257-391): there one wave per row spins on a
 // separate volatile int flag; the backward solve scatters with double atomics.
 // ---------------------------------------------------------------------------
-// acc - sum of lu[e] * x[ci[e]] over up to 4 entries e, e+dir, ... (stopping
-// at `stop`), subtracted in that order: the 4 loads are issued together and
+// acc - sum of lu[e] * x[ci[e]] over up to G entries e, e+dir, ... (stopping
+// at `stop`), subtracted in that order: the G loads are issued together and
 // the values still holding the sentinel are re-polled together.
-__device__ __forceinline__ double trsv_gather4(double acc, int e, int stop, int dir,
+template <int G>
+__device__ __forceinline__ double trsv_gather(double acc, int e, int stop, int dir,
                                                const int *__restrict__ ci,
                                                const double *__restrict__ lu, const double *x,
                                                unsigned *err) {
-  unsigned long long u[4];
-  int c[4];
-  double a[4];
+  unsigned long long u[G];
+  int c[G];
+  double a[G];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < G; ++q) {
     const int eq = e + q * dir;
     const bool in = dir > 0 ? eq < stop : eq > stop;
     c[q] = in ? ci[eq] : -1;
     a[q] = in ? lu[eq] : 0.0;
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) u[q] = c[q] >= 0 ? ld_relaxed(x + c[q]) : 0ull;
+  for (int q = 0; q < G; ++q) u[q] = c[q] >= 0 ? ld_relaxed(x + c[q]) : 0ull;
   unsigned polls = 0;
   for (;;) {
     bool pend = false;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) pend |= (c[q] >= 0 && u[q] == XS_SENTINEL);
+    for (int q = 0; q < G; ++q) pend |= (c[q] >= 0 && u[q] == XS_SENTINEL);
     if (!pend) break;
     if (++polls > XS_SPIN_LIMIT) {  // bounded wait: flag it, continue with NaN
       atomicOr(err, 1u);
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < G; ++q)
         if (u[q] == XS_SENTINEL) u[q] = 0x7FF8000000000000ull;
       break;
     }
     __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < G; ++q)
       if (c[q] >= 0 && u[q] == XS_SENTINEL) u[q] = ld_relaxed(x + c[q]);
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
+  for (int q = 0; q < G; ++q)
     if (c[q] >= 0) acc -= a[q] * __longlong_as_double((long long)u[q]);
   return acc;
 }
@@ -1006,9 +1007,9 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
         if (q < ee - eb) { iw_c[q] = ci[e]; iw_v[q] = lu[e]; }
       }
       if (!UPPER) {
-        for (int e = xb; e < xe; e += 4) acc = trsv_gather4(acc, e, xe, 1, ci, lu, x, err);
+        for (int e = xb; e < xe; e += XS_GATHER) acc = trsv_gather<XS_GATHER>(acc, e, xe, 1, ci, lu, x, err);
       } else {
-        for (int e = xe - 1; e >= xb; e -= 4) acc = trsv_gather4(acc, e, xb - 1, -1, ci, lu, x, err);
+        for (int e = xe - 1; e >= xb; e -= XS_GATHER) acc = trsv_gather<XS_GATHER>(acc, e, xb - 1, -1, ci, lu, x, err);
       }
     }
     const int nint = ee - eb;

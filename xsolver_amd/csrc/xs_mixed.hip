@@ -286,19 +286,19 @@ __device__ __forceinline__ void st_ready32(float *p, float v) {
   __hip_atomic_store(reinterpret_cast<unsigned *>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// acc - sum of lu[e] * x[ci[e]] over up to 4 entries e, e+dir, ... (stopping
+// acc - sum of lu[e] * x[ci[e]] over up to G entries e, e+dir, ... (stopping
 // at `stop`), subtracted in that order: the loads are issued together and
 // the values still holding the sentinel are re-polled together.
-template <class TF>
-__device__ __forceinline__ float trsv_gather4_32(float acc, int e, int stop, int dir,
+template <int G, class TF>
+__device__ __forceinline__ float trsv_gather_32(float acc, int e, int stop, int dir,
                                                  const int *__restrict__ ci,
                                                  const TF *__restrict__ lu, const float *x,
                                                  unsigned *err) {
-  unsigned u[4];
-  int c[4];
-  float a[4];
+  unsigned u[G];
+  int c[G];
+  float a[G];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < G; ++q) {
     const int eq = e + q * dir;
     const bool in = dir > 0 ? eq < stop : eq > stop;
     c[q] = in ? ci[eq] : -1;
@@ -309,27 +309,27 @@ __device__ __forceinline__ float trsv_gather4_32(float acc, int e, int stop, int
                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   };
 #pragma unroll
-  for (int q = 0; q < 4; ++q) u[q] = c[q] >= 0 ? ld(q) : 0u;
+  for (int q = 0; q < G; ++q) u[q] = c[q] >= 0 ? ld(q) : 0u;
   unsigned polls = 0;
   for (;;) {
     bool pend = false;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) pend |= (c[q] >= 0 && u[q] == XS_SENTINEL32);
+    for (int q = 0; q < G; ++q) pend |= (c[q] >= 0 && u[q] == XS_SENTINEL32);
     if (!pend) break;
     if (++polls > XS_SPIN_LIMIT32) {  // bounded wait: flag it, continue with NaN
       atomicOr(err, 1u);
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < G; ++q)
         if (u[q] == XS_SENTINEL32) u[q] = 0x7FC00000u;
       break;
     }
     __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < G; ++q)
       if (c[q] >= 0 && u[q] == XS_SENTINEL32) u[q] = ld(q);
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
+  for (int q = 0; q < G; ++q)
     if (c[q] >= 0) acc -= a[q] * __uint_as_float(u[q]);
   return acc;
 }
@@ -389,9 +389,9 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
         if (q < ee - eb) { iw_c[q] = ci[e]; iw_v[q] = fl(lu, e); }
       }
       if (!UPPER) {
-        for (int e = xb; e < xe; e += 4) acc = trsv_gather4_32(acc, e, xe, 1, ci, lu, x, err);
+        for (int e = xb; e < xe; e += XS_GATHER) acc = trsv_gather_32<XS_GATHER>(acc, e, xe, 1, ci, lu, x, err);
       } else {
-        for (int e = xe - 1; e >= xb; e -= 4) acc = trsv_gather4_32(acc, e, xb - 1, -1, ci, lu, x, err);
+        for (int e = xe - 1; e >= xb; e -= XS_GATHER) acc = trsv_gather_32<XS_GATHER>(acc, e, xb - 1, -1, ci, lu, x, err);
       }
     }
     const int nint = ee - eb;
