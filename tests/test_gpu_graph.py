@@ -32,6 +32,12 @@ CASES = [
     ("grf2d_40_cg_ilu0", lambda: gen.vardiff2d(gen.grf_kappa(40, 0.1, 4)), "cg", "ilu0", 1e-10, 2000, {}),
     ("convdiff3d_20_mixed_bicgstab", lambda: gen.convdiff3d(20), "bicgstab", "ilu0", 1e-10, 1000,
      {"precision": 1}),
+    ("convdiff3d_20_bicgstab_sweeps_5_2", lambda: gen.convdiff3d(20), "bicgstab", "ilu0", 1e-10,
+     1000, {"lsolve_sweeps": 5, "usolve_sweeps": 2}),
+    ("convdiff3d_20_mixed_sweeps_5_2", lambda: gen.convdiff3d(20), "bicgstab", "ilu0", 1e-10,
+     1000, {"precision": 1, "lsolve_sweeps": 5, "usolve_sweeps": 2}),
+    ("lognormal27_10_gmres_ilut_sweeps", lambda: gen.lognormal27(10, seed=2), "gmres", "ilut",
+     1e-10, 500, {"lsolve_sweeps": 10, "usolve_sweeps": 15}),
 ]
 
 
