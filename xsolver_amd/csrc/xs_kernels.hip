@@ -187,12 +187,12 @@ static void spmv_vec_launch(hipStream_t s, const xs_csr &A, const xs_spmv_plan &
 }
 
 // ELL SpMV over column-major slabs v[k*ld + row]; two rows per thread with
-// 16-byte loads, the slabs taken eight at a time (all index and value loads
+// 16-byte loads, the slabs taken four at a time (all index and value loads
 // of a batch issued, then all x gathers) and the 512-row blocks placed by
 // xcd_chunk.  Padding uses the row's own index and a zero value.
 // Reference: matMultELL / bmAxELL (DeviceMatrixELL.cpp:3-34), width fixed at
 // 16 there; here the width is the true maximum row length.
-#define XS_ELL_UK 8
+#define XS_ELL_UK 4
 template <int EPI, bool O16>
 __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell(
     int n, int ld, int W, const int *__restrict__ ci, const short *__restrict__ co,

@@ -176,7 +176,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell32(
   const int r = 2 * (xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x);
   if (r < n) {
     float s0 = 0.f, s1 = 0.f;
-    constexpr int UK = 8;   // slabs per batch: all loads of a batch, then all gathers
+    constexpr int UK = 4;   // slabs per batch: all loads of a batch, then all gathers
     for (int k0 = 0; k0 < W; k0 += UK) {
       int2 c[UK];
       float2 a[UK];
