@@ -1003,9 +1003,9 @@ int xs_parilut_factor(xs_ctx *c) {
     xs_pt_symbolic(s, true, n, c->A.rp, c->A.ci, F.rp_own, F.ci_own, F.dpos, cnt, crp, cci, crow, ovf);
     xs_pt_dpos(s, n, crp, cci, cdpos, ovf);
     // 2-3. embed and sweep over the candidates
-    xs_pt_embed(s, nc, crow, cci, c->A.rp, c->A.ci, c->A.v, F.scale, F.rp_own, F.ci_own, F.lu,
+    xs_pt_embed(s, n, crp, cci, c->A.rp, c->A.ci, c->A.v, F.scale, F.rp_own, F.ci_own, F.lu,
                 F.dpos, cv);
-    xs_pt_sweep(s, nc, crow, crp, cci, cdpos, cv, cv2, c->A.rp, c->A.ci, c->A.v, F.scale);
+    xs_pt_sweep(s, n, crp, cci, cdpos, cv, cv2, c->A.rp, c->A.ci, c->A.v, F.scale);
     std::swap(cv, cv2);
     // 4. thresholds
     xs_pt_count_parts(s, nc, crow, cci, cnt2);
@@ -1030,8 +1030,7 @@ int xs_parilut_factor(xs_ctx *c) {
     xs_pt_keep(s, true, n, crp, cci, cv, thr, dtol, nullptr, F.rp_own, F.ci_own, F.lu, F.rowidx);
     xs_pt_dpos(s, n, F.rp_own, F.ci_own, F.dpos, ovf);
     // 5. sweep over the kept pattern
-    xs_pt_sweep(s, fnnz, F.rowidx, F.rp_own, F.ci_own, F.dpos, F.lu, fv2, c->A.rp, c->A.ci,
-                c->A.v, F.scale);
+    xs_pt_sweep(s, n, F.rp_own, F.ci_own, F.dpos, F.lu, fv2, c->A.rp, c->A.ci, c->A.v, F.scale);
     std::swap(F.lu, fv2);
   }
   xs_pt_rescale(s, fnnz, F.rowidx, F.ci_own, F.lu, F.scale, 1);

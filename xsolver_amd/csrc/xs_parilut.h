@@ -8,12 +8,14 @@ void xs_pt_symbolic(hipStream_t s, bool fill, int n, const int *arp, const int *
                     const int *frp, const int *fci, const int *fdpos, int *cnt, const int *crp,
                     int *cci, int *crow, int *ovf);
 void xs_pt_dpos(hipStream_t s, int n, const int *rp, const int *ci, int *dpos, int *bad);
-void xs_pt_embed(hipStream_t s, int nc, const int *crow, const int *cci, const int *arp,
+// values of the old factor F embedded into the candidate pattern C (n rows,
+// CSR crp / cci); one ParILUT sweep over a pattern (rp, ci, dpos)
+void xs_pt_embed(hipStream_t s, int n, const int *crp, const int *cci, const int *arp,
                  const int *aci, const double *av, const double *sc, const int *frp,
                  const int *fci, const double *fv, const int *fdpos, double *cv);
-void xs_pt_sweep(hipStream_t s, int nnz, const int *row, const int *rp, const int *ci,
-                 const int *dpos, const double *vin, double *vout, const int *arp,
-                 const int *aci, const double *av, const double *sc);
+void xs_pt_sweep(hipStream_t s, int n, const int *rp, const int *ci, const int *dpos,
+                 const double *vin, double *vout, const int *arp, const int *aci,
+                 const double *av, const double *sc);
 void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const double *v,
                   int part, unsigned long long k, unsigned long long *sel, unsigned *hist);
 void xs_pt_count_parts(hipStream_t s, int nnz, const int *row, const int *ci,

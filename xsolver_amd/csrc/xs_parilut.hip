@@ -63,10 +63,29 @@ __global__ __launch_bounds__(XS_WAVE *PT_WAVES) void k_pt_symbolic(
   if (i < n) {
     for (int e = arp[i] + lane; e < arp[i + 1]; e += XS_WAVE) pt_insert(tab, aci[e], ovf);
     for (int e = frp[i] + lane; e < frp[i + 1]; e += XS_WAVE) pt_insert(tab, fci[e], ovf);
-    // union of the upper parts (incl. pivot) of the rows k in L_i
-    for (int e = frp[i]; e < fdpos[i]; ++e) {
-      const int k = fci[e];
-      for (int q = fdpos[k] + lane; q < frp[k + 1]; q += XS_WAVE) pt_insert(tab, fci[q], ovf);
+    // union of the upper parts (incl. pivot) of the rows k in L_i, the
+    // first 64 entries of four rows loaded together (a set: order is free)
+    constexpr int D = 4;
+    const int le = fdpos[i];
+    for (int e0 = frp[i]; e0 < le; e0 += D) {
+      int qb[D], qe[D], jj[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const bool ok = e0 + d < le;
+        const int k = ok ? fci[e0 + d] : 0;
+        qb[d] = ok ? fdpos[k] : 0;
+        qe[d] = ok ? frp[k + 1] : 0;
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const int q = qb[d] + lane;
+        jj[d] = q < qe[d] ? fci[q] : -1;
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (jj[d] >= 0) pt_insert(tab, jj[d], ovf);
+        for (int q = qb[d] + XS_WAVE + lane; q < qe[d]; q += XS_WAVE) pt_insert(tab, fci[q], ovf);
+      }
     }
   }
   __syncthreads();
@@ -147,44 +166,130 @@ __global__ void k_pt_dpos(int n, const int *__restrict__ rp, const int *__restri
   else { dpos[i] = a; atomicOr(bad, 1); }
 }
 
-// Embed values of the old factor F into the candidate pattern C.
-__global__ __launch_bounds__(XS_BLOCK) void k_pt_embed(
-    int nc, const int *__restrict__ crow, const int *__restrict__ cci, const int *__restrict__ arp,
+// ---------------------------------------------------------------------------
+// Row-wise forms of the embed and the sweep: one wave per pattern row, the
+// row's columns in LDS, and the partial sums S_ij = sum_{k in L_i, k <
+// min(i,j)} l_ik u_kj accumulated in LDS by walking, in ascending k, the
+// strictly upper part of F's row k for every k of row i's L part -- each U
+// row read once (coalesced) per L entry instead of a binary search per
+// product.  Contributions reach S_ij in ascending k, the order of
+// pt_partial, so the results are identical.  Rows longer than PT_MAXR fall
+// back to the per-entry form.
+// ---------------------------------------------------------------------------
+#define PT_ROWW 4
+#define PT_MAXR 256
+
+__device__ __forceinline__ void pt_row_partial(int i, const int *__restrict__ frp,
+                                               const int *__restrict__ fci,
+                                               const double *__restrict__ fv,
+                                               const int *__restrict__ fdpos,
+                                               const int *cols, double *S, int len, int lane) {
+  constexpr int D = 4;   // U rows whose first 64 entries are loaded together
+  for (int t = lane; t < len; t += XS_WAVE) S[t] = 0.0;
+  const int le = fdpos[i];
+  for (int e0 = frp[i]; e0 < le; e0 += D) {
+    int qb[D], qe[D], jj[D];
+    double lk[D], uv[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const bool ok = e0 + d < le;
+      const int k = ok ? fci[e0 + d] : 0;
+      lk[d] = ok ? fv[e0 + d] : 0.0;
+      qb[d] = ok ? fdpos[k] + 1 : 0;
+      qe[d] = ok ? frp[k + 1] : 0;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int q = qb[d] + lane;
+      jj[d] = q < qe[d] ? fci[q] : -1;
+      uv[d] = q < qe[d] ? fv[q] : 0.0;
+    }
+    // updates strictly in ascending k: all of row d (first chunk from
+    // registers, the rest from memory) before row d + 1
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      for (int q0 = qb[d]; q0 < qe[d]; q0 += XS_WAVE) {
+        int j;
+        double u;
+        if (q0 == qb[d]) { j = jj[d]; u = uv[d]; }
+        else {
+          const int q = q0 + lane;
+          j = q < qe[d] ? fci[q] : -1;
+          u = q < qe[d] ? fv[q] : 0.0;
+        }
+        if (j < 0) continue;
+        int lo = 0, hi = len;
+        while (lo < hi) {
+          const int m = (lo + hi) >> 1;
+          if (cols[m] < j) lo = m + 1;
+          else hi = m;
+        }
+        if (lo < len && cols[lo] == j) S[lo] += lk[d] * u;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_sweep_rows(
+    int n, const int *__restrict__ rp, const int *__restrict__ ci, const int *__restrict__ dpos,
+    const double *__restrict__ vin, double *__restrict__ vout, const int *__restrict__ arp,
+    const int *__restrict__ aci, const double *__restrict__ av, const double *__restrict__ sc) {
+  __shared__ int cols_s[PT_ROWW][PT_MAXR];
+  __shared__ double S_s[PT_ROWW][PT_MAXR];
+  const int w = threadIdx.x / XS_WAVE, lane = threadIdx.x & (XS_WAVE - 1);
+  const int i = blockIdx.x * PT_ROWW + w;
+  if (i >= n) return;   // wave-uniform, no workgroup barrier below
+  const int a = rp[i], len = rp[i + 1] - a;
+  int *cols = cols_s[w];
+  double *S = S_s[w];
+  const bool rows = len <= PT_MAXR;
+  if (rows) {
+    for (int t = lane; t < len; t += XS_WAVE) cols[t] = ci[a + t];
+    pt_row_partial(i, rp, ci, vin, dpos, cols, S, len, lane);
+  }
+  for (int t = lane; t < len; t += XS_WAVE) {
+    const int e = a + t, j = ci[e];
+    const double part = rows ? S[t] : pt_partial(rp, ci, vin, dpos, i, j);
+    const double s = pt_a(arp, aci, av, sc, i, j) - part;
+    double r;
+    if (i > j) r = s / pt_clamp(vin[dpos[j]]);
+    else r = (i == j) ? pt_clamp(s) : s;
+    vout[e] = isfinite(r) ? r : vin[e];
+  }
+}
+
+__global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_embed_rows(
+    int n, const int *__restrict__ crp, const int *__restrict__ cci, const int *__restrict__ arp,
     const int *__restrict__ aci, const double *__restrict__ av, const double *__restrict__ sc,
     const int *__restrict__ frp, const int *__restrict__ fci, const double *__restrict__ fv,
     const int *__restrict__ fdpos, double *__restrict__ cv) {
-  const int e = blockIdx.x * XS_BLOCK + threadIdx.x;
-  if (e >= nc) return;
-  const int i = crow[e], j = cci[e];
-  const int a = frp[i], b = frp[i + 1];
-  const int p = pt_find(fci, a, b, j);
-  double val;
-  if (p < b && fci[p] == j) {
-    val = fv[p];
-  } else {
-    const double r = pt_a(arp, aci, av, sc, i, j) - pt_partial(frp, fci, fv, fdpos, i, j);
-    val = (i > j) ? r / pt_clamp(fv[fdpos[j]]) : r;
+  __shared__ int cols_s[PT_ROWW][PT_MAXR];
+  __shared__ double S_s[PT_ROWW][PT_MAXR];
+  const int w = threadIdx.x / XS_WAVE, lane = threadIdx.x & (XS_WAVE - 1);
+  const int i = blockIdx.x * PT_ROWW + w;
+  if (i >= n) return;
+  const int a = crp[i], len = crp[i + 1] - a;
+  int *cols = cols_s[w];
+  double *S = S_s[w];
+  const bool rows = len <= PT_MAXR;
+  if (rows) {
+    for (int t = lane; t < len; t += XS_WAVE) cols[t] = cci[a + t];
+    pt_row_partial(i, frp, fci, fv, fdpos, cols, S, len, lane);
   }
-  cv[e] = isfinite(val) ? val : 0.0;
-}
-
-// One Jacobi sweep over the pattern (rp, ci) with values vin -> vout.
-__global__ __launch_bounds__(XS_BLOCK) void k_pt_sweep(
-    int nnz, const int *__restrict__ row, const int *__restrict__ rp, const int *__restrict__ ci,
-    const int *__restrict__ dpos, const double *__restrict__ vin, double *__restrict__ vout,
-    const int *__restrict__ arp, const int *__restrict__ aci, const double *__restrict__ av,
-    const double *__restrict__ sc) {
-  const int e = blockIdx.x * XS_BLOCK + threadIdx.x;
-  if (e >= nnz) return;
-  const int i = row[e], j = ci[e];
-  const double s = pt_a(arp, aci, av, sc, i, j) - pt_partial(rp, ci, vin, dpos, i, j);
-  double t;
-  if (i > j) {
-    t = s / pt_clamp(vin[dpos[j]]);
-  } else {
-    t = (i == j) ? pt_clamp(s) : s;
+  const int fa = frp[i], fb = frp[i + 1];
+  for (int t = lane; t < len; t += XS_WAVE) {
+    const int j = cci[a + t];
+    const int p = pt_find(fci, fa, fb, j);
+    double val;
+    if (p < fb && fci[p] == j) {
+      val = fv[p];
+    } else {
+      const double part = rows ? S[t] : pt_partial(frp, fci, fv, fdpos, i, j);
+      const double r = pt_a(arp, aci, av, sc, i, j) - part;
+      val = (i > j) ? r / pt_clamp(fv[fdpos[j]]) : r;
+    }
+    cv[a + t] = isfinite(val) ? val : 0.0;
   }
-  vout[e] = isfinite(t) ? t : vin[e];
 }
 
 // ---------------------------------------------------------------------------
@@ -418,18 +523,18 @@ void xs_pt_dpos(hipStream_t s, int n, const int *rp, const int *ci, int *dpos, i
   hipLaunchKernelGGL(k_pt_dpos, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, dpos, bad);
 }
 
-void xs_pt_embed(hipStream_t s, int nc, const int *crow, const int *cci, const int *arp,
+void xs_pt_embed(hipStream_t s, int n, const int *crp, const int *cci, const int *arp,
                  const int *aci, const double *av, const double *sc, const int *frp,
                  const int *fci, const double *fv, const int *fdpos, double *cv) {
-  hipLaunchKernelGGL(k_pt_embed, dim3((nc + XS_BLOCK - 1) / XS_BLOCK), dim3(XS_BLOCK), 0, s, nc,
-                     crow, cci, arp, aci, av, sc, frp, fci, fv, fdpos, cv);
+  hipLaunchKernelGGL(k_pt_embed_rows, dim3((n + PT_ROWW - 1) / PT_ROWW), dim3(XS_WAVE * PT_ROWW), 0,
+                     s, n, crp, cci, arp, aci, av, sc, frp, fci, fv, fdpos, cv);
 }
 
-void xs_pt_sweep(hipStream_t s, int nnz, const int *row, const int *rp, const int *ci,
-                 const int *dpos, const double *vin, double *vout, const int *arp,
-                 const int *aci, const double *av, const double *sc) {
-  hipLaunchKernelGGL(k_pt_sweep, dim3((nnz + XS_BLOCK - 1) / XS_BLOCK), dim3(XS_BLOCK), 0, s, nnz,
-                     row, rp, ci, dpos, vin, vout, arp, aci, av, sc);
+void xs_pt_sweep(hipStream_t s, int n, const int *rp, const int *ci, const int *dpos,
+                 const double *vin, double *vout, const int *arp, const int *aci,
+                 const double *av, const double *sc) {
+  hipLaunchKernelGGL(k_pt_sweep_rows, dim3((n + PT_ROWW - 1) / PT_ROWW), dim3(XS_WAVE * PT_ROWW), 0,
+                     s, n, rp, ci, dpos, vin, vout, arp, aci, av, sc);
 }
 
 void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const double *v,
