@@ -771,6 +771,23 @@ static int build_trsv_schedules(xs_ctx *c, const int *rp, const int *ci, const i
   return XS_OK;
 }
 
+// The groups of a pattern built on the device (ParILUT), on the first solve
+// that applies a factor by the exact triangular solve (sweep counts < 0);
+// until then the solves would fall back to fixed 64-row groups in row order.
+static int ensure_trsv_sched(xs_ctx *c) {
+  xs_ilu &F = c->ilu;
+  if (!F.sched_pending || (c->lsweeps >= 0 && c->usweeps >= 0)) return XS_OK;
+  const int n = F.n, nnz = F.nnz;
+  std::vector<int> hrp(n + 1), hci(nnz), hdp(n);
+  XS_HIP(hipStreamSynchronize(c->stream));
+  XS_HIP(hipMemcpy(hrp.data(), F.rp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  XS_HIP(hipMemcpy(hci.data(), F.ci, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+  XS_HIP(hipMemcpy(hdp.data(), F.dpos, sizeof(int) * n, hipMemcpyDeviceToHost));
+  const int rc = build_trsv_schedules(c, hrp.data(), hci.data(), hdp.data());
+  if (rc == XS_OK) F.sched_pending = false;
+  return rc;
+}
+
 // Update lists of the ParILU(0) sweeps (once per pattern, on the device):
 // count the products of every entry, scan, fill.
 static int build_parilu_lists(xs_ctx *c) {
@@ -932,18 +949,22 @@ int xs_parilut_factor(xs_ctx *c) {
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own);
   dfree(F.ucp); dfree(F.uri); dfree(F.upos);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);   // ILU(0) lists only
+  dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
+  F.lgroups = F.ugroups = 0;
+  F.sched_pending = false;
   F.nact = 0;
   F.tell_shape = false;   // a new pattern: the sweep copies are re-shaped
   int rc;
   double *dtol = nullptr, *thr = nullptr;
   int *cnt = nullptr, *bsum = nullptr, *ovf = nullptr;
-  unsigned long long *sel = nullptr, *cnt2 = nullptr;
-  unsigned *hist = nullptr;
+  unsigned long long *sel = nullptr, *cnt2 = nullptr, *list = nullptr;
+  unsigned *hist = nullptr, *lcnt = nullptr;
   int *crp = nullptr, *cci = nullptr, *crow = nullptr, *cdpos = nullptr;
   double *cv = nullptr, *cv2 = nullptr, *fv2 = nullptr;
   auto cleanup = [&]() {
     dfree(dtol); dfree(thr); dfree(cnt); dfree(bsum); dfree(ovf); dfree(sel); dfree(cnt2);
-    dfree(hist); dfree(crp); dfree(cci); dfree(crow); dfree(cdpos); dfree(cv); dfree(cv2); dfree(fv2);
+    dfree(list); dfree(hist); dfree(lcnt); dfree(crp); dfree(cci); dfree(crow); dfree(cdpos);
+    dfree(cv); dfree(cv2); dfree(fv2);
   };
 #define PT_TRY(x) do { if ((rc = (x))) { cleanup(); return rc; } } while (0)
   PT_TRY(F.scale ? XS_OK : dalloc(&F.scale, n));
@@ -952,9 +973,11 @@ int xs_parilut_factor(xs_ctx *c) {
   PT_TRY(dalloc(&cnt, n));
   PT_TRY(dalloc(&bsum, xs_scan_bsum_len(n)));
   PT_TRY(dalloc(&ovf, 1));
-  PT_TRY(dalloc(&sel, 3));
+  PT_TRY(dalloc(&sel, xs_pt_select_words()));
   PT_TRY(dalloc(&cnt2, 2));
-  PT_TRY(dalloc(&hist, 256));
+  PT_TRY(dalloc(&list, xs_pt_list_len()));
+  PT_TRY(dalloc(&hist, 512));
+  PT_TRY(dalloc(&lcnt, 2));
   PT_TRY(dalloc(&crp, n + 1));
   PT_TRY(dalloc(&cdpos, n));
   PT_TRY(dalloc(&F.rp_own, n + 1));
@@ -963,6 +986,17 @@ int xs_parilut_factor(xs_ctx *c) {
   PT_TRY(dalloc(&F.rowidx, nnz));
   PT_TRY(dalloc(&F.dpos, n));
   PT_TRY(dalloc(&F.dinv, n));
+  // the candidate and factor buffers grow only (by a quarter beyond the
+  // need), so most steps reuse the previous step's allocations
+  size_t cap_c = 0, cap_ci = (size_t)nnz, cap_lu = (size_t)nnz, cap_row = (size_t)nnz, cap_v2 = 0;
+  auto grow = [](size_t need, size_t &cap, auto **pp) -> int {
+    if (need <= cap) return XS_OK;
+    dfree(*pp);
+    const size_t want = need + need / 4;
+    const int r = dalloc(pp, want);
+    cap = r == XS_OK ? want : 0;
+    return r;
+  };
   XS_HIP(hipMemsetAsync(ovf, 0, sizeof(int), s));
   // initial factor = A (scaled): L = strict lower part, U = upper part
   XS_HIP(hipMemcpyAsync(F.rp_own, c->A.rp, sizeof(int) * (n + 1), hipMemcpyDeviceToDevice, s));
@@ -973,12 +1007,12 @@ int xs_parilut_factor(xs_ctx *c) {
   xs_pt_rescale(s, nnz, F.rowidx, F.ci_own, F.lu, F.scale, 0);
   xs_pt_dtol(s, n, c->A.rp, c->A.ci, c->A.v, F.scale, c->ilut_tol, dtol);
   xs_pt_dpos(s, n, F.rp_own, F.ci_own, F.dpos, ovf);
-  long nl0 = 0, nu0 = 0;
-  for (int i = 0; i < n; i++)
-    for (int e = c->h_rp[i]; e < c->h_rp[i + 1]; e++) {
-      if (c->h_ci[e] < i) nl0++;
-      else if (c->h_ci[e] > i) nu0++;
-    }
+  // entries of A's strict parts: the base of the keep counts
+  unsigned long long base[2] = {0, 0};
+  xs_pt_count_parts(s, nnz, F.rowidx, F.ci_own, cnt2);
+  XS_HIP(hipMemcpyAsync(base, cnt2, sizeof(base), hipMemcpyDeviceToHost, s));
+  XS_HIP(hipStreamSynchronize(s));
+  const long nl0 = (long)base[0], nu0 = (long)base[1];
   int fnnz = nnz;
   const int steps = c->parilut_steps;
   const double fill = (double)c->ilut_fill;
@@ -995,11 +1029,14 @@ int xs_parilut_factor(xs_ctx *c) {
     XS_HIP(hipStreamSynchronize(s));
     if (hdr[1]) { snprintf(xs_errbuf, sizeof(xs_errbuf), "ParILUT: pattern overflow"); cleanup(); return XS_ERR_MEMORY; }
     const int nc = hdr[0];
-    dfree(cci); dfree(crow); dfree(cv); dfree(cv2);
-    PT_TRY(dalloc(&cci, nc));
-    PT_TRY(dalloc(&crow, nc));
-    PT_TRY(dalloc(&cv, nc));
-    PT_TRY(dalloc(&cv2, nc));
+    {
+      size_t c1 = cap_c, c2 = cap_c, c3 = cap_c, c4 = cap_c;
+      PT_TRY(grow((size_t)nc, c1, &cci));
+      PT_TRY(grow((size_t)nc, c2, &crow));
+      PT_TRY(grow((size_t)nc, c3, &cv));
+      PT_TRY(grow((size_t)nc, c4, &cv2));
+      cap_c = c1;
+    }
     xs_pt_symbolic(s, true, n, c->A.rp, c->A.ci, F.rp_own, F.ci_own, F.dpos, cnt, crp, cci, crow, ovf);
     xs_pt_dpos(s, n, crp, cci, cdpos, ovf);
     // 2-3. embed and sweep over the candidates
@@ -1007,31 +1044,25 @@ int xs_parilut_factor(xs_ctx *c) {
                 F.dpos, cv);
     xs_pt_sweep(s, n, crp, cci, cdpos, cv, cv2, c->A.rp, c->A.ci, c->A.v, F.scale);
     std::swap(cv, cv2);
-    // 4. thresholds
+    // 4. thresholds (the part counts stay on the device)
     xs_pt_count_parts(s, nc, crow, cci, cnt2);
-    unsigned long long parts[2] = {0, 0};
-    XS_HIP(hipMemcpyAsync(parts, cnt2, sizeof(parts), hipMemcpyDeviceToHost, s));
-    XS_HIP(hipStreamSynchronize(s));
-    const bool dropL = (long long)parts[0] > keepL, dropU = (long long)parts[1] > keepU;
-    if (dropL) xs_pt_select(s, nc, crow, cci, cv, 0, parts[0] - (unsigned long long)keepL, sel, hist);
-    xs_pt_thr(s, sel, dropL, thr);
-    if (dropU) xs_pt_select(s, nc, crow, cci, cv, 2, parts[1] - (unsigned long long)keepU, sel, hist);
-    xs_pt_thr(s, sel, dropU, thr + 1);
+    xs_pt_select(s, nc, crow, cci, cv, cnt2, keepL, keepU, sel, hist, lcnt, list, thr);
     // compaction into the new factor
     xs_pt_keep(s, false, n, crp, cci, cv, thr, dtol, cnt, nullptr, nullptr, nullptr, nullptr);
     xs_scan_excl(s, n, cnt, F.rp_own, bsum);
     XS_HIP(hipMemcpyAsync(&fnnz, F.rp_own + n, sizeof(int), hipMemcpyDeviceToHost, s));
     XS_HIP(hipStreamSynchronize(s));
-    dfree(F.ci_own); dfree(F.lu); dfree(F.rowidx); dfree(fv2);
-    PT_TRY(dalloc(&F.ci_own, fnnz));
-    PT_TRY(dalloc(&F.lu, fnnz));
-    PT_TRY(dalloc(&F.rowidx, fnnz));
-    PT_TRY(dalloc(&fv2, fnnz));
+    // the old factor was last read by the embed (the stream is idle here)
+    PT_TRY(grow((size_t)fnnz, cap_ci, &F.ci_own));
+    PT_TRY(grow((size_t)fnnz, cap_lu, &F.lu));
+    PT_TRY(grow((size_t)fnnz, cap_row, &F.rowidx));
+    PT_TRY(grow((size_t)fnnz, cap_v2, &fv2));
     xs_pt_keep(s, true, n, crp, cci, cv, thr, dtol, nullptr, F.rp_own, F.ci_own, F.lu, F.rowidx);
     xs_pt_dpos(s, n, F.rp_own, F.ci_own, F.dpos, ovf);
     // 5. sweep over the kept pattern
     xs_pt_sweep(s, n, F.rp_own, F.ci_own, F.dpos, F.lu, fv2, c->A.rp, c->A.ci, c->A.v, F.scale);
     std::swap(F.lu, fv2);
+    std::swap(cap_lu, cap_v2);
   }
   xs_pt_rescale(s, fnnz, F.rowidx, F.ci_own, F.lu, F.scale, 1);
   F.n = n;
@@ -1048,12 +1079,9 @@ int xs_parilut_factor(xs_ctx *c) {
 #undef PT_TRY
   if (ov) return XS_ERR_MEMORY;
   if (bad != 0x7f7f7f7f) return XS_ERR_PIVOT;
-  // SpTRSV order of the new pattern
-  std::vector<int> hrp(n + 1), hci(fnnz), hdp(n);
-  XS_HIP(hipMemcpy(hrp.data(), F.rp_own, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
-  XS_HIP(hipMemcpy(hci.data(), F.ci_own, sizeof(int) * fnnz, hipMemcpyDeviceToHost));
-  XS_HIP(hipMemcpy(hdp.data(), F.dpos, sizeof(int) * n, hipMemcpyDeviceToHost));
-  return build_trsv_schedules(c, hrp.data(), hci.data(), hdp.data());
+  // row groups of the SpTRSV: built from the new pattern on the first exact solve
+  F.sched_pending = true;
+  return XS_OK;
 }
 
 extern "C" int xs_pc_setup(xs_ctx *c) {
@@ -1299,7 +1327,7 @@ static int prepare_solve(xs_ctx *c) {
   if ((c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) && (c->lsweeps >= 2 || c->usweeps >= 2) &&
       (rc = ensure_tri_ell(c)))
     return rc;
-  return XS_OK;
+  return ensure_trsv_sched(c);
 }
 
 static int run_solve(xs_ctx *c, const double *d_b, double *d_x) {
@@ -1614,6 +1642,7 @@ extern "C" int xs_pc_apply(xs_ctx *c, const double *x, double *y, void *stream) 
   if ((c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) && (c->lsweeps >= 2 || c->usweeps >= 2) &&
       (rc = ensure_tri_ell(c)))
     return rc;
+  if ((rc = ensure_trsv_sched(c))) return rc;
   hipStream_t keep = c->stream;
   c->stream = pick(c, stream);
   xs_op_pc(c, x, y, xs_ctx_vec(c, XS_NWORK - 1), nullptr);
@@ -1656,6 +1685,7 @@ extern "C" int xs_pc_apply_multi(xs_ctx *c, int nrhs, const double *x, double *y
     case XS_PC_ILUT: {
       if ((rc = xs_multi_reserve(c, nrhs, 1))) return rc;
       if ((c->lsweeps >= 2 || c->usweeps >= 2) && (rc = ensure_tri_ell(c))) return rc;
+      if ((rc = ensure_trsv_sched(c))) return rc;
       xs_multi_ilu_apply(c, s, nrhs, x, y, xs_multi_vec(c, nrhs, 8), xs_multi_vec(c, nrhs, 9),
                          nullptr);
       break;

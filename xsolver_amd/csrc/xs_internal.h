@@ -144,6 +144,9 @@ struct xs_ilu {
   // on its neighbour in solve order) and their counts
   int *lgb = nullptr, *ugb = nullptr;
   int lgroups = 0, ugroups = 0;
+  // the factor pattern changed on the device (ParILUT) and the groups and
+  // their order are built from it on the first exact triangular solve
+  bool sched_pending = false;
   int trsv_wg = 256;        // persistent workgroups of the triangular solves
   xs_tell tl, tu;           // sweep copies of L and U (built on demand)
   bool tell_shape = false;  // tl/tu widths and buffers match the pattern

@@ -16,11 +16,19 @@ void xs_pt_embed(hipStream_t s, int n, const int *crp, const int *cci, const int
 void xs_pt_sweep(hipStream_t s, int n, const int *rp, const int *ci, const int *dpos,
                  const double *vin, double *vout, const int *arp, const int *aci,
                  const double *av, const double *sc);
+// thresholds of both strict parts: thr[0] (resp. thr[1]) = the value of
+// rank cnt2[0] - keepL (resp. cnt2[1] - keepU) in ascending |v| of the
+// strict lower (upper) entries, 0 when the part has no more than keep
+// entries; sel: xs_pt_select_words() words, hist: 512, lcnt: 2, list:
+// xs_pt_list_len()
 void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const double *v,
-                  int part, unsigned long long k, unsigned long long *sel, unsigned *hist);
+                  const unsigned long long *cnt2, long long keepL, long long keepU,
+                  unsigned long long *sel, unsigned *hist, unsigned *lcnt,
+                  unsigned long long *list, double *thr);
+int xs_pt_select_words();
+int xs_pt_list_len();
 void xs_pt_count_parts(hipStream_t s, int nnz, const int *row, const int *ci,
                        unsigned long long *cnt2);
-void xs_pt_thr(hipStream_t s, const unsigned long long *sel, int use, double *thr_slot);
 void xs_pt_dtol(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
                 const double *sc, double droptol, double *dtol);
 void xs_pt_keep(hipStream_t s, bool write, int n, const int *crp, const int *cci,

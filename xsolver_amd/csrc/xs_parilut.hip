@@ -293,65 +293,140 @@ __global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_embed_rows(
 }
 
 // ---------------------------------------------------------------------------
-// Radix select: value of rank k (0-based) in ascending |v| over the entries
-// of part `part` (0: col < row, 2: col > row).  |v| bit patterns are ordered
-// like the values; 8 passes of 8-bit digits from the most significant.
-// sel[0] = prefix bits, sel[1] = remaining rank, sel[2] = mask of fixed bits.
+// Radix select of both thresholds at once: for the strict lower part (slot
+// 0, col < row) and the strict upper part (slot 1, col > row), the value of
+// rank k (0-based) in ascending |v|.  |v| bit patterns are ordered like the
+// values; passes of 8-bit digits from the most significant.  Once the bin of
+// rank k holds at most PT_SEL_CAP entries, the next pass collects them and
+// the rank is found among them in one block; later passes return at once.
+// State per slot p: sel[4p + 0] prefix bits, + 1 remaining rank, + 2 mask of
+// fixed bits, + 3 mode (0 histogram, 1 collect, 2 done).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int pt_part(int i, int j) { return j < i ? 0 : (j == i ? 1 : 2); }
+#define PT_SEL_CAP 1024
+static_assert(XS_BLOCK == 256, "k_pt_hist clears its 512 bins with two stores per thread");
+
+__device__ __forceinline__ int pt_slot(int i, int j) { return j < i ? 0 : (j > i ? 1 : -1); }
+
+__global__ void k_pt_sel_init(const unsigned long long *__restrict__ cnt2, long long keepL,
+                              long long keepU, unsigned long long *sel, unsigned *hist,
+                              unsigned *lcnt, double *thr) {
+  for (int q = threadIdx.x; q < 512; q += blockDim.x) hist[q] = 0;
+  if (threadIdx.x < 2) {
+    const int p = threadIdx.x;
+    const long long tot = (long long)cnt2[p], keep = p ? keepU : keepL;
+    sel[4 * p + 0] = 0;
+    sel[4 * p + 1] = tot > keep ? (unsigned long long)(tot - keep) : 0ull;
+    sel[4 * p + 2] = 0;
+    sel[4 * p + 3] = tot > keep ? 0ull : 2ull;
+    lcnt[p] = 0;
+    thr[p] = 0.0;   // nothing dropped unless the select finishes
+  }
+}
 
 __global__ __launch_bounds__(XS_BLOCK) void k_pt_hist(int nnz, const int *__restrict__ row,
                                                       const int *__restrict__ ci,
-                                                      const double *__restrict__ v, int part,
-                                                      int shift, const unsigned long long *sel,
-                                                      unsigned *__restrict__ hist) {
-  __shared__ unsigned h[256];
+                                                      const double *__restrict__ v, int shift,
+                                                      const unsigned long long *sel,
+                                                      unsigned *__restrict__ hist,
+                                                      unsigned *lcnt,
+                                                      unsigned long long *__restrict__ list) {
+  const unsigned long long m0 = sel[3], m1 = sel[7];
+  if (m0 == 2 && m1 == 2) return;
+  __shared__ unsigned h[512];
   h[threadIdx.x] = 0;
+  h[threadIdx.x + XS_BLOCK] = 0;
   __syncthreads();
-  const unsigned long long prefix = sel[0], mask = sel[2];
+  const unsigned long long pre[2] = {sel[0], sel[4]}, msk[2] = {sel[2], sel[6]};
+  const unsigned long long mode[2] = {m0, m1};
   for (int e = blockIdx.x * XS_BLOCK + threadIdx.x; e < nnz; e += gridDim.x * XS_BLOCK) {
-    if (pt_part(row[e], ci[e]) != part) continue;
+    const int p = pt_slot(row[e], ci[e]);
+    if (p < 0 || mode[p] == 2) continue;
     const unsigned long long bits =
         (unsigned long long)__double_as_longlong(fabs(v[e])) & 0x7FFFFFFFFFFFFFFFull;
-    if ((bits & mask) != prefix) continue;
-    atomicAdd(&h[(bits >> shift) & 0xFF], 1u);
+    if ((bits & msk[p]) != pre[p]) continue;
+    if (mode[p] == 1) {
+      const unsigned q = atomicAdd(&lcnt[p], 1u);
+      if (q < PT_SEL_CAP) list[p * PT_SEL_CAP + q] = bits;
+    } else {
+      atomicAdd(&h[p * 256 + ((bits >> shift) & 0xFF)], 1u);
+    }
   }
   __syncthreads();
-  if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+  for (int q = threadIdx.x; q < 512; q += XS_BLOCK)
+    if (h[q]) atomicAdd(&hist[q], h[q]);
 }
 
-__global__ void k_pt_pick(int shift, unsigned long long *sel, unsigned *hist) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  unsigned long long k = sel[1];
-  unsigned long long cum = 0;
-  int d = 0;
-  for (; d < 256; ++d) {
-    if (cum + hist[d] > k) break;
-    cum += hist[d];
+// one block of 256 threads per slot: pick the digit of rank k (or the rank
+// among the collected entries) and advance the state
+__global__ __launch_bounds__(256) void k_pt_pick(int shift, unsigned long long *sel,
+                                                 unsigned *hist, unsigned *lcnt,
+                                                 const unsigned long long *__restrict__ list,
+                                                 double *thr) {
+  const int p = blockIdx.x, t = threadIdx.x;
+  unsigned long long *st = sel + 4 * p;
+  const unsigned long long mode = st[3];
+  if (mode == 2) return;
+  __shared__ unsigned long long vals[PT_SEL_CAP];
+  __shared__ unsigned cum[256];
+  const unsigned long long k = st[1];
+  __syncthreads();
+  if (mode == 1) {
+    const int c = (int)min(lcnt[p], (unsigned)PT_SEL_CAP);
+    for (int q = t; q < c; q += 256) vals[q] = list[p * PT_SEL_CAP + q];
+    __syncthreads();
+    for (int q = t; q < c; q += 256) {
+      const unsigned long long x = vals[q];
+      unsigned long long less = 0, eq = 0;
+      for (int r = 0; r < c; ++r) {
+        less += vals[r] < x;
+        eq += vals[r] == x;
+      }
+      // every thread that holds the value of rank k writes that same value
+      if (less <= k && k < less + eq) st[0] = x;
+    }
+    __syncthreads();
+    if (t == 0) {
+      st[2] = ~0ull;
+      st[3] = 2;
+      thr[p] = __longlong_as_double((long long)st[0]);
+    }
+    return;
   }
-  if (d == 256) d = 255;
-  sel[0] |= ((unsigned long long)d) << shift;
-  sel[1] = k - cum;
-  sel[2] |= 0xFFull << shift;
-  for (int q = 0; q < 256; ++q) hist[q] = 0;
-}
-
-__global__ void k_pt_sel_init(unsigned long long *sel, unsigned long long k, unsigned *hist) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    sel[0] = 0; sel[1] = k; sel[2] = 0;
+  // inclusive scan of the 256 bin counts
+  const unsigned hc = hist[p * 256 + t];
+  cum[t] = hc;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const unsigned add = t >= off ? cum[t - off] : 0u;
+    __syncthreads();
+    cum[t] += add;
+    __syncthreads();
   }
-  for (int q = threadIdx.x; q < 256; q += blockDim.x) hist[q] = 0;
+  const unsigned long long below = cum[t] - hc;
+  if (hc && below <= k && k < below + hc) {
+    st[0] |= (unsigned long long)t << shift;
+    st[1] = k - below;
+    st[2] |= 0xFFull << shift;
+    if (shift == 0) {
+      st[3] = 2;
+      thr[p] = __longlong_as_double((long long)st[0]);
+    } else if (hc <= PT_SEL_CAP) {
+      st[3] = 1;
+      lcnt[p] = 0;
+    }
+  }
+  hist[p * 256 + t] = 0;
 }
 
-// count of entries per part (0, 2) -> cnt2[0], cnt2[1]
+// counts of the strict lower and strict upper entries -> cnt2[0], cnt2[1]
 __global__ __launch_bounds__(XS_BLOCK) void k_pt_count_parts(int nnz, const int *__restrict__ row,
                                                              const int *__restrict__ ci,
                                                              unsigned long long *cnt2) {
   unsigned long long a = 0, b = 0;
   for (int e = blockIdx.x * XS_BLOCK + threadIdx.x; e < nnz; e += gridDim.x * XS_BLOCK) {
-    const int p = pt_part(row[e], ci[e]);
+    const int p = pt_slot(row[e], ci[e]);
     a += p == 0;
-    b += p == 2;
+    b += p == 1;
   }
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_xor(a, off, XS_WAVE);
@@ -372,34 +447,52 @@ __device__ __forceinline__ bool pt_keep(int i, int j, double v, double tl, doubl
   return j < i ? a >= tl : a >= tu;
 }
 
-// per-row kept counts: one thread per row
-__global__ void k_pt_keep_count(int n, const int *__restrict__ crp, const int *__restrict__ cci,
-                                const double *__restrict__ cv, const double *thr,
-                                const double *__restrict__ dtol, int *__restrict__ cnt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// per-row kept counts and the compaction: a wave per row, kept entries
+// placed in column order by ballot
+__global__ __launch_bounds__(XS_BLOCK) void k_pt_keep_count(
+    int n, const int *__restrict__ crp, const int *__restrict__ cci,
+    const double *__restrict__ cv, const double *thr, const double *__restrict__ dtol,
+    int *__restrict__ cnt) {
+  const int lane = threadIdx.x & (XS_WAVE - 1);
+  const int i = blockIdx.x * (XS_BLOCK / XS_WAVE) + threadIdx.x / XS_WAVE;
   if (i >= n) return;
   const double tl = thr[0], tu = thr[1], dt = dtol[i];
   int c = 0;
-  for (int e = crp[i]; e < crp[i + 1]; ++e) c += pt_keep(i, cci[e], cv[e], tl, tu, dt);
-  cnt[i] = c;
+  for (int e = crp[i] + lane; e < crp[i + 1]; e += XS_WAVE) c += pt_keep(i, cci[e], cv[e], tl, tu, dt);
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, XS_WAVE);
+  if (lane == 0) cnt[i] = c;
 }
 
-__global__ void k_pt_keep_write(int n, const int *__restrict__ crp, const int *__restrict__ cci,
-                                const double *__restrict__ cv, const double *thr,
-                                const double *__restrict__ dtol, const int *__restrict__ frp,
-                                int *__restrict__ fci, double *__restrict__ fv,
-                                int *__restrict__ frow) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(XS_BLOCK) void k_pt_keep_write(
+    int n, const int *__restrict__ crp, const int *__restrict__ cci,
+    const double *__restrict__ cv, const double *thr, const double *__restrict__ dtol,
+    const int *__restrict__ frp, int *__restrict__ fci, double *__restrict__ fv,
+    int *__restrict__ frow) {
+  const int lane = threadIdx.x & (XS_WAVE - 1);
+  const int i = blockIdx.x * (XS_BLOCK / XS_WAVE) + threadIdx.x / XS_WAVE;
   if (i >= n) return;
   const double tl = thr[0], tu = thr[1], dt = dtol[i];
+  const unsigned long long below = (1ull << lane) - 1ull;
   int w = frp[i];
-  for (int e = crp[i]; e < crp[i + 1]; ++e) {
-    if (pt_keep(i, cci[e], cv[e], tl, tu, dt)) {
-      fci[w] = cci[e];
-      fv[w] = cv[e];
-      frow[w] = i;
-      ++w;
+  const int re = crp[i + 1];
+  for (int e0 = crp[i]; e0 < re; e0 += XS_WAVE) {
+    const int e = e0 + lane;
+    int j = 0;
+    double x = 0.0;
+    bool keep = false;
+    if (e < re) {
+      j = cci[e];
+      x = cv[e];
+      keep = pt_keep(i, j, x, tl, tu, dt);
     }
+    const unsigned long long m = __ballot(keep);
+    if (keep) {
+      const int q = w + __popcll(m & below);
+      fci[q] = j;
+      fv[q] = x;
+      frow[q] = i;
+    }
+    w += __popcll(m);
   }
 }
 
@@ -538,16 +631,23 @@ void xs_pt_sweep(hipStream_t s, int n, const int *rp, const int *ci, const int *
 }
 
 void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const double *v,
-                  int part, unsigned long long k, unsigned long long *sel, unsigned *hist) {
-  hipLaunchKernelGGL(k_pt_sel_init, dim3(1), dim3(256), 0, s, sel, k, hist);
-  int g = (nnz + XS_BLOCK * 8 - 1) / (XS_BLOCK * 8);
-  if (g > 4096) g = 4096;
+                  const unsigned long long *cnt2, long long keepL, long long keepU,
+                  unsigned long long *sel, unsigned *hist, unsigned *lcnt,
+                  unsigned long long *list, double *thr) {
+  hipLaunchKernelGGL(k_pt_sel_init, dim3(1), dim3(256), 0, s, cnt2, keepL, keepU, sel, hist, lcnt,
+                     thr);
+  int g = (nnz + XS_BLOCK * 16 - 1) / (XS_BLOCK * 16);
+  if (g > 2048) g = 2048;
   if (g < 1) g = 1;
   for (int shift = 56; shift >= 0; shift -= 8) {
-    hipLaunchKernelGGL(k_pt_hist, dim3(g), dim3(XS_BLOCK), 0, s, nnz, row, ci, v, part, shift, sel, hist);
-    hipLaunchKernelGGL(k_pt_pick, dim3(1), dim3(64), 0, s, shift, sel, hist);
+    hipLaunchKernelGGL(k_pt_hist, dim3(g), dim3(XS_BLOCK), 0, s, nnz, row, ci, v, shift, sel, hist,
+                       lcnt, list);
+    hipLaunchKernelGGL(k_pt_pick, dim3(2), dim3(256), 0, s, shift, sel, hist, lcnt, list, thr);
   }
 }
+
+int xs_pt_select_words() { return 8; }
+int xs_pt_list_len() { return 2 * PT_SEL_CAP; }
 
 void xs_pt_count_parts(hipStream_t s, int nnz, const int *row, const int *ci,
                        unsigned long long *cnt2) {
@@ -556,10 +656,6 @@ void xs_pt_count_parts(hipStream_t s, int nnz, const int *row, const int *ci,
   if (g < 1) g = 1;
   (void)hipMemsetAsync(cnt2, 0, 2 * sizeof(unsigned long long), s);
   hipLaunchKernelGGL(k_pt_count_parts, dim3(g), dim3(XS_BLOCK), 0, s, nnz, row, ci, cnt2);
-}
-
-void xs_pt_thr(hipStream_t s, const unsigned long long *sel, int use, double *thr_slot) {
-  hipLaunchKernelGGL(k_pt_thr, dim3(1), dim3(64), 0, s, sel, use, thr_slot);
 }
 
 void xs_pt_dtol(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
@@ -571,12 +667,13 @@ void xs_pt_dtol(hipStream_t s, int n, const int *arp, const int *aci, const doub
 void xs_pt_keep(hipStream_t s, bool write, int n, const int *crp, const int *cci,
                 const double *cv, const double *thr, const double *dtol, int *cnt,
                 const int *frp, int *fci, double *fv, int *frow) {
+  const int rows = XS_BLOCK / XS_WAVE, g = (n + rows - 1) / rows;
   if (!write)
-    hipLaunchKernelGGL(k_pt_keep_count, dim3((n + 255) / 256), dim3(256), 0, s, n, crp, cci, cv,
-                       thr, dtol, cnt);
+    hipLaunchKernelGGL(k_pt_keep_count, dim3(g), dim3(XS_BLOCK), 0, s, n, crp, cci, cv, thr, dtol,
+                       cnt);
   else
-    hipLaunchKernelGGL(k_pt_keep_write, dim3((n + 255) / 256), dim3(256), 0, s, n, crp, cci, cv,
-                       thr, dtol, frp, fci, fv, frow);
+    hipLaunchKernelGGL(k_pt_keep_write, dim3(g), dim3(XS_BLOCK), 0, s, n, crp, cci, cv, thr, dtol,
+                       frp, fci, fv, frow);
 }
 
 void xs_pt_scale_vec(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
