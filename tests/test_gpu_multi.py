@@ -31,7 +31,7 @@ SPMM_MATS = {
 }
 
 
-@pytest.mark.parametrize("k", [1, 5, 32, 64])
+@pytest.mark.parametrize("k", [1, 2, 5, 6, 12, 32, 64])
 @pytest.mark.parametrize("name", list(SPMM_MATS))
 def test_spmm_matches_oracle(dev, name, k):
     import torch
@@ -49,6 +49,41 @@ def test_spmm_matches_oracle(dev, name, k):
     for j in range(k):
         assert np.abs(Y[:, j] - oracle.spmv(A, X[:, j])).max() <= 1e-14 * scale
     S.close()
+
+
+@pytest.mark.parametrize("k", [2, 6, 12, 32, 64])
+def test_spmm_ell_matches_csr_bitwise(dev, k):
+    """On a matrix with an automatic ELL copy, even k runs the ELL SpMM (two
+    or four columns per thread, 16-bit offsets); every column sums in the
+    row's entry order, so it equals the CSR SpMM bit for bit.  A block that
+    is not 16-byte aligned takes the CSR kernel."""
+    import torch
+    A = gen.convdiff3d(20)
+    n = len(A[0]) - 1
+    X = np.random.default_rng(10 + k).uniform(-1, 1, (n, k))
+    outs = []
+    for kernel in (None, 1):
+        S = XSolver()
+        S.mat_setup(*A)
+        if kernel is not None:
+            S.set_option("spmv_kernel", kernel)     # vector CSR, no ELL copy
+        assert S.time_ms("spmv:kernel") == (2 if kernel is None else 1)
+        Yd = torch.empty((n, k), dtype=torch.float64, device=dev)
+        S.spmm(dev_tensor(X, dev), Yd, torch_stream())
+        torch.cuda.synchronize()
+        outs.append(Yd.cpu().numpy())
+        if kernel is None:
+            # blocks shifted by one double: 8-byte aligned only
+            xb = torch.empty(n * k + 1, dtype=torch.float64, device=dev)
+            yb = torch.empty(n * k + 1, dtype=torch.float64, device=dev)
+            xb[1:].copy_(dev_tensor(X.reshape(-1), dev))
+            S.spmm(xb[1:].view(n, k), yb[1:].view(n, k), torch_stream())
+            torch.cuda.synchronize()
+            outs.append(yb[1:].cpu().numpy().reshape(n, k))
+        S.close()
+    say(f"[spmm-ell] k={k} max |ell - csr| = {np.abs(outs[0] - outs[2]).max():.3e}")
+    assert np.array_equal(outs[0], outs[2])
+    assert np.array_equal(outs[1], outs[2])
 
 
 TRSV_MATS = {

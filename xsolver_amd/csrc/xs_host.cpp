@@ -1663,7 +1663,11 @@ extern "C" int xs_pc_apply_t(xs_ctx *c, const double *x, double *y, void *stream
 extern "C" int xs_spmm(xs_ctx *c, int nrhs, const double *x, double *y, void *stream) {
   if (!c || !x || !y || nrhs < 1 || nrhs > XS_MAX_RHS) return XS_ERR_ARG;
   if (!c->mat_ready) return XS_ERR_STATE;
-  xs_mspmm(pick(c, stream), c->A, nrhs, x, y, EPI_Y, nullptr, nullptr, nullptr, nullptr);
+  const void *vecs[2] = {x, y};
+  if (c->use_ell && xs_mspmm_ell_ok(c->ell, nrhs, vecs, 2))
+    xs_mspmm_ell(pick(c, stream), c->ell, nrhs, x, y, EPI_Y, nullptr, nullptr, nullptr, nullptr);
+  else
+    xs_mspmm(pick(c, stream), c->A, nrhs, x, y, EPI_Y, nullptr, nullptr, nullptr, nullptr);
   XS_HIP(hipGetLastError());
   return xs_take_launch_guard() ? XS_ERR_STATE : XS_OK;
 }

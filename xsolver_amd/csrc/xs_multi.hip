@@ -10,7 +10,11 @@
 // that has converged stops updating while the others go on, and once every
 // column is done all kernels exit at entry.
 // No reference counterpart (the reference solves one right-hand side).
+#include <algorithm>
 #include <climits>
+#include <cstdint>
+#include <mutex>
+#include <unordered_map>
 
 #include "xs_device.h"
 #include "xs_multi.h"
@@ -100,19 +104,14 @@ __device__ inline void m_finish(const xs_mred &R, int j, int k, const double *to
 // combines the partials in a fixed order (deterministic), writes the sentinel
 // back and runs the finishing op for every column (the grid-reduction scheme
 // of reduce_finish, no fence and no counter).
+// Cross-workgroup part: threads t < k hold the workgroup's column sums bs.
 template <int NV>
-__device__ __forceinline__ void mcol_reduce(int k, const double (&v)[NV], const xs_mred &R) {
+__device__ __forceinline__ void mcol_cross(int k, const double (&bs)[NV], const xs_mred &R) {
   __shared__ double red[NV][XS_BLOCK];
   const int per = XS_BLOCK / k;
-  for (int q = 0; q < NV; ++q) red[q][threadIdx.x] = v[q];
-  __syncthreads();
-  if ((int)threadIdx.x < k) {
-    for (int q = 0; q < NV; ++q) {
-      double s = 0.0;
-      for (int r = 0; r < per; ++r) s += red[q][r * k + threadIdx.x];
-      st_ready(&R.partial[((size_t)blockIdx.x * NV + q) * k + threadIdx.x], s);
-    }
-  }
+  if ((int)threadIdx.x < k)
+    for (int q = 0; q < NV; ++q)
+      st_ready(&R.partial[((size_t)blockIdx.x * NV + q) * k + threadIdx.x], bs[q]);
   if (blockIdx.x != gridDim.x - 1) return;
   const int j = threadIdx.x % k, g = threadIdx.x / k;
   double s[NV];
@@ -144,6 +143,22 @@ __device__ __forceinline__ void mcol_reduce(int k, const double (&v)[NV], const 
     for (int c = 0; c < k; ++c) all &= (R.ms->done[c] != 0);
     R.ms->all = all;
   }
+}
+
+// Thread t holds the values of column t % k (rows t / k).
+template <int NV>
+__device__ __forceinline__ void mcol_reduce(int k, const double (&v)[NV], const xs_mred &R) {
+  __shared__ double rowv[NV][XS_BLOCK];
+  const int per = XS_BLOCK / k;
+  for (int q = 0; q < NV; ++q) rowv[q][threadIdx.x] = v[q];
+  __syncthreads();
+  double bs[NV];
+  for (int q = 0; q < NV; ++q) {
+    bs[q] = 0.0;
+    if ((int)threadIdx.x < k)
+      for (int r = 0; r < per; ++r) bs[q] += rowv[q][r * k + threadIdx.x];
+  }
+  mcol_cross<NV>(k, bs, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -197,6 +212,141 @@ __global__ __launch_bounds__(XS_BLOCK) void k_mspmm(int n, int k, const int *__r
   }
   if (EPI == EPI_Y) return;
   mcol_reduce<NV>(k, acc, R);
+}
+
+// Slab entries k0 .. k0 + 3 of row i of an ELL copy: columns (-1 for
+// padding) and values.
+template <bool O16>
+__device__ __forceinline__ void ell_batch(int i, int k0, int ld, int W, const int *__restrict__ eci,
+                                          const short *__restrict__ eco,
+                                          const double *__restrict__ ev, int (&col)[4],
+                                          double (&a)[4]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (O16) {
+      const int o = k0 + q < W ? (int)eco[(size_t)(k0 + q) * ld + i] : XS_OFF16_PAD;
+      col[q] = o == XS_OFF16_PAD ? -1 : i + o;
+    } else {
+      col[q] = k0 + q < W ? eci[(size_t)(k0 + q) * ld + i] : -1;
+    }
+    a[q] = col[q] >= 0 ? ev[(size_t)(k0 + q) * ld + i] : 0.0;
+  }
+}
+
+// SpMM over the ELL copy of A, CW adjacent columns per thread (k / CW
+// threads per row): the row's slab entries are loaded once per CW columns
+// and each gather moves CW contiguous values.  Per column the entry-order
+// sum of k_mspmm.  Same epilogues; the workgroup's column sums go through
+// LDS (rows in order) to the shared cross-workgroup combine.
+template <int EPI, bool O16, int CW>
+__global__ __launch_bounds__(XS_BLOCK) void k_mspmm_ell(
+    int n, int k, int ld, int W, const int *__restrict__ eci, const short *__restrict__ eco,
+    const double *__restrict__ ev, const double *__restrict__ X, double *__restrict__ Y,
+    const double *__restrict__ B, const double *__restrict__ Wv, xs_mred R, const xs_mstate *ms) {
+  if (ms && ms->all) return;
+  constexpr int NV = (EPI == EPI_DOT2) ? 2 : 1;
+  const int L = k / CW, per = XS_BLOCK / L;
+  const int g = threadIdx.x % L, rl = threadIdx.x / L, j0 = g * CW;
+  bool live[CW];
+  bool any = false;
+#pragma unroll
+  for (int c = 0; c < CW; ++c) {
+    live[c] = rl < per && !(ms && ms->done[j0 + c]);
+    any |= live[c];
+  }
+  double acc[NV][CW];
+#pragma unroll
+  for (int c = 0; c < CW; ++c)
+    for (int q = 0; q < NV; ++q) acc[q][c] = 0.0;
+  if (any) {
+    // the first slab batch of the thread's next row is loaded while the
+    // current row's gathers are in flight
+    const int stride = (int)gridDim.x * per;
+    int i = (int)blockIdx.x * per + rl;
+    int ncol[4];
+    double na[4];
+    if (i < n) ell_batch<O16>(i, 0, ld, W, eci, eco, ev, ncol, na);
+    for (; i < n; i += stride) {
+      // the epilogue's operand row (B or W), loaded before the gathers
+      double ep[CW];
+#pragma unroll
+      for (int c = 0; c < CW; ++c) ep[c] = 0.0;
+      if (EPI != EPI_Y) {
+        const double2 *src = reinterpret_cast<const double2 *>(
+            (EPI == EPI_BMAXN ? B : Wv) + (size_t)i * k + j0);
+#pragma unroll
+        for (int h = 0; h < CW / 2; ++h) {
+          const double2 t = src[h];
+          ep[2 * h] = t.x;
+          ep[2 * h + 1] = t.y;
+        }
+      }
+      double s[CW];
+#pragma unroll
+      for (int c = 0; c < CW; ++c) s[c] = 0.0;
+      for (int k0 = 0; k0 < W; k0 += 4) {
+        int col[4];
+        double a[4];
+        if (k0 == 0) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) { col[q] = ncol[q]; a[q] = na[q]; }
+          if (i + stride < n) ell_batch<O16>(i + stride, 0, ld, W, eci, eco, ev, ncol, na);
+        } else {
+          ell_batch<O16>(i, k0, ld, W, eci, eco, ev, col, a);
+        }
+        double xv[4][CW];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (col[q] >= 0) {
+            const double2 *src = reinterpret_cast<const double2 *>(X + (size_t)col[q] * k + j0);
+#pragma unroll
+            for (int h = 0; h < CW / 2; ++h) {
+              const double2 t = src[h];
+              xv[q][2 * h] = t.x;
+              xv[q][2 * h + 1] = t.y;
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < CW; ++c) xv[q][c] = 0.0;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (col[q] >= 0)
+#pragma unroll
+            for (int c = 0; c < CW; ++c) s[c] += a[q] * xv[q][c];
+      }
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        if (!live[c]) continue;
+        const size_t p = (size_t)i * k + j0 + c;
+        if (EPI == EPI_BMAXN) {
+          const double t = ep[c] - s[c];
+          Y[p] = t;
+          acc[0][c] += t * t;
+        } else {
+          Y[p] = s[c];
+          if (EPI == EPI_DOT1 || EPI == EPI_DOT2) acc[0][c] += ep[c] * s[c];
+          if (EPI == EPI_DOT2) acc[NV - 1][c] += s[c] * s[c];
+        }
+      }
+    }
+  }
+  if (EPI == EPI_Y) return;
+  // workgroup column sums: rows in increasing order (fixed)
+  __shared__ double cs[NV][XS_BLOCK * CW];
+  if (rl < per)
+#pragma unroll
+    for (int c = 0; c < CW; ++c)
+      for (int q = 0; q < NV; ++q) cs[q][rl * k + j0 + c] = acc[q][c];
+  __syncthreads();
+  double bs[NV];
+  for (int q = 0; q < NV; ++q) {
+    bs[q] = 0.0;
+    if ((int)threadIdx.x < k)
+      for (int r = 0; r < per; ++r) bs[q] += cs[q][r * k + threadIdx.x];
+  }
+  mcol_cross<NV>(k, bs, R);
 }
 
 // a.b (and c.d) per column
@@ -555,13 +705,13 @@ static inline int mgrid_flat(size_t m) {
   if (g > 8192) g = 8192;
   return g < 1 ? 1 : (int)g;
 }
-// k-column Jacobi sweep over the ELL copy of a factor part (n x k row-major
-// blocks): thread (row, j) of a workgroup covering 256/k rows x k columns;
-// the row's slab entries are the same for its k threads, the iterate is
-// gathered k columns at a time.  Per column the arithmetic of
-// k_tri_sweep_ell (FUSED: sweeps 1 and 2 from zero), so every column equals
-// the single right-hand side sweep.
-template <bool UPPER, bool FUSED, bool O16>
+// k-column Jacobi sweep of a factor over its ELL copy (one pass, columns
+// interleaved): CW adjacent columns per thread, k / CW threads per row; the
+// row's slab entries are loaded once per CW columns and each gather moves CW
+// contiguous values.  Per column the arithmetic of k_tri_sweep_ell (FUSED:
+// sweeps 1 and 2 from zero), so every column equals the single right-hand
+// side sweep.
+template <bool UPPER, bool FUSED, bool O16, int CW>
 __global__ __launch_bounds__(XS_BLOCK) void k_msweep_ell(int n, int k, int ld, int W,
                                                          const int *__restrict__ eci,
                                                          const short *__restrict__ eco,
@@ -572,40 +722,79 @@ __global__ __launch_bounds__(XS_BLOCK) void k_msweep_ell(int n, int k, int ld, i
                                                          double *__restrict__ Yn,
                                                          const xs_mstate *ms) {
   if (ms && ms->all) return;
-  const int per = XS_BLOCK / k;
-  const int j = threadIdx.x % k, rl = threadIdx.x / k;
-  if (rl >= per || (ms && ms->done[j])) return;
-  for (int i = blockIdx.x * per + rl; i < n; i += gridDim.x * per) {
-    double r = 0.0;
-    for (int k0 = 0; k0 < W; k0 += 4) {
-      int c[4];
-      double a[4], y[4];
+  const int L = k / CW, per = XS_BLOCK / L;
+  const int g = threadIdx.x % L, rl = threadIdx.x / L, j0 = g * CW;
+  if (rl >= per) return;
+  bool live[CW];
+  bool any = false;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (O16) {
-          const int o = k0 + q < W ? (int)eco[(size_t)(k0 + q) * ld + i] : XS_OFF16_PAD;
-          c[q] = o == XS_OFF16_PAD ? -1 : i + o;
-        } else {
-          c[q] = k0 + q < W ? eci[(size_t)(k0 + q) * ld + i] : -1;
-        }
-        a[q] = c[q] >= 0 ? ev[(size_t)(k0 + q) * ld + i] : 0.0;
+  for (int c = 0; c < CW; ++c) {
+    live[c] = !(ms && ms->done[j0 + c]);
+    any |= live[c];
+  }
+  if (!any) return;
+  // CW contiguous values of row r of an interleaved block
+  auto load = [&](const double *base, int r, double (&out)[CW]) {
+    const double *src = base + (size_t)r * k + j0;
+    if (CW == 1) {
+      out[0] = src[0];
+    } else {
+#pragma unroll
+      for (int h = 0; h < CW / 2; ++h) {
+        const double2 t = reinterpret_cast<const double2 *>(src)[h];
+        out[2 * h] = t.x;
+        out[2 * h + 1] = t.y;
       }
+    }
+  };
+  // the first slab batch of the thread's next row is loaded while the
+  // current row's gathers are in flight
+  const int stride = (int)gridDim.x * per;
+  int i = (int)blockIdx.x * per + rl;
+  int ncol[4];
+  double na[4];
+  if (i < n) ell_batch<O16>(i, 0, ld, W, eci, eco, ev, ncol, na);
+  for (; i < n; i += stride) {
+    double bi[CW];
+    load(B, i, bi);
+    const double di = UPPER ? dinv[i] : 1.0;
+    double r[CW];
+#pragma unroll
+    for (int c = 0; c < CW; ++c) r[c] = 0.0;
+    for (int k0 = 0; k0 < W; k0 += 4) {
+      int col[4];
+      double a[4];
+      if (k0 == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { col[q] = ncol[q]; a[q] = na[q]; }
+        if (i + stride < n) ell_batch<O16>(i + stride, 0, ld, W, eci, eco, ev, ncol, na);
+      } else {
+        ell_batch<O16>(i, k0, ld, W, eci, eco, ev, col, a);
+      }
+      double y[4][CW];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        if (c[q] >= 0) {
-          const double bc = B[(size_t)c[q] * k + j];
-          if (FUSED) y[q] = UPPER ? dinv[c[q]] * (bc - 0.0) : bc - 0.0;
-          else y[q] = Yo[(size_t)c[q] * k + j];
+        if (col[q] >= 0) {
+          load(FUSED ? B : Yo, col[q], y[q]);
+          if (FUSED) {
+            const double dc = UPPER ? dinv[col[q]] : 1.0;
+#pragma unroll
+            for (int c = 0; c < CW; ++c) y[q][c] = UPPER ? dc * (y[q][c] - 0.0) : y[q][c] - 0.0;
+          }
         } else {
-          y[q] = 0.0;
+#pragma unroll
+          for (int c = 0; c < CW; ++c) y[q][c] = 0.0;
         }
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        if (c[q] >= 0) r += a[q] * y[q];
+        if (col[q] >= 0)
+#pragma unroll
+          for (int c = 0; c < CW; ++c) r[c] += a[q] * y[q][c];
     }
-    const double bi = B[(size_t)i * k + j];
-    Yn[(size_t)i * k + j] = UPPER ? dinv[i] * (bi - r) : bi - r;
+#pragma unroll
+    for (int c = 0; c < CW; ++c)
+      if (live[c]) Yn[(size_t)i * k + j0 + c] = UPPER ? di * (bi[c] - r[c]) : bi[c] - r[c];
   }
 }
 
@@ -634,6 +823,77 @@ void xs_mspmm(hipStream_t s, const xs_csr &A, int k, const double *X, double *Y,
       break;
     default:
       XS_GUARD(false);
+  }
+}
+
+// Workgroups of a grid-stride kernel that are resident at once (CUs x
+// workgroups per CU at the kernel's register and LDS use), per kernel:
+// launching no more than that leaves no partial second round of workgroups.
+static int resident_blocks(const void *kernel) {
+  static std::mutex mu;
+  static std::unordered_map<const void *, int> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(kernel);
+  if (it != cache.end()) return it->second;
+  int dev = 0, cus = 0, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, XS_BLOCK, 0);
+  const int r = cus > 0 && per_cu > 0 ? cus * per_cu : XS_MGRID;
+  cache.emplace(kernel, r);
+  return r;
+}
+
+static dim3 ell_grid(const void *kernel, int n, int per) {
+  long g = ((long)n + per - 1) / per;
+  const long cap = std::min(XS_MGRID, resident_blocks(kernel));
+  if (g > cap) g = cap;
+  return dim3(g < 1 ? 1 : (unsigned)g);
+}
+
+// ELL SpMM: k even, every vector 16-byte aligned (checked by the caller)
+template <bool O16, int CW>
+static void mspmm_ell_launch(hipStream_t s, const xs_ell &E, int k, const double *X, double *Y,
+                             int epi, const double *B, const double *W, const xs_mred &R,
+                             const xs_mstate *ms) {
+  const int per = XS_BLOCK / (k / CW);
+  const dim3 blk(XS_BLOCK);
+#define XS_MSPMM_ELL(EP)                                                                        \
+  hipLaunchKernelGGL((k_mspmm_ell<EP, O16, CW>),                                               \
+                     ell_grid(reinterpret_cast<const void *>(&k_mspmm_ell<EP, O16, CW>), E.n, per), \
+                     blk, 0, s, E.n, k, E.ld, E.W, E.ci, E.co, E.v, X, Y, B, W, R, ms)
+  switch (epi) {
+    case EPI_Y: XS_MSPMM_ELL(EPI_Y); break;
+    case EPI_DOT1: XS_MSPMM_ELL(EPI_DOT1); break;
+    case EPI_DOT2: XS_MSPMM_ELL(EPI_DOT2); break;
+    case EPI_BMAXN: XS_MSPMM_ELL(EPI_BMAXN); break;
+    default: XS_GUARD(false);
+  }
+#undef XS_MSPMM_ELL
+}
+
+bool xs_mspmm_ell_ok(const xs_ell &E, int k, const void *const *vecs, int nvec) {
+  if (!E.v || !(E.ci || E.co) || k < 2 || (k & 1) || k > XS_MAX_RHS) return false;
+  for (int q = 0; q < nvec; ++q)
+    if (vecs[q] && (reinterpret_cast<uintptr_t>(vecs[q]) & 15)) return false;
+  return true;
+}
+
+void xs_mspmm_ell(hipStream_t s, const xs_ell &E, int k, const double *X, double *Y, int epi,
+                  const double *B, const double *W, const xs_mred *red, const xs_mstate *ms) {
+  const void *vecs[4] = {X, Y, B, W};
+  XS_GUARD(X && Y && xs_mspmm_ell_ok(E, k, vecs, 4));
+  XS_GUARD(epi == EPI_Y || (red && red->partial && red->counter && red->ms));
+  XS_GUARD(epi != EPI_BMAXN || B);
+  XS_GUARD(!(epi == EPI_DOT1 || epi == EPI_DOT2) || W);
+  const xs_mred R = red ? *red : xs_mred{};
+  const bool o16 = E.co != nullptr;
+  if (k % 4 == 0) {
+    if (o16) mspmm_ell_launch<true, 4>(s, E, k, X, Y, epi, B, W, R, ms);
+    else mspmm_ell_launch<false, 4>(s, E, k, X, Y, epi, B, W, R, ms);
+  } else {
+    if (o16) mspmm_ell_launch<true, 2>(s, E, k, X, Y, epi, B, W, R, ms);
+    else mspmm_ell_launch<false, 2>(s, E, k, X, Y, epi, B, W, R, ms);
   }
 }
 
@@ -687,12 +947,22 @@ void xs_msweep_ell(hipStream_t s, const xs_tell &E, int n, int k, bool upper, bo
                    const xs_mstate *ms) {
   XS_GUARD((E.W == 0 || (E.ci && E.v)) && E.ld >= n && k_ok(k) && B && Yn && (!upper || dinv) &&
            (fused || Yo));
-  const dim3 grid(mgrid_rows(n, k)), blk(XS_BLOCK);
-#define XS_MSK(U, FU, O) hipLaunchKernelGGL((k_msweep_ell<U, FU, O>), grid, blk, 0, s, n, k, E.ld, E.W, E.ci, E.co, E.v, dinv, B, Yo, Yn, ms)
-#define XS_MSE(U, FU) do { if (E.co) XS_MSK(U, FU, true); else XS_MSK(U, FU, false); } while (0)
+  // columns per thread: 4 or 2 when k allows and the blocks are 16-byte aligned
+  const bool al = ((reinterpret_cast<uintptr_t>(B) | reinterpret_cast<uintptr_t>(Yo) |
+                    reinterpret_cast<uintptr_t>(Yn)) & 15) == 0;
+  const int cw = !al ? 1 : (k % 4 == 0 ? 4 : (k % 2 == 0 ? 2 : 1));
+  const int per = XS_BLOCK / (k / cw);
+  const dim3 blk(XS_BLOCK);
+#define XS_MSK(U, FU, O, C)                                                                    \
+  hipLaunchKernelGGL((k_msweep_ell<U, FU, O, C>),                                             \
+                     ell_grid(reinterpret_cast<const void *>(&k_msweep_ell<U, FU, O, C>), n, per), \
+                     blk, 0, s, n, k, E.ld, E.W, E.ci, E.co, E.v, dinv, B, Yo, Yn, ms)
+#define XS_MSC(U, FU, O) do { if (cw == 4) XS_MSK(U, FU, O, 4); else if (cw == 2) XS_MSK(U, FU, O, 2); else XS_MSK(U, FU, O, 1); } while (0)
+#define XS_MSE(U, FU) do { if (E.co) XS_MSC(U, FU, true); else XS_MSC(U, FU, false); } while (0)
   if (upper) { if (fused) XS_MSE(true, true); else XS_MSE(true, false); }
   else { if (fused) XS_MSE(false, true); else XS_MSE(false, false); }
 #undef XS_MSE
+#undef XS_MSC
 #undef XS_MSK
 }
 

@@ -53,6 +53,17 @@ int xs_multi_reserve(xs_ctx *c, int k, int extra_vecs) {
   return XS_OK;
 }
 
+// k-column SpMM over the ELL copy of A when the product uses it (even k,
+// aligned blocks), else over the CSR
+static void mspmm(xs_ctx *c, hipStream_t s, int k, const double *X, double *Y, int epi,
+                  const double *B, const double *W, const xs_mred *red, const xs_mstate *ms) {
+  const void *vecs[4] = {X, Y, B, W};
+  if (c->use_ell && xs_mspmm_ell_ok(c->ell, k, vecs, 4))
+    xs_mspmm_ell(s, c->ell, k, X, Y, epi, B, W, red, ms);
+  else
+    xs_mspmm(s, c->A, k, X, Y, epi, B, W, red, ms);
+}
+
 double *xs_multi_vec(xs_ctx *c, int k, int idx) {
   return c->mwork + (size_t)idx * (size_t)c->A.n * (size_t)k;
 }
@@ -204,15 +215,15 @@ int xs_solve_block(xs_ctx *c, int k, const double *B, double *X) {
     XS_CHK(hipMemsetAsync(U, 0, bytes, s));
     const xs_mred r0 = m.red(MF_BICG_RES0), rg = m.red(MF_BICG_GAMA);
     const xs_mred ra = m.red(MF_BICG_ALPHA), rr = m.red(MF_BICG_RES);
-    xs_mspmm(s, c->A, k, X, R, EPI_BMAXN, B, nullptr, &r0, m.ms);      // R = B - A X, res0
+    mspmm(c, s, k, X, R, EPI_BMAXN, B, nullptr, &r0, m.ms);      // R = B - A X, res0
     XS_CHK(hipMemcpyAsync(Rh, R, bytes, hipMemcpyDeviceToDevice, s));  // rhat = r0
     return m.drive([&]() {
       xs_mbicg_p(s, n, k, R, U, P, jac, Z, m.ms);                     // p = r + w (p - a u)
       if (!jac) m.pc(P, Z, T);                                         // z = M^-1 p
-      xs_mspmm(s, c->A, k, Z, U, EPI_DOT1, nullptr, Rh, &rg, m.ms);   // u = A z, gama
+      mspmm(c, s, k, Z, U, EPI_DOT1, nullptr, Rh, &rg, m.ms);   // u = A z, gama
       xs_mbicg_s(s, n, k, R, U, Sv, jac, Z1, m.ms);                   // s = r - gama u
       if (!jac) m.pc(Sv, Z1, T);                                       // z1 = M^-1 s
-      xs_mspmm(s, c->A, k, Z1, V, EPI_DOT2, nullptr, Sv, &ra, m.ms);  // v = A z1, alpha
+      mspmm(c, s, k, Z1, V, EPI_DOT2, nullptr, Sv, &ra, m.ms);  // v = A z1, alpha
       xs_mbicg_xr(s, n, k, X, R, Z, Z1, Sv, V, Rh, rr);               // x, r, ||r||, beta
     });
   }
@@ -221,12 +232,12 @@ int xs_solve_block(xs_ctx *c, int k, const double *B, double *X) {
   double *AP = xs_multi_vec(c, k, 3), *T = xs_multi_vec(c, k, 4);
   const xs_mred rnone = m.red(MF_NONE), rinit = m.red(MF_CG_INIT);
   const xs_mred ralpha = m.red(MF_CG_ALPHA), rres = m.red(MF_CG_RES), rrr = m.red(MF_CG_RR);
-  xs_mspmm(s, c->A, k, X, R, EPI_BMAXN, B, nullptr, &rnone, m.ms);     // R = B - A X
+  mspmm(c, s, k, X, R, EPI_BMAXN, B, nullptr, &rnone, m.ms);     // R = B - A X
   m.pc(R, Z, T);                                                       // Z = M^-1 R
   xs_mdot(s, n, k, R, Z, R, R, rinit);                                 // tau, res0
   XS_CHK(hipMemcpyAsync(P, Z, bytes, hipMemcpyDeviceToDevice, s));     // P = Z
   return m.drive([&]() {
-    xs_mspmm(s, c->A, k, P, AP, EPI_DOT1, nullptr, P, &ralpha, m.ms);  // AP, alpha
+    mspmm(c, s, k, P, AP, EPI_DOT1, nullptr, P, &ralpha, m.ms);  // AP, alpha
     if (jac) {
       xs_mcg_xr(s, n, k, X, R, P, AP, jac, Z, rres);                   // x, r, z, tau, ||r||
     } else {
