@@ -113,6 +113,10 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *                    GMRES (GMRES/FGMRES)
  *   "factor_storage" ILU factors of the FP32 inner solve: 0 FP32, 1 BF16, 2 FP16
  *   "inner_tol"      relative residual reduction asked of each inner solve (1e-4)
+ *   "mixed_reliable" precision 1 with BiCGSTAB: 1 (default) = one FP32
+ *                    BiCGSTAB recurrence whose residual is replaced by the FP64
+ *                    true residual every check_every iterations (reliable
+ *                    updates; inner_tol unused), 0 = restarted refinement
  *   "parilu_scale"   1 (default) = factorise S A S, S = |diag(A)|^-1/2, and
  *                    scale back (ParILU/ParILUT fixed-point stability)
  *   "parilut_steps"  ParILUT add/sweep/select/sweep steps (reference: 10)

@@ -268,6 +268,36 @@ def test_mixed_precision_reaches_fp64_accuracy(dev, pc, factor):
     g["S"].close()
 
 
+@pytest.mark.parametrize("apply", [None, (5, 2)], ids=["exact", "sweeps52"])
+@pytest.mark.parametrize("pc", ["ilu0", "jacobi"])
+def test_mixed_reliable_updates(dev, pc, apply):
+    """Mixed BiCGSTAB with reliable updates (one FP32 recurrence whose
+    residual is replaced by the FP64 true residual every check_every
+    iterations) keeps the FP64 solve's iteration count (within 15% + 2) and
+    reaches its FP64-verified residual; the restarted refinement
+    (mixed_reliable = 0) reaches it too.  The residual history holds the
+    true residual at every replacement."""
+    A = gen.convdiff3d(32)
+    xs, b = gen.manufactured(A, 5)
+    opts = {} if apply is None or pc != "ilu0" else dict(lsolve_sweeps=apply[0], usolve_sweeps=apply[1])
+    f64 = gpu_solve(A, b, "bicgstab", pc, 1e-10, 4000, **opts)
+    rel = gpu_solve(A, b, "bicgstab", pc, 1e-10, 4000, precision=1, **opts)
+    ref = gpu_solve(A, b, "bicgstab", pc, 1e-10, 4000, precision=1, mixed_reliable=0, **opts)
+    rr_rel, rr_ref = true_relres(A, b, rel["x"]), true_relres(A, b, ref["x"])
+    say(f"[mixed-reliable] pc={pc} apply={apply} fp64 it={f64['it']} reliable it={rel['it']} "
+        f"(replacements {len(rel['res']) - 1}) refinement it={ref['it']} relres {rr_rel:.2e} / {rr_ref:.2e}")
+    for g in (f64, rel, ref):
+        assert g["st"] == 0
+    assert rr_rel < 1.05e-10 and rr_ref < 1.05e-10
+    assert rel["it"] <= 1.15 * f64["it"] + 2
+    res = rel["res"]
+    assert len(res) >= 2 and res[-1] / res[0] < 1e-10
+    assert res[-1] == pytest.approx(np.linalg.norm(b - gen.spmv(A, rel["x"])), rel=1e-3)
+    assert np.linalg.norm(rel["x"] - f64["x"]) / np.linalg.norm(f64["x"]) < 1e-7
+    for g in (f64, rel, ref):
+        g["S"].close()
+
+
 @pytest.mark.parametrize("factor", [0, 1], ids=["fp32", "bf16"])
 def test_mixed_gmres_parilut(dev, factor):
     """GMRES(30) + ParILUT in mixed precision (FP32 inner FGMRES, FP32/BF16

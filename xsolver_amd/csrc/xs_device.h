@@ -248,8 +248,20 @@ __device__ inline void do_finish(const xs_reduce &R, const double *tot, int NV) 
       conv_check(st, resl);
       double beta = tot[1];
       sc[S_OMEGA] = (beta * sc[S_GAMA]) / (sc[S_ALPHA] * sc[S_BETO] + 1e-40);
+      sc[S_RHOO] = sc[S_BETO];
       sc[S_BETO] = beta;
       sc[S_BETA] = beta;
+      break;
+    }
+    case F_BICG_REPL: {   // reliable update: the recursive residual r replaced by
+                          // the true one; beta' and the next omega from its rhat.r
+      const double beta = tot[1], resl = sqrt(tot[0]);
+      sc[S_RESL] = resl;
+      sc[S_OMEGA] = (beta * sc[S_GAMA]) / (sc[S_ALPHA] * sc[S_RHOO] + 1e-40);
+      sc[S_BETO] = beta;
+      sc[S_BETA] = beta;
+      if (!isfinite(resl) || !isfinite(beta)) st->done = 2;
+      else if (st->done == 1) st->done = 0;   // the host decides on the true residual
       break;
     }
     default:

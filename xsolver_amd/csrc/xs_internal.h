@@ -27,6 +27,7 @@ enum xs_slot {
   S_TAU, S_TAUO, S_CGALPHA, S_CGBETA,
   // GCR
   S_GCR_T0, S_GCR_T1,
+  S_RHOO,           // BiCGSTAB: rhat.r of the iteration before the last
   S_SCRATCH0 = 32
 };
 
@@ -49,7 +50,8 @@ enum xs_finish_op {
   F_BICG_GAMA,      // gama = beta / tot0
   F_BICG_ALPHA,     // alpha = tot0 / tot1
   F_BICG_RES,       // resl = sqrt(tot0), beta' = tot1, omega, convergence
-  F_GCR_COEF        // t0 = sqrt(tot0), t1 = tot1 -> scale and step
+  F_GCR_COEF,       // t0 = sqrt(tot0), t1 = tot1 -> scale and step
+  F_BICG_REPL       // residual replaced by the true one: resl = sqrt(tot0), beta' = tot1
 };
 
 struct xs_reduce {

@@ -29,6 +29,9 @@ void xs_d2f(hipStream_t s, int n, const double *a, float *b);
 void xs_vec_d2f(hipStream_t s, int n, const double *d, float *f);
 void xs_d2f_scaled(hipStream_t s, int n, const double *a, float *b, const xs_state *sref, int slot);
 void xs_add_f2d(hipStream_t s, int n, const float *d, double *x, const xs_state *sref, int slot);
+void xs_flush_f2d(hipStream_t s, int n, float *d, double *x, const xs_state *sref, int slot);
+void xs_replace32(hipStream_t s, int n, const double *r64, float *r, const float *rh,
+                  const xs_state *sref, int slot, const xs_reduce &red);
 void xs_zero32(hipStream_t s, int n, float *a);
 void xs_copy32(hipStream_t s, int n, const float *a, float *b);
 void xs_scale32(hipStream_t s, int n, const float *d, const float *a, float *b);

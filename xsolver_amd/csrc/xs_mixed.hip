@@ -610,6 +610,42 @@ void xs_add_f2d(hipStream_t s, int n, const float *d, double *x, const xs_state 
   hipLaunchKernelGGL(k_add_f2d, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s, n, d, x, sref, slot);
 }
 
+// x64 += sigma d32, d32 = 0 (sigma = sqrt(sc[slot]))
+__global__ void k_flush_f2d(int n, float *__restrict__ d, double *__restrict__ x,
+                            const xs_state *sref, int slot) {
+  const double sc = sqrt(sref->sc[slot]);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] += sc * (double)d[i];
+    d[i] = 0.f;
+  }
+}
+void xs_flush_f2d(hipStream_t s, int n, float *d, double *x, const xs_state *sref, int slot) {
+  hipLaunchKernelGGL(k_flush_f2d, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s, n, d, x, sref, slot);
+}
+
+// r32 = r64 / sigma (sigma = sqrt(sref->sc[slot])) with r32.r32 and rhat.r32
+// reduced into R (the replacement of BiCGSTAB's recursive residual)
+__global__ __launch_bounds__(XS_BLOCK) void k_replace32(int n, const double *__restrict__ r64,
+                                                        float *__restrict__ r,
+                                                        const float *__restrict__ rh,
+                                                        const xs_state *sref, int slot,
+                                                        xs_reduce R) {
+  const double inv = 1.0 / sqrt(sref->sc[slot]);
+  double acc[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) {
+    const float ri = (float)(r64[i] * inv);
+    r[i] = ri;
+    acc[0] += (double)ri * (double)ri;
+    acc[1] += (double)rh[i] * (double)ri;
+  }
+  reduce_finish<2>(acc, R);
+}
+void xs_replace32(hipStream_t s, int n, const double *r64, float *r, const float *rh,
+                  const xs_state *sref, int slot, const xs_reduce &red) {
+  hipLaunchKernelGGL(k_replace32, dim3(vgrid(n)), dim3(XS_BLOCK), 0, s, n, r64, r, rh, sref, slot,
+                     red);
+}
+
 __global__ void k_zero32(int n, float *__restrict__ a) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0.f;
 }

@@ -573,12 +573,100 @@ static int inner_fgmres32(xs_ctx *c, const float *rhs, float *d, int maxit, int 
   return h.done == 2 ? XS_ERR_BREAKDOWN : XS_OK;
 }
 
+// FP32 BiCGSTAB with reliable updates (Sleijpen & van der Vorst, "Reliable
+// updated residuals in hybrid Bi-CG methods", Computing 56, 1996): one Krylov
+// recurrence for the whole solve, in FP32 on the system scaled by sigma =
+// ||b - A x0||; every check_every iterations the FP32 correction d is added
+// into the FP64 iterate (x += sigma d), the true residual b - A x is formed
+// in FP64 (the convergence-deciding norm) and replaces the recursive FP32
+// residual, and beta' is recomputed from it (F_BICG_REPL).  Unlike the
+// restarted refinement, the Krylov space is never discarded.
+static int mixed_bicgstab_reliable(xs_ctx *c, const double *b, double *x) {
+  hipStream_t s = c->stream;
+  const int n = c->A.n;
+  double *r64 = xs_ctx_vec(c, 0);
+  float *W = c->work32;
+  float *r = W, *rh = W + (size_t)n, *p = W + 2 * (size_t)n, *u = W + 3 * (size_t)n;
+  float *sv = W + 4 * (size_t)n, *v = W + 5 * (size_t)n, *z = W + 6 * (size_t)n;
+  float *z1 = W + 7 * (size_t)n, *t = W + 8 * (size_t)n;
+  float *rhs = W + 9 * (size_t)n, *d = W + 10 * (size_t)n;
+  xs_state *st = c->st, *si = c->st_in;
+  const float *jac = (c->pc == XS_PC_JACOBI) ? c->jac32 : nullptr;
+  xs_init_state(s, st, c->tol, c->abstol, c->maxiter);
+  const int SIG = S_SCRATCH0 + 1, TRES = S_SCRATCH0 + 2;
+  const xs_reduce rs = xs_ctx_reduce(c, F_STORE, SIG);   // sigma^2 = ||b - A x0||^2
+  const xs_reduce rt = xs_ctx_reduce(c, F_STORE, TRES);  // ||b - A x||^2
+  xs_op_spmv(c, x, r64, EPI_BMAXN, b, nullptr, &rs);
+  xs_state h{}, hi{};
+  XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
+  XS_CHK(hipStreamSynchronize(s));
+  const double res0 = std::sqrt(h.sc[SIG]);
+  std::vector<double> hist(1, res0);
+  double res = res0;
+  int rc = XS_NOT_CONVERGED, outer = 0;
+  c->iters = 0;
+  if (!std::isfinite(res0)) rc = XS_ERR_BREAKDOWN;
+  else if (res0 == 0.0 || res0 <= c->abstol) rc = XS_OK;
+  if (rc == XS_NOT_CONVERGED) {
+    // scaled start: r = (b - A x0) / sigma, rhat = r, d = p = u = 0
+    xs_d2f_scaled(s, n, r64, rhs, st, SIG);
+    xs_init_state(s, si, c->tol, 0.0, c->maxiter);
+    xs_zero32(s, n, d);
+    xs_zero32(s, n, p);
+    xs_zero32(s, n, u);
+    xs_reduce r0{}; r0.partial = c->partial; r0.counter = c->counters; r0.st = si; r0.op = F_RES0;
+    op_spmv32(c, d, r, EPI_BMAXN, rhs, nullptr, &r0);
+    xs_copy32(s, n, r, rh);
+    xs_reduce rg = r0, ra = r0, rr = r0, rp = r0;
+    rg.op = F_BICG_GAMA; ra.op = F_BICG_ALPHA; rr.op = F_BICG_RES; rp.op = F_BICG_REPL;
+    auto iteration = [&]() {
+      xs_bicg_p32(s, n, r, u, p, jac, z, si);
+      if (!jac) op_pc32(c, p, z, t, si);
+      op_spmv32(c, z, u, EPI_DOT1, nullptr, rh, &rg);
+      xs_bicg_s32(s, n, r, u, sv, jac, z1, si);
+      if (!jac) op_pc32(c, sv, z1, t, si);
+      op_spmv32(c, z1, v, EPI_DOT2, nullptr, sv, &ra);
+      xs_bicg_xr32(s, n, d, r, z, z1, sv, v, rh, rr);
+    };
+    xs_graph_cache gc;
+    int launched = 0;
+    for (;;) {
+      const int k = std::min(c->check_every, c->maxiter - launched);
+      XS_CHK(xs_run_batch(c, gc, k, iteration) == 0 ? hipSuccess : hipErrorUnknown);
+      launched += k;
+      // reliable update: x += sigma d, true residual in FP64, replacement
+      xs_flush_f2d(s, n, d, x, st, SIG);
+      xs_op_spmv(c, x, r64, EPI_BMAXN, b, nullptr, &rt);
+      xs_replace32(s, n, r64, r, rh, st, SIG, rp);
+      outer++;
+      XS_CHK(hipGetLastError());
+      XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
+      XS_CHK(hipMemcpyAsync(&hi, si, sizeof(xs_state), hipMemcpyDeviceToHost, s));
+      XS_CHK(hipStreamSynchronize(s));
+      res = std::sqrt(h.sc[TRES]);
+      hist.push_back(res);
+      c->iters = hi.iter;
+      if (!std::isfinite(res) || hi.done == 2) { rc = XS_ERR_BREAKDOWN; break; }
+      if (res == 0.0 || res / res0 < c->tol || res <= c->abstol) { rc = XS_OK; break; }
+      if (hi.iter >= c->maxiter || launched >= c->maxiter) break;
+    }
+  }
+  XS_CHK(hipGetLastError());
+  c->outer_iters = outer;
+  c->res0 = res0;
+  c->resl = res;
+  const int m = std::min<int>((int)hist.size(), c->resvec_cap);
+  XS_CHK(hipMemcpy(c->resvec_d, hist.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+  return rc;
+}
+
 int xs_solve_mixed(xs_ctx *c, const double *b, double *x) {
   if (c->ksp != XS_KSP_BICGSTAB && c->ksp != XS_KSP_GMRES && c->ksp != XS_KSP_FGMRES)
     return XS_ERR_UNSUPPORTED;
   int rc = xs_ctx_ensure_work(c, XS_NWORK);
   if (rc) return rc;
   if ((rc = mixed_prepare(c))) return rc;
+  if (c->ksp == XS_KSP_BICGSTAB && c->mixed_reliable) return mixed_bicgstab_reliable(c, b, x);
   hipStream_t s = c->stream;
   const int n = c->A.n;
   double *r64 = xs_ctx_vec(c, 0);

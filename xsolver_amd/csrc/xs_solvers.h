@@ -98,6 +98,8 @@ struct xs_ctx {
   // iterative-refinement loop; factors stored as FP32, BF16 or FP16
   int factor_type = 0;          // XS_F32 / XS_BF16 / XS_F16
   double inner_tol = 1e-4;      // relative reduction asked of each inner solve
+  bool mixed_reliable = true;   // mixed BiCGSTAB: one FP32 recurrence with FP64
+                                // residual replacement (else restarted refinement)
   bool mixed_ready = false;
   bool tell_ready = false;      // ELL sweep copies of the factors match ilu.lu
   bool tell32_ready = false;    // and their FP32 values (mixed inner solver)
