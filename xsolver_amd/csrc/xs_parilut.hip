@@ -53,44 +53,62 @@ __global__ __launch_bounds__(XS_WAVE *PT_WAVES) void k_pt_symbolic(
     const int *__restrict__ fci, const int *__restrict__ fdpos, int *__restrict__ cnt,
     const int *__restrict__ crp, int *__restrict__ cci, int *__restrict__ crow, int *ovf) {
   __shared__ int tab_all[PT_WAVES][PT_HASH];
-  __shared__ int keys_all[PT_WAVES][PT_HASH];
   const int lane = threadIdx.x & (XS_WAVE - 1), wv = threadIdx.x / XS_WAVE;
   const int i = blockIdx.x * PT_WAVES + wv;
   int *tab = tab_all[wv];
-  int *keys = keys_all[wv];
+  int *keys = tab;   // compacted in place (see below)
   for (int q = lane; q < PT_HASH; q += XS_WAVE) tab[q] = PT_EMPTY;
   __syncthreads();
   if (i < n) {
     for (int e = arp[i] + lane; e < arp[i + 1]; e += XS_WAVE) pt_insert(tab, aci[e], ovf);
     for (int e = frp[i] + lane; e < frp[i + 1]; e += XS_WAVE) pt_insert(tab, fci[e], ovf);
-    // union of the upper parts (incl. pivot) of the rows k in L_i, the
-    // first 64 entries of four rows loaded together (a set: order is free)
+    // union of the upper parts (incl. pivot) of the rows k in L_i (a set:
+    // order is free).  Up to 64 L entries at once: lane t holds the bounds of
+    // row k_t; then the first 64 entries of four such rows are loaded
+    // together, the next four while the current ones are inserted.
     constexpr int D = 4;
-    const int le = fdpos[i];
-    for (int e0 = frp[i]; e0 < le; e0 += D) {
-      int qb[D], qe[D], jj[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        const bool ok = e0 + d < le;
-        const int k = ok ? fci[e0 + d] : 0;
-        qb[d] = ok ? fdpos[k] : 0;
-        qe[d] = ok ? frp[k + 1] : 0;
+    const int lb = frp[i], le = fdpos[i];
+    for (int c0 = lb; c0 < le; c0 += XS_WAVE) {
+      const int nl = min(XS_WAVE, le - c0);
+      int ub = 0, ue = 0;
+      if (lane < nl) {
+        const int k = fci[c0 + lane];
+        ub = fdpos[k];
+        ue = frp[k + 1];
       }
+      int qb[D], qe[D], jj[D], nqb[D], nqe[D], njj[D];
+      auto load_group = [&](int g0, int (&gb)[D], int (&ge)[D], int (&gj)[D]) {
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        const int q = qb[d] + lane;
-        jj[d] = q < qe[d] ? fci[q] : -1;
-      }
+        for (int d = 0; d < D; ++d) {
+          const bool ok = g0 + d < nl;
+          gb[d] = ok ? __shfl(ub, g0 + d, XS_WAVE) : 0;
+          ge[d] = ok ? __shfl(ue, g0 + d, XS_WAVE) : 0;
+        }
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        if (jj[d] >= 0) pt_insert(tab, jj[d], ovf);
-        for (int q = qb[d] + XS_WAVE + lane; q < qe[d]; q += XS_WAVE) pt_insert(tab, fci[q], ovf);
+        for (int d = 0; d < D; ++d) {
+          const int q = gb[d] + lane;
+          gj[d] = q < ge[d] ? fci[q] : -1;
+        }
+      };
+      load_group(0, qb, qe, jj);
+      for (int g0 = 0; g0 < nl; g0 += D) {
+        if (g0 + D < nl) load_group(g0 + D, nqb, nqe, njj);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          if (jj[d] >= 0) pt_insert(tab, jj[d], ovf);
+          for (int q = qb[d] + XS_WAVE + lane; q < qe[d]; q += XS_WAVE) pt_insert(tab, fci[q], ovf);
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) { qb[d] = nqb[d]; qe[d] = nqe[d]; jj[d] = njj[d]; }
       }
     }
   }
   __syncthreads();
   if (i < n) {
-    // compact the keys (slot order), then rank-sort them
+    // compact the keys (slot order) to the front of the same table, then
+    // rank-sort them.  In place is safe: a chunk's 64 slots are all read
+    // (one wave-wide load) before its keys are stored, and they go to
+    // positions base + before <= q0 + lane, all read already.
     int base = 0;
     for (int q0 = 0; q0 < PT_HASH; q0 += XS_WAVE) {
       const int c = tab[q0 + lane];
@@ -178,53 +196,103 @@ __global__ void k_pt_dpos(int n, const int *__restrict__ rp, const int *__restri
 // ---------------------------------------------------------------------------
 #define PT_ROWW 4
 #define PT_MAXR 256
+#define PT_RTAB 512           // slots of a row's column table (load <= 1/2)
+
+// Column table of a pattern row held in LDS: slot t + 1 of cols[t] at the
+// multiplicative hash of cols[t], linear probing, 0 = empty.  A lookup takes
+// one or two probes where a binary search over the row takes log2(len) + 1
+// dependent LDS reads.
+__device__ __forceinline__ unsigned pt_rhash(int c) {
+  return ((unsigned)c * 2654435761u) >> (32 - 9);
+}
+static_assert(PT_RTAB == 512 && 2 * PT_MAXR <= PT_RTAB, "pt_rhash yields 9 bits");
+
+__device__ __forceinline__ void pt_row_table(const int *cols, int len, int *tab, int lane) {
+  for (int q = lane; q < PT_RTAB; q += XS_WAVE) tab[q] = 0;
+  for (int t = lane; t < len; t += XS_WAVE) {
+    unsigned h = pt_rhash(cols[t]);
+    // the table has room for every column (len <= PT_MAXR = PT_RTAB / 2)
+    while (atomicCAS(&tab[h], 0, t + 1) != 0) h = (h + 1) & (PT_RTAB - 1);
+  }
+}
+
+// slot of column j in the row, or -1
+__device__ __forceinline__ int pt_row_slot(const int *cols, const int *tab, int j) {
+  unsigned h = pt_rhash(j);
+  for (;;) {
+    const int t = tab[h];
+    if (t == 0) return -1;
+    if (cols[t - 1] == j) return t - 1;
+    h = (h + 1) & (PT_RTAB - 1);
+  }
+}
 
 __device__ __forceinline__ void pt_row_partial(int i, const int *__restrict__ frp,
                                                const int *__restrict__ fci,
                                                const double *__restrict__ fv,
                                                const int *__restrict__ fdpos,
-                                               const int *cols, double *S, int len, int lane) {
+                                               const int *cols, int *tab, double *S, int len,
+                                               int lane) {
   constexpr int D = 4;   // U rows whose first 64 entries are loaded together
   for (int t = lane; t < len; t += XS_WAVE) S[t] = 0.0;
-  const int le = fdpos[i];
-  for (int e0 = frp[i]; e0 < le; e0 += D) {
-    int qb[D], qe[D], jj[D];
-    double lk[D], uv[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const bool ok = e0 + d < le;
-      const int k = ok ? fci[e0 + d] : 0;
-      lk[d] = ok ? fv[e0 + d] : 0.0;
-      qb[d] = ok ? fdpos[k] + 1 : 0;
-      qe[d] = ok ? frp[k + 1] : 0;
+  pt_row_table(cols, len, tab, lane);
+  const int lb = frp[i], le = fdpos[i];
+  for (int c0 = lb; c0 < le; c0 += XS_WAVE) {
+    // up to 64 L entries of the row at once: lane t holds entry c0 + t, its
+    // value and the bounds of the strict upper part of row k = its column
+    const int cnt = min(XS_WAVE, le - c0);
+    int ub = 0, ue = 0;
+    double lv = 0.0;
+    if (lane < cnt) {
+      const int k = fci[c0 + lane];
+      lv = fv[c0 + lane];
+      ub = fdpos[k] + 1;
+      ue = frp[k + 1];
     }
+    // groups of D rows; the next group's first 64 entries are loaded while
+    // the current group's products are added
+    int qb[D], qe[D], jj[D], nqb[D], nqe[D], njj[D];
+    double lk[D], uv[D], nlk[D], nuv[D];
+    auto load_group = [&](int g0, int (&gb)[D], int (&ge)[D], double (&gl)[D], int (&gj)[D],
+                          double (&gu)[D]) {
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const int q = qb[d] + lane;
-      jj[d] = q < qe[d] ? fci[q] : -1;
-      uv[d] = q < qe[d] ? fv[q] : 0.0;
-    }
-    // updates strictly in ascending k: all of row d (first chunk from
-    // registers, the rest from memory) before row d + 1
+      for (int d = 0; d < D; ++d) {
+        const bool ok = g0 + d < cnt;
+        gb[d] = ok ? __shfl(ub, g0 + d, XS_WAVE) : 0;
+        ge[d] = ok ? __shfl(ue, g0 + d, XS_WAVE) : 0;
+        gl[d] = ok ? __shfl(lv, g0 + d, XS_WAVE) : 0.0;
+      }
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      for (int q0 = qb[d]; q0 < qe[d]; q0 += XS_WAVE) {
-        int j;
-        double u;
-        if (q0 == qb[d]) { j = jj[d]; u = uv[d]; }
-        else {
-          const int q = q0 + lane;
-          j = q < qe[d] ? fci[q] : -1;
-          u = q < qe[d] ? fv[q] : 0.0;
+      for (int d = 0; d < D; ++d) {
+        const int q = gb[d] + lane;
+        gj[d] = q < ge[d] ? fci[q] : -1;
+        gu[d] = q < ge[d] ? fv[q] : 0.0;
+      }
+    };
+    load_group(0, qb, qe, lk, jj, uv);
+    for (int g0 = 0; g0 < cnt; g0 += D) {
+      if (g0 + D < cnt) load_group(g0 + D, nqb, nqe, nlk, njj, nuv);
+      // updates strictly in ascending k: all of row d (first chunk from
+      // registers, the rest from memory) before row d + 1
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        for (int q0 = qb[d]; q0 < qe[d]; q0 += XS_WAVE) {
+          int j;
+          double u;
+          if (q0 == qb[d]) { j = jj[d]; u = uv[d]; }
+          else {
+            const int q = q0 + lane;
+            j = q < qe[d] ? fci[q] : -1;
+            u = q < qe[d] ? fv[q] : 0.0;
+          }
+          if (j < 0) continue;
+          const int t = pt_row_slot(cols, tab, j);
+          if (t >= 0) S[t] += lk[d] * u;
         }
-        if (j < 0) continue;
-        int lo = 0, hi = len;
-        while (lo < hi) {
-          const int m = (lo + hi) >> 1;
-          if (cols[m] < j) lo = m + 1;
-          else hi = m;
-        }
-        if (lo < len && cols[lo] == j) S[lo] += lk[d] * u;
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        qb[d] = nqb[d]; qe[d] = nqe[d]; jj[d] = njj[d]; lk[d] = nlk[d]; uv[d] = nuv[d];
       }
     }
   }
@@ -235,6 +303,7 @@ __global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_sweep_rows(
     const double *__restrict__ vin, double *__restrict__ vout, const int *__restrict__ arp,
     const int *__restrict__ aci, const double *__restrict__ av, const double *__restrict__ sc) {
   __shared__ int cols_s[PT_ROWW][PT_MAXR];
+  __shared__ int tab_s[PT_ROWW][PT_RTAB];
   __shared__ double S_s[PT_ROWW][PT_MAXR];
   const int w = threadIdx.x / XS_WAVE, lane = threadIdx.x & (XS_WAVE - 1);
   const int i = blockIdx.x * PT_ROWW + w;
@@ -245,7 +314,7 @@ __global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_sweep_rows(
   const bool rows = len <= PT_MAXR;
   if (rows) {
     for (int t = lane; t < len; t += XS_WAVE) cols[t] = ci[a + t];
-    pt_row_partial(i, rp, ci, vin, dpos, cols, S, len, lane);
+    pt_row_partial(i, rp, ci, vin, dpos, cols, tab_s[w], S, len, lane);
   }
   for (int t = lane; t < len; t += XS_WAVE) {
     const int e = a + t, j = ci[e];
@@ -264,6 +333,7 @@ __global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_embed_rows(
     const int *__restrict__ frp, const int *__restrict__ fci, const double *__restrict__ fv,
     const int *__restrict__ fdpos, double *__restrict__ cv) {
   __shared__ int cols_s[PT_ROWW][PT_MAXR];
+  __shared__ int tab_s[PT_ROWW][PT_RTAB];
   __shared__ double S_s[PT_ROWW][PT_MAXR];
   const int w = threadIdx.x / XS_WAVE, lane = threadIdx.x & (XS_WAVE - 1);
   const int i = blockIdx.x * PT_ROWW + w;
@@ -274,7 +344,7 @@ __global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_embed_rows(
   const bool rows = len <= PT_MAXR;
   if (rows) {
     for (int t = lane; t < len; t += XS_WAVE) cols[t] = cci[a + t];
-    pt_row_partial(i, frp, fci, fv, fdpos, cols, S, len, lane);
+    pt_row_partial(i, frp, fci, fv, fdpos, cols, tab_s[w], S, len, lane);
   }
   const int fa = frp[i], fb = frp[i + 1];
   for (int t = lane; t < len; t += XS_WAVE) {
