@@ -298,6 +298,21 @@ __device__ __forceinline__ void pt_row_partial(int i, const int *__restrict__ fr
   }
 }
 
+// a_ij of the scaled matrix on the row's slots (0 where A has no entry):
+// the value pt_a returns, without a search per entry
+__device__ __forceinline__ void pt_row_scatter_a(int i, const int *__restrict__ arp,
+                                                 const int *__restrict__ aci,
+                                                 const double *__restrict__ av,
+                                                 const double *__restrict__ sc, const int *cols,
+                                                 const int *tab, double *Av, int len, int lane) {
+  for (int t = lane; t < len; t += XS_WAVE) Av[t] = 0.0;
+  for (int e = arp[i] + lane; e < arp[i + 1]; e += XS_WAVE) {
+    const int j = aci[e];
+    const int t = pt_row_slot(cols, tab, j);
+    if (t >= 0) Av[t] = av[e] * sc[i] * sc[j];
+  }
+}
+
 __global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_sweep_rows(
     int n, const int *__restrict__ rp, const int *__restrict__ ci, const int *__restrict__ dpos,
     const double *__restrict__ vin, double *__restrict__ vout, const int *__restrict__ arp,
@@ -305,21 +320,23 @@ __global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_sweep_rows(
   __shared__ int cols_s[PT_ROWW][PT_MAXR];
   __shared__ int tab_s[PT_ROWW][PT_RTAB];
   __shared__ double S_s[PT_ROWW][PT_MAXR];
+  __shared__ double A_s[PT_ROWW][PT_MAXR];
   const int w = threadIdx.x / XS_WAVE, lane = threadIdx.x & (XS_WAVE - 1);
   const int i = blockIdx.x * PT_ROWW + w;
   if (i >= n) return;   // wave-uniform, no workgroup barrier below
   const int a = rp[i], len = rp[i + 1] - a;
   int *cols = cols_s[w];
-  double *S = S_s[w];
+  double *S = S_s[w], *Av = A_s[w];
   const bool rows = len <= PT_MAXR;
   if (rows) {
     for (int t = lane; t < len; t += XS_WAVE) cols[t] = ci[a + t];
     pt_row_partial(i, rp, ci, vin, dpos, cols, tab_s[w], S, len, lane);
+    pt_row_scatter_a(i, arp, aci, av, sc, cols, tab_s[w], Av, len, lane);
   }
   for (int t = lane; t < len; t += XS_WAVE) {
     const int e = a + t, j = ci[e];
     const double part = rows ? S[t] : pt_partial(rp, ci, vin, dpos, i, j);
-    const double s = pt_a(arp, aci, av, sc, i, j) - part;
+    const double s = (rows ? Av[t] : pt_a(arp, aci, av, sc, i, j)) - part;
     double r;
     if (i > j) r = s / pt_clamp(vin[dpos[j]]);
     else r = (i == j) ? pt_clamp(s) : s;
@@ -335,27 +352,42 @@ __global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_embed_rows(
   __shared__ int cols_s[PT_ROWW][PT_MAXR];
   __shared__ int tab_s[PT_ROWW][PT_RTAB];
   __shared__ double S_s[PT_ROWW][PT_MAXR];
+  __shared__ double A_s[PT_ROWW][PT_MAXR];
+  __shared__ int F_s[PT_ROWW][PT_MAXR];
   const int w = threadIdx.x / XS_WAVE, lane = threadIdx.x & (XS_WAVE - 1);
   const int i = blockIdx.x * PT_ROWW + w;
   if (i >= n) return;
   const int a = crp[i], len = crp[i + 1] - a;
-  int *cols = cols_s[w];
-  double *S = S_s[w];
+  int *cols = cols_s[w], *Fp = F_s[w];
+  double *S = S_s[w], *Av = A_s[w];
   const bool rows = len <= PT_MAXR;
+  const int fa = frp[i], fb = frp[i + 1];
   if (rows) {
     for (int t = lane; t < len; t += XS_WAVE) cols[t] = cci[a + t];
     pt_row_partial(i, frp, fci, fv, fdpos, cols, tab_s[w], S, len, lane);
+    pt_row_scatter_a(i, arp, aci, av, sc, cols, tab_s[w], Av, len, lane);
+    // positions of the old factor's entries of row i on the row's slots
+    for (int t = lane; t < len; t += XS_WAVE) Fp[t] = -1;
+    for (int e = fa + lane; e < fb; e += XS_WAVE) {
+      const int t = pt_row_slot(cols, tab_s[w], fci[e]);
+      if (t >= 0) Fp[t] = e;
+    }
   }
-  const int fa = frp[i], fb = frp[i + 1];
   for (int t = lane; t < len; t += XS_WAVE) {
     const int j = cci[a + t];
-    const int p = pt_find(fci, fa, fb, j);
+    int p;
+    if (rows) {
+      p = Fp[t];
+    } else {
+      p = pt_find(fci, fa, fb, j);
+      if (!(p < fb && fci[p] == j)) p = -1;
+    }
     double val;
-    if (p < fb && fci[p] == j) {
+    if (p >= 0) {
       val = fv[p];
     } else {
       const double part = rows ? S[t] : pt_partial(frp, fci, fv, fdpos, i, j);
-      const double r = pt_a(arp, aci, av, sc, i, j) - part;
+      const double r = (rows ? Av[t] : pt_a(arp, aci, av, sc, i, j)) - part;
       val = (i > j) ? r / pt_clamp(fv[fdpos[j]]) : r;
     }
     cv[a + t] = isfinite(val) ? val : 0.0;
