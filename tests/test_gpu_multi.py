@@ -150,7 +150,7 @@ def test_pc_apply_multi_parilut_factors(dev):
 
 
 @pytest.mark.parametrize("kl,ku", [(0, 1), (2, 2), (5, 2), (3, 6)])
-@pytest.mark.parametrize("k", [1, 7, 32])
+@pytest.mark.parametrize("k", [1, 6, 7, 12, 32])
 def test_pc_apply_multi_sweeps(dev, k, kl, ku):
     """k-column Jacobi-sweep apply (ELL copies, first two sweeps in one pass)
     against the oracle's sweeps on the GPU's own factors, column by column,
