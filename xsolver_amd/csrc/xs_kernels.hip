@@ -41,6 +41,21 @@ __device__ __forceinline__ void epilogue(int r, double s, double *__restrict__ y
   }
 }
 
+// The same with the epilogue's operand (b[r] or w[r]) already loaded.
+template <int EPI, int NV>
+__device__ __forceinline__ void epilogue_v(int r, double s, double bw, double *__restrict__ y,
+                                           double (&acc)[NV]) {
+  if (EPI == EPI_BMAX || EPI == EPI_BMAXN) {
+    const double t = bw - s;
+    y[r] = t;
+    if (EPI == EPI_BMAXN) acc[0] += t * t;
+  } else {
+    y[r] = s;
+    if (EPI == EPI_DOT1 || EPI == EPI_DOT2) acc[0] += bw * s;
+    if (EPI == EPI_DOT2) acc[NV - 1] += s * s;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // CSR SpMV: LDS-staged row blocks ("stream" rows of one nnz chunk through LDS
 // with fully coalesced loads of val/colidx, then reduce each row from LDS);
@@ -206,6 +221,15 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell(
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
   const int r = 2 * (xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x);
   if (r < n) {
+    // the epilogue's operand is loaded before the slabs, off the critical path
+    constexpr bool OPB = (EPI == EPI_BMAX || EPI == EPI_BMAXN);
+    constexpr bool OPW = (EPI == EPI_DOT1 || EPI == EPI_DOT2);
+    double e[2] = {0.0, 0.0};
+    if (OPB || OPW) {
+      const double *op = OPB ? b : w;
+      e[0] = op[r];
+      if (r + 1 < n) e[1] = op[r + 1];
+    }
     double s0 = 0.0, s1 = 0.0;
     for (int k0 = 0; k0 < W; k0 += XS_ELL_UK) {
       int2 c[XS_ELL_UK];
@@ -236,7 +260,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell(
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int rr = r + q;
-      if (rr < n) epilogue<EPI, NV>(rr, s[q], y, b, w, acc);
+      if (rr < n) epilogue_v<EPI, NV>(rr, s[q], e[q], y, acc);
     }
   }
   if (RED) reduce_finish<NV>(acc, R);
@@ -1168,6 +1192,9 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W
   if (st != nullptr && st->done) return;
   const int i = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
   if (i >= n) return;
+  // the row's own operands, loaded before the slabs (off the critical path)
+  const double bi = b[i];
+  const double di = UPPER ? dinv[i] : 1.0;
   double r = 0.0;
   for (int k0 = 0; k0 < W; k0 += 4) {
     int c[4];
@@ -1195,7 +1222,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W
     for (int q = 0; q < 4; ++q)
       if (c[q] >= 0) r += a[q] * y[q];
   }
-  yn[i] = UPPER ? dinv[i] * (b[i] - r) : b[i] - r;
+  yn[i] = UPPER ? di * (bi - r) : bi - r;
 }
 
 void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, const double *b,

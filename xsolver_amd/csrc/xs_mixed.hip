@@ -31,6 +31,21 @@ __device__ __forceinline__ void epi32(int r, float s, float *__restrict__ y,
   }
 }
 
+// The same with the epilogue's operand (b[r] or w[r]) already loaded.
+template <int EPI, int NV>
+__device__ __forceinline__ void epi32_v(int r, float s, float bw, float *__restrict__ y,
+                                        double (&acc)[NV]) {
+  if (EPI == EPI_BMAX || EPI == EPI_BMAXN) {
+    const float t = bw - s;
+    y[r] = t;
+    if (EPI == EPI_BMAXN) acc[0] += (double)t * (double)t;
+  } else {
+    y[r] = s;
+    if (EPI == EPI_DOT1 || EPI == EPI_DOT2) acc[0] += (double)bw * (double)s;
+    if (EPI == EPI_DOT2) acc[NV - 1] += (double)s * (double)s;
+  }
+}
+
 // CSR SpMV in FP32 over the same row-block plan as the FP64 kernel.
 template <int EPI>
 __global__ __launch_bounds__(XS_BLOCK) void k_spmv_csr32(
@@ -175,6 +190,15 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell32(
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
   const int r = 2 * (xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x);
   if (r < n) {
+    // the epilogue's operand is loaded before the slabs, off the critical path
+    constexpr bool OPB = (EPI == EPI_BMAX || EPI == EPI_BMAXN);
+    constexpr bool OPW = (EPI == EPI_DOT1 || EPI == EPI_DOT2);
+    float e0 = 0.f, e1 = 0.f;
+    if (OPB || OPW) {
+      const float *op = OPB ? b : w;
+      e0 = op[r];
+      if (r + 1 < n) e1 = op[r + 1];
+    }
     float s0 = 0.f, s1 = 0.f;
     constexpr int UK = 4;   // slabs per batch: all loads of a batch, then all gathers
     for (int k0 = 0; k0 < W; k0 += UK) {
@@ -202,8 +226,8 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell32(
         if (k0 + q < W) { s0 += a[q].x * x0[q]; s1 += a[q].y * x1[q]; }
       }
     }
-    if (r < n) epi32<EPI, NV>(r, s0, y, b, w, acc);
-    if (r + 1 < n) epi32<EPI, NV>(r + 1, s1, y, b, w, acc);
+    epi32_v<EPI, NV>(r, s0, e0, y, acc);
+    if (r + 1 < n) epi32_v<EPI, NV>(r + 1, s1, e1, y, acc);
   }
   if (RED) reduce_finish<NV>(acc, R);
 }
@@ -487,6 +511,9 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
   if (st != nullptr && st->done) return;
   const int i = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
   if (i >= n) return;
+  // the row's own operands, loaded before the slabs (off the critical path)
+  const float bi = b[i];
+  const float di = UPPER ? dinv[i] : 1.f;
   float r = 0.f;
   for (int k0 = 0; k0 < W; k0 += 4) {
     int c[4];
@@ -514,7 +541,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
     for (int q = 0; q < 4; ++q)
       if (c[q] >= 0) r += a[q] * y[q];
   }
-  yn[i] = UPPER ? dinv[i] * (b[i] - r) : b[i] - r;
+  yn[i] = UPPER ? di * (bi - r) : bi - r;
 }
 
 void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_tell &E, int ftype,
