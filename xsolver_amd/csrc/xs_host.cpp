@@ -69,8 +69,8 @@ static void free_precond(xs_ctx *c) {
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
   dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);
-  dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.vlo); dfree(F.tl.co);
-  dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co);
+  dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.vlo); dfree(F.tl.co); dfree(F.tl.ww);
+  dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co); dfree(F.tu.ww);
   c->tell_ready = c->tell32_ready = false;
   F = xs_ilu();
   F.trsv_wg = c->trsv_wg;
@@ -1215,10 +1215,13 @@ static int ensure_tri_ell(xs_ctx *c) {
     }
   }
   for (int part = 0; part < 2; ++part) xs_build_tell(c->stream, F, part == 1, part ? F.tu : F.tl);
-  if (!F.tell_shape) {   // offsets depend on the pattern only
+  if (!F.tell_shape) {   // offsets and group widths depend on the pattern only
     for (int part = 0; part < 2; ++part) {
       xs_tell &E = part ? F.tu : F.tl;
       if ((rc = make_offsets(c, F.n, E.ld, E.W, E.ci, &E.co))) return rc;
+      dfree(E.ww);
+      if ((rc = dalloc(&E.ww, ((size_t)F.n + XS_WAVE - 1) / XS_WAVE))) return rc;
+      xs_tri_wave_widths(c->stream, F, part == 1, E.ww);
     }
     F.tell_shape = true;
   }

@@ -105,6 +105,8 @@ struct xs_tell {
   double *v = nullptr;
   void *vlo = nullptr;       // values for the mixed inner solver (factor storage type)
   short *co = nullptr;       // column - row as 16 bits, XS_OFF16_PAD = padding
+  int *ww = nullptr;         // per 64-row group (rows 64g ..): its longest row, the
+                             // slab count the sweeps walk for those rows
 };
 
 // Work order of a sync-free triangular solve: the row groups (gb: ngroups + 1
@@ -244,6 +246,8 @@ void xs_tri_sweep(hipStream_t s, const xs_ilu &F, bool upper, const double *b, c
 // returns the widths of L and U (synchronous; nonzero on failure).
 int xs_tri_widths(hipStream_t s, const xs_ilu &F, int *d_tmp, int *wl, int *wu);
 void xs_build_tell(hipStream_t s, const xs_ilu &F, bool upper, xs_tell &E);
+// E.ww[g] = the longest strictly lower (upper) row among rows 64g .. 64g + 63
+void xs_tri_wave_widths(hipStream_t s, const xs_ilu &F, bool upper, int *ww);
 void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, const double *b,
                       const double *yo, double *yn, const xs_state *st);
 void xs_permute_vals(hipStream_t s, int nnz, const int *perm, const double *src, double *dst);

@@ -1175,6 +1175,25 @@ void xs_build_tell(hipStream_t s, const xs_ilu &F, bool upper, xs_tell &E) {
     hipLaunchKernelGGL(k_build_tell<false>, grid, blk, 0, s, F.n, E.ld, E.W, F.rp, F.ci, F.lu, F.dpos, E.ci, E.v);
 }
 
+template <bool UPPER>
+__global__ __launch_bounds__(XS_BLOCK) void k_tri_wave_widths(int n, const int *__restrict__ rp,
+                                                              const int *__restrict__ dpos,
+                                                              int *__restrict__ ww) {
+  const int i = blockIdx.x * XS_BLOCK + threadIdx.x;
+  int len = 0;
+  if (i < n) len = UPPER ? rp[i + 1] - dpos[i] - 1 : dpos[i] - rp[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off, XS_WAVE));
+  if ((threadIdx.x & (XS_WAVE - 1)) == 0 && i < n) ww[i / XS_WAVE] = len;
+}
+
+void xs_tri_wave_widths(hipStream_t s, const xs_ilu &F, bool upper, int *ww) {
+  XS_GUARD(F.rp && F.dpos && ww);
+  const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+  if (upper) hipLaunchKernelGGL(k_tri_wave_widths<true>, grid, blk, 0, s, F.n, F.rp, F.dpos, ww);
+  else hipLaunchKernelGGL(k_tri_wave_widths<false>, grid, blk, 0, s, F.n, F.rp, F.dpos, ww);
+}
+
 // yn = D (b - T yo), T given by its ELL slabs; FUSED: yo = b (L) or D b (U),
 // i.e. sweeps 1 and 2 from y = 0 in one pass.  Slabs four at a time: all
 // index/value loads of a batch, then all gathers.  Rows in 256-row blocks
@@ -1184,6 +1203,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W
                                                             const int *__restrict__ eci,
                                                             const short *__restrict__ eco,
                                                             const double *__restrict__ ev,
+                                                            const int *__restrict__ ww,
                                                             const double *__restrict__ dinv,
                                                             const double *__restrict__ b,
                                                             const double *__restrict__ yo,
@@ -1192,9 +1212,11 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W
   if (st != nullptr && st->done) return;
   const int i = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
   if (i >= n) return;
-  // the row's own operands, loaded before the slabs (off the critical path)
+  // the row's own operands, loaded before the slabs (off the critical path);
+  // the slabs past the longest row of the row's 64-row group are all padding
   const double bi = b[i];
   const double di = UPPER ? dinv[i] : 1.0;
+  W = ww ? ww[i / XS_WAVE] : W;
   double r = 0.0;
   for (int k0 = 0; k0 < W; k0 += 4) {
     int c[4];
@@ -1231,7 +1253,7 @@ void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, co
   XS_GUARD((E.W == 0 || (E.ci && E.v)) && E.ld >= F.n && b && yn && (!upper || F.dinv) &&
            (fused || yo));
   const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
-#define XS_SWK(U, FU, O) hipLaunchKernelGGL((k_tri_sweep_ell<U, FU, O>), grid, blk, 0, s, F.n, E.ld, E.W, E.ci, E.co, E.v, F.dinv, b, yo, yn, st)
+#define XS_SWK(U, FU, O) hipLaunchKernelGGL((k_tri_sweep_ell<U, FU, O>), grid, blk, 0, s, F.n, E.ld, E.W, E.ci, E.co, E.v, E.ww, F.dinv, b, yo, yn, st)
 #define XS_SWE(U, FU) do { if (E.co) XS_SWK(U, FU, true); else XS_SWK(U, FU, false); } while (0)
   if (upper) { if (fused) XS_SWE(true, true); else XS_SWE(true, false); }
   else { if (fused) XS_SWE(false, true); else XS_SWE(false, false); }

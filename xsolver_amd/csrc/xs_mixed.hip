@@ -503,6 +503,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
                                                               const int *__restrict__ eci,
                                                               const short *__restrict__ eco,
                                                               const TF *__restrict__ ev,
+                                                              const int *__restrict__ ww,
                                                               const float *__restrict__ dinv,
                                                               const float *__restrict__ b,
                                                               const float *__restrict__ yo,
@@ -514,6 +515,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
   // the row's own operands, loaded before the slabs (off the critical path)
   const float bi = b[i];
   const float di = UPPER ? dinv[i] : 1.f;
+  W = ww ? ww[i / XS_WAVE] : W;   // slabs past the group's longest row are padding
   float r = 0.f;
   for (int k0 = 0; k0 < W; k0 += 4) {
     int c[4];
@@ -553,7 +555,7 @@ void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_t
     return;
   }
   const dim3 grid((n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
-#define XS_SWK32(U, FU, O, T) hipLaunchKernelGGL((k_tri_sweep_ell32<U, FU, O, T>), grid, blk, 0, s, n, E.ld, E.W, E.ci, E.co, static_cast<const T *>(E.vlo), dinv, b, yo, yn, st)
+#define XS_SWK32(U, FU, O, T) hipLaunchKernelGGL((k_tri_sweep_ell32<U, FU, O, T>), grid, blk, 0, s, n, E.ld, E.W, E.ci, E.co, static_cast<const T *>(E.vlo), E.ww, dinv, b, yo, yn, st)
 #define XS_SWT32(U, FU, O) do { if (ftype == XS_BF16) XS_SWK32(U, FU, O, __hip_bfloat16); else if (ftype == XS_F16) XS_SWK32(U, FU, O, __half); else XS_SWK32(U, FU, O, float); } while (0)
 #define XS_SWE32(U, FU) do { if (E.co) XS_SWT32(U, FU, true); else XS_SWT32(U, FU, false); } while (0)
   if (upper) { if (fused) XS_SWE32(true, true); else XS_SWE32(true, false); }

@@ -716,6 +716,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_msweep_ell(int n, int k, int ld, i
                                                          const int *__restrict__ eci,
                                                          const short *__restrict__ eco,
                                                          const double *__restrict__ ev,
+                                                         const int *__restrict__ ww,
                                                          const double *__restrict__ dinv,
                                                          const double *__restrict__ B,
                                                          const double *__restrict__ Yo,
@@ -755,19 +756,27 @@ __global__ __launch_bounds__(XS_BLOCK) void k_msweep_ell(int n, int k, int ld, i
   double na[4];
   if (i < n) ell_batch<O16>(i, 0, ld, W, eci, eco, ev, ncol, na);
   for (; i < n; i += stride) {
+    // this row's first batch; the next row's is loaded now, whether or not
+    // this row walks its slabs
+    int col0[4];
+    double a0[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { col0[q] = ncol[q]; a0[q] = na[q]; }
+    if (i + stride < n) ell_batch<O16>(i + stride, 0, ld, W, eci, eco, ev, ncol, na);
     double bi[CW];
     load(B, i, bi);
     const double di = UPPER ? dinv[i] : 1.0;
     double r[CW];
 #pragma unroll
     for (int c = 0; c < CW; ++c) r[c] = 0.0;
-    for (int k0 = 0; k0 < W; k0 += 4) {
+    // slabs past the longest row of the row's 64-row group are padding
+    const int Wi = ww ? ww[i / XS_WAVE] : W;
+    for (int k0 = 0; k0 < Wi; k0 += 4) {
       int col[4];
       double a[4];
       if (k0 == 0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) { col[q] = ncol[q]; a[q] = na[q]; }
-        if (i + stride < n) ell_batch<O16>(i + stride, 0, ld, W, eci, eco, ev, ncol, na);
+        for (int q = 0; q < 4; ++q) { col[q] = col0[q]; a[q] = a0[q]; }
       } else {
         ell_batch<O16>(i, k0, ld, W, eci, eco, ev, col, a);
       }
@@ -956,7 +965,7 @@ void xs_msweep_ell(hipStream_t s, const xs_tell &E, int n, int k, bool upper, bo
 #define XS_MSK(U, FU, O, C)                                                                    \
   hipLaunchKernelGGL((k_msweep_ell<U, FU, O, C>),                                             \
                      ell_grid(reinterpret_cast<const void *>(&k_msweep_ell<U, FU, O, C>), n, per), \
-                     blk, 0, s, n, k, E.ld, E.W, E.ci, E.co, E.v, dinv, B, Yo, Yn, ms)
+                     blk, 0, s, n, k, E.ld, E.W, E.ci, E.co, E.v, E.ww, dinv, B, Yo, Yn, ms)
 #define XS_MSC(U, FU, O) do { if (cw == 4) XS_MSK(U, FU, O, 4); else if (cw == 2) XS_MSK(U, FU, O, 2); else XS_MSK(U, FU, O, 1); } while (0)
 #define XS_MSE(U, FU) do { if (E.co) XS_MSC(U, FU, true); else XS_MSC(U, FU, false); } while (0)
   if (upper) { if (fused) XS_MSE(true, true); else XS_MSE(true, false); }
