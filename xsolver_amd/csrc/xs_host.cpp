@@ -1203,7 +1203,8 @@ static int ensure_tri_ell(xs_ctx *c) {
     if (xs_tri_widths(c->stream, F, c->d_int, &w[0], &w[1])) return XS_ERR_DEVICE;
     for (int part = 0; part < 2; ++part) {
       xs_tell &E = part ? F.tu : F.tl;
-      const size_t need = (size_t)w[part] * F.n;
+      const int ld = (F.n + 1) & ~1;   // even: the FP32 sweeps load row pairs
+      const size_t need = (size_t)w[part] * ld;
       if (need > E.cap) {
         dfree(E.ci); dfree(E.v); dfree(E.vlo); dfree(E.co);
         E.cap = 0;
@@ -1211,7 +1212,7 @@ static int ensure_tri_ell(xs_ctx *c) {
         E.cap = need;
       }
       E.W = w[part];
-      E.ld = F.n;
+      E.ld = ld;
     }
   }
   for (int part = 0; part < 2; ++part) xs_build_tell(c->stream, F, part == 1, part ? F.tu : F.tl);

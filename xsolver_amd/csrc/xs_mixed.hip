@@ -498,6 +498,23 @@ void xs_tri_sweep32(hipStream_t s, bool upper, int n, const int *rp, const int *
 
 // FP32 Jacobi sweep over the ELL copy of a factor part (FP32 values); the
 // arithmetic of k_tri_sweep32 entry by entry, FUSED as in the FP64 kernel.
+// pairs of factor values at i, i + 1 (i even: 8- or 4-byte aligned), converted
+// exactly as fl() converts one
+__device__ __forceinline__ float2 fl2(const float *p, size_t i) {
+  return *reinterpret_cast<const float2 *>(p + i);
+}
+__device__ __forceinline__ float2 fl2(const __hip_bfloat16 *p, size_t i) {
+  const unsigned u = *reinterpret_cast<const unsigned *>(p + i);   // [i] low, [i + 1] high
+  return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xFFFF0000u));
+}
+__device__ __forceinline__ float2 fl2(const __half *p, size_t i) {
+  return __half22float2(*reinterpret_cast<const __half2 *>(p + i));
+}
+
+// Two rows per thread (r, r + 1; ld even): one 4-byte offset load and one
+// paired value load per slab cover both rows, half the load instructions of
+// a row per thread (which matters most for 2-byte BF16 / FP16 values).  Per
+// row the arithmetic of the one-row form, slab by slab.
 template <bool UPPER, bool FUSED, bool O16, typename TF>
 __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int W,
                                                               const int *__restrict__ eci,
@@ -510,51 +527,69 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
                                                               float *__restrict__ yn,
                                                               const xs_state *st) {
   if (st != nullptr && st->done) return;
-  const int i = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
-  if (i >= n) return;
-  // the row's own operands, loaded before the slabs (off the critical path)
-  const float bi = b[i];
-  const float di = UPPER ? dinv[i] : 1.f;
-  W = ww ? ww[i / XS_WAVE] : W;   // slabs past the group's longest row are padding
-  float r = 0.f;
+  const int r = 2 * (xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x);
+  if (r >= n) return;
+  const bool two = r + 1 < n;
+  // the rows' own operands, loaded before the slabs (off the critical path)
+  const float b0 = b[r], b1 = two ? b[r + 1] : 0.f;
+  const float d0 = UPPER ? dinv[r] : 1.f, d1 = (UPPER && two) ? dinv[r + 1] : 1.f;
+  W = ww ? ww[r / XS_WAVE] : W;   // slabs past the group's longest row are padding
+  float s0 = 0.f, s1 = 0.f;
   for (int k0 = 0; k0 < W; k0 += 4) {
-    int c[4];
-    float a[4], y[4];
+    int c0[4], c1[4];
+    float a0[4], a1[4], y0[4], y1[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (O16) {
-        const int o = k0 + q < W ? (int)eco[(size_t)(k0 + q) * ld + i] : XS_OFF16_PAD;
-        c[q] = o == XS_OFF16_PAD ? -1 : i + o;
-      } else {
-        c[q] = k0 + q < W ? eci[(size_t)(k0 + q) * ld + i] : -1;
+      c0[q] = c1[q] = -1;
+      a0[q] = a1[q] = 0.f;
+      if (k0 + q < W) {
+        const size_t p = (size_t)(k0 + q) * ld + r;
+        if (O16) {
+          const short2 o = *reinterpret_cast<const short2 *>(eco + p);
+          c0[q] = o.x == XS_OFF16_PAD ? -1 : r + o.x;
+          c1[q] = (o.y == XS_OFF16_PAD || !two) ? -1 : r + 1 + o.y;
+        } else {
+          const int2 cc = *reinterpret_cast<const int2 *>(eci + p);
+          c0[q] = cc.x;
+          c1[q] = two ? cc.y : -1;
+        }
+        const float2 av = fl2(ev, p);
+        a0[q] = c0[q] >= 0 ? av.x : 0.f;
+        a1[q] = c1[q] >= 0 ? av.y : 0.f;
       }
-      a[q] = c[q] >= 0 ? fl(ev, (size_t)(k0 + q) * ld + i) : 0.f;
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (c[q] >= 0) {
-        if (FUSED) y[q] = UPPER ? dinv[c[q]] * (b[c[q]] - 0.f) : b[c[q]] - 0.f;
-        else y[q] = yo[c[q]];
-      } else {
-        y[q] = 0.f;
+      y0[q] = y1[q] = 0.f;
+      if (c0[q] >= 0) {
+        if (FUSED) y0[q] = UPPER ? dinv[c0[q]] * (b[c0[q]] - 0.f) : b[c0[q]] - 0.f;
+        else y0[q] = yo[c0[q]];
+      }
+      if (c1[q] >= 0) {
+        if (FUSED) y1[q] = UPPER ? dinv[c1[q]] * (b[c1[q]] - 0.f) : b[c1[q]] - 0.f;
+        else y1[q] = yo[c1[q]];
       }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (c[q] >= 0) r += a[q] * y[q];
+    for (int q = 0; q < 4; ++q) {
+      if (c0[q] >= 0) s0 += a0[q] * y0[q];
+      if (c1[q] >= 0) s1 += a1[q] * y1[q];
+    }
   }
-  yn[i] = UPPER ? di * (bi - r) : bi - r;
+  yn[r] = UPPER ? d0 * (b0 - s0) : b0 - s0;
+  if (two) yn[r + 1] = UPPER ? d1 * (b1 - s1) : b1 - s1;
 }
 
 void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_tell &E, int ftype,
                         const float *dinv, const float *b, const float *yo, float *yn,
                         const xs_state *st) {
-  if (!((E.W == 0 || (E.ci && E.vlo)) && E.ld >= n && b && yn && (!upper || dinv) &&
-        (fused || yo))) {
+  if (!((E.W == 0 || (E.ci && E.vlo)) && E.ld >= n && E.ld % 2 == 0 && b && yn &&
+        (!upper || dinv) && (fused || yo))) {
     xs_raise_launch_guard();
     return;
   }
-  const dim3 grid((n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+  const int pairs = (n + 1) / 2;
+  const dim3 grid((pairs + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
 #define XS_SWK32(U, FU, O, T) hipLaunchKernelGGL((k_tri_sweep_ell32<U, FU, O, T>), grid, blk, 0, s, n, E.ld, E.W, E.ci, E.co, static_cast<const T *>(E.vlo), E.ww, dinv, b, yo, yn, st)
 #define XS_SWT32(U, FU, O) do { if (ftype == XS_BF16) XS_SWK32(U, FU, O, __hip_bfloat16); else if (ftype == XS_F16) XS_SWK32(U, FU, O, __half); else XS_SWK32(U, FU, O, float); } while (0)
 #define XS_SWE32(U, FU) do { if (E.co) XS_SWT32(U, FU, true); else XS_SWT32(U, FU, false); } while (0)
