@@ -11,7 +11,7 @@ backend applies them -- Jacobi sweeps per factor -- with 5 sweeps on L and 2
 on U in the headline configuration (the reference hard-codes 20 / 20; 5 / 2
 gave the shortest time-to-rtol of the sweep counts tried on this system);
 the exact sync-free SpTRSV apply, the reference's 20/20 sweeps and the
-FP32-inner mixed solve are timed beside it.
+FP32-inner mixed solves (FP32- and BF16-stored factors) are timed beside it.
 
 Printed: one JSON line (value = seconds per solve, lower is better) with the
 dominant kernel's roofline (algorithmic bytes / mean time from HIP events
@@ -442,6 +442,9 @@ def main():
     exactp = run_gpu(A, b, "csr", 1, 1)
     mixed = None if args.no_mixed else run_gpu(A, b, "csr", args.steps, args.warmup,
                                                profile=False, precision=1, sweeps=APPLY)
+    mixed_bf16 = None if args.no_mixed else run_gpu(A, b, "csr", args.steps, args.warmup,
+                                                    profile=False, precision=1,
+                                                    factor_storage=1, sweeps=APPLY)
     mixed_exact = None if args.no_mixed else run_gpu(A, b, "csr", args.steps, args.warmup,
                                                      profile=False, precision=1)
     # the reference GPU backend's own apply: 20 Jacobi sweeps per factor
@@ -505,6 +508,11 @@ def main():
             "mixed_true_relres": relres_of(mixed),
             "mixed_speedup_vs_fp64": (round(head["sec_per_step"] / mixed["sec_per_step"], 3)
                                       if mixed else None),
+            "mixed_bf16_factors_sec_per_step": sec(mixed_bf16),
+            "mixed_bf16_factors_iterations": its(mixed_bf16),
+            "mixed_bf16_factors_true_relres": relres_of(mixed_bf16),
+            "mixed_bf16_factors_speedup_vs_fp64": (
+                round(head["sec_per_step"] / mixed_bf16["sec_per_step"], 3) if mixed_bf16 else None),
             "mixed_exact_trsv_sec_per_step": sec(mixed_exact),
             "mixed_exact_trsv_speedup_vs_fp64": (
                 round(exact["sec_per_step"] / mixed_exact["sec_per_step"], 3)
