@@ -126,7 +126,9 @@ static int alloc_base(xs_ctx *c) {
   XS_HIP(hipMemset(c->counters, 0, sizeof(unsigned) * 64));
   if ((rc = dalloc(&c->d_int, 8))) return rc;
   XS_HIP(hipMemset(c->d_int, 0, sizeof(int) * 8));
-  XS_HIP(hipHostMalloc((void **)&c->h_state, sizeof(xs_state), hipHostMallocDefault));
+  // pinned: [0] the solve's state, [1] the inner solve's (mixed precision);
+  // a device-to-host copy into pageable memory goes through a staging blit
+  XS_HIP(hipHostMalloc((void **)&c->h_state, 2 * sizeof(xs_state), hipHostMallocDefault));
   return XS_OK;
 }
 

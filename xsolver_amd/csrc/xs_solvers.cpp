@@ -470,7 +470,7 @@ static int inner_bicgstab32(xs_ctx *c, const float *rhs, float *d, int maxit, in
   xs_reduce rg = r0, ra = r0, rr = r0;
   rg.op = F_BICG_GAMA; ra.op = F_BICG_ALPHA; rr.op = F_BICG_RES;
   int launched = 0;
-  xs_state h{};
+  xs_state &h = c->h_state[1];   // pinned
   XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
   XS_CHK(hipStreamSynchronize(s));
   auto iteration = [&]() {
@@ -529,7 +529,7 @@ static int inner_fgmres32(xs_ctx *c, const float *rhs, float *d, int maxit, int 
   xs_gmres_cycle(s, m, 1, g, st, nullptr);
   xs_ts_scale<float>(s, n, r, V, g.misc + XS_G_SCALE, st);
   int launched = 0, i = 0, cycle_start = 0;
-  xs_state h{};
+  xs_state &h = c->h_state[1];   // pinned
   XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
   XS_CHK(hipStreamSynchronize(s));
   bool done = h.done != 0;
@@ -597,7 +597,7 @@ static int mixed_bicgstab_reliable(xs_ctx *c, const double *b, double *x) {
   const xs_reduce rs = xs_ctx_reduce(c, F_STORE, SIG);   // sigma^2 = ||b - A x0||^2
   const xs_reduce rt = xs_ctx_reduce(c, F_STORE, TRES);  // ||b - A x||^2
   xs_op_spmv(c, x, r64, EPI_BMAXN, b, nullptr, &rs);
-  xs_state h{}, hi{};
+  xs_state &h = c->h_state[0], &hi = c->h_state[1];   // pinned
   XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
   XS_CHK(hipStreamSynchronize(s));
   const double res0 = std::sqrt(h.sc[SIG]);
@@ -680,7 +680,7 @@ int xs_solve_mixed(xs_ctx *c, const double *b, double *x) {
   rc = XS_NOT_CONVERGED;
   for (;;) {
     xs_op_spmv(c, x, r64, EPI_BMAXN, b, nullptr, &rn);           // FP64 true residual
-    xs_state h{};
+    xs_state &h = c->h_state[0];   // pinned (the inner solves use [1])
     XS_CHK(hipMemcpyAsync(&h, st, sizeof(xs_state), hipMemcpyDeviceToHost, s));
     XS_CHK(hipStreamSynchronize(s));
     res = std::sqrt(h.sc[S_SCRATCH0 + 1]);
