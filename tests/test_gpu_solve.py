@@ -298,6 +298,41 @@ def test_mixed_reliable_updates(dev, pc, apply):
         g["S"].close()
 
 
+def test_mixed_copies_follow_options(dev):
+    """The FP32 / BF16 copies of the mixed solve are made once and reused by
+    later solves; changing the factor storage or the SpMV format between
+    solves gives bit for bit what a fresh context with those options gives."""
+    A = gen.convdiff3d(24)
+    xs, b = gen.manufactured(A, 7)
+    base = dict(parilu_async=0, lsolve_sweeps=5, usolve_sweeps=2, precision=1)
+
+    def fresh(**extra):
+        g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000, **base, **extra)
+        g["S"].close()
+        return g["x"]
+
+    S = XSolver()
+    S.mat_setup(*A)
+    S.ksp_settype("bicgstab")
+    S.ksp_setoption(1e-10, 1000)
+    S.pc_settype("ilu0")
+    for k, v in dict(base, factor_storage=1).items():
+        S.set_option(k, v)
+    x1, st1 = S.solve(b)
+    x2, st2 = S.solve(b)
+    assert st1 == 0 and st2 == 0 and np.array_equal(x1, x2)
+    assert np.array_equal(x1, fresh(factor_storage=1))
+    S.set_option("factor_storage", 0)
+    x3, st3 = S.solve(b)
+    assert st3 == 0 and np.array_equal(x3, fresh(factor_storage=0))
+    S.set_option("spmv_kernel", 1)
+    x4, st4 = S.solve(b)
+    assert st4 == 0 and np.array_equal(x4, fresh(factor_storage=0, spmv_kernel=1))
+    for x in (x1, x3, x4):
+        assert true_relres(A, b, x) < 1.05e-10
+    S.close()
+
+
 @pytest.mark.parametrize("factor", [0, 1], ids=["fp32", "bf16"])
 def test_mixed_gmres_parilut(dev, factor):
     """GMRES(30) + ParILUT in mixed precision (FP32 inner FGMRES, FP32/BF16

@@ -267,6 +267,7 @@ static int make_ell(xs_ctx *c) {
   int rc;
   xs_csr &A = c->A;
   if (c->ell.v) return XS_OK;
+  c->mixed_ready = false;   // no FP32 copy of the new ELL yet
   int W = xs_max_row_len(c->stream, A, c->d_int);
   c->ell.n = A.n;
   c->ell.W = W;
@@ -415,6 +416,7 @@ extern "C" int xs_autotune(xs_ctx *c) {
 }
 
 static int setup_common(xs_ctx *c, int n, int nnz, int use_ell) {
+  c->mixed_ready = false;
   c->use_ell = use_ell ? 1 : 0;
   c->A.n = n;
   c->A.nnz = nnz;
@@ -630,6 +632,7 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     if (value != 0 && value != 1 && value != -1) return XS_ERR_ARG;
     c->spmv_vec = (int)value;
     if (c->ell_auto && value >= 0) { c->use_ell = 0; c->ell_auto = false; }
+    c->mixed_ready = false;   // the FP32 SpMV follows the format
     if (c->mat_ready) {
       c->plan.vec = c->spmv_vec < 0 ? (regular_rows(c->h_rp, c->plan) ? 1 : 0) : c->spmv_vec;
       c->planT.vec = c->spmv_vec < 0 ? c->plan.vec : c->spmv_vec;
@@ -661,6 +664,7 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     }
     c->use_ell = want;
     c->ell_auto = false;
+    c->mixed_ready = false;
   } else return XS_ERR_ARG;
   return XS_OK;
 }

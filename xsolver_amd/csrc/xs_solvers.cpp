@@ -346,7 +346,11 @@ void xs_mixed_free(xs_ctx *c) {
   c->mixed_ready = false;
 }
 
+// FP32 / BF16 / FP16 copies of A, the factors and the sweep copies: made
+// once and kept until the matrix values, the preconditioner, the factor
+// storage type or the SpMV format change (each clears mixed_ready).
 static int mixed_prepare(xs_ctx *c) {
+  if (c->mixed_ready) return XS_OK;
   hipStream_t s = c->stream;
   const int n = c->A.n;
   int rc;
