@@ -112,7 +112,8 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *                    iterative refinement (BiCGSTAB/CG) or FP64 flexible
  *                    GMRES (GMRES/FGMRES)
  *   "factor_storage" ILU factors of the FP32 inner solve: 0 FP32, 1 BF16, 2 FP16
- *   "inner_tol"      relative residual reduction asked of each inner solve (1e-4)
+ *   "inner_tol"      relative residual reduction asked of each inner solve;
+ *                    0 (default) = sqrt(tol), at least 1e-6
  *   "mixed_reliable" precision 1 with BiCGSTAB: 1 (default) = one FP32
  *                    BiCGSTAB recurrence whose residual is replaced by the FP64
  *                    true residual every check_every iterations (reliable

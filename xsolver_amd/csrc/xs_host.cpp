@@ -649,7 +649,10 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   else if (k == "refactor") { c->pc_ready = false; c->mixed_ready = false; c->tell_ready = c->tell32_ready = false; }
   else if (k == "factor_storage") { if (value < 0 || value > 2) return XS_ERR_ARG; c->factor_type = (int)value; c->mixed_ready = false; }
   else if (k == "mixed_reliable") { c->mixed_reliable = value != 0; }
-  else if (k == "inner_tol") { if (!(value > 0 && value < 1)) return XS_ERR_ARG; c->inner_tol = value; }
+  else if (k == "inner_tol") {   // 0 = automatic (sqrt of the solve's rtol, at least 1e-6)
+    if (!(value >= 0 && value < 1)) return XS_ERR_ARG;
+    c->inner_tol = value;
+  }
   else if (k == "parilu_scale") { c->parilu_scale = value != 0; c->pc_ready = false; }
   else if (k == "parilut_steps") { if (value < 1) return XS_ERR_ARG; c->parilut_steps = (int)value; c->pc_ready = false; }
   else if (k == "profile") { c->prof_kind = (int)value; for (int i = 0; i < 4; i++) { c->prof_ms[i] = 0; c->prof_cnt[i] = 0; } }

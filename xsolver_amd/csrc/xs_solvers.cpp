@@ -454,6 +454,14 @@ static void op_pc32(xs_ctx *c, const float *x, float *y, float *tmp, const xs_st
   }
 }
 
+// Relative reduction asked of an inner FP32 solve.  By default sqrt(rtol):
+// two refinement steps reach rtol when each inner solve meets its target,
+// and 1e-6 is about the reduction FP32 Krylov iterations still deliver.
+static double inner_tol_of(const xs_ctx *c) {
+  if (c->inner_tol > 0.0) return c->inner_tol;
+  return std::min(0.5, std::max(std::sqrt(c->tol), 1e-6));
+}
+
 // FP32 BiCGSTAB on A d = rhs from d = 0 (inner solve); returns iterations.
 static int inner_bicgstab32(xs_ctx *c, const float *rhs, float *d, int maxit, int *iters) {
   hipStream_t s = c->stream;
@@ -464,7 +472,7 @@ static int inner_bicgstab32(xs_ctx *c, const float *rhs, float *d, int maxit, in
   float *z1 = W + 7 * (size_t)n, *t = W + 8 * (size_t)n;
   xs_state *st = c->st_in;
   const float *jac = (c->pc == XS_PC_JACOBI) ? c->jac32 : nullptr;
-  xs_init_state(s, st, c->inner_tol, 0.0, maxit);
+  xs_init_state(s, st, inner_tol_of(c), 0.0, maxit);
   xs_zero32(s, n, d);
   xs_zero32(s, n, p);
   xs_zero32(s, n, u);
@@ -526,7 +534,7 @@ static int inner_fgmres32(xs_ctx *c, const float *rhs, float *d, int maxit, int 
   unsigned *tc = c->counters + 10;
   double *P = c->ts_partial;
   xs_state *st = c->st_in;
-  xs_init_state(s, st, c->inner_tol, 0.0, maxit);
+  xs_init_state(s, st, inner_tol_of(c), 0.0, maxit);
   xs_zero32(s, n, d);
   xs_copy32(s, n, rhs, r);
   xs_ts_dot<float>(s, n, 1, r, ld, r, P, tc, g.misc + XS_G_NRM2, nullptr);

@@ -97,7 +97,8 @@ struct xs_ctx {
   // mixed precision (precision = 1): FP32 inner BiCGSTAB / CG inside an FP64
   // iterative-refinement loop; factors stored as FP32, BF16 or FP16
   int factor_type = 0;          // XS_F32 / XS_BF16 / XS_F16
-  double inner_tol = 1e-4;      // relative reduction asked of each inner solve
+  double inner_tol = 0.0;       // relative reduction asked of each inner solve;
+                                // 0 = sqrt(tol), at least 1e-6 (two refinement steps)
   bool mixed_reliable = true;   // mixed BiCGSTAB: one FP32 recurrence with FP64
                                 // residual replacement (else restarted refinement)
   bool mixed_ready = false;
