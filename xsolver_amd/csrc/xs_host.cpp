@@ -1216,6 +1216,7 @@ static int ensure_tri_ell(xs_ctx *c) {
       const size_t need = (size_t)w[part] * ld;
       if (need > E.cap) {
         dfree(E.ci); dfree(E.v); dfree(E.vlo); dfree(E.co);
+        E.vlo_bytes = 0;
         E.cap = 0;
         if ((rc = dalloc(&E.ci, need)) || (rc = dalloc(&E.v, need))) return rc;
         E.cap = need;

@@ -104,6 +104,7 @@ struct xs_tell {
   int *ci = nullptr;
   double *v = nullptr;
   void *vlo = nullptr;       // values for the mixed inner solver (factor storage type)
+  size_t vlo_bytes = 0;      // allocated bytes of vlo
   short *co = nullptr;       // column - row as 16 bits, XS_OFF16_PAD = padding
   int *ww = nullptr;         // per 64-row group (rows 64g ..): its longest row, the
                              // slab count the sweeps walk for those rows

@@ -343,6 +343,7 @@ void xs_mixed_free(xs_ctx *c) {
   f(c->basis32);
   c->basis32_len = 0;
   c->work32_n = 0;
+  c->F32_bytes = 0;
   c->mixed_ready = false;
 }
 
@@ -364,9 +365,18 @@ static int mixed_prepare(xs_ctx *c) {
     if ((rc = ensure_buf(&c->jac32, (size_t)n))) return rc;
     xs_vec_d2f(s, n, c->jac_dinv, c->jac32);
   } else if (c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) {
-    if (c->F32) { (void)hipFree(c->F32); c->F32 = nullptr; }
+    // the low-precision buffers grow only: a refactorisation reuses them
+    auto grow = [](void *&buf, size_t &have, size_t need) -> int {
+      if (buf && have >= need) return XS_OK;
+      if (buf) (void)hipFree(buf);
+      buf = nullptr;
+      have = 0;
+      if (hipMalloc(&buf, need ? need : 1) != hipSuccess) { (void)hipGetLastError(); return XS_ERR_MEMORY; }
+      have = need;
+      return XS_OK;
+    };
     const size_t es = c->factor_type == XS_F32 ? 4 : 2;
-    if (hipMalloc(&c->F32, es * (size_t)c->ilu.nnz) != hipSuccess) return XS_ERR_MEMORY;
+    if ((rc = grow(c->F32, c->F32_bytes, es * (size_t)c->ilu.nnz))) return rc;
     xs_cvt_factor(s, c->ilu.nnz, c->ilu.lu, c->F32, c->factor_type);
     if ((rc = ensure_buf(&c->dinv32, (size_t)n))) return rc;
     xs_vec_d2f(s, n, c->ilu.dinv, c->dinv32);
@@ -374,8 +384,7 @@ static int mixed_prepare(xs_ctx *c) {
     if (c->tell_ready) {   // sweep copies in the factor storage type
       for (xs_tell *E : {&c->ilu.tl, &c->ilu.tu}) {
         if (E->W == 0) continue;
-        if (E->vlo) { (void)hipFree(E->vlo); E->vlo = nullptr; }
-        if (hipMalloc(&E->vlo, es * E->cap) != hipSuccess) return XS_ERR_MEMORY;
+        if ((rc = grow(E->vlo, E->vlo_bytes, es * E->cap))) return rc;
         xs_cvt_factor(s, E->W * E->ld, E->v, E->vlo, c->factor_type);
       }
       c->tell32_ready = true;

@@ -106,6 +106,7 @@ struct xs_ctx {
   bool tell32_ready = false;    // and their FP32 values (mixed inner solver)
   float *A32 = nullptr, *ell32 = nullptr, *dinv32 = nullptr, *jac32 = nullptr;
   void *F32 = nullptr;
+  size_t F32_bytes = 0;         // allocated bytes of F32
   float *work32 = nullptr;
   float *basis32 = nullptr;
   size_t basis32_len = 0;
