@@ -333,6 +333,40 @@ def test_mixed_copies_follow_options(dev):
     S.close()
 
 
+def test_parilut_exact_apply_after_sweeps(dev):
+    """ParILUT builds its SpTRSV row groups on the first exact solve: a
+    context that first solves with the sweep apply and then switches to the
+    exact apply gives bit for bit the solution of a context that used the
+    exact apply from the start."""
+    A = gen.lognormal27(12, seed=2)
+    xs, b = gen.manufactured(A, 4)
+
+    def ctx():
+        S = XSolver()
+        S.mat_setup(*A)
+        S.ksp_settype("fgmres")
+        S.ksp_setoption(1e-10, 500, 30)
+        S.pc_settype("ilut")
+        S.pc_setilut(1e-4, 2)
+        return S
+
+    S = ctx()
+    S.set_option("lsolve_sweeps", 10)
+    S.set_option("usolve_sweeps", 15)
+    xa, sta = S.solve(b)
+    S.set_option("lsolve_sweeps", -1)
+    S.set_option("usolve_sweeps", -1)
+    xb, stb = S.solve(b)
+    T = ctx()
+    xc, stc = T.solve(b)
+    say(f"[parilut-lazy] sweeps it={sta} exact-after it={S.iterations} fresh exact it={T.iterations}")
+    assert sta == 0 and stb == 0 and stc == 0
+    assert np.array_equal(xb, xc)
+    assert true_relres(A, b, xa) < 1.05e-10 and true_relres(A, b, xb) < 1.05e-10
+    S.close()
+    T.close()
+
+
 @pytest.mark.parametrize("factor", [0, 1], ids=["fp32", "bf16"])
 def test_mixed_gmres_parilut(dev, factor):
     """GMRES(30) + ParILUT in mixed precision (FP32 inner FGMRES, FP32/BF16
