@@ -119,7 +119,15 @@ struct xs_tsched {
   const int *order = nullptr;
   int ngroups = 0;
   int max_wg = 256;
+  int gather = 4;   // off-group entries a lane loads and polls together (4 or 16)
 };
+
+// Joint polling width of a factor's triangular solves: 16 off-group entries
+// at a time when a triangle holds more than 6 entries per row on average
+// (ParILUT fill), else 4 (wider gathers measured slower on 7-point factors).
+inline int xs_trsv_gather_width(long nnz, int n) {
+  return n > 0 && (nnz - (long)n) > 12L * n ? 16 : 4;
+}
 
 struct xs_ilu {
   int n = 0, nnz = 0;
@@ -167,6 +175,7 @@ inline xs_tsched xs_ilu_tsched(const xs_ilu &F, bool upper) {
   t.ngroups = upper ? F.ugroups : F.lgroups;
   t.order = upper ? F.usched : F.lsched;
   t.max_wg = F.trsv_wg;
+  t.gather = xs_trsv_gather_width(F.nnz, F.n);
   return t;
 }
 

@@ -976,7 +976,7 @@ __device__ __forceinline__ double trsv_gather(double acc, int e, int stop, int d
   return acc;
 }
 
-template <bool UPPER>
+template <bool UPPER, int G>
 __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict__ rp,
                                                    const int *__restrict__ ci,
                                                    const double *__restrict__ lu,
@@ -1031,9 +1031,9 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
         if (q < ee - eb) { iw_c[q] = ci[e]; iw_v[q] = lu[e]; }
       }
       if (!UPPER) {
-        for (int e = xb; e < xe; e += XS_GATHER) acc = trsv_gather<XS_GATHER>(acc, e, xe, 1, ci, lu, x, err);
+        for (int e = xb; e < xe; e += G) acc = trsv_gather<G>(acc, e, xe, 1, ci, lu, x, err);
       } else {
-        for (int e = xe - 1; e >= xb; e -= XS_GATHER) acc = trsv_gather<XS_GATHER>(acc, e, xb - 1, -1, ci, lu, x, err);
+        for (int e = xe - 1; e >= xb; e -= G) acc = trsv_gather<G>(acc, e, xb - 1, -1, ci, lu, x, err);
       }
     }
     const int nint = ee - eb;
@@ -1086,8 +1086,10 @@ static void trsv_launch(hipStream_t s, bool upper, int n, const int *rp, const i
   const int nw = S.gb ? S.ngroups : (n + XS_WAVE - 1) / XS_WAVE;
   const int tk = (nw + (XS_BLOCK / XS_WAVE) - 1) / (XS_BLOCK / XS_WAVE);
   dim3 grid(tk < S.max_wg ? tk : S.max_wg), blk(XS_BLOCK);
-  if (upper) hipLaunchKernelGGL(k_trsv<true>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, S, st);
-  else hipLaunchKernelGGL(k_trsv<false>, grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, S, st);
+#define XS_TRK(U, G) hipLaunchKernelGGL((k_trsv<U, G>), grid, blk, 0, s, n, rp, ci, lu, dpos, dinv, b, x, tickets, S, st)
+  if (S.gather == 16) { if (upper) XS_TRK(true, 16); else XS_TRK(false, 16); }
+  else { if (upper) XS_TRK(true, XS_GATHER); else XS_TRK(false, XS_GATHER); }
+#undef XS_TRK
 }
 
 void xs_trsv_lower(hipStream_t s, const xs_ilu &F, const double *b, double *x, unsigned *tickets,
@@ -1105,12 +1107,14 @@ void xs_trsv_lower_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
   xs_tsched S;
   S.max_wg = F.trsv_wg;
+  S.gather = xs_trsv_gather_width(F.nnz, F.n);
   trsv_launch(s, false, F.n, F.trp, F.tci, F.tlu, F.tdpos, F.dinv, b, x, tickets, S, st);
 }
 void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
   xs_tsched S;
   S.max_wg = F.trsv_wg;
+  S.gather = xs_trsv_gather_width(F.nnz, F.n);
   trsv_launch(s, true, F.n, F.trp, F.tci, F.tlu, F.tdpos, nullptr, b, x, tickets, S, st);
 }
 

@@ -360,7 +360,7 @@ __device__ __forceinline__ float trsv_gather_32(float acc, int e, int stop, int 
 
 // FP32 sync-free solve: the algorithm of k_trsv (xs_kernels.hip) with FP32
 // vectors and FP32/BF16/FP16 factor values.
-template <bool UPPER, class TF>
+template <bool UPPER, class TF, int G>
 __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restrict__ rp,
                                                      const int *__restrict__ ci,
                                                      const TF *__restrict__ lu,
@@ -413,9 +413,9 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
         if (q < ee - eb) { iw_c[q] = ci[e]; iw_v[q] = fl(lu, e); }
       }
       if (!UPPER) {
-        for (int e = xb; e < xe; e += XS_GATHER) acc = trsv_gather_32<XS_GATHER>(acc, e, xe, 1, ci, lu, x, err);
+        for (int e = xb; e < xe; e += G) acc = trsv_gather_32<G>(acc, e, xe, 1, ci, lu, x, err);
       } else {
-        for (int e = xe - 1; e >= xb; e -= XS_GATHER) acc = trsv_gather_32<XS_GATHER>(acc, e, xb - 1, -1, ci, lu, x, err);
+        for (int e = xe - 1; e >= xb; e -= G) acc = trsv_gather_32<G>(acc, e, xb - 1, -1, ci, lu, x, err);
       }
     }
     const int nint = ee - eb;
@@ -623,11 +623,13 @@ void xs_trsv32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, c
   const int nw = S.gb ? S.ngroups : (n + XS_WAVE - 1) / XS_WAVE;
   const int tk = (nw + 3) / 4;
   dim3 grid(tk < S.max_wg ? tk : S.max_wg), blk(XS_BLOCK);
-#define TRSV32(U, T) hipLaunchKernelGGL((k_trsv32<U, T>), grid, blk, 0, s, n, rp, ci, (const T *)lu, dpos, dinv, b, x, tickets, S, st)
+#define TRSV32G(U, T, G) hipLaunchKernelGGL((k_trsv32<U, T, G>), grid, blk, 0, s, n, rp, ci, (const T *)lu, dpos, dinv, b, x, tickets, S, st)
+#define TRSV32(U, T) do { if (S.gather == 16) TRSV32G(U, T, 16); else TRSV32G(U, T, XS_GATHER); } while (0)
   if (ftype == XS_F32) { if (upper) TRSV32(true, float); else TRSV32(false, float); }
   else if (ftype == XS_BF16) { if (upper) TRSV32(true, __hip_bfloat16); else TRSV32(false, __hip_bfloat16); }
   else { if (upper) TRSV32(true, __half); else TRSV32(false, __half); }
 #undef TRSV32
+#undef TRSV32G
 }
 
 // ---------------------------------------------------------------------------

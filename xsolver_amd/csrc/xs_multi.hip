@@ -840,16 +840,19 @@ void xs_mspmm(hipStream_t s, const xs_csr &A, int k, const double *X, double *Y,
 // launching no more than that leaves no partial second round of workgroups.
 static int resident_blocks(const void *kernel) {
   static std::mutex mu;
-  static std::unordered_map<const void *, int> cache;
-  std::lock_guard<std::mutex> lock(mu);
-  auto it = cache.find(kernel);
-  if (it != cache.end()) return it->second;
-  int dev = 0, cus = 0, per_cu = 0;
+  static std::unordered_map<unsigned long long, int> cache;   // (device, kernel)
+  int dev = 0;
   (void)hipGetDevice(&dev);
+  const unsigned long long key =
+      (reinterpret_cast<uintptr_t>(kernel) << 8) ^ static_cast<unsigned long long>(dev & 0xFF);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int cus = 0, per_cu = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, XS_BLOCK, 0);
   const int r = cus > 0 && per_cu > 0 ? cus * per_cu : XS_MGRID;
-  cache.emplace(kernel, r);
+  cache.emplace(key, r);
   return r;
 }
 
