@@ -69,6 +69,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
   dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);
+  dfree(F.app); dfree(F.apd);
   dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.vlo); dfree(F.tl.co); dfree(F.tl.ww);
   dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co); dfree(F.tu.ww);
   c->tell_ready = c->tell32_ready = false;
@@ -829,6 +830,9 @@ static int build_parilu_lists(xs_ctx *c) {
   if ((rc = dalloc(&F.act, (size_t)std::max(na, 1)))) return done2(rc);
   xs_compact_flagged(c->stream, nnz, flag, pos, F.act);
   F.nact = na;
+  if ((rc = dalloc(&F.app, (size_t)na + 1)) || (rc = dalloc(&F.apd, (size_t)std::max(na, 1))))
+    return done2(rc);
+  xs_parilu_active_lists(c->stream, na, nnz, F.act, F.pp, F.pd, F.app, F.apd);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return done2(XS_ERR_DEVICE);
   if (xs_take_launch_guard()) return done2(XS_ERR_STATE);
   return done2(XS_OK);
@@ -959,6 +963,7 @@ int xs_parilut_factor(xs_ctx *c) {
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own);
   dfree(F.ucp); dfree(F.uri); dfree(F.upos);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);   // ILU(0) lists only
+  dfree(F.app); dfree(F.apd);
   dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
   F.lgroups = F.ugroups = 0;
   F.sched_pending = false;

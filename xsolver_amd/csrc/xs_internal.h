@@ -149,6 +149,9 @@ struct xs_ilu {
   // entries a sweep can change (those with products, the pivots and the
   // entries below the diagonal); the others keep their start value
   int *act = nullptr;
+  // per active entry t (entry act[t]): its products pl/pu[app[t] .. app[t+1])
+  // and pivot code, so that a sweep reads them coalesced
+  int *app = nullptr, *apd = nullptr;
   int nact = 0;
   // SpTRSV work order: the 64-row groups of L (resp. U, counted from the
   // bottom) sorted by dependency level; ticket t solves group sched[t]
@@ -231,6 +234,10 @@ void xs_parilu_pairs(hipStream_t s, const xs_csr &A, const xs_ilu &F, int *cnt, 
 void xs_parilu_scaled(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
                       double *as);
 // one ParILU sweep over the update lists (same arithmetic as xs_parilu_sweep)
+// app[t] = pp[act[t]], apd[t] = pd[act[t]], app[nact] = pp[nnz] (the
+// entries between two active ones have no products)
+void xs_parilu_active_lists(hipStream_t s, int nact, int nnz, const int *act, const int *pp,
+                            const int *pd, int *app, int *apd);
 void xs_parilu_sweep_pairs(hipStream_t s, const xs_ilu &F, const double *as, const double *lu_in,
                            double *lu_out);
 // active-entry list: flag[k] = 1 for entries a sweep can change; compaction

@@ -840,13 +840,13 @@ void xs_parilu_scaled(hipStream_t s, const xs_csr &A, const xs_ilu &F, const dou
 // one is not finite.
 __global__ __launch_bounds__(XS_BLOCK) void k_parilu_sweep_pairs(
     int nact, const int *__restrict__ act, const double *__restrict__ as,
-    const int *__restrict__ pp, const int *__restrict__ pl, const int *__restrict__ pu,
-    const int *__restrict__ pd, const double *lu_in, double *lu_out) {
+    const int *__restrict__ app, const int *__restrict__ pl, const int *__restrict__ pu,
+    const int *__restrict__ apd, const double *lu_in, double *lu_out) {
   const int t = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
   if (t >= nact) return;
-  const int k = act[t];
+  // entry index, product range and pivot code: coalesced loads by t
+  const int k = act[t], e0 = app[t], e1 = app[t + 1], d = apd[t];
   double s = as[k];
-  const int e0 = pp[k], e1 = pp[k + 1], d = pd[k];
   for (int e = e0; e < e1; ++e) s -= lu_in[pl[e]] * lu_in[pu[e]];
   double nv;
   if (d >= 0) {
@@ -862,10 +862,27 @@ __global__ __launch_bounds__(XS_BLOCK) void k_parilu_sweep_pairs(
 
 void xs_parilu_sweep_pairs(hipStream_t s, const xs_ilu &F, const double *as, const double *lu_in,
                            double *lu_out) {
-  XS_GUARD(F.pp && F.pd && F.pl && F.pu && (F.act || F.nact == 0) && as && lu_in && lu_out);
+  XS_GUARD(F.app && F.apd && F.pl && F.pu && (F.act || F.nact == 0) && as && lu_in && lu_out);
   if (F.nact == 0) return;
   hipLaunchKernelGGL(k_parilu_sweep_pairs, dim3((F.nact + XS_BLOCK - 1) / XS_BLOCK), dim3(XS_BLOCK),
-                     0, s, F.nact, F.act, as, F.pp, F.pl, F.pu, F.pd, lu_in, lu_out);
+                     0, s, F.nact, F.act, as, F.app, F.pl, F.pu, F.apd, lu_in, lu_out);
+}
+
+__global__ void k_parilu_active_lists(int nact, int nnz, const int *__restrict__ act,
+                                      const int *__restrict__ pp, const int *__restrict__ pd,
+                                      int *__restrict__ app, int *__restrict__ apd) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nact) {
+    app[t] = pp[act[t]];
+    apd[t] = pd[act[t]];
+  }
+  if (t == 0) app[nact] = pp[nnz];
+}
+void xs_parilu_active_lists(hipStream_t s, int nact, int nnz, const int *act, const int *pp,
+                            const int *pd, int *app, int *apd) {
+  XS_GUARD(pp && pd && app && apd && (act || nact == 0));
+  hipLaunchKernelGGL(k_parilu_active_lists, dim3((std::max(nact, 1) + 255) / 256), dim3(256), 0, s,
+                     nact, nnz, act, pp, pd, app, apd);
 }
 
 __global__ void k_parilu_active_flags(int nnz, const int *__restrict__ cnt,
