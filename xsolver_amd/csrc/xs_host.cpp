@@ -71,7 +71,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);
   dfree(F.app); dfree(F.apd);
   dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.vlo); dfree(F.tl.co); dfree(F.tl.ww);
-  dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co); dfree(F.tu.ww);
+  dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co); dfree(F.tu.ww); dfree(F.tu.vs);
   c->tell_ready = c->tell32_ready = false;
   F = xs_ilu();
   F.trsv_wg = c->trsv_wg;
@@ -1220,10 +1220,11 @@ static int ensure_tri_ell(xs_ctx *c) {
       const int ld = (F.n + 1) & ~1;   // even: the FP32 sweeps load row pairs
       const size_t need = (size_t)w[part] * ld;
       if (need > E.cap) {
-        dfree(E.ci); dfree(E.v); dfree(E.vlo); dfree(E.co);
+        dfree(E.ci); dfree(E.v); dfree(E.vlo); dfree(E.co); dfree(E.vs);
         E.vlo_bytes = 0;
         E.cap = 0;
         if ((rc = dalloc(&E.ci, need)) || (rc = dalloc(&E.v, need))) return rc;
+        if (part == 1 && (rc = dalloc(&E.vs, need))) return rc;
         E.cap = need;
       }
       E.W = w[part];
@@ -1231,6 +1232,7 @@ static int ensure_tri_ell(xs_ctx *c) {
     }
   }
   for (int part = 0; part < 2; ++part) xs_build_tell(c->stream, F, part == 1, part ? F.tu : F.tl);
+  xs_tell_scale_cols(c->stream, F.tu, F.dinv);   // U's values for the fused first two sweeps
   if (!F.tell_shape) {   // offsets and group widths depend on the pattern only
     for (int part = 0; part < 2; ++part) {
       xs_tell &E = part ? F.tu : F.tl;
@@ -1250,8 +1252,9 @@ static int ensure_tri_ell(xs_ctx *c) {
 }
 
 // k >= 2 sweeps over the ELL copy: sweeps 1 and 2 fused, then k - 2 more,
-// ping-ponging so that the last lands in out (identical results to
-// ilu_sweeps)
+// ping-ponging so that the last lands in out (the results of ilu_sweeps, to
+// rounding: the fused U pass multiplies by u_ij / u_jj where ilu_sweeps
+// multiplies u_ij by (b_j / u_jj); L and the later passes are identical)
 static void ilu_sweeps_ell(xs_ctx *c, bool upper, int k, const double *b, double *out,
                            double *w, const xs_state *st) {
   double *cur = ((k - 2) % 2 == 0) ? out : w;

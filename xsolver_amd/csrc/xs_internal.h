@@ -108,6 +108,8 @@ struct xs_tell {
   short *co = nullptr;       // column - row as 16 bits, XS_OFF16_PAD = padding
   int *ww = nullptr;         // per 64-row group (rows 64g ..): its longest row, the
                              // slab count the sweeps walk for those rows
+  double *vs = nullptr;      // U only: u_ij / u_jj, the values of the first two
+                             // sweeps in one pass (no gather of 1/u_jj)
 };
 
 // Work order of a sync-free triangular solve: the row groups (gb: ngroups + 1
@@ -263,6 +265,8 @@ void xs_tri_sweep(hipStream_t s, const xs_ilu &F, bool upper, const double *b, c
 // returns the widths of L and U (synchronous; nonzero on failure).
 int xs_tri_widths(hipStream_t s, const xs_ilu &F, int *d_tmp, int *wl, int *wu);
 void xs_build_tell(hipStream_t s, const xs_ilu &F, bool upper, xs_tell &E);
+// E.vs = E.v with every entry divided by the pivot of its column (dinv)
+void xs_tell_scale_cols(hipStream_t s, const xs_tell &E, const double *dinv);
 // E.ww[g] = the longest strictly lower (upper) row among rows 64g .. 64g + 63
 void xs_tri_wave_widths(hipStream_t s, const xs_ilu &F, bool upper, int *ww);
 void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, const double *b,

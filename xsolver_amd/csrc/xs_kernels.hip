@@ -1136,9 +1136,10 @@ void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
 }
 
 // ---------------------------------------------------------------------------
-// Jacobi sweeps over ELL copies of the factor parts (same arithmetic as
-// k_tri_sweep, entry by entry in column order, so the results are identical;
-// coalesced slab loads instead of a thread walking its CSR row).
+// Jacobi sweeps over ELL copies of the factor parts (the arithmetic of
+// k_tri_sweep, entry by entry in column order, so the results are identical
+// except in the fused U pass, which reads u_ij / u_jj (to rounding); coalesced
+// slab loads instead of a thread walking its CSR row).
 // ---------------------------------------------------------------------------
 // strictly-lower / strictly-upper entry counts: maxima into w[0], w[1]
 __global__ __launch_bounds__(XS_BLOCK) void k_tri_widths(int n, const int *__restrict__ rp,
@@ -1215,6 +1216,24 @@ void xs_tri_wave_widths(hipStream_t s, const xs_ilu &F, bool upper, int *ww) {
   else hipLaunchKernelGGL(k_tri_wave_widths<false>, grid, blk, 0, s, F.n, F.rp, F.dpos, ww);
 }
 
+__global__ void k_tell_scale_cols(int W, int ld, const int *__restrict__ eci,
+                                  const double *__restrict__ ev, const double *__restrict__ dinv,
+                                  double *__restrict__ vs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ld) return;
+  for (int k = 0; k < W; ++k) {
+    const size_t p = (size_t)k * ld + i;
+    const int c = eci[p];
+    vs[p] = c >= 0 ? ev[p] * dinv[c] : 0.0;
+  }
+}
+void xs_tell_scale_cols(hipStream_t s, const xs_tell &E, const double *dinv) {
+  XS_GUARD((E.W == 0 || (E.ci && E.v && E.vs)) && dinv);
+  if (E.W == 0) return;
+  hipLaunchKernelGGL(k_tell_scale_cols, dim3((E.ld + 255) / 256), dim3(256), 0, s, E.W, E.ld, E.ci,
+                     E.v, dinv, E.vs);
+}
+
 // yn = D (b - T yo), T given by its ELL slabs; FUSED: yo = b (L) or D b (U),
 // i.e. sweeps 1 and 2 from y = 0 in one pass.  Slabs four at a time: all
 // index/value loads of a batch, then all gathers.  Rows in 256-row blocks
@@ -1255,7 +1274,9 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if (c[q] >= 0) {
-        if (FUSED) y[q] = UPPER ? dinv[c[q]] * (b[c[q]] - 0.0) : b[c[q]] - 0.0;
+        // FUSED: the previous iterate is D b (U) or b (L); for U the slab
+        // values are u_ij / u_jj (E.vs), so b is gathered alone
+        if (FUSED) y[q] = b[c[q]] - 0.0;
         else y[q] = yo[c[q]];
       } else {
         y[q] = 0.0;
@@ -1274,7 +1295,9 @@ void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, co
   XS_GUARD((E.W == 0 || (E.ci && E.v)) && E.ld >= F.n && b && yn && (!upper || F.dinv) &&
            (fused || yo));
   const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
-#define XS_SWK(U, FU, O) hipLaunchKernelGGL((k_tri_sweep_ell<U, FU, O>), grid, blk, 0, s, F.n, E.ld, E.W, E.ci, E.co, E.v, E.ww, F.dinv, b, yo, yn, st)
+  XS_GUARD(!(upper && fused) || E.W == 0 || E.vs);
+  const double *ev = (upper && fused) ? E.vs : E.v;
+#define XS_SWK(U, FU, O) hipLaunchKernelGGL((k_tri_sweep_ell<U, FU, O>), grid, blk, 0, s, F.n, E.ld, E.W, E.ci, E.co, ev, E.ww, F.dinv, b, yo, yn, st)
 #define XS_SWE(U, FU) do { if (E.co) XS_SWK(U, FU, true); else XS_SWK(U, FU, false); } while (0)
   if (upper) { if (fused) XS_SWE(true, true); else XS_SWE(true, false); }
   else { if (fused) XS_SWE(false, true); else XS_SWE(false, false); }

@@ -785,10 +785,9 @@ __global__ __launch_bounds__(XS_BLOCK) void k_msweep_ell(int n, int k, int ld, i
       for (int q = 0; q < 4; ++q) {
         if (col[q] >= 0) {
           load(FUSED ? B : Yo, col[q], y[q]);
-          if (FUSED) {
-            const double dc = UPPER ? dinv[col[q]] : 1.0;
+          if (FUSED) {   // U: the slab values are u_ij / u_jj (E.vs)
 #pragma unroll
-            for (int c = 0; c < CW; ++c) y[q][c] = UPPER ? dc * (y[q][c] - 0.0) : y[q][c] - 0.0;
+            for (int c = 0; c < CW; ++c) y[q][c] = y[q][c] - 0.0;
           }
         } else {
 #pragma unroll
@@ -959,6 +958,8 @@ void xs_msweep_ell(hipStream_t s, const xs_tell &E, int n, int k, bool upper, bo
                    const xs_mstate *ms) {
   XS_GUARD((E.W == 0 || (E.ci && E.v)) && E.ld >= n && k_ok(k) && B && Yn && (!upper || dinv) &&
            (fused || Yo));
+  XS_GUARD(!(upper && fused) || E.W == 0 || E.vs);
+  const double *ev = (upper && fused) ? E.vs : E.v;
   // columns per thread: 4 or 2 when k allows and the blocks are 16-byte aligned
   const bool al = ((reinterpret_cast<uintptr_t>(B) | reinterpret_cast<uintptr_t>(Yo) |
                     reinterpret_cast<uintptr_t>(Yn)) & 15) == 0;
@@ -968,7 +969,7 @@ void xs_msweep_ell(hipStream_t s, const xs_tell &E, int n, int k, bool upper, bo
 #define XS_MSK(U, FU, O, C)                                                                    \
   hipLaunchKernelGGL((k_msweep_ell<U, FU, O, C>),                                             \
                      ell_grid(reinterpret_cast<const void *>(&k_msweep_ell<U, FU, O, C>), n, per), \
-                     blk, 0, s, n, k, E.ld, E.W, E.ci, E.co, E.v, E.ww, dinv, B, Yo, Yn, ms)
+                     blk, 0, s, n, k, E.ld, E.W, E.ci, E.co, ev, E.ww, dinv, B, Yo, Yn, ms)
 #define XS_MSC(U, FU, O) do { if (cw == 4) XS_MSK(U, FU, O, 4); else if (cw == 2) XS_MSK(U, FU, O, 2); else XS_MSK(U, FU, O, 1); } while (0)
 #define XS_MSE(U, FU) do { if (E.co) XS_MSC(U, FU, true); else XS_MSC(U, FU, false); } while (0)
   if (upper) { if (fused) XS_MSE(true, true); else XS_MSE(true, false); }
