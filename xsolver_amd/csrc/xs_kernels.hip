@@ -232,25 +232,24 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell(
     }
     double s0 = 0.0, s1 = 0.0;
     for (int k0 = 0; k0 < W; k0 += XS_ELL_UK) {
+      // branch-free loads (a slot past W re-reads slab W - 1, masked from the
+      // sums) so that the batch's loads are all issued before any wait
       int2 c[XS_ELL_UK];
       double2 a[XS_ELL_UK];
 #pragma unroll
       for (int q = 0; q < XS_ELL_UK; ++q) {
-        if (k0 + q < W) {
-          if (O16) {   // 16-bit offsets: 4 bytes of indices for the two rows
-            const short2 o = *reinterpret_cast<const short2 *>(co + (size_t)(k0 + q) * ld + r);
-            c[q] = make_int2(r + o.x, r + 1 + o.y);
-          } else {
-            c[q] = *reinterpret_cast<const int2 *>(ci + (size_t)(k0 + q) * ld + r);
-          }
-          a[q] = *reinterpret_cast<const double2 *>(v + (size_t)(k0 + q) * ld + r);
+        const size_t p = (size_t)min(k0 + q, W - 1) * ld + r;
+        if (O16) {   // 16-bit offsets: 4 bytes of indices for the two rows
+          const short2 o = *reinterpret_cast<const short2 *>(co + p);
+          c[q] = make_int2(r + o.x, r + 1 + o.y);
+        } else {
+          c[q] = *reinterpret_cast<const int2 *>(ci + p);
         }
+        a[q] = *reinterpret_cast<const double2 *>(v + p);
       }
       double x0[XS_ELL_UK], x1[XS_ELL_UK];
 #pragma unroll
-      for (int q = 0; q < XS_ELL_UK; ++q) {
-        if (k0 + q < W) { x0[q] = x[c[q].x]; x1[q] = x[c[q].y]; }
-      }
+      for (int q = 0; q < XS_ELL_UK; ++q) { x0[q] = x[c[q].x]; x1[q] = x[c[q].y]; }
 #pragma unroll
       for (int q = 0; q < XS_ELL_UK; ++q) {
         if (k0 + q < W) { s0 += a[q].x * x0[q]; s1 += a[q].y * x1[q]; }
@@ -1259,28 +1258,28 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W
   W = ww ? ww[i / XS_WAVE] : W;
   double r = 0.0;
   for (int k0 = 0; k0 < W; k0 += 4) {
+    // branch-free loads: a slot past W re-reads slab W - 1 and a padding
+    // entry gathers the row's own value, both masked from the sum, so that
+    // the batch's loads are all issued before any wait
     int c[4];
     double a[4], y[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
+      const size_t p = (size_t)min(k0 + q, W - 1) * ld + i;
       if (O16) {
-        const int o = k0 + q < W ? (int)eco[(size_t)(k0 + q) * ld + i] : XS_OFF16_PAD;
-        c[q] = o == XS_OFF16_PAD ? -1 : i + o;
+        const int o = (int)eco[p];
+        c[q] = (k0 + q >= W || o == XS_OFF16_PAD) ? -1 : i + o;
       } else {
-        c[q] = k0 + q < W ? eci[(size_t)(k0 + q) * ld + i] : -1;
+        c[q] = k0 + q < W ? eci[p] : -1;
       }
-      a[q] = c[q] >= 0 ? ev[(size_t)(k0 + q) * ld + i] : 0.0;
+      a[q] = ev[p];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (c[q] >= 0) {
-        // FUSED: the previous iterate is D b (U) or b (L); for U the slab
-        // values are u_ij / u_jj (E.vs), so b is gathered alone
-        if (FUSED) y[q] = b[c[q]] - 0.0;
-        else y[q] = yo[c[q]];
-      } else {
-        y[q] = 0.0;
-      }
+      // FUSED: the previous iterate is D b (U) or b (L); for U the slab
+      // values are u_ij / u_jj (E.vs), so b is gathered alone
+      const int cc = c[q] >= 0 ? c[q] : i;
+      y[q] = FUSED ? b[cc] - 0.0 : yo[cc];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q)
