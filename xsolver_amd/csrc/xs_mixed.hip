@@ -202,25 +202,24 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell32(
     float s0 = 0.f, s1 = 0.f;
     constexpr int UK = 4;   // slabs per batch: all loads of a batch, then all gathers
     for (int k0 = 0; k0 < W; k0 += UK) {
+      // branch-free loads (a slot past W re-reads slab W - 1, masked from the
+      // sums) so that the batch's loads are all issued before any wait
       int2 c[UK];
       float2 a[UK];
 #pragma unroll
       for (int q = 0; q < UK; ++q) {
-        if (k0 + q < W) {
-          if (O16) {
-            const short2 o = *reinterpret_cast<const short2 *>(co + (size_t)(k0 + q) * ld + r);
-            c[q] = make_int2(r + o.x, r + 1 + o.y);
-          } else {
-            c[q] = *reinterpret_cast<const int2 *>(ci + (size_t)(k0 + q) * ld + r);
-          }
-          a[q] = *reinterpret_cast<const float2 *>(v + (size_t)(k0 + q) * ld + r);
+        const size_t p = (size_t)min(k0 + q, W - 1) * ld + r;
+        if (O16) {
+          const short2 o = *reinterpret_cast<const short2 *>(co + p);
+          c[q] = make_int2(r + o.x, r + 1 + o.y);
+        } else {
+          c[q] = *reinterpret_cast<const int2 *>(ci + p);
         }
+        a[q] = *reinterpret_cast<const float2 *>(v + p);
       }
       float x0[UK], x1[UK];
 #pragma unroll
-      for (int q = 0; q < UK; ++q) {
-        if (k0 + q < W) { x0[q] = x[c[q].x]; x1[q] = x[c[q].y]; }
-      }
+      for (int q = 0; q < UK; ++q) { x0[q] = x[c[q].x]; x1[q] = x[c[q].y]; }
 #pragma unroll
       for (int q = 0; q < UK; ++q) {
         if (k0 + q < W) { s0 += a[q].x * x0[q]; s1 += a[q].y * x1[q]; }
@@ -538,36 +537,35 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
   for (int k0 = 0; k0 < W; k0 += 4) {
     int c0[4], c1[4];
     float a0[4], a1[4], y0[4], y1[4];
+    // branch-free loads: a slot past W re-reads slab W - 1 and a padding
+    // entry gathers the row's own values, all masked from the sums, so that
+    // the batch's loads are issued before any wait
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      c0[q] = c1[q] = -1;
-      a0[q] = a1[q] = 0.f;
-      if (k0 + q < W) {
-        const size_t p = (size_t)(k0 + q) * ld + r;
-        if (O16) {
-          const short2 o = *reinterpret_cast<const short2 *>(eco + p);
-          c0[q] = o.x == XS_OFF16_PAD ? -1 : r + o.x;
-          c1[q] = (o.y == XS_OFF16_PAD || !two) ? -1 : r + 1 + o.y;
-        } else {
-          const int2 cc = *reinterpret_cast<const int2 *>(eci + p);
-          c0[q] = cc.x;
-          c1[q] = two ? cc.y : -1;
-        }
-        const float2 av = fl2(ev, p);
-        a0[q] = c0[q] >= 0 ? av.x : 0.f;
-        a1[q] = c1[q] >= 0 ? av.y : 0.f;
+      const bool in = k0 + q < W;
+      const size_t p = (size_t)min(k0 + q, W - 1) * ld + r;
+      if (O16) {
+        const short2 o = *reinterpret_cast<const short2 *>(eco + p);
+        c0[q] = (!in || o.x == XS_OFF16_PAD) ? -1 : r + o.x;
+        c1[q] = (!in || o.y == XS_OFF16_PAD || !two) ? -1 : r + 1 + o.y;
+      } else {
+        const int2 cc = *reinterpret_cast<const int2 *>(eci + p);
+        c0[q] = in ? cc.x : -1;
+        c1[q] = (in && two) ? cc.y : -1;
       }
+      const float2 av = fl2(ev, p);
+      a0[q] = av.x;
+      a1[q] = av.y;
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      y0[q] = y1[q] = 0.f;
-      if (c0[q] >= 0) {
-        if (FUSED) y0[q] = UPPER ? dinv[c0[q]] * (b[c0[q]] - 0.f) : b[c0[q]] - 0.f;
-        else y0[q] = yo[c0[q]];
-      }
-      if (c1[q] >= 0) {
-        if (FUSED) y1[q] = UPPER ? dinv[c1[q]] * (b[c1[q]] - 0.f) : b[c1[q]] - 0.f;
-        else y1[q] = yo[c1[q]];
+      const int g0 = c0[q] >= 0 ? c0[q] : r, g1 = c1[q] >= 0 ? c1[q] : r;
+      if (FUSED) {
+        y0[q] = UPPER ? dinv[g0] * (b[g0] - 0.f) : b[g0] - 0.f;
+        y1[q] = UPPER ? dinv[g1] * (b[g1] - 0.f) : b[g1] - 0.f;
+      } else {
+        y0[q] = yo[g0];
+        y1[q] = yo[g1];
       }
     }
 #pragma unroll
