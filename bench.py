@@ -125,8 +125,8 @@ PROFILE_NAMES = {
                         ("k_tri_sweep_ell<false, false, true>", APPLY[0] - 2)],
     "ilu_upper_apply": [("k_tri_sweep_ell<true, true, true>", 1),
                         ("k_tri_sweep_ell<true, false, true>", APPLY[1] - 2)],
-    "trsv_lower_exact": [("k_trsv<false>", 1)],
-    "trsv_upper_exact": [("k_trsv<true>", 1)],
+    "trsv_lower_exact": [("k_trsv<false, 4>", 1)],
+    "trsv_upper_exact": [("k_trsv<true, 4>", 1)],
 }
 
 
