@@ -959,15 +959,19 @@ __device__ __forceinline__ double trsv_gather(double acc, int e, int stop, int d
   unsigned long long u[G];
   int c[G];
   double a[G];
+  // branch-free loads (entry e is in range: a slot past the stop re-reads
+  // it and is masked by c = -1), so that they are all in flight at once
 #pragma unroll
   for (int q = 0; q < G; ++q) {
     const int eq = e + q * dir;
     const bool in = dir > 0 ? eq < stop : eq > stop;
-    c[q] = in ? ci[eq] : -1;
-    a[q] = in ? lu[eq] : 0.0;
+    const int cv = ci[in ? eq : e];
+    const double av = lu[in ? eq : e];
+    c[q] = in ? cv : -1;
+    a[q] = in ? av : 0.0;
   }
 #pragma unroll
-  for (int q = 0; q < G; ++q) u[q] = c[q] >= 0 ? ld_relaxed(x + c[q]) : 0ull;
+  for (int q = 0; q < G; ++q) u[q] = ld_relaxed(x + (c[q] >= 0 ? c[q] : c[0]));
   unsigned polls = 0;
   for (;;) {
     bool pend = false;
