@@ -69,7 +69,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
   dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);
-  dfree(F.app); dfree(F.apd);
+  dfree(F.app); dfree(F.apd); dfree(F.cmap); dfree(F.lc);
   dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.vlo); dfree(F.tl.co); dfree(F.tl.ww);
   dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co); dfree(F.tu.ww); dfree(F.tu.vs);
   c->tell_ready = c->tell32_ready = false;
@@ -830,9 +830,14 @@ static int build_parilu_lists(xs_ctx *c) {
   if ((rc = dalloc(&F.act, (size_t)std::max(na, 1)))) return done2(rc);
   xs_compact_flagged(c->stream, nnz, flag, pos, F.act);
   F.nact = na;
-  if ((rc = dalloc(&F.app, (size_t)na + 1)) || (rc = dalloc(&F.apd, (size_t)std::max(na, 1))))
+  if ((rc = dalloc(&F.app, (size_t)na + 1)) || (rc = dalloc(&F.apd, (size_t)std::max(na, 1))) ||
+      (rc = dalloc(&F.cmap, nnz)))
     return done2(rc);
-  xs_parilu_active_lists(c->stream, na, nnz, F.act, F.pp, F.pd, F.app, F.apd);
+  // sweep order: the lists' factor and pivot positions point into it
+  xs_parilu_cmap(c->stream, nnz, na, flag, pos, F.cmap);
+  xs_parilu_active_lists(c->stream, na, nnz, F.act, F.pp, F.pd, F.cmap, F.app, F.apd);
+  xs_remap_idx(c->stream, np, F.cmap, F.pl);
+  xs_remap_idx(c->stream, np, F.cmap, F.pu);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return done2(XS_ERR_DEVICE);
   if (xs_take_launch_guard()) return done2(XS_ERR_STATE);
   return done2(XS_OK);
@@ -907,28 +912,19 @@ static int factor_parilu(xs_ctx *c) {
     sc = F.scale;
   }
   if (F.pp) {
-    // start values: the (scaled) matrix, in both buffers of a deterministic
-    // run; entries off the active list keep them
-    const double *a0 = c->A.v;
-    if (sc) {
-      if (!F.as) {
-        int rc = dalloc(&F.as, c->A.nnz);
-        if (rc) return rc;
-      }
-      xs_parilu_scaled(c->stream, c->A, F, sc, F.as);
-      a0 = F.as;
-    }
-    const size_t bytes = sizeof(double) * c->A.nnz;
-    XS_HIP(hipMemcpyAsync(F.lu, a0, bytes, hipMemcpyDeviceToDevice, c->stream));
-    if (!c->parilu_async) XS_HIP(hipMemcpyAsync(c->lu_tmp, a0, bytes, hipMemcpyDeviceToDevice, c->stream));
+    // start values: the (scaled) matrix in sweep order, in both buffers of a
+    // deterministic run; entries off the active list keep them.  The last
+    // iterate goes back to CSR order, scaled back, in one pass.
+    int rc;
+    if ((!F.as && (rc = dalloc(&F.as, c->A.nnz))) || (!F.lc && (rc = dalloc(&F.lc, c->A.nnz))))
+      return rc;
+    double *cur = F.lc, *nxt = c->parilu_async ? F.lc : c->lu_tmp;
+    xs_parilu_start(c->stream, c->A, F, sc, c->parilu_async ? nullptr : c->lu_tmp);
     for (int s = 0; s < c->parilu_sweeps; s++) {
-      if (c->parilu_async) {
-        xs_parilu_sweep_pairs(c->stream, F, a0, F.lu, F.lu);
-      } else {
-        xs_parilu_sweep_pairs(c->stream, F, a0, F.lu, c->lu_tmp);
-        std::swap(F.lu, c->lu_tmp);
-      }
+      xs_parilu_sweep_pairs(c->stream, F, cur, nxt);
+      std::swap(cur, nxt);
     }
+    xs_parilu_finish(c->stream, c->A, F, sc, cur);
   } else {
     xs_ilu_copy_values(c->stream, c->A, F);
     if (sc) xs_pt_rescale(c->stream, c->A.nnz, F.rowidx, c->A.ci, F.lu, F.scale, 0);
@@ -940,8 +936,8 @@ static int factor_parilu(xs_ctx *c) {
         std::swap(F.lu, c->lu_tmp);
       }
     }
+    if (sc) xs_pt_rescale(c->stream, c->A.nnz, F.rowidx, c->A.ci, F.lu, F.scale, 1);
   }
-  if (sc) xs_pt_rescale(c->stream, c->A.nnz, F.rowidx, c->A.ci, F.lu, F.scale, 1);
   XS_HIP(hipMemsetAsync(c->d_int, 0x7f, sizeof(int), c->stream));
   xs_ilu_dinv(c->stream, F, c->d_int);
   int bad = 0;
@@ -963,7 +959,7 @@ int xs_parilut_factor(xs_ctx *c) {
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own);
   dfree(F.ucp); dfree(F.uri); dfree(F.upos);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);   // ILU(0) lists only
-  dfree(F.app); dfree(F.apd);
+  dfree(F.app); dfree(F.apd); dfree(F.cmap); dfree(F.lc);
   dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
   F.lgroups = F.ugroups = 0;
   F.sched_pending = false;

@@ -145,9 +145,8 @@ struct xs_ilu {
   // ParILU(0) update lists (no reference counterpart): entry k's products
   // l_ik' u_k'j are lu[pl[e]] * lu[pu[e]] for e in [pp[k], pp[k+1]) in
   // ascending k'; pd[k] = position of u_jj below the diagonal, -2 on it, -1
-  // above.  as = the (scaled) matrix values the sweeps start from.
+  // above.
   int *pp = nullptr, *pl = nullptr, *pu = nullptr, *pd = nullptr;
-  double *as = nullptr;
   // entries a sweep can change (those with products, the pivots and the
   // entries below the diagonal); the others keep their start value
   int *act = nullptr;
@@ -155,6 +154,12 @@ struct xs_ilu {
   // and pivot code, so that a sweep reads them coalesced
   int *app = nullptr, *apd = nullptr;
   int nact = 0;
+  // The sweeps run on the values in sweep order: entry k at cmap[k], the
+  // active entries first (act[t] at t), the others after them in CSR order.
+  // pl, pu and apd hold sweep-order positions; as = the (scaled) start
+  // values and lc the iterate, both in sweep order.
+  int *cmap = nullptr;
+  double *as = nullptr, *lc = nullptr;
   // SpTRSV work order: the 64-row groups of L (resp. U, counted from the
   // bottom) sorted by dependency level; ticket t solves group sched[t]
   int *lsched = nullptr, *usched = nullptr;
@@ -233,15 +238,24 @@ void xs_ilu_copy_values(hipStream_t s, const xs_csr &A, xs_ilu &F);
 // fill (pl, pu at offsets pp)
 void xs_parilu_pairs(hipStream_t s, const xs_csr &A, const xs_ilu &F, int *cnt, const int *pp,
                      int *pl, int *pu, int *pd);
-void xs_parilu_scaled(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
-                      double *as);
-// one ParILU sweep over the update lists (same arithmetic as xs_parilu_sweep)
-// app[t] = pp[act[t]], apd[t] = pd[act[t]], app[nact] = pp[nnz] (the
-// entries between two active ones have no products)
+// sweep order: cmap[k] = pos[k] for a flagged entry, nact + k - pos[k]
+// otherwise (pos = exclusive scan of flag); idx[e] <- cmap[idx[e]]
+void xs_parilu_cmap(hipStream_t s, int nnz, int nact, const int *flag, const int *pos, int *cmap);
+void xs_remap_idx(hipStream_t s, int n, const int *cmap, int *idx);
+// start values a_k (times s_i s_j with sc) in sweep order into F.as, F.lc
+// and lc2 (when given)
+void xs_parilu_start(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
+                     double *lc2);
+// F.lu[k] = lc[cmap[k]], scaled back with sc (as xs_pt_rescale with unscale)
+void xs_parilu_finish(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
+                      const double *lc);
+// one ParILU sweep over the update lists in sweep order (same arithmetic as
+// xs_parilu_sweep).  app[t] = pp[act[t]], apd[t] = cmap[pd[act[t]]] (codes
+// -1 / -2 kept), app[nact] = pp[nnz] (the entries between two active ones
+// have no products)
 void xs_parilu_active_lists(hipStream_t s, int nact, int nnz, const int *act, const int *pp,
-                            const int *pd, int *app, int *apd);
-void xs_parilu_sweep_pairs(hipStream_t s, const xs_ilu &F, const double *as, const double *lu_in,
-                           double *lu_out);
+                            const int *pd, const int *cmap, int *app, int *apd);
+void xs_parilu_sweep_pairs(hipStream_t s, const xs_ilu &F, const double *lc_in, double *lc_out);
 // active-entry list: flag[k] = 1 for entries a sweep can change; compaction
 // of the flagged indices at their scanned positions
 void xs_parilu_active_flags(hipStream_t s, int nnz, const int *cnt, const int *pd, int *flag);

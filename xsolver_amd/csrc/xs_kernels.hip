@@ -816,72 +816,124 @@ void xs_parilu_pairs(hipStream_t s, const xs_csr &A, const xs_ilu &F, int *cnt, 
   }
 }
 
-// as[k] = a_k s_i s_j, the starting values of the scaled sweeps
-__global__ void k_parilu_scaled(int nnz, const int *__restrict__ rowidx,
-                                const int *__restrict__ ci, const double *__restrict__ a,
-                                const double *__restrict__ sc, double *__restrict__ as) {
+// start values in sweep order: a_k s_i s_j (a_k without sc) at cmap[k] of
+// as, lc and lc2
+__global__ void k_parilu_start(int nnz, const int *__restrict__ rowidx,
+                               const int *__restrict__ ci, const double *__restrict__ a,
+                               const double *__restrict__ sc, const int *__restrict__ cmap,
+                               double *__restrict__ as, double *__restrict__ lc,
+                               double *__restrict__ lc2) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
-  as[k] = a[k] * sc[rowidx[k]] * sc[ci[k]];
+  const double v = sc ? a[k] * sc[rowidx[k]] * sc[ci[k]] : a[k];
+  const int p = cmap[k];
+  as[p] = v;
+  lc[p] = v;
+  if (lc2) lc2[p] = v;
 }
 
-void xs_parilu_scaled(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
-                      double *as) {
-  XS_GUARD(A.ci && A.v && F.rowidx && sc && as);
-  hipLaunchKernelGGL(k_parilu_scaled, dim3((A.nnz + 255) / 256), dim3(256), 0, s, A.nnz,
-                     F.rowidx, A.ci, A.v, sc, as);
+void xs_parilu_start(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
+                     double *lc2) {
+  XS_GUARD(A.ci && A.v && F.rowidx && F.cmap && F.as && F.lc);
+  hipLaunchKernelGGL(k_parilu_start, dim3((A.nnz + 255) / 256), dim3(256), 0, s, A.nnz,
+                     F.rowidx, A.ci, A.v, sc, F.cmap, F.as, F.lc, lc2);
+}
+
+// factor values back in CSR order, scaled back as k_pt_rescale does
+__global__ void k_parilu_finish(int nnz, const int *__restrict__ rowidx,
+                                const int *__restrict__ ci, const double *__restrict__ sc,
+                                const int *__restrict__ cmap, const double *__restrict__ lc,
+                                double *__restrict__ lu) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  double v = lc[cmap[k]];
+  if (sc) {
+    const int i = rowidx[k], j = ci[k];
+    if (j < i) v *= sc[j] / sc[i];
+    else v /= sc[i] * sc[j];
+  }
+  lu[k] = v;
+}
+
+void xs_parilu_finish(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
+                      const double *lc) {
+  XS_GUARD(A.ci && F.rowidx && F.cmap && F.lu && lc);
+  hipLaunchKernelGGL(k_parilu_finish, dim3((A.nnz + 255) / 256), dim3(256), 0, s, A.nnz,
+                     F.rowidx, A.ci, sc, F.cmap, lc, F.lu);
 }
 
 // One sweep over the update lists: the arithmetic of k_parilu_sweep (same
 // products in the same order, same pivot guard), without the merge, over the
 // active entries only (an entry above the diagonal without products is its
-// start value at every sweep).  The previous value is read only when the new
-// one is not finite.
+// start value at every sweep).  Values are in sweep order, so entry t's
+// start value, list data and result are all read / written coalesced.  The
+// previous value is read only when the new one is not finite.
 __global__ __launch_bounds__(XS_BLOCK) void k_parilu_sweep_pairs(
-    int nact, const int *__restrict__ act, const double *__restrict__ as,
-    const int *__restrict__ app, const int *__restrict__ pl, const int *__restrict__ pu,
-    const int *__restrict__ apd, const double *lu_in, double *lu_out) {
+    int nact, const double *__restrict__ as, const int *__restrict__ app,
+    const int *__restrict__ pl, const int *__restrict__ pu, const int *__restrict__ apd,
+    const double *lc_in, double *lc_out) {
   const int t = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
   if (t >= nact) return;
-  // entry index, product range and pivot code: coalesced loads by t
-  const int k = act[t], e0 = app[t], e1 = app[t + 1], d = apd[t];
-  double s = as[k];
-  for (int e = e0; e < e1; ++e) s -= lu_in[pl[e]] * lu_in[pu[e]];
+  const int e0 = app[t], e1 = app[t + 1], d = apd[t];
+  double s = as[t];
+  for (int e = e0; e < e1; ++e) s -= lc_in[pl[e]] * lc_in[pu[e]];
   double nv;
   if (d >= 0) {
-    double dd = lu_in[d];
+    double dd = lc_in[d];
     if (fabs(dd) < 1e-10) dd = dd >= 0.0 ? 1e-10 : -1e-10;
     nv = s / dd;
   } else {
     if (d == -2 && fabs(s) < 1e-10) s = s >= 0.0 ? 1e-10 : -1e-10;
     nv = s;
   }
-  lu_out[k] = isfinite(nv) ? nv : lu_in[k];
+  lc_out[t] = isfinite(nv) ? nv : lc_in[t];
 }
 
-void xs_parilu_sweep_pairs(hipStream_t s, const xs_ilu &F, const double *as, const double *lu_in,
-                           double *lu_out) {
-  XS_GUARD(F.app && F.apd && F.pl && F.pu && (F.act || F.nact == 0) && as && lu_in && lu_out);
+void xs_parilu_sweep_pairs(hipStream_t s, const xs_ilu &F, const double *lc_in, double *lc_out) {
+  XS_GUARD(F.app && F.apd && F.pl && F.pu && F.as && lc_in && lc_out);
   if (F.nact == 0) return;
   hipLaunchKernelGGL(k_parilu_sweep_pairs, dim3((F.nact + XS_BLOCK - 1) / XS_BLOCK), dim3(XS_BLOCK),
-                     0, s, F.nact, F.act, as, F.app, F.pl, F.pu, F.apd, lu_in, lu_out);
+                     0, s, F.nact, F.as, F.app, F.pl, F.pu, F.apd, lc_in, lc_out);
 }
 
 __global__ void k_parilu_active_lists(int nact, int nnz, const int *__restrict__ act,
                                       const int *__restrict__ pp, const int *__restrict__ pd,
-                                      int *__restrict__ app, int *__restrict__ apd) {
+                                      const int *__restrict__ cmap, int *__restrict__ app,
+                                      int *__restrict__ apd) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < nact) {
-    app[t] = pp[act[t]];
-    apd[t] = pd[act[t]];
+    const int k = act[t], d = pd[k];
+    app[t] = pp[k];
+    apd[t] = d >= 0 ? cmap[d] : d;
   }
   if (t == 0) app[nact] = pp[nnz];
 }
 void xs_parilu_active_lists(hipStream_t s, int nact, int nnz, const int *act, const int *pp,
-                            const int *pd, int *app, int *apd) {
-  XS_GUARD(pp && pd && app && apd && (act || nact == 0));
+                            const int *pd, const int *cmap, int *app, int *apd) {
+  XS_GUARD(pp && pd && cmap && app && apd && (act || nact == 0));
   hipLaunchKernelGGL(k_parilu_active_lists, dim3((std::max(nact, 1) + 255) / 256), dim3(256), 0, s,
-                     nact, nnz, act, pp, pd, app, apd);
+                     nact, nnz, act, pp, pd, cmap, app, apd);
+}
+
+__global__ void k_parilu_cmap(int nnz, int nact, const int *__restrict__ flag,
+                              const int *__restrict__ pos, int *__restrict__ cmap) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nnz) cmap[k] = flag[k] ? pos[k] : nact + k - pos[k];
+}
+void xs_parilu_cmap(hipStream_t s, int nnz, int nact, const int *flag, const int *pos, int *cmap) {
+  XS_GUARD(flag && pos && cmap);
+  hipLaunchKernelGGL(k_parilu_cmap, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, nact, flag, pos,
+                     cmap);
+}
+
+__global__ void k_remap_idx(int n, const int *__restrict__ cmap, int *__restrict__ idx) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) idx[e] = cmap[idx[e]];
+}
+void xs_remap_idx(hipStream_t s, int n, const int *cmap, int *idx) {
+  XS_GUARD(cmap && (idx || n == 0));
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_remap_idx, dim3((n + 255) / 256), dim3(256), 0, s, n, cmap, idx);
 }
 
 __global__ void k_parilu_active_flags(int nnz, const int *__restrict__ cnt,
