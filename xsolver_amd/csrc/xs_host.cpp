@@ -1250,11 +1250,19 @@ static int ensure_tri_ell(xs_ctx *c) {
 // k >= 2 sweeps over the ELL copy: sweeps 1 and 2 fused, then k - 2 more,
 // ping-ponging so that the last lands in out (the results of ilu_sweeps, to
 // rounding: the fused U pass multiplies by u_ij / u_jj where ilu_sweeps
-// multiplies u_ij by (b_j / u_jj); L and the later passes are identical)
+// multiplies u_ij by (b_j / u_jj); L and the later passes are identical).
+// With ru (L only): b is r, and the first pass forms s = r - gama u, writing
+// it to sv, which the later passes then read as b.
 static void ilu_sweeps_ell(xs_ctx *c, bool upper, int k, const double *b, double *out,
-                           double *w, const xs_state *st) {
+                           double *w, const xs_state *st, const double *ru = nullptr,
+                           double *sv = nullptr) {
   double *cur = ((k - 2) % 2 == 0) ? out : w;
-  xs_tri_sweep_ell(c->stream, c->ilu, upper, true, b, nullptr, cur, st);
+  if (ru) {
+    xs_tri_sweep_ell_s(c->stream, c->ilu, b, ru, sv, cur, st);
+    b = sv;
+  } else {
+    xs_tri_sweep_ell(c->stream, c->ilu, upper, true, b, nullptr, cur, st);
+  }
   for (int j = 3; j <= k; ++j) {
     double *nxt = (cur == out) ? w : out;
     xs_tri_sweep_ell(c->stream, c->ilu, upper, false, b, cur, nxt, st);
@@ -1303,6 +1311,31 @@ void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state
     default:
       xs_copy(c->stream, c->A.n, x, y);
   }
+}
+
+// BiCGSTAB's s = r - gama u followed by y = M^{-1} s.  With an ILU applied
+// by two or more L sweeps over the ELL copies, s is formed inside the first
+// L pass (one kernel less, same values); otherwise the s update runs first.
+void xs_op_pc_s(xs_ctx *c, const double *r, const double *u, double *sv, double *y, double *tmp,
+                xs_state *st) {
+  const bool fuse = (c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) && !c->use_t &&
+                    c->lsweeps >= 2 && c->tell_ready && st != nullptr;
+  if (!fuse) {
+    xs_bicg_s(c->stream, c->A.n, r, u, sv, nullptr, nullptr, st);
+    xs_op_pc(c, sv, y, tmp, st);
+    return;
+  }
+  xs_prof_begin(c, 2);
+  ilu_sweeps_ell(c, false, c->lsweeps, r, tmp, xs_ctx_vec(c, 10), st, u, sv);
+  xs_prof_end(c, 2);
+  xs_prof_begin(c, 3);
+  if (c->usweeps >= 0) {
+    apply_sweeps(c, true, c->usweeps, tmp, y, xs_ctx_vec(c, 10), st);
+  } else {
+    xs_fill_sentinel(c->stream, y, c->A.n);
+    xs_trsv_upper(c->stream, c->ilu, tmp, y, c->counters + 4, st);
+  }
+  xs_prof_end(c, 3);
 }
 
 // ---------------------------------------------------------------------------

@@ -285,6 +285,10 @@ void xs_tell_scale_cols(hipStream_t s, const xs_tell &E, const double *dinv);
 void xs_tri_wave_widths(hipStream_t s, const xs_ilu &F, bool upper, int *ww);
 void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, const double *b,
                       const double *yo, double *yn, const xs_state *st);
+// the fused first L pass with BiCGSTAB's s = r - gama u (gama in st) formed
+// in the pass and written to sv
+void xs_tri_sweep_ell_s(hipStream_t s, const xs_ilu &F, const double *r, const double *u,
+                        double *sv, double *yn, const xs_state *st);
 void xs_permute_vals(hipStream_t s, int nnz, const int *perm, const double *src, double *dst);
 void xs_grad_vals(hipStream_t s, const xs_csr &A, const double *lam, const double *x, double *g);
 void xs_jacobi_dinv(hipStream_t s, const xs_csr &A, double *dinv, int *d_bad);

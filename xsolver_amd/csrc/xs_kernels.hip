@@ -567,7 +567,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_bicg_s(int n, const double *__rest
   if (st->done) return;
   const double gama = st->sc[S_GAMA];
   for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) {
-    const double si = r[i] - gama * u[i];
+    const double si = fma(-gama, u[i], r[i]);
     sv[i] = si;
     if (dinv) z1[i] = dinv[i] * si;
   }
@@ -1290,10 +1290,13 @@ void xs_tell_scale_cols(hipStream_t s, const xs_tell &E, const double *dinv) {
 }
 
 // yn = D (b - T yo), T given by its ELL slabs; FUSED: yo = b (L) or D b (U),
-// i.e. sweeps 1 and 2 from y = 0 in one pass.  Slabs four at a time: all
+// i.e. sweeps 1 and 2 from y = 0 in one pass.  SX (fused L pass only): the
+// right-hand side is BiCGSTAB's s = r - gama u, formed here from b = r and
+// ru = u at every row it is read and written to sv for the later kernels
+// (the s update of k_bicg_s, same expression).  Slabs four at a time: all
 // index/value loads of a batch, then all gathers.  Rows in 256-row blocks
 // placed by xcd_chunk.
-template <bool UPPER, bool FUSED, bool O16>
+template <bool UPPER, bool FUSED, bool O16, bool SX = false>
 __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W,
                                                             const int *__restrict__ eci,
                                                             const short *__restrict__ eco,
@@ -1303,13 +1306,18 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W
                                                             const double *__restrict__ b,
                                                             const double *__restrict__ yo,
                                                             double *__restrict__ yn,
-                                                            const xs_state *st) {
+                                                            const xs_state *st,
+                                                            const double *__restrict__ ru = nullptr,
+                                                            double *__restrict__ sv = nullptr) {
+  static_assert(!SX || (FUSED && !UPPER), "s is formed in the fused L pass only");
   if (st != nullptr && st->done) return;
   const int i = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
   if (i >= n) return;
+  const double gama = SX ? st->sc[S_GAMA] : 0.0;
   // the row's own operands, loaded before the slabs (off the critical path);
   // the slabs past the longest row of the row's 64-row group are all padding
-  const double bi = b[i];
+  const double bi = SX ? fma(-gama, ru[i], b[i]) : b[i];
+  if (SX) sv[i] = bi;
   const double di = UPPER ? dinv[i] : 1.0;
   W = ww ? ww[i / XS_WAVE] : W;
   double r = 0.0;
@@ -1335,7 +1343,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W
       // FUSED: the previous iterate is D b (U) or b (L); for U the slab
       // values are u_ij / u_jj (E.vs), so b is gathered alone
       const int cc = c[q] >= 0 ? c[q] : i;
-      y[q] = FUSED ? b[cc] - 0.0 : yo[cc];
+      y[q] = SX ? fma(-gama, ru[cc], b[cc]) - 0.0 : FUSED ? b[cc] - 0.0 : yo[cc];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q)
@@ -1358,6 +1366,19 @@ void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, co
   else { if (fused) XS_SWE(false, true); else XS_SWE(false, false); }
 #undef XS_SWE
 #undef XS_SWK
+}
+
+void xs_tri_sweep_ell_s(hipStream_t s, const xs_ilu &F, const double *r, const double *u,
+                        double *sv, double *yn, const xs_state *st) {
+  const xs_tell &E = F.tl;
+  XS_GUARD((E.W == 0 || (E.ci && E.v)) && E.ld >= F.n && r && u && sv && yn && st);
+  const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+  if (E.co)
+    hipLaunchKernelGGL((k_tri_sweep_ell<false, true, true, true>), grid, blk, 0, s, F.n, E.ld, E.W,
+                       E.ci, E.co, E.v, E.ww, F.dinv, r, nullptr, yn, st, u, sv);
+  else
+    hipLaunchKernelGGL((k_tri_sweep_ell<false, true, false, true>), grid, blk, 0, s, F.n, E.ld, E.W,
+                       E.ci, E.co, E.v, E.ww, F.dinv, r, nullptr, yn, st, u, sv);
 }
 
 // ---------------------------------------------------------------------------

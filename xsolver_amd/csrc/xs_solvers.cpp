@@ -99,8 +99,8 @@ int xs_solve_bicgstab(xs_ctx *c, const double *b, double *x) {
     xs_bicg_p(s, n, r, u, p, jac, z, c->st);                    // p = r + w (p - a u)
     if (!jac) xs_op_pc(c, p, z, t, c->st);                      // z = M^-1 p
     xs_op_spmv(c, z, u, EPI_DOT1, nullptr, rh, &rg);            // u = A z, gama
-    xs_bicg_s(s, n, r, u, sv, jac, z1, c->st);                  // s = r - gama u
-    if (!jac) xs_op_pc(c, sv, z1, t, c->st);                    // z1 = M^-1 s
+    if (jac) xs_bicg_s(s, n, r, u, sv, jac, z1, c->st);         // s = r - gama u, z1 = D s
+    else xs_op_pc_s(c, r, u, sv, z1, t, c->st);                 // s = r - gama u, z1 = M^-1 s
     xs_op_spmv(c, z1, v, EPI_DOT2, nullptr, sv, &ra);           // v = A z1, alpha
     xs_bicg_xr(s, n, x, r, z, z1, sv, v, rh, rr);               // x, r, ||r||, next beta
   });

@@ -180,6 +180,9 @@ xs_reduce xs_ctx_reduce(xs_ctx *c, int op, int slot);
 void xs_op_spmv(xs_ctx *c, const double *x, double *y, int epi, const double *b,
                 const double *w, const xs_reduce *red);
 void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state *st);
+// s = r - gama u (gama in st), then y = M^{-1} s
+void xs_op_pc_s(xs_ctx *c, const double *r, const double *u, double *sv, double *y, double *tmp,
+                xs_state *st);
 int xs_op_pc_t(xs_ctx *c, const double *x, double *y, double *tmp, hipStream_t s);
 
 int xs_solve_cg(xs_ctx *c, const double *b, double *x);
