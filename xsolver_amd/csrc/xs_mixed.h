@@ -21,6 +21,10 @@ void xs_tri_sweep32(hipStream_t s, bool upper, int n, const int *rp, const int *
 void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_tell &E, int ftype,
                         const float *dinv, const float *b, const float *yo, float *yn,
                         const xs_state *st);
+// the fused first L pass with BiCGSTAB's s = r - gama u (gama in st) formed
+// in the pass and written to sv
+void xs_tri_sweep_ell32_s(hipStream_t s, int n, const xs_tell &E, int ftype, const float *r,
+                          const float *u, float *sv, float *yn, const xs_state *st);
 void xs_trsv32(hipStream_t s, bool upper, int n, const int *rp, const int *ci, const void *lu,
                int ftype, const int *dpos, const float *dinv, const float *b, float *x,
                unsigned *tickets, const xs_tsched &S, const xs_state *st);

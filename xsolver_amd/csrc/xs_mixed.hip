@@ -513,8 +513,10 @@ __device__ __forceinline__ float2 fl2(const __half *p, size_t i) {
 // Two rows per thread (r, r + 1; ld even): one 4-byte offset load and one
 // paired value load per slab cover both rows, half the load instructions of
 // a row per thread (which matters most for 2-byte BF16 / FP16 values).  Per
-// row the arithmetic of the one-row form, slab by slab.
-template <bool UPPER, bool FUSED, bool O16, typename TF>
+// row the arithmetic of the one-row form, slab by slab.  SX (fused L pass
+// only): the right-hand side is BiCGSTAB's s = r - gama u, formed from b = r
+// and ru = u at every row read and written to sv (k_bicg_s32's expression).
+template <bool UPPER, bool FUSED, bool O16, typename TF, bool SX = false>
 __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int W,
                                                               const int *__restrict__ eci,
                                                               const short *__restrict__ eco,
@@ -524,13 +526,22 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
                                                               const float *__restrict__ b,
                                                               const float *__restrict__ yo,
                                                               float *__restrict__ yn,
-                                                              const xs_state *st) {
+                                                              const xs_state *st,
+                                                              const float *__restrict__ ru = nullptr,
+                                                              float *__restrict__ sv = nullptr) {
+  static_assert(!SX || (FUSED && !UPPER), "s is formed in the fused L pass only");
   if (st != nullptr && st->done) return;
   const int r = 2 * (xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x);
   if (r >= n) return;
   const bool two = r + 1 < n;
+  const float gama = SX ? (float)st->sc[S_GAMA] : 0.f;
   // the rows' own operands, loaded before the slabs (off the critical path)
-  const float b0 = b[r], b1 = two ? b[r + 1] : 0.f;
+  const float b0 = SX ? fmaf(-gama, ru[r], b[r]) : b[r];
+  const float b1 = two ? (SX ? fmaf(-gama, ru[r + 1], b[r + 1]) : b[r + 1]) : 0.f;
+  if (SX) {
+    sv[r] = b0;
+    if (two) sv[r + 1] = b1;
+  }
   const float d0 = UPPER ? dinv[r] : 1.f, d1 = (UPPER && two) ? dinv[r + 1] : 1.f;
   W = ww ? ww[r / XS_WAVE] : W;   // slabs past the group's longest row are padding
   float s0 = 0.f, s1 = 0.f;
@@ -560,7 +571,10 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int g0 = c0[q] >= 0 ? c0[q] : r, g1 = c1[q] >= 0 ? c1[q] : r;
-      if (FUSED) {
+      if (SX) {
+        y0[q] = fmaf(-gama, ru[g0], b[g0]) - 0.f;
+        y1[q] = fmaf(-gama, ru[g1], b[g1]) - 0.f;
+      } else if (FUSED) {
         y0[q] = UPPER ? dinv[g0] * (b[g0] - 0.f) : b[g0] - 0.f;
         y1[q] = UPPER ? dinv[g1] * (b[g1] - 0.f) : b[g1] - 0.f;
       } else {
@@ -596,6 +610,21 @@ void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_t
 #undef XS_SWE32
 #undef XS_SWT32
 #undef XS_SWK32
+}
+
+void xs_tri_sweep_ell32_s(hipStream_t s, int n, const xs_tell &E, int ftype, const float *r,
+                          const float *u, float *sv, float *yn, const xs_state *st) {
+  if (!((E.W == 0 || (E.ci && E.vlo)) && E.ld >= n && E.ld % 2 == 0 && r && u && sv && yn && st)) {
+    xs_raise_launch_guard();
+    return;
+  }
+  const int pairs = (n + 1) / 2;
+  const dim3 grid((pairs + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+#define XS_SWS32(O, T) hipLaunchKernelGGL((k_tri_sweep_ell32<false, true, O, T, true>), grid, blk, 0, s, n, E.ld, E.W, E.ci, E.co, static_cast<const T *>(E.vlo), E.ww, nullptr, r, nullptr, yn, st, u, sv)
+#define XS_SWT32S(O) do { if (ftype == XS_BF16) XS_SWS32(O, __hip_bfloat16); else if (ftype == XS_F16) XS_SWS32(O, __half); else XS_SWS32(O, float); } while (0)
+  if (E.co) XS_SWT32S(true); else XS_SWT32S(false);
+#undef XS_SWT32S
+#undef XS_SWS32
 }
 
 __global__ void k_fill_u32(unsigned *p, int n, unsigned val) {
@@ -756,7 +785,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_bicg_s32(int n, const float *__res
   if (st->done) return;
   const float gama = (float)st->sc[S_GAMA];
   for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) {
-    const float si = r[i] - gama * u[i];
+    const float si = fmaf(-gama, u[i], r[i]);
     sv[i] = si;
     if (dinv) z1[i] = dinv[i] * si;
   }

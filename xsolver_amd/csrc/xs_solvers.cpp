@@ -412,14 +412,22 @@ static void op_spmv32(xs_ctx *c, const float *x, float *y, int epi, const float 
 }
 
 // FP32 counterpart of the host's ilu_sweeps (k Jacobi sweeps of one factor);
-// k >= 2 over the FP32 ELL copies when they are current (sweeps 1 and 2 fused)
+// k >= 2 over the FP32 ELL copies when they are current (sweeps 1 and 2
+// fused).  With ru (L, k >= 2, ELL copies current): b is r, and the first
+// pass forms s = r - gama u into sv, which the later passes read as b.
 static void ilu_sweeps32(xs_ctx *c, bool upper, int k, const float *b, float *out, float *w,
-                         const xs_state *st) {
+                         const xs_state *st, const float *ru = nullptr, float *sv = nullptr) {
   const int n = c->A.n, m = k < 1 ? 1 : k;
   if (m >= 2 && c->tell32_ready) {
     const xs_tell &E = upper ? c->ilu.tu : c->ilu.tl;
     float *cur = ((m - 2) % 2 == 0) ? out : w;
-    xs_tri_sweep_ell32(c->stream, upper, true, n, E, c->factor_type, c->dinv32, b, nullptr, cur, st);
+    if (ru) {
+      xs_tri_sweep_ell32_s(c->stream, n, E, c->factor_type, b, ru, sv, cur, st);
+      b = sv;
+    } else {
+      xs_tri_sweep_ell32(c->stream, upper, true, n, E, c->factor_type, c->dinv32, b, nullptr, cur,
+                         st);
+    }
     for (int j = 3; j <= m; ++j) {
       float *nxt = (cur == out) ? w : out;
       xs_tri_sweep_ell32(c->stream, upper, false, n, E, c->factor_type, c->dinv32, b, cur, nxt, st);
@@ -463,6 +471,30 @@ static void op_pc32(xs_ctx *c, const float *x, float *y, float *tmp, const xs_st
   }
 }
 
+// BiCGSTAB's s = r - gama u, then y = M^{-1} s: s formed inside the first L
+// pass when the ILU is applied by two or more L sweeps over the FP32 ELL
+// copies, else by its own kernel first
+static void op_pc32_s(xs_ctx *c, const float *r, const float *u, float *sv, float *y, float *tmp,
+                      xs_state *st) {
+  const int n = c->A.n;
+  if (!((c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) && c->lsweeps >= 2 && c->tell32_ready)) {
+    xs_bicg_s32(c->stream, n, r, u, sv, nullptr, nullptr, st);
+    op_pc32(c, sv, y, tmp, st);
+    return;
+  }
+  float *w = c->work32 + 11 * (size_t)n;   // sweep scratch
+  xs_prof_begin(c, 2);
+  ilu_sweeps32(c, false, c->lsweeps, r, tmp, w, st, u, sv);
+  xs_prof_end(c, 2);
+  xs_prof_begin(c, 3);
+  if (c->usweeps >= 0)
+    ilu_sweeps32(c, true, c->usweeps, tmp, y, w, st);
+  else
+    xs_trsv32(c->stream, true, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type, c->ilu.dpos,
+              c->dinv32, tmp, y, c->counters + 4, xs_ilu_tsched(c->ilu, true), st);
+  xs_prof_end(c, 3);
+}
+
 // Relative reduction asked of an inner FP32 solve.  By default sqrt(rtol):
 // two refinement steps reach rtol when each inner solve meets its target,
 // and 1e-6 is about the reduction FP32 Krylov iterations still deliver.
@@ -498,8 +530,8 @@ static int inner_bicgstab32(xs_ctx *c, const float *rhs, float *d, int maxit, in
     xs_bicg_p32(s, n, r, u, p, jac, z, st);
     if (!jac) op_pc32(c, p, z, t, st);
     op_spmv32(c, z, u, EPI_DOT1, nullptr, rh, &rg);
-    xs_bicg_s32(s, n, r, u, sv, jac, z1, st);
-    if (!jac) op_pc32(c, sv, z1, t, st);
+    if (jac) xs_bicg_s32(s, n, r, u, sv, jac, z1, st);
+    else op_pc32_s(c, r, u, sv, z1, t, st);
     op_spmv32(c, z1, v, EPI_DOT2, nullptr, sv, &ra);
     xs_bicg_xr32(s, n, d, r, z, z1, sv, v, rh, rr);
   };
@@ -644,8 +676,8 @@ static int mixed_bicgstab_reliable(xs_ctx *c, const double *b, double *x) {
       xs_bicg_p32(s, n, r, u, p, jac, z, si);
       if (!jac) op_pc32(c, p, z, t, si);
       op_spmv32(c, z, u, EPI_DOT1, nullptr, rh, &rg);
-      xs_bicg_s32(s, n, r, u, sv, jac, z1, si);
-      if (!jac) op_pc32(c, sv, z1, t, si);
+      if (jac) xs_bicg_s32(s, n, r, u, sv, jac, z1, si);
+      else op_pc32_s(c, r, u, sv, z1, t, si);
       op_spmv32(c, z1, v, EPI_DOT2, nullptr, sv, &ra);
       xs_bicg_xr32(s, n, d, r, z, z1, sv, v, rh, rr);
     };
