@@ -10,13 +10,27 @@ residual 1e-10.  The ILU factors are applied the way the reference's GPU
 backend applies them -- Jacobi sweeps per factor -- with 5 sweeps on L and 2
 on U in the headline configuration (the reference hard-codes 20 / 20; 5 / 2
 gave the shortest time-to-rtol of the sweep counts tried on this system);
-the exact sync-free SpTRSV apply, the reference's 20/20 sweeps and the
-FP32-inner mixed solves (FP32- and BF16-stored factors) are timed beside it.
+the exact sync-free SpTRSV apply, the reference's 20/20 sweeps, ELL input and
+the FP32-inner mixed solves (FP32- and BF16-stored factors) are timed beside
+it.
+
+The same JSON line carries, under "workloads", every other configuration of
+the hot path, each timed on the GPU with the matrix resident in HBM: the
+27-point lognormal 100^3 GMRES(30) + ParILUT system (FP64 and mixed
+precision, set-up included), 512 independent systems in one persistent
+launch, 32 right-hand sides on the 128^3 matrix, the 512^2 adjoint
+(forward + transposed solve + gradient gather), and SpMV kernel timings
+(ELL and row-binned vector CSR, with warm and with flushed last-level cache,
+beside a flushed-cache device copy that sets the practical bandwidth
+ceiling).
 
 Printed: one JSON line (value = seconds per solve, lower is better) with the
 dominant kernel's roofline (algorithmic bytes / mean time from HIP events
 recorded on the library's stream during the timed steps) and a CPU baseline
-(the oracle's C restatement of the same solve, one core, on a bounded sample).
+(the oracle's C restatement of the reference CPU path -- ILU(0) set-up, then
+BiCGSTAB with the exact LU apply -- run to convergence on the host cores,
+SpMV and BLAS-1 on OpenMP threads, triangular solves serial as in the
+reference).
 """
 import argparse
 import json
@@ -150,35 +164,51 @@ def measured_traffic(family):
         return None
 
 
-def cpu_baseline(A, b, iters_needed, sweeps, sample_iters=8):
-    """Oracle (C restatement, 1 core): ILU(0) set-up plus `sample_iters`
-    BiCGSTAB iterations with the same sweep apply on the same system, scaled
-    to `iters_needed`."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(A, b, sweeps):
+    """The oracle (the C restatement of the reference CPU path) running the
+    whole solve on the host: ILU(0) set-up (seqilu0,
+    hip/HostMatrix/HostMatrixCSR.cpp:164-197), then BiCGSTAB
+    (hip/Solver/xsolver_bicgstab.cpp:20-124) with the exact LU apply
+    (LUsolve, :385-409) to rtol 1e-10.  SpMV and BLAS-1 run on OMP_NUM_THREADS
+    OpenMP threads (the reference uses threaded MKL there); the triangular
+    solves are serial as in the reference.  Also timed: the same solve with
+    the headline's Jacobi-sweep apply."""
     import oracle
 
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     oracle.lib()
     t0 = time.perf_counter()
-    oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=0,
-                 lsweeps=sweeps[0], usweeps=sweeps[1])
-    t_setup = time.perf_counter() - t0
+    _, st, it, _ = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=2000)
+    t_exact = time.perf_counter() - t0
     t0 = time.perf_counter()
-    _, st, it, _ = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=sample_iters,
-                                lsweeps=sweeps[0], usweeps=sweeps[1])
-    t_run = time.perf_counter() - t0
-    per_iter = max(t_run - t_setup, 0.0) / max(it, 1)
-    est = t_setup + per_iter * iters_needed
+    _, st2, it2, _ = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=2000,
+                                  lsweeps=sweeps[0], usweeps=sweeps[1])
+    t_sw = time.perf_counter() - t0
     return {
-        "value": round(est, 4),
+        "value": round(t_exact, 4),
         "unit": "s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": (f"oracle ILU(0) set-up ({t_setup:.2f} s) + {it} BiCGSTAB iterations "
-                   f"with {sweeps[0]}/{sweeps[1]} Jacobi-sweep ILU apply ({per_iter:.3f} s/iter) "
-                   f"on the same system, scaled to {iters_needed} iterations"),
+        "cpu": _cpu_model(),
+        "sample": (f"the whole solve, no extrapolation: ILU(0) set-up + {it} BiCGSTAB iterations "
+                   f"with the exact LU apply to rtol 1e-10 (status {st}); SpMV/BLAS-1 on {threads} "
+                   f"OpenMP threads, triangular solves serial"),
+        "sweeps_apply_s": round(t_sw, 4),
+        "sweeps_apply_iterations": it2,
     }
 
 
-def bench_batch(args):
+def bench_batch(steps=3, warmup=1):
     """512 independent 2D variable-coefficient diffusion systems (s ~ U{32..160},
     kappa = exp(N(0, 0.5)) per cell, harmonic faces, seed 3), CG + Jacobi to
     rtol 1e-8 in one persistent launch, FP64 and FP32-inner/FP64-refine."""
@@ -197,34 +227,28 @@ def bench_batch(args):
     out = {}
     for prec in (0, 1):
         xd = torch.zeros_like(bd)
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             xd.zero_()
             B.solve_device(bd, xd, tol=1e-8, maxiter=5000, precision=prec)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             xd.zero_()
             B.solve_device(bd, xd, tol=1e-8, maxiter=5000, precision=prec)
         torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / args.steps
+        dt = (time.perf_counter() - t0) / steps
         its, rel = B.results()
         out[prec] = {"sec": dt, "max_iters": int(its.max()), "max_relres": float(rel.max())}
     B.close()
-    print(json.dumps({
-        "metric": "batched solve time, 512 systems (s)", "value": round(out[0]["sec"], 5),
-        "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1000 * out[0]["sec"], 3), "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded generator, xsolver_amd/gen.py)",
-        "config": {"workload": "batch512 vardiff2d s~U{32..160} CG+Jacobi rtol 1e-8",
-                   "rows": n, "nnz": int(len(v)), "systems_per_s": round(512 / out[0]["sec"], 1),
-                   "max_iterations": out[0]["max_iters"], "max_relres": out[0]["max_relres"],
-                   "mixed_sec": round(out[1]["sec"], 5),
-                   "mixed_max_relres": out[1]["max_relres"],
-                   "mixed_speedup_vs_fp64": round(out[0]["sec"] / out[1]["sec"], 3)}}), flush=True)
+    return {"workload": "batch512 vardiff2d s~U{32..160} CG+Jacobi rtol 1e-8, one persistent launch",
+            "sec": round(out[0]["sec"], 5), "rows": n, "nnz": int(len(v)),
+            "systems_per_s": round(512 / out[0]["sec"], 1),
+            "max_iterations": out[0]["max_iters"], "max_relres": out[0]["max_relres"],
+            "mixed_sec": round(out[1]["sec"], 5), "mixed_max_relres": out[1]["max_relres"],
+            "mixed_speedup_vs_fp64": round(out[0]["sec"] / out[1]["sec"], 3)}
 
 
-def bench_rhs32(args):
+def bench_rhs32(grid=128, steps=2, warmup=1):
     """32 right-hand sides on the 128^3 convection-diffusion matrix (RHS j:
     x*_j with seed 1 + j), BiCGSTAB + ParILU(0) with the headline's sweep
     apply to rtol 1e-10, one block solve (SpMM + 32-column sweeps) against 32
@@ -232,7 +256,7 @@ def bench_rhs32(args):
     import torch
     from xsolver_amd import XSolver, gen
 
-    N, k = args.grid, 32
+    N, k = grid, 32
     A = gen.convdiff3d(N)
     n = len(A[0]) - 1
     B = np.stack([gen.manufactured(A, 1 + j)[1] for j in range(k)], 1)
@@ -249,16 +273,16 @@ def bench_rhs32(args):
     S.pc_setup()
     Bd = torch.from_numpy(np.ascontiguousarray(B)).to(dev)
     Xd = torch.zeros_like(Bd)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         Xd.zero_()
         S.solve_multi_device(Bd, Xd)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         Xd.zero_()
         S.solve_multi_device(Bd, Xd)
     torch.cuda.synchronize()
-    block = (time.perf_counter() - t0) / args.steps
+    block = (time.perf_counter() - t0) / steps
     its, r0, rl = S.multi_results()
     X = Xd.cpu().numpy()
     rel = max(float(np.linalg.norm(B[:, j] - gen.spmv(A, X[:, j])) / np.linalg.norm(B[:, j]))
@@ -295,31 +319,24 @@ def bench_rhs32(args):
     spmm_us = e0.elapsed_time(e1) * 100.0
     spmm_bytes = (8 + idx_bytes) * nnz + (0 if spmm_ell else 4 * (n + 1)) + 16 * n * k
     S.close()
-    print(json.dumps({
-        "metric": "32-RHS block solve time (s)", "value": round(block, 5), "unit": "s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1000 * block, 3), "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded generator, xsolver_amd/gen.py)",
-        "config": {"workload": (f"convdiff3d_{N} x 32 RHS BiCGSTAB+ParILU(0) FP64 rtol 1e-10, "
-                                f"ILU apply {APPLY[0]}/{APPLY[1]} Jacobi sweeps"),
-                   "n": n, "k": k, "sec_per_rhs": round(block / k, 5),
-                   "single_solve_sec": round(single, 5),
-                   "speedup_vs_single_solves": round(single * k / block, 3),
-                   "max_iterations": int(its.max()), "min_iterations": int(its.min()),
-                   "max_true_relres": rel, "spmm_us": round(spmm_us, 2),
-                   "spmm_GBs": round(spmm_bytes / spmm_us / 1e3, 1),
-                   "spmm_bytes": spmm_bytes,
-                   "spmm_kernel": ("ELL, 16-bit offsets" if idx_bytes == 2 else "ELL")
-                   if spmm_ell else "CSR"}}), flush=True)
+    return {"workload": (f"convdiff3d_{N} x 32 RHS BiCGSTAB+ParILU(0) FP64 rtol 1e-10, "
+                         f"ILU apply {APPLY[0]}/{APPLY[1]} Jacobi sweeps"),
+            "sec": round(block, 5), "n": n, "k": k, "sec_per_rhs": round(block / k, 5),
+            "single_solve_sec": round(single, 5),
+            "speedup_vs_single_solves": round(single * k / block, 3),
+            "max_iterations": int(its.max()), "min_iterations": int(its.min()),
+            "max_true_relres": rel, "spmm_us": round(spmm_us, 2),
+            "spmm_GBs": round(spmm_bytes / spmm_us / 1e3, 1), "spmm_bytes": spmm_bytes,
+            "spmm_kernel": ("ELL, 16-bit offsets" if idx_bytes == 2 else "ELL")
+            if spmm_ell else "CSR"}
 
 
-def bench_c3(args):
+def bench_c3(steps=3, warmup=1, exact=True):
     """27-point lognormal system on 100^3 (n = 1,000,000, nnz = 26,463,592,
     seed 2), GMRES(30) + ParILUT (fill 2, drop 1e-4) to rtol 1e-10: FP64
     against FP32-inner / FP64-outer with BF16- and FP32-stored factors, the
     factors applied by the reference's ParILUT sweeps (10 on L, 15 on U) and
-    by the exact sync-free solve.  Set-up (ParILUT) timed separately."""
+    by the exact sync-free solve.  Time-to-rtol = ParILUT set-up + solve."""
     import torch
     from xsolver_amd import XSolver, gen
 
@@ -330,9 +347,10 @@ def bench_c3(args):
     rp, ci, v = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in A)
     bd = torch.from_numpy(b).to(dev)
     out = {}
-    for name, prec, fs, sw in (("fp64", 0, 0, (10, 15)), ("mixed_bf16", 1, 1, (10, 15)),
-                               ("mixed_fp32", 1, 0, (10, 15)), ("fp64_exact", 0, 0, None),
-                               ("mixed_bf16_exact", 1, 1, None)):
+    runs = [("fp64", 0, 0, (10, 15)), ("mixed_bf16", 1, 1, (10, 15)), ("mixed_fp32", 1, 0, (10, 15))]
+    if exact:
+        runs += [("fp64_exact", 0, 0, None), ("mixed_bf16_exact", 1, 1, None)]
+    for name, prec, fs, sw in runs:
         S = XSolver()
         S.mat_setup_device(rp, ci, v)
         S.ksp_settype("gmres")
@@ -344,39 +362,157 @@ def bench_c3(args):
         if sw:
             S.set_option("lsolve_sweeps", sw[0])
             S.set_option("usolve_sweeps", sw[1])
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        S.pc_setup()
-        torch.cuda.synchronize()
-        setup = time.perf_counter() - t0
+        setups, ts = [], []
         xd = torch.zeros_like(bd)
-        ts = []
-        for _ in range(args.warmup + args.steps):
+        for rep in range(warmup + steps):
+            S.set_option("refactor", 1)
             xd.zero_()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
+            S.pc_setup()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
             st = S.solve_device(bd, xd)
             torch.cuda.synchronize()
-            ts.append(time.perf_counter() - t0)
+            t2 = time.perf_counter()
             assert st == 0, f"{name}: status {st}"
+            if rep >= warmup:
+                setups.append(t1 - t0)
+                ts.append(t2 - t1)
         x = xd.cpu().numpy()
-        out[name] = {"solve_s": round(float(np.mean(ts[args.warmup:])), 5),
-                     "setup_s": round(setup, 4), "iterations": S.iterations,
+        su, so = float(np.mean(setups)), float(np.mean(ts))
+        out[name] = {"time_to_rtol_s": round(su + so, 5), "setup_s": round(su, 5),
+                     "solve_s": round(so, 5), "iterations": S.iterations,
                      "true_relres": float(np.linalg.norm(b - gen.spmv(A, x)) / np.linalg.norm(b))}
         S.close()
-    f64 = out["fp64"]["solve_s"]
-    print(json.dumps({
-        "metric": "mixed-precision vs FP64 solve time, GMRES(30)+ParILUT (s)",
-        "value": out["mixed_bf16"]["solve_s"], "unit": "s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(1000 * out["mixed_bf16"]["solve_s"], 3),
-        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32/bf16+f64",
-        "data": "synthetic (seeded generator, xsolver_amd/gen.py)",
-        "config": {"workload": "lognormal27_100 GMRES(30)+ParILUT(fill 2, drop 1e-4) rtol 1e-10, "
-                               "ILU apply 10/15 sweeps",
-                   "n": n, "nnz": nnz, "runs": out,
-                   "mixed_bf16_speedup_vs_fp64": round(f64 / out["mixed_bf16"]["solve_s"], 3),
-                   "mixed_fp32_speedup_vs_fp64": round(f64 / out["mixed_fp32"]["solve_s"], 3)}}),
-        flush=True)
+    f64 = out["fp64"]
+    return {"workload": "lognormal27_100 GMRES(30)+ParILUT(fill 2, drop 1e-4) rtol 1e-10, "
+                        "ILU apply 10/15 sweeps; time-to-rtol = ParILUT set-up + solve",
+            "n": n, "nnz": nnz, "runs": out,
+            "mixed_bf16_speedup_vs_fp64": round(f64["time_to_rtol_s"] / out["mixed_bf16"]["time_to_rtol_s"], 3),
+            "mixed_bf16_solve_speedup_vs_fp64": round(f64["solve_s"] / out["mixed_bf16"]["solve_s"], 3),
+            "mixed_fp32_speedup_vs_fp64": round(f64["time_to_rtol_s"] / out["mixed_fp32"]["time_to_rtol_s"], 3)}
+
+
+def bench_adjoint(steps=3, warmup=1):
+    """Adjoint on the 512^2 variable-coefficient diffusion system (kappa =
+    exp of a squared-exponential Gaussian random field, length 0.1, seed 4),
+    CG + ParILU(0) FP64: forward solve to rtol 1e-10, transposed solve
+    A^T lam = 2 (x - x_t) (transposed SpMV, transposed ILU triangular
+    solves), gradient gather dL/dA_ij = -lam_i x_j on the pattern."""
+    import torch
+    from xsolver_amd import XSolver, gen
+
+    A = gen.vardiff2d(gen.grf_kappa(512, 0.1, 4))
+    rp, ci, v = A
+    n, nnz = len(rp) - 1, len(v)
+    xt = np.random.default_rng([4, 1]).uniform(-1, 1, n)
+    b = gen.spmv(A, np.random.default_rng([4, 2]).uniform(-1, 1, n))
+    dev = torch.device("cuda:0")
+    S = XSolver()
+    S.mat_setup(rp, ci, v)
+    S.ksp_settype("cg")
+    S.ksp_setoption(1e-10, 20000)
+    S.pc_settype("ilu0")
+    S.pc_setup()
+    bd = torch.from_numpy(b).to(dev)
+    xtd = torch.from_numpy(xt).to(dev)
+    xd = torch.zeros_like(bd)
+    lam = torch.zeros_like(bd)
+    gv = torch.empty(nnz, dtype=torch.float64, device=dev)
+    tf, ta = [], []
+    for rep in range(warmup + steps):
+        xd.zero_()
+        lam.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        assert S.solve_device(bd, xd) == 0
+        itf = S.iterations
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        g = 2.0 * (xd - xtd)
+        assert S.solve_transpose_device(g, lam) == 0
+        ita = S.iterations
+        S.grad_values(lam, xd, gv, torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if rep >= warmup:
+            tf.append(t1 - t0)
+            ta.append(t2 - t1)
+    S.close()
+    return {"workload": "grf2d_512 CG+ParILU(0) FP64 forward + adjoint (A^T lam = dL/dx) + "
+                        "gradient gather",
+            "n": n, "nnz": nnz, "forward_s": round(float(np.mean(tf)), 5),
+            "adjoint_and_gradient_s": round(float(np.mean(ta)), 5),
+            "time_to_gradient_s": round(float(np.mean(tf) + np.mean(ta)), 5),
+            "forward_iterations": itf, "adjoint_iterations": ita}
+
+
+def bench_spmv_kernels(cases=("convdiff3d_128", "lognormal27_100"), reps=20):
+    """SpMV y = A x alone through xs_spmv on HIP events of the launching
+    stream: the ELL kernel and the row-binned vector-CSR kernel on each
+    matrix, back to back (warm: the 256 MB last-level cache keeps part of
+    the operands) and with the cache flushed before every launch (a 1 GiB
+    buffer written in between).  Beside them the practical ceiling measured
+    the same way: a flushed-cache device copy (torch copy_, 1 GiB)."""
+    import torch
+    from xsolver_amd import XSolver, gen
+
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream().cuda_stream
+    flush = torch.empty(1 << 27, dtype=torch.float64, device=dev)   # 1 GiB
+
+    def timed(fn, cold):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(reps)]
+        fn()
+        for e0, e1 in ev:
+            if cold:
+                flush.fill_(1.0)
+            e0.record()
+            fn()
+            e1.record()
+        torch.cuda.synchronize()
+        return float(np.median([e0.elapsed_time(e1) * 1000.0 for e0, e1 in ev]))
+
+    out = {}
+    src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(2.0)
+    dst = torch.empty_like(src)
+    cu = timed(lambda: dst.copy_(src), True)
+    wu = timed(lambda: dst.copy_(src), False)
+    out["copy_1GiB"] = {"cold_us": round(cu, 1), "cold_GBs": round(2 * 8 * src.numel() / cu / 1e3, 1),
+                        "warm_us": round(wu, 1), "warm_GBs": round(2 * 8 * src.numel() / wu / 1e3, 1)}
+    del src, dst
+    for name in cases:
+        A = gen.convdiff3d(128) if name == "convdiff3d_128" else gen.lognormal27(100, seed=2)
+        n, nnz = len(A[0]) - 1, len(A[2])
+        rp, ci, v = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in A)
+        x = torch.from_numpy(np.random.default_rng(0).uniform(-1, 1, n)).to(dev)
+        y = torch.empty_like(x)
+        rec = {"n": n, "nnz": nnz}
+        for kname, ell, vec in (("ell", True, None), ("vector_csr", False, 1)):
+            S = XSolver()
+            S.mat_setup_device(rp, ci, v, use_ell=ell)
+            if vec is not None:
+                S.set_option("spmv_kernel", vec)
+            kind = S.time_ms("spmv:kernel")
+            if ell and kind != 2:
+                S.close()
+                continue
+            idx = 2 if (ell and S.time_ms("spmv:idx16") == 1) else 4
+            # bytes the kernel must move: values + stored columns (+ row
+            # pointers for CSR), x once, y once
+            by = (8 + idx) * nnz + (0 if ell else 4 * (n + 1)) + 16 * n
+            warm = timed(lambda: S.spmv(x, y, st), False)
+            cold = timed(lambda: S.spmv(x, y, st), True)
+            rec[kname] = {"bytes": by, "warm_us": round(warm, 2),
+                          "warm_GBs": round(by / warm / 1e3, 1), "cold_us": round(cold, 2),
+                          "cold_GBs": round(by / cold / 1e3, 1),
+                          "cold_frac_of_peak": round(by / cold / 1e3 / HBM_PEAK_GBS, 3)}
+            S.close()
+        out[name] = rec
+        del rp, ci, v, x, y
+    return out
 
 
 def main():
@@ -391,9 +527,13 @@ def main():
     ap.add_argument("--no-sweeps", action="store_true",
                     help="skip the runs with the reference's Jacobi-sweep ILU apply")
     ap.add_argument("--workload", default="convdiff3d",
-                    choices=["convdiff3d", "batch512", "rhs32", "c3"],
-                    help="convdiff3d (default, the headline), batch512, rhs32, c3 (27-point "
-                         "100^3 GMRES+ParILUT, mixed precision vs FP64)")
+                    choices=["convdiff3d", "batch512", "rhs32", "c3", "adjoint", "spmv"],
+                    help="convdiff3d (default: the headline line, with every other workload "
+                         "under 'workloads'), or one workload alone (for profiling): batch512, "
+                         "rhs32, c3 (27-point 100^3 GMRES+ParILUT, mixed precision vs FP64), "
+                         "adjoint (512^2), spmv (SpMV kernels warm / flushed-cache, copy ceiling)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the other workloads of the default line")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -412,12 +552,16 @@ def main():
               "multi-GPU decomposition is design-only) -- run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "batch512":
-        return bench_batch(args)
-    if args.workload == "rhs32":
-        return bench_rhs32(args)
-    if args.workload == "c3":
-        return bench_c3(args)
+    single = {"batch512": lambda: bench_batch(args.steps, args.warmup),
+              "rhs32": lambda: bench_rhs32(args.grid, args.steps, args.warmup),
+              "c3": lambda: bench_c3(args.steps, args.warmup),
+              "adjoint": lambda: bench_adjoint(args.steps, args.warmup),
+              "spmv": lambda: bench_spmv_kernels()}
+    if args.workload in single:
+        rec = single[args.workload]()
+        print(json.dumps({"workload": args.workload, "steps": args.steps,
+                          "warmup": args.warmup, "result": rec}), flush=True)
+        return
 
     from xsolver_amd import gen
 
@@ -489,7 +633,18 @@ def main():
                 "bytes_per_launch": by[dom], "avg_us": round(rec["avg_us"], 2)}
 
     iters = its(head)
-    cpu = None if args.no_cpu else cpu_baseline(A, b, iters, APPLY)
+    cpu = None if args.no_cpu else cpu_baseline(A, b, APPLY)
+    # every other workload of the hot path, each bounded to a few timed runs
+    workloads = {}
+    if not args.headline_only:
+        for name, fn in (("spmv_kernels", bench_spmv_kernels), ("lognormal27_100", bench_c3),
+                         ("batch512", bench_batch), ("rhs32", bench_rhs32),
+                         ("grf2d_512_adjoint", bench_adjoint)):
+            t0 = time.perf_counter()
+            workloads[name] = fn()
+            workloads[name]["bench_wall_s"] = round(time.perf_counter() - t0, 1)
+            print(f"bench: {name} done ({workloads[name]['bench_wall_s']} s)", file=sys.stderr,
+                  flush=True)
 
     out = {
         "metric": METRIC,
@@ -539,6 +694,7 @@ def main():
         "kernels": kernels_gbs,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "workloads": workloads,
     }
     print(json.dumps(out), flush=True)
 

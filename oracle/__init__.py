@@ -59,8 +59,11 @@ def lib():
         L.or_ilu_sweeps.argtypes = [ctypes.c_int, _ip, _ip, _dp, _ip, ctypes.c_int,
                                     ctypes.c_int, _dp, _dp]
         L.or_parilut.argtypes = [ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_double,
-                                 ctypes.c_double, ctypes.c_int, _ip, _ip, _dp, _ip, _ip, _dp]
-        for f in ("or_jacobi_setup", "or_ilu0", "or_ilut", "or_solve", "or_parilut"):
+                                 ctypes.c_double, ctypes.c_int, _ip, _ip, _dp, _ip, _ip, _dp, _ip]
+        L.or_batch_cg.argtypes = [ctypes.c_int, _ip, _ip, _ip, _dp, _dp, _dp, ctypes.c_double,
+                                  ctypes.c_int, _ip, _dp]
+        for f in ("or_jacobi_setup", "or_ilu0", "or_ilut", "or_solve", "or_parilut",
+                  "or_batch_cg"):
             getattr(L, f).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -198,19 +201,39 @@ def parilu_sweep(A, L, Ucsc, n):
     return lo, uo
 
 
-def parilut(A, sweeps=10, fill=2.0, droptol=1e-4):
-    """ParILUT factors: L (strict lower CSR), U (upper CSR, diagonal first)."""
+def parilut(A, sweeps=10, fill=2.0, droptol=1e-4, trace=False):
+    """ParILUT factors: L (strict lower CSR), U (upper CSR, diagonal first).
+    With trace=True also an (sweeps, 4) array per outer step: strict-L
+    candidates, U candidates, kept strict-L entries, kept U entries (U counts
+    include the pivots)."""
     rp, ci, v = _csr(A)
     n = len(rp) - 1
-    cap = int((fill + 1.0) * len(v)) + n + 16
+    cap = int((fill + 3.0) * len(v)) + 2 * n + 16
     lrp = np.zeros(n + 1, np.int32); urp = np.zeros(n + 1, np.int32)
     lci = np.zeros(cap, np.int32); uci = np.zeros(cap, np.int32)
     lv = np.zeros(cap); uv = np.zeros(cap)
-    rc = lib().or_parilut(n, rp, ci, v, sweeps, fill, droptol, cap, lrp, lci, lv, urp, uci, uv)
+    tr = np.zeros(4 * max(sweeps, 1), np.int32)
+    rc = lib().or_parilut(n, rp, ci, v, sweeps, fill, droptol, cap, lrp, lci, lv, urp, uci, uv, tr)
     if rc:
         raise ValueError("ParILUT capacity exceeded")
-    return ((lrp, lci[:lrp[n]].copy(), lv[:lrp[n]].copy()),
-            (urp, uci[:urp[n]].copy(), uv[:urp[n]].copy()))
+    out = ((lrp, lci[:lrp[n]].copy(), lv[:lrp[n]].copy()),
+           (urp, uci[:urp[n]].copy(), uv[:urp[n]].copy()))
+    return out + (tr.reshape(-1, 4)[:sweeps].copy(),) if trace else out
+
+
+def batch_cg(sys_ptr, A, b, tol=1e-8, maxiter=5000, x0=None):
+    """CG + Jacobi on every diagonal block of a block-diagonal system.
+    Returns (x, iterations per system, final true relative residual per
+    system, number not converged)."""
+    rp, ci, v = _csr(A)
+    sp_ = np.ascontiguousarray(sys_ptr, np.int32)
+    nsys = len(sp_) - 1
+    x = np.zeros(len(rp) - 1) if x0 is None else np.array(x0, np.float64, copy=True)
+    its = np.zeros(nsys, np.int32)
+    rel = np.zeros(nsys)
+    bad = lib().or_batch_cg(nsys, sp_, rp, ci, v, np.ascontiguousarray(b, np.float64), x, tol,
+                            maxiter, its, rel)
+    return x, its, rel, bad
 
 
 def solve(A, b, ksp="cg", pc="jacobi", tol=1e-8, abstol=0.0, maxiter=1000,

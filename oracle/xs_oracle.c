@@ -14,17 +14,34 @@
 #include <string.h>
 
 /* ------------------------------------------------------------------------ */
-/* BLAS-1 helpers (sequential summation order)                              */
+/* BLAS-1 helpers.  Loops over vector entries and matrix rows run on OpenMP  */
+/* threads; a dot product sums fixed blocks of OR_BLK entries sequentially   */
+/* and then the block sums in block order, so every result is the same for  */
+/* any thread count (the serial build gives the same bits).                 */
 /* ------------------------------------------------------------------------ */
+#define OR_BLK 4096
 static double dot(int n, const double *a, const double *b) {
+  const int nb = (n + OR_BLK - 1) / OR_BLK;
+  double stackp[64];
+  double *part = nb <= 64 ? stackp : (double *)malloc(sizeof(double) * (size_t)nb);
+#pragma omp parallel for schedule(static) if (nb > 8)
+  for (int k = 0; k < nb; k++) {
+    const int e = (k + 1) * OR_BLK < n ? (k + 1) * OR_BLK : n;
+    double s = 0.0;
+    for (int i = k * OR_BLK; i < e; i++) s += a[i] * b[i];
+    part[k] = s;
+  }
   double s = 0.0;
-  for (int i = 0; i < n; i++) s += a[i] * b[i];
+  for (int k = 0; k < nb; k++) s += part[k];
+  if (part != stackp) free(part);
   return s;
 }
 static void axpy(int n, double a, const double *x, double *y) {
+#pragma omp parallel for schedule(static) if (n > 32768)
   for (int i = 0; i < n; i++) y[i] += a * x[i];
 }
 static void scal(int n, double a, double *x) {
+#pragma omp parallel for schedule(static) if (n > 32768)
   for (int i = 0; i < n; i++) x[i] *= a;
 }
 static void copy(int n, const double *x, double *y) { memcpy(y, x, sizeof(double) * (size_t)n); }
@@ -42,6 +59,7 @@ static void apply_M(or_precond_fn M, void *ctx, int n, const double *x, double *
  * accumulated in storage order and the diagonal term is added last. */
 void or_spmv(int n, const int *rp, const int *ci, const double *v,
              const double *x, double *y) {
+#pragma omp parallel for schedule(static) if (n > 16384)
   for (int i = 0; i < n; i++) {
     double r = 0.0, d = 0.0;
     for (int e = rp[i]; e < rp[i + 1]; e++) {
@@ -55,6 +73,7 @@ void or_spmv(int n, const int *rp, const int *ci, const double *v,
 /* y = b - A x.  intel/HostMatrix.h:105-125 (r = rhs; r -= offdiag; y = r - diag*x). */
 void or_bmax(int n, const int *rp, const int *ci, const double *v,
              const double *b, const double *x, double *y) {
+#pragma omp parallel for schedule(static) if (n > 16384)
   for (int i = 0; i < n; i++) {
     double r = b[i], d = 0.0;
     for (int e = rp[i]; e < rp[i + 1]; e++) {
@@ -122,6 +141,7 @@ int or_jacobi_setup(int n, const int *rp, const int *ci, const double *v,
 typedef struct { int n; const double *dinv; } or_jac_ctx_t;
 void or_jacobi_apply(void *ctx, const double *x, double *y) {
   const or_jac_ctx_t *c = (const or_jac_ctx_t *)ctx;
+#pragma omp parallel for schedule(static) if (c->n > 32768)
   for (int i = 0; i < c->n; i++) y[i] = c->dinv[i] * x[i];
 }
 
@@ -221,21 +241,25 @@ void or_ilu_sweeps(int n, const int *rp, const int *ci, const double *luval, con
   double *tmp = (double *)malloc(sizeof(double) * (size_t)n);
   if (kl == 0) memcpy(t, x, sizeof(double) * (size_t)n);
   for (int k = 0; k < kl; k++) {
+#pragma omp parallel for schedule(static) if (n > 16384)
     for (int i = 0; i < n; i++) {
       double r = 0.0;
       for (int e = rp[i]; e < uptr[i]; e++) r += luval[e] * t[ci[e]];
       tmp[i] = r;
     }
+#pragma omp parallel for schedule(static) if (n > 32768)
     for (int i = 0; i < n; i++) t[i] = x[i] - tmp[i];
   }
   if (ku == 0)
     for (int i = 0; i < n; i++) u[i] = (1.0 / luval[uptr[i]]) * t[i];
   for (int k = 0; k < ku; k++) {
+#pragma omp parallel for schedule(static) if (n > 16384)
     for (int i = 0; i < n; i++) {
       double r = 0.0;
       for (int e = uptr[i] + 1; e < rp[i + 1]; e++) r += luval[e] * u[ci[e]];
       tmp[i] = r;
     }
+#pragma omp parallel for schedule(static) if (n > 32768)
     for (int i = 0; i < n; i++) u[i] = (1.0 / luval[uptr[i]]) * (t[i] - tmp[i]);
   }
   memcpy(y, u, sizeof(double) * (size_t)n);
@@ -526,11 +550,13 @@ int or_bicgstab(int n, const int *rp, const int *ci, const double *v,
     double beta = dot(n, rh, r);
     double omega = (beta * gama) / (alpha * beto + 1e-40);
     beto = beta;
+#pragma omp parallel for schedule(static) if (n > 32768)
     for (int i = 0; i < n; i++) p[i] = r[i] + omega * (p[i] - alpha * u[i]);
     apply_M(M, Mctx, n, p, z);
     or_spmv(n, rp, ci, v, z, u);
     double tmp = dot(n, rh, u);
     gama = beta / tmp;
+#pragma omp parallel for schedule(static) if (n > 32768)
     for (int i = 0; i < n; i++) s[i] = r[i] - gama * u[i];
     apply_M(M, Mctx, n, s, z1);
     or_spmv(n, rp, ci, v, z1, vv);
@@ -539,6 +565,7 @@ int or_bicgstab(int n, const int *rp, const int *ci, const double *v,
        half step already solved the system (s = 0, so v = 0) that is 0/0 and
        every later value is NaN -- take alpha = 0, keeping x + gama z */
     alpha = t1 != 0.0 ? t0 / t1 : 0.0;
+#pragma omp parallel for schedule(static) if (n > 32768)
     for (int i = 0; i < n; i++) x[i] = x[i] + gama * z[i] + alpha * z1[i];
     copy(n, s, r);
     axpy(n, -alpha, vv, r);
@@ -716,6 +743,45 @@ done:
   return rc;
 }
 
+/* Independent systems (the diagonal blocks sys_ptr[s]..sys_ptr[s+1] of a
+ * block-diagonal CSR matrix), each solved by or_cg with Jacobi
+ * (intel/mpisolver-C-v1.cpp:193-272 per system; the batch itself has no
+ * reference counterpart).  Systems run on OpenMP threads, one system per
+ * thread at a time (the solves inside are serial), so results do not depend
+ * on the thread count.  iters[s] = iterations, relres[s] = ||b - A x|| / ||b||
+ * recomputed at the end; returns the number of systems that did not
+ * converge. */
+int or_batch_cg(int nsys, const int *sys_ptr, const int *rp, const int *ci, const double *v,
+                const double *b, double *x, double tol, int maxiter, int *iters,
+                double *relres) {
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bad)
+  for (int s = 0; s < nsys; s++) {
+    const int r0 = sys_ptr[s], m = sys_ptr[s + 1] - r0;
+    const int e0 = rp[r0], ne = rp[r0 + m] - e0;
+    int *lrp = malloc(sizeof(int) * (size_t)(m + 1));
+    int *lci = malloc(sizeof(int) * (size_t)(ne + 1));
+    double *dinv = malloc(sizeof(double) * (size_t)(m + 1));
+    double *res = malloc(sizeof(double) * (size_t)(m + 1));
+    for (int i = 0; i <= m; i++) lrp[i] = rp[r0 + i] - e0;
+    for (int e = 0; e < ne; e++) lci[e] = ci[e0 + e] - r0;
+    const double *lv = v + e0;
+    or_jac_ctx_t jac = {m, dinv};
+    int it = 0, rc;
+    if (or_jacobi_setup(m, lrp, lci, lv, 0.0, dinv)) rc = -1;
+    else rc = or_cg(m, lrp, lci, lv, b + r0, x + r0, tol, 0.0, maxiter, or_jacobi_apply, &jac,
+                    NULL, &it);
+    or_bmax(m, lrp, lci, lv, b + r0, x + r0, res);
+    double rn = 0.0, bn = 0.0;
+    for (int i = 0; i < m; i++) { rn += res[i] * res[i]; bn += b[r0 + i] * b[r0 + i]; }
+    iters[s] = it;
+    relres[s] = bn > 0.0 ? sqrt(rn / bn) : 0.0;
+    bad += rc != 0;
+    free(lrp); free(lci); free(dinv); free(res);
+  }
+  return bad;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Driver: build the preconditioner named by pc and run the solver named by  */
 /* ksp, the way hip/xsolver_c.cpp:150-292 wires them (Precond_jacobi,        */
@@ -725,7 +791,7 @@ done:
 static int streq(const char *a, const char *b) { return strcmp(a, b) == 0; }
 int or_parilut(int n, const int *arp, const int *aci, const double *av, int sweeps,
                double fill, double droptol, int cap, int *lrp, int *lci, double *lv,
-               int *urp, int *uci, double *uv);
+               int *urp, int *uci, double *uv, int *trace);
 
 int or_solve(const char *ksp, const char *pc, int n, const int *rp,
              const int *ci, const double *v, const double *b, double *x,
@@ -783,12 +849,12 @@ int or_solve(const char *ksp, const char *pc, int n, const int *rp,
   } else if (streq(pc, "parilut")) {
     /* ParILUT: 10 outer steps (hip/Precond/precon_gpuilut.cpp:5), fill factor
      * ilut_fill, relative drop tolerance ilut_tol; split as seqilut does. */
-    int cap = (int)((ilut_fill + 1.0) * rp[n]) + n + 16;
+    int cap = (int)((ilut_fill + 3.0) * rp[n]) + 2 * n + 16;
     lrp = malloc(sizeof(int) * (n + 1)); urp = malloc(sizeof(int) * (n + 1));
     lci = malloc(sizeof(int) * (size_t)cap); uci = malloc(sizeof(int) * (size_t)cap);
     lv = malloc(sizeof(double) * (size_t)cap); uv = malloc(sizeof(double) * (size_t)cap);
     dinv = malloc(sizeof(double) * n);
-    if (or_parilut(n, rp, ci, v, 10, (double)ilut_fill, ilut_tol, cap, lrp, lci, lv, urp, uci, uv))
+    if (or_parilut(n, rp, ci, v, 10, (double)ilut_fill, ilut_tol, cap, lrp, lci, lv, urp, uci, uv, NULL))
       goto out;
     /* U rows: diagonal first; strict part scaled by 1/u_ii, stored from the
      * second entry on (the split of hip/HostMatrix/HostMatrixCSR.cpp:230-268) */
@@ -829,29 +895,40 @@ out:
 /* Restates the structure of hip/Precond/parilut.cpp:621-1206 (parilut_clean):
  * per outer step s (of `sweeps`):
  *   1. candidates C_i = pattern(A_i) u L_i u U_i u (union over k in L_i of U_k)
- *      (the pattern of A - L U, :760-767);
+ *      (the pattern of A - L U with L's unit diagonal, :760-767);
  *   2. new candidate entries are initialised from R = A - L U:
  *      l_ij = r_ij / u_jj, u_ij = r_ij; existing entries keep their values
  *      (embedA_intoB_L2/U2, :178-233);
- *   3. one fixed-point sweep over the candidates (LU_sweep, :508);
- *   4. per factor, entries below the threshold are dropped: the threshold is
- *      the value of rank (count - keep) in ascending |v| (find_kth_element,
- *      select_kth.h:501-634; thresold_remove2, parilut.cpp:303-384) with
- *      keep = (1 + (fill - 1) (s + 1) / sweeps) * (strict nnz of that factor
- *      in A) (the reference's ramp, :692, with fill = 3); the diagonal of U
- *      is always kept; additionally |v| < droptol * ||a_i||_2 is dropped
- *      (the dual-threshold rule of ilutp_final, ilut_diag_in_csr.cpp:67-80);
+ *   3. one fixed-point sweep over the candidates (LU_sweep, :915-919);
+ *   4. selection (:921-1085), on the magnitudes of the unscaled factors:
+ *      - nothing is dropped by rank unless the candidate count exceeds
+ *        round(rate * nnz(A)) (:926-931), rate = 1 + (fill - 1)(s + 1)/sweeps
+ *        (:692);
+ *      - L's set holds its strict entries and the n unit diagonals (value 1,
+ *        last in each L row, :91-123); U's set its entries with the pivots.
+ *        keep = (int)(rate * base) with base = that factor's count in A,
+ *        diagonal included (l0nnz / u0nnz, :681-682, :1008-1013); the
+ *        threshold is the value of rank (count - keep) in ascending |v|
+ *        (find_kth_element, select_kth.h:501-634);
+ *      - an entry survives when |v| >= threshold; the diagonals always do
+ *        (thresold_remove_nnz2 / thresold_remove2, :303-384);
+ *      - additionally |v| < droptol * ||a_i||_2 is dropped, the dual-threshold
+ *        rule of ilutp_final (ilut_diag_in_csr.cpp:67-80; the GPU ParILUT of
+ *        the reference has no drop tolerance, droptol = 0 reproduces it);
  *   5. one fixed-point sweep over the kept pattern (L_sweep/U_sweep, :1155-1169).
  * Sweeps are Jacobi (old values in, new values out); the reference sweeps
  * asynchronously in place.  The sum over k in each sweep runs in ascending k.
  * Factors: L strict lower CSR (unit diagonal implied), U upper CSR including
  * the diagonal (first in each row).  Returns 0, -1 on capacity overflow.
- * The factorisation runs on the symmetrically scaled matrix
- * S A S, S = |diag A|^{-1/2} (the reference starts from the unscaled values;
- * from that start the synchronous sweep diverges on the 27-point lognormal
- * systems), and the factors are scaled back: l_ij = l~_ij s_j / s_i,
- * u_ij = u~_ij / (s_i s_j).  ILU is equivariant under diagonal scaling, so
- * the fixed points are unchanged. */
+ * The sweeps run on the symmetrically scaled matrix S A S,
+ * S = |diag A|^{-1/2} (the reference sweeps the unscaled values; from that
+ * start the synchronous sweep diverges on the 27-point lognormal systems);
+ * ILU is equivariant under diagonal scaling (L = S^-1 L~ S, U = S^-1 U~ S^-1),
+ * so the fixed points are unchanged, and the selection looks at the unscaled
+ * values l_ij = l~_ij (s_j / s_i), u_ij = u~_ij / (s_i s_j), as the
+ * reference's does.
+ * trace (optional, 4 ints per step): strict-L candidates, U candidates
+ * (pivots included), kept strict-L entries, kept U entries (pivots included). */
 typedef struct { int n; int *rp, *ci; double *v; int cap; } or_fac_t;
 
 static int fac_find(const int *ci, int a, int b, int col) {
@@ -926,7 +1003,7 @@ static double kth_abs(const double *v, int m, int k) {
 
 int or_parilut(int n, const int *arp, const int *aci, const double *av_in, int sweeps,
                double fill, double droptol, int cap, int *lrp, int *lci, double *lv,
-               int *urp, int *uci, double *uv) {
+               int *urp, int *uci, double *uv, int *trace) {
   or_fac_t L = {n, lrp, lci, lv, cap}, U = {n, urp, uci, uv, cap};
   double *sc = malloc(sizeof(double) * n);
   double *av = malloc(sizeof(double) * (size_t)(arp[n] + 1));
@@ -946,11 +1023,13 @@ int or_parilut(int n, const int *arp, const int *aci, const double *av_in, int s
     }
     lrp[i + 1] = nl0; urp[i + 1] = nu0;
   }
-  const int strictU0 = nu0 - n;
-  double *anorm = malloc(sizeof(double) * n);
+  /* keep bases with the diagonals: nnz of L with its unit diagonal and of U
+     (parilut.cpp:681-682) */
+  const long baseL = (long)nl0 + n, baseU = (long)nu0;
+  double *anorm = malloc(sizeof(double) * n);   /* ||a_i||_2 of the unscaled rows */
   for (int i = 0; i < n; i++) {
     double s = 0.0;
-    for (int e = arp[i]; e < arp[i + 1]; e++) s += av[e] * av[e];
+    for (int e = arp[i]; e < arp[i + 1]; e++) s += av_in[e] * av_in[e];
     anorm[i] = sqrt(s);
   }
   int *mark = malloc(sizeof(int) * n);
@@ -1007,33 +1086,35 @@ int or_parilut(int n, const int *arp, const int *aci, const double *av_in, int s
     or_fac_t CL = {n, nlrp, nlci, nlv, cap}, CU = {n, nurp, nuci, nuv, cap};
     /* 3: sweep over the candidates */
     fac_sweep(n, arp, aci, av, &CL, &CU);
-    /* 4: thresholds */
+    /* 4: thresholds on the unscaled magnitudes */
     double rate = 1.0 + (fill - 1.0) * (double)(s + 1) / (double)sweeps;
-    long keepL = (long)(rate * (double)nl0), keepU = (long)(rate * (double)strictU0);
+    long keepL = (long)(rate * (double)baseL), keepU = (long)(rate * (double)baseU);
+    long keep_all = lround(rate * (double)arp[n]);
     double tL = 0.0, tU = 0.0;
-    if (nl > keepL) tL = kth_abs(nlv, nl, (int)(nl - keepL));
-    {
-      double *su = malloc(sizeof(double) * (size_t)(nu + 1));
-      int m = 0;
-      for (int i = 0; i < n; i++)
-        for (int e = nurp[i] + 1; e < nurp[i + 1]; e++) su[m++] = nuv[e];
-      if (m > keepU) tU = kth_abs(su, m, (int)(m - keepU));
-      free(su);
+    /* |l_ij| and |u_ij| of the unscaled factors; L's n unit diagonals after them */
+    double *ml = malloc(sizeof(double) * (size_t)(nl + n + 1));
+    double *mu = malloc(sizeof(double) * (size_t)(nu + 1));
+    for (int i = 0; i < n; i++) {
+      for (int e = nlrp[i]; e < nlrp[i + 1]; e++) ml[e] = fabs(nlv[e] * (sc[nlci[e]] / sc[i]));
+      for (int e = nurp[i]; e < nurp[i + 1]; e++) mu[e] = fabs(nuv[e] / (sc[i] * sc[nuci[e]]));
+      ml[nl + i] = 1.0;
+    }
+    if ((long)nl + nu - keep_all > 0) {
+      if ((long)nl + n > keepL) tL = kth_abs(ml, nl + n, (int)(nl + n - keepL));
+      if ((long)nu > keepU) tU = kth_abs(mu, nu, (int)(nu - keepU));
     }
     int ol = 0, ou = 0;
     for (int i = 0; i < n; i++) {
       double dt = droptol * anorm[i];
-      for (int e = nlrp[i]; e < nlrp[i + 1]; e++) {
-        double a = fabs(nlv[e]);
-        if (a >= tL && a >= dt) { L.ci[ol] = nlci[e]; L.v[ol++] = nlv[e]; }
-      }
+      for (int e = nlrp[i]; e < nlrp[i + 1]; e++)
+        if (ml[e] >= tL && ml[e] >= dt) { L.ci[ol] = nlci[e]; L.v[ol++] = nlv[e]; }
       L.rp[i + 1] = ol;
-      for (int e = nurp[i]; e < nurp[i + 1]; e++) {
-        double a = fabs(nuv[e]);
-        if (nuci[e] == i || (a >= tU && a >= dt)) { U.ci[ou] = nuci[e]; U.v[ou++] = nuv[e]; }
-      }
+      for (int e = nurp[i]; e < nurp[i + 1]; e++)
+        if (nuci[e] == i || (mu[e] >= tU && mu[e] >= dt)) { U.ci[ou] = nuci[e]; U.v[ou++] = nuv[e]; }
       U.rp[i + 1] = ou;
     }
+    free(ml); free(mu);
+    if (trace) { trace[4 * s] = nl; trace[4 * s + 1] = nu; trace[4 * s + 2] = ol; trace[4 * s + 3] = ou; }
     free(nlrp); free(nurp); free(nlci); free(nuci); free(nlv); free(nuv);
     /* 5: sweep over the kept pattern */
     fac_sweep(n, arp, aci, av, &L, &U);

@@ -82,10 +82,15 @@ int or_gmres(int n, const int *rp, const int *ci, const double *v,
              int flexible, or_precond_fn M, void *Mctx, double *resvec,
              int *usediter);
 
+/* ---- independent systems (block-diagonal CSR), CG + Jacobi each --------- */
+int or_batch_cg(int nsys, const int *sys_ptr, const int *rp, const int *ci, const double *v,
+                const double *b, double *x, double tol, int maxiter, int *iters,
+                double *relres);
+
 /* ---- ParILUT (deterministic synchronous form) --------------------------- */
 int or_parilut(int n, const int *arp, const int *aci, const double *av, int sweeps,
                double fill, double droptol, int cap, int *lrp, int *lci, double *lv,
-               int *urp, int *uci, double *uv);
+               int *urp, int *uci, double *uv, int *trace);
 
 /* ---- driver: pc in {"none","jacobi","ilu0","ilut"},
  *      ksp in {"cg","bicgstab","gcr","gmres","fgmres"} ------------------- */

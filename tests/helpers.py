@@ -53,3 +53,38 @@ def rel_err(a, b):
 
 def true_relres(A, b, x):
     return float(np.linalg.norm(b - gen.spmv(A, x)) / np.linalg.norm(b))
+
+
+# north-star acceptance of the plain FP64 path: the solution within 1e-10
+# relative of the oracle's FP64 solution on the same seeded input
+X_TOL = 1e-10
+# residual histories: every entry within this relative deviation of the
+# oracle's (absolute norms, entry 0 = ||r0||)
+RES_RTOL = 1e-6
+
+
+def resvec_dev(res, reso):
+    m = min(len(res), len(reso))
+    if m == 0:
+        return 0.0
+    den = np.maximum(np.abs(reso[:m]), 1e-300)
+    return float(np.max(np.abs(res[:m] - reso[:m]) / den))
+
+
+def check_fp64_parity(tag, A, b, x, it, res, xo, ito, reso, tol, it_slack=2,
+                      true_res=True):
+    """The acceptance of the plain FP64 path against the oracle run on the
+    same input: iterations within +-it_slack, true relative residual below
+    tol (recomputed in FP64 on the host), ||x - x_oracle|| / ||x_oracle|| <=
+    X_TOL, and the whole residual history within RES_RTOL of the oracle's."""
+    rr = true_relres(A, b, x)
+    dx = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
+    dres = resvec_dev(res, reso)
+    say(f"   {tag}: oracle it={ito} gpu it={it} true relres={rr:.3e} "
+        f"x-diff={dx:.2e} resvec dev={dres:.2e}")
+    assert abs(it - ito) <= it_slack
+    if true_res:
+        assert rr < tol
+    assert dx <= X_TOL
+    assert len(res) == it + 1
+    assert dres <= RES_RTOL

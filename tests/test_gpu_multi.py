@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 import xsolver_amd
-from tests.helpers import random_sparse, rel_err, say, split_factors, true_relres
+from tests.helpers import check_fp64_parity, random_sparse, rel_err, say, split_factors, true_relres
 from xsolver_amd import XSolver, gen
 
 pytestmark = pytest.mark.gpu
@@ -205,7 +205,7 @@ def test_block_solve_with_sweep_apply(dev):
                                        lsweeps=5, usweeps=2)
         rr = true_relres(A, B[:, j], X[:, j])
         say(f"   col {j}: oracle it={ito} gpu it={its[j]} relres={rr:.2e}")
-        assert sto == 0 and abs(int(its[j]) - ito) <= 2 and rr < 1.05e-10
+        assert sto == 0 and abs(int(its[j]) - ito) <= 2 and rr < 1e-10
     S.close()
 
 
@@ -242,22 +242,18 @@ def test_block_solve_matches_oracle_per_column(dev, case):
     S.pc_settype(pc)
     if pc == "ilu0":
         S.set_option("parilu_async", 0)
-        S.set_option("parilu_sweeps", 60)
+        S.set_option("parilu_sweeps", 200)
+        S.set_option("parilu_scale", 0)
     X, st = S.solve_multi(B)
     its, r0, rl = S.multi_results()
     assert st == xsolver_amd.XS_OK and len(its) == k
     for j in range(k):
         xo, sto, ito, reso = oracle.solve(A, B[:, j], ksp, pc, tol=tol, maxiter=maxiter)
-        rr = true_relres(A, B[:, j], X[:, j])
-        say(f"   col {j}: oracle it={ito} gpu it={its[j]} relres={rr:.2e} "
-            f"x-diff={rel_err(X[:, j], xo):.2e}")
         assert sto == 0
-        assert abs(int(its[j]) - ito) <= 2
-        assert rr < tol * 1.05
-        assert np.linalg.norm(X[:, j] - xo) / np.linalg.norm(xo) < 50 * tol
         assert r0[j] == pytest.approx(np.linalg.norm(B[:, j]), rel=1e-12)
         res = S.multi_resvec(j)
-        assert len(res) == its[j] + 1 and res[-1] == pytest.approx(rl[j])
+        assert res[-1] == pytest.approx(rl[j])
+        check_fp64_parity(f"col {j}", A, B[:, j], X[:, j], int(its[j]), res, xo, ito, reso, tol)
         n = min(len(res), len(reso), 20)
         assert np.allclose(res[:n], reso[:n], rtol=1e-6)
     S.close()
@@ -317,7 +313,7 @@ def test_block_per_column_masks(dev):
     assert its[1] == 0 and its[3] == 0 and its[0] > 0 and its[2] > 0
     assert np.array_equal(X[:, 1], np.zeros(B.shape[0]))
     for j in (0, 2):
-        assert true_relres(A, B[:, j], X[:, j]) < 1.05e-10
+        assert true_relres(A, B[:, j], X[:, j]) < 1e-10
     S.close()
 
 

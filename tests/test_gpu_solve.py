@@ -1,7 +1,8 @@
 """GPU solver parity through the C-ABI against the oracle (same seeded
 matrices, same x0 = 0): iterations within +-2 (HIP conventions: iterations
-performed, absolute resvec), true FP64 relative residual below rtol, and the
-solution close to the oracle's solution."""
+performed, absolute resvec), true FP64 relative residual below rtol, the
+solution within 1e-10 relative of the oracle's FP64 solution and the whole
+residual history within 1e-6 of the oracle's."""
 import ctypes
 
 import numpy as np
@@ -9,7 +10,7 @@ import pytest
 
 import oracle
 import xsolver_amd
-from tests.helpers import rel_err, say, true_relres
+from tests.helpers import X_TOL, check_fp64_parity, rel_err, say, true_relres
 from xsolver_amd import XSolver, XSolverError, gen
 
 pytestmark = pytest.mark.gpu
@@ -47,21 +48,16 @@ def test_solve_parity(dev, case, fmt):
     xs, b = gen.manufactured(A, seed)
     say(f"[solve] {name} {fmt} n={len(xs)}")
     xo, sto, ito, reso = oracle.solve(A, b, ksp, pc, tol=tol, maxiter=maxiter)
-    kw = {"parilu_async": 0, "parilu_sweeps": 60} if pc == "ilu0" else {}
+    # deterministic ParILU run to its fixed point on the unscaled matrix:
+    # the oracle's ILU(0) to rounding
+    kw = {"parilu_async": 0, "parilu_sweeps": 200, "parilu_scale": 0} if pc == "ilu0" else {}
     kw["spmv_kernel"] = 1 if fmt == "vec" else 0
     g = gpu_solve(A, b, ksp, pc, tol, maxiter, ell=1 if fmt == "ell" else 0, **kw)
-    say(f"   oracle it={ito} gpu it={g['it']} status={g['st']} "
-        f"x-diff={rel_err(g['x'], xo):.2e} relres={true_relres(A, b, g['x']):.2e}")
     assert sto == 0 and g["st"] == xsolver_amd.XS_OK
-    assert abs(g["it"] - ito) <= 2
-    assert true_relres(A, b, g["x"]) < tol * 1.05
-    assert np.linalg.norm(g["x"] - xo) / np.linalg.norm(xo) < 50 * tol
     res = g["res"]
-    assert len(res) == g["it"] + 1
     assert res[0] == pytest.approx(np.linalg.norm(b), rel=1e-12)
     assert res[-1] / res[0] < tol
-    n = min(len(res), len(reso), 20)
-    assert np.allclose(res[:n], reso[:n], rtol=1e-6)
+    check_fp64_parity(name, A, b, g["x"], g["it"], res, xo, ito, reso, tol)
     g["S"].close()
 
 
@@ -71,9 +67,14 @@ def test_parilu_async_reference_default(dev):
     xs, b = gen.manufactured(A, 1)
     xo, sto, ito, _ = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=1000)
     g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000)
-    say(f"[async] oracle it={ito} gpu it={g['it']}")
+    rr = true_relres(A, b, g["x"])
+    dx = np.linalg.norm(g["x"] - xo) / np.linalg.norm(xo)
+    say(f"[async] oracle it={ito} gpu it={g['it']} relres={rr:.2e} x-diff={dx:.2e}")
+    # 20 asynchronous sweeps approximate ILU(0): same iterations, FP64
+    # residual below rtol; the solution agrees with the exact-ILU(0) oracle
+    # to the accuracy the residual allows
     assert g["st"] == 0 and abs(g["it"] - ito) <= 2
-    assert true_relres(A, b, g["x"]) < 1.05e-10
+    assert rr < 1e-10 and dx < 1e-8
     g["S"].close()
 
 
@@ -204,7 +205,7 @@ def test_legacy_singleton_api(dev):
     assert used.value == L.xsolver_get_iterations()
     assert resvec[0] == pytest.approx(np.linalg.norm(b))
     assert resvec[used.value] / resvec[0] < 1e-10
-    assert true_relres(A, b, sol) < 1.05e-10
+    assert true_relres(A, b, sol) < 1e-10
     L.xsolver_free()
     L.xsolver_ksp_settype(b"cg")
     assert L.xsolver_last_error() == xsolver_amd.XS_ERR_STATE
@@ -228,20 +229,16 @@ def test_basis_krylov_parity(dev, case):
     xs, b = gen.manufactured(A, seed)
     say(f"[krylov] {name} n={len(xs)}")
     xo, sto, ito, reso = oracle.solve(A, b, ksp, pc, tol=tol, maxiter=maxiter, restart=restart)
-    kw = {"parilu_async": 0, "parilu_sweeps": 60} if pc == "ilu0" else {}
+    kw = {"parilu_async": 0, "parilu_sweeps": 200, "parilu_scale": 0} if pc == "ilu0" else {}
     g = gpu_solve(A, b, ksp, pc, tol, maxiter, restart=restart, **kw)
-    rr = true_relres(A, b, g["x"])
-    say(f"   oracle it={ito} gpu it={g['it']} status={g['st']} x-diff={rel_err(g['x'], xo):.2e} relres={rr:.2e}")
     assert sto == 0 and g["st"] == xsolver_amd.XS_OK
-    assert abs(g["it"] - ito) <= 2
     res = g["res"]
     assert res[-1] / res[0] < tol
-    n = min(len(res), len(reso), 15)
-    assert np.allclose(res[:n], reso[:n], rtol=1e-6)
-    # left GMRES measures the preconditioned residual: check the true one loosely
-    bound = tol * (1.05 if ksp in ("fgmres", "gcr") else 1e3)
-    assert rr < bound
-    assert np.linalg.norm(g["x"] - xo) / np.linalg.norm(xo) < 1e3 * tol
+    # left-preconditioned GMRES converges on ||M^-1 r|| (the reference's
+    # measure, intel/mpisolver-C-v1.cpp:933-1087): its true residual is not
+    # the stopping test; the solution still matches the oracle's
+    check_fp64_parity(name, A, b, g["x"], g["it"], res, xo, ito, reso, tol,
+                      true_res=ksp != "gmres")
     g["S"].close()
 
 
@@ -261,7 +258,7 @@ def test_mixed_precision_reaches_fp64_accuracy(dev, pc, factor):
     err = np.linalg.norm(g["x"] - xo) / np.linalg.norm(xo)
     say(f"   oracle it={ito} mixed inner it={g['it']} outer={len(g['res']) - 1} relres={rr:.2e} err-vs-fp64={err:.2e}")
     assert g["st"] == 0
-    assert rr < 1.05e-10
+    assert rr < 1e-10
     assert err < 1e-7
     res = g["res"]
     assert res[-1] / res[0] < 1e-10 and len(res) >= 2
@@ -288,7 +285,7 @@ def test_mixed_reliable_updates(dev, pc, apply):
         f"(replacements {len(rel['res']) - 1}) refinement it={ref['it']} relres {rr_rel:.2e} / {rr_ref:.2e}")
     for g in (f64, rel, ref):
         assert g["st"] == 0
-    assert rr_rel < 1.05e-10 and rr_ref < 1.05e-10
+    assert rr_rel < 1e-10 and rr_ref < 1e-10
     assert rel["it"] <= 1.15 * f64["it"] + 2
     res = rel["res"]
     assert len(res) >= 2 and res[-1] / res[0] < 1e-10
@@ -329,7 +326,7 @@ def test_mixed_copies_follow_options(dev):
     x4, st4 = S.solve(b)
     assert st4 == 0 and np.array_equal(x4, fresh(factor_storage=0, spmv_kernel=1))
     for x in (x1, x3, x4):
-        assert true_relres(A, b, x) < 1.05e-10
+        assert true_relres(A, b, x) < 1e-10
     S.close()
 
 
@@ -362,7 +359,7 @@ def test_parilut_exact_apply_after_sweeps(dev):
     say(f"[parilut-lazy] sweeps it={sta} exact-after it={S.iterations} fresh exact it={T.iterations}")
     assert sta == 0 and stb == 0 and stc == 0
     assert np.array_equal(xb, xc)
-    assert true_relres(A, b, xa) < 1.05e-10 and true_relres(A, b, xb) < 1.05e-10
+    assert true_relres(A, b, xa) < 1e-10 and true_relres(A, b, xb) < 1e-10
     S.close()
     T.close()
 
@@ -386,7 +383,7 @@ def test_mixed_gmres_parilut(dev, factor):
     rr = true_relres(A, b, x)
     err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
     say(f"   oracle it={ito} mixed inner it={it} relres={rr:.2e} err-vs-fp64={err:.2e}")
-    assert st == 0 and rr < 1.05e-10 and err < 1e-7
+    assert st == 0 and rr < 1e-10 and err < 1e-7
 
 
 @pytest.mark.parametrize("sweeps", [20, 2])
@@ -398,17 +395,18 @@ def test_iterative_ilu_apply_solve(dev, precision, sweeps):
     refinement)."""
     A = gen.convdiff3d(24)
     xs, b = gen.manufactured(A, 1)
-    xo, sto, ito, _ = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=1000,
-                                   lsweeps=sweeps, usweeps=sweeps)
-    g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000, parilu_async=0, parilu_sweeps=60,
-                  lsolve_sweeps=sweeps, usolve_sweeps=sweeps, precision=precision)
+    xo, sto, ito, reso = oracle.solve(A, b, "bicgstab", "ilu0", tol=1e-10, maxiter=1000,
+                                      lsweeps=sweeps, usweeps=sweeps)
+    g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000, parilu_async=0, parilu_sweeps=200,
+                  parilu_scale=0, lsolve_sweeps=sweeps, usolve_sweeps=sweeps,
+                  precision=precision)
     say(f"[sweep-solve] precision={precision} sweeps={sweeps} oracle it={ito} gpu it={g['it']} "
         f"relres={true_relres(A, b, g['x']):.2e}")
     assert sto == 0 and g["st"] == 0
     if precision == 0:
-        assert abs(g["it"] - ito) <= 2
-        assert np.linalg.norm(g["x"] - xo) / np.linalg.norm(xo) < 50 * 1e-10
-    assert true_relres(A, b, g["x"]) < 1.05e-10
+        check_fp64_parity(f"sweeps {sweeps}", A, b, g["x"], g["it"], g["res"], xo, ito, reso,
+                          1e-10)
+    assert true_relres(A, b, g["x"]) < 1e-10
     g["S"].close()
 
 
@@ -423,7 +421,7 @@ def test_mixed_sweep_apply_factor_storage(dev, factor):
                   precision=1, factor_storage=factor)
     rr = true_relres(A, b, g["x"])
     say(f"[mixed-sweeps] factor={factor} inner it={g['it']} relres={rr:.2e}")
-    assert g["st"] == 0 and rr < 1.05e-10
+    assert g["st"] == 0 and rr < 1e-10
     assert np.linalg.norm(g["x"] - xs) / np.linalg.norm(xs) < 1e-6
     g["S"].close()
 
@@ -441,7 +439,7 @@ def test_update_values_then_solve(dev, precision):
     S.set_option("precision", precision)
     _, b = gen.manufactured(A, 1)
     x1, st1 = S.solve(b)
-    assert st1 == 0 and true_relres(A, b, x1) < 1.05e-10
+    assert st1 == 0 and true_relres(A, b, x1) < 1e-10
     v2 = v.copy()
     for i in range(len(rp) - 1):                  # heavier diagonal, new values
         seg = slice(rp[i], rp[i + 1])
@@ -454,7 +452,7 @@ def test_update_values_then_solve(dev, precision):
     x2, st2 = S.solve(b2)
     xo, sto, ito, _ = oracle.solve(A2, b2, "bicgstab", "ilu0", tol=1e-10, maxiter=1000)
     say(f"[update] precision={precision} it={S.iterations} oracle it={ito}")
-    assert st2 == 0 and true_relres(A2, b2, x2) < 1.05e-10
+    assert st2 == 0 and true_relres(A2, b2, x2) < 1e-10
     if precision == 0:
         assert abs(S.iterations - ito) <= 2
     S.close()
@@ -465,7 +463,7 @@ def test_mixed_with_vector_csr(dev):
     A = gen.convdiff3d(24)
     _, b = gen.manufactured(A, 1)
     g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000, precision=1, spmv_kernel=1)
-    assert g["st"] == 0 and true_relres(A, b, g["x"]) < 1.05e-10
+    assert g["st"] == 0 and true_relres(A, b, g["x"]) < 1e-10
     g["S"].close()
 
 

@@ -40,7 +40,7 @@ def test_autotune_choice_and_parity(dev, name, build, tmp_path, monkeypatch):
     S.ksp_setoption(1e-10, 1000)
     S.pc_settype("jacobi")
     xg, st = S.solve(b)
-    assert st == 0 and true_relres(A, b, xg) < 1.05e-10
+    assert st == 0 and true_relres(A, b, xg) < 1e-10
     S.close()
     # a second context on the same matrix takes the cached choice
     T = XSolver()

@@ -320,9 +320,10 @@ def test_parilut_invariants():
         uc = U[1][U[0][i]:U[0][i + 1]]
         assert np.all(lc < i) and np.all(np.diff(lc) > 0)
         assert uc[0] == i and np.all(np.diff(uc) > 0)
-    # per-factor counts within the final keep (ties may add a few)
-    assert len(L[2]) <= 2 * nl0 + 8
-    assert len(U[2]) - n <= 2 * nu0 + 8
+    # final keep: 2 x each factor's count in A with its diagonal
+    # (hip/Precond/parilut.cpp:681-682, 1008-1013)
+    assert len(L[2]) + n <= 2 * (nl0 + n)
+    assert len(U[2]) <= 2 * (nu0 + n)
     # residual A - LU on the kept pattern is small relative to A
     Ld = sp.csr_matrix((L[2], L[1], L[0]), shape=(n, n)) + sp.eye(n)
     Ud = sp.csr_matrix((U[2], U[1], U[0]), shape=(n, n))
@@ -335,6 +336,93 @@ def test_parilut_invariants():
     _, s2, it2, _ = oracle.solve(A, b, "fgmres", "parilut", tol=1e-10, maxiter=300, restart=30,
                                  ilut_tol=1e-4, ilut_fill=2)
     assert s1 == 0 and s2 == 0 and it2 < it1
+
+
+def _parts(A):
+    rp, ci, _ = A
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    return n, int(np.sum(ci < rows)), int(np.sum(ci > rows))
+
+
+@pytest.mark.parametrize("fill,sweeps", [(2.0, 10), (3.0, 5), (1.5, 4), (0.6, 1)])
+def test_parilut_kept_counts_follow_the_ramp(fill, sweeps):
+    """Without a drop tolerance the selection keeps, at every outer step s,
+    exactly keep_s = (int)(rate_s * base) entries per factor, with
+    rate_s = 1 + (fill - 1)(s + 1)/sweeps (hip/Precond/parilut.cpp:692) and
+    base = the factor's count in A including its diagonal (L's unit diagonal
+    counted, :681-682, 1008-1013), whenever the candidates exceed
+    round(rate_s * nnz(A)) (:926-931).  A wrong ramp, an off-by-one rank or
+    the strict base each change these counts."""
+    A = gen.lognormal27(6, seed=3)
+    n, nl0, nu0 = _parts(A)
+    nnz = len(A[2])
+    L, U, tr = oracle.parilut(A, sweeps, fill, 0.0, trace=True)
+    assert tr.shape == (sweeps, 4)
+    for s, (cl, cu, kl, ku) in enumerate(tr):
+        rate = 1.0 + (fill - 1.0) * (s + 1) / sweeps
+        keepL, keepU = int(rate * (nl0 + n)), int(rate * (nu0 + n))
+        if cl + cu - round(rate * nnz) > 0:
+            assert kl + n == min(keepL, cl + n), (s, kl, keepL)
+            assert ku == min(keepU, cu), (s, ku, keepU)
+        else:
+            assert (kl, ku) == (cl, cu)
+    assert len(L[2]) == tr[-1, 2] and len(U[2]) == tr[-1, 3]
+
+
+def test_parilut_fill_below_droptol_gives_ilu0():
+    """Closed form: when every fill-in entry of the factorisation is below
+    droptol * ||a_i||_2 and every entry of A's pattern above it, the dual
+    threshold keeps exactly pattern(A) and the values are ILU(0)'s up to
+    the one sweep that follows each selection (the candidate sweep before it
+    includes the fill, O(1e-6) here, so the last step's values carry an
+    O(1e-12) trace of it)."""
+    rng = np.random.default_rng(11)
+    n = 60
+    M = sp.random(n, n, density=0.08, random_state=12, format="csr")
+    M.data = 1e-3 * (0.5 + rng.random(M.nnz))            # off-diagonal ~1e-3
+    D = sp.diags(1.0 + rng.random(n))
+    S = (M - sp.diags(M.diagonal()) + D).tocsr()
+    S.sort_indices()
+    A = (S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.copy())
+    L, U = oracle.parilut(A, 10, 50.0, 1e-4)              # keep never binds
+    lu, uptr = oracle.ilu0(A)
+    rp, ci, _ = A
+    # pattern identical to A's
+    Lp = sp.csr_matrix((np.ones(len(L[1])), L[1], L[0]), shape=(n, n))
+    Up = sp.csr_matrix((np.ones(len(U[1])), U[1], U[0]), shape=(n, n))
+    assert ((Lp + Up) != (S != 0)).nnz == 0
+    Ls, Us, piv = _dense_factors(A, lu, uptr)
+    Ld = sp.csr_matrix((L[2], L[1], L[0]), shape=(n, n)).toarray()
+    Ud = sp.csr_matrix((U[2], U[1], U[0]), shape=(n, n)).toarray()
+    assert np.abs(Ld - Ls).max() < 1e-10
+    assert np.abs(Ud - (Us + np.diag(piv))).max() < 1e-10
+
+
+def test_parilut_exact_lu_when_fill_fits():
+    """Closed form: a banded matrix whose exact LU fill fits in the keep and
+    lies above droptol gives the exact LU after enough steps."""
+    n, bw = 40, 3
+    rng = np.random.default_rng(5)
+    Dn = np.zeros((n, n))
+    for k in range(-bw, bw + 1):
+        if k:
+            Dn += np.diag(-rng.random(n - abs(k)) - 0.2, k)
+    Dn += np.diag(np.abs(Dn).sum(1) + 1.0)
+    # pattern of A: only the outer band diagonals +-bw and the main diagonal,
+    # so the LU fill (the full band) is new
+    mask = (np.abs(np.subtract.outer(np.arange(n), np.arange(n))) % bw == 0) & \
+        (np.abs(np.subtract.outer(np.arange(n), np.arange(n))) <= bw)
+    Ad = np.where(mask, Dn, 0.0)
+    S = sp.csr_matrix(Ad)
+    S.sort_indices()
+    A = (S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.copy())
+    L, U = oracle.parilut(A, 30, 20.0, 0.0)
+    Ld = np.eye(n) + sp.csr_matrix((L[2], L[1], L[0]), shape=(n, n)).toarray()
+    Ud = sp.csr_matrix((U[2], U[1], U[0]), shape=(n, n)).toarray()
+    P, Le, Ue = sla.lu(Ad)
+    assert np.allclose(P, np.eye(n))
+    assert np.abs(Ld - Le).max() < 1e-12 and np.abs(Ud - Ue).max() < 1e-11
 
 
 def _dense_factors(A, lu, uptr):

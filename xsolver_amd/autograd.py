@@ -2,7 +2,9 @@
 
 forward:  x = A(vals)^{-1} b with the context's Krylov method and
           preconditioner (xs_mat_update_values + xs_solve_device).
-backward: A^T lam = dL/dx by xs_solve_transpose_device (transposed SpMV,
+backward: the forward's values are put back into the context
+          (xs_mat_update_values: a later forward may have replaced them), then
+          A^T lam = dL/dx by xs_solve_transpose_device (transposed SpMV,
           transposed ILU triangular solves), dL/db = lam and
           dL/dA_ij = -lam_i x_j on the sparsity pattern (xs_grad_values_device).
 No reference counterpart.
@@ -18,22 +20,25 @@ class SparseSolve(torch.autograd.Function):
         vals = vals.detach().contiguous()
         b = b.detach().contiguous()
         torch.cuda.synchronize()
-        rc = solver_lib_update(solver, vals)
+        solver_lib_update(solver, vals)
         x = torch.zeros_like(b)
         st = solver.solve_device(b, x)
         if st != 0:
             raise XSolverError(st, f"forward solve did not converge (status {st})")
         ctx.solver = solver
-        ctx.save_for_backward(x)
+        ctx.save_for_backward(x, vals)
         return x
 
     @staticmethod
     def backward(ctx, gx):
-        (x,) = ctx.saved_tensors
+        # the values of this forward's A: the shared context may hold those of
+        # a later forward by now
+        x, vals = ctx.saved_tensors
         solver = ctx.solver
         gx = gx.detach().contiguous()
         lam = torch.zeros_like(gx)
         torch.cuda.synchronize()
+        solver_lib_update(solver, vals)
         st = solver.solve_transpose_device(gx, lam)
         if st != 0:
             raise XSolverError(st, f"adjoint solve did not converge (status {st})")

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1016,9 +1017,11 @@ int xs_parilut_factor(xs_ctx *c) {
   xs_pt_rowidx(s, n, F.rp_own, F.rowidx);
   xs_pt_scale_vec(s, n, c->A.rp, c->A.ci, c->A.v, F.scale);
   xs_pt_rescale(s, nnz, F.rowidx, F.ci_own, F.lu, F.scale, 0);
-  xs_pt_dtol(s, n, c->A.rp, c->A.ci, c->A.v, F.scale, c->ilut_tol, dtol);
+  xs_pt_dtol(s, n, c->A.rp, c->A.v, c->ilut_tol, dtol);
   xs_pt_dpos(s, n, F.rp_own, F.ci_own, F.dpos, ovf);
-  // entries of A's strict parts: the base of the keep counts
+  // entries of A's lower part with the diagonal and of its upper part with
+  // the diagonal: the bases of the keep counts (l0nnz / u0nnz,
+  // hip/Precond/parilut.cpp:681-682)
   unsigned long long base[2] = {0, 0};
   xs_pt_count_parts(s, nnz, F.rowidx, F.ci_own, cnt2);
   XS_HIP(hipMemcpyAsync(base, cnt2, sizeof(base), hipMemcpyDeviceToHost, s));
@@ -1028,8 +1031,9 @@ int xs_parilut_factor(xs_ctx *c) {
   const int steps = c->parilut_steps;
   const double fill = (double)c->ilut_fill;
   for (int step = 0; step < steps; step++) {
+    // keep-rate ramp (hip/Precond/parilut.cpp:692) and keep counts (:1008-1013)
     const double rate = 1.0 + (fill - 1.0) * (double)(step + 1) / (double)steps;
-    const long keepL = (long)(rate * (double)nl0), keepU = (long)(rate * (double)nu0);
+    long keepL = (long)(rate * (double)nl0), keepU = (long)(rate * (double)nu0);
     // 1. candidate pattern
     xs_pt_symbolic(s, false, n, c->A.rp, c->A.ci, F.rp_own, F.ci_own, F.dpos, cnt, nullptr,
                    nullptr, nullptr, ovf);
@@ -1055,11 +1059,15 @@ int xs_parilut_factor(xs_ctx *c) {
                 F.dpos, cv);
     xs_pt_sweep(s, n, crp, cci, cdpos, cv, cv2, c->A.rp, c->A.ci, c->A.v, F.scale);
     std::swap(cv, cv2);
-    // 4. thresholds (the part counts stay on the device)
+    // 4. thresholds (the part counts stay on the device); nothing is
+    // dropped by rank while the candidates number at most round(rate nnz(A))
+    // (hip/Precond/parilut.cpp:926-931)
+    if ((long)nc - std::lround(rate * (double)nnz) <= 0) keepL = keepU = LONG_MAX;
     xs_pt_count_parts(s, nc, crow, cci, cnt2);
-    xs_pt_select(s, nc, crow, cci, cv, cnt2, keepL, keepU, sel, hist, lcnt, list, thr);
+    xs_pt_select(s, nc, crow, cci, cv, F.scale, cnt2, keepL, keepU, sel, hist, lcnt, list, thr);
     // compaction into the new factor
-    xs_pt_keep(s, false, n, crp, cci, cv, thr, dtol, cnt, nullptr, nullptr, nullptr, nullptr);
+    xs_pt_keep(s, false, n, crp, cci, cv, thr, dtol, F.scale, cnt, nullptr, nullptr, nullptr,
+               nullptr);
     xs_scan_excl(s, n, cnt, F.rp_own, bsum);
     XS_HIP(hipMemcpyAsync(&fnnz, F.rp_own + n, sizeof(int), hipMemcpyDeviceToHost, s));
     XS_HIP(hipStreamSynchronize(s));
@@ -1068,7 +1076,8 @@ int xs_parilut_factor(xs_ctx *c) {
     PT_TRY(grow((size_t)fnnz, cap_lu, &F.lu));
     PT_TRY(grow((size_t)fnnz, cap_row, &F.rowidx));
     PT_TRY(grow((size_t)fnnz, cap_v2, &fv2));
-    xs_pt_keep(s, true, n, crp, cci, cv, thr, dtol, nullptr, F.rp_own, F.ci_own, F.lu, F.rowidx);
+    xs_pt_keep(s, true, n, crp, cci, cv, thr, dtol, F.scale, nullptr, F.rp_own, F.ci_own, F.lu,
+               F.rowidx);
     xs_pt_dpos(s, n, F.rp_own, F.ci_own, F.dpos, ovf);
     // 5. sweep over the kept pattern
     xs_pt_sweep(s, n, F.rp_own, F.ci_own, F.dpos, F.lu, fv2, c->A.rp, c->A.ci, c->A.v, F.scale);

@@ -16,24 +16,30 @@ void xs_pt_embed(hipStream_t s, int n, const int *crp, const int *cci, const int
 void xs_pt_sweep(hipStream_t s, int n, const int *rp, const int *ci, const int *dpos,
                  const double *vin, double *vout, const int *arp, const int *aci,
                  const double *av, const double *sc);
-// thresholds of both strict parts: thr[0] (resp. thr[1]) = the value of
-// rank cnt2[0] - keepL (resp. cnt2[1] - keepU) in ascending |v| of the
-// strict lower (upper) entries, 0 when the part has no more than keep
-// entries; sel: xs_pt_select_words() words, hist: 512, lcnt: 2, list:
+// thresholds of both factors on the unscaled magnitudes (sc = the scaling
+// S of the swept values): thr[0] (resp. thr[1]) = the value of rank
+// cnt2[0] - keepL (resp. cnt2[1] - keepU) in ascending magnitude over L with
+// its unit diagonal (U with its pivots), 0 when the factor has no more than
+// keep entries; sel: xs_pt_select_words() words, hist: 512, lcnt: 2, list:
 // xs_pt_list_len()
 void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const double *v,
-                  const unsigned long long *cnt2, long long keepL, long long keepU,
-                  unsigned long long *sel, unsigned *hist, unsigned *lcnt,
+                  const double *sc, const unsigned long long *cnt2, long long keepL,
+                  long long keepU, unsigned long long *sel, unsigned *hist, unsigned *lcnt,
                   unsigned long long *list, double *thr);
 int xs_pt_select_words();
 int xs_pt_list_len();
+// entries with col <= row (L with its unit diagonal) -> cnt2[0], col >= row
+// (U with its pivots) -> cnt2[1]
 void xs_pt_count_parts(hipStream_t s, int nnz, const int *row, const int *ci,
                        unsigned long long *cnt2);
-void xs_pt_dtol(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
-                const double *sc, double droptol, double *dtol);
+// dtol[i] = droptol * ||a_i||_2 (unscaled A)
+void xs_pt_dtol(hipStream_t s, int n, const int *arp, const double *av, double droptol,
+                double *dtol);
+// compaction of the candidates (count pass, then write pass) with the
+// thresholds of xs_pt_select and the drop tolerance on unscaled magnitudes
 void xs_pt_keep(hipStream_t s, bool write, int n, const int *crp, const int *cci,
-                const double *cv, const double *thr, const double *dtol, int *cnt,
-                const int *frp, int *fci, double *fv, int *frow);
+                const double *cv, const double *thr, const double *dtol, const double *sc,
+                int *cnt, const int *frp, int *fci, double *fv, int *frow);
 void xs_pt_scale_vec(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
                      double *sc);
 void xs_pt_rescale(hipStream_t s, int nnz, const int *row, const int *ci, double *v,

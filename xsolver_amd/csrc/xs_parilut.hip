@@ -395,19 +395,27 @@ __global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_embed_rows(
 }
 
 // ---------------------------------------------------------------------------
-// Radix select of both thresholds at once: for the strict lower part (slot
-// 0, col < row) and the strict upper part (slot 1, col > row), the value of
-// rank k (0-based) in ascending |v|.  |v| bit patterns are ordered like the
-// values; passes of 8-bit digits from the most significant.  Once the bin of
-// rank k holds at most PT_SEL_CAP entries, the next pass collects them and
-// the rank is found among them in one block; later passes return at once.
-// State per slot p: sel[4p + 0] prefix bits, + 1 remaining rank, + 2 mask of
-// fixed bits, + 3 mode (0 histogram, 1 collect, 2 done).
+// Radix select of both thresholds at once (find_kth_element,
+// hip/Precond/par_ilut_hip/select_kth.h:501-634, applied per factor as in
+// hip/Precond/parilut.cpp:1008-1019): the value of rank k (0-based) in
+// ascending magnitude over L (slot 0: its strict entries and its n unit
+// diagonals, value 1) and over U (slot 1: its entries with the pivots).  The
+// magnitudes are those of the unscaled factors (the sweeps run on S A S):
+// |l_ij| = |l~_ij (s_j / s_i)|, |u_ij| = |u~_ij / (s_i s_j)|, the expressions
+// the compaction compares against the thresholds.  |v| bit patterns are
+// ordered like the values; passes of 8-bit digits from the most significant.
+// Once the bin of rank k holds at most PT_SEL_CAP entries, the next pass
+// collects them and the rank is found among them in one block; later passes
+// return at once.  State per slot p: sel[4p + 0] prefix bits, + 1 remaining
+// rank, + 2 mask of fixed bits, + 3 mode (0 histogram, 1 collect, 2 done).
 // ---------------------------------------------------------------------------
 #define PT_SEL_CAP 1024
 static_assert(XS_BLOCK == 256, "k_pt_hist clears its 512 bins with two stores per thread");
 
-__device__ __forceinline__ int pt_slot(int i, int j) { return j < i ? 0 : (j > i ? 1 : -1); }
+// magnitude of entry (i, j) of the unscaled factor from its scaled value v
+__device__ __forceinline__ double pt_mag(int i, int j, double v, const double *__restrict__ sc) {
+  return j < i ? fabs(v * (sc[j] / sc[i])) : fabs(v / (sc[i] * sc[j]));
+}
 
 __global__ void k_pt_sel_init(const unsigned long long *__restrict__ cnt2, long long keepL,
                               long long keepU, unsigned long long *sel, unsigned *hist,
@@ -427,7 +435,8 @@ __global__ void k_pt_sel_init(const unsigned long long *__restrict__ cnt2, long 
 
 __global__ __launch_bounds__(XS_BLOCK) void k_pt_hist(int nnz, const int *__restrict__ row,
                                                       const int *__restrict__ ci,
-                                                      const double *__restrict__ v, int shift,
+                                                      const double *__restrict__ v,
+                                                      const double *__restrict__ sc, int shift,
                                                       const unsigned long long *sel,
                                                       unsigned *__restrict__ hist,
                                                       unsigned *lcnt,
@@ -440,18 +449,22 @@ __global__ __launch_bounds__(XS_BLOCK) void k_pt_hist(int nnz, const int *__rest
   __syncthreads();
   const unsigned long long pre[2] = {sel[0], sel[4]}, msk[2] = {sel[2], sel[6]};
   const unsigned long long mode[2] = {m0, m1};
-  for (int e = blockIdx.x * XS_BLOCK + threadIdx.x; e < nnz; e += gridDim.x * XS_BLOCK) {
-    const int p = pt_slot(row[e], ci[e]);
-    if (p < 0 || mode[p] == 2) continue;
-    const unsigned long long bits =
-        (unsigned long long)__double_as_longlong(fabs(v[e])) & 0x7FFFFFFFFFFFFFFFull;
-    if ((bits & msk[p]) != pre[p]) continue;
+  auto consider = [&](int p, double mag) {
+    if (mode[p] == 2) return;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(mag);
+    if ((bits & msk[p]) != pre[p]) return;
     if (mode[p] == 1) {
       const unsigned q = atomicAdd(&lcnt[p], 1u);
       if (q < PT_SEL_CAP) list[p * PT_SEL_CAP + q] = bits;
     } else {
       atomicAdd(&h[p * 256 + ((bits >> shift) & 0xFF)], 1u);
     }
+  };
+  for (int e = blockIdx.x * XS_BLOCK + threadIdx.x; e < nnz; e += gridDim.x * XS_BLOCK) {
+    const int i = row[e], j = ci[e];
+    const double mag = pt_mag(i, j, v[e], sc);
+    if (j == i) consider(0, 1.0);    // L's unit diagonal
+    consider(j < i ? 0 : 1, mag);
   }
   __syncthreads();
   for (int q = threadIdx.x; q < 512; q += XS_BLOCK)
@@ -520,15 +533,17 @@ __global__ __launch_bounds__(256) void k_pt_pick(int shift, unsigned long long *
   hist[p * 256 + t] = 0;
 }
 
-// counts of the strict lower and strict upper entries -> cnt2[0], cnt2[1]
+// entries of L with its unit diagonal (col <= row) -> cnt2[0] and of U with
+// the pivots (col >= row) -> cnt2[1]: the counts the selection ranks over
+// (hip/Precond/parilut.cpp:35-63 counts each factor's diagonal)
 __global__ __launch_bounds__(XS_BLOCK) void k_pt_count_parts(int nnz, const int *__restrict__ row,
                                                              const int *__restrict__ ci,
                                                              unsigned long long *cnt2) {
   unsigned long long a = 0, b = 0;
   for (int e = blockIdx.x * XS_BLOCK + threadIdx.x; e < nnz; e += gridDim.x * XS_BLOCK) {
-    const int p = pt_slot(row[e], ci[e]);
-    a += p == 0;
-    b += p == 1;
+    const int i = row[e], j = ci[e];
+    a += j <= i;
+    b += j >= i;
   }
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_xor(a, off, XS_WAVE);
@@ -541,10 +556,13 @@ __global__ __launch_bounds__(XS_BLOCK) void k_pt_count_parts(int nnz, const int 
 }
 
 // keep predicate for the compaction
+// (thresold_remove_nnz2, hip/Precond/parilut.cpp:303-323: |v| >= threshold
+// survives, the diagonal always does; plus the drop tolerance on the same
+// unscaled magnitude)
 __device__ __forceinline__ bool pt_keep(int i, int j, double v, double tl, double tu,
-                                        double dt) {
+                                        double dt, const double *__restrict__ sc) {
   if (i == j) return true;
-  const double a = fabs(v);
+  const double a = pt_mag(i, j, v, sc);
   if (a < dt) return false;
   return j < i ? a >= tl : a >= tu;
 }
@@ -554,13 +572,13 @@ __device__ __forceinline__ bool pt_keep(int i, int j, double v, double tl, doubl
 __global__ __launch_bounds__(XS_BLOCK) void k_pt_keep_count(
     int n, const int *__restrict__ crp, const int *__restrict__ cci,
     const double *__restrict__ cv, const double *thr, const double *__restrict__ dtol,
-    int *__restrict__ cnt) {
+    const double *__restrict__ sc, int *__restrict__ cnt) {
   const int lane = threadIdx.x & (XS_WAVE - 1);
   const int i = blockIdx.x * (XS_BLOCK / XS_WAVE) + threadIdx.x / XS_WAVE;
   if (i >= n) return;
   const double tl = thr[0], tu = thr[1], dt = dtol[i];
   int c = 0;
-  for (int e = crp[i] + lane; e < crp[i + 1]; e += XS_WAVE) c += pt_keep(i, cci[e], cv[e], tl, tu, dt);
+  for (int e = crp[i] + lane; e < crp[i + 1]; e += XS_WAVE) c += pt_keep(i, cci[e], cv[e], tl, tu, dt, sc);
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, XS_WAVE);
   if (lane == 0) cnt[i] = c;
 }
@@ -568,8 +586,8 @@ __global__ __launch_bounds__(XS_BLOCK) void k_pt_keep_count(
 __global__ __launch_bounds__(XS_BLOCK) void k_pt_keep_write(
     int n, const int *__restrict__ crp, const int *__restrict__ cci,
     const double *__restrict__ cv, const double *thr, const double *__restrict__ dtol,
-    const int *__restrict__ frp, int *__restrict__ fci, double *__restrict__ fv,
-    int *__restrict__ frow) {
+    const double *__restrict__ sc, const int *__restrict__ frp, int *__restrict__ fci,
+    double *__restrict__ fv, int *__restrict__ frow) {
   const int lane = threadIdx.x & (XS_WAVE - 1);
   const int i = blockIdx.x * (XS_BLOCK / XS_WAVE) + threadIdx.x / XS_WAVE;
   if (i >= n) return;
@@ -585,7 +603,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_pt_keep_write(
     if (e < re) {
       j = cci[e];
       x = cv[e];
-      keep = pt_keep(i, j, x, tl, tu, dt);
+      keep = pt_keep(i, j, x, tl, tu, dt, sc);
     }
     const unsigned long long m = __ballot(keep);
     if (keep) {
@@ -604,17 +622,14 @@ __global__ void k_pt_thr(const unsigned long long *sel, int use, double *thr_slo
     *thr_slot = use ? __longlong_as_double((long long)sel[0]) : 0.0;
 }
 
-// droptol * ||a_i||_2 of the scaled matrix
-__global__ void k_pt_dtol(int n, const int *__restrict__ arp, const int *__restrict__ aci,
-                          const double *__restrict__ av, const double *__restrict__ sc,
+// droptol * ||a_i||_2 of the unscaled matrix (ilutp_final's tnorm,
+// hip/Precond/ilut_diag_in_csr.cpp:67-68)
+__global__ void k_pt_dtol(int n, const int *__restrict__ arp, const double *__restrict__ av,
                           double droptol, double *__restrict__ dtol) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
-  for (int e = arp[i]; e < arp[i + 1]; ++e) {
-    const double a = av[e] * sc[i] * sc[aci[e]];
-    s += a * a;
-  }
+  for (int e = arp[i]; e < arp[i + 1]; ++e) s += av[e] * av[e];
   dtol[i] = droptol * sqrt(s);
 }
 
@@ -733,7 +748,7 @@ void xs_pt_sweep(hipStream_t s, int n, const int *rp, const int *ci, const int *
 }
 
 void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const double *v,
-                  const unsigned long long *cnt2, long long keepL, long long keepU,
+                  const double *sc, const unsigned long long *cnt2, long long keepL, long long keepU,
                   unsigned long long *sel, unsigned *hist, unsigned *lcnt,
                   unsigned long long *list, double *thr) {
   hipLaunchKernelGGL(k_pt_sel_init, dim3(1), dim3(256), 0, s, cnt2, keepL, keepU, sel, hist, lcnt,
@@ -742,8 +757,8 @@ void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const d
   if (g > 2048) g = 2048;
   if (g < 1) g = 1;
   for (int shift = 56; shift >= 0; shift -= 8) {
-    hipLaunchKernelGGL(k_pt_hist, dim3(g), dim3(XS_BLOCK), 0, s, nnz, row, ci, v, shift, sel, hist,
-                       lcnt, list);
+    hipLaunchKernelGGL(k_pt_hist, dim3(g), dim3(XS_BLOCK), 0, s, nnz, row, ci, v, sc, shift, sel,
+                       hist, lcnt, list);
     hipLaunchKernelGGL(k_pt_pick, dim3(2), dim3(256), 0, s, shift, sel, hist, lcnt, list, thr);
   }
 }
@@ -760,22 +775,21 @@ void xs_pt_count_parts(hipStream_t s, int nnz, const int *row, const int *ci,
   hipLaunchKernelGGL(k_pt_count_parts, dim3(g), dim3(XS_BLOCK), 0, s, nnz, row, ci, cnt2);
 }
 
-void xs_pt_dtol(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
-                const double *sc, double droptol, double *dtol) {
-  hipLaunchKernelGGL(k_pt_dtol, dim3((n + 255) / 256), dim3(256), 0, s, n, arp, aci, av, sc,
-                     droptol, dtol);
+void xs_pt_dtol(hipStream_t s, int n, const int *arp, const double *av, double droptol,
+                double *dtol) {
+  hipLaunchKernelGGL(k_pt_dtol, dim3((n + 255) / 256), dim3(256), 0, s, n, arp, av, droptol, dtol);
 }
 
 void xs_pt_keep(hipStream_t s, bool write, int n, const int *crp, const int *cci,
-                const double *cv, const double *thr, const double *dtol, int *cnt,
-                const int *frp, int *fci, double *fv, int *frow) {
+                const double *cv, const double *thr, const double *dtol, const double *sc,
+                int *cnt, const int *frp, int *fci, double *fv, int *frow) {
   const int rows = XS_BLOCK / XS_WAVE, g = (n + rows - 1) / rows;
   if (!write)
     hipLaunchKernelGGL(k_pt_keep_count, dim3(g), dim3(XS_BLOCK), 0, s, n, crp, cci, cv, thr, dtol,
-                       cnt);
+                       sc, cnt);
   else
     hipLaunchKernelGGL(k_pt_keep_write, dim3(g), dim3(XS_BLOCK), 0, s, n, crp, cci, cv, thr, dtol,
-                       frp, fci, fv, frow);
+                       sc, frp, fci, fv, frow);
 }
 
 void xs_pt_scale_vec(hipStream_t s, int n, const int *arp, const int *aci, const double *av,

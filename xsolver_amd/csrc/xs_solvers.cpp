@@ -505,6 +505,7 @@ static double inner_tol_of(const xs_ctx *c) {
 
 // FP32 BiCGSTAB on A d = rhs from d = 0 (inner solve); returns iterations.
 static int inner_bicgstab32(xs_ctx *c, const float *rhs, float *d, int maxit, int *iters) {
+  *iters = 0;
   hipStream_t s = c->stream;
   const int n = c->A.n;
   float *W = c->work32;
@@ -552,6 +553,7 @@ static int inner_bicgstab32(xs_ctx *c, const float *rhs, float *d, int maxit, in
 // mixed-precision refinement): FP32 basis, SpMV and triangular solves; FP64
 // inner products, Hessenberg, Givens rotations and residual estimate.
 static int inner_fgmres32(xs_ctx *c, const float *rhs, float *d, int maxit, int *iters) {
+  *iters = 0;
   hipStream_t s = c->stream;
   const int n = c->A.n;
   const int m = c->restart > 0 ? c->restart : 30;
@@ -748,9 +750,14 @@ int xs_solve_mixed(xs_ctx *c, const double *b, double *x) {
                   ? inner_bicgstab32(c, rhs32, d32, c->maxiter - total, &it)
                   : inner_fgmres32(c, rhs32, d32, c->maxiter - total, &it);
     total += it;
+    // an inner failure other than a breakdown (memory, device) leaves d32
+    // undefined: stop before it reaches x; a breakdown before the first
+    // inner iteration, or an inner solve that made no iteration, would
+    // repeat forever
+    if (irc < 0 && irc != XS_ERR_BREAKDOWN) { rc = irc; break; }
+    if (it == 0) { rc = irc == XS_ERR_BREAKDOWN ? XS_ERR_BREAKDOWN : XS_NOT_CONVERGED; break; }
     xs_add_f2d(s, n, d32, x, st, S_SCRATCH0 + 1);
     outer++;
-    if (irc == XS_ERR_BREAKDOWN && it == 0) { rc = XS_ERR_BREAKDOWN; break; }
   }
   XS_CHK(hipGetLastError());
   c->iters = total;
