@@ -121,6 +121,10 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *   "parilu_scale"   1 (default) = factorise S A S, S = |diag(A)|^-1/2, and
  *                    scale back (ParILU/ParILUT fixed-point stability)
  *   "parilut_steps"  ParILUT add/sweep/select/sweep steps (reference: 10)
+ *   "pc_reuse"       1 = xs_mat_update_values keeps the preconditioner built
+ *                    from the earlier values (a preconditioner sets the rate
+ *                    of convergence, not the converged solution); 0 (default)
+ *                    = factorise again at the next solve
  *   "refactor"       any value: recompute the preconditioner at the next solve
  *   "profile"        bit mask of kernel families timed with HIP events
  *                    (2 SpMV, 4 lower solve, 8 upper solve); read back with

@@ -58,33 +58,68 @@ def true_relres(A, b, x):
 # north-star acceptance of the plain FP64 path: the solution within 1e-10
 # relative of the oracle's FP64 solution on the same seeded input
 X_TOL = 1e-10
-# residual histories: every entry within this relative deviation of the
-# oracle's (absolute norms, entry 0 = ||r0||)
+# residual histories (absolute norms, entry 0 = ||r0||): every entry within
+# RES_RTOL relative of the oracle's.  Where the recurrence itself amplifies
+# rounding past that (BiCGSTAB: a one-ulp change of b moves the oracle's own
+# 128^3 5/2-sweep history by 1.3e-4 and its 32^3 Jacobi history by 4e-6),
+# the bound is calibrated on the oracle: its own history with b perturbed by
+# one ulp (b (1 + u 2^-53), u in {-1, 0, 1}) sets the rounding sensitivity,
+# and the GPU history must lie within CAL_FACTOR times that deviation.
 RES_RTOL = 1e-6
+CAL_FACTOR = 10.0
 
 
 def resvec_dev(res, reso):
+    """Largest relative deviation |res_k - reso_k| / reso_k over the common
+    entries, with its index."""
     m = min(len(res), len(reso))
     if m == 0:
-        return 0.0
-    den = np.maximum(np.abs(reso[:m]), 1e-300)
-    return float(np.max(np.abs(res[:m] - reso[:m]) / den))
+        return 0.0, 0
+    r, ro = np.asarray(res[:m]), np.asarray(reso[:m])
+    q = np.abs(r - ro) / np.maximum(np.abs(ro), 1e-300)
+    k = int(np.argmax(q))
+    return float(q[k]), k
+
+
+def ulp_perturbed(b, seed=100):
+    u = np.random.default_rng(seed).integers(-1, 2, len(b))
+    return b * (1 + u * 2.0 ** -53)
 
 
 def check_fp64_parity(tag, A, b, x, it, res, xo, ito, reso, tol, it_slack=2,
-                      true_res=True):
+                      true_res=True, oracle_rerun=None, x_calibrated=False):
     """The acceptance of the plain FP64 path against the oracle run on the
     same input: iterations within +-it_slack, true relative residual below
     tol (recomputed in FP64 on the host), ||x - x_oracle|| / ||x_oracle|| <=
-    X_TOL, and the whole residual history within RES_RTOL of the oracle's."""
+    X_TOL, and the whole residual history within RES_RTOL of the oracle's.
+
+    oracle_rerun(b') -> (x', resvec') reruns the oracle on a one-ulp
+    perturbation of b.  When the history misses RES_RTOL it calibrates the
+    bound (CAL_FACTOR times the oracle's own deviation).  x_calibrated
+    applies the same calibration to x -- only for cases where the oracle's
+    own solution moves by more than X_TOL under a one-ulp change of b (short
+    restarts on ill-conditioned systems); every configuration-size test keeps
+    X_TOL strict."""
     rr = true_relres(A, b, x)
     dx = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
-    dres = resvec_dev(res, reso)
-    say(f"   {tag}: oracle it={ito} gpu it={it} true relres={rr:.3e} "
-        f"x-diff={dx:.2e} resvec dev={dres:.2e}")
+    q, k = resvec_dev(res, reso)
+    say(f"   {tag}: oracle it={ito} gpu it={it} true relres={rr:.3e} x-diff={dx:.2e} "
+        f"resvec: worst entry {k} (||r_k||/||r_0|| = {abs(reso[k]) / abs(reso[0]):.1e}) "
+        f"rel dev {q:.2e}")
     assert abs(it - ito) <= it_slack
     if true_res:
         assert rr < tol
-    assert dx <= X_TOL
     assert len(res) == it + 1
-    assert dres <= RES_RTOL
+    need_cal = q > RES_RTOL or (x_calibrated and dx > X_TOL)
+    if need_cal:
+        assert oracle_rerun is not None, "history beyond RES_RTOL and no calibration run"
+        xp, resp = oracle_rerun(ulp_perturbed(b))
+        qp, kp = resvec_dev(resp, reso)
+        dxp = float(np.linalg.norm(xp - xo) / np.linalg.norm(xo))
+        say(f"   {tag}: oracle vs one-ulp-perturbed oracle: resvec rel dev {qp:.2e} "
+            f"(entry {kp}), x-diff {dxp:.2e}")
+        assert q <= max(RES_RTOL, CAL_FACTOR * qp)
+        if x_calibrated:
+            assert dx <= max(X_TOL, CAL_FACTOR * dxp)
+    if not x_calibrated:
+        assert dx <= X_TOL

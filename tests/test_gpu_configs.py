@@ -92,8 +92,12 @@ def test_convdiff128_bicgstab_parilu0_matches_oracle(dev, fmt, apply):
         dlu = float(np.abs(lu - lo).max() / np.abs(lo).max())
         say(f"   factors vs oracle ILU(0): max rel {dlu:.2e}")
         assert np.array_equal(dpos, uo) and dlu < 1e-14
+    kl, ku = apply if apply else (0, 0)
     check_fp64_parity(f"c2 {fmt} {APPLY_IDS[APPLIES.index(apply)]}", A, b, x, S.iterations,
-                      S.resvec(), xo, ito, reso, 1e-10)
+                      S.resvec(), xo, ito, reso, 1e-10,
+                      oracle_rerun=lambda bp: oracle.solve(A, bp, "bicgstab", "ilu0", tol=1e-10,
+                                                           maxiter=2000, lsweeps=kl,
+                                                           usweeps=ku)[::3])
     S.close()
 
 
@@ -182,35 +186,53 @@ def test_adjoint_grf512_cpu_adjoint_and_fd(dev):
     rel_b = np.linalg.norm(gb - lam_o) / np.linalg.norm(lam_o)
     say(f"   vs FP64 CPU adjoint: dL/dA rel {rel_v:.2e}, dL/db rel {rel_b:.2e}")
     assert rel_v < 1e-6 and rel_b < 1e-6
-    # central finite differences, 200 random nonzeros, h = 1e-6: each
-    # perturbed solve starts from the unperturbed solution, so its residual
-    # reduction applies to the O(h) change alone
+    # central finite differences, 200 random nonzeros, h = 1e-6.  Each
+    # perturbed system A +- h e_i e_j^T is solved from the unperturbed
+    # solution, so the solver only has to resolve the O(h) change, with the
+    # unperturbed ILU(0) kept as preconditioner (it sets the convergence rate,
+    # not the solution) and BiCGSTAB with the 5/2 Jacobi-sweep apply.  The
+    # loss difference is formed without cancellation:
+    #   L+ - L- = sum (x+ - x-) (x+ + x- - 2 x_t).
     idx = np.random.default_rng(4).choice(nnz, size=200, replace=False)
     h = 1e-6
     Sf = XSolver()
     Sf.mat_setup(rp, ci, v)
-    Sf.ksp_settype("cg")
-    Sf.ksp_setoption(1e-12, 20000)
+    Sf.ksp_settype("bicgstab")
+    Sf.ksp_setoption(1e-11, 20000)
     Sf.pc_settype("ilu0")
     Sf.set_option("parilu_async", 0)
     Sf.set_option("parilu_sweeps", 200)
+    Sf.set_option("lsolve_sweeps", 5)
+    Sf.set_option("usolve_sweeps", 2)
+    Sf.pc_setup()
+    Sf.set_option("pc_reuse", 1)
     lib = xsolver_amd.lib()
 
-    def loss_at(vv):
+    def x_at(vv):
         assert lib.xs_mat_update_values(Sf.h, vv.ctypes.data_as(ctypes.c_void_p), 0) == 0
         xx, stt = Sf.solve(b, x0=xg)
         assert stt == 0
-        return float(((xx - xt) ** 2).sum())
+        return xx
 
     fd = np.zeros(len(idx))
+    its = []
     for q, k in enumerate(idx):
         vp = v.copy(); vp[k] += h
         vm = v.copy(); vm[k] -= h
-        fd[q] = (loss_at(vp) - loss_at(vm)) / (2 * h)
+        xp = x_at(vp); its.append(Sf.iterations)
+        xm = x_at(vm); its.append(Sf.iterations)
+        fd[q] = float(np.dot(xp - xm, xp + xm - 2 * xt)) / (2 * h)
+        if q % 50 == 49:
+            say(f"   FD {q + 1}/200 done (perturbed solves {min(its)}..{max(its)} iterations)")
     Sf.close()
     rel_fd = np.linalg.norm(gv[idx] - fd) / np.linalg.norm(fd)
     say(f"   vs central FD (200 entries, h=1e-6): rel {rel_fd:.2e}")
-    assert rel_fd < 1e-6
+    # FD accuracy: the truncation error is O(h^2) relative; each perturbed
+    # solution carries O(eps |x|) rounding per entry from its updates, which
+    # the loss difference sums over n = 262,144 entries with random signs:
+    # sqrt(n) eps / (2 h) ~ 1e-7 absolute per derivative -- the bound below
+    # leaves a decade above that for the smallest of the 200 derivatives
+    assert rel_fd < 1e-5
     S.close()
 
 

@@ -57,7 +57,9 @@ def test_solve_parity(dev, case, fmt):
     res = g["res"]
     assert res[0] == pytest.approx(np.linalg.norm(b), rel=1e-12)
     assert res[-1] / res[0] < tol
-    check_fp64_parity(name, A, b, g["x"], g["it"], res, xo, ito, reso, tol)
+    check_fp64_parity(name, A, b, g["x"], g["it"], res, xo, ito, reso, tol,
+                      oracle_rerun=lambda bp: oracle.solve(A, bp, ksp, pc, tol=tol,
+                                                           maxiter=maxiter)[::3])
     g["S"].close()
 
 
@@ -237,8 +239,14 @@ def test_basis_krylov_parity(dev, case):
     # left-preconditioned GMRES converges on ||M^-1 r|| (the reference's
     # measure, intel/mpisolver-C-v1.cpp:933-1087): its true residual is not
     # the stopping test; the solution still matches the oracle's
+    # FGMRES(5) on the 32^2 lognormal-field system: a one-ulp change of b
+    # moves the oracle's own x by 4e-11 .. 2.7e-10, so its x is compared
+    # against that sensitivity rather than the fixed 1e-10
     check_fp64_parity(name, A, b, g["x"], g["it"], res, xo, ito, reso, tol,
-                      true_res=ksp != "gmres")
+                      true_res=ksp != "gmres",
+                      oracle_rerun=lambda bp: oracle.solve(A, bp, ksp, pc, tol=tol, maxiter=maxiter,
+                                                           restart=restart)[::3],
+                      x_calibrated=restart == 5)
     g["S"].close()
 
 
@@ -405,7 +413,9 @@ def test_iterative_ilu_apply_solve(dev, precision, sweeps):
     assert sto == 0 and g["st"] == 0
     if precision == 0:
         check_fp64_parity(f"sweeps {sweeps}", A, b, g["x"], g["it"], g["res"], xo, ito, reso,
-                          1e-10)
+                          1e-10, oracle_rerun=lambda bp: oracle.solve(
+                              A, bp, "bicgstab", "ilu0", tol=1e-10, maxiter=1000,
+                              lsweeps=sweeps, usweeps=sweeps)[::3])
     assert true_relres(A, b, g["x"]) < 1e-10
     g["S"].close()
 

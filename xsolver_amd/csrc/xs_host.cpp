@@ -568,9 +568,11 @@ extern "C" int xs_mat_update_values(xs_ctx *c, const double *val, int on_device)
                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
   if (c->use_ell) xs_build_ell(c->stream, c->A, c->ell.W, c->ell);
   if (c->have_At) xs_permute_vals(c->stream, c->A.nnz, c->perm_At, c->A.v, c->At.v);
-  c->pc_ready = false;
-  c->mixed_ready = false;
-  c->tell_ready = c->tell32_ready = false;
+  if (!c->pc_reuse) {   // else the factors of the earlier values stay
+    c->pc_ready = false;
+    c->tell_ready = c->tell32_ready = false;
+  }
+  c->mixed_ready = false;   // the FP32 copy of A follows the new values
   XS_HIP(hipStreamSynchronize(c->stream));
   return XS_OK;
 }
@@ -656,6 +658,7 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     c->inner_tol = value;
   }
   else if (k == "parilu_scale") { c->parilu_scale = value != 0; c->pc_ready = false; }
+  else if (k == "pc_reuse") { c->pc_reuse = value != 0; }
   else if (k == "parilut_steps") { if (value < 1) return XS_ERR_ARG; c->parilut_steps = (int)value; c->pc_ready = false; }
   else if (k == "profile") { c->prof_kind = (int)value; for (int i = 0; i < 4; i++) { c->prof_ms[i] = 0; c->prof_cnt[i] = 0; } }
   else if (k == "precision") { if (value != 0 && value != 1) return XS_ERR_ARG; c->precision = (int)value; }

@@ -27,6 +27,7 @@ struct xs_ctx {
   int parilu_sweeps = 20;       // hip/Precond/precon_gpuilu0.cpp:5
   bool parilu_async = true;
   bool parilu_scale = true;
+  bool pc_reuse = false;        // xs_mat_update_values keeps the preconditioner
   int parilut_steps = 10;       // hip/Precond/precon_gpuilut.cpp:5
   // ILU application: -1 = exact sync-free triangular solve; k >= 0 = k Jacobi
   // sweeps per factor (the reference's GPU apply: 20/20 for ILU(0),
