@@ -248,14 +248,16 @@ typedef struct xs_batch xs_batch;
  * and its columns stay in that range.  Copied to the GPU. */
 int xs_batch_create(xs_batch **bt, int nsys, const int *sys_ptr, const int *rowptr,
                     const int *colidx, const double *val);
-/* CG + Jacobi on every system to ||b - A x|| < tol ||b|| (FP64 check), each
+/* CG + Jacobi on every system to ||b - A x|| < tol ||b - A x0|| (the
+ * reference's Krylov convention, hip/Solver/xsolver_cg.cpp:16-74; FP64 check), each
  * system with its own iteration count; precision 0 = FP64, 1 = FP32 CG inside
  * FP64 iterative refinement.  sol = x0 in, x out (host or device vectors). */
 int xs_batch_solve(xs_batch *bt, const double *rhs, double *sol, double tol, int maxiter,
                    int precision);
 int xs_batch_solve_device(xs_batch *bt, const double *d_rhs, double *d_sol, double tol,
                           int maxiter, int precision);
-/* Per-system iterations and final FP64 relative residual; returns nsys. */
+/* Per-system iterations and final FP64 relative residual
+ * ||b - A x|| / ||b - A x0||; returns nsys. */
 int xs_batch_get_results(const xs_batch *bt, int *iters, double *relres);
 double xs_batch_get_time_ms(const xs_batch *bt);
 int xs_batch_destroy(xs_batch *bt);

@@ -100,8 +100,28 @@ __device__ int bt_cg(int r0, int r1, const int *__restrict__ rp, const int *__re
   return it;
 }
 
+// 1 when every row of [r0, r1) has row pointers inside [0, nnz] and every
+// column inside [r0, r1) (block-diagonal pattern), else 0; workgroup-uniform.
+__device__ int bt_pattern_ok(int r0, int r1, int N, int nnz, const int *__restrict__ rp,
+                             const int *__restrict__ ci, double (*red)[2]) {
+  double bad = 0.0, dummy = 0.0;
+  if (r0 < 0 || r1 > N || r1 <= r0) bad = 1.0;
+  else
+    for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) {
+      const int e0 = rp[i], e1 = rp[i + 1];
+      if (e0 < 0 || e1 < e0 || e1 > nnz) { bad = 1.0; continue; }
+      for (int e = e0; e < e1; ++e)
+        if (ci[e] < r0 || ci[e] >= r1) bad = 1.0;
+    }
+  bt_sum2(bad, dummy, red);
+  return bad == 0.0;
+}
+
+// N: rows of the whole batch (the workspace stride); nnz: entries of the
+// whole batch.  A system whose rows or pattern fall outside those bounds is
+// not solved: iters = -1, relres = NaN.
 __global__ __launch_bounds__(BT_THREADS) void k_batch_cg(
-    int nsys, const int *__restrict__ order, const int *__restrict__ sys_ptr,
+    int nsys, int N, int nnz, const int *__restrict__ order, const int *__restrict__ sys_ptr,
     const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v,
     const float *__restrict__ v32, const double *__restrict__ dinv,
     const float *__restrict__ dinv32, const double *__restrict__ b, double *__restrict__ x,
@@ -112,7 +132,7 @@ __global__ __launch_bounds__(BT_THREADS) void k_batch_cg(
   for (;;) {
     if (threadIdx.x == 0) {
       const unsigned t = atomicAdd(queue, 1u);
-      s_sys = t < (unsigned)nsys ? order[t] : -1;
+      s_sys = t < (unsigned)nsys ? (order ? order[t] : (int)t) : -1;
     }
     __syncthreads();
     const int sys = s_sys;
@@ -121,16 +141,20 @@ __global__ __launch_bounds__(BT_THREADS) void k_batch_cg(
     const int r0 = sys_ptr[sys], r1 = sys_ptr[sys + 1];
     int it = 0;
     double rel = 0.0;
-    if (precision == 0) {
-      // FP64 CG from x0 (the workspace holds r, z, p, q, rhs)
-      double *r = w64, *z = w64 + sys_ptr[nsys], *p = z + sys_ptr[nsys], *q = p + sys_ptr[nsys];
-      double *rhs = q + sys_ptr[nsys], *d = rhs + sys_ptr[nsys];
-      double bb = 0.0, dummy = 0.0;
+    if (!bt_pattern_ok(r0, r1, N, nnz, rp, ci, red)) {
+      it = -1;
+      rel = __builtin_nan("");
+    } else if (precision == 0) {
+      // FP64 CG from x0 (the workspace holds r, z, p, q, rhs, d)
+      double *r = w64, *z = w64 + N, *p = z + N, *q = p + N;
+      double *rhs = q + N, *d = rhs + N;
+      double bb = 0.0, dummy = 0.0;   // ||b - A x0||^2: the base of tol and relres
       for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) {
         double s = 0.0;
         for (int e = rp[i]; e < rp[i + 1]; ++e) s += v[e] * x[ci[e]];
-        rhs[i] = b[i] - s;
-        bb += b[i] * b[i];
+        const double t = b[i] - s;
+        rhs[i] = t;
+        bb += t * t;
       }
       bt_sum2(bb, dummy, red);
       it = bt_cg<double, double>(r0, r1, rp, ci, v, dinv, rhs, d, r, z, p, q, tol, maxiter, red);
@@ -149,12 +173,9 @@ __global__ __launch_bounds__(BT_THREADS) void k_batch_cg(
       rel = bb > 0.0 ? sqrt(rn / bb) : 0.0;
     } else {
       // FP64 refinement around FP32 CG corrections
-      const int N = sys_ptr[nsys];
       float *r = w32, *z = w32 + N, *p = z + N, *q = p + N, *rhs = q + N, *d = rhs + N;
       double *res64 = w64;
-      double bb = 0.0, dm = 0.0;
-      for (int i = r0 + threadIdx.x; i < r1; i += BT_THREADS) bb += b[i] * b[i];
-      bt_sum2(bb, dm, red);
+      double bb = 0.0;   // ||b - A x0||^2 (the first outer residual)
       for (int outer = 0; outer < 50 && it < maxiter; ++outer) {
         double rn = 0.0, d2 = 0.0;
         __syncthreads();
@@ -166,6 +187,7 @@ __global__ __launch_bounds__(BT_THREADS) void k_batch_cg(
           rn += t * t;
         }
         bt_sum2(rn, d2, red);
+        if (outer == 0) bb = rn;
         rel = bb > 0.0 ? sqrt(rn / bb) : 0.0;
         if (!(rel >= tol)) break;
         const double sc = 1.0 / sqrt(rn);
@@ -200,18 +222,41 @@ __global__ void k_batch_prep(int n, const int *__restrict__ rp, const int *__res
   dinv32[i] = (float)(1.0 / d);
 }
 
+// FP64 Jacobi set-up only (1/a_ii; a zero or missing diagonal gives inf,
+// which the CG turns into a non-finite relres for that system), and the
+// work-queue counter cleared by the same launch (thread 0), so a caller
+// under graph capture records kernels only.
+__global__ void k_batch_prep64(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                               const double *__restrict__ v, double *__restrict__ dinv,
+                               unsigned *queue) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *queue = 0u;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int e = rp[i]; e < rp[i + 1]; ++e)
+    if (ci[e] == i) d = v[e];
+  dinv[i] = 1.0 / d;
+}
+
+void xs_batch_prep64(hipStream_t s, int n, const int *rp, const int *ci, const double *v,
+                     double *dinv, unsigned *queue) {
+  hipLaunchKernelGGL(k_batch_prep64, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, v, dinv,
+                     queue);
+}
+
 void xs_batch_prep(hipStream_t s, int n, const int *rp, const int *ci, const double *v,
                    double *dinv, float *dinv32, float *v32, int *bad) {
   hipLaunchKernelGGL(k_batch_prep, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, v, dinv,
                      dinv32, v32, bad);
 }
 
-void xs_batch_cg(hipStream_t s, int grid, int nsys, const int *order, const int *sys_ptr,
-                 const int *rp, const int *ci, const double *v, const float *v32,
-                 const double *dinv, const float *dinv32, const double *b, double *x,
-                 double *w64, float *w32, double tol, int maxiter, int precision,
+void xs_batch_cg(hipStream_t s, int grid, int nsys, int N, int nnz, const int *order,
+                 const int *sys_ptr, const int *rp, const int *ci, const double *v,
+                 const float *v32, const double *dinv, const float *dinv32, const double *b,
+                 double *x, double *w64, float *w32, double tol, int maxiter, int precision,
                  double inner_tol, unsigned *queue, int *iters, double *relres) {
-  hipLaunchKernelGGL(k_batch_cg, dim3(grid), dim3(BT_THREADS), 0, s, nsys, order, sys_ptr, rp, ci,
+  hipLaunchKernelGGL(k_batch_cg, dim3(grid), dim3(BT_THREADS), 0, s, nsys, N, nnz, order, sys_ptr,
+                     rp, ci,
                      v, v32, dinv, dinv32, b, x, w64, w32, tol, maxiter, precision, inner_tol,
                      queue, iters, relres);
 }
