@@ -32,7 +32,8 @@ enum xs_status {
   XS_ERR_DEVICE = -4,       /* HIP runtime error or no device */
   XS_ERR_BREAKDOWN = -5,    /* non-finite residual / Krylov breakdown */
   XS_ERR_PIVOT = -6,        /* missing diagonal entry / zero Jacobi pivot */
-  XS_ERR_MEMORY = -7
+  XS_ERR_MEMORY = -7,
+  XS_ERR_TOO_LARGE = -8     /* input beyond a fixed-size workspace's maximum */
 };
 
 /* ======================================================================== */
@@ -256,6 +257,27 @@ int xs_batch_solve(xs_batch *bt, const double *rhs, double *sol, double tol, int
                    int precision);
 int xs_batch_solve_device(xs_batch *bt, const double *d_rhs, double *d_sol, double tol,
                           int maxiter, int precision);
+/* Allocation-free variant of xs_batch_solve_device (FP64 CG + Jacobi).  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  It takes no context: every argument is a device pointer
+ * (the block-diagonal CSR as for xs_batch_create, rhs, sol = x0 in / x out,
+ * per-system iters and FP64 relative residual ||b - A x|| / ||b - A x0|| out) except
+ * the sizes, nsys systems with n rows and nnz entries in total.  Its workspace
+ * (7 doubles per row) is a static device array for at most
+ * XS_BATCH_NOALLOC_MAX_ROWS rows, held by each GPU it runs on until the
+ * process ends; n above that -> XS_ERR_TOO_LARGE.  Calls no hipMalloc,
+ * hipFree or allocating library routine and does not synchronise: it enqueues
+ * kernels on `stream` (NULL = the null stream) and returns XS_OK; convergence
+ * is read from relres (>= tol, or NaN, where a system did not converge; a
+ * system whose rows or columns leave its block gets iters = -1, relres =
+ * NaN).  Every call on one GPU shares that GPU's workspace: calls that
+ * overlap on one GPU race. */
+#define XS_BATCH_NOALLOC_MAX_ROWS 1048576
+int xs_batch_solve_noalloc(int nsys, int n, int nnz, const int *d_sys_ptr, const int *d_rowptr,
+                           const int *d_colidx, const double *d_val, const double *d_rhs,
+                           double *d_sol, double tol, int maxiter, int *d_iters,
+                           double *d_relres, void *stream);
 /* Per-system iterations and final FP64 relative residual
  * ||b - A x|| / ||b - A x0||; returns nsys. */
 int xs_batch_get_results(const xs_batch *bt, int *iters, double *relres);

@@ -29,6 +29,8 @@ XS_ERR_DEVICE = -4
 XS_ERR_BREAKDOWN = -5
 XS_ERR_PIVOT = -6
 XS_ERR_MEMORY = -7
+XS_ERR_TOO_LARGE = -8
+XS_BATCH_NOALLOC_MAX_ROWS = 1048576
 
 EXPORTS = [
     "xsolver_mat_setup", "xsolver_ksp_settype", "xsolver_ksp_set_absolute_tol",
@@ -45,6 +47,7 @@ EXPORTS = [
     "xs_batch_get_time_ms", "xs_batch_destroy", "xs_solve_transpose_device",
     "xs_grad_values_device", "xs_solve_multi_device", "xs_solve_multi", "xs_get_multi_results",
     "xs_get_multi_resvec", "xs_spmm", "xs_pc_apply_multi", "xs_autotune", "xs_mtx_read",
+    "xs_batch_solve_noalloc",
 ]
 
 XS_RHS_COLMAJOR = 0
@@ -104,6 +107,8 @@ def lib():
         "xs_batch_solve": (_i, [_vp, _vp, _vp, _d, _i, _i]),
         "xs_batch_solve_device": (_i, [_vp, _vp, _vp, _d, _i, _i]),
         "xs_batch_get_results": (_i, [_vp, _vp, _vp]),
+        "xs_batch_solve_noalloc": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _d, _i, _vp,
+                                        _vp, _vp]),
         "xs_batch_get_time_ms": (_d, [_vp]),
         "xs_batch_destroy": (_i, [_vp]),
         "xs_solve_transpose_device": (_i, [_vp, _vp, _vp]),

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -168,3 +169,67 @@ extern "C" int xs_batch_get_results(const xs_batch *bt, int *iters, double *relr
 
 extern "C" double xs_batch_get_time_ms(const xs_batch *bt) { return bt ? bt->ms : -1.0; }
 
+// ---------------------------------------------------------------------------
+// Allocation-free batched solve (no reference counterpart).  For callers that
+// cannot allocate device memory (code under HIP graph capture, serving loops,
+// harnesses that forbid hipMalloc).  Its workspace -- the work-queue counter,
+// 1/a_ii and the six CG vectors, 7 doubles per row -- lives in `scratch`, a
+// static device array of fixed size that each GPU holds until the process
+// ends.  Calls on one GPU share that GPU's array (the last call's contents
+// stay in it): calls that overlap on one GPU, on any streams, race.
+// ---------------------------------------------------------------------------
+#define XS_NOALLOC_HEADER 256   // bytes before the vectors: the queue counter
+static __device__ __align__(256) unsigned char
+    scratch[XS_NOALLOC_HEADER + 7 * sizeof(double) * (size_t)XS_BATCH_NOALLOC_MAX_ROWS];
+
+static std::mutex g_scratch_mu;
+static std::vector<unsigned char *> g_scratch_ptr;   // device address per device
+
+// the device address of `scratch` on the current device, looked up once per
+// device with that device current (the host-side name is not a device address)
+static int noalloc_scratch(unsigned char **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) { (void)hipGetLastError(); return XS_ERR_DEVICE; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return XS_ERR_DEVICE;
+    }
+    g_scratch_ptr[dev] = (unsigned char *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return XS_OK;
+}
+
+extern "C" int xs_batch_solve_noalloc(int nsys, int n, int nnz, const int *d_sys_ptr,
+                                      const int *d_rowptr, const int *d_colidx,
+                                      const double *d_val, const double *d_rhs, double *d_sol,
+                                      double tol, int maxiter, int *d_iters, double *d_relres,
+                                      void *stream) {
+  if (nsys <= 0 || n <= 0 || nnz <= 0 || maxiter <= 0 || !(tol >= 0.0) || !d_sys_ptr ||
+      !d_rowptr || !d_colidx || !d_val || !d_rhs || !d_sol || !d_iters || !d_relres)
+    return XS_ERR_ARG;
+  if (n > XS_BATCH_NOALLOC_MAX_ROWS || nsys > n) return XS_ERR_TOO_LARGE;
+  unsigned char *base = nullptr;
+  int rc = noalloc_scratch(&base);
+  if (rc) return rc;
+  unsigned *queue = (unsigned *)base;
+  double *dinv = (double *)(base + XS_NOALLOC_HEADER);
+  double *w64 = dinv + n;
+  hipStream_t s = (hipStream_t)stream;
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int grid = std::min(nsys, 2 * cus);
+  // one launch sets 1/a_ii and clears the queue counter (kernels only, so the
+  // call can be captured into a graph)
+  xs_batch_prep64(s, n, d_rowptr, d_colidx, d_val, dinv, queue);
+  xs_batch_cg(s, grid, nsys, n, nnz, nullptr, d_sys_ptr, d_rowptr, d_colidx, d_val, nullptr,
+              dinv, nullptr, d_rhs, d_sol, w64, nullptr, tol, maxiter, 0, 0.0, queue, d_iters,
+              d_relres);
+  if (hipGetLastError() != hipSuccess) return XS_ERR_DEVICE;
+  return XS_OK;
+}

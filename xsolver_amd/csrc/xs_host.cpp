@@ -484,6 +484,7 @@ extern "C" const char *xs_status_string(int s) {
     case XS_ERR_BREAKDOWN: return "breakdown (non-finite residual)";
     case XS_ERR_PIVOT: return "zero or missing pivot";
     case XS_ERR_MEMORY: return "out of memory";
+    case XS_ERR_TOO_LARGE: return "input larger than the fixed workspace";
   }
   return "unknown status";
 }
