@@ -62,8 +62,13 @@ def lib():
                                  ctypes.c_double, ctypes.c_int, _ip, _ip, _dp, _ip, _ip, _dp, _ip]
         L.or_batch_cg.argtypes = [ctypes.c_int, _ip, _ip, _ip, _dp, _dp, _dp, ctypes.c_double,
                                   ctypes.c_int, _ip, _dp]
+        L.or_ic0.argtypes = [ctypes.c_int, _ip, _ip, _dp, ctypes.c_double, ctypes.c_double,
+                             _ip, _ip, _dp]
+        L.or_ict.argtypes = [ctypes.c_int, _ip, _ip, _dp, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_int, _ip, _ip, _dp]
+        L.or_ic_split.argtypes = [ctypes.c_int, _ip, _ip, _dp, _ip, _ip, _dp, _ip, _ip, _dp, _dp]
         for f in ("or_jacobi_setup", "or_ilu0", "or_ilut", "or_solve", "or_parilut",
-                  "or_batch_cg"):
+                  "or_batch_cg", "or_ic0", "or_ict"):
             getattr(L, f).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -80,6 +85,10 @@ def ref_lib():
         R.ref_ilu0_simple.argtypes = [_dp, _dp, _ip, _ip, ctypes.c_int, _ip]
         R.ref_ilutp_final.argtypes = [_dp, _dp, _ip, _ip, ctypes.c_int, _dp, ctypes.c_int,
                                       _ip, _ip, _ip, ctypes.c_double]
+        R.ref_ic0_csr_A.argtypes = [ctypes.c_int, _ip, _dp, _ip, _ip, _dp, _ip, ctypes.c_double,
+                                    ctypes.c_double]
+        R.ref_ict_csr_A_l.argtypes = [ctypes.c_int, _ip, _dp, _ip, _ip, _dp, _ip,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_int]
         _REF = R
     return _REF
 
@@ -172,6 +181,77 @@ def ilut(A, lfil, droptol):
         raise ValueError("ILUT capacity exceeded")
     nnz = lurows[n]
     return lurows, lucols[:nnz].copy(), luval[:nnz].copy(), uptr[:n].copy(), diag
+
+
+def ic0(A, smallnum=1e-4, droptol=1e-4):
+    """IC(0) as the reference stores it: lower CSR (lp, lc, lv) with 1/l_ii
+    last in each row."""
+    rp, ci, v = _csr(A)
+    n = len(rp) - 1
+    cap = max(len(v), 1)
+    lp = np.zeros(n + 1, np.int32); lc = np.zeros(cap, np.int32); lv = np.zeros(cap)
+    rc = lib().or_ic0(n, rp, ci, v, smallnum, droptol, lp, lc, lv)
+    if rc:
+        raise ValueError("row %d: lower part does not end at the diagonal" % (rc - 1))
+    return lp, lc[:lp[n]].copy(), lv[:lp[n]].copy()
+
+
+def ict(A, droptol, maxfil, smallnum=1e-4):
+    """ICT (the reference's ict_csr_A_l) in the same layout as ic0()."""
+    rp, ci, v = _csr(A)
+    n = len(rp) - 1
+    cap = (maxfil + 1) * n
+    lp = np.zeros(n + 1, np.int32); lc = np.zeros(cap, np.int32); lv = np.zeros(cap)
+    rc = lib().or_ict(n, rp, ci, v, smallnum, droptol, maxfil, lp, lc, lv)
+    if rc:
+        raise ValueError("row %d has no diagonal entry" % (rc - 1))
+    return lp, lc[:lp[n]].copy(), lv[:lp[n]].copy()
+
+
+def ic_split(lp, lc, lv):
+    """seqic0 / seqict's conversion: (L~ strict lower CSR, U = L~^T strict
+    upper CSR, dinv = 1/l_ii^2)."""
+    lp = np.ascontiguousarray(lp, np.int32); lc = np.ascontiguousarray(lc, np.int32)
+    lv = np.ascontiguousarray(lv, np.float64)
+    n = len(lp) - 1
+    cap = max(int(lp[n]) - n, 1)
+    lrp = np.zeros(n + 1, np.int32); urp = np.zeros(n + 1, np.int32)
+    lci = np.zeros(cap, np.int32); uci = np.zeros(cap, np.int32)
+    lvs = np.zeros(cap); uvs = np.zeros(cap); dinv = np.zeros(n)
+    lib().or_ic_split(n, lp, lc, lv, lrp, lci, lvs, urp, uci, uvs, dinv)
+    m = lrp[n]
+    return (lrp, lci[:m].copy(), lvs[:m].copy()), (urp, uci[:m].copy(), uvs[:m].copy()), dinv
+
+
+def ic_combined(A, lp, lc, lv):
+    """An IC factor (ic0() layout, on a lower pattern contained in A's) as
+    combined LU values on A's pattern: L~_ij = l_ij / l_jj below the
+    diagonal, U = D L~^T (D = diag(l_ii^2)) on and above it; entries of A
+    outside the factor's pattern are 0."""
+    rp, ci, _ = _csr(A)
+    n = len(rp) - 1
+    d = np.empty(n)
+    vals = {}
+    for i in range(n):
+        for e in range(lp[i], lp[i + 1]):
+            if lc[e] == i:
+                d[i] = 1.0 / lv[e]
+    for i in range(n):
+        for e in range(lp[i], lp[i + 1]):
+            j = int(lc[e])
+            if j != i:
+                vals[(i, j)] = lv[e] / d[j]
+    lu = np.zeros(len(ci))
+    for i in range(n):
+        for e in range(rp[i], rp[i + 1]):
+            j = int(ci[e])
+            if j < i:
+                lu[e] = vals.get((i, j), 0.0)
+            elif j == i:
+                lu[e] = d[i] * d[i]
+            else:
+                lu[e] = d[i] * d[i] * vals.get((j, i), 0.0)
+    return lu
 
 
 def ilu_split(rp, ci, luval, uptr):

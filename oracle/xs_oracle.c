@@ -272,6 +272,209 @@ static void or_sweep_apply(void *ctx, const double *x, double *y) {
   or_ilu_sweeps(c->n, c->rp, c->ci, c->luval, c->uptr, c->kl, c->ku, x, y);
 }
 
+/* --- Incomplete Cholesky (CPU preconditioners of the reference) ---------- */
+/* IC(0), hip/Precond/ic0.cpp:5-75 (ic0_csr_A).  initLByA (:5-27): L = lower
+ * part of A with the diagonal (last in each row), tnorm_i = droptol ||a_i||_2
+ * over the whole row.  ic0_fact (:29-67), row by row:
+ *   l_ij = (a_ij - sum_{k<j} l_ik l_jk) / l_jj  (merge of row i's entries
+ *          before j with row j's strict entries; subtraction in ascending k),
+ *   l_ii = sqrt(a_ii - sum_j l_ij^2),
+ * with |l_ii| < 1e-20 replaced by smallnum / droptol tnorm_i + tnorm_i, and
+ * 1/l_ii stored in place of l_ii (so "/ l_jj" is a product with that
+ * entry).  smallnum = droptol = 1e-4 in the reference (precon_ic0.h:14-15). */
+int or_ic0(int n, const int *rp, const int *ci, const double *v, double smallnum,
+           double droptol, int *lp, int *lc, double *lv) {
+  double *tn = (double *)calloc((size_t)n, sizeof(double));
+  int len = 0;
+  lp[0] = 0;
+  for (int i = 0; i < n; i++) {
+    for (int e = rp[i]; e < rp[i + 1]; e++) {
+      if (ci[e] <= i) { lv[len] = v[e]; lc[len] = ci[e]; len++; }
+      tn[i] += v[e] * v[e];
+    }
+    lp[i + 1] = len;
+    tn[i] = sqrt(tn[i]) * droptol;
+    if (lp[i + 1] == lp[i] || lc[lp[i + 1] - 1] != i) { free(tn); return i + 1; }
+  }
+  lv[0] = sqrt(lv[0]);
+  if (fabs(lv[0]) < 1e-20) lv[0] = smallnum / droptol * tn[0] + tn[0];
+  lv[0] = 1.0 / lv[0];
+  for (int i = 1; i < n; i++) {
+    const int dl = lp[i + 1] - 1;
+    for (int j = lp[i]; j < dl; j++) {
+      const int c = lc[j];
+      int from = lp[c];
+      const int du = lp[c + 1] - 1;
+      for (int k = lp[i]; k < j; k++)
+        for (int p = from; p < du; p++) {
+          if (lc[p] == lc[k]) {
+            lv[j] -= lv[k] * lv[p];
+          } else if (lc[p] > lc[k]) {
+            from = p;
+            break;
+          }
+        }
+      lv[j] *= lv[du];
+    }
+    for (int j = lp[i]; j < dl; j++) lv[dl] -= lv[j] * lv[j];
+    lv[dl] = sqrt(lv[dl]);
+    if (fabs(lv[dl]) < 1e-20) lv[dl] = smallnum / droptol * tn[i] + tn[i];
+    lv[dl] = 1.0 / lv[dl];
+  }
+  free(tn);
+  return 0;
+}
+
+/* Min-heap on |val| over the first m entries (adjustHeap_l,
+ * hip/Precond/ict_l.cpp:4-24): sift down from i, the smaller child taken on
+ * |left| > |right|, moved up while strictly smaller than the sifted value. */
+static void ic_sift(double *val, int *col, int i, int m) {
+  const double tv = val[i];
+  const int tc = col[i];
+  for (int k = 2 * i + 1; k < m; k = 2 * k + 1) {
+    if (k + 1 < m && fabs(val[k]) > fabs(val[k + 1])) k++;
+    if (fabs(val[k]) < fabs(tv)) {
+      val[i] = val[k];
+      col[i] = col[k];
+      i = k;
+    } else {
+      break;
+    }
+  }
+  val[i] = tv;
+  col[i] = tc;
+}
+
+/* the m largest |val| of cnt candidates kept in val[0..m) (heapSort_part_l,
+ * ict_l.cpp:25-41): heapify from m/2 down, then a later candidate replaces
+ * the root only when strictly larger in magnitude */
+static void ic_keep_largest(double *val, int *col, int m, int cnt) {
+  for (int i = m / 2; i >= 0; i--) ic_sift(val, col, i, m);
+  for (int i = m; i < cnt; i++)
+    if (fabs(val[i]) > fabs(val[0])) {
+      val[0] = val[i];
+      col[0] = col[i];
+      ic_sift(val, col, 0, m);
+    }
+}
+
+/* ICT, hip/Precond/ict_l.cpp:139-232 (ict_csr_A_l / ict_fact_l).
+ * tnorm_i = droptol ||a_i||_2 (calTnorm :117-137).  Row i (left-looking):
+ * a dense work row w holds A's row i (every entry of the row, the upper part
+ * included); w is cleared first over [first column of L's row c0, i], c0 the
+ * first column of A's row i -- the one work array serves every row, and
+ * positions below that range keep what an earlier row left there, as in the
+ * reference.  Then for j = c0 .. i-1 in order:
+ *   w_j = (w_j - sum_{p in L_j strict} w[col p] l_p) / l_jj   (sum in p order),
+ *   kept when |w_j| > tnorm_i, and then w_i -= w_j^2 (a dropped w_j still
+ *   feeds the later columns of the row);
+ * |w_i| < 1e-20 -> smallnum / droptol tnorm_i + tnorm_i; 1/sqrt(|w_i|) is
+ * stored as the diagonal (last).  More than maxfil kept: the maxfil largest
+ * in magnitude (heap semantics above), in ascending column order.  Row 0 is
+ * a_00 alone.  The reference passes the macro small_num = 1e-4 as smallnum
+ * (HostMatrixCSR.cpp:331, ilut_diag_in_csr.h:7). */
+int or_ict(int n, const int *rp, const int *ci, const double *v, double smallnum,
+           double droptol, int maxfil, int *lp, int *lc, double *lv) {
+  double *tn = (double *)calloc((size_t)n, sizeof(double));
+  double *w = (double *)calloc((size_t)n, sizeof(double));
+  double *tv = (double *)malloc(sizeof(double) * (size_t)n);
+  int *tc = (int *)malloc(sizeof(int) * (size_t)n);
+  int rc = 0;
+  for (int i = 0; i < n; i++) {
+    int has = 0;
+    for (int e = rp[i]; e < rp[i + 1]; e++) {
+      tn[i] += v[e] * v[e];
+      if (ci[e] == i) has = 1;
+    }
+    if (!has) { rc = i + 1; goto out; }
+    tn[i] = sqrt(tn[i]) * droptol;
+  }
+  lp[0] = 0;
+  lv[0] = v[0];
+  lc[0] = 0;
+  if (fabs(lv[0]) < 1e-20) lv[0] = smallnum * (1.0 / droptol) * tn[0] + tn[0];
+  lv[0] = 1.0 / sqrt(fabs(lv[0]));
+  lp[1] = 1;
+  for (int i = 1; i < n; i++) {
+    int cnt = 0;
+    const int c0 = ci[rp[i]];
+    /* a row without entries left of the diagonal would make the reference
+     * read L's row i before it is written (lc[lp[i]]); w_i alone is cleared */
+    for (int j = c0 < i ? lc[lp[c0]] : i; j <= i; j++) w[j] = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; e++) w[ci[e]] = v[e];
+    for (int j = c0; j < i; j++) {
+      const int du = lp[j + 1] - 1;
+      double t = 0.0;
+      for (int p = lp[j]; p < du; p++) t += w[lc[p]] * lv[p];
+      w[j] -= t;
+      w[j] *= lv[du];
+      if (fabs(w[j]) > tn[i]) {
+        tv[cnt] = w[j];
+        tc[cnt] = j;
+        cnt++;
+        w[i] -= w[j] * w[j];
+      }
+    }
+    if (fabs(w[i]) < 1e-20) w[i] = smallnum * (1.0 / droptol) * tn[i] + tn[i];
+    w[i] = 1.0 / sqrt(fabs(w[i]));
+    int keep = cnt;
+    if (cnt > maxfil) {
+      ic_keep_largest(tv, tc, maxfil, cnt);
+      keep = maxfil;
+    }
+    /* ascending columns (the columns are distinct, so the order is unique) */
+    for (int a = 1; a < keep; a++) {
+      const int c = tc[a];
+      const double x = tv[a];
+      int b = a - 1;
+      while (b >= 0 && tc[b] > c) { tc[b + 1] = tc[b]; tv[b + 1] = tv[b]; b--; }
+      tc[b + 1] = c;
+      tv[b + 1] = x;
+    }
+    int at = lp[i];
+    for (int a = 0; a < keep; a++) { lv[at] = tv[a]; lc[at] = tc[a]; at++; }
+    lv[at] = w[i];
+    lc[at] = i;
+    lp[i + 1] = lp[i] + keep + 1;
+  }
+out:
+  free(tn); free(w); free(tv); free(tc);
+  return rc;
+}
+
+/* seqic0 / seqict (hip/HostMatrix/HostMatrixCSR.cpp:269-326, :327-384):
+ * d_i = stored 1/l_ii; L~_ij = l_ij d_j (strict, CSR in row order); U =
+ * L~^T as CSR (row j lists i ascending, filled in row order of i);
+ * dinv_i = d_i^2.  LUsolve (:385-409) then applies L~, dinv, U. */
+void or_ic_split(int n, const int *lp, const int *lc, const double *lv, int *lrp, int *lci,
+                 double *lvs, int *urp, int *uci, double *uvs, double *dinv) {
+  for (int i = 0; i <= n; i++) urp[i] = 0;
+  for (int i = 0; i < n; i++)
+    for (int e = lp[i]; e < lp[i + 1]; e++) {
+      if (lc[e] == i) dinv[i] = lv[e];
+      else urp[lc[e] + 1]++;
+    }
+  for (int i = 1; i <= n; i++) urp[i] += urp[i - 1];
+  int *fill = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+  memcpy(fill, urp, sizeof(int) * (size_t)(n + 1));
+  int il = 0;
+  lrp[0] = 0;
+  for (int i = 0; i < n; i++) {
+    for (int e = lp[i]; e < lp[i + 1]; e++) {
+      const int j = lc[e];
+      if (j == i) continue;
+      const double t = lv[e] * dinv[j];
+      lvs[il] = t;
+      lci[il++] = j;
+      uvs[fill[j]] = t;
+      uci[fill[j]++] = i;
+    }
+    lrp[i + 1] = il;
+  }
+  for (int i = 0; i < n; i++) dinv[i] *= dinv[i];
+  free(fill);
+}
+
 /* --- ILUT (dual threshold, per-row heaps) -------------------------------- */
 /* Min-heap of column ids ordered by |w[id]| ("less" = larger magnitude, so the
  * top is the smallest magnitude).  vh_push restates libstdc++'s push_heap
@@ -846,6 +1049,29 @@ int or_solve(const char *ksp, const char *pc, int n, const int *rp,
       lu.urp = urp; lu.uci = uci; lu.uv = uv; lu.dinv = dinv;
       M = or_lusolve; Mctx = &lu;
     }
+  } else if (streq(pc, "ic0") || streq(pc, "ict")) {
+    /* IC0: smallnum = droptol = 1e-4 (precon_ic0.h:14-15); ICT: droptol =
+     * ilut_tol (xsolver_pc_setict, precon_ict.h:27-30), maxfil = ilut_fill,
+     * capped at n - 1 (precon_ict.cpp:3-4), smallnum = 1e-4 (the macro the
+     * reference passes, HostMatrixCSR.cpp:331) */
+    int nnz = rp[n];
+    int maxfil = ilut_fill > n - 1 ? n - 1 : ilut_fill;
+    int cap = streq(pc, "ic0") ? nnz : (maxfil + 1) * n;
+    int *lp = malloc(sizeof(int) * (n + 1));
+    int *lc = malloc(sizeof(int) * (size_t)cap);
+    double *lvv = malloc(sizeof(double) * (size_t)cap);
+    int frc = streq(pc, "ic0") ? or_ic0(n, rp, ci, v, 1e-4, 1e-4, lp, lc, lvv)
+                               : or_ict(n, rp, ci, v, 1e-4, ilut_tol, maxfil, lp, lc, lvv);
+    if (frc) { free(lp); free(lc); free(lvv); goto out; }
+    lrp = malloc(sizeof(int) * (n + 1)); urp = malloc(sizeof(int) * (n + 1));
+    lci = malloc(sizeof(int) * (size_t)cap); uci = malloc(sizeof(int) * (size_t)cap);
+    lv = malloc(sizeof(double) * (size_t)cap); uv = malloc(sizeof(double) * (size_t)cap);
+    dinv = malloc(sizeof(double) * n);
+    or_ic_split(n, lp, lc, lvv, lrp, lci, lv, urp, uci, uv, dinv);
+    free(lp); free(lc); free(lvv);
+    lu.n = n; lu.lrp = lrp; lu.lci = lci; lu.lv = lv;
+    lu.urp = urp; lu.uci = uci; lu.uv = uv; lu.dinv = dinv;
+    M = or_lusolve; Mctx = &lu;
   } else if (streq(pc, "parilut")) {
     /* ParILUT: 10 outer steps (hip/Precond/precon_gpuilut.cpp:5), fill factor
      * ilut_fill, relative drop tolerance ilut_tol; split as seqilut does. */

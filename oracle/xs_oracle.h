@@ -92,7 +92,22 @@ int or_parilut(int n, const int *arp, const int *aci, const double *av, int swee
                double fill, double droptol, int cap, int *lrp, int *lci, double *lv,
                int *urp, int *uci, double *uv, int *trace);
 
-/* ---- driver: pc in {"none","jacobi","ilu0","ilut"},
+/* ---- incomplete Cholesky (the reference's CPU IC0 / ICT) ----------------
+ * Output L: lower CSR including the diagonal, stored last in each row as
+ * 1/l_ii (ic0.cpp / ict_l.cpp layout).  Return 0, or i+1 for the first row
+ * whose lower part does not end at the diagonal (IC0) / has no diagonal
+ * (ICT).  or_ic0: cap >= nnz(lower(A)); or_ict: cap >= (maxfil + 1) n. */
+int or_ic0(int n, const int *rp, const int *ci, const double *v, double smallnum,
+           double droptol, int *lp, int *lc, double *lv);
+int or_ict(int n, const int *rp, const int *ci, const double *v, double smallnum,
+           double droptol, int maxfil, int *lp, int *lc, double *lv);
+/* seqic0 / seqict's conversion to the LUsolve form (or_lu_t): L~ strict lower
+ * (l_ij / l_jj), U = L~^T strict upper (CSR by transposition), dinv = 1/l_ii^2.
+ * Arrays of capacity lp[n] - n (L~, U) and n (dinv, lrp/urp n + 1). */
+void or_ic_split(int n, const int *lp, const int *lc, const double *lv, int *lrp, int *lci,
+                 double *lvs, int *urp, int *uci, double *uvs, double *dinv);
+
+/* ---- driver: pc in {"none","jacobi","ilu0","ilut","parilut","ic0","ict"},
  *      ksp in {"cg","bicgstab","gcr","gmres","fgmres"} ------------------- */
 int or_solve(const char *ksp, const char *pc, int n, const int *rp,
              const int *ci, const double *v, const double *b, double *x,

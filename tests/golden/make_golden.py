@@ -1,10 +1,13 @@
-"""Write golden ILU(0)/ILUT vectors from the reference's own CPU ILU code.
+"""Write golden ILU(0)/ILUT and IC(0)/ICT vectors from the reference's own
+CPU code.
 
-Runs oracle/_ref/libxsref.so -- hip/Precond/ilut_diag_in_csr.cpp compiled in
-place from the reference tree by ``make -C oracle ref`` -- on small seeded
-matrices and stores inputs and outputs in tests/golden/ilu_golden.npz.  The
-fixture is data only (arrays); the tests compare the oracle's restatement
-(oracle/xs_oracle.c: or_ilu0, or_ilut) against it bit for bit.
+Runs oracle/_ref/libxsref.so -- hip/Precond/ilut_diag_in_csr.cpp, ic0.cpp and
+ict_l.cpp compiled in place from the reference tree by ``make -C oracle ref``
+-- on small seeded matrices and stores inputs and outputs in
+tests/golden/ilu_golden.npz and tests/golden/ic_golden.npz.  The fixtures are
+data only (arrays); the tests compare the oracle's restatements
+(oracle/xs_oracle.c: or_ilu0, or_ilut, or_ic0, or_ict) against them bit for
+bit.
 
 Usage:  python tests/golden/make_golden.py
 """
@@ -85,6 +88,54 @@ def main():
             out[key + "/uptr"] = up[:n]
             out[key + "/diag"] = diag
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ilu_golden.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, len(out), "arrays")
+    ic_main(R)
+
+
+def spd_matrices():
+    """Symmetric positive definite inputs for IC(0) / ICT."""
+    yield "poisson2d_6", gen.poisson2d(6)
+    yield "grf2d_8", gen.vardiff2d(gen.grf_kappa(8, 0.3, 4))
+    yield "lognormal27_4", gen.lognormal27(4, seed=2)
+    yield "poisson2d_11", gen.poisson2d(11)
+
+
+# (droptol, maxfil): the reference's defaults (1e-8, 100), a drop-dominated
+# setting, a fill-limited setting with many ties (constant-coefficient
+# stencils), no dropping at all
+ICT_PARAMS = [(1e-8, 100), (1e-2, 3), (1e-3, 1), (0.0, 50)]
+
+
+def ic_main(R):
+    out = {}
+    for name, (rp, ci, v) in spd_matrices():
+        rp = np.ascontiguousarray(rp, np.int32)
+        ci = np.ascontiguousarray(ci, np.int32)
+        v = np.ascontiguousarray(v, np.float64)
+        n = len(rp) - 1
+        out[f"{name}/rowptr"] = rp
+        out[f"{name}/colidx"] = ci
+        out[f"{name}/val"] = v
+        lp = np.zeros(n + 1, np.int32)
+        lc = np.zeros(len(v), np.int32)
+        lv = np.zeros(len(v))
+        R.ref_ic0_csr_A(n, rp, v.copy(), ci, lp, lv, lc, 1e-4, 1e-4)
+        out[f"{name}/ic0/lptr"] = lp
+        out[f"{name}/ic0/lcol"] = lc[:lp[n]]
+        out[f"{name}/ic0/lval"] = lv[:lp[n]]
+        for tol, mf in ICT_PARAMS:
+            mf = min(mf, n - 1)
+            cap = (mf + 1) * n
+            lp = np.zeros(n + 1, np.int32)
+            lc = np.zeros(cap, np.int32)
+            lv = np.zeros(cap)
+            R.ref_ict_csr_A_l(n, rp, v.copy(), ci, lp, lv, lc, 1e-4, tol, mf)
+            key = f"{name}/ict_{tol:g}_{mf}"
+            out[key + "/lptr"] = lp
+            out[key + "/lcol"] = lc[:lp[n]]
+            out[key + "/lval"] = lv[:lp[n]]
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ic_golden.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, len(out), "arrays")
 
