@@ -28,7 +28,7 @@ enum xs_status {
   XS_NOT_CONVERGED = 1,     /* maxiter reached (the reference's usediter = maxiter) */
   XS_ERR_ARG = -1,          /* invalid argument / unknown type string */
   XS_ERR_STATE = -2,        /* call out of order (e.g. solve before mat_setup) */
-  XS_ERR_UNSUPPORTED = -3,  /* e.g. IC0/ICT on GPU, MPI distribution, CPU backend */
+  XS_ERR_UNSUPPORTED = -3,  /* e.g. MPI distribution, CPU backend, IC0 on a nonsymmetric pattern */
   XS_ERR_DEVICE = -4,       /* HIP runtime error or no device */
   XS_ERR_BREAKDOWN = -5,    /* non-finite residual / Krylov breakdown */
   XS_ERR_PIVOT = -6,        /* missing diagonal entry / zero Jacobi pivot */
@@ -54,9 +54,14 @@ void xsolver_ksp_set_absolute_tol(double absolute_tol);
  * residual norms, resvec[0] = ||r0||) are written when xsolver_solve returns;
  * resvec may be NULL, otherwise it must hold maxiter + 1 doubles. */
 void xsolver_ksp_setoption(double tol, int maxiter, int *usediter, double *resvec, int restart);
-/* hip/xsolver_c.cpp:150-262: "Jacobi", "Ilu0", "Ilut", "Ilu0_mkl", "Ilut_mkl"
- * in the reference's spellings (on the GPU the *_mkl names map to ParILU/
- * ParILUT as in :196-211); "Ic0"/"Ict" -> XS_ERR_UNSUPPORTED (:174-177). */
+/* hip/xsolver_c.cpp:150-262: "Jacobi", "Ilu0", "Ilut", "Ilu0_mkl", "Ilut_mkl",
+ * "Ic0" in the reference's spellings (on the GPU the *_mkl names map to
+ * ParILU/ParILUT as in :196-211).  The reference runs IC0 on the host only
+ * (:174-177 exits on the GPU); here "Ic0" is ParIC(0) on the GPU, whose fixed
+ * point is the reference's IC(0) (hip/Precond/ic0.cpp), applied as
+ * L~ D L~^T; it needs a structurally symmetric pattern (else
+ * XS_ERR_UNSUPPORTED) and, like the reference, reads only the lower
+ * triangle's values. */
 void xsolver_pc_settype(char *fmt);
 /* hip/xsolver_c.cpp:264-273 */
 void xsolver_pc_setilut(double tol, int max_fill);

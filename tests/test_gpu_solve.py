@@ -130,7 +130,7 @@ def test_edge_cases(dev):
     with pytest.raises(XSolverError):
         S.ksp_settype("minres")
     with pytest.raises(XSolverError) as e:
-        S.pc_settype("ic0")
+        S.pc_settype("ict")
     assert e.value.status == xsolver_amd.XS_ERR_UNSUPPORTED
     S.close()
     # zero diagonal -> Jacobi pivot error (reference exits, HostVector.cpp:88-92)

@@ -21,6 +21,7 @@
 #include "xs_internal.h"
 #include "xs_multi.h"
 #include "xs_parilut.h"
+#include "xs_ic.h"
 #include "xs_solvers.h"
 
 // ---------------------------------------------------------------------------
@@ -70,7 +71,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
   dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);
-  dfree(F.app); dfree(F.apd); dfree(F.cmap); dfree(F.lc);
+  dfree(F.app); dfree(F.apd); dfree(F.cmap); dfree(F.lc); dfree(F.mpos);
   dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.vlo); dfree(F.tl.co); dfree(F.tl.ww);
   dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co); dfree(F.tu.ww); dfree(F.tu.vs);
   c->tell_ready = c->tell32_ready = false;
@@ -610,7 +611,8 @@ extern "C" int xs_pc_settype(xs_ctx *c, const char *t) {
   else if (streq_any(t, {"Ilu0", "ilu0", "ILU0", "Ilu0_mkl", "ilu0_mkl", "ILU0_MKL"})) pc = XS_PC_ILU0;
   else if (streq_any(t, {"Ilut", "ilut", "ILUT", "Ilut_mkl", "ilut_mkl", "ILUT_MKL"})) pc = XS_PC_ILUT;
   else if (streq_any(t, {"None", "none", "NONE"})) pc = XS_PC_NONE;
-  else if (streq_any(t, {"Ic0", "ic0", "IC0", "Ict", "ict", "ICT"})) return XS_ERR_UNSUPPORTED;
+  else if (streq_any(t, {"Ic0", "ic0", "IC0"})) pc = XS_PC_IC0;
+  else if (streq_any(t, {"Ict", "ict", "ICT"})) return XS_ERR_UNSUPPORTED;
   else return XS_ERR_ARG;
   if (pc != c->pc) free_precond(c);
   c->pc = pc;
@@ -804,9 +806,9 @@ static int ensure_trsv_sched(xs_ctx *c) {
   return rc;
 }
 
-// Update lists of the ParILU(0) sweeps (once per pattern, on the device):
-// count the products of every entry, scan, fill.
-static int build_parilu_lists(xs_ctx *c) {
+// Update lists of the ParILU(0) / ParIC(0) sweeps (once per pattern, on
+// the device): count the products of every entry, scan, fill.
+static int build_parilu_lists(xs_ctx *c, bool ic) {
   xs_ilu &F = c->ilu;
   const int nnz = c->A.nnz;
   int rc, *cnt = nullptr, *bsum = nullptr, np = 0;
@@ -814,14 +816,15 @@ static int build_parilu_lists(xs_ctx *c) {
   if ((rc = dalloc(&cnt, nnz)) || (rc = dalloc(&bsum, xs_scan_bsum_len(nnz))) ||
       (rc = dalloc(&F.pd, nnz)) || (rc = dalloc(&F.pp, (size_t)nnz + 1)))
     return done(rc);
-  xs_parilu_pairs(c->stream, c->A, F, cnt, nullptr, nullptr, nullptr, F.pd);
+  auto pairs = ic ? xs_ic_pairs : xs_parilu_pairs;
+  pairs(c->stream, c->A, F, cnt, nullptr, nullptr, nullptr, F.pd);
   xs_scan_excl(c->stream, nnz, cnt, F.pp, bsum);
   if (hipMemcpyAsync(&np, F.pp + nnz, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return done(XS_ERR_DEVICE);
   if ((rc = dalloc(&F.pl, (size_t)std::max(np, 1))) || (rc = dalloc(&F.pu, (size_t)std::max(np, 1))))
     return done(rc);
-  xs_parilu_pairs(c->stream, c->A, F, nullptr, F.pp, F.pl, F.pu, nullptr);
+  pairs(c->stream, c->A, F, nullptr, F.pp, F.pl, F.pu, nullptr);
   // active entries (a sweep can change them): flag, scan, compact
   int *flag = nullptr, *pos = nullptr;
   auto done2 = [&](int r) { dfree(flag); dfree(pos); return done(r); };
@@ -845,10 +848,11 @@ static int build_parilu_lists(xs_ctx *c) {
   xs_remap_idx(c->stream, np, F.cmap, F.pu);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return done2(XS_ERR_DEVICE);
   if (xs_take_launch_guard()) return done2(XS_ERR_STATE);
+  F.lists = ic ? 2 : 1;
   return done2(XS_OK);
 }
 
-static int setup_ilu_pattern(xs_ctx *c) {
+static int setup_ilu_pattern(xs_ctx *c, bool ic) {
   // Combined-factor layout on A's pattern plus the CSC of the upper triangle
   // used by the ParILU sweep (the reference builds L CSR / U CSC with
   // hipsparseDcsr2csc, hip/Precond/par_ilu_hip/Asplit_gpu.cpp:62-161).
@@ -888,7 +892,20 @@ static int setup_ilu_pattern(xs_ctx *c) {
   XS_HIP(hipMemcpy(F.ucp, ucnt.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
   XS_HIP(hipMemcpy(F.uri, uri.data(), sizeof(int) * nu, hipMemcpyHostToDevice));
   XS_HIP(hipMemcpy(F.upos, upos.data(), sizeof(int) * nu, hipMemcpyHostToDevice));
-  if ((rc = build_parilu_lists(c))) return rc;
+  if (ic) {
+    // every entry needs its mirror: U = D L~^T lives on A's upper pattern
+    if ((rc = dalloc(&F.mpos, nnz))) return rc;
+    XS_HIP(hipMemsetAsync(c->d_int, 0, sizeof(int), c->stream));
+    xs_ic_mirror(c->stream, c->A, F.rowidx, F.mpos, c->d_int);
+    int asym = 0;
+    XS_HIP(hipMemcpyAsync(&asym, c->d_int, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    XS_HIP(hipStreamSynchronize(c->stream));
+    if (asym) {
+      snprintf(xs_errbuf, sizeof(xs_errbuf), "IC0: the matrix pattern is not symmetric");
+      return XS_ERR_UNSUPPORTED;
+    }
+  }
+  if ((rc = build_parilu_lists(c, ic))) return rc;
   return build_trsv_schedules(c, rp.data(), ci.data(), dpos.data());
 }
 
@@ -899,8 +916,9 @@ static int setup_ilu_pattern(xs_ctx *c) {
 // 27-point lognormal systems.
 static int factor_parilu(xs_ctx *c) {
   xs_ilu &F = c->ilu;
+  if (F.lu && F.lists != 1) free_precond(c);   // a pattern set up for ParIC(0)
   if (!F.lu) {
-    int rc = setup_ilu_pattern(c);
+    int rc = setup_ilu_pattern(c, false);
     if (rc) return rc;
   }
   if (!c->parilu_async && !c->lu_tmp) {
@@ -952,6 +970,44 @@ static int factor_parilu(xs_ctx *c) {
   return XS_OK;
 }
 
+// ParIC(0) (no reference counterpart on the GPU; the fixed point is the
+// reference's IC(0), hip/Precond/ic0.cpp:29-67): parilu_sweeps sweeps of the
+// lower triangle (asynchronous in place, or deterministic double-buffered),
+// on S A S with parilu_scale, then the combined factors L~ | D L~^T on A's
+// pattern (kernels in xs_ic.hip).
+static int factor_paric(xs_ctx *c) {
+  xs_ilu &F = c->ilu;
+  if (F.lu && F.lists != 2) free_precond(c);   // a pattern set up for ParILU(0)
+  int rc;
+  if (!F.lu && (rc = setup_ilu_pattern(c, true))) {
+    free_precond(c);
+    return rc;
+  }
+  if (!c->parilu_async && !c->lu_tmp && (rc = dalloc(&c->lu_tmp, c->A.nnz))) return rc;
+  const double *sc = nullptr;
+  if (c->parilu_scale) {
+    if (!F.scale && (rc = dalloc(&F.scale, c->A.n))) return rc;
+    xs_pt_scale_vec(c->stream, c->A.n, c->A.rp, c->A.ci, c->A.v, F.scale);
+    sc = F.scale;
+  }
+  if ((!F.as && (rc = dalloc(&F.as, c->A.nnz))) || (!F.lc && (rc = dalloc(&F.lc, c->A.nnz))))
+    return rc;
+  double *cur = F.lc, *nxt = c->parilu_async ? F.lc : c->lu_tmp;
+  xs_ic_start(c->stream, c->A, F, sc, c->parilu_async ? nullptr : c->lu_tmp);
+  for (int s = 0; s < c->parilu_sweeps; s++) {
+    xs_ic_sweep_pairs(c->stream, F, cur, nxt);
+    std::swap(cur, nxt);
+  }
+  xs_ic_finish(c->stream, c->A, F, F.mpos, sc, cur);
+  XS_HIP(hipMemsetAsync(c->d_int, 0x7f, sizeof(int), c->stream));
+  xs_ilu_dinv(c->stream, F, c->d_int);
+  int bad = 0;
+  XS_HIP(hipMemcpyAsync(&bad, c->d_int, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  XS_HIP(hipStreamSynchronize(c->stream));
+  if (bad != 0x7f7f7f7f) return XS_ERR_PIVOT;
+  return XS_OK;
+}
+
 // ParILUT (hip/Precond/parilut.cpp:621-1206, precon_gpuilut.cpp:5: 10 outer
 // steps): kernels in xs_parilut.hip.  Fill factor = ilut_fill (final keep
 // rate per factor relative to A's strict parts), relative drop tolerance =
@@ -964,8 +1020,9 @@ int xs_parilut_factor(xs_ctx *c) {
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own);
   dfree(F.ucp); dfree(F.uri); dfree(F.upos);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);   // ILU(0) lists only
-  dfree(F.app); dfree(F.apd); dfree(F.cmap); dfree(F.lc);
+  dfree(F.app); dfree(F.apd); dfree(F.cmap); dfree(F.lc); dfree(F.mpos);
   dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
+  F.lists = 0;
   F.lgroups = F.ugroups = 0;
   F.sched_pending = false;
   F.nact = 0;
@@ -1128,6 +1185,9 @@ extern "C" int xs_pc_setup(xs_ctx *c) {
     }
     case XS_PC_ILU0:
       rc = factor_parilu(c);
+      break;
+    case XS_PC_IC0:
+      rc = factor_paric(c);
       break;
     case XS_PC_ILUT:
       rc = xs_parilut_factor(c);
@@ -1297,6 +1357,7 @@ void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state
       break;
     case XS_PC_ILU0:
     case XS_PC_ILUT:
+    case XS_PC_IC0:
       if (c->use_t) {
         xs_fill_sentinel(c->stream, tmp, c->A.n);
         xs_trsv_lower_t(c->stream, c->ilu, x, tmp, c->counters + 2, st);
@@ -1331,7 +1392,7 @@ void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state
 // L pass (one kernel less, same values); otherwise the s update runs first.
 void xs_op_pc_s(xs_ctx *c, const double *r, const double *u, double *sv, double *y, double *tmp,
                 xs_state *st) {
-  const bool fuse = (c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) && !c->use_t &&
+  const bool fuse = xs_pc_factored(c->pc) && !c->use_t &&
                     c->lsweeps >= 2 && c->tell_ready && st != nullptr;
   if (!fuse) {
     xs_bicg_s(c->stream, c->A.n, r, u, sv, nullptr, nullptr, st);
@@ -1389,7 +1450,7 @@ static int prepare_solve(xs_ctx *c) {
   if (c->maxiter <= 0) return XS_ERR_STATE;
   if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
   if ((rc = ensure_common(c))) return rc;
-  if ((c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) && (c->lsweeps >= 2 || c->usweeps >= 2) &&
+  if (xs_pc_factored(c->pc) && (c->lsweeps >= 2 || c->usweeps >= 2) &&
       (rc = ensure_tri_ell(c)))
     return rc;
   return ensure_trsv_sched(c);
@@ -1644,7 +1705,7 @@ int xs_build_factor_transpose(xs_ctx *c) {
 
 // y = M^{-T} x: U^T (lower, 1/u_ii) then L^T (upper, unit diagonal)
 int xs_op_pc_t(xs_ctx *c, const double *x, double *y, double *tmp, hipStream_t s) {
-  if (c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) {
+  if (xs_pc_factored(c->pc)) {
     int rc = xs_build_factor_transpose(c);
     if (rc) return rc;
     xs_fill_sentinel(s, tmp, c->A.n);
@@ -1704,7 +1765,7 @@ extern "C" int xs_pc_apply(xs_ctx *c, const double *x, double *y, void *stream) 
   int rc;
   if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
   if ((rc = xs_ctx_ensure_work(c, XS_NWORK))) return rc;
-  if ((c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) && (c->lsweeps >= 2 || c->usweeps >= 2) &&
+  if (xs_pc_factored(c->pc) && (c->lsweeps >= 2 || c->usweeps >= 2) &&
       (rc = ensure_tri_ell(c)))
     return rc;
   if ((rc = ensure_trsv_sched(c))) return rc;
@@ -1751,7 +1812,8 @@ extern "C" int xs_pc_apply_multi(xs_ctx *c, int nrhs, const double *x, double *y
       xs_mscale(s, n, nrhs, c->jac_dinv, x, y);
       break;
     case XS_PC_ILU0:
-    case XS_PC_ILUT: {
+    case XS_PC_ILUT:
+    case XS_PC_IC0: {
       if ((rc = xs_multi_reserve(c, nrhs, 1))) return rc;
       if ((c->lsweeps >= 2 || c->usweeps >= 2) && (rc = ensure_tri_ell(c))) return rc;
       if ((rc = ensure_trsv_sched(c))) return rc;
@@ -1769,7 +1831,7 @@ extern "C" int xs_pc_apply_multi(xs_ctx *c, int nrhs, const double *x, double *y
 extern "C" int xs_trsv(xs_ctx *c, int which, const double *b, double *x, void *stream) {
   if (!c || !b || !x || (which != 0 && which != 1)) return XS_ERR_ARG;
   int rc;
-  if (c->pc != XS_PC_ILU0 && c->pc != XS_PC_ILUT) return XS_ERR_STATE;
+  if (!xs_pc_factored(c->pc)) return XS_ERR_STATE;
   if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
   xs_fill_sentinel(pick(c, stream), x, c->A.n);
   if (which == 0) xs_trsv_lower(pick(c, stream), c->ilu, b, x, c->counters + 2, nullptr);
@@ -1781,7 +1843,7 @@ extern "C" int xs_trsv(xs_ctx *c, int which, const double *b, double *x, void *s
 extern "C" int xs_pc_get_factors(xs_ctx *c, int *rowptr, int *colidx, double *val, int *diagpos,
                                  int cap) {
   if (!c) return XS_ERR_ARG;
-  if (c->pc != XS_PC_ILU0 && c->pc != XS_PC_ILUT) return XS_ERR_STATE;
+  if (!xs_pc_factored(c->pc)) return XS_ERR_STATE;
   int rc;
   if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
   const xs_ilu &F = c->ilu;
@@ -1867,7 +1929,7 @@ extern "C" int xs_solve_transpose_device(xs_ctx *c, const double *d_rhs, double 
   int rc;
   if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
   if ((rc = xs_build_transpose(c))) return rc;
-  if ((c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) && (rc = xs_build_factor_transpose(c))) return rc;
+  if (xs_pc_factored(c->pc) && (rc = xs_build_factor_transpose(c))) return rc;
   c->use_t = true;
   rc = run_solve(c, d_rhs, d_sol);
   c->use_t = false;

@@ -160,6 +160,10 @@ struct xs_ilu {
   // values and lc the iterate, both in sweep order.
   int *cmap = nullptr;
   double *as = nullptr, *lc = nullptr;
+  // which update lists the pattern carries: 0 none, 1 ParILU(0), 2 ParIC(0)
+  int lists = 0;
+  // ParIC(0): position of each entry's mirror (ci[k], rowidx[k]) in A
+  int *mpos = nullptr;
   // SpTRSV work order: the 64-row groups of L (resp. U, counted from the
   // bottom) sorted by dependency level; ticket t solves group sched[t]
   int *lsched = nullptr, *usched = nullptr;

@@ -147,6 +147,7 @@ struct multi_run {
         break;
       case XS_PC_ILU0:
       case XS_PC_ILUT:
+      case XS_PC_IC0:
         xs_multi_ilu_apply(c, s, k, in, out, tmp, xs_multi_vec(c, k, 11), ms);
         break;
       default:

@@ -364,7 +364,7 @@ static int mixed_prepare(xs_ctx *c) {
   if (c->pc == XS_PC_JACOBI) {
     if ((rc = ensure_buf(&c->jac32, (size_t)n))) return rc;
     xs_vec_d2f(s, n, c->jac_dinv, c->jac32);
-  } else if (c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) {
+  } else if (xs_pc_factored(c->pc)) {
     // the low-precision buffers grow only: a refactorisation reuses them
     auto grow = [](void *&buf, size_t &have, size_t need) -> int {
       if (buf && have >= need) return XS_OK;
@@ -448,7 +448,7 @@ static void ilu_sweeps32(xs_ctx *c, bool upper, int k, const float *b, float *ou
 
 static void op_pc32(xs_ctx *c, const float *x, float *y, float *tmp, const xs_state *st) {
   const int n = c->A.n;
-  if (c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) {
+  if (xs_pc_factored(c->pc)) {
     float *w = c->work32 + 11 * (size_t)n;   // sweep scratch
     xs_prof_begin(c, 2);
     if (c->lsweeps >= 0)
@@ -477,7 +477,7 @@ static void op_pc32(xs_ctx *c, const float *x, float *y, float *tmp, const xs_st
 static void op_pc32_s(xs_ctx *c, const float *r, const float *u, float *sv, float *y, float *tmp,
                       xs_state *st) {
   const int n = c->A.n;
-  if (!((c->pc == XS_PC_ILU0 || c->pc == XS_PC_ILUT) && c->lsweeps >= 2 && c->tell32_ready)) {
+  if (!(xs_pc_factored(c->pc) && c->lsweeps >= 2 && c->tell32_ready)) {
     xs_bicg_s32(c->stream, n, r, u, sv, nullptr, nullptr, st);
     op_pc32(c, sv, y, tmp, st);
     return;

@@ -7,7 +7,12 @@
 #include "xs_internal.h"
 
 enum { XS_KSP_NONE = 0, XS_KSP_CG, XS_KSP_BICGSTAB, XS_KSP_GCR, XS_KSP_GMRES, XS_KSP_FGMRES };
-enum { XS_PC_NONE = 0, XS_PC_JACOBI, XS_PC_ILU0, XS_PC_ILUT };
+enum { XS_PC_NONE = 0, XS_PC_JACOBI, XS_PC_ILU0, XS_PC_ILUT, XS_PC_IC0 };
+// preconditioners held as combined ILU factors (xs_ilu) and applied by the
+// ILU solves: ParILU(0), ParILUT and ParIC(0) (L~ | D L~^T)
+inline bool xs_pc_factored(int pc) {
+  return pc == XS_PC_ILU0 || pc == XS_PC_ILUT || pc == XS_PC_IC0;
+}
 
 #define XS_NWORK 12   // work vectors of length n held by a context
 
