@@ -991,6 +991,273 @@ int or_batch_cg(int nsys, const int *sys_ptr, const int *rp, const int *ci, cons
 /* Precond_ilu0 = seqilu0 + LUsolve, Precond_ilup = seqilut + LUsolve).      */
 /* Returns the solver status, or -1 for an unknown name / setup failure.     */
 /* ------------------------------------------------------------------------ */
+/* ------------------------------------------------------------------------ */
+/* The rest of the intel Krylov family (intel/mpisolver-C-v1.cpp), each step  */
+/* for step, small = 1e-20 in every denominator as there.  Reported in this   */
+/* oracle's conventions: resvec[0] = ||r0||, resvec[k] = ||r_k|| (absolute    */
+/* norms of the residual each method monitors: M^-1 (b - A x) for the left-   */
+/* preconditioned ones), *usediter = iterations performed (the reference      */
+/* stores the 0-based index of the last one and relative norms).              */
+/* ------------------------------------------------------------------------ */
+#define OR_SMALL 1e-20
+
+/* the convergence test of every intel loop: ||r|| / (||r0|| + small) < tol */
+static int or_intel_check(double resl, double res0, double tol, double abstol, int it,
+                          double *resvec, int *usediter, int *rc) {
+  if (resvec) resvec[it] = resl;
+  *usediter = it;
+  if (!isfinite(resl)) { *rc = 2; return 1; }
+  if (resl / (res0 + OR_SMALL) < tol || resl <= abstol) { *rc = 0; return 1; }
+  return 0;
+}
+
+/* CG on M^-1 A (cpu_cg_solve, :26-102): r = M^-1 (b - A x); q = M^-1 A p;
+ * sigma = p.q; tao = r.r */
+int or_cg_left(int n, const int *rp, const int *ci, const double *v, const double *b,
+               double *x, double tol, double abstol, int maxiter, or_precond_fn M, void *Mctx,
+               double *resvec, int *usediter) {
+  double *r = malloc(sizeof(double) * n), *t = malloc(sizeof(double) * n);
+  double *p = malloc(sizeof(double) * n), *Ap = malloc(sizeof(double) * n);
+  double *q = malloc(sizeof(double) * n);
+  int rc = 1;
+  or_bmax(n, rp, ci, v, b, x, t);
+  apply_M(M, Mctx, n, t, r);
+  double taoo = dot(n, r, r);
+  double res0 = sqrt(taoo);
+  copy(n, r, p);
+  if (resvec) resvec[0] = res0;
+  *usediter = 0;
+  if (res0 == 0.0 || res0 <= abstol) { rc = 0; goto done; }
+  for (int it = 0; it < maxiter; it++) {
+    or_spmv(n, rp, ci, v, p, Ap);
+    apply_M(M, Mctx, n, Ap, q);
+    double sigma = dot(n, p, q);
+    double alpha = taoo / (sigma + OR_SMALL);
+    axpy(n, alpha, p, x);
+    axpy(n, -alpha, q, r);
+    double tao = dot(n, r, r);
+    if (or_intel_check(sqrt(tao), res0, tol, abstol, it + 1, resvec, usediter, &rc)) goto done;
+    double beta = tao / (taoo + OR_SMALL);
+    taoo = tao;
+    scal(n, beta, p);
+    axpy(n, 1.0, r, p);
+  }
+done:
+  free(r); free(t); free(p); free(Ap); free(q);
+  return rc;
+}
+
+/* ICG (cpu_icg_solve, :104-192): CG on M^-1 A with sigma = p.q, rho = r.q,
+ * theta = q.q in one reduction and the recurrence
+ * tao = taoo - 2 alpha rho + alpha^2 theta for the next beta (the
+ * convergence test still uses the computed r.r) */
+int or_icg(int n, const int *rp, const int *ci, const double *v, const double *b, double *x,
+           double tol, double abstol, int maxiter, or_precond_fn M, void *Mctx, double *resvec,
+           int *usediter) {
+  double *r = malloc(sizeof(double) * n), *t = malloc(sizeof(double) * n);
+  double *p = malloc(sizeof(double) * n), *Ap = malloc(sizeof(double) * n);
+  double *q = malloc(sizeof(double) * n);
+  int rc = 1;
+  or_bmax(n, rp, ci, v, b, x, t);
+  apply_M(M, Mctx, n, t, r);
+  double taoo = dot(n, r, r);
+  double res0 = sqrt(taoo);
+  copy(n, r, p);
+  if (resvec) resvec[0] = res0;
+  *usediter = 0;
+  if (res0 == 0.0 || res0 <= abstol) { rc = 0; goto done; }
+  for (int it = 0; it < maxiter; it++) {
+    or_spmv(n, rp, ci, v, p, Ap);
+    apply_M(M, Mctx, n, Ap, q);
+    double sigma = dot(n, p, q), rho = dot(n, r, q), theta = dot(n, q, q);
+    double alpha = taoo / (sigma + OR_SMALL);
+    axpy(n, alpha, p, x);
+    axpy(n, -alpha, q, r);
+    double resl = dot(n, r, r);
+    if (or_intel_check(sqrt(resl), res0, tol, abstol, it + 1, resvec, usediter, &rc)) goto done;
+    double tao = taoo - 2.0 * alpha * rho + alpha * alpha * theta;
+    double beta = tao / (taoo + OR_SMALL);
+    taoo = tao;
+    scal(n, beta, p);
+    axpy(n, 1.0, r, p);
+  }
+done:
+  free(r); free(t); free(p); free(Ap); free(q);
+  return rc;
+}
+
+/* ICGM (cpu_icgm_solve, :273-352): preconditioned CG with one reduction per
+ * iteration: sigma = p.Ap, rho = r.q, fai = Ap.z, theta = q.Ap, rr = r.r
+ * (q = M^-1 A p, z = M^-1 r), tau' = tau - alpha fai - alpha rho +
+ * alpha^2 theta; the test uses ||r|| from before the update (so x has moved
+ * one step further than the residual it is judged by) and ||r0|| of the
+ * unpreconditioned residual */
+int or_icgm(int n, const int *rp, const int *ci, const double *v, const double *b, double *x,
+            double tol, double abstol, int maxiter, or_precond_fn M, void *Mctx, double *resvec,
+            int *usediter) {
+  double *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+  double *p = malloc(sizeof(double) * n), *Ap = malloc(sizeof(double) * n);
+  double *q = malloc(sizeof(double) * n);
+  int rc = 1;
+  or_bmax(n, rp, ci, v, b, x, r);
+  apply_M(M, Mctx, n, r, z);
+  double res0 = sqrt(dot(n, r, r));
+  double tau = dot(n, r, z);
+  copy(n, z, p);
+  if (resvec) resvec[0] = res0;
+  *usediter = 0;
+  if (res0 == 0.0 || res0 <= abstol) { rc = 0; goto done; }
+  for (int it = 0; it < maxiter; it++) {
+    or_spmv(n, rp, ci, v, p, Ap);
+    apply_M(M, Mctx, n, Ap, q);
+    double sigma = dot(n, p, Ap), rho = dot(n, r, q), fai = dot(n, Ap, z);
+    double theta = dot(n, q, Ap), rr = dot(n, r, r);
+    double alpha = tau / (sigma + OR_SMALL);
+    axpy(n, alpha, p, x);
+    axpy(n, -alpha, Ap, r);
+    apply_M(M, Mctx, n, r, z);
+    double tauo = tau;
+    tau = tau - alpha * fai - alpha * rho + alpha * alpha * theta;
+    double beta = tau / (tauo + OR_SMALL);
+    scal(n, beta, p);
+    axpy(n, 1.0, z, p);
+    if (or_intel_check(sqrt(rr), res0, tol, abstol, it + 1, resvec, usediter, &rc)) goto done;
+  }
+done:
+  free(r); free(z); free(p); free(Ap); free(q);
+  return rc;
+}
+
+/* Improved BiCGSTAB (cpu_ibicgstab_solve, :484-684), right-preconditioned in
+ * y-space (x = x0 + M^-1 y), started from y = 0: t = A M^-1 r, p = 0
+ * (:539-545).  The reference takes a nonzero initial guess as y0 (:512) and
+ * its residual as b - A y0; here the guess is x0 and the correction is
+ * solved from y = 0 -- identical when x0 = 0.
+ * Per iteration two SpMVs, two preconditioner applications and six dots in
+ * one reduction (:588-600) plus ||r||. */
+int or_ibicgstab(int n, const int *rp, const int *ci, const double *v, const double *b,
+                 double *x, double tol, double abstol, int maxiter, or_precond_fn M, void *Mctx,
+                 double *resvec, int *usediter) {
+  double *buf = calloc((size_t)n * 12, sizeof(double));
+  double *res = buf, *reso = res + n, *tk = reso + n, *pk = tk + n, *qk = pk + n, *zk = qk + n;
+  double *vk = zk + n, *vko = vk + n, *sk = vko + n, *tmp = sk + n, *phi = tmp + n;
+  double *t2 = phi + n;
+  int rc = 1;
+  or_bmax(n, rp, ci, v, b, x, res);   /* x = y0 = 0: res = b */
+  double res0 = sqrt(dot(n, res, res));
+  copy(n, res, reso);
+  double alpha = 1.0, alphao = 1.0, rou = 1.0, rouo = 1.0, omega = 1.0;
+  double pi = 0.0, tau = 0.0, mu = 0.0;
+  apply_M(M, Mctx, n, res, tmp);
+  or_spmv(n, rp, ci, v, tmp, tk);
+  double fai = dot(n, res, res), sigma = dot(n, res, tk);
+  if (resvec) resvec[0] = res0;
+  *usediter = 0;
+  if (res0 == 0.0 || res0 <= abstol) { rc = 0; goto done; }
+  for (int it = 0; it < maxiter; it++) {
+    rouo = rou;
+    rou = fai - omega * mu;
+    double delta = (rou * alpha) / (rouo + OR_SMALL);
+    double beta = delta / (omega + OR_SMALL);
+    tau = sigma + beta * tau - delta * pi;
+    alphao = alpha;
+    alpha = rou / (tau + OR_SMALL);
+    copy(n, vk, vko);
+    copy(n, tk, vk);
+    axpy(n, -omega, pk, vk);
+    axpy(n, beta, vko, vk);
+    axpy(n, -delta, qk, vk);
+    apply_M(M, Mctx, n, vk, tmp);
+    or_spmv(n, rp, ci, v, tmp, qk);
+    copy(n, res, sk);
+    axpy(n, -alpha, vk, sk);
+    axpy(n, -omega, pk, tk);
+    axpy(n, -alpha, qk, tk);
+    double tmp1 = (beta * alpha) / (alphao + OR_SMALL), tmp2 = alpha * delta;
+    scal(n, tmp1, zk);
+    axpy(n, alpha, res, zk);
+    axpy(n, -tmp2, vko, zk);
+    apply_M(M, Mctx, n, tk, tmp);
+    or_spmv(n, rp, ci, v, tmp, pk);
+    fai = dot(n, reso, sk);
+    pi = dot(n, reso, qk);
+    double theta = dot(n, sk, tk), kappa = dot(n, tk, tk);
+    mu = dot(n, reso, tk);
+    double nu = dot(n, reso, pk);
+    omega = theta / (kappa + OR_SMALL);
+    sigma = mu - omega * nu;
+    copy(n, sk, res);
+    axpy(n, -omega, tk, res);
+    axpy(n, 1.0, zk, phi);
+    axpy(n, omega, sk, phi);
+    double resl = sqrt(dot(n, res, res));
+    if (or_intel_check(resl, res0, tol, abstol, it + 1, resvec, usediter, &rc)) break;
+  }
+  apply_M(M, Mctx, n, phi, t2);   /* sol = M^-1 y (:656, :667) */
+  axpy(n, 1.0, t2, x);
+done:
+  free(buf);
+  return rc;
+}
+
+/* Left-preconditioned GCR(restart) (cpu_gcr_solve, :685-810, classical
+ * Gram-Schmidt: every coefficient from the unmodified c_k) and
+ * (cpu_igcr_solve, :812-931, modified Gram-Schmidt): r = M^-1 (b - A x),
+ * u_k = r, c_k = M^-1 A u_k orthogonalised against c_0..c_{k-1} (the same
+ * coefficients applied to u_k), both scaled by 1/(||c_k|| + small), then
+ * x += (c_k.r / (||c_k|| + small)) u_k, r -= (same) c_k.  After `restart`
+ * directions k starts again at 0 (the stored directions are overwritten). */
+int or_gcr_left(int n, const int *rp, const int *ci, const double *v, const double *b,
+                double *x, double tol, double abstol, int maxiter, int restart, int mgs,
+                or_precond_fn M, void *Mctx, double *resvec, int *usediter) {
+  if (restart <= 0) restart = maxiter > 0 ? maxiter : 1;
+  double *U = calloc((size_t)restart * n, sizeof(double));
+  double *C = calloc((size_t)restart * n, sizeof(double));
+  double *r = malloc(sizeof(double) * n), *t = malloc(sizeof(double) * n);
+  double *al = calloc((size_t)restart, sizeof(double));
+  int rc = 1, it = 0;
+  or_bmax(n, rp, ci, v, b, x, t);
+  apply_M(M, Mctx, n, t, r);
+  double res0 = sqrt(dot(n, r, r));
+  if (resvec) resvec[0] = res0;
+  *usediter = 0;
+  if (res0 == 0.0 || res0 <= abstol) { rc = 0; goto done; }
+  while (it < maxiter) {
+    for (int k = 0; k < restart && it < maxiter; k++) {
+      double *uk = U + (size_t)k * n, *ck = C + (size_t)k * n;
+      copy(n, r, uk);
+      or_spmv(n, rp, ci, v, uk, t);
+      apply_M(M, Mctx, n, t, ck);
+      if (mgs) {
+        for (int i = 0; i < k; i++) {
+          double a = dot(n, C + (size_t)i * n, ck);
+          axpy(n, -a, C + (size_t)i * n, ck);
+          axpy(n, -a, U + (size_t)i * n, uk);
+        }
+      } else {
+        for (int i = 0; i < k; i++) al[i] = dot(n, C + (size_t)i * n, ck);
+        for (int i = 0; i < k; i++) {
+          axpy(n, -al[i], C + (size_t)i * n, ck);
+          axpy(n, -al[i], U + (size_t)i * n, uk);
+        }
+      }
+      double cc = sqrt(dot(n, ck, ck)), cr = dot(n, ck, r);
+      double sc = 1.0 / (cc + OR_SMALL);
+      scal(n, sc, uk);
+      scal(n, sc, ck);
+      double step = cr / (cc + OR_SMALL);
+      axpy(n, step, uk, x);
+      axpy(n, -step, ck, r);
+      it++;
+      if (or_intel_check(sqrt(dot(n, r, r)), res0, tol, abstol, it, resvec, usediter, &rc))
+        goto done;
+    }
+  }
+done:
+  free(U); free(C); free(r); free(t); free(al);
+  return rc;
+}
+
 static int streq(const char *a, const char *b) { return strcmp(a, b) == 0; }
 int or_parilut(int n, const int *arp, const int *aci, const double *av, int sweeps,
                double fill, double droptol, int cap, int *lrp, int *lci, double *lv,
@@ -1104,7 +1371,13 @@ int or_solve(const char *ksp, const char *pc, int n, const int *rp,
   } else {
     goto out;
   }
-  if (streq(ksp, "cg")) rc = or_cg(n, rp, ci, v, b, x, tol, abstol, maxiter, M, Mctx, resvec, usediter);
+  if (streq(ksp, "cg") || streq(ksp, "cgm")) rc = or_cg(n, rp, ci, v, b, x, tol, abstol, maxiter, M, Mctx, resvec, usediter);
+  else if (streq(ksp, "cg_left")) rc = or_cg_left(n, rp, ci, v, b, x, tol, abstol, maxiter, M, Mctx, resvec, usediter);
+  else if (streq(ksp, "icg")) rc = or_icg(n, rp, ci, v, b, x, tol, abstol, maxiter, M, Mctx, resvec, usediter);
+  else if (streq(ksp, "icgm")) rc = or_icgm(n, rp, ci, v, b, x, tol, abstol, maxiter, M, Mctx, resvec, usediter);
+  else if (streq(ksp, "ibicgstab")) rc = or_ibicgstab(n, rp, ci, v, b, x, tol, abstol, maxiter, M, Mctx, resvec, usediter);
+  else if (streq(ksp, "gcr_left")) rc = or_gcr_left(n, rp, ci, v, b, x, tol, abstol, maxiter, restart, 0, M, Mctx, resvec, usediter);
+  else if (streq(ksp, "igcr_left")) rc = or_gcr_left(n, rp, ci, v, b, x, tol, abstol, maxiter, restart, 1, M, Mctx, resvec, usediter);
   else if (streq(ksp, "bicgstab")) rc = or_bicgstab(n, rp, ci, v, b, x, tol, abstol, maxiter, M, Mctx, resvec, usediter);
   else if (streq(ksp, "gcr")) rc = or_gcr(n, rp, ci, v, b, x, tol, abstol, maxiter, restart, M, Mctx, resvec, usediter);
   else if (streq(ksp, "gmres")) rc = or_gmres(n, rp, ci, v, b, x, tol, maxiter, restart > 0 ? restart : 30, 0, M, Mctx, resvec, usediter);
