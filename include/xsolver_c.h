@@ -18,6 +18,8 @@
 #ifndef XSOLVER_C_H_
 #define XSOLVER_C_H_
 
+#include <stddef.h> /* NULL for callers of the Fortran-style surface */
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -46,7 +48,10 @@ enum xs_status {
  * selects ELL SpMV; advanced_pam[2] must be 1 (GPU). */
 void xsolver_mat_setup(int n, int *rowptr, int *colidx, double *val, int *advanced_pam);
 /* hip/xsolver_c.cpp:103-130: "Bicgstab|bicgstab|BICGSTAB", "Igcr|igcr|IGCR",
- * "Cg|cg|CG"; additionally "Gmres|gmres|GMRES" and "Fgmres|fgmres|FGMRES". */
+ * "Cg|cg|CG"; additionally "Gmres|gmres|GMRES", "Fgmres|fgmres|FGMRES" and the
+ * rest of the intel family (intel/mpisolver-C-v1.cpp): "cgm" (= "cg"),
+ * "cg_left" (CG on M^-1 A), "icg", "icgm", "ibicgstab", "gcr_left",
+ * "igcr_left". */
 void xsolver_ksp_settype(char *fmt);
 /* hip/xsolver_c.cpp:131-133 */
 void xsolver_ksp_set_absolute_tol(double absolute_tol);
@@ -305,6 +310,69 @@ int xs_batch_destroy(xs_batch *bt);
 int xs_mtx_read(const char *path, int flags, int *n, int *nnz, int *rowptr, int *colidx,
                 double *val, double *diag);
 
+/* ======================================================================== */
+/* Fortran-style MCSR surface of the reference's CUDA backend                */
+/* (cuda/HostMatrix.h:162-193, cuda/DeviceMatrix.h:121-169,                  */
+/* cuda/mpicudasolver-C-v1.cu:165-1530).  Every argument is passed by        */
+/* address, as from Fortran (the reference's C++ references are pointers at */
+/* the ABI).  One matrix at a time; not thread-safe; errors never exit:     */
+/* gpu_last_error() returns the status of the last call.                    */
+/* ======================================================================== */
+/* Registers (borrows) the host matrix in MCSR form, 0-based: diag_val[n],
+ * off-diagonals in CSR (offdiag_row_offset[n+1], offdiag_col_index /
+ * offdiag_val[nSizes]), columns in any order.  nHalo != 0 (MPI halo) ->
+ * XS_ERR_UNSUPPORTED. */
+void cpu_mat_setup(int *nInterior, int *nHalo, int *nSizes, double *diag_val, double *offdiag_val,
+                   int *offdiag_row_offset, int *offdiag_col_index);
+/* new value arrays for the registered pattern (cuda/HostMatrix.h:170-173) */
+void cpu_mat_update(double *diag_val, double *offdiag_val);
+/* copies the registered matrix to the GPU (cuda/DeviceMatrix.h:121-126) */
+void gpu_mat_setup(void);
+/* "ELL": ELL SpMV; *num_nz (1..32) is written back as 16 or 32, as the
+ * reference pads (:127-144); the ELL copy here has the true row width */
+void gpu_mat_format(char *fmt, int *num_nz);
+/* the values of the last cpu_mat_update to the GPU (:145-148) */
+void gpu_mat_update(void);
+void gpu_mat_free(void);
+/* "Jacobi" (:153-158), or any preconditioner name xs_pc_settype accepts;
+ * other strings: no preconditioner */
+void gpu_preconditioner_setup(char *fmt);
+/* refactorise on the current values (:159-165) */
+void gpu_preconditioner_update(char *fmt);
+void gpu_preconditioner_free(char *fmt);
+/* The intel-family solves (cuda/mpicudasolver-C-v1.cu; intel/mpisolver-C-v1.cpp):
+ * cg = CG on M^-1 A, icg = its one-reduction variant, cgm = preconditioned CG,
+ * icgm = PCG with one reduction per iteration, bicgstab / ibicgstab =
+ * right-preconditioned BiCGSTAB / improved BiCGSTAB, gcr / igcr =
+ * left-preconditioned GCR(restart) with classical / modified Gram-Schmidt,
+ * gmres / fgmres = GMRES(restart) left / flexible.  rhs, sol, sol_init and
+ * resvec are host arrays; sol_init NULL = start from sol.  On return:
+ * *usediter = 0-based index of the last iteration (maxiter - 1 at
+ * exhaustion; maxiter for gcr / igcr, as those loops leave it), resvec[k] =
+ * ||r_{k+1}|| / (||r_0|| + 1e-20) for the residual the method monitors. */
+void gpu_cg_solve(double *rhs, double *sol, double *tol, int *maxiter, double *sol_init,
+                  int *usediter, double *resvec);
+void gpu_icg_solve(double *rhs, double *sol, double *tol, int *maxiter, double *sol_init,
+                   int *usediter, double *resvec);
+void gpu_cgm_solve(double *rhs, double *sol, double *tol, int *maxiter, double *sol_init,
+                   int *usediter, double *resvec);
+void gpu_icgm_solve(double *rhs, double *sol, double *tol, int *maxiter, double *sol_init,
+                    int *usediter, double *resvec);
+void gpu_bicgstab_solve(double *rhs, double *sol, double *tol, int *maxiter, double *sol_init,
+                        int *usediter, double *resvec);
+void gpu_ibicgstab_solve(double *rhs, double *sol, double *tol, int *maxiter, double *sol_init,
+                         int *usediter, double *resvec);
+void gpu_gcr_solve(double *rhs, double *sol, int *restart, double *tol, int *maxiter,
+                   double *sol_init, int *usediter, double *resvec);
+void gpu_igcr_solve(double *rhs, double *sol, int *restart, double *tol, int *maxiter,
+                    double *sol_init, int *usediter, double *resvec);
+void gpu_gmres_solve(double *rhs, double *sol, int *restart, double *tol, int *maxiter,
+                     double *sol_init, int *usediter, double *resvec);
+void gpu_fgmres_solve(double *rhs, double *sol, int *restart, double *tol, int *maxiter,
+                      double *sol_init, int *usediter, double *resvec);
+/* status of the last call of this surface (no reference counterpart) */
+int gpu_last_error(void);
+
 #ifdef __cplusplus
 } /* extern "C" */
 
@@ -313,6 +381,36 @@ inline void xsolver_ksp_setoption(double tol, int maxiter, int &usediter,
                                   double *resvec, int restart = 0) {
   xsolver_ksp_setoption(tol, maxiter, &usediter, resvec, restart);
 }
+/* and for the MCSR surface (cuda/HostMatrix.h:162, cuda/DeviceMatrix.h:127,
+ * cuda/mpicudasolver-C-v1.cu): references in place of pointers */
+inline void cpu_mat_setup(int &nInterior, int &nHalo, int &nSizes, double *diag_val,
+                          double *offdiag_val, int *offdiag_row_offset, int *offdiag_col_index) {
+  cpu_mat_setup(&nInterior, &nHalo, &nSizes, diag_val, offdiag_val, offdiag_row_offset,
+                offdiag_col_index);
+}
+inline void gpu_mat_format(char *fmt, int &num_nz) { gpu_mat_format(fmt, &num_nz); }
+#define XS_MCSR_SOLVE_REF(name)                                                            \
+  inline void name(double *rhs, double *sol, double &tol, int &maxiter, double *sol_init,  \
+                   int &usediter, double *resvec = nullptr) {                              \
+    name(rhs, sol, &tol, &maxiter, sol_init, &usediter, resvec);                           \
+  }
+#define XS_MCSR_SOLVE_REF_R(name)                                                          \
+  inline void name(double *rhs, double *sol, int &restart, double &tol, int &maxiter,      \
+                   double *sol_init, int &usediter, double *resvec = nullptr) {            \
+    name(rhs, sol, &restart, &tol, &maxiter, sol_init, &usediter, resvec);                 \
+  }
+XS_MCSR_SOLVE_REF(gpu_cg_solve)
+XS_MCSR_SOLVE_REF(gpu_icg_solve)
+XS_MCSR_SOLVE_REF(gpu_cgm_solve)
+XS_MCSR_SOLVE_REF(gpu_icgm_solve)
+XS_MCSR_SOLVE_REF(gpu_bicgstab_solve)
+XS_MCSR_SOLVE_REF(gpu_ibicgstab_solve)
+XS_MCSR_SOLVE_REF_R(gpu_gcr_solve)
+XS_MCSR_SOLVE_REF_R(gpu_igcr_solve)
+XS_MCSR_SOLVE_REF_R(gpu_gmres_solve)
+XS_MCSR_SOLVE_REF_R(gpu_fgmres_solve)
+#undef XS_MCSR_SOLVE_REF
+#undef XS_MCSR_SOLVE_REF_R
 #endif
 
 #endif /* XSOLVER_C_H_ */

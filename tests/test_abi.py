@@ -76,6 +76,20 @@ def test_cpp_header_compiles_with_reference_signature(tmp_path):
     exe = tmp_path / "caller"
     subprocess.run(["g++", "-std=c++14", "-I", os.path.join(ROOT, "include"), str(src),
                     xsolver_amd.LIB_PATH, "-o", str(exe)], check=True)
+    # the CUDA backend's Fortran-style surface, called as the reference's
+    # C++ declarations allow (references, default resvec)
+    src2 = tmp_path / "caller_mcsr.cpp"
+    src2.write_text(
+        '#include "xsolver_c.h"\n'
+        'int main(){ int n=2,h=0,ns=2,ptr[3]={0,1,2},col[2]={1,0},nz=7,used=0,maxit=10,rs=5;\n'
+        ' double d[2]={4.0,4.0},off[2]={-1.0,-1.0},b[2]={1.0,1.0},x[2]={0,0},tol=1e-10;\n'
+        ' char jac[]="Jacobi", ell[]="ELL";\n'
+        ' cpu_mat_setup(n,h,ns,d,off,ptr,col); gpu_mat_setup(); gpu_mat_format(ell,nz);\n'
+        ' gpu_preconditioner_setup(jac); gpu_cg_solve(b,x,tol,maxit,NULL,used);\n'
+        ' gpu_gcr_solve(b,x,rs,tol,maxit,x,used); gpu_mat_free();\n'
+        ' return gpu_last_error()==0 ? 0 : 1; }\n')
+    subprocess.run(["g++", "-std=c++14", "-I", os.path.join(ROOT, "include"), str(src2),
+                    xsolver_amd.LIB_PATH, "-o", str(tmp_path / "caller_mcsr")], check=True)
     src_c = tmp_path / "caller.c"
     src_c.write_text('#include "xsolver_c.h"\nint main(void){ xs_ctx *c = 0; int rc = xs_create(&c);\n'
                      ' if (rc == 0) xs_destroy(c); return 0; }\n')

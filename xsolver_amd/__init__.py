@@ -48,6 +48,11 @@ EXPORTS = [
     "xs_grad_values_device", "xs_solve_multi_device", "xs_solve_multi", "xs_get_multi_results",
     "xs_get_multi_resvec", "xs_spmm", "xs_pc_apply_multi", "xs_autotune", "xs_mtx_read",
     "xs_batch_solve_noalloc",
+    "cpu_mat_setup", "cpu_mat_update", "gpu_mat_setup", "gpu_mat_format", "gpu_mat_update",
+    "gpu_mat_free", "gpu_preconditioner_setup", "gpu_preconditioner_update",
+    "gpu_preconditioner_free", "gpu_cg_solve", "gpu_icg_solve", "gpu_cgm_solve",
+    "gpu_icgm_solve", "gpu_bicgstab_solve", "gpu_ibicgstab_solve", "gpu_gcr_solve",
+    "gpu_igcr_solve", "gpu_gmres_solve", "gpu_fgmres_solve", "gpu_last_error",
 ]
 
 XS_RHS_COLMAJOR = 0
@@ -133,7 +138,23 @@ def lib():
         "xsolver_last_error": (_i, []),
         "xsolver_get_iterations": (_i, []),
         "xsolver_get_residual": (_d, []),
+        # Fortran-style MCSR surface: every argument by address
+        "cpu_mat_setup": (None, [_vp] * 7),
+        "cpu_mat_update": (None, [_vp, _vp]),
+        "gpu_mat_setup": (None, []),
+        "gpu_mat_format": (None, [_cp, _vp]),
+        "gpu_mat_update": (None, []),
+        "gpu_mat_free": (None, []),
+        "gpu_preconditioner_setup": (None, [_cp]),
+        "gpu_preconditioner_update": (None, [_cp]),
+        "gpu_preconditioner_free": (None, [_cp]),
+        "gpu_last_error": (_i, []),
     }
+    for name in ("gpu_cg_solve", "gpu_icg_solve", "gpu_cgm_solve", "gpu_icgm_solve",
+                 "gpu_bicgstab_solve", "gpu_ibicgstab_solve"):
+        sig[name] = (None, [_vp] * 7)
+    for name in ("gpu_gcr_solve", "gpu_igcr_solve", "gpu_gmres_solve", "gpu_fgmres_solve"):
+        sig[name] = (None, [_vp] * 8)
     for name, (res, args) in sig.items():
         f = getattr(L, name)
         f.restype = res

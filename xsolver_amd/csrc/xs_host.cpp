@@ -583,7 +583,14 @@ extern "C" int xs_ksp_settype(xs_ctx *c, const char *t) {
   if (!c || !t) return XS_ERR_ARG;
   if (streq_any(t, {"Bicgstab", "bicgstab", "BICGSTAB"})) c->ksp = XS_KSP_BICGSTAB;
   else if (streq_any(t, {"Igcr", "igcr", "IGCR", "Gcr", "gcr", "GCR"})) c->ksp = XS_KSP_GCR;
-  else if (streq_any(t, {"Cg", "cg", "CG"})) c->ksp = XS_KSP_CG;
+  else if (streq_any(t, {"Cg", "cg", "CG", "Cgm", "cgm", "CGM"})) c->ksp = XS_KSP_CG;
+  // the rest of the intel family (intel/mpisolver-C-v1.cpp)
+  else if (streq_any(t, {"cg_left", "CG_LEFT"})) c->ksp = XS_KSP_CG_LEFT;
+  else if (streq_any(t, {"Icg", "icg", "ICG"})) c->ksp = XS_KSP_ICG;
+  else if (streq_any(t, {"Icgm", "icgm", "ICGM"})) c->ksp = XS_KSP_ICGM;
+  else if (streq_any(t, {"Ibicgstab", "ibicgstab", "IBICGSTAB"})) c->ksp = XS_KSP_IBICGSTAB;
+  else if (streq_any(t, {"gcr_left", "GCR_LEFT"})) c->ksp = XS_KSP_GCR_LEFT;
+  else if (streq_any(t, {"igcr_left", "IGCR_LEFT"})) c->ksp = XS_KSP_IGCR_LEFT;
   else if (streq_any(t, {"Gmres", "gmres", "GMRES"})) c->ksp = XS_KSP_GMRES;
   else if (streq_any(t, {"Fgmres", "fgmres", "FGMRES"})) c->ksp = XS_KSP_FGMRES;
   else return XS_ERR_ARG;
@@ -1471,6 +1478,12 @@ static int run_solve(xs_ctx *c, const double *d_b, double *d_x) {
     case XS_KSP_CG: rc = xs_solve_cg(c, d_b, d_x); break;
     case XS_KSP_BICGSTAB: rc = xs_solve_bicgstab(c, d_b, d_x); break;
     case XS_KSP_GCR: rc = xs_solve_gcr(c, d_b, d_x); break;
+    case XS_KSP_CG_LEFT: rc = xs_solve_cg_left(c, d_b, d_x, false); break;
+    case XS_KSP_ICG: rc = xs_solve_cg_left(c, d_b, d_x, true); break;
+    case XS_KSP_ICGM: rc = xs_solve_icgm(c, d_b, d_x); break;
+    case XS_KSP_IBICGSTAB: rc = xs_solve_ibicgstab(c, d_b, d_x); break;
+    case XS_KSP_GCR_LEFT: rc = xs_solve_gcr(c, d_b, d_x, true, false); break;
+    case XS_KSP_IGCR_LEFT: rc = xs_solve_gcr(c, d_b, d_x, true, true); break;
     case XS_KSP_GMRES: rc = xs_solve_gmres(c, d_b, d_x, false); break;
     case XS_KSP_FGMRES: rc = xs_solve_gmres(c, d_b, d_x, true); break;
     default: return XS_ERR_STATE;

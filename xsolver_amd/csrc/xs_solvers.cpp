@@ -9,6 +9,7 @@
 #include <cstdio>
 
 #include "../../include/xsolver_c.h"
+#include "xs_intel.h"
 #include "xs_krylov.h"
 #include "xs_mixed.h"
 #include "xs_solvers.h"
@@ -246,8 +247,11 @@ int xs_solve_gmres(xs_ctx *c, const double *b, double *x, bool flexible) {
 // Right-preconditioned GCR(restart), hip/Solver/xsolver_igcr.cpp:28-108 (the
 // reference's default restart is maxiter, xsolver_igcr.h:36); c = A u is
 // orthogonalised against the stored c's with CGS2, the same coefficients
-// applied to u.
-int xs_solve_gcr(xs_ctx *c, const double *b, double *x) {
+// applied to u.  left = true: the intel left-preconditioned GCR
+// (intel/mpisolver-C-v1.cpp:685-810 with classical Gram-Schmidt, one pass;
+// mgs: :812-931, modified Gram-Schmidt, one coefficient at a time) on the
+// residual r = M^-1 (b - A x): u_k = r, c_k = M^-1 A u_k.
+int xs_solve_gcr(xs_ctx *c, const double *b, double *x, bool left, bool mgs) {
   int rc = xs_ctx_ensure_work(c, XS_NWORK);
   if (rc) return rc;
   const int n = c->A.n;
@@ -266,7 +270,13 @@ int xs_solve_gcr(xs_ctx *c, const double *b, double *x) {
   double *rv = c->resvec_d;
 
   xs_init_state(s, st, c->tol, c->abstol, c->maxiter);
-  xs_op_spmv(c, x, r, EPI_BMAX, b, nullptr, nullptr);
+  if (left) {
+    double *t = xs_ctx_vec(c, 2);
+    xs_op_spmv(c, x, t, EPI_BMAX, b, nullptr, nullptr);
+    xs_op_pc(c, t, r, tmp, nullptr);                               // r = M^-1 (b - A x)
+  } else {
+    xs_op_spmv(c, x, r, EPI_BMAX, b, nullptr, nullptr);
+  }
   xs_ts_dot<double>(s, n, 1, r, ld, r, P, tc, misc + XS_G_NRM2, nullptr);
   xs_res0_from(s, misc + XS_G_NRM2, st, rv);
   int k = 0, launched = 0;
@@ -276,9 +286,25 @@ int xs_solve_gcr(xs_ctx *c, const double *b, double *x) {
   double *t01 = misc + XS_G_T0;
   while (!done && launched < c->maxiter) {
     double *uk = U + (size_t)k * ld, *ck = C + (size_t)k * ld;
-    xs_op_pc(c, r, uk, tmp, st);                                   // u_k = M^-1 r
-    xs_op_spmv(c, uk, ck, EPI_Y, nullptr, nullptr, nullptr);       // c_k = A u_k
-    if (k > 0) {
+    if (left) {
+      double *t = xs_ctx_vec(c, 2);
+      xs_copy(s, n, r, uk);                                        // u_k = r
+      xs_op_spmv(c, uk, t, EPI_Y, nullptr, nullptr, nullptr);
+      xs_op_pc(c, t, ck, tmp, st);                                 // c_k = M^-1 A u_k
+    } else {
+      xs_op_pc(c, r, uk, tmp, st);                                 // u_k = M^-1 r
+      xs_op_spmv(c, uk, ck, EPI_Y, nullptr, nullptr, nullptr);     // c_k = A u_k
+    }
+    if (k > 0 && left && mgs) {                                    // one coefficient at a time
+      for (int i = 0; i < k; i++) {
+        xs_ts_dot<double>(s, n, 1, C + (size_t)i * ld, ld, ck, P, tc, h1 + i, st);
+        xs_ts_update<double>(s, n, 1, C + (size_t)i * ld, U + (size_t)i * ld, ld, h1 + i, ck,
+                             ck, uk, st);
+      }
+    } else if (k > 0 && left) {                                    // all from the unmodified c_k
+      xs_ts_dot<double>(s, n, k, C, ld, ck, P, tc, h1, st);
+      xs_ts_update<double>(s, n, k, C, U, ld, h1, ck, ck, uk, st);
+    } else if (k > 0) {
       xs_ts_dot<double>(s, n, k, C, ld, ck, P, tc, h1, st);
       xs_ts_update<double>(s, n, k, C, U, ld, h1, ck, ck, uk, st);
       xs_ts_dot<double>(s, n, k, C, ld, ck, P, tc, h2, st);
@@ -304,8 +330,160 @@ int xs_solve_gcr(xs_ctx *c, const double *b, double *x) {
   if (h.done == 1) return XS_OK;
   return XS_NOT_CONVERGED;
 }
+// ---------------------------------------------------------------------------
+// The rest of the intel Krylov family (intel/mpisolver-C-v1.cpp), each as a
+// fixed launch sequence per iteration over the generic kernels of
+// xs_intel.hip (multi-dot + scalar step, linear combinations) and the SpMV /
+// preconditioner operators; small = 1e-20 in the denominators as there.
+// ---------------------------------------------------------------------------
+static int ensure_mdot(xs_ctx *c) {
+  return ensure_dev(&c->mdot_partial, &c->mdot_len,
+                    (size_t)xs_mdot_blocks(c->A.n) * XS_MDOT_STRIDE);
+}
+
+// CG on M^-1 A (cpu_cg_solve, :26-102) and, improved = true, ICG
+// (cpu_icg_solve, :104-192: p.q, r.q and q.q in one reduction, the next
+// beta from the recurrence tao = taoo - 2 alpha rho + alpha^2 theta).  The
+// residual is the preconditioned one, r = M^-1 (b - A x).
+int xs_solve_cg_left(xs_ctx *c, const double *b, double *x, bool improved) {
+  int rc = xs_ctx_ensure_work(c, XS_NWORK);
+  if (rc || (rc = ensure_mdot(c))) return rc;
+  const int n = c->A.n;
+  hipStream_t s = c->stream;
+  double *r = xs_ctx_vec(c, 0), *p = xs_ctx_vec(c, 1), *Ap = xs_ctx_vec(c, 2);
+  double *q = xs_ctx_vec(c, 3), *t = xs_ctx_vec(c, 4);
+  double *P = c->mdot_partial, *rv = c->resvec_d;
+  xs_state *st = c->st;
+  xs_init_state(s, st, c->tol, c->abstol, c->maxiter);
+  xs_op_spmv(c, x, Ap, EPI_BMAX, b, nullptr, nullptr);
+  xs_op_pc(c, Ap, r, t, nullptr);                                  // r = M^-1 (b - A x)
+  {
+    const double *xs_[1] = {r}, *ys_[1] = {r};
+    xs_mdot_step(s, n, 1, xs_, ys_, P, IST_CGL_INIT, st, rv);
+  }
+  xs_copy(s, n, r, p);
+  const double *x_pq[3] = {p, r, q}, *y_pq[3] = {q, q, q};
+  const double *x_rr[1] = {r}, *y_rr[1] = {r};
+  const double *lx[2] = {x, p}, *lr[2] = {r, q}, *lp[2] = {p, r};
+  const int sx[2] = {IS_ONE, IS_ALPHA}, sr[2] = {IS_ONE, IS_NALPHA}, sp[2] = {IS_BETA, IS_ONE};
+  return drive(c, [&]() {
+    xs_op_spmv(c, p, Ap, EPI_Y, nullptr, nullptr, nullptr);        // Ap
+    xs_op_pc(c, Ap, q, t, st);                                     // q = M^-1 A p
+    xs_mdot_step(s, n, improved ? 3 : 1, x_pq, y_pq, P,
+                 improved ? IST_ICG_ALPHA : IST_CGL_ALPHA, st, rv);
+    xs_lincomb(s, n, 2, lx, sx, x, st);                            // x += alpha p
+    xs_lincomb(s, n, 2, lr, sr, r, st);                            // r -= alpha q
+    xs_mdot_step(s, n, 1, x_rr, y_rr, P, improved ? IST_ICG_RES : IST_CGL_RES, st, rv);
+    xs_lincomb(s, n, 2, lp, sp, p, st);                            // p = beta p + r
+  });
+}
+
+// ICGM (cpu_icgm_solve, :273-352): preconditioned CG with the five inner
+// products of an iteration in one reduction; convergence is judged on ||r||
+// from before the iteration's update (the x update of that iteration is
+// still applied, as in the reference).
+int xs_solve_icgm(xs_ctx *c, const double *b, double *x) {
+  int rc = xs_ctx_ensure_work(c, XS_NWORK);
+  if (rc || (rc = ensure_mdot(c))) return rc;
+  const int n = c->A.n;
+  hipStream_t s = c->stream;
+  double *r = xs_ctx_vec(c, 0), *z = xs_ctx_vec(c, 1), *p = xs_ctx_vec(c, 2);
+  double *Ap = xs_ctx_vec(c, 3), *q = xs_ctx_vec(c, 4), *t = xs_ctx_vec(c, 5);
+  double *P = c->mdot_partial, *rv = c->resvec_d;
+  xs_state *st = c->st;
+  xs_init_state(s, st, c->tol, c->abstol, c->maxiter);
+  xs_op_spmv(c, x, r, EPI_BMAX, b, nullptr, nullptr);              // r = b - A x
+  xs_op_pc(c, r, z, t, nullptr);                                   // z = M^-1 r
+  {
+    const double *xs_[2] = {r, r}, *ys_[2] = {r, z};
+    xs_mdot_step(s, n, 2, xs_, ys_, P, IST_ICGM_INIT, st, rv);
+  }
+  xs_copy(s, n, z, p);
+  const double *x5[5] = {p, r, Ap, q, r}, *y5[5] = {Ap, q, z, Ap, r};
+  const double *lx[2] = {x, p}, *lr[2] = {r, Ap}, *lp[2] = {p, z};
+  const int sx[2] = {IS_ONE, IS_ALPHA}, sr[2] = {IS_ONE, IS_NALPHA}, sp[2] = {IS_BETA, IS_ONE};
+  return drive(c, [&]() {
+    xs_op_spmv(c, p, Ap, EPI_Y, nullptr, nullptr, nullptr);
+    xs_op_pc(c, Ap, q, t, st);                                     // q = M^-1 A p
+    xs_mdot_step(s, n, 5, x5, y5, P, IST_ICGM_MAIN, st, rv);       // the one reduction
+    xs_lincomb(s, n, 2, lx, sx, x, st);                            // x += alpha p
+    xs_lincomb(s, n, 2, lr, sr, r, st);                            // r -= alpha A p
+    xs_op_pc(c, r, z, t, st);                                      // z = M^-1 r
+    xs_lincomb(s, n, 2, lp, sp, p, st);                            // p = beta p + z
+    xs_istep(s, IST_COMMIT, st, rv);                               // ||r|| of before the update
+  });
+}
+
+// Improved BiCGSTAB (cpu_ibicgstab_solve, :484-684), right-preconditioned in
+// y-space: the correction x - x0 = M^-1 y is built from y = 0 (the
+// reference starts from y0 = its initial guess; identical for x0 = 0).  Per
+// iteration: two SpMVs, two preconditioner applications, six inner products
+// in one reduction, and ||r||.
+int xs_solve_ibicgstab(xs_ctx *c, const double *b, double *x) {
+  int rc = xs_ctx_ensure_work(c, XS_NWORK);
+  if (rc || (rc = ensure_mdot(c))) return rc;
+  const int n = c->A.n;
+  hipStream_t s = c->stream;
+  double *res = xs_ctx_vec(c, 0), *reso = xs_ctx_vec(c, 1), *tk = xs_ctx_vec(c, 2);
+  double *pk = xs_ctx_vec(c, 3), *qk = xs_ctx_vec(c, 4), *zk = xs_ctx_vec(c, 5);
+  double *vk = xs_ctx_vec(c, 6), *vko = xs_ctx_vec(c, 7), *sk = xs_ctx_vec(c, 8);
+  double *tm = xs_ctx_vec(c, 9), *phi = xs_ctx_vec(c, 10), *t = xs_ctx_vec(c, 11);
+  double *P = c->mdot_partial, *rv = c->resvec_d;
+  xs_state *st = c->st;
+  xs_init_state(s, st, c->tol, c->abstol, c->maxiter);
+  for (double *v : {pk, qk, zk, vk, phi}) XS_CHK(hipMemsetAsync(v, 0, sizeof(double) * n, s));
+  xs_op_spmv(c, x, res, EPI_BMAX, b, nullptr, nullptr);            // res = b - A x0
+  xs_copy(s, n, res, reso);
+  xs_op_pc(c, res, tm, t, nullptr);
+  xs_op_spmv(c, tm, tk, EPI_Y, nullptr, nullptr, nullptr);         // tk = A M^-1 res
+  {
+    const double *xs_[2] = {res, res}, *ys_[2] = {res, tk};
+    xs_mdot_step(s, n, 2, xs_, ys_, P, IST_IBI_INIT, st, rv);
+  }
+  const double *l_vko[1] = {vk};
+  const int s_vko[1] = {IS_ONE};
+  const double *l_vk[4] = {tk, pk, vko, qk};
+  const int s_vk[4] = {IS_ONE, IS_NOMEGA, IS_BETA, IS_NDELTA};
+  const double *l_sk[2] = {res, vk};
+  const int s_sk[2] = {IS_ONE, IS_NALPHA};
+  const double *l_tk[3] = {tk, pk, qk};
+  const int s_tk[3] = {IS_ONE, IS_NOMEGA, IS_NALPHA};
+  const double *l_zk[3] = {zk, res, vko};
+  const int s_zk[3] = {IS_T1, IS_ALPHA, IS_NT2};
+  const double *x6[6] = {reso, reso, sk, tk, reso, reso}, *y6[6] = {sk, qk, tk, tk, tk, pk};
+  const double *l_res[2] = {sk, tk};
+  const int s_res[2] = {IS_ONE, IS_NOMEGA};
+  const double *l_phi[3] = {phi, zk, sk};
+  const int s_phi[3] = {IS_ONE, IS_ONE, IS_OMEGA};
+  const double *x_rr[1] = {res}, *y_rr[1] = {res};
+  rc = drive(c, [&]() {
+    xs_istep(s, IST_IBI_S1, st, rv);
+    xs_lincomb(s, n, 1, l_vko, s_vko, vko, st);                    // vko = vk
+    xs_lincomb(s, n, 4, l_vk, s_vk, vk, st);                       // vk = t - w p + b vko - d q
+    xs_op_pc(c, vk, tm, t, st);
+    xs_op_spmv(c, tm, qk, EPI_Y, nullptr, nullptr, nullptr);       // qk = A M^-1 vk
+    xs_lincomb(s, n, 2, l_sk, s_sk, sk, st);                       // sk = res - alpha vk
+    xs_lincomb(s, n, 3, l_tk, s_tk, tk, st);                       // tk -= w pk + alpha qk
+    xs_lincomb(s, n, 3, l_zk, s_zk, zk, st);                       // zk = t1 zk + a res - t2 vko
+    xs_op_pc(c, tk, tm, t, st);
+    xs_op_spmv(c, tm, pk, EPI_Y, nullptr, nullptr, nullptr);       // pk = A M^-1 tk
+    xs_mdot_step(s, n, 6, x6, y6, P, IST_IBI_S2, st, rv);          // the six inner products
+    xs_lincomb(s, n, 2, l_res, s_res, res, st);                    // res = sk - w tk
+    xs_lincomb(s, n, 3, l_phi, s_phi, phi, st);                    // y += zk + w sk
+    xs_mdot_step(s, n, 1, x_rr, y_rr, P, IST_IBI_RES, st, rv);
+  });
+  // x = x0 + M^-1 y (the reference's final preconditioner application, :656)
+  xs_op_pc(c, phi, tm, t, nullptr);
+  xs_add(s, n, tm, x);
+  XS_CHK(hipGetLastError());
+  return rc;
+}
+
 void xs_solvers_free(xs_ctx *c) {
   xs_multi_free(c);
+  if (c->mdot_partial) (void)hipFree(c->mdot_partial);
+  c->mdot_partial = nullptr;
+  c->mdot_len = 0;
   if (c->basis) (void)hipFree(c->basis);
   c->basis = nullptr;
   c->basis_len = 0;

@@ -6,7 +6,10 @@
 
 #include "xs_internal.h"
 
-enum { XS_KSP_NONE = 0, XS_KSP_CG, XS_KSP_BICGSTAB, XS_KSP_GCR, XS_KSP_GMRES, XS_KSP_FGMRES };
+enum { XS_KSP_NONE = 0, XS_KSP_CG, XS_KSP_BICGSTAB, XS_KSP_GCR, XS_KSP_GMRES, XS_KSP_FGMRES,
+       // the rest of the intel family (intel/mpisolver-C-v1.cpp)
+       XS_KSP_CG_LEFT, XS_KSP_ICG, XS_KSP_ICGM, XS_KSP_IBICGSTAB, XS_KSP_GCR_LEFT,
+       XS_KSP_IGCR_LEFT };
 enum { XS_PC_NONE = 0, XS_PC_JACOBI, XS_PC_ILU0, XS_PC_ILUT, XS_PC_IC0 };
 // preconditioners held as combined ILU factors (xs_ilu) and applied by the
 // ILU solves: ParILU(0), ParILUT and ParIC(0) (L~ | D L~^T)
@@ -88,6 +91,8 @@ struct xs_ctx {
   double *small_d = nullptr;
   size_t small_len = 0;
   double *ts_partial = nullptr;
+  double *mdot_partial = nullptr;   // intel-family multi-dot partials
+  size_t mdot_len = 0;
   size_t ts_partial_len = 0;
 
   // kernel timing (HIP events on the context stream around launches of one
@@ -193,7 +198,10 @@ int xs_op_pc_t(xs_ctx *c, const double *x, double *y, double *tmp, hipStream_t s
 
 int xs_solve_cg(xs_ctx *c, const double *b, double *x);
 int xs_solve_bicgstab(xs_ctx *c, const double *b, double *x);
-int xs_solve_gcr(xs_ctx *c, const double *b, double *x);
+int xs_solve_gcr(xs_ctx *c, const double *b, double *x, bool left = false, bool mgs = false);
+int xs_solve_cg_left(xs_ctx *c, const double *b, double *x, bool improved);
+int xs_solve_icgm(xs_ctx *c, const double *b, double *x);
+int xs_solve_ibicgstab(xs_ctx *c, const double *b, double *x);
 int xs_solve_gmres(xs_ctx *c, const double *b, double *x, bool flexible);
 int xs_parilut_factor(xs_ctx *c);
 int xs_solve_mixed(xs_ctx *c, const double *b, double *x);
