@@ -147,6 +147,11 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *                    ILUT, precon_gpuilut.cpp:19-20)
  *   "trsv_schedule"  1 (default) = hand out the triangular-solve row groups in
  *                    dependency-level order, 0 = row order
+ *   "trsv_progressive" -1 (default) = by factor width: factors with more than
+ *                    six entries per row and triangle solve every 64-row group
+ *                    lane by lane, publishing each row as soon as it is final;
+ *                    narrower factors solve a group whose rows depend at most
+ *                    on the row before with one affine scan; 0 / 1 force either
  *   "trsv_workgroups" persistent workgroups of a triangular solve (default 256):
  *                    bounds the number of polling waves
  *   "spmv_kernel"    SpMV kernel for CSR input: -1 (default) = automatic: an

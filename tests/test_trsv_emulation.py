@@ -4,7 +4,8 @@ along the solve order (upper solves: lane l holds row r1-1-l), dependencies
 outside the group read from the solution vector, dependencies inside the
 group resolved by the DPP affine scan (row shifts 1/2/4/8, then the
 row_bcast:15 / row_bcast:31 steps) when every row depends at most on the
-previous one, by the 64-step broadcast loop otherwise.  Waves are emulated in ticket order, which
+previous one, otherwise lane by lane (each row final and published at its
+turn, its in-wave products summed apart from the external ones).  Waves are emulated in ticket order, which
 is the order the kernel's waits resolve in.  Checked against the oracle's
 sequential solves (oracle/xs_oracle.c)."""
 import numpy as np
@@ -85,19 +86,24 @@ def emulate(rp, ci, lu, dpos, dinv, b, upper):
                 c, m = c + m * cj, m * mj
             val = c
         else:
+            # one lane at a time: lane k's row is (external part + in-wave
+            # part) * d once the rows of lanes < k are broadcast, published
+            # at once; the in-wave products are summed apart, in solve order
             paths.add("loop")
             e = np.where(upper, ee - 1, eb)
+            ain = np.zeros(W)
+            val = np.zeros(W)
             for k in range(nrows):
-                xk = acc[k] * di[k]
+                val[k] = (acc[k] + ain[k]) * di[k]
+                x[row[k]] = val[k]
                 rk = row[k]
                 for l in range(nrows):
                     if nint[l] == 0:
                         continue
                     inside = (e[l] >= eb[l]) if upper else (e[l] < ee[l])
                     if inside and ci[e[l]] == rk:
-                        acc[l] -= lu[e[l]] * xk
+                        ain[l] -= lu[e[l]] * val[k]
                         e[l] += -1 if upper else 1
-            val = acc * di
         for l in range(nrows):
             x[row[l]] = val[l]
     return x, paths

@@ -77,6 +77,7 @@ static void free_precond(xs_ctx *c) {
   c->tell_ready = c->tell32_ready = false;
   F = xs_ilu();
   F.trsv_wg = c->trsv_wg;
+  F.trsv_prog = c->trsv_prog;
   dfree(c->lu_tmp);
   c->pc_ready = false;
 }
@@ -657,6 +658,11 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     c->trsv_wg = (int)value;
     c->ilu.trsv_wg = c->trsv_wg;
   }
+  else if (k == "trsv_progressive") {   // -1 auto, 0 scan chain groups, 1 lane by lane
+    if (value < -1 || value > 1) return XS_ERR_ARG;
+    c->trsv_prog = (int)value;
+    c->ilu.trsv_prog = c->trsv_prog;
+  }
   else if (k == "trsv_schedule") { c->trsv_schedule = value != 0; c->pc_ready = false; free_precond(c); }
   else if (k == "lsolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->lsweeps = (int)value; }
   else if (k == "usolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->usweeps = (int)value; }
@@ -799,9 +805,9 @@ static int build_trsv_schedules(xs_ctx *c, const int *rp, const int *ci, const i
 // The groups of a pattern built on the device (ParILUT), on the first solve
 // that applies a factor by the exact triangular solve (sweep counts < 0);
 // until then the solves would fall back to fixed 64-row groups in row order.
-static int ensure_trsv_sched(xs_ctx *c) {
+static int ensure_trsv_sched(xs_ctx *c, bool exact = false) {
   xs_ilu &F = c->ilu;
-  if (!F.sched_pending || (c->lsweeps >= 0 && c->usweeps >= 0)) return XS_OK;
+  if (!F.sched_pending || (!exact && c->lsweeps >= 0 && c->usweeps >= 0)) return XS_OK;
   const int n = F.n, nnz = F.nnz;
   std::vector<int> hrp(n + 1), hci(nnz), hdp(n);
   XS_HIP(hipStreamSynchronize(c->stream));
@@ -1846,6 +1852,7 @@ extern "C" int xs_trsv(xs_ctx *c, int which, const double *b, double *x, void *s
   int rc;
   if (!xs_pc_factored(c->pc)) return XS_ERR_STATE;
   if (!c->pc_ready && (rc = xs_pc_setup(c))) return rc;
+  if ((rc = ensure_trsv_sched(c, true))) return rc;   // an exact solve whatever the apply
   xs_fill_sentinel(pick(c, stream), x, c->A.n);
   if (which == 0) xs_trsv_lower(pick(c, stream), c->ilu, b, x, c->counters + 2, nullptr);
   else xs_trsv_upper(pick(c, stream), c->ilu, b, x, c->counters + 4, nullptr);

@@ -122,6 +122,10 @@ struct xs_tsched {
   int ngroups = 0;
   int max_wg = 256;
   int gather = 4;   // off-group entries a lane loads and polls together (4 or 16)
+  // 1: every group is solved lane by lane, each row published as soon as it
+  // is final; 0: a group whose rows depend at most on the row before waits
+  // for all its external values and is solved by one affine scan
+  int prog = 0;
 };
 
 // Joint polling width of a factor's triangular solves: 16 off-group entries
@@ -175,6 +179,7 @@ struct xs_ilu {
   // their order are built from it on the first exact triangular solve
   bool sched_pending = false;
   int trsv_wg = 256;        // persistent workgroups of the triangular solves
+  int trsv_prog = -1;       // xs_tsched::prog; -1 = by factor width (wide -> 1)
   xs_tell tl, tu;           // sweep copies of L and U (built on demand)
   bool tell_shape = false;  // tl/tu widths and buffers match the pattern
   // transposed factors for the adjoint (built on demand)
@@ -190,6 +195,7 @@ inline xs_tsched xs_ilu_tsched(const xs_ilu &F, bool upper) {
   t.order = upper ? F.usched : F.lsched;
   t.max_wg = F.trsv_wg;
   t.gather = xs_trsv_gather_width(F.nnz, F.n);
+  t.prog = F.trsv_prog >= 0 ? F.trsv_prog : (t.gather == 16 ? 1 : 0);
   return t;
 }
 

@@ -1000,19 +1000,15 @@ void xs_ilu_dinv(hipStream_t s, xs_ilu &F, int *d_bad) {
 // (hip/Precond/gpuparilu.cpp:257-391): there one wave per row spins on a
 // separate volatile int flag; the backward solve scatters with double atomics.
 // ---------------------------------------------------------------------------
-// acc - sum of lu[e] * x[ci[e]] over up to G entries e, e+dir, ... (stopping
-// at `stop`), subtracted in that order: the G loads are issued together and
-// the values still holding the sentinel are re-polled together.
+// Up to G entries e, e+dir, ... (stopping at `stop`) of a row: their columns
+// c (-1 past the stop), values a and the current x of those columns u (which
+// may still hold the sentinel).  Branch-free loads (entry e is in range: a
+// slot past the stop re-reads it and is masked by c = -1), so that they are
+// all in flight at once.
 template <int G>
-__device__ __forceinline__ double trsv_gather(double acc, int e, int stop, int dir,
-                                               const int *__restrict__ ci,
-                                               const double *__restrict__ lu, const double *x,
-                                               unsigned *err) {
-  unsigned long long u[G];
-  int c[G];
-  double a[G];
-  // branch-free loads (entry e is in range: a slot past the stop re-reads
-  // it and is masked by c = -1), so that they are all in flight at once
+__device__ __forceinline__ void trsv_issue(int e, int stop, int dir, const int *__restrict__ ci,
+                                           const double *__restrict__ lu, const double *x, int *c,
+                                           double *a, unsigned long long *u) {
 #pragma unroll
   for (int q = 0; q < G; ++q) {
     const int eq = e + q * dir;
@@ -1024,13 +1020,24 @@ __device__ __forceinline__ double trsv_gather(double acc, int e, int stop, int d
   }
 #pragma unroll
   for (int q = 0; q < G; ++q) u[q] = ld_relaxed(x + (c[q] >= 0 ? c[q] : c[0]));
-  unsigned polls = 0;
-  for (;;) {
-    bool pend = false;
+}
+
+template <int G>
+__device__ __forceinline__ bool trsv_pending(const int *c, const unsigned long long *u) {
+  bool pend = false;
 #pragma unroll
-    for (int q = 0; q < G; ++q) pend |= (c[q] >= 0 && u[q] == XS_SENTINEL);
-    if (!pend) break;
-    if (++polls > XS_SPIN_LIMIT) {  // bounded wait: flag it, continue with NaN
+  for (int q = 0; q < G; ++q) pend |= (c[q] >= 0 && u[q] == XS_SENTINEL);
+  return pend;
+}
+
+// re-poll the values still holding the sentinel, all together, until none
+// is left (bounded: past the spin limit the flag is raised and NaN used)
+template <int G>
+__device__ __forceinline__ void trsv_poll(const int *c, unsigned long long *u, const double *x,
+                                          unsigned *err) {
+  unsigned polls = 0;
+  while (trsv_pending<G>(c, u)) {
+    if (++polls > XS_SPIN_LIMIT) {
       atomicOr(err, 1u);
 #pragma unroll
       for (int q = 0; q < G; ++q)
@@ -1042,12 +1049,34 @@ __device__ __forceinline__ double trsv_gather(double acc, int e, int stop, int d
     for (int q = 0; q < G; ++q)
       if (c[q] >= 0 && u[q] == XS_SENTINEL) u[q] = ld_relaxed(x + c[q]);
   }
+}
+
+// acc minus the products, subtracted in entry order
+template <int G>
+__device__ __forceinline__ double trsv_sub(double acc, const int *c, const double *a,
+                                           const unsigned long long *u) {
 #pragma unroll
   for (int q = 0; q < G; ++q)
     if (c[q] >= 0) acc -= a[q] * __longlong_as_double((long long)u[q]);
   return acc;
 }
 
+// acc - sum of lu[e] * x[ci[e]] over up to G entries from e, waiting for the
+// values that are not yet published
+template <int G>
+__device__ __forceinline__ double trsv_gather(double acc, int e, int stop, int dir,
+                                               const int *__restrict__ ci,
+                                               const double *__restrict__ lu, const double *x,
+                                               unsigned *err) {
+  unsigned long long u[G];
+  int c[G];
+  double a[G];
+  trsv_issue<G>(e, stop, dir, ci, lu, x, c, a, u);
+  trsv_poll<G>(c, u, x, err);
+  return trsv_sub<G>(acc, c, a, u);
+}
+
+#define XS_TRSV_AHEAD 4
 template <bool UPPER, int G>
 __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict__ rp,
                                                    const int *__restrict__ ci,
@@ -1074,6 +1103,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
     const int r = UPPER ? r1 - 1 - lane : r0 + lane;
     double acc = 0.0, di = 1.0;
     int eb = 0, ee = 0;  // in-wave entries [eb, ee)
+    int xb = 0, xe = 0;  // entries outside the group
     // the first XS_INW in-wave entries in solve order, loaded before any wait
     // so that no load sits on the in-wave chain
     int iw_c[XS_INW];
@@ -1083,7 +1113,6 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
     if (act) {
       acc = b[r];
       if (dinv) di = dinv[r];
-      int xb, xe;        // entries outside the group
       if (!UPPER) {
         xb = rp[r];
         const int end = dpos[r];
@@ -1102,33 +1131,96 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
         const int e = UPPER ? ee - 1 - q : eb + q;
         if (q < ee - eb) { iw_c[q] = ci[e]; iw_v[q] = lu[e]; }
       }
-      if (!UPPER) {
-        for (int e = xb; e < xe; e += G) acc = trsv_gather<G>(acc, e, xe, 1, ci, lu, x, err);
-      } else {
-        for (int e = xe - 1; e >= xb; e -= G) acc = trsv_gather<G>(acc, e, xb - 1, -1, ci, lu, x, err);
-      }
     }
+    // external entries in solve order: L from xb up, U from xe - 1 down
+    const int dir = UPPER ? -1 : 1;
+    const int estop = UPPER ? xb - 1 : xe;
+    int ep = UPPER ? xe - 1 : xb;
     const int nint = ee - eb;
     const int nb = UPPER ? r + 1 : r - 1;
     const bool chain = !act || nint == 0 || (nint == 1 && iw_c[0] == nb);
-    double val;
-    if (__all(chain)) {
-      // x_r = c + m * x_nb, composed along the lanes
+    if (!S.prog && __all(chain)) {
+      // every row depends at most on the one before it: wait for all
+      // external values, then x_r = c + m * x_nb composed along the lanes
+      if (act)
+        for (; UPPER ? ep > estop : ep < estop; ep += G * dir)
+          acc = trsv_gather<G>(acc, ep, estop, dir, ci, lu, x, err);
       double c[1] = {act ? di * acc : 0.0};
       double m = (act && nint == 1) ? -di * iw_v[0] : 0.0;
       wave_affine_scan<1>(c, m);
-      val = c[0];
+      if (act) st_ready(x + r, c[0]);
     } else {
-      // lane k's row is final once the rows of lanes < k are: broadcast it;
-      // each lane consumes its in-wave entries from registers in solve order
-      // (shifted down as they are used, refilled from memory past XS_INW)
+      // Rows depend on several earlier rows of the group: they are finished
+      // one lane at a time, and each is published as soon as it is final,
+      // so the groups that wait on it go on without waiting for the rest of
+      // this group.  Up front every lane subtracts its external products in
+      // order up to the first batch holding a value not yet published (that
+      // batch stays in registers); at its turn the lane waits for the rest.
+      // held: the first batch with an unpublished value and the batch after
+      // it (2G entries), so that a lane's turn rarely issues new loads
+      int pc[2 * G];
+      double pa[2 * G];
+      unsigned long long pu[2 * G];
+      bool stalled = false;
+      if (act) {
+        for (; UPPER ? ep > estop : ep < estop; ep += G * dir) {
+          trsv_issue<G>(ep, estop, dir, ci, lu, x, pc, pa, pu);
+          if (trsv_pending<G>(pc, pu)) { stalled = true; break; }
+          acc = trsv_sub<G>(acc, pc, pa, pu);
+        }
+      }
+      if (stalled) {
+        const int e2 = ep + G * dir;
+        if (UPPER ? e2 > estop : e2 < estop) {
+          trsv_issue<G>(e2, estop, dir, ci, lu, x, pc + G, pa + G, pu + G);
+        } else {
+#pragma unroll
+          for (int q = G; q < 2 * G; ++q) { pc[q] = -1; pa[q] = 0.0; pu[q] = 0; }
+        }
+      }
+      double ain = 0.0;   // the in-wave products, subtracted in solve order
       int left = act ? nint - XS_INW : 0;
       int enext = UPPER ? ee - 1 - XS_INW : eb + XS_INW;
       for (int k = 0; k < nrows; ++k) {
-        const double xk = __shfl(acc * di, k, XS_WAVE);
+        // while lane k's held batches still have an unpublished value, the
+        // next XS_TRSV_AHEAD lanes re-poll theirs in the same round trip, so
+        // they rarely wait at their own turn (a window, not the whole wave:
+        // a thousand waves re-polling every lane would flood the memory
+        // system with polls)
+        unsigned polls = 0;
+        for (;;) {
+          const int kwait = __shfl((stalled && trsv_pending<2 * G>(pc, pu)) ? 1 : 0, k, XS_WAVE);
+          if (!kwait) break;
+          if (++polls > XS_SPIN_LIMIT) {   // bounded wait: flag it, continue with NaN
+            if (lane == k) {
+              atomicOr(err, 1u);
+#pragma unroll
+              for (int q = 0; q < 2 * G; ++q)
+                if (pu[q] == XS_SENTINEL) pu[q] = 0x7FF8000000000000ull;
+            }
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+          if (stalled && lane >= k && lane < k + XS_TRSV_AHEAD) {
+#pragma unroll
+            for (int q = 0; q < 2 * G; ++q)
+              if (pc[q] >= 0 && pu[q] == XS_SENTINEL) pu[q] = ld_relaxed(x + pc[q]);
+          }
+        }
+        double vk = 0.0;
+        if (lane == k) {
+          if (stalled) {
+            acc = trsv_sub<2 * G>(acc, pc, pa, pu);
+            for (ep += 2 * G * dir; UPPER ? ep > estop : ep < estop; ep += G * dir)
+              acc = trsv_gather<G>(acc, ep, estop, dir, ci, lu, x, err);
+          }
+          vk = (acc + ain) * di;
+          st_ready(x + r, vk);
+        }
+        const double xk = __shfl(vk, k, XS_WAVE);
         const int rk = UPPER ? r1 - 1 - k : r0 + k;
         if (iw_c[0] == rk) {
-          acc -= iw_v[0] * xk;
+          ain -= iw_v[0] * xk;
 #pragma unroll
           for (int q = 0; q + 1 < XS_INW; ++q) { iw_c[q] = iw_c[q + 1]; iw_v[q] = iw_v[q + 1]; }
           if (left > 0) {
@@ -1141,9 +1233,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
           }
         }
       }
-      val = acc * di;
     }
-    if (act) st_ready(x + r, val);
   }
   }
   trsv_retire(tickets);
@@ -1180,6 +1270,7 @@ void xs_trsv_lower_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
   xs_tsched S;
   S.max_wg = F.trsv_wg;
   S.gather = xs_trsv_gather_width(F.nnz, F.n);
+  S.prog = F.trsv_prog >= 0 ? F.trsv_prog : (S.gather == 16 ? 1 : 0);
   trsv_launch(s, false, F.n, F.trp, F.tci, F.tlu, F.tdpos, F.dinv, b, x, tickets, S, st);
 }
 void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
@@ -1187,6 +1278,7 @@ void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
   xs_tsched S;
   S.max_wg = F.trsv_wg;
   S.gather = xs_trsv_gather_width(F.nnz, F.n);
+  S.prog = F.trsv_prog >= 0 ? F.trsv_prog : (S.gather == 16 ? 1 : 0);
   trsv_launch(s, true, F.n, F.trp, F.tci, F.tlu, F.tdpos, nullptr, b, x, tickets, S, st);
 }
 

@@ -48,6 +48,7 @@ struct xs_ctx {
   bool use_graph = true;        // replay batches of iterations as HIP graphs
   bool trsv_schedule = true;    // SpTRSV groups in dependency-level order
   int trsv_wg = 256;            // persistent workgroups of the SpTRSV
+  int trsv_prog = -1;           // SpTRSV group mode (xs_tsched::prog), -1 = by factor width
   int spmv_vec = -1;            // CSR kernel: -1 by row regularity, 0 LDS row blocks, 1 vector CSR
   int precision = 0;
   int use_ell = 0;
