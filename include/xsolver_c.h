@@ -132,6 +132,10 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *   "parilu_scale"   1 (default) = factorise S A S, S = |diag(A)|^-1/2, and
  *                    scale back (ParILU/ParILUT fixed-point stability)
  *   "parilut_steps"  ParILUT add/sweep/select/sweep steps (reference: 10)
+ *   "parilut_rows"   ParILUT row table of the candidate and sweep kernels:
+ *                    0 (default) = the smallest of 128 / 256 / 512 entries that
+ *                    leaves at most n/1024 rows to the per-entry form; 128,
+ *                    256 or 512 force it (longer rows take the per-entry form)
  *   "pc_reuse"       1 = xs_mat_update_values keeps the preconditioner built
  *                    from the earlier values (a preconditioner sets the rate
  *                    of convergence, not the converged solution); 0 (default)

@@ -299,20 +299,27 @@ PARILUT_MATS = {
 }
 
 
-@pytest.mark.parametrize("name", list(PARILUT_MATS))
-def test_parilut_matches_oracle(dev, name):
+PARILUT_CASES = [(m, 0) for m in PARILUT_MATS] + [("lognormal27_10", 128), ("lognormal27_10", 512)]
+
+
+@pytest.mark.parametrize("name,rows", PARILUT_CASES, ids=[f"{m}-rows{r}" for m, r in PARILUT_CASES])
+def test_parilut_matches_oracle(dev, name, rows):
     """GPU ParILUT (10 steps, fill 2, drop 1e-4) against the oracle's
     deterministic restatement: same pattern up to threshold ties, same values
-    to rounding, pivot always present, per-factor counts within the keep."""
+    to rounding, pivot always present, per-factor counts within the keep.
+    rows: the row table of the candidate and sweep kernels (0 = from the row
+    lengths; 128 sends the 27-point candidate rows beyond 128 entries to the
+    per-entry forms, 512 takes the two-wave workgroups)."""
     A = PARILUT_MATS[name]()
     n = len(A[0]) - 1
-    say(f"[parilut] {name} n={n} nnz={len(A[2])}")
+    say(f"[parilut] {name} n={n} nnz={len(A[2])} rows={rows}")
     L, U = oracle.parilut(A, 10, 2.0, 1e-4)
     S = XSolver()
     S.mat_setup(*A)
     S.ksp_settype("gmres")
     S.pc_settype("ilut")
     S.pc_setilut(1e-4, 2)
+    S.set_option("parilut_rows", rows)
     S.pc_setup()
     cap = 4 * len(A[2]) + n
     rp, ci, lu, dpos = S.factors(n, cap)

@@ -676,6 +676,11 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   else if (k == "parilu_scale") { c->parilu_scale = value != 0; c->pc_ready = false; }
   else if (k == "pc_reuse") { c->pc_reuse = value != 0; }
   else if (k == "parilut_steps") { if (value < 1) return XS_ERR_ARG; c->parilut_steps = (int)value; c->pc_ready = false; }
+  else if (k == "parilut_rows") {
+    if (value != 0 && value != 128 && value != 256 && value != 512) return XS_ERR_ARG;
+    c->parilut_rows = (int)value;
+    c->pc_ready = false;
+  }
   else if (k == "profile") { c->prof_kind = (int)value; for (int i = 0; i < 4; i++) { c->prof_ms[i] = 0; c->prof_cnt[i] = 0; } }
   else if (k == "precision") { if (value != 0 && value != 1) return XS_ERR_ARG; c->precision = (int)value; }
   else if (k == "use_ell") {
@@ -1044,13 +1049,13 @@ int xs_parilut_factor(xs_ctx *c) {
   double *dtol = nullptr, *thr = nullptr;
   int *cnt = nullptr, *bsum = nullptr, *ovf = nullptr;
   unsigned long long *sel = nullptr, *cnt2 = nullptr, *list = nullptr;
-  unsigned *hist = nullptr, *lcnt = nullptr;
+  unsigned *hist = nullptr, *lcnt = nullptr, *lstat = nullptr;
   int *crp = nullptr, *cci = nullptr, *crow = nullptr, *cdpos = nullptr;
   double *cv = nullptr, *cv2 = nullptr, *fv2 = nullptr;
   auto cleanup = [&]() {
     dfree(dtol); dfree(thr); dfree(cnt); dfree(bsum); dfree(ovf); dfree(sel); dfree(cnt2);
     dfree(list); dfree(hist); dfree(lcnt); dfree(crp); dfree(cci); dfree(crow); dfree(cdpos);
-    dfree(cv); dfree(cv2); dfree(fv2);
+    dfree(cv); dfree(cv2); dfree(fv2); dfree(lstat);
   };
 #define PT_TRY(x) do { if ((rc = (x))) { cleanup(); return rc; } } while (0)
   PT_TRY(F.scale ? XS_OK : dalloc(&F.scale, n));
@@ -1064,6 +1069,7 @@ int xs_parilut_factor(xs_ctx *c) {
   PT_TRY(dalloc(&list, xs_pt_list_len()));
   PT_TRY(dalloc(&hist, 512));
   PT_TRY(dalloc(&lcnt, 2));
+  PT_TRY(dalloc(&lstat, 8));
   PT_TRY(dalloc(&crp, n + 1));
   PT_TRY(dalloc(&cdpos, n));
   PT_TRY(dalloc(&F.rp_own, n + 1));
@@ -1097,7 +1103,7 @@ int xs_parilut_factor(xs_ctx *c) {
   // the diagonal: the bases of the keep counts (l0nnz / u0nnz,
   // hip/Precond/parilut.cpp:681-682)
   unsigned long long base[2] = {0, 0};
-  xs_pt_count_parts(s, nnz, F.rowidx, F.ci_own, cnt2);
+  xs_pt_count_parts(s, n, F.rp_own, F.dpos, cnt2);
   XS_HIP(hipMemcpyAsync(base, cnt2, sizeof(base), hipMemcpyDeviceToHost, s));
   XS_HIP(hipStreamSynchronize(s));
   const long nl0 = (long)base[0], nu0 = (long)base[1];
@@ -1109,14 +1115,18 @@ int xs_parilut_factor(xs_ctx *c) {
     const double rate = 1.0 + (fill - 1.0) * (double)(step + 1) / (double)steps;
     long keepL = (long)(rate * (double)nl0), keepU = (long)(rate * (double)nu0);
     // 1. candidate pattern
-    xs_pt_symbolic(s, false, n, c->A.rp, c->A.ci, F.rp_own, F.ci_own, F.dpos, cnt, nullptr,
-                   nullptr, nullptr, ovf);
+    xs_pt_symbolic(s, n, c->A.rp, c->A.ci, F.rp_own, F.ci_own, F.dpos, cnt, ovf);
     xs_scan_excl(s, n, cnt, crp, bsum);
+    xs_pt_len_stats(s, n, cnt, lstat);
     int hdr[2] = {0, 0};
+    unsigned cst[4];
     XS_HIP(hipMemcpyAsync(&hdr[0], crp + n, sizeof(int), hipMemcpyDeviceToHost, s));
     XS_HIP(hipMemcpyAsync(&hdr[1], ovf, sizeof(int), hipMemcpyDeviceToHost, s));
+    XS_HIP(hipMemcpyAsync(cst, lstat, sizeof(cst), hipMemcpyDeviceToHost, s));
     XS_HIP(hipStreamSynchronize(s));
-    if (hdr[1]) { snprintf(xs_errbuf, sizeof(xs_errbuf), "ParILUT: pattern overflow"); cleanup(); return XS_ERR_MEMORY; }
+    const int crows = c->parilut_rows ? c->parilut_rows : xs_pt_pick_rows(n, cst);
+    if (hdr[1] & 1) { snprintf(xs_errbuf, sizeof(xs_errbuf), "ParILUT: pattern overflow"); cleanup(); return XS_ERR_MEMORY; }
+    if (hdr[1] & 2) { snprintf(xs_errbuf, sizeof(xs_errbuf), "ParILUT: a row without its diagonal"); cleanup(); return XS_ERR_PIVOT; }
     const int nc = hdr[0];
     {
       size_t c1 = cap_c, c2 = cap_c, c3 = cap_c, c4 = cap_c;
@@ -1126,35 +1136,37 @@ int xs_parilut_factor(xs_ctx *c) {
       PT_TRY(grow((size_t)nc, c4, &cv2));
       cap_c = c1;
     }
-    xs_pt_symbolic(s, true, n, c->A.rp, c->A.ci, F.rp_own, F.ci_own, F.dpos, cnt, crp, cci, crow, ovf);
-    xs_pt_dpos(s, n, crp, cci, cdpos, ovf);
-    // 2-3. embed and sweep over the candidates
-    xs_pt_embed(s, n, crp, cci, c->A.rp, c->A.ci, c->A.v, F.scale, F.rp_own, F.ci_own, F.lu,
-                F.dpos, cv);
-    xs_pt_sweep(s, n, crp, cci, cdpos, cv, cv2, c->A.rp, c->A.ci, c->A.v, F.scale);
+    // 2-3. the candidates with the old values embedded, and a sweep over them
+    xs_pt_fill_embed(s, n, c->A.rp, c->A.ci, c->A.v, F.scale, F.rp_own, F.ci_own, F.lu, F.dpos,
+                     crp, cci, crow, cv, cdpos, ovf, crows, xs_pt_rows_over(crows, cst));
+    xs_pt_sweep(s, n, crp, cci, cdpos, cv, cv2, c->A.rp, c->A.ci, c->A.v, F.scale, crows);
     std::swap(cv, cv2);
     // 4. thresholds (the part counts stay on the device); nothing is
     // dropped by rank while the candidates number at most round(rate nnz(A))
     // (hip/Precond/parilut.cpp:926-931)
     if ((long)nc - std::lround(rate * (double)nnz) <= 0) keepL = keepU = LONG_MAX;
-    xs_pt_count_parts(s, nc, crow, cci, cnt2);
+    xs_pt_count_parts(s, n, crp, cdpos, cnt2);
     xs_pt_select(s, nc, crow, cci, cv, F.scale, cnt2, keepL, keepU, sel, hist, lcnt, list, thr);
     // compaction into the new factor
     xs_pt_keep(s, false, n, crp, cci, cv, thr, dtol, F.scale, cnt, nullptr, nullptr, nullptr,
-               nullptr);
+               nullptr, nullptr, nullptr);
     xs_scan_excl(s, n, cnt, F.rp_own, bsum);
+    xs_pt_len_stats(s, n, cnt, lstat);
+    unsigned fst[4];
     XS_HIP(hipMemcpyAsync(&fnnz, F.rp_own + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    XS_HIP(hipMemcpyAsync(fst, lstat, sizeof(fst), hipMemcpyDeviceToHost, s));
     XS_HIP(hipStreamSynchronize(s));
+    const int frows = c->parilut_rows ? c->parilut_rows : xs_pt_pick_rows(n, fst);
     // the old factor was last read by the embed (the stream is idle here)
     PT_TRY(grow((size_t)fnnz, cap_ci, &F.ci_own));
     PT_TRY(grow((size_t)fnnz, cap_lu, &F.lu));
     PT_TRY(grow((size_t)fnnz, cap_row, &F.rowidx));
     PT_TRY(grow((size_t)fnnz, cap_v2, &fv2));
     xs_pt_keep(s, true, n, crp, cci, cv, thr, dtol, F.scale, nullptr, F.rp_own, F.ci_own, F.lu,
-               F.rowidx);
-    xs_pt_dpos(s, n, F.rp_own, F.ci_own, F.dpos, ovf);
+               F.rowidx, F.dpos, ovf);
     // 5. sweep over the kept pattern
-    xs_pt_sweep(s, n, F.rp_own, F.ci_own, F.dpos, F.lu, fv2, c->A.rp, c->A.ci, c->A.v, F.scale);
+    xs_pt_sweep(s, n, F.rp_own, F.ci_own, F.dpos, F.lu, fv2, c->A.rp, c->A.ci, c->A.v, F.scale,
+                frows);
     std::swap(F.lu, fv2);
     std::swap(cap_lu, cap_v2);
   }
@@ -1171,8 +1183,8 @@ int xs_parilut_factor(xs_ctx *c) {
   XS_HIP(hipStreamSynchronize(s));
   cleanup();
 #undef PT_TRY
-  if (ov) return XS_ERR_MEMORY;
-  if (bad != 0x7f7f7f7f) return XS_ERR_PIVOT;
+  if (ov & 1) return XS_ERR_MEMORY;
+  if ((ov & 2) || bad != 0x7f7f7f7f) return XS_ERR_PIVOT;
   // row groups of the SpTRSV: built from the new pattern on the first exact solve
   F.sched_pending = true;
   return XS_OK;

@@ -4,18 +4,30 @@
 
 void xs_scan_excl(hipStream_t s, int n, const int *in, int *out, int *bsum);
 int xs_scan_bsum_len(int n);
-void xs_pt_symbolic(hipStream_t s, bool fill, int n, const int *arp, const int *aci,
-                    const int *frp, const int *fci, const int *fdpos, int *cnt, const int *crp,
-                    int *cci, int *crow, int *ovf);
+// per-row candidate counts |C_i|, C_i = A_i u F_i u (union over k in L_i of
+// the upper part of F_k with its pivot); *ovf |= 1 for a row beyond 1024
+void xs_pt_symbolic(hipStream_t s, int n, const int *arp, const int *aci, const int *frp,
+                    const int *fci, const int *fdpos, int *cnt, int *ovf);
+// pivot positions of a sorted pattern (*bad |= 2 for a row without its diagonal)
 void xs_pt_dpos(hipStream_t s, int n, const int *rp, const int *ci, int *dpos, int *bad);
-// values of the old factor F embedded into the candidate pattern C (n rows,
-// CSR crp / cci); one ParILUT sweep over a pattern (rp, ci, dpos)
-void xs_pt_embed(hipStream_t s, int n, const int *crp, const int *cci, const int *arp,
-                 const int *aci, const double *av, const double *sc, const int *frp,
-                 const int *fci, const double *fv, const int *fdpos, double *cv);
+// the candidate pattern C (crp from the counts; cci, crow, pivot positions
+// cdpos) with the old factor F's values embedded (cv); one ParILUT sweep over
+// a pattern (rp, ci, dpos).  maxr: the row table (128, 256 or 512 entries)
+// from xs_pt_pick_rows; has_long: some rows are longer (they take the
+// per-entry forms).  *ovf |= 2 for a row without its diagonal.
+void xs_pt_fill_embed(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
+                      const double *sc, const int *frp, const int *fci, const double *fv,
+                      const int *fdpos, const int *crp, int *cci, int *crow, double *cv,
+                      int *cdpos, int *ovf, int maxr, bool has_long);
 void xs_pt_sweep(hipStream_t s, int n, const int *rp, const int *ci, const int *dpos,
                  const double *vin, double *vout, const int *arp, const int *aci,
-                 const double *av, const double *sc);
+                 const double *av, const double *sc, int maxr);
+// row-length statistics from per-row counts (4 words: longest row, rows
+// longer than 128, 256, 512), the row table they call for and whether some
+// rows exceed it
+void xs_pt_len_stats(hipStream_t s, int n, const int *cnt, unsigned *stats);
+int xs_pt_pick_rows(int n, const unsigned *stats);
+bool xs_pt_rows_over(int maxr, const unsigned *stats);
 // thresholds of both factors on the unscaled magnitudes (sc = the scaling
 // S of the swept values): thr[0] (resp. thr[1]) = the value of rank
 // cnt2[0] - keepL (resp. cnt2[1] - keepU) in ascending magnitude over L with
@@ -29,17 +41,20 @@ void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const d
 int xs_pt_select_words();
 int xs_pt_list_len();
 // entries with col <= row (L with its unit diagonal) -> cnt2[0], col >= row
-// (U with its pivots) -> cnt2[1]
-void xs_pt_count_parts(hipStream_t s, int nnz, const int *row, const int *ci,
+// (U with its pivots) -> cnt2[1], from the pivot positions dpos of a sorted
+// n-row pattern rp whose rows all hold their diagonal
+void xs_pt_count_parts(hipStream_t s, int n, const int *rp, const int *dpos,
                        unsigned long long *cnt2);
 // dtol[i] = droptol * ||a_i||_2 (unscaled A)
 void xs_pt_dtol(hipStream_t s, int n, const int *arp, const double *av, double droptol,
                 double *dtol);
 // compaction of the candidates (count pass, then write pass) with the
-// thresholds of xs_pt_select and the drop tolerance on unscaled magnitudes
+// thresholds of xs_pt_select and the drop tolerance on unscaled magnitudes;
+// the write pass also places the pivots (fdpos; *bad |= 1 for a row without)
 void xs_pt_keep(hipStream_t s, bool write, int n, const int *crp, const int *cci,
                 const double *cv, const double *thr, const double *dtol, const double *sc,
-                int *cnt, const int *frp, int *fci, double *fv, int *frow);
+                int *cnt, const int *frp, int *fci, double *fv, int *frow, int *fdpos,
+                int *bad);
 void xs_pt_scale_vec(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
                      double *sc);
 void xs_pt_rescale(hipStream_t s, int nnz, const int *row, const int *ci, double *v,

@@ -22,7 +22,7 @@
 // the caller scales A's values in and the factors back out.
 #include <climits>
 
-#include "xs_internal.h"
+#include "xs_device.h"
 #include "xs_parilut.h"
 
 #define PT_HASH 2048          // LDS hash slots per wavefront
@@ -35,76 +35,162 @@ __device__ __forceinline__ int pt_wave_sum_i(int v) {
   return v;
 }
 
-__device__ __forceinline__ void pt_insert(int *tab, int c, int *ovf) {
-  unsigned h = ((unsigned)c * 2654435761u) & (PT_HASH - 1);
-  for (int probe = 0; probe < PT_HASH; ++probe) {
+// insert column c (when v) into a row's 2048-slot set; *ovf |= 1 when the
+// set is full.  One CAS per lane; the probe loop runs only when some lane's
+// first slot holds another column.
+__device__ int pt_insert_from(int *tab, int c, unsigned h, int *ovf) {
+  for (int probe = 1; probe < PT_HASH; ++probe) {
     const int old = atomicCAS(&tab[h], PT_EMPTY, c);
-    if (old == PT_EMPTY || old == c) return;
+    if (old == PT_EMPTY || old == c) return 0;
     h = (h + 1) & (PT_HASH - 1);
   }
   atomicOr(ovf, 1);
+  return 1;
 }
 
-// Pass 1 (fill == false): cnt[i] = |C_i|.  Pass 2 (fill == true): write C_i
-// sorted into cci[crp[i]..] and the row index into crow.
+__device__ __forceinline__ void pt_insert(int *tab, int c, bool v, int *ovf) {
+  const unsigned h = ((unsigned)c * 2654435761u) & (PT_HASH - 1);
+  int old = PT_EMPTY;
+  if (v) old = atomicCAS(&tab[h], PT_EMPTY, c);
+  const bool more = v && old != PT_EMPTY && old != c;
+  if (__any(more))
+    if (more) pt_insert_from(tab, c, (h + 1) & (PT_HASH - 1), ovf);
+}
+
+// The products l_ik u_kj of row i's L entries [lb, le) of F (columns k,
+// values l_ik) with the upper parts [fdpos[k] + off, frp[k + 1]) of the rows
+// k, as one load-balanced stream: the parts of up to 64 L entries laid end to
+// end (lane t holds entry t's part and the inclusive scan inc of the part
+// lengths), 128 products per step whatever the part lengths, each lane
+// taking positions p0 + 64 c + lane (c < PT_SPOS), two steps in flight.  The
+// parts a step spans are found by ballot (the owner of position p is the
+// number of lanes with inc <= p) and visited in a wave-uniform loop that
+// hands each lane its part's bounds and l_ik by readlane -- no LDS traffic.
+// fn(j, l_ik u_kj, first, valid) per lane and position, for every lane
+// (valid false: no product there, j a column of the matrix all the same);
+// value 0 without VALS; first marks a part's entry at fdpos[k] + off.
+// Every lane runs every step.
+__device__ __forceinline__ double pt_readlane(double v, int t) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, t);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), t);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+#define PT_SPOS 2   // stream positions per lane per step
+
+template <bool VALS, typename Fn>
+__device__ __forceinline__ void pt_stream(int lb, int le, int off, const int *__restrict__ frp,
+                                          const int *__restrict__ fci,
+                                          const double *__restrict__ fv,
+                                          const int *__restrict__ fdpos, int lane, Fn &&fn) {
+  constexpr int P = PT_SPOS, STEP = P * XS_WAVE;
+  struct step {
+    int j[P];
+    double l[P], u[P];
+    bool f[P], v[P];     // first entry of its part; a product at this position
+  };
+  for (int c0 = lb; c0 < le; c0 += XS_WAVE) {
+    // lane t: L entry c0 + t (loads clamped to the block's last entry, so
+    // they issue unconditionally), its value and its part of the stream
+    const int cnt = min(XS_WAVE, le - c0);
+    const int ce = c0 + min(lane, cnt - 1);
+    const int k = fci[ce];
+    const double lv = VALS ? fv[ce] : 0.0;
+    const int ub = fdpos[k] + off;
+    const int len = lane < cnt ? frp[k + 1] - ub : 0;
+    int inc = len;
+#pragma unroll
+    for (int o = 1; o < XS_WAVE; o <<= 1) {
+      const int y = __shfl_up(inc, o, XS_WAVE);
+      if (lane >= o) inc += y;
+    }
+    const int tot = __builtin_amdgcn_readlane(inc, XS_WAVE - 1);
+    const int st = inc - len;            // first stream position of this lane's part
+    const int base = ub - st;            // q = base + p for the positions of this part
+    // locate the step's positions and issue their loads, for every lane and
+    // whether or not the step holds products (a position past the end loads
+    // entry c0, so no load sits behind a branch)
+    auto issue = [&](int p0, step &z) {
+      const int pe = min(p0 + STEP, tot) - 1;
+      // owners of the first and the last position of the step (uniform;
+      // no iteration once p0 is past the end)
+      const int o0 = __popcll(__ballot(inc <= p0));
+      const int o1 = __popcll(__ballot(inc <= pe));
+      int q[P], sp[P];
+#pragma unroll
+      for (int c = 0; c < P; ++c) {
+        q[c] = c0;
+        sp[c] = -1;
+        z.l[c] = 0.0;
+      }
+      for (int t = o0; t <= o1; ++t) {
+        // parts in ascending order: each lane keeps the last part starting
+        // at or before its position (an empty part shares its start with
+        // the next one, which overrides it)
+        const int ts = __builtin_amdgcn_readlane(st, t);
+        const int tb = __builtin_amdgcn_readlane(base, t);
+        const double tl = VALS ? pt_readlane(lv, t) : 0.0;
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+          const int p = p0 + c * XS_WAVE + lane;
+          if (p >= ts) { q[c] = tb + p; z.l[c] = tl; sp[c] = ts; }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < P; ++c) {
+        const int p = p0 + c * XS_WAVE + lane;
+        z.v[c] = p < tot;
+        const int qq = z.v[c] ? q[c] : c0;
+        z.j[c] = fci[qq];
+        z.u[c] = VALS ? fv[qq] : 0.0;
+        // first: the entry at fdpos[k] + off (the pivot u_kk when off = 0)
+        z.f[c] = p == sp[c];
+      }
+    };
+    auto run = [&](const step &z) {
+#pragma unroll
+      for (int c = 0; c < P; ++c) fn(z.j[c], z.l[c] * z.u[c], z.f[c], z.v[c]);
+    };
+    // two steps in flight, in two register sets used in turn (no copies of
+    // registers whose loads are pending): the next step's loads are issued
+    // before the current step's products are added
+    step A, B;
+    issue(0, A);
+    for (int p0 = 0; p0 < tot; p0 += 2 * STEP) {
+      issue(p0 + STEP, B);
+      run(A);
+      issue(p0 + 2 * STEP, A);
+      run(B);
+    }
+  }
+}
+
+// Pass 1 (fill == false): cnt[i] = |C_i| for every row.  Pass 2 (fill ==
+// true), for the rows with more than minlen candidates only: C_i sorted into
+// cci[crp[i]..], the row index into crow and the pivot position into cdpos
+// (*ovf |= 2 for a row without its diagonal).  The 2048-slot hash takes any
+// row up to 1024 candidates (*ovf |= 1 beyond).
 template <bool FILL>
 __global__ __launch_bounds__(XS_WAVE *PT_WAVES) void k_pt_symbolic(
     int n, const int *__restrict__ arp, const int *__restrict__ aci, const int *__restrict__ frp,
     const int *__restrict__ fci, const int *__restrict__ fdpos, int *__restrict__ cnt,
-    const int *__restrict__ crp, int *__restrict__ cci, int *__restrict__ crow, int *ovf) {
+    const int *__restrict__ crp, int *__restrict__ cci, int *__restrict__ crow,
+    int *__restrict__ cdpos, int minlen, int *ovf) {
   __shared__ int tab_all[PT_WAVES][PT_HASH];
   const int lane = threadIdx.x & (XS_WAVE - 1), wv = threadIdx.x / XS_WAVE;
-  const int i = blockIdx.x * PT_WAVES + wv;
+  const int i = xcd_chunk(blockIdx.x, gridDim.x) * PT_WAVES + wv;
+  const bool act = i < n && (!FILL || crp[i + 1] - crp[i] > minlen);   // wave-uniform
   int *tab = tab_all[wv];
   int *keys = tab;   // compacted in place (see below)
-  for (int q = lane; q < PT_HASH; q += XS_WAVE) tab[q] = PT_EMPTY;
-  __syncthreads();
-  if (i < n) {
-    for (int e = arp[i] + lane; e < arp[i + 1]; e += XS_WAVE) pt_insert(tab, aci[e], ovf);
-    for (int e = frp[i] + lane; e < frp[i + 1]; e += XS_WAVE) pt_insert(tab, fci[e], ovf);
+  if (act) {
+    for (int q = lane; q < PT_HASH; q += XS_WAVE) tab[q] = PT_EMPTY;
+    for (int e = arp[i] + lane; e < arp[i + 1]; e += XS_WAVE) pt_insert(tab, aci[e], true, ovf);
+    for (int e = frp[i] + lane; e < frp[i + 1]; e += XS_WAVE) pt_insert(tab, fci[e], true, ovf);
     // union of the upper parts (incl. pivot) of the rows k in L_i (a set:
-    // order is free).  Up to 64 L entries at once: lane t holds the bounds of
-    // row k_t; then the first 64 entries of four such rows are loaded
-    // together, the next four while the current ones are inserted.
-    constexpr int D = 4;
-    const int lb = frp[i], le = fdpos[i];
-    for (int c0 = lb; c0 < le; c0 += XS_WAVE) {
-      const int nl = min(XS_WAVE, le - c0);
-      int ub = 0, ue = 0;
-      if (lane < nl) {
-        const int k = fci[c0 + lane];
-        ub = fdpos[k];
-        ue = frp[k + 1];
-      }
-      int qb[D], qe[D], jj[D], nqb[D], nqe[D], njj[D];
-      auto load_group = [&](int g0, int (&gb)[D], int (&ge)[D], int (&gj)[D]) {
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const bool ok = g0 + d < nl;
-          gb[d] = ok ? __shfl(ub, g0 + d, XS_WAVE) : 0;
-          ge[d] = ok ? __shfl(ue, g0 + d, XS_WAVE) : 0;
-        }
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const int q = gb[d] + lane;
-          gj[d] = q < ge[d] ? fci[q] : -1;
-        }
-      };
-      load_group(0, qb, qe, jj);
-      for (int g0 = 0; g0 < nl; g0 += D) {
-        if (g0 + D < nl) load_group(g0 + D, nqb, nqe, njj);
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          if (jj[d] >= 0) pt_insert(tab, jj[d], ovf);
-          for (int q = qb[d] + XS_WAVE + lane; q < qe[d]; q += XS_WAVE) pt_insert(tab, fci[q], ovf);
-        }
-#pragma unroll
-        for (int d = 0; d < D; ++d) { qb[d] = nqb[d]; qe[d] = nqe[d]; jj[d] = njj[d]; }
-      }
-    }
-  }
-  __syncthreads();
-  if (i < n) {
+    // order is free)
+    pt_stream<false>(frp[i], fdpos[i], 0, frp, fci, nullptr, fdpos, lane,
+                     [&](int j, double, bool, bool v) { pt_insert(tab, j, v, ovf); });
     // compact the keys (slot order) to the front of the same table, then
     // rank-sort them.  In place is safe: a chunk's 64 slots are all read
     // (one wave-wide load) before its keys are stored, and they go to
@@ -123,12 +209,21 @@ __global__ __launch_bounds__(XS_WAVE *PT_WAVES) void k_pt_symbolic(
       if (lane == 0) cnt[i] = u;
     } else {
       const int off = crp[i];
+      bool piv = false;
       for (int a = lane; a < u; a += XS_WAVE) {
         const int key = keys[a];
         int rank = 0;
         for (int b = 0; b < u; ++b) rank += keys[b] < key;
         cci[off + rank] = key;
         crow[off + rank] = i;
+        if (key == i) {
+          cdpos[i] = off + rank;
+          piv = true;
+        }
+      }
+      if (!__any(piv) && lane == 0) {
+        cdpos[i] = off;
+        atomicOr(ovf, 2);
       }
     }
   }
@@ -173,7 +268,7 @@ __device__ __forceinline__ double pt_clamp(double d) {
   return d;
 }
 
-// pivot positions of a sorted CSR pattern
+// pivot positions of a sorted CSR pattern (*bad |= 2 for a row without)
 __global__ void k_pt_dpos(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                           int *__restrict__ dpos, int *bad) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -181,216 +276,326 @@ __global__ void k_pt_dpos(int n, const int *__restrict__ rp, const int *__restri
   const int a = rp[i], b = rp[i + 1];
   const int p = pt_find(ci, a, b, i);
   if (p < b && ci[p] == i) dpos[i] = p;
-  else { dpos[i] = a; atomicOr(bad, 1); }
+  else { dpos[i] = a; atomicOr(bad, 2); }
 }
 
 // ---------------------------------------------------------------------------
-// Row-wise forms of the embed and the sweep: one wave per pattern row, the
-// row's columns in LDS, and the partial sums S_ij = sum_{k in L_i, k <
-// min(i,j)} l_ik u_kj accumulated in LDS by walking, in ascending k, the
-// strictly upper part of F's row k for every k of row i's L part -- each U
-// row read once (coalesced) per L entry instead of a binary search per
-// product.  Contributions reach S_ij in ascending k, the order of
-// pt_partial, so the results are identical.  Rows longer than PT_MAXR fall
-// back to the per-entry form.
+// Row-wise forms of the embed and the sweep: one wave per pattern row.  The
+// row's columns go into an LDS hash table (multiplicative hash, linear
+// probing, load <= 1/2) whose slots also hold the row's accumulators, so a
+// product reaches its accumulator in one or two probes.  The partial sums
+// S_ij = sum_{k in L_i, k < min(i,j)} l_ik u_kj come from pt_stream over the
+// strictly upper parts of the rows k of L_i: products outside the row's
+// pattern are dropped, the others added to their slots with LDS atomics (two
+// products of one step may share a slot).  Rows longer than MAXR fall back to
+// the per-entry form.  Rows are spread over the XCDs in contiguous chunks, so
+// the rows k a chunk reads stay in one XCD's L2.
 // ---------------------------------------------------------------------------
-#define PT_ROWW 4
-#define PT_MAXR 256
-#define PT_RTAB 512           // slots of a row's column table (load <= 1/2)
+constexpr int pt_log2(int v) { return v <= 1 ? 0 : 1 + pt_log2(v >> 1); }
 
-// Column table of a pattern row held in LDS: slot t + 1 of cols[t] at the
-// multiplicative hash of cols[t], linear probing, 0 = empty.  A lookup takes
-// one or two probes where a binary search over the row takes log2(len) + 1
-// dependent LDS reads.
+template <int RTAB>
 __device__ __forceinline__ unsigned pt_rhash(int c) {
-  return ((unsigned)c * 2654435761u) >> (32 - 9);
-}
-static_assert(PT_RTAB == 512 && 2 * PT_MAXR <= PT_RTAB, "pt_rhash yields 9 bits");
-
-__device__ __forceinline__ void pt_row_table(const int *cols, int len, int *tab, int lane) {
-  for (int q = lane; q < PT_RTAB; q += XS_WAVE) tab[q] = 0;
-  for (int t = lane; t < len; t += XS_WAVE) {
-    unsigned h = pt_rhash(cols[t]);
-    // the table has room for every column (len <= PT_MAXR = PT_RTAB / 2)
-    while (atomicCAS(&tab[h], 0, t + 1) != 0) h = (h + 1) & (PT_RTAB - 1);
-  }
+  return ((unsigned)c * 2654435761u) >> (32 - pt_log2(RTAB));
 }
 
-// slot of column j in the row, or -1
-__device__ __forceinline__ int pt_row_slot(const int *cols, const int *tab, int j) {
-  unsigned h = pt_rhash(j);
+// Slot of column j, inserted if absent (when v; -1 otherwise).  The table
+// has room (len <= RTAB / 2).  One CAS per lane; the probe loop runs only
+// when some lane's first slot holds another column.
+template <int RTAB>
+__device__ int pt_slot_insert_from(int *key, int j, unsigned h) {
   for (;;) {
-    const int t = tab[h];
-    if (t == 0) return -1;
-    if (cols[t - 1] == j) return t - 1;
-    h = (h + 1) & (PT_RTAB - 1);
+    const int old = atomicCAS(&key[h], PT_EMPTY, j);
+    if (old == PT_EMPTY || old == j) return (int)h;
+    h = (h + 1) & (RTAB - 1);
   }
 }
 
-__device__ __forceinline__ void pt_row_partial(int i, const int *__restrict__ frp,
-                                               const int *__restrict__ fci,
-                                               const double *__restrict__ fv,
-                                               const int *__restrict__ fdpos,
-                                               const int *cols, int *tab, double *S, int len,
-                                               int lane) {
-  constexpr int D = 4;   // U rows whose first 64 entries are loaded together
-  for (int t = lane; t < len; t += XS_WAVE) S[t] = 0.0;
-  pt_row_table(cols, len, tab, lane);
-  const int lb = frp[i], le = fdpos[i];
-  for (int c0 = lb; c0 < le; c0 += XS_WAVE) {
-    // up to 64 L entries of the row at once: lane t holds entry c0 + t, its
-    // value and the bounds of the strict upper part of row k = its column
-    const int cnt = min(XS_WAVE, le - c0);
-    int ub = 0, ue = 0;
-    double lv = 0.0;
-    if (lane < cnt) {
-      const int k = fci[c0 + lane];
-      lv = fv[c0 + lane];
-      ub = fdpos[k] + 1;
-      ue = frp[k + 1];
-    }
-    // groups of D rows; the next group's first 64 entries are loaded while
-    // the current group's products are added
-    int qb[D], qe[D], jj[D], nqb[D], nqe[D], njj[D];
-    double lk[D], uv[D], nlk[D], nuv[D];
-    auto load_group = [&](int g0, int (&gb)[D], int (&ge)[D], double (&gl)[D], int (&gj)[D],
-                          double (&gu)[D]) {
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        const bool ok = g0 + d < cnt;
-        gb[d] = ok ? __shfl(ub, g0 + d, XS_WAVE) : 0;
-        ge[d] = ok ? __shfl(ue, g0 + d, XS_WAVE) : 0;
-        gl[d] = ok ? __shfl(lv, g0 + d, XS_WAVE) : 0.0;
-      }
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        const int q = gb[d] + lane;
-        gj[d] = q < ge[d] ? fci[q] : -1;
-        gu[d] = q < ge[d] ? fv[q] : 0.0;
-      }
-    };
-    load_group(0, qb, qe, lk, jj, uv);
-    for (int g0 = 0; g0 < cnt; g0 += D) {
-      if (g0 + D < cnt) load_group(g0 + D, nqb, nqe, nlk, njj, nuv);
-      // updates strictly in ascending k: all of row d (first chunk from
-      // registers, the rest from memory) before row d + 1
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        for (int q0 = qb[d]; q0 < qe[d]; q0 += XS_WAVE) {
-          int j;
-          double u;
-          if (q0 == qb[d]) { j = jj[d]; u = uv[d]; }
-          else {
-            const int q = q0 + lane;
-            j = q < qe[d] ? fci[q] : -1;
-            u = q < qe[d] ? fv[q] : 0.0;
-          }
-          if (j < 0) continue;
-          const int t = pt_row_slot(cols, tab, j);
-          if (t >= 0) S[t] += lk[d] * u;
-        }
-      }
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        qb[d] = nqb[d]; qe[d] = nqe[d]; jj[d] = njj[d]; lk[d] = nlk[d]; uv[d] = nuv[d];
-      }
-    }
+template <int RTAB>
+__device__ __forceinline__ int pt_slot_insert_if(int *key, int j, bool v) {
+  const unsigned h = pt_rhash<RTAB>(j);
+  int old = PT_EMPTY;
+  if (v) old = atomicCAS(&key[h], PT_EMPTY, j);
+  const bool more = v && old != PT_EMPTY && old != j;
+  int r = v ? (int)h : -1;
+  if (__any(more))
+    if (more) r = pt_slot_insert_from<RTAB>(key, j, (h + 1) & (RTAB - 1));
+  return r;
+}
+
+template <int RTAB>
+__device__ __forceinline__ int pt_slot_insert(int *key, int j) {
+  return pt_slot_insert_if<RTAB>(key, j, true);
+}
+
+// Slot of column j, or -1.  The first two probes are read together; the
+// probe loop runs only when some lane found two other columns there.
+template <int RTAB>
+__device__ int pt_slot_find_from(const int *key, int j, unsigned h) {
+  for (;;) {
+    const int k = key[h];
+    if (k == j) return (int)h;
+    if (k == PT_EMPTY) return -1;
+    h = (h + 1) & (RTAB - 1);
   }
 }
 
-// a_ij of the scaled matrix on the row's slots (0 where A has no entry):
-// the value pt_a returns, without a search per entry
-__device__ __forceinline__ void pt_row_scatter_a(int i, const int *__restrict__ arp,
-                                                 const int *__restrict__ aci,
-                                                 const double *__restrict__ av,
-                                                 const double *__restrict__ sc, const int *cols,
-                                                 const int *tab, double *Av, int len, int lane) {
-  for (int t = lane; t < len; t += XS_WAVE) Av[t] = 0.0;
-  for (int e = arp[i] + lane; e < arp[i + 1]; e += XS_WAVE) {
-    const int j = aci[e];
-    const int t = pt_row_slot(cols, tab, j);
-    if (t >= 0) Av[t] = av[e] * sc[i] * sc[j];
-  }
+template <int RTAB>
+__device__ __forceinline__ int pt_slot_find(const int *key, int j) {
+  const unsigned h = pt_rhash<RTAB>(j), h1 = (h + 1) & (RTAB - 1);
+  const int k0 = key[h], k1 = key[h1];
+  // an empty first slot ends the search (no deletions: j was never placed
+  // beyond it)
+  int r = k0 == j ? (int)h : (k0 != PT_EMPTY && k1 == j) ? (int)h1 : -1;
+  const bool more = k0 != j && k0 != PT_EMPTY && k1 != j && k1 != PT_EMPTY;
+  if (__any(more))
+    if (more) r = pt_slot_find_from<RTAB>(key, j, (h + 2) & (RTAB - 1));
+  return r;
 }
 
-__global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_sweep_rows(
+// rows of at most MAXR entries per wave and waves per workgroup (12 bytes of
+// LDS per slot, two slots per entry)
+template <int MAXR> struct pt_rows { static constexpr int W = MAXR <= 256 ? 4 : 2; };
+
+// One sweep over a pattern.  a_ij enters after the products: each slot holds
+// sum l_ik u_kj, the lanes of A's entries subtract a_ij from their slots and
+// every value reads the negation -- a_ij - S_ij and 0 - S_ij exactly (IEEE
+// rounding is symmetric), without an LDS array for A.  The loads that do not
+// depend on LDS (the row's columns, the pivots u_jj of its L part, A's first
+// 64 entries) are issued together at the start.
+template <int MAXR>
+__global__ __launch_bounds__(XS_WAVE * pt_rows<MAXR>::W) void k_pt_sweep_rows(
     int n, const int *__restrict__ rp, const int *__restrict__ ci, const int *__restrict__ dpos,
     const double *__restrict__ vin, double *__restrict__ vout, const int *__restrict__ arp,
     const int *__restrict__ aci, const double *__restrict__ av, const double *__restrict__ sc) {
-  __shared__ int cols_s[PT_ROWW][PT_MAXR];
-  __shared__ int tab_s[PT_ROWW][PT_RTAB];
-  __shared__ double S_s[PT_ROWW][PT_MAXR];
-  __shared__ double A_s[PT_ROWW][PT_MAXR];
+  constexpr int W = pt_rows<MAXR>::W, RTAB = 2 * MAXR, NM = MAXR / XS_WAVE;
+  __shared__ int key_s[W][RTAB];
+  __shared__ double S_s[W][RTAB];
   const int w = threadIdx.x / XS_WAVE, lane = threadIdx.x & (XS_WAVE - 1);
-  const int i = blockIdx.x * PT_ROWW + w;
+  const int i = xcd_chunk(blockIdx.x, gridDim.x) * W + w;
   if (i >= n) return;   // wave-uniform, no workgroup barrier below
   const int a = rp[i], len = rp[i + 1] - a;
-  int *cols = cols_s[w];
-  double *S = S_s[w], *Av = A_s[w];
-  const bool rows = len <= PT_MAXR;
-  if (rows) {
-    for (int t = lane; t < len; t += XS_WAVE) cols[t] = ci[a + t];
-    pt_row_partial(i, rp, ci, vin, dpos, cols, tab_s[w], S, len, lane);
-    pt_row_scatter_a(i, arp, aci, av, sc, cols, tab_s[w], Av, len, lane);
+  if (len > MAXR) {
+    for (int t = lane; t < len; t += XS_WAVE) {
+      const int e = a + t, j = ci[e];
+      const double s = pt_a(arp, aci, av, sc, i, j) - pt_partial(rp, ci, vin, dpos, i, j);
+      double r;
+      if (i > j) r = s / pt_clamp(vin[dpos[j]]);
+      else r = (i == j) ? pt_clamp(s) : s;
+      vout[e] = isfinite(r) ? r : vin[e];
+    }
+    return;
   }
-  for (int t = lane; t < len; t += XS_WAVE) {
-    const int e = a + t, j = ci[e];
-    const double part = rows ? S[t] : pt_partial(rp, ci, vin, dpos, i, j);
-    const double s = (rows ? Av[t] : pt_a(arp, aci, av, sc, i, j)) - part;
-    double r;
-    if (i > j) r = s / pt_clamp(vin[dpos[j]]);
-    else r = (i == j) ? pt_clamp(s) : s;
-    vout[e] = isfinite(r) ? r : vin[e];
+  const int nl = dpos[i] - a, ab = arp[i], ae = arp[i + 1];
+  const double si = sc[i];
+  int cj[NM];
+  double piv[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const int t = lane + m * XS_WAVE;
+    cj[m] = t < len ? ci[a + t] : -1;
+  }
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const int t = lane + m * XS_WAVE;
+    piv[m] = t < nl ? vin[dpos[cj[m]]] : 1.0;
+  }
+  const int aj = ab + lane < ae ? aci[ab + lane] : -1;
+  const double aval = aj >= 0 ? av[ab + lane] * si * sc[aj] : 0.0;
+  int *key = key_s[w];
+  double *S = S_s[w];
+  for (int q = lane; q < RTAB; q += XS_WAVE) key[q] = PT_EMPTY;
+  int hs[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    hs[m] = -1;
+    if (cj[m] >= 0) {
+      const int h = pt_slot_insert<RTAB>(key, cj[m]);
+      S[h] = 0.0;
+      hs[m] = h;
+    }
+  }
+  const int ha = aj >= 0 ? pt_slot_find<RTAB>(key, aj) : -1;
+  pt_stream<true>(a, a + nl, 1, rp, ci, vin, dpos, lane, [&](int j, double x, bool, bool v) {
+    const int h = pt_slot_find<RTAB>(key, j);
+    if (v && h >= 0) atomicAdd(&S[h], x);
+  });
+  if (ha >= 0) S[ha] = S[ha] - aval;
+  for (int e = ab + XS_WAVE + lane; e < ae; e += XS_WAVE) {   // rows of A beyond 64 entries
+    const int j = aci[e];
+    const int h = pt_slot_find<RTAB>(key, j);
+    if (h >= 0) S[h] = S[h] - av[e] * si * sc[j];
+  }
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    if (cj[m] >= 0) {
+      const int e = a + lane + m * XS_WAVE, j = cj[m];
+      const double s = -S[hs[m]];
+      double r;
+      if (i > j) r = s / pt_clamp(piv[m]);
+      else r = (i == j) ? pt_clamp(s) : s;
+      vout[e] = isfinite(r) ? r : vin[e];
+    }
   }
 }
 
-__global__ __launch_bounds__(XS_WAVE * PT_ROWW) void k_pt_embed_rows(
-    int n, const int *__restrict__ crp, const int *__restrict__ cci, const int *__restrict__ arp,
-    const int *__restrict__ aci, const double *__restrict__ av, const double *__restrict__ sc,
-    const int *__restrict__ frp, const int *__restrict__ fci, const double *__restrict__ fv,
-    const int *__restrict__ fdpos, double *__restrict__ cv) {
-  __shared__ int cols_s[PT_ROWW][PT_MAXR];
-  __shared__ int tab_s[PT_ROWW][PT_RTAB];
-  __shared__ double S_s[PT_ROWW][PT_MAXR];
-  __shared__ double A_s[PT_ROWW][PT_MAXR];
-  __shared__ int F_s[PT_ROWW][PT_MAXR];
-  const int w = threadIdx.x / XS_WAVE, lane = threadIdx.x & (XS_WAVE - 1);
-  const int i = blockIdx.x * PT_ROWW + w;
+// The embed for the rows of more than minlen candidates (pt_fill_embed takes
+// the others): per entry, the old factor's value where it has the entry,
+// else the residual form with binary searches (embedA_intoB_L2 / _U2,
+// hip/Precond/parilut.cpp).  A wave per row.
+__global__ __launch_bounds__(XS_BLOCK) void k_pt_embed_long(
+    int n, int minlen, const int *__restrict__ crp, const int *__restrict__ cci,
+    const int *__restrict__ arp, const int *__restrict__ aci, const double *__restrict__ av,
+    const double *__restrict__ sc, const int *__restrict__ frp, const int *__restrict__ fci,
+    const double *__restrict__ fv, const int *__restrict__ fdpos, double *__restrict__ cv) {
+  const int lane = threadIdx.x & (XS_WAVE - 1);
+  const int i = blockIdx.x * (XS_BLOCK / XS_WAVE) + threadIdx.x / XS_WAVE;
   if (i >= n) return;
   const int a = crp[i], len = crp[i + 1] - a;
-  int *cols = cols_s[w], *Fp = F_s[w];
-  double *S = S_s[w], *Av = A_s[w];
-  const bool rows = len <= PT_MAXR;
+  if (len <= minlen) return;
   const int fa = frp[i], fb = frp[i + 1];
-  if (rows) {
-    for (int t = lane; t < len; t += XS_WAVE) cols[t] = cci[a + t];
-    pt_row_partial(i, frp, fci, fv, fdpos, cols, tab_s[w], S, len, lane);
-    pt_row_scatter_a(i, arp, aci, av, sc, cols, tab_s[w], Av, len, lane);
-    // positions of the old factor's entries of row i on the row's slots
-    for (int t = lane; t < len; t += XS_WAVE) Fp[t] = -1;
-    for (int e = fa + lane; e < fb; e += XS_WAVE) {
-      const int t = pt_row_slot(cols, tab_s[w], fci[e]);
-      if (t >= 0) Fp[t] = e;
-    }
-  }
   for (int t = lane; t < len; t += XS_WAVE) {
     const int j = cci[a + t];
-    int p;
-    if (rows) {
-      p = Fp[t];
-    } else {
-      p = pt_find(fci, fa, fb, j);
-      if (!(p < fb && fci[p] == j)) p = -1;
-    }
+    const int p = pt_find(fci, fa, fb, j);
     double val;
-    if (p >= 0) {
+    if (p < fb && fci[p] == j) {
       val = fv[p];
     } else {
-      const double part = rows ? S[t] : pt_partial(frp, fci, fv, fdpos, i, j);
-      const double r = (rows ? Av[t] : pt_a(arp, aci, av, sc, i, j)) - part;
+      const double r = pt_a(arp, aci, av, sc, i, j) - pt_partial(frp, fci, fv, fdpos, i, j);
       val = (i > j) ? r / pt_clamp(fv[fdpos[j]]) : r;
     }
     cv[a + t] = isfinite(val) ? val : 0.0;
+  }
+}
+
+// Candidate pattern and embedded values in one pass, for rows of at most
+// MAXR candidates (their count from pass 1 of k_pt_symbolic): the row's
+// columns -- A's, the old factor's and the union of the upper parts with
+// pivots of the rows k in L_i -- go into an LDS hash table whose slots
+// accumulate S_ij = sum_{k in L_i, k < min(i,j)} l_ik u_kj from the same
+// product stream (pivots excluded).  A's lanes subtract a_ij, every slot
+// takes r = a_ij - S_ij as the negation (exact), then (r / u_jj) below the
+// diagonal and r on and above it, and the old factor's lanes put back their
+// values (embedA_intoB_L2 / _U2, hip/Precond/parilut.cpp).  The keys are
+// compacted with their values and rank-sorted into cci / cv, with crow and
+// the pivot position (*ovf |= 2 for a row without its diagonal).
+template <int MAXR>
+__global__ __launch_bounds__(XS_WAVE * pt_rows<MAXR>::W) void k_pt_fill_embed(
+    int n, const int *__restrict__ arp, const int *__restrict__ aci, const double *__restrict__ av,
+    const double *__restrict__ sc, const int *__restrict__ frp, const int *__restrict__ fci,
+    const double *__restrict__ fv, const int *__restrict__ fdpos, const int *__restrict__ crp,
+    int *__restrict__ cci, int *__restrict__ crow, double *__restrict__ cv,
+    int *__restrict__ cdpos, int *ovf) {
+  constexpr int W = pt_rows<MAXR>::W, RTAB = 2 * MAXR;
+  __shared__ int key_s[W][RTAB];
+  __shared__ double S_s[W][RTAB];
+  const int w = threadIdx.x / XS_WAVE, lane = threadIdx.x & (XS_WAVE - 1);
+  const int i = xcd_chunk(blockIdx.x, gridDim.x) * W + w;
+  if (i >= n) return;   // wave-uniform, no workgroup barrier below
+  const int off = crp[i], u = crp[i + 1] - off;
+  if (u > MAXR) return;  // k_pt_symbolic<true> + k_pt_embed_long
+  const int ab = arp[i], ae = arp[i + 1], fa = frp[i], fb = frp[i + 1], fd = fdpos[i];
+  const double si = sc[i];
+  // A's and the old factor's first 64 entries of the row, loaded together
+  const int aj = ab + lane < ae ? aci[ab + lane] : -1;
+  const double aval = aj >= 0 ? av[ab + lane] * si * sc[aj] : 0.0;
+  const int fj = fa + lane < fb ? fci[fa + lane] : -1;
+  const double fval = fj >= 0 ? fv[fa + lane] : 0.0;
+  int *key = key_s[w];
+  double *S = S_s[w];
+  for (int q = lane; q < RTAB; q += XS_WAVE) {
+    key[q] = PT_EMPTY;
+    S[q] = 0.0;
+  }
+  const int ha = aj >= 0 ? pt_slot_insert<RTAB>(key, aj) : -1;
+  for (int e = ab + XS_WAVE + lane; e < ae; e += XS_WAVE) pt_slot_insert<RTAB>(key, aci[e]);
+  const int hf = fj >= 0 ? pt_slot_insert<RTAB>(key, fj) : -1;
+  for (int e = fa + XS_WAVE + lane; e < fb; e += XS_WAVE) pt_slot_insert<RTAB>(key, fci[e]);
+  pt_stream<true>(fa, fd, 0, frp, fci, fv, fdpos, lane,
+                  [&](int j, double x, bool first, bool v) {
+                    const int h = pt_slot_insert_if<RTAB>(key, j, v);
+                    if (v && !first) atomicAdd(&S[h], x);
+                  });
+  // S_ij - a_ij on A's slots; every value then reads the negation
+  if (ha >= 0) S[ha] = S[ha] - aval;
+  for (int e = ab + XS_WAVE + lane; e < ae; e += XS_WAVE) {
+    const int j = aci[e];
+    const int h = pt_slot_find<RTAB>(key, j);
+    S[h] = S[h] - av[e] * si * sc[j];
+  }
+  for (int q = lane; q < RTAB; q += XS_WAVE) {
+    const int j = key[q];
+    if (j == PT_EMPTY) continue;
+    const double r = -S[q];
+    const double val = (i > j) ? r / pt_clamp(fv[fdpos[j]]) : r;
+    S[q] = isfinite(val) ? val : 0.0;
+  }
+  // the old factor's entries keep their values
+  if (hf >= 0) S[hf] = isfinite(fval) ? fval : 0.0;
+  for (int e = fa + XS_WAVE + lane; e < fb; e += XS_WAVE) {
+    const int h = pt_slot_find<RTAB>(key, fci[e]);
+    const double x = fv[e];
+    S[h] = isfinite(x) ? x : 0.0;
+  }
+  // compact keys and values to the front of their tables (in place: a
+  // chunk's slots are all read before any of its entries is stored, at
+  // positions <= q0 + lane), then rank-sort
+  int base = 0;
+  for (int q0 = 0; q0 < RTAB; q0 += XS_WAVE) {
+    const int c = key[q0 + lane];
+    const double v = S[q0 + lane];
+    const bool has = c != PT_EMPTY;
+    const unsigned long long m = __ballot(has);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (has) {
+      key[base + before] = c;
+      S[base + before] = v;
+    }
+    base += __popcll(m);
+  }
+  if (base != u) {   // the count pass saw another set: leave the row alone
+    if (lane == 0) atomicOr(ovf, 1);
+    return;
+  }
+  bool piv = false;
+  for (int q = lane; q < base; q += XS_WAVE) {
+    const int kk = key[q];
+    int rank = 0;
+    for (int b = 0; b < base; ++b) rank += key[b] < kk;
+    cci[off + rank] = kk;
+    crow[off + rank] = i;
+    cv[off + rank] = S[q];
+    if (kk == i) {
+      cdpos[i] = off + rank;
+      piv = true;
+    }
+  }
+  if (!__any(piv) && lane == 0) {
+    cdpos[i] = off;
+    atomicOr(ovf, 2);
+  }
+}
+
+// row-length statistics of a pattern from its per-row counts: stats[0] the
+// longest row, stats[1 + b] the rows longer than 128 << b (b < 3)
+__global__ __launch_bounds__(XS_BLOCK) void k_pt_len_stats(int n, const int *__restrict__ cnt,
+                                                           unsigned *stats) {
+  unsigned mx = 0, o[3] = {0, 0, 0};
+  for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) {
+    const unsigned c = (unsigned)cnt[i];
+    mx = max(mx, c);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) o[b] += c > (128u << b);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    mx = max(mx, (unsigned)__shfl_xor((int)mx, off, XS_WAVE));
+#pragma unroll
+    for (int b = 0; b < 3; ++b) o[b] += __shfl_xor(o[b], off, XS_WAVE);
+  }
+  if ((threadIdx.x & (XS_WAVE - 1)) == 0) {
+    atomicMax(&stats[0], mx);
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      if (o[b]) atomicAdd(&stats[1 + b], o[b]);
   }
 }
 
@@ -535,21 +740,35 @@ __global__ __launch_bounds__(256) void k_pt_pick(int shift, unsigned long long *
 
 // entries of L with its unit diagonal (col <= row) -> cnt2[0] and of U with
 // the pivots (col >= row) -> cnt2[1]: the counts the selection ranks over
-// (hip/Precond/parilut.cpp:35-63 counts each factor's diagonal)
-__global__ __launch_bounds__(XS_BLOCK) void k_pt_count_parts(int nnz, const int *__restrict__ row,
-                                                             const int *__restrict__ ci,
+// (hip/Precond/parilut.cpp:35-63 counts each factor's diagonal).  From the
+// pivot positions of a sorted pattern whose rows all hold their diagonal:
+// row i has dpos_i - rp_i + 1 entries with col <= i and rp_{i+1} - dpos_i
+// with col >= i.  One atomic pair per workgroup.
+__global__ __launch_bounds__(XS_BLOCK) void k_pt_count_parts(int n, const int *__restrict__ rp,
+                                                             const int *__restrict__ dpos,
                                                              unsigned long long *cnt2) {
+  __shared__ unsigned long long red[2][XS_BLOCK / XS_WAVE];
   unsigned long long a = 0, b = 0;
-  for (int e = blockIdx.x * XS_BLOCK + threadIdx.x; e < nnz; e += gridDim.x * XS_BLOCK) {
-    const int i = row[e], j = ci[e];
-    a += j <= i;
-    b += j >= i;
+  for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) {
+    const int d = dpos[i];
+    a += (unsigned long long)(d - rp[i] + 1);
+    b += (unsigned long long)(rp[i + 1] - d);
   }
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_xor(a, off, XS_WAVE);
     b += __shfl_xor(b, off, XS_WAVE);
   }
+  const int w = threadIdx.x / XS_WAVE;
   if ((threadIdx.x & (XS_WAVE - 1)) == 0) {
+    red[0][w] = a;
+    red[1][w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < XS_BLOCK / XS_WAVE; ++q) {
+      a += red[0][q];
+      b += red[1][q];
+    }
     atomicAdd(&cnt2[0], a);
     atomicAdd(&cnt2[1], b);
   }
@@ -587,13 +806,13 @@ __global__ __launch_bounds__(XS_BLOCK) void k_pt_keep_write(
     int n, const int *__restrict__ crp, const int *__restrict__ cci,
     const double *__restrict__ cv, const double *thr, const double *__restrict__ dtol,
     const double *__restrict__ sc, const int *__restrict__ frp, int *__restrict__ fci,
-    double *__restrict__ fv, int *__restrict__ frow) {
+    double *__restrict__ fv, int *__restrict__ frow, int *__restrict__ fdpos, int *bad) {
   const int lane = threadIdx.x & (XS_WAVE - 1);
   const int i = blockIdx.x * (XS_BLOCK / XS_WAVE) + threadIdx.x / XS_WAVE;
   if (i >= n) return;
   const double tl = thr[0], tu = thr[1], dt = dtol[i];
   const unsigned long long below = (1ull << lane) - 1ull;
-  int w = frp[i];
+  int w = frp[i], dp = -1;
   const int re = crp[i + 1];
   for (int e0 = crp[i]; e0 < re; e0 += XS_WAVE) {
     const int e = e0 + lane;
@@ -606,13 +825,20 @@ __global__ __launch_bounds__(XS_BLOCK) void k_pt_keep_write(
       keep = pt_keep(i, j, x, tl, tu, dt, sc);
     }
     const unsigned long long m = __ballot(keep);
+    const int q = w + __popcll(m & below);
     if (keep) {
-      const int q = w + __popcll(m & below);
       fci[q] = j;
       fv[q] = x;
       frow[q] = i;
     }
+    // the pivot (always kept): its position, wave-uniform
+    const unsigned long long md = __ballot(keep && j == i);
+    if (md) dp = w + __popcll(m & ((1ull << (__ffsll((long long)md) - 1)) - 1ull));
     w += __popcll(m);
+  }
+  if (lane == 0) {
+    fdpos[i] = dp >= 0 ? dp : frp[i];
+    if (dp < 0) atomicOr(bad, 2);
   }
 }
 
@@ -719,32 +945,73 @@ void xs_scan_excl(hipStream_t s, int n, const int *in, int *out, int *bsum) {
 }
 int xs_scan_bsum_len(int n) { return (n + SCAN_B - 1) / SCAN_B + 1; }
 
-void xs_pt_symbolic(hipStream_t s, bool fill, int n, const int *arp, const int *aci,
-                    const int *frp, const int *fci, const int *fdpos, int *cnt, const int *crp,
-                    int *cci, int *crow, int *ovf) {
+void xs_pt_symbolic(hipStream_t s, int n, const int *arp, const int *aci, const int *frp,
+                    const int *fci, const int *fdpos, int *cnt, int *ovf) {
   dim3 grid((n + PT_WAVES - 1) / PT_WAVES), blk(XS_WAVE * PT_WAVES);
-  if (fill)
-    hipLaunchKernelGGL(k_pt_symbolic<true>, grid, blk, 0, s, n, arp, aci, frp, fci, fdpos, cnt, crp, cci, crow, ovf);
-  else
-    hipLaunchKernelGGL(k_pt_symbolic<false>, grid, blk, 0, s, n, arp, aci, frp, fci, fdpos, cnt, crp, cci, crow, ovf);
+  hipLaunchKernelGGL(k_pt_symbolic<false>, grid, blk, 0, s, n, arp, aci, frp, fci, fdpos, cnt,
+                     nullptr, nullptr, nullptr, nullptr, 0, ovf);
 }
 
 void xs_pt_dpos(hipStream_t s, int n, const int *rp, const int *ci, int *dpos, int *bad) {
   hipLaunchKernelGGL(k_pt_dpos, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, dpos, bad);
 }
 
-void xs_pt_embed(hipStream_t s, int n, const int *crp, const int *cci, const int *arp,
-                 const int *aci, const double *av, const double *sc, const int *frp,
-                 const int *fci, const double *fv, const int *fdpos, double *cv) {
-  hipLaunchKernelGGL(k_pt_embed_rows, dim3((n + PT_ROWW - 1) / PT_ROWW), dim3(XS_WAVE * PT_ROWW), 0,
-                     s, n, crp, cci, arp, aci, av, sc, frp, fci, fv, fdpos, cv);
+template <int MAXR>
+static dim3 pt_row_grid(int n) {
+  return dim3((n + pt_rows<MAXR>::W - 1) / pt_rows<MAXR>::W);
+}
+
+void xs_pt_fill_embed(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
+                      const double *sc, const int *frp, const int *fci, const double *fv,
+                      const int *fdpos, const int *crp, int *cci, int *crow, double *cv,
+                      int *cdpos, int *ovf, int maxr, bool has_long) {
+#define PT_FILL(M)                                                                              \
+  hipLaunchKernelGGL(k_pt_fill_embed<M>, pt_row_grid<M>(n), dim3(XS_WAVE * pt_rows<M>::W), 0, s, \
+                     n, arp, aci, av, sc, frp, fci, fv, fdpos, crp, cci, crow, cv, cdpos, ovf)
+  if (maxr <= 128) PT_FILL(128);
+  else if (maxr <= 256) PT_FILL(256);
+  else PT_FILL(512);
+#undef PT_FILL
+  if (!has_long) return;
+  const int m = maxr <= 128 ? 128 : maxr <= 256 ? 256 : 512;
+  hipLaunchKernelGGL(k_pt_symbolic<true>, dim3((n + PT_WAVES - 1) / PT_WAVES),
+                     dim3(XS_WAVE * PT_WAVES), 0, s, n, arp, aci, frp, fci, fdpos, nullptr, crp,
+                     cci, crow, cdpos, m, ovf);
+  const int rows = XS_BLOCK / XS_WAVE;
+  hipLaunchKernelGGL(k_pt_embed_long, dim3((n + rows - 1) / rows), dim3(XS_BLOCK), 0, s, n, m, crp,
+                     cci, arp, aci, av, sc, frp, fci, fv, fdpos, cv);
 }
 
 void xs_pt_sweep(hipStream_t s, int n, const int *rp, const int *ci, const int *dpos,
                  const double *vin, double *vout, const int *arp, const int *aci,
-                 const double *av, const double *sc) {
-  hipLaunchKernelGGL(k_pt_sweep_rows, dim3((n + PT_ROWW - 1) / PT_ROWW), dim3(XS_WAVE * PT_ROWW), 0,
-                     s, n, rp, ci, dpos, vin, vout, arp, aci, av, sc);
+                 const double *av, const double *sc, int maxr) {
+#define PT_SWEEP(M)                                                                              \
+  hipLaunchKernelGGL(k_pt_sweep_rows<M>, pt_row_grid<M>(n), dim3(XS_WAVE * pt_rows<M>::W), 0, s, \
+                     n, rp, ci, dpos, vin, vout, arp, aci, av, sc)
+  if (maxr <= 128) PT_SWEEP(128);
+  else if (maxr <= 256) PT_SWEEP(256);
+  else PT_SWEEP(512);
+#undef PT_SWEEP
+}
+
+void xs_pt_len_stats(hipStream_t s, int n, const int *cnt, unsigned *stats) {
+  int g = (n + XS_BLOCK * 8 - 1) / (XS_BLOCK * 8);
+  if (g > 1024) g = 1024;
+  if (g < 1) g = 1;
+  (void)hipMemsetAsync(stats, 0, 4 * sizeof(unsigned), s);
+  hipLaunchKernelGGL(k_pt_len_stats, dim3(g), dim3(XS_BLOCK), 0, s, n, cnt, stats);
+}
+
+bool xs_pt_rows_over(int maxr, const unsigned *stats) {
+  return stats[maxr <= 128 ? 1 : maxr <= 256 ? 2 : 3] != 0;
+}
+
+int xs_pt_pick_rows(int n, const unsigned *stats) {
+  // the smallest table that leaves at most n / 1024 rows to the per-entry form
+  const unsigned few = (unsigned)(n / 1024);
+  if (stats[1] <= few) return 128;
+  if (stats[2] <= few) return 256;
+  return 512;
 }
 
 void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const double *v,
@@ -766,13 +1033,13 @@ void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const d
 int xs_pt_select_words() { return 8; }
 int xs_pt_list_len() { return 2 * PT_SEL_CAP; }
 
-void xs_pt_count_parts(hipStream_t s, int nnz, const int *row, const int *ci,
+void xs_pt_count_parts(hipStream_t s, int n, const int *rp, const int *dpos,
                        unsigned long long *cnt2) {
-  int g = (nnz + XS_BLOCK * 8 - 1) / (XS_BLOCK * 8);
-  if (g > 4096) g = 4096;
+  int g = (n + XS_BLOCK * 8 - 1) / (XS_BLOCK * 8);
+  if (g > 512) g = 512;
   if (g < 1) g = 1;
   (void)hipMemsetAsync(cnt2, 0, 2 * sizeof(unsigned long long), s);
-  hipLaunchKernelGGL(k_pt_count_parts, dim3(g), dim3(XS_BLOCK), 0, s, nnz, row, ci, cnt2);
+  hipLaunchKernelGGL(k_pt_count_parts, dim3(g), dim3(XS_BLOCK), 0, s, n, rp, dpos, cnt2);
 }
 
 void xs_pt_dtol(hipStream_t s, int n, const int *arp, const double *av, double droptol,
@@ -782,14 +1049,15 @@ void xs_pt_dtol(hipStream_t s, int n, const int *arp, const double *av, double d
 
 void xs_pt_keep(hipStream_t s, bool write, int n, const int *crp, const int *cci,
                 const double *cv, const double *thr, const double *dtol, const double *sc,
-                int *cnt, const int *frp, int *fci, double *fv, int *frow) {
+                int *cnt, const int *frp, int *fci, double *fv, int *frow, int *fdpos,
+                int *bad) {
   const int rows = XS_BLOCK / XS_WAVE, g = (n + rows - 1) / rows;
   if (!write)
     hipLaunchKernelGGL(k_pt_keep_count, dim3(g), dim3(XS_BLOCK), 0, s, n, crp, cci, cv, thr, dtol,
                        sc, cnt);
   else
     hipLaunchKernelGGL(k_pt_keep_write, dim3(g), dim3(XS_BLOCK), 0, s, n, crp, cci, cv, thr, dtol,
-                       sc, frp, fci, fv, frow);
+                       sc, frp, fci, fv, frow, fdpos, bad);
 }
 
 void xs_pt_scale_vec(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
