@@ -1048,14 +1048,14 @@ int xs_parilut_factor(xs_ctx *c) {
   int rc;
   double *dtol = nullptr, *thr = nullptr;
   int *cnt = nullptr, *bsum = nullptr, *ovf = nullptr;
-  unsigned long long *sel = nullptr, *cnt2 = nullptr, *list = nullptr;
+  unsigned long long *sel = nullptr, *cnt2 = nullptr, *list = nullptr, *ckey = nullptr;
   unsigned *hist = nullptr, *lcnt = nullptr, *lstat = nullptr;
   int *crp = nullptr, *cci = nullptr, *crow = nullptr, *cdpos = nullptr;
   double *cv = nullptr, *cv2 = nullptr, *fv2 = nullptr;
   auto cleanup = [&]() {
     dfree(dtol); dfree(thr); dfree(cnt); dfree(bsum); dfree(ovf); dfree(sel); dfree(cnt2);
     dfree(list); dfree(hist); dfree(lcnt); dfree(crp); dfree(cci); dfree(crow); dfree(cdpos);
-    dfree(cv); dfree(cv2); dfree(fv2); dfree(lstat);
+    dfree(cv); dfree(cv2); dfree(fv2); dfree(lstat); dfree(ckey);
   };
 #define PT_TRY(x) do { if ((rc = (x))) { cleanup(); return rc; } } while (0)
   PT_TRY(F.scale ? XS_OK : dalloc(&F.scale, n));
@@ -1129,26 +1129,27 @@ int xs_parilut_factor(xs_ctx *c) {
     if (hdr[1] & 2) { snprintf(xs_errbuf, sizeof(xs_errbuf), "ParILUT: a row without its diagonal"); cleanup(); return XS_ERR_PIVOT; }
     const int nc = hdr[0];
     {
-      size_t c1 = cap_c, c2 = cap_c, c3 = cap_c, c4 = cap_c;
+      size_t c1 = cap_c, c2 = cap_c, c3 = cap_c, c4 = cap_c, c5 = cap_c;
       PT_TRY(grow((size_t)nc, c1, &cci));
       PT_TRY(grow((size_t)nc, c2, &crow));
       PT_TRY(grow((size_t)nc, c3, &cv));
       PT_TRY(grow((size_t)nc, c4, &cv2));
+      PT_TRY(grow((size_t)nc, c5, &ckey));
       cap_c = c1;
     }
     // 2-3. the candidates with the old values embedded, and a sweep over them
     xs_pt_fill_embed(s, n, c->A.rp, c->A.ci, c->A.v, F.scale, F.rp_own, F.ci_own, F.lu, F.dpos,
                      crp, cci, crow, cv, cdpos, ovf, crows, xs_pt_rows_over(crows, cst));
-    xs_pt_sweep(s, n, crp, cci, cdpos, cv, cv2, c->A.rp, c->A.ci, c->A.v, F.scale, crows);
+    xs_pt_sweep(s, n, crp, cci, cdpos, cv, cv2, c->A.rp, c->A.ci, c->A.v, F.scale, crows, ckey);
     std::swap(cv, cv2);
     // 4. thresholds (the part counts stay on the device); nothing is
     // dropped by rank while the candidates number at most round(rate nnz(A))
     // (hip/Precond/parilut.cpp:926-931)
     if ((long)nc - std::lround(rate * (double)nnz) <= 0) keepL = keepU = LONG_MAX;
     xs_pt_count_parts(s, n, crp, cdpos, cnt2);
-    xs_pt_select(s, nc, crow, cci, cv, F.scale, cnt2, keepL, keepU, sel, hist, lcnt, list, thr);
+    xs_pt_select(s, nc, ckey, n, cnt2, keepL, keepU, sel, hist, lcnt, list, thr);
     // compaction into the new factor
-    xs_pt_keep(s, false, n, crp, cci, cv, thr, dtol, F.scale, cnt, nullptr, nullptr, nullptr,
+    xs_pt_keep(s, false, n, crp, cci, cdpos, cv, ckey, thr, dtol, cnt, nullptr, nullptr, nullptr,
                nullptr, nullptr, nullptr);
     xs_scan_excl(s, n, cnt, F.rp_own, bsum);
     xs_pt_len_stats(s, n, cnt, lstat);
@@ -1162,11 +1163,11 @@ int xs_parilut_factor(xs_ctx *c) {
     PT_TRY(grow((size_t)fnnz, cap_lu, &F.lu));
     PT_TRY(grow((size_t)fnnz, cap_row, &F.rowidx));
     PT_TRY(grow((size_t)fnnz, cap_v2, &fv2));
-    xs_pt_keep(s, true, n, crp, cci, cv, thr, dtol, F.scale, nullptr, F.rp_own, F.ci_own, F.lu,
+    xs_pt_keep(s, true, n, crp, cci, cdpos, cv, ckey, thr, dtol, nullptr, F.rp_own, F.ci_own, F.lu,
                F.rowidx, F.dpos, ovf);
     // 5. sweep over the kept pattern
     xs_pt_sweep(s, n, F.rp_own, F.ci_own, F.dpos, F.lu, fv2, c->A.rp, c->A.ci, c->A.v, F.scale,
-                frows);
+                frows, nullptr);
     std::swap(F.lu, fv2);
     std::swap(cap_lu, cap_v2);
   }

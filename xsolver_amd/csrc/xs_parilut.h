@@ -19,24 +19,26 @@ void xs_pt_fill_embed(hipStream_t s, int n, const int *arp, const int *aci, cons
                       const double *sc, const int *frp, const int *fci, const double *fv,
                       const int *fdpos, const int *crp, int *cci, int *crow, double *cv,
                       int *cdpos, int *ovf, int maxr, bool has_long);
+// keys (may be null): each entry's selection key (bit pattern of the
+// unscaled magnitude, top bit set for U)
 void xs_pt_sweep(hipStream_t s, int n, const int *rp, const int *ci, const int *dpos,
                  const double *vin, double *vout, const int *arp, const int *aci,
-                 const double *av, const double *sc, int maxr);
+                 const double *av, const double *sc, int maxr, unsigned long long *keys);
 // row-length statistics from per-row counts (4 words: longest row, rows
 // longer than 128, 256, 512), the row table they call for and whether some
 // rows exceed it
 void xs_pt_len_stats(hipStream_t s, int n, const int *cnt, unsigned *stats);
 int xs_pt_pick_rows(int n, const unsigned *stats);
 bool xs_pt_rows_over(int maxr, const unsigned *stats);
-// thresholds of both factors on the unscaled magnitudes (sc = the scaling
-// S of the swept values): thr[0] (resp. thr[1]) = the value of rank
-// cnt2[0] - keepL (resp. cnt2[1] - keepU) in ascending magnitude over L with
-// its unit diagonal (U with its pivots), 0 when the factor has no more than
-// keep entries; sel: xs_pt_select_words() words, hist: 512, lcnt: 2, list:
-// xs_pt_list_len()
-void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const double *v,
-                  const double *sc, const unsigned long long *cnt2, long long keepL,
-                  long long keepU, unsigned long long *sel, unsigned *hist, unsigned *lcnt,
+// thresholds of both factors on the unscaled magnitudes from the entries'
+// selection keys (xs_pt_sweep) and nunit unit diagonals of L: thr[0] (resp.
+// thr[1]) = the value of rank cnt2[0] - keepL (resp. cnt2[1] - keepU) in
+// ascending magnitude over L with its unit diagonal (U with its pivots), 0
+// when the factor has no more than keep entries; sel: xs_pt_select_words()
+// words, hist: 512, lcnt: 2, list: xs_pt_list_len()
+void xs_pt_select(hipStream_t s, int nnz, const unsigned long long *keys, int nunit,
+                  const unsigned long long *cnt2, long long keepL, long long keepU,
+                  unsigned long long *sel, unsigned *hist, unsigned *lcnt,
                   unsigned long long *list, double *thr);
 int xs_pt_select_words();
 int xs_pt_list_len();
@@ -49,12 +51,13 @@ void xs_pt_count_parts(hipStream_t s, int n, const int *rp, const int *dpos,
 void xs_pt_dtol(hipStream_t s, int n, const int *arp, const double *av, double droptol,
                 double *dtol);
 // compaction of the candidates (count pass, then write pass) with the
-// thresholds of xs_pt_select and the drop tolerance on unscaled magnitudes;
-// the write pass also places the pivots (fdpos; *bad |= 1 for a row without)
+// thresholds of xs_pt_select and the drop tolerance, on the selection keys;
+// cdpos: the candidates' pivot positions (always kept).  The write pass also
+// places the pivots (fdpos; *bad |= 2 for a row without)
 void xs_pt_keep(hipStream_t s, bool write, int n, const int *crp, const int *cci,
-                const double *cv, const double *thr, const double *dtol, const double *sc,
-                int *cnt, const int *frp, int *fci, double *fv, int *frow, int *fdpos,
-                int *bad);
+                const int *cdpos, const double *cv, const unsigned long long *keys,
+                const double *thr, const double *dtol, int *cnt, const int *frp, int *fci,
+                double *fv, int *frow, int *fdpos, int *bad);
 void xs_pt_scale_vec(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
                      double *sc);
 void xs_pt_rescale(hipStream_t s, int nnz, const int *row, const int *ci, double *v,

@@ -352,11 +352,24 @@ __device__ __forceinline__ int pt_slot_find(const int *key, int j) {
   return r;
 }
 
+// Selection key of entry (i, j) from its scaled value v (s_i = si, s_j =
+// sj): the bit pattern of the unscaled factor's magnitude, |l_ij| =
+// |l~_ij (s_j / s_i)| or |u_ij| = |u~_ij / (s_i s_j)| (non-negative, so the
+// patterns order like the values), with the factor in the top bit (0 for L,
+// j < i; 1 for U, j >= i).
+#define PT_KEY_U (1ull << 63)
+__device__ __forceinline__ unsigned long long pt_key(int i, int j, double v, double si,
+                                                     double sj) {
+  const double m = j < i ? fabs(v * (sj / si)) : fabs(v / (si * sj));
+  return (unsigned long long)__double_as_longlong(m) | (j >= i ? PT_KEY_U : 0ull);
+}
+
 // rows of at most MAXR entries per wave and waves per workgroup (12 bytes of
 // LDS per slot, two slots per entry)
 template <int MAXR> struct pt_rows { static constexpr int W = MAXR <= 256 ? 4 : 2; };
 
-// One sweep over a pattern.  a_ij enters after the products: each slot holds
+// One sweep over a pattern (with keys: also each entry's selection key,
+// pt_key).  a_ij enters after the products: each slot holds
 // sum l_ik u_kj, the lanes of A's entries subtract a_ij from their slots and
 // every value reads the negation -- a_ij - S_ij and 0 - S_ij exactly (IEEE
 // rounding is symmetric), without an LDS array for A.  The loads that do not
@@ -366,7 +379,8 @@ template <int MAXR>
 __global__ __launch_bounds__(XS_WAVE * pt_rows<MAXR>::W) void k_pt_sweep_rows(
     int n, const int *__restrict__ rp, const int *__restrict__ ci, const int *__restrict__ dpos,
     const double *__restrict__ vin, double *__restrict__ vout, const int *__restrict__ arp,
-    const int *__restrict__ aci, const double *__restrict__ av, const double *__restrict__ sc) {
+    const int *__restrict__ aci, const double *__restrict__ av, const double *__restrict__ sc,
+    unsigned long long *__restrict__ keys) {
   constexpr int W = pt_rows<MAXR>::W, RTAB = 2 * MAXR, NM = MAXR / XS_WAVE;
   __shared__ int key_s[W][RTAB];
   __shared__ double S_s[W][RTAB];
@@ -374,6 +388,7 @@ __global__ __launch_bounds__(XS_WAVE * pt_rows<MAXR>::W) void k_pt_sweep_rows(
   const int i = xcd_chunk(blockIdx.x, gridDim.x) * W + w;
   if (i >= n) return;   // wave-uniform, no workgroup barrier below
   const int a = rp[i], len = rp[i + 1] - a;
+  const double si = sc[i];
   if (len > MAXR) {
     for (int t = lane; t < len; t += XS_WAVE) {
       const int e = a + t, j = ci[e];
@@ -381,12 +396,13 @@ __global__ __launch_bounds__(XS_WAVE * pt_rows<MAXR>::W) void k_pt_sweep_rows(
       double r;
       if (i > j) r = s / pt_clamp(vin[dpos[j]]);
       else r = (i == j) ? pt_clamp(s) : s;
-      vout[e] = isfinite(r) ? r : vin[e];
+      const double o = isfinite(r) ? r : vin[e];
+      vout[e] = o;
+      if (keys) keys[e] = pt_key(i, j, o, si, sc[j]);
     }
     return;
   }
   const int nl = dpos[i] - a, ab = arp[i], ae = arp[i + 1];
-  const double si = sc[i];
   int cj[NM];
   double piv[NM];
 #pragma unroll
@@ -394,10 +410,12 @@ __global__ __launch_bounds__(XS_WAVE * pt_rows<MAXR>::W) void k_pt_sweep_rows(
     const int t = lane + m * XS_WAVE;
     cj[m] = t < len ? ci[a + t] : -1;
   }
+  double scj[NM];
 #pragma unroll
   for (int m = 0; m < NM; ++m) {
     const int t = lane + m * XS_WAVE;
     piv[m] = t < nl ? vin[dpos[cj[m]]] : 1.0;
+    scj[m] = (keys && cj[m] >= 0) ? sc[cj[m]] : 1.0;
   }
   const int aj = ab + lane < ae ? aci[ab + lane] : -1;
   const double aval = aj >= 0 ? av[ab + lane] * si * sc[aj] : 0.0;
@@ -433,7 +451,9 @@ __global__ __launch_bounds__(XS_WAVE * pt_rows<MAXR>::W) void k_pt_sweep_rows(
       double r;
       if (i > j) r = s / pt_clamp(piv[m]);
       else r = (i == j) ? pt_clamp(s) : s;
-      vout[e] = isfinite(r) ? r : vin[e];
+      const double o = isfinite(r) ? r : vin[e];
+      vout[e] = o;
+      if (keys) keys[e] = pt_key(i, j, o, si, scj[m]);
     }
   }
 }
@@ -607,8 +627,10 @@ __global__ __launch_bounds__(XS_BLOCK) void k_pt_len_stats(int n, const int *__r
 // diagonals, value 1) and over U (slot 1: its entries with the pivots).  The
 // magnitudes are those of the unscaled factors (the sweeps run on S A S):
 // |l_ij| = |l~_ij (s_j / s_i)|, |u_ij| = |u~_ij / (s_i s_j)|, the expressions
-// the compaction compares against the thresholds.  |v| bit patterns are
-// ordered like the values; passes of 8-bit digits from the most significant.
+// the compaction compares against the thresholds, read from the entries'
+// selection keys (pt_key, written by the sweep over the candidates) plus the
+// n unit diagonals of L, counted at once.  |v| bit patterns are ordered like
+// the values; passes of 8-bit digits from the most significant.
 // Once the bin of rank k holds at most PT_SEL_CAP entries, the next pass
 // collects them and the rank is found among them in one block; later passes
 // return at once.  State per slot p: sel[4p + 0] prefix bits, + 1 remaining
@@ -617,10 +639,6 @@ __global__ __launch_bounds__(XS_BLOCK) void k_pt_len_stats(int n, const int *__r
 #define PT_SEL_CAP 1024
 static_assert(XS_BLOCK == 256, "k_pt_hist clears its 512 bins with two stores per thread");
 
-// magnitude of entry (i, j) of the unscaled factor from its scaled value v
-__device__ __forceinline__ double pt_mag(int i, int j, double v, const double *__restrict__ sc) {
-  return j < i ? fabs(v * (sc[j] / sc[i])) : fabs(v / (sc[i] * sc[j]));
-}
 
 __global__ void k_pt_sel_init(const unsigned long long *__restrict__ cnt2, long long keepL,
                               long long keepU, unsigned long long *sel, unsigned *hist,
@@ -638,10 +656,9 @@ __global__ void k_pt_sel_init(const unsigned long long *__restrict__ cnt2, long 
   }
 }
 
-__global__ __launch_bounds__(XS_BLOCK) void k_pt_hist(int nnz, const int *__restrict__ row,
-                                                      const int *__restrict__ ci,
-                                                      const double *__restrict__ v,
-                                                      const double *__restrict__ sc, int shift,
+__global__ __launch_bounds__(XS_BLOCK) void k_pt_hist(int nnz,
+                                                      const unsigned long long *__restrict__ key,
+                                                      int nunit, int shift,
                                                       const unsigned long long *sel,
                                                       unsigned *__restrict__ hist,
                                                       unsigned *lcnt,
@@ -649,27 +666,35 @@ __global__ __launch_bounds__(XS_BLOCK) void k_pt_hist(int nnz, const int *__rest
   const unsigned long long m0 = sel[3], m1 = sel[7];
   if (m0 == 2 && m1 == 2) return;
   __shared__ unsigned h[512];
+  __shared__ unsigned ubase;
   h[threadIdx.x] = 0;
   h[threadIdx.x + XS_BLOCK] = 0;
   __syncthreads();
   const unsigned long long pre[2] = {sel[0], sel[4]}, msk[2] = {sel[2], sel[6]};
   const unsigned long long mode[2] = {m0, m1};
-  auto consider = [&](int p, double mag) {
-    if (mode[p] == 2) return;
-    const unsigned long long bits = (unsigned long long)__double_as_longlong(mag);
-    if ((bits & msk[p]) != pre[p]) return;
+  for (int e = blockIdx.x * XS_BLOCK + threadIdx.x; e < nnz; e += gridDim.x * XS_BLOCK) {
+    const unsigned long long k = key[e];
+    const int p = (int)(k >> 63);
+    const unsigned long long bits = k & ~PT_KEY_U;
+    if (mode[p] == 2 || (bits & msk[p]) != pre[p]) continue;
     if (mode[p] == 1) {
       const unsigned q = atomicAdd(&lcnt[p], 1u);
       if (q < PT_SEL_CAP) list[p * PT_SEL_CAP + q] = bits;
     } else {
       atomicAdd(&h[p * 256 + ((bits >> shift) & 0xFF)], 1u);
     }
-  };
-  for (int e = blockIdx.x * XS_BLOCK + threadIdx.x; e < nnz; e += gridDim.x * XS_BLOCK) {
-    const int i = row[e], j = ci[e];
-    const double mag = pt_mag(i, j, v[e], sc);
-    if (j == i) consider(0, 1.0);    // L's unit diagonal
-    consider(j < i ? 0 : 1, mag);
+  }
+  // L's nunit unit diagonals (value 1), counted by the first workgroup
+  const unsigned long long one = (unsigned long long)__double_as_longlong(1.0);
+  if (blockIdx.x == 0 && m0 != 2 && (one & msk[0]) == pre[0]) {
+    if (m0 == 1) {
+      if (threadIdx.x == 0) ubase = atomicAdd(&lcnt[0], (unsigned)nunit);
+      __syncthreads();
+      for (int q = threadIdx.x; q < nunit; q += XS_BLOCK)
+        if (ubase + q < PT_SEL_CAP) list[ubase + q] = one;
+    } else if (threadIdx.x == 0) {
+      atomicAdd(&h[(one >> shift) & 0xFF], (unsigned)nunit);
+    }
   }
   __syncthreads();
   for (int q = threadIdx.x; q < 512; q += XS_BLOCK)
@@ -777,40 +802,42 @@ __global__ __launch_bounds__(XS_BLOCK) void k_pt_count_parts(int n, const int *_
 // keep predicate for the compaction
 // (thresold_remove_nnz2, hip/Precond/parilut.cpp:303-323: |v| >= threshold
 // survives, the diagonal always does; plus the drop tolerance on the same
-// unscaled magnitude)
-__device__ __forceinline__ bool pt_keep(int i, int j, double v, double tl, double tu,
-                                        double dt, const double *__restrict__ sc) {
-  if (i == j) return true;
-  const double a = pt_mag(i, j, v, sc);
+// unscaled magnitude), on the selection key of the entry
+__device__ __forceinline__ bool pt_keep(bool diag, unsigned long long k, double tl, double tu,
+                                        double dt) {
+  if (diag) return true;
+  const double a = __longlong_as_double((long long)(k & ~PT_KEY_U));
   if (a < dt) return false;
-  return j < i ? a >= tl : a >= tu;
+  return (k & PT_KEY_U) ? a >= tu : a >= tl;
 }
 
 // per-row kept counts and the compaction: a wave per row, kept entries
 // placed in column order by ballot
 __global__ __launch_bounds__(XS_BLOCK) void k_pt_keep_count(
-    int n, const int *__restrict__ crp, const int *__restrict__ cci,
-    const double *__restrict__ cv, const double *thr, const double *__restrict__ dtol,
-    const double *__restrict__ sc, int *__restrict__ cnt) {
+    int n, const int *__restrict__ crp, const int *__restrict__ cdpos,
+    const unsigned long long *__restrict__ key, const double *thr,
+    const double *__restrict__ dtol, int *__restrict__ cnt) {
   const int lane = threadIdx.x & (XS_WAVE - 1);
   const int i = blockIdx.x * (XS_BLOCK / XS_WAVE) + threadIdx.x / XS_WAVE;
   if (i >= n) return;
   const double tl = thr[0], tu = thr[1], dt = dtol[i];
+  const int dp = cdpos[i];
   int c = 0;
-  for (int e = crp[i] + lane; e < crp[i + 1]; e += XS_WAVE) c += pt_keep(i, cci[e], cv[e], tl, tu, dt, sc);
+  for (int e = crp[i] + lane; e < crp[i + 1]; e += XS_WAVE) c += pt_keep(e == dp, key[e], tl, tu, dt);
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, XS_WAVE);
   if (lane == 0) cnt[i] = c;
 }
 
 __global__ __launch_bounds__(XS_BLOCK) void k_pt_keep_write(
-    int n, const int *__restrict__ crp, const int *__restrict__ cci,
-    const double *__restrict__ cv, const double *thr, const double *__restrict__ dtol,
-    const double *__restrict__ sc, const int *__restrict__ frp, int *__restrict__ fci,
+    int n, const int *__restrict__ crp, const int *__restrict__ cci, const int *__restrict__ cdpos,
+    const double *__restrict__ cv, const unsigned long long *__restrict__ key, const double *thr,
+    const double *__restrict__ dtol, const int *__restrict__ frp, int *__restrict__ fci,
     double *__restrict__ fv, int *__restrict__ frow, int *__restrict__ fdpos, int *bad) {
   const int lane = threadIdx.x & (XS_WAVE - 1);
   const int i = blockIdx.x * (XS_BLOCK / XS_WAVE) + threadIdx.x / XS_WAVE;
   if (i >= n) return;
   const double tl = thr[0], tu = thr[1], dt = dtol[i];
+  const int cd = cdpos[i];
   const unsigned long long below = (1ull << lane) - 1ull;
   int w = frp[i], dp = -1;
   const int re = crp[i + 1];
@@ -822,7 +849,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_pt_keep_write(
     if (e < re) {
       j = cci[e];
       x = cv[e];
-      keep = pt_keep(i, j, x, tl, tu, dt, sc);
+      keep = pt_keep(e == cd, key[e], tl, tu, dt);
     }
     const unsigned long long m = __ballot(keep);
     const int q = w + __popcll(m & below);
@@ -984,10 +1011,10 @@ void xs_pt_fill_embed(hipStream_t s, int n, const int *arp, const int *aci, cons
 
 void xs_pt_sweep(hipStream_t s, int n, const int *rp, const int *ci, const int *dpos,
                  const double *vin, double *vout, const int *arp, const int *aci,
-                 const double *av, const double *sc, int maxr) {
+                 const double *av, const double *sc, int maxr, unsigned long long *keys) {
 #define PT_SWEEP(M)                                                                              \
   hipLaunchKernelGGL(k_pt_sweep_rows<M>, pt_row_grid<M>(n), dim3(XS_WAVE * pt_rows<M>::W), 0, s, \
-                     n, rp, ci, dpos, vin, vout, arp, aci, av, sc)
+                     n, rp, ci, dpos, vin, vout, arp, aci, av, sc, keys)
   if (maxr <= 128) PT_SWEEP(128);
   else if (maxr <= 256) PT_SWEEP(256);
   else PT_SWEEP(512);
@@ -1014,8 +1041,8 @@ int xs_pt_pick_rows(int n, const unsigned *stats) {
   return 512;
 }
 
-void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const double *v,
-                  const double *sc, const unsigned long long *cnt2, long long keepL, long long keepU,
+void xs_pt_select(hipStream_t s, int nnz, const unsigned long long *keys, int nunit,
+                  const unsigned long long *cnt2, long long keepL, long long keepU,
                   unsigned long long *sel, unsigned *hist, unsigned *lcnt,
                   unsigned long long *list, double *thr) {
   hipLaunchKernelGGL(k_pt_sel_init, dim3(1), dim3(256), 0, s, cnt2, keepL, keepU, sel, hist, lcnt,
@@ -1024,7 +1051,7 @@ void xs_pt_select(hipStream_t s, int nnz, const int *row, const int *ci, const d
   if (g > 2048) g = 2048;
   if (g < 1) g = 1;
   for (int shift = 56; shift >= 0; shift -= 8) {
-    hipLaunchKernelGGL(k_pt_hist, dim3(g), dim3(XS_BLOCK), 0, s, nnz, row, ci, v, sc, shift, sel,
+    hipLaunchKernelGGL(k_pt_hist, dim3(g), dim3(XS_BLOCK), 0, s, nnz, keys, nunit, shift, sel,
                        hist, lcnt, list);
     hipLaunchKernelGGL(k_pt_pick, dim3(2), dim3(256), 0, s, shift, sel, hist, lcnt, list, thr);
   }
@@ -1048,16 +1075,16 @@ void xs_pt_dtol(hipStream_t s, int n, const int *arp, const double *av, double d
 }
 
 void xs_pt_keep(hipStream_t s, bool write, int n, const int *crp, const int *cci,
-                const double *cv, const double *thr, const double *dtol, const double *sc,
-                int *cnt, const int *frp, int *fci, double *fv, int *frow, int *fdpos,
-                int *bad) {
+                const int *cdpos, const double *cv, const unsigned long long *keys,
+                const double *thr, const double *dtol, int *cnt, const int *frp, int *fci,
+                double *fv, int *frow, int *fdpos, int *bad) {
   const int rows = XS_BLOCK / XS_WAVE, g = (n + rows - 1) / rows;
   if (!write)
-    hipLaunchKernelGGL(k_pt_keep_count, dim3(g), dim3(XS_BLOCK), 0, s, n, crp, cci, cv, thr, dtol,
-                       sc, cnt);
+    hipLaunchKernelGGL(k_pt_keep_count, dim3(g), dim3(XS_BLOCK), 0, s, n, crp, cdpos, keys, thr,
+                       dtol, cnt);
   else
-    hipLaunchKernelGGL(k_pt_keep_write, dim3(g), dim3(XS_BLOCK), 0, s, n, crp, cci, cv, thr, dtol,
-                       sc, frp, fci, fv, frow, fdpos, bad);
+    hipLaunchKernelGGL(k_pt_keep_write, dim3(g), dim3(XS_BLOCK), 0, s, n, crp, cci, cdpos, cv, keys,
+                       thr, dtol, frp, fci, fv, frow, fdpos, bad);
 }
 
 void xs_pt_scale_vec(hipStream_t s, int n, const int *arp, const int *aci, const double *av,
