@@ -1072,19 +1072,22 @@ int xs_parilut_factor(xs_ctx *c) {
   PT_TRY(dalloc(&lstat, 8));
   PT_TRY(dalloc(&crp, n + 1));
   PT_TRY(dalloc(&cdpos, n));
+  // the factor ends near fill x nnz(A) entries (the keep counts): its
+  // buffers are sized for that at once; the candidate buffers grow only
+  // (to twice the need), so few steps pay for an allocation
+  const size_t fcap = (size_t)((c->ilut_fill + 0.25) * (double)nnz) + (size_t)n;
   PT_TRY(dalloc(&F.rp_own, n + 1));
-  PT_TRY(dalloc(&F.ci_own, nnz));
-  PT_TRY(dalloc(&F.lu, nnz));
-  PT_TRY(dalloc(&F.rowidx, nnz));
+  PT_TRY(dalloc(&F.ci_own, fcap));
+  PT_TRY(dalloc(&F.lu, fcap));
+  PT_TRY(dalloc(&F.rowidx, fcap));
+  PT_TRY(dalloc(&fv2, fcap));
   PT_TRY(dalloc(&F.dpos, n));
   PT_TRY(dalloc(&F.dinv, n));
-  // the candidate and factor buffers grow only (by a quarter beyond the
-  // need), so most steps reuse the previous step's allocations
-  size_t cap_c = 0, cap_ci = (size_t)nnz, cap_lu = (size_t)nnz, cap_row = (size_t)nnz, cap_v2 = 0;
-  auto grow = [](size_t need, size_t &cap, auto **pp) -> int {
+  size_t cap_c = 0, cap_ci = fcap, cap_lu = fcap, cap_row = fcap, cap_v2 = fcap;
+  auto grow = [](size_t need, size_t &cap, auto **pp, size_t factor) -> int {
     if (need <= cap) return XS_OK;
     dfree(*pp);
-    const size_t want = need + need / 4;
+    const size_t want = need * factor;
     const int r = dalloc(pp, want);
     cap = r == XS_OK ? want : 0;
     return r;
@@ -1130,11 +1133,11 @@ int xs_parilut_factor(xs_ctx *c) {
     const int nc = hdr[0];
     {
       size_t c1 = cap_c, c2 = cap_c, c3 = cap_c, c4 = cap_c, c5 = cap_c;
-      PT_TRY(grow((size_t)nc, c1, &cci));
-      PT_TRY(grow((size_t)nc, c2, &crow));
-      PT_TRY(grow((size_t)nc, c3, &cv));
-      PT_TRY(grow((size_t)nc, c4, &cv2));
-      PT_TRY(grow((size_t)nc, c5, &ckey));
+      PT_TRY(grow((size_t)nc, c1, &cci, 2));
+      PT_TRY(grow((size_t)nc, c2, &crow, 2));
+      PT_TRY(grow((size_t)nc, c3, &cv, 2));
+      PT_TRY(grow((size_t)nc, c4, &cv2, 2));
+      PT_TRY(grow((size_t)nc, c5, &ckey, 2));
       cap_c = c1;
     }
     // 2-3. the candidates with the old values embedded, and a sweep over them
@@ -1159,10 +1162,10 @@ int xs_parilut_factor(xs_ctx *c) {
     XS_HIP(hipStreamSynchronize(s));
     const int frows = c->parilut_rows ? c->parilut_rows : xs_pt_pick_rows(n, fst);
     // the old factor was last read by the embed (the stream is idle here)
-    PT_TRY(grow((size_t)fnnz, cap_ci, &F.ci_own));
-    PT_TRY(grow((size_t)fnnz, cap_lu, &F.lu));
-    PT_TRY(grow((size_t)fnnz, cap_row, &F.rowidx));
-    PT_TRY(grow((size_t)fnnz, cap_v2, &fv2));
+    PT_TRY(grow((size_t)fnnz, cap_ci, &F.ci_own, 2));
+    PT_TRY(grow((size_t)fnnz, cap_lu, &F.lu, 2));
+    PT_TRY(grow((size_t)fnnz, cap_row, &F.rowidx, 2));
+    PT_TRY(grow((size_t)fnnz, cap_v2, &fv2, 2));
     xs_pt_keep(s, true, n, crp, cci, cdpos, cv, ckey, thr, dtol, nullptr, F.rp_own, F.ci_own, F.lu,
                F.rowidx, F.dpos, ovf);
     // 5. sweep over the kept pattern

@@ -36,8 +36,8 @@ __device__ __forceinline__ int pt_wave_sum_i(int v) {
 }
 
 // insert column c (when v) into a row's 2048-slot set; *ovf |= 1 when the
-// set is full.  One CAS per lane; the probe loop runs only when some lane's
-// first slot holds another column.
+// set is full.  At most one CAS per lane; the probe loop runs only when some
+// lane's first slot holds another column.
 __device__ int pt_insert_from(int *tab, int c, unsigned h, int *ovf) {
   for (int probe = 1; probe < PT_HASH; ++probe) {
     const int old = atomicCAS(&tab[h], PT_EMPTY, c);
@@ -50,8 +50,10 @@ __device__ int pt_insert_from(int *tab, int c, unsigned h, int *ovf) {
 
 __device__ __forceinline__ void pt_insert(int *tab, int c, bool v, int *ovf) {
   const unsigned h = ((unsigned)c * 2654435761u) & (PT_HASH - 1);
-  int old = PT_EMPTY;
-  if (v) old = atomicCAS(&tab[h], PT_EMPTY, c);
+  // most inserts repeat a column already in its first slot: a plain read
+  // settles those, the CAS is left to the others
+  int old = tab[h];
+  if (v && old != c) old = atomicCAS(&tab[h], PT_EMPTY, c);
   const bool more = v && old != PT_EMPTY && old != c;
   if (__any(more))
     if (more) pt_insert_from(tab, c, (h + 1) & (PT_HASH - 1), ovf);
@@ -79,90 +81,126 @@ __device__ __forceinline__ double pt_readlane(double v, int t) {
 
 #define PT_SPOS 2   // stream positions per lane per step
 
-template <bool VALS, typename Fn>
-__device__ __forceinline__ void pt_stream(int lb, int le, int off, const int *__restrict__ frp,
-                                          const int *__restrict__ fci,
-                                          const double *__restrict__ fv,
-                                          const int *__restrict__ fdpos, int lane, Fn &&fn) {
-  constexpr int P = PT_SPOS, STEP = P * XS_WAVE;
+// One block of the stream (up to 64 L entries) in two phases: begin() takes
+// the block's per-lane inputs (or load_begin() loads them), scans the part
+// lengths and issues the first step's loads; finish() runs the steps.  A
+// kernel that already holds the first block's inputs begins it before its
+// own LDS set-up, so the first loads are in flight meanwhile.
+template <bool VALS>
+struct pt_stream_t {
+  static constexpr int P = PT_SPOS, STEP = P * XS_WAVE;
   struct step {
     int j[P];
     double l[P], u[P];
     bool f[P], v[P];     // first entry of its part; a product at this position
   };
-  for (int c0 = lb; c0 < le; c0 += XS_WAVE) {
-    // lane t: L entry c0 + t (loads clamped to the block's last entry, so
-    // they issue unconditionally), its value and its part of the stream
-    const int cnt = min(XS_WAVE, le - c0);
-    const int ce = c0 + min(lane, cnt - 1);
-    const int k = fci[ce];
-    const double lv = VALS ? fv[ce] : 0.0;
-    const int ub = fdpos[k] + off;
-    const int len = lane < cnt ? frp[k + 1] - ub : 0;
-    int inc = len;
+  const int *__restrict__ fci;
+  const double *__restrict__ fv;
+  int lane, c0, inc, tot, st, base;
+  double lv;
+  step A, B;
+
+  // lane t < cnt: L entry c0 + t, its value lv_ (l_ik) and its part [ub,
+  // ub + len) of (fci, fv); lanes t >= cnt hold no part
+  __device__ __forceinline__ void begin(int c0_, int cnt, int ub, int len, double lv_) {
+    c0 = c0_;
+    lv = lv_;
+    if (lane >= cnt) len = 0;
+    inc = len;
 #pragma unroll
     for (int o = 1; o < XS_WAVE; o <<= 1) {
       const int y = __shfl_up(inc, o, XS_WAVE);
       if (lane >= o) inc += y;
     }
-    const int tot = __builtin_amdgcn_readlane(inc, XS_WAVE - 1);
-    const int st = inc - len;            // first stream position of this lane's part
-    const int base = ub - st;            // q = base + p for the positions of this part
-    // locate the step's positions and issue their loads, for every lane and
-    // whether or not the step holds products (a position past the end loads
-    // entry c0, so no load sits behind a branch)
-    auto issue = [&](int p0, step &z) {
-      const int pe = min(p0 + STEP, tot) - 1;
-      // owners of the first and the last position of the step (uniform;
-      // no iteration once p0 is past the end)
-      const int o0 = __popcll(__ballot(inc <= p0));
-      const int o1 = __popcll(__ballot(inc <= pe));
-      int q[P], sp[P];
+    tot = __builtin_amdgcn_readlane(inc, XS_WAVE - 1);
+    st = inc - len;            // first stream position of this lane's part
+    base = ub - st;            // q = base + p for the positions of this part
+    issue(0, A);
+  }
+
+  // block c0_ of the L entries [c0_, le): loads clamped to the block's last
+  // entry, so they issue unconditionally
+  __device__ __forceinline__ void load_begin(int c0_, int le, int off, const int *__restrict__ frp,
+                                             const int *__restrict__ fdpos) {
+    const int cnt = min(XS_WAVE, le - c0_);
+    const int ce = c0_ + min(lane, cnt - 1);
+    const int k = fci[ce];
+    const double l = VALS ? fv[ce] : 0.0;
+    const int ub = fdpos[k] + off;
+    begin(c0_, cnt, ub, frp[k + 1] - ub, l);
+  }
+
+  // locate the step's positions and issue their loads, for every lane and
+  // whether or not the step holds products (a position past the end loads
+  // entry c0, so no load sits behind a branch)
+  __device__ __forceinline__ void issue(int p0, step &z) {
+    const int pe = min(p0 + STEP, tot) - 1;
+    // owners of the first and the last position of the step (uniform; no
+    // iteration once p0 is past the end)
+    const int o0 = __popcll(__ballot(inc <= p0));
+    const int o1 = __popcll(__ballot(inc <= pe));
+    int q[P], sp[P];
 #pragma unroll
-      for (int c = 0; c < P; ++c) {
-        q[c] = c0;
-        sp[c] = -1;
-        z.l[c] = 0.0;
-      }
-      for (int t = o0; t <= o1; ++t) {
-        // parts in ascending order: each lane keeps the last part starting
-        // at or before its position (an empty part shares its start with
-        // the next one, which overrides it)
-        const int ts = __builtin_amdgcn_readlane(st, t);
-        const int tb = __builtin_amdgcn_readlane(base, t);
-        const double tl = VALS ? pt_readlane(lv, t) : 0.0;
-#pragma unroll
-        for (int c = 0; c < P; ++c) {
-          const int p = p0 + c * XS_WAVE + lane;
-          if (p >= ts) { q[c] = tb + p; z.l[c] = tl; sp[c] = ts; }
-        }
-      }
+    for (int c = 0; c < P; ++c) {
+      q[c] = c0;
+      sp[c] = -1;
+      z.l[c] = 0.0;
+    }
+    for (int t = o0; t <= o1; ++t) {
+      // parts in ascending order: each lane keeps the last part starting at
+      // or before its position (an empty part shares its start with the
+      // next one, which overrides it)
+      const int ts = __builtin_amdgcn_readlane(st, t);
+      const int tb = __builtin_amdgcn_readlane(base, t);
+      const double tl = VALS ? pt_readlane(lv, t) : 0.0;
 #pragma unroll
       for (int c = 0; c < P; ++c) {
         const int p = p0 + c * XS_WAVE + lane;
-        z.v[c] = p < tot;
-        const int qq = z.v[c] ? q[c] : c0;
-        z.j[c] = fci[qq];
-        z.u[c] = VALS ? fv[qq] : 0.0;
-        // first: the entry at fdpos[k] + off (the pivot u_kk when off = 0)
-        z.f[c] = p == sp[c];
+        if (p >= ts) { q[c] = tb + p; z.l[c] = tl; sp[c] = ts; }
       }
-    };
-    auto run = [&](const step &z) {
+    }
 #pragma unroll
-      for (int c = 0; c < P; ++c) fn(z.j[c], z.l[c] * z.u[c], z.f[c], z.v[c]);
-    };
-    // two steps in flight, in two register sets used in turn (no copies of
-    // registers whose loads are pending): the next step's loads are issued
-    // before the current step's products are added
-    step A, B;
-    issue(0, A);
+    for (int c = 0; c < P; ++c) {
+      const int p = p0 + c * XS_WAVE + lane;
+      z.v[c] = p < tot;
+      const int qq = z.v[c] ? q[c] : c0;
+      z.j[c] = fci[qq];
+      z.u[c] = VALS ? fv[qq] : 0.0;
+      // first: the part's first entry (the pivot u_kk when the parts start
+      // at fdpos[k])
+      z.f[c] = p == sp[c];
+    }
+  }
+
+  template <typename Fn>
+  __device__ __forceinline__ void run(const step &z, Fn &fn) {
+#pragma unroll
+    for (int c = 0; c < P; ++c) fn(z.j[c], z.l[c] * z.u[c], z.f[c], z.v[c]);
+  }
+
+  // two steps in flight, in two register sets used in turn (no copies of
+  // registers whose loads are pending): the next step's loads are issued
+  // before the current step's products are added
+  template <typename Fn>
+  __device__ __forceinline__ void finish(Fn &fn) {
     for (int p0 = 0; p0 < tot; p0 += 2 * STEP) {
       issue(p0 + STEP, B);
-      run(A);
+      run(A, fn);
       issue(p0 + 2 * STEP, A);
-      run(B);
+      run(B, fn);
     }
+  }
+};
+
+template <bool VALS, typename Fn>
+__device__ __forceinline__ void pt_stream(int lb, int le, int off, const int *__restrict__ frp,
+                                          const int *__restrict__ fci,
+                                          const double *__restrict__ fv,
+                                          const int *__restrict__ fdpos, int lane, Fn &&fn) {
+  pt_stream_t<VALS> str{fci, fv, lane};
+  for (int c0 = lb; c0 < le; c0 += XS_WAVE) {
+    str.load_begin(c0, le, off, frp, fdpos);
+    str.finish(fn);
   }
 }
 
@@ -184,13 +222,27 @@ __global__ __launch_bounds__(XS_WAVE *PT_WAVES) void k_pt_symbolic(
   int *tab = tab_all[wv];
   int *keys = tab;   // compacted in place (see below)
   if (act) {
+    // the union of the upper parts (incl. pivot) of the rows k in L_i (a set:
+    // order is free): its first block begun from the row's first 64 entries
+    // before the table is set up
+    const int fa = frp[i], fb = frp[i + 1], fd = fdpos[i], nlf = fd - fa;
+    const int fj = fci[fb > fa ? fa + min(lane, fb - fa - 1) : 0];
+    const int kf = lane < nlf ? fj : i;
+    const int dkf = fdpos[kf], rkf = frp[kf + 1];
+    pt_stream_t<false> str{fci, nullptr, lane};
+    if (nlf > 0) str.begin(fa, min(XS_WAVE, nlf), dkf, rkf - dkf, 0.0);
     for (int q = lane; q < PT_HASH; q += XS_WAVE) tab[q] = PT_EMPTY;
     for (int e = arp[i] + lane; e < arp[i + 1]; e += XS_WAVE) pt_insert(tab, aci[e], true, ovf);
-    for (int e = frp[i] + lane; e < frp[i + 1]; e += XS_WAVE) pt_insert(tab, fci[e], true, ovf);
-    // union of the upper parts (incl. pivot) of the rows k in L_i (a set:
-    // order is free)
-    pt_stream<false>(frp[i], fdpos[i], 0, frp, fci, nullptr, fdpos, lane,
-                     [&](int j, double, bool, bool v) { pt_insert(tab, j, v, ovf); });
+    pt_insert(tab, fj, lane < fb - fa, ovf);
+    for (int e = fa + XS_WAVE + lane; e < fb; e += XS_WAVE) pt_insert(tab, fci[e], true, ovf);
+    auto add = [&](int j, double, bool, bool v) { pt_insert(tab, j, v, ovf); };
+    if (nlf > 0) {
+      str.finish(add);
+      for (int c0 = fa + XS_WAVE; c0 < fd; c0 += XS_WAVE) {
+        str.load_begin(c0, fd, 0, frp, fdpos);
+        str.finish(add);
+      }
+    }
     // compact the keys (slot order) to the front of the same table, then
     // rank-sort them.  In place is safe: a chunk's 64 slots are all read
     // (one wave-wide load) before its keys are stored, and they go to
@@ -299,8 +351,8 @@ __device__ __forceinline__ unsigned pt_rhash(int c) {
 }
 
 // Slot of column j, inserted if absent (when v; -1 otherwise).  The table
-// has room (len <= RTAB / 2).  One CAS per lane; the probe loop runs only
-// when some lane's first slot holds another column.
+// has room (len <= RTAB / 2).  At most one CAS per lane; the probe loop runs
+// only when some lane's first slot holds another column.
 template <int RTAB>
 __device__ int pt_slot_insert_from(int *key, int j, unsigned h) {
   for (;;) {
@@ -313,8 +365,8 @@ __device__ int pt_slot_insert_from(int *key, int j, unsigned h) {
 template <int RTAB>
 __device__ __forceinline__ int pt_slot_insert_if(int *key, int j, bool v) {
   const unsigned h = pt_rhash<RTAB>(j);
-  int old = PT_EMPTY;
-  if (v) old = atomicCAS(&key[h], PT_EMPTY, j);
+  int old = key[h];   // a column already in its first slot needs no CAS
+  if (v && old != j) old = atomicCAS(&key[h], PT_EMPTY, j);
   const bool more = v && old != PT_EMPTY && old != j;
   int r = v ? (int)h : -1;
   if (__any(more))
@@ -403,22 +455,35 @@ __global__ __launch_bounds__(XS_WAVE * pt_rows<MAXR>::W) void k_pt_sweep_rows(
     return;
   }
   const int nl = dpos[i] - a, ab = arp[i], ae = arp[i + 1];
-  int cj[NM];
-  double piv[NM];
+  // the row's columns; for its L entries j = k: the pivot positions dpos[k]
+  // (u_kk divides the entry; the product parts start after it), the row
+  // ends rp[k + 1] and the values l_ik -- loads with clamped indices, so
+  // they issue together
+  int cj[NM], dk[NM];
+  double piv[NM], scj[NM];
 #pragma unroll
   for (int m = 0; m < NM; ++m) {
     const int t = lane + m * XS_WAVE;
     cj[m] = t < len ? ci[a + t] : -1;
   }
-  double scj[NM];
 #pragma unroll
   for (int m = 0; m < NM; ++m) {
     const int t = lane + m * XS_WAVE;
-    piv[m] = t < nl ? vin[dpos[cj[m]]] : 1.0;
+    dk[m] = dpos[t < nl ? cj[m] : i];
+  }
+  const int rk0 = rp[(lane < nl ? cj[0] : i) + 1];
+  const double lv0 = vin[len > 0 ? a + min(lane, len - 1) : 0];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const int t = lane + m * XS_WAVE;
+    piv[m] = t < nl ? vin[dk[m]] : 1.0;
     scj[m] = (keys && cj[m] >= 0) ? sc[cj[m]] : 1.0;
   }
   const int aj = ab + lane < ae ? aci[ab + lane] : -1;
   const double aval = aj >= 0 ? av[ab + lane] * si * sc[aj] : 0.0;
+  // the product stream's first block, begun before the table is built
+  pt_stream_t<true> str{ci, vin, lane};
+  if (nl > 0) str.begin(a, min(XS_WAVE, nl), dk[0] + 1, rk0 - (dk[0] + 1), lv0);
   int *key = key_s[w];
   double *S = S_s[w];
   for (int q = lane; q < RTAB; q += XS_WAVE) key[q] = PT_EMPTY;
@@ -433,10 +498,17 @@ __global__ __launch_bounds__(XS_WAVE * pt_rows<MAXR>::W) void k_pt_sweep_rows(
     }
   }
   const int ha = aj >= 0 ? pt_slot_find<RTAB>(key, aj) : -1;
-  pt_stream<true>(a, a + nl, 1, rp, ci, vin, dpos, lane, [&](int j, double x, bool, bool v) {
+  auto add = [&](int j, double x, bool, bool v) {
     const int h = pt_slot_find<RTAB>(key, j);
     if (v && h >= 0) atomicAdd(&S[h], x);
-  });
+  };
+  if (nl > 0) {
+    str.finish(add);
+    for (int c0 = a + XS_WAVE; c0 < a + nl; c0 += XS_WAVE) {
+      str.load_begin(c0, a + nl, 1, rp, dpos);
+      str.finish(add);
+    }
+  }
   if (ha >= 0) S[ha] = S[ha] - aval;
   for (int e = ab + XS_WAVE + lane; e < ae; e += XS_WAVE) {   // rows of A beyond 64 entries
     const int j = aci[e];
@@ -515,11 +587,18 @@ __global__ __launch_bounds__(XS_WAVE * pt_rows<MAXR>::W) void k_pt_fill_embed(
   if (u > MAXR) return;  // k_pt_symbolic<true> + k_pt_embed_long
   const int ab = arp[i], ae = arp[i + 1], fa = frp[i], fb = frp[i + 1], fd = fdpos[i];
   const double si = sc[i];
-  // A's and the old factor's first 64 entries of the row, loaded together
+  // A's and the old factor's first 64 entries of the row, loaded together,
+  // with what the product stream's first block needs of the old factor's L
+  // entries k (pivot positions, row ends)
+  const int nlf = fd - fa;
   const int aj = ab + lane < ae ? aci[ab + lane] : -1;
   const double aval = aj >= 0 ? av[ab + lane] * si * sc[aj] : 0.0;
   const int fj = fa + lane < fb ? fci[fa + lane] : -1;
   const double fval = fj >= 0 ? fv[fa + lane] : 0.0;
+  const int kf = lane < nlf ? fj : i;
+  const int dkf = fdpos[kf], rkf = frp[kf + 1];
+  pt_stream_t<true> str{fci, fv, lane};
+  if (nlf > 0) str.begin(fa, min(XS_WAVE, nlf), dkf, rkf - dkf, fval);
   int *key = key_s[w];
   double *S = S_s[w];
   for (int q = lane; q < RTAB; q += XS_WAVE) {
@@ -530,11 +609,17 @@ __global__ __launch_bounds__(XS_WAVE * pt_rows<MAXR>::W) void k_pt_fill_embed(
   for (int e = ab + XS_WAVE + lane; e < ae; e += XS_WAVE) pt_slot_insert<RTAB>(key, aci[e]);
   const int hf = fj >= 0 ? pt_slot_insert<RTAB>(key, fj) : -1;
   for (int e = fa + XS_WAVE + lane; e < fb; e += XS_WAVE) pt_slot_insert<RTAB>(key, fci[e]);
-  pt_stream<true>(fa, fd, 0, frp, fci, fv, fdpos, lane,
-                  [&](int j, double x, bool first, bool v) {
-                    const int h = pt_slot_insert_if<RTAB>(key, j, v);
-                    if (v && !first) atomicAdd(&S[h], x);
-                  });
+  auto add = [&](int j, double x, bool first, bool v) {
+    const int h = pt_slot_insert_if<RTAB>(key, j, v);
+    if (v && !first) atomicAdd(&S[h], x);
+  };
+  if (nlf > 0) {
+    str.finish(add);
+    for (int c0 = fa + XS_WAVE; c0 < fd; c0 += XS_WAVE) {
+      str.load_begin(c0, fd, 0, frp, fdpos);
+      str.finish(add);
+    }
+  }
   // S_ij - a_ij on A's slots; every value then reads the negation
   if (ha >= 0) S[ha] = S[ha] - aval;
   for (int e = ab + XS_WAVE + lane; e < ae; e += XS_WAVE) {
