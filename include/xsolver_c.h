@@ -270,8 +270,10 @@ int xs_batch_create(xs_batch **bt, int nsys, const int *sys_ptr, const int *rowp
                     const int *colidx, const double *val);
 /* CG + Jacobi on every system to ||b - A x|| < tol ||b - A x0|| (the
  * reference's Krylov convention, hip/Solver/xsolver_cg.cpp:16-74; FP64 check), each
- * system with its own iteration count; precision 0 = FP64, 1 = FP32 CG inside
- * FP64 iterative refinement.  sol = x0 in, x out (host or device vectors). */
+ * system with its own iteration count; precision 0 = FP64, 1 = one FP32 CG
+ * recurrence (FP32 matrix values and vectors) with FP64 reliable updates: at
+ * every 1e-2 drop of the FP32 residual the FP64 residual b - A x replaces it
+ * and decides convergence.  sol = x0 in, x out (host or device vectors). */
 int xs_batch_solve(xs_batch *bt, const double *rhs, double *sol, double tol, int maxiter,
                    int precision);
 int xs_batch_solve_device(xs_batch *bt, const double *d_rhs, double *d_sol, double tol,

@@ -124,7 +124,7 @@ static int batch_run(xs_batch *bt, const double *d_b, double *d_x, double tol, i
   BT_CHK(hipMemsetAsync(bt->queue, 0, sizeof(unsigned), s));
   BT_CHK(hipEventRecord(bt->e0, s));
   xs_batch_cg(s, grid, bt->nsys, bt->n, bt->nnz, bt->order, bt->sys_ptr, bt->rp, bt->ci, bt->v, bt->v32,
-              bt->dinv, bt->dinv32, d_b, d_x, bt->w64, bt->w32, tol, maxiter, precision, 1e-4,
+              bt->dinv, bt->dinv32, d_b, d_x, bt->w64, bt->w32, tol, maxiter, precision, 1e-2,
               bt->queue, bt->iters, bt->relres);
   BT_CHK(hipGetLastError());
   BT_CHK(hipEventRecord(bt->e1, s));
