@@ -158,6 +158,11 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *                    on the row before with one affine scan; 0 / 1 force either
  *   "trsv_workgroups" persistent workgroups of a triangular solve (default 256):
  *                    bounds the number of polling waves
+ *   "trsv_gather"    off-group entries a lane of the triangular solve loads
+ *                    and polls together: 0 (default) = 16 for factors with
+ *                    more than six entries per row and triangle, else 4
+ *   "spmm_cw"        columns per thread of the k-RHS ELL SpMM: 0 (default) =
+ *                    4 when k % 4 == 0, else 2
  *   "spmv_kernel"    SpMV kernel for CSR input: -1 (default) = automatic: an
  *                    internal ELL copy when the padding to the longest row is
  *                    under 20% of nnz, else vector CSR when the row lengths are
@@ -168,13 +173,21 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *   "use_ell"        1 = ELL SpMV (same as advanced_pam[1] = 1) */
 int xs_set_option(xs_ctx *ctx, const char *name, double value);
 
-/* SpMV autotuner (no reference counterpart): times the CSR row-block kernel
- * at nnz budgets per workgroup of 512..4096, the row-binned vector-CSR kernel
- * (reported as chunk 0) and the ELL kernel (when its padding is under 50%) on
- * this matrix and keeps the fastest; with the
- * environment variable XS_TUNE_CACHE naming a file, results are read from
- * and appended to it per matrix signature.  Choice and time are queried with
- * xs_get_time_ms("tune:chunk" | "tune:ell" | "tune:us").  The SpMV kernel in
+/* Autotuner (no reference counterpart): times, on this matrix, the CSR
+ * row-block kernel at nnz budgets per workgroup of 512..4096, the row-binned
+ * vector-CSR kernel (reported as chunk 0) with lanes per row of half, one and
+ * two times the mean row length rounded up to a power of two and long rows
+ * past 4, 8 or 16 lane groups, and the ELL kernel (when its padding is under
+ * 50%, with the k-RHS SpMM's 2 or 4 columns per thread timed at k = 16), and
+ * keeps the fastest.  Called after xs_pc_setup with an ILU-type
+ * preconditioner it also times the exact triangular solves over persistent
+ * workgroups 128..1024, gather widths 4 / 16 and both group modes and keeps
+ * the fastest.  With the environment variable XS_TUNE_CACHE naming a file,
+ * results are read from and appended to it per matrix (and factor)
+ * signature.  Choices and times are queried with xs_get_time_ms("tune:chunk"
+ * | "tune:ell" | "tune:us" | "tune:vec_lanes" | "tune:long_thr" |
+ * "tune:spmm_cw" | "tune:trsv_wg" | "tune:trsv_gather" | "tune:trsv_prog" |
+ * "tune:trsv_us"; -1 when not tuned).  The SpMV kernel in
  * use: xs_get_time_ms("spmv:kernel") = 0 LDS row blocks, 1 vector CSR, 2 ELL;
  * xs_get_time_ms("spmv:idx16") = 1 when the ELL copy stores 16-bit column
  * offsets (every entry within 32767 of its row), 0 when it reads int32

@@ -78,6 +78,7 @@ static void free_precond(xs_ctx *c) {
   F = xs_ilu();
   F.trsv_wg = c->trsv_wg;
   F.trsv_prog = c->trsv_prog;
+  F.trsv_gather = c->trsv_gather;
   dfree(c->lu_tmp);
   c->pc_ready = false;
 }
@@ -210,15 +211,18 @@ static int make_plan(xs_ctx *c, const xs_csr &M, xs_spmv_plan &P, int chunk) {
 
 // Row-binned vector-CSR parameters from the host row pointer: lanes per row
 // = average row length rounded up to a power of two (2..64); rows longer than
-// 8 such groups are listed for one wavefront each.
-static int make_vec(xs_ctx *c, const std::vector<int> &hrp, xs_spmv_plan &P) {
+// 8 such groups are listed for one wavefront each (lanes and the multiple:
+// the autotuner's, when given).
+static int make_vec(xs_ctx *c, const std::vector<int> &hrp, xs_spmv_plan &P, int lanes = 0,
+                    int long_mult = 0) {
   (void)c;
   const int n = (int)hrp.size() - 1;
   const double avg = n > 0 ? (double)hrp[n] / n : 1.0;
   int L = 2;
   while (L < avg && L < XS_WAVE) L <<= 1;
+  if (lanes) L = lanes;               // the autotuner's choice
   P.vec_L = L;
-  P.long_thr = 8 * L;
+  P.long_thr = (long_mult ? long_mult : 8) * L;
   std::vector<int> lr;
   for (int i = 0; i < n; i++)
     if (hrp[i + 1] - hrp[i] > P.long_thr) lr.push_back(i);
@@ -322,27 +326,38 @@ static std::string tune_key(const xs_ctx *c) {
   return buf;
 }
 
-static bool tune_cache_get(const std::string &key, int *chunk, int *ell, double *us) {
+// XS_TUNE_CACHE: one line per tuned key, "key v0 v1 ...", the last line of
+// a key winning
+static bool tune_cache_get(const std::string &key, std::vector<double> &vals, size_t count) {
   const char *path = getenv("XS_TUNE_CACHE");
   if (!path) return false;
   FILE *f = fopen(path, "r");
   if (!f) return false;
-  char k[96];
-  int ch, el;
-  double t;
+  char line[512];
   bool hit = false;
-  while (fscanf(f, "%95s %d %d %lf", k, &ch, &el, &t) == 4)
-    if (key == k) { *chunk = ch; *ell = el; *us = t; hit = true; }
+  while (fgets(line, sizeof(line), f)) {
+    char k[128];
+    int used = 0;
+    if (sscanf(line, "%127s%n", k, &used) != 1 || key != k) continue;
+    std::vector<double> v;
+    const char *q = line + used;
+    double x;
+    int m = 0;
+    while (sscanf(q, "%lf%n", &x, &m) == 1) { v.push_back(x); q += m; }
+    if (v.size() == count) { vals = v; hit = true; }
+  }
   fclose(f);
   return hit;
 }
 
-static void tune_cache_put(const std::string &key, int chunk, int ell, double us) {
+static void tune_cache_put(const std::string &key, const std::vector<double> &vals) {
   const char *path = getenv("XS_TUNE_CACHE");
   if (!path) return;
   FILE *f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %.3f\n", key.c_str(), chunk, ell, us);
+  fprintf(f, "%s", key.c_str());
+  for (double v : vals) fprintf(f, " %.6g", v);
+  fprintf(f, "\n");
   fclose(f);
 }
 
@@ -359,24 +374,125 @@ static int time_spmv(xs_ctx *c, const double *x, double *y, int reps, double *us
   return XS_OK;
 }
 
+// mean microseconds of one k-column ELL SpMM (Y = A X), cw columns per thread
+static int time_spmm(xs_ctx *c, int k, const double *X, double *Y, int cw, int reps,
+                     double *us) {
+  for (int r = 0; r < 2; r++)
+    xs_mspmm_ell(c->stream, c->ell, k, X, Y, EPI_Y, nullptr, nullptr, nullptr, nullptr, cw);
+  XS_HIP(hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; r++)
+    xs_mspmm_ell(c->stream, c->ell, k, X, Y, EPI_Y, nullptr, nullptr, nullptr, nullptr, cw);
+  XS_HIP(hipEventRecord(c->ev1, c->stream));
+  XS_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  XS_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *us = 1000.0 * ms / reps;
+  return XS_OK;
+}
+
+// mean microseconds of a lower plus an upper exact triangular solve (the
+// solution vector refilled with the sentinel before each, as a solve does)
+static int time_trsv(xs_ctx *c, const double *b, double *x, int reps, double *us) {
+  const int n = c->A.n;
+  auto both = [&]() {
+    xs_fill_sentinel(c->stream, x, n);
+    xs_trsv_lower(c->stream, c->ilu, b, x, c->counters + 2, nullptr);
+    xs_fill_sentinel(c->stream, x, n);
+    xs_trsv_upper(c->stream, c->ilu, b, x, c->counters + 4, nullptr);
+  };
+  both();
+  XS_HIP(hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; r++) both();
+  XS_HIP(hipEventRecord(c->ev1, c->stream));
+  XS_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  XS_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *us = 1000.0 * ms / reps;
+  return xs_take_launch_guard() ? XS_ERR_STATE : XS_OK;
+}
+
+// n, nnz and an FNV-1a hash of about 4096 sampled row lengths of a pattern
+static std::string shape_key(const char *kind, const int *hrp, int n, int nnz) {
+  unsigned long long h = 1469598103934665603ull;
+  const int step = std::max(1, n / 4096);
+  for (int i = 0; i < n; i += step) {
+    h ^= (unsigned long long)(hrp[i + 1] - hrp[i]);
+    h *= 1099511628211ull;
+  }
+  char buf[128];
+  snprintf(buf, sizeof(buf), "%s:%d:%d:%016llx", kind, n, nnz, h);
+  return buf;
+}
+
+static int ensure_trsv_sched(xs_ctx *c, bool exact = false);
+
+// The triangular solves of a set-up factor: persistent workgroups, gather
+// width and group mode, each combination timed on a lower plus an upper
+// solve; the fastest is kept (and cached by the factor's shape).
+static int tune_trsv(xs_ctx *c) {
+  int rc;
+  if ((rc = ensure_trsv_sched(c, true))) return rc;
+  xs_ilu &F = c->ilu;
+  std::vector<int> hrp(F.n + 1);
+  XS_HIP(hipStreamSynchronize(c->stream));
+  XS_HIP(hipMemcpy(hrp.data(), F.rp, sizeof(int) * (F.n + 1), hipMemcpyDeviceToHost));
+  const std::string key = shape_key("trsv", hrp.data(), F.n, F.nnz);
+  std::vector<double> tv;
+  int bwg = F.trsv_wg, bga = xs_ilu_gather(F), bpr = F.trsv_prog;
+  double bus = 0.0;
+  if (tune_cache_get(key, tv, 4)) {
+    bwg = (int)tv[0]; bga = (int)tv[1]; bpr = (int)tv[2]; bus = tv[3];
+  } else {
+    double *b = xs_ctx_vec(c, 0), *x = xs_ctx_vec(c, 1);
+    XS_HIP(hipMemsetAsync(b, 0, sizeof(double) * c->A.n, c->stream));
+    bus = 1e30;
+    static const int wgs[] = {128, 256, 512, 1024}, gathers[] = {4, 16};
+    for (int wg : wgs)
+      for (int ga : gathers)
+        for (int pr = 0; pr < 2; ++pr) {
+          F.trsv_wg = wg;
+          F.trsv_gather = ga;
+          F.trsv_prog = pr;
+          double us;
+          if ((rc = time_trsv(c, b, x, 3, &us))) return rc;
+          if (us < bus) { bus = us; bwg = wg; bga = ga; bpr = pr; }
+        }
+    tune_cache_put(key, {(double)bwg, (double)bga, (double)bpr, bus});
+  }
+  c->trsv_wg = F.trsv_wg = bwg;
+  c->trsv_gather = F.trsv_gather = bga;
+  c->trsv_prog = F.trsv_prog = bpr;
+  c->tune_trsv_us = bus;
+  return XS_OK;
+}
+
+// The autotuner: for the SpMV, the LDS row-block size, the row-binned
+// vector-CSR kernel's lanes per row and long-row threshold, and the ELL copy
+// (with its k-RHS SpMM's columns per thread); for a set-up factor, the
+// triangular solves (tune_trsv).  Every candidate is timed on this matrix;
+// choices are cached in XS_TUNE_CACHE by the pattern's shape.
 extern "C" int xs_autotune(xs_ctx *c) {
   if (!c) return XS_ERR_ARG;
   if (!c->mat_ready) return XS_ERR_STATE;
   int rc;
   if ((rc = xs_ctx_ensure_work(c, XS_NWORK))) return rc;
   const std::string key = tune_key(c);
-  int best_chunk = 0, best_ell = 0;
+  int best_chunk = 0, best_ell = 0, best_L = 0, best_mult = 0, best_cw = 0;
   double best_us = 0.0;
-  if (!tune_cache_get(key, &best_chunk, &best_ell, &best_us)) {
+  std::vector<double> tv;
+  struct keep_prof {  // no kernel timing while tuning; restored on every path
+    xs_ctx *c;
+    int kind;
+    ~keep_prof() { c->prof_kind = kind; }
+  } kp{c, c->prof_kind};
+  c->prof_kind = 0;
+  if (tune_cache_get(key, tv, 6)) {
+    best_chunk = (int)tv[0]; best_ell = (int)tv[1]; best_us = tv[2];
+    best_L = (int)tv[3]; best_mult = (int)tv[4]; best_cw = (int)tv[5];
+  } else {
     double *x = xs_ctx_vec(c, 0), *y = xs_ctx_vec(c, 1);
     XS_HIP(hipMemsetAsync(x, 0, sizeof(double) * c->A.n, c->stream));
     const int reps = 20;
-    struct keep_prof {  // no kernel timing while tuning; restored on every path
-      xs_ctx *c;
-      int kind;
-      ~keep_prof() { c->prof_kind = kind; }
-    } kp{c, c->prof_kind};
-    c->prof_kind = 0;
     best_us = 1e30;
     c->use_ell = 0;
     c->plan.vec = 0;
@@ -387,11 +503,21 @@ extern "C" int xs_autotune(xs_ctx *c) {
       if ((rc = time_spmv(c, x, y, reps, &us))) return rc;
       if (us < best_us) { best_us = us; best_chunk = ch; best_ell = 0; }
     }
-    {  // row-binned vector CSR (chunk reported as 0)
+    {  // row-binned vector CSR (chunk reported as 0): lanes per row around
+       // the average row length, long rows past 4, 8 or 16 lane groups
+      if ((rc = make_vec(c, c->h_rp, c->plan))) return rc;
+      const int L0 = c->plan.vec_L;
+      const int lanes[] = {L0 / 2, L0, 2 * L0}, mults[] = {4, 8, 16};
       c->plan.vec = 1;
-      double us;
-      if ((rc = time_spmv(c, x, y, reps, &us))) return rc;
-      if (us < best_us) { best_us = us; best_chunk = 0; best_ell = 0; }
+      for (int L : lanes) {
+        if (L < 2 || L > XS_WAVE) continue;
+        for (int m : mults) {
+          if ((rc = make_vec(c, c->h_rp, c->plan, L, m))) return rc;
+          double us;
+          if ((rc = time_spmv(c, x, y, reps, &us))) return rc;
+          if (us < best_us) { best_us = us; best_chunk = 0; best_ell = 0; best_L = L; best_mult = m; }
+        }
+      }
       c->plan.vec = 0;
     }
     // ELL when padding stays under 50%
@@ -402,18 +528,41 @@ extern "C" int xs_autotune(xs_ctx *c) {
       double us;
       if ((rc = time_spmv(c, x, y, reps, &us))) return rc;
       if (us < best_us) { best_us = us; best_ell = 1; }
+      // the k-RHS SpMM over the same copy: columns per thread at k = 16
+      const int k = 16;
+      double *X = nullptr, *Y = nullptr;
+      const size_t nk = (size_t)c->A.n * k;
+      if ((rc = dalloc(&X, nk))) return rc;
+      if ((rc = dalloc(&Y, nk))) { dfree(X); return rc; }
+      XS_HIP(hipMemsetAsync(X, 0, sizeof(double) * nk, c->stream));
+      double best_mm = 1e30;
+      static const int cws[] = {2, 4};
+      for (int cw : cws) {
+        double mm;
+        if ((rc = time_spmm(c, k, X, Y, cw, 5, &mm))) { dfree(X); dfree(Y); return rc; }
+        if (mm < best_mm) { best_mm = mm; best_cw = cw; }
+      }
+      XS_HIP(hipStreamSynchronize(c->stream));
+      dfree(X);
+      dfree(Y);
     }
-    tune_cache_put(key, best_chunk, best_ell, best_us);
+    tune_cache_put(key, {(double)best_chunk, (double)best_ell, best_us, (double)best_L,
+                         (double)best_mult, (double)best_cw});
   }
   if ((rc = make_plan(c, c->A, c->plan, best_chunk > 0 ? best_chunk : xs_default_chunk(c->A))))
     return rc;
+  if ((rc = make_vec(c, c->h_rp, c->plan, best_L, best_mult))) return rc;
   c->plan.vec = (best_chunk == 0 && !best_ell) ? 1 : 0;
   c->spmv_vec = c->plan.vec;
   c->use_ell = best_ell;
   if (best_ell && (rc = make_ell(c))) return rc;
+  c->spmm_cw = best_cw;
   c->tune_chunk = best_chunk;
   c->tune_us = best_us;
+  c->tune_L = c->plan.vec_L;
+  c->tune_long_thr = c->plan.long_thr;
   c->tuned = true;
+  if (c->pc_ready && xs_pc_factored(c->pc) && (rc = tune_trsv(c))) return rc;
   XS_HIP(hipStreamSynchronize(c->stream));
   if (xs_take_launch_guard()) return XS_ERR_STATE;
   return ensure_common(c);
@@ -663,6 +812,15 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     c->trsv_prog = (int)value;
     c->ilu.trsv_prog = c->trsv_prog;
   }
+  else if (k == "trsv_gather") {   // 0 by factor width, 4 or 16 off-group entries per poll
+    if (value != 0 && value != 4 && value != 16) return XS_ERR_ARG;
+    c->trsv_gather = (int)value;
+    c->ilu.trsv_gather = c->trsv_gather;
+  }
+  else if (k == "spmm_cw") {   // 0 auto, 2 or 4 columns per thread of the k-RHS ELL SpMM
+    if (value != 0 && value != 2 && value != 4) return XS_ERR_ARG;
+    c->spmm_cw = (int)value;
+  }
   else if (k == "trsv_schedule") { c->trsv_schedule = value != 0; c->pc_ready = false; free_precond(c); }
   else if (k == "lsolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->lsweeps = (int)value; }
   else if (k == "usolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->usweeps = (int)value; }
@@ -810,7 +968,7 @@ static int build_trsv_schedules(xs_ctx *c, const int *rp, const int *ci, const i
 // The groups of a pattern built on the device (ParILUT), on the first solve
 // that applies a factor by the exact triangular solve (sweep counts < 0);
 // until then the solves would fall back to fixed 64-row groups in row order.
-static int ensure_trsv_sched(xs_ctx *c, bool exact = false) {
+static int ensure_trsv_sched(xs_ctx *c, bool exact) {
   xs_ilu &F = c->ilu;
   if (!F.sched_pending || (!exact && c->lsweeps >= 0 && c->usweeps >= 0)) return XS_OK;
   const int n = F.n, nnz = F.nnz;
@@ -1644,6 +1802,13 @@ extern "C" double xs_get_time_ms(const xs_ctx *c, const char *phase) {
   if (!strcmp(phase, "tune:chunk")) return c->tuned ? c->tune_chunk : -1.0;
   if (!strcmp(phase, "tune:ell")) return c->tuned ? c->use_ell : -1.0;
   if (!strcmp(phase, "tune:us")) return c->tuned ? c->tune_us : -1.0;
+  if (!strcmp(phase, "tune:vec_lanes")) return c->tuned ? c->tune_L : -1.0;
+  if (!strcmp(phase, "tune:long_thr")) return c->tuned ? c->tune_long_thr : -1.0;
+  if (!strcmp(phase, "tune:spmm_cw")) return c->tuned ? c->spmm_cw : -1.0;
+  if (!strcmp(phase, "tune:trsv_wg")) return c->tune_trsv_us > 0 ? c->trsv_wg : -1.0;
+  if (!strcmp(phase, "tune:trsv_gather")) return c->tune_trsv_us > 0 ? c->trsv_gather : -1.0;
+  if (!strcmp(phase, "tune:trsv_prog")) return c->tune_trsv_us > 0 ? c->trsv_prog : -1.0;
+  if (!strcmp(phase, "tune:trsv_us")) return c->tune_trsv_us > 0 ? c->tune_trsv_us : -1.0;
   // SpMV kernel in use: 0 LDS row blocks, 1 vector CSR, 2 ELL
   if (!strcmp(phase, "spmv:kernel"))
     return !c->mat_ready ? -1.0 : (c->use_ell && c->ell.v) ? 2.0 : (double)c->plan.vec;
@@ -1826,7 +1991,8 @@ extern "C" int xs_spmm(xs_ctx *c, int nrhs, const double *x, double *y, void *st
   if (!c->mat_ready) return XS_ERR_STATE;
   const void *vecs[2] = {x, y};
   if (c->use_ell && xs_mspmm_ell_ok(c->ell, nrhs, vecs, 2))
-    xs_mspmm_ell(pick(c, stream), c->ell, nrhs, x, y, EPI_Y, nullptr, nullptr, nullptr, nullptr);
+    xs_mspmm_ell(pick(c, stream), c->ell, nrhs, x, y, EPI_Y, nullptr, nullptr, nullptr, nullptr,
+                 c->spmm_cw);
   else
     xs_mspmm(pick(c, stream), c->A, nrhs, x, y, EPI_Y, nullptr, nullptr, nullptr, nullptr);
   XS_HIP(hipGetLastError());

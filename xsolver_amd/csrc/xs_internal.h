@@ -180,6 +180,7 @@ struct xs_ilu {
   bool sched_pending = false;
   int trsv_wg = 256;        // persistent workgroups of the triangular solves
   int trsv_prog = -1;       // xs_tsched::prog; -1 = by factor width (wide -> 1)
+  int trsv_gather = 0;      // xs_tsched::gather (4 or 16); 0 = by factor width
   xs_tell tl, tu;           // sweep copies of L and U (built on demand)
   bool tell_shape = false;  // tl/tu widths and buffers match the pattern
   // transposed factors for the adjoint (built on demand)
@@ -188,13 +189,18 @@ struct xs_ilu {
   double *tlu = nullptr;
 };
 
+// the gather width of a factor's triangular solves: the set one, else by width
+inline int xs_ilu_gather(const xs_ilu &F) {
+  return F.trsv_gather > 0 ? F.trsv_gather : xs_trsv_gather_width(F.nnz, F.n);
+}
+
 inline xs_tsched xs_ilu_tsched(const xs_ilu &F, bool upper) {
   xs_tsched t;
   t.gb = upper ? F.ugb : F.lgb;
   t.ngroups = upper ? F.ugroups : F.lgroups;
   t.order = upper ? F.usched : F.lsched;
   t.max_wg = F.trsv_wg;
-  t.gather = xs_trsv_gather_width(F.nnz, F.n);
+  t.gather = xs_ilu_gather(F);
   t.prog = F.trsv_prog >= 0 ? F.trsv_prog : (t.gather == 16 ? 1 : 0);
   return t;
 }

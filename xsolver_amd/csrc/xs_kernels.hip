@@ -1269,7 +1269,7 @@ void xs_trsv_lower_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
   xs_tsched S;
   S.max_wg = F.trsv_wg;
-  S.gather = xs_trsv_gather_width(F.nnz, F.n);
+  S.gather = xs_ilu_gather(F);
   S.prog = F.trsv_prog >= 0 ? F.trsv_prog : (S.gather == 16 ? 1 : 0);
   trsv_launch(s, false, F.n, F.trp, F.tci, F.tlu, F.tdpos, F.dinv, b, x, tickets, S, st);
 }
@@ -1277,7 +1277,7 @@ void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
   xs_tsched S;
   S.max_wg = F.trsv_wg;
-  S.gather = xs_trsv_gather_width(F.nnz, F.n);
+  S.gather = xs_ilu_gather(F);
   S.prog = F.trsv_prog >= 0 ? F.trsv_prog : (S.gather == 16 ? 1 : 0);
   trsv_launch(s, true, F.n, F.trp, F.tci, F.tlu, F.tdpos, nullptr, b, x, tickets, S, st);
 }

@@ -49,10 +49,12 @@ struct xs_mred {
 void xs_mspmm(hipStream_t s, const xs_csr &A, int k, const double *X, double *Y, int epi,
               const double *B, const double *W, const xs_mred *red, const xs_mstate *ms);
 // the same over the ELL copy of A: k even and every vector 16-byte aligned
-// (xs_mspmm_ell_ok); products in each row's entry order, as xs_mspmm
+// (xs_mspmm_ell_ok); products in each row's entry order, as xs_mspmm.  cw:
+// columns per thread, 2 or 4 (4 needs k % 4 == 0); 0 = 4 when k allows.
 bool xs_mspmm_ell_ok(const xs_ell &E, int k, const void *const *vecs, int nvec);
 void xs_mspmm_ell(hipStream_t s, const xs_ell &E, int k, const double *X, double *Y, int epi,
-                  const double *B, const double *W, const xs_mred *red, const xs_mstate *ms);
+                  const double *B, const double *W, const xs_mred *red, const xs_mstate *ms,
+                  int cw = 0);
 void xs_mdot(hipStream_t s, int n, int k, const double *a, const double *b, const double *c,
              const double *d, const xs_mred &red);
 void xs_mbicg_p(hipStream_t s, int n, int k, const double *R, const double *U, double *P,

@@ -891,7 +891,8 @@ bool xs_mspmm_ell_ok(const xs_ell &E, int k, const void *const *vecs, int nvec) 
 }
 
 void xs_mspmm_ell(hipStream_t s, const xs_ell &E, int k, const double *X, double *Y, int epi,
-                  const double *B, const double *W, const xs_mred *red, const xs_mstate *ms) {
+                  const double *B, const double *W, const xs_mred *red, const xs_mstate *ms,
+                  int cw) {
   const void *vecs[4] = {X, Y, B, W};
   XS_GUARD(X && Y && xs_mspmm_ell_ok(E, k, vecs, 4));
   XS_GUARD(epi == EPI_Y || (red && red->partial && red->counter && red->ms));
@@ -899,7 +900,7 @@ void xs_mspmm_ell(hipStream_t s, const xs_ell &E, int k, const double *X, double
   XS_GUARD(!(epi == EPI_DOT1 || epi == EPI_DOT2) || W);
   const xs_mred R = red ? *red : xs_mred{};
   const bool o16 = E.co != nullptr;
-  if (k % 4 == 0) {
+  if (k % 4 == 0 && cw != 2) {
     if (o16) mspmm_ell_launch<true, 4>(s, E, k, X, Y, epi, B, W, R, ms);
     else mspmm_ell_launch<false, 4>(s, E, k, X, Y, epi, B, W, R, ms);
   } else {

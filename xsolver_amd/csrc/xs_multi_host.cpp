@@ -59,7 +59,7 @@ static void mspmm(xs_ctx *c, hipStream_t s, int k, const double *X, double *Y, i
                   const double *B, const double *W, const xs_mred *red, const xs_mstate *ms) {
   const void *vecs[4] = {X, Y, B, W};
   if (c->use_ell && xs_mspmm_ell_ok(c->ell, k, vecs, 4))
-    xs_mspmm_ell(s, c->ell, k, X, Y, epi, B, W, red, ms);
+    xs_mspmm_ell(s, c->ell, k, X, Y, epi, B, W, red, ms, c->spmm_cw);
   else
     xs_mspmm(s, c->A, k, X, Y, epi, B, W, red, ms);
 }

@@ -46,10 +46,14 @@ struct xs_ctx {
   bool tuned = false;           // SpMV format chosen by the autotuner
   int tune_chunk = 0;
   double tune_us = 0.0;
+  int tune_L = 0, tune_long_thr = 0;  // vector-CSR lanes per row, long-row threshold
+  double tune_trsv_us = 0.0;    // > 0 once the triangular solves are tuned
   bool use_graph = true;        // replay batches of iterations as HIP graphs
   bool trsv_schedule = true;    // SpTRSV groups in dependency-level order
   int trsv_wg = 256;            // persistent workgroups of the SpTRSV
   int trsv_prog = -1;           // SpTRSV group mode (xs_tsched::prog), -1 = by factor width
+  int trsv_gather = 0;          // SpTRSV gather width (4 or 16), 0 = by factor width
+  int spmm_cw = 0;              // k-RHS ELL SpMM columns per thread (2 or 4), 0 = 4 when k % 4 == 0
   int spmv_vec = -1;            // CSR kernel: -1 by row regularity, 0 LDS row blocks, 1 vector CSR
   int precision = 0;
   int use_ell = 0;
