@@ -396,6 +396,32 @@ def bench_c3(steps=3, warmup=1, exact=True):
             "mixed_fp32_speedup_vs_fp64": round(f64["time_to_rtol_s"] / out["mixed_fp32"]["time_to_rtol_s"], 3)}
 
 
+def bench_mixed(grid=128, steps=3, warmup=1):
+    """The headline system's precision comparison alone (for profiling the
+    per-iteration kernel budget): FP64 and the FP32 BiCGSTAB recurrence with
+    FP64 reliable updates, FP32- and BF16-stored sweep copies, 5/2 and 5/3
+    sweeps, re-factorisation inside every step as in the headline."""
+    from xsolver_amd import gen
+
+    A = gen.convdiff3d(grid)
+    _, b = gen.manufactured(A, 1)
+    out = {}
+    for name, prec, fs, sw in (("fp64", 0, 0, APPLY), ("mixed_fp32", 1, 0, APPLY),
+                               ("mixed_bf16", 1, 1, APPLY), ("mixed_bf16_5_3", 1, 1, (5, 3))):
+        r = run_gpu(A, b, "csr", steps, warmup, profile=False, precision=prec,
+                    factor_storage=fs, sweeps=sw)
+        out[name] = {"sec_per_step": round(r["sec_per_step"], 5),
+                     "pc_setup_ms": round(r["pc_setup_ms"], 3),
+                     "iterations": int(np.median(r["iterations"])),
+                     "true_relres": float(np.linalg.norm(b - gen.spmv(A, r["x"])) / np.linalg.norm(b))}
+        print(f"bench: mixed {name} done", file=sys.stderr, flush=True)
+    f64 = out["fp64"]["sec_per_step"]
+    for name in out:
+        out[name]["speedup_vs_fp64"] = round(f64 / out[name]["sec_per_step"], 3)
+    return {"workload": f"convdiff3d_{grid} BiCGSTAB+ParILU(0) rtol 1e-10, FP64 vs mixed",
+            "runs": out}
+
+
 def bench_adjoint(steps=3, warmup=1):
     """Adjoint on the 512^2 variable-coefficient diffusion system (kappa =
     exp of a squared-exponential Gaussian random field, length 0.1, seed 4),
@@ -529,11 +555,12 @@ def main():
     ap.add_argument("--no-sweeps", action="store_true",
                     help="skip the runs with the reference's Jacobi-sweep ILU apply")
     ap.add_argument("--workload", default="convdiff3d",
-                    choices=["convdiff3d", "batch512", "rhs32", "c3", "adjoint", "spmv"],
+                    choices=["convdiff3d", "batch512", "rhs32", "c3", "adjoint", "spmv", "mixed"],
                     help="convdiff3d (default: the headline line, with every other workload "
                          "under 'workloads'), or one workload alone (for profiling): batch512, "
                          "rhs32, c3 (27-point 100^3 GMRES+ParILUT, mixed precision vs FP64), "
-                         "adjoint (512^2), spmv (SpMV kernels warm / flushed-cache, copy ceiling)")
+                         "adjoint (512^2), spmv (SpMV kernels warm / flushed-cache, copy ceiling), "
+                         "mixed (the headline system, FP64 vs mixed precision only)")
     ap.add_argument("--headline-only", action="store_true",
                     help="skip the other workloads of the default line")
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -558,7 +585,8 @@ def main():
               "rhs32": lambda: bench_rhs32(args.grid, args.steps, args.warmup),
               "c3": lambda: bench_c3(args.steps, args.warmup),
               "adjoint": lambda: bench_adjoint(args.steps, args.warmup),
-              "spmv": lambda: bench_spmv_kernels()}
+              "spmv": lambda: bench_spmv_kernels(),
+              "mixed": lambda: bench_mixed(args.grid, args.steps, args.warmup)}
     if args.workload in single:
         rec = single[args.workload]()
         print(json.dumps({"workload": args.workload, "steps": args.steps,
