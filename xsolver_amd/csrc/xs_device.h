@@ -31,6 +31,13 @@ __device__ __forceinline__ double ld_ready(const double *p, unsigned *err) {
   }
   return __longlong_as_double((long long)u);
 }
+// lane t's value of v in every lane (t uniform): two v_readlane, no LDS trip
+__device__ __forceinline__ double wave_readlane_f64(double v, int t) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, t);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), t);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 __device__ __forceinline__ void st_ready(double *p, double v) {
   unsigned long long u = (unsigned long long)__double_as_longlong(v);
   if (u == XS_SENTINEL) u = 0x7FF8000000000000ull;
@@ -286,7 +293,9 @@ __device__ inline void do_finish(const xs_reduce &R, const double *tot, int NV) 
 #define XS_RED_GROUP 64
 // in-wave entries a triangular-solve lane keeps in registers, and the
 // off-group entries whose values it loads together before waiting
+#ifndef XS_INW
 #define XS_INW 8
+#endif
 #define XS_GATHER 4
 template <int NV>
 __device__ __forceinline__ void group_gather(const double *slots, unsigned m, const xs_reduce &R,

@@ -1051,14 +1051,35 @@ __device__ __forceinline__ void trsv_poll(const int *c, unsigned long long *u, c
   }
 }
 
-// acc minus the products, subtracted in entry order
+// Sum of G values (G a power of two) by a fixed pairwise tree: log2(G)
+// dependent additions instead of G.
+template <int G>
+__device__ __forceinline__ double trsv_tree(const double *p) {
+  double t[G];
+#pragma unroll
+  for (int q = 0; q < G; ++q) t[q] = p[q];
+#pragma unroll
+  for (int w = G / 2; w >= 1; w /= 2) {
+#pragma unroll
+    for (int q = 0; q < w; ++q) t[q] = t[q] + t[q + w];
+  }
+  return t[0];
+}
+
+// acc minus one batch of products (0 where there is no entry), the batch
+// summed by the fixed tree; batches always start at the same entries (the
+// row's first external entry in solve order, then every G), so every row's
+// value is the same sequence of roundings whatever the timing.  Products are
+// rounded on their own (no fused multiply-add): the lane-by-lane path forms
+// some of them ahead of the subtraction and must round them the same way.
 template <int G>
 __device__ __forceinline__ double trsv_sub(double acc, const int *c, const double *a,
                                            const unsigned long long *u) {
+#pragma clang fp contract(off)
+  double p[G];
 #pragma unroll
-  for (int q = 0; q < G; ++q)
-    if (c[q] >= 0) acc -= a[q] * __longlong_as_double((long long)u[q]);
-  return acc;
+  for (int q = 0; q < G; ++q) p[q] = c[q] >= 0 ? a[q] * __longlong_as_double((long long)u[q]) : 0.0;
+  return acc - trsv_tree<G>(p);
 }
 
 // acc - sum of lu[e] * x[ci[e]] over up to G entries from e, waiting for the
@@ -1076,7 +1097,6 @@ __device__ __forceinline__ double trsv_gather(double acc, int e, int stop, int d
   return trsv_sub<G>(acc, c, a, u);
 }
 
-#define XS_TRSV_AHEAD 4
 template <bool UPPER, int G>
 __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict__ rp,
                                                    const int *__restrict__ ci,
@@ -1086,6 +1106,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
                                                    const double *__restrict__ b, double *x,
                                                    unsigned *tickets, const xs_tsched S,
                                                    const xs_state *st) {
+#pragma clang fp contract(off)
   if (st != nullptr && st->done) return;
   unsigned *err = tickets + 4;  // wait-expired flag, 4 words past the tickets
   const int lane = threadIdx.x & (XS_WAVE - 1);
@@ -1157,67 +1178,97 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
       // order up to the first batch holding a value not yet published (that
       // batch stays in registers); at its turn the lane waits for the rest.
       // held: the first batch with an unpublished value and the batch after
-      // it (2G entries), so that a lane's turn rarely issues new loads
-      int pc[2 * G];
-      double pa[2 * G];
-      unsigned long long pu[2 * G];
+      // it (2G entries), so that a lane's turn rarely issues new loads.  A
+      // held entry keeps its column (hc, -1: none) and in hv its value until
+      // x of that column arrives, then the product (0 for no entry); hm has
+      // a bit per entry still waiting.  Products are rounded on their own
+      // (no fused multiply-add, as in trsv_sub), so an entry subtracts the
+      // same bits whether its product was formed up front or held.
+      int hc[2 * G];
+      double hv[2 * G];
+      unsigned hm = 0;
       bool stalled = false;
-      if (act) {
-        for (; UPPER ? ep > estop : ep < estop; ep += G * dir) {
-          trsv_issue<G>(ep, estop, dir, ci, lu, x, pc, pa, pu);
-          if (trsv_pending<G>(pc, pu)) { stalled = true; break; }
-          acc = trsv_sub<G>(acc, pc, pa, pu);
-        }
-      }
-      if (stalled) {
-        const int e2 = ep + G * dir;
-        if (UPPER ? e2 > estop : e2 < estop) {
-          trsv_issue<G>(e2, estop, dir, ci, lu, x, pc + G, pa + G, pu + G);
-        } else {
+      {
+        int pc[G];
+        double pa[G];
+        unsigned long long pu[G];
 #pragma unroll
-          for (int q = G; q < 2 * G; ++q) { pc[q] = -1; pa[q] = 0.0; pu[q] = 0; }
+        for (int q = 0; q < G; ++q) { pc[q] = -1; pa[q] = 0.0; pu[q] = 0; }
+        if (act) {
+          for (; UPPER ? ep > estop : ep < estop; ep += G * dir) {
+            trsv_issue<G>(ep, estop, dir, ci, lu, x, pc, pa, pu);
+            if (trsv_pending<G>(pc, pu)) { stalled = true; break; }
+            acc = trsv_sub<G>(acc, pc, pa, pu);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+          const bool wait = pc[q] >= 0 && pu[q] == XS_SENTINEL;
+          hc[q] = stalled ? pc[q] : -1;
+          hv[q] = (!stalled || pc[q] < 0) ? 0.0
+                  : wait ? pa[q] : pa[q] * __longlong_as_double((long long)pu[q]);
+          if (stalled && wait) hm |= 1u << q;
+        }
+        const int e2 = ep + G * dir;
+        const bool more = stalled && (UPPER ? e2 > estop : e2 < estop);
+        if (more) trsv_issue<G>(e2, estop, dir, ci, lu, x, pc, pa, pu);
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+          const bool wait = pc[q] >= 0 && pu[q] == XS_SENTINEL;
+          hc[G + q] = more ? pc[q] : -1;
+          hv[G + q] = (!more || pc[q] < 0) ? 0.0
+                      : wait ? pa[q] : pa[q] * __longlong_as_double((long long)pu[q]);
+          if (more && wait) hm |= 1u << (G + q);
         }
       }
       double ain = 0.0;   // the in-wave products, subtracted in solve order
       int left = act ? nint - XS_INW : 0;
       int enext = UPPER ? ee - 1 - XS_INW : eb + XS_INW;
+      // The turn check reads only hm (a register refreshed where polls
+      // return), so a turn whose values are known to be there waits for no
+      // memory result: on this target a wait for any load also waits for
+      // the previous turn's publishing store.
       for (int k = 0; k < nrows; ++k) {
-        // while lane k's held batches still have an unpublished value, the
-        // next XS_TRSV_AHEAD lanes re-poll theirs in the same round trip, so
-        // they rarely wait at their own turn (a window, not the whole wave:
-        // a thousand waves re-polling every lane would flood the memory
-        // system with polls)
+        // while lane k's held batches still have an unpublished value, every
+        // lane from k on re-polls its own in the same round trip, so that
+        // they rarely wait at their own turn: a round trip (~2.5 us on the
+        // 100^3 ParILUT factors) is worth more than the polls it carries --
+        // re-polling only the next 4 lanes was 8% slower there, and the same
+        // on the 7-point factors
         unsigned polls = 0;
-        for (;;) {
-          const int kwait = __shfl((stalled && trsv_pending<2 * G>(pc, pu)) ? 1 : 0, k, XS_WAVE);
-          if (!kwait) break;
+        bool expired = false;
+        while (__builtin_amdgcn_readlane((int)hm, k)) {
           if (++polls > XS_SPIN_LIMIT) {   // bounded wait: flag it, continue with NaN
-            if (lane == k) {
-              atomicOr(err, 1u);
-#pragma unroll
-              for (int q = 0; q < 2 * G; ++q)
-                if (pu[q] == XS_SENTINEL) pu[q] = 0x7FF8000000000000ull;
-            }
+            if (lane == k) atomicOr(err, 1u);
+            expired = true;
             break;
           }
           __builtin_amdgcn_s_sleep(2);
-          if (stalled && lane >= k && lane < k + XS_TRSV_AHEAD) {
+          if (hm && lane >= k) {
+            unsigned long long pu[2 * G];
 #pragma unroll
             for (int q = 0; q < 2 * G; ++q)
-              if (pc[q] >= 0 && pu[q] == XS_SENTINEL) pu[q] = ld_relaxed(x + pc[q]);
+              pu[q] = ((hm >> q) & 1u) ? ld_relaxed(x + hc[q]) : XS_SENTINEL;
+#pragma unroll
+            for (int q = 0; q < 2 * G; ++q)
+              if (pu[q] != XS_SENTINEL) {
+                hv[q] = hv[q] * __longlong_as_double((long long)pu[q]);
+                hm &= ~(1u << q);
+              }
           }
         }
         double vk = 0.0;
         if (lane == k) {
           if (stalled) {
-            acc = trsv_sub<2 * G>(acc, pc, pa, pu);
+            acc -= trsv_tree<G>(hv);
+            acc -= trsv_tree<G>(hv + G);
             for (ep += 2 * G * dir; UPPER ? ep > estop : ep < estop; ep += G * dir)
               acc = trsv_gather<G>(acc, ep, estop, dir, ci, lu, x, err);
           }
-          vk = (acc + ain) * di;
+          vk = expired ? __longlong_as_double(0x7FF8000000000000ll) : (acc + ain) * di;
           st_ready(x + r, vk);
         }
-        const double xk = __shfl(vk, k, XS_WAVE);
+        const double xk = wave_readlane_f64(vk, k);
         const int rk = UPPER ? r1 - 1 - k : r0 + k;
         if (iw_c[0] == rk) {
           ain -= iw_v[0] * xk;
