@@ -287,58 +287,85 @@ void xs_launch_spmv_ell32(hipStream_t s, const xs_ell &E, const float *v32, cons
 #define XS_SENTINEL32 0xFFA0DEADu
 #define XS_SPIN_LIMIT32 (1u << 22)
 
-__device__ __forceinline__ float ld_ready32(const float *p, unsigned *err) {
-  unsigned u = __hip_atomic_load(reinterpret_cast<unsigned *>(const_cast<float *>(p)),
-                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  unsigned polls = 0;
-  while (u == XS_SENTINEL32) {
-    if (++polls > XS_SPIN_LIMIT32) {
-      atomicOr(err, 1u);
-      u = 0x7FC00000u;
-      break;
-    }
-    __builtin_amdgcn_s_sleep(1);
-    u = __hip_atomic_load(reinterpret_cast<unsigned *>(const_cast<float *>(p)), __ATOMIC_RELAXED,
-                          __HIP_MEMORY_SCOPE_AGENT);
-  }
-  return __uint_as_float(u);
-}
 __device__ __forceinline__ void st_ready32(float *p, float v) {
   unsigned u = __float_as_uint(v);
   if (u == XS_SENTINEL32) u = 0x7FC00000u;
   __hip_atomic_store(reinterpret_cast<unsigned *>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// acc - sum of lu[e] * x[ci[e]] over up to G entries e, e+dir, ... (stopping
-// at `stop`), subtracted in that order: the loads are issued together and
-// the values still holding the sentinel are re-polled together.
+__device__ __forceinline__ unsigned ld_relaxed32(const float *p) {
+  return __hip_atomic_load(reinterpret_cast<unsigned *>(const_cast<float *>(p)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Up to G entries e, e+dir, ... (stopping at `stop`) of a row: columns c (-1
+// past the stop), values a and the current x of those columns u (possibly
+// the sentinel); branch-free loads, all in flight at once (as trsv_issue in
+// xs_kernels.hip).
 template <int G, class TF>
-__device__ __forceinline__ float trsv_gather_32(float acc, int e, int stop, int dir,
-                                                 const int *__restrict__ ci,
-                                                 const TF *__restrict__ lu, const float *x,
-                                                 unsigned *err) {
-  unsigned u[G];
-  int c[G];
-  float a[G];
+__device__ __forceinline__ void trsv_issue32(int e, int stop, int dir, const int *__restrict__ ci,
+                                             const TF *__restrict__ lu, const float *x, int *c,
+                                             float *a, unsigned *u) {
 #pragma unroll
   for (int q = 0; q < G; ++q) {
     const int eq = e + q * dir;
     const bool in = dir > 0 ? eq < stop : eq > stop;
-    c[q] = in ? ci[eq] : -1;
-    a[q] = in ? fl(lu, eq) : 0.f;
+    const int cv = ci[in ? eq : e];
+    const float av = fl(lu, in ? eq : e);
+    c[q] = in ? cv : -1;
+    a[q] = in ? av : 0.f;
   }
-  auto ld = [&](int q) {
-    return __hip_atomic_load(reinterpret_cast<unsigned *>(const_cast<float *>(x + c[q])),
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  };
 #pragma unroll
-  for (int q = 0; q < G; ++q) u[q] = c[q] >= 0 ? ld(q) : 0u;
+  for (int q = 0; q < G; ++q) u[q] = ld_relaxed32(x + (c[q] >= 0 ? c[q] : c[0]));
+}
+
+template <int G>
+__device__ __forceinline__ bool trsv_pending32(const int *c, const unsigned *u) {
+  bool pend = false;
+#pragma unroll
+  for (int q = 0; q < G; ++q) pend |= (c[q] >= 0 && u[q] == XS_SENTINEL32);
+  return pend;
+}
+
+// fixed pairwise tree over G values (G a power of two)
+template <int G>
+__device__ __forceinline__ float trsv_tree32(const float *p) {
+  float t[G];
+#pragma unroll
+  for (int q = 0; q < G; ++q) t[q] = p[q];
+#pragma unroll
+  for (int w = G / 2; w >= 1; w /= 2) {
+#pragma unroll
+    for (int q = 0; q < w; ++q) t[q] = t[q] + t[q + w];
+  }
+  return t[0];
+}
+
+// acc minus one batch of products (0 where there is no entry), summed by the
+// fixed tree, each product rounded on its own: the same roundings whatever
+// the timing (as trsv_sub in xs_kernels.hip)
+template <int G>
+__device__ __forceinline__ float trsv_sub32(float acc, const int *c, const float *a,
+                                            const unsigned *u) {
+#pragma clang fp contract(off)
+  float p[G];
+#pragma unroll
+  for (int q = 0; q < G; ++q) p[q] = c[q] >= 0 ? a[q] * __uint_as_float(u[q]) : 0.f;
+  return acc - trsv_tree32<G>(p);
+}
+
+// acc - one batch of products, waiting (bounded) for the unpublished values
+template <int G, class TF>
+__device__ __forceinline__ float trsv_gather32(float acc, int e, int stop, int dir,
+                                               const int *__restrict__ ci,
+                                               const TF *__restrict__ lu, const float *x,
+                                               unsigned *err) {
+  unsigned u[G];
+  int c[G];
+  float a[G];
+  trsv_issue32<G>(e, stop, dir, ci, lu, x, c, a, u);
   unsigned polls = 0;
-  for (;;) {
-    bool pend = false;
-#pragma unroll
-    for (int q = 0; q < G; ++q) pend |= (c[q] >= 0 && u[q] == XS_SENTINEL32);
-    if (!pend) break;
+  while (trsv_pending32<G>(c, u)) {
     if (++polls > XS_SPIN_LIMIT32) {  // bounded wait: flag it, continue with NaN
       atomicOr(err, 1u);
 #pragma unroll
@@ -349,16 +376,17 @@ __device__ __forceinline__ float trsv_gather_32(float acc, int e, int stop, int 
     __builtin_amdgcn_s_sleep(1);
 #pragma unroll
     for (int q = 0; q < G; ++q)
-      if (c[q] >= 0 && u[q] == XS_SENTINEL32) u[q] = ld(q);
+      if (c[q] >= 0 && u[q] == XS_SENTINEL32) u[q] = ld_relaxed32(x + c[q]);
   }
-#pragma unroll
-  for (int q = 0; q < G; ++q)
-    if (c[q] >= 0) acc -= a[q] * __uint_as_float(u[q]);
-  return acc;
+  return trsv_sub32<G>(acc, c, a, u);
 }
 
 // FP32 sync-free solve: the algorithm of k_trsv (xs_kernels.hip) with FP32
-// vectors and FP32/BF16/FP16 factor values.
+// vectors and FP32/BF16/FP16 factor values -- row groups handed out by
+// tickets, chain groups by an affine scan after all external values are in,
+// the other groups lane by lane with every row published as soon as it is
+// final (two batches of external entries held, a bit mask of the entries
+// still unpublished, every later lane re-polled per round trip).
 template <bool UPPER, class TF, int G>
 __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restrict__ rp,
                                                      const int *__restrict__ ci,
@@ -368,6 +396,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
                                                      const float *__restrict__ b, float *x,
                                                      unsigned *tickets, const xs_tsched S,
                                                      const xs_state *st) {
+#pragma clang fp contract(off)
   if (st != nullptr && st->done) return;
   unsigned *err = tickets + 4;
   const int lane = threadIdx.x & (XS_WAVE - 1);
@@ -384,14 +413,14 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
     const bool act = lane < nrows;
     const int r = UPPER ? r1 - 1 - lane : r0 + lane;
     float acc = 0.f, di = 1.f;
-    int eb = 0, ee = 0;
-    int iw_c[XS_INW];    // first in-wave entries in solve order, loaded before any wait
+    int eb = 0, ee = 0;   // in-wave entries [eb, ee)
+    int xb = 0, xe = 0;   // entries outside the group
+    int iw_c[XS_INW];     // first in-wave entries in solve order, loaded before any wait
     float iw_v[XS_INW];
 #pragma unroll
     for (int q = 0; q < XS_INW; ++q) { iw_c[q] = -1; iw_v[q] = 0.f; }
     if (act) {
       acc = b[r];
-      int xb, xe;
       if (!UPPER) {
         xb = rp[r];
         const int end = dpos[r];
@@ -411,30 +440,104 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
         const int e = UPPER ? ee - 1 - q : eb + q;
         if (q < ee - eb) { iw_c[q] = ci[e]; iw_v[q] = fl(lu, e); }
       }
-      if (!UPPER) {
-        for (int e = xb; e < xe; e += G) acc = trsv_gather_32<G>(acc, e, xe, 1, ci, lu, x, err);
-      } else {
-        for (int e = xe - 1; e >= xb; e -= G) acc = trsv_gather_32<G>(acc, e, xb - 1, -1, ci, lu, x, err);
-      }
     }
+    // external entries in solve order: L from xb up, U from xe - 1 down
+    const int dir = UPPER ? -1 : 1;
+    const int estop = UPPER ? xb - 1 : xe;
+    int ep = UPPER ? xe - 1 : xb;
     const int nint = ee - eb;
     const int nb = UPPER ? r + 1 : r - 1;
     const bool chain = !act || nint == 0 || (nint == 1 && iw_c[0] == nb);
-    float val;
-    if (__all(chain)) {
+    if (!S.prog && __all(chain)) {
+      if (act)
+        for (; UPPER ? ep > estop : ep < estop; ep += G * dir)
+          acc = trsv_gather32<G>(acc, ep, estop, dir, ci, lu, x, err);
       float c = act ? di * acc : 0.f;
       float m = (act && nint == 1) ? -di * iw_v[0] : 0.f;
       wave_affine_scan_f(c, m);
-      val = c;
+      if (act) st_ready32(x + r, c);
     } else {
-      // broadcast loop over register-resident in-wave entries (as k_trsv)
+      // held: the first batch with an unpublished value and the one after
+      // it; hc the columns (-1: none), hv the value until x of the column
+      // arrives, then the product (0 for no entry), hm the waiting entries
+      int hc[2 * G];
+      float hv[2 * G];
+      unsigned hm = 0;
+      bool stalled = false;
+      {
+        int pc[G];
+        float pa[G];
+        unsigned pu[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) { pc[q] = -1; pa[q] = 0.f; pu[q] = 0u; }
+        if (act) {
+          for (; UPPER ? ep > estop : ep < estop; ep += G * dir) {
+            trsv_issue32<G>(ep, estop, dir, ci, lu, x, pc, pa, pu);
+            if (trsv_pending32<G>(pc, pu)) { stalled = true; break; }
+            acc = trsv_sub32<G>(acc, pc, pa, pu);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+          const bool wait = pc[q] >= 0 && pu[q] == XS_SENTINEL32;
+          hc[q] = stalled ? pc[q] : -1;
+          hv[q] = (!stalled || pc[q] < 0) ? 0.f : wait ? pa[q] : pa[q] * __uint_as_float(pu[q]);
+          if (stalled && wait) hm |= 1u << q;
+        }
+        const int e2 = ep + G * dir;
+        const bool more = stalled && (UPPER ? e2 > estop : e2 < estop);
+        if (more) trsv_issue32<G>(e2, estop, dir, ci, lu, x, pc, pa, pu);
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+          const bool wait = pc[q] >= 0 && pu[q] == XS_SENTINEL32;
+          hc[G + q] = more ? pc[q] : -1;
+          hv[G + q] = (!more || pc[q] < 0) ? 0.f : wait ? pa[q] : pa[q] * __uint_as_float(pu[q]);
+          if (more && wait) hm |= 1u << (G + q);
+        }
+      }
+      float ain = 0.f;   // the in-wave products, subtracted in solve order
       int left = act ? nint - XS_INW : 0;
       int enext = UPPER ? ee - 1 - XS_INW : eb + XS_INW;
       for (int k = 0; k < nrows; ++k) {
-        const float xk = __shfl(acc * di, k, XS_WAVE);
+        // the turn check reads only hm (no memory result: a wait for a load
+        // would also wait for the previous turn's publishing store)
+        unsigned polls = 0;
+        bool expired = false;
+        while (__builtin_amdgcn_readlane((int)hm, k)) {
+          if (++polls > XS_SPIN_LIMIT32) {   // bounded wait: flag it, continue with NaN
+            if (lane == k) atomicOr(err, 1u);
+            expired = true;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+          if (hm && lane >= k) {
+            unsigned pu[2 * G];
+#pragma unroll
+            for (int q = 0; q < 2 * G; ++q)
+              pu[q] = ((hm >> q) & 1u) ? ld_relaxed32(x + hc[q]) : XS_SENTINEL32;
+#pragma unroll
+            for (int q = 0; q < 2 * G; ++q)
+              if (pu[q] != XS_SENTINEL32) {
+                hv[q] = hv[q] * __uint_as_float(pu[q]);
+                hm &= ~(1u << q);
+              }
+          }
+        }
+        float vk = 0.f;
+        if (lane == k) {
+          if (stalled) {
+            acc -= trsv_tree32<G>(hv);
+            acc -= trsv_tree32<G>(hv + G);
+            for (ep += 2 * G * dir; UPPER ? ep > estop : ep < estop; ep += G * dir)
+              acc = trsv_gather32<G>(acc, ep, estop, dir, ci, lu, x, err);
+          }
+          vk = expired ? __uint_as_float(0x7FC00000u) : (acc + ain) * di;
+          st_ready32(x + r, vk);
+        }
+        const float xk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vk), k));
         const int rk = UPPER ? r1 - 1 - k : r0 + k;
         if (iw_c[0] == rk) {
-          acc -= iw_v[0] * xk;
+          ain -= iw_v[0] * xk;
 #pragma unroll
           for (int q = 0; q + 1 < XS_INW; ++q) { iw_c[q] = iw_c[q + 1]; iw_v[q] = iw_v[q + 1]; }
           if (left > 0) {
@@ -447,9 +550,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
           }
         }
       }
-      val = acc * di;
     }
-    if (act) st_ready32(x + r, val);
   }
   }
   trsv_retire(tickets);
