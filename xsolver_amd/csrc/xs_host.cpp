@@ -69,6 +69,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.ucp); dfree(F.uri); dfree(F.upos);
   dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own); dfree(F.trp); dfree(F.tci);
   dfree(F.tdpos); dfree(F.tlu); dfree(F.scale); dfree(F.tperm);
+  dfree(F.tlgb); dfree(F.tugb); dfree(F.tlsched); dfree(F.tusched);
   dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);
   dfree(F.app); dfree(F.apd); dfree(F.cmap); dfree(F.lc); dfree(F.mpos);
@@ -1889,9 +1890,29 @@ int xs_build_factor_transpose(xs_ctx *c) {
       if (tci[e] == i) tdpos[i] = e;
   int rc;
   dfree(F.trp); dfree(F.tci); dfree(F.tdpos); dfree(F.tlu); dfree(F.tperm);
+  dfree(F.tlgb); dfree(F.tugb); dfree(F.tlsched); dfree(F.tusched);
+  F.tlgroups = F.tugroups = 0;
   if ((rc = dalloc(&F.trp, n + 1)) || (rc = dalloc(&F.tci, nnz)) || (rc = dalloc(&F.tdpos, n)) ||
       (rc = dalloc(&F.tlu, nnz)) || (rc = dalloc(&F.tperm, nnz)))
     return rc;
+  // row groups and level order of the transposed triangular solves
+  {
+    std::vector<int> gb, ord;
+    for (int part = 0; part < 2; ++part) {
+      const bool upper = part == 1;
+      trsv_groups(n, trp.data(), tci.data(), tdpos.data(), upper, gb);
+      const int G = (int)gb.size() - 1;
+      int *&dgb = upper ? F.tugb : F.tlgb;
+      if ((rc = dalloc(&dgb, (size_t)G + 1))) return rc;
+      XS_HIP(hipMemcpy(dgb, gb.data(), sizeof(int) * (G + 1), hipMemcpyHostToDevice));
+      (upper ? F.tugroups : F.tlgroups) = G;
+      if (!c->trsv_schedule) continue;
+      group_order(n, trp.data(), tci.data(), tdpos.data(), upper, gb, ord);
+      int *&ds = upper ? F.tusched : F.tlsched;
+      if ((rc = dalloc(&ds, (size_t)G))) return rc;
+      XS_HIP(hipMemcpy(ds, ord.data(), sizeof(int) * G, hipMemcpyHostToDevice));
+    }
+  }
   XS_HIP(hipMemcpy(F.trp, trp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
   XS_HIP(hipMemcpy(F.tci, tci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
   XS_HIP(hipMemcpy(F.tdpos, tdpos.data(), sizeof(int) * n, hipMemcpyHostToDevice));

@@ -183,10 +183,13 @@ struct xs_ilu {
   int trsv_gather = 0;      // xs_tsched::gather (4 or 16); 0 = by factor width
   xs_tell tl, tu;           // sweep copies of L and U (built on demand)
   bool tell_shape = false;  // tl/tu widths and buffers match the pattern
-  // transposed factors for the adjoint (built on demand)
+  // transposed factors for the adjoint (built on demand), with the row
+  // groups and level order of their triangular solves (as lgb / lsched)
   bool have_t = false;
   int *trp = nullptr, *tci = nullptr, *tdpos = nullptr, *tperm = nullptr;
   double *tlu = nullptr;
+  int *tlgb = nullptr, *tugb = nullptr, *tlsched = nullptr, *tusched = nullptr;
+  int tlgroups = 0, tugroups = 0;
 };
 
 // the gather width of a factor's triangular solves: the set one, else by width

@@ -1319,6 +1319,9 @@ void xs_trsv_upper(hipStream_t s, const xs_ilu &F, const double *b, double *x, u
 void xs_trsv_lower_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
   xs_tsched S;
+  S.gb = F.tlgb;
+  S.ngroups = F.tlgroups;
+  S.order = F.tlsched;
   S.max_wg = F.trsv_wg;
   S.gather = xs_ilu_gather(F);
   S.prog = F.trsv_prog >= 0 ? F.trsv_prog : (S.gather == 16 ? 1 : 0);
@@ -1327,6 +1330,9 @@ void xs_trsv_lower_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
 void xs_trsv_upper_t(hipStream_t s, const xs_ilu &F, const double *b, double *x,
                      unsigned *tickets, const xs_state *st) {
   xs_tsched S;
+  S.gb = F.tugb;
+  S.ngroups = F.tugroups;
+  S.order = F.tusched;
   S.max_wg = F.trsv_wg;
   S.gather = xs_ilu_gather(F);
   S.prog = F.trsv_prog >= 0 ? F.trsv_prog : (S.gather == 16 ? 1 : 0);
