@@ -420,6 +420,28 @@ def test_iterative_ilu_apply_solve(dev, precision, sweeps):
     g["S"].close()
 
 
+def test_bicgstab_fused_updates_same_result(dev):
+    """With the sweep apply BiCGSTAB forms p (and s) inside the first L pass,
+    p alternating between two buffers; an odd batch of iterations per
+    convergence check (7) keeps the separate update kernels instead.  Both
+    give bit for bit the same iterate and iteration count, and the graph
+    replays of the fused batch (8) keep the buffers' alternation.  (The
+    factorisation runs its synchronous sweeps: the asynchronous ones are
+    not reproducible bit for bit by design.)"""
+    A = gen.convdiff3d(24)
+    xs, b = gen.manufactured(A, 3)
+    out = []
+    for every in (8, 7):
+        g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000, lsolve_sweeps=5, usolve_sweeps=2,
+                      check_every=every, parilu_async=0)
+        say(f"[bicgstab fused] check_every={every} it={g['it']} "
+            f"relres={true_relres(A, b, g['x']):.2e}")
+        assert g["st"] == 0 and true_relres(A, b, g["x"]) < 1e-10
+        out.append((g["x"], g["it"]))
+        g["S"].close()
+    assert out[0][1] == out[1][1] and np.array_equal(out[0][0], out[1][0])
+
+
 @pytest.mark.parametrize("factor", [0, 1, 2], ids=["f32", "bf16", "f16"])
 def test_mixed_sweep_apply_factor_storage(dev, factor):
     """FP32-inner refinement with the sweep apply reading the factors' sweep

@@ -1575,13 +1575,41 @@ void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state
   }
 }
 
+bool xs_pc_fuse_l(const xs_ctx *c) {
+  return xs_pc_factored(c->pc) && !c->use_t && c->lsweeps >= 2 && c->tell_ready;
+}
+
+// BiCGSTAB's p = r + omega (pin - alpha u) into pout, then y = M^{-1} p with
+// the update formed inside the first L pass (one kernel less, same values).
+void xs_op_pc_p(xs_ctx *c, const double *r, const double *u, const double *pin, double *pout,
+                double *y, double *tmp, xs_state *st) {
+  double *w = xs_ctx_vec(c, 10);
+  const int k = c->lsweeps;
+  double *cur = ((k - 2) % 2 == 0) ? tmp : w;
+  xs_prof_begin(c, 2);
+  xs_tri_sweep_ell_p(c->stream, c->ilu, r, u, pin, pout, cur, st);
+  for (int j = 3; j <= k; ++j) {
+    double *nxt = (cur == tmp) ? w : tmp;
+    xs_tri_sweep_ell(c->stream, c->ilu, false, false, pout, cur, nxt, st);
+    cur = nxt;
+  }
+  xs_prof_end(c, 2);
+  xs_prof_begin(c, 3);
+  if (c->usweeps >= 0) {
+    apply_sweeps(c, true, c->usweeps, tmp, y, w, st);
+  } else {
+    xs_fill_sentinel(c->stream, y, c->A.n);
+    xs_trsv_upper(c->stream, c->ilu, tmp, y, c->counters + 4, st);
+  }
+  xs_prof_end(c, 3);
+}
+
 // BiCGSTAB's s = r - gama u followed by y = M^{-1} s.  With an ILU applied
 // by two or more L sweeps over the ELL copies, s is formed inside the first
 // L pass (one kernel less, same values); otherwise the s update runs first.
 void xs_op_pc_s(xs_ctx *c, const double *r, const double *u, double *sv, double *y, double *tmp,
                 xs_state *st) {
-  const bool fuse = xs_pc_factored(c->pc) && !c->use_t &&
-                    c->lsweeps >= 2 && c->tell_ready && st != nullptr;
+  const bool fuse = xs_pc_fuse_l(c) && st != nullptr;
   if (!fuse) {
     xs_bicg_s(c->stream, c->A.n, r, u, sv, nullptr, nullptr, st);
     xs_op_pc(c, sv, y, tmp, st);

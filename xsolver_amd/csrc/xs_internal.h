@@ -308,6 +308,10 @@ void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, co
 // in the pass and written to sv
 void xs_tri_sweep_ell_s(hipStream_t s, const xs_ilu &F, const double *r, const double *u,
                         double *sv, double *yn, const xs_state *st);
+// the fused first L pass with BiCGSTAB's p = r + omega (pin - alpha u) formed
+// in the pass and written to pout (pin != pout)
+void xs_tri_sweep_ell_p(hipStream_t s, const xs_ilu &F, const double *r, const double *u,
+                        const double *pin, double *pout, double *yn, const xs_state *st);
 void xs_permute_vals(hipStream_t s, int nnz, const int *perm, const double *src, double *dst);
 void xs_grad_vals(hipStream_t s, const xs_csr &A, const double *lam, const double *x, double *g);
 void xs_jacobi_dinv(hipStream_t s, const xs_csr &A, double *dinv, int *d_bad);

@@ -538,6 +538,14 @@ void xs_launch_dot(hipStream_t s, int n, const double *a, const double *b, const
 
 // BiCGSTAB p = r + omega (p - alpha u) [+ z = dinv o p]
 // (DeviceVector.cpp:55-60 kernel1HIP; jacobiHIP :17-23)
+// BiCGSTAB's direction update for one entry, p = r + omega (p - alpha u)
+// (the reference's names for beta and omega), written with explicit fused
+// multiply-adds so that every kernel forming it rounds the same way.
+__device__ __forceinline__ double bicg_p_val(double r, double p, double u, double omega,
+                                             double alpha) {
+  return fma(omega, fma(-alpha, u, p), r);
+}
+
 __global__ __launch_bounds__(XS_BLOCK) void k_bicg_p(int n, const double *__restrict__ r,
                                                      const double *__restrict__ u,
                                                      double *__restrict__ p,
@@ -546,7 +554,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_bicg_p(int n, const double *__rest
   if (st->done) return;
   const double omega = st->sc[S_OMEGA], alpha = st->sc[S_ALPHA];
   for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) {
-    const double pi = r[i] + omega * (p[i] - alpha * u[i]);
+    const double pi = bicg_p_val(r[i], p[i], u[i], omega, alpha);
     p[i] = pi;
     if (dinv) z[i] = dinv[i] * pi;
   }
@@ -1515,6 +1523,71 @@ void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, co
   else { if (fused) XS_SWE(false, true); else XS_SWE(false, false); }
 #undef XS_SWE
 #undef XS_SWK
+}
+
+// The fused first L pass with BiCGSTAB's direction update folded in: the
+// right-hand side p_j = r_j + omega (pin_j - alpha u_j) (bicg_p_val, as in
+// k_bicg_p) is formed at every row read, written to pout for the row itself;
+// pin and pout are different vectors (rows read their neighbours' pin).
+// Otherwise the arithmetic of k_tri_sweep_ell<false, true, O16>.
+template <bool O16>
+__global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell_px(int n, int ld, int W,
+                                                               const int *__restrict__ eci,
+                                                               const short *__restrict__ eco,
+                                                               const double *__restrict__ ev,
+                                                               const int *__restrict__ ww,
+                                                               const double *__restrict__ r,
+                                                               const double *__restrict__ u,
+                                                               const double *__restrict__ pin,
+                                                               double *__restrict__ pout,
+                                                               double *__restrict__ yn,
+                                                               const xs_state *st) {
+  if (st->done) return;
+  const int i = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
+  if (i >= n) return;
+  const double omega = st->sc[S_OMEGA], alpha = st->sc[S_ALPHA];
+  const double bi = bicg_p_val(r[i], pin[i], u[i], omega, alpha);
+  pout[i] = bi;
+  W = ww ? ww[i / XS_WAVE] : W;
+  double acc = 0.0;
+  for (int k0 = 0; k0 < W; k0 += 4) {
+    int c[4];
+    double a[4], y[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const size_t p = (size_t)min(k0 + q, W - 1) * ld + i;
+      if (O16) {
+        const int o = (int)eco[p];
+        c[q] = (k0 + q >= W || o == XS_OFF16_PAD) ? -1 : i + o;
+      } else {
+        c[q] = k0 + q < W ? eci[p] : -1;
+      }
+      a[q] = ev[p];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int cc = c[q] >= 0 ? c[q] : i;
+      y[q] = bicg_p_val(r[cc], pin[cc], u[cc], omega, alpha) - 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c[q] >= 0) acc += a[q] * y[q];
+  }
+  yn[i] = bi - acc;
+}
+
+void xs_tri_sweep_ell_p(hipStream_t s, const xs_ilu &F, const double *r, const double *u,
+                        const double *pin, double *pout, double *yn, const xs_state *st) {
+  const xs_tell &E = F.tl;
+  XS_GUARD((E.W == 0 || (E.ci && E.v)) && E.ld >= F.n && r && u && pin && pout && pin != pout &&
+           yn && st);
+  const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+  if (E.co)
+    hipLaunchKernelGGL((k_tri_sweep_ell_px<true>), grid, blk, 0, s, F.n, E.ld, E.W, E.ci, E.co,
+                       E.v, E.ww, r, u, pin, pout, yn, st);
+  else
+    hipLaunchKernelGGL((k_tri_sweep_ell_px<false>), grid, blk, 0, s, F.n, E.ld, E.W, E.ci, E.co,
+                       E.v, E.ww, r, u, pin, pout, yn, st);
 }
 
 void xs_tri_sweep_ell_s(hipStream_t s, const xs_ilu &F, const double *r, const double *u,

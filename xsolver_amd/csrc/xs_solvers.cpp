@@ -96,9 +96,21 @@ int xs_solve_bicgstab(xs_ctx *c, const double *b, double *x) {
   const xs_reduce rg = xs_ctx_reduce(c, F_BICG_GAMA, 0);
   const xs_reduce ra = xs_ctx_reduce(c, F_BICG_ALPHA, 0);
   const xs_reduce rr = xs_ctx_reduce(c, F_BICG_RES, 0);
+  // With the sweep apply, p's update is formed inside the first L pass; the
+  // pass reads the old p at neighbouring rows, so the new p goes to the other
+  // of two buffers, alternating.  An even batch of captured iterations ends
+  // on the parity it started with, which replays of the batch rely on.
+  const bool pfuse = !jac && xs_pc_fuse_l(c) && c->check_every % 2 == 0;
+  double *p2 = xs_ctx_vec(c, 9);
+  bool odd = false;
   return drive(c, [&]() {
-    xs_bicg_p(s, n, r, u, p, jac, z, c->st);                    // p = r + w (p - a u)
-    if (!jac) xs_op_pc(c, p, z, t, c->st);                      // z = M^-1 p
+    if (pfuse) {
+      xs_op_pc_p(c, r, u, odd ? p2 : p, odd ? p : p2, z, t, c->st);   // p update, z = M^-1 p
+      odd = !odd;
+    } else {
+      xs_bicg_p(s, n, r, u, p, jac, z, c->st);                  // p = r + w (p - a u)
+      if (!jac) xs_op_pc(c, p, z, t, c->st);                    // z = M^-1 p
+    }
     xs_op_spmv(c, z, u, EPI_DOT1, nullptr, rh, &rg);            // u = A z, gama
     if (jac) xs_bicg_s(s, n, r, u, sv, jac, z1, c->st);         // s = r - gama u, z1 = D s
     else xs_op_pc_s(c, r, u, sv, z1, t, c->st);                 // s = r - gama u, z1 = M^-1 s

@@ -200,6 +200,13 @@ void xs_op_pc(xs_ctx *c, const double *x, double *y, double *tmp, const xs_state
 // s = r - gama u (gama in st), then y = M^{-1} s
 void xs_op_pc_s(xs_ctx *c, const double *r, const double *u, double *sv, double *y, double *tmp,
                 xs_state *st);
+// true when the ILU is applied by two or more L sweeps over the ELL copies:
+// BiCGSTAB's vector updates can then be formed inside the first L pass
+bool xs_pc_fuse_l(const xs_ctx *c);
+// BiCGSTAB's p = r + omega (pin - alpha u) into pout, then y = M^{-1} p, the
+// update formed inside the first L pass (xs_pc_fuse_l must hold)
+void xs_op_pc_p(xs_ctx *c, const double *r, const double *u, const double *pin, double *pout,
+                double *y, double *tmp, xs_state *st);
 int xs_op_pc_t(xs_ctx *c, const double *x, double *y, double *tmp, hipStream_t s);
 
 int xs_solve_cg(xs_ctx *c, const double *b, double *x);
