@@ -1195,7 +1195,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
       int hc[2 * G];
       double hv[2 * G];
       unsigned hm = 0;
-      bool stalled = false;
+      bool stalled = false, more = false;   // more: the second batch is held
       {
         int pc[G];
         double pa[G];
@@ -1218,7 +1218,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
           if (stalled && wait) hm |= 1u << q;
         }
         const int e2 = ep + G * dir;
-        const bool more = stalled && (UPPER ? e2 > estop : e2 < estop);
+        more = stalled && (UPPER ? e2 > estop : e2 < estop);
         if (more) trsv_issue<G>(e2, estop, dir, ci, lu, x, pc, pa, pu);
 #pragma unroll
         for (int q = 0; q < G; ++q) {
@@ -1269,7 +1269,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv(int n, const int *__restrict_
         if (lane == k) {
           if (stalled) {
             acc -= trsv_tree<G>(hv);
-            acc -= trsv_tree<G>(hv + G);
+            if (more) acc -= trsv_tree<G>(hv + G);   // else all zeros: acc is unchanged
             for (ep += 2 * G * dir; UPPER ? ep > estop : ep < estop; ep += G * dir)
               acc = trsv_gather<G>(acc, ep, estop, dir, ci, lu, x, err);
           }

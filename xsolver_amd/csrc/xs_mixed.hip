@@ -463,7 +463,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
       int hc[2 * G];
       float hv[2 * G];
       unsigned hm = 0;
-      bool stalled = false;
+      bool stalled = false, more = false;   // more: the second batch is held
       {
         int pc[G];
         float pa[G];
@@ -485,7 +485,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
           if (stalled && wait) hm |= 1u << q;
         }
         const int e2 = ep + G * dir;
-        const bool more = stalled && (UPPER ? e2 > estop : e2 < estop);
+        more = stalled && (UPPER ? e2 > estop : e2 < estop);
         if (more) trsv_issue32<G>(e2, estop, dir, ci, lu, x, pc, pa, pu);
 #pragma unroll
         for (int q = 0; q < G; ++q) {
@@ -527,7 +527,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_trsv32(int n, const int *__restric
         if (lane == k) {
           if (stalled) {
             acc -= trsv_tree32<G>(hv);
-            acc -= trsv_tree32<G>(hv + G);
+            if (more) acc -= trsv_tree32<G>(hv + G);   // else all zeros: acc is unchanged
             for (ep += 2 * G * dir; UPPER ? ep > estop : ep < estop; ep += G * dir)
               acc = trsv_gather32<G>(acc, ep, estop, dir, ci, lu, x, err);
           }
