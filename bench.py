@@ -124,7 +124,7 @@ def run_gpu(A, b, fmt, steps, warmup, profile=True, precision=0, factor_storage=
         if cnt > 0:
             kern[k] = {"count": int(cnt), "total_ms": tot, "avg_us": 1000.0 * tot / cnt}
     x = xd.cpu().numpy()
-    idx = (S.time_ms("spmv:idx16"), S.time_ms("pc:idx16"))
+    idx = (S.time_ms("spmv:idx16"), S.time_ms("pc:idx16"), S.time_ms("spmv:dia"))
     S.close()
     return {
         "idx16": idx,
@@ -139,7 +139,7 @@ def run_gpu(A, b, fmt, steps, warmup, profile=True, precision=0, factor_storage=
 
 # library timing families -> (kernel name in the rocprofv3 summary, launches)
 PROFILE_NAMES = {
-    "spmv": [("k_spmv_ell<2, true>", 1)],
+    "spmv": [("k_spmv_dia<2>", 1)],
     "ilu_lower_apply": [("k_tri_sweep_ell_px<true>", 0.5),
                         ("k_tri_sweep_ell<false, true, true, true>", 0.5),
                         ("k_tri_sweep_ell<false, false, true, false>", APPLY[0] - 2)],
@@ -520,16 +520,18 @@ def bench_spmv_kernels(cases=("convdiff3d_128", "lognormal27_100"), reps=20):
         x = torch.from_numpy(np.random.default_rng(0).uniform(-1, 1, n)).to(dev)
         y = torch.empty_like(x)
         rec = {"n": n, "nnz": nnz}
-        for kname, ell, vec in (("ell", True, None), ("vector_csr", False, 1)):
+        for kname, ell, vec, dia in (("diagonal_slots", True, None, 1), ("ell", True, None, 0),
+                                     ("vector_csr", False, 1, 0)):
             S = XSolver()
+            S.set_option("spmv_dia", dia)
             S.mat_setup_device(rp, ci, v, use_ell=ell)
             if vec is not None:
                 S.set_option("spmv_kernel", vec)
             kind = S.time_ms("spmv:kernel")
-            if ell and kind != 2:
+            if (ell and kind != 2) or (dia and S.time_ms("spmv:dia") == 0):
                 S.close()
                 continue
-            idx = 2 if (ell and S.time_ms("spmv:idx16") == 1) else 4
+            idx = 0 if dia else 2 if (ell and S.time_ms("spmv:idx16") == 1) else 4
             # bytes the kernel must move: values + stored columns (+ row
             # pointers for CSR), x once, y once
             by = (8 + idx) * nnz + (0 if ell else 4 * (n + 1)) + 16 * n
@@ -639,8 +641,10 @@ def main():
     ell = None if args.no_ell else run_gpu(A, b, "ell", args.steps, args.warmup,
                                            profile=False, sweeps=APPLY)
 
-    ia, ifac = headp["idx16"]
-    by = algorithmic_bytes(n, nnz, nnz_l, nnz_l, APPLY, 2 if ia == 1 else 4, 2 if ifac == 1 else 4)
+    ia, ifac, dia = headp["idx16"]
+    # A's copy: diagonal slots (no index bytes), else ELL with 16- or 32-bit columns
+    by = algorithmic_bytes(n, nnz, nnz_l, nnz_l, APPLY, 0 if dia > 0 else 2 if ia == 1 else 4,
+                           2 if ifac == 1 else 4)
     # kernel families: the headline run's SpMV and ILU applies, the exact run's SpTRSV
     fam = {"spmv": headp["kernels"].get("spmv"),
            "ilu_lower_apply": headp["kernels"].get("trsv_lower"),

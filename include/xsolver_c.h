@@ -170,7 +170,14 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *                    1 = row-binned vector CSR (lanes per row = mean row length
  *                    rounded up to a power of two, long rows one wavefront
  *                    each).  0 and 1 also drop the automatic ELL copy.
- *   "use_ell"        1 = ELL SpMV (same as advanced_pam[1] = 1) */
+ *   "use_ell"        1 = ELL SpMV (same as advanced_pam[1] = 1)
+ *   "spmv_dia"       1 (default) = when the ELL copy is built and the matrix
+ *                    has at most 8 distinct column - row offsets (5- and
+ *                    7-point stencils), the FP64 single-vector SpMV reads a
+ *                    diagonal-slot copy (one value slab per offset, zeros
+ *                    where a row lacks that diagonal, no per-entry index);
+ *                    same entries in the same column order as the ELL rows.
+ *                    Set before xs_mat_setup; 0 = ELL only */
 int xs_set_option(xs_ctx *ctx, const char *name, double value);
 
 /* Autotuner (no reference counterpart): times, on this matrix, the CSR
@@ -192,7 +199,9 @@ int xs_set_option(xs_ctx *ctx, const char *name, double value);
  * xs_get_time_ms("spmv:idx16") = 1 when the ELL copy stores 16-bit column
  * offsets (every entry within 32767 of its row), 0 when it reads int32
  * columns; xs_get_time_ms("pc:idx16") the same for the ELL copies of L and U
- * used by the Jacobi-sweep apply (-1 before they are built). */
+ * used by the Jacobi-sweep apply (-1 before they are built);
+ * xs_get_time_ms("spmv:dia") = the number of diagonal slots of the FP64
+ * SpMV's copy (0: none). */
 int xs_autotune(xs_ctx *ctx);
 
 /* Factorise / prepare the preconditioner now (otherwise done by the first

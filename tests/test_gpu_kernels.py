@@ -73,6 +73,42 @@ def test_spmv_full_size_convdiff128(dev):
         S.close()
 
 
+@pytest.mark.parametrize("name", ["convdiff3d_24", "grf2d_48", "random_3000"])
+def test_spmv_diagonal_slots(dev, name):
+    """A matrix with at most 8 distinct column - row offsets (the 7-point and
+    5-point stencils) gets a diagonal-slot copy for the FP64 SpMV: no index
+    per entry, the same entries summed in the same column order (zero
+    products where a boundary row lacks a diagonal), so y and b - Ax are bit
+    for bit those of the ELL kernel; other matrices keep the ELL copy."""
+    import torch
+    A = gen.convdiff3d(24) if name == "convdiff3d_24" else MATS[name]()
+    n = len(A[0]) - 1
+    offs = set()
+    rp, ci = A[0], A[1]
+    for i in range(n):
+        offs.update((ci[rp[i]:rp[i + 1]] - i).tolist())
+    rng = np.random.default_rng(6)
+    x, b = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    out = []
+    for dia in (1, 0):
+        S = XSolver()
+        S.set_option("spmv_dia", dia)
+        S.mat_setup(*A, advanced_pam=(0, 1, 1))
+        nd = S.time_ms("spmv:dia")
+        say(f"[spmv-dia] {name} dia={dia}: {int(nd)} slots ({len(offs)} offsets)")
+        assert nd == (len(offs) if dia and len(offs) <= 8 else 0)
+        yd = torch.empty(n, dtype=torch.float64, device=dev)
+        S.spmv(dev_tensor(x, dev), yd, torch_stream())
+        zd = torch.empty(n, dtype=torch.float64, device=dev)
+        S.bmax(dev_tensor(b, dev), dev_tensor(x, dev), zd, torch_stream())
+        torch.cuda.synchronize()
+        out.append((yd.cpu().numpy(), zd.cpu().numpy()))
+        S.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    scale = np.abs(A[2]).max() * np.abs(x).max() * np.diff(A[0]).max()
+    assert np.abs(out[0][0] - oracle.spmv(A, x)).max() <= 1e-14 * scale
+
+
 def test_default_spmv_kernel_choice(dev):
     """Automatic SpMV kernel for CSR input: ELL copy for near-uniform rows
     (7-point stencil), vector CSR for regular rows with a wider spread, LDS

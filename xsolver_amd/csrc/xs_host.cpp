@@ -90,7 +90,7 @@ static void free_matrix(xs_ctx *c) {
   dfree(c->plan.blk_row);
   dfree(c->plan.long_rows);
   c->plan = xs_spmv_plan();
-  dfree(c->ell.ci); dfree(c->ell.v); dfree(c->ell.co);
+  dfree(c->ell.ci); dfree(c->ell.v); dfree(c->ell.co); dfree(c->ell.dv);
   c->ell = xs_ell();
   dfree(c->At.rp); dfree(c->At.ci); dfree(c->At.v);
   c->At = xs_csr();
@@ -130,8 +130,8 @@ static int alloc_base(xs_ctx *c) {
   if ((rc = dalloc(&c->st, 1))) return rc;
   if ((rc = dalloc(&c->counters, 64))) return rc;
   XS_HIP(hipMemset(c->counters, 0, sizeof(unsigned) * 64));
-  if ((rc = dalloc(&c->d_int, 8))) return rc;
-  XS_HIP(hipMemset(c->d_int, 0, sizeof(int) * 8));
+  if ((rc = dalloc(&c->d_int, 16))) return rc;
+  XS_HIP(hipMemset(c->d_int, 0, sizeof(int) * 16));
   // pinned: [0] the solve's state, [1] the inner solve's (mixed precision);
   // a device-to-host copy into pageable memory goes through a staging blit
   XS_HIP(hipHostMalloc((void **)&c->h_state, 2 * sizeof(xs_state), hipHostMallocDefault));
@@ -284,7 +284,18 @@ static int make_ell(xs_ctx *c) {
   if ((rc = dalloc(&c->ell.ci, (size_t)W * c->ell.ld))) return rc;
   if ((rc = dalloc(&c->ell.v, (size_t)W * c->ell.ld))) return rc;
   xs_build_ell(c->stream, A, W, c->ell);
-  return make_offsets(c, A.n, c->ell.ld, W, c->ell.ci, &c->ell.co);
+  if ((rc = make_offsets(c, A.n, c->ell.ld, W, c->ell.ci, &c->ell.co))) return rc;
+  // diagonal slots for the FP64 SpMV when A has few distinct offsets
+  if (!c->spmv_dia) return XS_OK;
+  xs_dia_offs off{};
+  const int nd = xs_dia_offsets(c->stream, A, c->d_int, off);
+  if (nd < 0) return XS_ERR_DEVICE;
+  if (nd == 0) return XS_OK;
+  if ((rc = dalloc(&c->ell.dv, (size_t)nd * c->ell.ld))) return rc;
+  c->ell.nd = nd;
+  c->ell.doff = off;
+  xs_build_dia(c->stream, A, c->ell);
+  return XS_OK;
 }
 
 static int build_spmv(xs_ctx *c) {
@@ -807,6 +818,11 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     if (value < 1 || value > 65536) return XS_ERR_ARG;
     c->trsv_wg = (int)value;
     c->ilu.trsv_wg = c->trsv_wg;
+  }
+  else if (k == "spmv_dia") {   // 1: diagonal-slot copy for the FP64 SpMV when possible
+    if (value < 0 || value > 1) return XS_ERR_ARG;
+    c->spmv_dia = value != 0;
+    if (c->ell.dv && !c->spmv_dia) { dfree(c->ell.dv); c->ell.nd = 0; }
   }
   else if (k == "trsv_progressive") {   // -1 auto, 0 scan chain groups, 1 lane by lane
     if (value < -1 || value > 1) return XS_ERR_ARG;
@@ -1843,6 +1859,7 @@ extern "C" double xs_get_time_ms(const xs_ctx *c, const char *phase) {
     return !c->mat_ready ? -1.0 : (c->use_ell && c->ell.v) ? 2.0 : (double)c->plan.vec;
   // 1 when the ELL copy carries 16-bit column offsets
   if (!strcmp(phase, "spmv:idx16")) return c->ell.co ? 1.0 : 0.0;
+  if (!strcmp(phase, "spmv:dia")) return (c->use_ell && c->ell.dv) ? (double)c->ell.nd : 0.0;
   // 1 when both sweep copies of the factors carry 16-bit offsets (-1: not built)
   if (!strcmp(phase, "pc:idx16"))
     return !c->tell_ready ? -1.0

@@ -82,11 +82,24 @@ struct xs_spmv_plan {   // row blocks for the LDS-staged CSR kernel
   int *long_rows = nullptr;
 };
 
+// Diagonal slots: at most this many distinct column - row offsets in a matrix
+// (a 7-point stencil has 7) give its single-vector FP64 SpMV a copy without
+// any per-entry index.
+#define XS_DIA_MAX 8
+struct xs_dia_offs {    // by value into the kernel: slot k holds column row + o[k]
+  int o[XS_DIA_MAX];
+};
 struct xs_ell {         // column-major slabs: v[k * ld + row], width W
   int n = 0, ld = 0, W = 0;
   int *ci = nullptr;
   double *v = nullptr;
   short *co = nullptr;  // column - row as 16 bits (when every entry fits), else null
+  // diagonal-slot copy (nd > 0): dv[k * ld + row] = a(row, row + doff.o[k])
+  // or 0 where that entry is absent, the offsets ascending (the slots are in
+  // column order, as the ELL rows); null when the matrix has more offsets
+  int nd = 0;
+  xs_dia_offs doff{};
+  double *dv = nullptr;
 };
 
 // 16-bit column offsets: padding of the sweep copies (column -1)
@@ -223,6 +236,12 @@ void xs_launch_spmv_t(hipStream_t s, const xs_csr &A, const double *x, double *y
 void xs_build_plan(hipStream_t s, const xs_csr &A, xs_spmv_plan &P, int *blk, int chunk);
 int xs_default_chunk(const xs_csr &A);
 void xs_build_ell(hipStream_t s, const xs_csr &A, int W, xs_ell &E);
+// Distinct column - row offsets of A when there are at most XS_DIA_MAX of
+// them (ascending, count returned), else 0; synchronous, -1 on failure.
+// d_tmp: XS_DIA_MAX + 1 ints of device scratch.
+int xs_dia_offsets(hipStream_t s, const xs_csr &A, int *d_tmp, xs_dia_offs &out);
+// E.dv from A's values in the slots of E.doff (E.nd slots, E.ld rows)
+void xs_build_dia(hipStream_t s, const xs_csr &A, xs_ell &E);
 // Largest |column - row| over the non-padding entries of ELL slabs (rows < n,
 // column >= 0); synchronous, -1 on failure.  Then the 16-bit offsets: padding
 // columns (-1) become XS_OFF16_PAD, rows >= n point at row n - 1.

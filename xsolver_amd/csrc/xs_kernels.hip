@@ -265,6 +265,63 @@ __global__ __launch_bounds__(XS_BLOCK) void k_spmv_ell(
   if (RED) reduce_finish<NV>(acc, R);
 }
 
+// SpMV over the diagonal-slot copy: k_spmv_ell without index loads.  Slot k
+// of row r reads x at r + doff.o[k] (clamped into [0, n) for the zero-valued
+// slots of rows that lack that diagonal), so the sums run over the same
+// entries in the same column order as the ELL rows, with zero products in
+// place of absent entries (as the ELL padding adds them at the row's end).
+template <int EPI>
+__global__ __launch_bounds__(XS_BLOCK) void k_spmv_dia(
+    int n, int ld, int W, const xs_dia_offs doff, const double *__restrict__ v,
+    const double *__restrict__ x, double *__restrict__ y, const double *__restrict__ b,
+    const double *__restrict__ w, xs_reduce R) {
+  constexpr int NV = (EPI == EPI_DOT2) ? 2 : 1;
+  constexpr bool RED = (EPI == EPI_DOT1 || EPI == EPI_DOT2 || EPI == EPI_BMAXN);
+  if (R.st != nullptr && R.st->done) return;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  const int r = 2 * (xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x);
+  if (r < n) {
+    constexpr bool OPB = (EPI == EPI_BMAX || EPI == EPI_BMAXN);
+    constexpr bool OPW = (EPI == EPI_DOT1 || EPI == EPI_DOT2);
+    double e[2] = {0.0, 0.0};
+    if (OPB || OPW) {
+      const double *op = OPB ? b : w;
+      e[0] = op[r];
+      if (r + 1 < n) e[1] = op[r + 1];
+    }
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int k0 = 0; k0 < XS_DIA_MAX; k0 += XS_ELL_UK) {
+      if (k0 >= W) break;
+      double2 a[XS_ELL_UK];
+      int c0[XS_ELL_UK], c1[XS_ELL_UK];
+#pragma unroll
+      for (int q = 0; q < XS_ELL_UK; ++q) {
+        const int k = min(k0 + q, W - 1);
+        a[q] = *reinterpret_cast<const double2 *>(v + (size_t)k * ld + r);
+        c0[q] = min(max(r + doff.o[k], 0), n - 1);
+        c1[q] = min(max(r + 1 + doff.o[k], 0), n - 1);
+      }
+      double x0[XS_ELL_UK], x1[XS_ELL_UK];
+#pragma unroll
+      for (int q = 0; q < XS_ELL_UK; ++q) { x0[q] = x[c0[q]]; x1[q] = x[c1[q]]; }
+#pragma unroll
+      for (int q = 0; q < XS_ELL_UK; ++q) {
+        if (k0 + q < W) { s0 += a[q].x * x0[q]; s1 += a[q].y * x1[q]; }
+      }
+    }
+    double s[2] = {s0, s1};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int rr = r + q;
+      if (rr < n) epilogue_v<EPI, NV>(rr, s[q], e[q], y, acc);
+    }
+  }
+  if (RED) reduce_finish<NV>(acc, R);
+}
+
 static xs_reduce no_reduce() {
   xs_reduce r{};
   return r;
@@ -309,6 +366,18 @@ void xs_launch_spmv_ell(hipStream_t s, const xs_ell &E, const double *x, double 
   xs_reduce R = red ? *red : no_reduce();
   int threads = (E.n + 1) / 2;
   dim3 grid((threads + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+  if (E.dv && E.nd > 0 && E.nd <= XS_DIA_MAX) {
+#define XS_DIAK(E_) hipLaunchKernelGGL((k_spmv_dia<E_>), grid, blk, 0, s, E.n, E.ld, E.nd, E.doff, E.dv, x, y, b, w, R)
+    switch (epi) {
+      case EPI_Y: XS_DIAK(EPI_Y); break;
+      case EPI_BMAX: XS_DIAK(EPI_BMAX); break;
+      case EPI_DOT1: XS_DIAK(EPI_DOT1); break;
+      case EPI_DOT2: XS_DIAK(EPI_DOT2); break;
+      case EPI_BMAXN: XS_DIAK(EPI_BMAXN); break;
+    }
+#undef XS_DIAK
+    return;
+  }
 #define XS_ELLK(E_, O) hipLaunchKernelGGL((k_spmv_ell<E_, O>), grid, blk, 0, s, E.n, E.ld, E.W, E.ci, E.co, E.v, x, y, b, w, R)
 #define XS_ELLE(E_) do { if (E.co) XS_ELLK(E_, true); else XS_ELLK(E_, false); } while (0)
   switch (epi) {
@@ -455,6 +524,73 @@ void xs_build_ell(hipStream_t s, const xs_csr &A, int W, xs_ell &E) {
   XS_GUARD(A.rp && A.ci && A.v && E.ci && E.v);
   hipLaunchKernelGGL(k_build_ell, dim3((E.ld + 255) / 256), dim3(256), 0, s, A.n, E.ld, W, A.rp,
                      A.ci, A.v, E.ci, E.v);
+  if (E.dv) xs_build_dia(s, A, E);
+}
+
+// Distinct offsets column - row, gathered into t[0 .. XS_DIA_MAX) (INT_MIN:
+// free slot); t[XS_DIA_MAX] = 1 when there are more.  A plain read finds an
+// offset already there (nearly every entry), a CAS claims a free slot.
+__global__ void k_dia_offsets(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                              int *t) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    for (int e = rp[r]; e < rp[r + 1]; ++e) {
+      const int o = ci[e] - r;
+      if (e > rp[r] && ci[e] == ci[e - 1]) atomicOr(t + XS_DIA_MAX, 1);   // a repeated column
+      bool done = false;
+      for (int k = 0; k < XS_DIA_MAX && !done; ++k) {
+        int cur = __hip_atomic_load(t + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == INT_MIN) cur = atomicCAS(t + k, INT_MIN, o);
+        if (cur == INT_MIN || cur == o) done = true;
+      }
+      if (!done) atomicOr(t + XS_DIA_MAX, 1);
+    }
+  }
+}
+
+int xs_dia_offsets(hipStream_t s, const xs_csr &A, int *d_tmp, xs_dia_offs &out) {
+  if (!(A.rp && A.ci && d_tmp) || A.n <= 0) return -1;
+  int h[XS_DIA_MAX + 1];
+  for (int k = 0; k < XS_DIA_MAX; ++k) h[k] = INT_MIN;
+  h[XS_DIA_MAX] = 0;
+  if (hipMemcpyAsync(d_tmp, h, sizeof(h), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
+  hipLaunchKernelGGL(k_dia_offsets, dim3(std::min(4096, (A.n + 255) / 256)), dim3(256), 0, s, A.n,
+                     A.rp, A.ci, d_tmp);
+  if (hipMemcpyAsync(h, d_tmp, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return -1;
+  if (h[XS_DIA_MAX]) return 0;
+  int nd = 0;
+  for (int k = 0; k < XS_DIA_MAX; ++k)
+    if (h[k] != INT_MIN) out.o[nd++] = h[k];
+  std::sort(out.o, out.o + nd);
+  for (int k = nd; k < XS_DIA_MAX; ++k) out.o[k] = 0;
+  return nd;
+}
+
+__global__ void k_build_dia(int n, int ld, int nd, const xs_dia_offs doff,
+                            const int *__restrict__ rp, const int *__restrict__ ci,
+                            const double *__restrict__ v, double *__restrict__ dv) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ld) return;
+  double slot[XS_DIA_MAX];
+#pragma unroll
+  for (int k = 0; k < XS_DIA_MAX; ++k) slot[k] = 0.0;
+  if (r < n)
+    for (int e = rp[r]; e < rp[r + 1]; ++e) {
+      const int o = ci[e] - r;
+#pragma unroll
+      for (int k = 0; k < XS_DIA_MAX; ++k)
+        if (k < nd && doff.o[k] == o) slot[k] = v[e];
+    }
+#pragma unroll
+  for (int k = 0; k < XS_DIA_MAX; ++k)
+    if (k < nd) dv[(size_t)k * ld + r] = slot[k];
+}
+
+void xs_build_dia(hipStream_t s, const xs_csr &A, xs_ell &E) {
+  XS_GUARD(A.rp && A.ci && A.v && E.dv && E.nd > 0 && E.nd <= XS_DIA_MAX && E.ld >= A.n);
+  hipLaunchKernelGGL(k_build_dia, dim3((E.ld + 255) / 256), dim3(256), 0, s, A.n, E.ld, E.nd,
+                     E.doff, A.rp, A.ci, A.v, E.dv);
 }
 
 // Transposed SpMV y = A^T x by a gather over an explicitly transposed CSR is
