@@ -177,7 +177,11 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *                    diagonal-slot copy (one value slab per offset, zeros
  *                    where a row lacks that diagonal, no per-entry index);
  *                    same entries in the same column order as the ELL rows.
- *                    Set before xs_mat_setup; 0 = ELL only */
+ *                    Set before xs_mat_setup; 0 = ELL only
+ *   "sweep_dia"      1 (default) = the same for the strictly lower and upper
+ *                    factor parts in the FP64 Jacobi-sweep apply (at most 8
+ *                    distinct offsets each: ILU(0) of a 7-point stencil has
+ *                    3); 0 = ELL copies only */
 int xs_set_option(xs_ctx *ctx, const char *name, double value);
 
 /* Autotuner (no reference counterpart): times, on this matrix, the CSR
@@ -201,7 +205,8 @@ int xs_set_option(xs_ctx *ctx, const char *name, double value);
  * columns; xs_get_time_ms("pc:idx16") the same for the ELL copies of L and U
  * used by the Jacobi-sweep apply (-1 before they are built);
  * xs_get_time_ms("spmv:dia") = the number of diagonal slots of the FP64
- * SpMV's copy (0: none). */
+ * SpMV's copy (0: none), xs_get_time_ms("pc:dia") those of L's sweep copy
+ * (-1 before the sweep copies are built). */
 int xs_autotune(xs_ctx *ctx);
 
 /* Factorise / prepare the preconditioner now (otherwise done by the first

@@ -442,6 +442,35 @@ def test_bicgstab_fused_updates_same_result(dev):
     assert out[0][1] == out[1][1] and np.array_equal(out[0][0], out[1][0])
 
 
+@pytest.mark.parametrize("name", ["convdiff3d_24", "lognormal27_12"])
+def test_sweep_diagonal_slots_same_result(dev, name):
+    """ILU(0) factors of a 7-point stencil have three offsets per triangle:
+    the FP64 sweeps then read diagonal-slot copies of L and U (no index per
+    entry) -- the same entries in the same order as the ELL copies, so the
+    solve is bit for bit the one with the ELL sweeps.  The 27-point factors
+    (13 offsets per triangle) keep the ELL copies."""
+    A = gen.convdiff3d(24) if name == "convdiff3d_24" else gen.lognormal27(12, seed=2)
+    xs, b = gen.manufactured(A, 5)
+    out = []
+    for dia in (1, 0):
+        S = XSolver()
+        S.mat_setup(*A)
+        S.ksp_settype("bicgstab")
+        S.ksp_setoption(1e-10, 1000, 0)
+        S.pc_settype("ilu0")
+        for k, v in (("lsolve_sweeps", 5), ("usolve_sweeps", 2), ("parilu_async", 0),
+                     ("sweep_dia", dia)):
+            S.set_option(k, v)
+        x, st = S.solve(b)
+        nd = S.time_ms("pc:dia")
+        say(f"[sweep-dia] {name} sweep_dia={dia}: {int(nd)} slots it={S.iterations}")
+        assert st == 0 and true_relres(A, b, x) < 1e-10
+        assert nd == ((3 if name == "convdiff3d_24" else 0) if dia else 0)
+        out.append((x, S.iterations))
+        S.close()
+    assert out[0][1] == out[1][1] and np.array_equal(out[0][0], out[1][0])
+
+
 @pytest.mark.parametrize("factor", [0, 1, 2], ids=["f32", "bf16", "f16"])
 def test_mixed_sweep_apply_factor_storage(dev, factor):
     """FP32-inner refinement with the sweep apply reading the factors' sweep

@@ -75,6 +75,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.app); dfree(F.apd); dfree(F.cmap); dfree(F.lc); dfree(F.mpos);
   dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.vlo); dfree(F.tl.co); dfree(F.tl.ww);
   dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co); dfree(F.tu.ww); dfree(F.tu.vs);
+  dfree(F.tl.dv); dfree(F.tu.dv); dfree(F.tu.dvs);
   c->tell_ready = c->tell32_ready = false;
   F = xs_ilu();
   F.trsv_wg = c->trsv_wg;
@@ -824,6 +825,12 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     c->spmv_dia = value != 0;
     if (c->ell.dv && !c->spmv_dia) { dfree(c->ell.dv); c->ell.nd = 0; }
   }
+  else if (k == "sweep_dia") {   // 1: diagonal-slot copies of L and U for the FP64 sweeps
+    if (value < 0 || value > 1) return XS_ERR_ARG;
+    c->sweep_dia = value != 0;
+    c->ilu.tell_shape = false;   // rebuilt (with or without them) before the next sweep
+    c->tell_ready = c->tell32_ready = false;
+  }
   else if (k == "trsv_progressive") {   // -1 auto, 0 scan chain groups, 1 lane by lane
     if (value < -1 || value > 1) return XS_ERR_ARG;
     c->trsv_prog = (int)value;
@@ -1513,8 +1520,27 @@ static int ensure_tri_ell(xs_ctx *c) {
       dfree(E.ww);
       if ((rc = dalloc(&E.ww, ((size_t)F.n + XS_WAVE - 1) / XS_WAVE))) return rc;
       xs_tri_wave_widths(c->stream, F, part == 1, E.ww);
+      // diagonal slots for the FP64 sweeps when the part has few offsets
+      dfree(E.dv);
+      dfree(E.dvs);
+      E.nd = 0;
+      if (c->sweep_dia && E.W > 0) {
+        xs_dia_offs off{};
+        const int nd = xs_tri_dia_offsets(c->stream, F, part == 1, c->d_int, off);
+        if (nd < 0) return XS_ERR_DEVICE;
+        if (nd > 0) {
+          if ((rc = dalloc(&E.dv, (size_t)nd * E.ld))) return rc;
+          if (part == 1 && (rc = dalloc(&E.dvs, (size_t)nd * E.ld))) return rc;
+          E.nd = nd;
+          E.doff = off;
+        }
+      }
     }
     F.tell_shape = true;
+  }
+  for (int part = 0; part < 2; ++part) {
+    xs_tell &E = part ? F.tu : F.tl;
+    if (E.nd > 0) xs_build_tri_dia(c->stream, F, part == 1, E);
   }
   XS_HIP(hipStreamSynchronize(c->stream));
   if (xs_take_launch_guard()) return XS_ERR_STATE;
@@ -1860,6 +1886,7 @@ extern "C" double xs_get_time_ms(const xs_ctx *c, const char *phase) {
   // 1 when the ELL copy carries 16-bit column offsets
   if (!strcmp(phase, "spmv:idx16")) return c->ell.co ? 1.0 : 0.0;
   if (!strcmp(phase, "spmv:dia")) return (c->use_ell && c->ell.dv) ? (double)c->ell.nd : 0.0;
+  if (!strcmp(phase, "pc:dia")) return c->tell_ready ? (double)c->ilu.tl.nd : -1.0;
   // 1 when both sweep copies of the factors carry 16-bit offsets (-1: not built)
   if (!strcmp(phase, "pc:idx16"))
     return !c->tell_ready ? -1.0

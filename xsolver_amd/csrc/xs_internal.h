@@ -123,6 +123,11 @@ struct xs_tell {
                              // slab count the sweeps walk for those rows
   double *vs = nullptr;      // U only: u_ij / u_jj, the values of the first two
                              // sweeps in one pass (no gather of 1/u_jj)
+  // diagonal-slot copy for the FP64 sweeps (nd > 0): dv[k * ld + row] =
+  // t(row, row + doff.o[k]) or 0, offsets ascending; dvs as vs (U only)
+  int nd = 0;
+  xs_dia_offs doff{};
+  double *dv = nullptr, *dvs = nullptr;
 };
 
 // Work order of a sync-free triangular solve: the row groups (gb: ngroups + 1
@@ -242,6 +247,10 @@ void xs_build_ell(hipStream_t s, const xs_csr &A, int W, xs_ell &E);
 int xs_dia_offsets(hipStream_t s, const xs_csr &A, int *d_tmp, xs_dia_offs &out);
 // E.dv from A's values in the slots of E.doff (E.nd slots, E.ld rows)
 void xs_build_dia(hipStream_t s, const xs_csr &A, xs_ell &E);
+// The same for the strictly lower (upper) part of the factors: offsets, then
+// E.dv (and for U E.dvs = u_ij / u_jj) from the current factor values.
+int xs_tri_dia_offsets(hipStream_t s, const xs_ilu &F, bool upper, int *d_tmp, xs_dia_offs &out);
+void xs_build_tri_dia(hipStream_t s, const xs_ilu &F, bool upper, xs_tell &E);
 // Largest |column - row| over the non-padding entries of ELL slabs (rows < n,
 // column >= 0); synchronous, -1 on failure.  Then the 16-bit offsets: padding
 // columns (-1) become XS_OFF16_PAD, rows >= n point at row n - 1.

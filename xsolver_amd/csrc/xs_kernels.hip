@@ -528,33 +528,53 @@ void xs_build_ell(hipStream_t s, const xs_csr &A, int W, xs_ell &E) {
 }
 
 // Distinct offsets column - row, gathered into t[0 .. XS_DIA_MAX) (INT_MIN:
-// free slot); t[XS_DIA_MAX] = 1 when there are more.  A plain read finds an
-// offset already there (nearly every entry), a CAS claims a free slot.
-__global__ void k_dia_offsets(int n, const int *__restrict__ rp, const int *__restrict__ ci,
-                              int *t) {
+// free slot); t[XS_DIA_MAX] = 1 when there are more, or a row repeats a
+// column.  A slot only ever changes from INT_MIN to its offset, so a plain
+// read that finds the offset is final and a CAS claims a free slot.  Each
+// workgroup collects into an LDS table first and merges its (at most
+// XS_DIA_MAX) offsets into t at the end: the global table sees a handful of
+// atomics per workgroup instead of a read per entry.
+__device__ __forceinline__ bool dia_insert(int *tab, int o) {
+  for (int k = 0; k < XS_DIA_MAX; ++k) {
+    int cur = tab[k];
+    if (cur == INT_MIN) cur = atomicCAS(tab + k, INT_MIN, o);
+    if (cur == INT_MIN || cur == o) return true;
+  }
+  return false;
+}
+
+// part: 0 all entries, 1 strictly lower, 2 strictly upper (dpos: pivots)
+__global__ __launch_bounds__(256) void k_dia_offsets(int n, const int *__restrict__ rp,
+                                                     const int *__restrict__ ci,
+                                                     const int *__restrict__ dpos, int part,
+                                                     int *t) {
+  __shared__ int lt[XS_DIA_MAX + 1];
+  if (threadIdx.x <= XS_DIA_MAX) lt[threadIdx.x] = threadIdx.x < XS_DIA_MAX ? INT_MIN : 0;
+  __syncthreads();
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-    for (int e = rp[r]; e < rp[r + 1]; ++e) {
+    const int e0 = part == 2 ? dpos[r] + 1 : rp[r], e1 = part == 1 ? dpos[r] : rp[r + 1];
+    for (int e = e0; e < e1; ++e) {
       const int o = ci[e] - r;
-      if (e > rp[r] && ci[e] == ci[e - 1]) atomicOr(t + XS_DIA_MAX, 1);   // a repeated column
-      bool done = false;
-      for (int k = 0; k < XS_DIA_MAX && !done; ++k) {
-        int cur = __hip_atomic_load(t + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (cur == INT_MIN) cur = atomicCAS(t + k, INT_MIN, o);
-        if (cur == INT_MIN || cur == o) done = true;
-      }
-      if (!done) atomicOr(t + XS_DIA_MAX, 1);
+      if ((e > e0 && ci[e] == ci[e - 1]) || !dia_insert(lt, o)) atomicOr(lt + XS_DIA_MAX, 1);
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < XS_DIA_MAX && lt[threadIdx.x] != INT_MIN) {
+    if (!dia_insert(t, lt[threadIdx.x])) atomicOr(t + XS_DIA_MAX, 1);
+  } else if (threadIdx.x == XS_DIA_MAX && lt[XS_DIA_MAX]) {
+    atomicOr(t + XS_DIA_MAX, 1);
   }
 }
 
-int xs_dia_offsets(hipStream_t s, const xs_csr &A, int *d_tmp, xs_dia_offs &out) {
-  if (!(A.rp && A.ci && d_tmp) || A.n <= 0) return -1;
+static int dia_offsets(hipStream_t s, int n, const int *rp, const int *ci, const int *dpos,
+                       int part, int *d_tmp, xs_dia_offs &out) {
+  if (!(rp && ci && d_tmp && (part == 0 || dpos)) || n <= 0) return -1;
   int h[XS_DIA_MAX + 1];
   for (int k = 0; k < XS_DIA_MAX; ++k) h[k] = INT_MIN;
   h[XS_DIA_MAX] = 0;
   if (hipMemcpyAsync(d_tmp, h, sizeof(h), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
-  hipLaunchKernelGGL(k_dia_offsets, dim3(std::min(4096, (A.n + 255) / 256)), dim3(256), 0, s, A.n,
-                     A.rp, A.ci, d_tmp);
+  hipLaunchKernelGGL(k_dia_offsets, dim3(std::min(4096, (n + 255) / 256)), dim3(256), 0, s, n,
+                     rp, ci, dpos, part, d_tmp);
   if (hipMemcpyAsync(h, d_tmp, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return -1;
@@ -565,6 +585,60 @@ int xs_dia_offsets(hipStream_t s, const xs_csr &A, int *d_tmp, xs_dia_offs &out)
   std::sort(out.o, out.o + nd);
   for (int k = nd; k < XS_DIA_MAX; ++k) out.o[k] = 0;
   return nd;
+}
+
+int xs_dia_offsets(hipStream_t s, const xs_csr &A, int *d_tmp, xs_dia_offs &out) {
+  return dia_offsets(s, A.n, A.rp, A.ci, nullptr, 0, d_tmp, out);
+}
+
+int xs_tri_dia_offsets(hipStream_t s, const xs_ilu &F, bool upper, int *d_tmp, xs_dia_offs &out) {
+  return dia_offsets(s, F.n, F.rp, F.ci, F.dpos, upper ? 2 : 1, d_tmp, out);
+}
+
+// diagonal slots of the strictly lower (upper) factor part; U also writes the
+// values scaled by the pivot inverse of their column (as k_tell_scale_cols)
+template <bool UPPER>
+__global__ void k_build_tri_dia(int n, int ld, int nd, const xs_dia_offs doff,
+                                const int *__restrict__ rp, const int *__restrict__ ci,
+                                const double *__restrict__ lu, const int *__restrict__ dpos,
+                                const double *__restrict__ dinv, double *__restrict__ dv,
+                                double *__restrict__ dvs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ld) return;
+  double slot[XS_DIA_MAX], sc[XS_DIA_MAX];
+#pragma unroll
+  for (int k = 0; k < XS_DIA_MAX; ++k) { slot[k] = 0.0; sc[k] = 0.0; }
+  if (r < n) {
+    const int e0 = UPPER ? dpos[r] + 1 : rp[r], e1 = UPPER ? rp[r + 1] : dpos[r];
+    for (int e = e0; e < e1; ++e) {
+      const int c = ci[e], o = c - r;
+      const double v = lu[e];
+#pragma unroll
+      for (int k = 0; k < XS_DIA_MAX; ++k)
+        if (k < nd && doff.o[k] == o) {
+          slot[k] = v;
+          if (UPPER) sc[k] = v * dinv[c];
+        }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < XS_DIA_MAX; ++k)
+    if (k < nd) {
+      dv[(size_t)k * ld + r] = slot[k];
+      if (UPPER) dvs[(size_t)k * ld + r] = sc[k];
+    }
+}
+
+void xs_build_tri_dia(hipStream_t s, const xs_ilu &F, bool upper, xs_tell &E) {
+  XS_GUARD(F.rp && F.ci && F.lu && F.dpos && E.dv && E.nd > 0 && E.nd <= XS_DIA_MAX &&
+           E.ld >= F.n && (!upper || (E.dvs && F.dinv)));
+  const dim3 grid((E.ld + 255) / 256), blk(256);
+  if (upper)
+    hipLaunchKernelGGL(k_build_tri_dia<true>, grid, blk, 0, s, F.n, E.ld, E.nd, E.doff, F.rp, F.ci,
+                       F.lu, F.dpos, F.dinv, E.dv, E.dvs);
+  else
+    hipLaunchKernelGGL(k_build_tri_dia<false>, grid, blk, 0, s, F.n, E.ld, E.nd, E.doff, F.rp,
+                       F.ci, F.lu, F.dpos, F.dinv, E.dv, E.dvs);
 }
 
 __global__ void k_build_dia(int n, int ld, int nd, const xs_dia_offs doff,
@@ -1645,9 +1719,104 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell(int n, int ld, int W
   yn[i] = UPPER ? di * (bi - r) : bi - r;
 }
 
+// k_tri_sweep_ell over the diagonal-slot copy (E.nd slots, offsets by
+// value): no index loads; slot k of row i reads the iterate at i + o_k
+// (clamped for the zero slots of rows that lack the diagonal).  Same
+// entries in the same column order as the ELL rows, the zero slots adding
+// zero products.
+template <bool UPPER, bool FUSED, bool SX = false>
+__global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_dia(int n, int ld, int nd,
+                                                            const xs_dia_offs doff,
+                                                            const double *__restrict__ ev,
+                                                            const double *__restrict__ dinv,
+                                                            const double *__restrict__ b,
+                                                            const double *__restrict__ yo,
+                                                            double *__restrict__ yn,
+                                                            const xs_state *st,
+                                                            const double *__restrict__ ru = nullptr,
+                                                            double *__restrict__ sv = nullptr) {
+  static_assert(!SX || (FUSED && !UPPER), "s is formed in the fused L pass only");
+  if (st != nullptr && st->done) return;
+  const int i = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
+  if (i >= n) return;
+  const double gama = SX ? st->sc[S_GAMA] : 0.0;
+  const double bi = SX ? fma(-gama, ru[i], b[i]) : b[i];
+  if (SX) sv[i] = bi;
+  const double di = UPPER ? dinv[i] : 1.0;
+  double r = 0.0;
+#pragma unroll
+  for (int k0 = 0; k0 < XS_DIA_MAX; k0 += 4) {
+    if (k0 >= nd) break;
+    double a[4], y[4];
+    int cc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = min(k0 + q, nd - 1);
+      a[q] = ev[(size_t)k * ld + i];
+      cc[q] = min(max(i + doff.o[k], 0), n - 1);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      y[q] = SX ? fma(-gama, ru[cc[q]], b[cc[q]]) - 0.0 : FUSED ? b[cc[q]] - 0.0 : yo[cc[q]];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (k0 + q < nd) r += a[q] * y[q];
+  }
+  yn[i] = UPPER ? di * (bi - r) : bi - r;
+}
+
+// k_tri_sweep_ell_px over the diagonal-slot copy of L
+__global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_dia_px(int n, int ld, int nd,
+                                                               const xs_dia_offs doff,
+                                                               const double *__restrict__ ev,
+                                                               const double *__restrict__ r,
+                                                               const double *__restrict__ u,
+                                                               const double *__restrict__ pin,
+                                                               double *__restrict__ pout,
+                                                               double *__restrict__ yn,
+                                                               const xs_state *st) {
+  if (st->done) return;
+  const int i = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
+  if (i >= n) return;
+  const double omega = st->sc[S_OMEGA], alpha = st->sc[S_ALPHA];
+  const double bi = bicg_p_val(r[i], pin[i], u[i], omega, alpha);
+  pout[i] = bi;
+  double acc = 0.0;
+#pragma unroll
+  for (int k0 = 0; k0 < XS_DIA_MAX; k0 += 4) {
+    if (k0 >= nd) break;
+    double a[4], y[4];
+    int cc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = min(k0 + q, nd - 1);
+      a[q] = ev[(size_t)k * ld + i];
+      cc[q] = min(max(i + doff.o[k], 0), n - 1);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) y[q] = bicg_p_val(r[cc[q]], pin[cc[q]], u[cc[q]], omega, alpha) - 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (k0 + q < nd) acc += a[q] * y[q];
+  }
+  yn[i] = bi - acc;
+}
+
+static bool tell_dia(const xs_tell &E) { return E.nd > 0 && E.nd <= XS_DIA_MAX && E.dv; }
+
 void xs_tri_sweep_ell(hipStream_t s, const xs_ilu &F, bool upper, bool fused, const double *b,
                       const double *yo, double *yn, const xs_state *st) {
   const xs_tell &E = upper ? F.tu : F.tl;
+  if (tell_dia(E)) {
+    XS_GUARD(E.ld >= F.n && b && yn && (!upper || (F.dinv && E.dvs)) && (fused || yo));
+    const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+    const double *ev = (upper && fused) ? E.dvs : E.dv;
+#define XS_SWD(U, FU) hipLaunchKernelGGL((k_tri_sweep_dia<U, FU>), grid, blk, 0, s, F.n, E.ld, E.nd, E.doff, ev, F.dinv, b, yo, yn, st)
+    if (upper) { if (fused) XS_SWD(true, true); else XS_SWD(true, false); }
+    else { if (fused) XS_SWD(false, true); else XS_SWD(false, false); }
+#undef XS_SWD
+    return;
+  }
   XS_GUARD((E.W == 0 || (E.ci && E.v)) && E.ld >= F.n && b && yn && (!upper || F.dinv) &&
            (fused || yo));
   const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
@@ -1715,6 +1884,13 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell_px(int n, int ld, in
 void xs_tri_sweep_ell_p(hipStream_t s, const xs_ilu &F, const double *r, const double *u,
                         const double *pin, double *pout, double *yn, const xs_state *st) {
   const xs_tell &E = F.tl;
+  if (tell_dia(E)) {
+    XS_GUARD(E.ld >= F.n && r && u && pin && pout && pin != pout && yn && st);
+    const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+    hipLaunchKernelGGL(k_tri_sweep_dia_px, grid, blk, 0, s, F.n, E.ld, E.nd, E.doff, E.dv, r, u,
+                       pin, pout, yn, st);
+    return;
+  }
   XS_GUARD((E.W == 0 || (E.ci && E.v)) && E.ld >= F.n && r && u && pin && pout && pin != pout &&
            yn && st);
   const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
@@ -1729,6 +1905,13 @@ void xs_tri_sweep_ell_p(hipStream_t s, const xs_ilu &F, const double *r, const d
 void xs_tri_sweep_ell_s(hipStream_t s, const xs_ilu &F, const double *r, const double *u,
                         double *sv, double *yn, const xs_state *st) {
   const xs_tell &E = F.tl;
+  if (tell_dia(E)) {
+    XS_GUARD(E.ld >= F.n && r && u && sv && yn && st);
+    const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+    hipLaunchKernelGGL((k_tri_sweep_dia<false, true, true>), grid, blk, 0, s, F.n, E.ld, E.nd,
+                       E.doff, E.dv, F.dinv, r, nullptr, yn, st, u, sv);
+    return;
+  }
   XS_GUARD((E.W == 0 || (E.ci && E.v)) && E.ld >= F.n && r && u && sv && yn && st);
   const dim3 grid((F.n + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
   if (E.co)

@@ -54,6 +54,7 @@ struct xs_ctx {
   int trsv_prog = -1;           // SpTRSV group mode (xs_tsched::prog), -1 = by factor width
   int trsv_gather = 0;          // SpTRSV gather width (4 or 16), 0 = by factor width
   bool spmv_dia = true;         // diagonal-slot copy of A for the FP64 SpMV when possible
+  bool sweep_dia = true;        // diagonal-slot copies of L / U for the FP64 sweeps
   int spmm_cw = 0;              // k-RHS ELL SpMM columns per thread (2 or 4), 0 = 4 when k % 4 == 0
   int spmv_vec = -1;            // CSR kernel: -1 by row regularity, 0 LDS row blocks, 1 vector CSR
   int precision = 0;
