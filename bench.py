@@ -151,81 +151,6 @@ PROFILE_NAMES = {
 }
 
 
-def run_gpu(A, b, fmt, steps, warmup, profile=True, precision=0, factor_storage=0, sweeps=None):
-    import torch
-    from xsolver_amd import XSolver
-
-    dev = torch.device("cuda:0")
-    rp, ci, v = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in A)
-    bd = torch.from_numpy(b).to(dev)
-    xd = torch.zeros_like(bd)
-    S = XSolver()
-    S.mat_setup_device(rp, ci, v, use_ell=(fmt == "ell"))
-    S.ksp_settype("bicgstab")
-    S.ksp_setoption(1e-10, 2000)
-    S.pc_settype("ilu0")
-    S.set_option("parilu_sweeps", 20)
-    S.set_option("parilu_async", 1)
-    S.set_option("precision", precision)
-    S.set_option("factor_storage", factor_storage)
-    if sweeps is not None:   # the reference's iterative ILU apply (Jacobi sweeps)
-        S.set_option("lsolve_sweeps", sweeps[0])
-        S.set_option("usolve_sweeps", sweeps[1])
-
-    def step():
-        S.set_option("refactor", 1)
-        S.pc_setup()
-        xd.zero_()
-        st = S.solve_device(bd, xd)
-        assert st == 0, f"solve status {st}"
-
-    for _ in range(warmup):
-        step()
-    S.set_option("profile", (1 << 1) | (1 << 2) | (1 << 3) if profile else 0)
-    pc_ms, solve_ms, its = [], [], []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
-        pc_ms.append(S.time_ms("pc_setup"))
-        solve_ms.append(S.time_ms("solve"))
-        its.append(S.iterations)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    kern = {}
-    for k in ("spmv", "trsv_lower", "trsv_upper"):
-        cnt = S.time_ms("count:" + k)
-        tot = S.time_ms("total:" + k)
-        if cnt > 0:
-            kern[k] = {"count": int(cnt), "total_ms": tot, "avg_us": 1000.0 * tot / cnt}
-    x = xd.cpu().numpy()
-    idx = (S.time_ms("spmv:idx16"), S.time_ms("pc:idx16"), S.time_ms("spmv:dia"),
-           S.time_ms("pc:dia"))
-    S.close()
-    return {
-        "idx16": idx,
-        "sec_per_step": (t1 - t0) / steps,
-        "pc_setup_ms": float(np.mean(pc_ms)),
-        "solve_ms": float(np.mean(solve_ms)),
-        "iterations": its,
-        "kernels": kern,
-        "x": x,
-    }
-
-
-# library timing families -> (kernel name in the rocprofv3 summary, launches)
-PROFILE_NAMES = {
-    "spmv": [("k_spmv_dia<2>", 1)],
-    "ilu_lower_apply": [("k_tri_sweep_ell_px<true>", 0.5),
-                        ("k_tri_sweep_ell<false, true, true, true>", 0.5),
-                        ("k_tri_sweep_ell<false, false, true, false>", APPLY[0] - 2)],
-    "ilu_upper_apply": [("k_tri_sweep_ell<true, true, true, false>", 1),
-                        ("k_tri_sweep_ell<true, false, true, false>", APPLY[1] - 2)],
-    "trsv_lower_exact": [("k_trsv<false, 4>", 1)],
-    "trsv_upper_exact": [("k_trsv<true, 4>", 1)],
-}
-
-
 def measured_traffic(family):
     """HBM bytes per launch (per apply sequence) of `family` on the 128^3
     workload from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
