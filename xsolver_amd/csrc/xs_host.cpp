@@ -75,7 +75,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.app); dfree(F.apd); dfree(F.cmap); dfree(F.lc); dfree(F.mpos);
   dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.vlo); dfree(F.tl.co); dfree(F.tl.ww);
   dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co); dfree(F.tu.ww); dfree(F.tu.vs);
-  dfree(F.tl.dv); dfree(F.tu.dv); dfree(F.tu.dvs);
+  dfree(F.tl.dv); dfree(F.tu.dv); dfree(F.tu.dvs); dfree(F.tl.dvlo); dfree(F.tu.dvlo);
   c->tell_ready = c->tell32_ready = false;
   F = xs_ilu();
   F.trsv_wg = c->trsv_wg;

@@ -128,6 +128,8 @@ struct xs_tell {
   int nd = 0;
   xs_dia_offs doff{};
   double *dv = nullptr, *dvs = nullptr;
+  void *dvlo = nullptr;      // dv in the factor storage type (mixed inner solver)
+  size_t dvlo_bytes = 0;
 };
 
 // Work order of a sync-free triangular solve: the row groups (gb: ngroups + 1

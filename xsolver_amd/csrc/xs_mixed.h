@@ -9,6 +9,10 @@ enum { XS_F32 = 0, XS_BF16 = 1, XS_F16 = 2 };   // factor storage types
 void xs_launch_spmv_csr32(hipStream_t s, const xs_csr &A, const float *v32,
                           const xs_spmv_plan &P, const float *x, float *y, int epi,
                           const float *b, const float *w, const xs_reduce *red);
+// FP32 SpMV over d32, the FP32 copy of E's diagonal-slot values (E.nd > 0)
+void xs_launch_spmv_dia32(hipStream_t s, const xs_ell &E, const float *d32, const float *x,
+                          float *y, int epi, const float *b, const float *w,
+                          const xs_reduce *red);
 void xs_launch_spmv_ell32(hipStream_t s, const xs_ell &E, const float *v32, const float *x,
                           float *y, int epi, const float *b, const float *w,
                           const xs_reduce *red);

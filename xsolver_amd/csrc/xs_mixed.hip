@@ -259,6 +259,79 @@ void xs_launch_spmv_csr32(hipStream_t s, const xs_csr &A, const float *v32,
   }
 }
 
+// k_spmv_ell32 over the FP32 copy of A's diagonal-slot values (no index
+// loads; the zero slots of rows that lack a diagonal read a clamped x)
+template <int EPI>
+__global__ __launch_bounds__(XS_BLOCK) void k_spmv_dia32(
+    int n, int ld, int W, const xs_dia_offs doff, const float *__restrict__ v,
+    const float *__restrict__ x, float *__restrict__ y, const float *__restrict__ b,
+    const float *__restrict__ w, xs_reduce R) {
+  constexpr int NV = (EPI == EPI_DOT2) ? 2 : 1;
+  constexpr bool RED = (EPI == EPI_DOT1 || EPI == EPI_DOT2 || EPI == EPI_BMAXN);
+  if (R.st != nullptr && R.st->done) return;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  const int r = 2 * (xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x);
+  if (r < n) {
+    constexpr bool OPB = (EPI == EPI_BMAX || EPI == EPI_BMAXN);
+    constexpr bool OPW = (EPI == EPI_DOT1 || EPI == EPI_DOT2);
+    float e0 = 0.f, e1 = 0.f;
+    if (OPB || OPW) {
+      const float *op = OPB ? b : w;
+      e0 = op[r];
+      if (r + 1 < n) e1 = op[r + 1];
+    }
+    float s0 = 0.f, s1 = 0.f;
+    constexpr int UK = 4;
+#pragma unroll
+    for (int k0 = 0; k0 < XS_DIA_MAX; k0 += UK) {
+      if (k0 >= W) break;
+      float2 a[UK];
+      int c0[UK], c1[UK];
+#pragma unroll
+      for (int q = 0; q < UK; ++q) {
+        const int k = min(k0 + q, W - 1);
+        a[q] = *reinterpret_cast<const float2 *>(v + (size_t)k * ld + r);
+        c0[q] = min(max(r + doff.o[k], 0), n - 1);
+        c1[q] = min(max(r + 1 + doff.o[k], 0), n - 1);
+      }
+      float x0[UK], x1[UK];
+#pragma unroll
+      for (int q = 0; q < UK; ++q) { x0[q] = x[c0[q]]; x1[q] = x[c1[q]]; }
+#pragma unroll
+      for (int q = 0; q < UK; ++q) {
+        if (k0 + q < W) { s0 += a[q].x * x0[q]; s1 += a[q].y * x1[q]; }
+      }
+    }
+    epi32_v<EPI, NV>(r, s0, e0, y, acc);
+    if (r + 1 < n) epi32_v<EPI, NV>(r + 1, s1, e1, y, acc);
+  }
+  if (RED) reduce_finish<NV>(acc, R);
+}
+
+void xs_launch_spmv_dia32(hipStream_t s, const xs_ell &E, const float *d32, const float *x,
+                          float *y, int epi, const float *b, const float *w,
+                          const xs_reduce *red) {
+  xs_reduce R{};
+  if (red) R = *red;
+  if (!(d32 && E.nd > 0 && E.nd <= XS_DIA_MAX && E.ld % 2 == 0 && x && y)) {
+    xs_raise_launch_guard();
+    return;
+  }
+  const int threads = (E.n + 1) / 2;
+  dim3 grid((threads + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+#define XS_DIAK32(E_) hipLaunchKernelGGL((k_spmv_dia32<E_>), grid, blk, 0, s, E.n, E.ld, E.nd, E.doff, d32, x, y, b, w, R)
+  switch (epi) {
+    case EPI_Y: XS_DIAK32(EPI_Y); break;
+    case EPI_BMAX: XS_DIAK32(EPI_BMAX); break;
+    case EPI_DOT1: XS_DIAK32(EPI_DOT1); break;
+    case EPI_DOT2: XS_DIAK32(EPI_DOT2); break;
+    case EPI_BMAXN: XS_DIAK32(EPI_BMAXN); break;
+  }
+#undef XS_DIAK32
+}
+
 void xs_launch_spmv_ell32(hipStream_t s, const xs_ell &E, const float *v32, const float *x,
                           float *y, int epi, const float *b, const float *w,
                           const xs_reduce *red) {
@@ -693,9 +766,97 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
   if (two) yn[r + 1] = UPPER ? d1 * (b1 - s1) : b1 - s1;
 }
 
+// k_tri_sweep_ell32 over the diagonal-slot copy of a factor part (E.dvlo in
+// the factor storage type, E.nd slots): two rows per thread, one paired
+// value load per slot and no index loads; the zero slots of rows that lack a
+// diagonal read a clamped column and add zero products.
+template <bool UPPER, bool FUSED, typename TF, bool SX = false>
+__global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_dia32(int n, int ld, int nd,
+                                                              const xs_dia_offs doff,
+                                                              const TF *__restrict__ ev,
+                                                              const float *__restrict__ dinv,
+                                                              const float *__restrict__ b,
+                                                              const float *__restrict__ yo,
+                                                              float *__restrict__ yn,
+                                                              const xs_state *st,
+                                                              const float *__restrict__ ru = nullptr,
+                                                              float *__restrict__ sv = nullptr) {
+  static_assert(!SX || (FUSED && !UPPER), "s is formed in the fused L pass only");
+  if (st != nullptr && st->done) return;
+  const int r = 2 * (xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x);
+  if (r >= n) return;
+  const bool two = r + 1 < n;
+  const float gama = SX ? (float)st->sc[S_GAMA] : 0.f;
+  const float b0 = SX ? fmaf(-gama, ru[r], b[r]) : b[r];
+  const float b1 = two ? (SX ? fmaf(-gama, ru[r + 1], b[r + 1]) : b[r + 1]) : 0.f;
+  if (SX) {
+    sv[r] = b0;
+    if (two) sv[r + 1] = b1;
+  }
+  const float d0 = UPPER ? dinv[r] : 1.f, d1 = (UPPER && two) ? dinv[r + 1] : 1.f;
+  float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+  for (int k0 = 0; k0 < XS_DIA_MAX; k0 += 4) {
+    if (k0 >= nd) break;
+    int c0[4], c1[4];
+    float a0[4], a1[4], y0[4], y1[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = min(k0 + q, nd - 1);
+      const float2 av = fl2(ev, (size_t)k * ld + r);
+      a0[q] = av.x;
+      a1[q] = av.y;
+      c0[q] = min(max(r + doff.o[k], 0), n - 1);
+      c1[q] = min(max(r + 1 + doff.o[k], 0), n - 1);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int g0 = c0[q], g1 = c1[q];
+      if (SX) {
+        y0[q] = fmaf(-gama, ru[g0], b[g0]) - 0.f;
+        y1[q] = fmaf(-gama, ru[g1], b[g1]) - 0.f;
+      } else if (FUSED) {
+        y0[q] = UPPER ? dinv[g0] * (b[g0] - 0.f) : b[g0] - 0.f;
+        y1[q] = UPPER ? dinv[g1] * (b[g1] - 0.f) : b[g1] - 0.f;
+      } else {
+        y0[q] = yo[g0];
+        y1[q] = yo[g1];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (k0 + q < nd) {
+        s0 += a0[q] * y0[q];
+        s1 += a1[q] * y1[q];
+      }
+    }
+  }
+  yn[r] = UPPER ? d0 * (b0 - s0) : b0 - s0;
+  if (two) yn[r + 1] = UPPER ? d1 * (b1 - s1) : b1 - s1;
+}
+
+static bool tell_dia32(const xs_tell &E) {
+  return E.nd > 0 && E.nd <= XS_DIA_MAX && E.dvlo != nullptr;
+}
+
 void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_tell &E, int ftype,
                         const float *dinv, const float *b, const float *yo, float *yn,
                         const xs_state *st) {
+  if (tell_dia32(E)) {
+    if (!(E.ld >= n && E.ld % 2 == 0 && b && yn && (!upper || dinv) && (fused || yo))) {
+      xs_raise_launch_guard();
+      return;
+    }
+    const int pairs = (n + 1) / 2;
+    const dim3 grid((pairs + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+#define XS_SWD32(U, FU, T) hipLaunchKernelGGL((k_tri_sweep_dia32<U, FU, T>), grid, blk, 0, s, n, E.ld, E.nd, E.doff, static_cast<const T *>(E.dvlo), dinv, b, yo, yn, st)
+#define XS_SWT(U, FU) do { if (ftype == XS_BF16) XS_SWD32(U, FU, __hip_bfloat16); else if (ftype == XS_F16) XS_SWD32(U, FU, __half); else XS_SWD32(U, FU, float); } while (0)
+    if (upper) { if (fused) XS_SWT(true, true); else XS_SWT(true, false); }
+    else { if (fused) XS_SWT(false, true); else XS_SWT(false, false); }
+#undef XS_SWT
+#undef XS_SWD32
+    return;
+  }
   if (!((E.W == 0 || (E.ci && E.vlo)) && E.ld >= n && E.ld % 2 == 0 && b && yn &&
         (!upper || dinv) && (fused || yo))) {
     xs_raise_launch_guard();
@@ -715,6 +876,20 @@ void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_t
 
 void xs_tri_sweep_ell32_s(hipStream_t s, int n, const xs_tell &E, int ftype, const float *r,
                           const float *u, float *sv, float *yn, const xs_state *st) {
+  if (tell_dia32(E)) {
+    if (!(E.ld >= n && E.ld % 2 == 0 && r && u && sv && yn && st)) {
+      xs_raise_launch_guard();
+      return;
+    }
+    const int pairs = (n + 1) / 2;
+    const dim3 grid((pairs + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+#define XS_SWS(T) hipLaunchKernelGGL((k_tri_sweep_dia32<false, true, T, true>), grid, blk, 0, s, n, E.ld, E.nd, E.doff, static_cast<const T *>(E.dvlo), nullptr, r, nullptr, yn, st, u, sv)
+    if (ftype == XS_BF16) XS_SWS(__hip_bfloat16);
+    else if (ftype == XS_F16) XS_SWS(__half);
+    else XS_SWS(float);
+#undef XS_SWS
+    return;
+  }
   if (!((E.W == 0 || (E.ci && E.vlo)) && E.ld >= n && E.ld % 2 == 0 && r && u && sv && yn && st)) {
     xs_raise_launch_guard();
     return;

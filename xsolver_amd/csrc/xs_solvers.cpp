@@ -530,6 +530,7 @@ static int ensure_buf(T **p, size_t need) {
 void xs_mixed_free(xs_ctx *c) {
   auto f = [](auto *&p) { if (p) (void)hipFree((void *)p); p = nullptr; };
   f(c->A32); f(c->ell32); f(c->dinv32); f(c->jac32); f(c->F32); f(c->work32); f(c->st_in);
+  f(c->dia32);
   f(c->basis32);
   c->basis32_len = 0;
   c->work32_n = 0;
@@ -550,6 +551,10 @@ static int mixed_prepare(xs_ctx *c) {
   if (c->use_ell && c->ell.v) {
     if ((rc = ensure_buf(&c->ell32, (size_t)c->ell.W * c->ell.ld))) return rc;
     xs_d2f(s, c->ell.W * c->ell.ld, c->ell.v, c->ell32);
+    if (c->ell.dv) {
+      if ((rc = ensure_buf(&c->dia32, (size_t)c->ell.nd * c->ell.ld))) return rc;
+      xs_d2f(s, c->ell.nd * c->ell.ld, c->ell.dv, c->dia32);
+    }
   }
   if (c->pc == XS_PC_JACOBI) {
     if ((rc = ensure_buf(&c->jac32, (size_t)n))) return rc;
@@ -576,6 +581,10 @@ static int mixed_prepare(xs_ctx *c) {
         if (E->W == 0) continue;
         if ((rc = grow(E->vlo, E->vlo_bytes, es * E->cap))) return rc;
         xs_cvt_factor(s, E->W * E->ld, E->v, E->vlo, c->factor_type);
+        if (E->nd > 0 && E->dv) {   // the diagonal-slot copy too
+          if ((rc = grow(E->dvlo, E->dvlo_bytes, es * (size_t)E->nd * E->ld))) return rc;
+          xs_cvt_factor(s, E->nd * E->ld, E->dv, E->dvlo, c->factor_type);
+        }
       }
       c->tell32_ready = true;
     }
@@ -594,7 +603,9 @@ static int mixed_prepare(xs_ctx *c) {
 static void op_spmv32(xs_ctx *c, const float *x, float *y, int epi, const float *b,
                       const float *w, const xs_reduce *red) {
   xs_prof_begin(c, 1);
-  if (c->use_ell && c->ell32)
+  if (c->use_ell && c->ell32 && c->ell.dv && c->dia32)
+    xs_launch_spmv_dia32(c->stream, c->ell, c->dia32, x, y, epi, b, w, red);
+  else if (c->use_ell && c->ell32)
     xs_launch_spmv_ell32(c->stream, c->ell, c->ell32, x, y, epi, b, w, red);
   else
     xs_launch_spmv_csr32(c->stream, c->A, c->A32, c->plan, x, y, epi, b, w, red);

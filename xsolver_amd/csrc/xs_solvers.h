@@ -124,6 +124,7 @@ struct xs_ctx {
   bool tell_ready = false;      // ELL sweep copies of the factors match ilu.lu
   bool tell32_ready = false;    // and their FP32 values (mixed inner solver)
   float *A32 = nullptr, *ell32 = nullptr, *dinv32 = nullptr, *jac32 = nullptr;
+  float *dia32 = nullptr;       // FP32 copy of A's diagonal-slot values (ell.dv)
   void *F32 = nullptr;
   size_t F32_bytes = 0;         // allocated bytes of F32
   float *work32 = nullptr;
