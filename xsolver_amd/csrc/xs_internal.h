@@ -250,7 +250,7 @@ int xs_dia_offsets(hipStream_t s, const xs_csr &A, int *d_tmp, xs_dia_offs &out)
 // E.dv from A's values in the slots of E.doff (E.nd slots, E.ld rows)
 void xs_build_dia(hipStream_t s, const xs_csr &A, xs_ell &E);
 // The same for the strictly lower (upper) part of the factors: offsets, then
-// E.dv (and for U E.dvs = u_ij / u_jj) from the current factor values.
+// E.dv (and for U E.dvs = u_ij / u_jj) from the part's current ELL copy.
 int xs_tri_dia_offsets(hipStream_t s, const xs_ilu &F, bool upper, int *d_tmp, xs_dia_offs &out);
 void xs_build_tri_dia(hipStream_t s, const xs_ilu &F, bool upper, xs_tell &E);
 // Largest |column - row| over the non-padding entries of ELL slabs (rows < n,

@@ -595,32 +595,35 @@ int xs_tri_dia_offsets(hipStream_t s, const xs_ilu &F, bool upper, int *d_tmp, x
   return dia_offsets(s, F.n, F.rp, F.ci, F.dpos, upper ? 2 : 1, d_tmp, out);
 }
 
-// diagonal slots of the strictly lower (upper) factor part; U also writes the
-// values scaled by the pivot inverse of their column (as k_tell_scale_cols)
+// Diagonal slots of a factor part from its ELL copy (slab-major, read
+// coalesced): slab j of row r holds column eci[j ld + r] (-1: padding) with
+// value ev (and, for U, u_ij / u_jj in evs, the values of the fused first
+// pass); each goes to the slot of its offset, the others stay 0.
 template <bool UPPER>
-__global__ void k_build_tri_dia(int n, int ld, int nd, const xs_dia_offs doff,
-                                const int *__restrict__ rp, const int *__restrict__ ci,
-                                const double *__restrict__ lu, const int *__restrict__ dpos,
-                                const double *__restrict__ dinv, double *__restrict__ dv,
-                                double *__restrict__ dvs) {
+__global__ void k_tell_to_dia(int n, int ld, int W, int nd, const xs_dia_offs doff,
+                              const int *__restrict__ eci, const double *__restrict__ ev,
+                              const double *__restrict__ evs, double *__restrict__ dv,
+                              double *__restrict__ dvs) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= ld) return;
   double slot[XS_DIA_MAX], sc[XS_DIA_MAX];
 #pragma unroll
   for (int k = 0; k < XS_DIA_MAX; ++k) { slot[k] = 0.0; sc[k] = 0.0; }
-  if (r < n) {
-    const int e0 = UPPER ? dpos[r] + 1 : rp[r], e1 = UPPER ? rp[r + 1] : dpos[r];
-    for (int e = e0; e < e1; ++e) {
-      const int c = ci[e], o = c - r;
-      const double v = lu[e];
+  if (r < n)
+    for (int j = 0; j < W; ++j) {
+      const size_t p = (size_t)j * ld + r;
+      const int c = eci[p];
+      if (c < 0) continue;
+      const int o = c - r;
+      const double v = ev[p];
+      const double vs = UPPER ? evs[p] : 0.0;
 #pragma unroll
       for (int k = 0; k < XS_DIA_MAX; ++k)
         if (k < nd && doff.o[k] == o) {
           slot[k] = v;
-          if (UPPER) sc[k] = v * dinv[c];
+          if (UPPER) sc[k] = vs;
         }
     }
-  }
 #pragma unroll
   for (int k = 0; k < XS_DIA_MAX; ++k)
     if (k < nd) {
@@ -629,16 +632,18 @@ __global__ void k_build_tri_dia(int n, int ld, int nd, const xs_dia_offs doff,
     }
 }
 
+// from the ELL copy, which must be current (xs_build_tell, and for U
+// xs_tell_scale_cols)
 void xs_build_tri_dia(hipStream_t s, const xs_ilu &F, bool upper, xs_tell &E) {
-  XS_GUARD(F.rp && F.ci && F.lu && F.dpos && E.dv && E.nd > 0 && E.nd <= XS_DIA_MAX &&
-           E.ld >= F.n && (!upper || (E.dvs && F.dinv)));
+  XS_GUARD(E.ci && E.v && E.dv && E.nd > 0 && E.nd <= XS_DIA_MAX && E.ld >= F.n &&
+           (!upper || (E.dvs && E.vs)));
   const dim3 grid((E.ld + 255) / 256), blk(256);
   if (upper)
-    hipLaunchKernelGGL(k_build_tri_dia<true>, grid, blk, 0, s, F.n, E.ld, E.nd, E.doff, F.rp, F.ci,
-                       F.lu, F.dpos, F.dinv, E.dv, E.dvs);
+    hipLaunchKernelGGL(k_tell_to_dia<true>, grid, blk, 0, s, F.n, E.ld, E.W, E.nd, E.doff, E.ci,
+                       E.v, E.vs, E.dv, E.dvs);
   else
-    hipLaunchKernelGGL(k_build_tri_dia<false>, grid, blk, 0, s, F.n, E.ld, E.nd, E.doff, F.rp,
-                       F.ci, F.lu, F.dpos, F.dinv, E.dv, E.dvs);
+    hipLaunchKernelGGL(k_tell_to_dia<false>, grid, blk, 0, s, F.n, E.ld, E.W, E.nd, E.doff, E.ci,
+                       E.v, nullptr, E.dv, nullptr);
 }
 
 __global__ void k_build_dia(int n, int ld, int nd, const xs_dia_offs doff,
