@@ -181,7 +181,11 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *   "sweep_dia"      1 (default) = the same for the strictly lower and upper
  *                    factor parts in the FP64 Jacobi-sweep apply (at most 8
  *                    distinct offsets each: ILU(0) of a 7-point stencil has
- *                    3); 0 = ELL copies only */
+ *                    3); 0 = ELL copies only
+ *   "parilu_dia"     1 (default) = ParILU(0) sweeps over diagonal slots
+ *                    (no index lists) when A has at most 8 offsets; the
+ *                    deterministic sweeps give bit for bit the factors of
+ *                    the list sweeps; 0 = the update lists */
 int xs_set_option(xs_ctx *ctx, const char *name, double value);
 
 /* Autotuner (no reference counterpart): times, on this matrix, the CSR
@@ -206,7 +210,8 @@ int xs_set_option(xs_ctx *ctx, const char *name, double value);
  * used by the Jacobi-sweep apply (-1 before they are built);
  * xs_get_time_ms("spmv:dia") = the number of diagonal slots of the FP64
  * SpMV's copy (0: none), xs_get_time_ms("pc:dia") those of L's sweep copy
- * (-1 before the sweep copies are built). */
+ * (-1 before the sweep copies are built), xs_get_time_ms("parilu:dia")
+ * those of the ParILU(0) sweeps (0: update lists). */
 int xs_autotune(xs_ctx *ctx);
 
 /* Factorise / prepare the preconditioner now (otherwise done by the first

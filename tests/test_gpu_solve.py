@@ -80,6 +80,42 @@ def test_parilu_async_reference_default(dev):
     g["S"].close()
 
 
+@pytest.mark.parametrize("name", ["convdiff3d_20", "grf2d_40", "lognormal27_10"])
+def test_parilu_diagonal_slots_same_factors(dev, name):
+    """ParILU(0) of a matrix with at most 8 offsets (5- and 7-point stencils)
+    sweeps over diagonal slots instead of index lists: with the deterministic
+    sweeps the factors are bit for bit those of the list version (same
+    products in the same order, same pivot guards); the asynchronous sweeps
+    give a solve to rtol.  The 27-point matrix keeps the lists."""
+    A = {"convdiff3d_20": lambda: gen.convdiff3d(20),
+         "grf2d_40": lambda: gen.vardiff2d(gen.grf_kappa(40, 0.1, 4)),
+         "lognormal27_10": lambda: gen.lognormal27(10, seed=2)}[name]()
+    n, nnz = len(A[0]) - 1, len(A[2])
+    xs, b = gen.manufactured(A, 6)
+    facs = []
+    for dia in (1, 0):
+        S = XSolver()
+        S.mat_setup(*A)
+        S.ksp_settype("bicgstab")
+        S.ksp_setoption(1e-10, 1000, 0)
+        S.pc_settype("ilu0")
+        S.set_option("parilu_dia", dia)
+        S.set_option("parilu_async", 0)
+        S.pc_setup()
+        nd = S.time_ms("parilu:dia")
+        say(f"[parilu-dia] {name} parilu_dia={dia}: {int(nd)} slots")
+        assert nd == ((0 if name == "lognormal27_10" else 7 if name == "convdiff3d_20" else 5)
+                      if dia else 0)
+        facs.append(S.factors(n, nnz)[2].copy())
+        x, st = S.solve(b)
+        assert st == 0 and true_relres(A, b, x) < 1e-10
+        S.close()
+    assert np.array_equal(facs[0], facs[1])
+    g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000)   # asynchronous sweeps
+    assert g["st"] == 0 and true_relres(A, b, g["x"]) < 1e-10
+    g["S"].close()
+
+
 def test_device_resident_solve_matches_host_solve(dev):
     import torch
     A = gen.convdiff3d(24)

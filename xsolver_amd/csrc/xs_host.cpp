@@ -73,6 +73,7 @@ static void free_precond(xs_ctx *c) {
   dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);
   dfree(F.app); dfree(F.apd); dfree(F.cmap); dfree(F.lc); dfree(F.mpos);
+  dfree(F.pmask); dfree(F.dmap); dfree(F.das); dfree(F.dlc); dfree(F.dlc2);
   dfree(F.tl.ci); dfree(F.tl.v); dfree(F.tl.vlo); dfree(F.tl.co); dfree(F.tl.ww);
   dfree(F.tu.ci); dfree(F.tu.v); dfree(F.tu.vlo); dfree(F.tu.co); dfree(F.tu.ww); dfree(F.tu.vs);
   dfree(F.tl.dv); dfree(F.tu.dv); dfree(F.tu.dvs); dfree(F.tl.dvlo); dfree(F.tu.dvlo);
@@ -831,6 +832,10 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     c->ilu.tell_shape = false;   // rebuilt (with or without them) before the next sweep
     c->tell_ready = c->tell32_ready = false;
   }
+  else if (k == "parilu_dia") {   // 1: ParILU(0) sweeps over diagonal slots when possible
+    if (value < 0 || value > 1) return XS_ERR_ARG;
+    if ((value != 0) != c->parilu_dia) { c->parilu_dia = value != 0; free_precond(c); }
+  }
   else if (k == "trsv_progressive") {   // -1 auto, 0 scan chain groups, 1 lane by lane
     if (value < -1 || value > 1) return XS_ERR_ARG;
     c->trsv_prog = (int)value;
@@ -1008,6 +1013,54 @@ static int ensure_trsv_sched(xs_ctx *c, bool exact) {
 
 // Update lists of the ParILU(0) / ParIC(0) sweeps (once per pattern, on
 // the device): count the products of every entry, scan, fill.
+// ParILU(0) over diagonal slots when A has at most XS_DIA_MAX distinct
+// offsets (stencils): the slot pairs of every offset's products (host, from
+// the offsets alone), the row masks and the CSR -> slot map (device).  Left
+// off (F.pnd = 0: the update lists are used) otherwise.
+static int build_parilu_dia(xs_ctx *c) {
+  xs_ilu &F = c->ilu;
+  xs_dia_offs off{};
+  const int nd = xs_dia_offsets(c->stream, c->A, c->d_int, off);
+  if (nd < 0) return XS_ERR_DEVICE;
+  if (nd == 0) return XS_OK;
+  xs_dia_pairs P{};
+  P.diag = -1;
+  for (int s = 0; s < nd; ++s)
+    if (off.o[s] == 0) P.diag = s;
+  if (P.diag < 0) return XS_OK;
+  for (int s = 0; s < nd; ++s) {
+    const int o = off.o[s];
+    P.kind[s] = o < 0 ? 0 : o == 0 ? 1 : 2;
+    P.np[s] = 0;
+    for (int a = 0; a < nd; ++a) {   // offsets ascending: l's column ascending
+      const int o1 = off.o[a];
+      if (o1 >= std::min(o, 0)) break;
+      for (int b = 0; b < nd; ++b)
+        if (off.o[b] == o - o1) {
+          P.pl[s][P.np[s]] = (signed char)a;
+          P.pu[s][P.np[s]] = (signed char)b;
+          P.np[s]++;
+        }
+    }
+    if (P.kind[s] == 2 && P.np[s] == 0) P.kind[s] = 3;
+  }
+  const int n = c->A.n;
+  const size_t len = (size_t)nd * n;
+  int rc;
+  if ((rc = dalloc(&F.pmask, n)) || (rc = dalloc(&F.dmap, c->A.nnz)) || (rc = dalloc(&F.das, len)) ||
+      (rc = dalloc(&F.dlc, len)))
+    return rc;
+  XS_HIP(hipMemsetAsync(F.das, 0, sizeof(double) * len, c->stream));
+  XS_HIP(hipMemsetAsync(F.dlc, 0, sizeof(double) * len, c->stream));
+  F.pnd = nd;
+  F.pld = n;
+  F.poff = off;
+  F.ppairs = P;
+  xs_parilu_dia_map(c->stream, c->A, F);
+  XS_HIP(hipStreamSynchronize(c->stream));
+  return xs_take_launch_guard() ? XS_ERR_STATE : XS_OK;
+}
+
 static int build_parilu_lists(xs_ctx *c, bool ic) {
   xs_ilu &F = c->ilu;
   const int nnz = c->A.nnz;
@@ -1049,6 +1102,7 @@ static int build_parilu_lists(xs_ctx *c, bool ic) {
   if (hipStreamSynchronize(c->stream) != hipSuccess) return done2(XS_ERR_DEVICE);
   if (xs_take_launch_guard()) return done2(XS_ERR_STATE);
   F.lists = ic ? 2 : 1;
+  if (!ic && c->parilu_dia && (rc = build_parilu_dia(c))) return done2(rc);
   return done2(XS_OK);
 }
 
@@ -1134,7 +1188,21 @@ static int factor_parilu(xs_ctx *c) {
     xs_pt_scale_vec(c->stream, c->A.n, c->A.rp, c->A.ci, c->A.v, F.scale);
     sc = F.scale;
   }
-  if (F.pp) {
+  if (F.pnd > 0) {
+    // the same over diagonal slots (no index lists)
+    if (!c->parilu_async && !F.dlc2) {
+      int rc = dalloc(&F.dlc2, (size_t)F.pnd * F.pld);
+      if (rc) return rc;
+      XS_HIP(hipMemsetAsync(F.dlc2, 0, sizeof(double) * (size_t)F.pnd * F.pld, c->stream));
+    }
+    double *cur = F.dlc, *nxt = c->parilu_async ? F.dlc : F.dlc2;
+    xs_parilu_dia_start(c->stream, c->A, F, sc, c->parilu_async ? nullptr : F.dlc2);
+    for (int s = 0; s < c->parilu_sweeps; s++) {
+      xs_parilu_dia_sweep(c->stream, F, cur, nxt);
+      std::swap(cur, nxt);
+    }
+    xs_parilu_dia_finish(c->stream, c->A, F, sc, cur);
+  } else if (F.pp) {
     // start values: the (scaled) matrix in sweep order, in both buffers of a
     // deterministic run; entries off the active list keep them.  The last
     // iterate goes back to CSR order, scaled back, in one pass.
@@ -1220,6 +1288,8 @@ int xs_parilut_factor(xs_ctx *c) {
   dfree(F.lu); dfree(F.dpos); dfree(F.dinv); dfree(F.rowidx); dfree(F.rp_own); dfree(F.ci_own);
   dfree(F.ucp); dfree(F.uri); dfree(F.upos);
   dfree(F.pp); dfree(F.pl); dfree(F.pu); dfree(F.pd); dfree(F.as); dfree(F.act);   // ILU(0) lists only
+  dfree(F.pmask); dfree(F.dmap); dfree(F.das); dfree(F.dlc); dfree(F.dlc2);
+  F.pnd = 0;
   dfree(F.app); dfree(F.apd); dfree(F.cmap); dfree(F.lc); dfree(F.mpos);
   dfree(F.lsched); dfree(F.usched); dfree(F.lgb); dfree(F.ugb);
   F.lists = 0;
@@ -1887,6 +1957,7 @@ extern "C" double xs_get_time_ms(const xs_ctx *c, const char *phase) {
   if (!strcmp(phase, "spmv:idx16")) return c->ell.co ? 1.0 : 0.0;
   if (!strcmp(phase, "spmv:dia")) return (c->use_ell && c->ell.dv) ? (double)c->ell.nd : 0.0;
   if (!strcmp(phase, "pc:dia")) return c->tell_ready ? (double)c->ilu.tl.nd : -1.0;
+  if (!strcmp(phase, "parilu:dia")) return (double)c->ilu.pnd;
   // 1 when both sweep copies of the factors carry 16-bit offsets (-1: not built)
   if (!strcmp(phase, "pc:idx16"))
     return !c->tell_ready ? -1.0

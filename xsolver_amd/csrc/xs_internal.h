@@ -89,6 +89,18 @@ struct xs_spmv_plan {   // row blocks for the LDS-staged CSR kernel
 struct xs_dia_offs {    // by value into the kernel: slot k holds column row + o[k]
   int o[XS_DIA_MAX];
 };
+// ParILU(0) over diagonal slots (xs_ilu::pnd > 0): for the slot of offset
+// o = doff.o[s], the products l(i, i + o1) u(i + o1, i + o) as pairs of
+// slots (pl, pu) with o1 = doff.o[pl] < min(o, 0) and doff.o[pu] = o - o1 > 0,
+// ascending in o1 (the column of l); kind: 0 below the diagonal (divided by
+// the pivot of its column), 1 the diagonal, 2 above with products, 3 above
+// without any (never swept).
+struct xs_dia_pairs {
+  int np[XS_DIA_MAX];
+  signed char pl[XS_DIA_MAX][XS_DIA_MAX], pu[XS_DIA_MAX][XS_DIA_MAX];
+  signed char kind[XS_DIA_MAX];
+  int diag;
+};
 struct xs_ell {         // column-major slabs: v[k * ld + row], width W
   int n = 0, ld = 0, W = 0;
   int *ci = nullptr;
@@ -186,6 +198,17 @@ struct xs_ilu {
   double *as = nullptr, *lc = nullptr;
   // which update lists the pattern carries: 0 none, 1 ParILU(0), 2 ParIC(0)
   int lists = 0;
+  // ParILU(0) over diagonal slots (A with at most XS_DIA_MAX offsets): pnd
+  // slots of pld rows; pmask[i] bit s = row i has slot s; dmap[k] = the slot
+  // position (s * pld + i) of CSR entry k; das the (scaled) start values and
+  // dlc / dlc2 the iterates there (dlc2: the deterministic sweeps' second
+  // buffer).  Positions without an entry hold 0 and are never written.
+  int pnd = 0, pld = 0;
+  xs_dia_offs poff{};
+  xs_dia_pairs ppairs{};
+  unsigned char *pmask = nullptr;
+  int *dmap = nullptr;
+  double *das = nullptr, *dlc = nullptr, *dlc2 = nullptr;
   // ParIC(0): position of each entry's mirror (ci[k], rowidx[k]) in A
   int *mpos = nullptr;
   // SpTRSV work order: the 64-row groups of L (resp. U, counted from the
@@ -253,6 +276,15 @@ void xs_build_dia(hipStream_t s, const xs_csr &A, xs_ell &E);
 // E.dv (and for U E.dvs = u_ij / u_jj) from the part's current ELL copy.
 int xs_tri_dia_offsets(hipStream_t s, const xs_ilu &F, bool upper, int *d_tmp, xs_dia_offs &out);
 void xs_build_tri_dia(hipStream_t s, const xs_ilu &F, bool upper, xs_tell &E);
+// ParILU(0) over diagonal slots: the slot map and row masks of A's pattern,
+// the start values (and copies into lc2 when not null), one sweep (lc_in ==
+// lc_out: asynchronous in place), and the factor back into F.lu in CSR order
+void xs_parilu_dia_map(hipStream_t s, const xs_csr &A, xs_ilu &F);
+void xs_parilu_dia_start(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
+                         double *lc2);
+void xs_parilu_dia_sweep(hipStream_t s, const xs_ilu &F, const double *lc_in, double *lc_out);
+void xs_parilu_dia_finish(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
+                          const double *lc);
 // Largest |column - row| over the non-padding entries of ELL slabs (rows < n,
 // column >= 0); synchronous, -1 on failure.  Then the 16-bit offsets: padding
 // columns (-1) become XS_OFF16_PAD, rows >= n point at row n - 1.

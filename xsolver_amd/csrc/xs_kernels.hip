@@ -1085,6 +1085,91 @@ void xs_parilu_finish(hipStream_t s, const xs_csr &A, const xs_ilu &F, const dou
                      F.rowidx, A.ci, sc, F.cmap, lc, F.lu);
 }
 
+// --- ParILU(0) over diagonal slots -----------------------------------------
+__global__ void k_parilu_dia_map(int n, int ld, int nd, const xs_dia_offs doff,
+                                 const int *__restrict__ rp, const int *__restrict__ ci,
+                                 unsigned char *__restrict__ mask, int *__restrict__ dmap) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  unsigned m = 0;
+  for (int e = rp[r]; e < rp[r + 1]; ++e) {
+    const int o = ci[e] - r;
+    int slot = -1;
+#pragma unroll
+    for (int k = 0; k < XS_DIA_MAX; ++k)
+      if (k < nd && doff.o[k] == o) slot = k;
+    if (slot < 0) continue;   // not reached: the offsets were detected on this pattern
+    m |= 1u << slot;
+    dmap[e] = slot * ld + r;
+  }
+  mask[r] = (unsigned char)m;
+}
+
+void xs_parilu_dia_map(hipStream_t s, const xs_csr &A, xs_ilu &F) {
+  XS_GUARD(A.rp && A.ci && F.pmask && F.dmap && F.pnd > 0 && F.pnd <= XS_DIA_MAX && F.pld >= A.n);
+  hipLaunchKernelGGL(k_parilu_dia_map, dim3((A.n + 255) / 256), dim3(256), 0, s, A.n, F.pld, F.pnd,
+                     F.poff, A.rp, A.ci, F.pmask, F.dmap);
+}
+
+void xs_parilu_dia_start(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
+                         double *lc2) {
+  XS_GUARD(A.ci && A.v && F.rowidx && F.dmap && F.das && F.dlc);
+  hipLaunchKernelGGL(k_parilu_start, dim3((A.nnz + 255) / 256), dim3(256), 0, s, A.nnz,
+                     F.rowidx, A.ci, A.v, sc, F.dmap, F.das, F.dlc, lc2);
+}
+
+void xs_parilu_dia_finish(hipStream_t s, const xs_csr &A, const xs_ilu &F, const double *sc,
+                          const double *lc) {
+  XS_GUARD(A.ci && F.rowidx && F.dmap && F.lu && lc);
+  hipLaunchKernelGGL(k_parilu_finish, dim3((A.nnz + 255) / 256), dim3(256), 0, s, A.nnz,
+                     F.rowidx, A.ci, sc, F.dmap, lc, F.lu);
+}
+
+// One ParILU(0) sweep over the diagonal slots, a row per thread: for each of
+// the row's entries the arithmetic of k_parilu_sweep_pairs -- the start
+// value minus the products l(i, k) u(k, j) over the pattern in ascending k,
+// then the pivot guard and division (below the diagonal) or the diagonal's
+// guard -- reading the slots of the row and of the rows k and j instead of
+// index lists.  A pair counts only where both of its entries are in the
+// pattern (the row masks), as the list version's pattern merge.
+__global__ __launch_bounds__(XS_BLOCK) void k_parilu_dia_sweep(
+    int n, int ld, int nd, const xs_dia_offs doff, const xs_dia_pairs P,
+    const unsigned char *__restrict__ mask, const double *__restrict__ as, const double *lc_in,
+    double *lc_out) {
+  const int i = xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x;
+  if (i >= n) return;
+  const unsigned m = mask[i];
+#pragma unroll
+  for (int sl = 0; sl < XS_DIA_MAX; ++sl) {
+    if (sl >= nd || !((m >> sl) & 1u) || P.kind[sl] == 3) continue;
+    const size_t p = (size_t)sl * ld + i;
+    double v = as[p];
+    for (int q = 0; q < P.np[sl]; ++q) {
+      const int a = P.pl[sl][q], b = P.pu[sl][q];
+      if (!((m >> a) & 1u)) continue;
+      const int k = i + doff.o[a];
+      if (!((mask[k] >> b) & 1u)) continue;
+      v -= lc_in[(size_t)a * ld + i] * lc_in[(size_t)b * ld + k];
+    }
+    double nv;
+    if (P.kind[sl] == 0) {
+      double dd = lc_in[(size_t)P.diag * ld + i + doff.o[sl]];
+      if (fabs(dd) < 1e-10) dd = dd >= 0.0 ? 1e-10 : -1e-10;
+      nv = v / dd;
+    } else {
+      if (P.kind[sl] == 1 && fabs(v) < 1e-10) v = v >= 0.0 ? 1e-10 : -1e-10;
+      nv = v;
+    }
+    lc_out[p] = isfinite(nv) ? nv : lc_in[p];
+  }
+}
+
+void xs_parilu_dia_sweep(hipStream_t s, const xs_ilu &F, const double *lc_in, double *lc_out) {
+  XS_GUARD(F.pmask && F.das && lc_in && lc_out && F.pnd > 0 && F.pnd <= XS_DIA_MAX);
+  hipLaunchKernelGGL(k_parilu_dia_sweep, dim3((F.n + XS_BLOCK - 1) / XS_BLOCK), dim3(XS_BLOCK), 0,
+                     s, F.n, F.pld, F.pnd, F.poff, F.ppairs, F.pmask, F.das, lc_in, lc_out);
+}
+
 // One sweep over the update lists: the arithmetic of k_parilu_sweep (same
 // products in the same order, same pivot guard), without the merge, over the
 // active entries only (an entry above the diagonal without products is its

@@ -55,6 +55,7 @@ struct xs_ctx {
   int trsv_gather = 0;          // SpTRSV gather width (4 or 16), 0 = by factor width
   bool spmv_dia = true;         // diagonal-slot copy of A for the FP64 SpMV when possible
   bool sweep_dia = true;        // diagonal-slot copies of L / U for the FP64 sweeps
+  bool parilu_dia = true;       // ParILU(0) sweeps over diagonal slots when possible
   int spmm_cw = 0;              // k-RHS ELL SpMM columns per thread (2 or 4), 0 = 4 when k % 4 == 0
   int spmv_vec = -1;            // CSR kernel: -1 by row regularity, 0 LDS row blocks, 1 vector CSR
   int precision = 0;
