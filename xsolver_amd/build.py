@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libxsolver_amd.so")
 ARCH = os.environ.get("XS_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["xs_kernels.hip", "xs_krylov.hip", "xs_parilut.hip", "xs_ic.hip", "xs_intel.hip", "xs_mixed.hip", "xs_batch.hip", "xs_multi.hip", "xs_host.cpp", "xs_solvers.cpp", "xs_batch_host.cpp", "xs_multi_host.cpp", "xs_io.cpp", "xs_mcsr.cpp"]
+SOURCES = ["xs_kernels.hip", "xs_krylov.hip", "xs_parilut.hip", "xs_split.hip", "xs_ic.hip", "xs_intel.hip", "xs_mixed.hip", "xs_batch.hip", "xs_multi.hip", "xs_host.cpp", "xs_solvers.cpp", "xs_batch_host.cpp", "xs_multi_host.cpp", "xs_io.cpp", "xs_mcsr.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed-FP32 instructions are disabled for all device code (-packed-fp32-ops).
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",

@@ -994,9 +994,10 @@ static int build_trsv_schedules(xs_ctx *c, const int *rp, const int *ci, const i
   return XS_OK;
 }
 
-// The groups of a pattern built on the device (ParILUT), on the first solve
-// that applies a factor by the exact triangular solve (sweep counts < 0);
-// until then the solves would fall back to fixed 64-row groups in row order.
+// The row groups of a factor pattern (ILU(0) / IC(0) on A's pattern,
+// ParILUT's own), on the first solve that applies the factor by the exact
+// triangular solve (sweep counts < 0): a host pass over a copy of the
+// pattern.
 static int ensure_trsv_sched(xs_ctx *c, bool exact) {
   xs_ilu &F = c->ilu;
   if (!F.sched_pending || (!exact && c->lsweeps >= 0 && c->usweeps >= 0)) return XS_OK;
@@ -1106,31 +1107,65 @@ static int build_parilu_lists(xs_ctx *c, bool ic) {
   return done2(XS_OK);
 }
 
+// Column view of m (column, position) pairs on the device (xs_split.hip):
+// a stable sort by column, the column pointers cp (ncol + 1), rows[q] =
+// rowidx[pos[q]], the sorted positions (pos, when given) and perm[pos[q]] = q
+// (when given).  key / val are overwritten.
+static int device_columns(xs_ctx *c, int m, int ncol, unsigned *key, int *val, const int *rowidx,
+                          int *cp, int *rows, int *pos, int *perm) {
+  hipStream_t s = c->stream;
+  unsigned *key2 = nullptr;
+  int *val2 = nullptr, *hist = nullptr, *off = nullptr, *bsum = nullptr;
+  auto done = [&](int r) {
+    dfree(key2); dfree(val2); dfree(hist); dfree(off); dfree(bsum);
+    return r;
+  };
+  const size_t H = xs_colsort_hist_len(m);
+  int rc;
+  if ((rc = dalloc(&key2, (size_t)std::max(m, 1))) || (rc = dalloc(&val2, (size_t)std::max(m, 1))) ||
+      (rc = dalloc(&hist, H)) || (rc = dalloc(&off, H)) ||
+      (rc = dalloc(&bsum, (size_t)xs_scan_bsum_len((int)(H - 1)))))
+    return done(rc);
+  const int which = xs_colsort(s, m, ncol, key, val, key2, val2, hist, off, bsum);
+  const unsigned *sk = which ? key2 : key;
+  const int *sv = which ? val2 : val;
+  xs_key_ptr(s, m, ncol, sk, cp);
+  xs_sorted_rows(s, m, rowidx, sv, rows, perm);
+  if (pos && m > 0 &&
+      hipMemcpyAsync(pos, sv, sizeof(int) * (size_t)m, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return done(XS_ERR_DEVICE);
+  // the scratch is freed below: the sort must be done with it
+  if (hipStreamSynchronize(s) != hipSuccess) return done(XS_ERR_DEVICE);
+  if (xs_take_launch_guard()) return done(XS_ERR_STATE);
+  return done(XS_OK);
+}
+
+// CSR transpose on the device: trp / tci (rows of the transpose sorted) and
+// perm[e] = position of entry e in the transpose; rowidx = row of each entry.
+static int device_transpose(xs_ctx *c, int n, int nnz, const int *ci, const int *rowidx, int *trp,
+                            int *tci, int *perm) {
+  unsigned *key = nullptr;
+  int *val = nullptr, rc;
+  if ((rc = dalloc(&key, (size_t)std::max(nnz, 1))) || (rc = dalloc(&val, (size_t)std::max(nnz, 1)))) {
+    dfree(key);
+    return rc;
+  }
+  xs_all_pairs(c->stream, nnz, ci, key, val);
+  rc = device_columns(c, nnz, n, key, val, rowidx, trp, tci, nullptr, perm);
+  dfree(key);
+  dfree(val);
+  return rc;
+}
+
 static int setup_ilu_pattern(xs_ctx *c, bool ic) {
   // Combined-factor layout on A's pattern plus the CSC of the upper triangle
-  // used by the ParILU sweep (the reference builds L CSR / U CSC with
-  // hipsparseDcsr2csc, hip/Precond/par_ilu_hip/Asplit_gpu.cpp:62-161).
+  // used by the ParILU sweep, built on the device (the reference builds L CSR
+  // / U CSC with a count pass and hipsparseDcsr2csc,
+  // hip/Precond/par_ilu_hip/Asplit_gpu.cpp:4-49, 62-161).  A's rows are
+  // sorted by this point (xs_mat_setup sorts them, xs_mat_setup_device
+  // rejects unsorted ones).
   const int n = c->A.n, nnz = c->A.nnz;
-  const std::vector<int> &rp = c->h_rp, &ci = c->h_ci;
-  std::vector<int> dpos(n), rowidx(nnz), ucnt(n + 1, 0);
-  for (int i = 0; i < n; i++) {
-    dpos[i] = -1;
-    for (int e = rp[i]; e < rp[i + 1]; e++) {
-      rowidx[e] = i;
-      if (ci[e] == i) dpos[i] = e;
-      if (ci[e] >= i) ucnt[ci[e] + 1]++;
-    }
-    if (dpos[i] < 0) return XS_ERR_PIVOT;
-  }
-  for (int j = 0; j < n; j++) ucnt[j + 1] += ucnt[j];
-  const int nu = ucnt[n];
-  std::vector<int> uri(nu), upos(nu), fill(ucnt.begin(), ucnt.end() - 1);
-  for (int i = 0; i < n; i++)
-    for (int e = dpos[i]; e < rp[i + 1]; e++) {
-      const int q = fill[ci[e]]++;
-      uri[q] = i;
-      upos[q] = e;
-    }
+  hipStream_t s = c->stream;
   xs_ilu &F = c->ilu;
   F.n = n; F.nnz = nnz; F.rp = c->A.rp; F.ci = c->A.ci;
   int rc;
@@ -1139,13 +1174,37 @@ static int setup_ilu_pattern(xs_ctx *c, bool ic) {
   if ((rc = dalloc(&F.dinv, n))) return rc;
   if ((rc = dalloc(&F.rowidx, nnz))) return rc;
   if ((rc = dalloc(&F.ucp, n + 1))) return rc;
-  if ((rc = dalloc(&F.uri, nu))) return rc;
-  if ((rc = dalloc(&F.upos, nu))) return rc;
-  XS_HIP(hipMemcpy(F.dpos, dpos.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-  XS_HIP(hipMemcpy(F.rowidx, rowidx.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-  XS_HIP(hipMemcpy(F.ucp, ucnt.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-  XS_HIP(hipMemcpy(F.uri, uri.data(), sizeof(int) * nu, hipMemcpyHostToDevice));
-  XS_HIP(hipMemcpy(F.upos, upos.data(), sizeof(int) * nu, hipMemcpyHostToDevice));
+  XS_HIP(hipMemsetAsync(c->d_int, 0, sizeof(int), s));
+  xs_pt_rowidx(s, n, c->A.rp, F.rowidx);
+  xs_pt_dpos(s, n, c->A.rp, c->A.ci, F.dpos, c->d_int);
+  int bad = 0;
+  XS_HIP(hipMemcpyAsync(&bad, c->d_int, sizeof(int), hipMemcpyDeviceToHost, s));
+  XS_HIP(hipStreamSynchronize(s));
+  if (bad) return XS_ERR_PIVOT;
+  {
+    int *cnt = nullptr, *uoff = nullptr, *bsum = nullptr, *val = nullptr;
+    unsigned *key = nullptr;
+    auto done = [&](int r) {
+      dfree(cnt); dfree(uoff); dfree(bsum); dfree(val); dfree(key);
+      return r;
+    };
+    if ((rc = dalloc(&cnt, n)) || (rc = dalloc(&uoff, n + 1)) ||
+        (rc = dalloc(&bsum, xs_scan_bsum_len(n))))
+      return done(rc);
+    xs_upper_count(s, n, c->A.rp, F.dpos, cnt);
+    xs_scan_excl(s, n, cnt, uoff, bsum);
+    int nu = 0;
+    if (hipMemcpyAsync(&nu, uoff + n, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return done(XS_ERR_DEVICE);
+    if ((rc = dalloc(&F.uri, nu)) || (rc = dalloc(&F.upos, nu)) || (rc = dalloc(&key, nu)) ||
+        (rc = dalloc(&val, nu)))
+      return done(rc);
+    xs_upper_pairs(s, n, c->A.rp, c->A.ci, F.dpos, uoff, key, val);
+    if ((rc = device_columns(c, nu, n, key, val, F.rowidx, F.ucp, F.uri, F.upos, nullptr)))
+      return done(rc);
+    done(XS_OK);
+  }
   if (ic) {
     // every entry needs its mirror: U = D L~^T lives on A's upper pattern
     if ((rc = dalloc(&F.mpos, nnz))) return rc;
@@ -1160,7 +1219,10 @@ static int setup_ilu_pattern(xs_ctx *c, bool ic) {
     }
   }
   if ((rc = build_parilu_lists(c, ic))) return rc;
-  return build_trsv_schedules(c, rp.data(), ci.data(), dpos.data());
+  // the SpTRSV row groups and their order: on the first solve that applies
+  // the factor by exact triangular solves (the sweep applies use none)
+  F.sched_pending = true;
+  return XS_OK;
 }
 
 // ParILU(0) (hip/Precond/gpuparilu.cpp:150-165, 20 sweeps by default).  With
@@ -1974,39 +2036,26 @@ extern "C" double xs_get_time_ms(const xs_ctx *c, const char *phase) {
 // ---------------------------------------------------------------------------
 // transposes for the adjoint (no reference counterpart)
 // ---------------------------------------------------------------------------
-// Transposed pattern on the host (rows of the transpose are produced in
-// ascending column order, so they come out sorted); perm[k] = position of
-// entry k in the transpose.
-static void transpose_pattern(int n, const int *rp, const int *ci, std::vector<int> &trp,
-                              std::vector<int> &tci, std::vector<int> &perm) {
-  const int nnz = rp[n];
-  trp.assign(n + 1, 0);
-  for (int e = 0; e < nnz; e++) trp[ci[e] + 1]++;
-  for (int j = 0; j < n; j++) trp[j + 1] += trp[j];
-  std::vector<int> fill(trp.begin(), trp.end() - 1);
-  tci.resize(nnz);
-  perm.resize(nnz);
-  for (int i = 0; i < n; i++)
-    for (int e = rp[i]; e < rp[i + 1]; e++) {
-      const int q = fill[ci[e]]++;
-      tci[q] = i;
-      perm[e] = q;
-    }
-}
-
 int xs_build_transpose(xs_ctx *c) {
   if (c->have_At) return XS_OK;
   const int n = c->A.n, nnz = c->A.nnz;
-  std::vector<int> trp, tci, perm;
-  transpose_pattern(n, c->h_rp.data(), c->h_ci.data(), trp, tci, perm);
   int rc;
   dfree(c->At.rp); dfree(c->At.ci); dfree(c->At.v); dfree(c->perm_At); dfree(c->planT.blk_row);
   if ((rc = dalloc(&c->At.rp, n + 1)) || (rc = dalloc(&c->At.ci, nnz)) ||
       (rc = dalloc(&c->At.v, nnz)) || (rc = dalloc(&c->perm_At, nnz)))
     return rc;
-  XS_HIP(hipMemcpy(c->At.rp, trp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-  XS_HIP(hipMemcpy(c->At.ci, tci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-  XS_HIP(hipMemcpy(c->perm_At, perm.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+  // the transpose on the device; only its row pointers come back (the
+  // row-length bins of its SpMV plan)
+  {
+    int *row = nullptr;
+    if ((rc = dalloc(&row, nnz))) return rc;
+    xs_pt_rowidx(c->stream, n, c->A.rp, row);
+    rc = device_transpose(c, n, nnz, c->A.ci, row, c->At.rp, c->At.ci, c->perm_At);
+    dfree(row);
+    if (rc) return rc;
+  }
+  std::vector<int> trp(n + 1);
+  XS_HIP(hipMemcpy(trp.data(), c->At.rp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
   c->At.n = n;
   c->At.nnz = nnz;
   xs_permute_vals(c->stream, nnz, c->perm_At, c->A.v, c->At.v);
@@ -2023,14 +2072,6 @@ int xs_build_factor_transpose(xs_ctx *c) {
   xs_ilu &F = c->ilu;
   if (F.have_t) return XS_OK;
   const int n = F.n, nnz = F.nnz;
-  std::vector<int> frp(n + 1), fci(nnz), trp, tci, perm;
-  XS_HIP(hipMemcpy(frp.data(), F.rp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
-  XS_HIP(hipMemcpy(fci.data(), F.ci, sizeof(int) * nnz, hipMemcpyDeviceToHost));
-  transpose_pattern(n, frp.data(), fci.data(), trp, tci, perm);
-  std::vector<int> tdpos(n, -1);
-  for (int i = 0; i < n; i++)
-    for (int e = trp[i]; e < trp[i + 1]; e++)
-      if (tci[e] == i) tdpos[i] = e;
   int rc;
   dfree(F.trp); dfree(F.tci); dfree(F.tdpos); dfree(F.tlu); dfree(F.tperm);
   dfree(F.tlgb); dfree(F.tugb); dfree(F.tlsched); dfree(F.tusched);
@@ -2038,7 +2079,18 @@ int xs_build_factor_transpose(xs_ctx *c) {
   if ((rc = dalloc(&F.trp, n + 1)) || (rc = dalloc(&F.tci, nnz)) || (rc = dalloc(&F.tdpos, n)) ||
       (rc = dalloc(&F.tlu, nnz)) || (rc = dalloc(&F.tperm, nnz)))
     return rc;
-  // row groups and level order of the transposed triangular solves
+  // the transposed pattern and its pivot positions on the device (every
+  // factor row holds its pivot, so every transposed row does)
+  if ((rc = device_transpose(c, n, nnz, F.ci, F.rowidx, F.trp, F.tci, F.tperm))) return rc;
+  XS_HIP(hipMemsetAsync(c->d_int, 0, sizeof(int), c->stream));
+  xs_pt_dpos(c->stream, n, F.trp, F.tci, F.tdpos, c->d_int);
+  // row groups and level order of the transposed triangular solves (host,
+  // from a copy of the transposed pattern)
+  std::vector<int> trp(n + 1), tci(nnz), tdpos(n);
+  XS_HIP(hipStreamSynchronize(c->stream));
+  XS_HIP(hipMemcpy(trp.data(), F.trp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  XS_HIP(hipMemcpy(tci.data(), F.tci, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+  XS_HIP(hipMemcpy(tdpos.data(), F.tdpos, sizeof(int) * n, hipMemcpyDeviceToHost));
   {
     std::vector<int> gb, ord;
     for (int part = 0; part < 2; ++part) {
@@ -2056,10 +2108,6 @@ int xs_build_factor_transpose(xs_ctx *c) {
       XS_HIP(hipMemcpy(ds, ord.data(), sizeof(int) * G, hipMemcpyHostToDevice));
     }
   }
-  XS_HIP(hipMemcpy(F.trp, trp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-  XS_HIP(hipMemcpy(F.tci, tci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-  XS_HIP(hipMemcpy(F.tdpos, tdpos.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-  XS_HIP(hipMemcpy(F.tperm, perm.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
   xs_permute_vals(c->stream, nnz, F.tperm, F.lu, F.tlu);
   XS_HIP(hipStreamSynchronize(c->stream));
   if (xs_take_launch_guard()) return XS_ERR_STATE;

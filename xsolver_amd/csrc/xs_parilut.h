@@ -63,3 +63,21 @@ void xs_pt_scale_vec(hipStream_t s, int n, const int *arp, const int *aci, const
 void xs_pt_rescale(hipStream_t s, int nnz, const int *row, const int *ci, double *v,
                    const double *sc, int unscale);
 void xs_pt_rowidx(hipStream_t s, int n, const int *rp, int *row);
+
+// Pattern analysis on the device (xs_split.hip).  xs_colsort: stable sort of
+// m (key, val) pairs by key < ncol; returns 1 when the result is in key2 /
+// val2, 0 when in key / val.  hist and off hold xs_colsort_hist_len(m) ints,
+// bsum xs_scan_bsum_len(xs_colsort_hist_len(m)).
+size_t xs_colsort_hist_len(int m);
+int xs_colsort(hipStream_t s, int m, int ncol, unsigned *key, int *val, unsigned *key2, int *val2,
+               int *hist, int *off, int *bsum);
+// cp[j] = first sorted position with key >= j, j = 0..ncol
+void xs_key_ptr(hipStream_t s, int m, int ncol, const unsigned *skey, int *cp);
+// entries on and right of the pivot per row, and their (column, position)
+// pairs in row-major order (uoff from the exclusive scan of the counts)
+void xs_upper_count(hipStream_t s, int n, const int *rp, const int *dpos, int *cnt);
+void xs_upper_pairs(hipStream_t s, int n, const int *rp, const int *ci, const int *dpos,
+                    const int *uoff, unsigned *key, int *val);
+void xs_all_pairs(hipStream_t s, int nnz, const int *ci, unsigned *key, int *val);
+// out[q] = rowidx[pos[q]]; perm[pos[q]] = q when perm is given
+void xs_sorted_rows(hipStream_t s, int m, const int *rowidx, const int *pos, int *out, int *perm);
