@@ -117,8 +117,11 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *   "parilu_async"   1 = in-place asynchronous sweeps (reference behaviour),
  *                    0 = deterministic double-buffered sweeps
  *   "check_every"    iterations launched between host convergence checks
- *   "use_graph"      1 (default) = replay each batch of check_every iterations
- *                    as a captured HIP graph, 0 = launch kernels one by one
+ *   "use_graph"      1 = replay each batch of check_every iterations as a
+ *                    captured HIP graph, 0 = launch kernels one by one,
+ *                    -1 (default) = graphs for k-RHS block solves only (a
+ *                    single solve's short kernels ran faster launched one by
+ *                    one, and its graph is captured anew in every solve)
  *   "precision"      0 = FP64, 1 = FP32 inner Krylov solve inside FP64
  *                    iterative refinement (BiCGSTAB/CG) or FP64 flexible
  *                    GMRES (GMRES/FGMRES)

@@ -217,6 +217,17 @@ def test_one_based_and_unsorted_input(dev):
     with pytest.raises(XSolverError) as e:
         S.mat_setup_device(*(torch.from_numpy(a).to(dev) for a in (rp, ci2, v2)))
     assert e.value.status == xsolver_amd.XS_ERR_ARG
+    # the device checks (only the row pointers come to the host): a repeated
+    # column, a column out of range, row pointers that go back
+    ci4 = ci.copy()
+    ci4[rp[7 + 1] - 1] = len(rp) - 1
+    rp5 = rp.copy()
+    rp5[3], rp5[4] = rp5[4], rp5[3]
+    for bad in ((rp, ci3, v), (rp, ci4, v), (rp5, ci, v)):
+        with pytest.raises(XSolverError) as e:
+            S.mat_setup_device(*(torch.from_numpy(a).to(dev) for a in bad))
+        assert e.value.status == xsolver_amd.XS_ERR_ARG
+    S.mat_setup_device(*(torch.from_numpy(a).to(dev) for a in (rp, ci, v)))
     S.close()
 
 

@@ -717,12 +717,22 @@ extern "C" int xs_mat_setup_device(xs_ctx *c, int n, int nnz, const int *d_rowpt
   c->A.rp = const_cast<int *>(d_rowptr);
   c->A.ci = const_cast<int *>(d_colidx);
   c->A.v = const_cast<double *>(d_val);
+  // only the row pointers come to the host (the SpMV plan's row bins); the
+  // column checks run on the device
   c->h_rp.resize(n + 1);
-  c->h_ci.resize(nnz);
+  c->h_ci.clear();
   XS_HIP(hipMemcpy(c->h_rp.data(), d_rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
-  XS_HIP(hipMemcpy(c->h_ci.data(), d_colidx, sizeof(int) * nnz, hipMemcpyDeviceToHost));
   if (c->h_rp[0] != 0 || c->h_rp[n] != nnz) return XS_ERR_ARG;
-  analyse_pattern(c);
+  for (int i = 0; i < n; i++)
+    if (c->h_rp[i + 1] < c->h_rp[i]) return XS_ERR_ARG;
+  if (!c->d_int) return XS_ERR_STATE;
+  int flag = 0;
+  XS_HIP(hipMemsetAsync(c->d_int, 0, sizeof(int), c->stream));
+  xs_check_pattern(c->stream, n, d_rowptr, d_colidx, c->d_int);
+  XS_HIP(hipMemcpyAsync(&flag, c->d_int, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  XS_HIP(hipStreamSynchronize(c->stream));
+  c->sorted = !(flag & 1);
+  c->has_diag = !(flag & 2);
   if (!c->sorted) return XS_ERR_ARG;
   return setup_common(c, n, nnz, use_ell);
 }
@@ -805,7 +815,7 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   if (k == "parilu_sweeps") { if (value < 0) return XS_ERR_ARG; c->parilu_sweeps = (int)value; c->pc_ready = false; }
   else if (k == "parilu_async") { c->parilu_async = value != 0; c->pc_ready = false; }
   else if (k == "check_every") { if (value < 1) return XS_ERR_ARG; c->check_every = (int)value; }
-  else if (k == "use_graph") { c->use_graph = value != 0; }
+  else if (k == "use_graph") { if (value < -1 || value > 1) return XS_ERR_ARG; c->use_graph = (int)value; }
   else if (k == "spmv_kernel") {   // -1 = automatic, 0 = LDS row blocks, 1 = vector CSR
     if (value != 0 && value != 1 && value != -1) return XS_ERR_ARG;
     c->spmv_vec = (int)value;

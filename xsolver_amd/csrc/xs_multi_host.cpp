@@ -164,7 +164,7 @@ struct multi_run {
     xs_graph_cache gc;
     while (!h->all && launched < c->maxiter) {
       const int step = std::min(c->check_every, c->maxiter - launched);
-      if (xs_run_batch(c, gc, step, iteration)) return XS_ERR_DEVICE;
+      if (xs_run_batch(c, gc, step, iteration, true)) return XS_ERR_DEVICE;
       launched += step;
       XS_CHK(hipGetLastError());
       XS_CHK(hipMemcpyAsync(h, ms, sizeof(xs_mstate), hipMemcpyDeviceToHost, s));

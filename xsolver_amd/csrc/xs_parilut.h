@@ -81,3 +81,6 @@ void xs_upper_pairs(hipStream_t s, int n, const int *rp, const int *ci, const in
 void xs_all_pairs(hipStream_t s, int nnz, const int *ci, unsigned *key, int *val);
 // out[q] = rowidx[pos[q]]; perm[pos[q]] = q when perm is given
 void xs_sorted_rows(hipStream_t s, int m, const int *rowidx, const int *pos, int *out, int *perm);
+// *flag |= 1: a column out of [0, n) or not strictly ascending in its row;
+// |= 2: a row without its diagonal (rp must be monotone with rp[n] = nnz)
+void xs_check_pattern(hipStream_t s, int n, const int *rp, const int *ci, int *flag);

@@ -48,7 +48,7 @@ struct xs_ctx {
   double tune_us = 0.0;
   int tune_L = 0, tune_long_thr = 0;  // vector-CSR lanes per row, long-row threshold
   double tune_trsv_us = 0.0;    // > 0 once the triangular solves are tuned
-  bool use_graph = true;        // replay batches of iterations as HIP graphs
+  int use_graph = -1;           // replay batches as HIP graphs: 1 all solves, 0 none, -1 k-RHS blocks
   bool trsv_schedule = true;    // SpTRSV groups in dependency-level order
   int trsv_wg = 256;            // persistent workgroups of the SpTRSV
   int trsv_prog = -1;           // SpTRSV group mode (xs_tsched::prog), -1 = by factor width
@@ -170,8 +170,9 @@ struct xs_graph_cache {
 };
 
 template <class F>
-int xs_run_batch(xs_ctx *c, xs_graph_cache &gc, int count, F &iteration) {
-  const bool want = c->use_graph && c->prof_kind == 0 && !gc.failed && count == c->check_every;
+int xs_run_batch(xs_ctx *c, xs_graph_cache &gc, int count, F &iteration, bool block = false) {
+  const bool graphs = c->use_graph > 0 || (c->use_graph < 0 && block);
+  const bool want = graphs && c->prof_kind == 0 && !gc.failed && count == c->check_every;
   if (want && !gc.exec) {
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {

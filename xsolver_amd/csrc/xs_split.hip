@@ -153,3 +153,26 @@ void xs_sorted_rows(hipStream_t s, int m, const int *rowidx, const int *pos, int
   hipLaunchKernelGGL(k_sorted_rows, dim3((m + 255) / 256), dim3(256), 0, s, m, rowidx, pos, out,
                      perm);
 }
+
+// pattern checks of a device CSR (row pointers already checked monotone on
+// the host, so every row's range lies in [0, nnz)): *flag |= 1 for a column
+// out of range or not strictly ascending within its row, |= 2 for a row
+// without its diagonal
+__global__ void k_check_pattern(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                int *flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int f = 0, prev = -1;
+  bool d = false;
+  for (int e = rp[i]; e < rp[i + 1]; ++e) {
+    const int j = ci[e];
+    if (j < 0 || j >= n || j <= prev) f |= 1;
+    d |= j == i;
+    prev = j;
+  }
+  if (!d) f |= 2;
+  if (f) atomicOr(flag, f);
+}
+void xs_check_pattern(hipStream_t s, int n, const int *rp, const int *ci, int *flag) {
+  hipLaunchKernelGGL(k_check_pattern, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, flag);
+}
