@@ -57,15 +57,15 @@ def algorithmic_bytes(n, nnz, nnz_l, nnz_u, sweeps, idx_a=4, idx_f=4):
     entry read or written.  nnz_l / nnz_u: strictly lower / upper entries of
     the factors.  An apply of k >= 2 sweeps is one fused launch (sweeps 1-2,
     no previous iterate) plus k - 2 plain ones; U also reads 1/u_ii.
-    BiCGSTAB's two L applies per iteration form its vector updates in their
-    fused launches: p = r + omega (p - alpha u) in the first (reads r, the
-    old p and u, writes p: 24n more than the plain fused launch's 16n) and
-    s = r - gama u in the second (reads r and u, writes s: 16n more), so the
-    L apply is counted at their mean, 20n above the plain fused launch.  The
-    exact SpTRSV reads CSR factors (4-byte columns)."""
+    Of BiCGSTAB's two L applies per iteration the second forms s = r - gama u
+    in its fused launch (reads r and u, writes s: 16n more than the plain
+    fused launch's 16n); the first is the plain one (p has its own update
+    kernel by default), so the L apply is counted at their mean, 8n above the
+    plain fused launch.  The exact SpTRSV reads CSR factors (4-byte
+    columns)."""
     kl, ku = sweeps
     ea, ef = 8 + idx_a, 8 + idx_f
-    l_fused, l_plain = ef * nnz_l + 36 * n, ef * nnz_l + 24 * n
+    l_fused, l_plain = ef * nnz_l + 24 * n, ef * nnz_l + 24 * n
     u_fused, u_plain = ef * nnz_u + 24 * n, ef * nnz_u + 32 * n
     return {
         "spmv": ea * nnz + 16 * n,                  # ELL values + columns, x, y
@@ -141,7 +141,7 @@ def run_gpu(A, b, fmt, steps, warmup, profile=True, precision=0, factor_storage=
 # library timing families -> (kernel name in the rocprofv3 summary, launches)
 PROFILE_NAMES = {
     "spmv": [("k_spmv_dia<2>", 1)],
-    "ilu_lower_apply": [("k_tri_sweep_dia_px", 0.5),
+    "ilu_lower_apply": [("k_tri_sweep_dia<false, true, false>", 0.5),
                         ("k_tri_sweep_dia<false, true, true>", 0.5),
                         ("k_tri_sweep_dia<false, false, false>", APPLY[0] - 2)],
     "ilu_upper_apply": [("k_tri_sweep_dia<true, true, false>", 1),

@@ -117,6 +117,9 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *   "parilu_async"   1 = in-place asynchronous sweeps (reference behaviour),
  *                    0 = deterministic double-buffered sweeps
  *   "check_every"    iterations launched between host convergence checks
+ *   "fuse_p"         1 = BiCGSTAB forms p inside the first L sweep of M^{-1} p
+ *                    (sweep apply, even check_every; same iterates), 0
+ *                    (default) = a separate update kernel
  *   "use_graph"      1 = replay each batch of check_every iterations as a
  *                    captured HIP graph, 0 = launch kernels one by one,
  *                    -1 (default) = graphs for k-RHS block solves only (a

@@ -480,13 +480,36 @@ def test_bicgstab_fused_updates_same_result(dev):
     out = []
     for every in (8, 7):
         g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000, lsolve_sweeps=5, usolve_sweeps=2,
-                      check_every=every, parilu_async=0)
+                      check_every=every, parilu_async=0, fuse_p=1, use_graph=1)
         say(f"[bicgstab fused] check_every={every} it={g['it']} "
             f"relres={true_relres(A, b, g['x']):.2e}")
         assert g["st"] == 0 and true_relres(A, b, g["x"]) < 1e-10
         out.append((g["x"], g["it"]))
         g["S"].close()
     assert out[0][1] == out[1][1] and np.array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("factor", [0, 1])
+def test_mixed_fused_p_same_result(dev, factor):
+    """The FP32 BiCGSTAB of the mixed solve forms p inside the first L pass
+    over the diagonal-slot copy (p alternating between two buffers), as the
+    FP64 loop does: bit for bit the iterate and iteration count of the
+    separate update kernel (fuse_p = 0), launched one by one and replayed as
+    graphs alike."""
+    A = gen.convdiff3d(24)
+    xs, b = gen.manufactured(A, 5)
+    out = []
+    for fuse, graph in ((1, 0), (0, 0), (1, 1)):
+        g = gpu_solve(A, b, "bicgstab", "ilu0", 1e-10, 1000, lsolve_sweeps=5, usolve_sweeps=3,
+                      precision=1, factor_storage=factor, parilu_async=0, fuse_p=fuse,
+                      use_graph=graph)
+        say(f"[mixed fuse_p] factor={factor} fuse_p={fuse} graph={graph}: it={g['it']} "
+            f"relres={true_relres(A, b, g['x']):.2e}")
+        assert g["st"] == 0 and true_relres(A, b, g["x"]) < 1e-10
+        out.append((g["x"].copy(), g["it"]))
+        g["S"].close()
+    for x, it in out[1:]:
+        assert it == out[0][1] and np.array_equal(x, out[0][0])
 
 
 @pytest.mark.parametrize("name", ["convdiff3d_24", "lognormal27_12"])

@@ -278,7 +278,7 @@ static int make_ell(xs_ctx *c) {
   int rc;
   xs_csr &A = c->A;
   if (c->ell.v) return XS_OK;
-  c->mixed_ready = false;   // no FP32 copy of the new ELL yet
+  c->mixed_ready = c->mixed_mat_ready = false;   // no FP32 copy of the new ELL yet
   int W = xs_max_row_len(c->stream, A, c->d_int);
   c->ell.n = A.n;
   c->ell.W = W;
@@ -583,7 +583,7 @@ extern "C" int xs_autotune(xs_ctx *c) {
 }
 
 static int setup_common(xs_ctx *c, int n, int nnz, int use_ell) {
-  c->mixed_ready = false;
+  c->mixed_ready = c->mixed_mat_ready = false;
   c->use_ell = use_ell ? 1 : 0;
   c->A.n = n;
   c->A.nnz = nnz;
@@ -748,7 +748,7 @@ extern "C" int xs_mat_update_values(xs_ctx *c, const double *val, int on_device)
     c->pc_ready = false;
     c->tell_ready = c->tell32_ready = false;
   }
-  c->mixed_ready = false;   // the FP32 copy of A follows the new values
+  c->mixed_ready = c->mixed_mat_ready = false;   // the FP32 copy of A follows the new values
   XS_HIP(hipStreamSynchronize(c->stream));
   return XS_OK;
 }
@@ -815,12 +815,13 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   if (k == "parilu_sweeps") { if (value < 0) return XS_ERR_ARG; c->parilu_sweeps = (int)value; c->pc_ready = false; }
   else if (k == "parilu_async") { c->parilu_async = value != 0; c->pc_ready = false; }
   else if (k == "check_every") { if (value < 1) return XS_ERR_ARG; c->check_every = (int)value; }
+  else if (k == "fuse_p") { c->fuse_p = value != 0; }
   else if (k == "use_graph") { if (value < -1 || value > 1) return XS_ERR_ARG; c->use_graph = (int)value; }
   else if (k == "spmv_kernel") {   // -1 = automatic, 0 = LDS row blocks, 1 = vector CSR
     if (value != 0 && value != 1 && value != -1) return XS_ERR_ARG;
     c->spmv_vec = (int)value;
     if (c->ell_auto && value >= 0) { c->use_ell = 0; c->ell_auto = false; }
-    c->mixed_ready = false;   // the FP32 SpMV follows the format
+    c->mixed_ready = c->mixed_mat_ready = false;   // the FP32 SpMV follows the format
     if (c->mat_ready) {
       c->plan.vec = c->spmv_vec < 0 ? (regular_rows(c->h_rp, c->plan) ? 1 : 0) : c->spmv_vec;
       c->planT.vec = c->spmv_vec < 0 ? c->plan.vec : c->spmv_vec;
@@ -835,6 +836,7 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     if (value < 0 || value > 1) return XS_ERR_ARG;
     c->spmv_dia = value != 0;
     if (c->ell.dv && !c->spmv_dia) { dfree(c->ell.dv); c->ell.nd = 0; }
+    c->mixed_ready = c->mixed_mat_ready = false;   // the FP32 SpMV follows the format
   }
   else if (k == "sweep_dia") {   // 1: diagonal-slot copies of L and U for the FP64 sweeps
     if (value < 0 || value > 1) return XS_ERR_ARG;
@@ -861,8 +863,8 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     c->spmm_cw = (int)value;
   }
   else if (k == "trsv_schedule") { c->trsv_schedule = value != 0; c->pc_ready = false; free_precond(c); }
-  else if (k == "lsolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->lsweeps = (int)value; }
-  else if (k == "usolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->usweeps = (int)value; }
+  else if (k == "lsolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->lsweeps = (int)value; c->mixed_ready = false; }
+  else if (k == "usolve_sweeps") { if (value < -1) return XS_ERR_ARG; c->usweeps = (int)value; c->mixed_ready = false; }
   else if (k == "refactor") { c->pc_ready = false; c->mixed_ready = false; c->tell_ready = c->tell32_ready = false; }
   else if (k == "factor_storage") { if (value < 0 || value > 2) return XS_ERR_ARG; c->factor_type = (int)value; c->mixed_ready = false; }
   else if (k == "mixed_reliable") { c->mixed_reliable = value != 0; }
@@ -890,7 +892,7 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
     }
     c->use_ell = want;
     c->ell_auto = false;
-    c->mixed_ready = false;
+    c->mixed_ready = c->mixed_mat_ready = false;
   } else return XS_ERR_ARG;
   return XS_OK;
 }

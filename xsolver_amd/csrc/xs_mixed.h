@@ -25,6 +25,12 @@ void xs_tri_sweep32(hipStream_t s, bool upper, int n, const int *rp, const int *
 void xs_tri_sweep_ell32(hipStream_t s, bool upper, bool fused, int n, const xs_tell &E, int ftype,
                         const float *dinv, const float *b, const float *yo, float *yn,
                         const xs_state *st);
+// the first L pass of M^{-1} p over the diagonal-slot copy with BiCGSTAB's
+// p = r + beta (pin - omega u) formed in it and written to pout (pin != pout)
+bool xs_tell32_dia(const xs_tell &E);
+void xs_tri_sweep_ell32_p(hipStream_t s, int n, const xs_tell &E, int ftype, const float *r,
+                          const float *u, const float *pin, float *pout, float *yn,
+                          const xs_state *st);
 // the fused first L pass with BiCGSTAB's s = r - gama u (gama in st) formed
 // in the pass and written to sv
 void xs_tri_sweep_ell32_s(hipStream_t s, int n, const xs_tell &E, int ftype, const float *r,

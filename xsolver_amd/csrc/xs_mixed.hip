@@ -766,11 +766,20 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_ell32(int n, int ld, int
   if (two) yn[r + 1] = UPPER ? d1 * (b1 - s1) : b1 - s1;
 }
 
+// BiCGSTAB's p = r + beta (p - omega u) in FP32 (beta in sc[S_OMEGA], omega
+// in sc[S_ALPHA]): one expression shared by the update kernel and the L pass
+// that forms p, so both round alike
+__device__ __forceinline__ float bicg_p32_val(float r, float p, float u, float beta, float omega) {
+  return r + beta * (p - omega * u);
+}
+
 // k_tri_sweep_ell32 over the diagonal-slot copy of a factor part (E.dvlo in
 // the factor storage type, E.nd slots): two rows per thread, one paired
 // value load per slot and no index loads; the zero slots of rows that lack a
 // diagonal read a clamped column and add zero products.
-template <bool UPPER, bool FUSED, typename TF, bool SX = false>
+// SX: s = b - gama ru formed here (b = r, ru = u), written to sv; PX: p =
+// b + beta (pin - omega ru) formed here (b = r, ru = u), written to sv
+template <bool UPPER, bool FUSED, typename TF, bool SX = false, bool PX = false>
 __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_dia32(int n, int ld, int nd,
                                                               const xs_dia_offs doff,
                                                               const TF *__restrict__ ev,
@@ -780,16 +789,21 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_dia32(int n, int ld, int
                                                               float *__restrict__ yn,
                                                               const xs_state *st,
                                                               const float *__restrict__ ru = nullptr,
-                                                              float *__restrict__ sv = nullptr) {
+                                                              float *__restrict__ sv = nullptr,
+                                                              const float *__restrict__ pin = nullptr) {
   static_assert(!SX || (FUSED && !UPPER), "s is formed in the fused L pass only");
+  static_assert(!PX || (FUSED && !UPPER && !SX), "p is formed in the fused L pass only");
   if (st != nullptr && st->done) return;
   const int r = 2 * (xcd_chunk((int)blockIdx.x, (int)gridDim.x) * XS_BLOCK + (int)threadIdx.x);
   if (r >= n) return;
   const bool two = r + 1 < n;
   const float gama = SX ? (float)st->sc[S_GAMA] : 0.f;
-  const float b0 = SX ? fmaf(-gama, ru[r], b[r]) : b[r];
-  const float b1 = two ? (SX ? fmaf(-gama, ru[r + 1], b[r + 1]) : b[r + 1]) : 0.f;
-  if (SX) {
+  const float beta = PX ? (float)st->sc[S_OMEGA] : 0.f, omega = PX ? (float)st->sc[S_ALPHA] : 0.f;
+  const float b0 = SX ? fmaf(-gama, ru[r], b[r]) : PX ? bicg_p32_val(b[r], pin[r], ru[r], beta, omega) : b[r];
+  const float b1 = two ? (SX ? fmaf(-gama, ru[r + 1], b[r + 1])
+                         : PX ? bicg_p32_val(b[r + 1], pin[r + 1], ru[r + 1], beta, omega) : b[r + 1])
+                       : 0.f;
+  if (SX || PX) {
     sv[r] = b0;
     if (two) sv[r + 1] = b1;
   }
@@ -815,6 +829,9 @@ __global__ __launch_bounds__(XS_BLOCK) void k_tri_sweep_dia32(int n, int ld, int
       if (SX) {
         y0[q] = fmaf(-gama, ru[g0], b[g0]) - 0.f;
         y1[q] = fmaf(-gama, ru[g1], b[g1]) - 0.f;
+      } else if (PX) {
+        y0[q] = bicg_p32_val(b[g0], pin[g0], ru[g0], beta, omega) - 0.f;
+        y1[q] = bicg_p32_val(b[g1], pin[g1], ru[g1], beta, omega) - 0.f;
       } else if (FUSED) {
         y0[q] = UPPER ? dinv[g0] * (b[g0] - 0.f) : b[g0] - 0.f;
         y1[q] = UPPER ? dinv[g1] * (b[g1] - 0.f) : b[g1] - 0.f;
@@ -901,6 +918,25 @@ void xs_tri_sweep_ell32_s(hipStream_t s, int n, const xs_tell &E, int ftype, con
   if (E.co) XS_SWT32S(true); else XS_SWT32S(false);
 #undef XS_SWT32S
 #undef XS_SWS32
+}
+
+bool xs_tell32_dia(const xs_tell &E) { return tell_dia32(E); }
+
+void xs_tri_sweep_ell32_p(hipStream_t s, int n, const xs_tell &E, int ftype, const float *r,
+                          const float *u, const float *pin, float *pout, float *yn,
+                          const xs_state *st) {
+  if (!(tell_dia32(E) && E.ld >= n && E.ld % 2 == 0 && r && u && pin && pout && pin != pout && yn &&
+        st)) {
+    xs_raise_launch_guard();
+    return;
+  }
+  const int pairs = (n + 1) / 2;
+  const dim3 grid((pairs + XS_BLOCK - 1) / XS_BLOCK), blk(XS_BLOCK);
+#define XS_SWP(T) hipLaunchKernelGGL((k_tri_sweep_dia32<false, true, T, false, true>), grid, blk, 0, s, n, E.ld, E.nd, E.doff, static_cast<const T *>(E.dvlo), nullptr, r, nullptr, yn, st, u, pout, pin)
+  if (ftype == XS_BF16) XS_SWP(__hip_bfloat16);
+  else if (ftype == XS_F16) XS_SWP(__half);
+  else XS_SWP(float);
+#undef XS_SWP
 }
 
 __global__ void k_fill_u32(unsigned *p, int n, unsigned val) {
@@ -1043,7 +1079,7 @@ __global__ __launch_bounds__(XS_BLOCK) void k_bicg_p32(int n, const float *__res
   if (st->done) return;
   const float omega = (float)st->sc[S_OMEGA], alpha = (float)st->sc[S_ALPHA];
   for (int i = blockIdx.x * XS_BLOCK + threadIdx.x; i < n; i += gridDim.x * XS_BLOCK) {
-    const float pi = r[i] + omega * (p[i] - alpha * u[i]);
+    const float pi = bicg_p32_val(r[i], p[i], u[i], omega, alpha);
     p[i] = pi;
     if (dinv) z[i] = dinv[i] * pi;
   }

@@ -100,7 +100,7 @@ int xs_solve_bicgstab(xs_ctx *c, const double *b, double *x) {
   // pass reads the old p at neighbouring rows, so the new p goes to the other
   // of two buffers, alternating.  An even batch of captured iterations ends
   // on the parity it started with, which replays of the batch rely on.
-  const bool pfuse = !jac && xs_pc_fuse_l(c) && c->check_every % 2 == 0;
+  const bool pfuse = !jac && c->fuse_p && xs_pc_fuse_l(c) && c->check_every % 2 == 0;
   double *p2 = xs_ctx_vec(c, 9);
   bool odd = false;
   return drive(c, [&]() {
@@ -533,6 +533,7 @@ void xs_mixed_free(xs_ctx *c) {
   f(c->dia32);
   f(c->basis32);
   c->basis32_len = 0;
+  c->mixed_mat_ready = false;
   c->work32_n = 0;
   c->F32_bytes = 0;
   c->mixed_ready = false;
@@ -546,15 +547,20 @@ static int mixed_prepare(xs_ctx *c) {
   hipStream_t s = c->stream;
   const int n = c->A.n;
   int rc;
-  if ((rc = ensure_buf(&c->A32, (size_t)c->A.nnz))) return rc;
-  xs_d2f(s, c->A.nnz, c->A.v, c->A32);
-  if (c->use_ell && c->ell.v) {
-    if ((rc = ensure_buf(&c->ell32, (size_t)c->ell.W * c->ell.ld))) return rc;
-    xs_d2f(s, c->ell.W * c->ell.ld, c->ell.v, c->ell32);
-    if (c->ell.dv) {
+  // the FP32 copy of A that op_spmv32 reads (diagonal slots, ELL or CSR);
+  // a refactorisation leaves it as it is
+  if (!c->mixed_mat_ready) {
+    if (c->use_ell && c->ell.v && c->ell.dv) {
       if ((rc = ensure_buf(&c->dia32, (size_t)c->ell.nd * c->ell.ld))) return rc;
       xs_d2f(s, c->ell.nd * c->ell.ld, c->ell.dv, c->dia32);
+    } else if (c->use_ell && c->ell.v) {
+      if ((rc = ensure_buf(&c->ell32, (size_t)c->ell.W * c->ell.ld))) return rc;
+      xs_d2f(s, c->ell.W * c->ell.ld, c->ell.v, c->ell32);
+    } else {
+      if ((rc = ensure_buf(&c->A32, (size_t)c->A.nnz))) return rc;
+      xs_d2f(s, c->A.nnz, c->A.v, c->A32);
     }
+    c->mixed_mat_ready = true;
   }
   if (c->pc == XS_PC_JACOBI) {
     if ((rc = ensure_buf(&c->jac32, (size_t)n))) return rc;
@@ -571,8 +577,13 @@ static int mixed_prepare(xs_ctx *c) {
       return XS_OK;
     };
     const size_t es = c->factor_type == XS_F32 ? 4 : 2;
-    if ((rc = grow(c->F32, c->F32_bytes, es * (size_t)c->ilu.nnz))) return rc;
-    xs_cvt_factor(s, c->ilu.nnz, c->ilu.lu, c->F32, c->factor_type);
+    // the CSR factor in low precision: read by the exact triangular solves
+    // and by sweeps that do not run over the sweep copies (fewer than two
+    // sweeps, no copies yet); the sweep counts clear mixed_ready
+    if (!(c->tell_ready && c->lsweeps >= 2 && c->usweeps >= 2)) {
+      if ((rc = grow(c->F32, c->F32_bytes, es * (size_t)c->ilu.nnz))) return rc;
+      xs_cvt_factor(s, c->ilu.nnz, c->ilu.lu, c->F32, c->factor_type);
+    }
     if ((rc = ensure_buf(&c->dinv32, (size_t)n))) return rc;
     xs_vec_d2f(s, n, c->ilu.dinv, c->dinv32);
     c->tell32_ready = false;
@@ -580,10 +591,12 @@ static int mixed_prepare(xs_ctx *c) {
       for (xs_tell *E : {&c->ilu.tl, &c->ilu.tu}) {
         if (E->W == 0) continue;
         if ((rc = grow(E->vlo, E->vlo_bytes, es * E->cap))) return rc;
-        xs_cvt_factor(s, E->W * E->ld, E->v, E->vlo, c->factor_type);
-        if (E->nd > 0 && E->dv) {   // the diagonal-slot copy too
+        if (E->nd > 0 && E->nd <= XS_DIA_MAX && E->dv) {
+          // the sweeps read the diagonal-slot copy only
           if ((rc = grow(E->dvlo, E->dvlo_bytes, es * (size_t)E->nd * E->ld))) return rc;
           xs_cvt_factor(s, E->nd * E->ld, E->dv, E->dvlo, c->factor_type);
+        } else {
+          xs_cvt_factor(s, E->W * E->ld, E->v, E->vlo, c->factor_type);
         }
       }
       c->tell32_ready = true;
@@ -592,7 +605,7 @@ static int mixed_prepare(xs_ctx *c) {
   if (c->work32_n != (size_t)n) {
     if (c->work32) (void)hipFree(c->work32);
     c->work32 = nullptr;
-    if ((rc = ensure_buf(&c->work32, (size_t)12 * n))) return rc;
+    if ((rc = ensure_buf(&c->work32, (size_t)13 * n))) return rc;
     c->work32_n = n;
   }
   if ((rc = ensure_buf(&c->st_in, 1))) return rc;
@@ -603,7 +616,7 @@ static int mixed_prepare(xs_ctx *c) {
 static void op_spmv32(xs_ctx *c, const float *x, float *y, int epi, const float *b,
                       const float *w, const xs_reduce *red) {
   xs_prof_begin(c, 1);
-  if (c->use_ell && c->ell32 && c->ell.dv && c->dia32)
+  if (c->use_ell && c->ell.dv && c->dia32)
     xs_launch_spmv_dia32(c->stream, c->ell, c->dia32, x, y, epi, b, w, red);
   else if (c->use_ell && c->ell32)
     xs_launch_spmv_ell32(c->stream, c->ell, c->ell32, x, y, epi, b, w, red);
@@ -686,6 +699,35 @@ static void op_pc32_s(xs_ctx *c, const float *r, const float *u, float *sv, floa
   float *w = c->work32 + 11 * (size_t)n;   // sweep scratch
   xs_prof_begin(c, 2);
   ilu_sweeps32(c, false, c->lsweeps, r, tmp, w, st, u, sv);
+  xs_prof_end(c, 2);
+  xs_prof_begin(c, 3);
+  if (c->usweeps >= 0)
+    ilu_sweeps32(c, true, c->usweeps, tmp, y, w, st);
+  else
+    xs_trsv32(c->stream, true, n, c->ilu.rp, c->ilu.ci, c->F32, c->factor_type, c->ilu.dpos,
+              c->dinv32, tmp, y, c->counters + 4, xs_ilu_tsched(c->ilu, true), st);
+  xs_prof_end(c, 3);
+}
+
+// BiCGSTAB's p = r + beta (pin - omega u) formed inside the first L pass over
+// the FP32 diagonal-slot copy and written to pout, then y = M^{-1} p
+static bool pc32_fuse_p(const xs_ctx *c) {
+  return xs_pc_factored(c->pc) && c->fuse_p && c->lsweeps >= 2 && c->tell32_ready &&
+         xs_tell32_dia(c->ilu.tl);
+}
+static void op_pc32_p(xs_ctx *c, const float *r, const float *u, const float *pin, float *pout,
+                      float *y, float *tmp, xs_state *st) {
+  const int n = c->A.n, m = c->lsweeps;
+  float *w = c->work32 + 11 * (size_t)n;   // sweep scratch
+  xs_prof_begin(c, 2);
+  float *cur = ((m - 2) % 2 == 0) ? tmp : w;
+  xs_tri_sweep_ell32_p(c->stream, n, c->ilu.tl, c->factor_type, r, u, pin, pout, cur, st);
+  for (int j = 3; j <= m; ++j) {
+    float *nxt = (cur == tmp) ? w : tmp;
+    xs_tri_sweep_ell32(c->stream, false, false, n, c->ilu.tl, c->factor_type, c->dinv32, pout, cur,
+                       nxt, st);
+    cur = nxt;
+  }
   xs_prof_end(c, 2);
   xs_prof_begin(c, 3);
   if (c->usweeps >= 0)
@@ -875,9 +917,19 @@ static int mixed_bicgstab_reliable(xs_ctx *c, const double *b, double *x) {
     xs_copy32(s, n, r, rh);
     xs_reduce rg = r0, ra = r0, rr = r0, rp = r0;
     rg.op = F_BICG_GAMA; ra.op = F_BICG_ALPHA; rr.op = F_BICG_RES; rp.op = F_BICG_REPL;
+    // p formed inside the first L pass, alternating between p and p2 as in
+    // the FP64 loop (an even batch ends on the parity it started with)
+    const bool pfuse = !jac && pc32_fuse_p(c) && c->check_every % 2 == 0;
+    float *p2 = W + 12 * (size_t)n;
+    bool odd = false;
     auto iteration = [&]() {
-      xs_bicg_p32(s, n, r, u, p, jac, z, si);
-      if (!jac) op_pc32(c, p, z, t, si);
+      if (pfuse) {
+        op_pc32_p(c, r, u, odd ? p2 : p, odd ? p : p2, z, t, si);
+        odd = !odd;
+      } else {
+        xs_bicg_p32(s, n, r, u, p, jac, z, si);
+        if (!jac) op_pc32(c, p, z, t, si);
+      }
       op_spmv32(c, z, u, EPI_DOT1, nullptr, rh, &rg);
       if (jac) xs_bicg_s32(s, n, r, u, sv, jac, z1, si);
       else op_pc32_s(c, r, u, sv, z1, t, si);

@@ -48,6 +48,7 @@ struct xs_ctx {
   double tune_us = 0.0;
   int tune_L = 0, tune_long_thr = 0;  // vector-CSR lanes per row, long-row threshold
   double tune_trsv_us = 0.0;    // > 0 once the triangular solves are tuned
+  bool fuse_p = false;          // BiCGSTAB forms p inside the first L pass (slower when launched directly)
   int use_graph = -1;           // replay batches as HIP graphs: 1 all solves, 0 none, -1 k-RHS blocks
   bool trsv_schedule = true;    // SpTRSV groups in dependency-level order
   int trsv_wg = 256;            // persistent workgroups of the SpTRSV
@@ -121,7 +122,8 @@ struct xs_ctx {
                                 // 0 = sqrt(tol), at least 1e-6 (two refinement steps)
   bool mixed_reliable = true;   // mixed BiCGSTAB: one FP32 recurrence with FP64
                                 // residual replacement (else restarted refinement)
-  bool mixed_ready = false;
+  bool mixed_ready = false;     // FP32 / low-precision copies of the factors
+  bool mixed_mat_ready = false; // FP32 copy of A for the SpMV format in use
   bool tell_ready = false;      // ELL sweep copies of the factors match ilu.lu
   bool tell32_ready = false;    // and their FP32 values (mixed inner solver)
   float *A32 = nullptr, *ell32 = nullptr, *dinv32 = nullptr, *jac32 = nullptr;
