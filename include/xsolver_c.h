@@ -97,8 +97,9 @@ const char *xs_status_string(int status);
 int xs_mat_setup(xs_ctx *ctx, int n, const int *rowptr, const int *colidx,
                  const double *val, const int *advanced_pam);
 /* Matrix already resident on the GPU (0-based CSR device pointers, borrowed
- * until xs_destroy / the next setup).  No reference counterpart: used when the
- * caller owns HBM-resident data. */
+ * until xs_destroy / the next setup; columns strictly ascending within rows,
+ * else XS_ERR_ARG).  No reference counterpart: used when the caller owns
+ * HBM-resident data. */
 int xs_mat_setup_device(xs_ctx *ctx, int n, int nnz, const int *d_rowptr,
                         const int *d_colidx, const double *d_val, int use_ell);
 /* Replace the values of the current pattern (host or device pointer), the
