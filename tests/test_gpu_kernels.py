@@ -338,6 +338,34 @@ PARILUT_MATS = {
 PARILUT_CASES = [(m, 0) for m in PARILUT_MATS] + [("lognormal27_10", 128), ("lognormal27_10", 512)]
 
 
+@pytest.mark.parametrize("name", ["lognormal27_10", "longrows_1500"])
+def test_parilut_small_sets_same_factors(dev, name):
+    """The candidate counts of every ParILUT step come from 512-slot sets,
+    the rows that fill theirs counted again in 2048-slot sets
+    (longrows_1500: rows of 600 and 450 entries, and the rows whose lower part
+    reaches them, take the second set).  Same counts, so the same factor as
+    counting every row in the 2048-slot set: same pattern, values to the
+    rounding of the sweeps' atomic sums."""
+    A = (gen.lognormal27(10, seed=2) if name == "lognormal27_10"
+         else random_sparse(1500, 4, 29, long_rows=[(3, 600), (300, 450)]))
+    n = len(A[0]) - 1
+    out = []
+    for small in (1, 0):
+        S = XSolver()
+        S.mat_setup(*A)
+        S.ksp_settype("gmres")
+        S.pc_settype("ilut")
+        S.pc_setilut(1e-4, 2)
+        S.set_option("parilut_small_sets", small)
+        S.pc_setup()
+        rp, ci, lu, dpos = S.factors(n, 4 * len(A[2]) + n)
+        out.append((rp.copy(), ci.copy(), lu.copy()))
+        S.close()
+    say(f"[parilut small sets] {name}: nnz {len(out[0][1])} / {len(out[1][1])}")
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert rel_err(out[0][2], out[1][2]) < 1e-12
+
+
 @pytest.mark.parametrize("name,rows", PARILUT_CASES, ids=[f"{m}-rows{r}" for m, r in PARILUT_CASES])
 def test_parilut_matches_oracle(dev, name, rows):
     """GPU ParILUT (10 steps, fill 2, drop 1e-4) against the oracle's

@@ -875,6 +875,7 @@ extern "C" int xs_set_option(xs_ctx *c, const char *name, double value) {
   else if (k == "parilu_scale") { c->parilu_scale = value != 0; c->pc_ready = false; }
   else if (k == "pc_reuse") { c->pc_reuse = value != 0; }
   else if (k == "parilut_steps") { if (value < 1) return XS_ERR_ARG; c->parilut_steps = (int)value; c->pc_ready = false; }
+  else if (k == "parilut_small_sets") { c->parilut_small_sets = value != 0; c->pc_ready = false; }
   else if (k == "parilut_rows") {
     if (value != 0 && value != 128 && value != 256 && value != 512) return XS_ERR_ARG;
     c->parilut_rows = (int)value;
@@ -1373,13 +1374,14 @@ int xs_parilut_factor(xs_ctx *c) {
   F.tell_shape = false;   // a new pattern: the sweep copies are re-shaped
   int rc;
   double *dtol = nullptr, *thr = nullptr;
-  int *cnt = nullptr, *bsum = nullptr, *ovf = nullptr;
+  int *cnt = nullptr, *bsum = nullptr, *ovf = nullptr, *lrows = nullptr, *nlong = nullptr;
   unsigned long long *sel = nullptr, *cnt2 = nullptr, *list = nullptr, *ckey = nullptr;
   unsigned *hist = nullptr, *lcnt = nullptr, *lstat = nullptr;
   int *crp = nullptr, *cci = nullptr, *crow = nullptr, *cdpos = nullptr;
   double *cv = nullptr, *cv2 = nullptr, *fv2 = nullptr;
   auto cleanup = [&]() {
     dfree(dtol); dfree(thr); dfree(cnt); dfree(bsum); dfree(ovf); dfree(sel); dfree(cnt2);
+    dfree(lrows); dfree(nlong);
     dfree(list); dfree(hist); dfree(lcnt); dfree(crp); dfree(cci); dfree(crow); dfree(cdpos);
     dfree(cv); dfree(cv2); dfree(fv2); dfree(lstat); dfree(ckey);
   };
@@ -1390,6 +1392,10 @@ int xs_parilut_factor(xs_ctx *c) {
   PT_TRY(dalloc(&cnt, n));
   PT_TRY(dalloc(&bsum, xs_scan_bsum_len(n)));
   PT_TRY(dalloc(&ovf, 1));
+  if (c->parilut_small_sets) {
+    PT_TRY(dalloc(&lrows, n));
+    PT_TRY(dalloc(&nlong, 1));
+  }
   PT_TRY(dalloc(&sel, xs_pt_select_words()));
   PT_TRY(dalloc(&cnt2, 2));
   PT_TRY(dalloc(&list, xs_pt_list_len()));
@@ -1444,7 +1450,7 @@ int xs_parilut_factor(xs_ctx *c) {
     const double rate = 1.0 + (fill - 1.0) * (double)(step + 1) / (double)steps;
     long keepL = (long)(rate * (double)nl0), keepU = (long)(rate * (double)nu0);
     // 1. candidate pattern
-    xs_pt_symbolic(s, n, c->A.rp, c->A.ci, F.rp_own, F.ci_own, F.dpos, cnt, ovf);
+    xs_pt_symbolic(s, n, c->A.rp, c->A.ci, F.rp_own, F.ci_own, F.dpos, cnt, ovf, lrows, nlong);
     xs_scan_excl(s, n, cnt, crp, bsum);
     xs_pt_len_stats(s, n, cnt, lstat);
     int hdr[2] = {0, 0};

@@ -6,8 +6,12 @@ void xs_scan_excl(hipStream_t s, int n, const int *in, int *out, int *bsum);
 int xs_scan_bsum_len(int n);
 // per-row candidate counts |C_i|, C_i = A_i u F_i u (union over k in L_i of
 // the upper part of F_k with its pivot); *ovf |= 1 for a row beyond 1024
+// longrows (n ints) and nlong (1 int): scratch of the two-tier count (512-slot
+// sets, the rows that fill theirs counted again in 2048-slot sets); nullptr
+// = every row in the 2048-slot set
 void xs_pt_symbolic(hipStream_t s, int n, const int *arp, const int *aci, const int *frp,
-                    const int *fci, const int *fdpos, int *cnt, int *ovf);
+                    const int *fci, const int *fdpos, int *cnt, int *ovf, int *longrows = nullptr,
+                    int *nlong = nullptr);
 // pivot positions of a sorted pattern (*bad |= 2 for a row without its diagonal)
 void xs_pt_dpos(hipStream_t s, int n, const int *rp, const int *ci, int *dpos, int *bad);
 // the candidate pattern C (crp from the counts; cci, crow, pivot positions

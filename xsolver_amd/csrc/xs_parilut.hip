@@ -214,10 +214,13 @@ __global__ __launch_bounds__(XS_WAVE *PT_WAVES) void k_pt_symbolic(
     int n, const int *__restrict__ arp, const int *__restrict__ aci, const int *__restrict__ frp,
     const int *__restrict__ fci, const int *__restrict__ fdpos, int *__restrict__ cnt,
     const int *__restrict__ crp, int *__restrict__ cci, int *__restrict__ crow,
-    int *__restrict__ cdpos, int minlen, int *ovf) {
+    int *__restrict__ cdpos, int minlen, int *ovf, const int *__restrict__ rows = nullptr,
+    const int *__restrict__ nrows = nullptr) {
   __shared__ int tab_all[PT_WAVES][PT_HASH];
   const int lane = threadIdx.x & (XS_WAVE - 1), wv = threadIdx.x / XS_WAVE;
-  const int i = xcd_chunk(blockIdx.x, gridDim.x) * PT_WAVES + wv;
+  const int w = xcd_chunk(blockIdx.x, gridDim.x) * PT_WAVES + wv;
+  // with a row list (counts only): the w-th listed row, for w < *nrows
+  const int i = rows ? (w < *nrows ? rows[w] : n) : w;
   const bool act = i < n && (!FILL || crp[i + 1] - crp[i] > minlen);   // wave-uniform
   int *tab = tab_all[wv];
   int *keys = tab;   // compacted in place (see below)
@@ -277,6 +280,76 @@ __global__ __launch_bounds__(XS_WAVE *PT_WAVES) void k_pt_symbolic(
         cdpos[i] = off;
         atomicOr(ovf, 2);
       }
+    }
+  }
+}
+
+// Candidate counts in a TS-slot set per wave (TS < PT_HASH: a quarter of
+// the LDS, so more waves per CU, and a quarter of the clearing), counted as
+// the inserts that claim an empty slot (no compaction pass).  A row whose
+// set fills up is listed in longrows and counted again by k_pt_symbolic's
+// 2048-slot set (k_pt_count_long).
+template <int TS>
+__device__ __forceinline__ void pt_insert_cnt(int *tab, int c, bool v, int &nnew, bool &full) {
+  unsigned h = ((unsigned)c * 2654435761u) & (TS - 1);
+  int old = tab[h];
+  if (v && old != c) {
+    old = atomicCAS(&tab[h], PT_EMPTY, c);
+    if (old == PT_EMPTY) ++nnew;
+  }
+  const bool more = v && old != PT_EMPTY && old != c;
+  if (__any(more)) {
+    if (more) {
+      int probe = 1;
+      for (; probe < TS; ++probe) {
+        h = (h + 1) & (TS - 1);
+        const int o = atomicCAS(&tab[h], PT_EMPTY, c);
+        if (o == PT_EMPTY) { ++nnew; break; }
+        if (o == c) break;
+      }
+      if (probe == TS) full = true;
+    }
+  }
+}
+
+template <int TS>
+__global__ __launch_bounds__(XS_WAVE *PT_WAVES) void k_pt_count(
+    int n, const int *__restrict__ arp, const int *__restrict__ aci, const int *__restrict__ frp,
+    const int *__restrict__ fci, const int *__restrict__ fdpos, int *__restrict__ cnt,
+    int *__restrict__ longrows, int *nlong) {
+  __shared__ int tab_all[PT_WAVES][TS];
+  const int lane = threadIdx.x & (XS_WAVE - 1), wv = threadIdx.x / XS_WAVE;
+  const int i = xcd_chunk(blockIdx.x, gridDim.x) * PT_WAVES + wv;
+  if (i >= n) return;   // wave-uniform; the waves share no barrier
+  int *tab = tab_all[wv];
+  int nnew = 0;
+  bool full = false;
+  const int fa = frp[i], fb = frp[i + 1], fd = fdpos[i], nlf = fd - fa;
+  const int fj = fci[fb > fa ? fa + min(lane, fb - fa - 1) : 0];
+  const int kf = lane < nlf ? fj : i;
+  const int dkf = fdpos[kf], rkf = frp[kf + 1];
+  pt_stream_t<false> str{fci, nullptr, lane};
+  if (nlf > 0) str.begin(fa, min(XS_WAVE, nlf), dkf, rkf - dkf, 0.0);
+  for (int q = lane; q < TS; q += XS_WAVE) tab[q] = PT_EMPTY;
+  for (int e = arp[i] + lane; e < arp[i + 1]; e += XS_WAVE) pt_insert_cnt<TS>(tab, aci[e], true, nnew, full);
+  pt_insert_cnt<TS>(tab, fj, lane < fb - fa, nnew, full);
+  for (int e = fa + XS_WAVE + lane; e < fb; e += XS_WAVE) pt_insert_cnt<TS>(tab, fci[e], true, nnew, full);
+  auto add = [&](int j, double, bool, bool v) { pt_insert_cnt<TS>(tab, j, v, nnew, full); };
+  if (nlf > 0) {
+    str.finish(add);
+    for (int c0 = fa + XS_WAVE; c0 < fd; c0 += XS_WAVE) {
+      str.load_begin(c0, fd, 0, frp, fdpos);
+      str.finish(add);
+    }
+  }
+  const int u = pt_wave_sum_i(nnew);
+  const bool any_full = __any(full);
+  if (lane == 0) {
+    if (any_full) {
+      cnt[i] = 0;
+      longrows[atomicAdd(nlong, 1)] = i;
+    } else {
+      cnt[i] = u;
     }
   }
 }
@@ -1046,6 +1119,10 @@ __global__ void k_pt_rescale(int nnz, const int *__restrict__ row, const int *__
   else v[e] /= sc[i] * sc[j];
 }
 
+__global__ void k_pt_fill_i32(int *p, int n, int v) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) p[k] = v;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1058,10 +1135,22 @@ void xs_scan_excl(hipStream_t s, int n, const int *in, int *out, int *bsum) {
 int xs_scan_bsum_len(int n) { return (n + SCAN_B - 1) / SCAN_B + 1; }
 
 void xs_pt_symbolic(hipStream_t s, int n, const int *arp, const int *aci, const int *frp,
-                    const int *fci, const int *fdpos, int *cnt, int *ovf) {
+                    const int *fci, const int *fdpos, int *cnt, int *ovf, int *longrows,
+                    int *nlong) {
   dim3 grid((n + PT_WAVES - 1) / PT_WAVES), blk(XS_WAVE * PT_WAVES);
+  if (!longrows || !nlong) {   // every row in the 2048-slot set
+    hipLaunchKernelGGL(k_pt_symbolic<false>, grid, blk, 0, s, n, arp, aci, frp, fci, fdpos, cnt,
+                       nullptr, nullptr, nullptr, nullptr, 0, ovf, nullptr, nullptr);
+    return;
+  }
+  // 512-slot sets first (a full set lists its row), then the listed rows in
+  // the 2048-slot set; the second grid covers every row, the waves past the
+  // list's length exit at once
+  hipLaunchKernelGGL(k_pt_fill_i32, dim3(1), dim3(64), 0, s, nlong, 1, 0);
+  hipLaunchKernelGGL(k_pt_count<512>, grid, blk, 0, s, n, arp, aci, frp, fci, fdpos, cnt, longrows,
+                     nlong);
   hipLaunchKernelGGL(k_pt_symbolic<false>, grid, blk, 0, s, n, arp, aci, frp, fci, fdpos, cnt,
-                     nullptr, nullptr, nullptr, nullptr, 0, ovf);
+                     nullptr, nullptr, nullptr, nullptr, 0, ovf, longrows, nlong);
 }
 
 void xs_pt_dpos(hipStream_t s, int n, const int *rp, const int *ci, int *dpos, int *bad) {

@@ -38,6 +38,7 @@ struct xs_ctx {
   bool pc_reuse = false;        // xs_mat_update_values keeps the preconditioner
   int parilut_steps = 10;       // hip/Precond/precon_gpuilut.cpp:5
   int parilut_rows = 0;         // ParILUT row table (128 / 256 / 512), 0 = from the row lengths
+  bool parilut_small_sets = true;  // ParILUT candidate counts in 512-slot sets first
   // ILU application: -1 = exact sync-free triangular solve; k >= 0 = k Jacobi
   // sweeps per factor (the reference's GPU apply: 20/20 for ILU(0),
   // hip/Precond/precon_gpuilu0.cpp:17-18; 10/15 for ILUT, precon_gpuilut.cpp:19-20)
