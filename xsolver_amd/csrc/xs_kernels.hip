@@ -549,13 +549,22 @@ __global__ __launch_bounds__(256) void k_dia_offsets(int n, const int *__restric
                                                      const int *__restrict__ dpos, int part,
                                                      int *t) {
   __shared__ int lt[XS_DIA_MAX + 1];
+  __shared__ int done;
   if (threadIdx.x <= XS_DIA_MAX) lt[threadIdx.x] = threadIdx.x < XS_DIA_MAX ? INT_MIN : 0;
+  // a workgroup that starts after another has found more offsets than slots
+  // has nothing to add (the flag only goes from 0 to 1; a stale 0 costs time)
+  if (threadIdx.x == 0) done = __hip_atomic_load(t + XS_DIA_MAX, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
+  if (done) return;   // the same value for the whole workgroup
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    if (lt[XS_DIA_MAX]) break;   // this workgroup has its answer: no diagonal slots
     const int e0 = part == 2 ? dpos[r] + 1 : rp[r], e1 = part == 1 ? dpos[r] : rp[r + 1];
     for (int e = e0; e < e1; ++e) {
       const int o = ci[e] - r;
-      if ((e > e0 && ci[e] == ci[e - 1]) || !dia_insert(lt, o)) atomicOr(lt + XS_DIA_MAX, 1);
+      if ((e > e0 && ci[e] == ci[e - 1]) || !dia_insert(lt, o)) {
+        lt[XS_DIA_MAX] = 1;
+        break;
+      }
     }
   }
   __syncthreads();
@@ -573,12 +582,23 @@ static int dia_offsets(hipStream_t s, int n, const int *rp, const int *ci, const
   for (int k = 0; k < XS_DIA_MAX; ++k) h[k] = INT_MIN;
   h[XS_DIA_MAX] = 0;
   if (hipMemcpyAsync(d_tmp, h, sizeof(h), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
-  hipLaunchKernelGGL(k_dia_offsets, dim3(std::min(4096, (n + 255) / 256)), dim3(256), 0, s, n,
-                     rp, ci, dpos, part, d_tmp);
+  // the first rows first: a wide stencil (27 points) already shows more
+  // offsets than slots there, and the pass over every row is skipped
+  const int ns = std::min(n, 4096);
+  hipLaunchKernelGGL(k_dia_offsets, dim3((ns + 255) / 256), dim3(256), 0, s, ns, rp, ci, dpos,
+                     part, d_tmp);
   if (hipMemcpyAsync(h, d_tmp, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return -1;
   if (h[XS_DIA_MAX]) return 0;
+  if (ns < n) {
+    hipLaunchKernelGGL(k_dia_offsets, dim3(std::min(4096, (n + 255) / 256)), dim3(256), 0, s, n,
+                       rp, ci, dpos, part, d_tmp);
+    if (hipMemcpyAsync(h, d_tmp, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return -1;
+    if (h[XS_DIA_MAX]) return 0;
+  }
   int nd = 0;
   for (int k = 0; k < XS_DIA_MAX; ++k)
     if (h[k] != INT_MIN) out.o[nd++] = h[k];
