@@ -143,6 +143,9 @@ int xs_pc_setilut(xs_ctx *ctx, double tol, int max_fill);
  *                    0 (default) = the smallest of 128 / 256 / 512 entries that
  *                    leaves at most n/1024 rows to the per-entry form; 128,
  *                    256 or 512 force it (longer rows take the per-entry form)
+ *   "parilut_small_sets" 1 (default) = ParILUT candidate counts in 512-slot
+ *                    sets first (rows that fill theirs counted again in
+ *                    2048-slot sets; same counts), 0 = 2048-slot sets only
  *   "pc_reuse"       1 = xs_mat_update_values keeps the preconditioner built
  *                    from the earlier values (a preconditioner sets the rate
  *                    of convergence, not the converged solution); 0 (default)
